@@ -1,0 +1,208 @@
+/*
+ * stratum2.h — C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * This is the drop-in boundary for Stratum2's TestRenderer: the scene buffer
+ * formats, the view, the push-constant block and the HDR output buffer are
+ * the reference's own; the Vulkan objects around them are replaced by plain
+ * pointers and sizes.  No C++ or torch types appear in any signature.
+ *
+ * Conventions
+ *   - every call returns an int status (ST_OK == 0); st_last_error() returns a
+ *     thread-local message for the last failing call on this thread;
+ *   - one HIP stream per context; calls on one context must be serialised by
+ *     the caller (the reference renderer is not re-entrant either,
+ *     src/App/TestRenderer.cpp:284-649);
+ *   - host arrays in st_scene_desc are copied, the caller keeps ownership;
+ *   - output pointers in st_outputs are DEVICE pointers (hipMalloc'd or torch
+ *     tensors) owned by the caller.
+ */
+#ifndef STRATUM2_H
+#define STRATUM2_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ST_OK 0
+#define ST_ERROR_INVALID_ARGUMENT 1
+#define ST_ERROR_UNSUPPORTED 2
+#define ST_ERROR_HIP 3
+#define ST_ERROR_OUT_OF_MEMORY 4
+#define ST_ERROR_LIMIT 5
+
+#define ST_INVALID_INSTANCE 0xFFFFu  /* src/Shaders/compat/scene.h:17 */
+#define ST_INVALID_PRIMITIVE 0xFFFFu /* src/Shaders/compat/scene.h:18 */
+#define ST_NO_IMAGE 0xFFFFFFFFu      /* any index >= 2048 means "no image" (scene.h:14) */
+#define ST_IMAGE_LIMIT 2048u         /* gImageCount, src/Shaders/compat/scene.h:14 */
+
+/* InstanceData, src/Shaders/compat/scene.h:30-80.
+ * type in bits 0..3 (0 mesh, 1 sphere, 2 volume), material byte address in 4..31.
+ * mesh:   data = vertexInfoIndex | primitiveCount << 16
+ * volume: data = volume index (0xFFFFFFFF = unit-cube homogeneous medium). */
+typedef struct st_instance {
+    uint32_t type_material_address;
+    uint32_t data;
+} st_instance;
+
+/* TransformData: 3x4 row-major object-to-world, src/Shaders/compat/transform.h:7-24. */
+typedef struct st_transform {
+    float m[3][4];
+} st_transform;
+
+/* MeshVertexInfo, src/Shaders/compat/scene.h:89-132.
+ * buffer indices: [0] = index | position << 16, [1] = normal | texcoord << 16
+ * strides (bytes, 8 bits each): index | position << 8 | normal << 16 | texcoord << 24
+ * offsets (bytes): index, position, normal, texcoord.  A buffer index >= 2048 means absent. */
+typedef struct st_mesh_vertex_info {
+    uint32_t packed_buffer_indices[2];
+    uint32_t packed_strides;
+    uint32_t pad;
+    uint32_t packed_offsets[4];
+} st_mesh_vertex_info;
+
+/* VolumeInfo, src/Shaders/compat/scene.h:82-87 (object-space bounds of a medium instance). */
+typedef struct st_volume_info {
+    float min[3];
+    uint32_t instance_index;
+    float max[3];
+    uint32_t pad;
+} st_volume_info;
+
+typedef struct st_buffer_view {
+    const void* data;
+    uint64_t size_bytes;
+} st_buffer_view;
+
+/* RGBA32F image, row 0 at the top (v = 0), the layout of an env map loaded by
+ * src/Core/Image.cpp:57-73. */
+typedef struct st_image_desc {
+    const float* rgba;
+    uint32_t width;
+    uint32_t height;
+} st_image_desc;
+
+/* The scene buffers of Scene::FrameData (src/App/Scene.hpp:70-116), as
+ * produced by Scene::updateFrameData (src/App/Scene.cpp:428-791). */
+typedef struct st_scene_desc {
+    uint32_t instance_count;
+    const st_instance* instances;
+    const st_transform* transforms;          /* object-to-world, one per instance */
+    uint32_t material_word_count;
+    const uint32_t* material_data;           /* Material::store (64 B), Medium::store (28 B),
+                                                EnvironmentMap::store (16 B) records */
+    uint32_t mesh_vertex_info_count;
+    const st_mesh_vertex_info* mesh_vertex_info;
+    uint32_t vertex_buffer_count;
+    const st_buffer_view* vertex_buffers;    /* byte buffers (indices u16/u32, positions f32x3) */
+    uint32_t image_count;
+    const st_image_desc* images;             /* only the environment image may be referenced */
+    uint32_t environment_material_address;   /* 0xFFFFFFFF = no environment */
+    uint32_t volume_count;
+    const st_buffer_view* volumes;           /* NanoVDB 32.3 float grids (bytes) */
+    uint32_t volume_info_count;
+    const st_volume_info* volume_info;
+} st_scene_desc;
+
+/* ProjectionData, src/Shaders/compat/transform.h:128-191. */
+typedef struct st_projection {
+    float scale[2];
+    float offset[2];
+    float near_plane;
+    float far_plane;
+    float sensor_area;
+    float vertical_fov;
+} st_projection;
+
+/* ViewData (scene.h:134-167) + the camera node's TransformData (camera-to-world). */
+typedef struct st_view {
+    st_projection projection;
+    int32_t image_min[2];
+    int32_t image_max[2];
+    st_transform camera_to_world;
+} st_view;
+
+/* Compile-time defines of TestRenderer (src/App/TestRenderer.cpp:19-36) become flags. */
+#define ST_FLAG_LAMBERTIAN              (1u << 0)  /* gLambertian: all materials Lambert */
+#define ST_FLAG_SAMPLE_DIRECT           (1u << 1)  /* gSampleDirectIllumination (default on) */
+#define ST_FLAG_RESTIR_DI               (1u << 2)  /* gReSTIR_DI: RIS over candidate_samples */
+#define ST_FLAG_RESTIR_DI_REUSE         (1u << 3)  /* screen-space temporal + spatial reuse */
+#define ST_FLAG_ENV_ALIAS               (1u << 4)  /* alias-table env sampling (else mip warp) */
+#define ST_FLAG_ENV_MIS_BALANCED        (1u << 5)  /* env hits weighted against env pdfW */
+#define ST_FLAG_FLOAT_NEE               (1u << 6)  /* NEE added in fp32, not 1/16384 fixed point */
+#define ST_FLAG_PIXEL_JITTER            (1u << 7)  /* gPixelJitter */
+#define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
+
+/* PushConstants of testrenderer.slang:1-21 plus the CLI surface
+ * (--maxPathLength, --minPathLength, --exposure) and the spp loop. */
+typedef struct st_render_params {
+    uint32_t width, height;            /* mOutputExtent */
+    uint32_t max_depth;                /* mMaxDepth == --maxPathLength (segments) */
+    uint32_t min_path_length;          /* --minPathLength: drop contributions of shorter paths */
+    uint32_t seed;                     /* mRandomSeed of the first sample */
+    uint32_t spp;                      /* samples: one launch each, seeds seed..seed+spp-1 */
+    uint32_t max_null_collisions;      /* mMaxNullCollisions */
+    float environment_sample_probability; /* mEnvironmentSampleProbability */
+    uint32_t candidate_samples;        /* mCandidateSamples (ReSTIR DI) */
+    float max_m;                       /* mMaxM */
+    uint32_t flags;                    /* ST_FLAG_* */
+    uint32_t rr_start_depth;           /* Russian roulette from this path length on (0: every bounce) */
+    float exposure;                    /* --exposure: ldr = hdr * 2^exposure (only for st_tonemap) */
+    uint32_t spatial_neighbours;       /* ReSTIR spatial reuse neighbours */
+    float spatial_radius;              /* ReSTIR spatial reuse radius in pixels */
+    uint32_t samples_per_wave;         /* 0 = automatic: samples traced concurrently */
+} st_render_params;
+
+/* Device outputs.  hdr is required; the others may be NULL. */
+typedef struct st_outputs {
+    float* hdr;                 /* RGBA32F W*H: mean radiance over spp (mOutput) */
+    uint32_t* visibility;       /* RG32UI W*H: {instance|primitive<<16, packed normal}, sample 0 */
+    uint32_t* path_length;      /* u32 W*H*spp: segments traced by each sample's path */
+    uint64_t* ray_counts;       /* u64[2]: {extension rays, shadow rays} added by this call */
+} st_outputs;
+
+typedef struct st_context st_context;
+typedef struct st_scene st_scene;
+
+const char* st_last_error(void);
+int st_version(void);
+
+int st_context_create(int device, st_context** out);
+int st_context_destroy(st_context* ctx);
+int st_sync(st_context* ctx);
+
+/* Replaces Scene::updateFrameData + buildAccelerationStructure
+ * (src/App/Scene.cpp:17-55, 428-791): copies the buffers, flattens mesh
+ * instances and builds the SAH BVH on the host, uploads everything. */
+int st_scene_create(st_context* ctx, const st_scene_desc* desc, st_scene** out);
+int st_scene_destroy(st_scene* scene);
+/* BVH statistics: node count, triangle count, max depth, bytes on device. */
+int st_scene_stats(const st_scene* scene, uint64_t out[4]);
+
+/* Replaces TestRenderer::render (src/App/TestRenderer.cpp:284-649):
+ * Render, RenderIteration x (maxDepth-1), ProcessShadowRays, ProcessAtomicOutput
+ * (testrenderer.slang:1087-1183) for every sample, averaged into outputs->hdr. */
+int st_render(st_scene* scene, const st_view* view, const st_render_params* params,
+              const st_outputs* outputs);
+
+/* Primary-ray closest hit only (the minimum slice): visibility for each pixel. */
+int st_trace_primary(st_scene* scene, const st_view* view, uint32_t width, uint32_t height,
+                     uint32_t* visibility, float* t_out);
+
+/* Raw ray queries (device pointers): rays are float4 {o.xyz, tmax}, float4 {d.xyz, 0};
+ * closest: hits are uint4 {t bits, instance|primitive<<16, bary.x bits, bary.y bits};
+ * any-hit: occluded[i] = 1 when something lies in (0, tmax). */
+int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closest,
+                  uint32_t* hits);
+
+/* Duration in ms of the dominant kernel over the last st_render, and the
+ * traversal statistics needed for the algorithmic-bytes roofline. */
+int st_last_timings(st_context* ctx, double out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

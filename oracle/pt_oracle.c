@@ -1,0 +1,1484 @@
+/*
+ * pt_oracle.c — CPU ORACLE for the path-tracing hot path (test infrastructure).
+ *
+ * Plain C99 restatement of Stratum2's TestRenderer (multi-dispatch, deferred
+ * shadow rays, fixed-point NEE accumulation), built with -ffp-contract=off so
+ * every float operation is a single IEEE-754 operation in the order the
+ * reference writes it.  Each function cites the reference lines it restates.
+ * Only tests, the smoke entry and the bench CPU-baseline leg may load it.
+ *
+ * Parity status of each part (the check that pins it is in tests/):
+ *   rng / hash / packing / f16 / rayOffset  pinned: hand-checked constants and
+ *                                           closed-form properties (tests/test_oracle_units.py)
+ *   watertight triangle test + BVH          pinned: FP64 brute force over all triangles
+ *   camera                                  pinned: closed-form pinhole invariants
+ *   BSDFs                                   pinned: furnace / reciprocity / pdf integration
+ *   env mip-warp sampling                   pinned: sample pdf == SampleTexelPdf, integrates to 1
+ *   bounce loop                             pinned: closed-form furnace scene (white Lambert
+ *                                           sphere interior), emission-only analytic image
+ *   driver BVH tie-breaking, f32tof16 and   parity unpinned (Vulkan driver / Slang codegen,
+ *   bilinear weights of the reference        not runnable here)
+ */
+#include "pt_oracle.h"
+
+#include <math.h>
+#include <float.h>
+#include <limits.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PI 3.14159265358979323846f
+#define OR_1_PI (1.0f / OR_PI)
+#define OR_Q 16384 /* gRadianceQuantizationFactor, testrenderer.slang:25 */
+
+static _Thread_local char g_err[256];
+const char* or_last_error(void) { return g_err; }
+static int fail(const char* m) {
+    snprintf(g_err, sizeof g_err, "%s", m);
+    return 1;
+}
+
+/* ------------------------------------------------------------------ math */
+typedef struct { float x, y, z; } f3;
+static inline f3 v3(float x, float y, float z) { f3 r = {x, y, z}; return r; }
+static inline f3 add(f3 a, f3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline f3 sub(f3 a, f3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline f3 mul(f3 a, f3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline f3 mulf(f3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline f3 divv(f3 a, f3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
+static inline f3 divf(f3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+static inline f3 neg(f3 a) { return v3(-a.x, -a.y, -a.z); }
+static inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline f3 cross(f3 a, f3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline float length3(f3 a) { return sqrtf(dot(a, a)); }
+static inline f3 normalize3(f3 a) { return divf(a, sqrtf(dot(a, a))); }
+static inline float fget(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+static inline int any_gt0(f3 a) { return a.x > 0 || a.y > 0 || a.z > 0; }
+static inline int all_le0(f3 a) { return a.x <= 0 && a.y <= 0 && a.z <= 0; }
+static inline float minf(float a, float b) { return a < b ? a : b; }
+static inline float maxf(float a, float b) { return a > b ? a : b; }
+static inline float saturate(float a) { return a < 0 ? 0 : (a > 1 ? 1 : a); }
+static inline float max3f(f3 a) { return maxf(maxf(a.x, a.y), a.z); }
+static inline float pow2(float a) { return a * a; }
+/* luminance, src/Shaders/compat/common.h:51 */
+static inline float luminance(f3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+
+static inline uint32_t asuint(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float asfloat(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* ------------------------------------------------------------ rng.hlsli */
+/* xxhash32, src/Shaders/common/rng.hlsli:5-13 */
+uint32_t or_xxhash32(uint32_t p) {
+    const uint32_t P2 = 2246822519U, P3 = 3266489917U, P4 = 668265263U, P5 = 374761393U;
+    uint32_t h = p + P5;
+    h = P4 * ((h << 17) | (h >> 15));
+    h = P2 * (h ^ (h >> 15));
+    h = P3 * (h ^ (h >> 13));
+    return h ^ (h >> 16);
+}
+/* pcg, rng.hlsli:17-21 */
+uint32_t or_pcg(uint32_t v) {
+    uint32_t s = v * 747796405u + 2891336453u;
+    uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+/* pcg4d, rng.hlsli:23-35 */
+void or_pcg4d(const uint32_t in[4], uint32_t o[4]) {
+    uint32_t x = in[0] * 1664525u + 1013904223u, y = in[1] * 1664525u + 1013904223u;
+    uint32_t z = in[2] * 1664525u + 1013904223u, w = in[3] * 1664525u + 1013904223u;
+    x += y * w; y += z * x; z += x * y; w += y * z;
+    x ^= x >> 16; y ^= y >> 16; z ^= z >> 16; w ^= w >> 16;
+    x += y * w; y += z * x; z += x * y; w += y * z;
+    o[0] = x; o[1] = y; o[2] = z; o[3] = w;
+}
+/* RandomSampler, rng.hlsli:39-60: state (pixel.x, pixel.y, seed, offset). */
+typedef struct { uint32_t s[4]; } rng_t;
+static void rng_init(rng_t* r, uint32_t seed, uint32_t px, uint32_t py, uint32_t offset) {
+    r->s[0] = px; r->s[1] = py; r->s[2] = seed; r->s[3] = offset;
+}
+static void rng_next(rng_t* r, uint32_t o[4]) { r->s[3]++; or_pcg4d(r->s, o); }
+static void rng_nextf(rng_t* r, float o[4]) {
+    uint32_t v[4];
+    rng_next(r, v);
+    for (int i = 0; i < 4; i++) o[i] = asfloat(0x3f800000u | (v[i] >> 9)) - 1.0f;
+}
+
+/* ------------------------------------------------------------- packing */
+/* D3DX_FLOAT_to_INT / D3DX_INT_to_FLOAT / SaturateSigned,
+ * src/Shaders/common/D3DX_DXGIFormatConvert.inl:290-322 */
+static int32_t snorm16(float v) {
+    if (v != v) v = 0;
+    v = minf(maxf(v, -1.0f), 1.0f);
+    return (int32_t)truncf(v * 32767.0f + (v >= 0 ? 0.5f : -0.5f));
+}
+static float unsnorm16(int32_t i) { return maxf((float)i / 32767.0f, -1.0f); }
+/* packNormal2 / packNormal, src/Shaders/common/common.hlsli:3-19 */
+static uint32_t pack_normal(f3 v) {
+    float r = 1.0f / (fabsf(v.x) + fabsf(v.y) + fabsf(v.z));
+    float px = v.x * r, py = v.y * r;
+    if (v.z <= 0) {
+        float nx = (1 - fabsf(py)) * (px >= 0 ? 1.0f : -1.0f);
+        float ny = (1 - fabsf(px)) * (py >= 0 ? 1.0f : -1.0f);
+        px = nx; py = ny;
+    }
+    return ((uint32_t)snorm16(px) & 0xffffu) | ((uint32_t)snorm16(py) << 16);
+}
+/* unpackNormal2 / unpackNormal, common.hlsli:8-22 */
+static f3 unpack_normal(uint32_t p) {
+    float x = unsnorm16((int32_t)(p << 16) >> 16), y = unsnorm16((int32_t)(p & 0xffff0000u) >> 16);
+    f3 v = v3(x, y, 1 - (fabsf(x) + fabsf(y)));
+    if (v.z < 0) {
+        float nx = (1 - fabsf(v.y)) * (v.x >= 0 ? 1.0f : -1.0f);
+        float ny = (1 - fabsf(v.x)) * (v.y >= 0 ? 1.0f : -1.0f);
+        v.x = nx; v.y = ny;
+    }
+    return normalize3(v);
+}
+uint32_t or_pack_normal(const float v[3]) { return pack_normal(v3(v[0], v[1], v[2])); }
+void or_unpack_normal(uint32_t p, float o[3]) { f3 v = unpack_normal(p); o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+
+/* f32tof16 / f16tof32 (testrenderer.slang:75-97), round-to-nearest-even. */
+uint16_t or_f32_to_f16(float f) {
+    uint32_t x = asuint(f), sign = (x >> 16) & 0x8000u, ex = (x >> 23) & 0xffu, m = x & 0x7fffffu;
+    if (ex == 0xffu) return (uint16_t)(sign | 0x7c00u | (m ? 0x200u : 0));
+    int e = (int)ex - 127 + 15;
+    if (e >= 0x1f) return (uint16_t)(sign | 0x7c00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        m |= 0x800000u;
+        int sh = 14 - e;
+        uint32_t hm = m >> sh, rem = m & ((1u << sh) - 1), half = 1u << (sh - 1);
+        if (rem > half || (rem == half && (hm & 1u))) hm++;
+        return (uint16_t)(sign | hm);
+    }
+    uint32_t r = sign | ((uint32_t)e << 10) | (m >> 13), rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (r & 1u))) r++;
+    return (uint16_t)r;
+}
+float or_f16_to_f32(uint16_t h) {
+    uint32_t sign = ((uint32_t)h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0) {
+        float v = (float)m * (1.0f / 16777216.0f);
+        return sign ? -v : v;
+    }
+    if (e == 0x1f) return asfloat(sign | 0x7f800000u | (m << 13));
+    return asfloat(sign | ((e + 112) << 23) | (m << 13));
+}
+static float q16(float f) { return or_f16_to_f32(or_f32_to_f16(f)); }
+
+/* unorm8 material fields, src/Shaders/compat/material_data.h:43-64 */
+static float un8(uint32_t w, int shift) { return (float)((w >> shift) & 0xffu) / 255.0f; }
+
+/* rayOffset, src/Shaders/common/intersection.hlsli:17-32 */
+static float roff1(float p, float n) {
+    const float origin = 1.0f / 32.0f, fs = 1.0f / 65536.0f;
+    int32_t of = (int32_t)(256.0f * n);
+    if (fabsf(p) < origin) return p + fs * n;
+    return asfloat((uint32_t)((int32_t)asuint(p) + ((p < 0.0f) ? -of : of)));
+}
+static f3 ray_offset(f3 p, f3 n) { return v3(roff1(p.x, n.x), roff1(p.y, n.y), roff1(p.z, n.z)); }
+static f3 ray_offset_dir(f3 p, f3 n, f3 d) { return ray_offset(p, dot(n, d) < 0 ? neg(n) : n); }
+void or_ray_offset(const float p[3], const float n[3], float o[3]) {
+    f3 r = ray_offset(v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]));
+    o[0] = r.x; o[1] = r.y; o[2] = r.z;
+}
+
+/* makeOrthonormal, common.hlsli:24-34 (rows r0, r1, r2 = N) */
+static void make_orthonormal(f3 N, f3 r[3]) {
+    if (N.x != N.y || N.x != N.z) r[0] = v3(N.z - N.y, N.x - N.z, N.y - N.x);
+    else r[0] = v3(N.z - N.y, N.x + N.z, -N.y - N.x);
+    r[0] = normalize3(r[0]);
+    r[1] = cross(N, r[0]);
+    r[2] = N;
+}
+/* warps, common.hlsli:36-77 */
+static void sample_uniform_triangle(float u1, float u2, float b[2]) {
+    float a = sqrtf(u1);
+    b[0] = 1 - a; b[1] = a * u2;
+}
+static f3 sample_uniform_sphere(float u1, float u2) {
+    float z = u2 * 2 - 1, r = sqrtf(maxf(1 - z * z, 0)), phi = 2 * OR_PI * u1;
+    return v3(r * cosf(phi), r * sinf(phi), z);
+}
+static void sample_concentric_disc(float u1, float u2, float o[2]) {
+    float ux = 2 * u1 - 1, uy = 2 * u2 - 1, theta, r;
+    if (ux == 0 && uy == 0) { o[0] = o[1] = 0; return; }
+    if (fabsf(ux) > fabsf(uy)) { r = ux; theta = (OR_PI / 4) * (uy / ux); }
+    else { r = uy; theta = OR_PI / 2 - (OR_PI / 4) * (ux / uy); }
+    o[0] = r * cosf(theta); o[1] = r * sinf(theta);
+}
+static f3 sample_cos_hemisphere(float u1, float u2) {
+    float xy[2];
+    sample_concentric_disc(u1, u2, xy);
+    return v3(xy[0], xy[1], sqrtf(maxf(0, 1 - (xy[0] * xy[0] + xy[1] * xy[1]))));
+}
+static float cos_hemisphere_pdfw(float c) { return maxf(c, 0.0f) / OR_PI; }
+static float concentric_disc_pdfa(void) { return 1.0f / OR_PI; }
+static float mis1(float a) { return a * a; }
+/* mis, testrenderer.slang:284-290 */
+static float mis2(float a, float b) { float a2 = mis1(a); return a2 / (a2 + mis1(b)); }
+
+/* stableAtan2 / cartesianToSphericalUv / sphericalUvToCartesian, compat/common.h:137-150 */
+static float stable_atan2(float y, float x) {
+    return x == 0.0f ? (y == 0 ? 0 : (y < 0 ? -OR_PI / 2 : OR_PI / 2)) : atan2f(y, x);
+}
+static void cart_to_uv(f3 v, float uv[2]) {
+    float th = stable_atan2(v.z, v.x);
+    uv[0] = th * OR_1_PI * 0.5f + 0.5f;
+    uv[1] = acosf(minf(maxf(v.y, -1.0f), 1.0f)) * OR_1_PI;
+}
+static f3 uv_to_cart(float u, float v) {
+    u = u * 2 - 1;
+    u *= OR_PI; v *= OR_PI;
+    float sp = sinf(v);
+    return v3(sp * cosf(u), cosf(v), sp * sinf(u));
+}
+
+/* ------------------------------------------------------------- scene */
+typedef struct { float lo[3], hi[3]; uint32_t a, n; } bnode; /* n>0: leaf [a, a+n); else children a, a+1 */
+typedef struct { int w, h; float* rgba; } level_t;
+
+struct or_scene {
+    uint32_t ninst;
+    st_instance* inst;
+    st_transform* xf;
+    uint32_t nmatw;
+    uint32_t* mat;
+    uint32_t nvi;
+    st_mesh_vertex_info* vi;
+    uint32_t nvb;
+    uint8_t** vb;
+    uint64_t* vbsz;
+    /* environment */
+    int has_env;
+    uint32_t env_addr;
+    f3 env_emission;
+    int env_has_image;
+    int nlevels;
+    level_t lv[16];
+    /* lights: emissive instances in instance order (Scene.cpp:487-491) */
+    uint32_t nlights;
+    uint32_t* light_inst;
+    float sphere[4];
+    /* flattened world-space triangles */
+    uint32_t ntris;
+    float* tv;     /* 9 floats each */
+    uint32_t* tid; /* instance | primitive << 16 */
+    bnode* nodes;
+    uint32_t nnodes;
+    uint32_t* order;
+};
+
+static f3 xf_point(const st_transform* t, f3 v) {
+    return v3(t->m[0][0] * v.x + t->m[0][1] * v.y + t->m[0][2] * v.z + t->m[0][3],
+              t->m[1][0] * v.x + t->m[1][1] * v.y + t->m[1][2] * v.z + t->m[1][3],
+              t->m[2][0] * v.x + t->m[2][1] * v.y + t->m[2][2] * v.z + t->m[2][3]);
+}
+static f3 xf_vector(const st_transform* t, f3 v) {
+    return v3(t->m[0][0] * v.x + t->m[0][1] * v.y + t->m[0][2] * v.z,
+              t->m[1][0] * v.x + t->m[1][1] * v.y + t->m[1][2] * v.z,
+              t->m[2][0] * v.x + t->m[2][1] * v.y + t->m[2][2] * v.z);
+}
+
+static uint32_t vi_field(uint32_t w, int start, int len) { return (w >> start) & ((len == 32) ? 0xffffffffu : ((1u << len) - 1)); }
+
+/* LoadTriangleIndices, src/Shaders/common/scene.hlsli:43-61 */
+static int load_tri(const or_scene* s, const st_mesh_vertex_info* vi, uint32_t prim, uint32_t tri[3]) {
+    uint32_t ib = vi_field(vi->packed_buffer_indices[0], 0, 16), stride = vi_field(vi->packed_strides, 0, 8);
+    if (ib >= s->nvb) return 0;
+    uint64_t off = (uint64_t)vi->packed_offsets[0] + (uint64_t)prim * 3 * stride;
+    if (off + 3 * stride > s->vbsz[ib]) return 0;
+    const uint8_t* b = s->vb[ib] + off;
+    for (int i = 0; i < 3; i++) {
+        if (stride == 2) { uint16_t v; memcpy(&v, b + 2 * i, 2); tri[i] = v; }
+        else { uint32_t v; memcpy(&v, b + 4 * i, 4); tri[i] = v; }
+    }
+    return 1;
+}
+static int load_f3(const or_scene* s, uint32_t buf, uint32_t offset, uint32_t stride, uint32_t idx, f3* o) {
+    if (buf >= s->nvb) return 0;
+    uint64_t off = (uint64_t)offset + (uint64_t)stride * idx;
+    if (off + 12 > s->vbsz[buf]) return 0;
+    float f[3];
+    memcpy(f, s->vb[buf] + off, 12);
+    *o = v3(f[0], f[1], f[2]);
+    return 1;
+}
+static int load_f2(const or_scene* s, uint32_t buf, uint32_t offset, uint32_t stride, uint32_t idx, float o[2]) {
+    if (buf >= s->nvb) return 0;
+    uint64_t off = (uint64_t)offset + (uint64_t)stride * idx;
+    if (off + 8 > s->vbsz[buf]) return 0;
+    memcpy(o, s->vb[buf] + off, 8);
+    return 1;
+}
+
+typedef struct {
+    f3 pos;
+    uint32_t flags_material;
+    uint32_t gn, sn, tan;
+    float area;
+    float uv[2];
+} shading_t; /* ShadingData, src/Shaders/compat/scene.h:185-198 (fields used here) */
+
+/* makeTriangleShadingDataWithoutPosition + makeTriangleShadingData,
+ * src/Shaders/common/shading_data.hlsli:59-151 */
+static void make_tri_shading(const or_scene* s, uint32_t ii, uint32_t prim, float bx, float by, shading_t* r) {
+    const st_instance* in = &s->inst[ii];
+    const st_transform* T = &s->xf[ii];
+    const st_mesh_vertex_info* vi = &s->vi[in->data & 0xffffu];
+    uint32_t tri[3] = {0, 0, 0};
+    load_tri(s, vi, prim, tri);
+    uint32_t pb = vi_field(vi->packed_buffer_indices[0], 16, 16), ps = vi_field(vi->packed_strides, 8, 8);
+    f3 v0 = v3(0, 0, 0), v1 = v0, v2 = v0;
+    load_f3(s, pb, vi->packed_offsets[1], ps, tri[0], &v0);
+    load_f3(s, pb, vi->packed_offsets[1], ps, tri[1], &v1);
+    load_f3(s, pb, vi->packed_offsets[1], ps, tri[2], &v2);
+    r->flags_material = (in->type_material_address >> 4) << 4;
+    f3 dPds = xf_vector(T, sub(v0, v2)), dPdt = xf_vector(T, sub(v1, v2));
+    f3 gn = cross(dPds, dPdt);
+    float area2 = length3(gn);
+    gn = divf(gn, area2);
+    r->gn = pack_normal(gn);
+    r->area = area2 / 2;
+    float t0[2] = {0, 0}, t1[2] = {0, 0}, t2[2] = {0, 0};
+    uint32_t tb = vi_field(vi->packed_buffer_indices[1], 16, 16), ts = vi_field(vi->packed_strides, 24, 8);
+    if (tb < ST_IMAGE_LIMIT) {
+        load_f2(s, tb, vi->packed_offsets[3], ts, tri[0], t0);
+        load_f2(s, tb, vi->packed_offsets[3], ts, tri[1], t1);
+        load_f2(s, tb, vi->packed_offsets[3], ts, tri[2], t2);
+    }
+    float duvds[2] = {t2[0] - t0[0], t2[1] - t0[1]}, duvdt[2] = {t2[0] - t1[0], t2[1] - t1[1]};
+    r->uv[0] = t0[0] + (t1[0] - t0[0]) * bx + duvds[0] * by;
+    r->uv[1] = t0[1] + (t1[1] - t0[1]) * bx + duvds[1] * by;
+    float det = duvds[0] * duvdt[1] - duvdt[0] * duvds[1];
+    float invDet = 1 / det;
+    float dsdu = duvdt[1] * invDet, dtdu = -duvds[1] * invDet;
+    f3 dPdu;
+    if (det != 0) {
+        dPdu = neg(add(mulf(dPds, dsdu), mulf(dPdt, dtdu)));
+    } else {
+        f3 onb[3];
+        make_orthonormal(gn, onb);
+        dPdu = onb[0];
+    }
+    int snvalid = 0;
+    f3 sn = v3(0, 0, 0);
+    uint32_t nb = vi_field(vi->packed_buffer_indices[1], 0, 16), ns = vi_field(vi->packed_strides, 16, 8);
+    if (nb < ST_IMAGE_LIMIT) { /* gShadingNormals defaults to true (TestRenderer.cpp:22) */
+        f3 n0, n1, n2;
+        if (load_f3(s, nb, vi->packed_offsets[2], ns, tri[0], &n0) && load_f3(s, nb, vi->packed_offsets[2], ns, tri[1], &n1) &&
+            load_f3(s, nb, vi->packed_offsets[2], ns, tri[2], &n2)) {
+            sn = add(add(n0, mulf(sub(n1, n0), bx)), mulf(sub(n2, n0), by));
+            snvalid = !((sn.x == 0 && sn.y == 0 && sn.z == 0) || sn.x != sn.x || sn.y != sn.y || sn.z != sn.z);
+        }
+    }
+    if (!snvalid) {
+        r->sn = r->gn;
+        r->tan = pack_normal(normalize3(dPdu));
+    } else {
+        sn = normalize3(xf_vector(T, sn));
+        f3 tg = normalize3(sub(dPdu, mulf(sn, dot(sn, dPdu))));
+        r->sn = pack_normal(sn);
+        r->tan = pack_normal(tg);
+        if (dot(sn, gn) < 0) r->gn = pack_normal(neg(gn));
+    }
+    r->pos = xf_point(T, add(add(v0, mulf(sub(v1, v0), bx)), mulf(sub(v2, v0), by)));
+}
+
+/* ShadingData::toLocal / toWorld / shadingNormalCorrection, shading_data.hlsli:24-54 */
+static f3 to_local(const shading_t* sd, f3 v) {
+    f3 n = unpack_normal(sd->sn), t = unpack_normal(sd->tan);
+    f3 b = cross(n, t); /* bitangent flip flag is never set for triangles */
+    return v3(dot(v, t), dot(v, b), dot(v, n));
+}
+static f3 to_world(const shading_t* sd, f3 v) {
+    f3 n = unpack_normal(sd->sn), t = unpack_normal(sd->tan);
+    f3 b = cross(n, t);
+    return add(add(mulf(t, v.x), mulf(b, v.y)), mulf(n, v.z));
+}
+static float signf(float x) { return x > 0 ? 1.0f : (x < 0 ? -1.0f : 0.0f); }
+static float shading_normal_correction(const shading_t* sd, f3 din, f3 dout) {
+    f3 lg = to_local(sd, unpack_normal(sd->gn));
+    float ngin = dot(lg, din), ngout = dot(lg, dout);
+    if (signf(ngout) != signf(dout.z) || signf(ngin) != signf(din.z)) return 0;
+    return 1; /* Adjoint == false on camera paths */
+}
+
+/* --------------------------------------------------------- materials */
+typedef struct { uint32_t p[4]; float escale; } material_t;
+static material_t load_material(const or_scene* s, uint32_t addr) {
+    material_t m;
+    memset(&m, 0, sizeof m);
+    if ((uint64_t)addr / 4 + 9 > s->nmatw) return m;
+    memcpy(m.p, s->mat + addr / 4, 16);
+    m.escale = asfloat(s->mat[addr / 4 + 8]);
+    return m;
+}
+static f3 m_base(const material_t* m) { return v3(un8(m->p[0], 0), un8(m->p[0], 8), un8(m->p[0], 16)); }
+static f3 m_emission(const material_t* m) {
+    return mulf(v3(un8(m->p[1], 0), un8(m->p[1], 8), un8(m->p[1], 16)), m->escale);
+}
+#define M_SPEC(m) un8((m)->p[0], 24)
+#define M_SHEEN(m) un8((m)->p[1], 24)
+#define M_METAL(m) un8((m)->p[2], 0)
+#define M_ROUGH(m) un8((m)->p[2], 8)
+#define M_ANISO(m) un8((m)->p[2], 16)
+#define M_SUBSURF(m) un8((m)->p[2], 24)
+#define M_CC(m) un8((m)->p[3], 0)
+#define M_CCGLOSS(m) un8((m)->p[3], 8)
+#define M_TRANS(m) un8((m)->p[3], 16)
+#define M_ETA1(m) (un8((m)->p[3], 24) + 1)
+
+/* microfacet.hlsli:22-179 */
+static float powf5(float x) { return powf(x, 5.0f); }
+static f3 schlick3(f3 F0, float c) {
+    float k = powf5(maxf(1 - c, 0.0f));
+    return v3(F0.x + (1 - F0.x) * k, F0.y + (1 - F0.y) * k, F0.z + (1 - F0.z) * k);
+}
+static float schlick1(float F0, float c) { return F0 + (1 - F0) * powf5(maxf(1 - c, 0.0f)); }
+static float fresnel_dielectric3(float ni, float nt, float eta) {
+    float rs = (ni - eta * nt) / (ni + eta * nt);
+    float rp = (eta * ni - nt) / (eta * ni + nt);
+    return (rs * rs + rp * rp) / 2;
+}
+static float fresnel_dielectric(float ni, float eta) {
+    float nt2 = 1 - (1 - ni * ni) / (eta * eta);
+    if (nt2 < 0) return 1;
+    return fresnel_dielectric3(fabsf(ni), sqrtf(nt2), eta);
+}
+static float GTR1(float ndh, float a) {
+    float a2 = a * a, t = 1 + (a2 - 1) * ndh * ndh;
+    return (a2 - 1) / (OR_PI * logf(a2) * t);
+}
+static float GTR2a(f3 h, float ax, float ay) {
+    f3 hs = divv(h, v3(ax, ay, 1));
+    float l2 = dot(hs, hs);
+    return 1 / (OR_PI * ax * ay * l2 * l2);
+}
+static float smith_gtr2(f3 v, float ax, float ay) {
+    float ax2 = ax * ax, ay2 = ay * ay;
+    f3 v2 = mul(v, v);
+    float L = (-1 + sqrtf(1 + (v2.x * ax2 + v2.y * ay2) / v2.z)) / 2;
+    return 1 / (1 + L);
+}
+static float smith_gtr1(f3 v) { return smith_gtr2(v, 0.25f, 0.25f); }
+static f3 sample_vndf(f3 din, float ax, float ay, float r0, float r1) {
+    int flip = din.z < 0;
+    if (flip) din = neg(din);
+    f3 hin = normalize3(v3(ax * din.x, ay * din.y, din.z));
+    float r = sqrtf(r0), phi = 2 * OR_PI * r1;
+    float t1 = r * cosf(phi), t2 = r * sinf(phi);
+    float s = (1 + hin.z) / 2;
+    t2 = (1 - s) * sqrtf(1 - t1 * t1) + s * t2;
+    f3 dn = v3(t1, t2, sqrtf(maxf(0.0f, 1 - t1 * t1 - t2 * t2)));
+    f3 fr[3];
+    make_orthonormal(hin, fr);
+    f3 hn = add(add(mulf(fr[0], dn.x), mulf(fr[1], dn.y)), mulf(hin, dn.z));
+    f3 o = normalize3(v3(ax * hn.x, ay * hn.y, maxf(0.0f, hn.z)));
+    return flip ? neg(o) : o;
+}
+static f3 lerp3(f3 a, f3 b, float t) { return add(a, mulf(sub(b, a), t)); }
+static float lerp1(float a, float b, float t) { return a + (b - a) * t; }
+
+/* Disney eval, src/Shaders/materials/disney.hlsli:9-139 (Adjoint = false) */
+static f3 disney_eval(const material_t* m, f3 din, f3 dout) {
+    float aspect = sqrtf(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
+    float ax = maxf(1e-4f, alpha / aspect), ay = maxf(1e-4f, alpha * aspect), metallic = M_METAL(m);
+    if (din.z * dout.z > 0) {
+        float ndin = fabsf(din.z), ndout = dout.z;
+        f3 h = normalize3(add(din, dout));
+        if (h.z * din.z < 0) h = neg(h);
+        float ndh = h.z, hdin = dot(h, din), hdout = dot(h, dout);
+        f3 c = v3(0, 0, 0);
+        if (din.z >= 0 && dout.z >= 0) {
+            float Fd90 = 0.5f + 2 * M_ROUGH(m) * hdout * hdout;
+            float sno = powf(1 - ndout, 5.0f), sni = powf(1 - ndin, 5.0f), sho = powf(1 - hdout, 5.0f);
+            float based = (1 + (Fd90 - 1) * sno) * (1 + (Fd90 - 1) * sni);
+            float Fss90 = hdout * hdout * M_ROUGH(m);
+            float Fss = (1 + (Fss90 - 1) * sno) * (1 + (Fss90 - 1) * sni);
+            float ss = 1.25f * (Fss * (1 / (ndout + ndin) - 0.5f) + 0.5f);
+            f3 bc = m_base(m);
+            c = add(c, mulf(mulf(bc, (1 - M_TRANS(m)) * (1 - metallic)), (lerp1(based, ss, M_SUBSURF(m)) / OR_PI) * ndout));
+            float lum = luminance(bc);
+            f3 ctint = lum > 0 ? divf(bc, lum) : v3(1, 1, 1);
+            f3 csheen = lerp3(v3(1, 1, 1), ctint, 0.5f);
+            c = add(c, mulf(csheen, (1 - metallic) * M_SHEEN(m) * sho * ndout));
+            if (ndin > 0 && hdout > 0 && ndh > 0) {
+                float eta = M_ETA1(m);
+                float f0 = (eta - 1) * (eta - 1) / ((eta + 1) * (eta + 1));
+                f3 scol = lerp3(v3(1, 1, 1), ctint, 0.5f);
+                f3 c0 = lerp3(mulf(scol, M_SPEC(m) * f0), m_base(m), metallic);
+                f3 F = schlick3(c0, hdout);
+                float D = GTR2a(h, ax, ay), G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
+                float sw = 1 - M_TRANS(m) * (1 - metallic);
+                c = add(c, divf(mulf(mulf(F, sw), D * G), 4 * ndin));
+            }
+            if (ndin > 0 && ndh > 0) {
+                float ac = lerp1(0.1f, 0.001f, M_CCGLOSS(m));
+                float Fc = schlick1(0.04f, hdout), Dc = GTR1(ndh, ac), Gc = smith_gtr1(din) * smith_gtr1(dout);
+                float v = M_CC(m) * Fc * Dc * Gc / (4 * ndin);
+                c = add(c, v3(v, v, v));
+            }
+        }
+        float gw = (1 - metallic) * M_TRANS(m);
+        if (gw > 0) {
+            float eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+            float Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay);
+            float G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
+            c = add(c, mulf(m_base(m), gw * (Fg * D * G) / (4 * fabsf(ndin))));
+        }
+        return c;
+    }
+    float gw = (1 - metallic) * M_TRANS(m);
+    if (gw <= 0) return v3(0, 0, 0);
+    float eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+    f3 h = normalize3(add(din, mulf(dout, eta)));
+    if (h.z * din.z < 0) h = neg(h);
+    float hdin = dot(h, din), hdout = dot(h, dout), sd = hdin + eta * hdout;
+    float Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay);
+    float G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
+    f3 bc = m_base(m);
+    f3 sq = v3(sqrtf(bc.x), sqrtf(bc.y), sqrtf(bc.z));
+    return mulf(sq, gw * ((1 - Fg) * D * G * eta * eta * fabsf(hdout * hdin)) / (fabsf(din.z) * sd * sd));
+}
+
+/* Disney evalPdf, disney.hlsli:141-231 */
+static float disney_pdf(const material_t* m, f3 din, f3 dout) {
+    float alpha = pow2(M_ROUGH(m)), metallic = M_METAL(m), aspect = sqrtf(1 - M_ANISO(m) * 0.9f);
+    float ax = maxf(0.0001f, alpha / aspect), ay = maxf(0.0001f, alpha * aspect);
+    float ac = (1 - M_CCGLOSS(m)) * 0.1f + M_CCGLOSS(m) * 0.001f, tr = M_TRANS(m);
+    float dw = (1 - metallic) * (1 - tr), mw = (1 - tr * (1 - metallic)), gw = (1 - metallic) * tr, cw = M_CC(m);
+    float tw = dw + mw + gw + cw;
+    float dp = dw / tw, mp = mw / tw, gp = gw / tw, cp = cw / tw;
+    if (din.z < 0) {
+        dp = 0; mp = 0; cp = 0;
+        if (gp > 0) gp = 1;
+    }
+    if (din.z * dout.z > 0) {
+        f3 h = normalize3(add(din, dout));
+        if (h.z < 0) h = neg(h);
+        float ndin = din.z, ndh = h.z, hdin = dot(h, din), hdout = dot(h, dout);
+        if (din.z >= 0 && dout.z >= 0) {
+            dp *= maxf(dout.z, 0.0f) / OR_PI;
+            if (ndin > 0) {
+                float D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay);
+                mp *= (D * Gi / (4 * ndin));
+            } else mp = 0;
+            if (ndh > 0 && hdout > 0) {
+                float Dc = GTR1(ndh, ac);
+                cp *= (Dc * ndh / (4 * hdout));
+            } else cp = 0;
+        }
+        float eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+        float Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay);
+        gp *= (Fg * D * Gi / (4 * fabsf(ndin)));
+    } else {
+        float eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+        f3 h = normalize3(add(din, mulf(dout, eta)));
+        if (h.z < 0) h = neg(h);
+        float hdin = dot(h, din), hdout = dot(h, dout);
+        float D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay), Fg = fresnel_dielectric(hdin, eta);
+        float sd = hdin + eta * hdout, dh = eta * eta * hdout / (sd * sd);
+        gp *= (1 - Fg) * D * Gi * fabsf(dh * hdin / din.z);
+    }
+    return dp + mp + gp + cp;
+}
+
+typedef struct { f3 dir, f; float fwd, rev, eta, rough; } dsr_t;   /* DirectionSampleRecord */
+typedef struct { f3 f; float fwd, rev; } rer_t;                    /* ReflectanceEvalRecord */
+
+/* Disney sample, disney.hlsli:233-380 (returns direction, eta, roughness) */
+static dsr_t disney_sample(const material_t* m, f3 din, f3 rnd) {
+    dsr_t r;
+    memset(&r, 0, sizeof r);
+    float aspect = sqrtf(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
+    float ax = maxf(0.0001f, alpha / aspect), ay = maxf(0.0001f, alpha * aspect);
+    float ac = (1 - M_CCGLOSS(m)) * 0.1f + M_CCGLOSS(m) * 0.001f;
+    float metallic = M_METAL(m), tr = M_TRANS(m);
+    float dw = (1 - metallic) * (1 - tr), mw = (1 - tr * (1 - metallic)), gw = (1 - metallic) * tr, cw = M_CC(m);
+    if (din.z >= 0) {
+        float tw = dw + mw + gw + cw;
+        float dp = dw / tw, mp = mw / tw, gp = gw / tw;
+        if (rnd.z <= dp) {
+            r.dir = sample_cos_hemisphere(rnd.x, rnd.y);
+            r.eta = 0; r.rough = 1;
+            return r;
+        } else if (rnd.z <= (dp + mp)) {
+            f3 h = sample_vndf(din, ax, ay, rnd.x, rnd.y);
+            r.dir = normalize3(add(neg(din), mulf(h, 2 * dot(din, h))));
+            r.eta = 0; r.rough = M_ROUGH(m);
+            return r;
+        } else if (rnd.z <= (dp + mp + gp)) {
+            if (gp <= 0) return r;
+            f3 h = sample_vndf(din, ax, ay, rnd.x, rnd.y);
+            float hdin = dot(h, din), eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+            float F = fresnel_dielectric(hdin, eta), u = (rnd.z - (dp + mp)) / gp;
+            if (u <= F) {
+                r.dir = normalize3(add(neg(din), mulf(h, 2 * dot(din, h))));
+                r.eta = 0;
+                return r; /* roughness stays 0: reference returns before setting it */
+            }
+            float hdo2 = 1 - (1 - hdin * hdin) / (eta * eta);
+            if (hdo2 <= 0) { memset(&r, 0, sizeof r); return r; }
+            if (hdin < 0) h = neg(h);
+            float hdo = sqrtf(hdo2);
+            r.dir = add(divf(neg(din), eta), mulf(h, fabsf(hdin) / eta - hdo));
+            r.eta = eta;
+            r.rough = M_ROUGH(m);
+            return r;
+        } else {
+            float a2 = ac * ac;
+            float ch = sqrtf(maxf(0.0f, (1 - powf(a2, 1 - rnd.x)) / (1 - a2)));
+            float sh = sqrtf(maxf(1 - ch * ch, 0.0f)), az = 2 * OR_PI * rnd.y;
+            f3 mn = v3(sh * cosf(az), sh * sinf(az), ch);
+            r.dir = normalize3(add(neg(din), mulf(mn, 2 * dot(din, mn))));
+            r.eta = 0;
+            r.rough = sqrtf(ac);
+            return r;
+        }
+    }
+    f3 h = sample_vndf(din, ax, ay, rnd.x, rnd.y);
+    if (h.z < 0) h = neg(h);
+    float hdin = dot(h, din), eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+    float F = fresnel_dielectric(hdin, eta);
+    if (rnd.z <= F) {
+        r.dir = normalize3(add(neg(din), mulf(h, 2 * dot(din, h))));
+        r.eta = 0;
+    } else {
+        float hdo2 = 1 - (1 - hdin * hdin) / (eta * eta);
+        if (hdo2 <= 0) { memset(&r, 0, sizeof r); return r; }
+        if (hdin < 0) h = neg(h);
+        float hdo = sqrtf(hdo2);
+        r.dir = add(divf(neg(din), eta), mulf(h, fabsf(hdin) / eta - hdo));
+        r.eta = eta;
+    }
+    r.rough = M_ROUGH(m);
+    return r;
+}
+
+/* evaluateReflectance for Disney (disney.hlsli:391-403) or Lambert (lambertian.hlsli:13-24) */
+static rer_t bsdf_eval(const material_t* m, int lambert, f3 din, f3 dout) {
+    rer_t r;
+    if (lambert) {
+        if (din.z * dout.z < 0) { r.f = v3(0, 0, 0); r.fwd = r.rev = 0; return r; }
+        r.f = divf(mulf(m_base(m), fabsf(dout.z)), OR_PI);
+        r.fwd = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(dout.z)));
+        r.rev = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(din.z)));
+        return r;
+    }
+    r.f = disney_eval(m, din, dout);
+    r.fwd = disney_pdf(m, din, dout);
+    r.rev = disney_pdf(m, dout, din);
+    return r;
+}
+/* sampleDirection (disney.hlsli:415-429, lambertian.hlsli:29-39) */
+static dsr_t bsdf_sample(const material_t* m, int lambert, f3 rnd, f3 din) {
+    dsr_t r;
+    if (lambert) {
+        r.dir = sample_cos_hemisphere(rnd.x, rnd.y);
+        if (din.z < 0) r.dir = neg(r.dir);
+        r.f = divf(mulf(m_base(m), fabsf(r.dir.z)), OR_PI);
+        r.fwd = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(r.dir.z)));
+        r.rev = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(din.z)));
+        r.rough = 1; r.eta = 0;
+        return r;
+    }
+    r = disney_sample(m, din, rnd);
+    rer_t f = bsdf_eval(m, 0, din, r.dir);
+    r.f = f.f; r.fwd = f.fwd; r.rev = f.rev;
+    return r;
+}
+static int m_singular(const material_t* m, int lambert) {
+    if (lambert) return 0;
+    return maxf(maxf(M_SPEC(m), M_METAL(m)), M_TRANS(m)) * (1 - M_ROUGH(m)) > 0.75f;
+}
+static int m_can_eval(const material_t* m) { f3 b = m_base(m); return any_gt0(b); }
+static float m_cont_prob(const material_t* m) { return saturate(luminance(m_base(m))); }
+
+void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], const float dout[3], float o[5]) {
+    material_t m;
+    memcpy(m.p, mat, 16);
+    m.escale = asfloat(mat[4]);
+    rer_t r = bsdf_eval(&m, lambertian, v3(din[0], din[1], din[2]), v3(dout[0], dout[1], dout[2]));
+    o[0] = r.f.x; o[1] = r.f.y; o[2] = r.f.z; o[3] = r.fwd; o[4] = r.rev;
+}
+void or_bsdf_sample(const uint32_t mat[5], int lambertian, const float rnd[3], const float din[3], float o[10]) {
+    material_t m;
+    memcpy(m.p, mat, 16);
+    m.escale = asfloat(mat[4]);
+    dsr_t r = bsdf_sample(&m, lambertian, v3(rnd[0], rnd[1], rnd[2]), v3(din[0], din[1], din[2]));
+    o[0] = r.dir.x; o[1] = r.dir.y; o[2] = r.dir.z; o[3] = r.f.x; o[4] = r.f.y; o[5] = r.f.z;
+    o[6] = r.fwd; o[7] = r.rev; o[8] = r.eta; o[9] = r.rough;
+}
+
+/* ------------------------------------------------------- environment */
+static const float* texel(const level_t* l, int x, int y) { return l->rgba + 4 * ((size_t)y * l->w + x); }
+static float texel_lum(const level_t* l, int x, int y) {
+    const float* t = texel(l, x, y);
+    return luminance(v3(t[0], t[1], t[2]));
+}
+/* mip chain by 2x2 box (vkCmdBlitImage with linear filter, src/Core/Image.cpp:282-308) */
+static void build_mips(or_scene* s) {
+    int l = 0;
+    while (s->lv[l].w > 1 || s->lv[l].h > 1) {
+        level_t* a = &s->lv[l];
+        level_t* b = &s->lv[l + 1];
+        b->w = a->w > 1 ? a->w / 2 : 1;
+        b->h = a->h > 1 ? a->h / 2 : 1;
+        b->rgba = (float*)malloc(sizeof(float) * 4 * (size_t)b->w * b->h);
+        for (int y = 0; y < b->h; y++)
+            for (int x = 0; x < b->w; x++) {
+                int x0 = a->w > 1 ? 2 * x : 0, x1 = a->w > 1 ? 2 * x + 1 : 0;
+                int y0 = a->h > 1 ? 2 * y : 0, y1 = a->h > 1 ? 2 * y + 1 : 0;
+                for (int c = 0; c < 4; c++) {
+                    float v = ((texel(a, x0, y0)[c] + texel(a, x1, y0)[c]) + (texel(a, x0, y1)[c] + texel(a, x1, y1)[c])) * 0.25f;
+                    b->rgba[4 * ((size_t)y * b->w + x) + c] = v;
+                }
+            }
+        l++;
+    }
+    s->nlevels = l + 1;
+}
+/* bilinear SampleLevel(uv, 0) with repeat addressing (TestRenderer.cpp:54-57) */
+static f3 env_bilinear(const level_t* l, float u, float v) {
+    float x = u * (float)l->w - 0.5f, y = v * (float)l->h - 0.5f;
+    float fx0 = floorf(x), fy0 = floorf(y), ax = x - fx0, ay = y - fy0;
+    int x0 = (int)fx0, y0 = (int)fy0;
+    int xa = ((x0 % l->w) + l->w) % l->w, xb = (((x0 + 1) % l->w) + l->w) % l->w;
+    int ya = ((y0 % l->h) + l->h) % l->h, yb = (((y0 + 1) % l->h) + l->h) % l->h;
+    const float *t00 = texel(l, xa, ya), *t10 = texel(l, xb, ya), *t01 = texel(l, xa, yb), *t11 = texel(l, xb, yb);
+    f3 r;
+    float* rr = &r.x;
+    for (int c = 0; c < 3; c++) {
+        float top = t00[c] * (1 - ax) + t10[c] * ax;
+        float bot = t01[c] * (1 - ax) + t11[c] * ax;
+        rr[c] = top * (1 - ay) + bot * ay;
+    }
+    return r;
+}
+/* SampleTexel, src/Shaders/common/lights.hlsli:49-97 */
+static void sample_texel(const or_scene* s, float rx, float ry, float uv[2], float* pdf) {
+    int levelCount = s->nlevels, cx = 0, cy = 0, lw = 1, lh = 1;
+    *pdf = 1;
+    int iend = levelCount - 1 < 11 ? levelCount - 1 : 11;
+    for (int i = 1; i < iend; i++) {
+        const level_t* L = &s->lv[levelCount - 1 - i];
+        float inv_h = 1 / (float)L->h;
+        cx *= L->w / lw; cy *= L->h / lh;
+        float p[4] = {0, 0, 0, 0};
+        if ((uint32_t)L->w - (uint32_t)cx > 1) {
+            float sy = sinf(OR_PI * ((float)cy + 0.5f) * inv_h);
+            p[0] = texel_lum(L, cx, cy) * sy;
+            p[1] = texel_lum(L, cx + 1, cy) * sy;
+        }
+        if ((uint32_t)L->h - (uint32_t)cy > 1) {
+            float sy = sinf(OR_PI * ((float)cy + 1.5f) * inv_h);
+            p[2] = texel_lum(L, cx, cy + 1) * sy;
+            p[3] = texel_lum(L, cx + 1, cy + 1) * sy;
+        }
+        float sum = p[0] + p[1] + p[2] + p[3];
+        if (sum < 1e-6f) continue;
+        for (int j = 0; j < 4; j++) p[j] = p[j] / sum;
+        for (int j = 0; j < 4; j++) {
+            if (j == 3 || rx < p[j]) {
+                cx += j % 2; cy += j / 2;
+                *pdf *= p[j];
+                rx /= p[j];
+                break;
+            }
+            rx -= p[j];
+        }
+        lw = L->w; lh = L->h;
+    }
+    *pdf *= (float)((uint32_t)lw * (uint32_t)lh);
+    uv[0] = ((float)cx + rx) / (float)lw;
+    uv[1] = ((float)cy + ry) / (float)lh;
+}
+/* SampleTexelPdf, lights.hlsli:98-139 */
+static float sample_texel_pdf(const or_scene* s, float u, float v) {
+    int levelCount = s->nlevels, lw = 1, lh = 1;
+    float pdf = 1;
+    int iend = levelCount - 1 < 11 ? levelCount - 1 : 11;
+    for (int i = 1; i < iend; i++) {
+        const level_t* L = &s->lv[levelCount - 1 - i];
+        float inv_h = 1 / (float)L->h;
+        int ux = (int)((float)L->w * u), uy = (int)((float)L->h * v);
+        int cx = (ux / 2) * 2, cy = (uy / 2) * 2;
+        int ox = ux - cx < 1 ? ux - cx : 1, oy = uy - cy < 1 ? uy - cy : 1;
+        float p[4] = {0, 0, 0, 0};
+        if ((uint32_t)L->w - (uint32_t)cx > 1) {
+            float sy = sinf(OR_PI * ((float)cy + 0.5f) * inv_h);
+            p[0] = texel_lum(L, cx, cy) * sy;
+            p[1] = texel_lum(L, cx + 1, cy) * sy;
+        }
+        if ((uint32_t)L->h - (uint32_t)cy > 1) {
+            float sy = sinf(OR_PI * ((float)cy + 1.5f) * inv_h);
+            p[2] = texel_lum(L, cx, cy + 1) * sy;
+            p[3] = texel_lum(L, cx + 1, cy + 1) * sy;
+        }
+        float sum = p[0] + p[1] + p[2] + p[3];
+        if (sum < 1e-6f) continue;
+        for (int j = 0; j < 4; j++) p[j] = p[j] / sum;
+        pdf *= p[oy * 2 + ox];
+        lw = L->w; lh = L->h;
+    }
+    return pdf * (float)((uint32_t)lw * (uint32_t)lh);
+}
+int or_env_sample_texel(or_scene* s, const float rnd[2], float uv[2], float* pdf) {
+    if (!s->env_has_image) return 1;
+    sample_texel(s, rnd[0], rnd[1], uv, pdf);
+    return 0;
+}
+float or_env_sample_texel_pdf(or_scene* s, const float uv[2]) {
+    return s->env_has_image ? sample_texel_pdf(s, uv[0], uv[1]) : 0;
+}
+
+typedef struct {
+    uint32_t flags;
+    float env_prob;
+    uint32_t max_depth, min_len, cand;
+    int lambert;
+} cfg_t;
+
+/* EvaluateEnvironment, lights.hlsli:143-163 */
+static f3 evaluate_environment(const or_scene* s, const cfg_t* c, f3 d, float* pdfw) {
+    if (!s->has_env) { *pdfw = 0; return v3(0, 0, 0); }
+    f3 e = s->env_emission;
+    if (s->env_has_image) {
+        float uv[2];
+        cart_to_uv(d, uv);
+        e = mul(e, env_bilinear(&s->lv[0], uv[0], uv[1]));
+        *pdfw = sample_texel_pdf(s, uv[0], uv[1]) / (2 * OR_PI * OR_PI * sqrtf(1 - d.y * d.y));
+    } else {
+        *pdfw = 1 / (4 * OR_PI);
+    }
+    if (s->nlights > 0) *pdfw *= c->env_prob;
+    return e;
+}
+
+typedef struct {
+    f3 radiance, dir, pos;
+    float pdf, dist, cos_light;
+    uint32_t normal;
+    int finite;
+} isr_t; /* IlluminationSampleRecord, lights.hlsli:32-45 */
+
+/* SampleEmission + SampleIllumination, lights.hlsli:166-260.  The reference
+ * leaves mDirectionToLight unset for environment samples (lights.hlsli:234-244);
+ * here it is the sampled direction. */
+static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const float rnd_in[4], f3 ref) {
+    isr_t r;
+    memset(&r, 0, sizeof r);
+    float rnd[4] = {rnd_in[0], rnd_in[1], rnd_in[2], rnd_in[3]};
+    if (s->has_env) {
+        if (s->nlights == 0 || rnd[3] < c->env_prob) {
+            f3 d;
+            float pdf;
+            if (s->env_has_image) {
+                float uv[2];
+                sample_texel(s, rnd[0], rnd[1], uv, &pdf);
+                d = uv_to_cart(uv[0], uv[1]);
+                pdf /= (2 * OR_PI * OR_PI * sqrtf(1 - pow2(d.y)));
+            } else {
+                d = sample_uniform_sphere(rnd[0], rnd[1]);
+                pdf = 1 / (4 * OR_PI);
+            }
+            if (s->nlights > 0) pdf *= c->env_prob;
+            r.pdf = pdf;
+            r.dist = INFINITY;
+            r.pos = v3(INFINITY, INFINITY, INFINITY);
+            r.cos_light = 1;
+            r.finite = 0;
+            r.dir = d;
+            float tmp;
+            r.radiance = evaluate_environment(s, c, d, &tmp);
+            return r;
+        }
+        if (s->nlights > 0) rnd[3] = (rnd[3] - c->env_prob) / (1 - c->env_prob);
+    }
+    if (s->nlights == 0) return r;
+    uint32_t li = (uint32_t)(rnd[2] * (float)s->nlights) % s->nlights;
+    uint32_t ii = s->light_inst[li];
+    float pdf = 1 / (float)s->nlights;
+    if (s->has_env) pdf *= 1 - c->env_prob;
+    const st_instance* in = &s->inst[ii];
+    if ((in->type_material_address & 0xfu) != 0) { memset(&r, 0, sizeof r); return r; }
+    uint32_t pc = in->data >> 16;
+    uint32_t prim = (uint32_t)(rnd[3] * (float)pc) % pc;
+    pdf /= (float)pc;
+    float b[2];
+    sample_uniform_triangle(rnd[0], rnd[1], b);
+    shading_t sd;
+    make_tri_shading(s, ii, prim, b[0], b[1], &sd);
+    pdf /= sd.area;
+    r.pdf = pdf;
+    r.finite = 1;
+    material_t m = load_material(s, sd.flags_material >> 4);
+    r.radiance = m_emission(&m);
+    r.pos = sd.pos;
+    r.normal = sd.gn;
+    r.dir = sub(r.pos, ref);
+    r.dist = length3(r.dir);
+    r.dir = divf(r.dir, r.dist);
+    r.cos_light = -dot(unpack_normal(r.normal), r.dir);
+    if (r.cos_light <= 0) { memset(&r, 0, sizeof r); }
+    return r;
+}
+
+/* ------------------------------------------------------------- BVH */
+static float tri_c(const or_scene* s, uint32_t t, int ax) {
+    const float* v = s->tv + 9 * (size_t)t;
+    return (v[ax] + v[3 + ax] + v[6 + ax]) * (1.0f / 3.0f);
+}
+static void select_nth(const or_scene* s, uint32_t* a, int64_t n, int64_t k, int ax) {
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        float piv = tri_c(s, a[(lo + hi) / 2], ax);
+        int64_t i = lo, j = hi;
+        while (i <= j) {
+            while (tri_c(s, a[i], ax) < piv) i++;
+            while (tri_c(s, a[j], ax) > piv) j--;
+            if (i <= j) { uint32_t t = a[i]; a[i] = a[j]; a[j] = t; i++; j--; }
+        }
+        if (k <= j) hi = j;
+        else if (k >= i) lo = i;
+        else break;
+    }
+}
+static void node_bounds(const or_scene* s, bnode* nd, const uint32_t* a, uint32_t n) {
+    for (int k = 0; k < 3; k++) { nd->lo[k] = INFINITY; nd->hi[k] = -INFINITY; }
+    for (uint32_t i = 0; i < n; i++) {
+        const float* v = s->tv + 9 * (size_t)a[i];
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                nd->lo[k] = minf(nd->lo[k], v[3 * j + k]);
+                nd->hi[k] = maxf(nd->hi[k], v[3 * j + k]);
+            }
+    }
+    for (int k = 0; k < 3; k++) { nd->lo[k] = nextafterf(nd->lo[k], -INFINITY); nd->hi[k] = nextafterf(nd->hi[k], INFINITY); }
+}
+static void build_rec(or_scene* s, uint32_t ni, uint32_t start, uint32_t n) {
+    bnode* nd = &s->nodes[ni];
+    node_bounds(s, nd, s->order + start, n);
+    if (n <= 4) { nd->a = start; nd->n = n; return; }
+    float c_lo[3] = {INFINITY, INFINITY, INFINITY}, c_hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            float c = tri_c(s, s->order[start + i], k);
+            c_lo[k] = minf(c_lo[k], c); c_hi[k] = maxf(c_hi[k], c);
+        }
+    int ax = 0;
+    for (int k = 1; k < 3; k++) if (c_hi[k] - c_lo[k] > c_hi[ax] - c_lo[ax]) ax = k;
+    select_nth(s, s->order + start, n, n / 2, ax);
+    uint32_t c0 = s->nnodes;
+    s->nnodes += 2;
+    nd = &s->nodes[ni];
+    nd->a = c0; nd->n = 0;
+    build_rec(s, c0, start, n / 2);
+    build_rec(s, c0 + 1, start + n / 2, n - n / 2);
+}
+
+/* Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1) 2013).
+ * Returns 1 with t, b1 (weight of v1), b2 (weight of v2) on a hit with t > 0. */
+typedef struct { f3 o, d; int kx, ky, kz; float Sx, Sy, Sz; } wray_t;
+static wray_t wray_make(f3 o, f3 d) {
+    wray_t r;
+    r.o = o; r.d = d;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    r.kx = (r.kz + 1) % 3; r.ky = (r.kx + 1) % 3;
+    if (fget(d, r.kz) < 0) { int t = r.kx; r.kx = r.ky; r.ky = t; }
+    r.Sx = fget(d, r.kx) / fget(d, r.kz);
+    r.Sy = fget(d, r.ky) / fget(d, r.kz);
+    r.Sz = 1.0f / fget(d, r.kz);
+    return r;
+}
+static int wt_intersect(const wray_t* r, const float* v, float* t, float* b1, float* b2) {
+    f3 A = sub(v3(v[0], v[1], v[2]), r->o), B = sub(v3(v[3], v[4], v[5]), r->o), C = sub(v3(v[6], v[7], v[8]), r->o);
+    float Ax = fget(A, r->kx) - r->Sx * fget(A, r->kz), Ay = fget(A, r->ky) - r->Sy * fget(A, r->kz);
+    float Bx = fget(B, r->kx) - r->Sx * fget(B, r->kz), By = fget(B, r->ky) - r->Sy * fget(B, r->kz);
+    float Cx = fget(C, r->kx) - r->Sx * fget(C, r->kz), Cy = fget(C, r->ky) - r->Sy * fget(C, r->kz);
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return 0;
+    float det = U + V + W;
+    if (det == 0.0f) return 0;
+    float Az = r->Sz * fget(A, r->kz), Bz = r->Sz * fget(B, r->kz), Cz = r->Sz * fget(C, r->kz);
+    float T = U * Az + V * Bz + W * Cz;
+    float rcp = 1.0f / det;
+    float tt = T * rcp;
+    if (!(tt > 0.0f)) return 0;
+    *t = tt; *b1 = V * rcp; *b2 = W * rcp;
+    return 1;
+}
+
+typedef struct { float t, b1, b2; uint32_t id; int hit; } hit_t;
+
+static int box_hit(const bnode* n, f3 o, f3 inv, float tmax, float* tnear) {
+    float t0x = (n->lo[0] - o.x) * inv.x, t1x = (n->hi[0] - o.x) * inv.x;
+    float t0y = (n->lo[1] - o.y) * inv.y, t1y = (n->hi[1] - o.y) * inv.y;
+    float t0z = (n->lo[2] - o.z) * inv.z, t1z = (n->hi[2] - o.z) * inv.z;
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    *tnear = tn;
+    return tn <= tf * 1.00000024f;
+}
+
+/* closest (any=0) or any hit (any=1) in (0, tmax) */
+static hit_t trace(const or_scene* s, f3 o, f3 d, float tmax, int any, uint64_t* nodes, uint64_t* tris) {
+    hit_t h;
+    h.hit = 0; h.t = tmax; h.b1 = h.b2 = 0; h.id = 0xffffffffu;
+    if (s->nnodes == 0) return h;
+    wray_t wr = wray_make(o, d);
+    f3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const bnode* n = &s->nodes[stack[--sp]];
+        float tn;
+        (*nodes)++;
+        if (!box_hit(n, o, inv, h.t, &tn)) continue;
+        if (n->n > 0) {
+            for (uint32_t i = 0; i < n->n; i++) {
+                uint32_t ti = s->order[n->a + i];
+                float t, b1, b2;
+                (*tris)++;
+                if (!wt_intersect(&wr, s->tv + 9 * (size_t)ti, &t, &b1, &b2)) continue;
+                uint32_t id = s->tid[ti];
+                if (t < h.t || (t == h.t && h.hit && id < h.id)) {
+                    h.t = t; h.b1 = b1; h.b2 = b2; h.id = id; h.hit = 1;
+                    if (any) return h;
+                }
+            }
+        } else {
+            stack[sp++] = n->a + 1;
+            stack[sp++] = n->a;
+        }
+    }
+    return h;
+}
+
+or_scene* or_scene_create(const st_scene_desc* d) {
+    if (!d || (d->instance_count && (!d->instances || !d->transforms))) { fail("invalid scene description"); return NULL; }
+    or_scene* s = (or_scene*)calloc(1, sizeof *s);
+    s->ninst = d->instance_count;
+    s->inst = (st_instance*)malloc(sizeof(st_instance) * (s->ninst + 1));
+    s->xf = (st_transform*)malloc(sizeof(st_transform) * (s->ninst + 1));
+    memcpy(s->inst, d->instances, sizeof(st_instance) * s->ninst);
+    memcpy(s->xf, d->transforms, sizeof(st_transform) * s->ninst);
+    s->nmatw = d->material_word_count;
+    s->mat = (uint32_t*)malloc(4 * (s->nmatw + 16));
+    memcpy(s->mat, d->material_data, 4 * (size_t)s->nmatw);
+    s->nvi = d->mesh_vertex_info_count;
+    s->vi = (st_mesh_vertex_info*)malloc(sizeof(st_mesh_vertex_info) * (s->nvi + 1));
+    memcpy(s->vi, d->mesh_vertex_info, sizeof(st_mesh_vertex_info) * s->nvi);
+    s->nvb = d->vertex_buffer_count;
+    s->vb = (uint8_t**)calloc(s->nvb + 1, sizeof(uint8_t*));
+    s->vbsz = (uint64_t*)calloc(s->nvb + 1, sizeof(uint64_t));
+    for (uint32_t i = 0; i < s->nvb; i++) {
+        s->vbsz[i] = d->vertex_buffers[i].size_bytes;
+        s->vb[i] = (uint8_t*)malloc(s->vbsz[i] + 16);
+        memcpy(s->vb[i], d->vertex_buffers[i].data, s->vbsz[i]);
+    }
+    /* environment record: EnvironmentMap::store, src/App/Scene.hpp:42-45 */
+    s->has_env = d->environment_material_address != 0xFFFFFFFFu;
+    if (s->has_env) {
+        uint32_t a = d->environment_material_address / 4;
+        if (a + 4 > s->nmatw) { fail("environment address out of range"); or_scene_destroy(s); return NULL; }
+        s->env_addr = d->environment_material_address;
+        s->env_emission = v3(asfloat(s->mat[a]), asfloat(s->mat[a + 1]), asfloat(s->mat[a + 2]));
+        uint32_t img = s->mat[a + 3];
+        if (img < ST_IMAGE_LIMIT) {
+            if (img >= d->image_count) { fail("environment image index out of range"); or_scene_destroy(s); return NULL; }
+            const st_image_desc* im = &d->images[img];
+            s->env_has_image = 1;
+            s->lv[0].w = (int)im->width; s->lv[0].h = (int)im->height;
+            s->lv[0].rgba = (float*)malloc(sizeof(float) * 4 * (size_t)im->width * im->height);
+            memcpy(s->lv[0].rgba, im->rgba, sizeof(float) * 4 * (size_t)im->width * im->height);
+            build_mips(s);
+        }
+    }
+    /* lights, scene AABB and flattened triangles */
+    s->light_inst = (uint32_t*)malloc(sizeof(uint32_t) * (s->ninst + 1));
+    float amin[3] = {INFINITY, INFINITY, INFINITY}, amax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    uint64_t ntri = 0;
+    for (uint32_t i = 0; i < s->ninst; i++) {
+        if ((s->inst[i].type_material_address & 0xfu) == 0) ntri += s->inst[i].data >> 16;
+    }
+    s->tv = (float*)malloc(sizeof(float) * 9 * (ntri + 1));
+    s->tid = (uint32_t*)malloc(sizeof(uint32_t) * (ntri + 1));
+    for (uint32_t i = 0; i < s->ninst; i++) {
+        const st_instance* in = &s->inst[i];
+        uint32_t type = in->type_material_address & 0xfu;
+        if (type != 0) continue;
+        material_t m = load_material(s, in->type_material_address >> 4);
+        f3 e = v3(un8(m.p[1], 0), un8(m.p[1], 8), un8(m.p[1], 16));
+        if (any_gt0(e)) s->light_inst[s->nlights++] = i;
+        uint32_t vii = in->data & 0xffffu, pc = in->data >> 16;
+        if (vii >= s->nvi) { fail("vertex info index out of range"); or_scene_destroy(s); return NULL; }
+        const st_mesh_vertex_info* vi = &s->vi[vii];
+        uint32_t pb = vi_field(vi->packed_buffer_indices[0], 16, 16), ps = vi_field(vi->packed_strides, 8, 8);
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t p = 0; p < pc; p++) {
+            uint32_t tri[3];
+            if (!load_tri(s, vi, p, tri)) { fail("index buffer out of range"); or_scene_destroy(s); return NULL; }
+            for (int j = 0; j < 3; j++) {
+                f3 v;
+                if (!load_f3(s, pb, vi->packed_offsets[1], ps, tri[j], &v)) { fail("position buffer out of range"); or_scene_destroy(s); return NULL; }
+                lo[0] = minf(lo[0], v.x); lo[1] = minf(lo[1], v.y); lo[2] = minf(lo[2], v.z);
+                hi[0] = maxf(hi[0], v.x); hi[1] = maxf(hi[1], v.y); hi[2] = maxf(hi[2], v.z);
+                f3 w = xf_point(&s->xf[i], v);
+                s->tv[9 * (size_t)s->ntris + 3 * j] = w.x;
+                s->tv[9 * (size_t)s->ntris + 3 * j + 1] = w.y;
+                s->tv[9 * (size_t)s->ntris + 3 * j + 2] = w.z;
+            }
+            s->tid[s->ntris] = i | (p << 16);
+            s->ntris++;
+        }
+        if (pc == 0) continue;
+        /* transformed corners of the mesh AABB, Scene.cpp:622-631 */
+        for (int c = 0; c < 8; c++) {
+            f3 corner = v3((c % 2) == 0 ? lo[0] : hi[0], ((c % 4) / 2) == 0 ? lo[1] : hi[1], (c / 4) == 0 ? lo[2] : hi[2]);
+            corner = xf_point(&s->xf[i], corner);
+            amin[0] = minf(amin[0], corner.x); amin[1] = minf(amin[1], corner.y); amin[2] = minf(amin[2], corner.z);
+            amax[0] = maxf(amax[0], corner.x); amax[1] = maxf(amax[1], corner.y); amax[2] = maxf(amax[2], corner.z);
+        }
+    }
+    /* scene sphere, TestRenderer.cpp:458-461 */
+    f3 cen = divf(add(v3(amax[0], amax[1], amax[2]), v3(amin[0], amin[1], amin[2])), 2);
+    s->sphere[0] = cen.x; s->sphere[1] = cen.y; s->sphere[2] = cen.z;
+    s->sphere[3] = length3(sub(v3(amax[0], amax[1], amax[2]), cen));
+    if (s->ntris > 0) {
+        s->order = (uint32_t*)malloc(sizeof(uint32_t) * s->ntris);
+        for (uint32_t i = 0; i < s->ntris; i++) s->order[i] = i;
+        s->nodes = (bnode*)malloc(sizeof(bnode) * 2 * (size_t)s->ntris);
+        s->nnodes = 1;
+        build_rec(s, 0, 0, s->ntris);
+    }
+    return s;
+}
+
+void or_scene_destroy(or_scene* s) {
+    if (!s) return;
+    free(s->inst); free(s->xf); free(s->mat); free(s->vi);
+    for (uint32_t i = 0; i < s->nvb; i++) free(s->vb[i]);
+    free(s->vb); free(s->vbsz);
+    for (int l = 0; l < 16; l++) free(s->lv[l].rgba);
+    free(s->light_inst); free(s->tv); free(s->tid); free(s->nodes); free(s->order);
+    free(s);
+}
+
+int or_trace_rays(or_scene* s, const float* rays, uint32_t count, int closest, uint32_t* hits) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < (int64_t)count; i++) {
+        const float* r = rays + 8 * i;
+        uint64_t a = 0, b = 0;
+        hit_t h = trace(s, v3(r[0], r[1], r[2]), v3(r[4], r[5], r[6]), r[3], !closest, &a, &b);
+        if (closest) {
+            hits[4 * i + 0] = h.hit ? asuint(h.t) : asuint(r[3]);
+            hits[4 * i + 1] = h.hit ? h.id : 0xffffffffu;
+            hits[4 * i + 2] = asuint(h.b1);
+            hits[4 * i + 3] = asuint(h.b2);
+        } else {
+            hits[i] = (uint32_t)h.hit;
+        }
+    }
+    return 0;
+}
+
+int or_trace_rays_bruteforce_f64(or_scene* s, const float* rays, uint32_t count, uint32_t* hits) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < (int64_t)count; i++) {
+        const float* r = rays + 8 * i;
+        double o[3] = {r[0], r[1], r[2]}, dd[3] = {r[4], r[5], r[6]};
+        double best = r[3];
+        uint32_t bid = 0xffffffffu;
+        for (uint32_t t = 0; t < s->ntris; t++) {
+            const float* v = s->tv + 9 * (size_t)t;
+            double e1[3], e2[3], pv[3], tv[3], qv[3];
+            for (int k = 0; k < 3; k++) { e1[k] = (double)v[3 + k] - v[k]; e2[k] = (double)v[6 + k] - v[k]; tv[k] = o[k] - v[k]; }
+            pv[0] = dd[1] * e2[2] - dd[2] * e2[1]; pv[1] = dd[2] * e2[0] - dd[0] * e2[2]; pv[2] = dd[0] * e2[1] - dd[1] * e2[0];
+            double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+            if (det == 0) continue;
+            double u = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) / det;
+            if (u < 0 || u > 1) continue;
+            qv[0] = tv[1] * e1[2] - tv[2] * e1[1]; qv[1] = tv[2] * e1[0] - tv[0] * e1[2]; qv[2] = tv[0] * e1[1] - tv[1] * e1[0];
+            double vv = (dd[0] * qv[0] + dd[1] * qv[1] + dd[2] * qv[2]) / det;
+            if (vv < 0 || u + vv > 1) continue;
+            double tt = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) / det;
+            if (tt > 0 && tt < best) { best = tt; bid = s->tid[t]; }
+        }
+        hits[4 * i + 0] = asuint((float)best);
+        hits[4 * i + 1] = bid;
+        hits[4 * i + 2] = 0;
+        hits[4 * i + 3] = 0;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------ path tracer */
+typedef struct {
+    f3 origin, thr;
+    uint32_t dir;       /* packed oct-snorm16 (testrenderer.slang:119-122) */
+    uint32_t path_len;  /* mPathLength */
+    uint32_t inst_prim; /* mInstancePrimitiveIndex */
+    float t16, pdfa16, dirpdf16; /* FP16-stored fields, kept as their f32 values */
+    rng_t rng;
+} pstate_t; /* PathState, testrenderer.slang:66-125 */
+
+typedef struct { f3 o, d, c; float tmax, w; } shadow_t;
+
+typedef struct {
+    const or_scene* s;
+    const cfg_t* c;
+    pstate_t ps;
+    f3 out;           /* mOutput.rgb of this launch */
+    uint32_t nee[3];  /* mOutputAtomic */
+    shadow_t sh[64];
+    int nsh;
+    uint32_t vis[2];
+    uint64_t rays[2], trav[4];
+} pctx_t;
+
+/* AddRadiance, testrenderer.slang:220-235 */
+static void add_radiance(pctx_t* x, f3 rad, float w) {
+    if (w <= 0 || all_le0(rad)) return;
+    x->out = add(x->out, mulf(rad, w));
+}
+/* AddEmission, testrenderer.slang:788-808 (+ --minPathLength gate, vcm.slang:960) */
+static void add_emission(pctx_t* x, f3 e, float directPdfA, float emissionPdfW, float WtoA) {
+    (void)emissionPdfW;
+    float w = 1;
+    if (x->ps.path_len > 1) { /* isSpecular is never set on camera paths without VC */
+        if ((x->c->flags & ST_FLAG_SAMPLE_DIRECT) && x->ps.dirpdf16 > 0) w = mis2(x->ps.dirpdf16 * WtoA, directPdfA);
+    }
+    if (x->ps.path_len < x->c->min_len) return;
+    add_radiance(x, e, w);
+}
+/* ConnectToLight (plain NEE), testrenderer.slang:751-779 */
+static void connect_to_light(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
+    float rnd[4];
+    rng_nextf(&x->ps.rng, rnd);
+    isr_t r = sample_illumination(x->s, x->c, rnd, sd->pos);
+    if (all_le0(r.radiance) || r.pdf <= 0) return;
+    float AtoW = 1;
+    if (r.finite) AtoW = pow2(r.dist) / r.cos_light;
+    float dpw = r.pdf * AtoW;
+    f3 ldout = to_local(sd, r.dir);
+    rer_t f = bsdf_eval(m, x->c->lambert, ldin, ldout);
+    f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
+    float cp = m_cont_prob(m);
+    f.fwd *= cp; f.rev *= cp;
+    f3 contrib = divf(mul(mul(x->ps.thr, r.radiance), f.f), dpw);
+    float w = mis2(r.pdf, f.fwd / AtoW);
+    if (x->ps.path_len + 1 < x->c->min_len) return;
+    if (x->nsh >= 64) return;
+    shadow_t* sr = &x->sh[x->nsh++];
+    sr->o = ray_offset_dir(sd->pos, unpack_normal(sd->gn), r.dir);
+    sr->d = r.dir;
+    sr->tmax = maxf(r.dist * 0.99f, r.dist - 1e-2f); /* AdjustIntersectionDistance, intersection.hlsli:65 */
+    sr->c = contrib;
+    sr->w = w;
+}
+
+/* TraceRay + ExtendPath + ProcessInteraction, testrenderer.slang:816-935, 1026-1071.
+ * Returns 0 when the path ends. */
+static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view) {
+    (void)p; (void)view;
+    const or_scene* s = x->s;
+    pstate_t* ps = &x->ps;
+    f3 dir = unpack_normal(ps->dir);
+    hit_t h = trace(s, ps->origin, dir, INFINITY, 0, &x->trav[0], &x->trav[1]);
+    x->rays[0]++;
+    ps->path_len++;
+    shading_t sd;
+    memset(&sd, 0, sizeof sd);
+    float pick = 0;
+    if (h.hit) {
+        uint32_t ii = h.id & 0xffffu, prim = h.id >> 16;
+        pick = 1.0f / (float)(s->inst[ii].data >> 16);
+        make_tri_shading(s, ii, prim, h.b1, h.b2, &sd);
+        ps->inst_prim = h.id;
+    } else {
+        ps->inst_prim = 0xffffffffu;
+        sd.area = -1;
+        sd.sn = pack_normal(dir);
+    }
+    ps->t16 = q16(h.hit ? h.t : INFINITY);
+    /* LightSamplePdfA, lights.hlsli:264-271 */
+    float pdfa = 0;
+    if (s->nlights > 0) {
+        pdfa = pick / (sd.area * (float)s->nlights);
+        if (s->has_env) pdfa *= 1 - x->c->env_prob;
+    }
+    ps->pdfa16 = q16(pdfa);
+    if (!h.hit) {
+        if (ps->path_len == 1) { x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn; }
+        float dpw;
+        f3 e = mul(ps->thr, evaluate_environment(s, x->c, dir, &dpw));
+        if (any_gt0(e)) {
+            float dpa = concentric_disc_pdfa() / pow2(s->sphere[3]);
+            if (x->c->flags & ST_FLAG_ENV_MIS_BALANCED) dpa = dpw;
+            add_emission(x, e, dpa, dpw, 1);
+        }
+        ps->thr = v3(0, 0, 0);
+        return 0;
+    }
+    if (all_le0(ps->thr)) return 0; /* ExtendPath, testrenderer.slang:1062-1063 */
+    material_t m = load_material(s, sd.flags_material >> 4);
+    f3 ldin = to_local(&sd, neg(dir));
+    float cos_in = fabsf(ldin.z);
+    if (ps->path_len == 1) { x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn; }
+    if (ldin.z > 0) {
+        f3 e = mul(ps->thr, m_emission(&m));
+        if (any_gt0(e)) {
+            float pa = ps->pdfa16;
+            add_emission(x, e, pa, pa * cos_hemisphere_pdfw(ldin.z), cos_in / pow2(ps->t16));
+        }
+    }
+    if (ps->path_len >= x->c->max_depth) return 0;
+    if (!m_can_eval(&m)) return 0;
+    if (!m_singular(&m, x->c->lambert) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) connect_to_light(x, &m, &sd, ldin);
+    float cp = minf(1, m_cont_prob(&m));
+    float rr[4];
+    rng_nextf(&ps->rng, rr);
+    if (rr[0] > cp) return 0;
+    float rb[4];
+    rng_nextf(&ps->rng, rb);
+    dsr_t smp = bsdf_sample(&m, x->c->lambert, v3(rb[0], rb[1], rb[2]), ldin);
+    smp.fwd *= cp; smp.rev *= cp;
+    ps->thr = mul(ps->thr, divf(smp.f, smp.fwd));
+    if (smp.rough == 0) ps->dirpdf16 = 0;
+    else ps->dirpdf16 = q16(smp.fwd);
+    ps->thr = mulf(ps->thr, shading_normal_correction(&sd, ldin, smp.dir));
+    f3 wd = to_world(&sd, smp.dir);
+    ps->dir = pack_normal(wd);
+    ps->origin = ray_offset_dir(sd.pos, unpack_normal(sd.gn), wd);
+    if (ps->thr.x != ps->thr.x || ps->thr.y != ps->thr.y || ps->thr.z != ps->thr.z) return 0;
+    return 1;
+}
+
+/* One launch (1 spp) for one pixel: Render + RenderIteration x (maxDepth-1)
+ * + ProcessShadowRays + ProcessAtomicOutput (testrenderer.slang:1087-1183). */
+static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* view, uint32_t px, uint32_t py,
+                         uint32_t seed, f3* result, uint32_t* plen) {
+    pstate_t* ps = &x->ps;
+    memset(ps, 0, sizeof *ps);
+    x->out = v3(0, 0, 0);
+    x->nee[0] = x->nee[1] = x->nee[2] = 0;
+    x->nsh = 0;
+    rng_init(&ps->rng, seed, px, py, 0);
+    float ox = 0.5f, oy = 0.5f;
+    if (x->c->flags & ST_FLAG_PIXEL_JITTER) {
+        float j[4];
+        rng_nextf(&ps->rng, j);
+        ox = j[0]; oy = j[1];
+    }
+    /* InitializeCameraPath, testrenderer.slang:938-982; ViewData::toWorld scene.h:152-157;
+     * ProjectionData::backProject transform.h:159-170 */
+    float ew = (float)(view->image_max[0] - view->image_min[0]), eh = (float)(view->image_max[1] - view->image_min[1]);
+    float u = (((float)px + ox) - (float)view->image_min[0]) / ew, v = (((float)py + oy) - (float)view->image_min[1]) / eh;
+    float cx = 2 * u - 1, cy = -(2 * v - 1);
+    const st_projection* pr = &view->projection;
+    float sg = pr->near_plane > 0 ? 1.0f : (pr->near_plane < 0 ? -1.0f : 0.0f);
+    f3 bp = v3(pr->near_plane * (cx * sg - pr->offset[0]) / pr->scale[0], pr->near_plane * (cy * sg - pr->offset[1]) / pr->scale[1],
+               pr->near_plane);
+    f3 ldir = normalize3(bp);
+    const st_transform* T = &view->camera_to_world;
+    ps->origin = v3(T->m[0][3], T->m[1][3], T->m[2][3]);
+    ps->path_len = 0;
+    ps->thr = v3(1, 1, 1);
+    ps->dir = pack_normal(normalize3(xf_vector(T, ldir)));
+    /* Render always extends once; RenderIteration returns early when the
+     * throughput is zero or the depth is reached (testrenderer.slang:1105, 1137-1141) */
+    for (int first = 1;; first = 0) {
+        if (!first && (all_le0(ps->thr) || ps->path_len + 1 > x->c->max_depth)) break;
+        if (!extend_path(x, p, view)) { ps->thr = v3(0, 0, 0); break; }
+    }
+    /* ProcessShadowRays + AddRadianceInterlocked, testrenderer.slang:237-261, 1147-1166 */
+    for (int i = 0; i < x->nsh; i++) {
+        shadow_t* sr = &x->sh[i];
+        hit_t h = trace(x->s, sr->o, sr->d, sr->tmax, 1, &x->trav[2], &x->trav[3]);
+        x->rays[1]++;
+        if (h.hit) continue;
+        if (sr->w <= 0 || all_le0(sr->c)) continue;
+        f3 rw = mulf(sr->c, sr->w);
+        float q[3] = {maxf(0, rw.x) * (float)OR_Q, maxf(0, rw.y) * (float)OR_Q, maxf(0, rw.z) * (float)OR_Q};
+        for (int k = 0; k < 3; k++) {
+            if (x->c->flags & ST_FLAG_FLOAT_NEE) continue;
+            int32_t iv = q[k] >= 2147483647.0f ? INT_MAX : (int32_t)q[k];
+            x->nee[k] += (uint32_t)iv;
+        }
+        if (x->c->flags & ST_FLAG_FLOAT_NEE) x->out = add(x->out, v3(maxf(0, rw.x), maxf(0, rw.y), maxf(0, rw.z)));
+    }
+    /* ProcessAtomicOutput, testrenderer.slang:1173-1183 */
+    result->x = x->out.x + (float)x->nee[0] / (float)OR_Q;
+    result->y = x->out.y + (float)x->nee[1] / (float)OR_Q;
+    result->z = x->out.z + (float)x->nee[2] / (float)OR_Q;
+    *plen = ps->path_len;
+}
+
+int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
+    if (!s || !view || !p || !out) return fail("null argument");
+    if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
+    if (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_ENV_ALIAS)) return fail("mode not supported by this oracle");
+    cfg_t c;
+    c.flags = p->flags;
+    c.env_prob = p->environment_sample_probability;
+    c.max_depth = p->max_depth;
+    c.min_len = p->min_path_length;
+    c.cand = p->candidate_samples;
+    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    const uint32_t W = p->width, H = p->height;
+#pragma omp parallel
+    {
+        pctx_t x;
+        memset(&x, 0, sizeof x);
+        x.s = s;
+        x.c = &c;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t pix = 0; pix < (int64_t)W * H; pix++) {
+            uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
+            f3 sum = v3(0, 0, 0);
+            for (uint32_t k = 0; k < p->spp; k++) {
+                f3 r;
+                uint32_t pl;
+                render_pixel(&x, p, view, px, py, p->seed + k, &r, &pl);
+                if (k == 0 && out->visibility) { out->visibility[2 * pix] = x.vis[0]; out->visibility[2 * pix + 1] = x.vis[1]; }
+                if (out->path_length) out->path_length[(size_t)k * W * H + pix] = pl;
+                sum = add(sum, r);
+            }
+            if (out->hdr) {
+                out->hdr[4 * pix + 0] = sum.x / (float)p->spp;
+                out->hdr[4 * pix + 1] = sum.y / (float)p->spp;
+                out->hdr[4 * pix + 2] = sum.z / (float)p->spp;
+                out->hdr[4 * pix + 3] = 1;
+            }
+        }
+#pragma omp critical
+        {
+            rays[0] += x.rays[0]; rays[1] += x.rays[1];
+            for (int k = 0; k < 4; k++) trav[k] += x.trav[k];
+        }
+    }
+    if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
+    if (out->traversal) for (int k = 0; k < 4; k++) out->traversal[k] = trav[k];
+    return 0;
+}

@@ -1,0 +1,73 @@
+/*
+ * pt_oracle.h — CPU ORACLE (test infrastructure only).
+ *
+ * A plain-C restatement of Stratum2's TestRenderer path tracer
+ * (src/Shaders/kernels/testrenderer.slang and the headers it includes) used
+ * as the parity checker of the HIP implementation.  Only tests, the smoke
+ * entry and the bench's CPU-baseline leg may load it; the product never does.
+ *
+ * It consumes the same scene buffers as the C ABI (include/stratum2.h), with
+ * its own triangle flattening, its own median-split BVH and its own
+ * watertight triangle test, so that it shares no code with the product.
+ */
+#ifndef PT_ORACLE_H
+#define PT_ORACLE_H
+
+#include "../include/stratum2.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_scene or_scene;
+
+/* Per-pixel/per-sample debug streams filled by or_render (NULL to skip). */
+typedef struct or_outputs {
+    float* hdr;              /* RGBA32F W*H, mean over spp */
+    uint32_t* visibility;    /* RG32UI W*H (sample 0) */
+    uint32_t* path_length;   /* W*H*spp */
+    uint64_t* ray_counts;    /* [2]: extension, shadow */
+    uint64_t* traversal;     /* [4]: nodes visited (closest), tris tested (closest), nodes (shadow), tris (shadow) */
+} or_outputs;
+
+or_scene* or_scene_create(const st_scene_desc* desc);
+void or_scene_destroy(or_scene* s);
+const char* or_last_error(void);
+
+/* Render like st_render.  threads: OpenMP threads (0 = all). */
+int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out,
+              int threads);
+
+/* Closest-hit / any-hit of raw rays, same layout as st_trace_rays. */
+int or_trace_rays(or_scene* s, const float* rays, uint32_t count, int closest, uint32_t* hits);
+
+/* Brute force closest hit in FP64 over every triangle (independent check of
+ * the watertight test + BVH on tiny scenes). hits: {t (float bits), id, 0, 0}. */
+int or_trace_rays_bruteforce_f64(or_scene* s, const float* rays, uint32_t count, uint32_t* hits);
+
+/* Bit-exact helper exports (for golden vectors). */
+void or_pcg4d(const uint32_t in[4], uint32_t out[4]);
+uint32_t or_xxhash32(uint32_t p);
+uint32_t or_pcg(uint32_t v);
+uint32_t or_pack_normal(const float v[3]);
+void or_unpack_normal(uint32_t p, float out[3]);
+uint16_t or_f32_to_f16(float f);
+float or_f16_to_f32(uint16_t h);
+void or_ray_offset(const float p[3], const float n[3], float out[3]);
+
+/* Environment sampling restated (lights.hlsli:49-163): returns uv and pdf. */
+int or_env_sample_texel(or_scene* s, const float rnd[2], float uv[2], float* pdf);
+float or_env_sample_texel_pdf(or_scene* s, const float uv[2]);
+
+/* BSDF restated: evaluateReflectance / sampleDirection (disney.hlsli:391-429,
+ * lambertian.hlsli:13-39). mat = 4 packed words + emission scale bits.
+ * out_eval: {f.rgb, fwdPdf, revPdf}; out_sample: {dir.xyz, f.rgb, fwdPdf, revPdf, eta, roughness}. */
+void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], const float dout[3],
+                  float out_eval[5]);
+void or_bsdf_sample(const uint32_t mat[5], int lambertian, const float rnd[3], const float din[3],
+                    float out_sample[10]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,157 @@
+"""ctypes mirror of include/stratum2.h and the loader of the HIP library.
+
+The structures mirror the reference's scene buffer formats
+(src/Shaders/compat/scene.h:30-198, transform.h:7-191) and the TestRenderer
+push constants (src/Shaders/kernels/testrenderer.slang:1-21).  The loader
+raises when the built library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+u32, i32, f32, u64 = C.c_uint32, C.c_int32, C.c_float, C.c_uint64
+
+ST_OK = 0
+ST_INVALID_INSTANCE = 0xFFFF
+ST_NO_IMAGE = 0xFFFFFFFF
+
+ST_FLAG_LAMBERTIAN = 1 << 0
+ST_FLAG_SAMPLE_DIRECT = 1 << 1
+ST_FLAG_RESTIR_DI = 1 << 2
+ST_FLAG_RESTIR_DI_REUSE = 1 << 3
+ST_FLAG_ENV_ALIAS = 1 << 4
+ST_FLAG_ENV_MIS_BALANCED = 1 << 5
+ST_FLAG_FLOAT_NEE = 1 << 6
+ST_FLAG_PIXEL_JITTER = 1 << 7
+ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
+
+
+class st_instance(C.Structure):
+    _fields_ = [("type_material_address", u32), ("data", u32)]
+
+
+class st_transform(C.Structure):
+    _fields_ = [("m", (f32 * 4) * 3)]
+
+
+class st_mesh_vertex_info(C.Structure):
+    _fields_ = [("packed_buffer_indices", u32 * 2), ("packed_strides", u32), ("pad", u32),
+                ("packed_offsets", u32 * 4)]
+
+
+class st_volume_info(C.Structure):
+    _fields_ = [("min", f32 * 3), ("instance_index", u32), ("max", f32 * 3), ("pad", u32)]
+
+
+class st_buffer_view(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size_bytes", u64)]
+
+
+class st_image_desc(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("width", u32), ("height", u32)]
+
+
+class st_scene_desc(C.Structure):
+    _fields_ = [
+        ("instance_count", u32), ("instances", C.c_void_p), ("transforms", C.c_void_p),
+        ("material_word_count", u32), ("material_data", C.c_void_p),
+        ("mesh_vertex_info_count", u32), ("mesh_vertex_info", C.c_void_p),
+        ("vertex_buffer_count", u32), ("vertex_buffers", C.c_void_p),
+        ("image_count", u32), ("images", C.c_void_p),
+        ("environment_material_address", u32),
+        ("volume_count", u32), ("volumes", C.c_void_p),
+        ("volume_info_count", u32), ("volume_info", C.c_void_p),
+    ]
+
+
+class st_projection(C.Structure):
+    _fields_ = [("scale", f32 * 2), ("offset", f32 * 2), ("near_plane", f32), ("far_plane", f32),
+                ("sensor_area", f32), ("vertical_fov", f32)]
+
+
+class st_view(C.Structure):
+    _fields_ = [("projection", st_projection), ("image_min", i32 * 2), ("image_max", i32 * 2),
+                ("camera_to_world", st_transform)]
+
+
+class st_render_params(C.Structure):
+    _fields_ = [
+        ("width", u32), ("height", u32), ("max_depth", u32), ("min_path_length", u32),
+        ("seed", u32), ("spp", u32), ("max_null_collisions", u32),
+        ("environment_sample_probability", f32), ("candidate_samples", u32), ("max_m", f32),
+        ("flags", u32), ("rr_start_depth", u32), ("exposure", f32),
+        ("spatial_neighbours", u32), ("spatial_radius", f32), ("samples_per_wave", u32),
+    ]
+
+
+class st_outputs(C.Structure):
+    _fields_ = [("hdr", C.c_void_p), ("visibility", C.c_void_p), ("path_length", C.c_void_p),
+                ("ray_counts", C.c_void_p)]
+
+
+def default_params(width, height, **kw):
+    """TestRenderer defaults (src/App/TestRenderer.cpp:38-43)."""
+    p = st_render_params()
+    p.width, p.height = width, height
+    p.max_depth = 5
+    p.min_path_length = 0
+    p.seed = 0
+    p.spp = 1
+    p.max_null_collisions = 1000
+    p.environment_sample_probability = 0.9
+    p.candidate_samples = 32
+    p.max_m = 3.0
+    p.flags = ST_FLAG_DEFAULT
+    p.rr_start_depth = 0
+    p.exposure = 0.0
+    p.spatial_neighbours = 5
+    p.spatial_radius = 30.0
+    p.samples_per_wave = 0
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libstratum2.so")
+
+EXPORTS = [
+    "st_last_error", "st_version", "st_context_create", "st_context_destroy", "st_sync",
+    "st_scene_create", "st_scene_destroy", "st_scene_stats", "st_render", "st_trace_primary",
+    "st_trace_rays", "st_last_timings",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"stratum2 HIP library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.st_last_error.restype = C.c_char_p
+    lib.st_version.restype = C.c_int
+    lib.st_context_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.st_context_destroy.argtypes = [vp]
+    lib.st_sync.argtypes = [vp]
+    lib.st_scene_create.argtypes = [vp, C.POINTER(st_scene_desc), C.POINTER(vp)]
+    lib.st_scene_destroy.argtypes = [vp]
+    lib.st_scene_stats.argtypes = [vp, C.POINTER(u64)]
+    lib.st_render.argtypes = [vp, C.POINTER(st_view), C.POINTER(st_render_params), C.POINTER(st_outputs)]
+    lib.st_trace_primary.argtypes = [vp, C.POINTER(st_view), u32, u32, vp, vp]
+    lib.st_trace_rays.argtypes = [vp, vp, u32, C.c_int, vp]
+    lib.st_last_timings.argtypes = [vp, C.POINTER(C.c_double)]
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def check(lib, status, what):
+    if status != ST_OK:
+        msg = lib.st_last_error()
+        raise RuntimeError(f"{what} failed ({status}): {msg.decode() if msg else ''}")

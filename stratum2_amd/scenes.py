@@ -1,0 +1,471 @@
+"""Seeded synthetic scenes in the reference's scene-buffer formats.
+
+SceneBuilder packs meshes, materials and instances exactly like
+Scene::updateFrameData does (src/App/Scene.cpp:428-791): 64-byte material
+records (Material::store, src/App/Material.hpp:22-40), MeshVertexInfo records
+(src/Shaders/compat/scene.h:89-132), InstanceData words (scene.h:30-80) and a
+16-byte environment record (EnvironmentMap::store, src/App/Scene.hpp:42-45).
+The workloads (Cornell box with a displaced icosphere; 2000 displaced spheres
+with 4096 emissive triangles under a procedural sky) are generated from seeds.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+
+NO_BUFFER = 0xFFFF  # buffer index >= gVertexBufferCount (2048) means "absent"
+
+
+def unorm8(v):
+    """(uint)floor(saturate(v)*255 + 0.5f), src/Shaders/compat/material_data.h:51-64."""
+    v = np.float32(min(max(float(v), 0.0), 1.0))
+    return int(math.floor(np.float32(v * np.float32(255)) + np.float32(0.5)))
+
+
+def pack4(a, b, c, d):
+    return unorm8(a) | unorm8(b) << 8 | unorm8(c) << 16 | unorm8(d) << 24
+
+
+def material_record(base_color=(0.8, 0.8, 0.8), specular=0.5, emission=(0, 0, 0), sheen=0.0,
+                    metallic=0.0, roughness=0.5, anisotropic=0.0, subsurface=0.0, clearcoat=0.0,
+                    clearcoat_gloss=0.0, transmission=0.0, eta=1.5):
+    """64-byte Material::store record (Material.hpp:22-40) as 16 u32 words."""
+    e = np.asarray(emission, np.float32)
+    scale = np.float32(e.max())
+    en = e / scale if scale > 0 else np.zeros(3, np.float32)
+    w = np.zeros(16, np.uint32)
+    w[0] = pack4(*base_color, specular)
+    w[1] = pack4(en[0], en[1], en[2], sheen)
+    w[2] = pack4(metallic, roughness, anisotropic, subsurface)
+    w[3] = pack4(clearcoat, clearcoat_gloss, transmission, eta - 1.0)
+    w[4:8] = abi.ST_NO_IMAGE
+    w[8] = np.float32(scale).view(np.uint32)
+    w[9] = abi.ST_NO_IMAGE          # bump image
+    w[10] = np.float32(0).view(np.uint32)
+    w[11] = 0
+    w[12] = abi.ST_NO_IMAGE         # alpha mask
+    w[13] = np.float32(0.5).view(np.uint32)
+    return w
+
+
+class SceneData:
+    """Host copy of one scene's buffers, convertible to st_scene_desc."""
+
+    def __init__(self):
+        self.instances = []      # (type_material_address, data)
+        self.transforms = []     # 3x4
+        self.material_words = []
+        self.vertex_infos = []
+        self.buffers = []        # bytes objects
+        self.images = []         # float32 HxWx4
+        self.env_address = 0xFFFFFFFF
+        self.volumes = []
+        self.volume_info = []
+        self._keep = []
+
+    # -- building -------------------------------------------------------
+    def add_material(self, words):
+        addr = 4 * sum(len(w) for w in self.material_words)
+        self.material_words.append(np.asarray(words, np.uint32))
+        return addr
+
+    def add_buffer(self, arr):
+        self.buffers.append(np.ascontiguousarray(arr).tobytes())
+        return len(self.buffers) - 1
+
+    def add_mesh(self, positions, indices, normals=None):
+        """One mesh in its own position/index buffers (u32 indices, f32x3 positions)."""
+        positions = np.ascontiguousarray(positions, np.float32)
+        indices = np.ascontiguousarray(indices, np.uint32)
+        ib = self.add_buffer(indices)
+        pb = self.add_buffer(positions)
+        nb = self.add_buffer(np.ascontiguousarray(normals, np.float32)) if normals is not None else NO_BUFFER
+        vi = abi.st_mesh_vertex_info()
+        vi.packed_buffer_indices[0] = ib | pb << 16
+        vi.packed_buffer_indices[1] = nb | NO_BUFFER << 16
+        vi.packed_strides = 4 | 12 << 8 | (12 if normals is not None else 0) << 16
+        vi.packed_offsets[0] = 0
+        vi.packed_offsets[1] = 0
+        vi.packed_offsets[2] = 0
+        vi.packed_offsets[3] = 0
+        self.vertex_infos.append(vi)
+        return len(self.vertex_infos) - 1, len(indices) // 3
+
+    def add_mesh_shared(self, ib, pb, index_offset_bytes, position_offset_bytes, tri_count):
+        vi = abi.st_mesh_vertex_info()
+        vi.packed_buffer_indices[0] = ib | pb << 16
+        vi.packed_buffer_indices[1] = NO_BUFFER | NO_BUFFER << 16
+        vi.packed_strides = 4 | 12 << 8
+        vi.packed_offsets[0] = index_offset_bytes
+        vi.packed_offsets[1] = position_offset_bytes
+        self.vertex_infos.append(vi)
+        return len(self.vertex_infos) - 1, tri_count
+
+    def add_mesh_instance(self, vertex_info, tri_count, material_address, transform=None):
+        if tri_count > 0xFFFF or vertex_info > 0xFFFF:
+            raise ValueError("16-bit primitive/vertex-info limit (scene.h:46-58)")
+        self.instances.append((0 | (material_address << 4), vertex_info | tri_count << 16))
+        self.transforms.append(np.eye(3, 4, dtype=np.float32) if transform is None else np.asarray(transform, np.float32))
+        return len(self.instances) - 1
+
+    def set_environment(self, emission=(1, 1, 1), image=None):
+        img = abi.ST_NO_IMAGE
+        if image is not None:
+            self.images.append(np.ascontiguousarray(image, np.float32))
+            img = len(self.images) - 1
+        e = np.asarray(emission, np.float32).view(np.uint32)
+        self.env_address = self.add_material(np.array([e[0], e[1], e[2], img], np.uint32))
+
+    # -- export ---------------------------------------------------------
+    def desc(self):
+        """st_scene_desc pointing into arrays kept alive by this object."""
+        keep = []
+        n = len(self.instances)
+        inst = (abi.st_instance * max(n, 1))()
+        for i, (a, b) in enumerate(self.instances):
+            inst[i].type_material_address, inst[i].data = a, b
+        xf = np.ascontiguousarray(np.stack(self.transforms) if n else np.zeros((1, 3, 4)), np.float32)
+        mats = np.ascontiguousarray(np.concatenate(self.material_words) if self.material_words else np.zeros(1), np.uint32)
+        vis = (abi.st_mesh_vertex_info * max(len(self.vertex_infos), 1))(*self.vertex_infos)
+        bufs = (abi.st_buffer_view * max(len(self.buffers), 1))()
+        for i, b in enumerate(self.buffers):
+            cb = C.create_string_buffer(b, len(b))
+            keep.append(cb)
+            bufs[i].data = C.cast(cb, C.c_void_p)
+            bufs[i].size_bytes = len(b)
+        imgs = (abi.st_image_desc * max(len(self.images), 1))()
+        for i, im in enumerate(self.images):
+            imgs[i].rgba = im.ctypes.data
+            imgs[i].height, imgs[i].width = im.shape[0], im.shape[1]
+        vols = (abi.st_buffer_view * max(len(self.volumes), 1))()
+        for i, v in enumerate(self.volumes):
+            vols[i].data = v.ctypes.data
+            vols[i].size_bytes = v.nbytes
+        vinf = (abi.st_volume_info * max(len(self.volume_info), 1))(*self.volume_info)
+        d = abi.st_scene_desc()
+        d.instance_count = n
+        d.instances = C.cast(inst, C.c_void_p)
+        d.transforms = xf.ctypes.data
+        d.material_word_count = mats.size if self.material_words else 0
+        d.material_data = mats.ctypes.data
+        d.mesh_vertex_info_count = len(self.vertex_infos)
+        d.mesh_vertex_info = C.cast(vis, C.c_void_p)
+        d.vertex_buffer_count = len(self.buffers)
+        d.vertex_buffers = C.cast(bufs, C.c_void_p)
+        d.image_count = len(self.images)
+        d.images = C.cast(imgs, C.c_void_p)
+        d.environment_material_address = self.env_address
+        d.volume_count = len(self.volumes)
+        d.volumes = C.cast(vols, C.c_void_p)
+        d.volume_info_count = len(self.volume_info)
+        d.volume_info = C.cast(vinf, C.c_void_p)
+        keep += [inst, xf, mats, vis, bufs, imgs, vols, vinf]
+        self._keep = keep
+        return d
+
+    @property
+    def triangle_count(self):
+        return sum(b >> 16 for a, b in self.instances if (a & 0xF) == 0)
+
+
+# ---------------------------------------------------------------- camera
+def quat_to_transform(pos, q):
+    """TransformData(t, quatf, 1), src/Shaders/compat/transform.h:73-104 (q = x, y, z, w)."""
+    x, y, z, w = [np.float32(v) for v in q]
+    sqw, sqx, sqy, sqz = w * w, x * x, y * y, z * z
+    invs = np.float32(1) / (sqx + sqy + sqz + sqw)
+    m = np.zeros((3, 4), np.float32)
+    m[0, 0] = (sqx - sqy - sqz + sqw) * invs
+    m[1, 1] = (-sqx + sqy - sqz + sqw) * invs
+    m[2, 2] = (-sqx - sqy + sqz + sqw) * invs
+    t1, t2 = x * z, y * w
+    m[2, 0] = 2 * (t1 + t2) * invs
+    m[0, 2] = 2 * (t1 - t2) * invs
+    t1, t2 = x * y, z * w
+    m[1, 0] = 2 * (t1 - t2) * invs
+    m[0, 1] = 2 * (t1 + t2) * invs
+    t1, t2 = z * y, x * w
+    m[1, 2] = 2 * (t1 + t2) * invs
+    m[2, 1] = 2 * (t1 - t2) * invs
+    m[:, 3] = pos
+    return m
+
+
+def make_view(width, height, position=(0, 1.5, 0), orientation=(0, 0, 0, 1), fovy_deg=70.0):
+    """Camera of src/main.cpp:104-116 with the aspect fix of main.cpp:219-224
+    (ProjectionData::makePerspective, transform.h:181-190, near = -0.001)."""
+    v = abi.st_view()
+    fovy = np.float32(fovy_deg) * np.float32(math.pi) / np.float32(180)
+    aspect = np.float32(height) / np.float32(width)
+    s1 = np.float32(1) / np.float32(math.tan(float(fovy) / 2))
+    v.projection.scale[1] = s1
+    v.projection.scale[0] = aspect * s1
+    v.projection.offset[0] = v.projection.offset[1] = 0
+    v.projection.near_plane = -0.001
+    v.projection.far_plane = -10000.0
+    v.projection.sensor_area = 0
+    v.projection.vertical_fov = fovy
+    v.image_min[0] = v.image_min[1] = 0
+    v.image_max[0], v.image_max[1] = width, height
+    m = quat_to_transform(position, orientation)
+    for r in range(3):
+        for c in range(4):
+            v.camera_to_world.m[r][c] = float(m[r, c])
+    return v
+
+
+def axis_angle_quat(axis, deg):
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    h = math.radians(deg) / 2
+    return (a[0] * math.sin(h), a[1] * math.sin(h), a[2] * math.sin(h), math.cos(h))
+
+
+# ------------------------------------------------------------ geometry
+def quad(p0, p1, p2, p3):
+    """Two triangles (p0,p1,p2), (p0,p2,p3); normal = cross(v0-v2, v1-v2) side."""
+    return np.array([p0, p1, p2, p3], np.float32), np.array([0, 1, 2, 0, 2, 3], np.uint32)
+
+
+def icosphere(subdiv):
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = list(f)
+    for _ in range(subdiv):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[k] = len(verts) - 1
+            return cache[k]
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts), np.array(faces, np.int64)
+
+
+def seeded_noise(rng, pts, octaves=4):
+    """Sum of seeded random sinusoids, normalised to [-1, 1]."""
+    acc = np.zeros(len(pts))
+    norm = 0.0
+    for o in range(octaves):
+        freq = 2.0 ** o * 2.0
+        amp = 0.5 ** o
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        ph = rng.uniform(0, 2 * math.pi)
+        acc += amp * np.sin(freq * pts @ d * math.pi + ph)
+        norm += amp
+    return acc / norm
+
+
+def _orient_outward(verts, faces, center):
+    """Make cross(v0-v2, v1-v2) point away from center (the reference's geometric normal)."""
+    v0, v1, v2 = verts[faces[:, 0]], verts[faces[:, 1]], verts[faces[:, 2]]
+    n = np.cross(v0 - v2, v1 - v2)
+    c = (v0 + v1 + v2) / 3 - center
+    flip = (n * c).sum(1) < 0
+    faces = faces.copy()
+    faces[flip] = faces[flip][:, [0, 2, 1]]
+    return faces
+
+
+def _tri_normal_ok(verts, faces, want):
+    v0, v1, v2 = verts[faces[:, 0]], verts[faces[:, 1]], verts[faces[:, 2]]
+    n = np.cross(v0 - v2, v1 - v2)
+    return faces if (n @ np.asarray(want)).sum() > 0 else faces[:, [0, 2, 1]]
+
+
+def _random_material(rng, kind=None):
+    """Config material: albedo U(0.2,0.9)^3, roughness U(0.1,0.8); 'lambert' is
+    Disney {metallic 0, specular 0}, 'ggx' is Disney {metallic 1}."""
+    albedo = rng.uniform(0.2, 0.9, 3)
+    rough = rng.uniform(0.1, 0.8)
+    if kind is None:
+        kind = "lambert" if rng.uniform() < 0.5 else "ggx"
+    if kind == "lambert":
+        return material_record(base_color=albedo, specular=0.0, metallic=0.0, roughness=rough)
+    return material_record(base_color=albedo, specular=0.0, metallic=1.0, roughness=rough)
+
+
+def cornell_box(seed=0, light_radiance=15.0):
+    """Seeded Cornell box: 12 axis-aligned quads (5 walls, ceiling emitter,
+    6-sided block) + an icosphere (subdivision 4, 5120 triangles) displaced by
+    seeded noise of amplitude 0.05.  Camera at (0, 1, 2.4) looking down -z."""
+    rng = np.random.default_rng(seed)
+    s = SceneData()
+    center = np.array([0, 1, 0], np.float64)
+    walls = [  # (quad corners), all facing the box interior
+        [(-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)],    # floor
+        [(-1, 2, -1), (-1, 2, 1), (1, 2, 1), (1, 2, -1)],    # ceiling
+        [(-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)],  # back
+        [(-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)],  # left
+        [(1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1)],      # right
+    ]
+    for w in walls:
+        p, i = quad(*w)
+        f = i.reshape(-1, 3)
+        c = p.mean(0)
+        f = _tri_normal_ok(p, f, center - c)
+        vi, n = s.add_mesh(p, f.reshape(-1))
+        s.add_mesh_instance(vi, n, s.add_material(_random_material(rng)))
+    # ceiling emitter, facing down
+    p, i = quad((-0.3, 1.98, -0.3), (0.3, 1.98, -0.3), (0.3, 1.98, 0.3), (-0.3, 1.98, 0.3))
+    f = _tri_normal_ok(p, i.reshape(-1, 3), (0, -1, 0))
+    vi, n = s.add_mesh(p, f.reshape(-1))
+    s.add_mesh_instance(vi, n, s.add_material(material_record(base_color=(0.8, 0.8, 0.8), specular=0, roughness=1.0,
+                                                                emission=(light_radiance,) * 3)))
+    # block: 6 quads, outward
+    lo, hi = np.array([-0.75, 0.0, -0.6]), np.array([-0.15, 0.7, 0.0])
+    bc = (lo + hi) / 2
+    corners = lambda a, b, c, d: [tuple(lo + (hi - lo) * np.array(k)) for k in (a, b, c, d)]
+    faces6 = [corners((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)), corners((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)),
+              corners((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)), corners((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1)),
+              corners((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)), corners((0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))]
+    bp, bi = [], []
+    for q in faces6:
+        p, i = quad(*q)
+        f = _tri_normal_ok(p, i.reshape(-1, 3), p.mean(0) - bc)
+        bi.append(f + len(bp) * 4)
+        bp.append(p)
+    vi, n = s.add_mesh(np.concatenate(bp), np.concatenate(bi).reshape(-1))
+    s.add_mesh_instance(vi, n, s.add_material(_random_material(rng)))
+    # displaced icosphere
+    v, f = icosphere(4)
+    v = v * (1 + 0.05 / 0.4 * seeded_noise(rng, v))[:, None]
+    v = v * 0.4 + np.array([0.45, 0.45, -0.2])
+    f = _orient_outward(v, f, np.array([0.45, 0.45, -0.2]))
+    vi, n = s.add_mesh(v.astype(np.float32), f.reshape(-1).astype(np.uint32))
+    s.add_mesh_instance(vi, n, s.add_material(_random_material(rng)))
+    return s
+
+
+def cornell_view(width=256, height=256):
+    return make_view(width, height, position=(0.0, 1.0, 2.4))
+
+
+def sky_environment(width=1024, height=512, sun_elevation_deg=35.0, sun_azimuth_deg=60.0, sun_peak=1e4,
+                    sun_sigma_deg=1.0):
+    """Procedural RGBA32F HDR sky: gradient + a Gaussian sun lobe of peak sun_peak.
+    Pixel (x, y) is the direction of sphericalUvToCartesian(((x+.5)/W, (y+.5)/H))
+    (src/Shaders/compat/common.h:145-150)."""
+    u = (np.arange(width) + 0.5) / width
+    v = (np.arange(height) + 0.5) / height
+    uu, vv = np.meshgrid(u, v)
+    phi = (uu * 2 - 1) * math.pi
+    th = vv * math.pi
+    d = np.stack([np.sin(th) * np.cos(phi), np.cos(th), np.sin(th) * np.sin(phi)], -1)
+    up = np.clip(d[..., 1], 0, 1)
+    zen = np.array([0.25, 0.45, 0.9])
+    hor = np.array([0.9, 0.85, 0.75])
+    gnd = np.array([0.12, 0.1, 0.08])
+    sky = hor[None, None] * (1 - up[..., None] ** 0.5) + zen[None, None] * up[..., None] ** 0.5
+    col = np.where(d[..., 1:2] >= 0, sky, gnd[None, None] * (0.5 + 0.5 * (1 + d[..., 1:2])))
+    el, az = math.radians(sun_elevation_deg), math.radians(sun_azimuth_deg)
+    sd = np.array([math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)])
+    ang = np.arccos(np.clip(d @ sd, -1, 1))
+    sun = sun_peak * np.exp(-0.5 * (ang / math.radians(sun_sigma_deg)) ** 2)
+    col = col + sun[..., None] * np.array([1.0, 0.95, 0.85])[None, None]
+    img = np.ones((height, width, 4), np.float32)
+    img[..., :3] = col
+    return img
+
+
+def uv_sphere(segments=25, bands=11):
+    """Closed UV sphere with pole caps: 2*segments + 2*segments*(bands-2) triangles
+    (500 for 25 x 11)."""
+    lat = np.linspace(0, math.pi, bands + 1)[1:-1]
+    lon = np.arange(segments) * 2 * math.pi / segments
+    ring = np.stack([np.outer(np.sin(lat), np.cos(lon)), np.repeat(np.cos(lat)[:, None], segments, 1),
+                     np.outer(np.sin(lat), np.sin(lon))], -1).reshape(-1, 3)
+    verts = np.concatenate([[[0, 1, 0]], ring, [[0, -1, 0]]])
+    nr = bands - 1
+    faces = []
+    for j in range(segments):
+        faces.append((0, 1 + (j + 1) % segments, 1 + j))
+    for r in range(nr - 1):
+        for j in range(segments):
+            a, b = 1 + r * segments + j, 1 + r * segments + (j + 1) % segments
+            c, d = a + segments, b + segments
+            faces += [(a, b, d), (a, d, c)]
+    last = len(verts) - 1
+    base = 1 + (nr - 1) * segments
+    for j in range(segments):
+        faces.append((last, base + j, base + (j + 1) % segments))
+    return verts, np.array(faces, np.int64)
+
+
+def spheres_field(seed=1, n_spheres=2000, n_emitters=256, tris_per_emitter=16, env=True):
+    """1M triangles: n_spheres seeded displaced UV spheres (500 triangles each)
+    on a ground plane, n_emitters emissive instances of tris_per_emitter
+    triangles each (radiance lognormal(mu=1, sigma=1)), procedural sky."""
+    rng = np.random.default_rng(seed)
+    s = SceneData()
+    base_v, base_f = uv_sphere()
+    assert len(base_f) == 500
+    nx = int(math.ceil(math.sqrt(n_spheres * 1.25)))
+    nz = int(math.ceil(n_spheres / nx))
+    spacing = 1.6
+    allp, alli, metas = [], [], []
+    voff, ioff = 0, 0
+    for k in range(n_spheres):
+        gx, gz = k % nx, k // nx
+        r = rng.uniform(0.35, 0.6)
+        c = np.array([(gx - nx / 2) * spacing + rng.uniform(-0.3, 0.3), r,
+                      (gz - nz / 2) * spacing + rng.uniform(-0.3, 0.3)])
+        v = base_v * (1 + 0.15 * seeded_noise(rng, base_v, 3))[:, None] * r
+        f = _orient_outward(v, base_f, np.zeros(3))
+        allp.append(v.astype(np.float32))
+        alli.append(f.astype(np.uint32).reshape(-1))
+        metas.append((c, voff * 12, ioff * 4, len(f)))
+        voff += len(v)
+        ioff += f.size
+    pb = s.add_buffer(np.concatenate(allp))
+    ib = s.add_buffer(np.concatenate(alli))
+    for c, po, io, nt in metas:
+        vi, n = s.add_mesh_shared(ib, pb, io, po, nt)
+        xf = np.eye(3, 4, dtype=np.float32)
+        xf[:, 3] = c
+        s.add_mesh_instance(vi, n, s.add_material(_random_material(rng)), xf)
+    ext = max(nx, nz) * spacing
+    p, i = quad((-ext, 0, -ext), (ext, 0, -ext), (ext, 0, ext), (-ext, 0, ext))
+    f = _tri_normal_ok(p, i.reshape(-1, 3), (0, 1, 0))
+    vi, n = s.add_mesh(p, f.reshape(-1))
+    s.add_mesh_instance(vi, n, s.add_material(material_record(base_color=(0.5, 0.5, 0.5), specular=0, roughness=0.9)))
+    # emitters: flat 4x2 grids of quads (16 triangles) facing down, above the field
+    gx = np.linspace(-0.4, 0.4, 5)
+    gz = np.linspace(-0.2, 0.2, 3)
+    gp = np.array([(x, 0, z) for z in gz for x in gx], np.float32)
+    gf = []
+    for zz in range(2):
+        for xx in range(4):
+            a = zz * 5 + xx
+            gf += [(a, a + 1, a + 6), (a, a + 6, a + 5)]
+    gf = _tri_normal_ok(gp, np.array(gf), (0, -1, 0))
+    assert len(gf) == tris_per_emitter
+    evi, en = s.add_mesh(gp, gf.reshape(-1))
+    for k in range(n_emitters):
+        L = float(np.exp(rng.normal(1.0, 1.0)))
+        xf = np.eye(3, 4, dtype=np.float32)
+        xf[:, 3] = (rng.uniform(-ext * 0.8, ext * 0.8), rng.uniform(2.5, 4.0), rng.uniform(-ext * 0.8, ext * 0.8))
+        s.add_mesh_instance(evi, en, s.add_material(material_record(base_color=(0.8, 0.8, 0.8), specular=0,
+                                                                      roughness=1.0, emission=(L, L, L))), xf)
+    if env:
+        s.set_environment((1, 1, 1), sky_environment())
+    return s
+
+
+def spheres_view(width=1920, height=1080):
+    return make_view(width, height, position=(0.0, 7.0, 40.0), orientation=axis_angle_quat((1, 0, 0), -12.0))
