@@ -133,6 +133,7 @@ typedef struct st_view {
 #define ST_FLAG_ENV_MIS_BALANCED        (1u << 5)  /* env hits weighted against env pdfW */
 #define ST_FLAG_FLOAT_NEE               (1u << 6)  /* NEE added in fp32, not 1/16384 fixed point */
 #define ST_FLAG_PIXEL_JITTER            (1u << 7)  /* gPixelJitter */
+#define ST_FLAG_COUNT_TRAVERSAL         (1u << 8)  /* count nodes/triangles visited (gCountRays-like) */
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface
@@ -197,9 +198,14 @@ int st_trace_primary(st_scene* scene, const st_view* view, uint32_t width, uint3
 int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closest,
                   uint32_t* hits);
 
-/* Duration in ms of the dominant kernel over the last st_render, and the
- * traversal statistics needed for the algorithmic-bytes roofline. */
-int st_last_timings(st_context* ctx, double out[8]);
+/* Device time of the last st_render per kernel class (HIP events on the
+ * context stream) and traversal statistics for the algorithmic-bytes roofline:
+ * out[0..2]  ms in closest-hit trace, shade, shadow trace
+ * out[3..5]  launches of each class
+ * out[6..7]  nodes visited / triangles tested by closest-hit rays (COUNT_TRAVERSAL)
+ * out[8..9]  nodes visited / triangles tested by shadow rays (COUNT_TRAVERSAL)
+ * out[10..11] closest-hit rays, shadow rays traced */
+int st_last_timings(st_context* ctx, double out[12]);
 
 #ifdef __cplusplus
 }

@@ -504,11 +504,11 @@ static f3 disney_eval(const material_t* m, f3 din, f3 dout) {
             float Fss = (1 + (Fss90 - 1) * sno) * (1 + (Fss90 - 1) * sni);
             float ss = 1.25f * (Fss * (1 / (ndout + ndin) - 0.5f) + 0.5f);
             f3 bc = m_base(m);
-            c = add(c, mulf(mulf(bc, (1 - M_TRANS(m)) * (1 - metallic)), (lerp1(based, ss, M_SUBSURF(m)) / OR_PI) * ndout));
+            c = add(c, mulf(mulf(mulf(bc, (1 - M_TRANS(m)) * (1 - metallic)), lerp1(based, ss, M_SUBSURF(m)) / OR_PI), ndout));
             float lum = luminance(bc);
             f3 ctint = lum > 0 ? divf(bc, lum) : v3(1, 1, 1);
             f3 csheen = lerp3(v3(1, 1, 1), ctint, 0.5f);
-            c = add(c, mulf(csheen, (1 - metallic) * M_SHEEN(m) * sho * ndout));
+            c = add(c, mulf(mulf(mulf(csheen, (1 - metallic) * M_SHEEN(m)), sho), ndout));
             if (ndin > 0 && hdout > 0 && ndh > 0) {
                 float eta = M_ETA1(m);
                 float f0 = (eta - 1) * (eta - 1) / ((eta + 1) * (eta + 1));
@@ -517,7 +517,7 @@ static f3 disney_eval(const material_t* m, f3 din, f3 dout) {
                 f3 F = schlick3(c0, hdout);
                 float D = GTR2a(h, ax, ay), G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
                 float sw = 1 - M_TRANS(m) * (1 - metallic);
-                c = add(c, divf(mulf(mulf(F, sw), D * G), 4 * ndin));
+                c = add(c, divf(mulf(mulf(mulf(F, sw), D), G), 4 * ndin));
             }
             if (ndin > 0 && ndh > 0) {
                 float ac = lerp1(0.1f, 0.001f, M_CCGLOSS(m));

@@ -22,6 +22,7 @@ ST_FLAG_ENV_ALIAS = 1 << 4
 ST_FLAG_ENV_MIS_BALANCED = 1 << 5
 ST_FLAG_FLOAT_NEE = 1 << 6
 ST_FLAG_PIXEL_JITTER = 1 << 7
+ST_FLAG_COUNT_TRAVERSAL = 1 << 8
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 

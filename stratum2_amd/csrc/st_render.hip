@@ -1,0 +1,934 @@
+// st_render.hip — the wavefront bounce loop on gfx950 and the C ABI.
+//
+// The reference runs one thread per pixel for every bounce and keeps a 64-byte
+// PathState per pixel (testrenderer.slang:1087-1144, gMultiDispatch).  Here a
+// wave of W*H*k paths (k samples at once) lives in SoA arrays in HBM and only
+// live paths are re-launched: each bounce is
+//   trace   — closest hit of every queued ray                   (k_trace_closest)
+//   shade   — emission MIS, NEE sampling, RR, BSDF sampling;
+//             appends shadow rays and surviving paths to compacted
+//             queues with one atomic per wavefront (ballot+popcount)  (k_shade)
+//   shadow  — any-hit of the appended shadow rays, fixed-point NEE  (k_trace_shadow)
+// and a final resolve sums each pixel's samples in sample order (k_resolve).
+// All kernels are persistent grid-stride loops whose item counts are read on
+// the device, so the host never waits inside a frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/stratum2.h"
+#include "st_bvh.h"
+#include "st_layout.h"
+#include "st_shade.h"
+#include "st_trace.h"
+
+using namespace st;
+
+namespace {
+
+thread_local std::string g_error;
+
+int set_error(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(x)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (x);                                                                               \
+        if (e_ != hipSuccess)                                                                              \
+            throw HipError(std::string(#x) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" +        \
+                           std::to_string(__LINE__) + ")");                                                \
+    } while (0)
+
+constexpr int kTraceBlock = 128;
+constexpr int kShadeBlock = 256;
+
+// ----------------------------------------------------------------- device state
+struct WaveState {
+    float4* ray;        // per path: o.xyz, packed direction bits
+    float4* thr;        // per path: throughput.xyz, unused
+    uint4* st;          // per path: rng counter, path length, t16|pdfA16<<16, dirPdf16
+    float4* em;         // per path: emission accumulated in fp32 (AddRadiance)
+    uint4* nee;         // per path: NEE in 1/16384 fixed point (AddRadianceInterlocked)
+    uint4* hit;         // per queue slot: t, id, b1, b2
+    uint32_t* queue[2]; // live path ids, ping-pong
+    float4* shadow;     // per shadow record: {o, tmax}, {d, path}, {contrib, weight}
+    uint32_t* counters; // [0..1] queue counts (ping-pong), [2] shadow count, [4..5] u64 rays
+};
+
+struct RenderConsts {
+    uint32_t W, H, npix, wave_paths, k, sample_base, seed, max_depth, min_len, flags, rr_start;
+    float env_prob;
+    float vx, vy;  // unused
+    st_view view;
+};
+
+// -------------------------------------------------------------------- kernels
+__global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveState w) {
+    const uint32_t n = c.wave_paths;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t pix = p % c.npix, k = p / c.npix;
+        const uint32_t px = pix % c.W, py = pix / c.W;
+        Rng rng{px, py, c.seed + c.sample_base + k, 0};
+        float ox = 0.5f, oy = 0.5f;
+        if (c.flags & ST_FLAG_PIXEL_JITTER) {
+            float4 j = rng.nextf();
+            ox = j.x;
+            oy = j.y;
+        }
+        // InitializeCameraPath, testrenderer.slang:938-982; ViewData::toWorld,
+        // compat/scene.h:152-157; ProjectionData::backProject, transform.h:159-170
+        const st_view& V = c.view;
+        const float ew = (float)(V.image_max[0] - V.image_min[0]), eh = (float)(V.image_max[1] - V.image_min[1]);
+        const float u = (((float)px + ox) - (float)V.image_min[0]) / ew;
+        const float v = (((float)py + oy) - (float)V.image_min[1]) / eh;
+        const float cx = 2 * u - 1, cy = -(2 * v - 1);
+        const st_projection& pr = V.projection;
+        const float sg = pr.near_plane > 0 ? 1.0f : (pr.near_plane < 0 ? -1.0f : 0.0f);
+        const v3 bp = mk(pr.near_plane * (cx * sg - pr.offset[0]) / pr.scale[0],
+                         pr.near_plane * (cy * sg - pr.offset[1]) / pr.scale[1], pr.near_plane);
+        const v3 ld = normalize(bp);
+        const st_transform& T = V.camera_to_world;
+        const uint32_t dir = pack_normal(normalize(xf_vector(T, ld)));
+        w.ray[p] = make_float4(T.m[0][3], T.m[1][3], T.m[2][3], __uint_as_float(dir));
+        w.thr[p] = make_float4(1, 1, 1, 0);
+        w.st[p] = make_uint4(rng.w, 0, 0, 0);
+        w.em[p] = make_float4(0, 0, 0, 0);
+        w.nee[p] = make_uint4(0, 0, 0, 0);
+        w.queue[0][p] = p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.counters[0] = n;
+        w.counters[1] = 0;
+        w.counters[2] = 0;
+    }
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_closest(DevScene S, WaveState w, int qsel, uint32_t* spill,
+                                                               unsigned long long* stats) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t n = w.counters[qsel];
+    const uint32_t* q = w.queue[qsel];
+    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraverseStats ts{0, 0};
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const uint32_t p = q[i];
+        const float4 r = w.ray[p];
+        const v3 d = unpack_normal(__float_as_uint(r.w));
+        const Hit h = traverse<false, COUNT>(S, mk(r.x, r.y, r.z), d, __int_as_float(0x7f800000), lds, spill, stride, gtid, ts);
+        w.hit[i] = make_uint4(__float_as_uint(h.t), h.hit ? h.id : 0xffffffffu, __float_as_uint(h.b1), __float_as_uint(h.b2));
+    }
+    if (COUNT) {
+        atomicAdd(&stats[0], (unsigned long long)ts.nodes);
+        atomicAdd(&stats[1], (unsigned long long)ts.tris);
+    }
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_shadow(DevScene S, WaveState w, uint32_t flags, uint32_t* spill,
+                                                              unsigned long long* stats) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t n = w.counters[2];
+    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraverseStats ts{0, 0};
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const float4 a = w.shadow[3 * (size_t)i], b = w.shadow[3 * (size_t)i + 1], c = w.shadow[3 * (size_t)i + 2];
+        const Hit h = traverse<true, COUNT>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, lds, spill, stride, gtid, ts);
+        if (h.hit) continue;
+        // AddRadianceInterlocked, testrenderer.slang:237-261
+        const float wgt = c.w;
+        const v3 ct = mk(c.x, c.y, c.z);
+        if (wgt <= 0 || all_nonpos(ct)) continue;
+        const v3 rw = ct * wgt;
+        const uint32_t p = __float_as_uint(b.w);
+        if (flags & ST_FLAG_FLOAT_NEE) {
+            float4 e = w.em[p];
+            e.x += fmax_(0.0f, rw.x);
+            e.y += fmax_(0.0f, rw.y);
+            e.z += fmax_(0.0f, rw.z);
+            w.em[p] = e;
+            continue;
+        }
+        const float q0 = fmax_(0.0f, rw.x) * (float)kQuant, q1 = fmax_(0.0f, rw.y) * (float)kQuant,
+                    q2 = fmax_(0.0f, rw.z) * (float)kQuant;
+        // one shadow ray per path per bounce: a plain read-modify-write is race free
+        uint4 acc = w.nee[p];
+        acc.x += (uint32_t)(q0 >= 2147483647.0f ? 2147483647 : (int32_t)q0);
+        acc.y += (uint32_t)(q1 >= 2147483647.0f ? 2147483647 : (int32_t)q1);
+        acc.z += (uint32_t)(q2 >= 2147483647.0f ? 2147483647 : (int32_t)q2);
+        w.nee[p] = acc;
+    }
+    if (COUNT) {
+        atomicAdd(&stats[2], (unsigned long long)ts.nodes);
+        atomicAdd(&stats[3], (unsigned long long)ts.tris);
+    }
+}
+
+// Wavefront-aggregated append: one atomic per wave, lane order preserved.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+    const unsigned long long mask = __ballot(want);
+    const uint32_t lane = __lane_id();
+    const uint32_t cnt = (uint32_t)__popcll(mask);
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (cnt && (int)lane == leader) base = atomicAdd(counter, cnt);
+    base = __shfl(base, leader < 0 ? 0 : leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+template <bool LAMBERT, bool ALIAS>
+__global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
+                                                       uint32_t* visibility, int first_wave) {
+    const uint32_t n = w.counters[qsel];
+    const uint32_t* q = w.queue[qsel];
+    uint32_t* qout = w.queue[qsel ^ 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // every lane of a wave must reach the ballots: iterate to a wave-uniform bound
+    const uint32_t base0 = blockIdx.x * blockDim.x + threadIdx.x - __lane_id();
+    for (uint32_t i0 = base0; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + __lane_id();
+        const bool valid = i < n;
+        bool emit_shadow = false, cont = false;
+        float4 sh0 = make_float4(0, 0, 0, 0), sh1 = sh0, sh2 = sh0;
+        uint32_t p = 0;
+        float4 ray_out = sh0, thr_out = sh0;
+        if (valid) {
+            p = q[i];
+            const uint4 hit = w.hit[i];
+            const float4 r = w.ray[p];
+            float4 thr4 = w.thr[p];
+            uint4 s4 = w.st[p];
+            const uint32_t pix = p % c.npix, k = p / c.npix;
+            Rng rng{pix % c.W, pix / c.W, c.seed + c.sample_base + k, s4.x};
+            uint32_t path_len = s4.y + 1;  // TraceRay, testrenderer.slang:1026-1058
+            const v3 dir = unpack_normal(__float_as_uint(r.w));
+            v3 thr = mk(thr4.x, thr4.y, thr4.z);
+            const bool hit_any = hit.y != 0xffffffffu;
+            float dirpdf = h2f(s4.w);
+            float4 em = make_float4(0, 0, 0, 0);
+            bool em_dirty = false;
+            // AddEmission, testrenderer.slang:788-808 (+ --minPathLength gate, vcm.slang:960)
+            auto add_emission = [&](v3 e, float dpa, float WtoA) {
+                float wgt = 1;
+                if (path_len > 1 && (c.flags & ST_FLAG_SAMPLE_DIRECT) && dirpdf > 0) wgt = mis2(dirpdf * WtoA, dpa);
+                if (path_len < c.min_len) return;
+                if (wgt <= 0 || all_nonpos(e)) return;
+                if (!em_dirty) {
+                    em = w.em[p];
+                    em_dirty = true;
+                }
+                const v3 a = e * wgt;
+                em.x += a.x;
+                em.y += a.y;
+                em.z += a.z;
+            };
+            if (!hit_any) {
+                if (path_len == 1 && first_wave && k == 0 && visibility) {
+                    visibility[2 * pix] = 0xffffffffu;
+                    visibility[2 * pix + 1] = pack_normal(dir);
+                }
+                float dpw;
+                const v3 e = thr * evaluate_environment<ALIAS>(S, c.env_prob, dir, dpw);
+                if (any_pos(e)) {
+                    float dpa = (1.0f / ST_PI) / sq(S.sphere[3]);  // concentricDiscPdfA()/R^2
+                    if (c.flags & ST_FLAG_ENV_MIS_BALANCED) dpa = dpw;
+                    add_emission(e, dpa, 1);
+                }
+                s4.y = path_len;
+            } else {
+                const uint32_t ii = hit.y & 0xffffu, prim = hit.y >> 16;
+                Shading sd;
+                tri_shading(S, ii, prim, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
+                const float pick = 1.0f / (float)(S.inst[ii].data >> 16);
+                float pdfa = 0;  // LightSamplePdfA, lights.hlsli:264-271
+                if (S.light_count > 0) {
+                    pdfa = pick / (sd.area * (float)S.light_count);
+                    if (S.has_env) pdfa *= 1 - c.env_prob;
+                }
+                const float t16 = q16(__uint_as_float(hit.x)), pdfa16 = q16(pdfa);
+                if (path_len == 1 && first_wave && k == 0 && visibility) {
+                    visibility[2 * pix] = hit.y;
+                    visibility[2 * pix + 1] = sd.sn;
+                }
+                const Material m = load_material(S, sd.material);
+                const Frame f = make_frame(sd);
+                const v3 ldin = f.to_local(-dir);
+                const v3 lgn = f.to_local(unpack_normal(sd.gn));
+                const float cos_in = fabsf(ldin.z);
+                if (ldin.z > 0) {
+                    const v3 e = thr * m.emission();
+                    if (any_pos(e)) add_emission(e, pdfa16, cos_in / sq(t16));
+                }
+                bool alive = path_len < c.max_depth && any_pos(m.base());
+                // Russian roulette from path length rr_start (0: every bounce, testrenderer.slang:890-892)
+                const bool rr_on = path_len >= c.rr_start;
+                const float cp_eff = rr_on ? m_cont_prob(m) : 1.0f;
+                if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
+                    // ConnectToLight, testrenderer.slang:751-779
+                    const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), sd.pos);
+                    if (!all_nonpos(L.radiance) && L.pdf > 0) {
+                        const float AtoW = L.finite ? sq(L.dist) / L.cos_light : 1.0f;
+                        const float dpw = L.pdf * AtoW;
+                        const v3 ldout = f.to_local(L.dir);
+                        Eval fe = bsdf_eval<LAMBERT>(m, ldin, ldout);
+                        fe.f = fe.f * shading_normal_correction(f, lgn, ldin, ldout);
+                        fe.fwd *= cp_eff;
+                        fe.rev *= cp_eff;
+                        const v3 contrib = ((thr * L.radiance) * fe.f) / dpw;
+                        const float wgt = mis2(L.pdf, fe.fwd / AtoW);
+                        if (path_len + 1 >= c.min_len) {
+                            const v3 o = ray_offset(sd.pos, unpack_normal(sd.gn), L.dir);
+                            const float tmax = fmax_(L.dist * 0.99f, L.dist - 1e-2f);  // AdjustIntersectionDistance
+                            sh0 = make_float4(o.x, o.y, o.z, tmax);
+                            sh1 = make_float4(L.dir.x, L.dir.y, L.dir.z, __uint_as_float(p));
+                            sh2 = make_float4(contrib.x, contrib.y, contrib.z, wgt);
+                            emit_shadow = true;
+                        }
+                    }
+                }
+                if (alive) {
+                    const float cp = fmin_(1.0f, m_cont_prob(m));
+                    const float rr = rng.nextf().x;
+                    if (rr_on && rr > cp) alive = false;
+                    if (alive) {
+                        const float cpx = rr_on ? cp : 1.0f;
+                        const float4 rb = rng.nextf();
+                        Sample s = bsdf_sample<LAMBERT>(m, rb.x, rb.y, rb.z, ldin);
+                        s.fwd *= cpx;
+                        s.rev *= cpx;
+                        thr = thr * (s.f / s.fwd);
+                        dirpdf = s.rough == 0 ? 0.0f : s.fwd;
+                        thr = thr * shading_normal_correction(f, lgn, ldin, s.dir);
+                        const v3 wd = f.to_world(s.dir);
+                        const uint32_t pd = pack_normal(wd);
+                        const v3 o = ray_offset(sd.pos, unpack_normal(sd.gn), wd);
+                        const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
+                        // the next RenderIteration skips zero-throughput paths (testrenderer.slang:1137)
+                        cont = !nan && !all_nonpos(thr) && path_len + 1 <= c.max_depth;
+                        ray_out = make_float4(o.x, o.y, o.z, __uint_as_float(pd));
+                        thr_out = make_float4(thr.x, thr.y, thr.z, 0);
+                        s4.w = f2h(dirpdf);
+                    }
+                }
+                s4.y = path_len;
+                s4.z = f2h(t16) | (f2h(pdfa16) << 16);
+            }
+            s4.x = rng.w;
+            w.st[p] = s4;
+            if (em_dirty) w.em[p] = em;
+            if (cont) {
+                w.ray[p] = ray_out;
+                w.thr[p] = thr_out;
+            }
+        }
+        const uint32_t si = wave_append(&w.counters[2], emit_shadow);
+        if (emit_shadow) {
+            w.shadow[3 * (size_t)si] = sh0;
+            w.shadow[3 * (size_t)si + 1] = sh1;
+            w.shadow[3 * (size_t)si + 2] = sh2;
+        }
+        const uint32_t qi = wave_append(&w.counters[qsel ^ 1], cont);
+        if (cont) qout[qi] = p;
+    }
+}
+
+// Sum each pixel's samples in sample order (ProcessAtomicOutput + running mean).
+__global__ void k_resolve(RenderConsts c, WaveState w, float4* acc, float* hdr, uint32_t* path_length, float inv_spp,
+                          int last) {
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < c.npix; pix += gridDim.x * blockDim.x) {
+        float4 a = acc[pix];
+        for (uint32_t k = 0; k < c.k; k++) {
+            const uint32_t p = k * c.npix + pix;
+            const float4 e = w.em[p];
+            const uint4 q = w.nee[p];
+            a.x += e.x + (float)q.x / (float)kQuant;
+            a.y += e.y + (float)q.y / (float)kQuant;
+            a.z += e.z + (float)q.z / (float)kQuant;
+            if (path_length) path_length[(size_t)(c.sample_base + k) * c.npix + pix] = w.st[p].y;
+        }
+        acc[pix] = a;
+        if (last) {
+            hdr[4 * (size_t)pix + 0] = a.x * inv_spp;
+            hdr[4 * (size_t)pix + 1] = a.y * inv_spp;
+            hdr[4 * (size_t)pix + 2] = a.z * inv_spp;
+            hdr[4 * (size_t)pix + 3] = 1.0f;
+        }
+    }
+}
+
+__global__ void k_count(WaveState w, int qsel) {
+    // rays traced this bounce: extension queue + shadow queue
+    unsigned long long* r = reinterpret_cast<unsigned long long*>(w.counters + 4);
+    r[0] += w.counters[qsel];
+    r[1] += w.counters[2];
+    w.counters[2] = 0;
+    w.counters[qsel] = 0;
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int closest,
+                                                            uint32_t* out, uint32_t* spill) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraverseStats ts{0, 0};
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+        if (closest) {
+            const Hit h = traverse<false, false>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, lds, spill, stride, gtid, ts);
+            reinterpret_cast<uint4*>(out)[i] =
+                make_uint4(__float_as_uint(h.t), h.hit ? h.id : 0xffffffffu, __float_as_uint(h.b1), __float_as_uint(h.b2));
+        } else {
+            const Hit h = traverse<true, false>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, lds, spill, stride, gtid, ts);
+            out[i] = h.hit ? 1u : 0u;
+        }
+    }
+}
+
+// -------------------------------------------------------------------- host side
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void reserve(size_t bytes) {
+        if (bytes <= n) return;
+        if (p) HIP_CHECK(hipFree(p));
+        p = nullptr;
+        HIP_CHECK(hipMalloc(&p, bytes));
+        n = bytes;
+    }
+    template <class T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+}  // namespace
+
+struct st_context {
+    int device = 0;
+    int cus = 256;
+    hipStream_t stream = nullptr;
+    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, acc, stats, raysbuf, outbuf;
+    double timings[12] = {0};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+    size_t ev_used[3] = {0, 0, 0};
+};
+
+struct st_scene {
+    st_context* ctx = nullptr;
+    DevScene d{};
+    std::vector<DevBuf*> bufs;
+    uint64_t stats[4] = {0, 0, 0, 0};
+    ~st_scene() {
+        for (auto* b : bufs) delete b;
+    }
+    template <class T>
+    T* upload(const void* src, size_t bytes) {
+        DevBuf* b = new DevBuf();
+        bufs.push_back(b);
+        b->reserve(std::max<size_t>(bytes, 16));
+        if (bytes) HIP_CHECK(hipMemcpy(b->p, src, bytes, hipMemcpyHostToDevice));
+        return b->as<T>();
+    }
+};
+
+namespace {
+
+hipEvent_t get_event(st_context* ctx, int kind, bool start) {
+    auto& v = ctx->ev[kind];
+    size_t i = ctx->ev_used[kind];
+    if (i >= v.size()) {
+        hipEvent_t a, b;
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+        v.push_back({a, b});
+    }
+    return start ? v[i].first : v[i].second;
+}
+
+struct Timed {
+    st_context* ctx;
+    int kind;
+    Timed(st_context* c, int k) : ctx(c), kind(k) { HIP_CHECK(hipEventRecord(get_event(c, k, true), c->stream)); }
+    ~Timed() {
+        (void)hipEventRecord(get_event(ctx, kind, false), ctx->stream);
+        ctx->ev_used[kind]++;
+    }
+};
+
+void validate_desc(const st_scene_desc* d) {
+    if (!d) throw std::invalid_argument("null scene description");
+    if (d->instance_count && (!d->instances || !d->transforms)) throw std::invalid_argument("instances/transforms missing");
+    if (d->instance_count > 0xFFFF) throw std::invalid_argument("more than 65535 instances (16-bit instance ids)");
+    for (uint32_t i = 0; i < d->instance_count; i++) {
+        const st_instance& in = d->instances[i];
+        const uint32_t type = in.type_material_address & 0xfu;
+        if (type != 0) throw std::domain_error("only mesh instances are supported");
+        const uint32_t addr = in.type_material_address >> 4;
+        if ((uint64_t)addr / 4 + 16 > d->material_word_count) throw std::invalid_argument("material address out of range");
+        for (int k = 0; k < 4; k++)
+            if (d->material_data[addr / 4 + 4 + k] < ST_IMAGE_LIMIT) throw std::domain_error("material textures are not supported");
+        if ((in.data & 0xffffu) >= d->mesh_vertex_info_count) throw std::invalid_argument("vertex info index out of range");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* st_last_error(void) { return g_error.c_str(); }
+int st_version(void) { return 1; }
+
+int st_context_create(int device, st_context** out) {
+    if (!out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null output");
+    try {
+        auto ctx = std::make_unique<st_context>();
+        ctx->device = device;
+        HIP_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device));
+        ctx->cus = prop.multiProcessorCount;
+        HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        *out = ctx.release();
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_context_destroy(st_context* ctx) {
+    if (!ctx) return ST_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& v : ctx->ev)
+        for (auto& e : v) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return ST_OK;
+}
+
+int st_sync(st_context* ctx) {
+    if (!ctx) return set_error(ST_ERROR_INVALID_ARGUMENT, "null context");
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return set_error(ST_ERROR_HIP, hipGetErrorString(e));
+    return ST_OK;
+}
+
+int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
+    if (!ctx || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    try {
+        validate_desc(d);
+        HIP_CHECK(hipSetDevice(ctx->device));
+        auto s = std::make_unique<st_scene>();
+        s->ctx = ctx;
+        DevScene& D = s->d;
+        // reference-format buffers
+        D.inst = s->upload<st_instance>(d->instances, sizeof(st_instance) * d->instance_count);
+        D.xf = s->upload<st_transform>(d->transforms, sizeof(st_transform) * d->instance_count);
+        D.mat = s->upload<uint32_t>(d->material_data, 4ull * d->material_word_count + 64);
+        D.vi = s->upload<st_mesh_vertex_info>(d->mesh_vertex_info, sizeof(st_mesh_vertex_info) * d->mesh_vertex_info_count);
+        D.inst_count = d->instance_count;
+        D.mat_words = d->material_word_count;
+        D.vi_count = d->mesh_vertex_info_count;
+        std::vector<const uint8_t*> vbp(d->vertex_buffer_count + 1, nullptr);
+        std::vector<uint64_t> vbs(d->vertex_buffer_count + 1, 0);
+        for (uint32_t i = 0; i < d->vertex_buffer_count; i++) {
+            vbp[i] = s->upload<uint8_t>(d->vertex_buffers[i].data, d->vertex_buffers[i].size_bytes);
+            vbs[i] = d->vertex_buffers[i].size_bytes;
+        }
+        D.vb = s->upload<const uint8_t*>(vbp.data(), sizeof(void*) * vbp.size());
+        D.vb_size = s->upload<uint64_t>(vbs.data(), 8 * vbs.size());
+        D.vb_count = d->vertex_buffer_count;
+        // flatten mesh instances into world-space triangles; lights; scene AABB
+        auto u32at = [&](uint32_t buf, uint64_t off) {
+            if (buf >= d->vertex_buffer_count || off + 4 > d->vertex_buffers[buf].size_bytes)
+                throw std::invalid_argument("vertex/index buffer access out of range");
+            uint32_t v;
+            std::memcpy(&v, (const uint8_t*)d->vertex_buffers[buf].data + off, 4);
+            return v;
+        };
+        auto u16at = [&](uint32_t buf, uint64_t off) {
+            if (buf >= d->vertex_buffer_count || off + 2 > d->vertex_buffers[buf].size_bytes)
+                throw std::invalid_argument("index buffer access out of range");
+            uint16_t v;
+            std::memcpy(&v, (const uint8_t*)d->vertex_buffers[buf].data + off, 2);
+            return (uint32_t)v;
+        };
+        auto f3at = [&](uint32_t buf, uint64_t off, float o[3]) {
+            if (buf >= d->vertex_buffer_count || off + 12 > d->vertex_buffers[buf].size_bytes)
+                throw std::invalid_argument("position buffer access out of range");
+            std::memcpy(o, (const uint8_t*)d->vertex_buffers[buf].data + off, 12);
+        };
+        auto xfp = [](const st_transform& T, const float v[3], float o[3]) {
+            for (int r = 0; r < 3; r++) o[r] = T.m[r][0] * v[0] + T.m[r][1] * v[1] + T.m[r][2] * v[2] + T.m[r][3];
+        };
+        uint64_t ntri = 0;
+        for (uint32_t i = 0; i < d->instance_count; i++) ntri += d->instances[i].data >> 16;
+        std::vector<float> verts(9 * ntri + 9);
+        std::vector<uint32_t> ids(ntri + 1);
+        std::vector<uint32_t> lights;
+        float amin[3] = {INFINITY, INFINITY, INFINITY}, amax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint64_t t = 0;
+        for (uint32_t i = 0; i < d->instance_count; i++) {
+            const st_instance& in = d->instances[i];
+            const uint32_t addr = in.type_material_address >> 4;
+            const uint32_t ew = d->material_data[addr / 4 + 1];
+            if ((ew & 0xffu) || ((ew >> 8) & 0xffu) || ((ew >> 16) & 0xffu)) lights.push_back(i);  // Scene.cpp:487-491
+            const st_mesh_vertex_info& vi = d->mesh_vertex_info[in.data & 0xffffu];
+            const uint32_t pc = in.data >> 16;
+            const uint32_t ib = vi.packed_buffer_indices[0] & 0xffffu, pb = vi.packed_buffer_indices[0] >> 16;
+            const uint32_t is = vi.packed_strides & 0xffu, ps = (vi.packed_strides >> 8) & 0xffu;
+            if (is != 2 && is != 4) throw std::invalid_argument("index stride must be 2 or 4");
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t p = 0; p < pc; p++, t++) {
+                for (int j = 0; j < 3; j++) {
+                    const uint64_t io = (uint64_t)vi.packed_offsets[0] + ((uint64_t)p * 3 + j) * is;
+                    const uint32_t idx = is == 2 ? u16at(ib, io) : u32at(ib, io);
+                    float v[3];
+                    f3at(pb, (uint64_t)vi.packed_offsets[1] + (uint64_t)ps * idx, v);
+                    for (int k = 0; k < 3; k++) {
+                        lo[k] = std::min(lo[k], v[k]);
+                        hi[k] = std::max(hi[k], v[k]);
+                    }
+                    xfp(d->transforms[i], v, &verts[9 * t + 3 * j]);
+                }
+                ids[t] = i | (p << 16);
+            }
+            if (pc == 0) continue;
+            for (int cidx = 0; cidx < 8; cidx++) {  // transformed AABB corners, Scene.cpp:622-631
+                const float corner[3] = {(cidx % 2) == 0 ? lo[0] : hi[0], ((cidx % 4) / 2) == 0 ? lo[1] : hi[1],
+                                         (cidx / 4) == 0 ? lo[2] : hi[2]};
+                float w[3];
+                xfp(d->transforms[i], corner, w);
+                for (int k = 0; k < 3; k++) {
+                    amin[k] = std::min(amin[k], w[k]);
+                    amax[k] = std::max(amax[k], w[k]);
+                }
+            }
+        }
+        // scene bounding sphere, TestRenderer.cpp:458-461
+        float cen[3], dd = 0;
+        for (int k = 0; k < 3; k++) cen[k] = (amax[k] + amin[k]) / 2;
+        for (int k = 0; k < 3; k++) dd += (amax[k] - cen[k]) * (amax[k] - cen[k]);
+        D.sphere[0] = cen[0];
+        D.sphere[1] = cen[1];
+        D.sphere[2] = cen[2];
+        D.sphere[3] = std::sqrt(dd);
+        D.light_inst = s->upload<uint32_t>(lights.data(), 4 * lights.size());
+        D.light_count = (uint32_t)lights.size();
+        BvhOut bvh = build_bvh(verts.data(), ids.data(), (uint32_t)ntri);
+        D.nodes = s->upload<float4>(bvh.nodes.data(), bvh.nodes.size() * sizeof(NodeOut));
+        D.tris = s->upload<float4>(bvh.tris.data(), bvh.tris.size() * sizeof(TriOut));
+        D.node_count = (uint32_t)bvh.nodes.size();
+        D.tri_count = (uint32_t)bvh.tris.size();
+        s->stats[0] = D.node_count;
+        s->stats[1] = D.tri_count;
+        s->stats[2] = (uint64_t)bvh.depth;
+        s->stats[3] = bvh.nodes.size() * 64 + bvh.tris.size() * 48;
+        // environment: record (EnvironmentMap::store, Scene.hpp:42-45), mip chain, alias table
+        D.has_env = d->environment_material_address != 0xFFFFFFFFu;
+        if (D.has_env) {
+            const uint32_t a = d->environment_material_address / 4;
+            if ((uint64_t)a + 4 > d->material_word_count) throw std::invalid_argument("environment address out of range");
+            for (int k = 0; k < 3; k++) std::memcpy(&D.env_emission[k], &d->material_data[a + k], 4);
+            const uint32_t img = d->material_data[a + 3];
+            if (img < ST_IMAGE_LIMIT) {
+                if (img >= d->image_count) throw std::invalid_argument("environment image index out of range");
+                const st_image_desc& im = d->images[img];
+                if (im.width == 0 || im.height == 0 || !im.rgba) throw std::invalid_argument("empty environment image");
+                D.env_has_image = 1;
+                std::vector<std::vector<float>> lv(1);
+                std::vector<int> lw{(int)im.width}, lh{(int)im.height};
+                lv[0].assign(im.rgba, im.rgba + 4ull * im.width * im.height);
+                while (lw.back() > 1 || lh.back() > 1) {  // 2x2 box mips, Image.cpp:282-308
+                    const int w0 = lw.back(), h0 = lh.back(), w1 = w0 > 1 ? w0 / 2 : 1, h1 = h0 > 1 ? h0 / 2 : 1;
+                    std::vector<float> nx(4ull * w1 * h1);
+                    const std::vector<float>& src = lv.back();
+                    for (int y = 0; y < h1; y++)
+                        for (int x = 0; x < w1; x++) {
+                            const int x0 = w0 > 1 ? 2 * x : 0, x1 = w0 > 1 ? 2 * x + 1 : 0;
+                            const int y0 = h0 > 1 ? 2 * y : 0, y1 = h0 > 1 ? 2 * y + 1 : 0;
+                            for (int ch = 0; ch < 4; ch++) {
+                                auto T = [&](int xx, int yy) { return src[4 * ((size_t)yy * w0 + xx) + ch]; };
+                                nx[4 * ((size_t)y * w1 + x) + ch] = ((T(x0, y0) + T(x1, y0)) + (T(x0, y1) + T(x1, y1))) * 0.25f;
+                            }
+                        }
+                    lv.push_back(std::move(nx));
+                    lw.push_back(w1);
+                    lh.push_back(h1);
+                }
+                if ((int)lv.size() > kEnvMaxLevels) throw std::invalid_argument("environment image too large");
+                D.env_levels = (int)lv.size();
+                for (size_t l = 0; l < lv.size(); l++) {
+                    D.env_level[l] = s->upload<float4>(lv[l].data(), lv[l].size() * 4);
+                    D.env_w[l] = lw[l];
+                    D.env_h[l] = lh[l];
+                }
+                // alias table over luminance * sin(theta) of level 0 (Walker/Vose)
+                const int W = lw[0], H = lh[0];
+                const size_t N = (size_t)W * H;
+                std::vector<double> wgt(N);
+                double total = 0;
+                for (int y = 0; y < H; y++) {
+                    const double st = std::sin(M_PI * (y + 0.5) / H);
+                    for (int x = 0; x < W; x++) {
+                        const float* px = &lv[0][4 * ((size_t)y * W + x)];
+                        const double l = 0.2126 * px[0] + 0.7152 * px[1] + 0.0722 * px[2];
+                        wgt[(size_t)y * W + x] = std::max(0.0, l) * st;
+                        total += wgt[(size_t)y * W + x];
+                    }
+                }
+                std::vector<float2> alias(N);
+                std::vector<float> pdf(N);
+                if (total <= 0) {
+                    for (size_t i = 0; i < N; i++) wgt[i] = 1;
+                    total = (double)N;
+                }
+                std::vector<double> sc(N);
+                std::vector<uint32_t> small, large;
+                for (size_t i = 0; i < N; i++) {
+                    sc[i] = wgt[i] / total * N;
+                    pdf[i] = (float)(wgt[i] / total * N);  // uv-space density of texel i
+                    (sc[i] < 1 ? small : large).push_back((uint32_t)i);
+                }
+                while (!small.empty() && !large.empty()) {
+                    uint32_t sidx = small.back(), lidx = large.back();
+                    small.pop_back();
+                    alias[sidx] = make_float2((float)sc[sidx], 0);
+                    uint32_t al = lidx;
+                    std::memcpy(&alias[sidx].y, &al, 4);
+                    sc[lidx] = (sc[lidx] + sc[sidx]) - 1;
+                    if (sc[lidx] < 1) {
+                        large.pop_back();
+                        small.push_back(lidx);
+                    }
+                }
+                for (uint32_t i : large) {
+                    alias[i] = make_float2(1.0f, 0);
+                    std::memcpy(&alias[i].y, &i, 4);
+                }
+                for (uint32_t i : small) {
+                    alias[i] = make_float2(1.0f, 0);
+                    std::memcpy(&alias[i].y, &i, 4);
+                }
+                D.env_alias = s->upload<float2>(alias.data(), N * sizeof(float2));
+                D.env_pdf = s->upload<float>(pdf.data(), N * sizeof(float));
+            }
+        }
+        *out = s.release();
+        return ST_OK;
+    } catch (const std::domain_error& e) {
+        return set_error(ST_ERROR_UNSUPPORTED, e.what());
+    } catch (const std::invalid_argument& e) {
+        return set_error(ST_ERROR_INVALID_ARGUMENT, e.what());
+    } catch (const std::bad_alloc& e) {
+        return set_error(ST_ERROR_OUT_OF_MEMORY, e.what());
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_scene_destroy(st_scene* s) {
+    if (!s) return ST_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+    return ST_OK;
+}
+
+int st_scene_stats(const st_scene* s, uint64_t out[4]) {
+    if (!s || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 4; i++) out[i] = s->stats[i];
+    return ST_OK;
+}
+
+static void ensure_spill(st_context* ctx, int grid, int block) {
+    ctx->spill.reserve((size_t)(kMaxDepth - kLdsStack + 1) * grid * block * 4);
+}
+
+int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
+    if (!s || !view || !p || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
+    if (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE))
+        return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI is not available in this build");
+    if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
+    st_context* ctx = s->ctx;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        const uint32_t npix = p->width * p->height;
+        uint32_t k = p->samples_per_wave;
+        if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, (uint32_t)((4u << 20) / npix)));
+        k = std::min(k, p->spp);
+        const uint64_t paths = (uint64_t)npix * k;
+        if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
+        ctx->ray.reserve(paths * 16);
+        ctx->thr.reserve(paths * 16);
+        ctx->stt.reserve(paths * 16);
+        ctx->em.reserve(paths * 16);
+        ctx->nee.reserve(paths * 16);
+        ctx->hit.reserve(paths * 16);
+        ctx->q0.reserve(paths * 4);
+        ctx->q1.reserve(paths * 4);
+        ctx->shadow.reserve(paths * 48);
+        ctx->counters.reserve(256);
+        ctx->acc.reserve((size_t)npix * 16);
+        ctx->stats.reserve(64);
+        const int tgrid = ctx->cus * 8, sgrid = ctx->cus * 8;
+        ensure_spill(ctx, tgrid, kTraceBlock);
+        WaveState w;
+        w.ray = ctx->ray.as<float4>();
+        w.thr = ctx->thr.as<float4>();
+        w.st = ctx->stt.as<uint4>();
+        w.em = ctx->em.as<float4>();
+        w.nee = ctx->nee.as<uint4>();
+        w.hit = ctx->hit.as<uint4>();
+        w.queue[0] = ctx->q0.as<uint32_t>();
+        w.queue[1] = ctx->q1.as<uint32_t>();
+        w.shadow = ctx->shadow.as<float4>();
+        w.counters = ctx->counters.as<uint32_t>();
+        hipStream_t sm = ctx->stream;
+        HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 256, sm));
+        HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, sm));
+        HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, sm));
+        for (int kk = 0; kk < 3; kk++) ctx->ev_used[kk] = 0;
+        RenderConsts c;
+        std::memset(&c, 0, sizeof c);
+        c.W = p->width;
+        c.H = p->height;
+        c.npix = npix;
+        c.seed = p->seed;
+        c.max_depth = p->max_depth;
+        c.min_len = p->min_path_length;
+        c.flags = p->flags;
+        c.rr_start = p->rr_start_depth;
+        c.env_prob = p->environment_sample_probability;
+        c.view = *view;
+        const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+        const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
+        auto* stats = ctx->stats.as<unsigned long long>();
+        for (uint32_t base = 0; base < p->spp; base += k) {
+            c.k = std::min(k, p->spp - base);
+            c.sample_base = base;
+            c.wave_paths = npix * c.k;
+            const int cgrid = std::min<int>((c.wave_paths + kShadeBlock - 1) / kShadeBlock, ctx->cus * 16);
+            hipLaunchKernelGGL(k_camera, dim3(cgrid), dim3(kShadeBlock), 0, sm, c, w);
+            int qsel = 0;
+            for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
+                {
+                    Timed tm(ctx, 0);
+                    if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                }
+                {
+                    Timed tm(ctx, 1);
+                    uint32_t* vis = (base == 0) ? o->visibility : nullptr;
+                    if (lambert && alias) hipLaunchKernelGGL((k_shade<true, true>), dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0);
+                    else if (lambert) hipLaunchKernelGGL((k_shade<true, false>), dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0);
+                    else if (alias) hipLaunchKernelGGL((k_shade<false, true>), dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0);
+                    else hipLaunchKernelGGL((k_shade<false, false>), dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0);
+                }
+                {
+                    Timed tm(ctx, 2);
+                    if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
+                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
+                }
+                hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel);
+                qsel ^= 1;
+            }
+            const bool last = base + c.k >= p->spp;
+            const int rgrid = std::min<int>((npix + 255) / 256, ctx->cus * 8);
+            hipLaunchKernelGGL(k_resolve, dim3(rgrid), dim3(256), 0, sm, c, w, ctx->acc.as<float4>(), o->hdr,
+                               o->path_length, 1.0f / (float)p->spp, last ? 1 : 0);
+            HIP_CHECK(hipGetLastError());
+        }
+        if (o->ray_counts)
+            HIP_CHECK(hipMemcpyAsync(o->ray_counts, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToDevice, sm));
+        HIP_CHECK(hipStreamSynchronize(sm));
+        // per-kernel-class device time (HIP events on this stream)
+        for (int kind = 0; kind < 3; kind++) {
+            double ms = 0;
+            for (size_t i = 0; i < ctx->ev_used[kind]; i++) {
+                float t = 0;
+                HIP_CHECK(hipEventElapsedTime(&t, ctx->ev[kind][i].first, ctx->ev[kind][i].second));
+                ms += t;
+            }
+            ctx->timings[kind] = ms;
+            ctx->timings[3 + kind] = (double)ctx->ev_used[kind];
+        }
+        unsigned long long hs[4], rc[2];
+        HIP_CHECK(hipMemcpy(hs, ctx->stats.p, 32, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(rc, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToHost));
+        for (int i = 0; i < 4; i++) ctx->timings[6 + i] = (double)hs[i];
+        ctx->timings[10] = (double)rc[0];
+        ctx->timings[11] = (double)rc[1];
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_last_timings(st_context* ctx, double out[12]) {
+    if (!ctx || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 12; i++) out[i] = ctx->timings[i];
+    return ST_OK;
+}
+
+int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, uint32_t* hits) {
+    if (!s || (count && (!rays || !hits))) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    st_context* ctx = s->ctx;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        if (count == 0) return ST_OK;
+        const int grid = std::min<int>((count + kTraceBlock - 1) / kTraceBlock, ctx->cus * 8);
+        ensure_spill(ctx, ctx->cus * 8, kTraceBlock);
+        hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
+                           closest, hits, ctx->spill.as<uint32_t>());
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_trace_primary(st_scene* s, const st_view* view, uint32_t width, uint32_t height, uint32_t* visibility, float* t_out) {
+    (void)t_out;
+    if (!s || !view || !visibility) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    st_render_params p;
+    std::memset(&p, 0, sizeof p);
+    p.width = width;
+    p.height = height;
+    p.max_depth = 1;
+    p.spp = 1;
+    p.flags = 0;
+    p.environment_sample_probability = 0.9f;
+    st_context* ctx = s->ctx;
+    try {
+        ctx->outbuf.reserve((size_t)width * height * 16);
+        st_outputs o{ctx->outbuf.as<float>(), visibility, nullptr, nullptr};
+        return st_render(s, view, &p, &o);
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,362 @@
+// st_shade.h — scene access on the device: triangle shading frames, materials,
+// light and environment sampling.  Restates src/Shaders/common/scene.hlsli,
+// shading_data.hlsli, material.hlsli and lights.hlsli over the flat HBM copies
+// of the reference's scene buffers.
+#pragma once
+
+#include "st_device.h"
+
+namespace st {
+
+struct Shading {  // ShadingData fields used on camera paths (compat/scene.h:185-198)
+    v3 pos;
+    uint32_t material;  // byte address
+    uint32_t gn, sn, tan;
+    float area;
+};
+
+__device__ __forceinline__ v3 xf_point(const st_transform& T, v3 v) {
+    return mk(T.m[0][0] * v.x + T.m[0][1] * v.y + T.m[0][2] * v.z + T.m[0][3],
+              T.m[1][0] * v.x + T.m[1][1] * v.y + T.m[1][2] * v.z + T.m[1][3],
+              T.m[2][0] * v.x + T.m[2][1] * v.y + T.m[2][2] * v.z + T.m[2][3]);
+}
+__device__ __forceinline__ v3 xf_vector(const st_transform& T, v3 v) {
+    return mk(T.m[0][0] * v.x + T.m[0][1] * v.y + T.m[0][2] * v.z, T.m[1][0] * v.x + T.m[1][1] * v.y + T.m[1][2] * v.z,
+              T.m[2][0] * v.x + T.m[2][1] * v.y + T.m[2][2] * v.z);
+}
+
+__device__ __forceinline__ uint32_t ld_u32(const uint8_t* p) {
+    uint32_t v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+__device__ __forceinline__ v3 ld_v3(const uint8_t* p) {
+    float f[3];
+    __builtin_memcpy(f, p, 12);
+    return mk(f[0], f[1], f[2]);
+}
+
+// LoadTriangleIndices, scene.hlsli:43-61 (16- or 32-bit indices)
+__device__ __forceinline__ void load_tri(const DevScene& S, const st_mesh_vertex_info& vi, uint32_t prim, uint32_t tri[3]) {
+    const uint32_t ib = vi.packed_buffer_indices[0] & 0xffffu, stride = vi.packed_strides & 0xffu;
+    const uint8_t* b = S.vb[ib] + vi.packed_offsets[0] + (size_t)prim * 3 * stride;
+    if (stride == 2) {
+        uint16_t v[3];
+        __builtin_memcpy(v, b, 6);
+        tri[0] = v[0];
+        tri[1] = v[1];
+        tri[2] = v[2];
+    } else {
+        tri[0] = ld_u32(b);
+        tri[1] = ld_u32(b + 4);
+        tri[2] = ld_u32(b + 8);
+    }
+}
+
+// makeTriangleShadingData(WithoutPosition), shading_data.hlsli:59-151
+__device__ inline void tri_shading(const DevScene& S, uint32_t ii, uint32_t prim, float bx, float by, Shading& r) {
+    const st_instance in = S.inst[ii];
+    const st_transform T = S.xf[ii];
+    const st_mesh_vertex_info vi = S.vi[in.data & 0xffffu];
+    uint32_t tri[3];
+    load_tri(S, vi, prim, tri);
+    const uint32_t pb = vi.packed_buffer_indices[0] >> 16, ps = (vi.packed_strides >> 8) & 0xffu;
+    const uint8_t* P = S.vb[pb] + vi.packed_offsets[1];
+    const v3 v0 = ld_v3(P + (size_t)ps * tri[0]), v1 = ld_v3(P + (size_t)ps * tri[1]), v2 = ld_v3(P + (size_t)ps * tri[2]);
+    r.material = in.type_material_address >> 4;
+    const v3 dPds = xf_vector(T, v0 - v2), dPdt = xf_vector(T, v1 - v2);
+    v3 gn = cross(dPds, dPdt);
+    const float area2 = len(gn);
+    gn = gn / area2;
+    r.gn = pack_normal(gn);
+    r.area = area2 / 2;
+    float t0[2] = {0, 0}, t1[2] = {0, 0}, t2[2] = {0, 0};
+    const uint32_t tb = vi.packed_buffer_indices[1] >> 16, ts = vi.packed_strides >> 24;
+    if (tb < ST_IMAGE_LIMIT) {
+        const uint8_t* Tp = S.vb[tb] + vi.packed_offsets[3];
+        __builtin_memcpy(t0, Tp + (size_t)ts * tri[0], 8);
+        __builtin_memcpy(t1, Tp + (size_t)ts * tri[1], 8);
+        __builtin_memcpy(t2, Tp + (size_t)ts * tri[2], 8);
+    }
+    const float duvds0 = t2[0] - t0[0], duvds1 = t2[1] - t0[1], duvdt0 = t2[0] - t1[0], duvdt1 = t2[1] - t1[1];
+    const float det = duvds0 * duvdt1 - duvdt0 * duvds1;
+    v3 dPdu;
+    if (det != 0) {
+        const float invDet = 1 / det;
+        const float dsdu = duvdt1 * invDet, dtdu = -duvds1 * invDet;
+        dPdu = -(dPds * dsdu + dPdt * dtdu);
+    } else {
+        v3 o1;
+        make_onb(gn, dPdu, o1);
+    }
+    bool snvalid = false;
+    v3 sn = mk(0, 0, 0);
+    const uint32_t nb = vi.packed_buffer_indices[1] & 0xffffu, ns = (vi.packed_strides >> 16) & 0xffu;
+    if (nb < ST_IMAGE_LIMIT) {  // gShadingNormals is on by default (TestRenderer.cpp:22)
+        const uint8_t* N = S.vb[nb] + vi.packed_offsets[2];
+        const v3 n0 = ld_v3(N + (size_t)ns * tri[0]), n1 = ld_v3(N + (size_t)ns * tri[1]), n2 = ld_v3(N + (size_t)ns * tri[2]);
+        sn = (n0 + (n1 - n0) * bx) + (n2 - n0) * by;
+        snvalid = !((sn.x == 0 && sn.y == 0 && sn.z == 0) || sn.x != sn.x || sn.y != sn.y || sn.z != sn.z);
+    }
+    if (!snvalid) {
+        r.sn = r.gn;
+        r.tan = pack_normal(normalize(dPdu));
+    } else {
+        sn = normalize(xf_vector(T, sn));
+        const v3 tg = normalize(dPdu - sn * dot(sn, dPdu));
+        r.sn = pack_normal(sn);
+        r.tan = pack_normal(tg);
+        if (dot(sn, gn) < 0) r.gn = pack_normal(-gn);
+    }
+    r.pos = xf_point(T, (v0 + (v1 - v0) * bx) + (v2 - v0) * by);
+}
+
+// ShadingData::toLocal / toWorld / shadingNormalCorrection, shading_data.hlsli:24-54
+struct Frame {
+    v3 t, b, n;
+    __device__ __forceinline__ v3 to_local(v3 v) const { return mk(dot(v, t), dot(v, b), dot(v, n)); }
+    __device__ __forceinline__ v3 to_world(v3 v) const { return (t * v.x + b * v.y) + n * v.z; }
+};
+__device__ __forceinline__ Frame make_frame(const Shading& sd) {
+    Frame f;
+    f.n = unpack_normal(sd.sn);
+    f.t = unpack_normal(sd.tan);
+    f.b = cross(f.n, f.t);
+    return f;
+}
+__device__ __forceinline__ float sgn(float x) { return x > 0 ? 1.0f : (x < 0 ? -1.0f : 0.0f); }
+__device__ __forceinline__ float shading_normal_correction(const Frame& f, v3 lgn, v3 din, v3 dout) {
+    const float ngin = dot(lgn, din), ngout = dot(lgn, dout);
+    if (sgn(ngout) != sgn(dout.z) || sgn(ngin) != sgn(din.z)) return 0;
+    return 1;
+}
+
+// LoadMaterial, material.hlsli:15-21 (configurations carry no material textures)
+__device__ __forceinline__ Material load_material(const DevScene& S, uint32_t addr) {
+    Material m;
+    const uint4 w = *reinterpret_cast<const uint4*>(S.mat + addr / 4);
+    m.p[0] = w.x;
+    m.p[1] = w.y;
+    m.p[2] = w.z;
+    m.p[3] = w.w;
+    m.escale = __uint_as_float(S.mat[addr / 4 + 8]);
+    return m;
+}
+
+// ------------------------------------------------------------- environment
+__device__ __forceinline__ float texel_lum(const DevScene& S, int level, int x, int y) {
+    const float4 t = S.env_level[level][(size_t)y * S.env_w[level] + x];
+    return lum(mk(t.x, t.y, t.z));
+}
+// bilinear SampleLevel(uv, 0), repeat addressing (TestRenderer.cpp:54-57)
+__device__ inline v3 env_bilinear(const DevScene& S, float u, float v) {
+    const int w = S.env_w[0], h = S.env_h[0];
+    const float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y), ax = x - fx0, ay = y - fy0;
+    const int x0 = (int)fx0, y0 = (int)fy0;
+    const int xa = ((x0 % w) + w) % w, xb = (((x0 + 1) % w) + w) % w;
+    const int ya = ((y0 % h) + h) % h, yb = (((y0 + 1) % h) + h) % h;
+    const float4* L = S.env_level[0];
+    const float4 t00 = L[(size_t)ya * w + xa], t10 = L[(size_t)ya * w + xb], t01 = L[(size_t)yb * w + xa],
+                 t11 = L[(size_t)yb * w + xb];
+    const float r0 = t00.x * (1 - ax) + t10.x * ax, r1 = t01.x * (1 - ax) + t11.x * ax;
+    const float g0 = t00.y * (1 - ax) + t10.y * ax, g1 = t01.y * (1 - ax) + t11.y * ax;
+    const float b0 = t00.z * (1 - ax) + t10.z * ax, b1 = t01.z * (1 - ax) + t11.z * ax;
+    return mk(r0 * (1 - ay) + r1 * ay, g0 * (1 - ay) + g1 * ay, b0 * (1 - ay) + b1 * ay);
+}
+// SampleTexel, lights.hlsli:49-97 (hierarchical warp down the mip pyramid)
+__device__ inline void sample_texel(const DevScene& S, float rx, float ry, float& u, float& v, float& pdf) {
+    const int lc = S.env_levels;
+    int cx = 0, cy = 0, lw = 1, lh = 1;
+    pdf = 1;
+    const int iend = lc - 1 < 11 ? lc - 1 : 11;
+    for (int i = 1; i < iend; i++) {
+        const int L = lc - 1 - i, W = S.env_w[L], H = S.env_h[L];
+        const float inv_h = 1 / (float)H;
+        cx *= W / lw;
+        cy *= H / lh;
+        float p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+        if ((uint32_t)W - (uint32_t)cx > 1) {
+            const float sy = sinf(ST_PI * ((float)cy + 0.5f) * inv_h);
+            p0 = texel_lum(S, L, cx, cy) * sy;
+            p1 = texel_lum(S, L, cx + 1, cy) * sy;
+        }
+        if ((uint32_t)H - (uint32_t)cy > 1) {
+            const float sy = sinf(ST_PI * ((float)cy + 1.5f) * inv_h);
+            p2 = texel_lum(S, L, cx, cy + 1) * sy;
+            p3 = texel_lum(S, L, cx + 1, cy + 1) * sy;
+        }
+        const float sum = p0 + p1 + p2 + p3;
+        if (sum < 1e-6f) continue;
+        const float q[4] = {p0 / sum, p1 / sum, p2 / sum, p3 / sum};
+        for (int j = 0; j < 4; j++) {
+            if (j == 3 || rx < q[j]) {
+                cx += j % 2;
+                cy += j / 2;
+                pdf *= q[j];
+                rx /= q[j];
+                break;
+            }
+            rx -= q[j];
+        }
+        lw = W;
+        lh = H;
+    }
+    pdf *= (float)((uint32_t)lw * (uint32_t)lh);
+    u = ((float)cx + rx) / (float)lw;
+    v = ((float)cy + ry) / (float)lh;
+}
+// SampleTexelPdf, lights.hlsli:98-139
+__device__ inline float sample_texel_pdf(const DevScene& S, float u, float v) {
+    const int lc = S.env_levels;
+    int lw = 1, lh = 1;
+    float pdf = 1;
+    const int iend = lc - 1 < 11 ? lc - 1 : 11;
+    for (int i = 1; i < iend; i++) {
+        const int L = lc - 1 - i, W = S.env_w[L], H = S.env_h[L];
+        const float inv_h = 1 / (float)H;
+        const int ux = (int)((float)W * u), uy = (int)((float)H * v);
+        const int cx = (ux / 2) * 2, cy = (uy / 2) * 2;
+        const int ox = ux - cx < 1 ? ux - cx : 1, oy = uy - cy < 1 ? uy - cy : 1;
+        float p[4] = {0, 0, 0, 0};
+        if ((uint32_t)W - (uint32_t)cx > 1) {
+            const float sy = sinf(ST_PI * ((float)cy + 0.5f) * inv_h);
+            p[0] = texel_lum(S, L, cx, cy) * sy;
+            p[1] = texel_lum(S, L, cx + 1, cy) * sy;
+        }
+        if ((uint32_t)H - (uint32_t)cy > 1) {
+            const float sy = sinf(ST_PI * ((float)cy + 1.5f) * inv_h);
+            p[2] = texel_lum(S, L, cx, cy + 1) * sy;
+            p[3] = texel_lum(S, L, cx + 1, cy + 1) * sy;
+        }
+        const float sum = p[0] + p[1] + p[2] + p[3];
+        if (sum < 1e-6f) continue;
+        pdf *= p[oy * 2 + ox] / sum;
+        lw = W;
+        lh = H;
+    }
+    return pdf * (float)((uint32_t)lw * (uint32_t)lh);
+}
+// Alias-table environment sampling (ENV_ALIAS): one texel by Walker's method
+// over luminance*sin(theta) of the full-resolution map, uniform inside it.
+__device__ inline void sample_alias(const DevScene& S, float rx, float ry, float& u, float& v, float& pdf) {
+    const int W = S.env_w[0], H = S.env_h[0];
+    const uint32_t N = (uint32_t)W * (uint32_t)H;
+    const float x = rx * (float)N;
+    uint32_t i = (uint32_t)x;
+    if (i >= N) i = N - 1;
+    float f = x - (float)i;
+    const float2 e = S.env_alias[i];
+    float jx;
+    if (f < e.x) {
+        jx = f / e.x;
+    } else {
+        jx = (f - e.x) / (1.0f - e.x);
+        i = __float_as_uint(e.y);
+    }
+    jx = fmin_(jx, 0.99999994f);
+    pdf = S.env_pdf[i];
+    u = ((float)(i % (uint32_t)W) + jx) / (float)W;
+    v = ((float)(i / (uint32_t)W) + ry) / (float)H;
+}
+__device__ __forceinline__ float alias_pdf(const DevScene& S, float u, float v) {
+    const int W = S.env_w[0], H = S.env_h[0];
+    int x = (int)(u * (float)W), y = (int)(v * (float)H);
+    x = x < 0 ? 0 : (x >= W ? W - 1 : x);
+    y = y < 0 ? 0 : (y >= H ? H - 1 : y);
+    return S.env_pdf[(size_t)y * W + x];
+}
+
+// EvaluateEnvironment, lights.hlsli:143-163
+template <bool ALIAS>
+__device__ inline v3 evaluate_environment(const DevScene& S, float env_prob, v3 d, float& pdfw) {
+    if (!S.has_env) {
+        pdfw = 0;
+        return mk(0, 0, 0);
+    }
+    v3 e = mk(S.env_emission[0], S.env_emission[1], S.env_emission[2]);
+    if (S.env_has_image) {
+        float u, v;
+        cart_to_uv(d, u, v);
+        e = e * env_bilinear(S, u, v);
+        const float p = ALIAS ? alias_pdf(S, u, v) : sample_texel_pdf(S, u, v);
+        pdfw = p / (2 * ST_PI * ST_PI * sqrtf(1 - d.y * d.y));
+    } else {
+        pdfw = 1 / (4 * ST_PI);
+    }
+    if (S.light_count > 0) pdfw *= env_prob;
+    return e;
+}
+
+struct LightSample {  // IlluminationSampleRecord, lights.hlsli:32-45
+    v3 radiance, dir;
+    float pdf, dist, cos_light;
+    bool finite;
+};
+
+// SampleEmission + SampleIllumination, lights.hlsli:166-260.  The reference
+// leaves mDirectionToLight unset for environment samples (lights.hlsli:234-244);
+// here it is the sampled direction.
+template <bool ALIAS>
+__device__ inline LightSample sample_illumination(const DevScene& S, float env_prob, float4 rnd, v3 ref) {
+    LightSample r;
+    r.radiance = mk(0, 0, 0);
+    r.dir = mk(0, 0, 0);
+    r.pdf = 0;
+    r.dist = 0;
+    r.cos_light = 0;
+    r.finite = false;
+    if (S.has_env) {
+        if (S.light_count == 0 || rnd.w < env_prob) {
+            v3 d;
+            float pdf;
+            if (S.env_has_image) {
+                float u, v;
+                if (ALIAS) sample_alias(S, rnd.x, rnd.y, u, v, pdf);
+                else sample_texel(S, rnd.x, rnd.y, u, v, pdf);
+                d = uv_to_cart(u, v);
+                pdf /= (2 * ST_PI * ST_PI * sqrtf(1 - sq(d.y)));
+            } else {
+                d = sample_uniform_sphere(rnd.x, rnd.y);
+                pdf = 1 / (4 * ST_PI);
+            }
+            if (S.light_count > 0) pdf *= env_prob;
+            r.pdf = pdf;
+            r.dist = __int_as_float(0x7f800000);
+            r.cos_light = 1;
+            r.dir = d;
+            float tmp;
+            r.radiance = evaluate_environment<ALIAS>(S, env_prob, d, tmp);
+            return r;
+        }
+        if (S.light_count > 0) rnd.w = (rnd.w - env_prob) / (1 - env_prob);
+    }
+    if (S.light_count == 0) return r;
+    const uint32_t li = (uint32_t)(rnd.z * (float)S.light_count) % S.light_count;
+    const uint32_t ii = S.light_inst[li];
+    float pdf = 1 / (float)S.light_count;
+    if (S.has_env) pdf *= 1 - env_prob;
+    const st_instance in = S.inst[ii];
+    const uint32_t pc = in.data >> 16;
+    const uint32_t prim = (uint32_t)(rnd.w * (float)pc) % pc;
+    pdf /= (float)pc;
+    const float a = sqrtf(rnd.x);  // sampleUniformTriangle, common.hlsli:36-39
+    Shading sd;
+    tri_shading(S, ii, prim, 1 - a, a * rnd.y, sd);
+    pdf /= sd.area;
+    const Material m = load_material(S, sd.material);
+    r.radiance = m.emission();
+    r.pdf = pdf;
+    r.finite = true;
+    v3 dd = sd.pos - ref;
+    r.dist = len(dd);
+    r.dir = dd / r.dist;
+    r.cos_light = -dot(unpack_normal(sd.gn), r.dir);
+    if (r.cos_light <= 0) {  // lights are one-sided
+        r.radiance = mk(0, 0, 0);
+        r.pdf = 0;
+    }
+    return r;
+}
+
+}  // namespace st

@@ -1,0 +1,164 @@
+// st_trace.h — gfx950 BVH traversal with the watertight ray/triangle test.
+//
+// One lane per ray.  Each step loads one 64-byte node (two child boxes),
+// tests both boxes, intersects leaf children immediately and descends into
+// the nearer inner child, pushing the farther one.  The per-lane stack lives in
+// LDS (kLdsStack entries, lane-interleaved so a wave's pushes hit distinct
+// banks) and spills to an HBM region only past that depth.
+//
+// Closest-hit ties (equal t, e.g. a ray through a shared edge) resolve to the
+// smaller instance|primitive id, so the result does not depend on BVH shape or
+// visiting order.
+#pragma once
+
+#include "st_device.h"
+
+namespace st {
+
+struct Hit {
+    float t, b1, b2;
+    uint32_t id;
+    bool hit;
+};
+
+struct WRay {  // Woop-Benthin-Wald shear constants, JCGT 2(1) 2013
+    v3 o;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ WRay wray_make(v3 o, v3 d) {
+    WRay r;
+    r.o = o;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    r.kx = r.kz == 2 ? 0 : r.kz + 1;
+    r.ky = r.kx == 2 ? 0 : r.kx + 1;
+    if (comp(d, r.kz) < 0) {
+        int t = r.kx;
+        r.kx = r.ky;
+        r.ky = t;
+    }
+    float dz = comp(d, r.kz);
+    r.Sx = comp(d, r.kx) / dz;
+    r.Sy = comp(d, r.ky) / dz;
+    r.Sz = 1.0f / dz;
+    return r;
+}
+
+// Watertight test; t > 0 required; b1/b2 are the weights of v1/v2.
+__device__ __forceinline__ bool wt_intersect(const WRay& r, v3 v0, v3 v1, v3 v2, float& t, float& b1, float& b2) {
+    v3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
+    float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
+    float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    float rcp = 1.0f / det;
+    float tt = T * rcp;
+    if (!(tt > 0.0f)) return false;
+    t = tt;
+    b1 = V * rcp;
+    b2 = W * rcp;
+    return true;
+}
+
+// Conservative slab test: boxes are rounded outward by the builder and the
+// exit distance is widened by 2 ulp, so a box that contains a hit is entered.
+__device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz, v3 o, v3 inv, float tmax,
+                                     float& tn) {
+    float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+    float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+    float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+    float a = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
+    float b = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    tn = a;
+    return a <= b * 1.00000024f;
+}
+
+struct TraverseStats {
+    uint32_t nodes, tris;
+};
+
+// ANY: stop at the first hit in (0, tmax).  lds: this block's stack,
+// [kLdsStack][blockDim.x]; spill: HBM stack region, [kMaxDepth - kLdsStack][stride].
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ Hit traverse(const DevScene& S, v3 o, v3 d, float tmax, uint32_t* lds, uint32_t* spill,
+                                        uint32_t spill_stride, uint32_t gtid, TraverseStats& st) {
+    Hit h;
+    h.hit = false;
+    h.t = tmax;
+    h.b1 = h.b2 = 0;
+    h.id = 0xffffffffu;
+    if (S.node_count == 0) return h;
+    const WRay wr = wray_make(o, d);
+    const v3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const uint32_t lane = threadIdx.x, bdim = blockDim.x;
+    int sp = 0;
+    uint32_t node = 0;
+    while (true) {
+        const float4* np = S.nodes + 4 * (size_t)node;
+        const float4 a = np[0], b = np[1], c = np[2];
+        const int4 m = *reinterpret_cast<const int4*>(np + 3);
+        if (COUNT) st.nodes++;
+        float tn0, tn1;
+        bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, h.t, tn0) && m.z >= 0;
+        bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, h.t, tn1) && m.w >= 0;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const bool hs = side == 0 ? h0 : h1;
+            const int cnt = side == 0 ? m.z : m.w;
+            if (hs && cnt > 0) {
+                const int first = side == 0 ? m.x : m.y;
+                for (int k = 0; k < cnt; k++) {
+                    const float4* tp = S.tris + 3 * (size_t)(first + k);
+                    const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                    if (COUNT) st.tris++;
+                    float t, b1, b2;
+                    if (!wt_intersect(wr, mk(t0.x, t0.y, t0.z), mk(t1.x, t1.y, t1.z), mk(t2.x, t2.y, t2.z), t, b1, b2))
+                        continue;
+                    const uint32_t id = __float_as_uint(t0.w);
+                    if (t < h.t || (t == h.t && h.hit && id < h.id)) {
+                        h.t = t;
+                        h.b1 = b1;
+                        h.b2 = b2;
+                        h.id = id;
+                        h.hit = true;
+                        if (ANY) return h;
+                    }
+                }
+                if (side == 0) h0 = false;
+                else h1 = false;
+            }
+        }
+        if (h0 && h1) {
+            const bool first0 = tn0 <= tn1;
+            const uint32_t near = first0 ? (uint32_t)m.x : (uint32_t)m.y;
+            const uint32_t far = first0 ? (uint32_t)m.y : (uint32_t)m.x;
+            if (sp < kLdsStack) lds[sp * bdim + lane] = far;
+            else spill[(size_t)(sp - kLdsStack) * spill_stride + gtid] = far;
+            sp++;
+            node = near;
+        } else if (h0) {
+            node = (uint32_t)m.x;
+        } else if (h1) {
+            node = (uint32_t)m.y;
+        } else {
+            if (sp == 0) break;
+            sp--;
+            node = sp < kLdsStack ? lds[sp * bdim + lane] : spill[(size_t)(sp - kLdsStack) * spill_stride + gtid];
+        }
+    }
+    return h;
+}
+
+}  // namespace st

@@ -328,7 +328,7 @@ def cornell_box(seed=0, light_radiance=15.0):
     s.add_mesh_instance(vi, n, s.add_material(material_record(base_color=(0.8, 0.8, 0.8), specular=0, roughness=1.0,
                                                                 emission=(light_radiance,) * 3)))
     # block: 6 quads, outward
-    lo, hi = np.array([-0.75, 0.0, -0.6]), np.array([-0.15, 0.7, 0.0])
+    lo, hi = np.array([-0.75, 0.002, -0.6]), np.array([-0.15, 0.7, 0.0])
     bc = (lo + hi) / 2
     corners = lambda a, b, c, d: [tuple(lo + (hi - lo) * np.array(k)) for k in (a, b, c, d)]
     faces6 = [corners((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)), corners((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)),
