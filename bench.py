@@ -134,7 +134,7 @@ def cpu_baseline(sd, view, args, log):
     from stratum2_amd import abi
     from oracle.oracle import OracleScene
     o = OracleScene(sd)
-    threads = os.cpu_count() or 1
+    threads = min(16, os.cpu_count() or 1)  # this host's share of cores
     spp = 1
     while True:
         p = abi.default_params(args.width, args.height, max_depth=8, spp=spp)

@@ -1292,6 +1292,65 @@ static void connect_to_light(pctx_t* x, const material_t* m, const shading_t* sd
     sr->w = w;
 }
 
+/* ReservoirSampleDataDI, testrenderer.slang:568-606 */
+typedef struct {
+    float smp[4];
+    float src_pdf, mis_w, wtoa;
+    f3 contrib, vo, vd;
+    float vtmax;
+} rsd_t;
+static float rsd_target(const rsd_t* r) { return luminance(r->contrib) * r->wtoa; }
+static void rsd_init(rsd_t* r, pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin, const float rnd[4]) {
+    memset(r, 0, sizeof *r);
+    memcpy(r->smp, rnd, sizeof r->smp);
+    isr_t L = sample_illumination(x->s, x->c, rnd, sd->pos);
+    r->src_pdf = L.pdf;
+    if (luminance(L.radiance) > 0 && L.pdf > 0) {
+        f3 ldout = to_local(sd, L.dir);
+        rer_t f = bsdf_eval(m, x->c->lambert, ldin, ldout);
+        f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
+        float cp = m_cont_prob(m);
+        f.fwd *= cp; f.rev *= cp;
+        r->contrib = mul(f.f, L.radiance);
+        r->wtoa = L.finite ? L.cos_light / pow2(L.dist) : 1;
+        float AtoW = L.finite ? pow2(L.dist) / L.cos_light : 1;
+        r->mis_w = mis2(L.pdf, f.fwd / AtoW); /* ComputeMIS_DI, testrenderer.slang:530-566 */
+        r->vo = ray_offset_dir(sd->pos, unpack_normal(sd->gn), L.dir);
+        r->vd = L.dir;
+        r->vtmax = maxf(L.dist * 0.99f, L.dist - 1e-2f);
+    }
+}
+/* SampleIlluminationReSTIR (initial RIS, no reuse) + ConnectToLight (ReSTIR),
+ * testrenderer.slang:665-684, 730-747 */
+static void connect_to_light_restir(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
+    rsd_t sample, xi;
+    memset(&sample, 0, sizeof sample);
+    float wsum = 0;
+    for (uint32_t i = 0; i < x->c->cand; i++) {
+        float rnd[4];
+        rng_nextf(&x->ps.rng, rnd);
+        rsd_init(&xi, x, m, sd, ldin, rnd);
+        if (rsd_target(&xi) <= 0) continue;
+        wsum += rsd_target(&xi) / xi.src_pdf;
+        float u[4];
+        rng_nextf(&x->ps.rng, u);
+        if (u[0] <= (rsd_target(&xi) / xi.src_pdf) / wsum) sample = xi;
+    }
+    float W = 0;
+    if (rsd_target(&sample) > 0) W = wsum / ((float)x->c->cand * rsd_target(&sample));
+    if (W <= 0) return;
+    W *= sample.wtoa;
+    f3 contrib = mulf(mul(x->ps.thr, sample.contrib), W);
+    if (x->ps.path_len + 1 < x->c->min_len) return;
+    if (x->nsh >= 64) return;
+    shadow_t* sr = &x->sh[x->nsh++];
+    sr->o = sample.vo;
+    sr->d = sample.vd;
+    sr->tmax = sample.vtmax;
+    sr->c = contrib;
+    sr->w = sample.mis_w;
+}
+
 /* TraceRay + ExtendPath + ProcessInteraction, testrenderer.slang:816-935, 1026-1071.
  * Returns 0 when the path ends. */
 static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view) {
@@ -1349,7 +1408,10 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     }
     if (ps->path_len >= x->c->max_depth) return 0;
     if (!m_can_eval(&m)) return 0;
-    if (!m_singular(&m, x->c->lambert) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) connect_to_light(x, &m, &sd, ldin);
+    if (!m_singular(&m, x->c->lambert) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
+        if (x->c->flags & ST_FLAG_RESTIR_DI) connect_to_light_restir(x, &m, &sd, ldin);
+        else connect_to_light(x, &m, &sd, ldin);
+    }
     float cp = minf(1, m_cont_prob(&m));
     float rr[4];
     rng_nextf(&ps->rng, rr);
@@ -1432,7 +1494,7 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
 int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
     if (!s || !view || !p || !out) return fail("null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
-    if (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_ENV_ALIAS)) return fail("mode not supported by this oracle");
+    if (p->flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_ENV_ALIAS)) return fail("mode not supported by this oracle");
     cfg_t c;
     c.flags = p->flags;
     c.env_prob = p->environment_sample_probability;

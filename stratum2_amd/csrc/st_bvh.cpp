@@ -6,8 +6,9 @@
 // (src/App/Scene.cpp:17-55, 543-754).  All mesh instances are flattened into one
 // world-space triangle set (each triangle keeps its 16|16 instance|primitive
 // id), so traversal needs no instance transforms.  Binned SAH (32 bins per axis),
-// leaves of at most 4 triangles, subtrees built in parallel; past depth 48 the
-// builder switches to object-median splits so the depth never exceeds 64.
+// leaves of at most 4 triangles, subtrees built in parallel; when the remaining
+// depth budget gets tight the builder switches to object-median splits, so the
+// depth never exceeds 64 (the traversal stack bound).
 #include "st_bvh.h"
 
 #include <algorithm>
@@ -91,7 +92,10 @@ struct Builder {
                              [ax](const Ref& x, const Ref& y) { return x.c[ax] < y.c[ax]; });
             return mid;
         };
-        if (ext[axis] <= 0.0f || depth >= 48) return median(axis);
+        // keep room for a balanced finish: median splits need ceil(log2(n / 4)) more levels
+        int need = 0;
+        while ((n >> need) > (uint32_t)kBvhMaxLeaf) need++;
+        if (ext[axis] <= 0.0f || depth + need + 2 >= kBvhMaxDepth - 4) return median(axis);
         constexpr int B = 32;
         float best = std::numeric_limits<float>::infinity();
         int best_axis = -1, best_bin = -1;

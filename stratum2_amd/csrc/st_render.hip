@@ -71,9 +71,9 @@ struct WaveState {
 };
 
 struct RenderConsts {
-    uint32_t W, H, npix, wave_paths, k, sample_base, seed, max_depth, min_len, flags, rr_start;
+    uint32_t W, H, npix, wave_paths, k, sample_base, seed, max_depth, min_len, flags, rr_start, cand;
     float env_prob;
-    float vx, vy;  // unused
+    float max_m;
     st_view view;
 };
 
@@ -191,7 +191,99 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// One light candidate evaluated at a shading point: ReservoirSampleDataDI
+// (testrenderer.slang:568-606) and the plain NEE record of ConnectToLight (:751-779).
+struct Cand {
+    v3 contrib;      // f * Le (ReSTIR) or throughput * Le * f / pdfW (plain NEE)
+    v3 dir;
+    float dist, src_pdf, wtoa, mis_w;
+};
+
 template <bool LAMBERT, bool ALIAS>
+__device__ __forceinline__ bool eval_candidate(const DevScene& S, const RenderConsts& c, const Material& m, const Frame& f,
+                                               v3 lgn, v3 ldin, v3 pos, float4 rnd, float cp_eff, Cand& x, float& lum_rad) {
+    const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rnd, pos);
+    x.src_pdf = L.pdf;
+    x.dir = L.dir;
+    x.dist = L.dist;
+    x.contrib = mk(0, 0, 0);
+    x.wtoa = 0;
+    x.mis_w = 0;
+    lum_rad = lum(L.radiance);
+    if (L.pdf <= 0) return false;
+    const float AtoW = L.finite ? sq(L.dist) / L.cos_light : 1.0f;
+    const v3 ldout = f.to_local(L.dir);
+    Eval fe = bsdf_eval<LAMBERT>(m, ldin, ldout);
+    fe.f = fe.f * shading_normal_correction(f, lgn, ldin, ldout);
+    fe.fwd *= cp_eff;
+    fe.rev *= cp_eff;
+    x.contrib = fe.f * L.radiance;
+    x.wtoa = L.finite ? L.cos_light / sq(L.dist) : 1.0f;
+    x.mis_w = mis2(L.pdf, fe.fwd / AtoW);  // ComputeMIS_DI, testrenderer.slang:530-566
+    x.src_pdf = L.pdf;
+    return true;
+}
+
+// Next-event estimation at a surface vertex.  Plain: one light sample
+// (ConnectToLight, testrenderer.slang:751-779).  RESTIR: resampled importance
+// sampling over c.cand candidates (SampleIlluminationReSTIR, :665-684; the
+// ReSTIR ConnectToLight, :730-747).  Returns false when no shadow ray is made.
+template <bool LAMBERT, bool ALIAS, bool RESTIR>
+__device__ __forceinline__ bool next_event(const DevScene& S, const RenderConsts& c, Rng& rng, const Material& m,
+                                           const Frame& f, v3 lgn, v3 ldin, v3 pos, uint32_t gn, v3 thr, float cp_eff,
+                                           float4& sh0, float4& sh1, float4& sh2, uint32_t p) {
+    Cand sel;
+    float wgt_out;
+    v3 contrib;
+    if (!RESTIR) {
+        const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), pos);
+        if (all_nonpos(L.radiance) || L.pdf <= 0) return false;
+        const float AtoW = L.finite ? sq(L.dist) / L.cos_light : 1.0f;
+        const float dpw = L.pdf * AtoW;
+        const v3 ldout = f.to_local(L.dir);
+        Eval fe = bsdf_eval<LAMBERT>(m, ldin, ldout);
+        fe.f = fe.f * shading_normal_correction(f, lgn, ldin, ldout);
+        fe.fwd *= cp_eff;
+        fe.rev *= cp_eff;
+        contrib = ((thr * L.radiance) * fe.f) / dpw;
+        wgt_out = mis2(L.pdf, fe.fwd / AtoW);
+        sel.dir = L.dir;
+        sel.dist = L.dist;
+    } else {
+        float wsum = 0, sel_target = 0;
+        sel.contrib = mk(0, 0, 0);
+        sel.dir = mk(0, 0, 0);
+        sel.dist = 0;
+        sel.src_pdf = 0;
+        sel.wtoa = 0;
+        sel.mis_w = 0;
+        for (uint32_t i = 0; i < c.cand; i++) {
+            Cand x;
+            float lr;
+            const bool ok = eval_candidate<LAMBERT, ALIAS>(S, c, m, f, lgn, ldin, pos, rng.nextf(), cp_eff, x, lr);
+            const float target = (ok && lr > 0) ? lum(x.contrib) * x.wtoa : 0.0f;
+            if (target <= 0) continue;
+            wsum += target / x.src_pdf;
+            if (rng.nextf().x <= (target / x.src_pdf) / wsum) {
+                sel = x;
+                sel_target = target;
+            }
+        }
+        float W = sel_target > 0 ? wsum / ((float)c.cand * sel_target) : 0.0f;
+        if (W <= 0) return false;
+        W *= sel.wtoa;
+        contrib = (thr * sel.contrib) * W;
+        wgt_out = sel.mis_w;
+    }
+    const v3 o = ray_offset(pos, unpack_normal(gn), sel.dir);
+    const float tmax = fmax_(sel.dist * 0.99f, sel.dist - 1e-2f);  // AdjustIntersectionDistance
+    sh0 = make_float4(o.x, o.y, o.z, tmax);
+    sh1 = make_float4(sel.dir.x, sel.dir.y, sel.dir.z, __uint_as_float(p));
+    sh2 = make_float4(contrib.x, contrib.y, contrib.z, wgt_out);
+    return true;
+}
+
+template <bool LAMBERT, bool ALIAS, bool RESTIR>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
                                                        uint32_t* visibility, int first_wave) {
     const uint32_t n = w.counters[qsel];
@@ -279,27 +371,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 const bool rr_on = path_len >= c.rr_start;
                 const float cp_eff = rr_on ? m_cont_prob(m) : 1.0f;
                 if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
-                    // ConnectToLight, testrenderer.slang:751-779
-                    const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), sd.pos);
-                    if (!all_nonpos(L.radiance) && L.pdf > 0) {
-                        const float AtoW = L.finite ? sq(L.dist) / L.cos_light : 1.0f;
-                        const float dpw = L.pdf * AtoW;
-                        const v3 ldout = f.to_local(L.dir);
-                        Eval fe = bsdf_eval<LAMBERT>(m, ldin, ldout);
-                        fe.f = fe.f * shading_normal_correction(f, lgn, ldin, ldout);
-                        fe.fwd *= cp_eff;
-                        fe.rev *= cp_eff;
-                        const v3 contrib = ((thr * L.radiance) * fe.f) / dpw;
-                        const float wgt = mis2(L.pdf, fe.fwd / AtoW);
-                        if (path_len + 1 >= c.min_len) {
-                            const v3 o = ray_offset(sd.pos, unpack_normal(sd.gn), L.dir);
-                            const float tmax = fmax_(L.dist * 0.99f, L.dist - 1e-2f);  // AdjustIntersectionDistance
-                            sh0 = make_float4(o.x, o.y, o.z, tmax);
-                            sh1 = make_float4(L.dir.x, L.dir.y, L.dir.z, __uint_as_float(p));
-                            sh2 = make_float4(contrib.x, contrib.y, contrib.z, wgt);
-                            emit_shadow = true;
-                        }
-                    }
+                    emit_shadow = next_event<LAMBERT, ALIAS, RESTIR>(S, c, rng, m, f, lgn, ldin, sd.pos, sd.gn, thr, cp_eff,
+                                                                    sh0, sh1, sh2, p) &&
+                                  path_len + 1 >= c.min_len;
                 }
                 if (alive) {
                     const float cp = fmin_(1.0f, m_cont_prob(m));
@@ -761,6 +835,15 @@ int st_scene_stats(const st_scene* s, uint64_t out[4]) {
     return ST_OK;
 }
 
+using ShadeFn = void (*)(DevScene, RenderConsts, WaveState, int, uint32_t*, int);
+static ShadeFn shade_kernel(bool lambert, bool alias, bool restir) {
+    const int i = (lambert ? 4 : 0) | (alias ? 2 : 0) | (restir ? 1 : 0);
+    static const ShadeFn table[8] = {k_shade<false, false, false>, k_shade<false, false, true>, k_shade<false, true, false>,
+                                     k_shade<false, true, true>,   k_shade<true, false, false>, k_shade<true, false, true>,
+                                     k_shade<true, true, false>,   k_shade<true, true, true>};
+    return table[i];
+}
+
 static void ensure_spill(st_context* ctx, int grid, int block) {
     ctx->spill.reserve((size_t)(kMaxDepth - kLdsStack + 1) * grid * block * 4);
 }
@@ -768,8 +851,10 @@ static void ensure_spill(st_context* ctx, int grid, int block) {
 int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if (!s || !view || !p || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
-    if (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE))
-        return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI is not available in this build");
+    if (p->flags & ST_FLAG_RESTIR_DI_REUSE)
+        return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI reuse is not available in this build");
+    if ((p->flags & ST_FLAG_RESTIR_DI) && (p->candidate_samples == 0 || p->candidate_samples > 1024))
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "candidate_samples must be in [1, 1024]");
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
     st_context* ctx = s->ctx;
     try {
@@ -823,6 +908,9 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         c.env_prob = p->environment_sample_probability;
         c.view = *view;
         const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+        const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
+        c.cand = p->candidate_samples;
+        c.max_m = p->max_m;
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
         auto* stats = ctx->stats.as<unsigned long long>();
         for (uint32_t base = 0; base < p->spp; base += k) {
@@ -841,10 +929,8 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                 {
                     Timed tm(ctx, 1);
                     uint32_t* vis = (base == 0) ? o->visibility : nullptr;
-                    if (lambert && alias) hipLaunchKernelGGL((k_shade<true, true>), dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0);
-                    else if (lambert) hipLaunchKernelGGL((k_shade<true, false>), dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0);
-                    else if (alias) hipLaunchKernelGGL((k_shade<false, true>), dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0);
-                    else hipLaunchKernelGGL((k_shade<false, false>), dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0);
+                    auto* kfn = shade_kernel(lambert, alias, restir);
+                    hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0 ? 1 : 0);
                 }
                 {
                     Timed tm(ctx, 2);

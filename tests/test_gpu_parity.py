@@ -124,3 +124,13 @@ def test_empty_and_tiny_scenes(gpu_ctx):
     torch.cuda.synchronize()
     vis = out["visibility"].cpu().numpy().view(np.uint32)
     assert (vis[..., 0] == 0).any() and (vis[..., 0] == 0xFFFFFFFF).any()
+
+
+def test_restir_di_candidates(gpu_ctx):
+    """ReSTIR DI initial RIS (32 candidates, no reuse): deterministic, so the same bar."""
+    sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+    W, H = 64, 48
+    params = abi.default_params(W, H, max_depth=3, spp=2, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI)
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    g, r = _render_both(gpu_ctx, sd, view, params)
+    _check_image(g, r, frac_px=0.98, frac_len=0.98)
