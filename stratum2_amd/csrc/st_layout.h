@@ -27,7 +27,7 @@ constexpr int kNodeBytes = 64;
 constexpr int kTriBytes = 48;
 constexpr int kMaxLeafTris = 4;
 constexpr int kMaxDepth = 64;      // builder guarantee (median-split fallback beyond it)
-constexpr int kLdsStack = 32;      // per-lane stack entries kept in LDS; the rest spill to HBM
+constexpr int kLdsStack = 16;      // per-lane stack entries kept in LDS; the rest spill to HBM
 constexpr int kEnvMaxLevels = 16;
 constexpr int kQuant = 16384;      // gRadianceQuantizationFactor, testrenderer.slang:25
 

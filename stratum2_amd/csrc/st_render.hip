@@ -56,6 +56,7 @@ struct HipError : std::runtime_error {
 
 constexpr int kTraceBlock = 128;
 constexpr int kShadeBlock = 256;
+constexpr int kTraceBlocksPerCU = 14;  // 28 wavefronts per CU (7 per SIMD at <= 72 VGPRs), 8 KB LDS each
 
 // ----------------------------------------------------------------- device state
 struct WaveState {
@@ -71,18 +72,37 @@ struct WaveState {
 };
 
 struct RenderConsts {
-    uint32_t W, H, npix, wave_paths, k, sample_base, seed, max_depth, min_len, flags, rr_start, cand;
+    uint32_t W, H, npix, wave_paths, k, sample_base, seed, max_depth, min_len, flags, rr_start, cand, tiled;
     float env_prob;
     float max_m;
     st_view view;
 };
 
 // -------------------------------------------------------------------- kernels
+// Path order inside one sample: 8x8 pixel tiles (one wavefront of primary rays
+// covers one tile) when the image is a multiple of 8 in both dimensions,
+// scanline order otherwise.  q is the path index within its sample.
+__device__ __forceinline__ void pixel_of(const RenderConsts& c, uint32_t q, uint32_t& px, uint32_t& py) {
+    if (c.tiled) {
+        const uint32_t t = q >> 6, j = q & 63u, tx = c.W >> 3;
+        px = (t % tx) * 8 + (j & 7u);
+        py = (t / tx) * 8 + (j >> 3);
+    } else {
+        px = q % c.W;
+        py = q / c.W;
+    }
+}
+__device__ __forceinline__ uint32_t path_of_pixel(const RenderConsts& c, uint32_t px, uint32_t py) {
+    if (!c.tiled) return py * c.W + px;
+    return (((py >> 3) * (c.W >> 3) + (px >> 3)) << 6) | ((py & 7u) << 3) | (px & 7u);
+}
+
 __global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveState w) {
     const uint32_t n = c.wave_paths;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t pix = p % c.npix, k = p / c.npix;
-        const uint32_t px = pix % c.W, py = pix / c.W;
+        const uint32_t k = p / c.npix;
+        uint32_t px, py;
+        pixel_of(c, p % c.npix, px, py);
         Rng rng{px, py, c.seed + c.sample_base + k, 0};
         float ox = 0.5f, oy = 0.5f;
         if (c.flags & ST_FLAG_PIXEL_JITTER) {
@@ -115,54 +135,64 @@ __global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveStat
         w.counters[0] = n;
         w.counters[1] = 0;
         w.counters[2] = 0;
+        w.counters[16] = 0;
+        w.counters[17] = 0;
     }
 }
 
-template <bool COUNT>
-__global__ void __launch_bounds__(kTraceBlock) k_trace_closest(DevScene S, WaveState w, int qsel, uint32_t* spill,
-                                                               unsigned long long* stats) {
-    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
-    const uint32_t n = w.counters[qsel];
-    const uint32_t* q = w.queue[qsel];
-    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraverseStats ts{0, 0};
-    for (uint32_t i = gtid; i < n; i += stride) {
-        const uint32_t p = q[i];
-        const float4 r = w.ray[p];
-        const v3 d = unpack_normal(__float_as_uint(r.w));
-        const Hit h = traverse<false, COUNT>(S, mk(r.x, r.y, r.z), d, __int_as_float(0x7f800000), lds, spill, stride, gtid, ts);
+struct ClosestSrc {
+    WaveState w;
+    const uint32_t* q;
+    __device__ __forceinline__ void load(uint32_t i, v3& o, v3& d, float& tmax) const {
+        const float4 r = w.ray[q[i]];
+        o = mk(r.x, r.y, r.z);
+        d = unpack_normal(__float_as_uint(r.w));
+        tmax = __int_as_float(0x7f800000);
+    }
+    __device__ __forceinline__ void done(uint32_t i, const Hit& h) const {
         w.hit[i] = make_uint4(__float_as_uint(h.t), h.hit ? h.id : 0xffffffffu, __float_as_uint(h.b1), __float_as_uint(h.b2));
     }
+};
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_trace_closest(DevScene S, WaveState w, int qsel, uint32_t* spill,
+                                                               unsigned long long* stats) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    ClosestSrc src{w, w.queue[qsel]};
+    TraverseStats ts{0, 0};
+    trace_queue<false, COUNT>(S, w.counters[qsel], &w.counters[16], src, lds, spill, ts);
     if (COUNT) {
         atomicAdd(&stats[0], (unsigned long long)ts.nodes);
         atomicAdd(&stats[1], (unsigned long long)ts.tris);
     }
 }
 
-template <bool COUNT>
-__global__ void __launch_bounds__(kTraceBlock) k_trace_shadow(DevScene S, WaveState w, uint32_t flags, uint32_t* spill,
-                                                              unsigned long long* stats) {
-    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
-    const uint32_t n = w.counters[2];
-    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraverseStats ts{0, 0};
-    for (uint32_t i = gtid; i < n; i += stride) {
-        const float4 a = w.shadow[3 * (size_t)i], b = w.shadow[3 * (size_t)i + 1], c = w.shadow[3 * (size_t)i + 2];
-        const Hit h = traverse<true, COUNT>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, lds, spill, stride, gtid, ts);
-        if (h.hit) continue;
-        // AddRadianceInterlocked, testrenderer.slang:237-261
+// Shadow rays: any-hit, then AddRadianceInterlocked (testrenderer.slang:237-261)
+// into the path's own fixed-point accumulator.
+struct ShadowSrc {
+    WaveState w;
+    uint32_t flags;
+    __device__ __forceinline__ void load(uint32_t i, v3& o, v3& d, float& tmax) const {
+        const float4 a = w.shadow[3 * (size_t)i], b = w.shadow[3 * (size_t)i + 1];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+        tmax = a.w;
+    }
+    __device__ __forceinline__ void done(uint32_t i, const Hit& h) const {
+        if (h.hit) return;
+        const float4 c = w.shadow[3 * (size_t)i + 2];
+        const uint32_t p = __float_as_uint(w.shadow[3 * (size_t)i + 1].w);
         const float wgt = c.w;
         const v3 ct = mk(c.x, c.y, c.z);
-        if (wgt <= 0 || all_nonpos(ct)) continue;
+        if (wgt <= 0 || all_nonpos(ct)) return;
         const v3 rw = ct * wgt;
-        const uint32_t p = __float_as_uint(b.w);
         if (flags & ST_FLAG_FLOAT_NEE) {
             float4 e = w.em[p];
             e.x += fmax_(0.0f, rw.x);
             e.y += fmax_(0.0f, rw.y);
             e.z += fmax_(0.0f, rw.z);
             w.em[p] = e;
-            continue;
+            return;
         }
         const float q0 = fmax_(0.0f, rw.x) * (float)kQuant, q1 = fmax_(0.0f, rw.y) * (float)kQuant,
                     q2 = fmax_(0.0f, rw.z) * (float)kQuant;
@@ -173,6 +203,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_shadow(DevScene S, WaveSt
         acc.z += (uint32_t)(q2 >= 2147483647.0f ? 2147483647 : (int32_t)q2);
         w.nee[p] = acc;
     }
+};
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_trace_shadow(DevScene S, WaveState w, uint32_t flags, uint32_t* spill,
+                                                              unsigned long long* stats) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    ShadowSrc src{w, flags};
+    TraverseStats ts{0, 0};
+    trace_queue<true, COUNT>(S, w.counters[2], &w.counters[17], src, lds, spill, ts);
     if (COUNT) {
         atomicAdd(&stats[2], (unsigned long long)ts.nodes);
         atomicAdd(&stats[3], (unsigned long long)ts.tris);
@@ -305,8 +344,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
             const float4 r = w.ray[p];
             float4 thr4 = w.thr[p];
             uint4 s4 = w.st[p];
-            const uint32_t pix = p % c.npix, k = p / c.npix;
-            Rng rng{pix % c.W, pix / c.W, c.seed + c.sample_base + k, s4.x};
+            const uint32_t k = p / c.npix;
+            uint32_t px, py;
+            pixel_of(c, p % c.npix, px, py);
+            const uint32_t pix = py * c.W + px;
+            Rng rng{px, py, c.seed + c.sample_base + k, s4.x};
             uint32_t path_len = s4.y + 1;  // TraceRay, testrenderer.slang:1026-1058
             const v3 dir = unpack_normal(__float_as_uint(r.w));
             v3 thr = mk(thr4.x, thr4.y, thr4.z);
@@ -426,8 +468,9 @@ __global__ void k_resolve(RenderConsts c, WaveState w, float4* acc, float* hdr, 
                           int last) {
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < c.npix; pix += gridDim.x * blockDim.x) {
         float4 a = acc[pix];
+        const uint32_t q = path_of_pixel(c, pix % c.W, pix / c.W);
         for (uint32_t k = 0; k < c.k; k++) {
-            const uint32_t p = k * c.npix + pix;
+            const uint32_t p = k * c.npix + q;
             const float4 e = w.em[p];
             const uint4 q = w.nee[p];
             a.x += e.x + (float)q.x / (float)kQuant;
@@ -452,24 +495,36 @@ __global__ void k_count(WaveState w, int qsel) {
     r[1] += w.counters[2];
     w.counters[2] = 0;
     w.counters[qsel] = 0;
+    w.counters[16] = 0;  // traversal fetch cursors
+    w.counters[17] = 0;
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int closest,
-                                                            uint32_t* out, uint32_t* spill) {
-    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
-    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraverseStats ts{0, 0};
-    for (uint32_t i = gtid; i < n; i += stride) {
+struct RaySrc {
+    const float4* rays;
+    uint32_t* out;
+    int closest;
+    __device__ __forceinline__ void load(uint32_t i, v3& o, v3& d, float& tmax) const {
         const float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
-        if (closest) {
-            const Hit h = traverse<false, false>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, lds, spill, stride, gtid, ts);
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+        tmax = a.w;
+    }
+    __device__ __forceinline__ void done(uint32_t i, const Hit& h) const {
+        if (closest)
             reinterpret_cast<uint4*>(out)[i] =
                 make_uint4(__float_as_uint(h.t), h.hit ? h.id : 0xffffffffu, __float_as_uint(h.b1), __float_as_uint(h.b2));
-        } else {
-            const Hit h = traverse<true, false>(S, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, lds, spill, stride, gtid, ts);
+        else
             out[i] = h.hit ? 1u : 0u;
-        }
     }
+};
+
+__global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int closest,
+                                                            uint32_t* out, uint32_t* spill, uint32_t* fetch) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    RaySrc src{rays, out, closest};
+    TraverseStats ts{0, 0};
+    if (closest) trace_queue<false, false>(S, n, fetch, src, lds, spill, ts);
+    else trace_queue<true, false>(S, n, fetch, src, lds, spill, ts);
 }
 
 // -------------------------------------------------------------------- host side
@@ -877,7 +932,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         ctx->counters.reserve(256);
         ctx->acc.reserve((size_t)npix * 16);
         ctx->stats.reserve(64);
-        const int tgrid = ctx->cus * 8, sgrid = ctx->cus * 8;
+        const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         ensure_spill(ctx, tgrid, kTraceBlock);
         WaveState w;
         w.ray = ctx->ray.as<float4>();
@@ -907,6 +962,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         c.rr_start = p->rr_start_depth;
         c.env_prob = p->environment_sample_probability;
         c.view = *view;
+        c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
         const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
         const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
         c.cand = p->candidate_samples;
@@ -984,10 +1040,12 @@ int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, u
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
         if (count == 0) return ST_OK;
-        const int grid = std::min<int>((count + kTraceBlock - 1) / kTraceBlock, ctx->cus * 8);
-        ensure_spill(ctx, ctx->cus * 8, kTraceBlock);
+        const int grid = std::min<int>((count + kTraceBlock - 1) / kTraceBlock, ctx->cus * kTraceBlocksPerCU);
+        ensure_spill(ctx, ctx->cus * kTraceBlocksPerCU, kTraceBlock);
+        ctx->counters.reserve(256);
+        HIP_CHECK(hipMemsetAsync(ctx->counters.as<uint8_t>() + 4 * 18, 0, 4, ctx->stream));
         hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
-                           closest, hits, ctx->spill.as<uint32_t>());
+                           closest, hits, ctx->spill.as<uint32_t>(), ctx->counters.as<uint32_t>() + 18);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return ST_OK;

@@ -89,28 +89,37 @@ struct TraverseStats {
     uint32_t nodes, tris;
 };
 
-// ANY: stop at the first hit in (0, tmax).  lds: this block's stack,
-// [kLdsStack][blockDim.x]; spill: HBM stack region, [kMaxDepth - kLdsStack][stride].
-template <bool ANY, bool COUNT>
-__device__ __forceinline__ Hit traverse(const DevScene& S, v3 o, v3 d, float tmax, uint32_t* lds, uint32_t* spill,
-                                        uint32_t spill_stride, uint32_t gtid, TraverseStats& st) {
+// Traversal state of one lane.  step() visits one node: both child boxes,
+// leaf children intersected immediately, nearer inner child next, farther
+// one pushed.  Returns false when the ray is finished.
+struct Lane {
+    WRay wr;
+    v3 inv;
     Hit h;
-    h.hit = false;
-    h.t = tmax;
-    h.b1 = h.b2 = 0;
-    h.id = 0xffffffffu;
-    if (S.node_count == 0) return h;
-    const WRay wr = wray_make(o, d);
-    const v3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    const uint32_t lane = threadIdx.x, bdim = blockDim.x;
-    int sp = 0;
-    uint32_t node = 0;
-    while (true) {
+    uint32_t node;
+    int sp;
+
+    __device__ __forceinline__ void start(v3 o, v3 d, float tmax) {
+        wr = wray_make(o, d);
+        inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        h.hit = false;
+        h.t = tmax;
+        h.b1 = h.b2 = 0;
+        h.id = 0xffffffffu;
+        node = 0;
+        sp = 0;
+    }
+
+    template <bool ANY, bool COUNT>
+    __device__ __forceinline__ bool step(const DevScene& S, uint32_t* lds, uint32_t* spill, uint32_t spill_stride,
+                                         uint32_t gtid, TraverseStats& st) {
+        const uint32_t lane = threadIdx.x, bdim = blockDim.x;
         const float4* np = S.nodes + 4 * (size_t)node;
         const float4 a = np[0], b = np[1], c = np[2];
         const int4 m = *reinterpret_cast<const int4*>(np + 3);
         if (COUNT) st.nodes++;
         float tn0, tn1;
+        const v3 o = wr.o;
         bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, h.t, tn0) && m.z >= 0;
         bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, h.t, tn1) && m.w >= 0;
 #pragma unroll
@@ -133,7 +142,7 @@ __device__ __forceinline__ Hit traverse(const DevScene& S, v3 o, v3 d, float tma
                         h.b2 = b2;
                         h.id = id;
                         h.hit = true;
-                        if (ANY) return h;
+                        if (ANY) return false;
                     }
                 }
                 if (side == 0) h0 = false;
@@ -153,12 +162,69 @@ __device__ __forceinline__ Hit traverse(const DevScene& S, v3 o, v3 d, float tma
         } else if (h1) {
             node = (uint32_t)m.y;
         } else {
-            if (sp == 0) break;
+            if (sp == 0) return false;
             sp--;
             node = sp < kLdsStack ? lds[sp * bdim + lane] : spill[(size_t)(sp - kLdsStack) * spill_stride + gtid];
         }
+        return true;
     }
-    return h;
+};
+
+// ANY: stop at the first hit in (0, tmax).  lds: this block's stack,
+// [kLdsStack][blockDim.x]; spill: HBM stack region, [kMaxDepth - kLdsStack][stride].
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ Hit traverse(const DevScene& S, v3 o, v3 d, float tmax, uint32_t* lds, uint32_t* spill,
+                                        uint32_t spill_stride, uint32_t gtid, TraverseStats& st) {
+    Lane L;
+    L.start(o, d, tmax);
+    if (S.node_count == 0) return L.h;
+    while (L.step<ANY, COUNT>(S, lds, spill, spill_stride, gtid, st)) {
+    }
+    return L.h;
+}
+
+// Persistent traversal over a queue of n rays with per-lane dynamic fetch:
+// a lane whose ray is finished takes the next ray index from *fetch (one
+// atomic per wavefront per refill), so lanes do not idle behind the slowest
+// ray of their wavefront.  Src supplies load(i, o, d, tmax) and done(i, hit).
+template <bool ANY, bool COUNT, class Src>
+__device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint32_t* fetch, Src& src, uint32_t* lds,
+                                            uint32_t* spill, TraverseStats& st) {
+    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane_id = __lane_id();
+    Lane L;
+    bool active = false, drained = false;
+    uint32_t idx = 0;
+    while (true) {
+        if (!drained) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const uint32_t cnt = (uint32_t)__popcll(need);
+                const int leader = __ffsll((long long)need) - 1;
+                uint32_t base = 0;
+                if ((int)lane_id == leader) base = atomicAdd(fetch, cnt);
+                base = __shfl(base, leader);
+                if (base + cnt >= n) drained = true;
+                if (!active) {
+                    const uint32_t i = base + (uint32_t)__popcll(need & ((1ull << lane_id) - 1ull));
+                    if (i < n) {
+                        v3 o, d;
+                        float tmax;
+                        src.load(i, o, d, tmax);
+                        L.start(o, d, tmax);
+                        idx = i;
+                        active = S.node_count != 0;
+                        if (!active) src.done(i, L.h);
+                    }
+                }
+            }
+        }
+        if (!__any(active)) break;
+        if (active && !L.step<ANY, COUNT>(S, lds, spill, stride, gtid, st)) {
+            src.done(idx, L.h);
+            active = false;
+        }
+    }
 }
 
 }  // namespace st
