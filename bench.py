@@ -91,12 +91,13 @@ def main():
     value = sum(rays) / total_s / 1e6
     ms = total_s / len(times) * 1e3
 
-    # dominant kernel class and its algorithmic bytes (32 B ray + 16 B hit or
-    # 4 B flag + 64 B per node + 48 B per triangle)
+    # dominant kernel class and its algorithmic bytes: 32 B ray + 16 B hit (or
+    # 48 B shadow record + 4 B flag) + 48 B per 4-wide node + 48 B per triangle
+    NODE_B, TRI_B = 48, 48
     t_last = kt[-1]
     classes = {
-        "trace_closest": (t_last["trace_ms"], t_last["trace_launches"], t_last["closest_rays"] * (48 + 64 * nodes_c + 48 * tris_c)),
-        "trace_shadow": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * (36 + 64 * nodes_s + 48 * tris_s)),
+        "trace_closest": (t_last["trace_ms"], t_last["trace_launches"], t_last["closest_rays"] * (48 + NODE_B * nodes_c + TRI_B * tris_c)),
+        "trace_shadow": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * (52 + NODE_B * nodes_s + TRI_B * tris_s)),
         "shade": (t_last["shade_ms"], t_last["shade_launches"], t_last["closest_rays"] * (16 + 64 + 64 + 64 + 48) + t_last["shadow_rays"] * 48),
     }
     dom = max(classes, key=lambda k: classes[k][0])

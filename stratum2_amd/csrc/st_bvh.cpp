@@ -15,6 +15,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <future>
 #include <limits>
 #include <stdexcept>
@@ -194,6 +195,150 @@ struct Builder {
     }
 };
 
+// ---------------------------------------------------------------- 4-wide collapse
+struct Kid {
+    float lo[3], hi[3];
+    int32_t index, count;  // count > 0 leaf (triangle start = index), 0 inner (binary node), < 0 empty
+};
+
+static float area(const Kid& k) {
+    float d[3];
+    for (int a = 0; a < 3; a++) d[a] = std::max(0.0f, k.hi[a] - k.lo[a]);
+    return d[0] * d[1] + d[1] * d[2] + d[2] * d[0];
+}
+
+static void binary_kids(const std::vector<NodeOut>& nodes, uint32_t ni, std::vector<Kid>& out) {
+    const NodeOut& n = nodes[ni];
+    for (int c = 0; c < 2; c++) {
+        Kid k;
+        for (int a = 0; a < 3; a++) {
+            k.lo[a] = n.f[6 * c + 2 * a];
+            k.hi[a] = n.f[6 * c + 2 * a + 1];
+        }
+        k.index = n.i[c];
+        k.count = n.i[2 + c];
+        if (k.count >= 0) out.push_back(k);
+    }
+}
+
+// decoded coordinate o + q * 2^(e-127), as the traversal computes it
+static float decode(float o, uint32_t q, int e) {
+    uint32_t bits = (uint32_t)e << 23;
+    float s;
+    std::memcpy(&s, &bits, 4);
+    const float qs = (float)q * s;  // exact: q < 256, s a power of two
+    return o + qs;
+}
+
+struct Collapser {
+    const std::vector<NodeOut>& bn;
+    const std::vector<TriOut>& bt;
+    std::vector<Node4Out> nodes;
+    std::vector<TriOut> tris;
+    int depth = 0;
+
+    void make(std::vector<Kid> kids, uint32_t slot, int d) {
+        depth = std::max(depth, d);
+        while (kids.size() < 4) {
+            int best = -1;
+            for (int i = 0; i < (int)kids.size(); i++)
+                if (kids[i].count == 0 && (best < 0 || area(kids[i]) > area(kids[best]))) best = i;
+            if (best < 0) break;
+            std::vector<Kid> sub;
+            binary_kids(bn, (uint32_t)kids[best].index, sub);
+            kids.erase(kids.begin() + best);
+            kids.insert(kids.begin() + best, sub.begin(), sub.end());
+        }
+        Node4Out n;
+        std::memset(&n, 0, sizeof n);
+        float L[3], U[3];
+        for (int a = 0; a < 3; a++) {
+            L[a] = std::numeric_limits<float>::infinity();
+            U[a] = -std::numeric_limits<float>::infinity();
+            for (const Kid& k : kids) {
+                L[a] = std::min(L[a], k.lo[a]);
+                U[a] = std::max(U[a], k.hi[a]);
+            }
+        }
+        int e[3];
+        for (int a = 0; a < 3; a++) {
+            n.o[a] = L[a];
+            const double ext = (double)U[a] - (double)L[a];
+            int k = ext > 0 ? (int)std::ceil(std::log2(ext / 255.0)) : -100;
+            k = std::max(-126, std::min(127, k));
+            while (k < 127 && decode(L[a], 255, k + 127) < U[a]) k++;
+            e[a] = k + 127;
+            if (decode(L[a], 255, e[a]) < U[a]) throw std::runtime_error("BVH quantisation range");
+        }
+        uint32_t types = 0;
+        uint32_t n_inner = 0;
+        for (const Kid& k : kids) n_inner += k.count == 0;
+        const uint32_t base_inner = (uint32_t)nodes.size();
+        nodes.resize(nodes.size() + n_inner);
+        const uint32_t base_tri = (uint32_t)tris.size();
+        for (int c = 0; c < 4; c++) {
+            uint32_t qlo[3] = {255, 255, 255}, qhi[3] = {0, 0, 0}, type = 0;
+            if (c < (int)kids.size()) {
+                const Kid& k = kids[c];
+                for (int a = 0; a < 3; a++) {
+                    uint32_t bits = (uint32_t)e[a] << 23;
+                    float sc;
+                    std::memcpy(&sc, &bits, 4);
+                    double ql = std::floor(((double)k.lo[a] - L[a]) / sc), qh = std::ceil(((double)k.hi[a] - L[a]) / sc);
+                    qlo[a] = (uint32_t)std::max(0.0, std::min(255.0, ql));
+                    qhi[a] = (uint32_t)std::max(0.0, std::min(255.0, qh));
+                    while (qlo[a] > 0 && decode(L[a], qlo[a], e[a]) > k.lo[a]) qlo[a]--;
+                    while (qhi[a] < 255 && decode(L[a], qhi[a], e[a]) < k.hi[a]) qhi[a]++;
+                    if (decode(L[a], qlo[a], e[a]) > k.lo[a] || decode(L[a], qhi[a], e[a]) < k.hi[a])
+                        throw std::runtime_error("BVH quantisation not conservative");
+                }
+                if (k.count == 0) {
+                    type = 7;
+                } else {
+                    type = (uint32_t)k.count;
+                    for (int t = 0; t < k.count; t++) tris.push_back(bt[(size_t)k.index + t]);
+                }
+            }
+            for (int a = 0; a < 3; a++) {
+                n.q[4 * a + c] = (uint8_t)qlo[a];
+                n.q[12 + 4 * a + c] = (uint8_t)qhi[a];
+            }
+            types |= type << (3 * c);
+        }
+        n.x0 = (uint32_t)e[0] | ((uint32_t)e[1] << 8) | ((uint32_t)e[2] << 16) | ((types & 0xffu) << 24);
+        if (base_inner >= (1u << 28)) throw std::runtime_error("too many BVH nodes");
+        n.inner = base_inner | ((types >> 8) << 28);
+        n.tri = base_tri;
+        nodes[slot] = n;
+        uint32_t r = 0;
+        for (const Kid& k : kids) {
+            if (k.count != 0) continue;
+            std::vector<Kid> sub;
+            binary_kids(bn, (uint32_t)k.index, sub);
+            queue.push_back({std::move(sub), base_inner + r, d + 1});
+            r++;
+        }
+    }
+
+    // Breadth-first: node i's children get the next free slots, so the top
+    // levels of the tree occupy the lowest node indices.
+    struct Job {
+        std::vector<Kid> kids;
+        uint32_t slot;
+        int depth;
+    };
+    std::deque<Job> queue;
+    void run(std::vector<Kid> root) {
+        nodes.resize(1);
+        queue.push_back({std::move(root), 0, 1});
+        while (!queue.empty()) {
+            Job j = std::move(queue.front());
+            queue.pop_front();
+            make(std::move(j.kids), j.slot, j.depth);
+        }
+    }
+};
+
 }  // namespace
 
 BvhOut build_bvh(const float* verts, const uint32_t* ids, uint32_t n) {
@@ -240,6 +385,14 @@ BvhOut build_bvh(const float* verts, const uint32_t* ids, uint32_t n) {
     }
     out.depth = bd.max_depth.load();
     if (out.depth > kBvhMaxDepth) throw std::runtime_error("BVH deeper than the traversal stack");
+    Collapser col{out.nodes, out.tris, {}, {}, 0, {}};
+    std::vector<Kid> root;
+    binary_kids(out.nodes, 0, root);
+    col.run(std::move(root));
+    out.nodes4 = std::move(col.nodes);
+    out.tris4 = std::move(col.tris);
+    out.depth4 = col.depth;
+    if (out.tris4.size() != out.tris.size()) throw std::runtime_error("4-wide collapse lost triangles");
     return out;
 }
 

@@ -1,14 +1,12 @@
 // st_layout.h — HBM layouts shared by the host builder and the gfx950 kernels.
 //
-// BVH: binary tree, one 64-byte node per inner node holding BOTH child boxes,
-// so one node fetch (4 x 16-byte loads of one 64-byte line) tests two boxes:
-//   f[0] = c0.lo.x c0.hi.x c0.lo.y c0.hi.y
-//   f[1] = c0.lo.z c0.hi.z c1.lo.x c1.hi.x
-//   f[2] = c1.lo.y c1.hi.y c1.lo.z c1.hi.z
-//   f[3] = (int) child0, child1, count0, count1
-// count == 0: child is an inner node index; count > 0: child is a leaf of
-// `count` triangles starting at triangle index `child`; count < 0: empty child
-// (only the root of a scene with at most 4 triangles has one).
+// BVH: 4-wide tree of 48-byte nodes (three 16-byte loads test four boxes):
+//   f[0] = origin.xyz (float), ex | ey << 8 | ez << 16 | types(0..7) << 24
+//   f[1] = qlo.x[4], qlo.y[4], qlo.z[4], qhi.x[4]          (uint8 each)
+//   f[2] = qhi.y[4], qhi.z[4], inner | types(8..11) << 28, first triangle
+// child box = origin + q * 2^(e-127) per axis (exactly conservative);
+// child type 0 empty, 1..4 leaf of that many triangles (contiguous from the
+// node's first triangle in child order), 7 inner (contiguous from `inner`).
 //
 // Triangles: 48 bytes, world space, in leaf order:
 //   t[0] = v0.xyz, id (instance | primitive << 16, as InstanceData ids)
@@ -27,6 +25,7 @@ constexpr int kNodeBytes = 64;
 constexpr int kTriBytes = 48;
 constexpr int kMaxLeafTris = 4;
 constexpr int kMaxDepth = 64;      // builder guarantee (median-split fallback beyond it)
+constexpr int kMaxStack = 3 * kMaxDepth + 1;  // a 4-wide step pushes at most 3 entries
 constexpr int kLdsStack = 16;      // per-lane stack entries kept in LDS; the rest spill to HBM
 constexpr int kEnvMaxLevels = 16;
 constexpr int kQuant = 16384;      // gRadianceQuantizationFactor, testrenderer.slang:25

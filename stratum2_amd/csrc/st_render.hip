@@ -56,7 +56,7 @@ struct HipError : std::runtime_error {
 
 constexpr int kTraceBlock = 128;
 constexpr int kShadeBlock = 256;
-constexpr int kTraceBlocksPerCU = 14;  // 28 wavefronts per CU (7 per SIMD at <= 72 VGPRs), 8 KB LDS each
+constexpr int kTraceBlocksPerCU = 16;  // 32 wavefronts per CU (7-8 per SIMD at <= 72 VGPRs), 8 KB LDS each
 
 // ----------------------------------------------------------------- device state
 struct WaveState {
@@ -765,14 +765,15 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
         D.light_inst = s->upload<uint32_t>(lights.data(), 4 * lights.size());
         D.light_count = (uint32_t)lights.size();
         BvhOut bvh = build_bvh(verts.data(), ids.data(), (uint32_t)ntri);
-        D.nodes = s->upload<float4>(bvh.nodes.data(), bvh.nodes.size() * sizeof(NodeOut));
-        D.tris = s->upload<float4>(bvh.tris.data(), bvh.tris.size() * sizeof(TriOut));
-        D.node_count = (uint32_t)bvh.nodes.size();
-        D.tri_count = (uint32_t)bvh.tris.size();
+        D.nodes = s->upload<float4>(bvh.nodes4.data(), bvh.nodes4.size() * sizeof(Node4Out));
+        D.tris = s->upload<float4>(bvh.tris4.data(), bvh.tris4.size() * sizeof(TriOut));
+        D.node_count = ntri ? (uint32_t)bvh.nodes4.size() : 0;
+        D.tri_count = (uint32_t)bvh.tris4.size();
+        if (3 * bvh.depth4 + 1 > kMaxStack) throw std::runtime_error("BVH deeper than the traversal stack");
         s->stats[0] = D.node_count;
         s->stats[1] = D.tri_count;
-        s->stats[2] = (uint64_t)bvh.depth;
-        s->stats[3] = bvh.nodes.size() * 64 + bvh.tris.size() * 48;
+        s->stats[2] = (uint64_t)bvh.depth4;
+        s->stats[3] = bvh.nodes4.size() * sizeof(Node4Out) + bvh.tris4.size() * sizeof(TriOut);
         // environment: record (EnvironmentMap::store, Scene.hpp:42-45), mip chain, alias table
         D.has_env = d->environment_material_address != 0xFFFFFFFFu;
         if (D.has_env) {
@@ -900,7 +901,7 @@ static ShadeFn shade_kernel(bool lambert, bool alias, bool restir) {
 }
 
 static void ensure_spill(st_context* ctx, int grid, int block) {
-    ctx->spill.reserve((size_t)(kMaxDepth - kLdsStack + 1) * grid * block * 4);
+    ctx->spill.reserve((size_t)(kMaxStack - kLdsStack + 1) * grid * block * 4);
 }
 
 int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {

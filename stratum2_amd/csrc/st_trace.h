@@ -1,10 +1,12 @@
 // st_trace.h — gfx950 BVH traversal with the watertight ray/triangle test.
 //
-// One lane per ray.  Each step loads one 64-byte node (two child boxes),
-// tests both boxes, intersects leaf children immediately and descends into
-// the nearer inner child, pushing the farther one.  The per-lane stack lives in
-// LDS (kLdsStack entries, lane-interleaved so a wave's pushes hit distinct
-// banks) and spills to an HBM region only past that depth.
+// One lane per ray.  Each step loads one 48-byte 4-wide node (three 16-byte
+// loads), tests its four quantised child boxes, intersects hit leaf children
+// immediately and descends into the nearest hit inner child, pushing the
+// others.  The per-lane stack lives in LDS (kLdsStack entries, lane-interleaved
+// so a wave's pushes hit distinct banks) and spills to HBM past that depth.
+// The cost of a divergent vector load is paid per instruction in the texture
+// address unit, so the node format minimises loads per box tested.
 //
 // Closest-hit ties (equal t, e.g. a ray through a shared edge) resolve to the
 // smaller instance|primitive id, so the result does not depend on BVH shape or
@@ -110,62 +112,104 @@ struct Lane {
         sp = 0;
     }
 
+    __device__ __forceinline__ bool tri_test(const DevScene& S, uint32_t ti) {
+        const float4* tp = S.tris + 3 * (size_t)ti;
+        const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+        float t, b1, b2;
+        if (!wt_intersect(wr, mk(t0.x, t0.y, t0.z), mk(t1.x, t1.y, t1.z), mk(t2.x, t2.y, t2.z), t, b1, b2)) return false;
+        const uint32_t id = __float_as_uint(t0.w);
+        if (t < h.t || (t == h.t && h.hit && id < h.id)) {
+            h.t = t;
+            h.b1 = b1;
+            h.b2 = b2;
+            h.id = id;
+            h.hit = true;
+            return true;
+        }
+        return false;
+    }
+
+    __device__ __forceinline__ void push(uint32_t v, uint32_t* lds, uint32_t* spill, uint32_t stride, uint32_t gtid) {
+        if (sp < kLdsStack) lds[sp * (int)blockDim.x + (int)threadIdx.x] = v;
+        else spill[(size_t)(sp - kLdsStack) * stride + gtid] = v;
+        sp++;
+    }
+
+    // One 4-wide node: decode four quantised child boxes (o + q * 2^e, exact
+    // in float), slab-test them, intersect hit leaf children at once, continue
+    // with the nearest hit inner child and push the others farthest first.
     template <bool ANY, bool COUNT>
     __device__ __forceinline__ bool step(const DevScene& S, uint32_t* lds, uint32_t* spill, uint32_t spill_stride,
                                          uint32_t gtid, TraverseStats& st) {
-        const uint32_t lane = threadIdx.x, bdim = blockDim.x;
-        const float4* np = S.nodes + 4 * (size_t)node;
-        const float4 a = np[0], b = np[1], c = np[2];
-        const int4 m = *reinterpret_cast<const int4*>(np + 3);
+        const float4* np = S.nodes + 3 * (size_t)node;
+        const float4 w0 = np[0], w1 = np[1], w2 = np[2];
         if (COUNT) st.nodes++;
-        float tn0, tn1;
+        const uint32_t x0 = __float_as_uint(w0.w), wi = __float_as_uint(w2.z);
+        const float sx = __uint_as_float((x0 & 0xffu) << 23), sy = __uint_as_float(((x0 >> 8) & 0xffu) << 23),
+                    sz = __uint_as_float(((x0 >> 16) & 0xffu) << 23);
+        const uint32_t types = (x0 >> 24) | ((wi >> 28) << 8);
+        const uint32_t base_inner = wi & 0x0fffffffu, base_tri = __float_as_uint(w2.w);
+        const uint32_t qlx = __float_as_uint(w1.x), qly = __float_as_uint(w1.y), qlz = __float_as_uint(w1.z);
+        const uint32_t qhx = __float_as_uint(w1.w), qhy = __float_as_uint(w2.x), qhz = __float_as_uint(w2.y);
         const v3 o = wr.o;
-        bool h0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, h.t, tn0) && m.z >= 0;
-        bool h1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, h.t, tn1) && m.w >= 0;
+        float tk[4];
+        uint32_t ck[4];
+        uint32_t tri_off = 0, inner_rank = 0, trimask = 0;
 #pragma unroll
-        for (int side = 0; side < 2; side++) {
-            const bool hs = side == 0 ? h0 : h1;
-            const int cnt = side == 0 ? m.z : m.w;
-            if (hs && cnt > 0) {
-                const int first = side == 0 ? m.x : m.y;
-                for (int k = 0; k < cnt; k++) {
-                    const float4* tp = S.tris + 3 * (size_t)(first + k);
-                    const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                    if (COUNT) st.tris++;
-                    float t, b1, b2;
-                    if (!wt_intersect(wr, mk(t0.x, t0.y, t0.z), mk(t1.x, t1.y, t1.z), mk(t2.x, t2.y, t2.z), t, b1, b2))
-                        continue;
-                    const uint32_t id = __float_as_uint(t0.w);
-                    if (t < h.t || (t == h.t && h.hit && id < h.id)) {
-                        h.t = t;
-                        h.b1 = b1;
-                        h.b2 = b2;
-                        h.id = id;
-                        h.hit = true;
-                        if (ANY) return false;
-                    }
+        for (int c = 0; c < 4; c++) {
+            const uint32_t type = (types >> (3 * c)) & 7u;
+            const int sh = 8 * c;
+            const float lx = w0.x + (float)((qlx >> sh) & 0xffu) * sx, hx = w0.x + (float)((qhx >> sh) & 0xffu) * sx;
+            const float ly = w0.y + (float)((qly >> sh) & 0xffu) * sy, hy = w0.y + (float)((qhy >> sh) & 0xffu) * sy;
+            const float lz = w0.z + (float)((qlz >> sh) & 0xffu) * sz, hz = w0.z + (float)((qhz >> sh) & 0xffu) * sz;
+            float tn;
+            const bool hit = type != 0 && slab(lx, hx, ly, hy, lz, hz, o, inv, h.t, tn);
+            tk[c] = __int_as_float(0x7f800000);
+            ck[c] = 0;
+            if (type == 7) {
+                if (hit) {
+                    tk[c] = tn;
+                    ck[c] = base_inner + inner_rank;
                 }
-                if (side == 0) h0 = false;
-                else h1 = false;
+                inner_rank++;
+            } else {
+                // leaf: mark its triangles (offsets tri_off .. tri_off + type - 1, at most 16 per node)
+                if (hit) trimask |= ((1u << type) - 1u) << tri_off;
+                tri_off += type;
             }
         }
-        if (h0 && h1) {
-            const bool first0 = tn0 <= tn1;
-            const uint32_t near = first0 ? (uint32_t)m.x : (uint32_t)m.y;
-            const uint32_t far = first0 ? (uint32_t)m.y : (uint32_t)m.x;
-            if (sp < kLdsStack) lds[sp * bdim + lane] = far;
-            else spill[(size_t)(sp - kLdsStack) * spill_stride + gtid] = far;
-            sp++;
-            node = near;
-        } else if (h0) {
-            node = (uint32_t)m.x;
-        } else if (h1) {
-            node = (uint32_t)m.y;
-        } else {
-            if (sp == 0) return false;
-            sp--;
-            node = sp < kLdsStack ? lds[sp * bdim + lane] : spill[(size_t)(sp - kLdsStack) * spill_stride + gtid];
+        // one loop over all marked triangles keeps the lanes of a wavefront
+        // that have leaf work active together
+        while (trimask) {
+            const uint32_t j = (uint32_t)__builtin_ctz(trimask);
+            trimask &= trimask - 1u;
+            if (COUNT) st.tris++;
+            if (tri_test(S, base_tri + j) && ANY) return false;
         }
+        // sort the (at most 4) inner candidates by entry distance
+#define ST_CSWAP(i, j)                                   \
+    if (tk[j] < tk[i]) {                                 \
+        const float tt = tk[i];                          \
+        tk[i] = tk[j];                                   \
+        tk[j] = tt;                                      \
+        const uint32_t cc = ck[i];                       \
+        ck[i] = ck[j];                                   \
+        ck[j] = cc;                                      \
+    }
+        ST_CSWAP(0, 1) ST_CSWAP(2, 3) ST_CSWAP(0, 2) ST_CSWAP(1, 3) ST_CSWAP(1, 2)
+#undef ST_CSWAP
+        const float inf = __int_as_float(0x7f800000);
+        if (tk[3] < inf) push(ck[3], lds, spill, spill_stride, gtid);
+        if (tk[2] < inf) push(ck[2], lds, spill, spill_stride, gtid);
+        if (tk[1] < inf) push(ck[1], lds, spill, spill_stride, gtid);
+        if (tk[0] < inf) {
+            node = ck[0];
+            return true;
+        }
+        if (sp == 0) return false;
+        sp--;
+        node = sp < kLdsStack ? lds[sp * (int)blockDim.x + (int)threadIdx.x]
+                              : spill[(size_t)(sp - kLdsStack) * spill_stride + gtid];
         return true;
     }
 };
