@@ -34,6 +34,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--samples-per-wave", type=int, default=0, help="0: library default")
+    ap.add_argument("--no-restir", action="store_true", help="plain NEE instead of ReSTIR DI")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -58,7 +60,10 @@ def main():
     log(f"scene uploaded + BVH built in {time.time() - t0:.1f}s: {sc.stats()}")
     r = TestRenderer(sc)
     flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
-    params = abi.default_params(args.width, args.height, max_depth=8, spp=args.spp, flags=flags, rr_start_depth=3)
+    if not args.no_restir:  # ReSTIR DI: 32 candidates, temporal + spatial (5 neighbours, 30 px) reuse
+        flags |= abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
+    params = abi.default_params(args.width, args.height, max_depth=8, spp=args.spp, flags=flags, rr_start_depth=3,
+                                samples_per_wave=args.samples_per_wave)
 
     # traversal statistics for the algorithmic-bytes model (untimed, 1 spp)
     cp = abi.default_params(args.width, args.height, max_depth=8, spp=1, flags=flags | abi.ST_FLAG_COUNT_TRAVERSAL,
@@ -117,9 +122,11 @@ def main():
         "metric": "Mrays/s 1080p/1M tris", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": "1920x1080 64spp maxPathLength 8 (RR from 3), 1,004,098 tris, 4096 emissive tris, "
-                               "1024x512 env alias-sampled" if (args.width, args.height, args.spp) == (1920, 1080, 64)
-                   else f"{args.width}x{args.height} {args.spp}spp maxPathLength 8",
+        "config": {"workload": ("1920x1080 64spp maxPathLength 8 (RR from 3), 1,004,098 tris, 4096 emissive tris, "
+                                "1024x512 env alias-sampled" if (args.width, args.height, args.spp) == (1920, 1080, 64)
+                                else f"{args.width}x{args.height} {args.spp}spp maxPathLength 8")
+                               + ("" if args.no_restir else ", ReSTIR DI 32 candidates + temporal + spatial 5x30px"),
+                   "restir_di": not args.no_restir,
                    "width": args.width, "height": args.height, "spp": args.spp, "max_path_length": 8,
                    "triangles": sd.triangle_count, "samples_per_px_per_s": round(args.width * args.height * args.spp / (ms / 1e3), 1)},
         "roofline": roofline,

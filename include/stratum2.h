@@ -128,12 +128,14 @@ typedef struct st_view {
 #define ST_FLAG_LAMBERTIAN              (1u << 0)  /* gLambertian: all materials Lambert */
 #define ST_FLAG_SAMPLE_DIRECT           (1u << 1)  /* gSampleDirectIllumination (default on) */
 #define ST_FLAG_RESTIR_DI               (1u << 2)  /* gReSTIR_DI: RIS over candidate_samples */
-#define ST_FLAG_RESTIR_DI_REUSE         (1u << 3)  /* screen-space temporal + spatial reuse */
+#define ST_FLAG_RESTIR_DI_REUSE         (1u << 3)  /* screen-space temporal + spatial reuse of the
+                                                       primary-vertex reservoirs (deeper vertices: plain NEE) */
 #define ST_FLAG_ENV_ALIAS               (1u << 4)  /* alias-table env sampling (else mip warp) */
 #define ST_FLAG_ENV_MIS_BALANCED        (1u << 5)  /* env hits weighted against env pdfW */
 #define ST_FLAG_FLOAT_NEE               (1u << 6)  /* NEE added in fp32, not 1/16384 fixed point */
 #define ST_FLAG_PIXEL_JITTER            (1u << 7)  /* gPixelJitter */
 #define ST_FLAG_COUNT_TRAVERSAL         (1u << 8)  /* count nodes/triangles visited (gCountRays-like) */
+#define ST_FLAG_DIRECT_ONLY             (1u << 9)  /* gSampleDirectIlluminationOnly: NEE, then stop */
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface

@@ -1281,7 +1281,7 @@ static void connect_to_light(pctx_t* x, const material_t* m, const shading_t* sd
     float cp = m_cont_prob(m);
     f.fwd *= cp; f.rev *= cp;
     f3 contrib = divf(mul(mul(x->ps.thr, r.radiance), f.f), dpw);
-    float w = mis2(r.pdf, f.fwd / AtoW);
+    float w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, f.fwd / AtoW); /* ComputeMIS_DI :530-566 */
     if (x->ps.path_len + 1 < x->c->min_len) return;
     if (x->nsh >= 64) return;
     shadow_t* sr = &x->sh[x->nsh++];
@@ -1314,7 +1314,7 @@ static void rsd_init(rsd_t* r, pctx_t* x, const material_t* m, const shading_t* 
         r->contrib = mul(f.f, L.radiance);
         r->wtoa = L.finite ? L.cos_light / pow2(L.dist) : 1;
         float AtoW = L.finite ? pow2(L.dist) / L.cos_light : 1;
-        r->mis_w = mis2(L.pdf, f.fwd / AtoW); /* ComputeMIS_DI, testrenderer.slang:530-566 */
+        r->mis_w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(L.pdf, f.fwd / AtoW); /* ComputeMIS_DI, :530-566 */
         r->vo = ray_offset_dir(sd->pos, unpack_normal(sd->gn), L.dir);
         r->vd = L.dir;
         r->vtmax = maxf(L.dist * 0.99f, L.dist - 1e-2f);
@@ -1412,6 +1412,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
         if (x->c->flags & ST_FLAG_RESTIR_DI) connect_to_light_restir(x, &m, &sd, ldin);
         else connect_to_light(x, &m, &sd, ldin);
     }
+    if (x->c->flags & ST_FLAG_DIRECT_ONLY) return 0; /* gSampleDirectIlluminationOnly, testrenderer.slang:886-888 */
     float cp = minf(1, m_cont_prob(&m));
     float rr[4];
     rng_nextf(&ps->rng, rr);

@@ -23,6 +23,7 @@ ST_FLAG_ENV_MIS_BALANCED = 1 << 5
 ST_FLAG_FLOAT_NEE = 1 << 6
 ST_FLAG_PIXEL_JITTER = 1 << 7
 ST_FLAG_COUNT_TRAVERSAL = 1 << 8
+ST_FLAG_DIRECT_ONLY = 1 << 9
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 

@@ -69,12 +69,17 @@ struct WaveState {
     uint32_t* queue[2]; // live path ids, ping-pong
     float4* shadow;     // per shadow record: {o, tmax}, {d, path}, {contrib, weight}
     uint32_t* counters; // [0..1] queue counts (ping-pong), [2] shadow count, [4..5] u64 rays
+    // ReSTIR DI screen-space reuse (one sample per wave), per pixel
+    uint4* pri;         // primary hit: id, b1, b2, packed ray direction
+    float4* r_init;     // 2 per pixel: light-sample seed (float4), {target, W, M, depth}
+    float4* r_temp;     // after temporal reuse
+    float4* r_prev;     // after spatial reuse: this frame's result, next frame's history
 };
 
 struct RenderConsts {
-    uint32_t W, H, npix, wave_paths, k, sample_base, seed, max_depth, min_len, flags, rr_start, cand, tiled;
+    uint32_t W, H, npix, wave_paths, k, sample_base, seed, max_depth, min_len, flags, rr_start, cand, tiled, spatial;
     float env_prob;
-    float max_m;
+    float max_m, radius;
     st_view view;
 };
 
@@ -258,7 +263,7 @@ __device__ __forceinline__ bool eval_candidate(const DevScene& S, const RenderCo
     fe.rev *= cp_eff;
     x.contrib = fe.f * L.radiance;
     x.wtoa = L.finite ? L.cos_light / sq(L.dist) : 1.0f;
-    x.mis_w = mis2(L.pdf, fe.fwd / AtoW);  // ComputeMIS_DI, testrenderer.slang:530-566
+    x.mis_w = (c.flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(L.pdf, fe.fwd / AtoW);  // ComputeMIS_DI, :530-566
     x.src_pdf = L.pdf;
     return true;
 }
@@ -285,7 +290,7 @@ __device__ __forceinline__ bool next_event(const DevScene& S, const RenderConsts
         fe.fwd *= cp_eff;
         fe.rev *= cp_eff;
         contrib = ((thr * L.radiance) * fe.f) / dpw;
-        wgt_out = mis2(L.pdf, fe.fwd / AtoW);
+        wgt_out = (c.flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(L.pdf, fe.fwd / AtoW);
         sel.dir = L.dir;
         sel.dist = L.dist;
     } else {
@@ -320,6 +325,58 @@ __device__ __forceinline__ bool next_event(const DevScene& S, const RenderConsts
     sh1 = make_float4(sel.dir.x, sel.dir.y, sel.dir.z, __uint_as_float(p));
     sh2 = make_float4(contrib.x, contrib.y, contrib.z, wgt_out);
     return true;
+}
+
+// Resampled importance sampling over c.cand candidates (SampleIlluminationReSTIR,
+// testrenderer.slang:665-684) keeping the winning seed for reuse.
+template <bool LAMBERT, bool ALIAS>
+__device__ __forceinline__ void ris_reservoir(const DevScene& S, const RenderConsts& c, Rng& rng, const Material& m,
+                                              const Frame& f, v3 lgn, v3 ldin, v3 pos, float cp_eff, float4& seed,
+                                              float& target, float& W) {
+    float wsum = 0;
+    target = 0;
+    seed = make_float4(0, 0, 0, 0);
+    for (uint32_t i = 0; i < c.cand; i++) {
+        const float4 rnd = rng.nextf();
+        Cand x;
+        float lr;
+        const bool ok = eval_candidate<LAMBERT, ALIAS>(S, c, m, f, lgn, ldin, pos, rnd, cp_eff, x, lr);
+        const float t = (ok && lr > 0) ? lum(x.contrib) * x.wtoa : 0.0f;
+        if (t <= 0) continue;
+        wsum += t / x.src_pdf;
+        if (rng.nextf().x <= (t / x.src_pdf) / wsum) {
+            seed = rnd;
+            target = t;
+        }
+    }
+    W = target > 0 ? wsum / ((float)c.cand * target) : 0.0f;
+}
+
+// Shading point of a pixel's primary hit, rebuilt from its stored record.
+struct Prim {
+    Shading sd;
+    Material m;
+    Frame f;
+    v3 lgn, ldin;
+};
+__device__ __forceinline__ void rebuild_primary(const DevScene& S, uint4 rec, Prim& P) {
+    tri_shading(S, rec.x & 0xffffu, rec.x >> 16, __uint_as_float(rec.y), __uint_as_float(rec.z), P.sd);
+    P.m = load_material(S, P.sd.material);
+    P.f = make_frame(P.sd);
+    P.lgn = P.f.to_local(unpack_normal(P.sd.gn));
+    P.ldin = P.f.to_local(-unpack_normal(rec.w));
+}
+// target function of a light-sample seed at a shading point (0 when unusable)
+template <bool LAMBERT, bool ALIAS>
+__device__ __forceinline__ float seed_target(const DevScene& S, const RenderConsts& c, const Prim& P, float4 seed, Cand& x) {
+    float lr;
+    const float cp = 1 >= c.rr_start ? m_cont_prob(P.m) : 1.0f;  // primary vertex: path length 1
+    const bool ok = eval_candidate<LAMBERT, ALIAS>(S, c, P.m, P.f, P.lgn, P.ldin, P.sd.pos, seed, cp, x, lr);
+    return (ok && lr > 0) ? lum(x.contrib) * x.wtoa : 0.0f;
+}
+__device__ __forceinline__ float mis_pair(float a, float b) {  // mis(a, b), testrenderer.slang:287-290
+    const float a2 = a * a, d = a2 + b * b;
+    return d > 0 ? a2 / d : 0.0f;
 }
 
 template <bool LAMBERT, bool ALIAS, bool RESTIR>
@@ -372,6 +429,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 em.z += a.z;
             };
             if (!hit_any) {
+                if (RESTIR && (c.flags & ST_FLAG_RESTIR_DI_REUSE) && path_len == 1)
+                    w.r_init[2 * pix + 1] = make_float4(0, 0, 0, 0);
                 if (path_len == 1 && first_wave && k == 0 && visibility) {
                     visibility[2 * pix] = 0xffffffffu;
                     visibility[2 * pix + 1] = pack_normal(dir);
@@ -412,11 +471,27 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 // Russian roulette from path length rr_start (0: every bounce, testrenderer.slang:890-892)
                 const bool rr_on = path_len >= c.rr_start;
                 const float cp_eff = rr_on ? m_cont_prob(m) : 1.0f;
-                if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
-                    emit_shadow = next_event<LAMBERT, ALIAS, RESTIR>(S, c, rng, m, f, lgn, ldin, sd.pos, sd.gn, thr, cp_eff,
-                                                                    sh0, sh1, sh2, p) &&
+                const bool reuse = RESTIR && (c.flags & ST_FLAG_RESTIR_DI_REUSE) && path_len == 1;
+                if (reuse) {
+                    // primary vertex of the screen-space reuse mode: keep the RIS
+                    // reservoir; the shadow ray is made after temporal + spatial reuse
+                    float4 seed = make_float4(0, 0, 0, 0);
+                    float target = 0, W = 0;
+                    if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT))
+                        ris_reservoir<LAMBERT, ALIAS>(S, c, rng, m, f, lgn, ldin, sd.pos, cp_eff, seed, target, W);
+                    w.pri[pix] = make_uint4(hit.y, hit.z, hit.w, __float_as_uint(r.w));
+                    w.r_init[2 * pix] = seed;
+                    w.r_init[2 * pix + 1] = make_float4(target, W, target > 0 ? (float)c.cand : 0.0f, __uint_as_float(hit.x));
+                } else if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
+                    // the screen-space reuse mode keeps reservoirs at primary vertices only
+                    const bool ris_here = RESTIR && !(c.flags & ST_FLAG_RESTIR_DI_REUSE);
+                    emit_shadow = (ris_here ? next_event<LAMBERT, ALIAS, true>(S, c, rng, m, f, lgn, ldin, sd.pos, sd.gn, thr,
+                                                                              cp_eff, sh0, sh1, sh2, p)
+                                            : next_event<LAMBERT, ALIAS, false>(S, c, rng, m, f, lgn, ldin, sd.pos, sd.gn, thr,
+                                                                               cp_eff, sh0, sh1, sh2, p)) &&
                                   path_len + 1 >= c.min_len;
                 }
+                if (c.flags & ST_FLAG_DIRECT_ONLY) alive = false;  // gSampleDirectIlluminationOnly (:886-888)
                 if (alive) {
                     const float cp = fmin_(1.0f, m_cont_prob(m));
                     const float rr = rng.nextf().x;
@@ -460,6 +535,113 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
         }
         const uint32_t qi = wave_append(&w.counters[qsel ^ 1], cont);
         if (cont) qout[qi] = p;
+    }
+}
+
+// Temporal reuse: combine this frame's RIS reservoir with the previous frame's
+// final reservoir of the same pixel (ReservoirReuse, testrenderer.slang:608-663).
+// The camera is static and unjittered, so both live in the same domain.
+template <bool LAMBERT, bool ALIAS>
+__global__ void __launch_bounds__(kShadeBlock) k_restir_temporal(DevScene S, RenderConsts c, WaveState w, int has_prev) {
+    for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < c.npix; pix += gridDim.x * blockDim.x) {
+        float4 seed = w.r_init[2 * pix], b = w.r_init[2 * pix + 1];
+        if (b.z > 0 && has_prev) {
+            const float4 qs = w.r_prev[2 * pix], qb = w.r_prev[2 * pix + 1];
+            if (qb.z > 0) {
+                Prim P;
+                rebuild_primary(S, w.pri[pix], P);
+                Cand x;
+                const float tq = seed_target<LAMBERT, ALIAS>(S, c, P, qs, x);  // q's seed here
+                const uint32_t px = pix % c.W, py = pix / c.W;
+                Rng rng{px, py, c.seed + c.sample_base, 1u << 20};
+                const float m0 = mis_pair(b.z * b.x, qb.z * b.x), m1 = mis_pair(qb.z * qb.x, b.z * tq);
+                const float w0 = m0 * b.x * b.y, w1 = m1 * tq * qb.y;
+                const float total = w0 + w1;
+                float M = b.z + qb.z, target = b.x;
+                if (rng.nextf().x * total <= w1 && tq > 0) {
+                    seed = qs;
+                    target = tq;
+                }
+                const float Wn = target > 0 ? total / target : 0.0f;
+                b = make_float4(target, Wn, fmin_(M, c.max_m * (float)c.cand), b.w);
+            }
+        }
+        w.r_temp[2 * pix] = seed;
+        w.r_temp[2 * pix + 1] = b;
+    }
+}
+
+// Spatial reuse over c.spatial neighbours in a disc of c.radius pixels, then the
+// shadow ray of the final sample (ConnectToLight with ReSTIR, :730-747).
+template <bool LAMBERT, bool ALIAS>
+__global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, RenderConsts c, WaveState w) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t base0 = blockIdx.x * blockDim.x + threadIdx.x - __lane_id();
+    for (uint32_t i0 = base0; i0 < c.npix; i0 += stride) {
+        const uint32_t pix = i0 + __lane_id();
+        bool emit = false;
+        float4 sh0 = make_float4(0, 0, 0, 0), sh1 = sh0, sh2 = sh0;
+        if (pix < c.npix) {
+            float4 seed = w.r_temp[2 * pix], b = w.r_temp[2 * pix + 1];
+            if (b.z > 0) {
+                Prim P;
+                rebuild_primary(S, w.pri[pix], P);
+                const v3 nc = unpack_normal(P.sd.sn);
+                Cand cur;
+                seed_target<LAMBERT, ALIAS>(S, c, P, seed, cur);
+                const uint32_t px = pix % c.W, py = pix / c.W;
+                Rng rng{px, py, c.seed + c.sample_base, (1u << 20) + 64};
+                for (uint32_t n = 0; n < c.spatial; n++) {
+                    const float4 u = rng.nextf();
+                    float dx, dy;
+                    concentric_disc(u.x, u.y, dx, dy);
+                    const int qx = (int)px + (int)floorf(dx * c.radius + 0.5f), qy = (int)py + (int)floorf(dy * c.radius + 0.5f);
+                    if (qx < 0 || qy < 0 || qx >= (int)c.W || qy >= (int)c.H || (qx == (int)px && qy == (int)py)) continue;
+                    const uint32_t qpix = (uint32_t)qy * c.W + (uint32_t)qx;
+                    const float4 qs = w.r_temp[2 * qpix], qb = w.r_temp[2 * qpix + 1];
+                    if (!(qb.z > 0)) continue;
+                    // similar surfaces only: depth within 10%, normals within ~25 degrees
+                    if (fabsf(qb.w - b.w) > 0.1f * b.w) continue;
+                    Prim Q;
+                    rebuild_primary(S, w.pri[qpix], Q);
+                    if (dot(unpack_normal(Q.sd.sn), nc) < 0.9f) continue;
+                    Cand xn, xq;
+                    const float tn = seed_target<LAMBERT, ALIAS>(S, c, P, qs, xn);     // neighbour's seed here
+                    const float tc_q = seed_target<LAMBERT, ALIAS>(S, c, Q, seed, xq); // our seed there
+                    const float m0 = mis_pair(b.z * b.x, qb.z * tc_q), m1 = mis_pair(qb.z * qb.x, b.z * tn);
+                    const float w0 = m0 * b.x * b.y, w1 = m1 * tn * qb.y;
+                    const float total = w0 + w1;
+                    if (u.z * total <= w1 && tn > 0) {
+                        seed = qs;
+                        b.x = tn;
+                        cur = xn;
+                    }
+                    b.y = b.x > 0 ? total / b.x : 0.0f;
+                    b.z = fmin_(b.z + qb.z, c.max_m * (float)c.cand);
+                }
+                if (b.y > 0 && b.x > 0 && 2 >= c.min_len) {
+                    const float Wp = b.y * cur.wtoa;  // convert 1/pdfA to 1/pdfW
+                    const v3 contrib = cur.contrib * Wp;  // primary throughput is 1
+                    const v3 o = ray_offset(P.sd.pos, unpack_normal(P.sd.gn), cur.dir);
+                    const float tmax = fmax_(cur.dist * 0.99f, cur.dist - 1e-2f);
+                    const uint32_t p = path_of_pixel(c, px, py);  // one sample per wave
+                    sh0 = make_float4(o.x, o.y, o.z, tmax);
+                    sh1 = make_float4(cur.dir.x, cur.dir.y, cur.dir.z, __uint_as_float(p));
+                    sh2 = make_float4(contrib.x, contrib.y, contrib.z, cur.mis_w);
+                    emit = true;
+                }
+            } else {
+                b = make_float4(0, 0, 0, 0);
+            }
+            w.r_prev[2 * pix] = seed;
+            w.r_prev[2 * pix + 1] = b;
+        }
+        const uint32_t si = wave_append(&w.counters[2], emit);
+        if (emit) {
+            w.shadow[3 * (size_t)si] = sh0;
+            w.shadow[3 * (size_t)si + 1] = sh1;
+            w.shadow[3 * (size_t)si + 2] = sh2;
+        }
     }
 }
 
@@ -553,7 +735,7 @@ struct st_context {
     int device = 0;
     int cus = 256;
     hipStream_t stream = nullptr;
-    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, acc, stats, raysbuf, outbuf;
+    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, acc, stats, raysbuf, outbuf, pri, rinit, rtemp, rprev;
     double timings[12] = {0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     size_t ev_used[3] = {0, 0, 0};
@@ -907,8 +1089,10 @@ static void ensure_spill(st_context* ctx, int grid, int block) {
 int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if (!s || !view || !p || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
-    if (p->flags & ST_FLAG_RESTIR_DI_REUSE)
-        return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI reuse is not available in this build");
+    if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && !(p->flags & ST_FLAG_RESTIR_DI))
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");
+    if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && (p->spatial_neighbours > 64 || !(p->spatial_radius >= 0) || p->spatial_radius > 4096))
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "spatial reuse: at most 64 neighbours in a radius of at most 4096 px");
     if ((p->flags & ST_FLAG_RESTIR_DI) && (p->candidate_samples == 0 || p->candidate_samples > 1024))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "candidate_samples must be in [1, 1024]");
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
@@ -916,7 +1100,8 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
         const uint32_t npix = p->width * p->height;
-        uint32_t k = p->samples_per_wave;
+        const bool reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
+        uint32_t k = reuse ? 1 : p->samples_per_wave;  // temporal reuse chains consecutive samples
         if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, (uint32_t)((4u << 20) / npix)));
         k = std::min(k, p->spp);
         const uint64_t paths = (uint64_t)npix * k;
@@ -932,6 +1117,12 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         ctx->shadow.reserve(paths * 48);
         ctx->counters.reserve(256);
         ctx->acc.reserve((size_t)npix * 16);
+        if (reuse) {
+            ctx->pri.reserve((size_t)npix * 16);
+            ctx->rinit.reserve((size_t)npix * 32);
+            ctx->rtemp.reserve((size_t)npix * 32);
+            ctx->rprev.reserve((size_t)npix * 32);
+        }
         ctx->stats.reserve(64);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         ensure_spill(ctx, tgrid, kTraceBlock);
@@ -946,6 +1137,10 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         w.queue[1] = ctx->q1.as<uint32_t>();
         w.shadow = ctx->shadow.as<float4>();
         w.counters = ctx->counters.as<uint32_t>();
+        w.pri = ctx->pri.as<uint4>();
+        w.r_init = ctx->rinit.as<float4>();
+        w.r_temp = ctx->rtemp.as<float4>();
+        w.r_prev = ctx->rprev.as<float4>();
         hipStream_t sm = ctx->stream;
         HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 256, sm));
         HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, sm));
@@ -968,6 +1163,8 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
         c.cand = p->candidate_samples;
         c.max_m = p->max_m;
+        c.spatial = p->spatial_neighbours;
+        c.radius = p->spatial_radius;
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
         auto* stats = ctx->stats.as<unsigned long long>();
         for (uint32_t base = 0; base < p->spp; base += k) {
@@ -988,6 +1185,15 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                     uint32_t* vis = (base == 0) ? o->visibility : nullptr;
                     auto* kfn = shade_kernel(lambert, alias, restir);
                     hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0 ? 1 : 0);
+                }
+                if (reuse && b == 0) {
+                    const int rgrid2 = std::min<int>((npix + kShadeBlock - 1) / kShadeBlock, ctx->cus * 8);
+                    auto kt = lambert ? (alias ? k_restir_temporal<true, true> : k_restir_temporal<true, false>)
+                                      : (alias ? k_restir_temporal<false, true> : k_restir_temporal<false, false>);
+                    auto ks = lambert ? (alias ? k_restir_spatial<true, true> : k_restir_spatial<true, false>)
+                                      : (alias ? k_restir_spatial<false, true> : k_restir_spatial<false, false>);
+                    hipLaunchKernelGGL(kt, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, base > 0 ? 1 : 0);
+                    hipLaunchKernelGGL(ks, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w);
                 }
                 {
                     Timed tm(ctx, 2);

@@ -65,6 +65,14 @@ def test_cornell_lambertian(gpu_ctx):
     _check_image(g, r)
 
 
+def test_direct_only_mode(gpu_ctx):
+    sd = scenes.cornell_box(2)
+    W = H = 64
+    params = abi.default_params(W, H, max_depth=4, spp=2, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_DIRECT_ONLY)
+    g, r = _render_both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
+    _check_image(g, r)
+
+
 def test_min_path_length_gate(gpu_ctx):
     sd = scenes.cornell_box(1)
     W = H = 64
@@ -134,3 +142,36 @@ def test_restir_di_candidates(gpu_ctx):
     view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
     g, r = _render_both(gpu_ctx, sd, view, params)
     _check_image(g, r, frac_px=0.98, frac_len=0.98)
+
+
+def _gpu_render(gpu_ctx, sd, view, params):
+    import torch
+    from stratum2_amd.renderer import Scene, TestRenderer
+    sc = Scene(gpu_ctx, sd)
+    out = TestRenderer(sc).render(view, params)
+    torch.cuda.synchronize()
+    return out["hdr"].cpu().numpy()[..., :3]
+
+
+def test_restir_reuse_is_consistent(gpu_ctx):
+    """Screen-space temporal + spatial reuse (statistical, like the reference's
+    hash-grid reuse) of direct illumination (gSampleDirectIlluminationOnly): same
+    expectation as RIS alone, and per-frame error no worse.  (Temporal reuse
+    correlates consecutive samples, so accumulated multi-spp error is not
+    smaller; the numbers are printed.)"""
+    sd = scenes.spheres_field(seed=6, n_spheres=60, n_emitters=40, env=False)
+    W, H = 96, 64
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 14.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    ris = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_DIRECT_ONLY
+    ref = _gpu_render(gpu_ctx, sd, view, abi.default_params(W, H, max_depth=2, spp=1024, flags=ris, seed=1000))
+    res = {}
+    for spp in (1, 4, 64):
+        a = _gpu_render(gpu_ctx, sd, view, abi.default_params(W, H, max_depth=2, spp=spp, flags=ris))
+        b = _gpu_render(gpu_ctx, sd, view, abi.default_params(W, H, max_depth=2, spp=spp,
+                                                              flags=ris | abi.ST_FLAG_RESTIR_DI_REUSE))
+        res[spp] = (float(np.sqrt(((a - ref) ** 2).mean())), float(np.sqrt(((b - ref) ** 2).mean())), a.mean(), b.mean())
+        print(f"spp {spp}: rmse ris {res[spp][0]:.5f} reuse {res[spp][1]:.5f}; mean ref {ref.mean():.5f} "
+              f"ris {res[spp][2]:.5f} reuse {res[spp][3]:.5f}")
+    assert abs(res[64][3] - ref.mean()) < 0.03 * ref.mean()
+    assert abs(res[64][2] - ref.mean()) < 0.03 * ref.mean()
+    assert res[1][1] <= 1.05 * res[1][0]
