@@ -48,6 +48,12 @@ def lib():
         l.or_env_sample_texel_pdf.argtypes = [vp, vp]
         l.or_bsdf_eval.argtypes = [vp, C.c_int, vp, vp, vp]
         l.or_bsdf_sample.argtypes = [vp, C.c_int, vp, vp, vp]
+        l.or_nvdb_read.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        l.or_nvdb_read.restype = C.c_float
+        l.or_nvdb_world_to_index.argtypes = [vp, vp, C.c_int, vp]
+        l.or_hg_eval.argtypes = [C.c_float, vp, vp]
+        l.or_hg_eval.restype = C.c_float
+        l.or_hg_sample.argtypes = [C.c_float, C.c_float, C.c_float, vp, vp]
         _lib = l
     return _lib
 

@@ -273,6 +273,15 @@ struct or_scene {
     bnode* nodes;
     uint32_t nnodes;
     uint32_t* order;
+    /* media: volume instances (InstanceType::eVolume) and their NanoVDB grids */
+    uint32_t nvolinst;
+    uint32_t volinst[8];
+    st_transform* inv_xf;          /* inverse transforms of all instances */
+    uint32_t nvols;
+    uint8_t* vol[8];
+    uint64_t volsz[8];
+    uint32_t nvolinfo;
+    st_volume_info* volinfo;
 };
 
 static f3 xf_point(const st_transform* t, f3 v) {
@@ -842,7 +851,7 @@ float or_env_sample_texel_pdf(or_scene* s, const float uv[2]) {
 typedef struct {
     uint32_t flags;
     float env_prob;
-    uint32_t max_depth, min_len, cand;
+    uint32_t max_depth, min_len, cand, max_null, seed_px, seed_py, seed;
     int lambert;
 } cfg_t;
 
@@ -929,6 +938,169 @@ static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const float 
     r.cos_light = -dot(unpack_normal(r.normal), r.dir);
     if (r.cos_light <= 0) { memset(&r, 0, sizeof r); }
     return r;
+}
+
+/* ------------------------------------------------------------ NanoVDB */
+/* Float-grid read path of NanoVDB 32.3 (extern/nanovdb/PNanoVDB.h:1237-1560, 1988-2058):
+ * grid (672 B) -> tree (64 B) -> root (64 B + 32 B tiles) -> upper (32^3) -> lower (16^3) -> leaf (8^3). */
+static uint32_t rd32(const uint8_t* b, uint64_t o) { uint32_t v; memcpy(&v, b + o, 4); return v; }
+static uint64_t rd64(const uint8_t* b, uint64_t o) { uint64_t v; memcpy(&v, b + o, 8); return v; }
+static float rdf(const uint8_t* b, uint64_t o) { float v; memcpy(&v, b + o, 4); return v; }
+static uint64_t nvdb_root(const uint8_t* b) { return 672 + rd64(b, 672 + 24); }
+static float nvdb_read(const uint8_t* b, int i, int j, int k) {
+    const uint64_t root = nvdb_root(b);
+    const uint32_t ntiles = rd32(b, root + 24);
+    const uint64_t key = (uint64_t)((uint32_t)k >> 12) | ((uint64_t)((uint32_t)j >> 12) << 21) | ((uint64_t)((uint32_t)i >> 12) << 42);
+    for (uint32_t t = 0; t < ntiles; t++) {
+        const uint64_t tile = root + 64 + 32 * (uint64_t)t;
+        if (rd64(b, tile) != key) continue;
+        const int64_t child = (int64_t)rd64(b, tile + 8);
+        if (child == 0) return rdf(b, tile + 20);
+        const uint64_t up = root + (uint64_t)child;
+        const uint32_t n2 = (uint32_t)(((i & 4095) >> 7) << 10) + (uint32_t)(((j & 4095) >> 7) << 5) + (uint32_t)((k & 4095) >> 7);
+        if (!((rd32(b, up + 4128 + 4 * (n2 >> 5)) >> (n2 & 31)) & 1)) return rdf(b, up + 8256 + 8 * (uint64_t)n2);
+        const uint64_t lo = up + rd64(b, up + 8256 + 8 * (uint64_t)n2);
+        const uint32_t n1 = (uint32_t)(((i & 127) >> 3) << 8) + (uint32_t)(((j & 127) >> 3) << 4) + (uint32_t)((k & 127) >> 3);
+        if (!((rd32(b, lo + 544 + 4 * (n1 >> 5)) >> (n1 & 31)) & 1)) return rdf(b, lo + 1088 + 8 * (uint64_t)n1);
+        const uint64_t leaf = lo + rd64(b, lo + 1088 + 8 * (uint64_t)n1);
+        const uint32_t n0 = (uint32_t)((i & 7) << 6) + (uint32_t)((j & 7) << 3) + (uint32_t)(k & 7);
+        return rdf(b, leaf + 96 + 4 * (uint64_t)n0);
+    }
+    return rdf(b, root + 28); /* background */
+}
+static f3 nvdb_map(const uint8_t* b, uint64_t off, f3 v, int translate, int inverse) {
+    const uint64_t m = 296 + off;
+    if (inverse && translate) v = v3(v.x - rdf(b, 296 + 72), v.y - rdf(b, 296 + 76), v.z - rdf(b, 296 + 80));
+    f3 r = v3(v.x * rdf(b, m) + v.y * rdf(b, m + 4) + v.z * rdf(b, m + 8),
+              v.x * rdf(b, m + 12) + v.y * rdf(b, m + 16) + v.z * rdf(b, m + 20),
+              v.x * rdf(b, m + 24) + v.y * rdf(b, m + 28) + v.z * rdf(b, m + 32));
+    if (!inverse && translate) r = v3(r.x + rdf(b, 296 + 72), r.y + rdf(b, 296 + 76), r.z + rdf(b, 296 + 80));
+    return r;
+}
+static f3 nvdb_world_to_index(const uint8_t* b, f3 v) { return nvdb_map(b, 36, v, 1, 1); }
+static f3 nvdb_world_to_index_dir(const uint8_t* b, f3 v) { return nvdb_map(b, 36, v, 0, 1); }
+static f3 nvdb_index_to_world(const uint8_t* b, f3 v) { return nvdb_map(b, 0, v, 1, 0); }
+static f3 nvdb_index_to_world_dir(const uint8_t* b, f3 v) { return nvdb_map(b, 0, v, 0, 0); }
+float or_nvdb_read(const void* grid, int i, int j, int k) { return nvdb_read((const uint8_t*)grid, i, j, k); }
+void or_nvdb_world_to_index(const void* grid, const float p[3], int dir, float out[3]) {
+    const f3 r = dir ? nvdb_world_to_index_dir((const uint8_t*)grid, v3(p[0], p[1], p[2])) : nvdb_world_to_index((const uint8_t*)grid, v3(p[0], p[1], p[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* Medium record (Material.hpp:43-59), medium.hlsli:17-40 */
+typedef struct {
+    f3 density;
+    uint32_t albedo_packed, dens_vol, alb_vol;
+    float g;
+} medium_t;
+static medium_t load_medium(const or_scene* s, uint32_t addr) {
+    medium_t m;
+    memset(&m, 0, sizeof m);
+    if ((uint64_t)addr / 4 + 7 > s->nmatw) return m;
+    const uint32_t* w = s->mat + addr / 4;
+    m.density = v3(asfloat(w[0]), asfloat(w[1]), asfloat(w[2]));
+    m.albedo_packed = w[3];
+    m.dens_vol = w[4];
+    m.alb_vol = w[5];
+    m.g = asfloat(w[6]);
+    return m;
+}
+static f3 med_albedo(const medium_t* m) { return v3(un8(m->albedo_packed, 0), un8(m->albedo_packed, 8), un8(m->albedo_packed, 16)); }
+static int med_can_eval(const medium_t* m) { return any_gt0(m->density) && m->albedo_packed > 0; }
+static int med_singular(const medium_t* m) { return fabsf(m->g) > 0.999f; }
+static float med_cont_prob(const medium_t* m) { return saturate(luminance(mul(med_albedo(m), m->density))); }
+/* Henyey-Greenstein evaluateReflectance / sampleDirection, medium.hlsli:44-78 */
+static float hg_eval(float g, f3 din, f3 dout) {
+    return 1 / (4 * OR_PI) * (1 - g * g) / powf(1 + g * g + 2 * g * dot(din, dout), 1.5f);
+}
+static f3 hg_sample(float g, float r0, float r1, f3 din) {
+    if (fabsf(g) < 1e-4f) return sample_uniform_sphere(r0, r1);
+    const float g2 = g * g;
+    const float tmp = (g2 - 1) / (2 * r0 * g - (g + 1));
+    const float ce = (tmp * tmp - (1 + g2)) / (2 * g);
+    const float se = sqrtf(maxf(1 - ce * ce, 0));
+    const float az = 2 * OR_PI * r1;
+    f3 fr[3];
+    make_orthonormal(din, fr);
+    return add(add(mulf(fr[0], se * cosf(az)), mulf(fr[1], se * sinf(az))), mulf(din, ce));
+}
+float or_hg_eval(float g, const float din[3], const float dout[3]) { return hg_eval(g, v3(din[0], din[1], din[2]), v3(dout[0], dout[1], dout[2])); }
+void or_hg_sample(float g, float r0, float r1, const float din[3], float out[3]) {
+    const f3 r = hg_sample(g, r0, r1, v3(din[0], din[1], din[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+static f3 expv(f3 a) { return v3(expf(a.x), expf(a.y), expf(a.z)); }
+static float avg3(f3 a) { return (a.x + a.y + a.z) / 3; }
+/* deltaTrack<Scattering>, medium.hlsli:104-181.  origin/direction in object space. */
+static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_t* rng, f3 origin, f3 dir, float tmax, f3* beta,
+                      f3* dirPdf, f3* neePdf, int* scattered, uint32_t max_null) {
+    *scattered = 0;
+    *dirPdf = v3(1, 1, 1);
+    *neePdf = v3(1, 1, 1);
+    uint32_t u4[4];
+    rng_next(rng, u4);
+    const int ch = (int)(u4[0] % 3);
+    if (m->dens_vol == 0xFFFFFFFFu || m->dens_vol >= s->nvols) {
+        const f3 st = m->density;
+        const float sc = fget(st, ch);
+        if (sc < 1e-6f) return v3(0, 0, 0);
+        float r[4];
+        rng_nextf(rng, r);
+        const float t = -logf(1 - r[0]) / sc;
+        if (t < tmax && scattering) {
+            *scattered = 1;
+            *beta = mul(*beta, mul(mul(expv(mulf(neg(st), t)), st), med_albedo(m)));
+            const float e = expf(-sc * t) * sc;
+            *dirPdf = mulf(*dirPdf, e);
+            return add(origin, mulf(dir, t));
+        }
+        *beta = mul(*beta, expv(mulf(neg(st), tmax)));
+        *dirPdf = mulf(*dirPdf, expf(-sc * tmax));
+        return v3(0, 0, 0);
+    }
+    const uint8_t* g = s->vol[m->dens_vol];
+    const float rmax = rdf(g, nvdb_root(g) + 36);
+    const f3 maj = mulf(m->density, rmax);
+    if (fget(maj, ch) < 1e-6f) return v3(0, 0, 0);
+    origin = nvdb_world_to_index(g, origin);
+    dir = nvdb_world_to_index_dir(g, dir);
+    const float invc = 1 / fget(maj, ch);
+    const float invmax = 1 / max3f(maj);
+    for (uint32_t it = 0; it < max_null && any_gt0(*beta); it++) {
+        float r[4];
+        rng_nextf(rng, r);
+        const float t = -logf(1 - r[0]) * invc;
+        if (t >= tmax) {
+            const f3 tr = expv(mulf(neg(maj), tmax));
+            *beta = mul(*beta, tr);
+            *neePdf = mul(*neePdf, tr);
+            *dirPdf = mul(*dirPdf, tr);
+            break;
+        }
+        origin = add(origin, mulf(dir, t));
+        tmax -= t;
+        const float dv = nvdb_read(g, (int)floorf(origin.x), (int)floorf(origin.y), (int)floorf(origin.z));
+        const f3 sig_t = mulf(m->density, dv);
+        const f3 sig_s = med_albedo(m); /* readAlbedo is 1 without an albedo volume */
+        const f3 tr = mulf(expv(mulf(neg(maj), t)), invmax);
+        int real = 0;
+        if (scattering) {
+            float r2[4];
+            rng_nextf(rng, r2);
+            real = r2[0] < fget(sig_t, ch) * invc;
+        }
+        if (real) {
+            *beta = mul(*beta, mul(mul(tr, sig_t), sig_s));
+            *dirPdf = mul(*dirPdf, mul(tr, sig_t));
+            *scattered = 1;
+            return nvdb_index_to_world(g, origin);
+        }
+        const f3 d = sub(maj, sig_t);
+        *beta = mul(*beta, mul(tr, d));
+        *dirPdf = mul(*dirPdf, mul(tr, d));
+        *neePdf = mul(*neePdf, mul(tr, maj));
+    }
+    return v3(0, 0, 0);
 }
 
 /* ------------------------------------------------------------- BVH */
@@ -1069,6 +1241,48 @@ static hit_t trace(const or_scene* s, f3 o, f3 d, float tmax, int any, uint64_t*
     return h;
 }
 
+/* traceRay over triangles and the procedural volume boxes, intersection.hlsli:68-193.
+ * *vol_gn receives the packed box normal when the closest hit is a volume box. */
+static hit_t trace_ray(const or_scene* s, f3 o, f3 d, float tmax, int any, int* is_vol, uint32_t* vol_gn, uint64_t* nodes,
+                       uint64_t* tris) {
+    hit_t h = trace(s, o, d, tmax, any, nodes, tris);
+    *is_vol = 0;
+    if (any && h.hit) return h;
+    for (uint32_t v = 0; v < s->nvolinst; v++) {
+        const uint32_t ii = s->volinst[v];
+        const f3 oo = xf_point(&s->inv_xf[ii], o), od = xf_vector(&s->inv_xf[ii], d);
+        const uint32_t vidx = s->inst[ii].data;
+        f3 org, dir, bmin, bmax;
+        const uint8_t* g = NULL;
+        if (vidx == 0xFFFFFFFFu || vidx >= s->nvols) {
+            org = oo; dir = od; bmin = v3(-1, -1, -1); bmax = v3(1, 1, 1);
+        } else {
+            g = s->vol[vidx];
+            const uint64_t root = nvdb_root(g);
+            org = nvdb_world_to_index(g, oo);
+            dir = nvdb_world_to_index_dir(g, od);
+            bmin = v3((float)(int32_t)rd32(g, root), (float)(int32_t)rd32(g, root + 4), (float)(int32_t)rd32(g, root + 8));
+            bmax = add(v3((float)(int32_t)rd32(g, root + 12), (float)(int32_t)rd32(g, root + 16), (float)(int32_t)rd32(g, root + 20)),
+                       v3(1, 1, 1));
+        }
+        const f3 t0 = divv(sub(bmin, org), dir), t1 = divv(sub(bmax, org), dir);
+        const float sx = maxf(maxf(minf(t0.x, t1.x), minf(t0.y, t1.y)), minf(t0.z, t1.z));
+        const float sy = minf(minf(maxf(t0.x, t1.x), maxf(t0.y, t1.y)), maxf(t0.z, t1.z));
+        if (!(sx <= sy)) continue; /* the box itself is missed */
+        const float t = sx > 0 ? sx : sy;
+        if (!(t < h.t && t > 0)) continue;
+        f3 ln = sub(v3(t1.x == t, t1.y == t, t1.z == t), v3(t0.x == t, t0.y == t, t0.z == t));
+        if (g) ln = nvdb_index_to_world_dir(g, ln);
+        *vol_gn = pack_normal(normalize3(xf_vector(&s->xf[ii], ln)));
+        h.t = t;
+        h.id = ii | (0xFFFFu << 16);
+        h.hit = 1;
+        h.b1 = h.b2 = 0;
+        *is_vol = 1;
+    }
+    return h;
+}
+
 or_scene* or_scene_create(const st_scene_desc* d) {
     if (!d || (d->instance_count && (!d->instances || !d->transforms))) { fail("invalid scene description"); return NULL; }
     or_scene* s = (or_scene*)calloc(1, sizeof *s);
@@ -1109,6 +1323,37 @@ or_scene* or_scene_create(const st_scene_desc* d) {
             build_mips(s);
         }
     }
+    /* media: volume instances, their grids and the inverse transforms (TransformData::inverse) */
+    s->inv_xf = (st_transform*)calloc(s->ninst + 1, sizeof(st_transform));
+    for (uint32_t i = 0; i < s->ninst; i++) {
+        const st_transform* T = &s->xf[i];
+        double a[3][3], inv[3][3];
+        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) a[r][c] = T->m[r][c];
+        double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                     a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+        if (det == 0) continue;
+        inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det; inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+        inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det; inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+        inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det; inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+        inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det; inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+        inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+        for (int r = 0; r < 3; r++) {
+            double tr = 0;
+            for (int c = 0; c < 3; c++) { s->inv_xf[i].m[r][c] = (float)inv[r][c]; tr -= inv[r][c] * T->m[c][3]; }
+            s->inv_xf[i].m[r][3] = (float)tr;
+        }
+    }
+    s->nvols = d->volume_count < 8 ? d->volume_count : 8;
+    for (uint32_t v = 0; v < s->nvols; v++) {
+        s->volsz[v] = d->volumes[v].size_bytes;
+        s->vol[v] = (uint8_t*)malloc(s->volsz[v] + 16);
+        memcpy(s->vol[v], d->volumes[v].data, s->volsz[v]);
+    }
+    for (uint32_t i = 0; i < s->ninst && s->nvolinst < 8; i++)
+        if ((s->inst[i].type_material_address & 0xfu) == 2) s->volinst[s->nvolinst++] = i;
+    s->nvolinfo = d->volume_info_count;
+    s->volinfo = (st_volume_info*)calloc(s->nvolinfo + 1, sizeof(st_volume_info));
+    if (s->nvolinfo) memcpy(s->volinfo, d->volume_info, sizeof(st_volume_info) * s->nvolinfo);
     /* lights, scene AABB and flattened triangles */
     s->light_inst = (uint32_t*)malloc(sizeof(uint32_t) * (s->ninst + 1));
     float amin[3] = {INFINITY, INFINITY, INFINITY}, amax[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -1121,6 +1366,25 @@ or_scene* or_scene_create(const st_scene_desc* d) {
     for (uint32_t i = 0; i < s->ninst; i++) {
         const st_instance* in = &s->inst[i];
         uint32_t type = in->type_material_address & 0xfu;
+        if (type == 2) { /* volume box corners, Scene.cpp:700-711 */
+            f3 mn = v3(-1, -1, -1), mx = v3(1, 1, 1);
+            if (in->data < s->nvols) {
+                double b[6];
+                memcpy(b, s->vol[in->data] + 560, 48);
+                mn = v3((float)b[0], (float)b[1], (float)b[2]);
+                mx = v3((float)b[3], (float)b[4], (float)b[5]);
+            }
+            /* as the reference does, the running min/max also feed the later
+             * corners (Scene.cpp:701-709 updates mn/mx inside the corner loop) */
+            for (int c = 0; c < 8; c++) {
+                f3 corner = v3((c % 2) == 0 ? mn.x : mx.x, ((c % 4) / 2) == 0 ? mn.y : mx.y, (c / 4) == 0 ? mn.z : mx.z);
+                corner = xf_point(&s->xf[i], corner);
+                mn = v3(minf(mn.x, corner.x), minf(mn.y, corner.y), minf(mn.z, corner.z));
+                mx = v3(maxf(mx.x, corner.x), maxf(mx.y, corner.y), maxf(mx.z, corner.z));
+            }
+            amin[0] = minf(amin[0], mn.x); amin[1] = minf(amin[1], mn.y); amin[2] = minf(amin[2], mn.z);
+            amax[0] = maxf(amax[0], mx.x); amax[1] = maxf(amax[1], mx.y); amax[2] = maxf(amax[2], mx.z);
+        }
         if (type != 0) continue;
         material_t m = load_material(s, in->type_material_address >> 4);
         f3 e = v3(un8(m.p[1], 0), un8(m.p[1], 8), un8(m.p[1], 16));
@@ -1176,6 +1440,8 @@ void or_scene_destroy(or_scene* s) {
     free(s->vb); free(s->vbsz);
     for (int l = 0; l < 16; l++) free(s->lv[l].rgba);
     free(s->light_inst); free(s->tv); free(s->tid); free(s->nodes); free(s->order);
+    free(s->inv_xf); free(s->volinfo);
+    for (uint32_t v = 0; v < s->nvols; v++) free(s->vol[v]);
     free(s);
 }
 
@@ -1235,9 +1501,12 @@ typedef struct {
     uint32_t inst_prim; /* mInstancePrimitiveIndex */
     float t16, pdfa16, dirpdf16; /* FP16-stored fields, kept as their f32 values */
     rng_t rng;
+    uint32_t medium;    /* mCurrentMedium */
 } pstate_t; /* PathState, testrenderer.slang:66-125 */
 
-typedef struct { f3 o, d, c; float tmax, w; } shadow_t;
+/* ShadowRay (testrenderer.slang:127-142) + the vertex medium and the key of
+ * its own random stream for ratio tracking */
+typedef struct { f3 o, d, c; float tmax, w; uint32_t medium, plen; } shadow_t;
 
 typedef struct {
     const or_scene* s;
@@ -1249,6 +1518,7 @@ typedef struct {
     int nsh;
     uint32_t vis[2];
     uint64_t rays[2], trav[4];
+    uint32_t view_medium;
 } pctx_t;
 
 /* AddRadiance, testrenderer.slang:220-235 */
@@ -1290,6 +1560,8 @@ static void connect_to_light(pctx_t* x, const material_t* m, const shading_t* sd
     sr->tmax = maxf(r.dist * 0.99f, r.dist - 1e-2f); /* AdjustIntersectionDistance, intersection.hlsli:65 */
     sr->c = contrib;
     sr->w = w;
+    sr->medium = x->ps.medium;
+    sr->plen = x->ps.path_len;
 }
 
 /* ReservoirSampleDataDI, testrenderer.slang:568-606 */
@@ -1349,6 +1621,154 @@ static void connect_to_light_restir(pctx_t* x, const material_t* m, const shadin
     sr->tmax = sample.vtmax;
     sr->c = contrib;
     sr->w = sample.mis_w;
+    sr->medium = x->ps.medium;
+    sr->plen = x->ps.path_len;
+}
+
+/* traceScatteringRay, intersection.hlsli:195-260: closest hits, delta tracking
+ * through the current medium, medium boundaries crossed in place.
+ * Returns 0 miss, 1 surface hit (h), 2 medium scattering at *pos. */
+static int trace_scattering(pctx_t* x, f3 origin, f3 dir, hit_t* h, f3* pos, float* dirpdf, float* neepdf) {
+    const or_scene* s = x->s;
+    pstate_t* ps = &x->ps;
+    *dirpdf = 1;
+    *neepdf = 1;
+    f3 o = origin;
+    float tmax = INFINITY;
+    while (tmax > 1e-6f) {
+        int is_vol = 0;
+        uint32_t vgn = 0;
+        *h = trace_ray(s, o, dir, tmax, 0, &is_vol, &vgn, &x->trav[0], &x->trav[1]);
+        x->rays[0]++;
+        if (ps->medium != 0xFFFFu) {
+            const medium_t m = load_medium(s, s->inst[ps->medium].type_material_address >> 4);
+            const st_transform* iT = &s->inv_xf[ps->medium];
+            f3 dp3, np3;
+            int sc;
+            f3 p = delta_track(s, &m, 1, &ps->rng, xf_point(iT, o), xf_vector(iT, dir), h->hit ? h->t : tmax, &ps->thr, &dp3, &np3,
+                               &sc, x->c->max_null);
+            *dirpdf *= avg3(dp3);
+            *neepdf *= avg3(np3);
+            if (sc) {
+                *pos = xf_point(&s->xf[ps->medium], p);
+                h->t = length3(sub(*pos, origin));
+                h->id = ps->medium | (0xFFFFu << 16);
+                return 2;
+            }
+        }
+        if (!h->hit) {
+            h->t = tmax;
+            return 0;
+        }
+        if (!is_vol) return 1;
+        const f3 bp = add(o, mulf(dir, h->t));
+        const f3 ng = unpack_normal(vgn);
+        if (dot(ng, dir) > 0) ps->medium = 0xFFFFu;
+        else ps->medium = h->id & 0xFFFFu;
+        o = ray_offset_dir(bp, ng, dir);
+        tmax -= h->t;
+    }
+    h->hit = 0;
+    h->t = tmax;
+    return 0;
+}
+
+/* traceVisibilityRay with media, intersection.hlsli:262-338 (ratio tracking).
+ * The deferred-shadow variant of the reference reads an uninitialised static
+ * path state here (testrenderer.slang:1153-1158); this uses the vertex's
+ * medium and a random stream of its own, keyed by pixel, sample and bounce. */
+static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, uint32_t py, uint32_t seed, f3* beta, float* neepdf) {
+    const or_scene* s = x->s;
+    rng_t rng;
+    rng_init(&rng, seed, px, py, 0x40000000u | (sr->plen << 16));
+    *neepdf = 1;
+    uint32_t cur = sr->medium;
+    f3 o = sr->o;
+    float tmax = sr->tmax;
+    while (tmax > 1e-6f) {
+        int is_vol = 0;
+        uint32_t vgn = 0;
+        hit_t h = trace_ray(s, o, sr->d, tmax, 0, &is_vol, &vgn, &x->trav[2], &x->trav[3]);
+        x->rays[1]++;
+        if (h.hit && !is_vol) {
+            *beta = v3(0, 0, 0);
+            *neepdf = 0;
+            return;
+        }
+        if (cur != 0xFFFFu) {
+            const medium_t m = load_medium(s, s->inst[cur].type_material_address >> 4);
+            const st_transform* iT = &s->inv_xf[cur];
+            f3 dp3, np3;
+            int sc;
+            delta_track(s, &m, 0, &rng, xf_point(iT, o), xf_vector(iT, sr->d), h.hit ? h.t : tmax, beta, &dp3, &np3, &sc,
+                        x->c->max_null);
+            *neepdf *= avg3(np3);
+            if (!(beta->x != 0 && beta->y != 0 && beta->z != 0)) return;
+        }
+        if (!h.hit) return;
+        const f3 bp = add(o, mulf(sr->d, h.t));
+        const f3 ng = unpack_normal(vgn);
+        if (dot(ng, sr->d) < 0) {
+            cur = h.id & 0xFFFFu;
+            o = ray_offset(bp, neg(ng));
+        } else {
+            cur = 0xFFFFu;
+            o = ray_offset(bp, ng);
+        }
+        tmax -= h.t;
+    }
+}
+
+/* ProcessInteraction for a Medium vertex (testrenderer.slang:816-935 with isMedium). */
+static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
+    const or_scene* s = x->s;
+    pstate_t* ps = &x->ps;
+    const medium_t m = load_medium(s, s->inst[ps->medium].type_material_address >> 4);
+    const f3 ldin = neg(dir);
+    if (ps->path_len == 1) { x->vis[0] = ps->inst_prim; x->vis[1] = 0; }
+    if (ps->path_len >= x->c->max_depth) return 0;
+    if (!med_can_eval(&m)) return 0;
+    if (!med_singular(&m) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
+        float rnd[4];
+        rng_nextf(&ps->rng, rnd);
+        isr_t r = sample_illumination(s, x->c, rnd, pos);
+        if (!(all_le0(r.radiance) || r.pdf <= 0)) {
+            float AtoW = 1;
+            if (r.finite) AtoW = pow2(r.dist) / r.cos_light;
+            const float dpw = r.pdf * AtoW;
+            const float v = hg_eval(m.g, ldin, r.dir);
+            const float cp = med_cont_prob(&m);
+            const float fwd = v * cp;
+            const f3 contrib = divf(mulf(mul(ps->thr, r.radiance), v), dpw);
+            const float w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, fwd / AtoW);
+            if (ps->path_len + 1 >= x->c->min_len && x->nsh < 64) {
+                shadow_t* sr = &x->sh[x->nsh++];
+                sr->o = pos; /* rayOffset of a non-surface point is the point itself */
+                sr->d = r.dir;
+                sr->tmax = maxf(r.dist * 0.99f, r.dist - 1e-2f);
+                sr->c = contrib;
+                sr->w = w;
+                sr->medium = ps->medium;
+                sr->plen = ps->path_len;
+            }
+        }
+    }
+    if (x->c->flags & ST_FLAG_DIRECT_ONLY) return 0;
+    const float cp = minf(1, med_cont_prob(&m));
+    float rr[4];
+    rng_nextf(&ps->rng, rr);
+    if (rr[0] > cp) return 0;
+    float rb[4];
+    rng_nextf(&ps->rng, rb);
+    const f3 sdir = hg_sample(m.g, rb[0], rb[1], ldin);
+    const float v = hg_eval(m.g, ldin, sdir);
+    const float fwd = v * cp;
+    ps->thr = mulf(ps->thr, v / fwd);
+    ps->dirpdf16 = (1 - fabsf(m.g)) == 0 ? 0 : q16(fwd);
+    ps->dir = pack_normal(sdir);
+    ps->origin = pos;
+    if (ps->thr.x != ps->thr.x || ps->thr.y != ps->thr.y || ps->thr.z != ps->thr.z) return 0;
+    return 1;
 }
 
 /* TraceRay + ExtendPath + ProcessInteraction, testrenderer.slang:816-935, 1026-1071.
@@ -1358,17 +1778,41 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     const or_scene* s = x->s;
     pstate_t* ps = &x->ps;
     f3 dir = unpack_normal(ps->dir);
-    hit_t h = trace(s, ps->origin, dir, INFINITY, 0, &x->trav[0], &x->trav[1]);
-    x->rays[0]++;
+    hit_t h;
+    int kind;
+    f3 mpos = v3(0, 0, 0);
+    if (s->nvolinst > 0) {
+        float dirpdf, neepdf;
+        kind = trace_scattering(x, ps->origin, dir, &h, &mpos, &dirpdf, &neepdf);
+        ps->thr = divf(ps->thr, dirpdf); /* TraceRay, testrenderer.slang:1030 */
+    } else {
+        h = trace(s, ps->origin, dir, INFINITY, 0, &x->trav[0], &x->trav[1]);
+        x->rays[0]++;
+        kind = h.hit ? 1 : 0;
+    }
     ps->path_len++;
     shading_t sd;
     memset(&sd, 0, sizeof sd);
     float pick = 0;
+    if (kind == 2) {
+        /* medium scattering: pickPdf 0 and shape area 0 (intersection.hlsli:232-236) */
+        ps->inst_prim = h.id;
+        ps->t16 = q16(h.t);
+        float pdfa = 0;
+        if (s->nlights > 0) {
+            pdfa = 0.0f / (0.0f * (float)s->nlights);
+            if (s->has_env) pdfa *= 1 - x->c->env_prob;
+        }
+        ps->pdfa16 = q16(pdfa);
+        if (all_le0(ps->thr)) return 0;
+        return medium_interaction(x, mpos, dir);
+    }
     if (h.hit) {
         uint32_t ii = h.id & 0xffffu, prim = h.id >> 16;
         pick = 1.0f / (float)(s->inst[ii].data >> 16);
         make_tri_shading(s, ii, prim, h.b1, h.b2, &sd);
         ps->inst_prim = h.id;
+        if (s->nvolinst > 0) h.t = length3(sub(sd.pos, ps->origin)); /* intersection.hlsli:244 */
     } else {
         ps->inst_prim = 0xffffffffu;
         sd.area = -1;
@@ -1460,6 +1904,7 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
     f3 ldir = normalize3(bp);
     const st_transform* T = &view->camera_to_world;
     ps->origin = v3(T->m[0][3], T->m[1][3], T->m[2][3]);
+    ps->medium = x->view_medium;
     ps->path_len = 0;
     ps->thr = v3(1, 1, 1);
     ps->dir = pack_normal(normalize3(xf_vector(T, ldir)));
@@ -1472,9 +1917,16 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
     /* ProcessShadowRays + AddRadianceInterlocked, testrenderer.slang:237-261, 1147-1166 */
     for (int i = 0; i < x->nsh; i++) {
         shadow_t* sr = &x->sh[i];
-        hit_t h = trace(x->s, sr->o, sr->d, sr->tmax, 1, &x->trav[2], &x->trav[3]);
-        x->rays[1]++;
-        if (h.hit) continue;
+        if (x->s->nvolinst > 0) {
+            float neepdf;
+            trace_visibility_media(x, sr, px, py, seed, &sr->c, &neepdf);
+            if (all_le0(sr->c) || neepdf <= 0) continue;
+            sr->c = divf(sr->c, neepdf);
+        } else {
+            hit_t h = trace(x->s, sr->o, sr->d, sr->tmax, 1, &x->trav[2], &x->trav[3]);
+            x->rays[1]++;
+            if (h.hit) continue;
+        }
         if (sr->w <= 0 || all_le0(sr->c)) continue;
         f3 rw = mulf(sr->c, sr->w);
         float q[3] = {maxf(0, rw.x) * (float)OR_Q, maxf(0, rw.y) * (float)OR_Q, maxf(0, rw.z) * (float)OR_Q};
@@ -1496,6 +1948,7 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
     if (!s || !view || !p || !out) return fail("null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
     if (p->flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_ENV_ALIAS)) return fail("mode not supported by this oracle");
+    if (s->nvolinst > 0 && (p->flags & ST_FLAG_RESTIR_DI)) return fail("ReSTIR DI with media is not supported");
     cfg_t c;
     c.flags = p->flags;
     c.env_prob = p->environment_sample_probability;
@@ -1503,6 +1956,19 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
     c.min_len = p->min_path_length;
     c.cand = p->candidate_samples;
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    c.max_null = p->max_null_collisions;
+    uint32_t view_medium = 0xFFFFu;
+    {
+        const f3 cam = v3(view->camera_to_world.m[0][3], view->camera_to_world.m[1][3], view->camera_to_world.m[2][3]);
+        for (uint32_t i = 0; i < s->nvolinfo; i++) {
+            const st_volume_info* vi = &s->volinfo[i];
+            if (vi->instance_index >= s->ninst) continue;
+            const f3 lp = xf_point(&s->inv_xf[vi->instance_index], cam);
+            if (lp.x >= vi->min[0] && lp.y >= vi->min[1] && lp.z >= vi->min[2] && lp.x <= vi->max[0] && lp.y <= vi->max[1] &&
+                lp.z <= vi->max[2])
+                view_medium = vi->instance_index;
+        }
+    }
     uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
@@ -1516,6 +1982,7 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
         memset(&x, 0, sizeof x);
         x.s = s;
         x.c = &c;
+        x.view_medium = view_medium;
 #pragma omp for schedule(dynamic, 64)
         for (int64_t pix = 0; pix < (int64_t)W * H; pix++) {
             uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);

@@ -67,6 +67,16 @@ void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], con
 void or_bsdf_sample(const uint32_t mat[5], int lambertian, const float rnd[3], const float din[3],
                     float out_sample[10]);
 
+/* NanoVDB 32.3 float grid read (PNanoVDB.h read accessor path, as
+ * medium.hlsli:85-88 uses it) and the grid's world->index map
+ * (pnanovdb_grid_world_to_index[_dir]f). */
+float or_nvdb_read(const void* grid, int i, int j, int k);
+void or_nvdb_world_to_index(const void* grid, const float p[3], int dir, float out[3]);
+
+/* Henyey-Greenstein phase function, medium.hlsli:44-78. */
+float or_hg_eval(float g, const float din[3], const float dout[3]);
+void or_hg_sample(float g, float r0, float r1, const float din[3], float out[3]);
+
 #ifdef __cplusplus
 }
 #endif

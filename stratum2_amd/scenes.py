@@ -50,6 +50,20 @@ def material_record(base_color=(0.8, 0.8, 0.8), specular=0.5, emission=(0, 0, 0)
     return w
 
 
+def medium_record(density=(1.0, 1.0, 1.0), albedo=(1.0, 1.0, 1.0), anisotropy=0.0,
+                  density_volume=0xFFFFFFFF, albedo_volume=0xFFFFFFFF):
+    """28-byte Medium::store record (src/App/Material.hpp:49-66) as 7 u32 words:
+    density scale f32x3, albedo RGBA8 (alpha 1), density/albedo volume
+    indices (0xFFFFFFFF = none), anisotropy g."""
+    w = np.zeros(7, np.uint32)
+    w[0:3] = np.asarray(density, np.float32).view(np.uint32)
+    w[3] = pack4(*albedo, 1.0)
+    w[4] = density_volume
+    w[5] = albedo_volume
+    w[6] = np.float32(anisotropy).view(np.uint32)
+    return w
+
+
 class SceneData:
     """Host copy of one scene's buffers, convertible to st_scene_desc."""
 
@@ -109,6 +123,30 @@ class SceneData:
         self.instances.append((0 | (material_address << 4), vertex_info | tri_count << 16))
         self.transforms.append(np.eye(3, 4, dtype=np.float32) if transform is None else np.asarray(transform, np.float32))
         return len(self.instances) - 1
+
+    def add_volume(self, grid):
+        """A NanoVDB float grid (bytes, as stratum2_amd.nanovdb writes it)."""
+        self.volumes.append(np.ascontiguousarray(grid, np.uint8))
+        return len(self.volumes) - 1
+
+    def add_volume_instance(self, medium_address, volume_index=0xFFFFFFFF, transform=None):
+        """VolumeInstanceData (compat/scene.h:71-80) plus its VolumeInfo
+        (object-space bounds: the grid's world bbox, or the [-1, 1] cube
+        without a grid; Scene.cpp:672-699)."""
+        idx = len(self.instances)
+        self.instances.append((2 | (medium_address << 4), volume_index))
+        self.transforms.append(np.eye(3, 4, dtype=np.float32) if transform is None else np.asarray(transform, np.float32))
+        if volume_index == 0xFFFFFFFF:
+            mn, mx = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+        else:
+            bb = np.frombuffer(self.volumes[volume_index][560:608].tobytes(), "<f8")
+            mn, mx = tuple(np.float32(bb[:3])), tuple(np.float32(bb[3:]))
+        info = abi.st_volume_info()
+        info.min[:] = mn
+        info.max[:] = mx
+        info.instance_index = idx
+        self.volume_info.append(info)
+        return idx
 
     def set_environment(self, emission=(1, 1, 1), image=None):
         img = abi.ST_NO_IMAGE
@@ -469,3 +507,38 @@ def spheres_field(seed=1, n_spheres=2000, n_emitters=256, tris_per_emitter=16, e
 
 def spheres_view(width=1920, height=1080):
     return make_view(width, height, position=(0.0, 7.0, 40.0), orientation=axis_angle_quat((1, 0, 0), -12.0))
+
+
+def unit_cube_transform(bbox_min, bbox_max):
+    """createTransform (src/App/loaders/load_volumes.cpp:18-27): centre the
+    volume's world bbox at the origin and scale its largest extent to 1."""
+    mn = np.asarray(bbox_min, np.float64)
+    mx = np.asarray(bbox_max, np.float64)
+    c = (mx + mn) / 2
+    s = np.float32(1 / np.float32((mx - mn).max()))
+    t = np.eye(3, 4, dtype=np.float32) * s
+    t[:, 3] = (-c.astype(np.float32)) * s
+    return t
+
+
+def cloud(seed=0, n=256, sigma_scale=20.0, albedo=0.9, anisotropy=0.3, env=True, env_size=(1024, 512)):
+    """Heterogeneous medium workload: an n^3 NanoVDB float grid of seeded
+    4-octave fBm thresholded at 0.4, density scale sigma_scale, albedo and
+    Henyey-Greenstein g as given, placed in a unit cube at the origin, lit by
+    the procedural sky."""
+    from . import nanovdb
+    sd = SceneData()
+    dense = nanovdb.fbm_density(n=n, seed=seed, octaves=4, threshold=0.4)
+    grid = nanovdb.write_float_grid(dense, voxel_size=1.0)
+    vol = sd.add_volume(grid)
+    bb = np.frombuffer(grid[560:608].tobytes(), "<f8")
+    med = sd.add_material(medium_record(density=(sigma_scale,) * 3, albedo=(albedo,) * 3, anisotropy=anisotropy,
+                                        density_volume=vol))
+    sd.add_volume_instance(med, vol, unit_cube_transform(bb[:3], bb[3:]))
+    if env:
+        sd.set_environment(image=sky_environment(*env_size))
+    return sd
+
+
+def cloud_view(width=1024, height=1024):
+    return make_view(width, height, position=(0.0, 0.0, 1.6))
