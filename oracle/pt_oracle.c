@@ -1625,6 +1625,7 @@ static void connect_to_light_restir(pctx_t* x, const material_t* m, const shadin
     sr->plen = x->ps.path_len;
 }
 
+#define OR_MAX_MEDIA_SEGMENTS 256
 /* traceScatteringRay, intersection.hlsli:195-260: closest hits, delta tracking
  * through the current medium, medium boundaries crossed in place.
  * Returns 0 miss, 1 surface hit (h), 2 medium scattering at *pos. */
@@ -1635,7 +1636,9 @@ static int trace_scattering(pctx_t* x, f3 origin, f3 dir, hit_t* h, f3* pos, flo
     *neepdf = 1;
     f3 o = origin;
     float tmax = INFINITY;
-    while (tmax > 1e-6f) {
+    /* at most OR_MAX_MEDIA_SEGMENTS traceRay calls; a ray still crossing
+     * boundaries after that ends as a miss (the reference loop is unbounded) */
+    for (int seg = 0; tmax > 1e-6f && seg < OR_MAX_MEDIA_SEGMENTS; seg++) {
         int is_vol = 0;
         uint32_t vgn = 0;
         *h = trace_ray(s, o, dir, tmax, 0, &is_vol, &vgn, &x->trav[0], &x->trav[1]);
@@ -1685,7 +1688,12 @@ static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, u
     uint32_t cur = sr->medium;
     f3 o = sr->o;
     float tmax = sr->tmax;
-    while (tmax > 1e-6f) {
+    for (int seg = 0; tmax > 1e-6f; seg++) {
+        if (seg == OR_MAX_MEDIA_SEGMENTS) { /* bounded like the scattering ray: counts as blocked */
+            *beta = v3(0, 0, 0);
+            *neepdf = 0;
+            return;
+        }
         int is_vol = 0;
         uint32_t vgn = 0;
         hit_t h = trace_ray(s, o, sr->d, tmax, 0, &is_vol, &vgn, &x->trav[2], &x->trav[3]);

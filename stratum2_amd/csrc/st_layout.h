@@ -29,6 +29,10 @@ constexpr int kMaxStack = 3 * kMaxDepth + 1;  // a 4-wide step pushes at most 3 
 constexpr int kLdsStack = 16;      // per-lane stack entries kept in LDS; the rest spill to HBM
 constexpr int kEnvMaxLevels = 16;
 constexpr int kQuant = 16384;      // gRadianceQuantizationFactor, testrenderer.slang:25
+constexpr int kMaxVolumes = 8;     // gVolumeCount, src/Shaders/compat/scene.h:13-15
+constexpr int kMaxVolumeInstances = 8;
+constexpr uint32_t kNoMedium = 0xffffu;  // INVALID_INSTANCE in the 16-bit medium field
+constexpr int kMaxMediaSegments = 256;   // traceRay calls per scattering / visibility ray with media
 
 struct DevScene {
     const float4* nodes;
@@ -52,6 +56,12 @@ struct DevScene {
     const float2* env_alias;       // per level-0 texel: {prob, alias index bits}
     const float* env_pdf;          // per level-0 texel: uv-space pdf
     float sphere[4];               // scene bounding sphere, TestRenderer.cpp:458-461
+    // media: volume instances (procedural boxes) and their NanoVDB float grids
+    const st_transform* inv_xf;    // per instance, TransformData::inverse
+    uint32_t vol_count, vinst_count;
+    const uint8_t* vol[kMaxVolumes];
+    uint64_t vol_root[kMaxVolumes];  // byte offset of each grid's root node
+    uint32_t vinst[kMaxVolumeInstances];
 };
 
 }  // namespace st

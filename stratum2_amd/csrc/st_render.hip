@@ -10,6 +10,10 @@
 //             queues with one atomic per wavefront (ballot+popcount)  (k_shade)
 //   shadow  — any-hit of the appended shadow rays, fixed-point NEE  (k_trace_shadow)
 // and a final resolve sums each pixel's samples in sample order (k_resolve).
+// Scenes with media swap the two traversal kernels for k_trace_media (delta
+// tracking through the current medium, boundary crossings, scattering
+// vertices) and k_shadow_media (ratio tracking), and shade medium vertices
+// with the Henyey-Greenstein phase function.
 // All kernels are persistent grid-stride loops whose item counts are read on
 // the device, so the host never waits inside a frame.
 #include <hip/hip_runtime.h>
@@ -30,6 +34,7 @@
 #include "st_layout.h"
 #include "st_shade.h"
 #include "st_trace.h"
+#include "st_volume.h"
 
 using namespace st;
 
@@ -74,12 +79,14 @@ struct WaveState {
     float4* r_init;     // 2 per pixel: light-sample seed (float4), {target, W, M, depth}
     float4* r_temp;     // after temporal reuse
     float4* r_prev;     // after spatial reuse: this frame's result, next frame's history
+    uint32_t* med;      // per path: current medium instance (kNoMedium outside media)
 };
 
 struct RenderConsts {
     uint32_t W, H, npix, wave_paths, k, sample_base, seed, max_depth, min_len, flags, rr_start, cand, tiled, spatial;
     float env_prob;
     float max_m, radius;
+    uint32_t view_medium, max_null;
     st_view view;
 };
 
@@ -134,6 +141,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveStat
         w.st[p] = make_uint4(rng.w, 0, 0, 0);
         w.em[p] = make_float4(0, 0, 0, 0);
         w.nee[p] = make_uint4(0, 0, 0, 0);
+        w.med[p] = c.view_medium;
         w.queue[0][p] = p;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -174,6 +182,27 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
 
 // Shadow rays: any-hit, then AddRadianceInterlocked (testrenderer.slang:237-261)
 // into the path's own fixed-point accumulator.
+__device__ __forceinline__ void add_shadow_contribution(const WaveState& w, uint32_t flags, uint32_t p, v3 ct, float wgt) {
+    if (wgt <= 0 || all_nonpos(ct)) return;
+    const v3 rw = ct * wgt;
+    if (flags & ST_FLAG_FLOAT_NEE) {
+        float4 e = w.em[p];
+        e.x += fmax_(0.0f, rw.x);
+        e.y += fmax_(0.0f, rw.y);
+        e.z += fmax_(0.0f, rw.z);
+        w.em[p] = e;
+        return;
+    }
+    const float q0 = fmax_(0.0f, rw.x) * (float)kQuant, q1 = fmax_(0.0f, rw.y) * (float)kQuant,
+                q2 = fmax_(0.0f, rw.z) * (float)kQuant;
+    // one shadow ray per path per bounce: a plain read-modify-write is race free
+    uint4 acc = w.nee[p];
+    acc.x += (uint32_t)(q0 >= 2147483647.0f ? 2147483647 : (int32_t)q0);
+    acc.y += (uint32_t)(q1 >= 2147483647.0f ? 2147483647 : (int32_t)q1);
+    acc.z += (uint32_t)(q2 >= 2147483647.0f ? 2147483647 : (int32_t)q2);
+    w.nee[p] = acc;
+}
+
 struct ShadowSrc {
     WaveState w;
     uint32_t flags;
@@ -187,26 +216,7 @@ struct ShadowSrc {
         if (h.hit) return;
         const float4 c = w.shadow[3 * (size_t)i + 2];
         const uint32_t p = __float_as_uint(w.shadow[3 * (size_t)i + 1].w);
-        const float wgt = c.w;
-        const v3 ct = mk(c.x, c.y, c.z);
-        if (wgt <= 0 || all_nonpos(ct)) return;
-        const v3 rw = ct * wgt;
-        if (flags & ST_FLAG_FLOAT_NEE) {
-            float4 e = w.em[p];
-            e.x += fmax_(0.0f, rw.x);
-            e.y += fmax_(0.0f, rw.y);
-            e.z += fmax_(0.0f, rw.z);
-            w.em[p] = e;
-            return;
-        }
-        const float q0 = fmax_(0.0f, rw.x) * (float)kQuant, q1 = fmax_(0.0f, rw.y) * (float)kQuant,
-                    q2 = fmax_(0.0f, rw.z) * (float)kQuant;
-        // one shadow ray per path per bounce: a plain read-modify-write is race free
-        uint4 acc = w.nee[p];
-        acc.x += (uint32_t)(q0 >= 2147483647.0f ? 2147483647 : (int32_t)q0);
-        acc.y += (uint32_t)(q1 >= 2147483647.0f ? 2147483647 : (int32_t)q1);
-        acc.z += (uint32_t)(q2 >= 2147483647.0f ? 2147483647 : (int32_t)q2);
-        w.nee[p] = acc;
+        add_shadow_contribution(w, flags, p, mk(c.x, c.y, c.z), c.w);
     }
 };
 
@@ -221,6 +231,167 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
         atomicAdd(&stats[2], (unsigned long long)ts.nodes);
         atomicAdd(&stats[3], (unsigned long long)ts.tris);
     }
+}
+
+// Sum of a per-lane count over the wavefront, added once per wave.
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+// traceScatteringRay (intersection.hlsli:195-260) for every queued path of a
+// scene with media: closest hit over triangles and volume boxes, delta
+// tracking through the path's current medium up to that hit, medium
+// boundaries crossed in place.  Updates the path's throughput (divided by the
+// average free-flight pdf, TraceRay testrenderer.slang:1029-1030), random
+// counter and medium.  A scattering vertex is reported as id medium | 0xffff
+// << 16 with t = its distance from the ray origin, and its position replaces
+// the ray origin; surface hits keep t of the last segment (the shading kernel
+// measures the distance from the origin, intersection.hlsli:244).  One lane
+// per path, grid-stride.
+__global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderConsts c, WaveState w, int qsel, uint32_t* spill) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t n = w.counters[qsel];
+    const uint32_t* q = w.queue[qsel];
+    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraverseStats ts{0, 0};
+    uint32_t segs = 0;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const uint32_t p = q[i];
+        const float4 r = w.ray[p];
+        const v3 origin = mk(r.x, r.y, r.z);
+        const v3 dir = unpack_normal(__float_as_uint(r.w));
+        const float4 t4 = w.thr[p];
+        v3 thr = mk(t4.x, t4.y, t4.z);
+        uint4 s4 = w.st[p];
+        uint32_t med = w.med[p];
+        const uint32_t k = p / c.npix;
+        uint32_t px, py;
+        pixel_of(c, p % c.npix, px, py);
+        Rng rng{px, py, c.seed + c.sample_base + k, s4.x};
+        float dirpdf = 1;
+        v3 o = origin;
+        float tmax = __int_as_float(0x7f800000);
+        uint4 out = make_uint4(0, 0xffffffffu, 0, 0);
+        bool scattered = false, done = false;
+        v3 spos = mk(0, 0, 0);
+        // bounded: a ray still crossing boundaries after kMaxMediaSegments
+        // traceRay calls ends as a miss (the reference loop is unbounded)
+        for (int seg = 0; tmax > 1e-6f && seg < kMaxMediaSegments; seg++) {
+            Hit h = traverse<false, false>(S, o, dir, tmax, lds, spill, stride, gtid, ts);
+            segs++;
+            bool is_vol;
+            uint32_t vgn = 0;
+            volume_boxes(S, o, dir, h, is_vol, vgn);
+            if (med != kNoMedium) {
+                const Medium m = load_medium(S, S.inst[med].type_material_address >> 4);
+                const st_transform& iT = S.inv_xf[med];
+                v3 dp3, np3;
+                bool sc;
+                const v3 pp = delta_track<true>(S, m, rng, xf_point(iT, o), xf_vector(iT, dir), h.hit ? h.t : tmax, thr, dp3,
+                                                np3, sc, c.max_null);
+                dirpdf *= avg3(dp3);
+                if (sc) {
+                    spos = xf_point(S.xf[med], pp);
+                    out = make_uint4(__float_as_uint(len(spos - origin)), med | (0xffffu << 16), 0, 0);
+                    scattered = done = true;
+                    break;
+                }
+            }
+            if (!h.hit) {
+                out = make_uint4(__float_as_uint(tmax), 0xffffffffu, 0, 0);
+                done = true;
+                break;
+            }
+            if (!is_vol) {
+                out = make_uint4(__float_as_uint(h.t), h.id, __float_as_uint(h.b1), __float_as_uint(h.b2));
+                done = true;
+                break;
+            }
+            const v3 bp = o + dir * h.t;
+            const v3 ng = unpack_normal(vgn);
+            med = dot(ng, dir) > 0 ? kNoMedium : (h.id & 0xffffu);
+            o = ray_offset(bp, ng, dir);
+            tmax -= h.t;
+        }
+        if (!done) out = make_uint4(__float_as_uint(tmax), 0xffffffffu, 0, 0);
+        thr = thr / dirpdf;
+        w.hit[i] = out;
+        w.thr[p] = make_float4(thr.x, thr.y, thr.z, t4.w);
+        s4.x = rng.w;
+        w.st[p] = s4;
+        w.med[p] = med;
+        if (scattered) w.ray[p] = make_float4(spos.x, spos.y, spos.z, r.w);
+    }
+    wave_add_u64(reinterpret_cast<unsigned long long*>(w.counters + 4), segs);
+}
+
+// traceVisibilityRay with media (intersection.hlsli:262-338): closest hits,
+// any surface blocks, ratio tracking through each medium crossed; the NEE
+// contribution is divided by the average tracking pdf (testrenderer.slang:
+// 1153-1162).  The reference reads an uninitialised static path state here;
+// this uses the vertex's medium and its own random stream keyed by pixel,
+// sample and path length (offset 0x40000000 | length << 16).
+__global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, RenderConsts c, WaveState w, uint32_t* spill) {
+    __shared__ uint32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t n = w.counters[2];
+    const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraverseStats ts{0, 0};
+    uint32_t segs = 0;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const float4 a = w.shadow[3 * (size_t)i], b = w.shadow[3 * (size_t)i + 1], cc = w.shadow[3 * (size_t)i + 2];
+        const uint32_t p = __float_as_uint(b.w);
+        const uint32_t plen = w.st[p].y;
+        const uint32_t k = p / c.npix;
+        uint32_t px, py;
+        pixel_of(c, p % c.npix, px, py);
+        Rng rng{px, py, c.seed + c.sample_base + k, 0x40000000u | (plen << 16)};
+        uint32_t cur = w.med[p];
+        v3 beta = mk(cc.x, cc.y, cc.z);
+        const v3 d = mk(b.x, b.y, b.z);
+        v3 o = mk(a.x, a.y, a.z);
+        float tmax = a.w, neepdf = 1;
+        for (int seg = 0; tmax > 1e-6f; seg++) {
+            if (seg == kMaxMediaSegments) {  // bounded like the scattering ray: counts as blocked
+                beta = mk(0, 0, 0);
+                neepdf = 0;
+                break;
+            }
+            Hit h = traverse<false, false>(S, o, d, tmax, lds, spill, stride, gtid, ts);
+            segs++;
+            bool is_vol;
+            uint32_t vgn = 0;
+            volume_boxes(S, o, d, h, is_vol, vgn);
+            if (h.hit && !is_vol) {
+                beta = mk(0, 0, 0);
+                neepdf = 0;
+                break;
+            }
+            if (cur != kNoMedium) {
+                const Medium m = load_medium(S, S.inst[cur].type_material_address >> 4);
+                const st_transform& iT = S.inv_xf[cur];
+                v3 dp3, np3;
+                bool sc;
+                delta_track<false>(S, m, rng, xf_point(iT, o), xf_vector(iT, d), h.hit ? h.t : tmax, beta, dp3, np3, sc,
+                                   c.max_null);
+                neepdf *= avg3(np3);
+                if (!(beta.x != 0 && beta.y != 0 && beta.z != 0)) break;
+            }
+            if (!h.hit) break;
+            const v3 bp = o + d * h.t;
+            const v3 ng = unpack_normal(vgn);
+            if (dot(ng, d) < 0) {
+                cur = h.id & 0xffffu;
+                o = ray_offset(bp, -ng);
+            } else {
+                cur = kNoMedium;
+                o = ray_offset(bp, ng);
+            }
+            tmax -= h.t;
+        }
+        if (!(all_nonpos(beta) || neepdf <= 0)) add_shadow_contribution(w, c.flags, p, beta / neepdf, cc.w);
+    }
+    wave_add_u64(reinterpret_cast<unsigned long long*>(w.counters + 4) + 1, segs);
 }
 
 // Wavefront-aggregated append: one atomic per wave, lane order preserved.
@@ -379,7 +550,7 @@ __device__ __forceinline__ float mis_pair(float a, float b) {  // mis(a, b), tes
     return d > 0 ? a2 / d : 0.0f;
 }
 
-template <bool LAMBERT, bool ALIAS, bool RESTIR>
+template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
                                                        uint32_t* visibility, int first_wave) {
     const uint32_t n = w.counters[qsel];
@@ -443,6 +614,66 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                     add_emission(e, dpa, 1);
                 }
                 s4.y = path_len;
+            } else if (MEDIA && (hit.y >> 16) == 0xffffu) {
+                // medium scattering vertex: ProcessInteraction<false>(Medium) (testrenderer.slang:816-935)
+                const uint32_t ii = hit.y & 0xffffu;
+                const v3 pos = mk(r.x, r.y, r.z);  // written by k_trace_media
+                float pdfa = 0;  // pickPdf 0 over shape area 0 (intersection.hlsli:232-236)
+                if (S.light_count > 0) {
+                    pdfa = 0.0f / (0.0f * (float)S.light_count);
+                    if (S.has_env) pdfa *= 1 - c.env_prob;
+                }
+                const float t16 = q16(__uint_as_float(hit.x)), pdfa16 = q16(pdfa);
+                if (!all_nonpos(thr)) {  // ExtendPath, testrenderer.slang:1062-1063
+                    if (path_len == 1 && first_wave && k == 0 && visibility) {
+                        visibility[2 * pix] = hit.y;
+                        visibility[2 * pix + 1] = 0;
+                    }
+                    const Medium m = load_medium(S, S.inst[ii].type_material_address >> 4);
+                    const v3 ldin = -dir;
+                    bool alive = path_len < c.max_depth && m.can_eval();
+                    const bool rr_on = path_len >= c.rr_start;
+                    if (alive && !m.singular() && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
+                        // ConnectToLight (:751-779) with the phase function; cosOut = 1
+                        const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), pos);
+                        if (!(all_nonpos(L.radiance) || L.pdf <= 0)) {
+                            const float AtoW = L.finite ? sq(L.dist) / L.cos_light : 1.0f;
+                            const float dpw = L.pdf * AtoW;
+                            const float v = hg_eval(m.g, ldin, L.dir);
+                            const float fwd = v * (rr_on ? m.cont_prob() : 1.0f);
+                            const v3 contrib = ((thr * L.radiance) * v) / dpw;
+                            const float wgt = (c.flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(L.pdf, fwd / AtoW);
+                            if (path_len + 1 >= c.min_len) {
+                                // rayOffset of a medium vertex is the vertex itself (intersection.hlsli:33-35)
+                                sh0 = make_float4(pos.x, pos.y, pos.z, fmax_(L.dist * 0.99f, L.dist - 1e-2f));
+                                sh1 = make_float4(L.dir.x, L.dir.y, L.dir.z, __uint_as_float(p));
+                                sh2 = make_float4(contrib.x, contrib.y, contrib.z, wgt);
+                                emit_shadow = true;
+                            }
+                        }
+                    }
+                    if (c.flags & ST_FLAG_DIRECT_ONLY) alive = false;
+                    if (alive) {
+                        const float cp = fmin_(1.0f, m.cont_prob());
+                        const float rr = rng.nextf().x;
+                        if (rr_on && rr > cp) alive = false;
+                        if (alive) {
+                            const float4 rb = rng.nextf();
+                            const v3 sdir = hg_sample(m.g, rb.x, rb.y, ldin);
+                            const float v = hg_eval(m.g, ldin, sdir);
+                            const float fwd = v * (rr_on ? cp : 1.0f);
+                            thr = thr * (v / fwd);
+                            dirpdf = (1 - fabsf(m.g)) == 0 ? 0.0f : fwd;
+                            const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
+                            cont = !nan && !all_nonpos(thr) && path_len + 1 <= c.max_depth;
+                            ray_out = make_float4(pos.x, pos.y, pos.z, __uint_as_float(pack_normal(sdir)));
+                            thr_out = make_float4(thr.x, thr.y, thr.z, 0);
+                            s4.w = f2h(dirpdf);
+                        }
+                    }
+                }
+                s4.y = path_len;
+                s4.z = f2h(t16) | (f2h(pdfa16) << 16);
             } else {
                 const uint32_t ii = hit.y & 0xffffu, prim = hit.y >> 16;
                 Shading sd;
@@ -453,8 +684,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                     pdfa = pick / (sd.area * (float)S.light_count);
                     if (S.has_env) pdfa *= 1 - c.env_prob;
                 }
-                const float t16 = q16(__uint_as_float(hit.x)), pdfa16 = q16(pdfa);
-                if (path_len == 1 && first_wave && k == 0 && visibility) {
+                // with media, the distance is measured from the ray origin (intersection.hlsli:244)
+                const float thit = MEDIA ? len(sd.pos - mk(r.x, r.y, r.z)) : __uint_as_float(hit.x);
+                const float t16 = q16(thit), pdfa16 = q16(pdfa);
+                const bool live = !MEDIA || !all_nonpos(thr);  // ExtendPath, testrenderer.slang:1062-1063
+                if (live && path_len == 1 && first_wave && k == 0 && visibility) {
                     visibility[2 * pix] = hit.y;
                     visibility[2 * pix + 1] = sd.sn;
                 }
@@ -463,11 +697,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 const v3 ldin = f.to_local(-dir);
                 const v3 lgn = f.to_local(unpack_normal(sd.gn));
                 const float cos_in = fabsf(ldin.z);
-                if (ldin.z > 0) {
+                if (live && ldin.z > 0) {
                     const v3 e = thr * m.emission();
                     if (any_pos(e)) add_emission(e, pdfa16, cos_in / sq(t16));
                 }
-                bool alive = path_len < c.max_depth && any_pos(m.base());
+                bool alive = live && path_len < c.max_depth && any_pos(m.base());
                 // Russian roulette from path length rr_start (0: every bounce, testrenderer.slang:890-892)
                 const bool rr_on = path_len >= c.rr_start;
                 const float cp_eff = rr_on ? m_cont_prob(m) : 1.0f;
@@ -670,11 +904,14 @@ __global__ void k_resolve(RenderConsts c, WaveState w, float4* acc, float* hdr, 
     }
 }
 
-__global__ void k_count(WaveState w, int qsel) {
-    // rays traced this bounce: extension queue + shadow queue
+__global__ void k_count(WaveState w, int qsel, int media) {
+    // rays traced this bounce: extension queue + shadow queue (the media
+    // kernels count their traceRay segments themselves)
     unsigned long long* r = reinterpret_cast<unsigned long long*>(w.counters + 4);
-    r[0] += w.counters[qsel];
-    r[1] += w.counters[2];
+    if (!media) {
+        r[0] += w.counters[qsel];
+        r[1] += w.counters[2];
+    }
     w.counters[2] = 0;
     w.counters[qsel] = 0;
     w.counters[16] = 0;  // traversal fetch cursors
@@ -735,7 +972,7 @@ struct st_context {
     int device = 0;
     int cus = 256;
     hipStream_t stream = nullptr;
-    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, acc, stats, raysbuf, outbuf, pri, rinit, rtemp, rprev;
+    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, acc, stats, raysbuf, outbuf, pri, rinit, rtemp, rprev, med;
     double timings[12] = {0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     size_t ev_used[3] = {0, 0, 0};
@@ -745,6 +982,9 @@ struct st_scene {
     st_context* ctx = nullptr;
     DevScene d{};
     std::vector<DevBuf*> bufs;
+    // host copies for the camera's medium (TestRenderer.cpp:431-446)
+    std::vector<st_volume_info> volume_info;
+    std::vector<st_transform> inv_xf;
     uint64_t stats[4] = {0, 0, 0, 0};
     ~st_scene() {
         for (auto* b : bufs) delete b;
@@ -787,16 +1027,105 @@ void validate_desc(const st_scene_desc* d) {
     if (!d) throw std::invalid_argument("null scene description");
     if (d->instance_count && (!d->instances || !d->transforms)) throw std::invalid_argument("instances/transforms missing");
     if (d->instance_count > 0xFFFF) throw std::invalid_argument("more than 65535 instances (16-bit instance ids)");
+    if (d->volume_count > (uint32_t)kMaxVolumes) throw std::invalid_argument("more than 8 volumes (gVolumeCount)");
+    if (d->volume_count && !d->volumes) throw std::invalid_argument("volumes missing");
+    if (d->volume_info_count && !d->volume_info) throw std::invalid_argument("volume info missing");
+    for (uint32_t i = 0; i < d->volume_info_count; i++) {
+        const uint32_t ii = d->volume_info[i].instance_index;
+        if (ii >= d->instance_count || (d->instances[ii].type_material_address & 0xfu) != 2)
+            throw std::invalid_argument("volume info must name a volume instance");
+    }
+    uint32_t nvol = 0;
     for (uint32_t i = 0; i < d->instance_count; i++) {
         const st_instance& in = d->instances[i];
         const uint32_t type = in.type_material_address & 0xfu;
-        if (type != 0) throw std::domain_error("only mesh instances are supported");
+        if (type == 2) {  // VolumeInstanceData (compat/scene.h:71-80) with a 28-byte medium record
+            if (++nvol > (uint32_t)kMaxVolumeInstances) throw std::domain_error("more than 8 volume instances");
+            const uint32_t addr = in.type_material_address >> 4;
+            if (addr % 4 || (uint64_t)addr / 4 + 7 > d->material_word_count) throw std::invalid_argument("medium address out of range");
+            const uint32_t dv = d->material_data[addr / 4 + 4];
+            if (in.data != 0xFFFFFFFFu && in.data >= d->volume_count) throw std::invalid_argument("volume index out of range");
+            if (dv != 0xFFFFFFFFu && dv >= d->volume_count) throw std::invalid_argument("density volume index out of range");
+            if (d->material_data[addr / 4 + 5] != 0xFFFFFFFFu) throw std::domain_error("albedo volumes are not supported");
+            continue;
+        }
+        if (type != 0) throw std::domain_error("only mesh and volume instances are supported");
         const uint32_t addr = in.type_material_address >> 4;
         if ((uint64_t)addr / 4 + 16 > d->material_word_count) throw std::invalid_argument("material address out of range");
         for (int k = 0; k < 4; k++)
             if (d->material_data[addr / 4 + 4 + k] < ST_IMAGE_LIMIT) throw std::domain_error("material textures are not supported");
         if ((in.data & 0xffffu) >= d->mesh_vertex_info_count) throw std::invalid_argument("vertex info index out of range");
     }
+}
+
+// NanoVDB 32.3 float grid: header checks and a walk of every child offset
+// (root tiles -> upper -> lower -> leaf), so the kernels' reads stay inside
+// the buffer.  Returns the byte offset of the root node.
+uint64_t validate_nanovdb(const uint8_t* b, uint64_t n) {
+    auto need = [&](uint64_t off, uint64_t len) {
+        if (off > n || len > n - off) throw std::invalid_argument("NanoVDB grid: offset out of range");
+    };
+    auto r32 = [&](uint64_t o) { need(o, 4); uint32_t v; std::memcpy(&v, b + o, 4); return v; };
+    auto r64 = [&](uint64_t o) { need(o, 8); uint64_t v; std::memcpy(&v, b + o, 8); return v; };
+    need(0, 672 + 64 + 64);
+    if (r64(0) != 0x304244566F6E614Eull) throw std::invalid_argument("NanoVDB grid: bad magic");
+    if ((r32(16) >> 21) != 32) throw std::domain_error("NanoVDB grid: major version must be 32");
+    if (r32(636) != 1) throw std::domain_error("NanoVDB grid: only float grids are supported");
+    const uint64_t root = 672 + r64(672 + 24);
+    if (root % 8) throw std::invalid_argument("NanoVDB grid: misaligned root");
+    need(root, 64);
+    const uint32_t ntiles = r32(root + 24);
+    need(root + 64, 32ull * ntiles);
+    for (uint32_t t = 0; t < ntiles; t++) {
+        const int64_t child = (int64_t)r64(root + 64 + 32ull * t + 8);
+        if (child == 0) continue;
+        if (child < 0 || child % 8) throw std::invalid_argument("NanoVDB grid: bad upper offset");
+        const uint64_t up = root + (uint64_t)child;
+        need(up, 270400);
+        for (uint32_t n2 = 0; n2 < 32768; n2++) {
+            if (!((r32(up + 4128 + 4 * (n2 >> 5)) >> (n2 & 31)) & 1)) continue;
+            const int64_t lo_off = (int64_t)r64(up + 8256 + 8ull * n2);
+            if (lo_off <= 0 || lo_off % 8) throw std::invalid_argument("NanoVDB grid: bad lower offset");
+            const uint64_t lo = up + (uint64_t)lo_off;
+            need(lo, 33856);
+            for (uint32_t n1 = 0; n1 < 4096; n1++) {
+                if (!((r32(lo + 544 + 4 * (n1 >> 5)) >> (n1 & 31)) & 1)) continue;
+                const int64_t lf = (int64_t)r64(lo + 1088 + 8ull * n1);
+                if (lf <= 0 || lf % 8) throw std::invalid_argument("NanoVDB grid: bad leaf offset");
+                need(lo + (uint64_t)lf, 2144);
+            }
+        }
+    }
+    return root;
+}
+
+// TransformData::inverse of a 3x4 affine transform, in double.
+st_transform inverse_transform(const st_transform& T) {
+    st_transform r{};
+    double a[3][3], inv[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a[i][j] = T.m[i][j];
+    const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                       a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    if (det == 0) return r;
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int i = 0; i < 3; i++) {
+        double tr = 0;
+        for (int j = 0; j < 3; j++) {
+            r.m[i][j] = (float)inv[i][j];
+            tr -= inv[i][j] * T.m[j][3];
+        }
+        r.m[i][3] = (float)tr;
+    }
+    return r;
 }
 
 }  // namespace
@@ -893,7 +1222,25 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
             for (int r = 0; r < 3; r++) o[r] = T.m[r][0] * v[0] + T.m[r][1] * v[1] + T.m[r][2] * v[2] + T.m[r][3];
         };
         uint64_t ntri = 0;
-        for (uint32_t i = 0; i < d->instance_count; i++) ntri += d->instances[i].data >> 16;
+        for (uint32_t i = 0; i < d->instance_count; i++)
+            if ((d->instances[i].type_material_address & 0xfu) == 0) ntri += d->instances[i].data >> 16;
+        // media: grids, volume instances, inverse transforms
+        std::vector<uint64_t> roots(d->volume_count);
+        for (uint32_t v = 0; v < d->volume_count; v++) {
+            const st_buffer_view& g = d->volumes[v];
+            if (!g.data) throw std::invalid_argument("volume data missing");
+            roots[v] = validate_nanovdb((const uint8_t*)g.data, g.size_bytes);
+            D.vol[v] = s->upload<uint8_t>(g.data, g.size_bytes);
+            D.vol_root[v] = roots[v];
+        }
+        D.vol_count = d->volume_count;
+        s->inv_xf.resize(d->instance_count + 1);
+        for (uint32_t i = 0; i < d->instance_count; i++) s->inv_xf[i] = inverse_transform(d->transforms[i]);
+        D.inv_xf = s->upload<st_transform>(s->inv_xf.data(), sizeof(st_transform) * s->inv_xf.size());
+        D.vinst_count = 0;
+        for (uint32_t i = 0; i < d->instance_count; i++)
+            if ((d->instances[i].type_material_address & 0xfu) == 2) D.vinst[D.vinst_count++] = i;
+        for (uint32_t i = 0; i < d->volume_info_count; i++) s->volume_info.push_back(d->volume_info[i]);
         std::vector<float> verts(9 * ntri + 9);
         std::vector<uint32_t> ids(ntri + 1);
         std::vector<uint32_t> lights;
@@ -901,6 +1248,34 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
         uint64_t t = 0;
         for (uint32_t i = 0; i < d->instance_count; i++) {
             const st_instance& in = d->instances[i];
+            if ((in.type_material_address & 0xfu) == 2) {
+                // volume box corners (Scene.cpp:672-711); as in the reference, the
+                // running min/max also feed the later corners
+                float mn[3] = {-1, -1, -1}, mx[3] = {1, 1, 1};
+                if (in.data < d->volume_count) {
+                    double bb[6];
+                    std::memcpy(bb, (const uint8_t*)d->volumes[in.data].data + 560, 48);
+                    for (int k = 0; k < 3; k++) {
+                        mn[k] = (float)bb[k];
+                        mx[k] = (float)bb[3 + k];
+                    }
+                }
+                for (int cidx = 0; cidx < 8; cidx++) {
+                    const float corner[3] = {(cidx % 2) == 0 ? mn[0] : mx[0], ((cidx % 4) / 2) == 0 ? mn[1] : mx[1],
+                                             (cidx / 4) == 0 ? mn[2] : mx[2]};
+                    float wc[3];
+                    xfp(d->transforms[i], corner, wc);
+                    for (int k = 0; k < 3; k++) {
+                        mn[k] = std::min(mn[k], wc[k]);
+                        mx[k] = std::max(mx[k], wc[k]);
+                    }
+                }
+                for (int k = 0; k < 3; k++) {
+                    amin[k] = std::min(amin[k], mn[k]);
+                    amax[k] = std::max(amax[k], mx[k]);
+                }
+                continue;
+            }
             const uint32_t addr = in.type_material_address >> 4;
             const uint32_t ew = d->material_data[addr / 4 + 1];
             if ((ew & 0xffu) || ((ew >> 8) & 0xffu) || ((ew >> 16) & 0xffu)) lights.push_back(i);  // Scene.cpp:487-491
@@ -1074,11 +1449,17 @@ int st_scene_stats(const st_scene* s, uint64_t out[4]) {
 }
 
 using ShadeFn = void (*)(DevScene, RenderConsts, WaveState, int, uint32_t*, int);
-static ShadeFn shade_kernel(bool lambert, bool alias, bool restir) {
+static ShadeFn shade_kernel(bool lambert, bool alias, bool restir, bool media) {
+    if (media) {  // ReSTIR DI with media is rejected before this
+        static const ShadeFn mt[4] = {k_shade<false, false, false, true>, k_shade<false, true, false, true>,
+                                      k_shade<true, false, false, true>, k_shade<true, true, false, true>};
+        return mt[(lambert ? 2 : 0) | (alias ? 1 : 0)];
+    }
     const int i = (lambert ? 4 : 0) | (alias ? 2 : 0) | (restir ? 1 : 0);
-    static const ShadeFn table[8] = {k_shade<false, false, false>, k_shade<false, false, true>, k_shade<false, true, false>,
-                                     k_shade<false, true, true>,   k_shade<true, false, false>, k_shade<true, false, true>,
-                                     k_shade<true, true, false>,   k_shade<true, true, true>};
+    static const ShadeFn table[8] = {
+        k_shade<false, false, false, false>, k_shade<false, false, true, false>, k_shade<false, true, false, false>,
+        k_shade<false, true, true, false>,   k_shade<true, false, false, false>, k_shade<true, false, true, false>,
+        k_shade<true, true, false, false>,   k_shade<true, true, true, false>};
     return table[i];
 }
 
@@ -1096,6 +1477,10 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
     if ((p->flags & ST_FLAG_RESTIR_DI) && (p->candidate_samples == 0 || p->candidate_samples > 1024))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "candidate_samples must be in [1, 1024]");
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
+    const bool media = s->d.vinst_count > 0;
+    if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
+    if (media && (p->flags & ST_FLAG_RESTIR_DI))
+        return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI is not supported in scenes with media");
     st_context* ctx = s->ctx;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
@@ -1115,6 +1500,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         ctx->q0.reserve(paths * 4);
         ctx->q1.reserve(paths * 4);
         ctx->shadow.reserve(paths * 48);
+        ctx->med.reserve(paths * 4);
         ctx->counters.reserve(256);
         ctx->acc.reserve((size_t)npix * 16);
         if (reuse) {
@@ -1141,6 +1527,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         w.r_init = ctx->rinit.as<float4>();
         w.r_temp = ctx->rtemp.as<float4>();
         w.r_prev = ctx->rprev.as<float4>();
+        w.med = ctx->med.as<uint32_t>();
         hipStream_t sm = ctx->stream;
         HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 256, sm));
         HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, sm));
@@ -1158,6 +1545,23 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         c.rr_start = p->rr_start_depth;
         c.env_prob = p->environment_sample_probability;
         c.view = *view;
+        c.max_null = p->max_null_collisions;
+        // the camera's medium: the last volume whose object-space bounds hold the
+        // camera position (TestRenderer.cpp:431-446)
+        c.view_medium = kNoMedium;
+        {
+            const st_transform& T = view->camera_to_world;
+            const float cam[3] = {T.m[0][3], T.m[1][3], T.m[2][3]};
+            for (const st_volume_info& vi : s->volume_info) {
+                if (vi.instance_index >= s->d.inst_count) continue;
+                const st_transform& iT = s->inv_xf[vi.instance_index];
+                float lp[3];
+                for (int r = 0; r < 3; r++) lp[r] = iT.m[r][0] * cam[0] + iT.m[r][1] * cam[1] + iT.m[r][2] * cam[2] + iT.m[r][3];
+                if (lp[0] >= vi.min[0] && lp[1] >= vi.min[1] && lp[2] >= vi.min[2] && lp[0] <= vi.max[0] &&
+                    lp[1] <= vi.max[1] && lp[2] <= vi.max[2])
+                    c.view_medium = vi.instance_index;
+            }
+        }
         c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
         const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
         const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
@@ -1177,13 +1581,14 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
             for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
                 {
                     Timed tm(ctx, 0);
-                    if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, ctx->spill.as<uint32_t>());
+                    else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
                     else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
                 }
                 {
                     Timed tm(ctx, 1);
                     uint32_t* vis = (base == 0) ? o->visibility : nullptr;
-                    auto* kfn = shade_kernel(lambert, alias, restir);
+                    auto* kfn = shade_kernel(lambert, alias, restir, media);
                     hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0 ? 1 : 0);
                 }
                 if (reuse && b == 0) {
@@ -1197,10 +1602,11 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                 }
                 {
                     Timed tm(ctx, 2);
-                    if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
+                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, ctx->spill.as<uint32_t>());
+                    else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
                     else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
                 }
-                hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel);
+                hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
                 qsel ^= 1;
             }
             const bool last = base + c.k >= p->spp;

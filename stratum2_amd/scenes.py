@@ -542,3 +542,50 @@ def cloud(seed=0, n=256, sigma_scale=20.0, albedo=0.9, anisotropy=0.3, env=True,
 
 def cloud_view(width=1024, height=1024):
     return make_view(width, height, position=(0.0, 0.0, 1.6))
+
+
+# ------------------------------------------------------------ media helpers
+def layered_grid(values_y):
+    """NanoVDB grid of 4 x len(values_y) x 4 voxels, constant in x and z."""
+    from . import nanovdb
+    d = np.zeros((4, len(values_y), 4), np.float32)
+    for j, v in enumerate(values_y):
+        d[:, j, :] = v
+    return nanovdb.write_float_grid(d, voxel_size=1.0)
+
+
+def add_medium_box(sd, sigma, albedo, g, box_min, box_max, grid=None):
+    """A medium filling the world box [box_min, box_max]: the [-1, 1] cube
+    (no grid) or the grid's index box mapped onto it."""
+    box_min, box_max = np.asarray(box_min, np.float64), np.asarray(box_max, np.float64)
+    if grid is None:
+        lo, hi, vol = np.full(3, -1.0), np.full(3, 1.0), 0xFFFFFFFF
+    else:
+        vol = sd.add_volume(grid)
+        bb = np.frombuffer(grid[560:608].tobytes(), "<f8")
+        lo, hi = bb[:3], bb[3:]
+    s = (box_max - box_min) / (hi - lo)
+    t = np.zeros((3, 4), np.float32)
+    t[:, :3] = np.diag(s)
+    t[:, 3] = box_min - lo * s
+    med = sd.add_material(medium_record(density=sigma, albedo=albedo, anisotropy=g, density_volume=vol))
+    return sd.add_volume_instance(med, vol, t)
+
+
+def emissive_inward_cube(sd, half, emission):
+    """Closed cube [-half, half]^3 of 12 triangles whose geometric normals
+    face inwards, with a black emissive material."""
+    c = np.array([[x, y, z] for z in (-half, half) for y in (-half, half) for x in (-half, half)], np.float32)
+    faces = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    idx = []
+    for a, b, cc, dd in faces:
+        for tri in ((a, b, cc), (a, cc, dd)):
+            v = c[list(tri)]
+            n = np.cross(v[0] - v[2], v[1] - v[2])
+            if n @ (-v.mean(0)) < 0:  # make the geometric normal point into the box
+                tri = (tri[0], tri[2], tri[1])
+            idx.extend(tri)
+    vi, nt = sd.add_mesh(c, np.array(idx, np.uint32))
+    # black walls that emit: every path that reaches a wall ends there
+    mat = sd.add_material(material_record(base_color=(0, 0, 0), emission=emission))
+    sd.add_mesh_instance(vi, nt, mat)

@@ -128,33 +128,6 @@ def _camera_dirs(view, W, H):
     return d / np.linalg.norm(d, axis=-1, keepdims=True)
 
 
-def _layered_grid(values_y):
-    """4 x len(values_y) x 4 grid, constant in x and z."""
-    d = np.zeros((4, len(values_y), 4), np.float32)
-    for j, v in enumerate(values_y):
-        d[:, j, :] = v
-    return nanovdb.write_float_grid(d, voxel_size=1.0)
-
-
-def _add_medium(sd, sigma, albedo, g, box_min, box_max, grid=None):
-    """A medium filling the world box [box_min, box_max]: the [-1, 1] cube
-    (no grid) or the grid's index box mapped onto it."""
-    box_min, box_max = np.asarray(box_min, np.float64), np.asarray(box_max, np.float64)
-    if grid is None:
-        lo, hi, vol = np.full(3, -1.0), np.full(3, 1.0), 0xFFFFFFFF
-    else:
-        vol = sd.add_volume(grid)
-        bb = np.frombuffer(grid[560:608].tobytes(), "<f8")
-        lo, hi = bb[:3], bb[3:]
-    s = (box_max - box_min) / (hi - lo)
-    t = np.zeros((3, 4), np.float32)
-    t[:, :3] = np.diag(s)
-    t[:, 3] = box_min - lo * s
-    med = sd.add_material(scenes.medium_record(density=sigma, albedo=albedo, anisotropy=g,
-                                               density_volume=vol))
-    return sd.add_volume_instance(med, vol, t)
-
-
 def _box_chord(o, d, lo, hi):
     with np.errstate(divide="ignore", invalid="ignore"):
         t0, t1 = (lo - o) / d, (hi - o) / d
@@ -172,7 +145,7 @@ def test_absorbing_medium_chord_transmittance(use_grid):
     sd.set_environment(emission=(1.0, 1.0, 1.0))
     sigma = np.array([0.6, 1.5, 3.0])
     lo, hi = np.array([-0.5, -0.4, -0.6]), np.array([0.5, 0.4, 0.6])
-    _add_medium(sd, tuple(sigma), (0.0, 0.0, 0.0), 0.0, lo, hi, _layered_grid([1.0] * 4) if use_grid else None)
+    scenes.add_medium_box(sd, tuple(sigma), (0.0, 0.0, 0.0), 0.0, lo, hi, scenes.layered_grid([1.0] * 4) if use_grid else None)
     W = H = 24
     view = scenes.make_view(W, H, position=(0.0, 0.0, 1.4))
     p = abi.default_params(W, H, max_depth=4, spp=384)
@@ -213,8 +186,8 @@ def test_lit_floor_under_absorbing_slab(use_grid):
     _floor(sd)
     h = 0.2
     sigma = np.array([1.0, 2.5, 5.0])
-    grid = _layered_grid([0.25, 1.0]) if use_grid else None
-    _add_medium(sd, tuple(sigma), (0.0, 0.0, 0.0), 0.0, (-50, 0.2, -50), (50, 0.2 + h, 50), grid)
+    grid = scenes.layered_grid([0.25, 1.0]) if use_grid else None
+    scenes.add_medium_box(sd, tuple(sigma), (0.0, 0.0, 0.0), 0.0, (-50, 0.2, -50), (50, 0.2 + h, 50), grid)
     tau = sigma * h * (0.625 if use_grid else 1.0)
     W = H = 16
     view = scenes.make_view(W, H, position=(0.0, 5.0, 0.0), orientation=scenes.axis_angle_quat((1, 0, 0), 90.0))
@@ -231,22 +204,6 @@ def test_lit_floor_under_absorbing_slab(use_grid):
     assert np.abs(rel).max() < 0.03, rel
 
 
-def _inward_cube(sd, half, emission):
-    c = np.array([[x, y, z] for z in (-half, half) for y in (-half, half) for x in (-half, half)], np.float32)
-    faces = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
-    idx = []
-    for a, b, cc, dd in faces:
-        for tri in ((a, b, cc), (a, cc, dd)):
-            v = c[list(tri)]
-            n = np.cross(v[0] - v[2], v[1] - v[2])
-            if n @ (-v.mean(0)) < 0:  # make the geometric normal point into the box
-                tri = (tri[0], tri[2], tri[1])
-            idx.extend(tri)
-    vi, nt = sd.add_mesh(c, np.array(idx, np.uint32))
-    mat = sd.add_material(scenes.material_record(base_color=(0, 0, 0), emission=emission))
-    sd.add_mesh_instance(vi, nt, mat)
-
-
 @pytest.mark.parametrize("g", [0.0, 0.3])
 def test_scattering_medium_furnace(g):
     """A white (albedo 1) scattering medium inside a closed box whose walls
@@ -254,8 +211,8 @@ def test_scattering_medium_furnace(g):
     every pixel.  Exercises free-flight sampling, HG sampling, NEE from
     medium vertices through transmittance, and its MIS with emission hits."""
     sd = scenes.SceneData()
-    _inward_cube(sd, 1.0, (1.0, 1.0, 1.0))
-    _add_medium(sd, (1.5, 1.5, 1.5), (1.0, 1.0, 1.0), g, (-0.99,) * 3, (0.99,) * 3)
+    scenes.emissive_inward_cube(sd, 1.0, (1.0, 1.0, 1.0))
+    scenes.add_medium_box(sd, (1.5, 1.5, 1.5), (1.0, 1.0, 1.0), g, (-0.99,) * 3, (0.99,) * 3)
     W = H = 16
     view = scenes.make_view(W, H, position=(0.1, -0.2, 0.3))
     p = abi.default_params(W, H, max_depth=64, spp=128)
