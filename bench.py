@@ -1,16 +1,24 @@
 """Headline benchmark: Mrays/s of the path-tracing hot path on one MI355X.
 
-Workload (one "step" = one frame): 1920x1080, 64 spp, maxPathLength 8 with
-Russian roulette from bounce 3, over 1,004,098 triangles (2000 seeded displaced
-spheres of 500 triangles + ground + 4096 emissive triangles in 256 emitters)
-under a procedural 1024x512 HDR sky sampled with an alias table.  Synthetic
-seeded scene, inputs resident in HBM before the timed region.
+Workload (one "step" = one frame), --workload spheres (default): 1920x1080,
+64 spp, maxPathLength 8 with Russian roulette from bounce 3, over 1,004,098
+triangles (2000 seeded displaced spheres of 500 triangles + ground + 4096
+emissive triangles in 256 emitters) under a procedural 1024x512 HDR sky
+sampled with an alias table, ReSTIR DI (32 candidates, temporal + spatial
+reuse over 5 neighbours in 30 px).  --workload cloud: 1024x1024, 32 spp,
+maxPathLength 16, a 256^3 NanoVDB fBm density grid (sigma_t scale 20, albedo
+0.9, g 0.3) under the same sky, delta tracking + ratio tracking.  Synthetic
+seeded scenes, inputs resident in HBM before the timed region.
 
 Prints one JSON line with the driver's contract plus:
   roofline      the dominant kernel's algorithmic HBM bytes per launch / its
-                mean HIP-event duration, against 8 TB/s
+                mean HIP-event duration, against 8 TB/s; traffic = the HBM
+                bytes per launch measured for that kernel by rocprofv3 PMC
+                (FETCH_SIZE, WRITE_SIZE) and recorded in profiles/
   cpu_baseline  the CPU oracle (a C port of the reference path tracer) on this
-                host's cores over a bounded crop of the same frame
+                host's cores over a bounded sample of the same frame
+--copy-calibration runs a plain device copy of known size (for scaling the
+PMC byte counters) and exits.
 """
 import argparse
 import json
@@ -36,6 +44,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-wave", type=int, default=0, help="0: library default")
     ap.add_argument("--no-restir", action="store_true", help="plain NEE instead of ReSTIR DI")
+    ap.add_argument("--workload", choices=["spheres", "cloud"], default="spheres")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -50,10 +60,26 @@ def main():
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
+    if args.copy_calibration:
+        copy_calibration(log)
+        return
+    cloud = args.workload == "cloud"
+    if cloud:
+        if args.width == 1920 and args.height == 1080:
+            args.width = args.height = 1024
+        if args.spp == 64:
+            args.spp = 32
+        args.no_restir = True  # ReSTIR DI is not offered with media
+    max_depth = 16 if cloud else 8
+
     t0 = time.time()
-    sd = scenes.spheres_field(seed=1)
-    view = scenes.spheres_view(args.width, args.height)
-    log(f"scene generated: {sd.triangle_count} triangles in {time.time() - t0:.1f}s")
+    if cloud:
+        sd = scenes.cloud(seed=0)
+        view = scenes.cloud_view(args.width, args.height)
+    else:
+        sd = scenes.spheres_field(seed=1)
+        view = scenes.spheres_view(args.width, args.height)
+    log(f"scene generated: {sd.triangle_count} triangles, {len(sd.volumes)} volumes in {time.time() - t0:.1f}s")
     ctx = Context(0)
     t0 = time.time()
     sc = Scene(ctx, sd)
@@ -62,12 +88,14 @@ def main():
     flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
     if not args.no_restir:  # ReSTIR DI: 32 candidates, temporal + spatial (5 neighbours, 30 px) reuse
         flags |= abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
-    params = abi.default_params(args.width, args.height, max_depth=8, spp=args.spp, flags=flags, rr_start_depth=3,
+    rr = 0 if cloud else 3
+    params = abi.default_params(args.width, args.height, max_depth=max_depth, spp=args.spp, flags=flags, rr_start_depth=rr,
                                 samples_per_wave=args.samples_per_wave)
 
-    # traversal statistics for the algorithmic-bytes model (untimed, 1 spp)
-    cp = abi.default_params(args.width, args.height, max_depth=8, spp=1, flags=flags | abi.ST_FLAG_COUNT_TRAVERSAL,
-                            rr_start_depth=3)
+    # traversal statistics for the algorithmic-bytes model (untimed, 1 spp);
+    # with media the node counters count density-grid reads
+    cp = abi.default_params(args.width, args.height, max_depth=max_depth, spp=1, flags=flags | abi.ST_FLAG_COUNT_TRAVERSAL,
+                            rr_start_depth=rr)
     r.render(view, cp)
     torch.cuda.synchronize()
     tc = ctx.timings()
@@ -96,38 +124,57 @@ def main():
     value = sum(rays) / total_s / 1e6
     ms = total_s / len(times) * 1e3
 
-    # dominant kernel class and its algorithmic bytes: 32 B ray + 16 B hit (or
-    # 48 B shadow record + 4 B flag) + 48 B per 4-wide node + 48 B per triangle
-    NODE_B, TRI_B = 48, 48
+    # dominant kernel class and its algorithmic bytes.  Surfaces: 32 B ray +
+    # 16 B hit (or 48 B shadow record + 4 B flag) + 48 B per 4-wide node + 48 B
+    # per triangle.  Media (per traceRay segment): 52 B path state read + 52 B
+    # written (or the 48 B shadow record + 16 B NEE accumulator), and 44 B per
+    # density-grid read (root tile key + child, upper mask word + table entry,
+    # lower mask word + table entry, leaf value).
+    NODE_B, TRI_B, GRID_B = 48, 48, 44
     t_last = kt[-1]
-    classes = {
-        "trace_closest": (t_last["trace_ms"], t_last["trace_launches"], t_last["closest_rays"] * (48 + NODE_B * nodes_c + TRI_B * tris_c)),
-        "trace_shadow": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * (52 + NODE_B * nodes_s + TRI_B * tris_s)),
-        "shade": (t_last["shade_ms"], t_last["shade_launches"], t_last["closest_rays"] * (16 + 64 + 64 + 64 + 48) + t_last["shadow_rays"] * 48),
-    }
+    if cloud:
+        classes = {
+            "k_trace_media": (t_last["trace_ms"], t_last["trace_launches"], t_last["closest_rays"] * 104 + GRID_B * nodes_c * t_last["closest_rays"]),
+            "k_shadow_media": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * 64 + GRID_B * nodes_s * t_last["shadow_rays"]),
+            "k_shade": (t_last["shade_ms"], t_last["shade_launches"], t_last["closest_rays"] * (16 + 64 + 64 + 28 + 48) + t_last["shadow_rays"] * 48),
+        }
+    else:
+        classes = {
+            "k_trace_closest": (t_last["trace_ms"], t_last["trace_launches"], t_last["closest_rays"] * (48 + NODE_B * nodes_c + TRI_B * tris_c)),
+            "k_trace_shadow": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * (52 + NODE_B * nodes_s + TRI_B * tris_s)),
+            "k_shade": (t_last["shade_ms"], t_last["shade_launches"], t_last["closest_rays"] * (16 + 64 + 64 + 64 + 48) + t_last["shadow_rays"] * 48),
+        }
     dom = max(classes, key=lambda k: classes[k][0])
     dms, dl, dbytes = classes[dom]
     achieved = dbytes / (dms / 1e3) / 1e9 if dms > 0 else 0.0
+    traffic, traffic_src = measured_traffic(args, dom)
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "bytes_per_launch": round(dbytes / max(1, dl)), "avg_launch_ms": round(dms / max(1, dl), 4),
-                "per_ray": {"closest_nodes": round(nodes_c, 2), "closest_tris": round(tris_c, 2),
-                            "shadow_nodes": round(nodes_s, 2), "shadow_tris": round(tris_s, 2)}}
+                "per_ray": ({"closest_grid_reads": round(nodes_c, 2), "shadow_grid_reads": round(nodes_s, 2)} if cloud else
+                            {"closest_nodes": round(nodes_c, 2), "closest_tris": round(tris_c, 2),
+                             "shadow_nodes": round(nodes_s, 2), "shadow_tris": round(tris_s, 2)})}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sd, view, args, log)
+        cpu = cpu_baseline(sd, view, args, max_depth, log)
 
+    if cloud:
+        workload = (f"{args.width}x{args.height} {args.spp}spp maxPathLength 16, 256^3 NanoVDB fBm grid (sigma_t 20, "
+                    "albedo 0.9, g 0.3), 1024x512 env alias-sampled, delta + ratio tracking")
+    elif (args.width, args.height, args.spp) == (1920, 1080, 64):
+        workload = ("1920x1080 64spp maxPathLength 8 (RR from 3), 1,004,098 tris, 4096 emissive tris, "
+                    "1024x512 env alias-sampled")
+    else:
+        workload = f"{args.width}x{args.height} {args.spp}spp maxPathLength 8"
+    if not cloud and not args.no_restir:
+        workload += ", ReSTIR DI 32 candidates + temporal + spatial 5x30px"
     line = {
-        "metric": "Mrays/s 1080p/1M tris", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "metric": "Mrays/s volume 1024^2" if cloud else "Mrays/s 1080p/1M tris", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": ("1920x1080 64spp maxPathLength 8 (RR from 3), 1,004,098 tris, 4096 emissive tris, "
-                                "1024x512 env alias-sampled" if (args.width, args.height, args.spp) == (1920, 1080, 64)
-                                else f"{args.width}x{args.height} {args.spp}spp maxPathLength 8")
-                               + ("" if args.no_restir else ", ReSTIR DI 32 candidates + temporal + spatial 5x30px"),
-                   "restir_di": not args.no_restir,
-                   "width": args.width, "height": args.height, "spp": args.spp, "max_path_length": 8,
+        "config": {"workload": workload, "restir_di": not args.no_restir,
+                   "width": args.width, "height": args.height, "spp": args.spp, "max_path_length": max_depth,
                    "triangles": sd.triangle_count, "samples_per_px_per_s": round(args.width * args.height * args.spp / (ms / 1e3), 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -135,17 +182,46 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(sd, view, args, log):
+def measured_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` from the PMC summary committed for
+    this workload (profiles/hbm_traffic_<workload>.json: FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by the copy calibration)."""
+    name = "cloud" if args.workload == "cloud" else ("spheres_plain" if args.no_restir else "spheres_restir")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{name}.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        rec = json.load(f)
+    k = rec.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    return int(k["hbm_bytes_per_dispatch"]), os.path.relpath(path, ROOT)
+
+
+def copy_calibration(log):
+    """A plain device-to-device copy of 256 MiB, three times, for scaling the
+    PMC byte counters (each copy reads and writes exactly 268435456 bytes)."""
+    import torch
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 copies of 268435456 bytes done")
+
+
+def cpu_baseline(sd, view, args, max_depth, log):
     """Oracle (C port of the reference tracer, OpenMP) on the same frame at a
     bounded number of samples (reference semantics: mip-warp env, RR at every
-    bounce), grown until it takes about a third of --cpu-seconds."""
+    bounce, plain NEE), grown until it takes about a third of --cpu-seconds."""
     from stratum2_amd import abi
     from oracle.oracle import OracleScene
     o = OracleScene(sd)
     threads = min(16, os.cpu_count() or 1)  # this host's share of cores
     spp = 1
     while True:
-        p = abi.default_params(args.width, args.height, max_depth=8, spp=spp)
+        p = abi.default_params(args.width, args.height, max_depth=max_depth, spp=spp)
         a = time.perf_counter()
         res = o.render(view, p, threads=threads, want_path_length=False)
         dt = time.perf_counter() - a
@@ -155,8 +231,8 @@ def cpu_baseline(sd, view, args, log):
         spp = min(64, spp * max(2, int(args.cpu_seconds / 3 / max(dt, 1e-3)) + 1))
     log(f"cpu baseline: {args.width}x{args.height} x {spp} spp, {nr} rays in {dt:.2f}s on {threads} threads")
     return {"value": round(nr / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (C port of the reference tracer), same {args.width}x{args.height} frame at {spp} spp, "
-                      f"maxPathLength 8, {nr} rays in {dt:.1f} s"}
+            "sample": f"CPU oracle (C port of the reference tracer, plain NEE), same {args.width}x{args.height} frame at "
+                      f"{spp} spp, maxPathLength {max_depth}, {nr} rays in {dt:.1f} s"}
 
 
 if __name__ == "__main__":

@@ -206,7 +206,10 @@ int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closes
  * out[3..5]  launches of each class
  * out[6..7]  nodes visited / triangles tested by closest-hit rays (COUNT_TRAVERSAL)
  * out[8..9]  nodes visited / triangles tested by shadow rays (COUNT_TRAVERSAL)
- * out[10..11] closest-hit rays, shadow rays traced */
+ * out[10..11] closest-hit rays, shadow rays traced (traceRay calls: with
+ *             media, one per segment between medium boundaries)
+ * In scenes with media, out[6] and out[8] count density-grid reads (free-flight
+ * collisions) of scattering and shadow rays instead of BVH nodes. */
 int st_last_timings(st_context* ctx, double out[12]);
 
 #ifdef __cplusplus

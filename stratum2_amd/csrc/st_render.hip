@@ -249,13 +249,14 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
 // the ray origin; surface hits keep t of the last segment (the shading kernel
 // measures the distance from the origin, intersection.hlsli:244).  One lane
 // per path, grid-stride.
-__global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderConsts c, WaveState w, int qsel, uint32_t* spill) {
+__global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderConsts c, WaveState w, int qsel, uint32_t* spill,
+                                                             unsigned long long* stats) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t n = w.counters[qsel];
     const uint32_t* q = w.queue[qsel];
     const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraverseStats ts{0, 0};
-    uint32_t segs = 0;
+    uint32_t segs = 0, reads = 0;
     for (uint32_t i = gtid; i < n; i += stride) {
         const uint32_t p = q[i];
         const float4 r = w.ray[p];
@@ -289,7 +290,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
                 v3 dp3, np3;
                 bool sc;
                 const v3 pp = delta_track<true>(S, m, rng, xf_point(iT, o), xf_vector(iT, dir), h.hit ? h.t : tmax, thr, dp3,
-                                                np3, sc, c.max_null);
+                                                np3, sc, c.max_null, reads);
                 dirpdf *= avg3(dp3);
                 if (sc) {
                     spos = xf_point(S.xf[med], pp);
@@ -324,6 +325,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
         if (scattered) w.ray[p] = make_float4(spos.x, spos.y, spos.z, r.w);
     }
     wave_add_u64(reinterpret_cast<unsigned long long*>(w.counters + 4), segs);
+    if (c.flags & ST_FLAG_COUNT_TRAVERSAL) wave_add_u64(&stats[0], reads);
 }
 
 // traceVisibilityRay with media (intersection.hlsli:262-338): closest hits,
@@ -332,12 +334,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
 // 1153-1162).  The reference reads an uninitialised static path state here;
 // this uses the vertex's medium and its own random stream keyed by pixel,
 // sample and path length (offset 0x40000000 | length << 16).
-__global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, RenderConsts c, WaveState w, uint32_t* spill) {
+__global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, RenderConsts c, WaveState w, uint32_t* spill,
+                                                              unsigned long long* stats) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t n = w.counters[2];
     const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraverseStats ts{0, 0};
-    uint32_t segs = 0;
+    uint32_t segs = 0, reads = 0;
     for (uint32_t i = gtid; i < n; i += stride) {
         const float4 a = w.shadow[3 * (size_t)i], b = w.shadow[3 * (size_t)i + 1], cc = w.shadow[3 * (size_t)i + 2];
         const uint32_t p = __float_as_uint(b.w);
@@ -373,7 +376,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, Render
                 v3 dp3, np3;
                 bool sc;
                 delta_track<false>(S, m, rng, xf_point(iT, o), xf_vector(iT, d), h.hit ? h.t : tmax, beta, dp3, np3, sc,
-                                   c.max_null);
+                                   c.max_null, reads);
                 neepdf *= avg3(np3);
                 if (!(beta.x != 0 && beta.y != 0 && beta.z != 0)) break;
             }
@@ -392,6 +395,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, Render
         if (!(all_nonpos(beta) || neepdf <= 0)) add_shadow_contribution(w, c.flags, p, beta / neepdf, cc.w);
     }
     wave_add_u64(reinterpret_cast<unsigned long long*>(w.counters + 4) + 1, segs);
+    if (c.flags & ST_FLAG_COUNT_TRAVERSAL) wave_add_u64(&stats[2], reads);
 }
 
 // Wavefront-aggregated append: one atomic per wave, lane order preserved.
@@ -1581,7 +1585,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
             for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
                 {
                     Timed tm(ctx, 0);
-                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, ctx->spill.as<uint32_t>());
+                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, ctx->spill.as<uint32_t>(), stats);
                     else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
                     else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
                 }
@@ -1602,7 +1606,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                 }
                 {
                     Timed tm(ctx, 2);
-                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, ctx->spill.as<uint32_t>());
+                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, ctx->spill.as<uint32_t>(), stats);
                     else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
                     else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
                 }

@@ -109,9 +109,10 @@ __device__ __forceinline__ float max3(v3 a) { return fmax_(fmax_(a.x, a.y), a.z)
 // instance's object space.  Multiplies beta by the transmittance (times
 // sigma_t * sigma_s at a real collision); returns the scattering position in
 // object space when *scattered.
+// *reads counts the grid reads (collisions) for the traffic model.
 template <bool SCATTER>
 __device__ inline v3 delta_track(const DevScene& S, const Medium& m, Rng& rng, v3 origin, v3 dir, float tmax, v3& beta,
-                                 v3& dir_pdf, v3& nee_pdf, bool& scattered, uint32_t max_null) {
+                                 v3& dir_pdf, v3& nee_pdf, bool& scattered, uint32_t max_null, uint32_t& reads) {
     scattered = false;
     dir_pdf = mk(1, 1, 1);
     nee_pdf = mk(1, 1, 1);
@@ -151,6 +152,7 @@ __device__ inline v3 delta_track(const DevScene& S, const Medium& m, Rng& rng, v
         }
         origin = origin + dir * t;
         tmax -= t;
+        reads++;
         const float dv = nvdb_read(g, root, (int)floorf(origin.x), (int)floorf(origin.y), (int)floorf(origin.z));
         const v3 sig_t = m.density * dv;
         const v3 sig_s = m.albedo();  // readAlbedo is 1 without an albedo volume
