@@ -164,6 +164,12 @@ typedef struct st_outputs {
     uint32_t* visibility;       /* RG32UI W*H: {instance|primitive<<16, packed normal}, sample 0 */
     uint32_t* path_length;      /* u32 W*H*spp: segments traced by each sample's path */
     uint64_t* ray_counts;       /* u64[2]: {extension rays, shadow rays} added by this call */
+    /* StoreAuxillaryData (testrenderer.slang:195-218) of sample 0's first vertex;
+     * each may be null.  prev_uv and the previous depth assume a static scene
+     * in steady state (identity motion transform, previous view = this view). */
+    float* albedo;              /* RGBA32F W*H: {albedo, 1} (1 on a miss) (mAlbedo) */
+    float* depth;               /* RGBA32F W*H: {fp16-stored distance, previous depth, 1, 1} (mDepth) */
+    float* prev_uv;             /* RG32F W*H: previous-frame pixel coordinate / extent (mPrevUVs) */
 } st_outputs;
 
 typedef struct st_context st_context;
@@ -190,7 +196,9 @@ int st_scene_stats(const st_scene* scene, uint64_t out[4]);
 int st_render(st_scene* scene, const st_view* view, const st_render_params* params,
               const st_outputs* outputs);
 
-/* Primary-ray closest hit only (the minimum slice): visibility for each pixel. */
+/* Primary-ray closest hit only (the minimum slice): visibility for each pixel
+ * and, when t_out is not null, the hit distance as the path state stores it
+ * (fp16-rounded, inf on a miss).  Device pointers. */
 int st_trace_primary(st_scene* scene, const st_view* view, uint32_t width, uint32_t height,
                      uint32_t* visibility, float* t_out);
 

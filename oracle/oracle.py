@@ -60,7 +60,8 @@ def lib():
 
 class or_outputs(C.Structure):
     _fields_ = [("hdr", C.c_void_p), ("visibility", C.c_void_p), ("path_length", C.c_void_p),
-                ("ray_counts", C.c_void_p), ("traversal", C.c_void_p)]
+                ("ray_counts", C.c_void_p), ("traversal", C.c_void_p), ("albedo", C.c_void_p), ("depth", C.c_void_p),
+                ("prev_uv", C.c_void_p)]
 
 
 class OracleScene:
@@ -83,12 +84,16 @@ class OracleScene:
         pl = np.zeros((S, H, W), np.uint32) if want_path_length else None
         rc = np.zeros(2, np.uint64)
         tr = np.zeros(4, np.uint64)
+        alb = np.zeros((H, W, 4), np.float32)
+        dep = np.zeros((H, W, 4), np.float32)
+        puv = np.zeros((H, W, 2), np.float32)
         o = or_outputs(hdr.ctypes.data, vis.ctypes.data, pl.ctypes.data if pl is not None else None,
-                       rc.ctypes.data, tr.ctypes.data)
+                       rc.ctypes.data, tr.ctypes.data, alb.ctypes.data, dep.ctypes.data, puv.ctypes.data)
         st = self.lib.or_render(self.h, C.byref(view), C.byref(params), C.byref(o), threads)
         if st != 0:
             raise RuntimeError("oracle render: " + self.lib.or_last_error().decode())
-        return dict(hdr=hdr, visibility=vis, path_length=pl, ray_counts=rc, traversal=tr)
+        return dict(hdr=hdr, visibility=vis, path_length=pl, ray_counts=rc, traversal=tr, albedo=alb, depth=dep,
+                    prev_uv=puv)
 
     def trace_rays(self, rays, closest=True):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

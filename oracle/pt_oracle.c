@@ -1283,6 +1283,28 @@ static hit_t trace_ray(const or_scene* s, f3 o, f3 d, float tmax, int any, int* 
     return h;
 }
 
+/* TransformData::inverse of a 3x4 affine transform, computed in double. */
+static st_transform inverse_xf(const st_transform* T) {
+    st_transform r;
+    memset(&r, 0, sizeof r);
+    double a[3][3], inv[3][3];
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) a[i][j] = T->m[i][j];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    if (det == 0) return r;
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det; inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det; inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det; inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det; inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int i = 0; i < 3; i++) {
+        double tr = 0;
+        for (int j = 0; j < 3; j++) { r.m[i][j] = (float)inv[i][j]; tr -= inv[i][j] * T->m[j][3]; }
+        r.m[i][3] = (float)tr;
+    }
+    return r;
+}
+
 or_scene* or_scene_create(const st_scene_desc* d) {
     if (!d || (d->instance_count && (!d->instances || !d->transforms))) { fail("invalid scene description"); return NULL; }
     or_scene* s = (or_scene*)calloc(1, sizeof *s);
@@ -1325,24 +1347,7 @@ or_scene* or_scene_create(const st_scene_desc* d) {
     }
     /* media: volume instances, their grids and the inverse transforms (TransformData::inverse) */
     s->inv_xf = (st_transform*)calloc(s->ninst + 1, sizeof(st_transform));
-    for (uint32_t i = 0; i < s->ninst; i++) {
-        const st_transform* T = &s->xf[i];
-        double a[3][3], inv[3][3];
-        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) a[r][c] = T->m[r][c];
-        double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
-                     a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
-        if (det == 0) continue;
-        inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det; inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
-        inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det; inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
-        inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det; inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
-        inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det; inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
-        inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
-        for (int r = 0; r < 3; r++) {
-            double tr = 0;
-            for (int c = 0; c < 3; c++) { s->inv_xf[i].m[r][c] = (float)inv[r][c]; tr -= inv[r][c] * T->m[c][3]; }
-            s->inv_xf[i].m[r][3] = (float)tr;
-        }
-    }
+    for (uint32_t i = 0; i < s->ninst; i++) s->inv_xf[i] = inverse_xf(&s->xf[i]);
     s->nvols = d->volume_count < 8 ? d->volume_count : 8;
     for (uint32_t v = 0; v < s->nvols; v++) {
         s->volsz[v] = d->volumes[v].size_bytes;
@@ -1519,7 +1524,56 @@ typedef struct {
     uint32_t vis[2];
     uint64_t rays[2], trav[4];
     uint32_t view_medium;
+    /* auxiliary outputs of the first vertex (StoreAuxillaryData) */
+    const st_view* view;
+    st_transform inv_cam;
+    uint32_t px, py, W, H;
+    float alb[3], dep[2], uv[2];
 } pctx_t;
+
+/* ViewData::toRaster (compat/scene.h:158-165) with ProjectionData::projectPoint
+ * (compat/transform.h:139-155); leaves *ox, *oy unchanged when p is not visible. */
+static void to_raster(const st_view* V, f3 p, float* ox, float* oy) {
+    const st_projection* pr = &V->projection;
+    const float sz = p.z > 0 ? 1.0f : (p.z < 0 ? -1.0f : 0.0f);
+    const float sn = pr->near_plane > 0 ? 1.0f : (pr->near_plane < 0 ? -1.0f : 0.0f);
+    if (sz != sn) return;
+    float r0, r1, r2, r3;
+    if (pr->vertical_fov < 0) {
+        r0 = p.x * pr->scale[0] + pr->offset[0];
+        r1 = p.y * pr->scale[1] + pr->offset[1];
+        r2 = (p.z - pr->far_plane) / (pr->near_plane - pr->far_plane);
+        r3 = 1;
+    } else {
+        r0 = p.x * pr->scale[0] + p.z * pr->offset[0];
+        r1 = p.y * pr->scale[1] + p.z * pr->offset[1];
+        r2 = fabsf(pr->near_plane);
+        r3 = p.z * sn;
+    }
+    const float aw = fabsf(r3);
+    if (fabsf(r0) >= aw || fabsf(r1) >= aw || fabsf(r2) >= aw) return;
+    const float ew = (float)(V->image_max[0] - V->image_min[0]), eh = (float)(V->image_max[1] - V->image_min[1]);
+    *ox = (float)V->image_min[0] + ew * ((r0 / r3) * 0.5f + 0.5f);
+    *oy = (float)V->image_min[1] + eh * ((-r1 / r3) * 0.5f + 0.5f);
+}
+
+/* StoreAuxillaryData (testrenderer.slang:195-218) for a static scene in steady
+ * state: the instance motion transform is the identity and the previous view
+ * is this view.  has_instance: the vertex is a surface or medium vertex. */
+static void store_aux(pctx_t* x, int has_instance, f3 pos, f3 albedo) {
+    float pcx = (float)x->px + 0.5f, pcy = (float)x->py + 0.5f;
+    float prev_depth = x->ps.t16;
+    if (has_instance) {
+        const f3 pc = xf_point(&x->inv_cam, pos);
+        prev_depth = length3(pc);
+        to_raster(x->view, pc, &pcx, &pcy);
+    }
+    x->alb[0] = albedo.x; x->alb[1] = albedo.y; x->alb[2] = albedo.z;
+    x->dep[0] = x->ps.t16;
+    x->dep[1] = prev_depth;
+    x->uv[0] = pcx / (float)x->W;
+    x->uv[1] = pcy / (float)x->H;
+}
 
 /* AddRadiance, testrenderer.slang:220-235 */
 static void add_radiance(pctx_t* x, f3 rad, float w) {
@@ -1733,7 +1787,10 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
     pstate_t* ps = &x->ps;
     const medium_t m = load_medium(s, s->inst[ps->medium].type_material_address >> 4);
     const f3 ldin = neg(dir);
-    if (ps->path_len == 1) { x->vis[0] = ps->inst_prim; x->vis[1] = 0; }
+    if (ps->path_len == 1) {
+        x->vis[0] = ps->inst_prim; x->vis[1] = 0;
+        store_aux(x, 1, pos, med_albedo(&m));
+    }
     if (ps->path_len >= x->c->max_depth) return 0;
     if (!med_can_eval(&m)) return 0;
     if (!med_singular(&m) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
@@ -1835,7 +1892,10 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     }
     ps->pdfa16 = q16(pdfa);
     if (!h.hit) {
-        if (ps->path_len == 1) { x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn; }
+        if (ps->path_len == 1) {
+            x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn;
+            store_aux(x, 0, v3(0, 0, 0), v3(1, 1, 1));
+        }
         float dpw;
         f3 e = mul(ps->thr, evaluate_environment(s, x->c, dir, &dpw));
         if (any_gt0(e)) {
@@ -1850,7 +1910,10 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     material_t m = load_material(s, sd.flags_material >> 4);
     f3 ldin = to_local(&sd, neg(dir));
     float cos_in = fabsf(ldin.z);
-    if (ps->path_len == 1) { x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn; }
+    if (ps->path_len == 1) {
+        x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn;
+        store_aux(x, 1, sd.pos, m_base(&m));
+    }
     if (ldin.z > 0) {
         f3 e = mul(ps->thr, m_emission(&m));
         if (any_gt0(e)) {
@@ -1909,6 +1972,7 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
     float sg = pr->near_plane > 0 ? 1.0f : (pr->near_plane < 0 ? -1.0f : 0.0f);
     f3 bp = v3(pr->near_plane * (cx * sg - pr->offset[0]) / pr->scale[0], pr->near_plane * (cy * sg - pr->offset[1]) / pr->scale[1],
                pr->near_plane);
+    if (pr->vertical_fov < 0) bp = v3((cx - pr->offset[0]) / pr->scale[0], (cy - pr->offset[1]) / pr->scale[1], pr->near_plane);
     f3 ldir = normalize3(bp);
     const st_transform* T = &view->camera_to_world;
     ps->origin = v3(T->m[0][3], T->m[1][3], T->m[2][3]);
@@ -1977,6 +2041,7 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
                 view_medium = vi->instance_index;
         }
     }
+    const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
@@ -1991,15 +2056,30 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
         x.s = s;
         x.c = &c;
         x.view_medium = view_medium;
+        x.view = view;
+        x.inv_cam = inv_cam;
+        x.W = W;
+        x.H = H;
 #pragma omp for schedule(dynamic, 64)
         for (int64_t pix = 0; pix < (int64_t)W * H; pix++) {
             uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
+            x.px = px;
+            x.py = py;
             f3 sum = v3(0, 0, 0);
             for (uint32_t k = 0; k < p->spp; k++) {
                 f3 r;
                 uint32_t pl;
                 render_pixel(&x, p, view, px, py, p->seed + k, &r, &pl);
                 if (k == 0 && out->visibility) { out->visibility[2 * pix] = x.vis[0]; out->visibility[2 * pix + 1] = x.vis[1]; }
+                if (k == 0 && out->albedo) {
+                    for (int j = 0; j < 3; j++) out->albedo[4 * pix + j] = x.alb[j];
+                    out->albedo[4 * pix + 3] = 1;
+                }
+                if (k == 0 && out->depth) {
+                    out->depth[4 * pix] = x.dep[0]; out->depth[4 * pix + 1] = x.dep[1];
+                    out->depth[4 * pix + 2] = 1; out->depth[4 * pix + 3] = 1;
+                }
+                if (k == 0 && out->prev_uv) { out->prev_uv[2 * pix] = x.uv[0]; out->prev_uv[2 * pix + 1] = x.uv[1]; }
                 if (out->path_length) out->path_length[(size_t)k * W * H + pix] = pl;
                 sum = add(sum, r);
             }

@@ -28,6 +28,9 @@ typedef struct or_outputs {
     uint32_t* path_length;   /* W*H*spp */
     uint64_t* ray_counts;    /* [2]: extension, shadow */
     uint64_t* traversal;     /* [4]: nodes visited (closest), tris tested (closest), nodes (shadow), tris (shadow) */
+    float* albedo;           /* RGBA32F W*H (sample 0), as st_outputs */
+    float* depth;            /* RGBA32F W*H (sample 0), as st_outputs */
+    float* prev_uv;          /* RG32F W*H (sample 0), as st_outputs */
 } or_outputs;
 
 or_scene* or_scene_create(const st_scene_desc* desc);

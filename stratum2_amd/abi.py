@@ -87,7 +87,7 @@ class st_render_params(C.Structure):
 
 class st_outputs(C.Structure):
     _fields_ = [("hdr", C.c_void_p), ("visibility", C.c_void_p), ("path_length", C.c_void_p),
-                ("ray_counts", C.c_void_p)]
+                ("ray_counts", C.c_void_p), ("albedo", C.c_void_p), ("depth", C.c_void_p), ("prev_uv", C.c_void_p)]
 
 
 def default_params(width, height, **kw):

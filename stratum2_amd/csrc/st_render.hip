@@ -88,6 +88,15 @@ struct RenderConsts {
     float max_m, radius;
     uint32_t view_medium, max_null;
     st_view view;
+    st_transform inv_cam;  // world to camera (TransformData::inverse of the camera transform)
+};
+
+// Auxiliary outputs of sample 0's first vertex (StoreAuxillaryData); null = not wanted.
+struct AuxOut {
+    uint32_t* visibility;
+    float* albedo;
+    float* depth;
+    float* prev_uv;
 };
 
 // -------------------------------------------------------------------- kernels
@@ -107,6 +116,53 @@ __device__ __forceinline__ void pixel_of(const RenderConsts& c, uint32_t q, uint
 __device__ __forceinline__ uint32_t path_of_pixel(const RenderConsts& c, uint32_t px, uint32_t py) {
     if (!c.tiled) return py * c.W + px;
     return (((py >> 3) * (c.W >> 3) + (px >> 3)) << 6) | ((py & 7u) << 3) | (px & 7u);
+}
+
+// ViewData::toRaster (compat/scene.h:158-165) with ProjectionData::projectPoint
+// (compat/transform.h:139-155); leaves ox, oy unchanged when p is not visible.
+__device__ inline void to_raster(const st_view& V, v3 p, float& ox, float& oy) {
+    const st_projection& pr = V.projection;
+    const float sz = p.z > 0 ? 1.0f : (p.z < 0 ? -1.0f : 0.0f);
+    const float sn = pr.near_plane > 0 ? 1.0f : (pr.near_plane < 0 ? -1.0f : 0.0f);
+    if (sz != sn) return;
+    float r0, r1, r2, r3;
+    if (pr.vertical_fov < 0) {
+        r0 = p.x * pr.scale[0] + pr.offset[0];
+        r1 = p.y * pr.scale[1] + pr.offset[1];
+        r2 = (p.z - pr.far_plane) / (pr.near_plane - pr.far_plane);
+        r3 = 1;
+    } else {
+        r0 = p.x * pr.scale[0] + p.z * pr.offset[0];
+        r1 = p.y * pr.scale[1] + p.z * pr.offset[1];
+        r2 = fabsf(pr.near_plane);
+        r3 = p.z * sn;
+    }
+    const float aw = fabsf(r3);
+    if (fabsf(r0) >= aw || fabsf(r1) >= aw || fabsf(r2) >= aw) return;
+    const float ew = (float)(V.image_max[0] - V.image_min[0]), eh = (float)(V.image_max[1] - V.image_min[1]);
+    ox = (float)V.image_min[0] + ew * ((r0 / r3) * 0.5f + 0.5f);
+    oy = (float)V.image_min[1] + eh * ((-r1 / r3) * 0.5f + 0.5f);
+}
+
+// StoreAuxillaryData (testrenderer.slang:195-218) for a static scene in steady
+// state: identity motion transform, previous view = this view.
+__device__ inline void store_aux(const RenderConsts& c, const AuxOut& aux, uint32_t pix, uint32_t px, uint32_t py,
+                                 uint32_t id, uint32_t sn, bool has_instance, v3 pos, v3 albedo, float t16) {
+    if (aux.visibility) {
+        aux.visibility[2 * pix] = id;
+        aux.visibility[2 * pix + 1] = sn;
+    }
+    if (!aux.albedo && !aux.depth && !aux.prev_uv) return;
+    float pcx = (float)px + 0.5f, pcy = (float)py + 0.5f;
+    float prev_depth = t16;
+    if (has_instance) {
+        const v3 pc = xf_point(c.inv_cam, pos);
+        prev_depth = len(pc);
+        to_raster(c.view, pc, pcx, pcy);
+    }
+    if (aux.albedo) reinterpret_cast<float4*>(aux.albedo)[pix] = make_float4(albedo.x, albedo.y, albedo.z, 1.0f);
+    if (aux.depth) reinterpret_cast<float4*>(aux.depth)[pix] = make_float4(t16, prev_depth, 1.0f, 1.0f);
+    if (aux.prev_uv) reinterpret_cast<float2*>(aux.prev_uv)[pix] = make_float2(pcx / (float)c.W, pcy / (float)c.H);
 }
 
 __global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveState w) {
@@ -131,8 +187,10 @@ __global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveStat
         const float cx = 2 * u - 1, cy = -(2 * v - 1);
         const st_projection& pr = V.projection;
         const float sg = pr.near_plane > 0 ? 1.0f : (pr.near_plane < 0 ? -1.0f : 0.0f);
-        const v3 bp = mk(pr.near_plane * (cx * sg - pr.offset[0]) / pr.scale[0],
-                         pr.near_plane * (cy * sg - pr.offset[1]) / pr.scale[1], pr.near_plane);
+        v3 bp = mk(pr.near_plane * (cx * sg - pr.offset[0]) / pr.scale[0],
+                   pr.near_plane * (cy * sg - pr.offset[1]) / pr.scale[1], pr.near_plane);
+        if (pr.vertical_fov < 0)  // orthographic projection (mVerticalFoV < 0)
+            bp = mk((cx - pr.offset[0]) / pr.scale[0], (cy - pr.offset[1]) / pr.scale[1], pr.near_plane);
         const v3 ld = normalize(bp);
         const st_transform& T = V.camera_to_world;
         const uint32_t dir = pack_normal(normalize(xf_vector(T, ld)));
@@ -556,7 +614,7 @@ __device__ __forceinline__ float mis_pair(float a, float b) {  // mis(a, b), tes
 
 template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
-                                                       uint32_t* visibility, int first_wave) {
+                                                       AuxOut aux, int first_wave) {
     const uint32_t n = w.counters[qsel];
     const uint32_t* q = w.queue[qsel];
     uint32_t* qout = w.queue[qsel ^ 1];
@@ -606,10 +664,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
             if (!hit_any) {
                 if (RESTIR && (c.flags & ST_FLAG_RESTIR_DI_REUSE) && path_len == 1)
                     w.r_init[2 * pix + 1] = make_float4(0, 0, 0, 0);
-                if (path_len == 1 && first_wave && k == 0 && visibility) {
-                    visibility[2 * pix] = 0xffffffffu;
-                    visibility[2 * pix + 1] = pack_normal(dir);
-                }
+                if (path_len == 1 && first_wave && k == 0)  // a miss stores albedo 1 (TraceRay, :1040-1041)
+                    store_aux(c, aux, pix, px, py, 0xffffffffu, pack_normal(dir), false, mk(0, 0, 0), mk(1, 1, 1),
+                              q16(__int_as_float(0x7f800000)));
                 float dpw;
                 const v3 e = thr * evaluate_environment<ALIAS>(S, c.env_prob, dir, dpw);
                 if (any_pos(e)) {
@@ -629,11 +686,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 }
                 const float t16 = q16(__uint_as_float(hit.x)), pdfa16 = q16(pdfa);
                 if (!all_nonpos(thr)) {  // ExtendPath, testrenderer.slang:1062-1063
-                    if (path_len == 1 && first_wave && k == 0 && visibility) {
-                        visibility[2 * pix] = hit.y;
-                        visibility[2 * pix + 1] = 0;
-                    }
                     const Medium m = load_medium(S, S.inst[ii].type_material_address >> 4);
+                    if (path_len == 1 && first_wave && k == 0)
+                        store_aux(c, aux, pix, px, py, hit.y, 0, true, pos, m.albedo(), t16);
                     const v3 ldin = -dir;
                     bool alive = path_len < c.max_depth && m.can_eval();
                     const bool rr_on = path_len >= c.rr_start;
@@ -692,11 +747,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 const float thit = MEDIA ? len(sd.pos - mk(r.x, r.y, r.z)) : __uint_as_float(hit.x);
                 const float t16 = q16(thit), pdfa16 = q16(pdfa);
                 const bool live = !MEDIA || !all_nonpos(thr);  // ExtendPath, testrenderer.slang:1062-1063
-                if (live && path_len == 1 && first_wave && k == 0 && visibility) {
-                    visibility[2 * pix] = hit.y;
-                    visibility[2 * pix + 1] = sd.sn;
-                }
                 const Material m = load_material(S, sd.material);
+                if (live && path_len == 1 && first_wave && k == 0)
+                    store_aux(c, aux, pix, px, py, hit.y, sd.sn, true, sd.pos, m.base(), t16);
                 const Frame f = make_frame(sd);
                 const v3 ldin = f.to_local(-dir);
                 const v3 lgn = f.to_local(unpack_normal(sd.gn));
@@ -1452,7 +1505,7 @@ int st_scene_stats(const st_scene* s, uint64_t out[4]) {
     return ST_OK;
 }
 
-using ShadeFn = void (*)(DevScene, RenderConsts, WaveState, int, uint32_t*, int);
+using ShadeFn = void (*)(DevScene, RenderConsts, WaveState, int, AuxOut, int);
 static ShadeFn shade_kernel(bool lambert, bool alias, bool restir, bool media) {
     if (media) {  // ReSTIR DI with media is rejected before this
         static const ShadeFn mt[4] = {k_shade<false, false, false, true>, k_shade<false, true, false, true>,
@@ -1549,6 +1602,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         c.rr_start = p->rr_start_depth;
         c.env_prob = p->environment_sample_probability;
         c.view = *view;
+        c.inv_cam = inverse_transform(view->camera_to_world);
         c.max_null = p->max_null_collisions;
         // the camera's medium: the last volume whose object-space bounds hold the
         // camera position (TestRenderer.cpp:431-446)
@@ -1591,9 +1645,10 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                 }
                 {
                     Timed tm(ctx, 1);
-                    uint32_t* vis = (base == 0) ? o->visibility : nullptr;
+                    AuxOut aux{nullptr, nullptr, nullptr, nullptr};
+                    if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
                     auto* kfn = shade_kernel(lambert, alias, restir, media);
-                    hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, vis, base == 0 ? 1 : 0);
+                    hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0);
                 }
                 if (reuse && b == 0) {
                     const int rgrid2 = std::min<int>((npix + kShadeBlock - 1) / kShadeBlock, ctx->cus * 8);
@@ -1672,7 +1727,6 @@ int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, u
 }
 
 int st_trace_primary(st_scene* s, const st_view* view, uint32_t width, uint32_t height, uint32_t* visibility, float* t_out) {
-    (void)t_out;
     if (!s || !view || !visibility) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
     st_render_params p;
     std::memset(&p, 0, sizeof p);
@@ -1684,9 +1738,17 @@ int st_trace_primary(st_scene* s, const st_view* view, uint32_t width, uint32_t 
     p.environment_sample_probability = 0.9f;
     st_context* ctx = s->ctx;
     try {
-        ctx->outbuf.reserve((size_t)width * height * 16);
-        st_outputs o{ctx->outbuf.as<float>(), visibility, nullptr, nullptr};
-        return st_render(s, view, &p, &o);
+        const size_t npix = (size_t)width * height;
+        ctx->outbuf.reserve(npix * 32);
+        float* hdr = ctx->outbuf.as<float>();
+        float* depth = t_out ? hdr + 4 * npix : nullptr;
+        st_outputs o{hdr, visibility, nullptr, nullptr, nullptr, depth, nullptr};
+        const int rc = st_render(s, view, &p, &o);
+        if (rc != ST_OK || !t_out) return rc;
+        // t_out[i] = depth[i].x: the first hit's fp16-stored distance (inf on a miss)
+        HIP_CHECK(hipMemcpy2DAsync(t_out, 4, depth, 16, 4, npix, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
     }

@@ -75,6 +75,16 @@ class Scene:
                   "st_trace_rays")
         return out
 
+    def trace_primary(self, view, width, height):
+        """st_trace_primary: (visibility (H, W, 2) int32, t (H, W) float32) device tensors."""
+        torch = _torch()
+        dev = torch.device("cuda", self.ctx.device)
+        vis = torch.empty((height, width, 2), dtype=torch.int32, device=dev)
+        t = torch.empty((height, width), dtype=torch.float32, device=dev)
+        abi.check(self.lib, self.lib.st_trace_primary(self.h, C.byref(view), width, height, vis.data_ptr(), t.data_ptr()),
+                  "st_trace_primary")
+        return vis, t
+
 
 class TestRenderer:
     """render(view, params) -> dict of device tensors (hdr, visibility, path_length, ray_counts)."""
@@ -82,7 +92,7 @@ class TestRenderer:
     def __init__(self, scene):
         self.scene = scene
 
-    def render(self, view, params, want_path_length=False, out=None):
+    def render(self, view, params, want_path_length=False, out=None, want_aux=False):
         torch = _torch()
         dev = torch.device("cuda", self.scene.ctx.device)
         W, H, S = params.width, params.height, params.spp
@@ -92,9 +102,15 @@ class TestRenderer:
             res["visibility"] = torch.empty((H, W, 2), dtype=torch.int32, device=dev)
             res["ray_counts"] = torch.zeros(2, dtype=torch.int64, device=dev)
             res["path_length"] = torch.empty((S, H, W), dtype=torch.int32, device=dev) if want_path_length else None
-        o = abi.st_outputs(res["hdr"].data_ptr(), res["visibility"].data_ptr(),
-                           res["path_length"].data_ptr() if res.get("path_length") is not None else None,
-                           res["ray_counts"].data_ptr())
+            if want_aux:
+                res["albedo"] = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+                res["depth"] = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+                res["prev_uv"] = torch.empty((H, W, 2), dtype=torch.float32, device=dev)
+
+        def ptr(k):
+            return res[k].data_ptr() if res.get(k) is not None else None
+        o = abi.st_outputs(ptr("hdr"), ptr("visibility"), ptr("path_length"), ptr("ray_counts"), ptr("albedo"), ptr("depth"),
+                           ptr("prev_uv"))
         abi.check(self.scene.lib, self.scene.lib.st_render(self.scene.h, C.byref(view), C.byref(params), C.byref(o)),
                   "st_render")
         return res

@@ -254,6 +254,19 @@ def make_view(width, height, position=(0, 1.5, 0), orientation=(0, 0, 0, 1), fov
     return v
 
 
+def make_ortho_view(width, height, size=(2.0, 2.0), position=(0, 1.5, 0), orientation=(0, 0, 0, 1), znear=-0.001,
+                    zfar=-10000.0):
+    """ProjectionData::makeOrthographic (compat/transform.h:172-180): scale
+    2/size, vertical_fov -1 marks the orthographic branch of backProject."""
+    v = make_view(width, height, position, orientation)
+    v.projection.scale[0] = np.float32(2) / np.float32(size[0])
+    v.projection.scale[1] = np.float32(2) / np.float32(size[1])
+    v.projection.near_plane = znear
+    v.projection.far_plane = zfar
+    v.projection.vertical_fov = -1.0
+    return v
+
+
 def axis_angle_quat(axis, deg):
     a = np.asarray(axis, np.float64)
     a = a / np.linalg.norm(a)

@@ -230,3 +230,40 @@ def test_media_camera_inside_scattering_furnace(gpu_ctx):
     big = abi.default_params(48, 48, max_depth=64, spp=256, seed=7)
     hb = _gpu_render(gpu_ctx, sd, view, big)
     assert abs(hb.mean() - 1) < 0.01, hb.mean()
+
+
+# ------------------------------------------------------------- aux outputs
+@pytest.mark.parametrize("ortho", [False, True])
+def test_auxiliary_outputs_and_cameras(gpu_ctx, ortho):
+    """albedo / depth / prev_uv (StoreAuxillaryData) and visibility of the
+    first vertex against the oracle, for the perspective and the orthographic
+    branch of backProject; st_trace_primary's distances equal depth.x."""
+    import torch
+    from stratum2_amd.renderer import Scene, TestRenderer
+    from oracle.oracle import OracleScene
+    sd = scenes.cornell_box(4)
+    scenes.add_medium_box(sd, (1.0, 1.0, 1.0), (0.7, 0.6, 0.5), 0.0, (-0.4, 0.3, -0.5), (0.3, 1.0, 0.2)) if ortho else None
+    W, H = 64, 48
+    view = (scenes.make_ortho_view(W, H, size=(1.6, 1.2), position=(0, 1, 2.4)) if ortho else scenes.cornell_view(W, H))
+    params = abi.default_params(W, H, max_depth=3, spp=2)
+    sc = Scene(gpu_ctx, sd)
+    g = TestRenderer(sc).render(view, params, want_aux=True)
+    torch.cuda.synchronize()
+    g = {k: (v.cpu().numpy() if v is not None else None) for k, v in g.items()}
+    r = OracleScene(sd).render(view, params)
+    vis = g["visibility"].view(np.uint32)
+    same = vis[..., 0] == r["visibility"][..., 0]
+    print(f"primary ids equal: {same.mean():.5f}")
+    assert same.mean() >= (0.995 if ortho else 1.0)
+    for k in ("albedo", "depth", "prev_uv"):
+        a, b = g[k][same], r[k][same]
+        fin = np.isfinite(b)
+        assert np.array_equal(np.isfinite(a), fin), k
+        assert np.allclose(a[fin], b[fin], rtol=1e-6, atol=1e-7), k
+    if not ortho:
+        v2, t = sc.trace_primary(view, W, H)
+        assert np.array_equal(v2.cpu().numpy().view(np.uint32), vis)
+        tt = t.cpu().numpy()
+        assert np.array_equal(np.isfinite(tt), np.isfinite(g["depth"][..., 0]))
+        m = np.isfinite(tt)
+        assert np.array_equal(tt[m], g["depth"][..., 0][m])

@@ -274,3 +274,53 @@ def test_furnace_closed_box():
         r = OracleScene(sd).render(scenes.make_view(W, H, position=(0, 0, 0)), params)
         mean = r["hdr"][..., :3].mean()
         assert abs(mean - 0.5 * depth) < 0.03 * depth, (depth, mean)
+
+
+def test_auxiliary_outputs_closed_form():
+    """StoreAuxillaryData outputs of the oracle: the first vertex projects back
+    into its own pixel (prev_uv), the stored depth is the fp16 rounding of the
+    camera distance, and albedo is the visible material's 8-bit base colour
+    (1 on a miss)."""
+    sd = scenes.cornell_box(0)
+    W = H = 48
+    view = scenes.cornell_view(W, H)
+    r = OracleScene(sd).render(view, abi.default_params(W, H, max_depth=3, spp=1))
+    vis, uv, dep, alb = r["visibility"][..., 0], r["prev_uv"], r["depth"], r["albedo"]
+    hit = vis != 0xFFFFFFFF
+    assert hit.mean() > 0.9
+    px = (np.arange(W) + 0.5) / W
+    py = (np.arange(H) + 0.5) / H
+    assert np.abs(uv[..., 0] - px[None, :])[hit].max() * W < 0.02
+    assert np.abs(uv[..., 1] - py[:, None])[hit].max() * H < 0.02
+    assert np.array_equal(uv[~hit][:, 0], np.broadcast_to(px[None, :], (H, W))[~hit].astype(np.float32))
+    assert np.array_equal(dep[..., 0][hit], dep[..., 1][hit].astype(np.float16).astype(np.float32))
+    assert np.all(dep[..., 2:] == 1)
+    words = np.concatenate(sd.material_words)
+    for (y, x) in np.argwhere(hit)[::37]:
+        inst = int(vis[y, x]) & 0xFFFF
+        addr = sd.instances[inst][0] >> 4
+        w0 = int(words[addr // 4])
+        base = [((w0 >> sh) & 0xFF) / 255.0 for sh in (0, 8, 16)]
+        assert np.allclose(alb[y, x, :3], base, rtol=0, atol=1e-7) and alb[y, x, 3] == 1
+
+
+def test_orthographic_camera_rays():
+    """backProject's orthographic branch (vertical_fov < 0): the primary hit
+    of every pixel is the closest hit of the ray the formula gives."""
+    sd = scenes.cornell_box(1)
+    W = H = 24
+    view = scenes.make_ortho_view(W, H, size=(1.5, 1.5), position=(0, 1, 2.4))
+    sc = OracleScene(sd)
+    r = sc.render(view, abi.default_params(W, H, max_depth=1, spp=1))
+    px, py = np.meshgrid(np.arange(W, dtype=np.float32) + np.float32(0.5), np.arange(H, dtype=np.float32) + np.float32(0.5))
+    cx = np.float32(2) * (px / np.float32(W)) - np.float32(1)
+    cy = -(np.float32(2) * (py / np.float32(H)) - np.float32(1))
+    pr = view.projection
+    d = np.stack([cx / np.float32(pr.scale[0]), cy / np.float32(pr.scale[1]), np.full_like(cx, pr.near_plane)], -1).astype(np.float64)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    rays = np.zeros((H * W, 8), np.float32)
+    rays[:, 0:3] = (0, 1, 2.4)
+    rays[:, 3] = np.inf
+    rays[:, 4:7] = d.reshape(-1, 3)  # camera rotation is the identity
+    ref = sc.trace_rays_bruteforce(rays)[:, 1].reshape(H, W)
+    assert (r["visibility"][..., 0] == ref).mean() > 0.97  # oct-snorm16 quantisation moves edge pixels
