@@ -220,6 +220,28 @@ int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closes
  * collisions) of scattering and shadow rays instead of BVH nodes. */
 int st_last_timings(st_context* ctx, double out[12]);
 
+/* Tonemapper modes (TonemapMode, src/Shaders/compat/tonemap.h:10-23). */
+#define ST_TONEMAP_RAW 0
+#define ST_TONEMAP_REINHARD 1
+#define ST_TONEMAP_REINHARD_EXTENDED 2
+#define ST_TONEMAP_REINHARD_LUMINANCE 3
+#define ST_TONEMAP_REINHARD_LUMINANCE_EXTENDED 4
+#define ST_TONEMAP_UNCHARTED2 5
+#define ST_TONEMAP_FILMIC 6
+#define ST_TONEMAP_ACES 7
+#define ST_TONEMAP_ACES_APPROX 8
+#define ST_TONEMAP_VIRIDIS_R 9
+#define ST_TONEMAP_VIRIDIS_LENGTH_RGB 10
+
+/* Replaces Tonemapper::render (src/App/Tonemapper.cpp:49-96; kernels
+ * maxReduce and main of src/Shaders/kernels/tonemap.slang:124-192): the
+ * maximum of the (albedo-modulated) input in 1/16384 fixed point, then
+ * radiance * 2^exposure, the mode's operator and, when gamma_correct, the
+ * reference's sRGB encode.  hdr and out are device RGBA32F W*H images (out may
+ * equal hdr); albedo (device RGBA32F) may be null.  --exposure maps here. */
+int st_tonemap(st_context* ctx, const float* hdr, const float* albedo, uint32_t width, uint32_t height, float exposure,
+               int mode, int gamma_correct, float* out);
+
 #ifdef __cplusplus
 }
 #endif

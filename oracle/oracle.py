@@ -54,6 +54,7 @@ def lib():
         l.or_hg_eval.argtypes = [C.c_float, vp, vp]
         l.or_hg_eval.restype = C.c_float
         l.or_hg_sample.argtypes = [C.c_float, C.c_float, C.c_float, vp, vp]
+        l.or_tonemap.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_int, vp, vp]
         _lib = l
     return _lib
 
@@ -107,3 +108,17 @@ class OracleScene:
         out = np.zeros((len(rays), 4), np.uint32)
         self.lib.or_trace_rays_bruteforce_f64(self.h, rays.ctypes.data, len(rays), out.ctypes.data)
         return out
+
+
+def tonemap(hdr, albedo=None, exposure=0.0, mode=0, gamma=True):
+    """or_tonemap on host arrays: returns (out (H, W, 4), maxima {r, g, b, lum})."""
+    hdr = np.ascontiguousarray(hdr, np.float32)
+    H, W = hdr.shape[:2]
+    alb = np.ascontiguousarray(albedo, np.float32) if albedo is not None else None
+    out = np.zeros((H, W, 4), np.float32)
+    mx = np.zeros(4, np.float32)
+    st = lib().or_tonemap(hdr.ctypes.data, alb.ctypes.data if alb is not None else None, W, H, exposure, mode, int(gamma),
+                          out.ctypes.data, mx.ctypes.data)
+    if st != 0:
+        raise RuntimeError("oracle tonemap: " + lib().or_last_error().decode())
+    return out, mx

@@ -2100,3 +2100,106 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
     if (out->traversal) for (int k = 0; k < 4; k++) out->traversal[k] = trav[k];
     return 0;
 }
+
+/* ------------------------------------------------------------ tonemapper */
+/* Tonemapper::render (src/App/Tonemapper.cpp:49-96): maxReduce then main of
+ * src/Shaders/kernels/tonemap.slang:124-192, operators :30-117, viridisQuintic
+ * and rgbToSrgb of compat/common.h:88-104, in the reference's operation order. */
+static f3 tm_div1p(f3 c) { return v3(c.x / (1 + c.x), c.y / (1 + c.y), c.z / (1 + c.z)); }
+static float tm_u2(float x) {
+    const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
+    return ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+}
+static float tm_filmic1(float c) {
+    c = maxf(0.0f, c - 0.004f);
+    return (c * (6.2f * c + 0.5f)) / (c * (6.2f * c + 1.7f) + 0.06f);
+}
+static float tm_rtt1(float v) {
+    const float a = v * (v + 0.0245786f) - 0.000090537f;
+    const float b = v * (0.983729f * v + 0.4329510f) + 0.238081f;
+    return a / b;
+}
+static float tm_aces_approx1(float v) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    v *= 0.6f;
+    return saturate((v * (a * v + b)) / (v * (c * v + d) + e));
+}
+static f3 tm_viridis(float x) {
+    const float x2 = x * x, x3 = x2 * x, x4 = x * x3, x5 = x2 * x3;
+    return v3((0.280268003f + -0.143510503f * x + 2.225793877f * x2 + -14.815088879f * x3) + (25.212752309f * x4 + -11.772589584f * x5),
+              (-0.002117546f + 1.617109353f * x + -1.909305070f * x2 + 2.701152864f * x3) + (-1.685288385f * x4 + 0.178738871f * x5),
+              (0.300805501f + 2.614650302f * x + -12.019139090f * x2 + 28.933559110f * x3) + (-33.491294770f * x4 + 13.762053843f * x5));
+}
+static float tm_srgb1(float c) { return c <= 0.0031308f ? c * 12.92f : powf(c * 1.055f, 1 / 2.4f) - 0.055f; }
+
+static f3 tm_apply(int mode, f3 c, const float mx[4]) {
+    switch (mode) {
+        case ST_TONEMAP_REINHARD: {
+            const float l = luminance(c);
+            const f3 tc = tm_div1p(c), a = divf(c, 1 + l);
+            return add(a, mul(sub(tc, a), tc));
+        }
+        case ST_TONEMAP_REINHARD_EXTENDED: {
+            const f3 m = v3(mx[0] == 0 ? 1.0f : mx[0], mx[1] == 0 ? 1.0f : mx[1], mx[2] == 0 ? 1.0f : mx[2]);
+            return mul(tm_div1p(c), add(v3(1, 1, 1), divv(c, mul(m, m))));
+        }
+        case ST_TONEMAP_REINHARD_LUMINANCE: {
+            const float l = luminance(c), l1 = l / (1 + l);
+            return mulf(c, l1 / l);
+        }
+        case ST_TONEMAP_REINHARD_LUMINANCE_EXTENDED: {
+            const float l = luminance(c), m = mx[3] == 0 ? 1.0f : mx[3];
+            const float l1 = (l / (1 + l)) * (1 + l / (m * m));
+            return mulf(c, l1 / l);
+        }
+        case ST_TONEMAP_UNCHARTED2: {
+            const float w = tm_u2(mx[3] == 0 ? 1.0f : mx[3]);
+            return divf(v3(tm_u2(c.x), tm_u2(c.y), tm_u2(c.z)), w);
+        }
+        case ST_TONEMAP_FILMIC: return v3(tm_filmic1(c.x), tm_filmic1(c.y), tm_filmic1(c.z));
+        case ST_TONEMAP_ACES: {
+            const f3 i = v3(0.59719f * c.x + 0.35458f * c.y + 0.04823f * c.z, 0.07600f * c.x + 0.90834f * c.y + 0.01566f * c.z,
+                            0.02840f * c.x + 0.13383f * c.y + 0.83777f * c.z);
+            const f3 r = v3(tm_rtt1(i.x), tm_rtt1(i.y), tm_rtt1(i.z));
+            return v3(saturate(1.60475f * r.x + -0.53108f * r.y + -0.07367f * r.z),
+                      saturate(-0.10208f * r.x + 1.10813f * r.y + -0.00605f * r.z),
+                      saturate(-0.00327f * r.x + -0.07276f * r.y + 1.07602f * r.z));
+        }
+        case ST_TONEMAP_ACES_APPROX: return v3(tm_aces_approx1(c.x), tm_aces_approx1(c.y), tm_aces_approx1(c.z));
+        case ST_TONEMAP_VIRIDIS_R: return tm_viridis(saturate(luminance(c)));
+        case ST_TONEMAP_VIRIDIS_LENGTH_RGB: return tm_viridis(saturate(luminance(c) / (mx[3] == 0 ? 1.0f : mx[3])));
+        default: return c;
+    }
+}
+
+int or_tonemap(const float* hdr, const float* albedo, uint32_t w, uint32_t h, float exposure, int mode, int gamma, float* out,
+               float max_out[4]) {
+    if (!hdr || !out) return fail("null argument");
+    const size_t n = (size_t)w * h;
+    uint32_t mq[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < n; i++) {
+        f3 v = v3(hdr[4 * i], hdr[4 * i + 1], hdr[4 * i + 2]);
+        if (albedo) v = mul(v, v3(albedo[4 * i], albedo[4 * i + 1], albedo[4 * i + 2]));
+        const float l = luminance(v);
+        if (v.x != v.x || v.y != v.y || v.z != v.z || l != l || l <= 0) continue;
+        const float q[4] = {v.x * 16384.0f, v.y * 16384.0f, v.z * 16384.0f, l * 16384.0f};
+        for (int k = 0; k < 4; k++) {
+            const float cl = minf(maxf(q[k], 0.0f), 4294967296.0f);
+            const uint32_t u = cl >= 4294967296.0f ? 0xffffffffu : (uint32_t)cl;
+            if (u > mq[k]) mq[k] = u;
+        }
+    }
+    float mx[4];
+    for (int k = 0; k < 4; k++) mx[k] = (float)mq[k] / 16384.0f;
+    if (max_out) for (int k = 0; k < 4; k++) max_out[k] = mx[k];
+    const float scale = powf(2.0f, exposure);
+    for (size_t i = 0; i < n; i++) {
+        f3 r = v3(hdr[4 * i], hdr[4 * i + 1], hdr[4 * i + 2]);
+        if (albedo) r = mul(r, add(v3(1e-2f, 1e-2f, 1e-2f), v3(albedo[4 * i], albedo[4 * i + 1], albedo[4 * i + 2])));
+        r = mulf(r, scale);
+        r = tm_apply(mode, r, mx);
+        if (gamma) r = v3(tm_srgb1(r.x), tm_srgb1(r.y), tm_srgb1(r.z));
+        out[4 * i] = r.x; out[4 * i + 1] = r.y; out[4 * i + 2] = r.z; out[4 * i + 3] = 1;
+    }
+    return 0;
+}

@@ -80,6 +80,11 @@ void or_nvdb_world_to_index(const void* grid, const float p[3], int dir, float o
 float or_hg_eval(float g, const float din[3], const float dout[3]);
 void or_hg_sample(float g, float r0, float r1, const float din[3], float out[3]);
 
+/* Tonemapper restated (tonemap.slang:30-192): like st_tonemap on host
+ * arrays; max_out (may be null) receives the reduced maxima {r, g, b, lum}. */
+int or_tonemap(const float* hdr, const float* albedo, uint32_t w, uint32_t h, float exposure, int mode, int gamma, float* out,
+               float max_out[4]);
+
 #ifdef __cplusplus
 }
 #endif

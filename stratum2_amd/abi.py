@@ -26,6 +26,11 @@ ST_FLAG_COUNT_TRAVERSAL = 1 << 8
 ST_FLAG_DIRECT_ONLY = 1 << 9
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
+# TonemapMode (src/Shaders/compat/tonemap.h:10-23)
+(ST_TONEMAP_RAW, ST_TONEMAP_REINHARD, ST_TONEMAP_REINHARD_EXTENDED, ST_TONEMAP_REINHARD_LUMINANCE,
+ ST_TONEMAP_REINHARD_LUMINANCE_EXTENDED, ST_TONEMAP_UNCHARTED2, ST_TONEMAP_FILMIC, ST_TONEMAP_ACES, ST_TONEMAP_ACES_APPROX,
+ ST_TONEMAP_VIRIDIS_R, ST_TONEMAP_VIRIDIS_LENGTH_RGB) = range(11)
+
 
 class st_instance(C.Structure):
     _fields_ = [("type_material_address", u32), ("data", u32)]
@@ -121,7 +126,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "libstratum2.so")
 EXPORTS = [
     "st_last_error", "st_version", "st_context_create", "st_context_destroy", "st_sync",
     "st_scene_create", "st_scene_destroy", "st_scene_stats", "st_render", "st_trace_primary",
-    "st_trace_rays", "st_last_timings",
+    "st_trace_rays", "st_last_timings", "st_tonemap",
 ]
 
 _lib = None
@@ -147,6 +152,7 @@ def load_library(path=LIB_PATH):
     lib.st_render.argtypes = [vp, C.POINTER(st_view), C.POINTER(st_render_params), C.POINTER(st_outputs)]
     lib.st_trace_primary.argtypes = [vp, C.POINTER(st_view), u32, u32, vp, vp]
     lib.st_trace_rays.argtypes = [vp, vp, u32, C.c_int, vp]
+    lib.st_tonemap.argtypes = [vp, vp, vp, u32, u32, C.c_float, C.c_int, C.c_int, vp]
     lib.st_last_timings.argtypes = [vp, C.POINTER(C.c_double)]
     if path == LIB_PATH:
         _lib = lib

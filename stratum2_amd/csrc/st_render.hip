@@ -33,6 +33,7 @@
 #include "st_bvh.h"
 #include "st_layout.h"
 #include "st_shade.h"
+#include "st_tonemap.h"
 #include "st_trace.h"
 #include "st_volume.h"
 
@@ -1003,6 +1004,60 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, const fl
     else trace_queue<true, false>(S, n, fetch, src, lds, spill, ts);
 }
 
+// Tonemapper::render's two passes (tonemap.slang:124-192).  maxReduce: the
+// maximum of rgb and luminance in 1/16384 fixed point (one atomic per wave).
+__global__ void k_tonemap_max(const float4* in, const float4* albedo, uint32_t n, uint32_t* mx) {
+    uint32_t m[4] = {0, 0, 0, 0};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 c = in[i];
+        v3 v = mk(c.x, c.y, c.z);
+        if (albedo) {
+            const float4 a = albedo[i];
+            v = v * mk(a.x, a.y, a.z);
+        }
+        const float l = lum(v);
+        if (v.x != v.x || v.y != v.y || v.z != v.z || l != l || l <= 0) continue;
+        const float q[4] = {v.x * kTonemapQuant, v.y * kTonemapQuant, v.z * kTonemapQuant, l * kTonemapQuant};
+        for (int k = 0; k < 4; k++) {
+            const float cl = fmin_(fmax_(q[k], 0.0f), 4294967296.0f);  // clamp(v * Q, 0, float(0xFFFFFFFF))
+            const uint32_t u = cl >= 4294967296.0f ? 0xffffffffu : (uint32_t)cl;
+            m[k] = m[k] > u ? m[k] : u;
+        }
+    }
+    for (int k = 0; k < 4; k++) {
+        uint32_t v = m[k];
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t o = __shfl_xor(v, off);
+            v = v > o ? v : o;
+        }
+        if (__lane_id() == 0 && v) atomicMax(&mx[k], v);
+    }
+}
+
+// main: exposure, operator, optional sRGB encode
+__global__ void k_tonemap(const float4* in, const float4* albedo, uint32_t n, const uint32_t* mx, float exposure, int mode,
+                          int gamma, float4* out) {
+    const float4 mv = make_float4((float)mx[0] / kTonemapQuant, (float)mx[1] / kTonemapQuant, (float)mx[2] / kTonemapQuant,
+                                  (float)mx[3] / kTonemapQuant);
+    const float scale = powf(2.0f, exposure);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 c = in[i];
+        v3 r = mk(c.x, c.y, c.z);
+        if (albedo) {
+            const float4 a = albedo[i];
+            r = r * (mk(1e-2f, 1e-2f, 1e-2f) + mk(a.x, a.y, a.z));
+        }
+        r = r * scale;
+        r = tonemap(mode, r, mv);
+        if (gamma) r = mk(tm_srgb1(r.x), tm_srgb1(r.y), tm_srgb1(r.z));
+        out[i] = make_float4(r.x, r.y, r.z, 1.0f);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 // -------------------------------------------------------------------- host side
 struct DevBuf {
     void* p = nullptr;
@@ -1029,7 +1084,8 @@ struct st_context {
     int device = 0;
     int cus = 256;
     hipStream_t stream = nullptr;
-    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, acc, stats, raysbuf, outbuf, pri, rinit, rtemp, rprev, med;
+    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, acc, stats, raysbuf, outbuf, pri, rinit, rtemp, rprev, med,
+        tmax;
     double timings[12] = {0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     size_t ev_used[3] = {0, 0, 0};
@@ -1747,6 +1803,32 @@ int st_trace_primary(st_scene* s, const st_view* view, uint32_t width, uint32_t 
         if (rc != ST_OK || !t_out) return rc;
         // t_out[i] = depth[i].x: the first hit's fp16-stored distance (inf on a miss)
         HIP_CHECK(hipMemcpy2DAsync(t_out, 4, depth, 16, 4, npix, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_tonemap(st_context* ctx, const float* hdr, const float* albedo, uint32_t width, uint32_t height, float exposure,
+               int mode, int gamma_correct, float* out) {
+    if (!ctx || !hdr || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (mode < ST_TONEMAP_RAW || mode > ST_TONEMAP_VIRIDIS_LENGTH_RGB) return set_error(ST_ERROR_INVALID_ARGUMENT, "unknown tonemap mode");
+    if ((uint64_t)width * height > (1ull << 30)) return set_error(ST_ERROR_LIMIT, "image too large");
+    const uint32_t n = width * height;
+    if (n == 0) return ST_OK;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        ctx->tmax.reserve(16);
+        uint32_t* mx = ctx->tmax.as<uint32_t>();
+        const int grid = std::min<int>((n + 255) / 256, ctx->cus * 8);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, mx, 4u, 0u);
+        hipLaunchKernelGGL(k_tonemap_max, dim3(grid), dim3(256), 0, ctx->stream, reinterpret_cast<const float4*>(hdr),
+                           reinterpret_cast<const float4*>(albedo), n, mx);
+        hipLaunchKernelGGL(k_tonemap, dim3(grid), dim3(256), 0, ctx->stream, reinterpret_cast<const float4*>(hdr),
+                           reinterpret_cast<const float4*>(albedo), n, (const uint32_t*)mx, exposure, mode, gamma_correct,
+                           reinterpret_cast<float4*>(out));
+        HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return ST_OK;
     } catch (const std::exception& e) {

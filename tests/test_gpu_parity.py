@@ -267,3 +267,29 @@ def test_auxiliary_outputs_and_cameras(gpu_ctx, ortho):
         assert np.array_equal(np.isfinite(tt), np.isfinite(g["depth"][..., 0]))
         m = np.isfinite(tt)
         assert np.array_equal(tt[m], g["depth"][..., 0][m])
+
+
+@pytest.mark.parametrize("mode", list(range(11)))
+def test_tonemap_matches_oracle(gpu_ctx, mode):
+    """st_tonemap against the oracle's restatement of tonemap.slang on a
+    rendered HDR image, with and without albedo modulation and sRGB encode."""
+    import torch
+    from stratum2_amd.renderer import Scene, TestRenderer
+    from oracle.oracle import tonemap
+    sd = scenes.cornell_box(0)
+    W = H = 48
+    out = TestRenderer(Scene(gpu_ctx, sd)).render(scenes.cornell_view(W, H), abi.default_params(W, H, max_depth=4, spp=2),
+                                                   want_aux=True)
+    torch.cuda.synchronize()
+    hdr, alb = out["hdr"], out["albedo"]
+    for use_alb, gamma in ((False, True), (True, False)):
+        res = torch.empty_like(hdr)
+        lib = gpu_ctx.lib
+        abi.check(lib, lib.st_tonemap(gpu_ctx.h, hdr.data_ptr(), alb.data_ptr() if use_alb else None, W, H, 1.5, mode,
+                                      int(gamma), res.data_ptr()), "st_tonemap")
+        torch.cuda.synchronize()
+        ref, _ = tonemap(hdr.cpu().numpy(), alb.cpu().numpy() if use_alb else None, 1.5, mode, gamma)
+        got = res.cpu().numpy()
+        # black pixels give 0/0 in the luminance operators, as in the reference
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        assert np.allclose(got, ref, rtol=2e-6, atol=2e-7, equal_nan=True), (mode, use_alb, gamma)

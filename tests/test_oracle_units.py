@@ -324,3 +324,36 @@ def test_orthographic_camera_rays():
     rays[:, 4:7] = d.reshape(-1, 3)  # camera rotation is the identity
     ref = sc.trace_rays_bruteforce(rays)[:, 1].reshape(H, W)
     assert (r["visibility"][..., 0] == ref).mean() > 0.97  # oct-snorm16 quantisation moves edge pixels
+
+
+def test_tonemap_operators_closed_forms():
+    """The oracle's tonemapper: Reinhard of a grey value is c/(1+c), the
+    extended operators map the image maximum to 1, exposure scales by 2^E,
+    the ACES / filmic curves start at 0 and saturate at 1, viridis runs from
+    its dark-purple to its yellow end, and the maxima are the 1/16384
+    fixed-point truncation of the largest rgb and luminance."""
+    from oracle.oracle import tonemap
+    g = np.array([0.05, 0.3, 1.0, 2.5, 7.0, 40.0], np.float32)
+    img = np.ones((1, len(g), 4), np.float32)
+    img[0, :, :3] = g[:, None]
+    for mode in (abi.ST_TONEMAP_REINHARD, abi.ST_TONEMAP_REINHARD_LUMINANCE):
+        out, _ = tonemap(img, mode=mode, gamma=False)
+        assert np.allclose(out[0, :, 0], g / (1 + g), rtol=2e-6)
+    for mode in (abi.ST_TONEMAP_REINHARD_EXTENDED, abi.ST_TONEMAP_REINHARD_LUMINANCE_EXTENDED, abi.ST_TONEMAP_UNCHARTED2):
+        out, mx = tonemap(img, mode=mode, gamma=False)
+        assert abs(out[0, -1, 0] - 1) < 2e-3, (mode, out[0, -1])  # the max (quantised to 1/16384) maps to white
+        assert np.all(np.diff(out[0, :, 0]) > 0)
+    out, mx = tonemap(img, mode=abi.ST_TONEMAP_RAW, exposure=2.0, gamma=False)
+    assert np.array_equal(out[0, :, :3], img[0, :, :3] * 4)
+    assert np.array_equal(mx[:3], np.floor(np.float32(40.0) * 16384) / 16384 * np.ones(3, np.float32))
+    z = np.zeros((1, 1, 4), np.float32)
+    for mode in (abi.ST_TONEMAP_ACES, abi.ST_TONEMAP_ACES_APPROX, abi.ST_TONEMAP_FILMIC):
+        lo, _ = tonemap(z, mode=mode, gamma=False)
+        hi, _ = tonemap(img * 1e4, mode=mode, gamma=False)
+        assert np.all(np.abs(lo[0, 0, :3]) < 1e-3) and np.all(np.abs(hi[0, -1, :3] - 1) < 2e-2), mode
+    v0, _ = tonemap(z, mode=abi.ST_TONEMAP_VIRIDIS_R, gamma=False)
+    v1, _ = tonemap(img * 100, mode=abi.ST_TONEMAP_VIRIDIS_R, gamma=False)
+    assert np.allclose(v0[0, 0, :3], [0.267, 0.005, 0.329], atol=0.04)
+    assert np.allclose(v1[0, -1, :3], [0.993, 0.906, 0.144], atol=0.05)  # the quintic fit ends at blue 0.10
+    s, _ = tonemap(img, mode=abi.ST_TONEMAP_REINHARD, gamma=True)
+    assert np.all(np.diff(s[0, :, 0]) > 0) and np.all(s[0, :, 0] < 1)
