@@ -187,6 +187,15 @@ int st_sync(st_context* ctx);
  * instances and builds the SAH BVH on the host, uploads everything. */
 int st_scene_create(st_context* ctx, const st_scene_desc* desc, st_scene** out);
 int st_scene_destroy(st_scene* scene);
+
+/* Moves the instances (instance_count host 3x4 transforms, the same
+ * TransformData layout as st_scene_desc) and rebuilds the BVH on the GPU:
+ * triangles re-flattened on the device, linear BVH (Morton order + radix
+ * tree) collapsed to the 4-wide node format.  Replaces the reference's
+ * per-update acceleration-structure rebuild (src/App/Scene.cpp:726-754).
+ * build_ms (may be null) receives the device build time.  Hits do not depend
+ * on the tree: ties resolve to the smaller instance|primitive id. */
+int st_scene_update(st_scene* scene, const st_transform* transforms, double* build_ms);
 /* BVH statistics: node count, triangle count, max depth, bytes on device. */
 int st_scene_stats(const st_scene* scene, uint64_t out[4]);
 

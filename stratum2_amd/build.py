@@ -18,7 +18,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=o
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     objs = []
-    for src in ["st_bvh.cpp", "st_render.hip"]:
+    for src in ["st_bvh.cpp", "st_lbvh.hip", "st_render.hip"]:
         obj = os.path.join(OUT_DIR, src.replace(".", "_") + ".o")
         cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):

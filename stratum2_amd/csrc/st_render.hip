@@ -32,6 +32,7 @@
 #include "../../include/stratum2.h"
 #include "st_bvh.h"
 #include "st_layout.h"
+#include "st_lbvh.h"
 #include "st_shade.h"
 #include "st_tonemap.h"
 #include "st_trace.h"
@@ -1004,6 +1005,27 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, const fl
     else trace_queue<true, false>(S, n, fetch, src, lds, spill, ts);
 }
 
+// World-space triangle records (v0.xyz|id, v1, v2) of every mesh instance
+// under the current transforms, for the device BVH builder.  tri_inst[t] is
+// the instance of flattened triangle t, inst_first[i] its first triangle.
+__global__ void k_flatten(DevScene S, const uint32_t* tri_inst, const uint32_t* inst_first, uint32_t ntri, float4* out) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntri; t += gridDim.x * blockDim.x) {
+        const uint32_t i = tri_inst[t], p = t - inst_first[i];
+        const st_instance in = S.inst[i];
+        const st_transform T = S.xf[i];
+        const st_mesh_vertex_info vi = S.vi[in.data & 0xffffu];
+        uint32_t tri[3];
+        load_tri(S, vi, p, tri);
+        const uint32_t pb = vi.packed_buffer_indices[0] >> 16, ps = (vi.packed_strides >> 8) & 0xffu;
+        const uint8_t* P = S.vb[pb] + vi.packed_offsets[1];
+        const v3 a = xf_point(T, ld_v3(P + (size_t)ps * tri[0])), b = xf_point(T, ld_v3(P + (size_t)ps * tri[1])),
+                 c = xf_point(T, ld_v3(P + (size_t)ps * tri[2]));
+        out[3 * (size_t)t] = make_float4(a.x, a.y, a.z, __uint_as_float(i | (p << 16)));
+        out[3 * (size_t)t + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+        out[3 * (size_t)t + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+}
+
 // Tonemapper::render's two passes (tonemap.slang:124-192).  maxReduce: the
 // maximum of rgb and luminance in 1/16384 fixed point (one atomic per wave).
 __global__ void k_tonemap_max(const float4* in, const float4* albedo, uint32_t n, uint32_t* mx) {
@@ -1099,8 +1121,23 @@ struct st_scene {
     std::vector<st_volume_info> volume_info;
     std::vector<st_transform> inv_xf;
     uint64_t stats[4] = {0, 0, 0, 0};
+    // for scene updates: per-instance object-space bounds (the scene sphere),
+    // the flattened triangles' owners, and the BVH buffers (replaced on update)
+    struct InstBox {
+        int kind;  // 0 none, 1 mesh, 2 volume
+        float lo[3], hi[3];
+    };
+    std::vector<InstBox> boxes;
+    std::vector<uint32_t> inst_first;
+    uint64_t ntri = 0;
+    const uint32_t* tri_inst = nullptr;
+    const uint32_t* inst_first_dev = nullptr;
+    void* bvh_nodes = nullptr;  // hipMalloc'ed by the device builder, else null
+    void* bvh_tris = nullptr;
     ~st_scene() {
         for (auto* b : bufs) delete b;
+        if (bvh_nodes) (void)hipFree(bvh_nodes);
+        if (bvh_tris) (void)hipFree(bvh_tris);
     }
     template <class T>
     T* upload(const void* src, size_t bytes) {
@@ -1169,6 +1206,102 @@ void validate_desc(const st_scene_desc* d) {
             if (d->material_data[addr / 4 + 4 + k] < ST_IMAGE_LIMIT) throw std::domain_error("material textures are not supported");
         if ((in.data & 0xffffu) >= d->mesh_vertex_info_count) throw std::invalid_argument("vertex info index out of range");
     }
+}
+
+// Scene AABB (Scene.cpp:622-631 mesh instances, :672-711 volumes, where the
+// reference lets the running min/max feed the later corners) and the bounding
+// sphere of TestRenderer.cpp:458-461.
+template <class Box>
+void scene_sphere(const std::vector<Box>& boxes, const st_transform* xf, float sphere[4]) {
+    auto xfp = [](const st_transform& T, const float v[3], float o[3]) {
+        for (int r = 0; r < 3; r++) o[r] = T.m[r][0] * v[0] + T.m[r][1] * v[1] + T.m[r][2] * v[2] + T.m[r][3];
+    };
+    float amin[3] = {INFINITY, INFINITY, INFINITY}, amax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < boxes.size(); i++) {
+        const Box& b = boxes[i];
+        if (b.kind == 2) {
+            float mn[3] = {b.lo[0], b.lo[1], b.lo[2]}, mx[3] = {b.hi[0], b.hi[1], b.hi[2]};
+            for (int cidx = 0; cidx < 8; cidx++) {
+                const float corner[3] = {(cidx % 2) == 0 ? mn[0] : mx[0], ((cidx % 4) / 2) == 0 ? mn[1] : mx[1],
+                                         (cidx / 4) == 0 ? mn[2] : mx[2]};
+                float wc[3];
+                xfp(xf[i], corner, wc);
+                for (int k = 0; k < 3; k++) {
+                    mn[k] = std::min(mn[k], wc[k]);
+                    mx[k] = std::max(mx[k], wc[k]);
+                }
+            }
+            for (int k = 0; k < 3; k++) {
+                amin[k] = std::min(amin[k], mn[k]);
+                amax[k] = std::max(amax[k], mx[k]);
+            }
+        } else if (b.kind == 1) {
+            for (int cidx = 0; cidx < 8; cidx++) {
+                const float corner[3] = {(cidx % 2) == 0 ? b.lo[0] : b.hi[0], ((cidx % 4) / 2) == 0 ? b.lo[1] : b.hi[1],
+                                         (cidx / 4) == 0 ? b.lo[2] : b.hi[2]};
+                float w[3];
+                xfp(xf[i], corner, w);
+                for (int k = 0; k < 3; k++) {
+                    amin[k] = std::min(amin[k], w[k]);
+                    amax[k] = std::max(amax[k], w[k]);
+                }
+            }
+        }
+    }
+    float cen[3], dd = 0;
+    for (int k = 0; k < 3; k++) cen[k] = (amax[k] + amin[k]) / 2;
+    for (int k = 0; k < 3; k++) dd += (amax[k] - cen[k]) * (amax[k] - cen[k]);
+    sphere[0] = cen[0];
+    sphere[1] = cen[1];
+    sphere[2] = cen[2];
+    sphere[3] = std::sqrt(dd);
+}
+
+// Structural check of a 4-wide tree before the traversal kernels may use it:
+// every inner child index in range and referenced once, every triangle in
+// exactly one leaf, depth within the traversal stack.  Returns "" when valid.
+std::string check_bvh4(const std::vector<Node4Out>& nodes, uint32_t tri_count) {
+    const size_t nn = nodes.size();
+    if (nn == 0) return "empty tree";
+    std::vector<uint8_t> seen_node(nn, 0), seen_tri(tri_count, 0);
+    std::vector<uint32_t> level{0}, next;
+    seen_node[0] = 1;
+    int depth = 0;
+    size_t reached = 0;
+    while (!level.empty()) {
+        if (++depth > kMaxDepth) return "tree deeper than the traversal stack";
+        next.clear();
+        for (uint32_t ni : level) {
+            reached++;
+            const Node4Out& n = nodes[ni];
+            const uint32_t types = (n.x0 >> 24) | ((n.inner >> 28) << 8);
+            const uint32_t base_inner = n.inner & 0x0fffffffu;
+            uint32_t r = 0, off = 0;
+            for (int c = 0; c < 4; c++) {
+                const uint32_t t = (types >> (3 * c)) & 7u;
+                if (t == 7) {
+                    const uint64_t ci = (uint64_t)base_inner + r++;
+                    if (ci >= nn || seen_node[ci]) return "bad inner child index";
+                    seen_node[ci] = 1;
+                    next.push_back((uint32_t)ci);
+                } else if (t >= 1 && t <= 4) {
+                    for (uint32_t k = 0; k < t; k++) {
+                        const uint64_t ti = (uint64_t)n.tri + off + k;
+                        if (ti >= tri_count || seen_tri[ti]) return "bad leaf triangle range";
+                        seen_tri[ti] = 1;
+                    }
+                    off += t;
+                } else if (t != 0) {
+                    return "bad child type";
+                }
+            }
+        }
+        level.swap(next);
+    }
+    if (reached != nn) return "unreachable nodes";
+    for (uint32_t t = 0; t < tri_count; t++)
+        if (!seen_tri[t]) return "triangle missing from the leaves";
+    return "";
 }
 
 // NanoVDB 32.3 float grid: header checks and a walk of every child offset
@@ -1355,15 +1488,18 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
             if ((d->instances[i].type_material_address & 0xfu) == 2) D.vinst[D.vinst_count++] = i;
         for (uint32_t i = 0; i < d->volume_info_count; i++) s->volume_info.push_back(d->volume_info[i]);
         std::vector<float> verts(9 * ntri + 9);
-        std::vector<uint32_t> ids(ntri + 1);
+        std::vector<uint32_t> ids(ntri + 1), owner(ntri + 1, 0);
+        s->boxes.resize(d->instance_count);
+        s->inst_first.assign(d->instance_count + 1, 0);
         std::vector<uint32_t> lights;
-        float amin[3] = {INFINITY, INFINITY, INFINITY}, amax[3] = {-INFINITY, -INFINITY, -INFINITY};
         uint64_t t = 0;
         for (uint32_t i = 0; i < d->instance_count; i++) {
             const st_instance& in = d->instances[i];
+            st_scene::InstBox& ib0 = s->boxes[i];
+            ib0.kind = 0;
+            s->inst_first[i] = (uint32_t)t;
             if ((in.type_material_address & 0xfu) == 2) {
-                // volume box corners (Scene.cpp:672-711); as in the reference, the
-                // running min/max also feed the later corners
+                // volume box: the grid's world bbox, or the [-1, 1] cube (Scene.cpp:672-680)
                 float mn[3] = {-1, -1, -1}, mx[3] = {1, 1, 1};
                 if (in.data < d->volume_count) {
                     double bb[6];
@@ -1373,19 +1509,10 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
                         mx[k] = (float)bb[3 + k];
                     }
                 }
-                for (int cidx = 0; cidx < 8; cidx++) {
-                    const float corner[3] = {(cidx % 2) == 0 ? mn[0] : mx[0], ((cidx % 4) / 2) == 0 ? mn[1] : mx[1],
-                                             (cidx / 4) == 0 ? mn[2] : mx[2]};
-                    float wc[3];
-                    xfp(d->transforms[i], corner, wc);
-                    for (int k = 0; k < 3; k++) {
-                        mn[k] = std::min(mn[k], wc[k]);
-                        mx[k] = std::max(mx[k], wc[k]);
-                    }
-                }
+                ib0.kind = 2;
                 for (int k = 0; k < 3; k++) {
-                    amin[k] = std::min(amin[k], mn[k]);
-                    amax[k] = std::max(amax[k], mx[k]);
+                    ib0.lo[k] = mn[k];
+                    ib0.hi[k] = mx[k];
                 }
                 continue;
             }
@@ -1411,27 +1538,19 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
                     xfp(d->transforms[i], v, &verts[9 * t + 3 * j]);
                 }
                 ids[t] = i | (p << 16);
+                owner[t] = i;
             }
             if (pc == 0) continue;
-            for (int cidx = 0; cidx < 8; cidx++) {  // transformed AABB corners, Scene.cpp:622-631
-                const float corner[3] = {(cidx % 2) == 0 ? lo[0] : hi[0], ((cidx % 4) / 2) == 0 ? lo[1] : hi[1],
-                                         (cidx / 4) == 0 ? lo[2] : hi[2]};
-                float w[3];
-                xfp(d->transforms[i], corner, w);
-                for (int k = 0; k < 3; k++) {
-                    amin[k] = std::min(amin[k], w[k]);
-                    amax[k] = std::max(amax[k], w[k]);
-                }
+            ib0.kind = 1;  // transformed AABB corners, Scene.cpp:622-631
+            for (int k = 0; k < 3; k++) {
+                ib0.lo[k] = lo[k];
+                ib0.hi[k] = hi[k];
             }
         }
-        // scene bounding sphere, TestRenderer.cpp:458-461
-        float cen[3], dd = 0;
-        for (int k = 0; k < 3; k++) cen[k] = (amax[k] + amin[k]) / 2;
-        for (int k = 0; k < 3; k++) dd += (amax[k] - cen[k]) * (amax[k] - cen[k]);
-        D.sphere[0] = cen[0];
-        D.sphere[1] = cen[1];
-        D.sphere[2] = cen[2];
-        D.sphere[3] = std::sqrt(dd);
+        scene_sphere(s->boxes, d->transforms, D.sphere);
+        s->ntri = ntri;
+        s->tri_inst = s->upload<uint32_t>(owner.data(), 4 * owner.size());
+        s->inst_first_dev = s->upload<uint32_t>(s->inst_first.data(), 4 * s->inst_first.size());
         D.light_inst = s->upload<uint32_t>(lights.data(), 4 * lights.size());
         D.light_count = (uint32_t)lights.size();
         BvhOut bvh = build_bvh(verts.data(), ids.data(), (uint32_t)ntri);
@@ -1542,6 +1661,59 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
         return set_error(ST_ERROR_INVALID_ARGUMENT, e.what());
     } catch (const std::bad_alloc& e) {
         return set_error(ST_ERROR_OUT_OF_MEMORY, e.what());
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_scene_update(st_scene* s, const st_transform* transforms, double* build_ms) {
+    if (!s || !transforms) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    st_context* ctx = s->ctx;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        DevScene& D = s->d;
+        const uint32_t ni = D.inst_count;
+        // transforms, their inverses and the scene sphere
+        HIP_CHECK(hipMemcpy(const_cast<st_transform*>(D.xf), transforms, sizeof(st_transform) * ni, hipMemcpyHostToDevice));
+        for (uint32_t i = 0; i < ni; i++) s->inv_xf[i] = inverse_transform(transforms[i]);
+        HIP_CHECK(hipMemcpy(const_cast<st_transform*>(D.inv_xf), s->inv_xf.data(), sizeof(st_transform) * ni, hipMemcpyHostToDevice));
+        scene_sphere(s->boxes, transforms, D.sphere);
+        double ms = 0;
+        if (s->ntri > 0) {
+            // flatten on the device and rebuild the BVH there
+            DevBuf flat;
+            flat.reserve(48 * s->ntri);
+            const uint32_t n = (uint32_t)s->ntri;
+            const int grid = std::min<int>((n + 255) / 256, ctx->cus * 16);
+            hipLaunchKernelGGL(k_flatten, dim3(grid), dim3(256), 0, ctx->stream, D, s->tri_inst, s->inst_first_dev, n,
+                               flat.as<float4>());
+            HIP_CHECK(hipGetLastError());
+            LbvhResult r = build_lbvh(flat.as<float4>(), n, ctx->stream);
+            std::vector<Node4Out> hn(r.node_count);
+            HIP_CHECK(hipMemcpy(hn.data(), r.nodes, sizeof(Node4Out) * hn.size(), hipMemcpyDeviceToHost));
+            const std::string bad = (3 * r.depth4 + 1 > kMaxStack) ? std::string("tree deeper than the traversal stack")
+                                                                     : check_bvh4(hn, r.tri_count);
+            if (!bad.empty() || r.tri_count != n) {
+                (void)hipFree(r.nodes);
+                (void)hipFree(r.tris);
+                return set_error(ST_ERROR_LIMIT, "device BVH rejected: " + (bad.empty() ? std::string("triangle count") : bad));
+            }
+            if (s->bvh_nodes) (void)hipFree(s->bvh_nodes);
+            if (s->bvh_tris) (void)hipFree(s->bvh_tris);
+            s->bvh_nodes = r.nodes;
+            s->bvh_tris = r.tris;
+            D.nodes = r.nodes;
+            D.tris = r.tris;
+            D.node_count = r.node_count;
+            D.tri_count = r.tri_count;
+            s->stats[0] = r.node_count;
+            s->stats[1] = r.tri_count;
+            s->stats[2] = (uint64_t)r.depth4;
+            s->stats[3] = (uint64_t)r.node_count * sizeof(Node4Out) + (uint64_t)r.tri_count * sizeof(TriOut);
+            ms = r.ms;
+        }
+        if (build_ms) *build_ms = ms;
+        return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
     }

@@ -50,6 +50,15 @@ class Scene:
         self.h = C.c_void_p()
         abi.check(self.lib, self.lib.st_scene_create(ctx.h, C.byref(self._desc), C.byref(self.h)), "st_scene_create")
 
+    def update(self, transforms):
+        """st_scene_update: new (instance_count, 3, 4) transforms, BVH rebuilt on
+        the GPU.  Returns the device build time in ms."""
+        import numpy as np
+        xf = np.ascontiguousarray(transforms, np.float32)
+        ms = C.c_double(0)
+        abi.check(self.lib, self.lib.st_scene_update(self.h, xf.ctypes.data, C.byref(ms)), "st_scene_update")
+        return ms.value
+
     def stats(self):
         out = (C.c_uint64 * 4)()
         abi.check(self.lib, self.lib.st_scene_stats(self.h, out), "st_scene_stats")
