@@ -292,7 +292,7 @@ __device__ inline v3 disney_eval(const Material& m, v3 din, v3 dout) {
                 float sw = 1 - m.trans() * (1 - metallic);
                 c = c + (((F * sw) * D) * G) / (4 * ndin);
             }
-            if (ndin > 0 && ndh > 0) {
+            if (m.cc() > 0 && ndin > 0 && ndh > 0) {  // a zero clearcoat weight adds exactly 0
                 float ac = lerp(0.1f, 0.001f, m.ccgloss());
                 float Fc = schlick1(0.04f, hdout), Dc = gtr1(ndh, ac);
                 float Gc = smith2(din, 0.25f, 0.25f) * smith2(dout, 0.25f, 0.25f);
@@ -348,12 +348,14 @@ __device__ inline float disney_pdf(const Material& m, v3 din, v3 dout) {
             } else {
                 mp = 0;
             }
-            if (ndh > 0 && hdout > 0) cp *= (gtr1(ndh, ac) * ndh / (4 * hdout));
+            if (cp != 0 && ndh > 0 && hdout > 0) cp *= (gtr1(ndh, ac) * ndh / (4 * hdout));
             else cp = 0;
         }
-        float eta = din.z > 0 ? m.eta1() : 1 / m.eta1();
-        float Fg = fresnel_dielectric(hdin, eta), D = gtr2(h, ax, ay), Gi = smith2(din, ax, ay);
-        gp *= (Fg * D * Gi / (4 * fabsf(ndin)));
+        if (gp != 0) {  // finite factor (ndin != 0 here): a zero weight stays exactly 0
+            float eta = din.z > 0 ? m.eta1() : 1 / m.eta1();
+            float Fg = fresnel_dielectric(hdin, eta), D = gtr2(h, ax, ay), Gi = smith2(din, ax, ay);
+            gp *= (Fg * D * Gi / (4 * fabsf(ndin)));
+        }
     } else {
         float eta = din.z > 0 ? m.eta1() : 1 / m.eta1();
         v3 h = normalize(din + dout * eta);

@@ -309,11 +309,13 @@ __device__ inline LightSample sample_illumination(const DevScene& S, float env_p
     if (S.has_env) {
         if (S.light_count == 0 || rnd.w < env_prob) {
             v3 d;
-            float pdf;
+            float pdf, su = 0, sv = 0;
             if (S.env_has_image) {
                 float u, v;
                 if (ALIAS) sample_alias(S, rnd.x, rnd.y, u, v, pdf);
                 else sample_texel(S, rnd.x, rnd.y, u, v, pdf);
+                su = u;
+                sv = v;
                 d = uv_to_cart(u, v);
                 pdf /= (2 * ST_PI * ST_PI * sqrtf(1 - sq(d.y)));
             } else {
@@ -325,8 +327,14 @@ __device__ inline LightSample sample_illumination(const DevScene& S, float env_p
             r.dist = __int_as_float(0x7f800000);
             r.cos_light = 1;
             r.dir = d;
-            float tmp;
-            r.radiance = evaluate_environment<ALIAS>(S, env_prob, d, tmp);
+            if (ALIAS && S.env_has_image) {
+                // alias-table mode: the radiance at the sampled uv itself (no
+                // direction -> uv round trip; equal up to float rounding)
+                r.radiance = mk(S.env_emission[0], S.env_emission[1], S.env_emission[2]) * env_bilinear(S, su, sv);
+            } else {
+                float tmp;
+                r.radiance = evaluate_environment<ALIAS>(S, env_prob, d, tmp);
+            }
             return r;
         }
         if (S.light_count > 0) rnd.w = (rnd.w - env_prob) / (1 - env_prob);
