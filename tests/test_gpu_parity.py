@@ -362,3 +362,56 @@ def test_gpu_bvh_rebuild_after_moving_instances(gpu_ctx):
     b = _render_np(Scene(gpu_ctx, sd), view, params)
     for k in ("hdr", "visibility", "path_length", "ray_counts"):
         assert np.array_equal(a[k], b[k]), k
+
+
+# --------------------------------------------------- feature-complete scene
+def _feature_scene(seed=11):
+    """Cornell box whose materials use every Disney lobe (sheen, clearcoat,
+    anisotropy, subsurface, transmission), the sphere carrying vertex normals
+    (shading normals) and 16-bit indices, the block under a rotated,
+    non-uniformly scaled transform."""
+    rng = np.random.default_rng(seed)
+    sd = scenes.cornell_box(seed)
+    for i in range(len(sd.instances)):
+        if i == 5:
+            continue  # the emitter
+        glass = i == 7
+        w = scenes.material_record(base_color=rng.uniform(0.3, 0.95, 3), specular=rng.uniform(0, 1), sheen=rng.uniform(0, 1),
+                                   metallic=0.0 if glass else rng.uniform(0, 1), roughness=rng.uniform(0.05, 0.9),
+                                   anisotropic=rng.uniform(0, 1), subsurface=rng.uniform(0, 1), clearcoat=rng.uniform(0, 1),
+                                   clearcoat_gloss=rng.uniform(0, 1), transmission=0.9 if glass else rng.uniform(0, 0.3),
+                                   eta=1.45)
+        a, b = sd.instances[i]
+        sd.instances[i] = ((a & 0xF) | (sd.add_material(w) << 4), b)
+    # sphere: rebuild with normals and 16-bit indices
+    v, f = scenes.icosphere(3)
+    n = v / np.linalg.norm(v, axis=1, keepdims=True)
+    v = (v * 0.35 + np.array([0.45, 0.4, -0.2])).astype(np.float32)
+    idx = f.reshape(-1).astype(np.uint16)
+    ib = sd.add_buffer(idx)
+    pb = sd.add_buffer(v)
+    nb = sd.add_buffer(n.astype(np.float32))
+    vi = abi.st_mesh_vertex_info()
+    vi.packed_buffer_indices[0] = ib | pb << 16
+    vi.packed_buffer_indices[1] = nb | scenes.NO_BUFFER << 16
+    vi.packed_strides = 2 | 12 << 8 | 12 << 16
+    sd.vertex_infos.append(vi)
+    a, _ = sd.instances[7]
+    sd.instances[7] = (a, (len(sd.vertex_infos) - 1) | (len(f) << 16))
+    # block: rotate about y and scale non-uniformly about its centre
+    c, s_ = np.cos(0.5), np.sin(0.5)
+    R = np.array([[c, 0, s_], [0, 1, 0], [-s_, 0, c]]) @ np.diag([1.2, 0.8, 1.0])
+    ctr = np.array([-0.45, 0.35, -0.3])
+    T = np.zeros((3, 4), np.float32)
+    T[:, :3] = R
+    T[:, 3] = ctr - R @ ctr
+    sd.transforms[6] = T
+    return sd
+
+
+def test_disney_lobes_shading_normals_16bit_indices_and_transforms(gpu_ctx):
+    sd = _feature_scene()
+    W = H = 96
+    params = abi.default_params(W, H, max_depth=5, spp=2)
+    g, r = _render_both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
+    _check_image(g, r, frac_px=0.98, frac_len=0.98)
