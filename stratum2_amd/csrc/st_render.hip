@@ -665,7 +665,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
             };
             if (!hit_any) {
                 if (RESTIR && (c.flags & ST_FLAG_RESTIR_DI_REUSE) && path_len == 1)
-                    w.r_init[2 * pix + 1] = make_float4(0, 0, 0, 0);
+                    w.r_init[2 * ((size_t)k * c.npix + pix) + 1] = make_float4(0, 0, 0, 0);
                 if (path_len == 1 && first_wave && k == 0)  // a miss stores albedo 1 (TraceRay, :1040-1041)
                     store_aux(c, aux, pix, px, py, 0xffffffffu, pack_normal(dir), false, mk(0, 0, 0), mk(1, 1, 1),
                               q16(__int_as_float(0x7f800000)));
@@ -772,9 +772,10 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                     float target = 0, W = 0;
                     if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT))
                         ris_reservoir<LAMBERT, ALIAS>(S, c, rng, m, f, lgn, ldin, sd.pos, cp_eff, seed, target, W);
-                    w.pri[pix] = make_uint4(hit.y, hit.z, hit.w, __float_as_uint(r.w));
-                    w.r_init[2 * pix] = seed;
-                    w.r_init[2 * pix + 1] = make_float4(target, W, target > 0 ? (float)c.cand : 0.0f, __uint_as_float(hit.x));
+                    const size_t rp = (size_t)k * c.npix + pix;  // sample k of the wave
+                    w.pri[rp] = make_uint4(hit.y, hit.z, hit.w, __float_as_uint(r.w));
+                    w.r_init[2 * rp] = seed;
+                    w.r_init[2 * rp + 1] = make_float4(target, W, target > 0 ? (float)c.cand : 0.0f, __uint_as_float(hit.x));
                 } else if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
                     // the screen-space reuse mode keeps reservoirs at primary vertices only
                     const bool ris_here = RESTIR && !(c.flags & ST_FLAG_RESTIR_DI_REUSE);
@@ -835,18 +836,22 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
 // final reservoir of the same pixel (ReservoirReuse, testrenderer.slang:608-663).
 // The camera is static and unjittered, so both live in the same domain.
 template <bool LAMBERT, bool ALIAS>
-__global__ void __launch_bounds__(kShadeBlock) k_restir_temporal(DevScene S, RenderConsts c, WaveState w, int has_prev) {
+// j: the sample of the wave (its reservoirs at j * npix); r_prev holds the
+// final reservoirs of the sample before it.
+__global__ void __launch_bounds__(kShadeBlock) k_restir_temporal(DevScene S, RenderConsts c, WaveState w, int has_prev,
+                                                                 uint32_t j) {
     for (uint32_t pix = blockIdx.x * blockDim.x + threadIdx.x; pix < c.npix; pix += gridDim.x * blockDim.x) {
-        float4 seed = w.r_init[2 * pix], b = w.r_init[2 * pix + 1];
+        const size_t rp = (size_t)j * c.npix + pix;
+        float4 seed = w.r_init[2 * rp], b = w.r_init[2 * rp + 1];
         if (b.z > 0 && has_prev) {
             const float4 qs = w.r_prev[2 * pix], qb = w.r_prev[2 * pix + 1];
             if (qb.z > 0) {
                 Prim P;
-                rebuild_primary(S, w.pri[pix], P);
+                rebuild_primary(S, w.pri[rp], P);
                 Cand x;
                 const float tq = seed_target<LAMBERT, ALIAS>(S, c, P, qs, x);  // q's seed here
                 const uint32_t px = pix % c.W, py = pix / c.W;
-                Rng rng{px, py, c.seed + c.sample_base, 1u << 20};
+                Rng rng{px, py, c.seed + c.sample_base + j, 1u << 20};
                 const float m0 = mis_pair(b.z * b.x, qb.z * b.x), m1 = mis_pair(qb.z * qb.x, b.z * tq);
                 const float w0 = m0 * b.x * b.y, w1 = m1 * tq * qb.y;
                 const float total = w0 + w1;
@@ -859,15 +864,16 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_temporal(DevScene S, Ren
                 b = make_float4(target, Wn, fmin_(M, c.max_m * (float)c.cand), b.w);
             }
         }
-        w.r_temp[2 * pix] = seed;
-        w.r_temp[2 * pix + 1] = b;
+        w.r_temp[2 * rp] = seed;
+        w.r_temp[2 * rp + 1] = b;
     }
 }
 
 // Spatial reuse over c.spatial neighbours in a disc of c.radius pixels, then the
 // shadow ray of the final sample (ConnectToLight with ReSTIR, :730-747).
 template <bool LAMBERT, bool ALIAS>
-__global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, RenderConsts c, WaveState w) {
+__global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, RenderConsts c, WaveState w, uint32_t j) {
+    const size_t jb = (size_t)j * c.npix;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t base0 = blockIdx.x * blockDim.x + threadIdx.x - __lane_id();
     for (uint32_t i0 = base0; i0 < c.npix; i0 += stride) {
@@ -875,15 +881,15 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
         bool emit = false;
         float4 sh0 = make_float4(0, 0, 0, 0), sh1 = sh0, sh2 = sh0;
         if (pix < c.npix) {
-            float4 seed = w.r_temp[2 * pix], b = w.r_temp[2 * pix + 1];
+            float4 seed = w.r_temp[2 * (jb + pix)], b = w.r_temp[2 * (jb + pix) + 1];
             if (b.z > 0) {
                 Prim P;
-                rebuild_primary(S, w.pri[pix], P);
+                rebuild_primary(S, w.pri[jb + pix], P);
                 const v3 nc = unpack_normal(P.sd.sn);
                 Cand cur;
                 seed_target<LAMBERT, ALIAS>(S, c, P, seed, cur);
                 const uint32_t px = pix % c.W, py = pix / c.W;
-                Rng rng{px, py, c.seed + c.sample_base, (1u << 20) + 64};
+                Rng rng{px, py, c.seed + c.sample_base + j, (1u << 20) + 64};
                 for (uint32_t n = 0; n < c.spatial; n++) {
                     const float4 u = rng.nextf();
                     float dx, dy;
@@ -891,12 +897,12 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
                     const int qx = (int)px + (int)floorf(dx * c.radius + 0.5f), qy = (int)py + (int)floorf(dy * c.radius + 0.5f);
                     if (qx < 0 || qy < 0 || qx >= (int)c.W || qy >= (int)c.H || (qx == (int)px && qy == (int)py)) continue;
                     const uint32_t qpix = (uint32_t)qy * c.W + (uint32_t)qx;
-                    const float4 qs = w.r_temp[2 * qpix], qb = w.r_temp[2 * qpix + 1];
+                    const float4 qs = w.r_temp[2 * (jb + qpix)], qb = w.r_temp[2 * (jb + qpix) + 1];
                     if (!(qb.z > 0)) continue;
                     // similar surfaces only: depth within 10%, normals within ~25 degrees
                     if (fabsf(qb.w - b.w) > 0.1f * b.w) continue;
                     Prim Q;
-                    rebuild_primary(S, w.pri[qpix], Q);
+                    rebuild_primary(S, w.pri[jb + qpix], Q);
                     if (dot(unpack_normal(Q.sd.sn), nc) < 0.9f) continue;
                     Cand xn, xq;
                     const float tn = seed_target<LAMBERT, ALIAS>(S, c, P, qs, xn);     // neighbour's seed here
@@ -917,7 +923,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
                     const v3 contrib = cur.contrib * Wp;  // primary throughput is 1
                     const v3 o = ray_offset(P.sd.pos, unpack_normal(P.sd.gn), cur.dir);
                     const float tmax = fmax_(cur.dist * 0.99f, cur.dist - 1e-2f);
-                    const uint32_t p = path_of_pixel(c, px, py);  // one sample per wave
+                    const uint32_t p = j * c.npix + path_of_pixel(c, px, py);
                     sh0 = make_float4(o.x, o.y, o.z, tmax);
                     sh1 = make_float4(cur.dir.x, cur.dir.y, cur.dir.z, __uint_as_float(p));
                     sh2 = make_float4(contrib.x, contrib.y, contrib.z, cur.mis_w);
@@ -1771,7 +1777,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         HIP_CHECK(hipSetDevice(ctx->device));
         const uint32_t npix = p->width * p->height;
         const bool reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
-        uint32_t k = reuse ? 1 : p->samples_per_wave;  // temporal reuse chains consecutive samples
+        uint32_t k = p->samples_per_wave;
         if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, (uint32_t)((4u << 20) / npix)));
         k = std::min(k, p->spp);
         const uint64_t paths = (uint64_t)npix * k;
@@ -1789,9 +1795,9 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         ctx->counters.reserve(256);
         ctx->acc.reserve((size_t)npix * 16);
         if (reuse) {
-            ctx->pri.reserve((size_t)npix * 16);
-            ctx->rinit.reserve((size_t)npix * 32);
-            ctx->rtemp.reserve((size_t)npix * 32);
+            ctx->pri.reserve((size_t)npix * k * 16);
+            ctx->rinit.reserve((size_t)npix * k * 32);
+            ctx->rtemp.reserve((size_t)npix * k * 32);
             ctx->rprev.reserve((size_t)npix * 32);
         }
         ctx->stats.reserve(64);
@@ -1884,8 +1890,12 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                                       : (alias ? k_restir_temporal<false, true> : k_restir_temporal<false, false>);
                     auto ks = lambert ? (alias ? k_restir_spatial<true, true> : k_restir_spatial<true, false>)
                                       : (alias ? k_restir_spatial<false, true> : k_restir_spatial<false, false>);
-                    hipLaunchKernelGGL(kt, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, base > 0 ? 1 : 0);
-                    hipLaunchKernelGGL(ks, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w);
+                    // temporal reuse chains consecutive samples: sample j of the wave
+                    // reuses sample j - 1's final reservoirs
+                    for (uint32_t j = 0; j < c.k; j++) {
+                        hipLaunchKernelGGL(kt, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, (base + j) > 0 ? 1 : 0, j);
+                        hipLaunchKernelGGL(ks, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, j);
+                    }
                 }
                 {
                     Timed tm(ctx, 2);

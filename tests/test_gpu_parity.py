@@ -415,3 +415,16 @@ def test_disney_lobes_shading_normals_16bit_indices_and_transforms(gpu_ctx):
     params = abi.default_params(W, H, max_depth=5, spp=2)
     g, r = _render_both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
     _check_image(g, r, frac_px=0.98, frac_len=0.98)
+
+
+def test_restir_reuse_independent_of_samples_per_wave(gpu_ctx):
+    """Temporal reuse chains consecutive samples; running two samples per wave
+    (reservoirs per sample, reuse passes in sample order) must give the same
+    image, bit for bit, as one sample per wave."""
+    sd = scenes.spheres_field(seed=6, n_spheres=60, n_emitters=40, env=True)
+    W, H = 96, 64
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 14.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
+    imgs = [_gpu_render(gpu_ctx, sd, view, abi.default_params(W, H, max_depth=4, spp=6, flags=flags, samples_per_wave=k))
+            for k in (1, 2, 3)]
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
