@@ -1,0 +1,299 @@
+"""On-disk formats for scenes rendered through the C ABI.
+
+* NanoVDB files (.nvdb): segments of a 16-byte header, per-grid 176-byte
+  metadata + name, then the grid buffers, uncompressed or zlib ("ZIP")
+  compressed -- the layout of extern/nanovdb/util/IO.h (Header, MetaData,
+  Internal::read/write), which src/App/loaders/load_volumes.cpp:100-109 reads.
+* Mitsuba volumes (.vol, version 3, float32, one channel), converted to a
+  NanoVDB float grid as load_volumes.cpp:53-98 does (voxel (i, j, k) takes
+  data[(k * yres + j) * xres + i]; values equal to the background 0 are
+  inactive).
+* Wavefront OBJ meshes -> SceneData (one mesh instance per object/group,
+  polygons fanned into triangles, optional .mtl diffuse colours).
+* Radiance RGBE images (.hdr) for environment maps, read with both scanline
+  encodings (flat and run-length), written flat.
+"""
+import os
+import struct
+import zlib
+
+import numpy as np
+
+from . import nanovdb, scenes
+
+NVDB_MAGIC = 0x304244566F6E614E  # "NanoVDB0"
+NVDB_VERSION = (32 << 21) | (3 << 10) | 3
+CODEC_NONE, CODEC_ZIP = 0, 1
+_META = struct.Struct("<4Q2I6d6i3dI4I3IHHI")
+assert _META.size == 176
+
+
+# ------------------------------------------------------------------ NanoVDB
+def _string_hash(name):
+    """stringHash of IO.h: h = h * 67 + c + (h >> 56), 64-bit."""
+    h = 0
+    for c in name.encode():
+        overflow = h >> 56
+        h = (h * 67 + c + overflow) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def grid_name(grid):
+    raw = bytes(grid[40:40 + 256])
+    return raw.split(b"\0", 1)[0].decode(errors="replace")
+
+
+def _grid_meta(grid, file_size, codec):
+    g = bytes(grid)
+    name = grid_name(grid)
+    tree = 672
+    node_count = struct.unpack_from("<3I", g, tree + 32)
+    tile_count = struct.unpack_from("<3I", g, tree + 44)
+    voxels = struct.unpack_from("<Q", g, tree + 56)[0]
+    root = tree + struct.unpack_from("<Q", g, tree + 24)[0]
+    ibox = struct.unpack_from("<6i", g, root)
+    wbox = struct.unpack_from("<6d", g, 560)
+    vsize = struct.unpack_from("<3d", g, 608)
+    gclass, gtype = struct.unpack_from("<2I", g, 632)
+    meta = _META.pack(len(g), file_size, _string_hash(name), voxels, gtype, gclass, *wbox, *ibox, *vsize,
+                      len(name.encode()) + 1, *node_count, 1, *tile_count, codec, 0, NVDB_VERSION)
+    return meta + name.encode() + b"\0"
+
+
+def write_nvdb(path, grids, codec=CODEC_NONE):
+    """Write one segment holding the given grids (uint8 arrays)."""
+    blobs = []
+    for g in grids:
+        raw = np.ascontiguousarray(g, np.uint8).tobytes()
+        if codec == CODEC_ZIP:
+            c = zlib.compress(raw)
+            blobs.append(struct.pack("<Q", len(c)) + c)
+        elif codec == CODEC_NONE:
+            blobs.append(raw)
+        else:
+            raise ValueError("codec must be CODEC_NONE or CODEC_ZIP")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<QIHH", NVDB_MAGIC, NVDB_VERSION, len(grids), codec))
+        for g, b in zip(grids, blobs):
+            f.write(_grid_meta(np.asarray(g, np.uint8), len(b), codec))
+        for b in blobs:
+            f.write(b)
+
+
+def read_nvdb(path, n=0):
+    """Grid n of a .nvdb file as a uint8 array (NONE and ZIP codecs)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    pos, counter = 0, 0
+    while pos < len(data):
+        magic, version, count, codec = struct.unpack_from("<QIHH", data, pos)
+        if magic != NVDB_MAGIC:
+            raise ValueError("not a NanoVDB file (bad magic)")
+        if version >> 21 != 32:
+            raise ValueError(f"NanoVDB major version {version >> 21} is not supported")
+        pos += 16
+        metas = []
+        for _ in range(count):
+            m = _META.unpack_from(data, pos)
+            pos += _META.size + m[21]  # name size
+            metas.append(m)
+        for m in metas:
+            grid_size, file_size = m[0], m[1]
+            if counter == n:
+                if codec == CODEC_NONE:
+                    raw = data[pos:pos + grid_size]
+                elif codec == CODEC_ZIP:
+                    size = struct.unpack_from("<Q", data, pos)[0]
+                    raw = zlib.decompress(data[pos + 8:pos + 8 + size])
+                else:
+                    raise ValueError("BLOSC-compressed NanoVDB files are not supported")
+                if len(raw) != grid_size:
+                    raise ValueError("truncated NanoVDB grid")
+                return np.frombuffer(raw, np.uint8).copy()
+            pos += file_size
+            counter += 1
+    raise IndexError(f"grid {n} not found in {path}")
+
+
+# ------------------------------------------------------------- Mitsuba .vol
+def read_vol(path):
+    """Mitsuba grid volume, version 3, float32, one channel: returns
+    (dense[i, j, k], pmin, pmax)."""
+    with open(path, "rb") as f:
+        head = f.read(48)
+        if head[:3] != b"VOL" or head[3] != 3:
+            raise ValueError("not a version-3 Mitsuba volume")
+        vtype, xres, yres, zres, channels = struct.unpack_from("<5i", head, 4)
+        if vtype != 1 or channels != 1:
+            raise ValueError("only float32 single-channel volumes are supported")
+        pmin = struct.unpack_from("<3f", head, 24)
+        pmax = struct.unpack_from("<3f", head, 36)
+        data = np.frombuffer(f.read(4 * xres * yres * zres), "<f4")
+    return data.reshape(zres, yres, xres).transpose(2, 1, 0).copy(), pmin, pmax
+
+
+def write_vol(path, dense, pmin=(0, 0, 0), pmax=(1, 1, 1)):
+    d = np.asarray(dense, np.float32)
+    xres, yres, zres = d.shape
+    with open(path, "wb") as f:
+        f.write(b"VOL" + bytes([3]) + struct.pack("<5i", 1, xres, yres, zres, 1) + struct.pack("<6f", *pmin, *pmax))
+        f.write(d.transpose(2, 1, 0).astype("<f4").tobytes())
+
+
+def vol_to_grid(path, name=None):
+    """load_volumes.cpp:53-98: a fog-volume float grid with voxel size 1 and
+    background 0 over the file's index box."""
+    dense, _, _ = read_vol(path)
+    return nanovdb.write_float_grid(dense, voxel_size=1.0, background=0.0,
+                                    name=name or os.path.splitext(os.path.basename(path))[0])
+
+
+# ---------------------------------------------------------------------- OBJ
+def _read_mtl(path):
+    mats, cur = {}, None
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if not t:
+                continue
+            if t[0] == "newmtl":
+                cur = t[1]
+                mats[cur] = {}
+            elif cur and t[0] in ("Kd", "Ke"):
+                mats[cur][t[0]] = tuple(float(x) for x in t[1:4])
+    return mats
+
+
+def read_obj(path, sd=None, transform=None):
+    """Adds the OBJ's meshes to a SceneData (new one by default): one mesh
+    instance per object/group (and material), polygons fanned to triangles,
+    Kd as base colour and Ke as emission from the .mtl files it names."""
+    sd = sd or scenes.SceneData()
+    base = os.path.dirname(os.path.abspath(path))
+    verts, mtl = [], {}
+    groups = {}  # (group, material) -> list of triangles (vertex indices)
+    g, m = "default", None
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if not t or t[0].startswith("#"):
+                continue
+            if t[0] == "v":
+                verts.append([float(x) for x in t[1:4]])
+            elif t[0] in ("o", "g"):
+                g = " ".join(t[1:]) or "default"
+            elif t[0] == "usemtl":
+                m = t[1] if len(t) > 1 else None
+            elif t[0] == "mtllib":
+                for name in t[1:]:
+                    p = os.path.join(base, name)
+                    if os.path.exists(p):
+                        mtl.update(_read_mtl(p))
+            elif t[0] == "f":
+                idx = []
+                for tok in t[1:]:
+                    i = int(tok.split("/")[0])
+                    idx.append(i - 1 if i > 0 else len(verts) + i)
+                for k in range(1, len(idx) - 1):
+                    groups.setdefault((g, m), []).append((idx[0], idx[k], idx[k + 1]))
+    V = np.asarray(verts, np.float32)
+    for (gname, mname), tris in groups.items():
+        tri = np.asarray(tris, np.int64)
+        used, remap = np.unique(tri, return_inverse=True)
+        for start in range(0, len(tri), 0xFFFF):  # 16-bit primitive ids per instance
+            part = remap.reshape(-1, 3)[start:start + 0xFFFF]
+            vi, nt = sd.add_mesh(V[used], part.reshape(-1).astype(np.uint32))
+            props = mtl.get(mname, {})
+            kd = props.get("Kd", (0.8, 0.8, 0.8))
+            ke = props.get("Ke", (0.0, 0.0, 0.0))
+            mat = sd.add_material(scenes.material_record(base_color=[min(max(x, 0.0), 1.0) for x in kd], specular=0.0,
+                                                         emission=ke))
+            sd.add_mesh_instance(vi, nt, mat, transform)
+    return sd
+
+
+def write_obj(path, positions, indices):
+    with open(path, "w") as f:
+        for v in np.asarray(positions, np.float32):
+            f.write(f"v {float(v[0])!r} {float(v[1])!r} {float(v[2])!r}\n")
+        for t in np.asarray(indices).reshape(-1, 3):
+            f.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
+
+
+# ---------------------------------------------------------------- RGBE .hdr
+def _rgbe_to_float(rgbe):
+    """RGBE -> float: mantissa * 2^(e - 136), black for e = 0."""
+    e = rgbe[..., 3].astype(np.int32)
+    scale = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136), np.float32(0.0)).astype(np.float32)
+    out = np.empty(rgbe.shape[:-1] + (4,), np.float32)
+    for c in range(3):
+        out[..., c] = rgbe[..., c].astype(np.float32) * scale
+    out[..., 3] = 1
+    return out
+
+
+def read_hdr(path):
+    """Radiance RGBE image as float32 (H, W, 4) (alpha 1)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    fmt_ok = False
+    while True:
+        end = data.index(b"\n", pos)
+        line = data[pos:end].strip()
+        pos = end + 1
+        if line.startswith(b"FORMAT=") and b"32-bit_rle_rgbe" in line:
+            fmt_ok = True
+        if line == b"":
+            break
+    if not fmt_ok:
+        raise ValueError("not an RGBE Radiance image")
+    end = data.index(b"\n", pos)
+    res = data[pos:end].split()
+    pos = end + 1
+    if res[0] != b"-Y" or res[2] != b"+X":
+        raise ValueError("only -Y H +X W Radiance images are supported")
+    H, W = int(res[1]), int(res[3])
+    img = np.zeros((H, W, 4), np.uint8)
+    buf = np.frombuffer(data, np.uint8)
+    for y in range(H):
+        if 8 <= W < 32768 and buf[pos] == 2 and buf[pos + 1] == 2 and (int(buf[pos + 2]) << 8 | int(buf[pos + 3])) == W:
+            pos += 4
+            for c in range(4):
+                x = 0
+                while x < W:
+                    n = int(buf[pos])
+                    pos += 1
+                    if n > 128:
+                        n -= 128
+                        img[y, x:x + n, c] = buf[pos]
+                        pos += 1
+                    else:
+                        img[y, x:x + n, c] = buf[pos:pos + n]
+                        pos += n
+                    x += n
+        else:
+            img[y] = buf[pos:pos + 4 * W].reshape(W, 4)
+            pos += 4 * W
+    return _rgbe_to_float(img)
+
+
+def float_to_rgbe(rgb):
+    rgb = np.asarray(rgb, np.float64)[..., :3]
+    m = rgb.max(-1)
+    out = np.zeros(rgb.shape[:-1] + (4,), np.uint8)
+    nz = m > 1e-32
+    mant, ex = np.frexp(m[nz])
+    scale = mant * 256.0 / m[nz]
+    for c in range(3):
+        out[..., c][nz] = np.floor(rgb[..., c][nz] * scale).astype(np.uint8)
+    out[..., 3][nz] = (ex + 128).astype(np.uint8)
+    return out
+
+
+def write_hdr(path, img):
+    img = np.asarray(img, np.float32)
+    H, W = img.shape[:2]
+    with open(path, "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {H} +X {W}\n".encode())
+        f.write(float_to_rgbe(img).tobytes())
