@@ -46,6 +46,7 @@ def main():
     ap.add_argument("--no-restir", action="store_true", help="plain NEE instead of ReSTIR DI")
     ap.add_argument("--workload", choices=["spheres", "cloud"], default="spheres")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--device-bvh", action="store_true", help="rebuild the BVH on the GPU (st_scene_update) before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -84,6 +85,11 @@ def main():
     t0 = time.time()
     sc = Scene(ctx, sd)
     log(f"scene uploaded + BVH built in {time.time() - t0:.1f}s: {sc.stats()}")
+    bvh_ms = None
+    if args.device_bvh:
+        import numpy as _np
+        bvh_ms = sc.update(_np.stack(sd.transforms))
+        log(f"device BVH rebuild: {bvh_ms:.2f} ms: {sc.stats()}")
     r = TestRenderer(sc)
     flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
     if not args.no_restir:  # ReSTIR DI: 32 candidates, temporal + spatial (5 neighbours, 30 px) reuse
@@ -175,7 +181,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": {"workload": workload, "restir_di": not args.no_restir,
                    "width": args.width, "height": args.height, "spp": args.spp, "max_path_length": max_depth,
-                   "triangles": sd.triangle_count, "samples_per_px_per_s": round(args.width * args.height * args.spp / (ms / 1e3), 1)},
+                   "triangles": sd.triangle_count, "bvh": "device linear BVH" if args.device_bvh else "host binned SAH",
+                   "bvh_build_ms": round(bvh_ms, 2) if bvh_ms is not None else None, "samples_per_px_per_s": round(args.width * args.height * args.spp / (ms / 1e3), 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
