@@ -136,6 +136,14 @@ typedef struct st_view {
 #define ST_FLAG_PIXEL_JITTER            (1u << 7)  /* gPixelJitter */
 #define ST_FLAG_COUNT_TRAVERSAL         (1u << 8)  /* count nodes/triangles visited (gCountRays-like) */
 #define ST_FLAG_DIRECT_ONLY             (1u << 9)  /* gSampleDirectIlluminationOnly: NEE, then stop */
+/* TestRenderer light tracing ("Light tracing", mLightTrace, gTraceFromLight;
+ * TestRenderer.cpp:106-125, 534-582): paths start on the lights and every
+ * non-singular vertex connects to the camera; hdr is the atomic 1/16384
+ * fixed-point image alone (ProcessAtomicOutput with gClearImage).  As the
+ * reference forces, NEE, ReSTIR DI and the direct-only mode are off in this
+ * mode (their flags are ignored), and visibility, albedo, depth, prev_uv and
+ * path_length are not written. */
+#define ST_FLAG_TRACE_FROM_LIGHT        (1u << 10)
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface
@@ -156,6 +164,8 @@ typedef struct st_render_params {
     uint32_t spatial_neighbours;       /* ReSTIR spatial reuse neighbours */
     float spatial_radius;              /* ReSTIR spatial reuse radius in pixels */
     uint32_t samples_per_wave;         /* 0 = automatic: samples traced concurrently */
+    float light_subpath_factor;        /* mLightSubpathCount (TestRenderer.hpp:29, default 1): light paths
+                                          per sample = max(1, uint(W*H * factor)) (TestRenderer.cpp:452) */
 } st_render_params;
 
 /* Device outputs.  hdr is required; the others may be NULL. */

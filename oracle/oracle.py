@@ -48,6 +48,7 @@ def lib():
         l.or_env_sample_texel_pdf.argtypes = [vp, vp]
         l.or_bsdf_eval.argtypes = [vp, C.c_int, vp, vp, vp]
         l.or_bsdf_sample.argtypes = [vp, C.c_int, vp, vp, vp]
+        l.or_bsdf_eval_adjoint.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
         l.or_nvdb_read.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         l.or_nvdb_read.restype = C.c_float
         l.or_nvdb_world_to_index.argtypes = [vp, vp, C.c_int, vp]

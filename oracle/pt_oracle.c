@@ -66,6 +66,8 @@ static inline float maxf(float a, float b) { return a > b ? a : b; }
 static inline float saturate(float a) { return a < 0 ? 0 : (a > 1 ? 1 : a); }
 static inline float max3f(f3 a) { return maxf(maxf(a.x, a.y), a.z); }
 static inline float pow2(float a) { return a * a; }
+/* max(0, x) of the reference's GPU code: NaN becomes 0 */
+static inline float clamp0(float x) { return x > 0 ? x : 0.0f; }
 /* luminance, src/Shaders/compat/common.h:51 */
 static inline float luminance(f3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
 
@@ -412,12 +414,19 @@ static f3 to_world(const shading_t* sd, f3 v) {
     return add(add(mulf(t, v.x), mulf(b, v.y)), mulf(n, v.z));
 }
 static float signf(float x) { return x > 0 ? 1.0f : (x < 0 ? -1.0f : 0.0f); }
-static float shading_normal_correction(const shading_t* sd, f3 din, f3 dout) {
+/* ShadingData::shadingNormalCorrection, shading_data.hlsli:35-54; adj: light paths (Adjoint) */
+static float shading_normal_correction_x(const shading_t* sd, f3 din, f3 dout, int adj) {
     f3 lg = to_local(sd, unpack_normal(sd->gn));
     float ngin = dot(lg, din), ngout = dot(lg, dout);
     if (signf(ngout) != signf(dout.z) || signf(ngin) != signf(din.z)) return 0;
-    return 1; /* Adjoint == false on camera paths */
+    float G = 1;
+    if (adj) {
+        const float num = ngout * din.z, denom = dout.z * ngin;
+        if (fabsf(denom) > 1e-5f) G *= fabsf(num / denom);
+    }
+    return G;
 }
+static float shading_normal_correction(const shading_t* sd, f3 din, f3 dout) { return shading_normal_correction_x(sd, din, dout, 0); }
 
 /* --------------------------------------------------------- materials */
 typedef struct { uint32_t p[4]; float escale; } material_t;
@@ -495,8 +504,9 @@ static f3 sample_vndf(f3 din, float ax, float ay, float r0, float r1) {
 static f3 lerp3(f3 a, f3 b, float t) { return add(a, mulf(sub(b, a), t)); }
 static float lerp1(float a, float b, float t) { return a + (b - a) * t; }
 
-/* Disney eval, src/Shaders/materials/disney.hlsli:9-139 (Adjoint = false) */
-static f3 disney_eval(const material_t* m, f3 din, f3 dout) {
+/* Disney eval, src/Shaders/materials/disney.hlsli:9-139; adj: Adjoint (light paths),
+ * refraction scaled by eta_factor = 1 / eta^2 (:123) */
+static f3 disney_eval(const material_t* m, f3 din, f3 dout, int adj) {
     float aspect = sqrtf(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
     float ax = maxf(1e-4f, alpha / aspect), ay = maxf(1e-4f, alpha * aspect), metallic = M_METAL(m);
     if (din.z * dout.z > 0) {
@@ -554,7 +564,8 @@ static f3 disney_eval(const material_t* m, f3 din, f3 dout) {
     float G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
     f3 bc = m_base(m);
     f3 sq = v3(sqrtf(bc.x), sqrtf(bc.y), sqrtf(bc.z));
-    return mulf(sq, gw * ((1 - Fg) * D * G * eta * eta * fabsf(hdout * hdin)) / (fabsf(din.z) * sd * sd));
+    const float eta_factor = adj ? (1 / (eta * eta)) : 1;
+    return mulf(sq, gw * (eta_factor * (1 - Fg) * D * G * eta * eta * fabsf(hdout * hdin)) / (fabsf(din.z) * sd * sd));
 }
 
 /* Disney evalPdf, disney.hlsli:141-231 */
@@ -672,7 +683,7 @@ static dsr_t disney_sample(const material_t* m, f3 din, f3 rnd) {
 }
 
 /* evaluateReflectance for Disney (disney.hlsli:391-403) or Lambert (lambertian.hlsli:13-24) */
-static rer_t bsdf_eval(const material_t* m, int lambert, f3 din, f3 dout) {
+static rer_t bsdf_eval_x(const material_t* m, int lambert, f3 din, f3 dout, int adj) {
     rer_t r;
     if (lambert) {
         if (din.z * dout.z < 0) { r.f = v3(0, 0, 0); r.fwd = r.rev = 0; return r; }
@@ -681,13 +692,14 @@ static rer_t bsdf_eval(const material_t* m, int lambert, f3 din, f3 dout) {
         r.rev = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(din.z)));
         return r;
     }
-    r.f = disney_eval(m, din, dout);
+    r.f = disney_eval(m, din, dout, adj);
     r.fwd = disney_pdf(m, din, dout);
     r.rev = disney_pdf(m, dout, din);
     return r;
 }
 /* sampleDirection (disney.hlsli:415-429, lambertian.hlsli:29-39) */
-static dsr_t bsdf_sample(const material_t* m, int lambert, f3 rnd, f3 din) {
+static rer_t bsdf_eval(const material_t* m, int lambert, f3 din, f3 dout) { return bsdf_eval_x(m, lambert, din, dout, 0); }
+static dsr_t bsdf_sample_x(const material_t* m, int lambert, f3 rnd, f3 din, int adj) {
     dsr_t r;
     if (lambert) {
         r.dir = sample_cos_hemisphere(rnd.x, rnd.y);
@@ -699,10 +711,11 @@ static dsr_t bsdf_sample(const material_t* m, int lambert, f3 rnd, f3 din) {
         return r;
     }
     r = disney_sample(m, din, rnd);
-    rer_t f = bsdf_eval(m, 0, din, r.dir);
+    rer_t f = bsdf_eval_x(m, 0, din, r.dir, adj);
     r.f = f.f; r.fwd = f.fwd; r.rev = f.rev;
     return r;
 }
+static dsr_t bsdf_sample(const material_t* m, int lambert, f3 rnd, f3 din) { return bsdf_sample_x(m, lambert, rnd, din, 0); }
 static int m_singular(const material_t* m, int lambert) {
     if (lambert) return 0;
     return maxf(maxf(M_SPEC(m), M_METAL(m)), M_TRANS(m)) * (1 - M_ROUGH(m)) > 0.75f;
@@ -715,6 +728,14 @@ void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], con
     memcpy(m.p, mat, 16);
     m.escale = asfloat(mat[4]);
     rer_t r = bsdf_eval(&m, lambertian, v3(din[0], din[1], din[2]), v3(dout[0], dout[1], dout[2]));
+    o[0] = r.f.x; o[1] = r.f.y; o[2] = r.f.z; o[3] = r.fwd; o[4] = r.rev;
+}
+void or_bsdf_eval_adjoint(const uint32_t mat[5], int lambertian, int adjoint, const float din[3], const float dout[3],
+                          float o[5]) {
+    material_t m;
+    memcpy(m.p, mat, 16);
+    m.escale = asfloat(mat[4]);
+    rer_t r = bsdf_eval_x(&m, lambertian, v3(din[0], din[1], din[2]), v3(dout[0], dout[1], dout[2]), adjoint);
     o[0] = r.f.x; o[1] = r.f.y; o[2] = r.f.z; o[3] = r.fwd; o[4] = r.rev;
 }
 void or_bsdf_sample(const uint32_t mat[5], int lambertian, const float rnd[3], const float din[3], float o[10]) {
@@ -1532,12 +1553,12 @@ typedef struct {
 } pctx_t;
 
 /* ViewData::toRaster (compat/scene.h:158-165) with ProjectionData::projectPoint
- * (compat/transform.h:139-155); leaves *ox, *oy unchanged when p is not visible. */
-static void to_raster(const st_view* V, f3 p, float* ox, float* oy) {
+ * (compat/transform.h:139-155); 0, with *ox, *oy unchanged, when p is not visible. */
+static int to_raster(const st_view* V, f3 p, float* ox, float* oy) {
     const st_projection* pr = &V->projection;
     const float sz = p.z > 0 ? 1.0f : (p.z < 0 ? -1.0f : 0.0f);
     const float sn = pr->near_plane > 0 ? 1.0f : (pr->near_plane < 0 ? -1.0f : 0.0f);
-    if (sz != sn) return;
+    if (sz != sn) return 0;
     float r0, r1, r2, r3;
     if (pr->vertical_fov < 0) {
         r0 = p.x * pr->scale[0] + pr->offset[0];
@@ -1551,10 +1572,11 @@ static void to_raster(const st_view* V, f3 p, float* ox, float* oy) {
         r3 = p.z * sn;
     }
     const float aw = fabsf(r3);
-    if (fabsf(r0) >= aw || fabsf(r1) >= aw || fabsf(r2) >= aw) return;
+    if (fabsf(r0) >= aw || fabsf(r1) >= aw || fabsf(r2) >= aw) return 0;
     const float ew = (float)(V->image_max[0] - V->image_min[0]), eh = (float)(V->image_max[1] - V->image_min[1]);
     *ox = (float)V->image_min[0] + ew * ((r0 / r3) * 0.5f + 0.5f);
     *oy = (float)V->image_min[1] + eh * ((-r1 / r3) * 0.5f + 0.5f);
+    return 1;
 }
 
 /* StoreAuxillaryData (testrenderer.slang:195-218) for a static scene in steady
@@ -2001,7 +2023,7 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
         }
         if (sr->w <= 0 || all_le0(sr->c)) continue;
         f3 rw = mulf(sr->c, sr->w);
-        float q[3] = {maxf(0, rw.x) * (float)OR_Q, maxf(0, rw.y) * (float)OR_Q, maxf(0, rw.z) * (float)OR_Q};
+        float q[3] = {clamp0(rw.x) * (float)OR_Q, clamp0(rw.y) * (float)OR_Q, clamp0(rw.z) * (float)OR_Q};
         for (int k = 0; k < 3; k++) {
             if (x->c->flags & ST_FLAG_FLOAT_NEE) continue;
             int32_t iv = q[k] >= 2147483647.0f ? INT_MAX : (int32_t)q[k];
@@ -2016,10 +2038,195 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
     *plen = ps->path_len;
 }
 
+/* ------------------------------------------------------- light tracing */
+/* TestRenderer with mLightTrace (gTraceFromLight, TestRenderer.cpp:106-125,
+ * 534-582): the light dispatch of Render / RenderIteration
+ * (testrenderer.slang:1087-1144) over mLightSubpathCount paths, then
+ * ProcessShadowRays and ProcessAtomicOutput with gClearImage (:1147-1183). */
+typedef struct {
+    const or_scene* s;
+    const cfg_t* c;
+    const st_view* view;
+    st_transform inv_cam;
+    uint32_t W, H, L;
+    float ipd;      /* ViewData::imagePlaneDist */
+    uint32_t* acc;  /* this thread's mOutputAtomic, 3 per pixel */
+    uint64_t rays[2], trav[4];
+} lctx_t;
+
+/* ConnectToCamera, testrenderer.slang:441-524 (no VC: weight 1).  The shadow
+ * ray runs from the offset vertex to the camera position, unshortened; its
+ * contribution goes to pixel int(y) * W + int(x) (dropped beyond the image). */
+static void connect_to_camera(lctx_t* x, f3 thr, const material_t* m, const shading_t* sd, f3 ldin) {
+    const f3 pc = xf_point(&x->inv_cam, sd->pos);
+    float pcx = 0, pcy = 0;
+    if (!to_raster(x->view, pc, &pcx, &pcy)) return;
+    const float cos_cam = fabsf(normalize3(pc).z);
+    const st_transform* T = &x->view->camera_to_world;
+    f3 tc = sub(v3(T->m[0][3], T->m[1][3], T->m[2][3]), sd->pos);
+    const float dist = length3(tc);
+    tc = divf(tc, dist);
+    const f3 ldout = to_local(sd, tc);
+    const float cos_out = fabsf(ldout.z);
+    rer_t f = bsdf_eval_x(m, x->c->lambert, ldin, ldout, 1);
+    f.f = mulf(f.f, shading_normal_correction_x(sd, ldin, ldout, 1));
+    const float image_point_to_camera = x->ipd / cos_cam;
+    const float image_to_solid_angle = pow2(image_point_to_camera) / cos_cam;
+    const float image_to_surface = image_to_solid_angle * cos_out / pow2(dist);
+    const float surface_to_image = 1.0f / image_to_surface;
+    const f3 contrib = divf(mul(thr, f.f), (float)x->L * surface_to_image * cos_out);
+    const uint32_t pix = (uint32_t)(int32_t)pcy * x->W + (uint32_t)(int32_t)pcx;
+    if (pix >= x->W * x->H) return;
+    const f3 o = ray_offset_dir(sd->pos, unpack_normal(sd->gn), tc);
+    hit_t h = trace(x->s, o, tc, dist, 1, &x->trav[2], &x->trav[3]);
+    x->rays[1]++;
+    if (h.hit || all_le0(contrib)) return;
+    /* AddRadianceInterlocked, testrenderer.slang:237-261 */
+    const float q[3] = {clamp0(contrib.x) * (float)OR_Q, clamp0(contrib.y) * (float)OR_Q, clamp0(contrib.z) * (float)OR_Q};
+    for (int k = 0; k < 3; k++) x->acc[3 * (size_t)pix + k] += (uint32_t)(q[k] >= 2147483647.0f ? INT_MAX : (int32_t)q[k]);
+}
+
+/* One light subpath: InitializeLightPath (testrenderer.slang:983-1024), then
+ * TraceRay<true> + ProcessInteraction<true> (:810-935, 1026-1074) per segment.
+ * A sample of zero pdf (no light, or a one-sided light facing away from the
+ * origin) or with a NaN throughput ends at once: the reference carries it on
+ * with a NaN throughput whose contributions clamp to zero. */
+static void trace_light_path(lctx_t* x, uint32_t q, uint32_t seed) {
+    const or_scene* s = x->s;
+    const cfg_t* c = x->c;
+    rng_t rng;
+    rng_init(&rng, seed, q % x->W, q / x->W, 0xFFFFu);
+    float rnd[4], dr[4];
+    rng_nextf(&rng, rnd);
+    const isr_t r = sample_illumination(s, c, rnd, v3(0, 0, 0));
+    rng_nextf(&rng, dr);
+    f3 thr = divf(r.radiance, r.pdf), o, d;
+    if (r.finite) {
+        const f3 ld = sample_cos_hemisphere(dr[0], dr[1]);
+        f3 onb[3];
+        make_orthonormal(unpack_normal(r.normal), onb);
+        o = ray_offset(r.pos, onb[2]);
+        d = normalize3(add(add(mulf(onb[0], ld.x), mulf(onb[1], ld.y)), mulf(onb[2], ld.z)));
+        const float dirpdf = q16(cos_hemisphere_pdfw(ld.z)); /* mDirPdfW is stored in fp16 */
+        thr = mulf(thr, ld.z / dirpdf);
+    } else {
+        float xy[2];
+        sample_concentric_disc(dr[0], dr[1], xy);
+        f3 onb[3];
+        make_orthonormal(neg(r.dir), onb);
+        const f3 ctr = v3(s->sphere[0], s->sphere[1], s->sphere[2]);
+        o = add(ctr, mulf(add(add(mulf(onb[0], xy[0]), mulf(onb[1], xy[1])), r.dir), s->sphere[3]));
+        d = neg(r.dir);
+    }
+    if (!(r.pdf > 0) || thr.x != thr.x || thr.y != thr.y || thr.z != thr.z || !any_gt0(thr)) return;
+    uint32_t dir = pack_normal(d), path_len = 0;
+    while (path_len + 1 <= c->max_depth) {
+        const f3 wd = unpack_normal(dir);
+        hit_t h = trace(s, o, wd, INFINITY, 0, &x->trav[0], &x->trav[1]);
+        x->rays[0]++;
+        path_len++;
+        if (!h.hit) return; /* no emission is gathered on light paths */
+        shading_t sd;
+        make_tri_shading(s, h.id & 0xffffu, h.id >> 16, h.b1, h.b2, &sd);
+        const material_t m = load_material(s, sd.flags_material >> 4);
+        const f3 ldin = to_local(&sd, neg(wd));
+        if (path_len >= c->max_depth || !m_can_eval(&m)) return;
+        if (!m_singular(&m, c->lambert) && path_len + 1 >= c->min_len) connect_to_camera(x, thr, &m, &sd, ldin);
+        const float cp = minf(1, m_cont_prob(&m));
+        float rr[4], rb[4];
+        rng_nextf(&rng, rr);
+        if (rr[0] > cp) return;
+        rng_nextf(&rng, rb);
+        dsr_t smp = bsdf_sample_x(&m, c->lambert, v3(rb[0], rb[1], rb[2]), ldin, 1);
+        smp.fwd *= cp; smp.rev *= cp;
+        thr = mul(thr, divf(smp.f, smp.fwd));
+        thr = mulf(thr, shading_normal_correction_x(&sd, ldin, smp.dir, 1));
+        const f3 w = to_world(&sd, smp.dir);
+        dir = pack_normal(w);
+        o = ray_offset_dir(sd.pos, unpack_normal(sd.gn), w);
+        if (thr.x != thr.x || thr.y != thr.y || thr.z != thr.z) return;
+        if (all_le0(thr)) return; /* RenderIteration skips zero-throughput paths */
+    }
+}
+
+static int render_light(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
+    if (s->nvolinst > 0) return fail("light tracing with media is not supported");
+    if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64)) return fail("light_subpath_factor out of range");
+    cfg_t c;
+    memset(&c, 0, sizeof c);
+    c.flags = p->flags & ~(uint32_t)(ST_FLAG_SAMPLE_DIRECT | ST_FLAG_DIRECT_ONLY | ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE |
+                                      ST_FLAG_FLOAT_NEE);
+    c.env_prob = p->environment_sample_probability;
+    c.max_depth = p->max_depth;
+    c.min_len = p->min_path_length;
+    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    const uint32_t W = p->width, H = p->height, npix = W * H;
+    /* mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452 */
+    uint32_t L = (uint32_t)((float)npix * p->light_subpath_factor);
+    if (L < 1) L = 1;
+    /* ViewData::imagePlaneDist, compat/scene.h:147 */
+    const float ipd = (float)abs(view->image_max[1] - view->image_min[1]) / (2 * tanf(view->projection.vertical_fov / 2));
+    const st_transform inv_cam = inverse_xf(&view->camera_to_world);
+    uint32_t* total = calloc((size_t)npix * 3, 4);
+    f3* sum = calloc(npix, sizeof(f3));
+    if (!total || !sum) { free(total); free(sum); return fail("out of memory"); }
+    uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
+    int oom = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    for (uint32_t k = 0; k < p->spp; k++) {
+        memset(total, 0, (size_t)npix * 3 * 4);
+#pragma omp parallel
+        {
+            lctx_t x;
+            memset(&x, 0, sizeof x);
+            x.s = s; x.c = &c; x.view = view; x.inv_cam = inv_cam;
+            x.W = W; x.H = H; x.L = L; x.ipd = ipd;
+            x.acc = calloc((size_t)npix * 3, 4);
+#pragma omp for schedule(dynamic, 256)
+            for (int64_t q = 0; q < (int64_t)L; q++)
+                if (x.acc) trace_light_path(&x, (uint32_t)q, p->seed + k);
+#pragma omp critical
+            {
+                /* integer sums: the same in any order, as the reference's atomics */
+                if (x.acc) for (size_t i = 0; i < (size_t)npix * 3; i++) total[i] += x.acc[i];
+                else oom = 1;
+                rays[0] += x.rays[0]; rays[1] += x.rays[1];
+                for (int j = 0; j < 4; j++) trav[j] += x.trav[j];
+            }
+            free(x.acc);
+        }
+        if (oom) { free(total); free(sum); return fail("out of memory"); }
+        /* ProcessAtomicOutput with gClearImage, then the mean over samples */
+        for (uint32_t i = 0; i < npix; i++) {
+            const f3 r = v3(0.0f + (float)total[3 * i] / (float)OR_Q, 0.0f + (float)total[3 * i + 1] / (float)OR_Q,
+                            0.0f + (float)total[3 * i + 2] / (float)OR_Q);
+            sum[i] = add(sum[i], r);
+        }
+    }
+    if (out->hdr)
+        for (uint32_t i = 0; i < npix; i++) {
+            out->hdr[4 * i + 0] = sum[i].x / (float)p->spp;
+            out->hdr[4 * i + 1] = sum[i].y / (float)p->spp;
+            out->hdr[4 * i + 2] = sum[i].z / (float)p->spp;
+            out->hdr[4 * i + 3] = 1;
+        }
+    if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
+    if (out->traversal) for (int j = 0; j < 4; j++) out->traversal[j] = trav[j];
+    free(total);
+    free(sum);
+    return 0;
+}
+
 int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
     if (!s || !view || !p || !out) return fail("null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
-    if (p->flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_ENV_ALIAS)) return fail("mode not supported by this oracle");
+    if (p->flags & ST_FLAG_ENV_ALIAS) return fail("mode not supported by this oracle");
+    if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, out, threads);
+    if (p->flags & ST_FLAG_RESTIR_DI_REUSE) return fail("mode not supported by this oracle");
     if (s->nvolinst > 0 && (p->flags & ST_FLAG_RESTIR_DI)) return fail("ReSTIR DI with media is not supported");
     cfg_t c;
     c.flags = p->flags;

@@ -67,6 +67,9 @@ float or_env_sample_texel_pdf(or_scene* s, const float uv[2]);
  * out_eval: {f.rgb, fwdPdf, revPdf}; out_sample: {dir.xyz, f.rgb, fwdPdf, revPdf, eta, roughness}. */
 void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], const float dout[3],
                   float out_eval[5]);
+/* evaluateReflectance<Adjoint> (disney.hlsli:391-403): adjoint = 1 for light paths */
+void or_bsdf_eval_adjoint(const uint32_t mat[5], int lambertian, int adjoint, const float din[3], const float dout[3],
+                          float o[5]);
 void or_bsdf_sample(const uint32_t mat[5], int lambertian, const float rnd[3], const float din[3],
                     float out_sample[10]);
 

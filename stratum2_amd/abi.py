@@ -24,6 +24,7 @@ ST_FLAG_FLOAT_NEE = 1 << 6
 ST_FLAG_PIXEL_JITTER = 1 << 7
 ST_FLAG_COUNT_TRAVERSAL = 1 << 8
 ST_FLAG_DIRECT_ONLY = 1 << 9
+ST_FLAG_TRACE_FROM_LIGHT = 1 << 10  # TestRenderer light tracing (gTraceFromLight)
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 # TonemapMode (src/Shaders/compat/tonemap.h:10-23)
@@ -87,6 +88,7 @@ class st_render_params(C.Structure):
         ("environment_sample_probability", f32), ("candidate_samples", u32), ("max_m", f32),
         ("flags", u32), ("rr_start_depth", u32), ("exposure", f32),
         ("spatial_neighbours", u32), ("spatial_radius", f32), ("samples_per_wave", u32),
+        ("light_subpath_factor", f32),
     ]
 
 
@@ -113,6 +115,7 @@ def default_params(width, height, **kw):
     p.spatial_neighbours = 5
     p.spatial_radius = 30.0
     p.samples_per_wave = 0
+    p.light_subpath_factor = 1.0  # mLightSubpathCount, TestRenderer.hpp:29
     for k, v in kw.items():
         if not hasattr(p, k):
             raise AttributeError(k)

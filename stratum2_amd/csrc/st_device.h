@@ -260,7 +260,9 @@ struct Sample {
     float fwd, rev, eta, rough;
 };
 
-// Disney eval, materials/disney.hlsli:9-139 (camera paths: Adjoint = false)
+// Disney eval, materials/disney.hlsli:9-139.  ADJ: light paths (Adjoint), whose
+// refracted radiance is not scaled by eta^2 (eta_factor, :123).
+template <bool ADJ = false>
 __device__ inline v3 disney_eval(const Material& m, v3 din, v3 dout) {
     float aspect = sqrtf(1 - m.aniso() * 0.9f), alpha = sq(m.rough());
     float ax = fmax_(1e-4f, alpha / aspect), ay = fmax_(1e-4f, alpha * aspect), metallic = m.metal();
@@ -319,6 +321,7 @@ __device__ inline v3 disney_eval(const Material& m, v3 din, v3 dout) {
     float G = smith2(din, ax, ay) * smith2(dout, ax, ay);
     v3 bc = m.base();
     v3 sqb = mk(sqrtf(bc.x), sqrtf(bc.y), sqrtf(bc.z));
+    if (ADJ) return sqb * (gw * ((1 / (eta * eta)) * (1 - Fg) * D * G * eta * eta * fabsf(hdout * hdin)) / (fabsf(din.z) * sd * sd));
     return sqb * (gw * ((1 - Fg) * D * G * eta * eta * fabsf(hdout * hdin)) / (fabsf(din.z) * sd * sd));
 }
 
@@ -437,7 +440,7 @@ __device__ inline Sample disney_sample(const Material& m, v3 din, float rx, floa
 }
 
 // evaluateReflectance: Lambert (lambertian.hlsli:13-24) or Disney (disney.hlsli:391-403)
-template <bool LAMBERT>
+template <bool LAMBERT, bool ADJ = false>
 __device__ inline Eval bsdf_eval(const Material& m, v3 din, v3 dout) {
     Eval r;
     if (LAMBERT) {
@@ -451,13 +454,13 @@ __device__ inline Eval bsdf_eval(const Material& m, v3 din, v3 dout) {
         r.rev = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(din.z)));
         return r;
     }
-    r.f = disney_eval(m, din, dout);
+    r.f = disney_eval<ADJ>(m, din, dout);
     r.fwd = disney_pdf(m, din, dout);
     r.rev = disney_pdf(m, dout, din);
     return r;
 }
 // sampleDirection: lambertian.hlsli:29-39, disney.hlsli:415-429
-template <bool LAMBERT>
+template <bool LAMBERT, bool ADJ = false>
 __device__ inline Sample bsdf_sample(const Material& m, float rx, float ry, float rz, v3 din) {
     if (LAMBERT) {
         Sample r;
@@ -471,7 +474,7 @@ __device__ inline Sample bsdf_sample(const Material& m, float rx, float ry, floa
         return r;
     }
     Sample r = disney_sample(m, din, rx, ry, rz);
-    Eval f = bsdf_eval<false>(m, din, r.dir);
+    Eval f = bsdf_eval<false, ADJ>(m, din, r.dir);
     r.f = f.f;
     r.fwd = f.fwd;
     r.rev = f.rev;

@@ -89,6 +89,8 @@ struct RenderConsts {
     float env_prob;
     float max_m, radius;
     uint32_t view_medium, max_null;
+    uint32_t light_paths;  // light tracing: paths per sample (mLightSubpathCount)
+    float ipd;             // light tracing: ViewData::imagePlaneDist (compat/scene.h:147)
     st_view view;
     st_transform inv_cam;  // world to camera (TransformData::inverse of the camera transform)
 };
@@ -121,12 +123,12 @@ __device__ __forceinline__ uint32_t path_of_pixel(const RenderConsts& c, uint32_
 }
 
 // ViewData::toRaster (compat/scene.h:158-165) with ProjectionData::projectPoint
-// (compat/transform.h:139-155); leaves ox, oy unchanged when p is not visible.
-__device__ inline void to_raster(const st_view& V, v3 p, float& ox, float& oy) {
+// (compat/transform.h:139-155); false, with ox, oy unchanged, when p is not visible.
+__device__ inline bool to_raster(const st_view& V, v3 p, float& ox, float& oy) {
     const st_projection& pr = V.projection;
     const float sz = p.z > 0 ? 1.0f : (p.z < 0 ? -1.0f : 0.0f);
     const float sn = pr.near_plane > 0 ? 1.0f : (pr.near_plane < 0 ? -1.0f : 0.0f);
-    if (sz != sn) return;
+    if (sz != sn) return false;
     float r0, r1, r2, r3;
     if (pr.vertical_fov < 0) {
         r0 = p.x * pr.scale[0] + pr.offset[0];
@@ -140,10 +142,11 @@ __device__ inline void to_raster(const st_view& V, v3 p, float& ox, float& oy) {
         r3 = p.z * sn;
     }
     const float aw = fabsf(r3);
-    if (fabsf(r0) >= aw || fabsf(r1) >= aw || fabsf(r2) >= aw) return;
+    if (fabsf(r0) >= aw || fabsf(r1) >= aw || fabsf(r2) >= aw) return false;
     const float ew = (float)(V.image_max[0] - V.image_min[0]), eh = (float)(V.image_max[1] - V.image_min[1]);
     ox = (float)V.image_min[0] + ew * ((r0 / r3) * 0.5f + 0.5f);
     oy = (float)V.image_min[1] + eh * ((-r1 / r3) * 0.5f + 0.5f);
+    return true;
 }
 
 // StoreAuxillaryData (testrenderer.slang:195-218) for a static scene in steady
@@ -242,9 +245,26 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
 
 // Shadow rays: any-hit, then AddRadianceInterlocked (testrenderer.slang:237-261)
 // into the path's own fixed-point accumulator.
+// AddRadianceInterlocked (testrenderer.slang:237-261): p is the path (camera
+// paths) or the pixel slot (light tracing).  max(0, x) maps NaN to 0, as the
+// GPU max of the reference does.
+__device__ __forceinline__ float clamp0(float x) { return x > 0 ? x : 0.0f; }
 __device__ __forceinline__ void add_shadow_contribution(const WaveState& w, uint32_t flags, uint32_t p, v3 ct, float wgt) {
     if (wgt <= 0 || all_nonpos(ct)) return;
     const v3 rw = ct * wgt;
+    if (flags & ST_FLAG_TRACE_FROM_LIGHT) {
+        // light paths of one launch meet in the same pixels: 32-bit atomics,
+        // whose sums do not depend on the order
+        const float q0 = clamp0(rw.x) * (float)kQuant, q1 = clamp0(rw.y) * (float)kQuant, q2 = clamp0(rw.z) * (float)kQuant;
+        uint32_t* a = reinterpret_cast<uint32_t*>(w.nee + p);
+        const uint32_t i0 = (uint32_t)(q0 >= 2147483647.0f ? 2147483647 : (int32_t)q0);
+        const uint32_t i1 = (uint32_t)(q1 >= 2147483647.0f ? 2147483647 : (int32_t)q1);
+        const uint32_t i2 = (uint32_t)(q2 >= 2147483647.0f ? 2147483647 : (int32_t)q2);
+        if (i0) atomicAdd(a, i0);
+        if (i1) atomicAdd(a + 1, i1);
+        if (i2) atomicAdd(a + 2, i2);
+        return;
+    }
     if (flags & ST_FLAG_FLOAT_NEE) {
         float4 e = w.em[p];
         e.x += fmax_(0.0f, rw.x);
@@ -253,8 +273,7 @@ __device__ __forceinline__ void add_shadow_contribution(const WaveState& w, uint
         w.em[p] = e;
         return;
     }
-    const float q0 = fmax_(0.0f, rw.x) * (float)kQuant, q1 = fmax_(0.0f, rw.y) * (float)kQuant,
-                q2 = fmax_(0.0f, rw.z) * (float)kQuant;
+    const float q0 = clamp0(rw.x) * (float)kQuant, q1 = clamp0(rw.y) * (float)kQuant, q2 = clamp0(rw.z) * (float)kQuant;
     // one shadow ray per path per bounce: a plain read-modify-write is race free
     uint4 acc = w.nee[p];
     acc.x += (uint32_t)(q0 >= 2147483647.0f ? 2147483647 : (int32_t)q0);
@@ -276,7 +295,8 @@ struct ShadowSrc {
         if (h.hit) return;
         const float4 c = w.shadow[3 * (size_t)i + 2];
         const uint32_t p = __float_as_uint(w.shadow[3 * (size_t)i + 1].w);
-        add_shadow_contribution(w, flags, p, mk(c.x, c.y, c.z), c.w);
+        // light tracing keeps the light path in .w; its weight is 1 (no MIS)
+        add_shadow_contribution(w, flags, p, mk(c.x, c.y, c.z), (flags & ST_FLAG_TRACE_FROM_LIGHT) ? 1.0f : c.w);
     }
 };
 
@@ -941,6 +961,188 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
             w.shadow[3 * (size_t)si + 1] = sh1;
             w.shadow[3 * (size_t)si + 2] = sh2;
         }
+    }
+}
+
+// ------------------------------------------------------------ light tracing
+// Light path p of a wave is light subpath p % light_paths of sample
+// p / light_paths.  Its random stream is keyed by its dispatch index
+// (q % W, q / W) with offset 0xFFFF (testrenderer.slang:983-984, 1092-1094).
+__device__ __forceinline__ Rng light_rng(const RenderConsts& c, uint32_t p, uint32_t counter) {
+    const uint32_t q = p % c.light_paths;
+    return Rng{q % c.W, q / c.W, c.seed + c.sample_base + p / c.light_paths, counter};
+}
+
+// InitializeLightPath (testrenderer.slang:983-1024): SampleIllumination(rnd, 0)
+// picks the emitter; a triangle emits cosine-distributed about its geometry
+// normal from the offset sample point, the environment emits along -d from a
+// disc of the scene sphere's radius facing d (throughput Le / pdfW, no disc
+// pdf, as the reference leaves it).  A sample with zero pdf (a one-sided light
+// facing away from the origin, or no light) or a NaN throughput starts no path:
+// the reference carries such a path on with a NaN throughput whose
+// contributions clamp to zero.  Also clears the wave's pixel accumulators.
+template <bool ALIAS>
+__global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderConsts c, WaveState w) {
+    const uint32_t n = c.wave_paths, nacc = c.k * c.npix, stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nacc; i += stride) {
+        w.em[i] = make_float4(0, 0, 0, 0);
+        w.nee[i] = make_uint4(0, 0, 0, 0);
+    }
+    const uint32_t base0 = blockIdx.x * blockDim.x + threadIdx.x - __lane_id();
+    for (uint32_t i0 = base0; i0 < n; i0 += stride) {
+        const uint32_t p = i0 + __lane_id();
+        bool live = false;
+        if (p < n) {
+            Rng rng = light_rng(c, p, 0xFFFFu);
+            const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), mk(0, 0, 0));
+            const float4 dr = rng.nextf();
+            v3 thr = L.radiance / L.pdf, o, d;
+            float dirpdf;
+            if (L.finite) {
+                const v3 ld = sample_cos_hemisphere(dr.x, dr.y);
+                const v3 nrm = unpack_normal(L.gn);
+                v3 t0, t1;
+                make_onb(nrm, t0, t1);
+                o = ray_offset(L.pos, nrm);
+                d = normalize(t0 * ld.x + t1 * ld.y + nrm * ld.z);
+                dirpdf = q16(cos_hemisphere_pdfw(ld.z));  // mDirPdfW is stored in fp16
+                thr = thr * (ld.z / dirpdf);
+            } else {
+                float x, y;
+                concentric_disc(dr.x, dr.y, x, y);
+                const v3 nd = -L.dir;
+                v3 t0, t1;
+                make_onb(nd, t0, t1);
+                const float R = S.sphere[3];
+                o = mk(S.sphere[0], S.sphere[1], S.sphere[2]) + (t0 * x + t1 * y + L.dir) * R;
+                d = nd;
+                dirpdf = q16(L.pdf);
+            }
+            const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
+            live = L.pdf > 0 && !nan && any_pos(thr) && c.max_depth > 0;
+            if (live) {
+                w.ray[p] = make_float4(o.x, o.y, o.z, __uint_as_float(pack_normal(d)));
+                w.thr[p] = make_float4(thr.x, thr.y, thr.z, 0);
+                w.st[p] = make_uint4(rng.w, 0, 0, f2h(dirpdf));
+                w.med[p] = kNoMedium;
+            }
+        }
+        const uint32_t qi = wave_append(&w.counters[0], live);
+        if (live) w.queue[0][qi] = p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        w.counters[16] = 0;
+        w.counters[17] = 0;
+    }
+}
+
+// ProcessInteraction<true> (testrenderer.slang:810-935) at the surface vertices
+// of the queued light paths: a vertex that can scatter and is not singular
+// connects to the camera (ConnectToCamera, :441-524: the adjoint BSDF toward
+// the camera, weighted by the image-to-surface pdf conversion and divided by
+// the number of light paths; its shadow ray ends at the camera, unshortened,
+// and adds to the pixel it projects to), then Russian roulette and an adjoint
+// BSDF sample extend the path.  A miss ends a light path without emission.
+template <bool LAMBERT>
+__global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderConsts c, WaveState w, int qsel) {
+    const uint32_t n = w.counters[qsel];
+    const uint32_t* q = w.queue[qsel];
+    uint32_t* qout = w.queue[qsel ^ 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t base0 = blockIdx.x * blockDim.x + threadIdx.x - __lane_id();
+    const st_transform& T = c.view.camera_to_world;
+    const v3 cam = mk(T.m[0][3], T.m[1][3], T.m[2][3]);  // transformPoint(0) of the view transform
+    for (uint32_t i0 = base0; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + __lane_id();
+        bool emit_shadow = false, cont = false;
+        float4 sh0 = make_float4(0, 0, 0, 0), sh1 = sh0, sh2 = sh0, ray_out = sh0, thr_out = sh0;
+        uint32_t p = 0;
+        if (i < n) {
+            p = q[i];
+            const uint4 hit = w.hit[i];
+            if (hit.y != 0xffffffffu) {
+                const float4 r = w.ray[p];
+                const float4 thr4 = w.thr[p];
+                uint4 s4 = w.st[p];
+                Rng rng = light_rng(c, p, s4.x);
+                const uint32_t path_len = s4.y + 1;
+                const v3 dir = unpack_normal(__float_as_uint(r.w));
+                v3 thr = mk(thr4.x, thr4.y, thr4.z);
+                Shading sd;
+                tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
+                const Material m = load_material(S, sd.material);
+                const Frame f = make_frame(sd);
+                const v3 ldin = f.to_local(-dir);
+                const v3 lgn = f.to_local(unpack_normal(sd.gn));
+                const v3 gn = unpack_normal(sd.gn);
+                bool alive = path_len < c.max_depth && any_pos(m.base());
+                const bool rr_on = path_len >= c.rr_start;
+                if (alive && !m_singular<LAMBERT>(m) && path_len + 1 >= c.min_len) {
+                    const v3 pc = xf_point(c.inv_cam, sd.pos);
+                    float pcx = 0, pcy = 0;
+                    if (to_raster(c.view, pc, pcx, pcy)) {
+                        const float cos_cam = fabsf(normalize(pc).z);
+                        v3 tc = cam - sd.pos;
+                        const float dist = len(tc);
+                        tc = tc / dist;
+                        const v3 ldout = f.to_local(tc);
+                        const float cos_out = fabsf(ldout.z);
+                        Eval fe = bsdf_eval<LAMBERT, true>(m, ldin, ldout);
+                        fe.f = fe.f * shading_normal_correction<true>(f, lgn, ldin, ldout);
+                        const float ipcd = c.ipd / cos_cam;
+                        const float i2s = (sq(ipcd) / cos_cam) * cos_out / sq(dist);
+                        const float s2i = 1.0f / i2s;
+                        const v3 contrib = (thr * fe.f) / ((float)c.light_paths * s2i * cos_out);
+                        // pixel index int(y) * W + int(x); one beyond the image is dropped
+                        const uint32_t pix = (uint32_t)(int32_t)pcy * c.W + (uint32_t)(int32_t)pcx;
+                        if (pix < c.npix) {
+                            const uint32_t slot = (p / c.light_paths) * c.npix + path_of_pixel(c, pix % c.W, pix / c.W);
+                            const v3 o = ray_offset(sd.pos, gn, tc);
+                            sh0 = make_float4(o.x, o.y, o.z, dist);
+                            sh1 = make_float4(tc.x, tc.y, tc.z, __uint_as_float(slot));
+                            sh2 = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(p));
+                            emit_shadow = true;
+                        }
+                    }
+                }
+                if (alive) {
+                    const float cp = fmin_(1.0f, m_cont_prob(m));
+                    const float rr = rng.nextf().x;
+                    if (rr_on && rr > cp) alive = false;
+                    if (alive) {
+                        const float cpx = rr_on ? cp : 1.0f;
+                        const float4 rb = rng.nextf();
+                        Sample sm = bsdf_sample<LAMBERT, true>(m, rb.x, rb.y, rb.z, ldin);
+                        sm.fwd *= cpx;
+                        sm.rev *= cpx;
+                        thr = thr * (sm.f / sm.fwd);
+                        thr = thr * shading_normal_correction<true>(f, lgn, ldin, sm.dir);
+                        const v3 wd = f.to_world(sm.dir);
+                        const v3 o = ray_offset(sd.pos, gn, wd);
+                        const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
+                        cont = !nan && !all_nonpos(thr) && path_len + 1 <= c.max_depth;
+                        ray_out = make_float4(o.x, o.y, o.z, __uint_as_float(pack_normal(wd)));
+                        thr_out = make_float4(thr.x, thr.y, thr.z, 0);
+                        s4.w = f2h(sm.rough == 0 ? 0.0f : sm.fwd);
+                    }
+                }
+                s4.x = rng.w;
+                s4.y = path_len;
+                w.st[p] = s4;
+                if (cont) {
+                    w.ray[p] = ray_out;
+                    w.thr[p] = thr_out;
+                }
+            }
+        }
+        const uint32_t si = wave_append(&w.counters[2], emit_shadow);
+        if (emit_shadow) {
+            w.shadow[3 * (size_t)si] = sh0;
+            w.shadow[3 * (size_t)si + 1] = sh1;
+            w.shadow[3 * (size_t)si + 2] = sh2;
+        }
+        const uint32_t qi = wave_append(&w.counters[qsel ^ 1], cont);
+        if (cont) qout[qi] = p;
     }
 }
 
@@ -1758,9 +1960,12 @@ static void ensure_spill(st_context* ctx, int grid, int block) {
     ctx->spill.reserve((size_t)(kMaxStack - kLdsStack + 1) * grid * block * 4);
 }
 
+static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o);
+
 int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if (!s || !view || !p || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
+    if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, o);
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && !(p->flags & ST_FLAG_RESTIR_DI))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && (p->spatial_neighbours > 64 || !(p->spatial_radius >= 0) || p->spatial_radius > 4096))
@@ -1916,6 +2121,142 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
             HIP_CHECK(hipMemcpyAsync(o->ray_counts, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToDevice, sm));
         HIP_CHECK(hipStreamSynchronize(sm));
         // per-kernel-class device time (HIP events on this stream)
+        for (int kind = 0; kind < 3; kind++) {
+            double ms = 0;
+            for (size_t i = 0; i < ctx->ev_used[kind]; i++) {
+                float t = 0;
+                HIP_CHECK(hipEventElapsedTime(&t, ctx->ev[kind][i].first, ctx->ev[kind][i].second));
+                ms += t;
+            }
+            ctx->timings[kind] = ms;
+            ctx->timings[3 + kind] = (double)ctx->ev_used[kind];
+        }
+        unsigned long long hs[4], rc[2];
+        HIP_CHECK(hipMemcpy(hs, ctx->stats.p, 32, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(rc, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToHost));
+        for (int i = 0; i < 4; i++) ctx->timings[6 + i] = (double)hs[i];
+        ctx->timings[10] = (double)rc[0];
+        ctx->timings[11] = (double)rc[1];
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+// Light tracing (TestRenderer with mLightTrace): light subpaths of every sample
+// in waves, connections to the camera accumulated per pixel and sample in 1/16384
+// fixed point, then the same resolve as camera paths.
+static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
+    if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
+    if (s->d.vinst_count > 0) return set_error(ST_ERROR_UNSUPPORTED, "light tracing is not supported in scenes with media");
+    if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64))
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "light_subpath_factor must be in [0, 64]");
+    st_context* ctx = s->ctx;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        const uint32_t npix = p->width * p->height;
+        // mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452
+        const uint32_t L = std::max<uint32_t>(1u, (uint32_t)((float)npix * p->light_subpath_factor));
+        const uint32_t span = std::max(L, npix);
+        uint32_t k = p->samples_per_wave;
+        if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, (uint32_t)((4u << 20) / span)));
+        k = std::min(k, p->spp);
+        const uint64_t paths = (uint64_t)span * k;
+        if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
+        ctx->ray.reserve(paths * 16);
+        ctx->thr.reserve(paths * 16);
+        ctx->stt.reserve(paths * 16);
+        ctx->em.reserve(paths * 16);
+        ctx->nee.reserve(paths * 16);
+        ctx->hit.reserve(paths * 16);
+        ctx->q0.reserve(paths * 4);
+        ctx->q1.reserve(paths * 4);
+        ctx->shadow.reserve(paths * 48);
+        ctx->med.reserve(paths * 4);
+        ctx->counters.reserve(256);
+        ctx->acc.reserve((size_t)npix * 16);
+        ctx->stats.reserve(64);
+        const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
+        ensure_spill(ctx, tgrid, kTraceBlock);
+        WaveState w;
+        std::memset(&w, 0, sizeof w);
+        w.ray = ctx->ray.as<float4>();
+        w.thr = ctx->thr.as<float4>();
+        w.st = ctx->stt.as<uint4>();
+        w.em = ctx->em.as<float4>();
+        w.nee = ctx->nee.as<uint4>();
+        w.hit = ctx->hit.as<uint4>();
+        w.queue[0] = ctx->q0.as<uint32_t>();
+        w.queue[1] = ctx->q1.as<uint32_t>();
+        w.shadow = ctx->shadow.as<float4>();
+        w.counters = ctx->counters.as<uint32_t>();
+        w.med = ctx->med.as<uint32_t>();
+        hipStream_t sm = ctx->stream;
+        HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 256, sm));
+        HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, sm));
+        HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, sm));
+        for (int kk = 0; kk < 3; kk++) ctx->ev_used[kk] = 0;
+        RenderConsts c;
+        std::memset(&c, 0, sizeof c);
+        c.W = p->width;
+        c.H = p->height;
+        c.npix = npix;
+        c.seed = p->seed;
+        c.max_depth = p->max_depth;
+        c.min_len = p->min_path_length;
+        // the reference turns NEE, ReSTIR DI and the direct-only mode off when
+        // light tracing (TestRenderer.cpp:119-124)
+        c.flags = (p->flags & ~(ST_FLAG_SAMPLE_DIRECT | ST_FLAG_DIRECT_ONLY | ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE |
+                                ST_FLAG_FLOAT_NEE));
+        c.rr_start = p->rr_start_depth;
+        c.env_prob = p->environment_sample_probability;
+        c.view = *view;
+        c.inv_cam = inverse_transform(view->camera_to_world);
+        c.view_medium = kNoMedium;
+        c.light_paths = L;
+        // ViewData::imagePlaneDist, compat/scene.h:147
+        c.ipd = (float)std::abs(view->image_max[1] - view->image_min[1]) / (2 * std::tan(view->projection.vertical_fov / 2));
+        c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
+        const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+        const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
+        auto* stats = ctx->stats.as<unsigned long long>();
+        for (uint32_t base = 0; base < p->spp; base += k) {
+            c.k = std::min(k, p->spp - base);
+            c.sample_base = base;
+            c.wave_paths = L * c.k;
+            HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 16, sm));  // queue and shadow counts
+            const int igrid = std::min<int>((std::max(c.wave_paths, c.k * npix) + kShadeBlock - 1) / kShadeBlock, ctx->cus * 16);
+            if (alias) hipLaunchKernelGGL(k_light_init<true>, dim3(igrid), dim3(kShadeBlock), 0, sm, s->d, c, w);
+            else hipLaunchKernelGGL(k_light_init<false>, dim3(igrid), dim3(kShadeBlock), 0, sm, s->d, c, w);
+            int qsel = 0;
+            for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
+                {
+                    Timed tm(ctx, 0);
+                    if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                }
+                {
+                    Timed tm(ctx, 1);
+                    if (lambert) hipLaunchKernelGGL(k_shade_light<true>, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
+                    else hipLaunchKernelGGL(k_shade_light<false>, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
+                }
+                {
+                    Timed tm(ctx, 2);
+                    if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
+                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
+                }
+                hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, 0);
+                qsel ^= 1;
+            }
+            const bool last = base + c.k >= p->spp;
+            const int rgrid = std::min<int>((npix + 255) / 256, ctx->cus * 8);
+            hipLaunchKernelGGL(k_resolve, dim3(rgrid), dim3(256), 0, sm, c, w, ctx->acc.as<float4>(), o->hdr,
+                               (uint32_t*)nullptr, 1.0f / (float)p->spp, last ? 1 : 0);
+            HIP_CHECK(hipGetLastError());
+        }
+        if (o->ray_counts)
+            HIP_CHECK(hipMemcpyAsync(o->ray_counts, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToDevice, sm));
+        HIP_CHECK(hipStreamSynchronize(sm));
         for (int kind = 0; kind < 3; kind++) {
             double ms = 0;
             for (size_t i = 0; i < ctx->ev_used[kind]; i++) {

@@ -125,10 +125,18 @@ __device__ __forceinline__ Frame make_frame(const Shading& sd) {
     return f;
 }
 __device__ __forceinline__ float sgn(float x) { return x > 0 ? 1.0f : (x < 0 ? -1.0f : 0.0f); }
+// ShadingData::shadingNormalCorrection, shading_data.hlsli:35-54.  ADJ: light
+// paths, whose throughput also takes the ratio of geometric to shading cosines.
+template <bool ADJ = false>
 __device__ __forceinline__ float shading_normal_correction(const Frame& f, v3 lgn, v3 din, v3 dout) {
     const float ngin = dot(lgn, din), ngout = dot(lgn, dout);
     if (sgn(ngout) != sgn(dout.z) || sgn(ngin) != sgn(din.z)) return 0;
-    return 1;
+    float G = 1;
+    if (ADJ) {
+        const float num = ngout * din.z, denom = dout.z * ngin;
+        if (fabsf(denom) > 1e-5f) G *= fabsf(num / denom);
+    }
+    return G;
 }
 
 // LoadMaterial, material.hlsli:15-21 (configurations carry no material textures)
@@ -292,6 +300,8 @@ struct LightSample {  // IlluminationSampleRecord, lights.hlsli:32-45
     v3 radiance, dir;
     float pdf, dist, cos_light;
     bool finite;
+    v3 pos;       // finite lights: the sampled point
+    uint32_t gn;  // finite lights: its packed geometry normal
 };
 
 // SampleEmission + SampleIllumination, lights.hlsli:166-260.  The reference
@@ -306,6 +316,8 @@ __device__ inline LightSample sample_illumination(const DevScene& S, float env_p
     r.dist = 0;
     r.cos_light = 0;
     r.finite = false;
+    r.pos = mk(0, 0, 0);
+    r.gn = 0;
     if (S.has_env) {
         if (S.light_count == 0 || rnd.w < env_prob) {
             v3 d;
@@ -356,6 +368,8 @@ __device__ inline LightSample sample_illumination(const DevScene& S, float env_p
     r.radiance = m.emission();
     r.pdf = pdf;
     r.finite = true;
+    r.pos = sd.pos;
+    r.gn = sd.gn;
     v3 dd = sd.pos - ref;
     r.dist = len(dd);
     r.dir = dd / r.dist;

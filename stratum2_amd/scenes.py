@@ -403,6 +403,22 @@ def cornell_box(seed=0, light_radiance=15.0):
     return s
 
 
+def cornell_box_smooth(seed=0, subdiv=1, light_radiance=15.0):
+    """The seeded Cornell box with its sphere replaced by a coarse icosphere
+    (subdivision subdiv) carrying per-vertex normals, so that shading normals
+    differ from the geometric normals by up to ~20 degrees."""
+    s = cornell_box(seed, light_radiance)
+    v, f = icosphere(subdiv)
+    n = v / np.linalg.norm(v, axis=1, keepdims=True)
+    ctr = np.array([0.45, 0.45, -0.2])
+    v = v * 0.4 + ctr
+    f = _orient_outward(v, f, ctr)
+    vi, nt = s.add_mesh(v.astype(np.float32), f.reshape(-1).astype(np.uint32), normals=n.astype(np.float32))
+    a, _ = s.instances[7]
+    s.instances[7] = (a, vi | (nt << 16))
+    return s
+
+
 def cornell_view(width=256, height=256):
     return make_view(width, height, position=(0.0, 1.0, 2.4))
 
