@@ -17,6 +17,8 @@ Prints one JSON line with the driver's contract plus:
                 (FETCH_SIZE, WRITE_SIZE) and recorded in profiles/
   cpu_baseline  the CPU oracle (a C port of the reference path tracer) on this
                 host's cores over a bounded sample of the same frame
+--light-trace times TestRenderer's light-tracing mode on the spheres workload
+instead (light subpaths, one per pixel and sample, connected to the camera).
 --copy-calibration runs a plain device copy of known size (for scaling the
 PMC byte counters) and exits.
 """
@@ -47,6 +49,7 @@ def main():
     ap.add_argument("--workload", choices=["spheres", "cloud"], default="spheres")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--device-bvh", action="store_true", help="rebuild the BVH on the GPU (st_scene_update) before timing")
+    ap.add_argument("--light-trace", action="store_true", help="light tracing (paths start on the lights)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -71,6 +74,11 @@ def main():
         if args.spp == 64:
             args.spp = 32
         args.no_restir = True  # ReSTIR DI is not offered with media
+    if args.light_trace:
+        if cloud:
+            print("--light-trace runs the spheres workload (light tracing with media is not offered)", file=sys.stderr)
+            sys.exit(2)
+        args.no_restir = True  # the reference turns ReSTIR DI off when light tracing
     max_depth = 16 if cloud else 8
 
     t0 = time.time()
@@ -92,6 +100,8 @@ def main():
         log(f"device BVH rebuild: {bvh_ms:.2f} ms: {sc.stats()}")
     r = TestRenderer(sc)
     flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
+    if args.light_trace:
+        flags |= abi.ST_FLAG_TRACE_FROM_LIGHT
     if not args.no_restir:  # ReSTIR DI: 32 candidates, temporal + spatial (5 neighbours, 30 px) reuse
         flags |= abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
     rr = 0 if cloud else 3
@@ -150,6 +160,8 @@ def main():
             "k_trace_shadow": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * (52 + NODE_B * nodes_s + TRI_B * tris_s)),
             "k_shade": (t_last["shade_ms"], t_last["shade_launches"], t_last["closest_rays"] * (16 + 64 + 64 + 64 + 48) + t_last["shadow_rays"] * 48),
         }
+        if args.light_trace:  # k_shade_light: hit, ray, throughput, state read; ray, throughput, state written
+            classes["k_shade_light"] = classes.pop("k_shade")
     dom = max(classes, key=lambda k: classes[k][0])
     dms, dl, dbytes = classes[dom]
     achieved = dbytes / (dms / 1e3) / 1e9 if dms > 0 else 0.0
@@ -175,11 +187,13 @@ def main():
         workload = f"{args.width}x{args.height} {args.spp}spp maxPathLength 8"
     if not cloud and not args.no_restir:
         workload += ", ReSTIR DI 32 candidates + temporal + spatial 5x30px"
+    if args.light_trace:
+        workload += ", light tracing (1 light subpath per pixel and sample, connected to the camera)"
     line = {
         "metric": "Mrays/s volume 1024^2" if cloud else "Mrays/s 1080p/1M tris", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": workload, "restir_di": not args.no_restir,
+        "config": {"workload": workload, "restir_di": not args.no_restir, "light_tracing": args.light_trace,
                    "width": args.width, "height": args.height, "spp": args.spp, "max_path_length": max_depth,
                    "triangles": sd.triangle_count, "bvh": "device linear BVH" if args.device_bvh else "host binned SAH",
                    "bvh_build_ms": round(bvh_ms, 2) if bvh_ms is not None else None, "samples_per_px_per_s": round(args.width * args.height * args.spp / (ms / 1e3), 1)},
@@ -193,7 +207,8 @@ def measured_traffic(args, kernel):
     """HBM bytes per launch of `kernel` from the PMC summary committed for
     this workload (profiles/hbm_traffic_<workload>.json: FETCH_SIZE and
     WRITE_SIZE per dispatch, scaled by the copy calibration)."""
-    name = "cloud" if args.workload == "cloud" else ("spheres_plain" if args.no_restir else "spheres_restir")
+    name = "cloud" if args.workload == "cloud" else ("spheres_light" if args.light_trace else
+                                                      ("spheres_plain" if args.no_restir else "spheres_restir"))
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{name}.json")
     if not os.path.exists(path):
         return None, None
@@ -227,8 +242,9 @@ def cpu_baseline(sd, view, args, max_depth, log):
     o = OracleScene(sd)
     threads = min(16, os.cpu_count() or 1)  # this host's share of cores
     spp = 1
+    fl = abi.ST_FLAG_DEFAULT | (abi.ST_FLAG_TRACE_FROM_LIGHT if args.light_trace else 0)
     while True:
-        p = abi.default_params(args.width, args.height, max_depth=max_depth, spp=spp)
+        p = abi.default_params(args.width, args.height, max_depth=max_depth, spp=spp, flags=fl)
         a = time.perf_counter()
         res = o.render(view, p, threads=threads, want_path_length=False)
         dt = time.perf_counter() - a
@@ -238,7 +254,8 @@ def cpu_baseline(sd, view, args, max_depth, log):
         spp = min(64, spp * max(2, int(args.cpu_seconds / 3 / max(dt, 1e-3)) + 1))
     log(f"cpu baseline: {args.width}x{args.height} x {spp} spp, {nr} rays in {dt:.2f}s on {threads} threads")
     return {"value": round(nr / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (C port of the reference tracer, plain NEE), same {args.width}x{args.height} frame at "
+            "sample": f"CPU oracle (C port of the reference tracer, {'light tracing' if args.light_trace else 'plain NEE'}), "
+                      f"same {args.width}x{args.height} frame at "
                       f"{spp} spp, maxPathLength {max_depth}, {nr} rays in {dt:.1f} s"}
 
 
