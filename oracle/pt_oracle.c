@@ -2044,62 +2044,100 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
  * (testrenderer.slang:1087-1144) over mLightSubpathCount paths, then
  * ProcessShadowRays and ProcessAtomicOutput with gClearImage (:1147-1183). */
 typedef struct {
-    const or_scene* s;
-    const cfg_t* c;
-    const st_view* view;
-    st_transform inv_cam;
-    uint32_t W, H, L;
+    pctx_t b;       /* path state, scene, counters, view */
+    uint32_t L;     /* light paths per sample */
     float ipd;      /* ViewData::imagePlaneDist */
+    uint32_t seed;  /* this sample's seed */
     uint32_t* acc;  /* this thread's mOutputAtomic, 3 per pixel */
-    uint64_t rays[2], trav[4];
 } lctx_t;
 
-/* ConnectToCamera, testrenderer.slang:441-524 (no VC: weight 1).  The shadow
- * ray runs from the offset vertex to the camera position, unshortened; its
+/* SceneParameters::GetMediumIndex (scene.hlsli:32-40): the first volume whose
+ * object-space bounds hold the point. */
+static uint32_t medium_at(const or_scene* s, f3 pos) {
+    for (uint32_t i = 0; i < s->nvolinfo; i++) {
+        const st_volume_info* vi = &s->volinfo[i];
+        const f3 lp = xf_point(&s->inv_xf[vi->instance_index], pos);
+        if (lp.x >= vi->min[0] && lp.y >= vi->min[1] && lp.z >= vi->min[2] && lp.x <= vi->max[0] && lp.y <= vi->max[1] &&
+            lp.z <= vi->max[2])
+            return vi->instance_index;
+    }
+    return 0xFFFFu;
+}
+
+/* ConnectToCamera, testrenderer.slang:441-524 (no VC: weight 1).  med != NULL:
+ * a medium vertex (phase function toward the camera, cosOut 1, the ray leaves
+ * from the vertex itself); else a surface vertex with the adjoint BSDF and
+ * shading-normal factor.  The shadow ray runs to the camera position,
+ * unshortened, through the vertex's medium (ratio tracking keyed like the
+ * camera paths' NEE, by the light path's dispatch index and length); its
  * contribution goes to pixel int(y) * W + int(x) (dropped beyond the image). */
-static void connect_to_camera(lctx_t* x, f3 thr, const material_t* m, const shading_t* sd, f3 ldin) {
-    const f3 pc = xf_point(&x->inv_cam, sd->pos);
+static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* sd, const medium_t* med, f3 pos, f3 ldin,
+                              uint32_t q) {
+    pctx_t* x = &lx->b;
+    const f3 pc = xf_point(&x->inv_cam, pos);
     float pcx = 0, pcy = 0;
     if (!to_raster(x->view, pc, &pcx, &pcy)) return;
     const float cos_cam = fabsf(normalize3(pc).z);
     const st_transform* T = &x->view->camera_to_world;
-    f3 tc = sub(v3(T->m[0][3], T->m[1][3], T->m[2][3]), sd->pos);
+    f3 tc = sub(v3(T->m[0][3], T->m[1][3], T->m[2][3]), pos);
     const float dist = length3(tc);
     tc = divf(tc, dist);
-    const f3 ldout = to_local(sd, tc);
-    const float cos_out = fabsf(ldout.z);
-    rer_t f = bsdf_eval_x(m, x->c->lambert, ldin, ldout, 1);
-    f.f = mulf(f.f, shading_normal_correction_x(sd, ldin, ldout, 1));
-    const float image_point_to_camera = x->ipd / cos_cam;
+    f3 f, o;
+    float cos_out;
+    if (med) {
+        const float v = hg_eval(med->g, ldin, tc);
+        f = v3(v, v, v);
+        cos_out = 1;
+        o = pos;
+    } else {
+        const f3 ldout = to_local(sd, tc);
+        cos_out = fabsf(ldout.z);
+        rer_t r = bsdf_eval_x(m, x->c->lambert, ldin, ldout, 1);
+        f = mulf(r.f, shading_normal_correction_x(sd, ldin, ldout, 1));
+        o = ray_offset_dir(pos, unpack_normal(sd->gn), tc);
+    }
+    const float image_point_to_camera = lx->ipd / cos_cam;
     const float image_to_solid_angle = pow2(image_point_to_camera) / cos_cam;
     const float image_to_surface = image_to_solid_angle * cos_out / pow2(dist);
     const float surface_to_image = 1.0f / image_to_surface;
-    const f3 contrib = divf(mul(thr, f.f), (float)x->L * surface_to_image * cos_out);
+    f3 contrib = divf(mul(x->ps.thr, f), (float)lx->L * surface_to_image * cos_out);
     const uint32_t pix = (uint32_t)(int32_t)pcy * x->W + (uint32_t)(int32_t)pcx;
     if (pix >= x->W * x->H) return;
-    const f3 o = ray_offset_dir(sd->pos, unpack_normal(sd->gn), tc);
-    hit_t h = trace(x->s, o, tc, dist, 1, &x->trav[2], &x->trav[3]);
-    x->rays[1]++;
-    if (h.hit || all_le0(contrib)) return;
+    if (x->s->nvolinst > 0) {
+        shadow_t sr;
+        sr.o = o; sr.d = tc; sr.tmax = dist; sr.c = contrib; sr.w = 1;
+        sr.medium = x->ps.medium; sr.plen = x->ps.path_len;
+        float neepdf;
+        trace_visibility_media(x, &sr, q % x->W, q / x->W, lx->seed, &contrib, &neepdf);
+        if (all_le0(contrib) || neepdf <= 0) return;
+        contrib = divf(contrib, neepdf);
+    } else {
+        hit_t h = trace(x->s, o, tc, dist, 1, &x->trav[2], &x->trav[3]);
+        x->rays[1]++;
+        if (h.hit) return;
+    }
+    if (all_le0(contrib)) return;
     /* AddRadianceInterlocked, testrenderer.slang:237-261 */
-    const float q[3] = {clamp0(contrib.x) * (float)OR_Q, clamp0(contrib.y) * (float)OR_Q, clamp0(contrib.z) * (float)OR_Q};
-    for (int k = 0; k < 3; k++) x->acc[3 * (size_t)pix + k] += (uint32_t)(q[k] >= 2147483647.0f ? INT_MAX : (int32_t)q[k]);
+    const float qv[3] = {clamp0(contrib.x) * (float)OR_Q, clamp0(contrib.y) * (float)OR_Q, clamp0(contrib.z) * (float)OR_Q};
+    for (int k = 0; k < 3; k++) lx->acc[3 * (size_t)pix + k] += (uint32_t)(qv[k] >= 2147483647.0f ? INT_MAX : (int32_t)qv[k]);
 }
 
 /* One light subpath: InitializeLightPath (testrenderer.slang:983-1024), then
- * TraceRay<true> + ProcessInteraction<true> (:810-935, 1026-1074) per segment.
- * A sample of zero pdf (no light, or a one-sided light facing away from the
- * origin) or with a NaN throughput ends at once: the reference carries it on
- * with a NaN throughput whose contributions clamp to zero. */
-static void trace_light_path(lctx_t* x, uint32_t q, uint32_t seed) {
+ * TraceRay<true> + ExtendPath + ProcessInteraction<true> (:810-935, 1026-1074)
+ * per segment.  A sample of zero pdf (no light, or a one-sided light facing
+ * away from the origin) or with a NaN throughput ends at once: the reference
+ * carries it on with a NaN throughput whose contributions clamp to zero. */
+static void trace_light_path(lctx_t* lx, uint32_t q) {
+    pctx_t* x = &lx->b;
     const or_scene* s = x->s;
     const cfg_t* c = x->c;
-    rng_t rng;
-    rng_init(&rng, seed, q % x->W, q / x->W, 0xFFFFu);
+    pstate_t* ps = &x->ps;
+    memset(ps, 0, sizeof *ps);
+    rng_init(&ps->rng, lx->seed, q % x->W, q / x->W, 0xFFFFu);
     float rnd[4], dr[4];
-    rng_nextf(&rng, rnd);
+    rng_nextf(&ps->rng, rnd);
     const isr_t r = sample_illumination(s, c, rnd, v3(0, 0, 0));
-    rng_nextf(&rng, dr);
+    rng_nextf(&ps->rng, dr);
     f3 thr = divf(r.radiance, r.pdf), o, d;
     if (r.finite) {
         const f3 ld = sample_cos_hemisphere(dr[0], dr[1]);
@@ -2119,38 +2157,69 @@ static void trace_light_path(lctx_t* x, uint32_t q, uint32_t seed) {
         d = neg(r.dir);
     }
     if (!(r.pdf > 0) || thr.x != thr.x || thr.y != thr.y || thr.z != thr.z || !any_gt0(thr)) return;
-    uint32_t dir = pack_normal(d), path_len = 0;
-    while (path_len + 1 <= c->max_depth) {
-        const f3 wd = unpack_normal(dir);
-        hit_t h = trace(s, o, wd, INFINITY, 0, &x->trav[0], &x->trav[1]);
-        x->rays[0]++;
-        path_len++;
-        if (!h.hit) return; /* no emission is gathered on light paths */
-        shading_t sd;
-        make_tri_shading(s, h.id & 0xffffu, h.id >> 16, h.b1, h.b2, &sd);
-        const material_t m = load_material(s, sd.flags_material >> 4);
-        const f3 ldin = to_local(&sd, neg(wd));
-        if (path_len >= c->max_depth || !m_can_eval(&m)) return;
-        if (!m_singular(&m, c->lambert) && path_len + 1 >= c->min_len) connect_to_camera(x, thr, &m, &sd, ldin);
-        const float cp = minf(1, m_cont_prob(&m));
-        float rr[4], rb[4];
-        rng_nextf(&rng, rr);
-        if (rr[0] > cp) return;
-        rng_nextf(&rng, rb);
-        dsr_t smp = bsdf_sample_x(&m, c->lambert, v3(rb[0], rb[1], rb[2]), ldin, 1);
-        smp.fwd *= cp; smp.rev *= cp;
-        thr = mul(thr, divf(smp.f, smp.fwd));
-        thr = mulf(thr, shading_normal_correction_x(&sd, ldin, smp.dir, 1));
-        const f3 w = to_world(&sd, smp.dir);
-        dir = pack_normal(w);
-        o = ray_offset_dir(sd.pos, unpack_normal(sd.gn), w);
-        if (thr.x != thr.x || thr.y != thr.y || thr.z != thr.z) return;
-        if (all_le0(thr)) return; /* RenderIteration skips zero-throughput paths */
+    ps->thr = thr;
+    ps->origin = o;
+    ps->dir = pack_normal(d);
+    ps->medium = (s->nvolinst > 0 && r.finite) ? medium_at(s, r.pos) : 0xFFFFu;
+    while (ps->path_len + 1 <= c->max_depth) {
+        const f3 wd = unpack_normal(ps->dir);
+        hit_t h;
+        int kind;
+        f3 mpos = v3(0, 0, 0);
+        if (s->nvolinst > 0) {
+            float dirpdf, neepdf;
+            kind = trace_scattering(x, ps->origin, wd, &h, &mpos, &dirpdf, &neepdf);
+            ps->thr = divf(ps->thr, dirpdf);
+        } else {
+            h = trace(s, ps->origin, wd, INFINITY, 0, &x->trav[0], &x->trav[1]);
+            x->rays[0]++;
+            kind = h.hit ? 1 : 0;
+        }
+        ps->path_len++;
+        if (kind == 0) return;          /* no emission is gathered on light paths */
+        if (all_le0(ps->thr)) return;   /* ExtendPath, testrenderer.slang:1062-1063 */
+        if (kind == 2) {
+            const medium_t m = load_medium(s, s->inst[ps->medium].type_material_address >> 4);
+            const f3 ldin = neg(wd);
+            if (ps->path_len >= c->max_depth || !med_can_eval(&m)) return;
+            if (!med_singular(&m) && ps->path_len + 1 >= c->min_len) connect_to_camera(lx, NULL, NULL, &m, mpos, ldin, q);
+            const float cp = minf(1, med_cont_prob(&m));
+            float rr[4], rb[4];
+            rng_nextf(&ps->rng, rr);
+            if (rr[0] > cp) return;
+            rng_nextf(&ps->rng, rb);
+            const f3 sdir = hg_sample(m.g, rb[0], rb[1], ldin);
+            const float v = hg_eval(m.g, ldin, sdir);
+            const float fwd = v * cp;
+            ps->thr = mulf(ps->thr, v / fwd);
+            ps->dir = pack_normal(sdir);
+            ps->origin = mpos;
+        } else {
+            shading_t sd;
+            make_tri_shading(s, h.id & 0xffffu, h.id >> 16, h.b1, h.b2, &sd);
+            const material_t m = load_material(s, sd.flags_material >> 4);
+            const f3 ldin = to_local(&sd, neg(wd));
+            if (ps->path_len >= c->max_depth || !m_can_eval(&m)) return;
+            if (!m_singular(&m, c->lambert) && ps->path_len + 1 >= c->min_len) connect_to_camera(lx, &m, &sd, NULL, sd.pos, ldin, q);
+            const float cp = minf(1, m_cont_prob(&m));
+            float rr[4], rb[4];
+            rng_nextf(&ps->rng, rr);
+            if (rr[0] > cp) return;
+            rng_nextf(&ps->rng, rb);
+            dsr_t smp = bsdf_sample_x(&m, c->lambert, v3(rb[0], rb[1], rb[2]), ldin, 1);
+            smp.fwd *= cp; smp.rev *= cp;
+            ps->thr = mul(ps->thr, divf(smp.f, smp.fwd));
+            ps->thr = mulf(ps->thr, shading_normal_correction_x(&sd, ldin, smp.dir, 1));
+            const f3 w = to_world(&sd, smp.dir);
+            ps->dir = pack_normal(w);
+            ps->origin = ray_offset_dir(sd.pos, unpack_normal(sd.gn), w);
+        }
+        if (ps->thr.x != ps->thr.x || ps->thr.y != ps->thr.y || ps->thr.z != ps->thr.z) return;
+        if (all_le0(ps->thr)) return; /* RenderIteration skips zero-throughput paths */
     }
 }
 
 static int render_light(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
-    if (s->nvolinst > 0) return fail("light tracing with media is not supported");
     if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64)) return fail("light_subpath_factor out of range");
     cfg_t c;
     memset(&c, 0, sizeof c);
@@ -2160,6 +2229,7 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
     c.max_depth = p->max_depth;
     c.min_len = p->min_path_length;
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    c.max_null = p->max_null_collisions;
     const uint32_t W = p->width, H = p->height, npix = W * H;
     /* mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452 */
     uint32_t L = (uint32_t)((float)npix * p->light_subpath_factor);
@@ -2183,19 +2253,20 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
         {
             lctx_t x;
             memset(&x, 0, sizeof x);
-            x.s = s; x.c = &c; x.view = view; x.inv_cam = inv_cam;
-            x.W = W; x.H = H; x.L = L; x.ipd = ipd;
+            x.b.s = s; x.b.c = &c; x.b.view = view; x.b.inv_cam = inv_cam;
+            x.b.W = W; x.b.H = H;
+            x.L = L; x.ipd = ipd; x.seed = p->seed + k;
             x.acc = calloc((size_t)npix * 3, 4);
 #pragma omp for schedule(dynamic, 256)
             for (int64_t q = 0; q < (int64_t)L; q++)
-                if (x.acc) trace_light_path(&x, (uint32_t)q, p->seed + k);
+                if (x.acc) trace_light_path(&x, (uint32_t)q);
 #pragma omp critical
             {
                 /* integer sums: the same in any order, as the reference's atomics */
                 if (x.acc) for (size_t i = 0; i < (size_t)npix * 3; i++) total[i] += x.acc[i];
                 else oom = 1;
-                rays[0] += x.rays[0]; rays[1] += x.rays[1];
-                for (int j = 0; j < 4; j++) trav[j] += x.trav[j];
+                rays[0] += x.b.rays[0]; rays[1] += x.b.rays[1];
+                for (int j = 0; j < 4; j++) trav[j] += x.b.trav[j];
             }
             free(x.acc);
         }

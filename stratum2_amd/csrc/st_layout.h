@@ -62,6 +62,8 @@ struct DevScene {
     const uint8_t* vol[kMaxVolumes];
     uint64_t vol_root[kMaxVolumes];  // byte offset of each grid's root node
     uint32_t vinst[kMaxVolumeInstances];
+    const st_volume_info* vinfo;   // VolumeInfo records, in the scene's order
+    uint32_t vinfo_count;
 };
 
 }  // namespace st

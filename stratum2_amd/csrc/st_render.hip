@@ -319,6 +319,32 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v
     if (__lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
+// Random stream of path p at counter `counter`: camera paths are keyed by their
+// pixel and sample; light paths (light_paths > 0) by their dispatch index
+// (q % W, q / W) and sample (testrenderer.slang:983-984, 1092-1094).
+__device__ __forceinline__ Rng path_rng(const RenderConsts& c, uint32_t p, uint32_t counter) {
+    if (c.light_paths) {
+        const uint32_t q = p % c.light_paths;
+        return Rng{q % c.W, q / c.W, c.seed + c.sample_base + p / c.light_paths, counter};
+    }
+    uint32_t px, py;
+    pixel_of(c, p % c.npix, px, py);
+    return Rng{px, py, c.seed + c.sample_base + p / c.npix, counter};
+}
+
+// SceneParameters::GetMediumIndex (scene.hlsli:32-40): the first volume whose
+// object-space bounds hold the point.
+__device__ inline uint32_t medium_at(const DevScene& S, v3 pos) {
+    for (uint32_t i = 0; i < S.vinfo_count; i++) {
+        const st_volume_info vi = S.vinfo[i];
+        const v3 lp = xf_point(S.inv_xf[vi.instance_index], pos);
+        if (lp.x >= vi.min[0] && lp.y >= vi.min[1] && lp.z >= vi.min[2] && lp.x <= vi.max[0] && lp.y <= vi.max[1] &&
+            lp.z <= vi.max[2])
+            return vi.instance_index;
+    }
+    return kNoMedium;
+}
+
 // traceScatteringRay (intersection.hlsli:195-260) for every queued path of a
 // scene with media: closest hit over triangles and volume boxes, delta
 // tracking through the path's current medium up to that hit, medium
@@ -346,10 +372,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
         v3 thr = mk(t4.x, t4.y, t4.z);
         uint4 s4 = w.st[p];
         uint32_t med = w.med[p];
-        const uint32_t k = p / c.npix;
-        uint32_t px, py;
-        pixel_of(c, p % c.npix, px, py);
-        Rng rng{px, py, c.seed + c.sample_base + k, s4.x};
+        Rng rng = path_rng(c, p, s4.x);
         float dirpdf = 1;
         v3 o = origin;
         float tmax = __int_as_float(0x7f800000);
@@ -412,8 +435,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
 // any surface blocks, ratio tracking through each medium crossed; the NEE
 // contribution is divided by the average tracking pdf (testrenderer.slang:
 // 1153-1162).  The reference reads an uninitialised static path state here;
-// this uses the vertex's medium and its own random stream keyed by pixel,
-// sample and path length (offset 0x40000000 | length << 16).
+// this uses the vertex's medium and its own random stream keyed like the
+// path's (pixel or light dispatch index, sample) and by path length (offset
+// 0x40000000 | length << 16).  Light tracing keeps the light path in the
+// record's weight slot and adds with weight 1 to the pixel slot.
 __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, RenderConsts c, WaveState w, uint32_t* spill,
                                                               unsigned long long* stats) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
@@ -424,12 +449,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, Render
     for (uint32_t i = gtid; i < n; i += stride) {
         const float4 a = w.shadow[3 * (size_t)i], b = w.shadow[3 * (size_t)i + 1], cc = w.shadow[3 * (size_t)i + 2];
         const uint32_t p = __float_as_uint(b.w);
-        const uint32_t plen = w.st[p].y;
-        const uint32_t k = p / c.npix;
-        uint32_t px, py;
-        pixel_of(c, p % c.npix, px, py);
-        Rng rng{px, py, c.seed + c.sample_base + k, 0x40000000u | (plen << 16)};
-        uint32_t cur = w.med[p];
+        const uint32_t path = c.light_paths ? __float_as_uint(cc.w) : p;
+        const uint32_t plen = w.st[path].y;
+        Rng rng = path_rng(c, path, 0x40000000u | (plen << 16));
+        uint32_t cur = w.med[path];
         v3 beta = mk(cc.x, cc.y, cc.z);
         const v3 d = mk(b.x, b.y, b.z);
         v3 o = mk(a.x, a.y, a.z);
@@ -472,7 +495,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, Render
             }
             tmax -= h.t;
         }
-        if (!(all_nonpos(beta) || neepdf <= 0)) add_shadow_contribution(w, c.flags, p, beta / neepdf, cc.w);
+        if (!(all_nonpos(beta) || neepdf <= 0))
+            add_shadow_contribution(w, c.flags, p, beta / neepdf, c.light_paths ? 1.0f : cc.w);
     }
     wave_add_u64(reinterpret_cast<unsigned long long*>(w.counters + 4) + 1, segs);
     if (c.flags & ST_FLAG_COUNT_TRAVERSAL) wave_add_u64(&stats[2], reads);
@@ -966,12 +990,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
 
 // ------------------------------------------------------------ light tracing
 // Light path p of a wave is light subpath p % light_paths of sample
-// p / light_paths.  Its random stream is keyed by its dispatch index
-// (q % W, q / W) with offset 0xFFFF (testrenderer.slang:983-984, 1092-1094).
-__device__ __forceinline__ Rng light_rng(const RenderConsts& c, uint32_t p, uint32_t counter) {
-    const uint32_t q = p % c.light_paths;
-    return Rng{q % c.W, q / c.W, c.seed + c.sample_base + p / c.light_paths, counter};
-}
+// p / light_paths; its random stream starts at offset 0xFFFF (path_rng).
 
 // InitializeLightPath (testrenderer.slang:983-1024): SampleIllumination(rnd, 0)
 // picks the emitter; a triangle emits cosine-distributed about its geometry
@@ -993,7 +1012,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderCo
         const uint32_t p = i0 + __lane_id();
         bool live = false;
         if (p < n) {
-            Rng rng = light_rng(c, p, 0xFFFFu);
+            Rng rng = path_rng(c, p, 0xFFFFu);
             const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), mk(0, 0, 0));
             const float4 dr = rng.nextf();
             v3 thr = L.radiance / L.pdf, o, d;
@@ -1024,7 +1043,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderCo
                 w.ray[p] = make_float4(o.x, o.y, o.z, __uint_as_float(pack_normal(d)));
                 w.thr[p] = make_float4(thr.x, thr.y, thr.z, 0);
                 w.st[p] = make_uint4(rng.w, 0, 0, f2h(dirpdf));
-                w.med[p] = kNoMedium;
+                // a finite light's medium (GetMediumIndex); environment paths start outside
+                w.med[p] = (L.finite && S.vinst_count > 0) ? medium_at(S, L.pos) : kNoMedium;
             }
         }
         const uint32_t qi = wave_append(&w.counters[0], live);
@@ -1036,14 +1056,47 @@ __global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderCo
     }
 }
 
-// ProcessInteraction<true> (testrenderer.slang:810-935) at the surface vertices
-// of the queued light paths: a vertex that can scatter and is not singular
-// connects to the camera (ConnectToCamera, :441-524: the adjoint BSDF toward
-// the camera, weighted by the image-to-surface pdf conversion and divided by
-// the number of light paths; its shadow ray ends at the camera, unshortened,
-// and adds to the pixel it projects to), then Russian roulette and an adjoint
-// BSDF sample extend the path.  A miss ends a light path without emission.
-template <bool LAMBERT>
+// ConnectToCamera (testrenderer.slang:441-524).  camera_link projects a vertex
+// (false when the camera does not see it); camera_record takes the BSDF value f
+// toward the camera and |cos| cos_out there, converts image-plane to surface
+// density, divides by the number of light paths and fills the shadow record:
+// it ends at the camera, unshortened, and adds to the pixel the vertex projects
+// to, int(y) * W + int(x) (one beyond the image is dropped).
+struct CamLink {
+    float cos_cam;  // |cos| at the camera of the direction to the vertex
+    float pcx, pcy; // raster position
+};
+__device__ __forceinline__ bool camera_link(const RenderConsts& c, v3 pos, CamLink& l) {
+    const v3 pc = xf_point(c.inv_cam, pos);
+    l.pcx = l.pcy = 0;
+    if (!to_raster(c.view, pc, l.pcx, l.pcy)) return false;
+    l.cos_cam = fabsf(normalize(pc).z);
+    return true;
+}
+__device__ __forceinline__ bool camera_record(const RenderConsts& c, uint32_t p, v3 thr, v3 f, float cos_cam, float cos_out,
+                                              float dist, v3 origin, v3 tc, float pcx, float pcy, float4& sh0, float4& sh1,
+                                              float4& sh2) {
+    const float ipcd = c.ipd / cos_cam;
+    const float i2s = (sq(ipcd) / cos_cam) * cos_out / sq(dist);
+    const float s2i = 1.0f / i2s;
+    const v3 contrib = (thr * f) / ((float)c.light_paths * s2i * cos_out);
+    const uint32_t pix = (uint32_t)(int32_t)pcy * c.W + (uint32_t)(int32_t)pcx;
+    if (pix >= c.npix) return false;
+    const uint32_t slot = (p / c.light_paths) * c.npix + path_of_pixel(c, pix % c.W, pix / c.W);
+    sh0 = make_float4(origin.x, origin.y, origin.z, dist);
+    sh1 = make_float4(tc.x, tc.y, tc.z, __uint_as_float(slot));
+    sh2 = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(p));
+    return true;
+}
+
+// ProcessInteraction<true> (testrenderer.slang:810-935) at the vertices of the
+// queued light paths: a vertex that can scatter and is not singular connects
+// to the camera (the adjoint BSDF, or the phase function in a medium, toward
+// it), then Russian roulette and an adjoint BSDF (or phase) sample extend the
+// path.  A miss ends a light path without emission.  MEDIA: scenes with
+// volumes, whose scattering vertices k_trace_media reports as id
+// medium | 0xffff << 16 at the position it wrote into the ray origin.
+template <bool LAMBERT, bool MEDIA>
 __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderConsts c, WaveState w, int qsel) {
     const uint32_t n = w.counters[qsel];
     const uint32_t* q = w.queue[qsel];
@@ -1060,70 +1113,87 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
         if (i < n) {
             p = q[i];
             const uint4 hit = w.hit[i];
-            if (hit.y != 0xffffffffu) {
-                const float4 r = w.ray[p];
-                const float4 thr4 = w.thr[p];
+            const float4 r = w.ray[p];
+            const float4 thr4 = w.thr[p];
+            v3 thr = mk(thr4.x, thr4.y, thr4.z);
+            // ExtendPath (testrenderer.slang:1062-1063): tracking through media
+            // may have zeroed the throughput
+            if (hit.y != 0xffffffffu && (!MEDIA || !all_nonpos(thr))) {
                 uint4 s4 = w.st[p];
-                Rng rng = light_rng(c, p, s4.x);
+                Rng rng = path_rng(c, p, s4.x);
                 const uint32_t path_len = s4.y + 1;
                 const v3 dir = unpack_normal(__float_as_uint(r.w));
-                v3 thr = mk(thr4.x, thr4.y, thr4.z);
-                Shading sd;
-                tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
-                const Material m = load_material(S, sd.material);
-                const Frame f = make_frame(sd);
-                const v3 ldin = f.to_local(-dir);
-                const v3 lgn = f.to_local(unpack_normal(sd.gn));
-                const v3 gn = unpack_normal(sd.gn);
-                bool alive = path_len < c.max_depth && any_pos(m.base());
                 const bool rr_on = path_len >= c.rr_start;
-                if (alive && !m_singular<LAMBERT>(m) && path_len + 1 >= c.min_len) {
-                    const v3 pc = xf_point(c.inv_cam, sd.pos);
-                    float pcx = 0, pcy = 0;
-                    if (to_raster(c.view, pc, pcx, pcy)) {
-                        const float cos_cam = fabsf(normalize(pc).z);
+                if (MEDIA && (hit.y >> 16) == 0xffffu) {
+                    const Medium m = load_medium(S, S.inst[hit.y & 0xffffu].type_material_address >> 4);
+                    const v3 pos = mk(r.x, r.y, r.z);
+                    const v3 ldin = -dir;  // media work in world space
+                    bool alive = path_len < c.max_depth && m.can_eval();
+                    CamLink l;
+                    if (alive && !m.singular() && path_len + 1 >= c.min_len && camera_link(c, pos, l)) {
+                        v3 tc = cam - pos;
+                        const float dist = len(tc);
+                        tc = tc / dist;
+                        const float v = hg_eval(m.g, ldin, tc);
+                        // rayOffset of a medium vertex is the vertex itself (intersection.hlsli:33-35)
+                        emit_shadow = camera_record(c, p, thr, mk(v, v, v), l.cos_cam, 1.0f, dist, pos, tc, l.pcx, l.pcy,
+                                                    sh0, sh1, sh2);
+                    }
+                    if (alive) {
+                        const float cp = fmin_(1.0f, m.cont_prob());
+                        const float rr = rng.nextf().x;
+                        if (rr_on && rr > cp) alive = false;
+                        if (alive) {
+                            const float4 rb = rng.nextf();
+                            const v3 sdir = hg_sample(m.g, rb.x, rb.y, ldin);
+                            const float v = hg_eval(m.g, ldin, sdir);
+                            const float fwd = v * (rr_on ? cp : 1.0f);
+                            thr = thr * (v / fwd);
+                            const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
+                            cont = !nan && !all_nonpos(thr) && path_len + 1 <= c.max_depth;
+                            ray_out = make_float4(pos.x, pos.y, pos.z, __uint_as_float(pack_normal(sdir)));
+                            thr_out = make_float4(thr.x, thr.y, thr.z, 0);
+                        }
+                    }
+                } else {
+                    Shading sd;
+                    tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
+                    const Material m = load_material(S, sd.material);
+                    const Frame f = make_frame(sd);
+                    const v3 ldin = f.to_local(-dir);
+                    const v3 lgn = f.to_local(unpack_normal(sd.gn));
+                    const v3 gn = unpack_normal(sd.gn);
+                    bool alive = path_len < c.max_depth && any_pos(m.base());
+                    CamLink l;
+                    if (alive && !m_singular<LAMBERT>(m) && path_len + 1 >= c.min_len && camera_link(c, sd.pos, l)) {
                         v3 tc = cam - sd.pos;
                         const float dist = len(tc);
                         tc = tc / dist;
                         const v3 ldout = f.to_local(tc);
-                        const float cos_out = fabsf(ldout.z);
                         Eval fe = bsdf_eval<LAMBERT, true>(m, ldin, ldout);
                         fe.f = fe.f * shading_normal_correction<true>(f, lgn, ldin, ldout);
-                        const float ipcd = c.ipd / cos_cam;
-                        const float i2s = (sq(ipcd) / cos_cam) * cos_out / sq(dist);
-                        const float s2i = 1.0f / i2s;
-                        const v3 contrib = (thr * fe.f) / ((float)c.light_paths * s2i * cos_out);
-                        // pixel index int(y) * W + int(x); one beyond the image is dropped
-                        const uint32_t pix = (uint32_t)(int32_t)pcy * c.W + (uint32_t)(int32_t)pcx;
-                        if (pix < c.npix) {
-                            const uint32_t slot = (p / c.light_paths) * c.npix + path_of_pixel(c, pix % c.W, pix / c.W);
-                            const v3 o = ray_offset(sd.pos, gn, tc);
-                            sh0 = make_float4(o.x, o.y, o.z, dist);
-                            sh1 = make_float4(tc.x, tc.y, tc.z, __uint_as_float(slot));
-                            sh2 = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(p));
-                            emit_shadow = true;
-                        }
+                        emit_shadow = camera_record(c, p, thr, fe.f, l.cos_cam, fabsf(ldout.z), dist,
+                                                    ray_offset(sd.pos, gn, tc), tc, l.pcx, l.pcy, sh0, sh1, sh2);
                     }
-                }
-                if (alive) {
-                    const float cp = fmin_(1.0f, m_cont_prob(m));
-                    const float rr = rng.nextf().x;
-                    if (rr_on && rr > cp) alive = false;
                     if (alive) {
-                        const float cpx = rr_on ? cp : 1.0f;
-                        const float4 rb = rng.nextf();
-                        Sample sm = bsdf_sample<LAMBERT, true>(m, rb.x, rb.y, rb.z, ldin);
-                        sm.fwd *= cpx;
-                        sm.rev *= cpx;
-                        thr = thr * (sm.f / sm.fwd);
-                        thr = thr * shading_normal_correction<true>(f, lgn, ldin, sm.dir);
-                        const v3 wd = f.to_world(sm.dir);
-                        const v3 o = ray_offset(sd.pos, gn, wd);
-                        const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
-                        cont = !nan && !all_nonpos(thr) && path_len + 1 <= c.max_depth;
-                        ray_out = make_float4(o.x, o.y, o.z, __uint_as_float(pack_normal(wd)));
-                        thr_out = make_float4(thr.x, thr.y, thr.z, 0);
-                        s4.w = f2h(sm.rough == 0 ? 0.0f : sm.fwd);
+                        const float cp = fmin_(1.0f, m_cont_prob(m));
+                        const float rr = rng.nextf().x;
+                        if (rr_on && rr > cp) alive = false;
+                        if (alive) {
+                            const float cpx = rr_on ? cp : 1.0f;
+                            const float4 rb = rng.nextf();
+                            Sample sm = bsdf_sample<LAMBERT, true>(m, rb.x, rb.y, rb.z, ldin);
+                            sm.fwd *= cpx;
+                            sm.rev *= cpx;
+                            thr = thr * (sm.f / sm.fwd);
+                            thr = thr * shading_normal_correction<true>(f, lgn, ldin, sm.dir);
+                            const v3 wd = f.to_world(sm.dir);
+                            const v3 o = ray_offset(sd.pos, gn, wd);
+                            const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
+                            cont = !nan && !all_nonpos(thr) && path_len + 1 <= c.max_depth;
+                            ray_out = make_float4(o.x, o.y, o.z, __uint_as_float(pack_normal(wd)));
+                            thr_out = make_float4(thr.x, thr.y, thr.z, 0);
+                        }
                     }
                 }
                 s4.x = rng.w;
@@ -1695,6 +1765,8 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
         for (uint32_t i = 0; i < d->instance_count; i++)
             if ((d->instances[i].type_material_address & 0xfu) == 2) D.vinst[D.vinst_count++] = i;
         for (uint32_t i = 0; i < d->volume_info_count; i++) s->volume_info.push_back(d->volume_info[i]);
+        D.vinfo = s->upload<st_volume_info>(s->volume_info.data(), sizeof(st_volume_info) * s->volume_info.size());
+        D.vinfo_count = (uint32_t)s->volume_info.size();
         std::vector<float> verts(9 * ntri + 9);
         std::vector<uint32_t> ids(ntri + 1), owner(ntri + 1, 0);
         s->boxes.resize(d->instance_count);
@@ -2148,7 +2220,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
 // fixed point, then the same resolve as camera paths.
 static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
-    if (s->d.vinst_count > 0) return set_error(ST_ERROR_UNSUPPORTED, "light tracing is not supported in scenes with media");
+    if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
     if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "light_subpath_factor must be in [0, 64]");
     st_context* ctx = s->ctx;
@@ -2213,12 +2285,14 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         c.view = *view;
         c.inv_cam = inverse_transform(view->camera_to_world);
         c.view_medium = kNoMedium;
+        c.max_null = p->max_null_collisions;
         c.light_paths = L;
         // ViewData::imagePlaneDist, compat/scene.h:147
         c.ipd = (float)std::abs(view->image_max[1] - view->image_min[1]) / (2 * std::tan(view->projection.vertical_fov / 2));
         c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
         const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
+        const bool media = s->d.vinst_count > 0;
         auto* stats = ctx->stats.as<unsigned long long>();
         for (uint32_t base = 0; base < p->spp; base += k) {
             c.k = std::min(k, p->spp - base);
@@ -2232,20 +2306,23 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
             for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
                 {
                     Timed tm(ctx, 0);
-                    if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                    else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
                     else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
                 }
                 {
                     Timed tm(ctx, 1);
-                    if (lambert) hipLaunchKernelGGL(k_shade_light<true>, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
-                    else hipLaunchKernelGGL(k_shade_light<false>, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
+                    auto kfn = media ? (lambert ? k_shade_light<true, true> : k_shade_light<false, true>)
+                                     : (lambert ? k_shade_light<true, false> : k_shade_light<false, false>);
+                    hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
                 }
                 {
                     Timed tm(ctx, 2);
-                    if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
+                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, ctx->spill.as<uint32_t>(), stats);
+                    else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
                     else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
                 }
-                hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, 0);
+                hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
                 qsel ^= 1;
             }
             const bool last = base + c.k >= p->spp;

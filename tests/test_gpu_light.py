@@ -82,12 +82,33 @@ def test_light_tracing_independent_of_samples_per_wave(gpu_ctx):
     assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
 
 
-def test_light_tracing_rejects_media_and_bad_factor(gpu_ctx):
+def test_light_tracing_rejects_bad_factor(gpu_ctx):
     from stratum2_amd.renderer import Scene, TestRenderer
     sc = Scene(gpu_ctx, scenes.cornell_box(0))
-    with pytest.raises(RuntimeError):
+    with pytest.raises(RuntimeError, match="light_subpath_factor"):
         TestRenderer(sc).render(scenes.cornell_view(16, 16), abi.default_params(16, 16, flags=FL, light_subpath_factor=-1.0))
+
+
+def test_light_tracing_medium_around_the_light(gpu_ctx):
+    """Light paths start inside a homogeneous medium (GetMediumIndex), scatter
+    in it and connect to the camera through it (ratio tracking)."""
     sd = scenes.cornell_box(0)
-    scenes.add_medium_box(sd, (2.0, 2.0, 2.0), (0.8, 0.8, 0.8), 0.0, (-0.5, 0.2, -0.5), (0.5, 1.2, 0.5))
-    with pytest.raises(RuntimeError, match="media"):
-        TestRenderer(Scene(gpu_ctx, sd)).render(scenes.cornell_view(16, 16), abi.default_params(16, 16, flags=FL))
+    scenes.add_medium_box(sd, (1.5, 1.5, 1.5), (0.8, 0.8, 0.8), 0.3, (-0.7, 0.9, -0.7), (0.7, 1.99, 0.7))
+    W = H = 64
+    params = abi.default_params(W, H, max_depth=5, spp=2, flags=FL | abi.ST_FLAG_LAMBERTIAN)
+    g, r = _both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
+    assert r["hdr"][..., :3].mean() > 0.01
+    _check(g, r, frac_px=0.98)
+
+
+def test_light_tracing_emitters_in_a_medium(gpu_ctx):
+    """Emissive spheres inside a large thin medium box: light paths start in
+    the medium, scatter in it and among the spheres."""
+    sd = scenes.spheres_field(seed=2, n_spheres=40, n_emitters=8, env=False)
+    scenes.add_medium_box(sd, (0.3, 0.3, 0.3), (0.9, 0.9, 0.9), 0.5, (-6, 0, -6), (6, 4, 6))
+    W, H = 96, 64
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 14.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    params = abi.default_params(W, H, max_depth=4, spp=2, flags=FL)
+    g, r = _both(gpu_ctx, sd, view, params)
+    assert r["hdr"][..., :3].mean() > 0
+    _check(g, r, frac_px=0.98)
