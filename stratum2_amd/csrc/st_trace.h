@@ -25,36 +25,41 @@ struct Hit {
 
 struct WRay {  // Woop-Benthin-Wald shear constants, JCGT 2(1) 2013
     v3 o;
-    int kx, ky, kz;
+    uint32_t k;  // kx | ky << 2 | kz << 4
     float Sx, Sy, Sz;
+    __device__ __forceinline__ int kx() const { return (int)(k & 3u); }
+    __device__ __forceinline__ int ky() const { return (int)((k >> 2) & 3u); }
+    __device__ __forceinline__ int kz() const { return (int)(k >> 4); }
 };
 
 __device__ __forceinline__ WRay wray_make(v3 o, v3 d) {
     WRay r;
     r.o = o;
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-    r.kx = r.kz == 2 ? 0 : r.kz + 1;
-    r.ky = r.kx == 2 ? 0 : r.kx + 1;
-    if (comp(d, r.kz) < 0) {
-        int t = r.kx;
-        r.kx = r.ky;
-        r.ky = t;
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    if (comp(d, kz) < 0) {
+        int t = kx;
+        kx = ky;
+        ky = t;
     }
-    float dz = comp(d, r.kz);
-    r.Sx = comp(d, r.kx) / dz;
-    r.Sy = comp(d, r.ky) / dz;
+    float dz = comp(d, kz);
+    r.Sx = comp(d, kx) / dz;
+    r.Sy = comp(d, ky) / dz;
     r.Sz = 1.0f / dz;
+    r.k = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
     return r;
 }
 
 // Watertight test; t > 0 required; b1/b2 are the weights of v1/v2.
 __device__ __forceinline__ bool wt_intersect(const WRay& r, v3 v0, v3 v1, v3 v2, float& t, float& b1, float& b2) {
     v3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
-    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
-    float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
-    float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
-    float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+    const int kx = r.kx(), ky = r.ky(), kz = r.kz();
+    float Akz = comp(A, kz), Bkz = comp(B, kz), Ckz = comp(C, kz);
+    float Ax = comp(A, kx) - r.Sx * Akz, Ay = comp(A, ky) - r.Sy * Akz;
+    float Bx = comp(B, kx) - r.Sx * Bkz, By = comp(B, ky) - r.Sy * Bkz;
+    float Cx = comp(C, kx) - r.Sx * Ckz, Cy = comp(C, ky) - r.Sy * Ckz;
     float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
         U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
@@ -91,9 +96,9 @@ struct TraverseStats {
     uint32_t nodes, tris;
 };
 
-// Traversal state of one lane.  step() visits one node: both child boxes,
-// leaf children intersected immediately, nearer inner child next, farther
-// one pushed.  Returns false when the ray is finished.
+// Traversal state of one lane.  step() visits one 4-wide node: its leaf
+// children are intersected at once, the nearest hit inner child is next and
+// the others are pushed.  Returns false when the ray is finished.
 struct Lane {
     WRay wr;
     v3 inv;
@@ -118,7 +123,7 @@ struct Lane {
         float t, b1, b2;
         if (!wt_intersect(wr, mk(t0.x, t0.y, t0.z), mk(t1.x, t1.y, t1.z), mk(t2.x, t2.y, t2.z), t, b1, b2)) return false;
         const uint32_t id = __float_as_uint(t0.w);
-        if (t < h.t || (t == h.t && h.hit && id < h.id)) {
+        if (t < h.t || (t == h.t && h.id != 0xffffffffu && id < h.id)) {
             h.t = t;
             h.b1 = b1;
             h.b2 = b2;
@@ -152,9 +157,10 @@ struct Lane {
         const uint32_t qlx = __float_as_uint(w1.x), qly = __float_as_uint(w1.y), qlz = __float_as_uint(w1.z);
         const uint32_t qhx = __float_as_uint(w1.w), qhy = __float_as_uint(w2.x), qhz = __float_as_uint(w2.y);
         const v3 o = wr.o;
-        float tk[4];
-        uint32_t ck[4];
-        uint32_t tri_off = 0, inner_rank = 0, trimask = 0;
+        // sort keys: entry distance bits (non-negative floats order as integers)
+        // with the child slot in the low 2 bits; empty = all ones
+        uint32_t key[4];
+        uint32_t tri_off = 0, innermask = 0, trimask = 0;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const uint32_t type = (types >> (3 * c)) & 7u;
@@ -164,14 +170,10 @@ struct Lane {
             const float lz = w0.z + (float)((qlz >> sh) & 0xffu) * sz, hz = w0.z + (float)((qhz >> sh) & 0xffu) * sz;
             float tn;
             const bool hit = type != 0 && slab(lx, hx, ly, hy, lz, hz, o, inv, h.t, tn);
-            tk[c] = __int_as_float(0x7f800000);
-            ck[c] = 0;
+            key[c] = 0xffffffffu;
             if (type == 7) {
-                if (hit) {
-                    tk[c] = tn;
-                    ck[c] = base_inner + inner_rank;
-                }
-                inner_rank++;
+                if (hit) key[c] = (__float_as_uint(tn) & ~3u) | (uint32_t)c;
+                innermask |= 1u << c;
             } else {
                 // leaf: mark its triangles (offsets tri_off .. tri_off + type - 1, at most 16 per node)
                 if (hit) trimask |= ((1u << type) - 1u) << tri_off;
@@ -188,22 +190,19 @@ struct Lane {
         }
         // sort the (at most 4) inner candidates by entry distance
 #define ST_CSWAP(i, j)                                   \
-    if (tk[j] < tk[i]) {                                 \
-        const float tt = tk[i];                          \
-        tk[i] = tk[j];                                   \
-        tk[j] = tt;                                      \
-        const uint32_t cc = ck[i];                       \
-        ck[i] = ck[j];                                   \
-        ck[j] = cc;                                      \
+    {                                                    \
+        const uint32_t lo = min(key[i], key[j]);         \
+        key[j] = max(key[i], key[j]);                    \
+        key[i] = lo;                                     \
     }
         ST_CSWAP(0, 1) ST_CSWAP(2, 3) ST_CSWAP(0, 2) ST_CSWAP(1, 3) ST_CSWAP(1, 2)
 #undef ST_CSWAP
-        const float inf = __int_as_float(0x7f800000);
-        if (tk[3] < inf) push(ck[3], lds, spill, spill_stride, gtid);
-        if (tk[2] < inf) push(ck[2], lds, spill, spill_stride, gtid);
-        if (tk[1] < inf) push(ck[1], lds, spill, spill_stride, gtid);
-        if (tk[0] < inf) {
-            node = ck[0];
+        auto child = [&](uint32_t k) { return base_inner + (uint32_t)__popc(innermask & ((1u << (k & 3u)) - 1u)); };
+        if (key[3] != 0xffffffffu) push(child(key[3]), lds, spill, spill_stride, gtid);
+        if (key[2] != 0xffffffffu) push(child(key[2]), lds, spill, spill_stride, gtid);
+        if (key[1] != 0xffffffffu) push(child(key[1]), lds, spill, spill_stride, gtid);
+        if (key[0] != 0xffffffffu) {
+            node = child(key[0]);
             return true;
         }
         if (sp == 0) return false;
