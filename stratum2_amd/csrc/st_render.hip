@@ -1380,12 +1380,23 @@ struct DevBuf {
 
 }  // namespace
 
+// Device state of one wave of paths.  Two waves can be in flight at once, each
+// in its own slot and on its own stream, so that one wave's last bounces (few
+// rays, bound by the slowest ray) overlap the next wave's first bounces.
+struct WaveBufs {
+    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, pri, rinit, rtemp, med;
+    hipStream_t stream = nullptr;        // slot 0: the context's stream
+    hipEvent_t reuse_done = nullptr;     // this slot's last wave finished its screen-space reuse
+    hipEvent_t resolve_done = nullptr;   // this slot's last wave finished its resolve
+};
+
 struct st_context {
     int device = 0;
     int cus = 256;
     hipStream_t stream = nullptr;
-    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, acc, stats, raysbuf, outbuf, pri, rinit, rtemp, rprev, med,
-        tmax;
+    WaveBufs slot[2];
+    hipEvent_t start_done = nullptr;     // the frame's clears on the context stream
+    DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
     double timings[12] = {0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
     size_t ev_used[3] = {0, 0, 0};
@@ -1444,9 +1455,12 @@ hipEvent_t get_event(st_context* ctx, int kind, bool start) {
 struct Timed {
     st_context* ctx;
     int kind;
-    Timed(st_context* c, int k) : ctx(c), kind(k) { HIP_CHECK(hipEventRecord(get_event(c, k, true), c->stream)); }
+    hipStream_t stream;
+    Timed(st_context* c, int k, hipStream_t s) : ctx(c), kind(k), stream(s) {
+        HIP_CHECK(hipEventRecord(get_event(c, k, true), s));
+    }
     ~Timed() {
-        (void)hipEventRecord(get_event(ctx, kind, false), ctx->stream);
+        (void)hipEventRecord(get_event(ctx, kind, false), stream);
         ctx->ev_used[kind]++;
     }
 };
@@ -1669,6 +1683,14 @@ int st_context_create(int device, st_context** out) {
         HIP_CHECK(hipGetDeviceProperties(&prop, device));
         ctx->cus = prop.multiProcessorCount;
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        // a second stream of the same (default) priority for the second wave slot
+        HIP_CHECK(hipStreamCreateWithFlags(&ctx->slot[1].stream, hipStreamNonBlocking));
+        ctx->slot[0].stream = ctx->stream;
+        for (WaveBufs& b : ctx->slot) {
+            HIP_CHECK(hipEventCreateWithFlags(&b.reuse_done, hipEventDisableTiming));
+            HIP_CHECK(hipEventCreateWithFlags(&b.resolve_done, hipEventDisableTiming));
+        }
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->start_done, hipEventDisableTiming));
         *out = ctx.release();
         return ST_OK;
     } catch (const std::exception& e) {
@@ -1680,11 +1702,18 @@ int st_context_destroy(st_context* ctx) {
     if (!ctx) return ST_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->slot[1].stream) (void)hipStreamSynchronize(ctx->slot[1].stream);
     for (auto& v : ctx->ev)
         for (auto& e : v) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
+    for (WaveBufs& b : ctx->slot) {
+        if (b.reuse_done) (void)hipEventDestroy(b.reuse_done);
+        if (b.resolve_done) (void)hipEventDestroy(b.resolve_done);
+    }
+    if (ctx->start_done) (void)hipEventDestroy(ctx->start_done);
+    if (ctx->slot[1].stream) (void)hipStreamDestroy(ctx->slot[1].stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return ST_OK;
@@ -2028,8 +2057,77 @@ static ShadeFn shade_kernel(bool lambert, bool alias, bool restir, bool media) {
     return table[i];
 }
 
-static void ensure_spill(st_context* ctx, int grid, int block) {
-    ctx->spill.reserve((size_t)(kMaxStack - kLdsStack + 1) * grid * block * 4);
+static void ensure_spill(WaveBufs& b, int grid, int block) {
+    b.spill.reserve((size_t)(kMaxStack - kLdsStack + 1) * grid * block * 4);
+}
+
+// Buffers of one wave slot for `paths` paths (reuse: k samples of npix pixels).
+static void reserve_wave(WaveBufs& b, uint64_t paths, bool reuse, uint64_t npix_k, int tgrid) {
+    b.ray.reserve(paths * 16);
+    b.thr.reserve(paths * 16);
+    b.stt.reserve(paths * 16);
+    b.em.reserve(paths * 16);
+    b.nee.reserve(paths * 16);
+    b.hit.reserve(paths * 16);
+    b.q0.reserve(paths * 4);
+    b.q1.reserve(paths * 4);
+    b.shadow.reserve(paths * 48);
+    b.med.reserve(paths * 4);
+    b.counters.reserve(256);
+    if (reuse) {
+        b.pri.reserve(npix_k * 16);
+        b.rinit.reserve(npix_k * 32);
+        b.rtemp.reserve(npix_k * 32);
+    }
+    ensure_spill(b, tgrid, kTraceBlock);
+}
+
+static WaveState wave_state(WaveBufs& b, st_context* ctx) {
+    WaveState w;
+    std::memset(&w, 0, sizeof w);
+    w.ray = b.ray.as<float4>();
+    w.thr = b.thr.as<float4>();
+    w.st = b.stt.as<uint4>();
+    w.em = b.em.as<float4>();
+    w.nee = b.nee.as<uint4>();
+    w.hit = b.hit.as<uint4>();
+    w.queue[0] = b.q0.as<uint32_t>();
+    w.queue[1] = b.q1.as<uint32_t>();
+    w.shadow = b.shadow.as<float4>();
+    w.counters = b.counters.as<uint32_t>();
+    w.pri = b.pri.as<uint4>();
+    w.r_init = b.rinit.as<float4>();
+    w.r_temp = b.rtemp.as<float4>();
+    w.r_prev = ctx->rprev.as<float4>();
+    w.med = b.med.as<uint32_t>();
+    return w;
+}
+
+// Per-class device times of the last render (HIP events), traversal counters
+// and the rays of both wave slots; writes the ray counts to `ray_counts`.
+static void finish_timings(st_context* ctx, int slots, uint64_t* ray_counts) {
+    for (int kind = 0; kind < 3; kind++) {
+        double ms = 0;
+        for (size_t i = 0; i < ctx->ev_used[kind]; i++) {
+            float t = 0;
+            HIP_CHECK(hipEventElapsedTime(&t, ctx->ev[kind][i].first, ctx->ev[kind][i].second));
+            ms += t;
+        }
+        ctx->timings[kind] = ms;
+        ctx->timings[3 + kind] = (double)ctx->ev_used[kind];
+    }
+    unsigned long long hs[4], rc[2] = {0, 0};
+    HIP_CHECK(hipMemcpy(hs, ctx->stats.p, 32, hipMemcpyDeviceToHost));
+    for (int sl = 0; sl < slots; sl++) {
+        unsigned long long r[2];
+        HIP_CHECK(hipMemcpy(r, ctx->slot[sl].counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToHost));
+        rc[0] += r[0];
+        rc[1] += r[1];
+    }
+    for (int i = 0; i < 4; i++) ctx->timings[6 + i] = (double)hs[i];
+    ctx->timings[10] = (double)rc[0];
+    ctx->timings[11] = (double)rc[1];
+    if (ray_counts) HIP_CHECK(hipMemcpy(ray_counts, rc, 16, hipMemcpyHostToDevice));
 }
 
 static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o);
@@ -2059,47 +2157,19 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         k = std::min(k, p->spp);
         const uint64_t paths = (uint64_t)npix * k;
         if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
-        ctx->ray.reserve(paths * 16);
-        ctx->thr.reserve(paths * 16);
-        ctx->stt.reserve(paths * 16);
-        ctx->em.reserve(paths * 16);
-        ctx->nee.reserve(paths * 16);
-        ctx->hit.reserve(paths * 16);
-        ctx->q0.reserve(paths * 4);
-        ctx->q1.reserve(paths * 4);
-        ctx->shadow.reserve(paths * 48);
-        ctx->med.reserve(paths * 4);
-        ctx->counters.reserve(256);
-        ctx->acc.reserve((size_t)npix * 16);
-        if (reuse) {
-            ctx->pri.reserve((size_t)npix * k * 16);
-            ctx->rinit.reserve((size_t)npix * k * 32);
-            ctx->rtemp.reserve((size_t)npix * k * 32);
-            ctx->rprev.reserve((size_t)npix * 32);
-        }
-        ctx->stats.reserve(64);
+        const uint32_t waves = (p->spp + k - 1) / k;
+        const int slots = waves > 1 ? 2 : 1;
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
-        ensure_spill(ctx, tgrid, kTraceBlock);
-        WaveState w;
-        w.ray = ctx->ray.as<float4>();
-        w.thr = ctx->thr.as<float4>();
-        w.st = ctx->stt.as<uint4>();
-        w.em = ctx->em.as<float4>();
-        w.nee = ctx->nee.as<uint4>();
-        w.hit = ctx->hit.as<uint4>();
-        w.queue[0] = ctx->q0.as<uint32_t>();
-        w.queue[1] = ctx->q1.as<uint32_t>();
-        w.shadow = ctx->shadow.as<float4>();
-        w.counters = ctx->counters.as<uint32_t>();
-        w.pri = ctx->pri.as<uint4>();
-        w.r_init = ctx->rinit.as<float4>();
-        w.r_temp = ctx->rtemp.as<float4>();
-        w.r_prev = ctx->rprev.as<float4>();
-        w.med = ctx->med.as<uint32_t>();
-        hipStream_t sm = ctx->stream;
-        HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 256, sm));
-        HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, sm));
-        HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, sm));
+        for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, reuse, paths, tgrid);
+        ctx->acc.reserve((size_t)npix * 16);
+        if (reuse) ctx->rprev.reserve((size_t)npix * 32);
+        ctx->stats.reserve(64);
+        hipStream_t s0 = ctx->stream;
+        HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, s0));
+        HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
+        HIP_CHECK(hipEventRecord(ctx->start_done, s0));
+        if (slots > 1) HIP_CHECK(hipStreamWaitEvent(ctx->slot[1].stream, ctx->start_done, 0));
         for (int kk = 0; kk < 3; kk++) ctx->ev_used[kk] = 0;
         RenderConsts c;
         std::memset(&c, 0, sizeof c);
@@ -2140,7 +2210,17 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         c.radius = p->spatial_radius;
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
         auto* stats = ctx->stats.as<unsigned long long>();
-        for (uint32_t base = 0; base < p->spp; base += k) {
+        // Wave i runs in slot i % 2 on that slot's stream; the slot's previous
+        // wave (i - 2) finished before it on the same stream.  Across slots,
+        // two orders matter: the screen-space reuse of wave i reads the final
+        // reservoirs of wave i - 1, and resolves add samples in sample order.
+        uint32_t wi = 0;
+        for (uint32_t base = 0; base < p->spp; base += k, wi++) {
+            WaveBufs& B = ctx->slot[wi % slots];
+            WaveBufs& prev = ctx->slot[(wi + 1) % slots];
+            hipStream_t sm = B.stream;
+            WaveState w = wave_state(B, ctx);
+            uint32_t* spill = B.spill.as<uint32_t>();
             c.k = std::min(k, p->spp - base);
             c.sample_base = base;
             c.wave_paths = npix * c.k;
@@ -2149,13 +2229,13 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
             int qsel = 0;
             for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
                 {
-                    Timed tm(ctx, 0);
-                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, ctx->spill.as<uint32_t>(), stats);
-                    else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
-                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                    Timed tm(ctx, 0, sm);
+                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats);
+                    else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                 }
                 {
-                    Timed tm(ctx, 1);
+                    Timed tm(ctx, 1, sm);
                     AuxOut aux{nullptr, nullptr, nullptr, nullptr};
                     if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
                     auto* kfn = shade_kernel(lambert, alias, restir, media);
@@ -2167,57 +2247,40 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                                       : (alias ? k_restir_temporal<false, true> : k_restir_temporal<false, false>);
                     auto ks = lambert ? (alias ? k_restir_spatial<true, true> : k_restir_spatial<true, false>)
                                       : (alias ? k_restir_spatial<false, true> : k_restir_spatial<false, false>);
+                    if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.reuse_done, 0));
                     // temporal reuse chains consecutive samples: sample j of the wave
                     // reuses sample j - 1's final reservoirs
                     for (uint32_t j = 0; j < c.k; j++) {
                         hipLaunchKernelGGL(kt, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, (base + j) > 0 ? 1 : 0, j);
                         hipLaunchKernelGGL(ks, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, j);
                     }
+                    HIP_CHECK(hipEventRecord(B.reuse_done, sm));
                 }
                 {
-                    Timed tm(ctx, 2);
-                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, ctx->spill.as<uint32_t>(), stats);
-                    else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
-                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
+                    Timed tm(ctx, 2, sm);
+                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats);
+                    else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
+                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
                 }
                 hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
                 qsel ^= 1;
             }
             const bool last = base + c.k >= p->spp;
             const int rgrid = std::min<int>((npix + 255) / 256, ctx->cus * 8);
+            if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.resolve_done, 0));
             hipLaunchKernelGGL(k_resolve, dim3(rgrid), dim3(256), 0, sm, c, w, ctx->acc.as<float4>(), o->hdr,
                                o->path_length, 1.0f / (float)p->spp, last ? 1 : 0);
+            HIP_CHECK(hipEventRecord(B.resolve_done, sm));
             HIP_CHECK(hipGetLastError());
         }
-        if (o->ray_counts)
-            HIP_CHECK(hipMemcpyAsync(o->ray_counts, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToDevice, sm));
-        HIP_CHECK(hipStreamSynchronize(sm));
-        // per-kernel-class device time (HIP events on this stream)
-        for (int kind = 0; kind < 3; kind++) {
-            double ms = 0;
-            for (size_t i = 0; i < ctx->ev_used[kind]; i++) {
-                float t = 0;
-                HIP_CHECK(hipEventElapsedTime(&t, ctx->ev[kind][i].first, ctx->ev[kind][i].second));
-                ms += t;
-            }
-            ctx->timings[kind] = ms;
-            ctx->timings[3 + kind] = (double)ctx->ev_used[kind];
-        }
-        unsigned long long hs[4], rc[2];
-        HIP_CHECK(hipMemcpy(hs, ctx->stats.p, 32, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(rc, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToHost));
-        for (int i = 0; i < 4; i++) ctx->timings[6 + i] = (double)hs[i];
-        ctx->timings[10] = (double)rc[0];
-        ctx->timings[11] = (double)rc[1];
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
+        finish_timings(ctx, slots, o->ray_counts);
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
     }
 }
 
-// Light tracing (TestRenderer with mLightTrace): light subpaths of every sample
-// in waves, connections to the camera accumulated per pixel and sample in 1/16384
-// fixed point, then the same resolve as camera paths.
 static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
     if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
@@ -2235,39 +2298,18 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         k = std::min(k, p->spp);
         const uint64_t paths = (uint64_t)span * k;
         if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
-        ctx->ray.reserve(paths * 16);
-        ctx->thr.reserve(paths * 16);
-        ctx->stt.reserve(paths * 16);
-        ctx->em.reserve(paths * 16);
-        ctx->nee.reserve(paths * 16);
-        ctx->hit.reserve(paths * 16);
-        ctx->q0.reserve(paths * 4);
-        ctx->q1.reserve(paths * 4);
-        ctx->shadow.reserve(paths * 48);
-        ctx->med.reserve(paths * 4);
-        ctx->counters.reserve(256);
+        const uint32_t waves = (p->spp + k - 1) / k;
+        const int slots = waves > 1 ? 2 : 1;
+        const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
+        for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, false, 0, tgrid);
         ctx->acc.reserve((size_t)npix * 16);
         ctx->stats.reserve(64);
-        const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
-        ensure_spill(ctx, tgrid, kTraceBlock);
-        WaveState w;
-        std::memset(&w, 0, sizeof w);
-        w.ray = ctx->ray.as<float4>();
-        w.thr = ctx->thr.as<float4>();
-        w.st = ctx->stt.as<uint4>();
-        w.em = ctx->em.as<float4>();
-        w.nee = ctx->nee.as<uint4>();
-        w.hit = ctx->hit.as<uint4>();
-        w.queue[0] = ctx->q0.as<uint32_t>();
-        w.queue[1] = ctx->q1.as<uint32_t>();
-        w.shadow = ctx->shadow.as<float4>();
-        w.counters = ctx->counters.as<uint32_t>();
-        w.med = ctx->med.as<uint32_t>();
-        hipStream_t sm = ctx->stream;
-        HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 256, sm));
-        HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, sm));
-        HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, sm));
-        for (int kk = 0; kk < 3; kk++) ctx->ev_used[kk] = 0;
+        hipStream_t s0 = ctx->stream;
+        HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, s0));
+        HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
+        HIP_CHECK(hipEventRecord(ctx->start_done, s0));
+        if (slots > 1) HIP_CHECK(hipStreamWaitEvent(ctx->slot[1].stream, ctx->start_done, 0));
         RenderConsts c;
         std::memset(&c, 0, sizeof c);
         c.W = p->width;
@@ -2294,62 +2336,54 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
         const bool media = s->d.vinst_count > 0;
         auto* stats = ctx->stats.as<unsigned long long>();
-        for (uint32_t base = 0; base < p->spp; base += k) {
+        // waves alternate between the two slots and streams; resolves keep sample order
+        uint32_t wi = 0;
+        for (uint32_t base = 0; base < p->spp; base += k, wi++) {
+            WaveBufs& B = ctx->slot[wi % slots];
+            WaveBufs& prev = ctx->slot[(wi + 1) % slots];
+            hipStream_t sm = B.stream;
+            WaveState w = wave_state(B, ctx);
+            uint32_t* spill = B.spill.as<uint32_t>();
             c.k = std::min(k, p->spp - base);
             c.sample_base = base;
             c.wave_paths = L * c.k;
-            HIP_CHECK(hipMemsetAsync(ctx->counters.p, 0, 16, sm));  // queue and shadow counts
+            HIP_CHECK(hipMemsetAsync(B.counters.p, 0, 16, sm));  // queue and shadow counts
             const int igrid = std::min<int>((std::max(c.wave_paths, c.k * npix) + kShadeBlock - 1) / kShadeBlock, ctx->cus * 16);
             if (alias) hipLaunchKernelGGL(k_light_init<true>, dim3(igrid), dim3(kShadeBlock), 0, sm, s->d, c, w);
             else hipLaunchKernelGGL(k_light_init<false>, dim3(igrid), dim3(kShadeBlock), 0, sm, s->d, c, w);
             int qsel = 0;
             for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
                 {
-                    Timed tm(ctx, 0);
-                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, ctx->spill.as<uint32_t>(), stats);
-                    else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
-                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, ctx->spill.as<uint32_t>(), stats);
+                    Timed tm(ctx, 0, sm);
+                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats);
+                    else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                 }
                 {
-                    Timed tm(ctx, 1);
+                    Timed tm(ctx, 1, sm);
                     auto kfn = media ? (lambert ? k_shade_light<true, true> : k_shade_light<false, true>)
                                      : (lambert ? k_shade_light<true, false> : k_shade_light<false, false>);
                     hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
                 }
                 {
-                    Timed tm(ctx, 2);
-                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, ctx->spill.as<uint32_t>(), stats);
-                    else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
-                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, ctx->spill.as<uint32_t>(), stats);
+                    Timed tm(ctx, 2, sm);
+                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats);
+                    else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
+                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
                 }
                 hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
                 qsel ^= 1;
             }
             const bool last = base + c.k >= p->spp;
             const int rgrid = std::min<int>((npix + 255) / 256, ctx->cus * 8);
+            if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.resolve_done, 0));
             hipLaunchKernelGGL(k_resolve, dim3(rgrid), dim3(256), 0, sm, c, w, ctx->acc.as<float4>(), o->hdr,
                                (uint32_t*)nullptr, 1.0f / (float)p->spp, last ? 1 : 0);
+            HIP_CHECK(hipEventRecord(B.resolve_done, sm));
             HIP_CHECK(hipGetLastError());
         }
-        if (o->ray_counts)
-            HIP_CHECK(hipMemcpyAsync(o->ray_counts, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToDevice, sm));
-        HIP_CHECK(hipStreamSynchronize(sm));
-        for (int kind = 0; kind < 3; kind++) {
-            double ms = 0;
-            for (size_t i = 0; i < ctx->ev_used[kind]; i++) {
-                float t = 0;
-                HIP_CHECK(hipEventElapsedTime(&t, ctx->ev[kind][i].first, ctx->ev[kind][i].second));
-                ms += t;
-            }
-            ctx->timings[kind] = ms;
-            ctx->timings[3 + kind] = (double)ctx->ev_used[kind];
-        }
-        unsigned long long hs[4], rc[2];
-        HIP_CHECK(hipMemcpy(hs, ctx->stats.p, 32, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(rc, ctx->counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToHost));
-        for (int i = 0; i < 4; i++) ctx->timings[6 + i] = (double)hs[i];
-        ctx->timings[10] = (double)rc[0];
-        ctx->timings[11] = (double)rc[1];
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
+        finish_timings(ctx, slots, o->ray_counts);
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
@@ -2369,11 +2403,12 @@ int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, u
         HIP_CHECK(hipSetDevice(ctx->device));
         if (count == 0) return ST_OK;
         const int grid = std::min<int>((count + kTraceBlock - 1) / kTraceBlock, ctx->cus * kTraceBlocksPerCU);
-        ensure_spill(ctx, ctx->cus * kTraceBlocksPerCU, kTraceBlock);
-        ctx->counters.reserve(256);
-        HIP_CHECK(hipMemsetAsync(ctx->counters.as<uint8_t>() + 4 * 18, 0, 4, ctx->stream));
+        WaveBufs& B = ctx->slot[0];
+        ensure_spill(B, ctx->cus * kTraceBlocksPerCU, kTraceBlock);
+        B.counters.reserve(256);
+        HIP_CHECK(hipMemsetAsync(B.counters.as<uint8_t>() + 4 * 18, 0, 4, ctx->stream));
         hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
-                           closest, hits, ctx->spill.as<uint32_t>(), ctx->counters.as<uint32_t>() + 18);
+                           closest, hits, B.spill.as<uint32_t>(), B.counters.as<uint32_t>() + 18);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return ST_OK;
