@@ -658,6 +658,50 @@ __device__ __forceinline__ float mis_pair(float a, float b) {  // mis(a, b), tes
     return d > 0 ? a2 / d : 0.0f;
 }
 
+// RIS at the primary vertices of the screen-space reuse mode
+// (SampleIlluminationReSTIR, testrenderer.slang:665-684), as a lean pass before
+// k_shade: it keeps each pixel's reservoir and primary record, and advances
+// the path's random counter past the candidates, where k_shade continues the
+// path (emission, RR and BSDF sampling draw no numbers before NEE).
+template <bool LAMBERT, bool ALIAS>
+__global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c, WaveState w, int qsel) {
+    const uint32_t n = w.counters[qsel];
+    const uint32_t* q = w.queue[qsel];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = q[i];
+        const uint4 hit = w.hit[i];
+        const uint32_t k = p / c.npix;
+        uint32_t px, py;
+        pixel_of(c, p % c.npix, px, py);
+        const size_t rp = (size_t)k * c.npix + py * c.W + px;  // sample k of the wave
+        if (hit.y == 0xffffffffu) {
+            w.r_init[2 * rp + 1] = make_float4(0, 0, 0, 0);
+            continue;
+        }
+        const float4 r = w.ray[p];
+        uint4 s4 = w.st[p];
+        Rng rng{px, py, c.seed + c.sample_base + k, s4.x};
+        Shading sd;
+        tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
+        const Material m = load_material(S, sd.material);
+        const Frame f = make_frame(sd);
+        const v3 ldin = f.to_local(-unpack_normal(__float_as_uint(r.w)));
+        const v3 lgn = f.to_local(unpack_normal(sd.gn));
+        const uint32_t path_len = s4.y + 1;
+        const bool alive = path_len < c.max_depth && any_pos(m.base());
+        const float cp_eff = path_len >= c.rr_start ? m_cont_prob(m) : 1.0f;
+        float4 seed = make_float4(0, 0, 0, 0);
+        float target = 0, W = 0;
+        if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT))
+            ris_reservoir<LAMBERT, ALIAS>(S, c, rng, m, f, lgn, ldin, sd.pos, cp_eff, seed, target, W);
+        w.pri[rp] = make_uint4(hit.y, hit.z, hit.w, __float_as_uint(r.w));
+        w.r_init[2 * rp] = seed;
+        w.r_init[2 * rp + 1] = make_float4(target, W, target > 0 ? (float)c.cand : 0.0f, __uint_as_float(hit.x));
+        s4.x = rng.w;
+        w.st[p] = s4;
+    }
+}
+
 template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
                                                        AuxOut aux, int first_wave) {
@@ -708,8 +752,6 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 em.z += a.z;
             };
             if (!hit_any) {
-                if (RESTIR && (c.flags & ST_FLAG_RESTIR_DI_REUSE) && path_len == 1)
-                    w.r_init[2 * ((size_t)k * c.npix + pix) + 1] = make_float4(0, 0, 0, 0);
                 if (path_len == 1 && first_wave && k == 0)  // a miss stores albedo 1 (TraceRay, :1040-1041)
                     store_aux(c, aux, pix, px, py, 0xffffffffu, pack_normal(dir), false, mk(0, 0, 0), mk(1, 1, 1),
                               q16(__int_as_float(0x7f800000)));
@@ -808,19 +850,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 // Russian roulette from path length rr_start (0: every bounce, testrenderer.slang:890-892)
                 const bool rr_on = path_len >= c.rr_start;
                 const float cp_eff = rr_on ? m_cont_prob(m) : 1.0f;
-                const bool reuse = RESTIR && (c.flags & ST_FLAG_RESTIR_DI_REUSE) && path_len == 1;
-                if (reuse) {
-                    // primary vertex of the screen-space reuse mode: keep the RIS
-                    // reservoir; the shadow ray is made after temporal + spatial reuse
-                    float4 seed = make_float4(0, 0, 0, 0);
-                    float target = 0, W = 0;
-                    if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT))
-                        ris_reservoir<LAMBERT, ALIAS>(S, c, rng, m, f, lgn, ldin, sd.pos, cp_eff, seed, target, W);
-                    const size_t rp = (size_t)k * c.npix + pix;  // sample k of the wave
-                    w.pri[rp] = make_uint4(hit.y, hit.z, hit.w, __float_as_uint(r.w));
-                    w.r_init[2 * rp] = seed;
-                    w.r_init[2 * rp + 1] = make_float4(target, W, target > 0 ? (float)c.cand : 0.0f, __uint_as_float(hit.x));
-                } else if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
+                // primary vertex of the screen-space reuse mode: k_ris kept the RIS
+                // reservoir (and advanced the random counter); the shadow ray is
+                // made after temporal + spatial reuse
+                const bool reuse = (c.flags & ST_FLAG_RESTIR_DI_REUSE) && path_len == 1;
+                if (!reuse && alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
                     // the screen-space reuse mode keeps reservoirs at primary vertices only
                     const bool ris_here = RESTIR && !(c.flags & ST_FLAG_RESTIR_DI_REUSE);
                     emit_shadow = (ris_here ? next_event<LAMBERT, ALIAS, true>(S, c, rng, m, f, lgn, ldin, sd.pos, sd.gn, thr,
@@ -1380,9 +1414,14 @@ struct DevBuf {
 
 }  // namespace
 
-// Device state of one wave of paths.  Two waves can be in flight at once, each
-// in its own slot and on its own stream, so that one wave's last bounces (few
-// rays, bound by the slowest ray) overlap the next wave's first bounces.
+// Device state of one wave of paths.  Up to kWaveSlots waves are in flight at
+// once, each in its own slot and on its own stream, so that one wave's last
+// bounces (few rays, bound by the slowest ray) overlap the next wave's first
+// bounces.  Four slots match the four hardware queues a process gets; with
+// waves of about 2M paths (one 1080p sample) this measured fastest at 1080p
+// (more slots share queues, larger waves were slower).
+constexpr int kWaveSlots = 4;
+constexpr uint32_t kWavePaths = 2u << 20;
 struct WaveBufs {
     DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, pri, rinit, rtemp, med;
     hipStream_t stream = nullptr;        // slot 0: the context's stream
@@ -1394,7 +1433,7 @@ struct st_context {
     int device = 0;
     int cus = 256;
     hipStream_t stream = nullptr;
-    WaveBufs slot[2];
+    WaveBufs slot[kWaveSlots];
     hipEvent_t start_done = nullptr;     // the frame's clears on the context stream
     DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
     double timings[12] = {0};
@@ -1683,9 +1722,9 @@ int st_context_create(int device, st_context** out) {
         HIP_CHECK(hipGetDeviceProperties(&prop, device));
         ctx->cus = prop.multiProcessorCount;
         HIP_CHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        // a second stream of the same (default) priority for the second wave slot
-        HIP_CHECK(hipStreamCreateWithFlags(&ctx->slot[1].stream, hipStreamNonBlocking));
+        // more streams of the same (default) priority for the other wave slots
         ctx->slot[0].stream = ctx->stream;
+        for (int sl = 1; sl < kWaveSlots; sl++) HIP_CHECK(hipStreamCreateWithFlags(&ctx->slot[sl].stream, hipStreamNonBlocking));
         for (WaveBufs& b : ctx->slot) {
             HIP_CHECK(hipEventCreateWithFlags(&b.reuse_done, hipEventDisableTiming));
             HIP_CHECK(hipEventCreateWithFlags(&b.resolve_done, hipEventDisableTiming));
@@ -1702,7 +1741,8 @@ int st_context_destroy(st_context* ctx) {
     if (!ctx) return ST_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->slot[1].stream) (void)hipStreamSynchronize(ctx->slot[1].stream);
+    for (int sl = 1; sl < kWaveSlots; sl++)
+        if (ctx->slot[sl].stream) (void)hipStreamSynchronize(ctx->slot[sl].stream);
     for (auto& v : ctx->ev)
         for (auto& e : v) {
             (void)hipEventDestroy(e.first);
@@ -1713,7 +1753,8 @@ int st_context_destroy(st_context* ctx) {
         if (b.resolve_done) (void)hipEventDestroy(b.resolve_done);
     }
     if (ctx->start_done) (void)hipEventDestroy(ctx->start_done);
-    if (ctx->slot[1].stream) (void)hipStreamDestroy(ctx->slot[1].stream);
+    for (int sl = 1; sl < kWaveSlots; sl++)
+        if (ctx->slot[sl].stream) (void)hipStreamDestroy(ctx->slot[sl].stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return ST_OK;
@@ -2153,12 +2194,12 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         const uint32_t npix = p->width * p->height;
         const bool reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
         uint32_t k = p->samples_per_wave;
-        if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, (uint32_t)((4u << 20) / npix)));
+        if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, kWavePaths / npix));
         k = std::min(k, p->spp);
         const uint64_t paths = (uint64_t)npix * k;
         if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
         const uint32_t waves = (p->spp + k - 1) / k;
-        const int slots = waves > 1 ? 2 : 1;
+        const int slots = (int)std::min<uint32_t>(kWaveSlots, waves);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, reuse, paths, tgrid);
         ctx->acc.reserve((size_t)npix * 16);
@@ -2169,7 +2210,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
         HIP_CHECK(hipEventRecord(ctx->start_done, s0));
-        if (slots > 1) HIP_CHECK(hipStreamWaitEvent(ctx->slot[1].stream, ctx->start_done, 0));
+        for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
         for (int kk = 0; kk < 3; kk++) ctx->ev_used[kk] = 0;
         RenderConsts c;
         std::memset(&c, 0, sizeof c);
@@ -2210,14 +2251,14 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         c.radius = p->spatial_radius;
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
         auto* stats = ctx->stats.as<unsigned long long>();
-        // Wave i runs in slot i % 2 on that slot's stream; the slot's previous
-        // wave (i - 2) finished before it on the same stream.  Across slots,
+        // Wave i runs in slot i % slots on that slot's stream; the slot's
+        // previous wave (i - slots) finished before it on the same stream.  Across slots,
         // two orders matter: the screen-space reuse of wave i reads the final
         // reservoirs of wave i - 1, and resolves add samples in sample order.
         uint32_t wi = 0;
         for (uint32_t base = 0; base < p->spp; base += k, wi++) {
             WaveBufs& B = ctx->slot[wi % slots];
-            WaveBufs& prev = ctx->slot[(wi + 1) % slots];
+            WaveBufs& prev = ctx->slot[(wi + slots - 1) % slots];
             hipStream_t sm = B.stream;
             WaveState w = wave_state(B, ctx);
             uint32_t* spill = B.spill.as<uint32_t>();
@@ -2238,7 +2279,15 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                     Timed tm(ctx, 1, sm);
                     AuxOut aux{nullptr, nullptr, nullptr, nullptr};
                     if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
-                    auto* kfn = shade_kernel(lambert, alias, restir, media);
+                    // with screen-space reuse, k_ris runs the primary vertices' RIS
+                    // first and every vertex after takes plain NEE: the shading
+                    // kernel without the candidate loop
+                    if (reuse && b == 0) {
+                        auto kr = lambert ? (alias ? k_ris<true, true> : k_ris<true, false>)
+                                          : (alias ? k_ris<false, true> : k_ris<false, false>);
+                        hipLaunchKernelGGL(kr, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
+                    }
+                    auto* kfn = shade_kernel(lambert, alias, restir && !reuse, media);
                     hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0);
                 }
                 if (reuse && b == 0) {
@@ -2294,12 +2343,12 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         const uint32_t L = std::max<uint32_t>(1u, (uint32_t)((float)npix * p->light_subpath_factor));
         const uint32_t span = std::max(L, npix);
         uint32_t k = p->samples_per_wave;
-        if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, (uint32_t)((4u << 20) / span)));
+        if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, kWavePaths / span));
         k = std::min(k, p->spp);
         const uint64_t paths = (uint64_t)span * k;
         if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
         const uint32_t waves = (p->spp + k - 1) / k;
-        const int slots = waves > 1 ? 2 : 1;
+        const int slots = (int)std::min<uint32_t>(kWaveSlots, waves);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, false, 0, tgrid);
         ctx->acc.reserve((size_t)npix * 16);
@@ -2309,7 +2358,7 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
         HIP_CHECK(hipEventRecord(ctx->start_done, s0));
-        if (slots > 1) HIP_CHECK(hipStreamWaitEvent(ctx->slot[1].stream, ctx->start_done, 0));
+        for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
         RenderConsts c;
         std::memset(&c, 0, sizeof c);
         c.W = p->width;
@@ -2336,11 +2385,11 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
         const bool media = s->d.vinst_count > 0;
         auto* stats = ctx->stats.as<unsigned long long>();
-        // waves alternate between the two slots and streams; resolves keep sample order
+        // waves rotate through the slots and their streams; resolves keep sample order
         uint32_t wi = 0;
         for (uint32_t base = 0; base < p->spp; base += k, wi++) {
             WaveBufs& B = ctx->slot[wi % slots];
-            WaveBufs& prev = ctx->slot[(wi + 1) % slots];
+            WaveBufs& prev = ctx->slot[(wi + slots - 1) % slots];
             hipStream_t sm = B.stream;
             WaveState w = wave_state(B, ctx);
             uint32_t* spill = B.spill.as<uint32_t>();
