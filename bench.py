@@ -112,9 +112,11 @@ def main():
     # with media the node counters count density-grid reads
     cp = abi.default_params(args.width, args.height, max_depth=max_depth, spp=1, flags=flags | abi.ST_FLAG_COUNT_TRAVERSAL,
                             rr_start_depth=rr)
-    r.render(view, cp)
+    first = r.render(view, cp)
     torch.cuda.synchronize()
     tc = ctx.timings()
+    # primary hits per sample (unjittered camera: the same for every sample)
+    prim_hits = int((first["visibility"][..., 0] != -1).sum().item())
     nodes_c = tc["closest_nodes"] / max(1, tc["closest_rays"])
     tris_c = tc["closest_tris"] / max(1, tc["closest_rays"])
     nodes_s = tc["shadow_nodes"] / max(1, tc["shadow_rays"])
@@ -145,8 +147,19 @@ def main():
     # per triangle.  Media (per traceRay segment): 52 B path state read + 52 B
     # written (or the 48 B shadow record + 16 B NEE accumulator), and 44 B per
     # density-grid read (root tile key + child, upper mask word + table entry,
-    # lower mask word + table entry, leaf value).
+    # lower mask word + table entry, leaf value).  Shading: 16 B hit + 64 B path
+    # state + 64 B material + 64 B triangle + 48 B shadow record per vertex; with
+    # ReSTIR reuse each primary hit also evaluates `cand` light candidates
+    # (environment, alias mode: 8 B alias entry + 4 B pdf + 4 texels of 16 B;
+    # emissive triangle: 4 B light index + 8 B instance + 48 B transform + 32 B
+    # vertex info + 12 B indices + 36 B positions + 36 B material), mixed by the
+    # environment sampling probability, and writes its 48 B reservoir + record.
     NODE_B, TRI_B, GRID_B = 48, 48, 44
+    CAND_ENV_B, CAND_TRI_B = 76, 176
+    ris_bytes = 0.0
+    if not cloud and not args.no_restir:
+        env_p = params.environment_sample_probability
+        ris_bytes = prim_hits * args.spp * (params.candidate_samples * (env_p * CAND_ENV_B + (1 - env_p) * CAND_TRI_B) + 48)
     t_last = kt[-1]
     if cloud:
         classes = {
@@ -158,7 +171,8 @@ def main():
         classes = {
             "k_trace_closest": (t_last["trace_ms"], t_last["trace_launches"], t_last["closest_rays"] * (48 + NODE_B * nodes_c + TRI_B * tris_c)),
             "k_trace_shadow": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * (52 + NODE_B * nodes_s + TRI_B * tris_s)),
-            "k_shade": (t_last["shade_ms"], t_last["shade_launches"], t_last["closest_rays"] * (16 + 64 + 64 + 64 + 48) + t_last["shadow_rays"] * 48),
+            "k_shade": (t_last["shade_ms"], t_last["shade_launches"],
+                        t_last["closest_rays"] * (16 + 64 + 64 + 64 + 48) + t_last["shadow_rays"] * 48 + ris_bytes),
         }
         if args.light_trace:  # k_shade_light: hit, ray, throughput, state read; ray, throughput, state written
             classes["k_shade_light"] = classes.pop("k_shade")
@@ -214,10 +228,14 @@ def measured_traffic(args, kernel):
         return None, None
     with open(path) as f:
         rec = json.load(f)
-    k = rec.get("kernels", {}).get(kernel)
+    ks = rec.get("kernels", {})
+    k = ks.get(kernel)
     if not k:
         return None, None
-    return int(k["hbm_bytes_per_dispatch"]), os.path.relpath(path, ROOT)
+    total = k["hbm_bytes_per_dispatch"] * k["dispatches"]
+    if kernel == "k_shade" and "k_ris" in ks:  # the RIS pass is timed with the shading class
+        total += ks["k_ris"]["hbm_bytes_per_dispatch"] * ks["k_ris"]["dispatches"]
+    return int(total / k["dispatches"]), os.path.relpath(path, ROOT)
 
 
 def copy_calibration(log):
