@@ -81,6 +81,7 @@ struct WaveState {
     float4* r_init;     // 2 per pixel: light-sample seed (float4), {target, W, M, depth}
     float4* r_temp;     // after temporal reuse
     float4* r_prev;     // after spatial reuse: this frame's result, next frame's history
+    float4* psd;        // 2 per pixel: the primary hit's shading data (pos, material, gn, sn, tangent, area)
     uint32_t* med;      // per path: current medium instance (kNoMedium outside media)
 };
 
@@ -638,8 +639,17 @@ struct Prim {
     Frame f;
     v3 lgn, ldin;
 };
-__device__ __forceinline__ void rebuild_primary(const DevScene& S, uint4 rec, Prim& P) {
-    tri_shading(S, rec.x & 0xffffu, rec.x >> 16, __uint_as_float(rec.y), __uint_as_float(rec.z), P.sd);
+// The triangle's shading data was computed once, by k_ris, and stored bit for
+// bit, so the reuse passes read one 32-byte record instead of the mesh buffers.
+__device__ __forceinline__ void rebuild_primary(const DevScene& S, const WaveState& w, size_t idx, Prim& P) {
+    const uint4 rec = w.pri[idx];
+    const float4 a = w.psd[2 * idx], b = w.psd[2 * idx + 1];
+    P.sd.pos = mk(a.x, a.y, a.z);
+    P.sd.material = __float_as_uint(a.w);
+    P.sd.gn = __float_as_uint(b.x);
+    P.sd.sn = __float_as_uint(b.y);
+    P.sd.tan = __float_as_uint(b.z);
+    P.sd.area = b.w;
     P.m = load_material(S, P.sd.material);
     P.f = make_frame(P.sd);
     P.lgn = P.f.to_local(unpack_normal(P.sd.gn));
@@ -695,6 +705,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c,
         if (alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT))
             ris_reservoir<LAMBERT, ALIAS>(S, c, rng, m, f, lgn, ldin, sd.pos, cp_eff, seed, target, W);
         w.pri[rp] = make_uint4(hit.y, hit.z, hit.w, __float_as_uint(r.w));
+        w.psd[2 * rp] = make_float4(sd.pos.x, sd.pos.y, sd.pos.z, __uint_as_float(sd.material));
+        w.psd[2 * rp + 1] = make_float4(__uint_as_float(sd.gn), __uint_as_float(sd.sn), __uint_as_float(sd.tan), sd.area);
         w.r_init[2 * rp] = seed;
         w.r_init[2 * rp + 1] = make_float4(target, W, target > 0 ? (float)c.cand : 0.0f, __uint_as_float(hit.x));
         s4.x = rng.w;
@@ -925,7 +937,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_temporal(DevScene S, Ren
             const float4 qs = w.r_prev[2 * pix], qb = w.r_prev[2 * pix + 1];
             if (qb.z > 0) {
                 Prim P;
-                rebuild_primary(S, w.pri[rp], P);
+                rebuild_primary(S, w, rp, P);
                 Cand x;
                 const float tq = seed_target<LAMBERT, ALIAS>(S, c, P, qs, x);  // q's seed here
                 const uint32_t px = pix % c.W, py = pix / c.W;
@@ -962,7 +974,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
             float4 seed = w.r_temp[2 * (jb + pix)], b = w.r_temp[2 * (jb + pix) + 1];
             if (b.z > 0) {
                 Prim P;
-                rebuild_primary(S, w.pri[jb + pix], P);
+                rebuild_primary(S, w, jb + pix, P);
                 const v3 nc = unpack_normal(P.sd.sn);
                 Cand cur;
                 seed_target<LAMBERT, ALIAS>(S, c, P, seed, cur);
@@ -980,7 +992,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
                     // similar surfaces only: depth within 10%, normals within ~25 degrees
                     if (fabsf(qb.w - b.w) > 0.1f * b.w) continue;
                     Prim Q;
-                    rebuild_primary(S, w.pri[jb + qpix], Q);
+                    rebuild_primary(S, w, jb + qpix, Q);
                     if (dot(unpack_normal(Q.sd.sn), nc) < 0.9f) continue;
                     Cand xn, xq;
                     const float tn = seed_target<LAMBERT, ALIAS>(S, c, P, qs, xn);     // neighbour's seed here
@@ -1423,7 +1435,7 @@ struct DevBuf {
 constexpr int kWaveSlots = 4;
 constexpr uint32_t kWavePaths = 2u << 20;
 struct WaveBufs {
-    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, pri, rinit, rtemp, med;
+    DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, pri, psd, rinit, rtemp, med;
     hipStream_t stream = nullptr;        // slot 0: the context's stream
     hipEvent_t reuse_done = nullptr;     // this slot's last wave finished its screen-space reuse
     hipEvent_t resolve_done = nullptr;   // this slot's last wave finished its resolve
@@ -2117,6 +2129,7 @@ static void reserve_wave(WaveBufs& b, uint64_t paths, bool reuse, uint64_t npix_
     b.counters.reserve(256);
     if (reuse) {
         b.pri.reserve(npix_k * 16);
+        b.psd.reserve(npix_k * 32);
         b.rinit.reserve(npix_k * 32);
         b.rtemp.reserve(npix_k * 32);
     }
@@ -2137,6 +2150,7 @@ static WaveState wave_state(WaveBufs& b, st_context* ctx) {
     w.shadow = b.shadow.as<float4>();
     w.counters = b.counters.as<uint32_t>();
     w.pri = b.pri.as<uint4>();
+    w.psd = b.psd.as<float4>();
     w.r_init = b.rinit.as<float4>();
     w.r_temp = b.rtemp.as<float4>();
     w.r_prev = ctx->rprev.as<float4>();
