@@ -48,6 +48,11 @@ struct DevScene {
     uint32_t inst_count, mat_words, vi_count, vb_count;
     const uint32_t* light_inst;
     uint32_t light_count;
+    // emitter table: per light, 4 float4 {transform rows 0-2, {triangles, first
+    // triangle, emission word, emission scale}}; per emissive triangle, 5 float4
+    // {v0 | has-normals, v1 | n0.x, v2 | n0.y, n0.z n1, n2} in object space
+    const float4* light_rec;
+    const float4* light_tri;
     // environment
     int has_env, env_has_image, env_levels;
     float env_emission[3];

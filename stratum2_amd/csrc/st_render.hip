@@ -1352,6 +1352,48 @@ __global__ void k_flatten(DevScene S, const uint32_t* tri_inst, const uint32_t* 
 
 // Tonemapper::render's two passes (tonemap.slang:124-192).  maxReduce: the
 // maximum of rgb and luminance in 1/16384 fixed point (one atomic per wave).
+// Emitter table (DevScene::light_rec, light_tri) from the scene buffers: one
+// block per light; tri_shading's loaders copy each triangle's object-space
+// vertices and normals bit for bit, and the light's transform and emission
+// words are copied as they are.
+__global__ void k_light_table(DevScene S, const uint32_t* tri_base, float4* rec, float4* tri) {
+    const uint32_t li = blockIdx.x;
+    if (li >= S.light_count) return;
+    const uint32_t ii = S.light_inst[li];
+    const st_instance in = S.inst[ii];
+    const uint32_t pc = in.data >> 16, base = tri_base[li];
+    if (threadIdx.x == 0) {
+        const st_transform& T = S.xf[ii];
+        for (int r = 0; r < 3; r++) rec[4 * (size_t)li + r] = make_float4(T.m[r][0], T.m[r][1], T.m[r][2], T.m[r][3]);
+        const uint32_t addr = in.type_material_address >> 4;
+        rec[4 * (size_t)li + 3] = make_float4(__uint_as_float(pc), __uint_as_float(base), __uint_as_float(S.mat[addr / 4 + 1]),
+                                              __uint_as_float(S.mat[addr / 4 + 8]));
+    }
+    const st_mesh_vertex_info vi = S.vi[in.data & 0xffffu];
+    const uint32_t pb = vi.packed_buffer_indices[0] >> 16, ps = (vi.packed_strides >> 8) & 0xffu;
+    const uint32_t nb = vi.packed_buffer_indices[1] & 0xffffu, ns = (vi.packed_strides >> 16) & 0xffu;
+    const bool has_n = nb < ST_IMAGE_LIMIT;
+    for (uint32_t prim = threadIdx.x; prim < pc; prim += blockDim.x) {
+        uint32_t t[3];
+        load_tri(S, vi, prim, t);
+        const uint8_t* P = S.vb[pb] + vi.packed_offsets[1];
+        const v3 v0 = ld_v3(P + (size_t)ps * t[0]), v1 = ld_v3(P + (size_t)ps * t[1]), v2 = ld_v3(P + (size_t)ps * t[2]);
+        v3 n0 = mk(0, 0, 0), n1 = n0, n2 = n0;
+        if (has_n) {
+            const uint8_t* N = S.vb[nb] + vi.packed_offsets[2];
+            n0 = ld_v3(N + (size_t)ns * t[0]);
+            n1 = ld_v3(N + (size_t)ns * t[1]);
+            n2 = ld_v3(N + (size_t)ns * t[2]);
+        }
+        float4* o = tri + 5 * ((size_t)base + prim);
+        o[0] = make_float4(v0.x, v0.y, v0.z, __uint_as_float(has_n ? 1u : 0u));
+        o[1] = make_float4(v1.x, v1.y, v1.z, n0.x);
+        o[2] = make_float4(v2.x, v2.y, v2.z, n0.y);
+        o[3] = make_float4(n0.z, n1.x, n1.y, n1.z);
+        o[4] = make_float4(n2.x, n2.y, n2.z, 0.0f);
+    }
+}
+
 __global__ void k_tonemap_max(const float4* in, const float4* albedo, uint32_t n, uint32_t* mx) {
     uint32_t m[4] = {0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1474,10 +1516,27 @@ struct st_scene {
     const uint32_t* inst_first_dev = nullptr;
     void* bvh_nodes = nullptr;  // hipMalloc'ed by the device builder, else null
     void* bvh_tris = nullptr;
+    // emitter table, rebuilt when the instances move
+    const uint32_t* light_tri_base = nullptr;
+    float4* light_rec = nullptr;
+    float4* light_tri = nullptr;
     ~st_scene() {
         for (auto* b : bufs) delete b;
         if (bvh_nodes) (void)hipFree(bvh_nodes);
         if (bvh_tris) (void)hipFree(bvh_tris);
+    }
+    void build_light_table(hipStream_t stream) {
+        if (d.light_count == 0) return;
+        hipLaunchKernelGGL(k_light_table, dim3(d.light_count), dim3(64), 0, stream, d, light_tri_base, light_rec, light_tri);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+    template <class T>
+    T* alloc(size_t bytes) {
+        DevBuf* b = new DevBuf();
+        bufs.push_back(b);
+        b->reserve(std::max<size_t>(bytes, 16));
+        return b->as<T>();
     }
     template <class T>
     T* upload(const void* src, size_t bytes) {
@@ -1915,6 +1974,22 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
         s->inst_first_dev = s->upload<uint32_t>(s->inst_first.data(), 4 * s->inst_first.size());
         D.light_inst = s->upload<uint32_t>(lights.data(), 4 * lights.size());
         D.light_count = (uint32_t)lights.size();
+        if (D.light_count > 0) {
+            // the emitter table: per light its first triangle in the table
+            std::vector<uint32_t> base(lights.size());
+            uint64_t total = 0;
+            for (size_t li = 0; li < lights.size(); li++) {
+                base[li] = (uint32_t)total;
+                total += d->instances[lights[li]].data >> 16;
+            }
+            if (total >= (1ull << 31)) throw std::domain_error("too many emissive triangles");
+            s->light_tri_base = s->upload<uint32_t>(base.data(), 4 * base.size());
+            s->light_rec = s->alloc<float4>(64 * lights.size());
+            s->light_tri = s->alloc<float4>(80 * std::max<uint64_t>(1, total));
+            D.light_rec = s->light_rec;
+            D.light_tri = s->light_tri;
+            s->build_light_table(ctx->stream);
+        }
         BvhOut bvh = build_bvh(verts.data(), ids.data(), (uint32_t)ntri);
         D.nodes = s->upload<float4>(bvh.nodes4.data(), bvh.nodes4.size() * sizeof(Node4Out));
         D.tris = s->upload<float4>(bvh.tris4.data(), bvh.tris4.size() * sizeof(TriOut));
@@ -2040,6 +2115,7 @@ int st_scene_update(st_scene* s, const st_transform* transforms, double* build_m
         for (uint32_t i = 0; i < ni; i++) s->inv_xf[i] = inverse_transform(transforms[i]);
         HIP_CHECK(hipMemcpy(const_cast<st_transform*>(D.inv_xf), s->inv_xf.data(), sizeof(st_transform) * ni, hipMemcpyHostToDevice));
         scene_sphere(s->boxes, transforms, D.sphere);
+        s->build_light_table(ctx->stream);  // the emitters' transforms moved too
         double ms = 0;
         if (s->ntri > 0) {
             // flatten on the device and rebuild the BVH there

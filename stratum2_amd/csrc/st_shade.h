@@ -296,6 +296,38 @@ __device__ inline v3 evaluate_environment(const DevScene& S, float env_prob, v3 
     return e;
 }
 
+// The position, geometry normal and area of a point on an emissive triangle,
+// as tri_shading computes them (makeTriangleShadingData, shading_data.hlsli:
+// 59-151: the geometry normal flips to the side of an interpolated vertex
+// normal), from the emitter table: two loads instead of the chain light ->
+// instance -> vertex info -> indices -> vertices.
+__device__ inline void light_point(const DevScene& S, uint32_t li, uint32_t prim, float bx, float by, v3& pos,
+                                   uint32_t& gnp, float& area) {
+    const float4* L = S.light_rec + 4 * (size_t)li;
+    const float4 r0 = L[0], r1 = L[1], r2 = L[2];
+    const st_transform T = {{{r0.x, r0.y, r0.z, r0.w}, {r1.x, r1.y, r1.z, r1.w}, {r2.x, r2.y, r2.z, r2.w}}};
+    const float4* R = S.light_tri + 5 * ((size_t)__float_as_uint(L[3].y) + prim);
+    const float4 f0 = R[0], f1 = R[1], f2 = R[2];
+    const v3 v0 = mk(f0.x, f0.y, f0.z), v1 = mk(f1.x, f1.y, f1.z), v2 = mk(f2.x, f2.y, f2.z);
+    const v3 dPds = xf_vector(T, v0 - v2), dPdt = xf_vector(T, v1 - v2);
+    v3 gn = cross(dPds, dPdt);
+    const float area2 = len(gn);
+    gn = gn / area2;
+    gnp = pack_normal(gn);
+    area = area2 / 2;
+    if (__float_as_uint(f0.w) & 1u) {
+        const float4 f3 = R[3], f4 = R[4];
+        const v3 n0 = mk(f1.w, f2.w, f3.x), n1 = mk(f3.y, f3.z, f3.w), n2 = mk(f4.x, f4.y, f4.z);
+        v3 sn = (n0 + (n1 - n0) * bx) + (n2 - n0) * by;
+        const bool snvalid = !((sn.x == 0 && sn.y == 0 && sn.z == 0) || sn.x != sn.x || sn.y != sn.y || sn.z != sn.z);
+        if (snvalid) {
+            sn = normalize(xf_vector(T, sn));
+            if (dot(sn, gn) < 0) gnp = pack_normal(-gn);
+        }
+    }
+    pos = xf_point(T, (v0 + (v1 - v0) * bx) + (v2 - v0) * by);
+}
+
 struct LightSample {  // IlluminationSampleRecord, lights.hlsli:32-45
     v3 radiance, dir;
     float pdf, dist, cos_light;
@@ -353,27 +385,28 @@ __device__ inline LightSample sample_illumination(const DevScene& S, float env_p
     }
     if (S.light_count == 0) return r;
     const uint32_t li = (uint32_t)(rnd.z * (float)S.light_count) % S.light_count;
-    const uint32_t ii = S.light_inst[li];
     float pdf = 1 / (float)S.light_count;
     if (S.has_env) pdf *= 1 - env_prob;
-    const st_instance in = S.inst[ii];
-    const uint32_t pc = in.data >> 16;
+    const float4 l3 = S.light_rec[4 * (size_t)li + 3];  // {triangles, first, emission word, scale}
+    const uint32_t pc = __float_as_uint(l3.x);
     const uint32_t prim = (uint32_t)(rnd.w * (float)pc) % pc;
     pdf /= (float)pc;
     const float a = sqrtf(rnd.x);  // sampleUniformTriangle, common.hlsli:36-39
-    Shading sd;
-    tri_shading(S, ii, prim, 1 - a, a * rnd.y, sd);
-    pdf /= sd.area;
-    const Material m = load_material(S, sd.material);
-    r.radiance = m.emission();
+    v3 lpos;
+    uint32_t lgn;
+    float larea;
+    light_point(S, li, prim, 1 - a, a * rnd.y, lpos, lgn, larea);
+    pdf /= larea;
+    const uint32_t ew = __float_as_uint(l3.z);  // Material::emission, as load_material's record holds it
+    r.radiance = mk(un8(ew, 0), un8(ew, 8), un8(ew, 16)) * l3.w;
     r.pdf = pdf;
     r.finite = true;
-    r.pos = sd.pos;
-    r.gn = sd.gn;
-    v3 dd = sd.pos - ref;
+    r.pos = lpos;
+    r.gn = lgn;
+    v3 dd = lpos - ref;
     r.dist = len(dd);
     r.dir = dd / r.dist;
-    r.cos_light = -dot(unpack_normal(sd.gn), r.dir);
+    r.cos_light = -dot(unpack_normal(lgn), r.dir);
     if (r.cos_light <= 0) {  // lights are one-sided
         r.radiance = mk(0, 0, 0);
         r.pdf = 0;

@@ -428,3 +428,30 @@ def test_restir_reuse_independent_of_samples_per_wave(gpu_ctx):
     imgs = [_gpu_render(gpu_ctx, sd, view, abi.default_params(W, H, max_depth=4, spp=6, flags=flags, samples_per_wave=k))
             for k in (1, 2, 3)]
     assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
+
+
+def _emissive_smooth_sphere_scene():
+    """Cornell box whose coarse sphere carries vertex normals and emits: light
+    samples on it take the geometry-normal flip of makeTriangleShadingData."""
+    sd = scenes.cornell_box_smooth(2, subdiv=1)
+    a, b = sd.instances[7]
+    mat = sd.add_material(scenes.material_record(base_color=(0.6, 0.6, 0.6), specular=0.2, roughness=0.7,
+                                                 emission=(4.0, 3.0, 2.0)))
+    sd.instances[7] = ((a & 0xF) | (mat << 4), b)
+    return sd
+
+
+@pytest.mark.parametrize("light_trace", [False, True])
+def test_emitter_with_vertex_normals(gpu_ctx, light_trace):
+    sd = _emissive_smooth_sphere_scene()
+    W = H = 64
+    fl = abi.ST_FLAG_DEFAULT | (abi.ST_FLAG_TRACE_FROM_LIGHT if light_trace else 0)
+    params = abi.default_params(W, H, max_depth=4, spp=4, flags=fl)
+    g, r = _render_both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
+    if light_trace:
+        hg, hr = g["hdr"][..., :3], r["hdr"][..., :3]
+        ok = np.abs(hg - hr) <= 1e-4 * np.maximum(np.abs(hr), 1e-3)
+        print(f"pixel channels within 1e-4 rel: {ok.mean():.5f}")
+        assert ok.mean() >= 0.99 and abs(hg.mean() - hr.mean()) <= 0.01 * hr.mean()
+    else:
+        _check_image(g, r)
