@@ -12,7 +12,12 @@
   polygons fanned into triangles, optional .mtl diffuse colours).
 * Radiance RGBE images (.hdr) for environment maps, read with both scanline
   encodings (flat and run-length), written flat.
+* glTF 2.0 scenes (.gltf with external or data-URI buffers, .glb) -> SceneData,
+  as src/App/loaders/load_gltf.cpp maps them: metallic-roughness materials,
+  triangle primitives, node TRS / matrix transforms down the node hierarchy.
 """
+import base64
+import json
 import os
 import struct
 import zlib
@@ -297,3 +302,198 @@ def write_hdr(path, img):
     with open(path, "wb") as f:
         f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {H} +X {W}\n".encode())
         f.write(float_to_rgbe(img).tobytes())
+
+
+# --------------------------------------------------------------------- glTF
+_GLTF_COMPONENTS = {5120: np.int8, 5121: np.uint8, 5122: np.int16, 5123: np.uint16, 5125: np.uint32, 5126: np.float32}
+_GLTF_WIDTH = {"SCALAR": 1, "VEC2": 2, "VEC3": 3, "VEC4": 4, "MAT2": 4, "MAT3": 9, "MAT4": 16}
+_GLB_JSON, _GLB_BIN = 0x4E4F534A, 0x004E4942
+
+
+def _gltf_document(path):
+    """The JSON document and its buffers (.glb container or .gltf text)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    binary = None
+    if data[:4] == b"glTF":  # 12-byte header, then chunks of {length, type, bytes}
+        _, length = struct.unpack_from("<II", data, 4)
+        off, doc = 12, None
+        while off + 8 <= min(length, len(data)):
+            clen, ctype = struct.unpack_from("<II", data, off)
+            chunk = data[off + 8:off + 8 + clen]
+            off += 8 + clen
+            if ctype == _GLB_JSON:
+                doc = json.loads(chunk.decode("utf-8"))
+            elif ctype == _GLB_BIN and binary is None:
+                binary = chunk
+        if doc is None:
+            raise ValueError(f"{path}: no JSON chunk")
+    else:
+        doc = json.loads(data.decode("utf-8"))
+    base = os.path.dirname(os.path.abspath(path))
+    buffers = []
+    for b in doc.get("buffers", []):
+        uri = b.get("uri")
+        if uri is None:
+            if binary is None:
+                raise ValueError(f"{path}: buffer without a URI outside a .glb")
+            buffers.append(binary)
+        elif uri.startswith("data:"):
+            buffers.append(base64.b64decode(uri.split(",", 1)[1]))
+        else:
+            with open(os.path.join(base, uri), "rb") as f:
+                buffers.append(f.read())
+    return doc, buffers
+
+
+def _gltf_accessor(doc, buffers, index):
+    """Accessor `index` as a (count, width) array (strided views copied)."""
+    a = doc["accessors"][index]
+    if "sparse" in a:
+        raise ValueError("sparse glTF accessors are not supported")
+    dt = np.dtype(_GLTF_COMPONENTS[a["componentType"]])
+    width, count = _GLTF_WIDTH[a["type"]], a["count"]
+    if "bufferView" not in a:
+        return np.zeros((count, width), dt)
+    bv = doc["bufferViews"][a["bufferView"]]
+    stride = bv.get("byteStride", dt.itemsize * width)
+    start = bv.get("byteOffset", 0) + a.get("byteOffset", 0)
+    buf = buffers[bv["buffer"]]
+    if count and start + stride * (count - 1) + dt.itemsize * width > len(buf):
+        raise ValueError(f"glTF accessor {index} reaches past its buffer")
+    arr = np.ndarray((count, width), dt, buf, start, (stride, dt.itemsize)).copy()
+    if a.get("normalized") and dt.kind in "iu":
+        arr = np.maximum(arr.astype(np.float32) / np.float32(np.iinfo(dt).max), np.float32(-1))
+    return arr
+
+
+def _lum(c):
+    c = np.asarray(c, np.float32)
+    return float(c[0] * np.float32(0.2126) + c[1] * np.float32(0.7152) + c[2] * np.float32(0.0722))
+
+
+def _gltf_material(m):
+    """Scene::makeMetallicRoughnessMaterial (Scene.cpp:195-209) on the factors of
+    a glTF material with the extensions load_gltf.cpp:92-128 reads (textures are
+    not sampled by the renderer and are left out)."""
+    pbr = m.get("pbrMetallicRoughness", {})
+    ext = m.get("extensions", {})
+    emission = np.asarray(m.get("emissiveFactor", [0, 0, 0]), np.float32)
+    if "KHR_materials_emissive_strength" in ext:
+        emission = emission * np.float32(ext["KHR_materials_emissive_strength"].get("emissiveStrength", 1.0))
+    eta = ext["KHR_materials_ior"].get("ior", 1.5) if "KHR_materials_ior" in ext else 1.5
+    t = ext["KHR_materials_transmission"].get("transmissionFactor", 0.0) if "KHR_materials_transmission" in ext else 0.0
+    specular = 1.0
+    if "KHR_materials_specular" in ext:
+        sp = ext["KHR_materials_specular"]
+        if "specularColorFactor" in sp:
+            specular = _lum(sp["specularColorFactor"])
+        elif "specularFactor" in sp:
+            specular = sp["specularFactor"]
+    clearcoat = ext["KHR_materials_clearcoat"].get("clearcoatFactor", 0.0) if "KHR_materials_clearcoat" in ext else 0.0
+    return scenes.material_record(base_color=pbr.get("baseColorFactor", [1, 1, 1, 1])[:3], specular=specular,
+                                  emission=emission, sheen=0.0, metallic=pbr.get("metallicFactor", 1.0),
+                                  roughness=pbr.get("roughnessFactor", 1.0), anisotropic=0.0, subsurface=0.0,
+                                  clearcoat=clearcoat, clearcoat_gloss=0.0, transmission=_lum([t, t, t]), eta=eta)
+
+
+def gltf_node_transform(node):
+    """TransformData of a glTF node (load_gltf.cpp:267-277): translation,
+    rotation quaternion normalised as quatf.h:64-67 does and scale, composed as
+    the host TransformData(t, r, s) constructor does (transform.h:61-64:
+    Eigen's Quaternion::toRotationMatrix times Scaling(s)), or the top 3x4 of
+    the column-major matrix; the identity when it has neither."""
+    if any(k in node for k in ("translation", "rotation", "scale")):
+        t = np.asarray(node.get("translation", [0, 0, 0]), np.float32)
+        s = np.asarray(node.get("scale", [1, 1, 1]), np.float32)
+        q = np.asarray(node.get("rotation", [0, 0, 0, 1]), np.float32)
+        q = q * (np.float32(1) / np.sqrt(np.float32(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]))
+        x, y, z, w = q
+        tx, ty, tz = np.float32(2) * x, np.float32(2) * y, np.float32(2) * z
+        twx, twy, twz = tx * w, ty * w, tz * w
+        txx, txy, txz = tx * x, ty * x, tz * x
+        tyy, tyz, tzz = ty * y, tz * y, tz * z
+        one = np.float32(1)
+        R = np.array([[one - (tyy + tzz), txy - twz, txz + twy],
+                      [txy + twz, one - (txx + tzz), tyz - twx],
+                      [txz - twy, tyz + twx, one - (txx + tyy)]], np.float32)
+        M = np.zeros((3, 4), np.float32)
+        M[:, :3] = R * s[None, :]
+        M[:, 3] = t
+        return M
+    if "matrix" in node:
+        return np.asarray(node["matrix"], np.float64).reshape(4, 4).T[:3, :].astype(np.float32)
+    return np.eye(3, 4, dtype=np.float32)
+
+
+def _compose(a, b):
+    """a o b for 3x4 affine transforms."""
+    m = np.zeros((3, 4), np.float32)
+    m[:, :3] = a[:, :3] @ b[:, :3]
+    m[:, 3] = a[:, :3] @ b[:, 3] + a[:, 3]
+    return m
+
+
+def read_gltf(path, sd=None):
+    """Adds a glTF 2.0 scene (.gltf or .glb) to a SceneData (new by default),
+    following src/App/loaders/load_gltf.cpp: every node is placed, with its
+    transform composed with its parents' (:297-299), and each triangle
+    primitive of its mesh becomes one mesh instance (positions, normals and
+    the first texture coordinates, indices widened to 32 bits).  Primitives
+    without a material get glTF's default material (base colour 1, metallic 1,
+    roughness 1).  Other primitive modes, meshes beyond 65535 triangles (the
+    16-bit primitive ids of InstanceData) and KHR_lights_punctual sphere lights
+    (procedural spheres, which the renderer does not take) raise ValueError."""
+    sd = sd or scenes.SceneData()
+    doc, buffers = _gltf_document(path)
+    mats = [sd.add_material(_gltf_material(m)) for m in doc.get("materials", [])]
+    default_mat = None
+    meshes = {}  # (mesh, primitive) -> (vertex info, triangle count, material address)
+    for mi, mesh in enumerate(doc.get("meshes", [])):
+        for pi, prim in enumerate(mesh.get("primitives", [])):
+            if prim.get("mode", 4) != 4:
+                raise ValueError(f"glTF mesh {mi} primitive {pi}: only triangle lists are supported")
+            attrs = prim["attributes"]
+            pos = _gltf_accessor(doc, buffers, attrs["POSITION"]).astype(np.float32)
+            nrm = _gltf_accessor(doc, buffers, attrs["NORMAL"]).astype(np.float32) if "NORMAL" in attrs else None
+            uv = _gltf_accessor(doc, buffers, attrs["TEXCOORD_0"]).astype(np.float32) if "TEXCOORD_0" in attrs else None
+            if "indices" in prim:
+                idx = _gltf_accessor(doc, buffers, prim["indices"]).reshape(-1).astype(np.uint32)
+            else:
+                idx = np.arange(len(pos), dtype=np.uint32)
+            if len(idx) % 3 or (len(idx) and int(idx.max()) >= len(pos)):
+                raise ValueError(f"glTF mesh {mi} primitive {pi}: malformed indices")
+            if len(idx) // 3 > 0xFFFF:
+                raise ValueError(f"glTF mesh {mi} primitive {pi}: more than 65535 triangles")
+            if "material" in prim:
+                mat = mats[prim["material"]]
+            else:
+                if default_mat is None:
+                    default_mat = sd.add_material(_gltf_material({}))
+                mat = default_mat
+            vi, nt = sd.add_mesh(pos, idx, normals=nrm, texcoords=uv)
+            meshes[(mi, pi)] = (vi, nt, mat)
+    nodes = doc.get("nodes", [])
+    parent = {}
+    for i, n in enumerate(nodes):
+        for c in n.get("children", []):
+            parent[c] = i
+    world = {}
+
+    def world_of(i):
+        if i not in world:
+            local = gltf_node_transform(nodes[i])
+            world[i] = _compose(world_of(parent[i]), local) if i in parent else local
+        return world[i]
+
+    for i, n in enumerate(nodes):
+        light = n.get("extensions", {}).get("KHR_lights_punctual")
+        if light is not None and "light" in light:
+            l = doc.get("extensions", {}).get("KHR_lights_punctual", {}).get("lights", [])[light["light"]]
+            if l.get("type") == "point" and "radius" in l.get("extras", {}):
+                raise ValueError(f"glTF node {i}: sphere lights (KHR_lights_punctual point with a radius) are not supported")
+        if "mesh" in n and n["mesh"] < len(doc.get("meshes", [])):
+            for pi in range(len(doc["meshes"][n["mesh"]].get("primitives", []))):
+                vi, nt, mat = meshes[(n["mesh"], pi)]
+                sd.add_mesh_instance(vi, nt, mat, world_of(i))
+    return sd

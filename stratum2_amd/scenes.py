@@ -89,17 +89,20 @@ class SceneData:
         self.buffers.append(np.ascontiguousarray(arr).tobytes())
         return len(self.buffers) - 1
 
-    def add_mesh(self, positions, indices, normals=None):
-        """One mesh in its own position/index buffers (u32 indices, f32x3 positions)."""
+    def add_mesh(self, positions, indices, normals=None, texcoords=None):
+        """One mesh in its own buffers (u32 indices, f32x3 positions, optional
+        f32x3 normals and f32x2 texture coordinates)."""
         positions = np.ascontiguousarray(positions, np.float32)
         indices = np.ascontiguousarray(indices, np.uint32)
         ib = self.add_buffer(indices)
         pb = self.add_buffer(positions)
         nb = self.add_buffer(np.ascontiguousarray(normals, np.float32)) if normals is not None else NO_BUFFER
+        tb = self.add_buffer(np.ascontiguousarray(texcoords, np.float32)) if texcoords is not None else NO_BUFFER
         vi = abi.st_mesh_vertex_info()
         vi.packed_buffer_indices[0] = ib | pb << 16
-        vi.packed_buffer_indices[1] = nb | NO_BUFFER << 16
-        vi.packed_strides = 4 | 12 << 8 | (12 if normals is not None else 0) << 16
+        vi.packed_buffer_indices[1] = nb | tb << 16
+        vi.packed_strides = (4 | 12 << 8 | (12 if normals is not None else 0) << 16
+                             | (8 if texcoords is not None else 0) << 24)
         vi.packed_offsets[0] = 0
         vi.packed_offsets[1] = 0
         vi.packed_offsets[2] = 0

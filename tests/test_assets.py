@@ -1,7 +1,7 @@
 """File formats: NanoVDB files against the reference's own NanoVDB IO
 (compiled from the reference tree into oracle/_ref when present), Mitsuba
-.vol conversion, OBJ meshes and Radiance .hdr images (round trips and the
-renders they give)."""
+.vol conversion, OBJ meshes, glTF scenes and Radiance .hdr images (round
+trips and the renders they give)."""
 import ctypes as C
 import os
 
@@ -10,6 +10,8 @@ import pytest
 
 from oracle.oracle import OracleScene, lib
 from stratum2_amd import abi, assets, nanovdb, scenes
+
+import gltf_files
 
 REF_IO = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "libref_nanovdb_io.so")
 
@@ -129,3 +131,127 @@ def test_hdr_round_trip_and_rle(tmp_path):
                         f.write(bytes([n]) + bytes(row[x:x + n]))
                         x += n
     assert np.array_equal(assets.read_hdr(q), back)
+
+
+# ------------------------------------------------------------------ glTF
+def _quat_matrix(axis, angle):
+    """Rodrigues' rotation formula."""
+    k = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * K @ K
+
+
+def test_gltf_transforms_materials_and_vertices(tmp_path):
+    doc, data, src = gltf_files.gltf_content()
+    p = str(tmp_path / "s.gltf")
+    gltf_files.write_gltf(p, doc, data)
+    sd = assets.read_gltf(p)
+    # node 0 -> quad, node 1 -> two icosphere primitives, node 2 -> quad
+    assert len(sd.instances) == 4
+    T0 = np.zeros((3, 4))
+    T0[:, :3] = _quat_matrix([0, 0, 1], np.pi / 4) @ np.diag([2.0, 1.0, 0.5])
+    T0[:, 3] = [1, 2, 3]
+    C = np.array([[0, 0, 2, 0.5], [0, 2, 0, -1], [-2, 0, 0, 3]], np.float64)   # column-major matrix, transposed
+    T1 = np.zeros((3, 4))
+    T1[:, :3] = T0[:, :3] @ C[:, :3]
+    T1[:, 3] = T0[:, :3] @ C[:, 3] + T0[:, 3]
+    np.testing.assert_allclose(sd.transforms[0], T0, atol=2e-7)
+    np.testing.assert_allclose(sd.transforms[1], T1, atol=2e-6)
+    np.testing.assert_allclose(sd.transforms[2], T1, atol=2e-6)
+    assert np.array_equal(sd.transforms[3], np.eye(3, 4, dtype=np.float32))
+    # materials: factors through Scene::makeMetallicRoughnessMaterial and the extensions
+    lum = lambda c: 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]  # noqa: E731
+    m0 = scenes.material_record(base_color=(0.9, 0.5, 0.25), specular=lum([0.2, 0.6, 1.0]), emission=(4.0, 2.0, 0.0),
+                                metallic=0.25, roughness=0.6, clearcoat=0.4, transmission=0.65, eta=1.33)
+    m1 = scenes.material_record(base_color=(0.3, 0.3, 0.8), specular=0.35, metallic=1.0, roughness=1.0)
+    m2 = scenes.material_record(base_color=(1, 1, 1), specular=1.0, metallic=1.0, roughness=1.0)
+    assert [np.array_equal(a, b) for a, b in zip(sd.material_words, [m0, m1, m2])] == [True] * 3
+    addr = [(a >> 4) for a, _ in sd.instances]
+    assert addr == [0, 64, 128, 0]
+    # vertex data round trip: the quad keeps normals and texture coordinates, indices widen to u32
+    qp, qn, quv, qi = src["quad"]
+    vi0, n0 = sd.instances[0][1] & 0xFFFF, sd.instances[0][1] >> 16
+    assert n0 == 2
+    v = sd.vertex_infos[vi0]
+    ib, pb = v.packed_buffer_indices[0] & 0xFFFF, v.packed_buffer_indices[0] >> 16
+    nb, tb = v.packed_buffer_indices[1] & 0xFFFF, v.packed_buffer_indices[1] >> 16
+    assert np.array_equal(np.frombuffer(sd.buffers[ib], np.uint32), qi.astype(np.uint32))
+    assert np.array_equal(np.frombuffer(sd.buffers[pb], np.float32).reshape(-1, 3), qp)
+    assert np.array_equal(np.frombuffer(sd.buffers[nb], np.float32).reshape(-1, 3), qn)
+    assert np.array_equal(np.frombuffer(sd.buffers[tb], np.float32).reshape(-1, 2), quv)
+    assert v.packed_strides == (4 | 12 << 8 | 12 << 16 | 8 << 24)
+    vi1 = sd.instances[1][1] & 0xFFFF
+    assert sd.vertex_infos[vi1].packed_buffer_indices[1] == (scenes.NO_BUFFER | scenes.NO_BUFFER << 16)
+
+
+def test_gltf_and_glb_read_the_same_scene(tmp_path):
+    doc, data, _ = gltf_files.gltf_content(3)
+    a, b = str(tmp_path / "s.gltf"), str(tmp_path / "s.glb")
+    gltf_files.write_gltf(a, doc, data)
+    gltf_files.write_glb(b, doc, data)
+    # external .bin buffer as a third container
+    import json
+    d = json.loads(json.dumps(doc))
+    d["buffers"] = [{"byteLength": len(data), "uri": "s.bin"}]
+    open(str(tmp_path / "s.bin"), "wb").write(data)
+    c = str(tmp_path / "ext.gltf")
+    open(c, "w").write(json.dumps(d))
+    A, B, Cs = assets.read_gltf(a), assets.read_gltf(b), assets.read_gltf(c)
+    for X in (B, Cs):
+        assert X.instances == A.instances
+        assert X.buffers == A.buffers
+        assert all(np.array_equal(x, y) for x, y in zip(X.transforms, A.transforms))
+        assert all(np.array_equal(x, y) for x, y in zip(X.material_words, A.material_words))
+
+
+def test_gltf_scene_renders_like_the_generated_scene(tmp_path):
+    """A glTF holding the Cornell icosphere (translated by a node) renders
+    bit-identically to the same scene built directly."""
+    v, f = scenes.icosphere(2)
+    v = (v * 0.5).astype(np.float32)
+    i = f.reshape(-1).astype(np.uint32)
+    mat = scenes.material_record(base_color=(0.6, 0.5, 0.4), specular=1.0, metallic=0.0, roughness=0.3)
+    a = scenes.SceneData()
+    vi, n = a.add_mesh(v, i)
+    a.add_mesh_instance(vi, n, a.add_material(mat), np.array([[1, 0, 0, 0], [0, 1, 0, 1], [0, 0, 1, 0]], np.float32))
+    a.set_environment(emission=(1.0, 0.9, 0.8))
+    doc = {"asset": {"version": "2.0"},
+           "buffers": [{"byteLength": v.nbytes + i.nbytes, "uri": "ico.bin"}],
+           "bufferViews": [{"buffer": 0, "byteOffset": 0, "byteLength": v.nbytes},
+                           {"buffer": 0, "byteOffset": v.nbytes, "byteLength": i.nbytes}],
+           "accessors": [{"bufferView": 0, "componentType": 5126, "type": "VEC3", "count": len(v)},
+                         {"bufferView": 1, "componentType": 5125, "type": "SCALAR", "count": len(i)}],
+           "materials": [{"pbrMetallicRoughness": {"baseColorFactor": [0.6, 0.5, 0.4, 1], "metallicFactor": 0.0,
+                                                   "roughnessFactor": 0.3}}],
+           "meshes": [{"primitives": [{"attributes": {"POSITION": 0}, "indices": 1, "material": 0}]}],
+           "nodes": [{"translation": [0, 1, 0], "mesh": 0}]}
+    import json
+    open(str(tmp_path / "ico.bin"), "wb").write(v.tobytes() + i.tobytes())
+    p = str(tmp_path / "ico.gltf")
+    open(p, "w").write(json.dumps(doc))
+    b = assets.read_gltf(p)
+    b.set_environment(emission=(1.0, 0.9, 0.8))
+    view = scenes.make_view(32, 32, position=(0, 1, 2.0))
+    params = abi.default_params(32, 32, max_depth=3, spp=2)
+    assert np.array_equal(OracleScene(a).render(view, params)["hdr"], OracleScene(b).render(view, params)["hdr"])
+
+
+def test_gltf_rejects_what_the_renderer_cannot_take(tmp_path):
+    import json
+    doc, data, _ = gltf_files.gltf_content()
+    cases = []
+    d = json.loads(json.dumps(doc))
+    d["meshes"][0]["primitives"][0]["mode"] = 5   # triangle strip
+    cases.append(d)
+    d = json.loads(json.dumps(doc))
+    d["extensions"] = {"KHR_lights_punctual": {"lights": [{"type": "point", "extras": {"radius": 0.1}}]}}
+    d["nodes"][2]["extensions"] = {"KHR_lights_punctual": {"light": 0}}
+    cases.append(d)
+    d = json.loads(json.dumps(doc))
+    d["accessors"][0]["count"] = 1 << 20          # reads past the buffer
+    cases.append(d)
+    for k, d in enumerate(cases):
+        p = str(tmp_path / f"bad{k}.gltf")
+        gltf_files.write_gltf(p, d, data)
+        with pytest.raises(ValueError):
+            assets.read_gltf(p)

@@ -455,3 +455,24 @@ def test_emitter_with_vertex_normals(gpu_ctx, light_trace):
         assert ok.mean() >= 0.99 and abs(hg.mean() - hr.mean()) <= 0.01 * hr.mean()
     else:
         _check_image(g, r)
+
+
+def test_gltf_scene_in_cornell_box(gpu_ctx, tmp_path):
+    """A .glb read into the Cornell box: an emissive quad carrying normals and
+    texture coordinates (16-bit indices, KHR emissive strength) under a TRS
+    node and a transmissive clearcoated icosphere under a matrix child node."""
+    import gltf_files
+    from stratum2_amd import assets
+    doc, data, _ = gltf_files.gltf_content(5)
+    doc["meshes"][1]["primitives"] = doc["meshes"][1]["primitives"][:1]
+    s, c = np.sin(np.pi / 12), np.cos(np.pi / 12)
+    doc["nodes"] = [{"translation": [-0.6, 1.0, -0.6], "rotation": [0.0, s, 0.0, c], "scale": [0.5, 0.4, 1.0],
+                     "mesh": 0, "children": [1]},
+                    {"matrix": [0.6, 0, 0, 0, 0, 0.75, 0, 0, 0, 0, 0.3, 0, 1.4, 0.2, 0.6, 1], "mesh": 1}]
+    p = str(tmp_path / "props.glb")
+    gltf_files.write_glb(p, doc, data)
+    sd = assets.read_gltf(p, sd=scenes.cornell_box(0))
+    W = H = 64
+    params = abi.default_params(W, H, max_depth=4, spp=4)
+    g, r = _render_both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
+    _check_image(g, r)
