@@ -261,6 +261,77 @@ int st_last_timings(st_context* ctx, double out[12]);
 int st_tonemap(st_context* ctx, const float* hdr, const float* albedo, uint32_t width, uint32_t height, float exposure,
                int mode, int gamma_correct, float* out);
 
+/* ---- SVGF denoiser: Denoiser::denoise (src/App/Denoiser.cpp:113-255) and
+ * the kernels of src/Shaders/kernels/svgf/ (temporal_accumulation.slang,
+ * estimate_variance.slang, atrous.slang).  It runs between st_render and
+ * st_tonemap, as TestRenderer::render does (TestRenderer.cpp:656-670). */
+
+/* Compile-time defines of the svgf kernels (Denoiser.cpp:160-168) become flags. */
+#define ST_DENOISE_REPROJECTION      (1u << 0)  /* gReprojection (default on; --noReprojection) */
+#define ST_DENOISE_DEMODULATE_ALBEDO (1u << 1)  /* gDemodulateAlbedo (default on) */
+#define ST_DENOISE_CHECK_NORMAL      (1u << 2)  /* gCheckNormal (default on; --noNormalCheck) */
+#define ST_DENOISE_CHECK_DEPTH       (1u << 3)  /* gCheckDepth (default on; --noDepthCheck) */
+#define ST_DENOISE_SHADOW_HYSTERESIS (1u << 4)  /* gShadowHysteresis (default off) */
+#define ST_DENOISE_DEFAULT (ST_DENOISE_REPROJECTION | ST_DENOISE_DEMODULATE_ALBEDO | ST_DENOISE_CHECK_NORMAL | ST_DENOISE_CHECK_DEPTH)
+
+/* FilterKernelType, src/Shaders/compat/filter_type.h:5-13. */
+#define ST_FILTER_ATROUS 0
+#define ST_FILTER_BOX3 1
+#define ST_FILTER_BOX5 2
+#define ST_FILTER_SUBSAMPLED 3
+#define ST_FILTER_BOX3_SUBSAMPLED 4
+#define ST_FILTER_BOX5_SUBSAMPLED 5
+
+/* Denoiser settings (Denoiser.hpp:41-58; defaults in brackets). */
+typedef struct st_denoise_params {
+    uint32_t flags;                 /* ST_DENOISE_* [ST_DENOISE_DEFAULT] */
+    uint32_t history_limit;         /* mHistoryLimit: cap on the accumulated sample count, 0 = none [0] */
+    uint32_t atrous_iterations;     /* mAtrousIterations: filter passes after accumulation, at most 15 [0] */
+    uint32_t history_tap;           /* mHistoryTap: after pass i == history_tap - 1 the first filter image's
+                                       colour replaces the history colour; 0 = never [0] */
+    uint32_t filter_type;           /* ST_FILTER_* [ST_FILTER_BOX3] */
+    float variance_boost_length;    /* estimate_variance's mVarianceBoostLength.  The reference pushes its
+                                       uint32 default 4 into this float constant, which reads as 5.6e-45
+                                       and leaves the variance unboosted: 0 gives the same result [0] */
+    float sigma_luminance_boost;    /* mSigmaLuminanceBoost [3] */
+    float shadow_preserve_offset;   /* mShadowPreserveOffset [0.1] */
+    float shadow_preserve_scale;    /* mShadowPreserveScale [1] */
+} st_denoise_params;
+
+/* One frame's inputs, all device W*H images as st_render writes them
+ * (DenoiserParameters, src/Shaders/compat/denoiser.h:36-52). */
+typedef struct st_denoise_inputs {
+    const float* radiance;               /* RGBA32F; alpha is the frame's sample weight (1) */
+    const float* albedo;                 /* RGBA32F; read only with ST_DENOISE_DEMODULATE_ALBEDO */
+    const float* prev_uv;                /* RG32F previous-frame pixel coordinate / extent */
+    const uint32_t* visibility;          /* 2 x u32 {instance|primitive, oct-snorm16 normal} */
+    const float* depth;                  /* RGBA32F {depth, previous depth, d/dx, d/dy} */
+    const uint32_t* instance_index_map;  /* device: current instance -> previous frame's index
+                                            (mInstanceIndexMap); null = identity */
+    uint32_t instance_count;             /* entries of the map; indices beyond it map to 0xFFFF */
+} st_denoise_inputs;
+
+typedef struct st_denoiser st_denoiser;
+
+/* History images for one extent (Denoiser's mPrevAccumColor, mPrevAccumMoments,
+ * mPrevVisibility, mPrevDepth and the two filter images). */
+int st_denoiser_create(st_context* ctx, uint32_t width, uint32_t height, st_denoiser** out);
+int st_denoiser_destroy(st_denoiser* d);
+/* Denoiser::resetAccumulation (Denoiser.hpp:30-33): the next call starts over. */
+int st_denoiser_reset(st_denoiser* d);
+/* Denoiser::accumulatedFrames: TestRenderer seeds its frames with it while
+ * denoising (TestRenderer.cpp:468). */
+uint32_t st_denoiser_accumulated_frames(const st_denoiser* d);
+/* Denoiser::denoise: without history (first call, after a reset) the output is
+ * the radiance and the history is cleared; otherwise temporal accumulation,
+ * then, with atrous_iterations > 0, variance estimation and the filter
+ * passes.  The frame's visibility and depth are kept as the next call's
+ * history.  out (device RGBA32F W*H, may equal in->radiance) receives the
+ * output image; with ST_DENOISE_DEMODULATE_ALBEDO it is demodulated, and
+ * st_tonemap with the same albedo restores it. */
+int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in, const st_denoise_params* p,
+               float* out);
+
 #ifdef __cplusplus
 }
 #endif

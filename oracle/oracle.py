@@ -56,6 +56,13 @@ def lib():
         l.or_hg_eval.restype = C.c_float
         l.or_hg_sample.argtypes = [C.c_float, C.c_float, C.c_float, vp, vp]
         l.or_tonemap.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_float, C.c_int, C.c_int, vp, vp]
+        u32 = C.c_uint32
+        pv, pp = C.POINTER(abi.st_view), C.POINTER(abi.st_denoise_params)
+        l.or_svgf_normal_threshold.restype = C.c_float
+        l.or_svgf_accumulate.argtypes = [pv, pp, u32, u32, vp, u32] + [vp] * 11
+        l.or_svgf_variance.argtypes = [pv, pp, u32, u32, vp, u32, vp, vp, vp, vp, vp]
+        l.or_svgf_atrous.argtypes = [pv, pp, u32, u32, vp, vp, u32, vp, vp]
+        l.or_svgf_copy_rgb.argtypes = [u32, u32, vp, vp]
         _lib = l
     return _lib
 
@@ -123,3 +130,61 @@ def tonemap(hdr, albedo=None, exposure=0.0, mode=0, gamma=True):
     if st != 0:
         raise RuntimeError("oracle tonemap: " + lib().or_last_error().decode())
     return out, mx
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+class OracleDenoiser:
+    """Denoiser::denoise (src/App/Denoiser.cpp:113-255) on host arrays, over
+    the kernel restatements of svgf_oracle.c.  Holds the history images
+    (previous accumulated colour and moments, visibility, depth) between
+    calls, as the reference's Denoiser node does."""
+
+    def __init__(self, width, height):
+        self.W, self.H = width, height
+        self.reset_pending = False
+        self.accumulated_frames = 0
+        self.prev = None  # (color, moments, visibility, depth)
+        self.last = {}
+
+    def reset(self):
+        self.reset_pending = True
+        self.accumulated_frames = 0
+
+    def denoise(self, view, radiance, albedo, prev_uv, visibility, depth, params, instance_index_map=None):
+        W, H, L = self.W, self.H, lib()
+        f = lambda a, c: np.ascontiguousarray(np.asarray(a, np.float32).reshape(H, W, c))  # noqa: E731
+        radiance, prev_uv, depth = f(radiance, 4), f(prev_uv, 2), f(depth, 4)
+        albedo = f(albedo, 4) if albedo is not None else np.zeros((H, W, 4), np.float32)
+        vis = np.ascontiguousarray(np.asarray(visibility).view(np.uint32).reshape(H, W, 2))
+        m = np.ascontiguousarray(instance_index_map, np.uint32) if instance_index_map is not None else None
+        mc = len(m) if m is not None else 0
+        color = np.zeros((H, W, 4), np.float32)
+        moments = np.zeros((H, W, 2), np.float32)
+        out = radiance.copy()
+        self.last = {}
+        if not self.reset_pending and self.prev is not None:
+            pc, pm, pvis, pdep = self.prev
+            L.or_svgf_accumulate(C.byref(view), C.byref(params), W, H, _p(m), mc, _p(vis), _p(pvis), _p(depth), _p(pdep),
+                                 _p(prev_uv), _p(radiance), _p(albedo), _p(pc), _p(pm), _p(color), _p(moments))
+            out = color
+            self.last["accum_color"], self.last["accum_moments"] = color.copy(), moments.copy()
+            if params.atrous_iterations > 0:
+                temp = [np.zeros((H, W, 4), np.float32), np.zeros((H, W, 4), np.float32)]
+                L.or_svgf_variance(C.byref(view), C.byref(params), W, H, _p(m), mc, _p(vis), _p(depth), _p(color),
+                                   _p(moments), _p(temp[0]))
+                self.last["variance"] = temp[0].copy()
+                for i in range(params.atrous_iterations):
+                    L.or_svgf_atrous(C.byref(view), C.byref(params), W, H, _p(vis), _p(depth), i, _p(temp[i % 2]),
+                                     _p(temp[(i + 1) % 2]))
+                    if i + 1 == params.history_tap:
+                        L.or_svgf_copy_rgb(W, H, _p(temp[0]), _p(color))
+                out = temp[params.atrous_iterations % 2]
+            self.accumulated_frames += 1
+        else:
+            self.reset_pending = False
+            self.accumulated_frames = 0
+        self.prev = (color, moments, vis.copy(), depth.copy())
+        return out.copy()

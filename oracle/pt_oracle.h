@@ -88,6 +88,21 @@ void or_hg_sample(float g, float r0, float r1, const float din[3], float out[3])
 int or_tonemap(const float* hdr, const float* albedo, uint32_t w, uint32_t h, float exposure, int mode, int gamma, float* out,
                float max_out[4]);
 
+/* SVGF denoiser kernels restated (svgf_oracle.c): one call per kernel over
+ * host images, as st_denoise runs them (Denoiser::denoise restated in
+ * oracle.py). */
+float or_svgf_normal_threshold(void);
+void or_svgf_accumulate(const st_view* view, const st_denoise_params* p, uint32_t W, uint32_t H, const uint32_t* map,
+                        uint32_t map_count, const uint32_t* vis, const uint32_t* prev_vis, const float* depth,
+                        const float* prev_depth, const float* prev_uv, const float* input, const float* albedo,
+                        const float* prev_color, const float* prev_moments, float* color, float* moments);
+void or_svgf_variance(const st_view* view, const st_denoise_params* p, uint32_t W, uint32_t H, const uint32_t* map,
+                      uint32_t map_count, const uint32_t* vis, const float* depth, const float* color,
+                      const float* moments, float* out);
+void or_svgf_atrous(const st_view* view, const st_denoise_params* p, uint32_t W, uint32_t H, const uint32_t* vis,
+                    const float* depth, uint32_t iteration, const float* in, float* out);
+void or_svgf_copy_rgb(uint32_t W, uint32_t H, const float* filtered, float* color);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,6 +123,52 @@ def default_params(width, height, **kw):
     return p
 
 
+# SVGF denoiser (Denoiser.cpp / svgf kernels)
+ST_DENOISE_REPROJECTION = 1 << 0
+ST_DENOISE_DEMODULATE_ALBEDO = 1 << 1
+ST_DENOISE_CHECK_NORMAL = 1 << 2
+ST_DENOISE_CHECK_DEPTH = 1 << 3
+ST_DENOISE_SHADOW_HYSTERESIS = 1 << 4
+ST_DENOISE_DEFAULT = (ST_DENOISE_REPROJECTION | ST_DENOISE_DEMODULATE_ALBEDO | ST_DENOISE_CHECK_NORMAL
+                      | ST_DENOISE_CHECK_DEPTH)
+ST_FILTER_ATROUS, ST_FILTER_BOX3, ST_FILTER_BOX5, ST_FILTER_SUBSAMPLED, ST_FILTER_BOX3_SUBSAMPLED, \
+    ST_FILTER_BOX5_SUBSAMPLED = range(6)
+
+
+class st_denoise_params(C.Structure):
+    _fields_ = [("flags", u32), ("history_limit", u32), ("atrous_iterations", u32), ("history_tap", u32),
+                ("filter_type", u32), ("variance_boost_length", f32), ("sigma_luminance_boost", f32),
+                ("shadow_preserve_offset", f32), ("shadow_preserve_scale", f32)]
+
+
+class st_denoise_inputs(C.Structure):
+    _fields_ = [("radiance", C.c_void_p), ("albedo", C.c_void_p), ("prev_uv", C.c_void_p),
+                ("visibility", C.c_void_p), ("depth", C.c_void_p), ("instance_index_map", C.c_void_p),
+                ("instance_count", u32)]
+
+
+def default_denoise_params(**kw):
+    """Denoiser defaults (src/App/Denoiser.hpp:41-58 and the constructor,
+    Denoiser.cpp:24-31).  The reference's variance boost length (uint32 4
+    pushed into a float constant) reads as 5.6e-45 and boosts nothing; 0 is
+    the same."""
+    p = st_denoise_params()
+    p.flags = ST_DENOISE_DEFAULT
+    p.history_limit = 0
+    p.atrous_iterations = 0
+    p.history_tap = 0
+    p.filter_type = ST_FILTER_BOX3
+    p.variance_boost_length = 0.0
+    p.sigma_luminance_boost = 3.0
+    p.shadow_preserve_offset = 0.1
+    p.shadow_preserve_scale = 1.0
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libstratum2.so")
 
@@ -130,6 +176,7 @@ EXPORTS = [
     "st_last_error", "st_version", "st_context_create", "st_context_destroy", "st_sync",
     "st_scene_create", "st_scene_destroy", "st_scene_stats", "st_render", "st_trace_primary",
     "st_trace_rays", "st_last_timings", "st_tonemap", "st_scene_update",
+    "st_denoiser_create", "st_denoiser_destroy", "st_denoiser_reset", "st_denoiser_accumulated_frames", "st_denoise",
 ]
 
 _lib = None
@@ -158,6 +205,12 @@ def load_library(path=LIB_PATH):
     lib.st_tonemap.argtypes = [vp, vp, vp, u32, u32, C.c_float, C.c_int, C.c_int, vp]
     lib.st_scene_update.argtypes = [vp, vp, C.POINTER(C.c_double)]
     lib.st_last_timings.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.st_denoiser_create.argtypes = [vp, u32, u32, C.POINTER(vp)]
+    lib.st_denoiser_destroy.argtypes = [vp]
+    lib.st_denoiser_reset.argtypes = [vp]
+    lib.st_denoiser_accumulated_frames.argtypes = [vp]
+    lib.st_denoiser_accumulated_frames.restype = u32
+    lib.st_denoise.argtypes = [vp, C.POINTER(st_view), C.POINTER(st_denoise_inputs), C.POINTER(st_denoise_params), vp]
     if path == LIB_PATH:
         _lib = lib
     return lib

@@ -34,6 +34,7 @@
 #include "st_layout.h"
 #include "st_lbvh.h"
 #include "st_shade.h"
+#include "st_svgf.h"
 #include "st_tonemap.h"
 #include "st_trace.h"
 #include "st_volume.h"
@@ -2604,6 +2605,179 @@ int st_tonemap(st_context* ctx, const float* hdr, const float* albedo, uint32_t 
                            reinterpret_cast<float4*>(out));
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+/* ---- SVGF denoiser ------------------------------------------------------ */
+
+}  // extern "C"
+
+// History of one extent: two slots of accumulated colour and moments and of
+// the kept visibility and depth (slot `cur` holds the last frame's), and the
+// two filter images of the a-trous passes.
+struct st_denoiser {
+    st_context* ctx = nullptr;
+    uint32_t W = 0, H = 0;
+    DevBuf color[2], moments[2], vis[2], depth[2], temp[2];
+    int cur = 0;
+    bool have_history = false, reset = false;
+    uint32_t frames = 0;
+};
+
+extern "C" {
+
+int st_denoiser_create(st_context* ctx, uint32_t width, uint32_t height, st_denoiser** out) {
+    if (!ctx || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (width == 0 || height == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty extent");
+    if ((uint64_t)width * height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        auto d = std::make_unique<st_denoiser>();
+        d->ctx = ctx;
+        d->W = width;
+        d->H = height;
+        const size_t n = (size_t)width * height;
+        for (int k = 0; k < 2; k++) {
+            d->color[k].reserve(n * 16);
+            d->moments[k].reserve(n * 8);
+            d->vis[k].reserve(n * 8);
+            d->depth[k].reserve(n * 4);
+            d->temp[k].reserve(n * 16);
+        }
+        *out = d.release();
+        return ST_OK;
+    } catch (const std::bad_alloc&) {
+        return set_error(ST_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_denoiser_destroy(st_denoiser* d) {
+    if (!d) return ST_OK;
+    (void)hipSetDevice(d->ctx->device);
+    delete d;
+    return ST_OK;
+}
+
+int st_denoiser_reset(st_denoiser* d) {
+    if (!d) return set_error(ST_ERROR_INVALID_ARGUMENT, "null denoiser");
+    d->reset = true;
+    d->frames = 0;
+    return ST_OK;
+}
+
+uint32_t st_denoiser_accumulated_frames(const st_denoiser* d) { return d ? d->frames : 0; }
+
+int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in, const st_denoise_params* p,
+               float* out) {
+    if (!d || !view || !in || !p || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!in->radiance || !in->prev_uv || !in->visibility || !in->depth)
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "radiance, prev_uv, visibility and depth are required");
+    if ((p->flags & ST_DENOISE_DEMODULATE_ALBEDO) && !in->albedo)
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "ST_DENOISE_DEMODULATE_ALBEDO needs the albedo image");
+    if (in->instance_index_map && in->instance_count == 0)
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "instance_index_map without instance_count");
+    if (view->image_min[0] < 0 || view->image_min[1] < 0 || view->image_max[0] > (int32_t)d->W ||
+        view->image_max[1] > (int32_t)d->H || view->image_min[0] >= view->image_max[0] ||
+        view->image_min[1] >= view->image_max[1])
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "view bounds must lie inside the denoiser's extent");
+    if (p->filter_type > ST_FILTER_BOX5_SUBSAMPLED) return set_error(ST_ERROR_INVALID_ARGUMENT, "unknown filter type");
+    if (p->atrous_iterations > 15) return set_error(ST_ERROR_LIMIT, "at most 15 filter passes (step 2^14)");
+    try {
+        st_context* ctx = d->ctx;
+        HIP_CHECK(hipSetDevice(ctx->device));
+        hipStream_t st = ctx->stream;
+        const SvgfTiles t = svgf_tiles(d->W, d->H);
+        const dim3 grid(t.per_xcd * kSvgfXcds), block(kSvgfTileX * kSvgfTileY);
+        const SvgfView v{view->image_min[0], view->image_min[1], view->image_max[0], view->image_max[1]};
+        const int c = d->cur ^ 1, pv = d->cur;
+        float4* out4 = reinterpret_cast<float4*>(out);
+        const float4* radiance = reinterpret_cast<const float4*>(in->radiance);
+        const uint2* vis = reinterpret_cast<const uint2*>(in->visibility);
+        const float4* depth = reinterpret_cast<const float4*>(in->depth);
+        if (!d->reset && d->have_history) {
+            const uint32_t iters = p->atrous_iterations;
+            SvgfAccumArgs a{};
+            a.t = t;
+            a.v = v;
+            a.flags = p->flags;
+            a.history_limit = p->history_limit;
+            a.cos_normal = std::cos(2.0f * (180.0f / 3.14159265358979f));  // cos(degrees(2)), a radian argument
+            a.shadow_offset = p->shadow_preserve_offset;
+            a.shadow_scale = p->shadow_preserve_scale;
+            a.map = in->instance_index_map;
+            a.map_count = in->instance_count;
+            a.vis = vis;
+            a.depth = depth;
+            a.prev_uv = reinterpret_cast<const float2*>(in->prev_uv);
+            a.input = radiance;
+            a.albedo = reinterpret_cast<const float4*>(in->albedo);
+            a.prev_vis = d->vis[pv].as<uint2>();
+            a.prev_depth = d->depth[pv].as<float>();
+            a.prev_color = d->color[pv].as<float4>();
+            a.prev_moments = d->moments[pv].as<float2>();
+            a.color = d->color[c].as<float4>();
+            a.moments = d->moments[c].as<float2>();
+            a.keep_vis = d->vis[c].as<uint2>();
+            a.keep_depth = d->depth[c].as<float>();
+            a.out = iters == 0 ? out4 : nullptr;
+            hipLaunchKernelGGL(k_svgf_accumulate, grid, block, 0, st, a);
+            if (iters > 0) {
+                SvgfVarianceArgs va{};
+                va.t = t;
+                va.v = v;
+                va.flags = p->flags;
+                va.history_limit = p->history_limit;
+                va.boost_length = p->variance_boost_length;
+                va.map = in->instance_index_map;
+                va.map_count = in->instance_count;
+                va.vis = d->vis[c].as<uint2>();
+                va.depth = depth;
+                va.color = d->color[c].as<float4>();
+                va.moments = d->moments[c].as<float2>();
+                va.out = d->temp[0].as<float4>();
+                hipLaunchKernelGGL(k_svgf_variance, grid, block, 0, st, va);
+                float4* temp[2] = {d->temp[0].as<float4>(), d->temp[1].as<float4>()};
+                for (uint32_t i = 0; i < iters; i++) {
+                    // the last pass writes the caller's image instead of filter image (i + 1) % 2
+                    float4* dst = i + 1 == iters ? out4 : temp[(i + 1) % 2];
+                    SvgfAtrousArgs aa{};
+                    aa.t = t;
+                    aa.v = v;
+                    aa.flags = p->flags;
+                    aa.filter_type = p->filter_type;
+                    aa.iteration = i;
+                    aa.step = 1 << i;
+                    aa.sigma_boost = p->sigma_luminance_boost;
+                    aa.vis = d->vis[c].as<uint2>();
+                    aa.depth = depth;
+                    aa.in = temp[i % 2];
+                    aa.out = dst;
+                    hipLaunchKernelGGL(k_svgf_atrous, grid, block, 0, st, aa);
+                    if (i + 1 == p->history_tap) {
+                        // copy_rgb reads filter image 0: this pass's output when (i + 1) is even
+                        const float4* f0 = (i + 1) % 2 == 0 ? dst : temp[0];
+                        hipLaunchKernelGGL(k_svgf_copy_rgb, grid, block, 0, st, t, f0, d->color[c].as<float4>());
+                    }
+                }
+            }
+            d->frames++;
+        } else {
+            hipLaunchKernelGGL(k_svgf_restart, grid, block, 0, st, t, vis, depth, radiance, d->color[c].as<float4>(),
+                               d->moments[c].as<float2>(), d->vis[c].as<uint2>(), d->depth[c].as<float>(),
+                               out == in->radiance ? nullptr : out4);
+            d->reset = false;
+            d->frames = 0;
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(st));
+        d->cur = c;
+        d->have_history = true;
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());

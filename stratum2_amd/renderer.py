@@ -123,3 +123,63 @@ class TestRenderer:
         abi.check(self.scene.lib, self.scene.lib.st_render(self.scene.h, C.byref(view), C.byref(params), C.byref(o)),
                   "st_render")
         return res
+
+
+class Denoiser:
+    """The reference's Denoiser node (src/App/Denoiser.hpp:14-59) over
+    st_denoise: temporal accumulation with reprojection, then optional
+    variance estimation and edge-aware filter passes.  Settings are the
+    reference's members (defaults from Denoiser.hpp and Denoiser.cpp:24-31)."""
+
+    def __init__(self, ctx, width, height, **settings):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.width, self.height = width, height
+        self.params = abi.default_denoise_params(**settings)
+        self.h = C.c_void_p()
+        abi.check(self.lib, self.lib.st_denoiser_create(ctx.h, width, height, C.byref(self.h)), "st_denoiser_create")
+
+    def close(self):
+        if self.h:
+            self.lib.st_denoiser_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset_accumulation(self):
+        abi.check(self.lib, self.lib.st_denoiser_reset(self.h), "st_denoiser_reset")
+
+    @property
+    def accumulated_frames(self):
+        return int(self.lib.st_denoiser_accumulated_frames(self.h))
+
+    @property
+    def demodulate_albedo(self):
+        return bool(self.params.flags & abi.ST_DENOISE_DEMODULATE_ALBEDO)
+
+    def denoise(self, view, frame, out=None, instance_index_map=None):
+        """frame: the dict TestRenderer.render returns with want_aux=True
+        (hdr, albedo, prev_uv, visibility, depth).  Returns the output image
+        (a new device tensor unless `out` is given)."""
+        torch = _torch()
+        H, W = self.height, self.width
+        for k, c in (("hdr", 4), ("albedo", 4), ("prev_uv", 2), ("visibility", 2), ("depth", 4)):
+            t = frame.get(k)
+            if t is None:
+                raise ValueError(f"denoise needs frame['{k}'] (render with want_aux=True)")
+            if tuple(t.shape) != (H, W, c) or not t.is_contiguous() or t.device.type != "cuda":
+                raise ValueError(f"frame['{k}'] must be a contiguous ({H}, {W}, {c}) device tensor")
+        if out is None:
+            out = torch.empty((H, W, 4), dtype=torch.float32, device=frame["hdr"].device)
+        m = instance_index_map
+        if m is not None and (m.dtype != torch.int32 or m.device.type != "cuda" or not m.is_contiguous()):
+            raise ValueError("instance_index_map must be a contiguous int32 device tensor")
+        inp = abi.st_denoise_inputs(frame["hdr"].data_ptr(), frame["albedo"].data_ptr(), frame["prev_uv"].data_ptr(),
+                                    frame["visibility"].data_ptr(), frame["depth"].data_ptr(),
+                                    m.data_ptr() if m is not None else None, m.numel() if m is not None else 0)
+        abi.check(self.lib, self.lib.st_denoise(self.h, C.byref(view), C.byref(inp), C.byref(self.params),
+                                                out.data_ptr()), "st_denoise")
+        return out
