@@ -19,6 +19,10 @@ Prints one JSON line with the driver's contract plus:
                 host's cores over a bounded sample of the same frame
 --light-trace times TestRenderer's light-tracing mode on the spheres workload
 instead (light subpaths, one per pixel and sample, connected to the camera).
+--workload svgf times the SVGF denoiser (st_denoise) on a 1920x1080 frame of
+the spheres workload: temporal accumulation with reprojection and albedo
+demodulation, variance estimate, five 5x5 a-trous passes (one step = one
+denoised frame, Mpixels/s).
 --copy-calibration runs a plain device copy of known size (for scaling the
 PMC byte counters) and exits.
 """
@@ -46,7 +50,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-wave", type=int, default=0, help="0: library default")
     ap.add_argument("--no-restir", action="store_true", help="plain NEE instead of ReSTIR DI")
-    ap.add_argument("--workload", choices=["spheres", "cloud"], default="spheres")
+    ap.add_argument("--workload", choices=["spheres", "cloud", "svgf"], default="spheres")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--device-bvh", action="store_true", help="rebuild the BVH on the GPU (st_scene_update) before timing")
     ap.add_argument("--light-trace", action="store_true", help="light tracing (paths start on the lights)")
@@ -66,6 +70,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(log)
+        return
+    if args.workload == "svgf":
+        bench_svgf(args, log)
         return
     cloud = args.workload == "cloud"
     if cloud:
@@ -217,12 +224,111 @@ def main():
     print(json.dumps(line), flush=True)
 
 
+def bench_svgf(args, log):
+    """SVGF denoiser: one step = one st_denoise call on a resident 1920x1080
+    frame (radiance, albedo, previous-frame uv, visibility, depth) rendered
+    once by the library at 1 spp; the history carries over between steps as
+    under a static camera."""
+    import numpy as np
+    import torch
+    from stratum2_amd import abi, scenes
+    from stratum2_amd.renderer import Context, Denoiser, Scene, TestRenderer
+    W, H = args.width, args.height
+    sd = scenes.spheres_field(seed=1)
+    view = scenes.spheres_view(W, H)
+    ctx = Context(0)
+    sc = Scene(ctx, sd)
+    fl = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
+    frame = TestRenderer(sc).render(view, abi.default_params(W, H, max_depth=8, spp=1, flags=fl, rr_start_depth=3),
+                                    want_aux=True)
+    torch.cuda.synchronize()
+    settings = dict(atrous_iterations=5, filter_type=abi.ST_FILTER_ATROUS, history_limit=32, history_tap=1)
+    den = Denoiser(ctx, W, H, **settings)
+    out = torch.empty_like(frame["hdr"])
+    den.denoise(view, frame, out=out)   # the first call starts the history
+    for _ in range(args.warmup):
+        den.denoise(view, frame, out=out)
+    torch.cuda.synchronize()
+    times, tims = [], []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        den.denoise(view, frame, out=out)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - a)
+        tims.append(den.timings())
+        log(f"step: {times[-1] * 1e3:.3f} ms, passes {tims[-1]}")
+    total_s = sum(times)
+    px = W * H
+    value = px * len(times) / total_s / 1e6
+    ms = total_s / len(times) * 1e3
+    # algorithmic bytes per pixel and launch: accumulation reads visibility 8,
+    # depth 16, previous uv 8, radiance 16, albedo 16 and the history at the
+    # reprojected pixel (visibility 8, depth 4, colour 16, moments 8) and writes
+    # colour 16, moments 8, visibility 8, depth 4 and the geometry image 16; the
+    # variance estimate reads colour 16, moments 8, visibility 8, depth 16,
+    # geometry 16 and writes 16; a filter pass reads colour 16, geometry 16,
+    # depth 16 and writes 16 (neighbours are counted once: they are L2 hits of
+    # other pixels' loads).
+    t = tims[-1]
+    classes = {"k_svgf_accumulate": (t["accumulate_ms"], t["accumulate_launches"], 152 * px),
+               "k_svgf_variance": (t["variance_ms"], t["variance_launches"], 80 * px),
+               "k_svgf_atrous": (t["atrous_ms"], t["atrous_launches"], 64 * px * t["atrous_launches"])}
+    dom = max(classes, key=lambda k: classes[k][0])
+    dms, dl, dbytes = classes[dom]
+    achieved = dbytes / (dms / 1e3) / 1e9 if dms > 0 else 0.0
+    traffic, src = measured_traffic(args, dom)
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": src,
+                "bytes_per_launch": round(dbytes / max(1, dl)), "avg_launch_ms": round(dms / max(1, dl), 4),
+                "passes_ms": {k: round(v[0], 4) for k, v in classes.items()}}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = svgf_cpu_baseline(frame, W, H, den.params, log)
+    line = {
+        "metric": "Mpixels/s SVGF 1080p", "value": round(value, 2), "unit": "Mpixels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (a rendered frame of the seeded scene)",
+        "config": {"workload": f"{W}x{H} SVGF: temporal accumulation (reprojection, albedo demodulation, normal and "
+                               "depth tests, history limit 32), variance estimate, 5 a-trous 5x5 passes",
+                   "width": W, "height": H, **settings},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def svgf_cpu_baseline(frame, W, H, params, log, seconds=10.0):
+    """The oracle's denoiser (single-threaded C) on the same frame: a history
+    start, then calls with all passes, repeated for about `seconds`."""
+    from stratum2_amd import scenes
+    from oracle.oracle import OracleDenoiser
+    host = {k: frame[k].cpu().numpy() for k in ("hdr", "albedo", "prev_uv", "visibility", "depth")}
+    view = scenes.make_view(W, H)
+    den = OracleDenoiser(W, H)
+    args = (host["hdr"], host["albedo"], host["prev_uv"], host["visibility"], host["depth"], params)
+    den.denoise(view, *args)
+    n, a = 0, time.perf_counter()
+    while True:
+        den.denoise(view, *args)
+        n += 1
+        dt = time.perf_counter() - a
+        if dt >= seconds or n >= 200:
+            break
+    log(f"cpu baseline: SVGF on {W}x{H}, {n} frames in {dt:.2f}s")
+    return {"value": round(W * H * n / dt / 1e6, 4), "unit": "Mpixels/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (C restatement of the svgf kernels, one thread), the same {W}x{H} frame, "
+                      f"{n} calls after the history start in {dt:.1f} s"}
+
+
 def measured_traffic(args, kernel):
     """HBM bytes per launch of `kernel` from the PMC summary committed for
     this workload (profiles/hbm_traffic_<workload>.json: FETCH_SIZE and
     WRITE_SIZE per dispatch, scaled by the copy calibration)."""
-    name = "cloud" if args.workload == "cloud" else ("spheres_light" if args.light_trace else
-                                                      ("spheres_plain" if args.no_restir else "spheres_restir"))
+    if args.workload in ("cloud", "svgf"):
+        name = args.workload
+    else:
+        name = "spheres_light" if args.light_trace else ("spheres_plain" if args.no_restir else "spheres_restir")
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{name}.json")
     if not os.path.exists(path):
         return None, None

@@ -331,6 +331,12 @@ uint32_t st_denoiser_accumulated_frames(const st_denoiser* d);
  * st_tonemap with the same albedo restores it. */
 int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in, const st_denoise_params* p,
                float* out);
+/* GPU time of the last st_denoise call per pass, as the reference's
+ * ProfilerScopes record it (Denoiser.cpp:186,204,219): out[0..3] = ms of
+ * accumulation (or the history restart), variance estimate, filter passes and
+ * history copy; out[4..7] = their launch counts.  HIP events on the context's
+ * stream. */
+int st_denoiser_last_timings(const st_denoiser* d, double out[8]);
 
 #ifdef __cplusplus
 }

@@ -177,6 +177,7 @@ EXPORTS = [
     "st_scene_create", "st_scene_destroy", "st_scene_stats", "st_render", "st_trace_primary",
     "st_trace_rays", "st_last_timings", "st_tonemap", "st_scene_update",
     "st_denoiser_create", "st_denoiser_destroy", "st_denoiser_reset", "st_denoiser_accumulated_frames", "st_denoise",
+    "st_denoiser_last_timings",
 ]
 
 _lib = None
@@ -211,6 +212,7 @@ def load_library(path=LIB_PATH):
     lib.st_denoiser_accumulated_frames.argtypes = [vp]
     lib.st_denoiser_accumulated_frames.restype = u32
     lib.st_denoise.argtypes = [vp, C.POINTER(st_view), C.POINTER(st_denoise_inputs), C.POINTER(st_denoise_params), vp]
+    lib.st_denoiser_last_timings.argtypes = [vp, C.POINTER(C.c_double)]
     if path == LIB_PATH:
         _lib = lib
     return lib

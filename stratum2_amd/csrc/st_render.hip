@@ -2618,13 +2618,34 @@ int st_tonemap(st_context* ctx, const float* hdr, const float* albedo, uint32_t 
 // History of one extent: two slots of accumulated colour and moments and of
 // the kept visibility and depth (slot `cur` holds the last frame's), and the
 // two filter images of the a-trous passes.
+// Kernel classes timed per call: accumulation (or restart), variance, filter
+// passes, history copy.
+constexpr int kSvgfClasses = 4, kSvgfMaxEvents = 20;
+
 struct st_denoiser {
     st_context* ctx = nullptr;
     uint32_t W = 0, H = 0;
-    DevBuf color[2], moments[2], vis[2], depth[2], temp[2];
+    DevBuf color[2], moments[2], vis[2], depth[2], temp[2], geo;
     int cur = 0;
     bool have_history = false, reset = false;
     uint32_t frames = 0;
+    hipEvent_t ev[2 * kSvgfMaxEvents] = {};
+    int ev_class[kSvgfMaxEvents] = {};
+    int nev = 0;
+    double timings[2 * kSvgfClasses] = {};
+    ~st_denoiser() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    // Brackets one kernel launch on `s` with the next event pair.
+    template <class F>
+    void timed(int cls, hipStream_t s, F&& launch) {
+        const int k = nev++;
+        ev_class[k] = cls;
+        HIP_CHECK(hipEventRecord(ev[2 * k], s));
+        launch();
+        HIP_CHECK(hipEventRecord(ev[2 * k + 1], s));
+    }
 };
 
 extern "C" {
@@ -2648,6 +2669,8 @@ int st_denoiser_create(st_context* ctx, uint32_t width, uint32_t height, st_deno
             d->depth[k].reserve(n * 4);
             d->temp[k].reserve(n * 16);
         }
+        d->geo.reserve(n * 16);
+        for (hipEvent_t& e : d->ev) HIP_CHECK(hipEventCreate(&e));
         *out = d.release();
         return ST_OK;
     } catch (const std::bad_alloc&) {
@@ -2673,6 +2696,12 @@ int st_denoiser_reset(st_denoiser* d) {
 
 uint32_t st_denoiser_accumulated_frames(const st_denoiser* d) { return d ? d->frames : 0; }
 
+int st_denoiser_last_timings(const st_denoiser* d, double out[8]) {
+    if (!d || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 2 * kSvgfClasses; i++) out[i] = d->timings[i];
+    return ST_OK;
+}
+
 int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in, const st_denoise_params* p,
                float* out) {
     if (!d || !view || !in || !p || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
@@ -2696,6 +2725,7 @@ int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in,
         const dim3 grid(t.per_xcd * kSvgfXcds), block(kSvgfTileX * kSvgfTileY);
         const SvgfView v{view->image_min[0], view->image_min[1], view->image_max[0], view->image_max[1]};
         const int c = d->cur ^ 1, pv = d->cur;
+        d->nev = 0;
         float4* out4 = reinterpret_cast<float4*>(out);
         const float4* radiance = reinterpret_cast<const float4*>(in->radiance);
         const uint2* vis = reinterpret_cast<const uint2*>(in->visibility);
@@ -2725,8 +2755,9 @@ int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in,
             a.moments = d->moments[c].as<float2>();
             a.keep_vis = d->vis[c].as<uint2>();
             a.keep_depth = d->depth[c].as<float>();
+            a.geo = iters == 0 ? nullptr : d->geo.as<float4>();
             a.out = iters == 0 ? out4 : nullptr;
-            hipLaunchKernelGGL(k_svgf_accumulate, grid, block, 0, st, a);
+            d->timed(0, st, [&] { hipLaunchKernelGGL(k_svgf_accumulate, grid, block, 0, st, a); });
             if (iters > 0) {
                 SvgfVarianceArgs va{};
                 va.t = t;
@@ -2738,10 +2769,11 @@ int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in,
                 va.map_count = in->instance_count;
                 va.vis = d->vis[c].as<uint2>();
                 va.depth = depth;
+                va.geo = d->geo.as<float4>();
                 va.color = d->color[c].as<float4>();
                 va.moments = d->moments[c].as<float2>();
                 va.out = d->temp[0].as<float4>();
-                hipLaunchKernelGGL(k_svgf_variance, grid, block, 0, st, va);
+                d->timed(1, st, [&] { hipLaunchKernelGGL(k_svgf_variance, grid, block, 0, st, va); });
                 float4* temp[2] = {d->temp[0].as<float4>(), d->temp[1].as<float4>()};
                 for (uint32_t i = 0; i < iters; i++) {
                     // the last pass writes the caller's image instead of filter image (i + 1) % 2
@@ -2754,28 +2786,39 @@ int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in,
                     aa.iteration = i;
                     aa.step = 1 << i;
                     aa.sigma_boost = p->sigma_luminance_boost;
-                    aa.vis = d->vis[c].as<uint2>();
+                    aa.geo = d->geo.as<float4>();
                     aa.depth = depth;
                     aa.in = temp[i % 2];
                     aa.out = dst;
-                    hipLaunchKernelGGL(k_svgf_atrous, grid, block, 0, st, aa);
+                    d->timed(2, st, [&] { hipLaunchKernelGGL(k_svgf_atrous, grid, block, 0, st, aa); });
                     if (i + 1 == p->history_tap) {
                         // copy_rgb reads filter image 0: this pass's output when (i + 1) is even
                         const float4* f0 = (i + 1) % 2 == 0 ? dst : temp[0];
-                        hipLaunchKernelGGL(k_svgf_copy_rgb, grid, block, 0, st, t, f0, d->color[c].as<float4>());
+                        d->timed(3, st, [&] {
+                            hipLaunchKernelGGL(k_svgf_copy_rgb, grid, block, 0, st, t, f0, d->color[c].as<float4>());
+                        });
                     }
                 }
             }
             d->frames++;
         } else {
-            hipLaunchKernelGGL(k_svgf_restart, grid, block, 0, st, t, vis, depth, radiance, d->color[c].as<float4>(),
-                               d->moments[c].as<float2>(), d->vis[c].as<uint2>(), d->depth[c].as<float>(),
-                               out == in->radiance ? nullptr : out4);
+            d->timed(0, st, [&] {
+                hipLaunchKernelGGL(k_svgf_restart, grid, block, 0, st, t, vis, depth, radiance, d->color[c].as<float4>(),
+                                   d->moments[c].as<float2>(), d->vis[c].as<uint2>(), d->depth[c].as<float>(),
+                                   out == in->radiance ? nullptr : out4);
+            });
             d->reset = false;
             d->frames = 0;
         }
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(st));
+        for (double& x : d->timings) x = 0;
+        for (int k = 0; k < d->nev; k++) {
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, d->ev[2 * k], d->ev[2 * k + 1]));
+            d->timings[d->ev_class[k]] += ms;
+            d->timings[kSvgfClasses + d->ev_class[k]] += 1;
+        }
         d->cur = c;
         d->have_history = true;
         return ST_OK;

@@ -8,7 +8,11 @@
 // keeps visibility and only the depth channel of the frame (the accumulation
 // reads nothing else of the previous frame's depth), double-buffered with the
 // accumulated colour and moments so that accumulation writes the next history
-// while it gathers the previous one.
+// while it gathers the previous one.  Accumulation also writes a geometry
+// image {decoded shading normal, depth}: the variance estimate and the filter
+// passes read a neighbour's normal and depth in one 16-byte load instead of
+// decoding the octahedral normal (five divisions and a square root) at every
+// tap.  The decode is the same function, so the weights are unchanged.
 //
 // Launch shape: 64x4-pixel tiles (one row of 64 pixels per wavefront, so each
 // image load of a wavefront is one contiguous 1 KiB span).  Tiles are handed
@@ -86,6 +90,7 @@ struct SvgfAccumArgs {
     float2* moments;     // accumulated {luminance, luminance^2}
     uint2* keep_vis;     // this frame's visibility and depth, the next call's history
     float* keep_depth;
+    float4* geo;         // {decoded normal, depth} of this frame, for the passes that follow (or null)
     float4* out;         // also receives the colour when no filter pass follows (else null)
 };
 
@@ -98,6 +103,8 @@ __global__ __launch_bounds__(256) void k_svgf_accumulate(SvgfAccumArgs a) {
     const float4 dep = a.depth[i];
     a.keep_vis[i] = vis;
     a.keep_depth[i] = dep.x;
+    const v3 n = unpack_normal(vis.y);
+    if (a.geo) a.geo[i] = make_float4(n.x, n.y, n.z, dep.x);
 
     float4 cp = make_float4(0, 0, 0, 0);
     float mp0 = 0, mp1 = 0, sum_w = 0;
@@ -111,7 +118,6 @@ __global__ __launch_bounds__(256) void k_svgf_accumulate(SvgfAccumArgs a) {
             const bool far = !(fabsf(px) < 16777216.0f && fabsf(py) < 16777216.0f);
             const int ix = far ? 0 : (int)px, iy = far ? 0 : (int)py;  // int2(): truncation
             const float wx = px - floorf(px), wy = py - floorf(py);    // frac()
-            const v3 n = unpack_normal(vis.y);
             const float dd = sqrtf(dep.z * dep.z + dep.w * dep.w);
             const uint32_t want = svgf_map(a.map, a.map_count, inst);
             for (int yy = 0; yy <= 1 && !far; yy++)
@@ -214,6 +220,7 @@ struct SvgfVarianceArgs {
     uint32_t map_count;
     const uint2* vis;
     const float4* depth;
+    const float4* geo;
     const float4* color;
     const float2* moments;
     float4* out;
@@ -234,7 +241,8 @@ __global__ __launch_bounds__(256) void k_svgf_variance(SvgfVarianceArgs a) {
         return;
     }
     const float4 dep = a.depth[i];
-    const v3 n = unpack_normal(vis.y);
+    const float4 g = a.geo[i];
+    const v3 n = mk(g.x, g.y, g.z);
     const uint32_t want = svgf_map(a.map, a.map_count, inst);
     float sum_w = 1;
     const int r = histlen > 1 ? 2 : 3;
@@ -244,14 +252,14 @@ __global__ __launch_bounds__(256) void k_svgf_variance(SvgfVarianceArgs a) {
             const int qx = x + xx, qy = y + yy;
             if (!svgf_inside(a.v, qx, qy)) continue;
             const size_t j = (size_t)qy * a.t.W + qx;
-            const uint2 pv = a.vis[j];
-            if (want != (pv.x & 0xFFFFu)) continue;
+            if (want != (a.vis[j].x & 0xFFFFu)) continue;
+            const float4 gj = a.geo[j];
             float w_z = 0, w_n = 1;
             if (a.flags & ST_DENOISE_CHECK_DEPTH) {
                 const float gx = dep.z * (float)xx, gy = dep.w * (float)yy;
-                w_z = fabsf(a.depth[j].x - dep.x) / (sqrtf(gx * gx + gy * gy) + 1e-2f);
+                w_z = fabsf(gj.w - dep.x) / (sqrtf(gx * gx + gy * gy) + 1e-2f);
             }
-            if (a.flags & ST_DENOISE_CHECK_NORMAL) w_n = powf(sat(dot(unpack_normal(pv.y), n)), 128.0f);
+            if (a.flags & ST_DENOISE_CHECK_NORMAL) w_n = powf(sat(dot(mk(gj.x, gj.y, gj.z), n)), 128.0f);
             const float w = expf(-w_z) * w_n;
             if (!isfinite(w)) continue;
             const float2 mj = a.moments[j];
@@ -280,7 +288,7 @@ struct SvgfAtrousArgs {
     uint32_t flags, filter_type, iteration;
     int step;
     float sigma_boost;
-    const uint2* vis;
+    const float4* geo;
     const float4* depth;
     const float4* in;
     float4* out;
@@ -291,8 +299,7 @@ struct SvgfTap {
     SvgfView v;
     uint32_t W, flags, iteration;
     int step;
-    const uint2* vis;
-    const float4* depth;
+    const float4* geo;
     const float4* in;
     int x, y;
     v3 n;
@@ -308,12 +315,15 @@ struct SvgfTap {
         const float lp = lum(mk(c.x, c.y, c.z));
         const float w_l = fabsf(lp - l) / fmax_(sigma_l, 1e-10f);
         float w_z = 0, w_n = 1;
-        if (flags & ST_DENOISE_CHECK_DEPTH) {
-            // the offset already carries the step; the reference scales it again
-            const float gx = dzx * (float)(ox * step), gy = dzy * (float)(oy * step);
-            w_z = fabsf(depth[j].x - z) / (sqrtf(gx * gx + gy * gy) + 1e-2f);
+        if (flags & (ST_DENOISE_CHECK_DEPTH | ST_DENOISE_CHECK_NORMAL)) {
+            const float4 g = geo[j];
+            if (flags & ST_DENOISE_CHECK_DEPTH) {
+                // the offset already carries the step; the reference scales it again
+                const float gx = dzx * (float)(ox * step), gy = dzy * (float)(oy * step);
+                w_z = fabsf(g.w - z) / (sqrtf(gx * gx + gy * gy) + 1e-2f);
+            }
+            if (flags & ST_DENOISE_CHECK_NORMAL) w_n = powf(fmax_(0.0f, dot(mk(g.x, g.y, g.z), n)), 256.0f);
         }
-        if (flags & ST_DENOISE_CHECK_NORMAL) w_n = powf(fmax_(0.0f, dot(unpack_normal(vis[j].y), n)), 256.0f);
         const float w = expf(-(w_l * w_l) - w_z) * kw * w_n;
         if (!isfinite(w)) return;
         sum.x += c.x * w;
@@ -365,12 +375,12 @@ __global__ __launch_bounds__(256) void k_svgf_atrous(SvgfAtrousArgs a) {
     t.flags = a.flags;
     t.iteration = a.iteration;
     t.step = a.step;
-    t.vis = a.vis;
-    t.depth = a.depth;
+    t.geo = a.geo;
     t.in = a.in;
     t.x = x;
     t.y = y;
-    t.n = unpack_normal(a.vis[i].y);
+    const float4 g = a.geo[i];
+    t.n = mk(g.x, g.y, g.z);
     const float4 dep = a.depth[i];
     t.z = dep.x;
     t.dzx = dep.z;

@@ -156,6 +156,14 @@ class Denoiser:
     def accumulated_frames(self):
         return int(self.lib.st_denoiser_accumulated_frames(self.h))
 
+    def timings(self):
+        """Per-pass GPU ms and launch counts of the last denoise call."""
+        out = (C.c_double * 8)()
+        abi.check(self.lib, self.lib.st_denoiser_last_timings(self.h, out), "st_denoiser_last_timings")
+        names = ["accumulate", "variance", "atrous", "copy_rgb"]
+        return {**{f"{n}_ms": out[i] for i, n in enumerate(names)},
+                **{f"{n}_launches": int(out[4 + i]) for i, n in enumerate(names)}}
+
     @property
     def demodulate_albedo(self):
         return bool(self.params.flags & abi.ST_DENOISE_DEMODULATE_ALBEDO)
