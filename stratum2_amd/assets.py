@@ -12,6 +12,10 @@
   polygons fanned into triangles, optional .mtl diffuse colours).
 * Radiance RGBE images (.hdr) for environment maps, read with both scanline
   encodings (flat and run-length), written flat.
+* OpenEXR images (.exr) for environment maps, read as src/Core/Image.cpp:57-72
+  loads them (tinyexr's LoadEXR: RGBA float32), for single-part scanline files
+  with NONE, RLE, ZIPS or ZIP compression and HALF or FLOAT channels; written
+  uncompressed or ZIP.
 * glTF 2.0 scenes (.gltf with external or data-URI buffers, .glb) -> SceneData,
   as src/App/loaders/load_gltf.cpp maps them: metallic-roughness materials,
   triangle primitives, node TRS / matrix transforms down the node hierarchy.
@@ -497,3 +501,172 @@ def read_gltf(path, sd=None):
                 vi, nt, mat = meshes[(n["mesh"], pi)]
                 sd.add_mesh_instance(vi, nt, mat, world_of(i))
     return sd
+
+
+# ---------------------------------------------------------------------- EXR
+EXR_NONE, EXR_RLE, EXR_ZIPS, EXR_ZIP, EXR_PIZ = 0, 1, 2, 3, 4
+_EXR_LINES = {EXR_NONE: 1, EXR_RLE: 1, EXR_ZIPS: 1, EXR_ZIP: 16}
+_EXR_HALF, _EXR_FLOAT, _EXR_UINT = 1, 2, 0
+_EXR_DTYPE = {_EXR_UINT: np.dtype("<u4"), _EXR_HALF: np.dtype("<f2"), _EXR_FLOAT: np.dtype("<f4")}
+
+
+def _exr_unpredict(buf):
+    """Undo the byte predictor and the even/odd split of the RLE and ZIP
+    codecs (OpenEXR's ImfZip / ImfRle reconstruction)."""
+    t = np.frombuffer(buf, np.uint8)
+    t = (np.cumsum(t.astype(np.int64)) - 128 * np.arange(len(t))) % 256  # t[i] = t[i-1] + t[i] - 128
+    t = t.astype(np.uint8)
+    half = (len(t) + 1) // 2
+    out = np.empty(len(t), np.uint8)
+    out[0::2] = t[:half]
+    out[1::2] = t[half:]
+    return out.tobytes()
+
+
+def _exr_rle_decode(data, expect):
+    out = bytearray()
+    i = 0
+    while i < len(data) and len(out) < expect:
+        n = struct.unpack_from("b", data, i)[0]
+        i += 1
+        if n < 0:
+            out += data[i:i - n]
+            i -= n
+        else:
+            out += data[i:i + 1] * (n + 1)
+            i += 1
+    if len(out) != expect:
+        raise ValueError("EXR: RLE block decodes to the wrong size")
+    return bytes(out)
+
+
+def read_exr(path):
+    """OpenEXR image as float32 (H, W, 4), as tinyexr's LoadEXR returns it
+    (extern/tiny_exr.h:5850-6080): channel names taken after their last '.',
+    R, G and B looked up among the first four channels in file order (A, or 1
+    when absent), a single channel copied into all four; HALF read as float,
+    UINT bits reinterpreted as float.  Tiled, multi-part, deep and PIZ, PXR24
+    or B44 files raise ValueError."""
+    with open(path, "rb") as f:
+        data = f.read()
+    if len(data) < 8 or struct.unpack_from("<I", data, 0)[0] != 20000630:
+        raise ValueError(f"{path}: not an OpenEXR file")
+    version = struct.unpack_from("<I", data, 4)[0]
+    if version & 0x200:
+        raise ValueError(f"{path}: tiled EXR images are not supported")
+    if version & 0x1800:
+        raise ValueError(f"{path}: multi-part and deep EXR images are not supported")
+    pos, attrs = 8, {}
+    while True:
+        end = data.index(b"\0", pos)
+        name = data[pos:end].decode()
+        pos = end + 1
+        if not name:
+            break
+        end = data.index(b"\0", pos)
+        typ = data[pos:end].decode()
+        size = struct.unpack_from("<i", data, end + 1)[0]
+        attrs[name] = (typ, data[end + 5:end + 5 + size])
+        pos = end + 5 + size
+    channels = []
+    raw = attrs["channels"][1]
+    p = 0
+    while raw[p] != 0:
+        end = raw.index(b"\0", p)
+        cname = raw[p:end].decode()
+        ptype, _, xs, ys = struct.unpack_from("<iB3xii", raw, end + 1)
+        if (xs, ys) != (1, 1):
+            raise ValueError(f"{path}: subsampled channels are not supported")
+        channels.append((cname, ptype))
+        p = end + 17
+    comp = attrs["compression"][1][0]
+    if comp not in _EXR_LINES:
+        raise ValueError(f"{path}: EXR compression {comp} is not supported (NONE, RLE, ZIPS, ZIP are)")
+    x0, y0, x1, y1 = struct.unpack("<4i", attrs["dataWindow"][1])
+    W, H = x1 - x0 + 1, y1 - y0 + 1
+    lines = _EXR_LINES[comp]
+    nblocks = (H + lines - 1) // lines
+    offsets = struct.unpack_from(f"<{nblocks}Q", data, pos)
+    planes = [np.zeros((H, W), _EXR_DTYPE[t]) for _, t in channels]
+    row_bytes = sum(W * _EXR_DTYPE[t].itemsize for _, t in channels)
+    for off in offsets:
+        y, size = struct.unpack_from("<ii", data, off)
+        ny = min(lines, y1 - y + 1)
+        expect = ny * row_bytes
+        blk = data[off + 8:off + 8 + size]
+        if size < expect:
+            if comp in (EXR_ZIPS, EXR_ZIP):
+                blk = _exr_unpredict(zlib.decompress(blk))
+            elif comp == EXR_RLE:
+                blk = _exr_unpredict(_exr_rle_decode(blk, expect))
+        if len(blk) != expect:
+            raise ValueError(f"{path}: EXR block at y={y} has {len(blk)} bytes, expected {expect}")
+        q = 0
+        for r in range(ny):
+            for c, (_, t) in enumerate(channels):
+                n = W * _EXR_DTYPE[t].itemsize
+                planes[c][y - y0 + r] = np.frombuffer(blk, _EXR_DTYPE[t], W, q)
+                q += n
+    fl = [pl.view(np.float32) if t == _EXR_UINT else pl.astype(np.float32) for pl, (_, t) in zip(planes, channels)]
+    short = [n.rsplit(".", 1)[-1] for n, _ in channels]
+    out = np.empty((H, W, 4), np.float32)
+    if len(channels) == 1:
+        out[...] = fl[0][..., None]
+        return out
+    idx = {n: i for i, n in enumerate(short[:4])}
+    for k in "RGB":
+        if k not in idx:
+            raise ValueError(f"{path}: {k} channel not found")
+    out[..., 0], out[..., 1], out[..., 2] = fl[idx["R"]], fl[idx["G"]], fl[idx["B"]]
+    out[..., 3] = fl[idx["A"]] if "A" in idx else 1.0
+    return out
+
+
+def _exr_attr(name, typ, value):
+    return name.encode() + b"\0" + typ.encode() + b"\0" + struct.pack("<i", len(value)) + value
+
+
+def write_exr(path, img, compression=EXR_ZIP, half=False):
+    """An (H, W, 3|4) image as a single-part scanline OpenEXR file with
+    channels A (if present), B, G, R in HALF or FLOAT and NONE or ZIP
+    compression."""
+    img = np.asarray(img, np.float32)
+    H, W, C = img.shape
+    if compression not in (EXR_NONE, EXR_ZIP):
+        raise ValueError("write_exr writes NONE or ZIP")
+    names = (["A"] if C == 4 else []) + ["B", "G", "R"]
+    src = {"R": 0, "G": 1, "B": 2, "A": 3}
+    ptype = _EXR_HALF if half else _EXR_FLOAT
+    dt = _EXR_DTYPE[ptype]
+    chl = b"".join(n.encode() + b"\0" + struct.pack("<iB3xii", ptype, 0, 1, 1) for n in names) + b"\0"
+    hdr = (struct.pack("<II", 20000630, 2)
+           + _exr_attr("channels", "chlist", chl)
+           + _exr_attr("compression", "compression", bytes([compression]))
+           + _exr_attr("dataWindow", "box2i", struct.pack("<4i", 0, 0, W - 1, H - 1))
+           + _exr_attr("displayWindow", "box2i", struct.pack("<4i", 0, 0, W - 1, H - 1))
+           + _exr_attr("lineOrder", "lineOrder", b"\0")
+           + _exr_attr("pixelAspectRatio", "float", struct.pack("<f", 1.0))
+           + _exr_attr("screenWindowCenter", "v2f", struct.pack("<2f", 0, 0))
+           + _exr_attr("screenWindowWidth", "float", struct.pack("<f", 1.0)) + b"\0")
+    lines = _EXR_LINES[compression]
+    blocks = []
+    for y in range(0, H, lines):
+        rows = img[y:y + lines]
+        raw = b"".join(np.ascontiguousarray(rows[r, :, src[n]]).astype(dt).tobytes()
+                       for r in range(rows.shape[0]) for n in names)
+        if compression == EXR_ZIP:
+            b = np.frombuffer(raw, np.uint8)
+            inter = np.concatenate([b[0::2], b[1::2]]).astype(np.int64)
+            pred = inter.copy()
+            pred[1:] = (inter[1:] - inter[:-1] + 128) % 256
+            z = zlib.compress(pred.astype(np.uint8).tobytes(), 6)
+            if len(z) < len(raw):
+                raw = z
+        blocks.append(struct.pack("<ii", y, len(raw)) + raw)
+    table_start = len(hdr)
+    offs, at = [], table_start + 8 * len(blocks)
+    for b in blocks:
+        offs.append(at)
+        at += len(b)
+    with open(path, "wb") as f:
+        f.write(hdr + struct.pack(f"<{len(offs)}Q", *offs) + b"".join(blocks))

@@ -1,7 +1,8 @@
 """File formats: NanoVDB files against the reference's own NanoVDB IO
 (compiled from the reference tree into oracle/_ref when present), Mitsuba
-.vol conversion, OBJ meshes, glTF scenes and Radiance .hdr images (round
-trips and the renders they give)."""
+.vol conversion, OBJ meshes, glTF scenes, Radiance .hdr images and OpenEXR
+images (against the reference's own tinyexr, compiled into oracle/_ref when
+present; round trips and the renders they give)."""
 import ctypes as C
 import os
 
@@ -255,3 +256,81 @@ def test_gltf_rejects_what_the_renderer_cannot_take(tmp_path):
         gltf_files.write_gltf(p, d, data)
         with pytest.raises(ValueError):
             assets.read_gltf(p)
+
+
+# ------------------------------------------------------------------- EXR
+REF_EXR = os.path.join(os.path.dirname(REF_IO), "libref_tinyexr.so")
+
+
+def _ref_exr():
+    if not os.path.exists(REF_EXR):
+        pytest.skip("oracle/_ref/libref_tinyexr.so not built (reference tree absent)")
+    L = C.CDLL(REF_EXR)
+    L.ref_exr_load.argtypes = [C.c_char_p, C.c_void_p, C.c_ulonglong, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.ref_exr_save.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_int]
+    return L
+
+
+def _ref_load(L, path, cap=1 << 22):
+    out = np.zeros(cap, np.float32)
+    w, h = C.c_int(), C.c_int()
+    assert L.ref_exr_load(path.encode(), out.ctypes.data, cap, C.byref(w), C.byref(h)) == 0
+    return out[:4 * w.value * h.value].reshape(h.value, w.value, 4)
+
+
+@pytest.mark.parametrize("names", ["ABGR", "BGR", "Y", "ABGRZ"])
+@pytest.mark.parametrize("comp", [assets.EXR_NONE, assets.EXR_RLE, assets.EXR_ZIPS, assets.EXR_ZIP])
+@pytest.mark.parametrize("half", [False, True])
+def test_exr_files_written_by_the_reference_loader_library(tmp_path, names, comp, half):
+    """Files written by tinyexr (the reference's own EXR library) with each
+    compression read the same through assets.read_exr and through LoadEXR."""
+    L = _ref_exr()
+    rng = np.random.default_rng(len(names) * 10 + comp)
+    H, W = 45, 70   # several 16-line ZIP blocks, the last one short
+    planes = rng.gamma(0.5, 3.0, (len(names), H, W)).astype(np.float32)
+    planes[:, 10:20, 5:40] = planes[:, 10, 5][:, None, None]   # runs for RLE
+    p = str(tmp_path / f"t_{names}_{comp}_{half}.exr")
+    assert L.ref_exr_save(p.encode(), planes.ctypes.data, W, H, names.encode(), comp, 1 if half else 2) == 0
+    mine, ref = assets.read_exr(p), _ref_load(L, p)
+    assert mine.shape == (H, W, 4)
+    np.testing.assert_array_equal(mine, ref)
+
+
+def test_exr_written_here_loads_in_the_reference_library(tmp_path):
+    L = _ref_exr()
+    rng = np.random.default_rng(3)
+    img = rng.uniform(0, 50, (33, 41, 4)).astype(np.float32)
+    for comp in (assets.EXR_NONE, assets.EXR_ZIP):
+        for half in (False, True):
+            p = str(tmp_path / f"w{comp}{half}.exr")
+            assets.write_exr(p, img, compression=comp, half=half)
+            want = img.astype(np.float16).astype(np.float32) if half else img
+            np.testing.assert_array_equal(_ref_load(L, p), want)
+            np.testing.assert_array_equal(assets.read_exr(p), want)
+
+
+def test_exr_round_trip_rgb_and_rejections(tmp_path):
+    rng = np.random.default_rng(4)
+    rgb = rng.uniform(0, 5, (20, 17, 3)).astype(np.float32)
+    p = str(tmp_path / "rgb.exr")
+    assets.write_exr(p, rgb)
+    r = assets.read_exr(p)
+    np.testing.assert_array_equal(r[..., :3], rgb)
+    assert np.all(r[..., 3] == 1.0)
+    data = bytearray(open(p, "rb").read())
+    data[5] |= 0x02          # version flags: tiled
+    open(str(tmp_path / "tiled.exr"), "wb").write(bytes(data))
+    with pytest.raises(ValueError):
+        assets.read_exr(str(tmp_path / "tiled.exr"))
+    with pytest.raises(ValueError):
+        assets.write_exr(str(tmp_path / "x.exr"), rgb, compression=assets.EXR_PIZ)
+
+
+def test_exr_environment_renders_like_the_array(tmp_path):
+    """An environment map read back from .exr renders like the array it was
+    written from (full float precision)."""
+    env = scenes.sky_environment(64, 32)
+    p = str(tmp_path / "sky.exr")
+    assets.write_exr(p, env)
+    img = assets.read_exr(p)
+    np.testing.assert_array_equal(img[..., :3], env[..., :3])
