@@ -144,6 +144,24 @@ typedef struct st_view {
  * mode (their flags are ignored), and visibility, albedo, depth, prev_uv and
  * path_length are not written. */
 #define ST_FLAG_TRACE_FROM_LIGHT        (1u << 10)
+/* TestRenderer's bidirectional mode with a light vertex cache (gUseVC;
+ * TestRenderer.cpp:126-129, 478-482, 566-582; testrenderer.slang:325-440,
+ * 483-497, 531-563, 788-806, 812-936, 958-1019).  Per sample, a light pass
+ * traces max(1, W*H*light_subpath_factor) light subpaths that store their
+ * non-singular vertices and connect them to the camera; then the camera paths
+ * add emission, NEE and one connection to a uniformly chosen cached light
+ * vertex at every non-singular vertex, all weighted with the VCM recursive MIS
+ * quantities (no merging).  The cache is ordered by light subpath and depth
+ * (the reference appends in atomic, run-dependent order; both pick uniformly).
+ * Supported for surfaces with plain NEE: media, ReSTIR DI, --minPathLength,
+ * fp32 NEE and the direct-only mode return ST_ERROR_UNSUPPORTED.
+ * ST_FLAG_TRACE_FROM_LIGHT takes precedence, as mLightTrace clears gUseVC. */
+#define ST_FLAG_BDPT_LVC                (1u << 11)
+/* With ST_FLAG_BDPT_LVC: keep dVC and dVCM in fp32.  The reference stores them
+ * in fp16 (testrenderer.slang:75-82), where they overflow past 65504 and zero
+ * the MIS weights of NEE and the connections: in a 64x64 Cornell box with one
+ * light subpath per pixel, paths of three segments come out 8% too dark. */
+#define ST_FLAG_VC_FP32_MIS             (1u << 12)
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface

@@ -339,7 +339,10 @@ typedef struct {
 
 /* makeTriangleShadingDataWithoutPosition + makeTriangleShadingData,
  * src/Shaders/common/shading_data.hlsli:59-151 */
-static void make_tri_shading(const or_scene* s, uint32_t ii, uint32_t prim, float bx, float by, shading_t* r) {
+/* makeTriangleShadingData, shading_data.hlsli:142-174: from barycentrics, or
+ * (lp != NULL) from an object-space point on the triangle, whose barycentrics
+ * are solved from the point and whose world position is T * lp. */
+static void make_tri_shading_x(const or_scene* s, uint32_t ii, uint32_t prim, float bx, float by, const f3* lp, shading_t* r) {
     const st_instance* in = &s->inst[ii];
     const st_transform* T = &s->xf[ii];
     const st_mesh_vertex_info* vi = &s->vi[in->data & 0xffffu];
@@ -350,6 +353,14 @@ static void make_tri_shading(const or_scene* s, uint32_t ii, uint32_t prim, floa
     load_f3(s, pb, vi->packed_offsets[1], ps, tri[0], &v0);
     load_f3(s, pb, vi->packed_offsets[1], ps, tri[1], &v1);
     load_f3(s, pb, vi->packed_offsets[1], ps, tri[2], &v2);
+    if (lp) {
+        const f3 v1v0 = sub(v1, v0), v2v0 = sub(v2, v0), p_v0 = sub(*lp, v0);
+        const float d00 = dot(v1v0, v1v0), d01 = dot(v1v0, v2v0), d11 = dot(v2v0, v2v0);
+        const float d20 = dot(p_v0, v1v0), d21 = dot(p_v0, v2v0);
+        const float den = d00 * d11 - d01 * d01;
+        bx = (d11 * d20 - d01 * d21) / den;
+        by = (d00 * d21 - d01 * d20) / den;
+    }
     r->flags_material = (in->type_material_address >> 4) << 4;
     f3 dPds = xf_vector(T, sub(v0, v2)), dPdt = xf_vector(T, sub(v1, v2));
     f3 gn = cross(dPds, dPdt);
@@ -399,7 +410,10 @@ static void make_tri_shading(const or_scene* s, uint32_t ii, uint32_t prim, floa
         r->tan = pack_normal(tg);
         if (dot(sn, gn) < 0) r->gn = pack_normal(neg(gn));
     }
-    r->pos = xf_point(T, add(add(v0, mulf(sub(v1, v0), bx)), mulf(sub(v2, v0), by)));
+    r->pos = lp ? xf_point(T, *lp) : xf_point(T, add(add(v0, mulf(sub(v1, v0), bx)), mulf(sub(v2, v0), by)));
+}
+static void make_tri_shading(const or_scene* s, uint32_t ii, uint32_t prim, float bx, float by, shading_t* r) {
+    make_tri_shading_x(s, ii, prim, bx, by, NULL, r);
 }
 
 /* ShadingData::toLocal / toWorld / shadingNormalCorrection, shading_data.hlsli:24-54 */
@@ -1528,7 +1542,23 @@ typedef struct {
     float t16, pdfa16, dirpdf16; /* FP16-stored fields, kept as their f32 values */
     rng_t rng;
     uint32_t medium;    /* mCurrentMedium */
+    float dvc, dvcm;    /* dVC, dVCM (FP16-stored, kept as their f32 values) */
+    int vc_f32;         /* ST_FLAG_VC_FP32_MIS: dVC, dVCM kept in fp32 */
+    int flag31;         /* isSpecular (camera paths) / isFiniteLight (light paths): one bit */
 } pstate_t; /* PathState, testrenderer.slang:66-125 */
+
+/* a dVC / dVCM store: fp16 as the reference's PathState (testrenderer.slang:75-82) */
+static float qvc(const pstate_t* ps, float f) { return ps->vc_f32 ? f : q16(f); }
+
+/* PackedLightVertex, testrenderer.slang:45-64 */
+typedef struct {
+    f3 lpos;            /* object-space position */
+    uint32_t inst_prim;
+    f3 thr;
+    uint32_t dirin;     /* packed local incoming direction */
+    uint32_t plen;
+    float dvc, dvcm;
+} lvert_t;
 
 /* ShadowRay (testrenderer.slang:127-142) + the vertex medium and the key of
  * its own random stream for ratio tracking */
@@ -1550,6 +1580,11 @@ typedef struct {
     st_transform inv_cam;
     uint32_t px, py, W, H;
     float alb[3], dep[2], uv[2];
+    /* bidirectional mode: this sample's light vertex cache and subpath count */
+    const lvert_t* lvc;
+    uint32_t lvc_count, L;
+    float ipd;
+    const uint32_t* light_acc;  /* this pixel's light-pass connections (3 x u32) or NULL */
 } pctx_t;
 
 /* ViewData::toRaster (compat/scene.h:158-165) with ProjectionData::projectPoint
@@ -1604,9 +1639,14 @@ static void add_radiance(pctx_t* x, f3 rad, float w) {
 }
 /* AddEmission, testrenderer.slang:788-808 (+ --minPathLength gate, vcm.slang:960) */
 static void add_emission(pctx_t* x, f3 e, float directPdfA, float emissionPdfW, float WtoA) {
-    (void)emissionPdfW;
     float w = 1;
-    if (x->ps.path_len > 1) { /* isSpecular is never set on camera paths without VC */
+    if (x->c->flags & ST_FLAG_BDPT_LVC) {
+        /* partial eye sub-path MIS weight [tech. rep. (43)] */
+        if (x->ps.path_len > 1 && !x->ps.flag31) {
+            const float wc = mis1(directPdfA) * x->ps.dvcm + mis1(emissionPdfW) * x->ps.dvc;
+            w = 1 / (1 + wc);
+        }
+    } else if (x->ps.path_len > 1) { /* isSpecular is never set on camera paths without VC */
         if ((x->c->flags & ST_FLAG_SAMPLE_DIRECT) && x->ps.dirpdf16 > 0) w = mis2(x->ps.dirpdf16 * WtoA, directPdfA);
     }
     if (x->ps.path_len < x->c->min_len) return;
@@ -1628,12 +1668,68 @@ static void connect_to_light(pctx_t* x, const material_t* m, const shading_t* sd
     f.fwd *= cp; f.rev *= cp;
     f3 contrib = divf(mul(mul(x->ps.thr, r.radiance), f.f), dpw);
     float w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, f.fwd / AtoW); /* ComputeMIS_DI :530-566 */
+    if (x->c->flags & ST_FLAG_BDPT_LVC) {
+        /* ComputeMIS_DI with gUseVC [tech. rep. (44), (45)] */
+        const float epw = r.pdf * cos_hemisphere_pdfw(r.cos_light);
+        const float wl = mis1(f.fwd / dpw);
+        const float wc = mis1(epw * fabsf(ldout.z) / (dpw * r.cos_light)) * (0 + x->ps.dvcm + x->ps.dvc * mis1(f.rev));
+        w = 1 / (wl + 1 + wc);
+    }
     if (x->ps.path_len + 1 < x->c->min_len) return;
     if (x->nsh >= 64) return;
     shadow_t* sr = &x->sh[x->nsh++];
     sr->o = ray_offset_dir(sd->pos, unpack_normal(sd->gn), r.dir);
     sr->d = r.dir;
     sr->tmax = maxf(r.dist * 0.99f, r.dist - 1e-2f); /* AdjustIntersectionDistance, intersection.hlsli:65 */
+    sr->c = contrib;
+    sr->w = w;
+    sr->medium = x->ps.medium;
+    sr->plen = x->ps.path_len;
+}
+
+/* SampleLightVertex + ConnectToLightSubpath, testrenderer.slang:355-437: one
+ * uniformly chosen vertex of this sample's light vertex cache (picked by
+ * next().x modulo the cache size), weighted by cache size / light subpaths. */
+static void connect_to_light_subpath(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
+    uint32_t rv[4];
+    rng_next(&x->ps.rng, rv);
+    if (x->lvc_count == 0) return;
+    const lvert_t* v = &x->lvc[rv[0] % x->lvc_count];
+    if (x->ps.path_len + 1 + v->plen > x->c->max_depth) return;
+    const f3 lp = v->lpos;
+    shading_t sv;
+    memset(&sv, 0, sizeof sv);
+    make_tri_shading_x(x->s, v->inst_prim & 0xffffu, v->inst_prim >> 16, 0, 0, &lp, &sv);
+    f3 dir = sub(sv.pos, sd->pos);
+    const float dist = length3(dir);
+    dir = divf(dir, dist);
+    const material_t mv = load_material(x->s, sv.flags_material >> 4);
+    const f3 vin = unpack_normal(v->dirin);
+    const f3 vout = to_local(&sv, neg(dir));
+    const float cos_light = fabsf(vout.z);
+    rer_t fv = bsdf_eval_x(&mv, x->c->lambert, vin, vout, 1);
+    fv.f = mulf(fv.f, shading_normal_correction_x(&sv, vin, vout, 1));
+    const float cpv = m_cont_prob(&mv);
+    fv.fwd *= cpv; fv.rev *= cpv;
+    if (all_le0(v->thr) || all_le0(fv.f)) return;
+    const f3 ldout = to_local(sd, dir);
+    rer_t f = bsdf_eval(m, x->c->lambert, ldin, ldout);
+    f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
+    const float cp = m_cont_prob(m);
+    f.fwd *= cp; f.rev *= cp;
+    const float wn = (float)x->lvc_count / (float)x->L;
+    const float inv_d2 = 1 / pow2(dist);
+    const f3 contrib = mul(mul(x->ps.thr, v->thr), mulf(mulf(mul(f.f, fv.f), inv_d2), wn));
+    const float cam_pdfa = f.fwd * cos_light * inv_d2;
+    const float light_pdfa = fv.fwd * fabsf(ldout.z) * inv_d2;
+    const float wl = mis1(cam_pdfa) * (0 + v->dvcm + v->dvc * mis1(fv.rev));   /* [tech. rep. (40)] */
+    const float wc = mis1(light_pdfa) * (0 + x->ps.dvcm + x->ps.dvc * mis1(f.rev)); /* [tech. rep. (41)] */
+    const float w = 1 / (wl + 1 + wc);
+    if (x->nsh >= 64) return;
+    shadow_t* sr = &x->sh[x->nsh++];
+    sr->o = ray_offset_dir(sd->pos, unpack_normal(sd->gn), dir);
+    sr->d = dir;
+    sr->tmax = maxf(dist * 0.99f, dist - 1e-2f); /* AdjustIntersectionDistance, intersection.hlsli:65 */
     sr->c = contrib;
     sr->w = w;
     sr->medium = x->ps.medium;
@@ -1860,6 +1956,20 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
 
 /* TraceRay + ExtendPath + ProcessInteraction, testrenderer.slang:816-935, 1026-1071.
  * Returns 0 when the path ends. */
+/* Sub-path MIS quantities after a BSDF sample, ProcessInteraction
+ * (testrenderer.slang:909-922): [tech. rep. (53)-(55)] for a singular sample,
+ * else (34)-(36); a camera path is no longer specular after a non-singular one. */
+static void vc_after_sample(pstate_t* ps, const dsr_t* smp, float cos_out, int from_light) {
+    if (smp->rough == 0) {
+        ps->dvcm = qvc(ps, 0.0f);
+        ps->dvc = qvc(ps, ps->dvc * mis1(cos_out));
+    } else {
+        if (!from_light) ps->flag31 = 0;
+        ps->dvc = qvc(ps, mis1(cos_out / smp->fwd) * (ps->dvc * mis1(smp->rev) + ps->dvcm + 0));
+        ps->dvcm = qvc(ps, mis1(1.0f / smp->fwd));
+    }
+}
+
 static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view) {
     (void)p; (void)view;
     const or_scene* s = x->s;
@@ -1932,6 +2042,12 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     material_t m = load_material(s, sd.flags_material >> 4);
     f3 ldin = to_local(&sd, neg(dir));
     float cos_in = fabsf(ldin.z);
+    const int vc = (x->c->flags & ST_FLAG_BDPT_LVC) != 0;
+    if (vc) { /* [tech. rep. (31)-(33)], completed at the hit (testrenderer.slang:826-834) */
+        ps->dvcm = qvc(ps, ps->dvcm * mis1(pow2(ps->t16)));
+        ps->dvcm = qvc(ps, ps->dvcm / mis1(cos_in));
+        ps->dvc = qvc(ps, ps->dvc / mis1(cos_in));
+    }
     if (ps->path_len == 1) {
         x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn;
         store_aux(x, 1, sd.pos, m_base(&m));
@@ -1945,7 +2061,10 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     }
     if (ps->path_len >= x->c->max_depth) return 0;
     if (!m_can_eval(&m)) return 0;
-    if (!m_singular(&m, x->c->lambert) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
+    if (!m_singular(&m, x->c->lambert) && vc) {
+        connect_to_light(x, &m, &sd, ldin);
+        connect_to_light_subpath(x, &m, &sd, ldin);
+    } else if (!m_singular(&m, x->c->lambert) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
         if (x->c->flags & ST_FLAG_RESTIR_DI) connect_to_light_restir(x, &m, &sd, ldin);
         else connect_to_light(x, &m, &sd, ldin);
     }
@@ -1961,6 +2080,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     ps->thr = mul(ps->thr, divf(smp.f, smp.fwd));
     if (smp.rough == 0) ps->dirpdf16 = 0;
     else ps->dirpdf16 = q16(smp.fwd);
+    if (vc) vc_after_sample(ps, &smp, fabsf(smp.dir.z), 0);
     ps->thr = mulf(ps->thr, shading_normal_correction(&sd, ldin, smp.dir));
     f3 wd = to_world(&sd, smp.dir);
     ps->dir = pack_normal(wd);
@@ -2002,6 +2122,16 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
     ps->path_len = 0;
     ps->thr = v3(1, 1, 1);
     ps->dir = pack_normal(normalize3(xf_vector(T, ldir)));
+    if (x->c->flags & ST_FLAG_BDPT_LVC) { /* testrenderer.slang:958-977 */
+        const float cos_cam = fabsf(ldir.z);
+        const float ipcd = x->ipd / cos_cam;
+        const float cam_pdfw = pow2(ipcd) / cos_cam;
+        ps->dirpdf16 = q16(cam_pdfw);
+        ps->flag31 = 1;
+        ps->vc_f32 = (x->c->flags & ST_FLAG_VC_FP32_MIS) != 0;
+        ps->dvcm = qvc(ps, mis1((float)x->L / cam_pdfw));
+        ps->dvc = qvc(ps, 0.0f);
+    }
     /* Render always extends once; RenderIteration returns early when the
      * throughput is zero or the depth is reached (testrenderer.slang:1105, 1137-1141) */
     for (int first = 1;; first = 0) {
@@ -2031,7 +2161,10 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
         }
         if (x->c->flags & ST_FLAG_FLOAT_NEE) x->out = add(x->out, v3(maxf(0, rw.x), maxf(0, rw.y), maxf(0, rw.z)));
     }
-    /* ProcessAtomicOutput, testrenderer.slang:1173-1183 */
+    /* ProcessAtomicOutput, testrenderer.slang:1173-1183: one atomic sum per pixel
+     * holds every connection (with VC, also the light pass's camera connections) */
+    if (x->light_acc)
+        for (int k = 0; k < 3; k++) x->nee[k] += x->light_acc[k];
     result->x = x->out.x + (float)x->nee[0] / (float)OR_Q;
     result->y = x->out.y + (float)x->nee[1] / (float)OR_Q;
     result->z = x->out.z + (float)x->nee[2] / (float)OR_Q;
@@ -2049,6 +2182,8 @@ typedef struct {
     float ipd;      /* ViewData::imagePlaneDist */
     uint32_t seed;  /* this sample's seed */
     uint32_t* acc;  /* this thread's mOutputAtomic, 3 per pixel */
+    lvert_t* verts; /* bidirectional mode: max_depth - 1 slots per light subpath */
+    uint8_t* nverts;
 } lctx_t;
 
 /* SceneParameters::GetMediumIndex (scene.hlsli:32-40): the first volume whose
@@ -2083,7 +2218,7 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
     const float dist = length3(tc);
     tc = divf(tc, dist);
     f3 f, o;
-    float cos_out;
+    float cos_out, frev = 0;
     if (med) {
         const float v = hg_eval(med->g, ldin, tc);
         f = v3(v, v, v);
@@ -2094,6 +2229,7 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
         cos_out = fabsf(ldout.z);
         rer_t r = bsdf_eval_x(m, x->c->lambert, ldin, ldout, 1);
         f = mulf(r.f, shading_normal_correction_x(sd, ldin, ldout, 1));
+        frev = r.rev * m_cont_prob(m);
         o = ray_offset_dir(pos, unpack_normal(sd->gn), tc);
     }
     const float image_point_to_camera = lx->ipd / cos_cam;
@@ -2101,6 +2237,11 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
     const float image_to_surface = image_to_solid_angle * cos_out / pow2(dist);
     const float surface_to_image = 1.0f / image_to_surface;
     f3 contrib = divf(mul(x->ps.thr, f), (float)lx->L * surface_to_image * cos_out);
+    float wmis = 1;
+    if (x->c->flags & ST_FLAG_BDPT_LVC) { /* [tech. rep. (46)]; the eye sub-path weight is 0 (47) */
+        const float wl = mis1(image_to_surface / (float)lx->L) * (0 + x->ps.dvcm + x->ps.dvc * mis1(frev));
+        wmis = 1.f / (wl + 1.f);
+    }
     const uint32_t pix = (uint32_t)(int32_t)pcy * x->W + (uint32_t)(int32_t)pcx;
     if (pix >= x->W * x->H) return;
     if (x->s->nvolinst > 0) {
@@ -2116,8 +2257,9 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
         x->rays[1]++;
         if (h.hit) return;
     }
-    if (all_le0(contrib)) return;
+    if (all_le0(contrib) || wmis <= 0) return;
     /* AddRadianceInterlocked, testrenderer.slang:237-261 */
+    contrib = mulf(contrib, wmis);
     const float qv[3] = {clamp0(contrib.x) * (float)OR_Q, clamp0(contrib.y) * (float)OR_Q, clamp0(contrib.z) * (float)OR_Q};
     for (int k = 0; k < 3; k++) lx->acc[3 * (size_t)pix + k] += (uint32_t)(qv[k] >= 2147483647.0f ? INT_MAX : (int32_t)qv[k]);
 }
@@ -2139,6 +2281,7 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
     const isr_t r = sample_illumination(s, c, rnd, v3(0, 0, 0));
     rng_nextf(&ps->rng, dr);
     f3 thr = divf(r.radiance, r.pdf), o, d;
+    float direct_pdfa = 1, emission_pdfw = 1, cos_light = 1;
     if (r.finite) {
         const f3 ld = sample_cos_hemisphere(dr[0], dr[1]);
         f3 onb[3];
@@ -2147,6 +2290,9 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
         d = normalize3(add(add(mulf(onb[0], ld.x), mulf(onb[1], ld.y)), mulf(onb[2], ld.z)));
         const float dirpdf = q16(cos_hemisphere_pdfw(ld.z)); /* mDirPdfW is stored in fp16 */
         thr = mulf(thr, ld.z / dirpdf);
+        direct_pdfa = r.pdf;
+        emission_pdfw = r.pdf * dirpdf;
+        cos_light = ld.z;
     } else {
         float xy[2];
         sample_concentric_disc(dr[0], dr[1], xy);
@@ -2161,6 +2307,13 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
     ps->origin = o;
     ps->dir = pack_normal(d);
     ps->medium = (s->nvolinst > 0 && r.finite) ? medium_at(s, r.pos) : 0xFFFFu;
+    const int vc = (c->flags & ST_FLAG_BDPT_LVC) != 0;
+    if (vc) { /* testrenderer.slang:1016-1019; light samples are never singular (lights.hlsli:168) */
+        ps->flag31 = r.finite;
+        ps->vc_f32 = (c->flags & ST_FLAG_VC_FP32_MIS) != 0;
+        ps->dvcm = qvc(ps, mis1(direct_pdfa / emission_pdfw));
+        ps->dvc = qvc(ps, mis1(cos_light / emission_pdfw));
+    }
     while (ps->path_len + 1 <= c->max_depth) {
         const f3 wd = unpack_normal(ps->dir);
         hit_t h;
@@ -2199,7 +2352,25 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
             make_tri_shading(s, h.id & 0xffffu, h.id >> 16, h.b1, h.b2, &sd);
             const material_t m = load_material(s, sd.flags_material >> 4);
             const f3 ldin = to_local(&sd, neg(wd));
+            if (vc) { /* testrenderer.slang:812-824 */
+                const float cos_in = fabsf(ldin.z);
+                ps->t16 = q16(h.t);
+                if (ps->path_len > 1 || ps->flag31) ps->dvcm = qvc(ps, ps->dvcm * mis1(pow2(ps->t16)));
+                ps->dvcm = qvc(ps, ps->dvcm / mis1(cos_in));
+                ps->dvc = qvc(ps, ps->dvc / mis1(cos_in));
+            }
             if (ps->path_len >= c->max_depth || !m_can_eval(&m)) return;
+            if (vc && !m_singular(&m, c->lambert) && ps->path_len + 2 <= c->max_depth) {
+                /* StoreLightVertex, testrenderer.slang:329-341 */
+                lvert_t* v = &lx->verts[(size_t)q * (c->max_depth - 1) + lx->nverts[q]++];
+                v->lpos = xf_point(&s->inv_xf[h.id & 0xffffu], sd.pos);
+                v->inst_prim = h.id;
+                v->thr = ps->thr;
+                v->dirin = pack_normal(ldin);
+                v->plen = ps->path_len;
+                v->dvc = ps->dvc;
+                v->dvcm = ps->dvcm;
+            }
             if (!m_singular(&m, c->lambert) && ps->path_len + 1 >= c->min_len) connect_to_camera(lx, &m, &sd, NULL, sd.pos, ldin, q);
             const float cp = minf(1, m_cont_prob(&m));
             float rr[4], rb[4];
@@ -2209,6 +2380,7 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
             dsr_t smp = bsdf_sample_x(&m, c->lambert, v3(rb[0], rb[1], rb[2]), ldin, 1);
             smp.fwd *= cp; smp.rev *= cp;
             ps->thr = mul(ps->thr, divf(smp.f, smp.fwd));
+            if (vc) vc_after_sample(ps, &smp, fabsf(smp.dir.z), 1);
             ps->thr = mulf(ps->thr, shading_normal_correction_x(&sd, ldin, smp.dir, 1));
             const f3 w = to_world(&sd, smp.dir);
             ps->dir = pack_normal(w);
@@ -2224,7 +2396,7 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
     cfg_t c;
     memset(&c, 0, sizeof c);
     c.flags = p->flags & ~(uint32_t)(ST_FLAG_SAMPLE_DIRECT | ST_FLAG_DIRECT_ONLY | ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE |
-                                      ST_FLAG_FLOAT_NEE);
+                                      ST_FLAG_FLOAT_NEE | ST_FLAG_BDPT_LVC);
     c.env_prob = p->environment_sample_probability;
     c.max_depth = p->max_depth;
     c.min_len = p->min_path_length;
@@ -2292,11 +2464,127 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
     return 0;
 }
 
+/* TestRenderer with gUseVC (TestRenderer.cpp:478-482, 559-629): per sample,
+ * the light dispatch (light subpaths store their vertices and connect to the
+ * camera), then the view dispatch (emission, NEE and one light-vertex
+ * connection per vertex), then ProcessShadowRays and ProcessAtomicOutput into
+ * the view paths' image.  The light vertex cache is ordered by light subpath
+ * and depth. */
+static int render_vc(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
+    if (s->nvolinst > 0 || (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_FLOAT_NEE | ST_FLAG_DIRECT_ONLY)) || p->min_path_length)
+        return fail("the bidirectional mode supports surfaces with plain NEE only");
+    if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64)) return fail("light_subpath_factor out of range");
+    if (p->max_depth < 1) return fail("max_depth must be at least 1");
+    cfg_t c;
+    memset(&c, 0, sizeof c);
+    c.flags = p->flags;
+    c.env_prob = p->environment_sample_probability;
+    c.max_depth = p->max_depth;
+    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    c.max_null = p->max_null_collisions;
+    const uint32_t W = p->width, H = p->height, npix = W * H;
+    uint32_t L = (uint32_t)((float)npix * p->light_subpath_factor);
+    if (L < 1) L = 1;
+    const float ipd = (float)abs(view->image_max[1] - view->image_min[1]) / (2 * tanf(view->projection.vertical_fov / 2));
+    const st_transform inv_cam = inverse_xf(&view->camera_to_world);
+    const size_t slots = (size_t)L * (p->max_depth - 1 ? p->max_depth - 1 : 1);
+    uint32_t* total = calloc((size_t)npix * 3, 4);
+    f3* sum = calloc(npix, sizeof(f3));
+    lvert_t* verts = malloc(sizeof(lvert_t) * slots);
+    lvert_t* lvc = malloc(sizeof(lvert_t) * slots);
+    uint8_t* nverts = calloc(L, 1);
+    if (!total || !sum || !verts || !lvc || !nverts) {
+        free(total); free(sum); free(verts); free(lvc); free(nverts);
+        return fail("out of memory");
+    }
+    uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
+    int oom = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    for (uint32_t k = 0; k < p->spp && !oom; k++) {
+        memset(total, 0, (size_t)npix * 3 * 4);
+        memset(nverts, 0, L);
+#pragma omp parallel
+        {
+            lctx_t x;
+            memset(&x, 0, sizeof x);
+            x.b.s = s; x.b.c = &c; x.b.view = view; x.b.inv_cam = inv_cam;
+            x.b.W = W; x.b.H = H;
+            x.L = L; x.ipd = ipd; x.seed = p->seed + k;
+            x.verts = verts; x.nverts = nverts;
+            x.acc = calloc((size_t)npix * 3, 4);
+#pragma omp for schedule(dynamic, 256)
+            for (int64_t q = 0; q < (int64_t)L; q++)
+                if (x.acc) trace_light_path(&x, (uint32_t)q);
+#pragma omp critical
+            {
+                if (x.acc) for (size_t i = 0; i < (size_t)npix * 3; i++) total[i] += x.acc[i];
+                else oom = 1;
+                rays[0] += x.b.rays[0]; rays[1] += x.b.rays[1];
+                for (int j = 0; j < 4; j++) trav[j] += x.b.trav[j];
+            }
+            free(x.acc);
+        }
+        uint32_t count = 0;
+        for (uint32_t q = 0; q < L; q++)
+            for (uint32_t j = 0; j < nverts[q]; j++) lvc[count++] = verts[(size_t)q * (p->max_depth - 1) + j];
+#pragma omp parallel
+        {
+            pctx_t x;
+            memset(&x, 0, sizeof x);
+            x.s = s; x.c = &c; x.view_medium = 0xFFFFu; x.view = view; x.inv_cam = inv_cam;
+            x.W = W; x.H = H;
+            x.lvc = lvc; x.lvc_count = count; x.L = L; x.ipd = ipd;
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+                const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
+                x.px = px; x.py = py;
+                x.light_acc = total + 3 * (size_t)pix;
+                f3 r;
+                uint32_t pl;
+                render_pixel(&x, p, view, px, py, p->seed + k, &r, &pl);
+                if (k == 0 && out->visibility) { out->visibility[2 * pix] = x.vis[0]; out->visibility[2 * pix + 1] = x.vis[1]; }
+                if (k == 0 && out->albedo) {
+                    for (int j = 0; j < 3; j++) out->albedo[4 * pix + j] = x.alb[j];
+                    out->albedo[4 * pix + 3] = 1;
+                }
+                if (k == 0 && out->depth) {
+                    out->depth[4 * pix] = x.dep[0]; out->depth[4 * pix + 1] = x.dep[1];
+                    out->depth[4 * pix + 2] = 1; out->depth[4 * pix + 3] = 1;
+                }
+                if (k == 0 && out->prev_uv) { out->prev_uv[2 * pix] = x.uv[0]; out->prev_uv[2 * pix + 1] = x.uv[1]; }
+                if (out->path_length) out->path_length[(size_t)k * npix + pix] = pl;
+                sum[pix] = add(sum[pix], r);
+            }
+#pragma omp critical
+            {
+                rays[0] += x.rays[0]; rays[1] += x.rays[1];
+                for (int j = 0; j < 4; j++) trav[j] += x.trav[j];
+            }
+        }
+    }
+    if (!oom && out->hdr)
+        for (uint32_t i = 0; i < npix; i++) {
+            out->hdr[4 * i + 0] = sum[i].x / (float)p->spp;
+            out->hdr[4 * i + 1] = sum[i].y / (float)p->spp;
+            out->hdr[4 * i + 2] = sum[i].z / (float)p->spp;
+            out->hdr[4 * i + 3] = 1;
+        }
+    if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
+    if (out->traversal) for (int j = 0; j < 4; j++) out->traversal[j] = trav[j];
+    free(total); free(sum); free(verts); free(lvc); free(nverts);
+    return oom ? fail("out of memory") : 0;
+}
+
 int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
     if (!s || !view || !p || !out) return fail("null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
     if (p->flags & ST_FLAG_ENV_ALIAS) return fail("mode not supported by this oracle");
     if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, out, threads);
+    if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, out, threads);
     if (p->flags & ST_FLAG_RESTIR_DI_REUSE) return fail("mode not supported by this oracle");
     if (s->nvolinst > 0 && (p->flags & ST_FLAG_RESTIR_DI)) return fail("ReSTIR DI with media is not supported");
     cfg_t c;

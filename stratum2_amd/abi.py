@@ -25,6 +25,8 @@ ST_FLAG_PIXEL_JITTER = 1 << 7
 ST_FLAG_COUNT_TRAVERSAL = 1 << 8
 ST_FLAG_DIRECT_ONLY = 1 << 9
 ST_FLAG_TRACE_FROM_LIGHT = 1 << 10  # TestRenderer light tracing (gTraceFromLight)
+ST_FLAG_BDPT_LVC = 1 << 11  # TestRenderer bidirectional mode with a light vertex cache (gUseVC)
+ST_FLAG_VC_FP32_MIS = 1 << 12  # dVC / dVCM in fp32 instead of the reference's fp16
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 # TonemapMode (src/Shaders/compat/tonemap.h:10-23)
