@@ -17,6 +17,7 @@
 // All kernels are persistent grid-stride loops whose item counts are read on
 // the device, so the host never waits inside a frame.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -84,6 +85,13 @@ struct WaveState {
     float4* r_prev;     // after spatial reuse: this frame's result, next frame's history
     float4* psd;        // 2 per pixel: the primary hit's shading data (pos, material, gn, sn, tangent, area)
     uint32_t* med;      // per path: current medium instance (kNoMedium outside media)
+    // bidirectional mode (gUseVC)
+    float2* vcm;        // per path: dVC, dVCM (fp16-rounded unless ST_FLAG_VC_FP32_MIS)
+    float4* lv_raw;     // light vertices, lv_stride slots per light subpath: {local pos, id}, {thr, dir in}, {length, dVC, dVCM}
+    uint32_t* lv_cnt;   // per light subpath: vertices stored
+    uint32_t* lv_off;   // exclusive prefix sums of lv_cnt
+    float4* lv;         // the cache: vertices in (subpath, depth) order
+    uint32_t* lv_total; // vertices in the cache
 };
 
 struct RenderConsts {
@@ -91,7 +99,8 @@ struct RenderConsts {
     float env_prob;
     float max_m, radius;
     uint32_t view_medium, max_null;
-    uint32_t light_paths;  // light tracing: paths per sample (mLightSubpathCount)
+    uint32_t light_paths;  // light tracing and the bidirectional mode: light subpaths per sample (mLightSubpathCount)
+    uint32_t lv_stride;    // bidirectional mode: light vertex slots per subpath (max_depth - 1)
     float ipd;             // light tracing: ViewData::imagePlaneDist (compat/scene.h:147)
     st_view view;
     st_transform inv_cam;  // world to camera (TransformData::inverse of the camera transform)
@@ -104,6 +113,12 @@ struct AuxOut {
     float* depth;
     float* prev_uv;
 };
+
+// A dVC / dVCM store: fp16, as the reference's PathState keeps them
+// (testrenderer.slang:75-82), unless ST_FLAG_VC_FP32_MIS.
+__device__ __forceinline__ float qvc(const RenderConsts& c, float x) {
+    return (c.flags & ST_FLAG_VC_FP32_MIS) ? x : q16(x);
+}
 
 // -------------------------------------------------------------------- kernels
 // Path order inside one sample: 8x8 pixel tiles (one wavefront of primary rays
@@ -203,9 +218,20 @@ __global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveStat
         const uint32_t dir = pack_normal(normalize(xf_vector(T, ld)));
         w.ray[p] = make_float4(T.m[0][3], T.m[1][3], T.m[2][3], __uint_as_float(dir));
         w.thr[p] = make_float4(1, 1, 1, 0);
-        w.st[p] = make_uint4(rng.w, 0, 0, 0);
         w.em[p] = make_float4(0, 0, 0, 0);
-        w.nee[p] = make_uint4(0, 0, 0, 0);
+        if (c.flags & ST_FLAG_BDPT_LVC) {
+            // eye sub-path MIS quantities (testrenderer.slang:958-977); the specular
+            // flag sits in bit 31 of the dirPdf word; the light pass has already
+            // added its camera connections to this path's accumulator
+            const float cos_cam = fabsf(ld.z);
+            const float ipcd = c.ipd / cos_cam;
+            const float cam_pdfw = sq(ipcd) / cos_cam;
+            w.st[p] = make_uint4(rng.w, 0, 0, f2h(cam_pdfw) | 0x80000000u);
+            w.vcm[p] = make_float2(qvc(c, 0.0f), qvc(c, mis1((float)c.light_paths / cam_pdfw)));
+        } else {
+            w.st[p] = make_uint4(rng.w, 0, 0, 0);
+            w.nee[p] = make_uint4(0, 0, 0, 0);
+        }
         w.med[p] = c.view_medium;
         w.queue[0][p] = p;
     }
@@ -254,9 +280,10 @@ __device__ __forceinline__ float clamp0(float x) { return x > 0 ? x : 0.0f; }
 __device__ __forceinline__ void add_shadow_contribution(const WaveState& w, uint32_t flags, uint32_t p, v3 ct, float wgt) {
     if (wgt <= 0 || all_nonpos(ct)) return;
     const v3 rw = ct * wgt;
-    if (flags & ST_FLAG_TRACE_FROM_LIGHT) {
-        // light paths of one launch meet in the same pixels: 32-bit atomics,
-        // whose sums do not depend on the order
+    if (flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC)) {
+        // light paths of one launch meet in the same pixels (and a bidirectional
+        // camera vertex makes two shadow rays): 32-bit atomics, whose sums do
+        // not depend on the order
         const float q0 = clamp0(rw.x) * (float)kQuant, q1 = clamp0(rw.y) * (float)kQuant, q2 = clamp0(rw.z) * (float)kQuant;
         uint32_t* a = reinterpret_cast<uint32_t*>(w.nee + p);
         const uint32_t i0 = (uint32_t)(q0 >= 2147483647.0f ? 2147483647 : (int32_t)q0);
@@ -298,7 +325,8 @@ struct ShadowSrc {
         const float4 c = w.shadow[3 * (size_t)i + 2];
         const uint32_t p = __float_as_uint(w.shadow[3 * (size_t)i + 1].w);
         // light tracing keeps the light path in .w; its weight is 1 (no MIS)
-        add_shadow_contribution(w, flags, p, mk(c.x, c.y, c.z), (flags & ST_FLAG_TRACE_FROM_LIGHT) ? 1.0f : c.w);
+        const bool lt_only = (flags & ST_FLAG_TRACE_FROM_LIGHT) && !(flags & ST_FLAG_BDPT_LVC);
+        add_shadow_contribution(w, flags, p, mk(c.x, c.y, c.z), lt_only ? 1.0f : c.w);
     }
 };
 
@@ -503,6 +531,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, Render
     wave_add_u64(reinterpret_cast<unsigned long long*>(w.counters + 4) + 1, segs);
     if (c.flags & ST_FLAG_COUNT_TRAVERSAL) wave_add_u64(&stats[2], reads);
 }
+
 
 // Wavefront-aggregated append: one atomic per wave, lane order preserved.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
@@ -1063,7 +1092,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderCo
             const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), mk(0, 0, 0));
             const float4 dr = rng.nextf();
             v3 thr = L.radiance / L.pdf, o, d;
-            float dirpdf;
+            float dirpdf, direct_pdfa = 1, emission_pdfw = 1, cos_light = 1;
             if (L.finite) {
                 const v3 ld = sample_cos_hemisphere(dr.x, dr.y);
                 const v3 nrm = unpack_normal(L.gn);
@@ -1073,6 +1102,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderCo
                 d = normalize(t0 * ld.x + t1 * ld.y + nrm * ld.z);
                 dirpdf = q16(cos_hemisphere_pdfw(ld.z));  // mDirPdfW is stored in fp16
                 thr = thr * (ld.z / dirpdf);
+                direct_pdfa = L.pdf;
+                emission_pdfw = L.pdf * dirpdf;
+                cos_light = ld.z;
             } else {
                 float x, y;
                 concentric_disc(dr.x, dr.y, x, y);
@@ -1086,10 +1118,15 @@ __global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderCo
             }
             const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
             live = L.pdf > 0 && !nan && any_pos(thr) && c.max_depth > 0;
+            if (c.flags & ST_FLAG_BDPT_LVC) w.lv_cnt[p] = 0;
             if (live) {
                 w.ray[p] = make_float4(o.x, o.y, o.z, __uint_as_float(pack_normal(d)));
                 w.thr[p] = make_float4(thr.x, thr.y, thr.z, 0);
-                w.st[p] = make_uint4(rng.w, 0, 0, f2h(dirpdf));
+                // bidirectional mode: light sub-path MIS quantities (testrenderer.slang:1016-1019;
+                // light samples are never singular, lights.hlsli:168) and isFiniteLight in bit 31
+                if (c.flags & ST_FLAG_BDPT_LVC)
+                    w.vcm[p] = make_float2(qvc(c, mis1(cos_light / emission_pdfw)), qvc(c, mis1(direct_pdfa / emission_pdfw)));
+                w.st[p] = make_uint4(rng.w, 0, 0, f2h(dirpdf) | (L.finite ? 0x80000000u : 0u));
                 // a finite light's medium (GetMediumIndex); environment paths start outside
                 w.med[p] = (L.finite && S.vinst_count > 0) ? medium_at(S, L.pos) : kNoMedium;
             }
@@ -1100,6 +1137,19 @@ __global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderCo
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         w.counters[16] = 0;
         w.counters[17] = 0;
+    }
+}
+
+// Sub-path MIS quantities after a BSDF sample (ProcessInteraction,
+// testrenderer.slang:909-922): [tech. rep. (53)-(55)] for a singular sample,
+// else (34)-(36).  vcm = {dVC, dVCM}.
+__device__ __forceinline__ void vc_after_sample(const RenderConsts& c, float2& vcm, const Sample& sm, float cos_out) {
+    if (sm.rough == 0) {
+        vcm.y = qvc(c, 0.0f);
+        vcm.x = qvc(c, vcm.x * mis1(cos_out));
+    } else {
+        vcm.x = qvc(c, mis1(cos_out / sm.fwd) * (vcm.x * mis1(sm.rev) + vcm.y + 0));
+        vcm.y = qvc(c, mis1(1.0f / sm.fwd));
     }
 }
 
@@ -1120,9 +1170,12 @@ __device__ __forceinline__ bool camera_link(const RenderConsts& c, v3 pos, CamLi
     l.cos_cam = fabsf(normalize(pc).z);
     return true;
 }
+// In the bidirectional mode (vc) the record carries the MIS weight [tech. rep.
+// (46)] from the vertex's dVC, dVCM and the reverse pdf frev; else the light path.
 __device__ __forceinline__ bool camera_record(const RenderConsts& c, uint32_t p, v3 thr, v3 f, float cos_cam, float cos_out,
                                               float dist, v3 origin, v3 tc, float pcx, float pcy, float4& sh0, float4& sh1,
-                                              float4& sh2) {
+                                              float4& sh2, bool vc = false, float2 vcm = make_float2(0, 0),
+                                              float frev = 0) {
     const float ipcd = c.ipd / cos_cam;
     const float i2s = (sq(ipcd) / cos_cam) * cos_out / sq(dist);
     const float s2i = 1.0f / i2s;
@@ -1130,9 +1183,14 @@ __device__ __forceinline__ bool camera_record(const RenderConsts& c, uint32_t p,
     const uint32_t pix = (uint32_t)(int32_t)pcy * c.W + (uint32_t)(int32_t)pcx;
     if (pix >= c.npix) return false;
     const uint32_t slot = (p / c.light_paths) * c.npix + path_of_pixel(c, pix % c.W, pix / c.W);
+    float tag = __uint_as_float(p);
+    if (vc) {
+        const float wl = mis1(i2s / (float)c.light_paths) * (0 + vcm.y + vcm.x * mis1(frev));
+        tag = 1.f / (wl + 1.f);
+    }
     sh0 = make_float4(origin.x, origin.y, origin.z, dist);
     sh1 = make_float4(tc.x, tc.y, tc.z, __uint_as_float(slot));
-    sh2 = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(p));
+    sh2 = make_float4(contrib.x, contrib.y, contrib.z, tag);
     return true;
 }
 
@@ -1143,7 +1201,12 @@ __device__ __forceinline__ bool camera_record(const RenderConsts& c, uint32_t p,
 // path.  A miss ends a light path without emission.  MEDIA: scenes with
 // volumes, whose scattering vertices k_trace_media reports as id
 // medium | 0xffff << 16 at the position it wrote into the ray origin.
-template <bool LAMBERT, bool MEDIA>
+// VC: the light pass of the bidirectional mode (surfaces only): the sub-path MIS
+// quantities are completed at each hit (testrenderer.slang:812-824), each
+// non-singular vertex short of max_depth - 1 segments is stored in its
+// subpath's slots (StoreLightVertex, :329-341), and the camera connection is
+// MIS-weighted.
+template <bool LAMBERT, bool MEDIA, bool VC>
 __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderConsts c, WaveState w, int qsel) {
     const uint32_t n = w.counters[qsel];
     const uint32_t* q = w.queue[qsel];
@@ -1156,6 +1219,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
         const uint32_t i = i0 + __lane_id();
         bool emit_shadow = false, cont = false;
         float4 sh0 = make_float4(0, 0, 0, 0), sh1 = sh0, sh2 = sh0, ray_out = sh0, thr_out = sh0;
+        float2 vcm_out = make_float2(0, 0);
         uint32_t p = 0;
         if (i < n) {
             p = q[i];
@@ -1210,7 +1274,24 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
                     const v3 ldin = f.to_local(-dir);
                     const v3 lgn = f.to_local(unpack_normal(sd.gn));
                     const v3 gn = unpack_normal(sd.gn);
+                    float2 vcm = make_float2(0, 0);
+                    if (VC) {
+                        const float cos_in = fabsf(ldin.z), t16 = q16(__uint_as_float(hit.x));
+                        vcm = w.vcm[p];
+                        if (path_len > 1 || (s4.w >> 31)) vcm.y = qvc(c, vcm.y * mis1(sq(t16)));
+                        vcm.y = qvc(c, vcm.y / mis1(cos_in));
+                        vcm.x = qvc(c, vcm.x / mis1(cos_in));
+                    }
                     bool alive = path_len < c.max_depth && any_pos(m.base());
+                    if (VC && alive && !m_singular<LAMBERT>(m) && path_len + 2 <= c.max_depth) {
+                        const uint32_t j = w.lv_cnt[p];
+                        float4* v = w.lv_raw + 3 * ((size_t)p * c.lv_stride + j);
+                        const v3 lp = xf_point(S.inv_xf[hit.y & 0xffffu], sd.pos);
+                        v[0] = make_float4(lp.x, lp.y, lp.z, __uint_as_float(hit.y));
+                        v[1] = make_float4(thr.x, thr.y, thr.z, __uint_as_float(pack_normal(ldin)));
+                        v[2] = make_float4(__uint_as_float(path_len), vcm.x, vcm.y, 0);
+                        w.lv_cnt[p] = j + 1;
+                    }
                     CamLink l;
                     if (alive && !m_singular<LAMBERT>(m) && path_len + 1 >= c.min_len && camera_link(c, sd.pos, l)) {
                         v3 tc = cam - sd.pos;
@@ -1220,7 +1301,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
                         Eval fe = bsdf_eval<LAMBERT, true>(m, ldin, ldout);
                         fe.f = fe.f * shading_normal_correction<true>(f, lgn, ldin, ldout);
                         emit_shadow = camera_record(c, p, thr, fe.f, l.cos_cam, fabsf(ldout.z), dist,
-                                                    ray_offset(sd.pos, gn, tc), tc, l.pcx, l.pcy, sh0, sh1, sh2);
+                                                    ray_offset(sd.pos, gn, tc), tc, l.pcx, l.pcy, sh0, sh1, sh2, VC, vcm,
+                                                    fe.rev * m_cont_prob(m));
                     }
                     if (alive) {
                         const float cp = fmin_(1.0f, m_cont_prob(m));
@@ -1233,6 +1315,10 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
                             sm.fwd *= cpx;
                             sm.rev *= cpx;
                             thr = thr * (sm.f / sm.fwd);
+                            if (VC) {
+                                vc_after_sample(c, vcm, sm, fabsf(sm.dir.z));
+                                vcm_out = vcm;
+                            }
                             thr = thr * shading_normal_correction<true>(f, lgn, ldin, sm.dir);
                             const v3 wd = f.to_world(sm.dir);
                             const v3 o = ray_offset(sd.pos, gn, wd);
@@ -1249,6 +1335,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
                 if (cont) {
                     w.ray[p] = ray_out;
                     w.thr[p] = thr_out;
+                    if (VC) w.vcm[p] = vcm_out;
                 }
             }
         }
@@ -1260,6 +1347,237 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
         }
         const uint32_t qi = wave_append(&w.counters[qsel ^ 1], cont);
         if (cont) qout[qi] = p;
+    }
+}
+
+// The camera pass of the bidirectional mode (gUseVC, surfaces only):
+// ProcessInteraction<false> (testrenderer.slang:826-935) with the eye sub-path
+// MIS quantities completed at each hit, emission weighted by [tech. rep. (43)]
+// (AddEmission, :788-806), NEE weighted by ComputeMIS_DI's VC branch
+// (:530-566), and one connection to a uniformly chosen vertex of this sample's
+// light vertex cache, weighted by cache size / light subpaths
+// (SampleLightVertex + ConnectToLightSubpath, :355-437).  Up to two shadow
+// records per vertex, both added to the path's own accumulator.
+template <bool LAMBERT, bool ALIAS>
+__global__ void __launch_bounds__(kShadeBlock) k_shade_vc(DevScene S, RenderConsts c, WaveState w, int qsel, AuxOut aux,
+                                                          int first_wave) {
+    const uint32_t n = w.counters[qsel];
+    const uint32_t* q = w.queue[qsel];
+    uint32_t* qout = w.queue[qsel ^ 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t base0 = blockIdx.x * blockDim.x + threadIdx.x - __lane_id();
+    const uint32_t lv_count = *w.lv_total;
+    for (uint32_t i0 = base0; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + __lane_id();
+        bool sh_a = false, sh_b = false, cont = false;
+        float4 a0 = make_float4(0, 0, 0, 0), a1 = a0, a2 = a0, b0 = a0, b1 = a0, b2 = a0, ray_out = a0, thr_out = a0;
+        float2 vcm_out = make_float2(0, 0);
+        uint32_t p = 0;
+        if (i < n) {
+            p = q[i];
+            const uint4 hit = w.hit[i];
+            const float4 r = w.ray[p];
+            const float4 thr4 = w.thr[p];
+            uint4 s4 = w.st[p];
+            float2 vcm = w.vcm[p];
+            const uint32_t k = p / c.npix;
+            uint32_t px, py;
+            pixel_of(c, p % c.npix, px, py);
+            const uint32_t pix = py * c.W + px;
+            Rng rng{px, py, c.seed + c.sample_base + k, s4.x};
+            const uint32_t path_len = s4.y + 1;
+            const v3 dir = unpack_normal(__float_as_uint(r.w));
+            v3 thr = mk(thr4.x, thr4.y, thr4.z);
+            bool specular = (s4.w >> 31) != 0;
+            float4 em = make_float4(0, 0, 0, 0);
+            bool em_dirty = false;
+            // partial eye sub-path MIS weight [tech. rep. (43)]
+            auto add_emission = [&](v3 e, float dpa, float epw) {
+                float wgt = 1;
+                if (path_len > 1 && !specular) wgt = 1 / (1 + (mis1(dpa) * vcm.y + mis1(epw) * vcm.x));
+                if (wgt <= 0 || all_nonpos(e)) return;
+                if (!em_dirty) {
+                    em = w.em[p];
+                    em_dirty = true;
+                }
+                const v3 a = e * wgt;
+                em.x += a.x;
+                em.y += a.y;
+                em.z += a.z;
+            };
+            if (hit.y == 0xffffffffu) {
+                if (path_len == 1 && first_wave && k == 0)
+                    store_aux(c, aux, pix, px, py, 0xffffffffu, pack_normal(dir), false, mk(0, 0, 0), mk(1, 1, 1),
+                              q16(__int_as_float(0x7f800000)));
+                float dpw;
+                const v3 e = thr * evaluate_environment<ALIAS>(S, c.env_prob, dir, dpw);
+                if (any_pos(e)) {
+                    float dpa = (1.0f / ST_PI) / sq(S.sphere[3]);  // concentricDiscPdfA()/R^2
+                    if (c.flags & ST_FLAG_ENV_MIS_BALANCED) dpa = dpw;
+                    add_emission(e, dpa, dpw);
+                }
+                s4.y = path_len;
+            } else {
+                const uint32_t ii = hit.y & 0xffffu, prim = hit.y >> 16;
+                Shading sd;
+                tri_shading(S, ii, prim, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
+                const float pick = 1.0f / (float)(S.inst[ii].data >> 16);
+                float pdfa = 0;  // LightSamplePdfA, lights.hlsli:264-271
+                if (S.light_count > 0) {
+                    pdfa = pick / (sd.area * (float)S.light_count);
+                    if (S.has_env) pdfa *= 1 - c.env_prob;
+                }
+                const float t16 = q16(__uint_as_float(hit.x)), pdfa16 = q16(pdfa);
+                const Material m = load_material(S, sd.material);
+                const Frame f = make_frame(sd);
+                const v3 ldin = f.to_local(-dir);
+                const v3 lgn = f.to_local(unpack_normal(sd.gn));
+                const v3 gn = unpack_normal(sd.gn);
+                const float cos_in = fabsf(ldin.z);
+                vcm.y = qvc(c, vcm.y * mis1(sq(t16)));  // [tech. rep. (31)-(33)], :826-834
+                vcm.y = qvc(c, vcm.y / mis1(cos_in));
+                vcm.x = qvc(c, vcm.x / mis1(cos_in));
+                if (path_len == 1 && first_wave && k == 0)
+                    store_aux(c, aux, pix, px, py, hit.y, sd.sn, true, sd.pos, m.base(), t16);
+                if (ldin.z > 0) {
+                    const v3 e = thr * m.emission();
+                    if (any_pos(e)) add_emission(e, pdfa16, pdfa16 * cos_hemisphere_pdfw(ldin.z));
+                }
+                bool alive = path_len < c.max_depth && any_pos(m.base());
+                const bool rr_on = path_len >= c.rr_start;
+                const float cp_eff = rr_on ? m_cont_prob(m) : 1.0f;
+                if (alive && !m_singular<LAMBERT>(m)) {
+                    // ConnectToLight (:751-779) with the VC weight [tech. rep. (44), (45)]
+                    const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), sd.pos);
+                    if (!(all_nonpos(L.radiance) || L.pdf <= 0)) {
+                        const float AtoW = L.finite ? sq(L.dist) / L.cos_light : 1.0f;
+                        const float dpw = L.pdf * AtoW;
+                        const v3 ldout = f.to_local(L.dir);
+                        Eval fe = bsdf_eval<LAMBERT>(m, ldin, ldout);
+                        fe.f = fe.f * shading_normal_correction(f, lgn, ldin, ldout);
+                        fe.fwd *= cp_eff;
+                        fe.rev *= cp_eff;
+                        const v3 contrib = ((thr * L.radiance) * fe.f) / dpw;
+                        const float epw = L.pdf * cos_hemisphere_pdfw(L.cos_light);
+                        const float wl = mis1(fe.fwd / dpw);
+                        const float wc = mis1(epw * fabsf(ldout.z) / (dpw * L.cos_light)) * (0 + vcm.y + vcm.x * mis1(fe.rev));
+                        const float wgt = 1 / (wl + 1 + wc);
+                        const v3 o = ray_offset(sd.pos, gn, L.dir);
+                        a0 = make_float4(o.x, o.y, o.z, fmax_(L.dist * 0.99f, L.dist - 1e-2f));
+                        a1 = make_float4(L.dir.x, L.dir.y, L.dir.z, __uint_as_float(p));
+                        a2 = make_float4(contrib.x, contrib.y, contrib.z, wgt);
+                        sh_a = true;
+                    }
+                    // SampleLightVertex: next().x picks a cached vertex uniformly
+                    const uint32_t rv = rng.next().x;
+                    if (lv_count > 0) {
+                        const float4* v = w.lv + 3 * (size_t)(rv % lv_count);
+                        const float4 v0 = v[0], v1 = v[1], v2 = v[2];
+                        if (path_len + 1 + __float_as_uint(v2.x) <= c.max_depth) {
+                            const uint32_t vid = __float_as_uint(v0.w);
+                            const v3 lp = mk(v0.x, v0.y, v0.z);
+                            Shading sv;
+                            tri_shading(S, vid & 0xffffu, vid >> 16, 0.0f, 0.0f, sv, &lp);
+                            v3 d = sv.pos - sd.pos;
+                            const float dist = len(d);
+                            d = d / dist;
+                            const Material mv = load_material(S, sv.material);
+                            const Frame fv = make_frame(sv);
+                            const v3 vin = unpack_normal(__float_as_uint(v1.w));
+                            const v3 vout = fv.to_local(-d);
+                            const v3 vlgn = fv.to_local(unpack_normal(sv.gn));
+                            const float cos_light = fabsf(vout.z);
+                            Eval fl = bsdf_eval<LAMBERT, true>(mv, vin, vout);
+                            fl.f = fl.f * shading_normal_correction<true>(fv, vlgn, vin, vout);
+                            const float cpv = m_cont_prob(mv);
+                            fl.fwd *= cpv;
+                            fl.rev *= cpv;
+                            const v3 vthr = mk(v1.x, v1.y, v1.z);
+                            if (!all_nonpos(vthr) && !all_nonpos(fl.f)) {
+                                const v3 ldout = f.to_local(d);
+                                Eval fe = bsdf_eval<LAMBERT>(m, ldin, ldout);
+                                fe.f = fe.f * shading_normal_correction(f, lgn, ldin, ldout);
+                                fe.fwd *= cp_eff;
+                                fe.rev *= cp_eff;
+                                const float wn = (float)lv_count / (float)c.light_paths;
+                                const float inv_d2 = 1 / sq(dist);
+                                const v3 contrib = (thr * vthr) * (((fe.f * fl.f) * inv_d2) * wn);
+                                const float cam_pdfa = fe.fwd * cos_light * inv_d2;
+                                const float light_pdfa = fl.fwd * fabsf(ldout.z) * inv_d2;
+                                const float wl = mis1(cam_pdfa) * (0 + v2.z + v2.y * mis1(fl.rev));   // [tech. rep. (40)]
+                                const float wc = mis1(light_pdfa) * (0 + vcm.y + vcm.x * mis1(fe.rev));  // [tech. rep. (41)]
+                                const float wgt = 1 / (wl + 1 + wc);
+                                const v3 o = ray_offset(sd.pos, gn, d);
+                                b0 = make_float4(o.x, o.y, o.z, fmax_(dist * 0.99f, dist - 1e-2f));
+                                b1 = make_float4(d.x, d.y, d.z, __uint_as_float(p));
+                                b2 = make_float4(contrib.x, contrib.y, contrib.z, wgt);
+                                sh_b = true;
+                            }
+                        }
+                    }
+                }
+                if (alive) {
+                    const float cp = fmin_(1.0f, m_cont_prob(m));
+                    const float rr = rng.nextf().x;
+                    if (rr_on && rr > cp) alive = false;
+                    if (alive) {
+                        const float cpx = rr_on ? cp : 1.0f;
+                        const float4 rb = rng.nextf();
+                        Sample sm = bsdf_sample<LAMBERT>(m, rb.x, rb.y, rb.z, ldin);
+                        sm.fwd *= cpx;
+                        sm.rev *= cpx;
+                        thr = thr * (sm.f / sm.fwd);
+                        const float dirpdf = sm.rough == 0 ? 0.0f : sm.fwd;
+                        vc_after_sample(c, vcm, sm, fabsf(sm.dir.z));
+                        if (sm.rough != 0) specular = false;
+                        thr = thr * shading_normal_correction(f, lgn, ldin, sm.dir);
+                        const v3 wd = f.to_world(sm.dir);
+                        const v3 o = ray_offset(sd.pos, gn, wd);
+                        const bool nan = thr.x != thr.x || thr.y != thr.y || thr.z != thr.z;
+                        cont = !nan && !all_nonpos(thr) && path_len + 1 <= c.max_depth;
+                        ray_out = make_float4(o.x, o.y, o.z, __uint_as_float(pack_normal(wd)));
+                        thr_out = make_float4(thr.x, thr.y, thr.z, 0);
+                        vcm_out = vcm;
+                        s4.w = f2h(dirpdf) | (specular ? 0x80000000u : 0u);
+                    }
+                }
+                s4.y = path_len;
+                s4.z = f2h(t16) | (f2h(pdfa16) << 16);
+            }
+            s4.x = rng.w;
+            w.st[p] = s4;
+            if (em_dirty) w.em[p] = em;
+            if (cont) {
+                w.ray[p] = ray_out;
+                w.thr[p] = thr_out;
+                w.vcm[p] = vcm_out;
+            }
+        }
+        const uint32_t ia = wave_append(&w.counters[2], sh_a);
+        if (sh_a) {
+            w.shadow[3 * (size_t)ia] = a0;
+            w.shadow[3 * (size_t)ia + 1] = a1;
+            w.shadow[3 * (size_t)ia + 2] = a2;
+        }
+        const uint32_t ib = wave_append(&w.counters[2], sh_b);
+        if (sh_b) {
+            w.shadow[3 * (size_t)ib] = b0;
+            w.shadow[3 * (size_t)ib + 1] = b1;
+            w.shadow[3 * (size_t)ib + 2] = b2;
+        }
+        const uint32_t qi = wave_append(&w.counters[qsel ^ 1], cont);
+        if (cont) qout[qi] = p;
+    }
+}
+
+// The light vertex cache of one sample: each subpath's stored vertices moved to
+// its exclusive prefix sum, so the cache is ordered by subpath and depth.
+__global__ void k_lvc_compact(RenderConsts c, WaveState w, uint32_t n) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const uint32_t cnt = w.lv_cnt[q], off = w.lv_off[q];
+        for (uint32_t j = 0; j < cnt; j++)
+            for (int e = 0; e < 3; e++) w.lv[3 * (size_t)(off + j) + e] = w.lv_raw[3 * ((size_t)q * c.lv_stride + j) + e];
+        if (q == n - 1) *w.lv_total = off + cnt;
     }
 }
 
@@ -1479,6 +1797,7 @@ constexpr int kWaveSlots = 4;
 constexpr uint32_t kWavePaths = 2u << 20;
 struct WaveBufs {
     DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, pri, psd, rinit, rtemp, med;
+    DevBuf vcm, lv_raw, lv, lv_cnt, lv_off, lv_total, scan_tmp;  // bidirectional mode
     hipStream_t stream = nullptr;        // slot 0: the context's stream
     hipEvent_t reuse_done = nullptr;     // this slot's last wave finished its screen-space reuse
     hipEvent_t resolve_done = nullptr;   // this slot's last wave finished its resolve
@@ -2232,6 +2551,12 @@ static WaveState wave_state(WaveBufs& b, st_context* ctx) {
     w.r_temp = b.rtemp.as<float4>();
     w.r_prev = ctx->rprev.as<float4>();
     w.med = b.med.as<uint32_t>();
+    w.vcm = b.vcm.as<float2>();
+    w.lv_raw = b.lv_raw.as<float4>();
+    w.lv = b.lv.as<float4>();
+    w.lv_cnt = b.lv_cnt.as<uint32_t>();
+    w.lv_off = b.lv_off.as<uint32_t>();
+    w.lv_total = b.lv_total.as<uint32_t>();
     return w;
 }
 
@@ -2263,11 +2588,13 @@ static void finish_timings(st_context* ctx, int slots, uint64_t* ray_counts) {
 }
 
 static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o);
+static int render_vc(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o);
 
 int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if (!s || !view || !p || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
     if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, o);
+    if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, o);
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && !(p->flags & ST_FLAG_RESTIR_DI))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && (p->spatial_neighbours > 64 || !(p->spatial_radius >= 0) || p->spatial_radius > 4096))
@@ -2501,8 +2828,8 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
                 }
                 {
                     Timed tm(ctx, 1, sm);
-                    auto kfn = media ? (lambert ? k_shade_light<true, true> : k_shade_light<false, true>)
-                                     : (lambert ? k_shade_light<true, false> : k_shade_light<false, false>);
+                    auto kfn = media ? (lambert ? k_shade_light<true, true, false> : k_shade_light<false, true, false>)
+                                     : (lambert ? k_shade_light<true, false, false> : k_shade_light<false, false, false>);
                     hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
                 }
                 {
@@ -2519,6 +2846,151 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
             if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.resolve_done, 0));
             hipLaunchKernelGGL(k_resolve, dim3(rgrid), dim3(256), 0, sm, c, w, ctx->acc.as<float4>(), o->hdr,
                                (uint32_t*)nullptr, 1.0f / (float)p->spp, last ? 1 : 0);
+            HIP_CHECK(hipEventRecord(B.resolve_done, sm));
+            HIP_CHECK(hipGetLastError());
+        }
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
+        finish_timings(ctx, slots, o->ray_counts);
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+// TestRenderer with gUseVC (TestRenderer.cpp:478-482, 559-629), one sample per
+// wave: the light pass (light subpaths store their vertices and connect to the
+// camera, into the pixels' accumulators), the cache compaction, then the camera
+// pass (emission, NEE and a cached-vertex connection at every vertex), then the
+// resolve.  Both passes of a sample run on its slot's stream.
+static int render_vc(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
+    if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
+    if (s->d.vinst_count > 0 || (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_FLOAT_NEE | ST_FLAG_DIRECT_ONLY)) ||
+        p->min_path_length)
+        return set_error(ST_ERROR_UNSUPPORTED, "the bidirectional mode supports surfaces with plain NEE only");
+    if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64))
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "light_subpath_factor must be in [0, 64]");
+    if (p->max_depth < 1 || p->max_depth > 256) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_depth must be in [1, 256]");
+    st_context* ctx = s->ctx;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        const uint32_t npix = p->width * p->height;
+        const uint32_t L = std::max<uint32_t>(1u, (uint32_t)((float)npix * p->light_subpath_factor));
+        const uint32_t stride = std::max<uint32_t>(1u, p->max_depth - 1);
+        const uint64_t paths = std::max<uint64_t>(L, npix);
+        if (paths >= (1ull << 31) || (uint64_t)L * stride >= (1ull << 31))
+            return set_error(ST_ERROR_LIMIT, "too many paths or light vertices per sample");
+        const int slots = (int)std::min<uint32_t>(kWaveSlots, p->spp);
+        const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
+        size_t scan_bytes = 0;
+        HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)L));
+        for (int sl = 0; sl < slots; sl++) {
+            WaveBufs& b = ctx->slot[sl];
+            reserve_wave(b, paths, false, 0, tgrid);
+            b.shadow.reserve(paths * 96);  // NEE and a light-vertex connection per vertex
+            b.vcm.reserve(paths * 8);
+            b.lv_raw.reserve((size_t)L * stride * 48);
+            b.lv.reserve((size_t)L * stride * 48);
+            b.lv_cnt.reserve((size_t)L * 4);
+            b.lv_off.reserve((size_t)L * 4);
+            b.lv_total.reserve(16);
+            b.scan_tmp.reserve(std::max<size_t>(scan_bytes, 16));
+        }
+        ctx->acc.reserve((size_t)npix * 16);
+        ctx->stats.reserve(64);
+        hipStream_t s0 = ctx->stream;
+        HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, s0));
+        HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
+        HIP_CHECK(hipEventRecord(ctx->start_done, s0));
+        for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
+        for (int kk = 0; kk < 3; kk++) ctx->ev_used[kk] = 0;
+        RenderConsts c;
+        std::memset(&c, 0, sizeof c);
+        c.W = p->width;
+        c.H = p->height;
+        c.npix = npix;
+        c.seed = p->seed;
+        c.max_depth = p->max_depth;
+        c.flags = p->flags & ~(uint32_t)ST_FLAG_SAMPLE_DIRECT;  // NEE runs as part of the bidirectional mode
+        c.rr_start = p->rr_start_depth;
+        c.env_prob = p->environment_sample_probability;
+        c.view = *view;
+        c.inv_cam = inverse_transform(view->camera_to_world);
+        c.view_medium = kNoMedium;
+        c.max_null = p->max_null_collisions;
+        c.light_paths = L;
+        c.lv_stride = stride;
+        c.ipd = (float)std::abs(view->image_max[1] - view->image_min[1]) / (2 * std::tan(view->projection.vertical_fov / 2));
+        c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
+        c.k = 1;
+        const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+        const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
+        auto* stats = ctx->stats.as<unsigned long long>();
+        const uint32_t depth = std::max<uint32_t>(1, p->max_depth);
+        auto bounces = [&](hipStream_t sm, const RenderConsts& cc, WaveState& w, uint32_t* spill, bool light,
+                           const AuxOut& aux, int first) {
+            int qsel = 0;
+            for (uint32_t b = 0; b < depth; b++) {
+                {
+                    Timed tm(ctx, 0, sm);
+                    if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                }
+                {
+                    Timed tm(ctx, 1, sm);
+                    if (light) {
+                        auto kfn = lambert ? k_shade_light<true, false, true> : k_shade_light<false, false, true>;
+                        hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, cc, w, qsel);
+                    } else {
+                        auto kfn = lambert ? (alias ? k_shade_vc<true, true> : k_shade_vc<true, false>)
+                                           : (alias ? k_shade_vc<false, true> : k_shade_vc<false, false>);
+                        hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, cc, w, qsel, aux, first);
+                    }
+                }
+                {
+                    Timed tm(ctx, 2, sm);
+                    if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, cc.flags, spill, stats);
+                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, cc.flags, spill, stats);
+                }
+                hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, 0);
+                qsel ^= 1;
+            }
+        };
+        uint32_t wi = 0;
+        for (uint32_t base = 0; base < p->spp; base++, wi++) {
+            WaveBufs& B = ctx->slot[wi % slots];
+            WaveBufs& prev = ctx->slot[(wi + slots - 1) % slots];
+            hipStream_t sm = B.stream;
+            WaveState w = wave_state(B, ctx);
+            uint32_t* spill = B.spill.as<uint32_t>();
+            c.sample_base = base;
+            // light pass: light paths keyed by their dispatch index (path_rng)
+            RenderConsts cl = c;
+            cl.flags |= ST_FLAG_TRACE_FROM_LIGHT;
+            cl.wave_paths = L;
+            HIP_CHECK(hipMemsetAsync(B.counters.p, 0, 16, sm));
+            const int igrid = std::min<int>((uint32_t)((paths + kShadeBlock - 1) / kShadeBlock), ctx->cus * 16);
+            if (alias) hipLaunchKernelGGL(k_light_init<true>, dim3(igrid), dim3(kShadeBlock), 0, sm, s->d, cl, w);
+            else hipLaunchKernelGGL(k_light_init<false>, dim3(igrid), dim3(kShadeBlock), 0, sm, s->d, cl, w);
+            bounces(sm, cl, w, spill, true, AuxOut{nullptr, nullptr, nullptr, nullptr}, 0);
+            // the cache, in (subpath, depth) order
+            size_t sb = B.scan_tmp.n;
+            HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(B.scan_tmp.p, sb, w.lv_cnt, w.lv_off, (int)L, sm));
+            const int lgrid = std::min<int>((L + kShadeBlock - 1) / kShadeBlock, ctx->cus * 16);
+            hipLaunchKernelGGL(k_lvc_compact, dim3(lgrid), dim3(kShadeBlock), 0, sm, c, w, L);
+            // camera pass
+            RenderConsts cc = c;
+            cc.wave_paths = npix;
+            const int cgrid = std::min<int>((npix + kShadeBlock - 1) / kShadeBlock, ctx->cus * 16);
+            hipLaunchKernelGGL(k_camera, dim3(cgrid), dim3(kShadeBlock), 0, sm, cc, w);
+            AuxOut aux{nullptr, nullptr, nullptr, nullptr};
+            if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
+            bounces(sm, cc, w, spill, false, aux, base == 0 ? 1 : 0);
+            const bool last = base + 1 >= p->spp;
+            const int rgrid = std::min<int>((npix + 255) / 256, ctx->cus * 8);
+            if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.resolve_done, 0));
+            hipLaunchKernelGGL(k_resolve, dim3(rgrid), dim3(256), 0, sm, cc, w, ctx->acc.as<float4>(), o->hdr,
+                               o->path_length, 1.0f / (float)p->spp, last ? 1 : 0);
             HIP_CHECK(hipEventRecord(B.resolve_done, sm));
             HIP_CHECK(hipGetLastError());
         }

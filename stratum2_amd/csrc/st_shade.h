@@ -54,7 +54,11 @@ __device__ __forceinline__ void load_tri(const DevScene& S, const st_mesh_vertex
 }
 
 // makeTriangleShadingData(WithoutPosition), shading_data.hlsli:59-151
-__device__ inline void tri_shading(const DevScene& S, uint32_t ii, uint32_t prim, float bx, float by, Shading& r) {
+// lp != nullptr: the shading data at an object-space point on the triangle
+// (makeTriangleShadingData from a local position, shading_data.hlsli:153-174):
+// barycentrics solved from the point, world position T * lp.
+__device__ inline void tri_shading(const DevScene& S, uint32_t ii, uint32_t prim, float bx, float by, Shading& r,
+                                   const v3* lp = nullptr) {
     const st_instance in = S.inst[ii];
     const st_transform T = S.xf[ii];
     const st_mesh_vertex_info vi = S.vi[in.data & 0xffffu];
@@ -63,6 +67,14 @@ __device__ inline void tri_shading(const DevScene& S, uint32_t ii, uint32_t prim
     const uint32_t pb = vi.packed_buffer_indices[0] >> 16, ps = (vi.packed_strides >> 8) & 0xffu;
     const uint8_t* P = S.vb[pb] + vi.packed_offsets[1];
     const v3 v0 = ld_v3(P + (size_t)ps * tri[0]), v1 = ld_v3(P + (size_t)ps * tri[1]), v2 = ld_v3(P + (size_t)ps * tri[2]);
+    if (lp) {
+        const v3 v1v0 = v1 - v0, v2v0 = v2 - v0, p_v0 = *lp - v0;
+        const float d00 = dot(v1v0, v1v0), d01 = dot(v1v0, v2v0), d11 = dot(v2v0, v2v0);
+        const float d20 = dot(p_v0, v1v0), d21 = dot(p_v0, v2v0);
+        const float den = d00 * d11 - d01 * d01;
+        bx = (d11 * d20 - d01 * d21) / den;
+        by = (d00 * d21 - d01 * d20) / den;
+    }
     r.material = in.type_material_address >> 4;
     const v3 dPds = xf_vector(T, v0 - v2), dPdt = xf_vector(T, v1 - v2);
     v3 gn = cross(dPds, dPdt);
@@ -108,7 +120,7 @@ __device__ inline void tri_shading(const DevScene& S, uint32_t ii, uint32_t prim
         r.tan = pack_normal(tg);
         if (dot(sn, gn) < 0) r.gn = pack_normal(-gn);
     }
-    r.pos = xf_point(T, (v0 + (v1 - v0) * bx) + (v2 - v0) * by);
+    r.pos = lp ? xf_point(T, *lp) : xf_point(T, (v0 + (v1 - v0) * bx) + (v2 - v0) * by);
 }
 
 // ShadingData::toLocal / toWorld / shadingNormalCorrection, shading_data.hlsli:24-54
