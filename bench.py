@@ -18,7 +18,8 @@ Prints one JSON line with the driver's contract plus:
   cpu_baseline  the CPU oracle (a C port of the reference path tracer) on this
                 host's cores over a bounded sample of the same frame
 --light-trace times TestRenderer's light-tracing mode on the spheres workload
-instead (light subpaths, one per pixel and sample, connected to the camera).
+instead (light subpaths, one per pixel and sample, connected to the camera);
+--bdpt its bidirectional mode with a light vertex cache (gUseVC).
 --workload svgf times the SVGF denoiser (st_denoise) on a 1920x1080 frame of
 the spheres workload: temporal accumulation with reprojection and albedo
 demodulation, variance estimate, five 5x5 a-trous passes (one step = one
@@ -54,6 +55,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--device-bvh", action="store_true", help="rebuild the BVH on the GPU (st_scene_update) before timing")
     ap.add_argument("--light-trace", action="store_true", help="light tracing (paths start on the lights)")
+    ap.add_argument("--bdpt", action="store_true", help="bidirectional mode with a light vertex cache (gUseVC)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -81,11 +83,11 @@ def main():
         if args.spp == 64:
             args.spp = 32
         args.no_restir = True  # ReSTIR DI is not offered with media
-    if args.light_trace:
+    if args.light_trace or args.bdpt:
         if cloud:
-            print("--light-trace runs the spheres workload (light tracing with media is not offered)", file=sys.stderr)
+            print("--light-trace and --bdpt run the spheres workload (neither is offered with media)", file=sys.stderr)
             sys.exit(2)
-        args.no_restir = True  # the reference turns ReSTIR DI off when light tracing
+        args.no_restir = True  # the reference turns ReSTIR DI off when light tracing; the bidirectional mode uses plain NEE
     max_depth = 16 if cloud else 8
 
     t0 = time.time()
@@ -109,6 +111,8 @@ def main():
     flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
     if args.light_trace:
         flags |= abi.ST_FLAG_TRACE_FROM_LIGHT
+    if args.bdpt:
+        flags |= abi.ST_FLAG_BDPT_LVC
     if not args.no_restir:  # ReSTIR DI: 32 candidates, temporal + spatial (5 neighbours, 30 px) reuse
         flags |= abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
     rr = 0 if cloud else 3
@@ -183,6 +187,10 @@ def main():
         }
         if args.light_trace:  # k_shade_light: hit, ray, throughput, state read; ray, throughput, state written
             classes["k_shade_light"] = classes.pop("k_shade")
+        if args.bdpt:  # k_shade_vc and k_shade_light share the shading class; a cached-vertex connection
+            # adds its 48 B vertex, 64 B triangle and 64 B material
+            ms, ml, mb = classes.pop("k_shade")
+            classes["k_shade_vc"] = (ms, ml, mb + t_last["shadow_rays"] * (48 + 64 + 64) / 2)
     dom = max(classes, key=lambda k: classes[k][0])
     dms, dl, dbytes = classes[dom]
     achieved = dbytes / (dms / 1e3) / 1e9 if dms > 0 else 0.0
@@ -210,11 +218,14 @@ def main():
         workload += ", ReSTIR DI 32 candidates + temporal + spatial 5x30px"
     if args.light_trace:
         workload += ", light tracing (1 light subpath per pixel and sample, connected to the camera)"
+    if args.bdpt:
+        workload += ", bidirectional with a light vertex cache (1 light subpath per pixel and sample)"
     line = {
         "metric": "Mrays/s volume 1024^2" if cloud else "Mrays/s 1080p/1M tris", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": {"workload": workload, "restir_di": not args.no_restir, "light_tracing": args.light_trace,
+                   "bdpt": args.bdpt,
                    "width": args.width, "height": args.height, "spp": args.spp, "max_path_length": max_depth,
                    "triangles": sd.triangle_count, "bvh": "device linear BVH" if args.device_bvh else "host binned SAH",
                    "bvh_build_ms": round(bvh_ms, 2) if bvh_ms is not None else None, "samples_per_px_per_s": round(args.width * args.height * args.spp / (ms / 1e3), 1)},
@@ -328,7 +339,8 @@ def measured_traffic(args, kernel):
     if args.workload in ("cloud", "svgf"):
         name = args.workload
     else:
-        name = "spheres_light" if args.light_trace else ("spheres_plain" if args.no_restir else "spheres_restir")
+        name = ("spheres_light" if args.light_trace else "spheres_bdpt" if args.bdpt else
+                "spheres_plain" if args.no_restir else "spheres_restir")
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{name}.json")
     if not os.path.exists(path):
         return None, None
@@ -366,7 +378,7 @@ def cpu_baseline(sd, view, args, max_depth, log):
     o = OracleScene(sd)
     threads = min(16, os.cpu_count() or 1)  # this host's share of cores
     spp = 1
-    fl = abi.ST_FLAG_DEFAULT | (abi.ST_FLAG_TRACE_FROM_LIGHT if args.light_trace else 0)
+    fl = abi.ST_FLAG_DEFAULT | (abi.ST_FLAG_TRACE_FROM_LIGHT if args.light_trace else 0) | (abi.ST_FLAG_BDPT_LVC if args.bdpt else 0)
     while True:
         p = abi.default_params(args.width, args.height, max_depth=max_depth, spp=spp, flags=fl)
         a = time.perf_counter()
@@ -378,7 +390,8 @@ def cpu_baseline(sd, view, args, max_depth, log):
         spp = min(64, spp * max(2, int(args.cpu_seconds / 3 / max(dt, 1e-3)) + 1))
     log(f"cpu baseline: {args.width}x{args.height} x {spp} spp, {nr} rays in {dt:.2f}s on {threads} threads")
     return {"value": round(nr / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (C port of the reference tracer, {'light tracing' if args.light_trace else 'plain NEE'}), "
+            "sample": f"CPU oracle (C port of the reference tracer, "
+                      f"{'light tracing' if args.light_trace else 'bidirectional, light vertex cache' if args.bdpt else 'plain NEE'}), "
                       f"same {args.width}x{args.height} frame at "
                       f"{spp} spp, maxPathLength {max_depth}, {nr} rays in {dt:.1f} s"}
 
