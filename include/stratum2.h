@@ -154,7 +154,9 @@ typedef struct st_view {
  * quantities (no merging).  The cache is ordered by light subpath and depth
  * (the reference appends in atomic, run-dependent order; both pick uniformly).
  * Supported for surfaces with plain NEE: media, ReSTIR DI, --minPathLength,
- * fp32 NEE and the direct-only mode return ST_ERROR_UNSUPPORTED.
+ * fp32 NEE and the direct-only mode return ST_ERROR_UNSUPPORTED.  Waves hold
+ * one sample (samples_per_wave is ignored); each of the up to four wave slots
+ * keeps a cache of 96 B per light subpath and path segment (max_depth - 1).
  * ST_FLAG_TRACE_FROM_LIGHT takes precedence, as mLightTrace clears gUseVC. */
 #define ST_FLAG_BDPT_LVC                (1u << 11)
 /* With ST_FLAG_BDPT_LVC: keep dVC and dVCM in fp32.  The reference stores them

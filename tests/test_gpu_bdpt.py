@@ -63,6 +63,16 @@ def test_bdpt_environment_emitters_and_ragged_image(gpu_ctx):
     _check(g, r)
 
 
+def test_bdpt_samples_rotate_through_the_wave_slots(gpu_ctx):
+    """Six samples: waves rotate through the four slots, each with its own light
+    vertex cache; the image and the per-sample path lengths still match."""
+    W = H = 32
+    sd = scenes.cornell_box(4)
+    params = abi.default_params(W, H, max_depth=4, spp=6, flags=VC | abi.ST_FLAG_LAMBERTIAN, light_subpath_factor=1.5)
+    g, r = _both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
+    _check(g, r)
+
+
 def test_bdpt_rejects_unsupported(gpu_ctx):
     from stratum2_amd.renderer import Scene, TestRenderer
     W = H = 16
