@@ -16,6 +16,14 @@
  *   env mip-warp sampling                   pinned: sample pdf == SampleTexelPdf, integrates to 1
  *   bounce loop                             pinned: closed-form furnace scene (white Lambert
  *                                           sphere interior), emission-only analytic image
+ *   ReSTIR DI RIS                           pinned: unbiased against one-sample NEE (test_oracle_units.py)
+ *   NanoVDB read path, delta / ratio        pinned: the reference's PNanoVDB compiled into oracle/_ref,
+ *   tracking, Henyey-Greenstein             chord transmittance, furnace, HG pdf (test_oracle_media.py)
+ *   light tracing, adjoint BSDF             pinned: estimates the camera-path image, adjoint
+ *                                           reciprocity (test_oracle_light.py)
+ *   bidirectional mode (light vertex cache) pinned: with fp32 MIS estimates the path tracer's
+ *                                           image; the fp16 overflow (test_oracle_bdpt.py)
+ *   tonemapper                              pinned: operator closed forms (test_oracle_units.py)
  *   driver BVH tie-breaking, f32tof16 and   parity unpinned (Vulkan driver / Slang codegen,
  *   bilinear weights of the reference        not runnable here)
  */

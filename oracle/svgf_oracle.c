@@ -10,7 +10,10 @@
  * is restated in oracle.py.  Images are row-major W*H; colours RGBA32F,
  * moments RG32F, visibility 2 x u32, depth RGBA32F {depth, previous depth,
  * derivative xy}.  getViewIndex is 0 for every pixel
- * (src/Shaders/common/scene.hlsli:109-113), so one view bounds the taps. */
+ * (src/Shaders/common/scene.hlsli:109-113), so one view bounds the taps.
+ * Parity: pinned by closed forms (running means and moments, exact shifts,
+ * the history cap, rejection thresholds, tap patterns as delta responses) in
+ * tests/test_oracle_svgf.py. */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>

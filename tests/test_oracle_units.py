@@ -357,3 +357,24 @@ def test_tonemap_operators_closed_forms():
     assert np.allclose(v1[0, -1, :3], [0.993, 0.906, 0.144], atol=0.05)  # the quintic fit ends at blue 0.10
     s, _ = tonemap(img, mode=abi.ST_TONEMAP_REINHARD, gamma=True)
     assert np.all(np.diff(s[0, :, 0]) > 0) and np.all(s[0, :, 0] < 1)
+
+
+def test_restir_ris_is_an_unbiased_direct_light_estimator():
+    """Resampled importance sampling over 32 light candidates (ReSTIR DI
+    without reuse) estimates the same direct illumination as one light sample:
+    image means within 1.5%, 8x8 block means within 6%, on an environment plus
+    emissive-triangle scene with direct lighting only."""
+    sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+    W, H = 64, 48
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    sc = OracleScene(sd)
+    fl = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_DIRECT_ONLY
+    nee = sc.render(view, abi.default_params(W, H, max_depth=2, spp=512, flags=fl))["hdr"][..., :3].astype(np.float64)
+    ris = sc.render(view, abi.default_params(W, H, max_depth=2, spp=64, seed=777,
+                                             flags=fl | abi.ST_FLAG_RESTIR_DI))["hdr"][..., :3].astype(np.float64)
+    assert abs(ris.mean() / nee.mean() - 1) < 0.015, (ris.mean(), nee.mean())
+    bn = nee.reshape(6, 8, 8, 8, 3).mean(axis=(1, 3, 4))
+    br = ris.reshape(6, 8, 8, 8, 3).mean(axis=(1, 3, 4))
+    lit = bn > 0.05 * bn.max()
+    print("block ratio range", (br[lit] / bn[lit]).min(), (br[lit] / bn[lit]).max())
+    assert np.all(np.abs(br[lit] / bn[lit] - 1) < 0.06)
