@@ -1698,44 +1698,73 @@ static void connect_to_light(pctx_t* x, const material_t* m, const shading_t* sd
 /* SampleLightVertex + ConnectToLightSubpath, testrenderer.slang:355-437: one
  * uniformly chosen vertex of this sample's light vertex cache (picked by
  * next().x modulo the cache size), weighted by cache size / light subpaths. */
-static void connect_to_light_subpath(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
+static void connect_to_light_subpath(pctx_t* x, const material_t* m, const shading_t* sd, const medium_t* cmed, f3 cpos,
+                                     f3 ldin) {
     uint32_t rv[4];
     rng_next(&x->ps.rng, rv);
     if (x->lvc_count == 0) return;
     const lvert_t* v = &x->lvc[rv[0] % x->lvc_count];
     if (x->ps.path_len + 1 + v->plen > x->c->max_depth) return;
     const f3 lp = v->lpos;
+    const uint32_t vi = v->inst_prim & 0xffffu;
+    const int vmed = (x->s->inst[vi].type_material_address & 0xfu) == 2; /* a medium vertex (isMedium) */
     shading_t sv;
     memset(&sv, 0, sizeof sv);
-    make_tri_shading_x(x->s, v->inst_prim & 0xffffu, v->inst_prim >> 16, 0, 0, &lp, &sv);
-    f3 dir = sub(sv.pos, sd->pos);
+    f3 vpos;
+    if (vmed) vpos = xf_point(&x->s->xf[vi], lp); /* makeVolumeShadingData, shading_data.hlsli:186-194 */
+    else {
+        make_tri_shading_x(x->s, vi, v->inst_prim >> 16, 0, 0, &lp, &sv);
+        vpos = sv.pos;
+    }
+    f3 dir = sub(vpos, cpos);
     const float dist = length3(dir);
     dir = divf(dir, dist);
-    const material_t mv = load_material(x->s, sv.flags_material >> 4);
     const f3 vin = unpack_normal(v->dirin);
-    const f3 vout = to_local(&sv, neg(dir));
-    const float cos_light = fabsf(vout.z);
-    rer_t fv = bsdf_eval_x(&mv, x->c->lambert, vin, vout, 1);
-    fv.f = mulf(fv.f, shading_normal_correction_x(&sv, vin, vout, 1));
-    const float cpv = m_cont_prob(&mv);
+    rer_t fv;
+    float cos_light, cpv;
+    if (vmed) {
+        const medium_t mv = load_medium(x->s, x->s->inst[vi].type_material_address >> 4);
+        const float ph = hg_eval(mv.g, vin, neg(dir));
+        fv.f = v3(ph, ph, ph); fv.fwd = ph; fv.rev = ph;
+        cos_light = 1;
+        cpv = med_cont_prob(&mv);
+    } else {
+        const material_t mv = load_material(x->s, sv.flags_material >> 4);
+        const f3 vout = to_local(&sv, neg(dir));
+        cos_light = fabsf(vout.z);
+        fv = bsdf_eval_x(&mv, x->c->lambert, vin, vout, 1);
+        fv.f = mulf(fv.f, shading_normal_correction_x(&sv, vin, vout, 1));
+        cpv = m_cont_prob(&mv);
+    }
     fv.fwd *= cpv; fv.rev *= cpv;
     if (all_le0(v->thr) || all_le0(fv.f)) return;
-    const f3 ldout = to_local(sd, dir);
-    rer_t f = bsdf_eval(m, x->c->lambert, ldin, ldout);
-    f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
-    const float cp = m_cont_prob(m);
-    f.fwd *= cp; f.rev *= cp;
+    rer_t f;
+    float cos_cam;
+    if (cmed) {
+        const float ph = hg_eval(cmed->g, ldin, dir);
+        f.f = v3(ph, ph, ph); f.fwd = ph; f.rev = ph;
+        cos_cam = 1;
+        const float cp = med_cont_prob(cmed);
+        f.fwd *= cp; f.rev *= cp;
+    } else {
+        const f3 ldout = to_local(sd, dir);
+        cos_cam = fabsf(ldout.z);
+        f = bsdf_eval(m, x->c->lambert, ldin, ldout);
+        f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
+        const float cp = m_cont_prob(m);
+        f.fwd *= cp; f.rev *= cp;
+    }
     const float wn = (float)x->lvc_count / (float)x->L;
     const float inv_d2 = 1 / pow2(dist);
     const f3 contrib = mul(mul(x->ps.thr, v->thr), mulf(mulf(mul(f.f, fv.f), inv_d2), wn));
     const float cam_pdfa = f.fwd * cos_light * inv_d2;
-    const float light_pdfa = fv.fwd * fabsf(ldout.z) * inv_d2;
+    const float light_pdfa = fv.fwd * cos_cam * inv_d2;
     const float wl = mis1(cam_pdfa) * (0 + v->dvcm + v->dvc * mis1(fv.rev));   /* [tech. rep. (40)] */
     const float wc = mis1(light_pdfa) * (0 + x->ps.dvcm + x->ps.dvc * mis1(f.rev)); /* [tech. rep. (41)] */
     const float w = 1 / (wl + 1 + wc);
     if (x->nsh >= 64) return;
     shadow_t* sr = &x->sh[x->nsh++];
-    sr->o = ray_offset_dir(sd->pos, unpack_normal(sd->gn), dir);
+    sr->o = cmed ? cpos : ray_offset_dir(sd->pos, unpack_normal(sd->gn), dir); /* rayOffset of a medium vertex: itself */
     sr->d = dir;
     sr->tmax = maxf(dist * 0.99f, dist - 1e-2f); /* AdjustIntersectionDistance, intersection.hlsli:65 */
     sr->c = contrib;
@@ -1908,18 +1937,66 @@ static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, u
 }
 
 /* ProcessInteraction for a Medium vertex (testrenderer.slang:816-935 with isMedium). */
+/* Sub-path MIS quantities after a BSDF sample, ProcessInteraction
+ * (testrenderer.slang:909-922): [tech. rep. (53)-(55)] for a singular sample,
+ * else (34)-(36); a camera path is no longer specular after a non-singular one. */
+static void vc_after_sample(pstate_t* ps, const dsr_t* smp, float cos_out, int from_light) {
+    if (smp->rough == 0) {
+        ps->dvcm = qvc(ps, 0.0f);
+        ps->dvc = qvc(ps, ps->dvc * mis1(cos_out));
+    } else {
+        if (!from_light) ps->flag31 = 0;
+        ps->dvc = qvc(ps, mis1(cos_out / smp->fwd) * (ps->dvc * mis1(smp->rev) + ps->dvcm + 0));
+        ps->dvcm = qvc(ps, mis1(1.0f / smp->fwd));
+    }
+}
+
 static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
     const or_scene* s = x->s;
     pstate_t* ps = &x->ps;
     const medium_t m = load_medium(s, s->inst[ps->medium].type_material_address >> 4);
     const f3 ldin = neg(dir);
+    const int vc = (x->c->flags & ST_FLAG_BDPT_LVC) != 0;
+    if (vc) { /* [tech. rep. (31)-(33)]; cosIn of a medium vertex is 1 (testrenderer.slang:826-834) */
+        ps->dvcm = qvc(ps, ps->dvcm * mis1(pow2(ps->t16)));
+        ps->dvcm = qvc(ps, ps->dvcm / mis1(1.0f));
+        ps->dvc = qvc(ps, ps->dvc / mis1(1.0f));
+    }
     if (ps->path_len == 1) {
         x->vis[0] = ps->inst_prim; x->vis[1] = 0;
         store_aux(x, 1, pos, med_albedo(&m));
     }
     if (ps->path_len >= x->c->max_depth) return 0;
     if (!med_can_eval(&m)) return 0;
-    if (!med_singular(&m) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
+    if (!med_singular(&m) && vc) {
+        float rnd[4];
+        rng_nextf(&ps->rng, rnd);
+        isr_t r = sample_illumination(s, x->c, rnd, pos);
+        if (!(all_le0(r.radiance) || r.pdf <= 0)) {
+            float AtoW = 1;
+            if (r.finite) AtoW = pow2(r.dist) / r.cos_light;
+            const float dpw = r.pdf * AtoW;
+            const float v = hg_eval(m.g, ldin, r.dir);
+            const float cp = med_cont_prob(&m);
+            const float fwd = v * cp, rev = v * cp;
+            const f3 contrib = divf(mulf(mul(ps->thr, r.radiance), v), dpw);
+            /* ComputeMIS_DI with gUseVC; cosOut of a medium vertex is 1 */
+            const float epw = r.pdf * cos_hemisphere_pdfw(r.cos_light);
+            const float wl = mis1(fwd / dpw);
+            const float wc = mis1(epw * 1.0f / (dpw * r.cos_light)) * (0 + ps->dvcm + ps->dvc * mis1(rev));
+            if (x->nsh < 64) {
+                shadow_t* sr = &x->sh[x->nsh++];
+                sr->o = pos;
+                sr->d = r.dir;
+                sr->tmax = maxf(r.dist * 0.99f, r.dist - 1e-2f);
+                sr->c = contrib;
+                sr->w = 1 / (wl + 1 + wc);
+                sr->medium = ps->medium;
+                sr->plen = ps->path_len;
+            }
+        }
+        connect_to_light_subpath(x, NULL, NULL, &m, pos, ldin);
+    } else if (!med_singular(&m) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
         float rnd[4];
         rng_nextf(&ps->rng, rnd);
         isr_t r = sample_illumination(s, x->c, rnd, pos);
@@ -1956,6 +2033,12 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
     const float fwd = v * cp;
     ps->thr = mulf(ps->thr, v / fwd);
     ps->dirpdf16 = (1 - fabsf(m.g)) == 0 ? 0 : q16(fwd);
+    if (vc) {
+        dsr_t smp;
+        memset(&smp, 0, sizeof smp);
+        smp.fwd = fwd; smp.rev = fwd; smp.rough = (1 - fabsf(m.g)) == 0 ? 0.0f : 1.0f;
+        vc_after_sample(ps, &smp, 1.0f, 0);
+    }
     ps->dir = pack_normal(sdir);
     ps->origin = pos;
     if (ps->thr.x != ps->thr.x || ps->thr.y != ps->thr.y || ps->thr.z != ps->thr.z) return 0;
@@ -1964,20 +2047,6 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
 
 /* TraceRay + ExtendPath + ProcessInteraction, testrenderer.slang:816-935, 1026-1071.
  * Returns 0 when the path ends. */
-/* Sub-path MIS quantities after a BSDF sample, ProcessInteraction
- * (testrenderer.slang:909-922): [tech. rep. (53)-(55)] for a singular sample,
- * else (34)-(36); a camera path is no longer specular after a non-singular one. */
-static void vc_after_sample(pstate_t* ps, const dsr_t* smp, float cos_out, int from_light) {
-    if (smp->rough == 0) {
-        ps->dvcm = qvc(ps, 0.0f);
-        ps->dvc = qvc(ps, ps->dvc * mis1(cos_out));
-    } else {
-        if (!from_light) ps->flag31 = 0;
-        ps->dvc = qvc(ps, mis1(cos_out / smp->fwd) * (ps->dvc * mis1(smp->rev) + ps->dvcm + 0));
-        ps->dvcm = qvc(ps, mis1(1.0f / smp->fwd));
-    }
-}
-
 static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view) {
     (void)p; (void)view;
     const or_scene* s = x->s;
@@ -2071,7 +2140,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     if (!m_can_eval(&m)) return 0;
     if (!m_singular(&m, x->c->lambert) && vc) {
         connect_to_light(x, &m, &sd, ldin);
-        connect_to_light_subpath(x, &m, &sd, ldin);
+        connect_to_light_subpath(x, &m, &sd, NULL, sd.pos, ldin);
     } else if (!m_singular(&m, x->c->lambert) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
         if (x->c->flags & ST_FLAG_RESTIR_DI) connect_to_light_restir(x, &m, &sd, ldin);
         else connect_to_light(x, &m, &sd, ldin);
@@ -2232,6 +2301,7 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
         f = v3(v, v, v);
         cos_out = 1;
         o = pos;
+        frev = v * med_cont_prob(med);
     } else {
         const f3 ldout = to_local(sd, tc);
         cos_out = fabsf(ldout.z);
@@ -2342,7 +2412,23 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
         if (kind == 2) {
             const medium_t m = load_medium(s, s->inst[ps->medium].type_material_address >> 4);
             const f3 ldin = neg(wd);
+            if (vc) { /* testrenderer.slang:812-824 with cosIn 1 */
+                ps->t16 = q16(h.t);
+                if (ps->path_len > 1 || ps->flag31) ps->dvcm = qvc(ps, ps->dvcm * mis1(pow2(ps->t16)));
+                ps->dvcm = qvc(ps, ps->dvcm / mis1(1.0f));
+                ps->dvc = qvc(ps, ps->dvc / mis1(1.0f));
+            }
             if (ps->path_len >= c->max_depth || !med_can_eval(&m)) return;
+            if (vc && !med_singular(&m) && ps->path_len + 2 <= c->max_depth) {
+                lvert_t* v = &lx->verts[(size_t)q * (c->max_depth - 1) + lx->nverts[q]++];
+                v->lpos = xf_point(&s->inv_xf[h.id & 0xffffu], mpos);
+                v->inst_prim = h.id;
+                v->thr = ps->thr;
+                v->dirin = pack_normal(ldin);
+                v->plen = ps->path_len;
+                v->dvc = ps->dvc;
+                v->dvcm = ps->dvcm;
+            }
             if (!med_singular(&m) && ps->path_len + 1 >= c->min_len) connect_to_camera(lx, NULL, NULL, &m, mpos, ldin, q);
             const float cp = minf(1, med_cont_prob(&m));
             float rr[4], rb[4];
@@ -2353,6 +2439,12 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
             const float v = hg_eval(m.g, ldin, sdir);
             const float fwd = v * cp;
             ps->thr = mulf(ps->thr, v / fwd);
+            if (vc) {
+                dsr_t smp;
+                memset(&smp, 0, sizeof smp);
+                smp.fwd = fwd; smp.rev = fwd; smp.rough = (1 - fabsf(m.g)) == 0 ? 0.0f : 1.0f;
+                vc_after_sample(ps, &smp, 1.0f, 1);
+            }
             ps->dir = pack_normal(sdir);
             ps->origin = mpos;
         } else {
@@ -2472,6 +2564,22 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
     return 0;
 }
 
+/* The camera's medium: the last volume whose object-space bounds hold the
+ * camera position (TestRenderer.cpp:431-446). */
+static uint32_t camera_medium(const or_scene* s, const st_view* view) {
+    uint32_t view_medium = 0xFFFFu;
+    const f3 cam = v3(view->camera_to_world.m[0][3], view->camera_to_world.m[1][3], view->camera_to_world.m[2][3]);
+    for (uint32_t i = 0; i < s->nvolinfo; i++) {
+        const st_volume_info* vi = &s->volinfo[i];
+        if (vi->instance_index >= s->ninst) continue;
+        const f3 lp = xf_point(&s->inv_xf[vi->instance_index], cam);
+        if (lp.x >= vi->min[0] && lp.y >= vi->min[1] && lp.z >= vi->min[2] && lp.x <= vi->max[0] && lp.y <= vi->max[1] &&
+            lp.z <= vi->max[2])
+            view_medium = vi->instance_index;
+    }
+    return view_medium;
+}
+
 /* TestRenderer with gUseVC (TestRenderer.cpp:478-482, 559-629): per sample,
  * the light dispatch (light subpaths store their vertices and connect to the
  * camera), then the view dispatch (emission, NEE and one light-vertex
@@ -2479,7 +2587,13 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
  * the view paths' image.  The light vertex cache is ordered by light subpath
  * and depth. */
 static int render_vc(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
-    if (s->nvolinst > 0 || (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_FLOAT_NEE | ST_FLAG_DIRECT_ONLY)) || p->min_path_length)
+    /* Participating media stay rejected: the medium-vertex branches below
+     * follow TestRenderer's volume code, but with them the four strategies'
+     * weights sum to about 1.18 on a Cornell box with a scattering block (each
+     * strategy alone matches the path tracer), so that combination is left
+     * open. */
+    if (s->nvolinst > 0 || (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_FLOAT_NEE | ST_FLAG_DIRECT_ONLY)) ||
+        p->min_path_length)
         return fail("the bidirectional mode supports surfaces with plain NEE only");
     if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64)) return fail("light_subpath_factor out of range");
     if (p->max_depth < 1) return fail("max_depth must be at least 1");
@@ -2496,6 +2610,7 @@ static int render_vc(or_scene* s, const st_view* view, const st_render_params* p
     const float ipd = (float)abs(view->image_max[1] - view->image_min[1]) / (2 * tanf(view->projection.vertical_fov / 2));
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     const size_t slots = (size_t)L * (p->max_depth - 1 ? p->max_depth - 1 : 1);
+    const uint32_t view_medium = camera_medium(s, view);
     uint32_t* total = calloc((size_t)npix * 3, 4);
     f3* sum = calloc(npix, sizeof(f3));
     lvert_t* verts = malloc(sizeof(lvert_t) * slots);
@@ -2543,7 +2658,7 @@ static int render_vc(or_scene* s, const st_view* view, const st_render_params* p
         {
             pctx_t x;
             memset(&x, 0, sizeof x);
-            x.s = s; x.c = &c; x.view_medium = 0xFFFFu; x.view = view; x.inv_cam = inv_cam;
+            x.s = s; x.c = &c; x.view_medium = view_medium; x.view = view; x.inv_cam = inv_cam;
             x.W = W; x.H = H;
             x.lvc = lvc; x.lvc_count = count; x.L = L; x.ipd = ipd;
 #pragma omp for schedule(dynamic, 64)
@@ -2603,18 +2718,7 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
     c.cand = p->candidate_samples;
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
     c.max_null = p->max_null_collisions;
-    uint32_t view_medium = 0xFFFFu;
-    {
-        const f3 cam = v3(view->camera_to_world.m[0][3], view->camera_to_world.m[1][3], view->camera_to_world.m[2][3]);
-        for (uint32_t i = 0; i < s->nvolinfo; i++) {
-            const st_volume_info* vi = &s->volinfo[i];
-            if (vi->instance_index >= s->ninst) continue;
-            const f3 lp = xf_point(&s->inv_xf[vi->instance_index], cam);
-            if (lp.x >= vi->min[0] && lp.y >= vi->min[1] && lp.z >= vi->min[2] && lp.x <= vi->max[0] && lp.y <= vi->max[1] &&
-                lp.z <= vi->max[2])
-                view_medium = vi->instance_index;
-        }
-    }
+    const uint32_t view_medium = camera_medium(s, view);
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
 #ifdef _OPENMP
