@@ -118,6 +118,56 @@ def test_trace_rays_closest_and_any(gpu_ctx):
     assert same > 0.999
 
 
+def test_deep_traversal_stacks_spill_to_memory(gpu_ctx):
+    """60,000 nearly parallel triangles stacked along -z.  A ray down the stack
+    enters all four child boxes of every node on its way, so nearest-first
+    traversal pushes three entries per level: with a tree of depth >= 6 the
+    per-lane stack runs past its 16 LDS entries into the HBM spill buffer.
+    Closest and any hits must equal the oracle's and the closest hits the
+    FP64 brute force, on the host SAH tree and on the device-built tree."""
+    import torch
+    from stratum2_amd.renderer import Scene
+    from oracle.oracle import OracleScene
+    rng = np.random.default_rng(11)
+    k = 60000
+    z = -1.0 - 0.0005 * np.arange(k)
+    j = rng.uniform(-0.3, 0.3, (k, 3))
+    pos = np.zeros((k, 3, 3), np.float32)
+    pos[:, 0] = np.stack([-2 + j[:, 0], -2 + j[:, 1], z], -1)
+    pos[:, 1] = np.stack([2 + j[:, 2], -2 + j[:, 1], z], -1)
+    pos[:, 2] = np.stack([j[:, 0], 2 + j[:, 2], z], -1)
+    sd = scenes.SceneData()
+    vi, n = sd.add_mesh(pos.reshape(-1, 3), np.arange(3 * k, dtype=np.uint32))
+    sd.add_mesh_instance(vi, n, sd.add_material(scenes.material_record()))
+    nr = 3000
+    o = np.zeros((nr, 3))
+    o[:, :2] = rng.uniform(-0.5, 0.5, (nr, 2))
+    o[nr // 2:, 2] = rng.uniform(-25, -2, nr - nr // 2)          # half start inside the stack
+    d = np.stack([rng.normal(0, 0.02, nr), rng.normal(0, 0.02, nr), -np.ones(nr)], -1)
+    d[nr // 2::2, 2] = 1.0                                          # some of those look back out
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    tmax = np.where(rng.uniform(size=nr) < 0.7, np.inf, rng.uniform(0.0001, 0.01, nr))
+    rays = np.zeros((nr, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7] = o, tmax, d
+    orc = OracleScene(sd)
+    rc, ra, bf = orc.trace_rays(rays, True), orc.trace_rays(rays, False), orc.trace_rays_bruteforce(rays)
+    assert (rc[:, 1] != 0xFFFFFFFF).mean() > 0.5 and (ra == 0).any()
+    assert (bf[:, 1] == rc[:, 1]).mean() > 0.999
+    sc = Scene(gpu_ctx, sd)
+    gt = torch.from_numpy(rays).cuda()
+    for tree in ("host SAH", "device"):
+        if tree == "device":
+            sc.update(np.stack(sd.transforms))
+        st = sc.stats()
+        print(f"{tree} tree: {st}")
+        assert st["triangles"] == k and 3 * st["depth"] > 16
+        hc = sc.trace_rays(gt, True).cpu().numpy().view(np.uint32)
+        ha = sc.trace_rays(gt, False).cpu().numpy().view(np.uint32)
+        assert np.array_equal(hc[:, 1], rc[:, 1]), tree
+        assert np.array_equal(hc[:, 0], rc[:, 0]), tree
+        assert np.array_equal(ha, ra), tree
+
+
 def test_empty_and_tiny_scenes(gpu_ctx):
     """A scene of one triangle (root with an empty child) and a miss-only render."""
     import torch
