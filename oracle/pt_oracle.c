@@ -2504,7 +2504,9 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
     c.max_null = p->max_null_collisions;
     const uint32_t W = p->width, H = p->height, npix = W * H;
     /* mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452 */
-    uint32_t L = (uint32_t)((float)npix * p->light_subpath_factor);
+    const float Lf = (float)npix * p->light_subpath_factor; /* bounded before the conversion */
+    if (!(Lf < 2147483648.0f)) return fail("too many light subpaths");
+    uint32_t L = (uint32_t)Lf;
     if (L < 1) L = 1;
     /* ViewData::imagePlaneDist, compat/scene.h:147 */
     const float ipd = (float)abs(view->image_max[1] - view->image_min[1]) / (2 * tanf(view->projection.vertical_fov / 2));
@@ -2605,7 +2607,9 @@ static int render_vc(or_scene* s, const st_view* view, const st_render_params* p
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
     c.max_null = p->max_null_collisions;
     const uint32_t W = p->width, H = p->height, npix = W * H;
-    uint32_t L = (uint32_t)((float)npix * p->light_subpath_factor);
+    const float Lf = (float)npix * p->light_subpath_factor; /* bounded before the conversion */
+    if (!(Lf < 2147483648.0f)) return fail("too many light subpaths");
+    uint32_t L = (uint32_t)Lf;
     if (L < 1) L = 1;
     const float ipd = (float)abs(view->image_max[1] - view->image_min[1]) / (2 * tanf(view->projection.vertical_fov / 2));
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);

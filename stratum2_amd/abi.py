@@ -11,6 +11,11 @@ import os
 u32, i32, f32, u64 = C.c_uint32, C.c_int32, C.c_float, C.c_uint64
 
 ST_OK = 0
+ST_ERROR_INVALID_ARGUMENT = 1
+ST_ERROR_UNSUPPORTED = 2
+ST_ERROR_HIP = 3
+ST_ERROR_OUT_OF_MEMORY = 4
+ST_ERROR_LIMIT = 5
 ST_INVALID_INSTANCE = 0xFFFF
 ST_NO_IMAGE = 0xFFFFFFFF
 

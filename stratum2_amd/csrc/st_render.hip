@@ -2757,8 +2757,11 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
         const uint32_t npix = p->width * p->height;
-        // mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452
-        const uint32_t L = std::max<uint32_t>(1u, (uint32_t)((float)npix * p->light_subpath_factor));
+        // mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452;
+        // the product is bounded first, so the conversion stays defined
+        const float Lf = (float)npix * p->light_subpath_factor;
+        if (!(Lf < 2147483648.0f)) return set_error(ST_ERROR_LIMIT, "too many light subpaths");
+        const uint32_t L = std::max<uint32_t>(1u, (uint32_t)Lf);
         const uint32_t span = std::max(L, npix);
         uint32_t k = p->samples_per_wave;
         if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, kWavePaths / span));
@@ -2874,7 +2877,9 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
         const uint32_t npix = p->width * p->height;
-        const uint32_t L = std::max<uint32_t>(1u, (uint32_t)((float)npix * p->light_subpath_factor));
+        const float Lf = (float)npix * p->light_subpath_factor;  // bounded before the conversion
+        if (!(Lf < 2147483648.0f)) return set_error(ST_ERROR_LIMIT, "too many light subpaths");
+        const uint32_t L = std::max<uint32_t>(1u, (uint32_t)Lf);
         const uint32_t stride = std::max<uint32_t>(1u, p->max_depth - 1);
         const uint64_t paths = std::max<uint64_t>(L, npix);
         if (paths >= (1ull << 31) || (uint64_t)L * stride >= (1ull << 31))

@@ -41,3 +41,23 @@ def test_errors_without_gpu_are_reported_not_crashing():
     lib = abi.load_library()
     assert lib.st_render(None, None, None, None) != 0
     assert lib.st_last_error()
+
+
+def test_python_constants_match_the_header():
+    """Every ST_FLAG_*, ST_ERROR_*, ST_DENOISE_*, ST_FILTER_* and ST_TONEMAP_*
+    value that abi.py defines equals the header's #define."""
+    text = open(os.path.join(ROOT, "include", "stratum2.h")).read()
+    defines = {}
+    for name, expr in re.findall(r"^#define\s+(ST_\w+)\s+(.+?)\s*(?:/[/*].*)?$", text, re.M):
+        try:
+            defines[name] = eval(expr.replace("u", ""), {}, dict(defines))
+        except Exception:
+            pass
+    checked = 0
+    for name in dir(abi):
+        if re.match(r"ST_(FLAG|ERROR|DENOISE|FILTER|TONEMAP)_", name) and name in defines:
+            assert getattr(abi, name) == defines[name], name
+            checked += 1
+    for name in ("ST_ERROR_INVALID_ARGUMENT", "ST_ERROR_UNSUPPORTED", "ST_ERROR_LIMIT", "ST_FLAG_BDPT_LVC"):
+        assert name in defines and hasattr(abi, name), name
+    assert checked >= 20

@@ -168,6 +168,54 @@ def test_deep_traversal_stacks_spill_to_memory(gpu_ctx):
         assert np.array_equal(ha, ra), tree
 
 
+LIMIT_CASES = [
+    # (flags, params, expected status); every case is refused before anything is allocated
+    (0, dict(width=0, height=8), "ST_ERROR_INVALID_ARGUMENT"),
+    (0, dict(width=8, height=8, spp=0), "ST_ERROR_INVALID_ARGUMENT"),
+    (0, dict(width=16385, height=16385), "ST_ERROR_LIMIT"),
+    (0, dict(width=1920, height=1080, spp=1100, samples_per_wave=1100), "ST_ERROR_LIMIT"),
+    ("ST_FLAG_RESTIR_DI_REUSE", dict(width=8, height=8), "ST_ERROR_INVALID_ARGUMENT"),
+    ("ST_FLAG_RESTIR_DI", dict(width=8, height=8, candidate_samples=0), "ST_ERROR_INVALID_ARGUMENT"),
+    ("ST_FLAG_RESTIR_DI", dict(width=8, height=8, candidate_samples=1025), "ST_ERROR_INVALID_ARGUMENT"),
+    (0, dict(width=8, height=8, max_null_collisions=(1 << 20) + 1), "ST_ERROR_INVALID_ARGUMENT"),
+    ("ST_FLAG_TRACE_FROM_LIGHT", dict(width=16385, height=16385), "ST_ERROR_LIMIT"),
+    ("ST_FLAG_TRACE_FROM_LIGHT", dict(width=1920, height=1080, spp=1100, samples_per_wave=1100), "ST_ERROR_LIMIT"),
+    ("ST_FLAG_TRACE_FROM_LIGHT", dict(width=16384, height=16384, light_subpath_factor=64.0), "ST_ERROR_LIMIT"),
+    ("ST_FLAG_TRACE_FROM_LIGHT", dict(width=8, height=8, light_subpath_factor=65.0), "ST_ERROR_INVALID_ARGUMENT"),
+    ("ST_FLAG_BDPT_LVC", dict(width=16384, height=16384, light_subpath_factor=64.0), "ST_ERROR_LIMIT"),
+    ("ST_FLAG_BDPT_LVC", dict(width=8, height=8, max_depth=300), "ST_ERROR_INVALID_ARGUMENT"),
+]
+
+
+@pytest.mark.parametrize("case", range(len(LIMIT_CASES)))
+def test_render_rejects_out_of_range_arguments(gpu_ctx, case):
+    """st_render's argument and size checks, called through the C-ABI with a
+    64-byte output: images above 2^28 pixels, waves of 2^31 paths or more,
+    light-subpath counts of 2^31 or more (W * H * factor, bounded before its
+    conversion to an integer) and out-of-range settings return their status
+    with nothing allocated or launched."""
+    import ctypes as C
+    import torch
+    from stratum2_amd.renderer import Scene
+    flags, kw, want = LIMIT_CASES[case]
+    fl = abi.ST_FLAG_DEFAULT | (getattr(abi, flags) if flags else 0)
+    if flags == "ST_FLAG_RESTIR_DI_REUSE":
+        fl = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI_REUSE   # reuse without ST_FLAG_RESTIR_DI
+    W, H = kw.pop("width"), kw.pop("height")
+    sc = Scene(gpu_ctx, scenes.cornell_box(0))
+    dummy = torch.zeros(16, dtype=torch.float32, device="cuda")
+    o = abi.st_outputs()
+    o.hdr = dummy.data_ptr()
+    p = abi.default_params(W, H, flags=fl, **kw)
+    view = scenes.make_view(max(W, 1), max(H, 1))
+    status = sc.lib.st_render(sc.h, C.byref(view), C.byref(p), C.byref(o))
+    torch.cuda.synchronize()
+    msg = sc.lib.st_last_error()
+    print(f"case {case}: status {status}: {msg.decode() if msg else ''}")
+    assert status == getattr(abi, want)
+    assert float(dummy.abs().sum()) == 0.0
+
+
 def test_empty_and_tiny_scenes(gpu_ctx):
     """A scene of one triangle (root with an empty child) and a miss-only render."""
     import torch
