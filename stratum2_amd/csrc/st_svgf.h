@@ -29,6 +29,19 @@ namespace st {
 constexpr int kSvgfTileX = 64, kSvgfTileY = 4, kSvgfXcds = 8;
 constexpr uint32_t kSvgfInvalidInstance = 0xFFFFu;
 
+// The edge-stopping weights' transcendentals, as shading hardware evaluates
+// them: pow(x, 2^K) by K squarings (within (2^K - 1)/2 ulp of the exact power,
+// about the error of exp2(y * log2 x) on a GPU) and exp(x) as the hardware
+// 2^x of x * log2(e).  Both are a few instructions where the library
+// functions take dozens, and these taps are ALU-bound.
+template <int K>
+__device__ __forceinline__ float pow_2k(float x) {
+#pragma unroll
+    for (int i = 0; i < K; i++) x = x * x;
+    return x;
+}
+__device__ __forceinline__ float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
+
 struct SvgfView {
     int min_x, min_y, max_x, max_y;  // ViewData::mImageMin / mImageMax
 };
@@ -259,8 +272,8 @@ __global__ __launch_bounds__(256) void k_svgf_variance(SvgfVarianceArgs a) {
                 const float gx = dep.z * (float)xx, gy = dep.w * (float)yy;
                 w_z = fabsf(gj.w - dep.x) / (sqrtf(gx * gx + gy * gy) + 1e-2f);
             }
-            if (a.flags & ST_DENOISE_CHECK_NORMAL) w_n = powf(sat(dot(mk(gj.x, gj.y, gj.z), n)), 128.0f);
-            const float w = expf(-w_z) * w_n;
+            if (a.flags & ST_DENOISE_CHECK_NORMAL) w_n = pow_2k<7>(sat(dot(mk(gj.x, gj.y, gj.z), n)));
+            const float w = exp_hw(-w_z) * w_n;
             if (!isfinite(w)) continue;
             const float2 mj = a.moments[j];
             const float4 cj = a.color[j];
@@ -322,9 +335,9 @@ struct SvgfTap {
                 const float gx = dzx * (float)(ox * step), gy = dzy * (float)(oy * step);
                 w_z = fabsf(g.w - z) / (sqrtf(gx * gx + gy * gy) + 1e-2f);
             }
-            if (flags & ST_DENOISE_CHECK_NORMAL) w_n = powf(fmax_(0.0f, dot(mk(g.x, g.y, g.z), n)), 256.0f);
+            if (flags & ST_DENOISE_CHECK_NORMAL) w_n = pow_2k<8>(fmax_(0.0f, dot(mk(g.x, g.y, g.z), n)));
         }
-        const float w = expf(-(w_l * w_l) - w_z) * kw * w_n;
+        const float w = exp_hw(-(w_l * w_l) - w_z) * kw * w_n;
         if (!isfinite(w)) return;
         sum.x += c.x * w;
         sum.y += c.y * w;
