@@ -46,6 +46,10 @@ def lib():
         l.or_env_sample_texel.argtypes = [vp, vp, vp, vp]
         l.or_env_sample_texel_pdf.restype = C.c_float
         l.or_env_sample_texel_pdf.argtypes = [vp, vp]
+        l.or_env_alias_sample.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        l.or_env_alias_pdf.restype = C.c_float
+        l.or_env_alias_pdf.argtypes = [vp, vp]
+        l.or_env_alias_table.argtypes = [vp, vp, vp, vp]
         l.or_bsdf_eval.argtypes = [vp, C.c_int, vp, vp, vp]
         l.or_bsdf_sample.argtypes = [vp, C.c_int, vp, vp, vp]
         l.or_bsdf_eval_adjoint.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
@@ -103,6 +107,25 @@ class OracleScene:
             raise RuntimeError("oracle render: " + self.lib.or_last_error().decode())
         return dict(hdr=hdr, visibility=vis, path_length=pl, ray_counts=rc, traversal=tr, albedo=alb, depth=dep,
                     prev_uv=puv)
+
+    def env_alias_table(self, n):
+        """(prob, alias, pdf) of the n texels of the environment's level 0."""
+        prob, alias, pdf = np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros(n, np.float32)
+        if self.lib.or_env_alias_table(self.h, prob.ctypes.data, alias.ctypes.data, pdf.ctypes.data) != 0:
+            raise RuntimeError("scene has no environment image")
+        return prob, alias, pdf
+
+    def env_alias_sample(self, rnd):
+        """Alias-table samples for rnd (n, 4) = {column, accept, u jitter, v jitter}: (uv (n, 2), pdf (n,))."""
+        rnd = np.ascontiguousarray(rnd, np.float32).reshape(-1, 4)
+        uv, pdf = np.zeros((len(rnd), 2), np.float32), np.zeros(len(rnd), np.float32)
+        if self.lib.or_env_alias_sample(self.h, len(rnd), rnd.ctypes.data, uv.ctypes.data, pdf.ctypes.data) != 0:
+            raise RuntimeError("scene has no environment image")
+        return uv, pdf
+
+    def env_alias_pdf(self, uv):
+        uv = np.ascontiguousarray(uv, np.float32).reshape(2)
+        return float(self.lib.or_env_alias_pdf(self.h, uv.ctypes.data))
 
     def trace_rays(self, rays, closest=True):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

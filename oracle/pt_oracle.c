@@ -272,6 +272,11 @@ struct or_scene {
     int env_has_image;
     int nlevels;
     level_t lv[16];
+    /* alias table over luminance * sin(theta) of level 0 (ST_FLAG_ENV_ALIAS) */
+    float* alias_prob;
+    uint32_t* alias_idx;
+    float* alias_pdf;
+    float alias_jmax; /* largest in-texel jitter: 1 - 2^(ceil(log2(max(W, H))) - 21) */
     /* lights: emissive instances in instance order (Scene.cpp:487-491) */
     uint32_t nlights;
     uint32_t* light_inst;
@@ -882,6 +887,108 @@ static float sample_texel_pdf(const or_scene* s, float u, float v) {
     }
     return pdf * (float)((uint32_t)lw * (uint32_t)lh);
 }
+/* Alias table over the full-resolution map (the north-star environment
+ * sampler, ST_FLAG_ENV_ALIAS; the reference warps the mip pyramid instead).
+ * Walker's alias method with Vose's two-worklist construction (M. D. Vose,
+ * "A linear algorithm for generating random numbers with a given
+ * distribution", IEEE TSE 17(9), 1991): texel weights w_i = max(0, lum) *
+ * sin(pi (y + 1/2) / H) in double; column i keeps probability min(1, N w_i /
+ * sum w) of itself and gives the rest to one donor.  Worklists are stacks
+ * (last in, first out), so the table is one fixed function of the image.
+ * pdf[i] = N w_i / sum w is the texel's uv-space density. */
+static int build_alias(or_scene* s) {
+    const level_t* l = &s->lv[0];
+    const size_t N = (size_t)l->w * l->h;
+    double* wgt = (double*)malloc(sizeof(double) * N);
+    double* sc = (double*)malloc(sizeof(double) * N);
+    uint32_t* small = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    uint32_t* large = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    s->alias_prob = (float*)malloc(sizeof(float) * N);
+    s->alias_idx = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    s->alias_pdf = (float*)malloc(sizeof(float) * N);
+    if (!wgt || !sc || !small || !large || !s->alias_prob || !s->alias_idx || !s->alias_pdf) {
+        free(wgt); free(sc); free(small); free(large);
+        return 1;
+    }
+    double total = 0;
+    for (int y = 0; y < l->h; y++) {
+        const double st = sin(3.14159265358979323846 * (y + 0.5) / l->h);
+        for (int x = 0; x < l->w; x++) {
+            const float* t = texel(l, x, y);
+            const double lum = 0.2126 * t[0] + 0.7152 * t[1] + 0.0722 * t[2];
+            wgt[(size_t)y * l->w + x] = (lum > 0 ? lum : 0.0) * st;
+            total += wgt[(size_t)y * l->w + x];
+        }
+    }
+    if (total <= 0) {
+        for (size_t i = 0; i < N; i++) wgt[i] = 1;
+        total = (double)N;
+    }
+    size_t ns = 0, nl = 0;
+    for (size_t i = 0; i < N; i++) {
+        sc[i] = wgt[i] / total * N;
+        s->alias_pdf[i] = (float)(wgt[i] / total * N);
+        if (sc[i] < 1) small[ns++] = (uint32_t)i;
+        else large[nl++] = (uint32_t)i;
+    }
+    while (ns > 0 && nl > 0) {
+        const uint32_t a = small[--ns], g = large[nl - 1];
+        s->alias_prob[a] = (float)sc[a];
+        s->alias_idx[a] = g;
+        sc[g] = (sc[g] + sc[a]) - 1;
+        if (sc[g] < 1) {
+            nl--;
+            small[ns++] = g;
+        }
+    }
+    /* leftovers hold probability 1 up to rounding */
+    for (size_t j = 0; j < nl; j++) { s->alias_prob[large[j]] = 1; s->alias_idx[large[j]] = large[j]; }
+    for (size_t j = 0; j < ns; j++) { s->alias_prob[small[j]] = 1; s->alias_idx[small[j]] = small[j]; }
+    free(wgt); free(sc); free(small); free(large);
+    /* a jitter this far below 1 keeps x + j and (int)((x + j) / W * W) inside
+     * texel x for every x < 2^lg, so the pdf a sample reports is the pdf
+     * evaluated at its point */
+    int lg = 0;
+    while ((1 << lg) < (l->w > l->h ? l->w : l->h)) lg++;
+    s->alias_jmax = 1.0f - ldexpf(1.0f, lg - 21);
+    return 0;
+}
+/* One texel by the alias table, a uniform point inside it.  rx picks the
+ * column, ra is the accept test, ju / jv place the point in the texel; *pdf
+ * is the uv-space density of the texel drawn. */
+static void sample_alias(const or_scene* s, float rx, float ra, float ju, float jv, float uv[2], float* pdf) {
+    const int W = s->lv[0].w, H = s->lv[0].h;
+    const uint32_t N = (uint32_t)W * (uint32_t)H;
+    uint32_t i = (uint32_t)(rx * (float)N);
+    if (i >= N) i = N - 1;
+    if (!(ra < s->alias_prob[i])) i = s->alias_idx[i];
+    *pdf = s->alias_pdf[i];
+    uv[0] = ((float)(i % (uint32_t)W) + minf(ju, s->alias_jmax)) / (float)W;
+    uv[1] = ((float)(i / (uint32_t)W) + minf(jv, s->alias_jmax)) / (float)H;
+}
+/* uv-space density of the alias sampler at uv */
+static float alias_pdf(const or_scene* s, float u, float v) {
+    const int W = s->lv[0].w, H = s->lv[0].h;
+    int x = (int)(u * (float)W), y = (int)(v * (float)H);
+    x = x < 0 ? 0 : (x >= W ? W - 1 : x);
+    y = y < 0 ? 0 : (y >= H ? H - 1 : y);
+    return s->alias_pdf[(size_t)y * W + x];
+}
+int or_env_alias_sample(or_scene* s, uint32_t n, const float* rnd, float* uv, float* pdf) {
+    if (!s->env_has_image) return 1;
+    for (uint32_t i = 0; i < n; i++)
+        sample_alias(s, rnd[4 * i], rnd[4 * i + 1], rnd[4 * i + 2], rnd[4 * i + 3], uv + 2 * i, pdf + i);
+    return 0;
+}
+float or_env_alias_pdf(or_scene* s, const float uv[2]) { return s->env_has_image ? alias_pdf(s, uv[0], uv[1]) : 0; }
+int or_env_alias_table(or_scene* s, float* prob, uint32_t* alias, float* pdf) {
+    if (!s->env_has_image) return 1;
+    const size_t N = (size_t)s->lv[0].w * s->lv[0].h;
+    if (prob) memcpy(prob, s->alias_prob, sizeof(float) * N);
+    if (alias) memcpy(alias, s->alias_idx, sizeof(uint32_t) * N);
+    if (pdf) memcpy(pdf, s->alias_pdf, sizeof(float) * N);
+    return 0;
+}
 int or_env_sample_texel(or_scene* s, const float rnd[2], float uv[2], float* pdf) {
     if (!s->env_has_image) return 1;
     sample_texel(s, rnd[0], rnd[1], uv, pdf);
@@ -895,8 +1002,14 @@ typedef struct {
     uint32_t flags;
     float env_prob;
     uint32_t max_depth, min_len, cand, max_null, seed_px, seed_py, seed;
+    uint32_t rr_start; /* Russian roulette from this path length on (0, 1: every vertex) */
     int lambert;
 } cfg_t;
+
+/* The continuation probability folded into the BSDF / phase pdfs at a vertex
+ * of path length plen: Russian roulette runs at every vertex in the reference
+ * (testrenderer.slang:890-892); the north-star mode starts it at rr_start. */
+static float rr_cp(const cfg_t* c, uint32_t plen, float cp) { return plen >= c->rr_start ? cp : 1.0f; }
 
 /* EvaluateEnvironment, lights.hlsli:143-163 */
 static f3 evaluate_environment(const or_scene* s, const cfg_t* c, f3 d, float* pdfw) {
@@ -906,7 +1019,8 @@ static f3 evaluate_environment(const or_scene* s, const cfg_t* c, f3 d, float* p
         float uv[2];
         cart_to_uv(d, uv);
         e = mul(e, env_bilinear(&s->lv[0], uv[0], uv[1]));
-        *pdfw = sample_texel_pdf(s, uv[0], uv[1]) / (2 * OR_PI * OR_PI * sqrtf(1 - d.y * d.y));
+        const float pu = (c->flags & ST_FLAG_ENV_ALIAS) ? alias_pdf(s, uv[0], uv[1]) : sample_texel_pdf(s, uv[0], uv[1]);
+        *pdfw = pu / (2 * OR_PI * OR_PI * sqrtf(1 - d.y * d.y));
     } else {
         *pdfw = 1 / (4 * OR_PI);
     }
@@ -932,9 +1046,17 @@ static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const float 
         if (s->nlights == 0 || rnd[3] < c->env_prob) {
             f3 d;
             float pdf;
+            const int alias = (c->flags & ST_FLAG_ENV_ALIAS) && s->env_has_image;
+            float uv[2] = {0, 0};
             if (s->env_has_image) {
-                float uv[2];
-                sample_texel(s, rnd[0], rnd[1], uv, &pdf);
+                if (alias) {
+                    /* rnd[2] is free on this branch; rnd[3], rescaled past the
+                     * environment-or-emitter choice, is uniform again */
+                    const float ju = s->nlights > 0 ? rnd[3] / c->env_prob : rnd[3];
+                    sample_alias(s, rnd[0], rnd[2], ju, rnd[1], uv, &pdf);
+                } else {
+                    sample_texel(s, rnd[0], rnd[1], uv, &pdf);
+                }
                 d = uv_to_cart(uv[0], uv[1]);
                 pdf /= (2 * OR_PI * OR_PI * sqrtf(1 - pow2(d.y)));
             } else {
@@ -948,8 +1070,14 @@ static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const float 
             r.cos_light = 1;
             r.finite = 0;
             r.dir = d;
-            float tmp;
-            r.radiance = evaluate_environment(s, c, d, &tmp);
+            if (alias) {
+                /* alias mode: the radiance at the sampled uv itself, without the
+                 * direction -> uv round trip */
+                r.radiance = mul(s->env_emission, env_bilinear(&s->lv[0], uv[0], uv[1]));
+            } else {
+                float tmp;
+                r.radiance = evaluate_environment(s, c, d, &tmp);
+            }
             return r;
         }
         if (s->nlights > 0) rnd[3] = (rnd[3] - c->env_prob) / (1 - c->env_prob);
@@ -1386,6 +1514,7 @@ or_scene* or_scene_create(const st_scene_desc* d) {
             s->lv[0].rgba = (float*)malloc(sizeof(float) * 4 * (size_t)im->width * im->height);
             memcpy(s->lv[0].rgba, im->rgba, sizeof(float) * 4 * (size_t)im->width * im->height);
             build_mips(s);
+            if (build_alias(s)) { or_scene_destroy(s); fail("out of memory"); return NULL; }
         }
     }
     /* media: volume instances, their grids and the inverse transforms (TransformData::inverse) */
@@ -1487,6 +1616,7 @@ void or_scene_destroy(or_scene* s) {
     for (uint32_t i = 0; i < s->nvb; i++) free(s->vb[i]);
     free(s->vb); free(s->vbsz);
     for (int l = 0; l < 16; l++) free(s->lv[l].rgba);
+    free(s->alias_prob); free(s->alias_idx); free(s->alias_pdf);
     free(s->light_inst); free(s->tv); free(s->tid); free(s->nodes); free(s->order);
     free(s->inv_xf); free(s->volinfo);
     for (uint32_t v = 0; v < s->nvols; v++) free(s->vol[v]);
@@ -1593,6 +1723,14 @@ typedef struct {
     uint32_t lvc_count, L;
     float ipd;
     const uint32_t* light_acc;  /* this pixel's light-pass connections (3 x u32) or NULL */
+    /* screen-space ReSTIR DI reuse: the primary vertex keeps its RIS reservoir
+     * (seed, target, W, M, hit distance) and shading record instead of making
+     * a shadow ray */
+    int reuse;
+    int prim_hit;               /* the primary ray hit a surface */
+    shading_t prim_sd;
+    uint32_t prim_dir;          /* packed primary ray direction */
+    float res[8];               /* seed[4], target, W, M, depth */
 } pctx_t;
 
 /* ViewData::toRaster (compat/scene.h:158-165) with ProjectionData::projectPoint
@@ -1672,7 +1810,7 @@ static void connect_to_light(pctx_t* x, const material_t* m, const shading_t* sd
     f3 ldout = to_local(sd, r.dir);
     rer_t f = bsdf_eval(m, x->c->lambert, ldin, ldout);
     f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
-    float cp = m_cont_prob(m);
+    float cp = rr_cp(x->c, x->ps.path_len, m_cont_prob(m));
     f.fwd *= cp; f.rev *= cp;
     f3 contrib = divf(mul(mul(x->ps.thr, r.radiance), f.f), dpw);
     float w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, f.fwd / AtoW); /* ComputeMIS_DI :530-566 */
@@ -1790,7 +1928,7 @@ static void rsd_init(rsd_t* r, pctx_t* x, const material_t* m, const shading_t* 
         f3 ldout = to_local(sd, L.dir);
         rer_t f = bsdf_eval(m, x->c->lambert, ldin, ldout);
         f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
-        float cp = m_cont_prob(m);
+        float cp = rr_cp(x->c, x->ps.path_len, m_cont_prob(m));
         f.fwd *= cp; f.rev *= cp;
         r->contrib = mul(f.f, L.radiance);
         r->wtoa = L.finite ? L.cos_light / pow2(L.dist) : 1;
@@ -1832,6 +1970,32 @@ static void connect_to_light_restir(pctx_t* x, const material_t* m, const shadin
     sr->w = sample.mis_w;
     sr->medium = x->ps.medium;
     sr->plen = x->ps.path_len;
+}
+
+/* RIS at a primary vertex of the screen-space reuse mode: the candidates of
+ * SampleIlluminationReSTIR (testrenderer.slang:665-684), in the same draws,
+ * with the winning seed kept for reuse instead of a shadow ray. */
+static void ris_reservoir(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
+    rsd_t xi;
+    float wsum = 0, target = 0;
+    for (int j = 0; j < 4; j++) x->res[j] = 0;
+    for (uint32_t i = 0; i < x->c->cand; i++) {
+        float rnd[4];
+        rng_nextf(&x->ps.rng, rnd);
+        rsd_init(&xi, x, m, sd, ldin, rnd);
+        const float t = rsd_target(&xi);
+        if (t <= 0) continue;
+        wsum += t / xi.src_pdf;
+        float u[4];
+        rng_nextf(&x->ps.rng, u);
+        if (u[0] <= (t / xi.src_pdf) / wsum) {
+            memcpy(x->res, rnd, sizeof rnd);
+            target = t;
+        }
+    }
+    x->res[4] = target;
+    x->res[5] = target > 0 ? wsum / ((float)x->c->cand * target) : 0.0f;
+    x->res[6] = target > 0 ? (float)x->c->cand : 0.0f;
 }
 
 #define OR_MAX_MEDIA_SEGMENTS 256
@@ -2005,7 +2169,7 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
             if (r.finite) AtoW = pow2(r.dist) / r.cos_light;
             const float dpw = r.pdf * AtoW;
             const float v = hg_eval(m.g, ldin, r.dir);
-            const float cp = med_cont_prob(&m);
+            const float cp = rr_cp(x->c, ps->path_len, med_cont_prob(&m));
             const float fwd = v * cp;
             const f3 contrib = divf(mulf(mul(ps->thr, r.radiance), v), dpw);
             const float w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, fwd / AtoW);
@@ -2023,14 +2187,15 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
     }
     if (x->c->flags & ST_FLAG_DIRECT_ONLY) return 0;
     const float cp = minf(1, med_cont_prob(&m));
+    const int rr_on = ps->path_len >= x->c->rr_start;
     float rr[4];
     rng_nextf(&ps->rng, rr);
-    if (rr[0] > cp) return 0;
+    if (rr_on && rr[0] > cp) return 0;
     float rb[4];
     rng_nextf(&ps->rng, rb);
     const f3 sdir = hg_sample(m.g, rb[0], rb[1], ldin);
     const float v = hg_eval(m.g, ldin, sdir);
-    const float fwd = v * cp;
+    const float fwd = v * (rr_on ? cp : 1.0f);
     ps->thr = mulf(ps->thr, v / fwd);
     ps->dirpdf16 = (1 - fabsf(m.g)) == 0 ? 0 : q16(fwd);
     if (vc) {
@@ -2128,6 +2293,12 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     if (ps->path_len == 1) {
         x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn;
         store_aux(x, 1, sd.pos, m_base(&m));
+        if (x->reuse) {
+            x->prim_hit = 1;
+            x->prim_sd = sd;
+            x->prim_dir = ps->dir;
+            x->res[7] = h.t; /* the raw hit distance */
+        }
     }
     if (ldin.z > 0) {
         f3 e = mul(ps->thr, m_emission(&m));
@@ -2142,14 +2313,21 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
         connect_to_light(x, &m, &sd, ldin);
         connect_to_light_subpath(x, &m, &sd, NULL, sd.pos, ldin);
     } else if (!m_singular(&m, x->c->lambert) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
-        if (x->c->flags & ST_FLAG_RESTIR_DI) connect_to_light_restir(x, &m, &sd, ldin);
+        /* with screen-space reuse, RIS runs at the primary vertex only (its
+         * shadow ray comes after the reuse passes); deeper vertices take plain NEE */
+        if (x->reuse && ps->path_len == 1) ris_reservoir(x, &m, &sd, ldin);
+        else if ((x->c->flags & ST_FLAG_RESTIR_DI) && !x->reuse) connect_to_light_restir(x, &m, &sd, ldin);
         else connect_to_light(x, &m, &sd, ldin);
     }
     if (x->c->flags & ST_FLAG_DIRECT_ONLY) return 0; /* gSampleDirectIlluminationOnly, testrenderer.slang:886-888 */
+    /* Russian roulette (testrenderer.slang:890-892); the draw is made at every
+     * vertex, the test only from rr_start on */
     float cp = minf(1, m_cont_prob(&m));
+    const int rr_on = ps->path_len >= x->c->rr_start;
     float rr[4];
     rng_nextf(&ps->rng, rr);
-    if (rr[0] > cp) return 0;
+    if (rr_on && rr[0] > cp) return 0;
+    if (!rr_on) cp = 1;
     float rb[4];
     rng_nextf(&ps->rng, rb);
     dsr_t smp = bsdf_sample(&m, x->c->lambert, v3(rb[0], rb[1], rb[2]), ldin);
@@ -2431,13 +2609,14 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
             }
             if (!med_singular(&m) && ps->path_len + 1 >= c->min_len) connect_to_camera(lx, NULL, NULL, &m, mpos, ldin, q);
             const float cp = minf(1, med_cont_prob(&m));
+            const int rr_on = ps->path_len >= c->rr_start;
             float rr[4], rb[4];
             rng_nextf(&ps->rng, rr);
-            if (rr[0] > cp) return;
+            if (rr_on && rr[0] > cp) return;
             rng_nextf(&ps->rng, rb);
             const f3 sdir = hg_sample(m.g, rb[0], rb[1], ldin);
             const float v = hg_eval(m.g, ldin, sdir);
-            const float fwd = v * cp;
+            const float fwd = v * (rr_on ? cp : 1.0f);
             ps->thr = mulf(ps->thr, v / fwd);
             if (vc) {
                 dsr_t smp;
@@ -2472,10 +2651,12 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
                 v->dvcm = ps->dvcm;
             }
             if (!m_singular(&m, c->lambert) && ps->path_len + 1 >= c->min_len) connect_to_camera(lx, &m, &sd, NULL, sd.pos, ldin, q);
-            const float cp = minf(1, m_cont_prob(&m));
+            const float cp0 = minf(1, m_cont_prob(&m));
+            const int rr_on = ps->path_len >= c->rr_start;
             float rr[4], rb[4];
             rng_nextf(&ps->rng, rr);
-            if (rr[0] > cp) return;
+            if (rr_on && rr[0] > cp0) return;
+            const float cp = rr_on ? cp0 : 1.0f;
             rng_nextf(&ps->rng, rb);
             dsr_t smp = bsdf_sample_x(&m, c->lambert, v3(rb[0], rb[1], rb[2]), ldin, 1);
             smp.fwd *= cp; smp.rev *= cp;
@@ -2502,6 +2683,7 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
     c.min_len = p->min_path_length;
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
     c.max_null = p->max_null_collisions;
+    c.rr_start = p->rr_start_depth;
     const uint32_t W = p->width, H = p->height, npix = W * H;
     /* mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452 */
     const float Lf = (float)npix * p->light_subpath_factor; /* bounded before the conversion */
@@ -2597,6 +2779,7 @@ static int render_vc(or_scene* s, const st_view* view, const st_render_params* p
     if (s->nvolinst > 0 || (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_FLOAT_NEE | ST_FLAG_DIRECT_ONLY)) ||
         p->min_path_length)
         return fail("the bidirectional mode supports surfaces with plain NEE only");
+    if (p->rr_start_depth > 1) return fail("the bidirectional mode runs Russian roulette at every vertex");
     if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64)) return fail("light_subpath_factor out of range");
     if (p->max_depth < 1) return fail("max_depth must be at least 1");
     cfg_t c;
@@ -2606,6 +2789,7 @@ static int render_vc(or_scene* s, const st_view* view, const st_render_params* p
     c.max_depth = p->max_depth;
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
     c.max_null = p->max_null_collisions;
+    c.rr_start = p->rr_start_depth;
     const uint32_t W = p->width, H = p->height, npix = W * H;
     const float Lf = (float)npix * p->light_subpath_factor; /* bounded before the conversion */
     if (!(Lf < 2147483648.0f)) return fail("too many light subpaths");
@@ -2706,15 +2890,233 @@ static int render_vc(or_scene* s, const st_view* view, const st_render_params* p
     return oom ? fail("out of memory") : 0;
 }
 
+/* ------------------------------------------- screen-space ReSTIR DI reuse */
+/* The north-star reuse mode (ST_FLAG_RESTIR_DI_REUSE; the reference reuses
+ * one entry of a world-space hash grid instead, testrenderer.slang:697-709).
+ * Per sample, in sample order:
+ *  1. every pixel's path, with RIS at the primary vertex keeping its
+ *     reservoir (ris_reservoir) and plain NEE deeper;
+ *  2. temporal reuse: the reservoir against the same pixel's final reservoir
+ *     of the previous sample (ReservoirReuse, :608-663: pairwise MIS with
+ *     mis(a, b) = a^2 / (a^2 + b^2), M clamped at maxM * candidates);
+ *  3. spatial reuse over `spatial_neighbours` pixels drawn in a disc of
+ *     `spatial_radius` pixels (depth within 10%, shading normals within
+ *     cos = 0.9), then the shadow ray of the final sample (ConnectToLight
+ *     with ReSTIR, :730-747) added to the pixel's fixed-point sum. */
+typedef struct {
+    int hit;
+    shading_t sd;
+    material_t m;
+    f3 ldin;
+} prim_t;
+static void prim_make(const or_scene* s, const shading_t* sd, uint32_t dir, prim_t* P) {
+    P->hit = 1;
+    P->sd = *sd;
+    P->m = load_material(s, sd->flags_material >> 4);
+    P->ldin = to_local(sd, neg(unpack_normal(dir)));
+}
+/* target function of a light-sample seed at a primary vertex (0 when unusable) */
+static float seed_target(pctx_t* x, const prim_t* P, const float seed[4], rsd_t* r) {
+    x->ps.path_len = 1;
+    rsd_init(r, x, &P->m, &P->sd, P->ldin, seed);
+    return r->src_pdf > 0 ? rsd_target(r) : 0.0f;
+}
+static float mis_pair(float a, float b) { /* mis(a, b), testrenderer.slang:287-290 */
+    const float a2 = a * a, d = a2 + b * b;
+    return d > 0 ? a2 / d : 0.0f;
+}
+
+static int render_reuse(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads,
+                        const cfg_t* c) {
+    const uint32_t W = p->width, H = p->height;
+    const size_t npix = (size_t)W * H;
+    prim_t* prim = (prim_t*)calloc(npix, sizeof(prim_t));
+    float* rinit = (float*)calloc(npix * 8, sizeof(float));
+    float* rtemp = (float*)calloc(npix * 8, sizeof(float));
+    float* rprev = (float*)calloc(npix * 8, sizeof(float));
+    f3* em = (f3*)calloc(npix, sizeof(f3));
+    uint32_t* nee = (uint32_t*)calloc(npix * 3, sizeof(uint32_t));
+    f3* sum = (f3*)calloc(npix, sizeof(f3));
+    if (!prim || !rinit || !rtemp || !rprev || !em || !nee || !sum) {
+        free(prim); free(rinit); free(rtemp); free(rprev); free(em); free(nee); free(sum);
+        return fail("out of memory");
+    }
+    const uint32_t view_medium = camera_medium(s, view);
+    const st_transform inv_cam = inverse_xf(&view->camera_to_world);
+    uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
+    const float mmax = p->max_m * (float)c->cand;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    for (uint32_t k = 0; k < p->spp; k++) {
+        const uint32_t seed = p->seed + k;
+#pragma omp parallel
+        {
+            pctx_t x;
+            memset(&x, 0, sizeof x);
+            x.s = s;
+            x.c = c;
+            x.view_medium = view_medium;
+            x.view = view;
+            x.inv_cam = inv_cam;
+            x.W = W;
+            x.H = H;
+            x.reuse = 1;
+            /* 1. paths, primary reservoirs */
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+                const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
+                x.px = px;
+                x.py = py;
+                x.prim_hit = 0;
+                memset(x.res, 0, sizeof x.res);
+                f3 r;
+                uint32_t pl;
+                render_pixel(&x, p, view, px, py, seed, &r, &pl);
+                if (k == 0 && out->visibility) { out->visibility[2 * pix] = x.vis[0]; out->visibility[2 * pix + 1] = x.vis[1]; }
+                if (k == 0 && out->albedo) {
+                    for (int j = 0; j < 3; j++) out->albedo[4 * pix + j] = x.alb[j];
+                    out->albedo[4 * pix + 3] = 1;
+                }
+                if (k == 0 && out->depth) {
+                    out->depth[4 * pix] = x.dep[0]; out->depth[4 * pix + 1] = x.dep[1];
+                    out->depth[4 * pix + 2] = 1; out->depth[4 * pix + 3] = 1;
+                }
+                if (k == 0 && out->prev_uv) { out->prev_uv[2 * pix] = x.uv[0]; out->prev_uv[2 * pix + 1] = x.uv[1]; }
+                if (out->path_length) out->path_length[(size_t)k * npix + pix] = pl;
+                em[pix] = x.out;
+                for (int j = 0; j < 3; j++) nee[3 * pix + j] = x.nee[j];
+                prim[pix].hit = 0;
+                if (x.prim_hit) prim_make(s, &x.prim_sd, x.prim_dir, &prim[pix]);
+                memcpy(&rinit[8 * pix], x.res, sizeof x.res);
+                if (!x.prim_hit) memset(&rinit[8 * pix], 0, sizeof x.res); /* a miss keeps no reservoir */
+            }
+            /* 2. temporal reuse against the previous sample's final reservoir */
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+                const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
+                float* b = &rtemp[8 * pix];
+                memcpy(b, &rinit[8 * pix], 8 * sizeof(float));
+                const float* q = &rprev[8 * pix];
+                if (b[6] > 0 && k > 0 && q[6] > 0) {
+                    rsd_t xr;
+                    const float tq = seed_target(&x, &prim[pix], q, &xr);
+                    rng_t rng;
+                    rng_init(&rng, seed, px, py, 1u << 20);
+                    float u[4];
+                    rng_nextf(&rng, u);
+                    const float m0 = mis_pair(b[6] * b[4], q[6] * b[4]), m1 = mis_pair(q[6] * q[4], b[6] * tq);
+                    const float w0 = m0 * b[4] * b[5], w1 = m1 * tq * q[5];
+                    const float total = w0 + w1;
+                    const float M = b[6] + q[6];
+                    float target = b[4];
+                    if (u[0] * total <= w1 && tq > 0) {
+                        memcpy(b, q, 4 * sizeof(float));
+                        target = tq;
+                    }
+                    b[4] = target;
+                    b[5] = target > 0 ? total / target : 0.0f;
+                    b[6] = minf(M, mmax);
+                }
+            }
+            /* 3. spatial reuse, then the final sample's shadow ray */
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+                const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
+                float b[8];
+                memcpy(b, &rtemp[8 * pix], sizeof b);
+                if (!(b[6] > 0)) {
+                    memset(&rprev[8 * pix], 0, 8 * sizeof(float));
+                    memcpy(&rprev[8 * pix], b, 4 * sizeof(float));
+                    continue;
+                }
+                const prim_t* P = &prim[pix];
+                const f3 nc = unpack_normal(P->sd.sn);
+                rsd_t cur;
+                seed_target(&x, P, b, &cur);
+                rng_t rng;
+                rng_init(&rng, seed, px, py, (1u << 20) + 64);
+                for (uint32_t n = 0; n < p->spatial_neighbours; n++) {
+                    float u[4], d[2];
+                    rng_nextf(&rng, u);
+                    sample_concentric_disc(u[0], u[1], d);
+                    const int qx = (int)px + (int)floorf(d[0] * p->spatial_radius + 0.5f);
+                    const int qy = (int)py + (int)floorf(d[1] * p->spatial_radius + 0.5f);
+                    if (qx < 0 || qy < 0 || qx >= (int)W || qy >= (int)H || (qx == (int)px && qy == (int)py)) continue;
+                    const size_t qp = (size_t)qy * W + (size_t)qx;
+                    const float* q = &rtemp[8 * qp];
+                    if (!(q[6] > 0)) continue;
+                    if (fabsf(q[7] - b[7]) > 0.1f * b[7]) continue;
+                    const prim_t* Q = &prim[qp];
+                    if (dot(unpack_normal(Q->sd.sn), nc) < 0.9f) continue;
+                    rsd_t xn, xq;
+                    const float tn = seed_target(&x, P, q, &xn);   /* the neighbour's seed here */
+                    const float tcq = seed_target(&x, Q, b, &xq);  /* our seed there */
+                    const float m0 = mis_pair(b[6] * b[4], q[6] * tcq), m1 = mis_pair(q[6] * q[4], b[6] * tn);
+                    const float w0 = m0 * b[4] * b[5], w1 = m1 * tn * q[5];
+                    const float total = w0 + w1;
+                    if (u[2] * total <= w1 && tn > 0) {
+                        memcpy(b, q, 4 * sizeof(float));
+                        b[4] = tn;
+                        cur = xn;
+                    }
+                    b[5] = b[4] > 0 ? total / b[4] : 0.0f;
+                    b[6] = minf(b[6] + q[6], mmax);
+                }
+                memcpy(&rprev[8 * pix], b, sizeof b);
+                if (b[5] > 0 && b[4] > 0 && 2 >= c->min_len) {
+                    const float Wp = b[5] * cur.wtoa; /* 1/pdfA -> 1/pdfW */
+                    const f3 contrib = mulf(cur.contrib, Wp); /* primary throughput is 1 */
+                    hit_t h = trace(s, cur.vo, cur.vd, cur.vtmax, 1, &x.trav[2], &x.trav[3]);
+                    x.rays[1]++;
+                    if (!h.hit && cur.mis_w > 0 && !all_le0(contrib)) {
+                        const f3 rw = mulf(contrib, cur.mis_w);
+                        const float qv[3] = {clamp0(rw.x) * (float)OR_Q, clamp0(rw.y) * (float)OR_Q, clamp0(rw.z) * (float)OR_Q};
+                        for (int j = 0; j < 3; j++) nee[3 * pix + j] += (uint32_t)(qv[j] >= 2147483647.0f ? INT_MAX : (int32_t)qv[j]);
+                    }
+                }
+            }
+            /* resolve: the running sum over samples (ProcessAtomicOutput) */
+#pragma omp for schedule(static)
+            for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+                sum[pix].x += em[pix].x + (float)nee[3 * pix] / (float)OR_Q;
+                sum[pix].y += em[pix].y + (float)nee[3 * pix + 1] / (float)OR_Q;
+                sum[pix].z += em[pix].z + (float)nee[3 * pix + 2] / (float)OR_Q;
+            }
+#pragma omp critical
+            {
+                rays[0] += x.rays[0]; rays[1] += x.rays[1];
+                for (int j = 0; j < 4; j++) trav[j] += x.trav[j];
+            }
+        }
+    }
+    if (out->hdr)
+        for (size_t i = 0; i < npix; i++) {
+            out->hdr[4 * i + 0] = sum[i].x / (float)p->spp;
+            out->hdr[4 * i + 1] = sum[i].y / (float)p->spp;
+            out->hdr[4 * i + 2] = sum[i].z / (float)p->spp;
+            out->hdr[4 * i + 3] = 1;
+        }
+    if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
+    if (out->traversal) for (int j = 0; j < 4; j++) out->traversal[j] = trav[j];
+    free(prim); free(rinit); free(rtemp); free(rprev); free(em); free(nee); free(sum);
+    return 0;
+}
+
 int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
     if (!s || !view || !p || !out) return fail("null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
-    if (p->flags & ST_FLAG_ENV_ALIAS) return fail("mode not supported by this oracle");
     if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, out, threads);
     if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, out, threads);
-    if (p->flags & ST_FLAG_RESTIR_DI_REUSE) return fail("mode not supported by this oracle");
     if (s->nvolinst > 0 && (p->flags & ST_FLAG_RESTIR_DI)) return fail("ReSTIR DI with media is not supported");
+    const int reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
+    if (reuse && !(p->flags & ST_FLAG_RESTIR_DI)) return fail("ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");
+    if (reuse && (p->flags & (ST_FLAG_FLOAT_NEE | ST_FLAG_PIXEL_JITTER)))
+        return fail("screen-space reuse with fp32 NEE or pixel jitter is not restated");
     cfg_t c;
+    memset(&c, 0, sizeof c);
     c.flags = p->flags;
     c.env_prob = p->environment_sample_probability;
     c.max_depth = p->max_depth;
@@ -2722,6 +3124,8 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
     c.cand = p->candidate_samples;
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
     c.max_null = p->max_null_collisions;
+    c.rr_start = p->rr_start_depth;
+    if (reuse) return render_reuse(s, view, p, out, threads, &c);
     const uint32_t view_medium = camera_medium(s, view);
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};

@@ -61,6 +61,13 @@ void or_ray_offset(const float p[3], const float n[3], float out[3]);
 /* Environment sampling restated (lights.hlsli:49-163): returns uv and pdf. */
 int or_env_sample_texel(or_scene* s, const float rnd[2], float uv[2], float* pdf);
 float or_env_sample_texel_pdf(or_scene* s, const float uv[2]);
+/* Alias-table sampling of the environment (ST_FLAG_ENV_ALIAS), n samples:
+ * rnd[4i..4i+3] = {column, accept test, u jitter, v jitter} -> uv[2i..], pdf[i];
+ * the table itself (prob, alias, uv pdf per texel of level 0; any pointer may
+ * be null). */
+int or_env_alias_sample(or_scene* s, uint32_t n, const float* rnd, float* uv, float* pdf);
+float or_env_alias_pdf(or_scene* s, const float uv[2]);
+int or_env_alias_table(or_scene* s, float* prob, uint32_t* alias, float* pdf);
 
 /* BSDF restated: evaluateReflectance / sampleDirection (disney.hlsli:391-429,
  * lambertian.hlsli:13-39). mat = 4 packed words + emission scale bits.

@@ -21,7 +21,6 @@
 
 namespace st {
 
-constexpr int kNodeBytes = 64;
 constexpr int kTriBytes = 48;
 constexpr int kMaxLeafTris = 4;
 constexpr int kMaxDepth = 64;      // builder guarantee (median-split fallback beyond it)
@@ -60,6 +59,7 @@ struct DevScene {
     int env_w[kEnvMaxLevels], env_h[kEnvMaxLevels];
     const float2* env_alias;       // per level-0 texel: {prob, alias index bits}
     const float* env_pdf;          // per level-0 texel: uv-space pdf
+    float env_jmax;                // largest in-texel jitter of the alias sampler (see st_scene_create)
     float sphere[4];               // scene bounding sphere, TestRenderer.cpp:458-461
     // media: volume instances (procedural boxes) and their NanoVDB float grids
     const st_transform* inv_xf;    // per instance, TransformData::inverse

@@ -1809,6 +1809,7 @@ struct st_context {
     hipStream_t stream = nullptr;
     WaveBufs slot[kWaveSlots];
     hipEvent_t start_done = nullptr;     // the frame's clears on the context stream
+    hipEvent_t entry = nullptr;          // the caller's null-stream work at a call's entry
     DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
     double timings[12] = {0};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
@@ -1894,6 +1895,15 @@ struct Timed {
         ctx->ev_used[kind]++;
     }
 };
+
+// The context's streams do not synchronise with the null stream, so every
+// call that reads caller memory first makes the context stream wait for the
+// work the caller has queued on the device's null stream (where torch's
+// default stream puts its kernels and copies).
+void order_after_caller(st_context* ctx) {
+    HIP_CHECK(hipEventRecord(ctx->entry, nullptr));
+    HIP_CHECK(hipStreamWaitEvent(ctx->stream, ctx->entry, 0));
+}
 
 void validate_desc(const st_scene_desc* d) {
     if (!d) throw std::invalid_argument("null scene description");
@@ -2121,6 +2131,7 @@ int st_context_create(int device, st_context** out) {
             HIP_CHECK(hipEventCreateWithFlags(&b.resolve_done, hipEventDisableTiming));
         }
         HIP_CHECK(hipEventCreateWithFlags(&ctx->start_done, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&ctx->entry, hipEventDisableTiming));
         *out = ctx.release();
         return ST_OK;
     } catch (const std::exception& e) {
@@ -2144,6 +2155,7 @@ int st_context_destroy(st_context* ctx) {
         if (b.resolve_done) (void)hipEventDestroy(b.resolve_done);
     }
     if (ctx->start_done) (void)hipEventDestroy(ctx->start_done);
+    if (ctx->entry) (void)hipEventDestroy(ctx->entry);
     for (int sl = 1; sl < kWaveSlots; sl++)
         if (ctx->slot[sl].stream) (void)hipStreamDestroy(ctx->slot[sl].stream);
     (void)hipStreamDestroy(ctx->stream);
@@ -2406,6 +2418,12 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
                     alias[i] = make_float2(1.0f, 0);
                     std::memcpy(&alias[i].y, &i, 4);
                 }
+                // the alias sampler's in-texel jitter stops short of 1 by a margin
+                // that keeps x + j / W and its evaluation (int)(u * W) inside texel
+                // x at this map size: 1 - 2^(ceil(log2(max(W, H))) - 21)
+                int lg = 0;
+                while ((1 << lg) < std::max(W, H)) lg++;
+                D.env_jmax = 1.0f - std::ldexp(1.0f, lg - 21);
                 D.env_alias = s->upload<float2>(alias.data(), N * sizeof(float2));
                 D.env_pdf = s->upload<float>(pdf.data(), N * sizeof(float));
             }
@@ -2609,6 +2627,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
     st_context* ctx = s->ctx;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
+        order_after_caller(ctx);
         const uint32_t npix = p->width * p->height;
         const bool reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
         uint32_t k = p->samples_per_wave;
@@ -2756,6 +2775,7 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
     st_context* ctx = s->ctx;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
+        order_after_caller(ctx);
         const uint32_t npix = p->width * p->height;
         // mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452;
         // the product is bounded first, so the conversion stays defined
@@ -2870,12 +2890,16 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
     if (s->d.vinst_count > 0 || (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_FLOAT_NEE | ST_FLAG_DIRECT_ONLY)) ||
         p->min_path_length)
         return set_error(ST_ERROR_UNSUPPORTED, "the bidirectional mode supports surfaces with plain NEE only");
+    // its MIS weights fold the continuation probability in at every vertex
+    if (p->rr_start_depth > 1)
+        return set_error(ST_ERROR_UNSUPPORTED, "the bidirectional mode runs Russian roulette at every vertex");
     if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "light_subpath_factor must be in [0, 64]");
     if (p->max_depth < 1 || p->max_depth > 256) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_depth must be in [1, 256]");
     st_context* ctx = s->ctx;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
+        order_after_caller(ctx);
         const uint32_t npix = p->width * p->height;
         const float Lf = (float)npix * p->light_subpath_factor;  // bounded before the conversion
         if (!(Lf < 2147483648.0f)) return set_error(ST_ERROR_LIMIT, "too many light subpaths");
@@ -3019,6 +3043,7 @@ int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, u
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
         if (count == 0) return ST_OK;
+        order_after_caller(ctx);
         const int grid = std::min<int>((count + kTraceBlock - 1) / kTraceBlock, ctx->cus * kTraceBlocksPerCU);
         WaveBufs& B = ctx->slot[0];
         ensure_spill(B, ctx->cus * kTraceBlocksPerCU, kTraceBlock);
@@ -3071,6 +3096,7 @@ int st_tonemap(st_context* ctx, const float* hdr, const float* albedo, uint32_t 
     if (n == 0) return ST_OK;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
+        order_after_caller(ctx);
         ctx->tmax.reserve(16);
         uint32_t* mx = ctx->tmax.as<uint32_t>();
         const int grid = std::min<int>((n + 255) / 256, ctx->cus * 8);
@@ -3197,6 +3223,7 @@ int st_denoise(st_denoiser* d, const st_view* view, const st_denoise_inputs* in,
     try {
         st_context* ctx = d->ctx;
         HIP_CHECK(hipSetDevice(ctx->device));
+        order_after_caller(ctx);
         hipStream_t st = ctx->stream;
         const SvgfTiles t = svgf_tiles(d->W, d->H);
         const dim3 grid(t.per_xcd * kSvgfXcds), block(kSvgfTileX * kSvgfTileY);

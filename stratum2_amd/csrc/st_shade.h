@@ -259,25 +259,21 @@ __device__ inline float sample_texel_pdf(const DevScene& S, float u, float v) {
 }
 // Alias-table environment sampling (ENV_ALIAS): one texel by Walker's method
 // over luminance*sin(theta) of the full-resolution map, uniform inside it.
-__device__ inline void sample_alias(const DevScene& S, float rx, float ry, float& u, float& v, float& pdf) {
+// Four independent uniforms: rx picks the table column, ra is the accept test
+// against the column's probability, ju and jv place the point in the texel.
+// (Taking the accept test and the jitter from the fraction of rx * N would
+// leave them 23 - log2(N) bits: 4 bits for a 1024x512 map.)
+__device__ inline void sample_alias(const DevScene& S, float rx, float ra, float ju, float jv, float& u, float& v,
+                                    float& pdf) {
     const int W = S.env_w[0], H = S.env_h[0];
     const uint32_t N = (uint32_t)W * (uint32_t)H;
-    const float x = rx * (float)N;
-    uint32_t i = (uint32_t)x;
+    uint32_t i = (uint32_t)(rx * (float)N);
     if (i >= N) i = N - 1;
-    float f = x - (float)i;
     const float2 e = S.env_alias[i];
-    float jx;
-    if (f < e.x) {
-        jx = f / e.x;
-    } else {
-        jx = (f - e.x) / (1.0f - e.x);
-        i = __float_as_uint(e.y);
-    }
-    jx = fmin_(jx, 0.99999994f);
+    if (!(ra < e.x)) i = __float_as_uint(e.y);
     pdf = S.env_pdf[i];
-    u = ((float)(i % (uint32_t)W) + jx) / (float)W;
-    v = ((float)(i / (uint32_t)W) + ry) / (float)H;
+    u = ((float)(i % (uint32_t)W) + fmin_(ju, S.env_jmax)) / (float)W;
+    v = ((float)(i / (uint32_t)W) + fmin_(jv, S.env_jmax)) / (float)H;
 }
 __device__ __forceinline__ float alias_pdf(const DevScene& S, float u, float v) {
     const int W = S.env_w[0], H = S.env_h[0];
@@ -368,8 +364,14 @@ __device__ inline LightSample sample_illumination(const DevScene& S, float env_p
             float pdf, su = 0, sv = 0;
             if (S.env_has_image) {
                 float u, v;
-                if (ALIAS) sample_alias(S, rnd.x, rnd.y, u, v, pdf);
-                else sample_texel(S, rnd.x, rnd.y, u, v, pdf);
+                if (ALIAS) {
+                    // rnd.z is free on this branch; rnd.w, rescaled past the
+                    // environment-or-emitter choice, is uniform again
+                    const float ju = S.light_count > 0 ? rnd.w / env_prob : rnd.w;
+                    sample_alias(S, rnd.x, rnd.z, ju, rnd.y, u, v, pdf);
+                } else {
+                    sample_texel(S, rnd.x, rnd.y, u, v, pdf);
+                }
                 su = u;
                 sv = v;
                 d = uv_to_cart(u, v);

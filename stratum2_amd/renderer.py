@@ -13,6 +13,23 @@ def _torch():
     return torch
 
 
+def _enter(device):
+    """Ordering contract of the C ABI: a call waits for the work queued on the
+    device's null stream (torch's default stream) before it reads its inputs,
+    and its outputs are complete when it returns.  Work on any other torch
+    stream is finished here first."""
+    torch = _torch()
+    cur = torch.cuda.current_stream(device)
+    if cur != torch.cuda.default_stream(device):
+        cur.synchronize()
+
+
+def _check_tensor(t, name, shape, dtype, device):
+    if (tuple(t.shape) != tuple(shape) or t.dtype != dtype or not t.is_contiguous() or t.device.type != "cuda"
+            or t.device.index != device):
+        raise ValueError(f"{name} must be a contiguous {tuple(shape)} {dtype} tensor on cuda:{device}")
+
+
 class Context:
     def __init__(self, device=0):
         self.lib = abi.load_library()
@@ -79,7 +96,9 @@ class Scene:
         """rays: (N, 8) float32 device tensor {o.xyz, tmax, d.xyz, 0}."""
         torch = _torch()
         n = rays.shape[0]
-        out = torch.zeros((n, 4) if closest else (n,), dtype=torch.int32, device=rays.device)
+        _check_tensor(rays, "rays", (n, 8), torch.float32, self.ctx.device)
+        out = torch.empty((n, 4) if closest else (n,), dtype=torch.int32, device=rays.device)  # every entry is written
+        _enter(self.ctx.device)
         abi.check(self.lib, self.lib.st_trace_rays(self.h, rays.data_ptr(), n, int(closest), out.data_ptr()),
                   "st_trace_rays")
         return out
@@ -90,6 +109,7 @@ class Scene:
         dev = torch.device("cuda", self.ctx.device)
         vis = torch.empty((height, width, 2), dtype=torch.int32, device=dev)
         t = torch.empty((height, width), dtype=torch.float32, device=dev)
+        _enter(self.ctx.device)
         abi.check(self.lib, self.lib.st_trace_primary(self.h, C.byref(view), width, height, vis.data_ptr(), t.data_ptr()),
                   "st_trace_primary")
         return vis, t
@@ -105,21 +125,32 @@ class TestRenderer:
         torch = _torch()
         dev = torch.device("cuda", self.scene.ctx.device)
         W, H, S = params.width, params.height, params.spp
+        layout = {"hdr": ((H, W, 4), torch.float32), "visibility": ((H, W, 2), torch.int32),
+                  "ray_counts": ((2,), torch.int64), "path_length": ((S, H, W), torch.int32),
+                  "albedo": ((H, W, 4), torch.float32), "depth": ((H, W, 4), torch.float32),
+                  "prev_uv": ((H, W, 2), torch.float32)}
         res = out or {}
         if "hdr" not in res:
             res["hdr"] = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
             res["visibility"] = torch.empty((H, W, 2), dtype=torch.int32, device=dev)
-            res["ray_counts"] = torch.zeros(2, dtype=torch.int64, device=dev)
+            res["ray_counts"] = torch.empty(2, dtype=torch.int64, device=dev)
             res["path_length"] = torch.empty((S, H, W), dtype=torch.int32, device=dev) if want_path_length else None
             if want_aux:
                 res["albedo"] = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
                 res["depth"] = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
                 res["prev_uv"] = torch.empty((H, W, 2), dtype=torch.float32, device=dev)
+        else:  # caller-provided outputs: the kernels write them in place, so they must fit exactly
+            for k, t in res.items():
+                if k not in layout:
+                    raise ValueError(f"unknown output '{k}'")
+                if t is not None:
+                    _check_tensor(t, f"out['{k}']", *layout[k], self.scene.ctx.device)
 
         def ptr(k):
             return res[k].data_ptr() if res.get(k) is not None else None
         o = abi.st_outputs(ptr("hdr"), ptr("visibility"), ptr("path_length"), ptr("ray_counts"), ptr("albedo"), ptr("depth"),
                            ptr("prev_uv"))
+        _enter(self.scene.ctx.device)
         abi.check(self.scene.lib, self.scene.lib.st_render(self.scene.h, C.byref(view), C.byref(params), C.byref(o)),
                   "st_render")
         return res
@@ -185,6 +216,8 @@ class Denoiser:
         m = instance_index_map
         if m is not None and (m.dtype != torch.int32 or m.device.type != "cuda" or not m.is_contiguous()):
             raise ValueError("instance_index_map must be a contiguous int32 device tensor")
+        _check_tensor(out, "out", (H, W, 4), torch.float32, self.ctx.device)
+        _enter(self.ctx.device)
         inp = abi.st_denoise_inputs(frame["hdr"].data_ptr(), frame["albedo"].data_ptr(), frame["prev_uv"].data_ptr(),
                                     frame["visibility"].data_ptr(), frame["depth"].data_ptr(),
                                     m.data_ptr() if m is not None else None, m.numel() if m is not None else 0)

@@ -342,11 +342,12 @@ def _tri_normal_ok(verts, faces, want):
     return faces if (n @ np.asarray(want)).sum() > 0 else faces[:, [0, 2, 1]]
 
 
-def _random_material(rng, kind=None):
-    """Config material: albedo U(0.2,0.9)^3, roughness U(0.1,0.8); 'lambert' is
-    Disney {metallic 0, specular 0}, 'ggx' is Disney {metallic 1}."""
+def _random_material(rng, kind=None, min_roughness=0.0):
+    """Config material: albedo U(0.2,0.9)^3, roughness U(0.1,0.8) (raised to
+    min_roughness); 'lambert' is Disney {metallic 0, specular 0}, 'ggx' is
+    Disney {metallic 1}."""
     albedo = rng.uniform(0.2, 0.9, 3)
-    rough = rng.uniform(0.1, 0.8)
+    rough = max(rng.uniform(0.1, 0.8), min_roughness)
     if kind is None:
         kind = "lambert" if rng.uniform() < 0.5 else "ggx"
     if kind == "lambert":
@@ -354,10 +355,13 @@ def _random_material(rng, kind=None):
     return material_record(base_color=albedo, specular=0.0, metallic=1.0, roughness=rough)
 
 
-def cornell_box(seed=0, light_radiance=15.0):
+def cornell_box(seed=0, light_radiance=15.0, min_roughness=0.0):
     """Seeded Cornell box: 12 axis-aligned quads (5 walls, ceiling emitter,
     6-sided block) + an icosphere (subdivision 4, 5120 triangles) displaced by
-    seeded noise of amplitude 0.05.  Camera at (0, 1, 2.4) looking down -z."""
+    seeded noise of amplitude 0.05.  Camera at (0, 1, 2.4) looking down -z.
+    min_roughness 0.3 keeps every metallic material out of the Disney BSDF's
+    singular class (max(specular, metallic) * (1 - roughness) > 0.75,
+    disney.hlsli:387), whose vertices make no NEE."""
     rng = np.random.default_rng(seed)
     s = SceneData()
     center = np.array([0, 1, 0], np.float64)
@@ -374,7 +378,7 @@ def cornell_box(seed=0, light_radiance=15.0):
         c = p.mean(0)
         f = _tri_normal_ok(p, f, center - c)
         vi, n = s.add_mesh(p, f.reshape(-1))
-        s.add_mesh_instance(vi, n, s.add_material(_random_material(rng)))
+        s.add_mesh_instance(vi, n, s.add_material(_random_material(rng, min_roughness=min_roughness)))
     # ceiling emitter, facing down
     p, i = quad((-0.3, 1.98, -0.3), (0.3, 1.98, -0.3), (0.3, 1.98, 0.3), (-0.3, 1.98, 0.3))
     f = _tri_normal_ok(p, i.reshape(-1, 3), (0, -1, 0))
@@ -395,14 +399,14 @@ def cornell_box(seed=0, light_radiance=15.0):
         bi.append(f + len(bp) * 4)
         bp.append(p)
     vi, n = s.add_mesh(np.concatenate(bp), np.concatenate(bi).reshape(-1))
-    s.add_mesh_instance(vi, n, s.add_material(_random_material(rng)))
+    s.add_mesh_instance(vi, n, s.add_material(_random_material(rng, min_roughness=min_roughness)))
     # displaced icosphere
     v, f = icosphere(4)
     v = v * (1 + 0.05 / 0.4 * seeded_noise(rng, v))[:, None]
     v = v * 0.4 + np.array([0.45, 0.45, -0.2])
     f = _orient_outward(v, f, np.array([0.45, 0.45, -0.2]))
     vi, n = s.add_mesh(v.astype(np.float32), f.reshape(-1).astype(np.uint32))
-    s.add_mesh_instance(vi, n, s.add_material(_random_material(rng)))
+    s.add_mesh_instance(vi, n, s.add_material(_random_material(rng, min_roughness=min_roughness)))
     return s
 
 

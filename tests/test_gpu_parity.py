@@ -275,6 +275,44 @@ def test_restir_reuse_is_consistent(gpu_ctx):
     assert res[1][1] <= 1.05 * res[1][0]
 
 
+HEADLINE_FLAGS = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
+
+
+def test_headline_frame_modes_vs_oracle(gpu_ctx):
+    """The headline workload's own frame, scaled down: the 1M-triangle spheres
+    field (seed 1) with bench.py's flags exactly (alias-table environment,
+    RIS over 32 candidates with temporal + spatial reuse over 5 neighbours in
+    30 px, Russian roulette from path length 3, maximum path length 8), at
+    480x272 (8x8 tiled path order) and 8 spp, against the oracle's restatement
+    of every one of those modes."""
+    sd = scenes.spheres_field(seed=1)
+    W, H = 480, 272
+    params = abi.default_params(W, H, max_depth=8, spp=8, flags=HEADLINE_FLAGS, rr_start_depth=3)
+    g, r = _render_both(gpu_ctx, sd, scenes.spheres_view(W, H), params)
+    _check_image(g, r, frac_px=0.98, frac_len=0.98)
+
+
+def test_alias_env_and_emitters_vs_oracle(gpu_ctx):
+    """Alias-table environment sampling with emitters, one-sample NEE, path length 5."""
+    sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+    W, H = 96, 64
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    params = abi.default_params(W, H, max_depth=5, spp=4, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS)
+    g, r = _render_both(gpu_ctx, sd, view, params)
+    _check_image(g, r, frac_px=0.98, frac_len=0.98)
+
+
+@pytest.mark.parametrize("rr", [2, 3])
+def test_rr_start_vs_oracle(gpu_ctx, rr):
+    """Russian roulette from path length 2 or 3 (pdfs with and without the
+    continuation probability on either side of it), Cornell box, length 6."""
+    sd = scenes.cornell_box(0)
+    W = H = 64
+    params = abi.default_params(W, H, max_depth=6, spp=4, rr_start_depth=rr)
+    g, r = _render_both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
+    _check_image(g, r)
+
+
 # ------------------------------------------------------------------- media
 def _check_media_image(g, r, frac_vis=0.995, frac_px=0.98):
     """Media add free-flight sampling (logf) and transmittance (expf) to every

@@ -1,0 +1,154 @@
+"""CPU checks that pin the oracle's restatement of the headline frame's own
+modes, which the reference does not have (it warps the environment's mip
+pyramid, runs Russian roulette at every vertex and reuses reservoirs through
+a world-space hash grid; testrenderer.slang:608-726, 890-892,
+lights.hlsli:49-139):
+
+- Russian roulette from path length k (rr_start_depth) changes nothing at a
+  maximum depth that no vertex with roulette reaches, and keeps the image's
+  expectation otherwise (where no vertex is singular: the reference loses
+  light at singular vertices, more with roulette, which is kept and shown);
+- alias-table environment sampling keeps the expectation of the mip warp;
+- screen-space temporal + spatial reuse of the primary vertices' RIS
+  reservoirs keeps the expectation of RIS alone and of one-sample NEE;
+- the headline estimator (alias table, RIS, reuse) keeps the expectation of
+  the reference's (mip warp, one-sample NEE).
+
+Expectations are compared over independent renders (separate seed ranges),
+so the spatial correlation that reuse brings inside one image does not enter
+the standard error: a paired t over R image means per estimator, the pairs
+sharing seeds."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle.oracle import OracleScene
+from stratum2_amd import abi, scenes
+
+W, H = 96, 64
+
+
+@pytest.fixture(scope="module")
+def field():
+    sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    return OracleScene(sd), view
+
+
+def image_means(osc, view, reps, spp, **kw):
+    out = []
+    for r in range(reps):
+        p = abi.default_params(W, H, spp=spp, seed=1000 + 997 * r, **kw)
+        out.append(float(osc.render(view, p, want_path_length=False)["hdr"][..., :3].mean()))
+    return np.array(out)
+
+
+def same_expectation(a, b, alpha=1e-3):
+    """Paired t-test over renders that share their seed ranges: the two
+    estimators draw the same camera paths' first numbers, so their per-render
+    difference varies far less than either image mean."""
+    d = a - b
+    t = stats.ttest_rel(a, b)
+    print(f"means {a.mean():.5f} vs {b.mean():.5f}, difference {d.mean():+.5f} +- {d.std(ddof=1) / np.sqrt(len(d)):.5f} "
+          f"({d.std(ddof=1) / np.sqrt(len(d)) / a.mean():.2%} of the mean): p = {t.pvalue:.4f}")
+    return t.pvalue > alpha
+
+
+HEADLINE = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
+
+
+def test_rr_start_beyond_depth_changes_nothing(field):
+    """A vertex of path length max_depth ends the path before roulette, so
+    rr_start_depth = max_depth is the same estimator as never: identical
+    images, bit for bit.  One step earlier, roulette runs at one vertex and
+    the images differ."""
+    osc, view = field
+    img = lambda rr, d: osc.render(view, abi.default_params(W, H, spp=2, max_depth=d, rr_start_depth=rr,  # noqa: E731
+                                                            flags=HEADLINE), want_path_length=False)["hdr"]
+    assert np.array_equal(img(3, 3), img(99, 3))
+    assert np.array_equal(img(1, 1), img(0, 1))
+    assert not np.array_equal(img(2, 3), img(99, 3))
+
+
+@pytest.fixture(scope="module")
+def cornell_rough():
+    sd = scenes.cornell_box(0, min_roughness=0.3)
+    return OracleScene(sd), scenes.cornell_view(64, 64)
+
+
+def cornell_means(osc, view, reps, spp, **kw):
+    out = []
+    for r in range(reps):
+        p = abi.default_params(64, 64, spp=spp, seed=1000 + 997 * r, **kw)
+        out.append(float(osc.render(view, p, want_path_length=False)["hdr"][..., :3].mean()))
+    return np.array(out)
+
+
+def test_rr_from_three_keeps_expectation(cornell_rough):
+    """Roulette at every vertex, from path length 3, and never: one
+    expectation (Disney materials, none singular, path length 8)."""
+    osc, view = cornell_rough
+    every = cornell_means(osc, view, 32, 16, max_depth=8, rr_start_depth=0)
+    assert same_expectation(every, cornell_means(osc, view, 32, 16, max_depth=8, rr_start_depth=3))
+    assert same_expectation(every, cornell_means(osc, view, 32, 16, max_depth=8, rr_start_depth=99))
+
+
+def test_rr_and_singular_vertices_lose_light_as_the_reference_does():
+    """A reference property the oracle keeps: a vertex on a singular Disney
+    material (isSingular, disney.hlsli:387) makes no NEE, yet the emission its
+    BSDF sample then hits is still MIS-weighted against light sampling
+    (testrenderer.slang:790-799, 869-884).  That share of the light is lost,
+    and folding the continuation probability into the BSDF pdf makes the
+    weight smaller: with the Cornell box's metallic materials of roughness
+    below 0.25, roulette at every vertex gives a darker image than no
+    roulette, by about 2%."""
+    sd = scenes.cornell_box(0)
+    osc, view = OracleScene(sd), scenes.cornell_view(64, 64)
+    every = cornell_means(osc, view, 24, 16, max_depth=8, rr_start_depth=0)
+    never = cornell_means(osc, view, 24, 16, max_depth=8, rr_start_depth=99)
+    d = (every - never) / never.mean()
+    se = d.std(ddof=1) / np.sqrt(len(d))
+    print(f"every vs never: {d.mean():+.4f} +- {se:.4f} of the mean")
+    assert d.mean() < -0.01 and d.mean() + 4 * se < 0
+
+
+def test_alias_table_keeps_expectation(field):
+    osc, view = field
+    a = image_means(osc, view, 12, 32, max_depth=4)
+    b = image_means(osc, view, 12, 32, max_depth=4, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS)
+    assert same_expectation(a, b)
+
+
+def test_reuse_keeps_direct_light_expectation(field):
+    """Direct light only (gSampleDirectIlluminationOnly): one-sample NEE, RIS
+    over 32 candidates and RIS with temporal + spatial reuse (5 neighbours in
+    30 px) estimate the same image."""
+    osc, view = field
+    d = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_DIRECT_ONLY | abi.ST_FLAG_ENV_ALIAS
+    nee = image_means(osc, view, 8, 8, max_depth=2, flags=d)
+    ris = image_means(osc, view, 8, 8, max_depth=2, flags=d | abi.ST_FLAG_RESTIR_DI)
+    reuse = image_means(osc, view, 8, 8, max_depth=2, flags=d | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE)
+    assert same_expectation(nee, ris)
+    assert same_expectation(ris, reuse)
+
+
+def test_headline_estimator_keeps_reference_expectation(field):
+    """The headline frame's estimator (alias table, RIS with reuse) against the
+    reference's (mip warp, one-sample NEE), both with roulette from path length
+    3 (roulette itself is pinned above), path length 8.  The emitters'
+    lognormal radiance makes this comparison loose: its standard error is
+    about 2% of the mean."""
+    osc, view = field
+    ref = image_means(osc, view, 16, 32, max_depth=8, rr_start_depth=3)
+    head = image_means(osc, view, 16, 32, max_depth=8, rr_start_depth=3, flags=HEADLINE)
+    assert same_expectation(ref, head)
+
+
+def test_reuse_is_a_function_of_its_inputs(field):
+    """Reuse reads neighbours' reservoirs after every path of the sample is
+    shaded: the image does not depend on the thread count."""
+    osc, view = field
+    p = abi.default_params(W, H, spp=3, max_depth=8, rr_start_depth=3, flags=HEADLINE)
+    a = osc.render(view, p, threads=1, want_path_length=False)["hdr"]
+    b = osc.render(view, p, threads=0, want_path_length=False)["hdr"]
+    assert np.array_equal(a, b)
