@@ -56,6 +56,9 @@ def main():
     ap.add_argument("--device-bvh", action="store_true", help="rebuild the BVH on the GPU (st_scene_update) before timing")
     ap.add_argument("--light-trace", action="store_true", help="light tracing (paths start on the lights)")
     ap.add_argument("--bdpt", action="store_true", help="bidirectional mode with a light vertex cache (gUseVC)")
+    ap.add_argument("--serial", action="store_true",
+                    help="time the frames with waves one after another on one stream (ST_FLAG_SERIAL_WAVES), "
+                         "so a profiler sees each kernel alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -116,7 +119,8 @@ def main():
     if not args.no_restir:  # ReSTIR DI: 32 candidates, temporal + spatial (5 neighbours, 30 px) reuse
         flags |= abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
     rr = 0 if cloud else 3
-    params = abi.default_params(args.width, args.height, max_depth=max_depth, spp=args.spp, flags=flags, rr_start_depth=rr,
+    params = abi.default_params(args.width, args.height, max_depth=max_depth, spp=args.spp,
+                                flags=flags | (abi.ST_FLAG_SERIAL_WAVES if args.serial else 0), rr_start_depth=rr,
                                 samples_per_wave=args.samples_per_wave)
 
     # traversal statistics for the algorithmic-bytes model (untimed, 1 spp);
@@ -153,58 +157,83 @@ def main():
     value = sum(rays) / total_s / 1e6
     ms = total_s / len(times) * 1e3
 
-    # dominant kernel class and its algorithmic bytes.  Surfaces: 32 B ray +
-    # 16 B hit (or 48 B shadow record + 4 B flag) + 48 B per 4-wide node + 48 B
-    # per triangle.  Media (per traceRay segment): 52 B path state read + 52 B
-    # written (or the 48 B shadow record + 16 B NEE accumulator), and 44 B per
-    # density-grid read (root tile key + child, upper mask word + table entry,
-    # lower mask word + table entry, leaf value).  Shading: 16 B hit + 64 B path
-    # state + 64 B material + 64 B triangle + 48 B shadow record per vertex; with
-    # ReSTIR reuse each primary hit also evaluates `cand` light candidates
-    # (environment, alias mode: 8 B alias entry + 4 B pdf + 4 texels of 16 B;
-    # emissive triangle: 4 B light index + 8 B instance + 48 B transform + 32 B
-    # vertex info + 12 B indices + 36 B positions + 36 B material), mixed by the
-    # environment sampling probability, and writes its 48 B reservoir + record.
+    # Exclusive per-kernel times: one more frame, untimed, with the waves one
+    # after another on one stream (the same image), HIP events around each
+    # launch.  The dominant kernel is the one with the most device time.
+    serial = abi.default_params(args.width, args.height, max_depth=max_depth, spp=args.spp,
+                                flags=flags | abi.ST_FLAG_SERIAL_WAVES, rr_start_depth=rr,
+                                samples_per_wave=args.samples_per_wave)
+    a = time.perf_counter()
+    r.render(view, serial, out=out)
+    torch.cuda.synchronize()
+    serial_ms = (time.perf_counter() - a) * 1e3
+    kx = ctx.kernel_times()
+    ts = ctx.timings()
+    log(f"serial frame {serial_ms:.1f} ms, per kernel (ms, launches): {kx}")
+
+    # Algorithmic bytes per kernel.  Traversal: 32 B ray + 16 B hit (or 48 B
+    # shadow record + 4 B flag) + 48 B per 4-wide node + 48 B per triangle.
+    # Media (per traceRay segment): 52 B path state read + 52 B written (or the
+    # 48 B shadow record + 16 B NEE accumulator), and 44 B per density-grid read
+    # (root tile key + child, upper mask word + table entry, lower mask word +
+    # table entry, leaf value).  Shading: 16 B hit + 64 B path state + 64 B
+    # material + 64 B triangle + 48 B shadow record per vertex.  k_ris, per
+    # primary hit: the same 208 B of hit, state, material and triangle, `cand`
+    # light candidates (environment, alias mode: 8 B alias entry + 4 B pdf + 4
+    # texels of 16 B; emissive triangle: 16 B light record + 80 B emitter
+    # triangle + 80 B transform and material words), mixed by the environment
+    # sampling probability, and its 48 B reservoir + 48 B record written.  Reuse,
+    # per pixel and sample: temporal 144 B (two reservoirs, the primary record,
+    # one written), spatial 160 B (own reservoir and record read, one reservoir
+    # and one shadow record written); the neighbours' records are other pixels'
+    # reads, and light-sample re-evaluations read the same few MB of emitter
+    # table and environment as k_ris, so neither is counted again.
     NODE_B, TRI_B, GRID_B = 48, 48, 44
     CAND_ENV_B, CAND_TRI_B = 76, 176
-    ris_bytes = 0.0
-    if not cloud and not args.no_restir:
-        env_p = params.environment_sample_probability
-        ris_bytes = prim_hits * args.spp * (params.candidate_samples * (env_p * CAND_ENV_B + (1 - env_p) * CAND_TRI_B) + 48)
-    t_last = kt[-1]
+    env_p = params.environment_sample_probability
+    cand_b = env_p * CAND_ENV_B + (1 - env_p) * CAND_TRI_B
+    cr, sr = ts["closest_rays"], ts["shadow_rays"]
     if cloud:
-        classes = {
-            "k_trace_media": (t_last["trace_ms"], t_last["trace_launches"], t_last["closest_rays"] * 104 + GRID_B * nodes_c * t_last["closest_rays"]),
-            "k_shadow_media": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * 64 + GRID_B * nodes_s * t_last["shadow_rays"]),
-            "k_shade": (t_last["shade_ms"], t_last["shade_launches"], t_last["closest_rays"] * (16 + 64 + 64 + 28 + 48) + t_last["shadow_rays"] * 48),
-        }
+        algo = {"k_trace_closest": cr * 104 + GRID_B * nodes_c * cr,
+                "k_trace_shadow": sr * 64 + GRID_B * nodes_s * sr,
+                "k_shade": cr * (16 + 64 + 64 + 28 + 48) + sr * 48}
+        names = {"k_trace_closest": "k_trace_media", "k_trace_shadow": "k_shadow_media"}
     else:
-        classes = {
-            "k_trace_closest": (t_last["trace_ms"], t_last["trace_launches"], t_last["closest_rays"] * (48 + NODE_B * nodes_c + TRI_B * tris_c)),
-            "k_trace_shadow": (t_last["shadow_ms"], t_last["shadow_launches"], t_last["shadow_rays"] * (52 + NODE_B * nodes_s + TRI_B * tris_s)),
-            "k_shade": (t_last["shade_ms"], t_last["shade_launches"],
-                        t_last["closest_rays"] * (16 + 64 + 64 + 64 + 48) + t_last["shadow_rays"] * 48 + ris_bytes),
-        }
-        if args.light_trace:  # k_shade_light: hit, ray, throughput, state read; ray, throughput, state written
-            classes["k_shade_light"] = classes.pop("k_shade")
-        if args.bdpt:  # k_shade_vc and k_shade_light share the shading class; a cached-vertex connection
-            # adds its 48 B vertex, 64 B triangle and 64 B material
-            ms, ml, mb = classes.pop("k_shade")
-            classes["k_shade_vc"] = (ms, ml, mb + t_last["shadow_rays"] * (48 + 64 + 64) / 2)
-    dom = max(classes, key=lambda k: classes[k][0])
-    dms, dl, dbytes = classes[dom]
-    achieved = dbytes / (dms / 1e3) / 1e9 if dms > 0 else 0.0
+        algo = {"k_trace_closest": cr * (48 + NODE_B * nodes_c + TRI_B * tris_c),
+                "k_trace_shadow": sr * (52 + NODE_B * nodes_s + TRI_B * tris_s),
+                "k_shade": cr * (16 + 64 + 64 + 64 + 48) + sr * 48}
+        names = {"k_shade": "k_shade_light" if args.light_trace else "k_shade_vc" if args.bdpt else "k_shade"}
+        if args.bdpt:  # a cached-vertex connection adds its 48 B vertex, 64 B triangle and 64 B material
+            algo["k_shade"] += sr * (48 + 64 + 64) / 2
+        if not args.no_restir:
+            algo["k_ris"] = prim_hits * args.spp * (208 + params.candidate_samples * cand_b + 96)
+            algo["k_restir_reuse"] = args.width * args.height * args.spp * (144 + 160)
+    kernels = {}
+    for k, b in algo.items():
+        ms_k, n_k = kx[k]
+        if n_k == 0 or ms_k <= 0:
+            continue
+        kernels[names.get(k, k)] = {"ms_per_frame": round(ms_k, 3), "launches": n_k, "avg_launch_ms": round(ms_k / n_k, 4),
+                                    "bytes_per_launch": round(b / n_k), "frac": round(b / (ms_k / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_frame"])
+    d = kernels[dom]
+    # a kernel's own time per frame cannot exceed the frame
+    assert d["ms_per_frame"] <= ms, f"{dom}: {d['ms_per_frame']} ms of kernel time in a {ms:.1f} ms frame"
+    achieved = d["bytes_per_launch"] / (d["avg_launch_ms"] / 1e3) / 1e9
     traffic, traffic_src = measured_traffic(args, dom)
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
-                "bytes_per_launch": round(dbytes / max(1, dl)), "avg_launch_ms": round(dms / max(1, dl), 4),
+                "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_launch_ms"],
+                "timing": "HIP events around each launch in a frame rendered with ST_FLAG_SERIAL_WAVES "
+                          f"(no kernel overlap; {serial_ms:.1f} ms for the whole frame)",
+                "kernels": kernels,
                 "per_ray": ({"closest_grid_reads": round(nodes_c, 2), "shadow_grid_reads": round(nodes_s, 2)} if cloud else
                             {"closest_nodes": round(nodes_c, 2), "closest_tris": round(tris_c, 2),
                              "shadow_nodes": round(nodes_s, 2), "shadow_tris": round(tris_s, 2)})}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(sd, view, args, max_depth, log)
+        cpu = cpu_baseline(sd, view, args, max_depth, flags, rr, log)
 
     if cloud:
         workload = (f"{args.width}x{args.height} {args.spp}spp maxPathLength 16, 256^3 NanoVDB fBm grid (sigma_t 20, "
@@ -215,7 +244,8 @@ def main():
     else:
         workload = f"{args.width}x{args.height} {args.spp}spp maxPathLength 8"
     if not cloud and not args.no_restir:
-        workload += ", ReSTIR DI 32 candidates + temporal + spatial 5x30px"
+        workload += (", ReSTIR DI at primary vertices (32 candidates, temporal + spatial reuse over 5 neighbours in 30 px), "
+                     "one-sample NEE deeper")
     if args.light_trace:
         workload += ", light tracing (1 light subpath per pixel and sample, connected to the camera)"
     if args.bdpt:
@@ -346,14 +376,10 @@ def measured_traffic(args, kernel):
         return None, None
     with open(path) as f:
         rec = json.load(f)
-    ks = rec.get("kernels", {})
-    k = ks.get(kernel)
+    k = rec.get("kernels", {}).get(kernel)
     if not k:
         return None, None
-    total = k["hbm_bytes_per_dispatch"] * k["dispatches"]
-    if kernel == "k_shade" and "k_ris" in ks:  # the RIS pass is timed with the shading class
-        total += ks["k_ris"]["hbm_bytes_per_dispatch"] * ks["k_ris"]["dispatches"]
-    return int(total / k["dispatches"]), os.path.relpath(path, ROOT)
+    return int(k["hbm_bytes_per_dispatch"]), os.path.relpath(path, ROOT)
 
 
 def copy_calibration(log):
@@ -369,18 +395,18 @@ def copy_calibration(log):
     log("copy calibration: 3 copies of 268435456 bytes done")
 
 
-def cpu_baseline(sd, view, args, max_depth, log):
-    """Oracle (C port of the reference tracer, OpenMP) on the same frame at a
-    bounded number of samples (reference semantics: mip-warp env, RR at every
-    bounce, plain NEE), grown until it takes about a third of --cpu-seconds."""
+def cpu_baseline(sd, view, args, max_depth, flags, rr, log):
+    """The oracle (C restatement of the same estimator: same flags, roulette
+    start, candidates and reuse, OpenMP over this host's share of cores) on
+    the same frame at a bounded number of samples, grown until it takes about
+    a third of --cpu-seconds."""
     from stratum2_amd import abi
     from oracle.oracle import OracleScene
     o = OracleScene(sd)
     threads = min(16, os.cpu_count() or 1)  # this host's share of cores
     spp = 1
-    fl = abi.ST_FLAG_DEFAULT | (abi.ST_FLAG_TRACE_FROM_LIGHT if args.light_trace else 0) | (abi.ST_FLAG_BDPT_LVC if args.bdpt else 0)
     while True:
-        p = abi.default_params(args.width, args.height, max_depth=max_depth, spp=spp, flags=fl)
+        p = abi.default_params(args.width, args.height, max_depth=max_depth, spp=spp, flags=flags, rr_start_depth=rr)
         a = time.perf_counter()
         res = o.render(view, p, threads=threads, want_path_length=False)
         dt = time.perf_counter() - a
@@ -390,10 +416,8 @@ def cpu_baseline(sd, view, args, max_depth, log):
         spp = min(64, spp * max(2, int(args.cpu_seconds / 3 / max(dt, 1e-3)) + 1))
     log(f"cpu baseline: {args.width}x{args.height} x {spp} spp, {nr} rays in {dt:.2f}s on {threads} threads")
     return {"value": round(nr / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (C port of the reference tracer, "
-                      f"{'light tracing' if args.light_trace else 'bidirectional, light vertex cache' if args.bdpt else 'plain NEE'}), "
-                      f"same {args.width}x{args.height} frame at "
-                      f"{spp} spp, maxPathLength {max_depth}, {nr} rays in {dt:.1f} s"}
+            "sample": f"CPU oracle (C restatement of this estimator, the same flags), the same "
+                      f"{args.width}x{args.height} frame at {spp} spp, maxPathLength {max_depth}, {nr} rays in {dt:.1f} s"}
 
 
 if __name__ == "__main__":

@@ -164,6 +164,11 @@ typedef struct st_view {
  * the MIS weights of NEE and the connections: in a 64x64 Cornell box with one
  * light subpath per pixel, paths of three segments come out 8% too dark. */
 #define ST_FLAG_VC_FP32_MIS             (1u << 12)
+/* Run the waves one after another on the context stream instead of rotating
+ * them through four wave slots on four streams.  The image is the same bit
+ * for bit; no two kernels of the call overlap, so the per-kernel HIP-event
+ * times of st_last_kernel_times are each kernel's own (profiling). */
+#define ST_FLAG_SERIAL_WAVES            (1u << 13)
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface
@@ -249,7 +254,8 @@ int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closes
 
 /* Device time of the last st_render per kernel class (HIP events on the
  * context stream) and traversal statistics for the algorithmic-bytes roofline:
- * out[0..2]  ms in closest-hit trace, shade, shadow trace
+ * out[0..2]  ms in closest-hit trace, shade (k_shade alone: k_ris and the reuse
+ *            passes are timed apart, st_last_kernel_times), shadow trace
  * out[3..5]  launches of each class
  * out[6..7]  nodes visited / triangles tested by closest-hit rays (COUNT_TRAVERSAL)
  * out[8..9]  nodes visited / triangles tested by shadow rays (COUNT_TRAVERSAL)
@@ -258,6 +264,20 @@ int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closes
  * In scenes with media, out[6] and out[8] count density-grid reads (free-flight
  * collisions) of scattering and shadow rays instead of BVH nodes. */
 int st_last_timings(st_context* ctx, double out[12]);
+
+/* Device time of the last st_render per kernel (HIP events around each
+ * launch, on the stream it runs on):
+ * out[0..4]  ms in k_trace_closest (or k_trace_media), shading (k_shade,
+ *            k_shade_light, k_shade_vc), k_trace_shadow (or k_shadow_media),
+ *            k_ris (primary-vertex RIS of the reuse mode), the reuse passes
+ *            (k_restir_temporal + k_restir_spatial)
+ * out[5..9]  launches of each (one per bounce and wave; reuse: per sample)
+ * Waves in flight on several streams overlap, so a launch's interval can hold
+ * other kernels too: render with ST_FLAG_SERIAL_WAVES for exclusive times.
+ * No counterpart in the reference (its profiler scopes, Profiler.hpp, time
+ * command-buffer regions on the host); replaces the GUI readout of
+ * ReSTIRPT.cpp:89-95 for the roofline. */
+int st_last_kernel_times(st_context* ctx, double out[10]);
 
 /* Tonemapper modes (TonemapMode, src/Shaders/compat/tonemap.h:10-23). */
 #define ST_TONEMAP_RAW 0

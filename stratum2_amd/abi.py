@@ -32,6 +32,7 @@ ST_FLAG_DIRECT_ONLY = 1 << 9
 ST_FLAG_TRACE_FROM_LIGHT = 1 << 10  # TestRenderer light tracing (gTraceFromLight)
 ST_FLAG_BDPT_LVC = 1 << 11  # TestRenderer bidirectional mode with a light vertex cache (gUseVC)
 ST_FLAG_VC_FP32_MIS = 1 << 12  # dVC / dVCM in fp32 instead of the reference's fp16
+ST_FLAG_SERIAL_WAVES = 1 << 13  # waves one after another on one stream (exclusive per-kernel times)
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 # TonemapMode (src/Shaders/compat/tonemap.h:10-23)
@@ -182,7 +183,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "libstratum2.so")
 EXPORTS = [
     "st_last_error", "st_version", "st_context_create", "st_context_destroy", "st_sync",
     "st_scene_create", "st_scene_destroy", "st_scene_stats", "st_render", "st_trace_primary",
-    "st_trace_rays", "st_last_timings", "st_tonemap", "st_scene_update",
+    "st_trace_rays", "st_last_timings", "st_last_kernel_times", "st_tonemap", "st_scene_update",
     "st_denoiser_create", "st_denoiser_destroy", "st_denoiser_reset", "st_denoiser_accumulated_frames", "st_denoise",
     "st_denoiser_last_timings",
 ]
@@ -213,6 +214,7 @@ def load_library(path=LIB_PATH):
     lib.st_tonemap.argtypes = [vp, vp, vp, u32, u32, C.c_float, C.c_int, C.c_int, vp]
     lib.st_scene_update.argtypes = [vp, vp, C.POINTER(C.c_double)]
     lib.st_last_timings.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.st_last_kernel_times.argtypes = [vp, C.POINTER(C.c_double)]
     lib.st_denoiser_create.argtypes = [vp, u32, u32, C.POINTER(vp)]
     lib.st_denoiser_destroy.argtypes = [vp]
     lib.st_denoiser_reset.argtypes = [vp]

@@ -1812,8 +1812,12 @@ struct st_context {
     hipEvent_t entry = nullptr;          // the caller's null-stream work at a call's entry
     DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
     double timings[12] = {0};
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
-    size_t ev_used[3] = {0, 0, 0};
+    // HIP-event timed kernel classes: 0 closest-hit trace, 1 shading, 2 shadow
+    // trace, 3 primary-vertex RIS (k_ris), 4 screen-space reuse passes
+    static constexpr int kKinds = 5;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[kKinds];
+    size_t ev_used[kKinds] = {0, 0, 0, 0, 0};
+    double kernel_ms[kKinds] = {0, 0, 0, 0, 0};
 };
 
 struct st_scene {
@@ -2580,16 +2584,22 @@ static WaveState wave_state(WaveBufs& b, st_context* ctx) {
 
 // Per-class device times of the last render (HIP events), traversal counters
 // and the rays of both wave slots; writes the ray counts to `ray_counts`.
+static void begin_timings(st_context* ctx) {
+    for (int kk = 0; kk < st_context::kKinds; kk++) ctx->ev_used[kk] = 0;
+}
 static void finish_timings(st_context* ctx, int slots, uint64_t* ray_counts) {
-    for (int kind = 0; kind < 3; kind++) {
+    for (int kind = 0; kind < st_context::kKinds; kind++) {
         double ms = 0;
         for (size_t i = 0; i < ctx->ev_used[kind]; i++) {
             float t = 0;
             HIP_CHECK(hipEventElapsedTime(&t, ctx->ev[kind][i].first, ctx->ev[kind][i].second));
             ms += t;
         }
-        ctx->timings[kind] = ms;
-        ctx->timings[3 + kind] = (double)ctx->ev_used[kind];
+        ctx->kernel_ms[kind] = ms;
+        if (kind < 3) {
+            ctx->timings[kind] = ms;
+            ctx->timings[3 + kind] = (double)ctx->ev_used[kind];
+        }
     }
     unsigned long long hs[4], rc[2] = {0, 0};
     HIP_CHECK(hipMemcpy(hs, ctx->stats.p, 32, hipMemcpyDeviceToHost));
@@ -2636,7 +2646,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         const uint64_t paths = (uint64_t)npix * k;
         if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
         const uint32_t waves = (p->spp + k - 1) / k;
-        const int slots = (int)std::min<uint32_t>(kWaveSlots, waves);
+        const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) ? 1 : (int)std::min<uint32_t>(kWaveSlots, waves);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, reuse, paths, tgrid);
         ctx->acc.reserve((size_t)npix * 16);
@@ -2648,7 +2658,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
         HIP_CHECK(hipEventRecord(ctx->start_done, s0));
         for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
-        for (int kk = 0; kk < 3; kk++) ctx->ev_used[kk] = 0;
+        begin_timings(ctx);
         RenderConsts c;
         std::memset(&c, 0, sizeof c);
         c.W = p->width;
@@ -2712,18 +2722,19 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                     else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                     else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                 }
+                // with screen-space reuse, k_ris runs the primary vertices' RIS
+                // first and every vertex after takes plain NEE: the shading
+                // kernel without the candidate loop
+                if (reuse && b == 0) {
+                    Timed tm(ctx, 3, sm);
+                    auto kr = lambert ? (alias ? k_ris<true, true> : k_ris<true, false>)
+                                      : (alias ? k_ris<false, true> : k_ris<false, false>);
+                    hipLaunchKernelGGL(kr, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
+                }
                 {
                     Timed tm(ctx, 1, sm);
                     AuxOut aux{nullptr, nullptr, nullptr, nullptr};
                     if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
-                    // with screen-space reuse, k_ris runs the primary vertices' RIS
-                    // first and every vertex after takes plain NEE: the shading
-                    // kernel without the candidate loop
-                    if (reuse && b == 0) {
-                        auto kr = lambert ? (alias ? k_ris<true, true> : k_ris<true, false>)
-                                          : (alias ? k_ris<false, true> : k_ris<false, false>);
-                        hipLaunchKernelGGL(kr, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
-                    }
                     auto* kfn = shade_kernel(lambert, alias, restir && !reuse, media);
                     hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0);
                 }
@@ -2737,6 +2748,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
                     // temporal reuse chains consecutive samples: sample j of the wave
                     // reuses sample j - 1's final reservoirs
                     for (uint32_t j = 0; j < c.k; j++) {
+                        Timed tm(ctx, 4, sm);
                         hipLaunchKernelGGL(kt, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, (base + j) > 0 ? 1 : 0, j);
                         hipLaunchKernelGGL(ks, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, j);
                     }
@@ -2789,9 +2801,10 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         const uint64_t paths = (uint64_t)span * k;
         if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
         const uint32_t waves = (p->spp + k - 1) / k;
-        const int slots = (int)std::min<uint32_t>(kWaveSlots, waves);
+        const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) ? 1 : (int)std::min<uint32_t>(kWaveSlots, waves);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, false, 0, tgrid);
+        begin_timings(ctx);
         ctx->acc.reserve((size_t)npix * 16);
         ctx->stats.reserve(64);
         hipStream_t s0 = ctx->stream;
@@ -2908,7 +2921,7 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
         const uint64_t paths = std::max<uint64_t>(L, npix);
         if (paths >= (1ull << 31) || (uint64_t)L * stride >= (1ull << 31))
             return set_error(ST_ERROR_LIMIT, "too many paths or light vertices per sample");
-        const int slots = (int)std::min<uint32_t>(kWaveSlots, p->spp);
+        const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) ? 1 : (int)std::min<uint32_t>(kWaveSlots, p->spp);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         size_t scan_bytes = 0;
         HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)L));
@@ -2932,7 +2945,7 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
         HIP_CHECK(hipEventRecord(ctx->start_done, s0));
         for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
-        for (int kk = 0; kk < 3; kk++) ctx->ev_used[kk] = 0;
+        begin_timings(ctx);
         RenderConsts c;
         std::memset(&c, 0, sizeof c);
         c.W = p->width;
@@ -3034,6 +3047,15 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
 int st_last_timings(st_context* ctx, double out[12]) {
     if (!ctx || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
     for (int i = 0; i < 12; i++) out[i] = ctx->timings[i];
+    return ST_OK;
+}
+
+int st_last_kernel_times(st_context* ctx, double out[10]) {
+    if (!ctx || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < st_context::kKinds; i++) {
+        out[i] = ctx->kernel_ms[i];
+        out[st_context::kKinds + i] = (double)ctx->ev_used[i];
+    }
     return ST_OK;
 }
 

@@ -55,6 +55,14 @@ class Context:
                 "closest_nodes", "closest_tris", "shadow_nodes", "shadow_tris", "closest_rays", "shadow_rays"]
         return dict(zip(keys, list(out)))
 
+    def kernel_times(self):
+        """Per-kernel device ms and launches of the last render
+        (st_last_kernel_times); exclusive when rendered with ST_FLAG_SERIAL_WAVES."""
+        out = (C.c_double * 10)()
+        abi.check(self.lib, self.lib.st_last_kernel_times(self.h, out), "st_last_kernel_times")
+        names = ["k_trace_closest", "k_shade", "k_trace_shadow", "k_ris", "k_restir_reuse"]
+        return {n: (out[i], int(out[5 + i])) for i, n in enumerate(names)}
+
 
 class Scene:
     """Device copy of a SceneData (Scene::updateFrameData + BVH build)."""

@@ -566,6 +566,24 @@ def test_restir_reuse_independent_of_samples_per_wave(gpu_ctx):
     assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
 
 
+@pytest.mark.parametrize("flags", [HEADLINE_FLAGS, abi.ST_FLAG_DEFAULT | abi.ST_FLAG_TRACE_FROM_LIGHT,
+                                   abi.ST_FLAG_DEFAULT | abi.ST_FLAG_BDPT_LVC])
+def test_serial_waves_give_identical_images(gpu_ctx, flags):
+    """ST_FLAG_SERIAL_WAVES (one stream, the profiling mode) renders the same
+    image, bit for bit, as waves rotating through four streams."""
+    sd = scenes.spheres_field(seed=6, n_spheres=60, n_emitters=40, env=True)
+    W, H = 96, 64
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 14.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    rr = 0 if flags & abi.ST_FLAG_BDPT_LVC else 3
+    a = _gpu_render(gpu_ctx, sd, view, abi.default_params(W, H, max_depth=6, spp=6, flags=flags, rr_start_depth=rr,
+                                                          samples_per_wave=1))
+    b = _gpu_render(gpu_ctx, sd, view, abi.default_params(W, H, max_depth=6, spp=6, rr_start_depth=rr, samples_per_wave=1,
+                                                          flags=flags | abi.ST_FLAG_SERIAL_WAVES))
+    assert np.array_equal(a, b)
+    t = gpu_ctx.kernel_times()
+    assert t["k_trace_closest"][1] > 0 and t["k_shade"][1] > 0
+
+
 def _emissive_smooth_sphere_scene():
     """Cornell box whose coarse sphere carries vertex normals and emits: light
     samples on it take the geometry-normal flip of makeTriangleShadingData."""
