@@ -252,6 +252,19 @@ int st_trace_primary(st_scene* scene, const st_view* view, uint32_t width, uint3
 int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closest,
                   uint32_t* hits);
 
+/* st_trace_rays without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as st_trace_rays.  Its traversal scratch (ray-fetch counter and the
+ * HBM part of the per-lane stacks) is one fixed 16 MiB + 256 B device array
+ * per GPU, held until the process ends; the traversal grid shrinks to what
+ * fits it (fewer lanes for deeper trees), and a tree too deep for even one
+ * block returns ST_ERROR_LIMIT, as do counts of 2^31 or more.  Calls on one
+ * GPU share the array: calls that overlap on one GPU (two threads, two
+ * contexts) race and must be serialised by the caller. */
+int st_trace_rays_noalloc(st_scene* scene, const float* rays, uint32_t count, int closest,
+                          uint32_t* hits);
+
 /* Device time of the last st_render per kernel class (HIP events on the
  * context stream) and traversal statistics for the algorithmic-bytes roofline:
  * out[0..2]  ms in closest-hit trace, shade (k_shade alone: k_ris and the reuse

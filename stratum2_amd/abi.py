@@ -183,7 +183,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "libstratum2.so")
 EXPORTS = [
     "st_last_error", "st_version", "st_context_create", "st_context_destroy", "st_sync",
     "st_scene_create", "st_scene_destroy", "st_scene_stats", "st_render", "st_trace_primary",
-    "st_trace_rays", "st_last_timings", "st_last_kernel_times", "st_tonemap", "st_scene_update",
+    "st_trace_rays", "st_trace_rays_noalloc", "st_last_timings", "st_last_kernel_times", "st_tonemap", "st_scene_update",
     "st_denoiser_create", "st_denoiser_destroy", "st_denoiser_reset", "st_denoiser_accumulated_frames", "st_denoise",
     "st_denoiser_last_timings",
 ]
@@ -211,6 +211,7 @@ def load_library(path=LIB_PATH):
     lib.st_render.argtypes = [vp, C.POINTER(st_view), C.POINTER(st_render_params), C.POINTER(st_outputs)]
     lib.st_trace_primary.argtypes = [vp, C.POINTER(st_view), u32, u32, vp, vp]
     lib.st_trace_rays.argtypes = [vp, vp, u32, C.c_int, vp]
+    lib.st_trace_rays_noalloc.argtypes = [vp, vp, u32, C.c_int, vp]
     lib.st_tonemap.argtypes = [vp, vp, vp, u32, u32, C.c_float, C.c_int, C.c_int, vp]
     lib.st_scene_update.argtypes = [vp, vp, C.POINTER(C.c_double)]
     lib.st_last_timings.argtypes = [vp, C.POINTER(C.c_double)]

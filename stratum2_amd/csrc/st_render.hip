@@ -1795,6 +1795,20 @@ struct DevBuf {
 // (more slots share queues, larger waves were slower).
 constexpr int kWaveSlots = 4;
 constexpr uint32_t kWavePaths = 2u << 20;
+
+// Fixed device scratch of st_trace_rays_noalloc: a 256-byte header holding the
+// traversal's ray-fetch counter, then the HBM part of the per-lane traversal
+// stacks.  One copy per GPU, held until the process ends; calls on one GPU
+// share it.  Host code never passes `scratch` itself to a kernel (on the host
+// it names a stand-in, not device memory): it uses the device address that
+// hipGetSymbolAddress returns, kept per device in g_scratch.
+constexpr size_t kScratchHeader = 256;
+constexpr size_t kScratchSpill = 16u << 20;
+constexpr size_t kScratchBytes = kScratchHeader + kScratchSpill;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxDevices] = {};
 struct WaveBufs {
     DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, pri, psd, rinit, rtemp, med;
     DevBuf vcm, lv_raw, lv, lv_cnt, lv_off, lv_total, scan_tmp;  // bidirectional mode
@@ -1810,6 +1824,7 @@ struct st_context {
     WaveBufs slot[kWaveSlots];
     hipEvent_t start_done = nullptr;     // the frame's clears on the context stream
     hipEvent_t entry = nullptr;          // the caller's null-stream work at a call's entry
+    uint8_t* scratch = nullptr;          // this device's fixed scratch (device address of `scratch`)
     DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
     double timings[12] = {0};
     // HIP-event timed kernel classes: 0 closest-hit trace, 1 shading, 2 shadow
@@ -2120,9 +2135,21 @@ int st_version(void) { return 1; }
 int st_context_create(int device, st_context** out) {
     if (!out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null output");
     try {
+        if (device < 0 || device >= kMaxDevices) return set_error(ST_ERROR_INVALID_ARGUMENT, "device index out of range");
         auto ctx = std::make_unique<st_context>();
         ctx->device = device;
         HIP_CHECK(hipSetDevice(device));
+        {
+            // the fixed scratch's device address, once per device, with the device current
+            std::lock_guard<std::mutex> lock(g_scratch_mu);
+            if (!g_scratch[device]) {
+                void* ptr = nullptr;
+                HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+                if (!ptr) throw HipError("hipGetSymbolAddress(scratch) returned no address");
+                g_scratch[device] = ptr;
+            }
+            ctx->scratch = static_cast<uint8_t*>(g_scratch[device]);
+        }
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, device));
         ctx->cus = prop.multiProcessorCount;
@@ -3073,6 +3100,38 @@ int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, u
         HIP_CHECK(hipMemsetAsync(B.counters.as<uint8_t>() + 4 * 18, 0, 4, ctx->stream));
         hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
                            closest, hits, B.spill.as<uint32_t>(), B.counters.as<uint32_t>() + 18);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_trace_rays_noalloc(st_scene* s, const float* rays, uint32_t count, int closest, uint32_t* hits) {
+    if (!s || (count && (!rays || !hits))) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (count > 0x7fffffffu) return set_error(ST_ERROR_LIMIT, "at most 2^31 - 1 rays per call");
+    st_context* ctx = s->ctx;
+    // Per lane, nearest-first traversal holds at most 3 stack entries per tree
+    // level; past the LDS part they go to the scratch's spill region, laid out
+    // [entry][lane] over the grid's lanes, so the grid is sized to fit it.
+    const uint64_t depth = s->stats[2];
+    const uint64_t entries = 3 * depth + 1 > (uint64_t)kLdsStack ? 3 * depth + 1 - kLdsStack : 1;
+    const uint64_t fit_blocks = kScratchSpill / (entries * 4 * kTraceBlock);
+    if (fit_blocks < 1)
+        return set_error(ST_ERROR_LIMIT, "BVH too deep for the fixed traversal scratch of st_trace_rays_noalloc");
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        if (count == 0) return ST_OK;
+        order_after_caller(ctx);
+        const uint64_t want = ((uint64_t)count + kTraceBlock - 1) / kTraceBlock;
+        const int grid = (int)std::min<uint64_t>({want, fit_blocks, (uint64_t)ctx->cus * kTraceBlocksPerCU});
+        uint32_t* fetch = reinterpret_cast<uint32_t*>(ctx->scratch);
+        uint32_t* spill = reinterpret_cast<uint32_t*>(ctx->scratch + kScratchHeader);
+        // the fetch counter starts at zero (a fill kernel: also right inside a captured graph)
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, fetch, 1u, 0u);
+        hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
+                           closest, hits, spill, fetch);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return ST_OK;
