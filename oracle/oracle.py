@@ -56,6 +56,7 @@ def lib():
         l.or_nvdb_read.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         l.or_nvdb_read.restype = C.c_float
         l.or_nvdb_world_to_index.argtypes = [vp, vp, C.c_int, vp]
+        l.or_math.argtypes = [C.c_int, C.c_uint32, vp, vp, vp]
         l.or_hg_eval.argtypes = [C.c_float, vp, vp]
         l.or_hg_eval.restype = C.c_float
         l.or_hg_sample.argtypes = [C.c_float, C.c_float, C.c_float, vp, vp]
@@ -139,6 +140,18 @@ class OracleScene:
         out = np.zeros((len(rays), 4), np.uint32)
         self.lib.or_trace_rays_bruteforce_f64(self.h, rays.ctypes.data, len(rays), out.ctypes.data)
         return out
+
+
+MATH_FNS = {"sin": 0, "cos": 1, "atan2": 2, "acos": 3, "log": 4, "exp": 5, "pow": 6}
+
+
+def math_fn(name, x, y=None):
+    """The oracle's deterministic transcendental `name` over float32 arrays."""
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    yy = np.ascontiguousarray(y, np.float32).reshape(-1) if y is not None else None
+    out = np.zeros_like(x)
+    lib().or_math(MATH_FNS[name], len(x), x.ctypes.data, yy.ctypes.data if yy is not None else None, out.ctypes.data)
+    return out
 
 
 def tonemap(hdr, albedo=None, exposure=0.0, mode=0, gamma=True):

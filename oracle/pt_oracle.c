@@ -200,6 +200,134 @@ void or_ray_offset(const float p[3], const float n[3], float o[3]) {
 }
 
 /* makeOrthonormal, common.hlsli:24-34 (rows r0, r1, r2 = N) */
+/* ------------------------------------------ deterministic transcendentals */
+/* The renderer's transcendental functions as fixed sequences of float
+ * operations (round to nearest, no contraction), the same sequences the GPU
+ * library runs, so that both sample the same directions and distances bit for
+ * bit.  Method: Cody-Waite argument reduction and the single-precision
+ * minimax polynomials of the Cephes library (S. L. Moshier, "Methods and
+ * Programs for Mathematical Functions", 1989: sinf.c, atanf.c, asinf.c,
+ * logf.c, expf.c), evaluated in float.  The reference's own precision
+ * (Slang -> SPIR-V -> driver) is unpinned; these are pinned against float64
+ * by their error bounds in tests/test_oracle_math.py. */
+static float om_poly_sin(float x, float z) {
+    return ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * x + x;
+}
+static float om_poly_cos(float z) {
+    const float y = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z;
+    return (y - 0.5f * z) + 1.0f;
+}
+static float om_reduce(float a, int* jo) {
+    int j = (int)(a * 1.27323954473516f);
+    float y = (float)j;
+    if (j & 1) { j += 1; y += 1.0f; }
+    j &= 7;
+    *jo = j;
+    return ((a - y * 0.78515625f) - y * 2.4187564849853515625e-4f) - y * 3.77489497744594108e-8f;
+}
+static float om_sin(float x) {
+    float sign = 1.0f, a = x;
+    if (a < 0) { a = -a; sign = -1.0f; }
+    int j;
+    const float r = om_reduce(a, &j);
+    if (j > 3) { sign = -sign; j -= 4; }
+    const float z = r * r;
+    const float y = (j == 1 || j == 2) ? om_poly_cos(z) : om_poly_sin(r, z);
+    return sign < 0 ? -y : y;
+}
+static float om_cos(float x) {
+    const float a = x < 0 ? -x : x;
+    int j;
+    const float r = om_reduce(a, &j);
+    float sign = 1.0f;
+    if (j > 3) { j -= 4; sign = -sign; }
+    if (j > 1) sign = -sign;
+    const float z = r * r;
+    const float y = (j == 1 || j == 2) ? om_poly_sin(r, z) : om_poly_cos(z);
+    return sign < 0 ? -y : y;
+}
+static float om_atan_pos(float x) {
+    float y = 0.0f;
+    if (x > 2.414213562373095f) { y = 1.5707963267948966f; x = -(1.0f / x); }
+    else if (x > 0.4142135623730950f) { y = 0.7853981633974483f; x = (x - 1.0f) / (x + 1.0f); }
+    const float z = x * x;
+    return y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z * x + x);
+}
+/* atan2 for x != 0 (stable_atan2 handles x == 0, compat/common.h:137-140) */
+static float om_atan2(float y, float x) {
+    const float q = y / x;
+    const float t = q < 0 ? -om_atan_pos(-q) : om_atan_pos(q);
+    if (x > 0) return t;
+    return y < 0 ? t - 3.14159265358979f : t + 3.14159265358979f;
+}
+static float om_asin_pos(float a, int* big) {
+    float x, z;
+    *big = a > 0.5f;
+    if (*big) { z = 0.5f * (1.0f - a); x = sqrtf(z); }
+    else { x = a; z = x * x; }
+    return ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z +
+            1.6666752422e-1f) * z * x + x;
+}
+static float om_acos(float x) {
+    const float a = x < 0 ? -x : x;
+    int big;
+    const float s = om_asin_pos(a, &big);
+    if (big) {
+        const float r = s + s;
+        return x < 0 ? 3.14159265358979f - r : r;
+    }
+    return x < 0 ? 1.5707963267948966f + s : 1.5707963267948966f - s;
+}
+static float om_log(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+    if (x == INFINITY) return x;
+    int e = 0;
+    if (x < 1.17549435e-38f) { x *= 16777216.0f; e = -24; }
+    const uint32_t b = asuint(x);
+    e += (int)((b >> 23) & 0xffu) - 126;
+    float m = asfloat((b & 0x007fffffu) | 0x3f000000u);
+    if (m < 0.707106781186547524f) { e -= 1; m = m + m - 1.0f; }
+    else m = m - 1.0f;
+    const float z = m * m;
+    float y = ((((((((7.0376836292e-2f * m - 1.1514610310e-1f) * m + 1.1676998740e-1f) * m - 1.2420140846e-1f) * m +
+                   1.4249322787e-1f) * m - 1.6668057665e-1f) * m + 2.0000714765e-1f) * m - 2.4999993993e-1f) * m +
+               3.3333331174e-1f) * m * z;
+    const float fe = (float)e;
+    y += -2.12194440e-4f * fe;
+    y += -0.5f * z;
+    return (m + y) + 0.693359375f * fe;
+}
+static float om_exp(float x) {
+    if (x != x) return x;
+    if (x > 88.7228391f) return INFINITY;
+    if (x < -87.0f) return 0.0f;
+    const float fz = floorf(1.44269504088896341f * x + 0.5f);
+    float r = x - fz * 0.693359375f;
+    r = r - fz * -2.12194440e-4f;
+    const int n = (int)fz;
+    const float z = r * r;
+    const float p = ((((((1.9875691500e-4f * r + 1.3981999507e-3f) * r + 8.3334519073e-3f) * r + 4.1665795894e-2f) * r +
+                      1.6666665459e-1f) * r + 5.0000001201e-1f) * z + r) + 1.0f;
+    const int n1 = n / 2, n2 = n - n1;
+    return (p * asfloat((uint32_t)(n1 + 127) << 23)) * asfloat((uint32_t)(n2 + 127) << 23);
+}
+static float om_pow(float x, float y) { return om_exp(y * om_log(x)); }
+static float om_eval(int fn, float x, float y) {
+    switch (fn) {
+        case 0: return om_sin(x);
+        case 1: return om_cos(x);
+        case 2: return om_atan2(x, y);
+        case 3: return om_acos(x);
+        case 4: return om_log(x);
+        case 5: return om_exp(x);
+        case 6: return om_pow(x, y);
+        default: return NAN;
+    }
+}
+void or_math(int fn, uint32_t n, const float* x, const float* y, float* out) {
+    for (uint32_t i = 0; i < n; i++) out[i] = om_eval(fn, x[i], y ? y[i] : 0.0f);
+}
+
 static void make_orthonormal(f3 N, f3 r[3]) {
     if (N.x != N.y || N.x != N.z) r[0] = v3(N.z - N.y, N.x - N.z, N.y - N.x);
     else r[0] = v3(N.z - N.y, N.x + N.z, -N.y - N.x);
@@ -214,14 +342,14 @@ static void sample_uniform_triangle(float u1, float u2, float b[2]) {
 }
 static f3 sample_uniform_sphere(float u1, float u2) {
     float z = u2 * 2 - 1, r = sqrtf(maxf(1 - z * z, 0)), phi = 2 * OR_PI * u1;
-    return v3(r * cosf(phi), r * sinf(phi), z);
+    return v3(r * om_cos(phi), r * om_sin(phi), z);
 }
 static void sample_concentric_disc(float u1, float u2, float o[2]) {
     float ux = 2 * u1 - 1, uy = 2 * u2 - 1, theta, r;
     if (ux == 0 && uy == 0) { o[0] = o[1] = 0; return; }
     if (fabsf(ux) > fabsf(uy)) { r = ux; theta = (OR_PI / 4) * (uy / ux); }
     else { r = uy; theta = OR_PI / 2 - (OR_PI / 4) * (ux / uy); }
-    o[0] = r * cosf(theta); o[1] = r * sinf(theta);
+    o[0] = r * om_cos(theta); o[1] = r * om_sin(theta);
 }
 static f3 sample_cos_hemisphere(float u1, float u2) {
     float xy[2];
@@ -236,18 +364,18 @@ static float mis2(float a, float b) { float a2 = mis1(a); return a2 / (a2 + mis1
 
 /* stableAtan2 / cartesianToSphericalUv / sphericalUvToCartesian, compat/common.h:137-150 */
 static float stable_atan2(float y, float x) {
-    return x == 0.0f ? (y == 0 ? 0 : (y < 0 ? -OR_PI / 2 : OR_PI / 2)) : atan2f(y, x);
+    return x == 0.0f ? (y == 0 ? 0 : (y < 0 ? -OR_PI / 2 : OR_PI / 2)) : om_atan2(y, x);
 }
 static void cart_to_uv(f3 v, float uv[2]) {
     float th = stable_atan2(v.z, v.x);
     uv[0] = th * OR_1_PI * 0.5f + 0.5f;
-    uv[1] = acosf(minf(maxf(v.y, -1.0f), 1.0f)) * OR_1_PI;
+    uv[1] = om_acos(minf(maxf(v.y, -1.0f), 1.0f)) * OR_1_PI;
 }
 static f3 uv_to_cart(float u, float v) {
     u = u * 2 - 1;
     u *= OR_PI; v *= OR_PI;
-    float sp = sinf(v);
-    return v3(sp * cosf(u), cosf(v), sp * sinf(u));
+    float sp = om_sin(v);
+    return v3(sp * om_cos(u), om_cos(v), sp * om_sin(u));
 }
 
 /* ------------------------------------------------------------- scene */
@@ -481,7 +609,9 @@ static f3 m_emission(const material_t* m) {
 #define M_ETA1(m) (un8((m)->p[3], 24) + 1)
 
 /* microfacet.hlsli:22-179 */
-static float powf5(float x) { return powf(x, 5.0f); }
+/* pow(x, 5) of disney.hlsli as (x^2)^2 x: within 3 ulp of the correctly
+ * rounded power, the same operations as the GPU library */
+static float powf5(float x) { const float x2 = x * x; return x2 * x2 * x; }
 static f3 schlick3(f3 F0, float c) {
     float k = powf5(maxf(1 - c, 0.0f));
     return v3(F0.x + (1 - F0.x) * k, F0.y + (1 - F0.y) * k, F0.z + (1 - F0.z) * k);
@@ -499,7 +629,7 @@ static float fresnel_dielectric(float ni, float eta) {
 }
 static float GTR1(float ndh, float a) {
     float a2 = a * a, t = 1 + (a2 - 1) * ndh * ndh;
-    return (a2 - 1) / (OR_PI * logf(a2) * t);
+    return (a2 - 1) / (OR_PI * om_log(a2) * t);
 }
 static float GTR2a(f3 h, float ax, float ay) {
     f3 hs = divv(h, v3(ax, ay, 1));
@@ -518,7 +648,7 @@ static f3 sample_vndf(f3 din, float ax, float ay, float r0, float r1) {
     if (flip) din = neg(din);
     f3 hin = normalize3(v3(ax * din.x, ay * din.y, din.z));
     float r = sqrtf(r0), phi = 2 * OR_PI * r1;
-    float t1 = r * cosf(phi), t2 = r * sinf(phi);
+    float t1 = r * om_cos(phi), t2 = r * om_sin(phi);
     float s = (1 + hin.z) / 2;
     t2 = (1 - s) * sqrtf(1 - t1 * t1) + s * t2;
     f3 dn = v3(t1, t2, sqrtf(maxf(0.0f, 1 - t1 * t1 - t2 * t2)));
@@ -544,7 +674,7 @@ static f3 disney_eval(const material_t* m, f3 din, f3 dout, int adj) {
         f3 c = v3(0, 0, 0);
         if (din.z >= 0 && dout.z >= 0) {
             float Fd90 = 0.5f + 2 * M_ROUGH(m) * hdout * hdout;
-            float sno = powf(1 - ndout, 5.0f), sni = powf(1 - ndin, 5.0f), sho = powf(1 - hdout, 5.0f);
+            float sno = powf5(1 - ndout), sni = powf5(1 - ndin), sho = powf5(1 - hdout);
             float based = (1 + (Fd90 - 1) * sno) * (1 + (Fd90 - 1) * sni);
             float Fss90 = hdout * hdout * M_ROUGH(m);
             float Fss = (1 + (Fss90 - 1) * sno) * (1 + (Fss90 - 1) * sni);
@@ -681,9 +811,9 @@ static dsr_t disney_sample(const material_t* m, f3 din, f3 rnd) {
             return r;
         } else {
             float a2 = ac * ac;
-            float ch = sqrtf(maxf(0.0f, (1 - powf(a2, 1 - rnd.x)) / (1 - a2)));
+            float ch = sqrtf(maxf(0.0f, (1 - om_pow(a2, 1 - rnd.x)) / (1 - a2)));
             float sh = sqrtf(maxf(1 - ch * ch, 0.0f)), az = 2 * OR_PI * rnd.y;
-            f3 mn = v3(sh * cosf(az), sh * sinf(az), ch);
+            f3 mn = v3(sh * om_cos(az), sh * om_sin(az), ch);
             r.dir = normalize3(add(neg(din), mulf(mn, 2 * dot(din, mn))));
             r.eta = 0;
             r.rough = sqrtf(ac);
@@ -830,12 +960,12 @@ static void sample_texel(const or_scene* s, float rx, float ry, float uv[2], flo
         cx *= L->w / lw; cy *= L->h / lh;
         float p[4] = {0, 0, 0, 0};
         if ((uint32_t)L->w - (uint32_t)cx > 1) {
-            float sy = sinf(OR_PI * ((float)cy + 0.5f) * inv_h);
+            float sy = om_sin(OR_PI * ((float)cy + 0.5f) * inv_h);
             p[0] = texel_lum(L, cx, cy) * sy;
             p[1] = texel_lum(L, cx + 1, cy) * sy;
         }
         if ((uint32_t)L->h - (uint32_t)cy > 1) {
-            float sy = sinf(OR_PI * ((float)cy + 1.5f) * inv_h);
+            float sy = om_sin(OR_PI * ((float)cy + 1.5f) * inv_h);
             p[2] = texel_lum(L, cx, cy + 1) * sy;
             p[3] = texel_lum(L, cx + 1, cy + 1) * sy;
         }
@@ -870,12 +1000,12 @@ static float sample_texel_pdf(const or_scene* s, float u, float v) {
         int ox = ux - cx < 1 ? ux - cx : 1, oy = uy - cy < 1 ? uy - cy : 1;
         float p[4] = {0, 0, 0, 0};
         if ((uint32_t)L->w - (uint32_t)cx > 1) {
-            float sy = sinf(OR_PI * ((float)cy + 0.5f) * inv_h);
+            float sy = om_sin(OR_PI * ((float)cy + 0.5f) * inv_h);
             p[0] = texel_lum(L, cx, cy) * sy;
             p[1] = texel_lum(L, cx + 1, cy) * sy;
         }
         if ((uint32_t)L->h - (uint32_t)cy > 1) {
-            float sy = sinf(OR_PI * ((float)cy + 1.5f) * inv_h);
+            float sy = om_sin(OR_PI * ((float)cy + 1.5f) * inv_h);
             p[2] = texel_lum(L, cx, cy + 1) * sy;
             p[3] = texel_lum(L, cx + 1, cy + 1) * sy;
         }
@@ -1182,7 +1312,9 @@ static int med_singular(const medium_t* m) { return fabsf(m->g) > 0.999f; }
 static float med_cont_prob(const medium_t* m) { return saturate(luminance(mul(med_albedo(m), m->density))); }
 /* Henyey-Greenstein evaluateReflectance / sampleDirection, medium.hlsli:44-78 */
 static float hg_eval(float g, f3 din, f3 dout) {
-    return 1 / (4 * OR_PI) * (1 - g * g) / powf(1 + g * g + 2 * g * dot(din, dout), 1.5f);
+    /* pow(d, 1.5) as d * sqrt(d), as the GPU library takes it */
+    const float d = 1 + g * g + 2 * g * dot(din, dout);
+    return 1 / (4 * OR_PI) * (1 - g * g) / (d * sqrtf(d));
 }
 static f3 hg_sample(float g, float r0, float r1, f3 din) {
     if (fabsf(g) < 1e-4f) return sample_uniform_sphere(r0, r1);
@@ -1193,14 +1325,14 @@ static f3 hg_sample(float g, float r0, float r1, f3 din) {
     const float az = 2 * OR_PI * r1;
     f3 fr[3];
     make_orthonormal(din, fr);
-    return add(add(mulf(fr[0], se * cosf(az)), mulf(fr[1], se * sinf(az))), mulf(din, ce));
+    return add(add(mulf(fr[0], se * om_cos(az)), mulf(fr[1], se * om_sin(az))), mulf(din, ce));
 }
 float or_hg_eval(float g, const float din[3], const float dout[3]) { return hg_eval(g, v3(din[0], din[1], din[2]), v3(dout[0], dout[1], dout[2])); }
 void or_hg_sample(float g, float r0, float r1, const float din[3], float out[3]) {
     const f3 r = hg_sample(g, r0, r1, v3(din[0], din[1], din[2]));
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
-static f3 expv(f3 a) { return v3(expf(a.x), expf(a.y), expf(a.z)); }
+static f3 expv(f3 a) { return v3(om_exp(a.x), om_exp(a.y), om_exp(a.z)); }
 static float avg3(f3 a) { return (a.x + a.y + a.z) / 3; }
 /* deltaTrack<Scattering>, medium.hlsli:104-181.  origin/direction in object space. */
 static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_t* rng, f3 origin, f3 dir, float tmax, f3* beta,
@@ -1217,16 +1349,16 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
         if (sc < 1e-6f) return v3(0, 0, 0);
         float r[4];
         rng_nextf(rng, r);
-        const float t = -logf(1 - r[0]) / sc;
+        const float t = -om_log(1 - r[0]) / sc;
         if (t < tmax && scattering) {
             *scattered = 1;
             *beta = mul(*beta, mul(mul(expv(mulf(neg(st), t)), st), med_albedo(m)));
-            const float e = expf(-sc * t) * sc;
+            const float e = om_exp(-sc * t) * sc;
             *dirPdf = mulf(*dirPdf, e);
             return add(origin, mulf(dir, t));
         }
         *beta = mul(*beta, expv(mulf(neg(st), tmax)));
-        *dirPdf = mulf(*dirPdf, expf(-sc * tmax));
+        *dirPdf = mulf(*dirPdf, om_exp(-sc * tmax));
         return v3(0, 0, 0);
     }
     const uint8_t* g = s->vol[m->dens_vol];
@@ -1240,7 +1372,7 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
     for (uint32_t it = 0; it < max_null && any_gt0(*beta); it++) {
         float r[4];
         rng_nextf(rng, r);
-        const float t = -logf(1 - r[0]) * invc;
+        const float t = -om_log(1 - r[0]) * invc;
         if (t >= tmax) {
             const f3 tr = expv(mulf(neg(maj), tmax));
             *beta = mul(*beta, tr);

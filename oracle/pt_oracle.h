@@ -86,6 +86,12 @@ void or_bsdf_sample(const uint32_t mat[5], int lambertian, const float rnd[3], c
 float or_nvdb_read(const void* grid, int i, int j, int k);
 void or_nvdb_world_to_index(const void* grid, const float p[3], int dir, float out[3]);
 
+/* The deterministic transcendentals (the same float operations as the GPU
+ * library) over n arguments: fn 0 sin(x), 1 cos(x), 2 atan2(x, y) (x the
+ * numerator, y != 0), 3 acos(x), 4 log(x), 5 exp(x), 6 pow(x, y) for x > 0;
+ * y may be null for the one-argument functions. */
+void or_math(int fn, uint32_t n, const float* x, const float* y, float* out);
+
 /* Henyey-Greenstein phase function, medium.hlsli:44-78. */
 float or_hg_eval(float g, const float din[3], const float dout[3]);
 void or_hg_sample(float g, float r0, float r1, const float din[3], float out[3]);

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "st_layout.h"
+#include "st_math.h"
 
 #define ST_PI 3.14159265358979323846f
 #define ST_1_PI (1.0f / ST_PI)
@@ -134,7 +135,7 @@ __device__ __forceinline__ void make_onb(v3 N, v3& r0, v3& r1) {
 // warps, common.hlsli:36-77
 __device__ __forceinline__ v3 sample_uniform_sphere(float u1, float u2) {
     float z = u2 * 2 - 1, r = sqrtf(fmax_(1 - z * z, 0)), phi = 2 * ST_PI * u1;
-    return mk(r * cosf(phi), r * sinf(phi), z);
+    return mk(r * cos_(phi), r * sin_(phi), z);
 }
 __device__ __forceinline__ void concentric_disc(float u1, float u2, float& ox, float& oy) {
     float ux = 2 * u1 - 1, uy = 2 * u2 - 1, th, r;
@@ -149,8 +150,8 @@ __device__ __forceinline__ void concentric_disc(float u1, float u2, float& ox, f
         r = uy;
         th = ST_PI / 2 - (ST_PI / 4) * (ux / uy);
     }
-    ox = r * cosf(th);
-    oy = r * sinf(th);
+    ox = r * cos_(th);
+    oy = r * sin_(th);
 }
 __device__ __forceinline__ v3 sample_cos_hemisphere(float u1, float u2) {
     float x, y;
@@ -166,19 +167,19 @@ __device__ __forceinline__ float mis2(float a, float b) {
 }
 // compat/common.h:137-150
 __device__ __forceinline__ float stable_atan2(float y, float x) {
-    return x == 0.0f ? (y == 0 ? 0 : (y < 0 ? -ST_PI / 2 : ST_PI / 2)) : atan2f(y, x);
+    return x == 0.0f ? (y == 0 ? 0 : (y < 0 ? -ST_PI / 2 : ST_PI / 2)) : atan2_(y, x);
 }
 __device__ __forceinline__ void cart_to_uv(v3 v, float& u, float& w) {
     float th = stable_atan2(v.z, v.x);
     u = th * ST_1_PI * 0.5f + 0.5f;
-    w = acosf(fmin_(fmax_(v.y, -1.0f), 1.0f)) * ST_1_PI;
+    w = acos_(fmin_(fmax_(v.y, -1.0f), 1.0f)) * ST_1_PI;
 }
 __device__ __forceinline__ v3 uv_to_cart(float u, float v) {
     u = u * 2 - 1;
     u *= ST_PI;
     v *= ST_PI;
-    float sp = sinf(v);
-    return mk(sp * cosf(u), cosf(v), sp * sinf(u));
+    float sp = sin_(v);
+    return mk(sp * cos_(u), cos_(v), sp * sin_(u));
 }
 
 // ---------------------------------------------------------- materials
@@ -222,7 +223,7 @@ __device__ __forceinline__ float fresnel_dielectric(float ni, float eta) {
 }
 __device__ __forceinline__ float gtr1(float ndh, float a) {
     float a2 = a * a, t = 1 + (a2 - 1) * ndh * ndh;
-    return (a2 - 1) / (ST_PI * logf(a2) * t);
+    return (a2 - 1) / (ST_PI * log_(a2) * t);
 }
 __device__ __forceinline__ float gtr2(v3 h, float ax, float ay) {
     v3 hs = h / mk(ax, ay, 1);
@@ -240,7 +241,7 @@ __device__ __forceinline__ v3 sample_vndf(v3 din, float ax, float ay, float r0, 
     if (flip) din = -din;
     v3 hin = normalize(mk(ax * din.x, ay * din.y, din.z));
     float r = sqrtf(r0), phi = 2 * ST_PI * r1;
-    float t1 = r * cosf(phi), t2 = r * sinf(phi);
+    float t1 = r * cos_(phi), t2 = r * sin_(phi);
     float s = (1 + hin.z) / 2;
     t2 = (1 - s) * sqrtf(1 - t1 * t1) + s * t2;
     v3 dn = mk(t1, t2, sqrtf(fmax_(0.0f, 1 - t1 * t1 - t2 * t2)));
@@ -413,9 +414,9 @@ __device__ inline Sample disney_sample(const Material& m, v3 din, float rx, floa
             return r;
         } else {
             float a2 = ac * ac;
-            float ch = sqrtf(fmax_(0.0f, (1 - powf(a2, 1 - rx)) / (1 - a2)));
+            float ch = sqrtf(fmax_(0.0f, (1 - pow_(a2, 1 - rx)) / (1 - a2)));
             float sh = sqrtf(fmax_(1 - ch * ch, 0.0f)), az = 2 * ST_PI * ry;
-            v3 mn = mk(sh * cosf(az), sh * sinf(az), ch);
+            v3 mn = mk(sh * cos_(az), sh * sin_(az), ch);
             r.dir = normalize(-din + mn * (2 * dot(din, mn)));
             r.rough = sqrtf(ac);
             return r;

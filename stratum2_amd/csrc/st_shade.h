@@ -197,12 +197,12 @@ __device__ inline void sample_texel(const DevScene& S, float rx, float ry, float
         cy *= H / lh;
         float p0 = 0, p1 = 0, p2 = 0, p3 = 0;
         if ((uint32_t)W - (uint32_t)cx > 1) {
-            const float sy = sinf(ST_PI * ((float)cy + 0.5f) * inv_h);
+            const float sy = sin_(ST_PI * ((float)cy + 0.5f) * inv_h);
             p0 = texel_lum(S, L, cx, cy) * sy;
             p1 = texel_lum(S, L, cx + 1, cy) * sy;
         }
         if ((uint32_t)H - (uint32_t)cy > 1) {
-            const float sy = sinf(ST_PI * ((float)cy + 1.5f) * inv_h);
+            const float sy = sin_(ST_PI * ((float)cy + 1.5f) * inv_h);
             p2 = texel_lum(S, L, cx, cy + 1) * sy;
             p3 = texel_lum(S, L, cx + 1, cy + 1) * sy;
         }
@@ -240,12 +240,12 @@ __device__ inline float sample_texel_pdf(const DevScene& S, float u, float v) {
         const int ox = ux - cx < 1 ? ux - cx : 1, oy = uy - cy < 1 ? uy - cy : 1;
         float p[4] = {0, 0, 0, 0};
         if ((uint32_t)W - (uint32_t)cx > 1) {
-            const float sy = sinf(ST_PI * ((float)cy + 0.5f) * inv_h);
+            const float sy = sin_(ST_PI * ((float)cy + 0.5f) * inv_h);
             p[0] = texel_lum(S, L, cx, cy) * sy;
             p[1] = texel_lum(S, L, cx + 1, cy) * sy;
         }
         if ((uint32_t)H - (uint32_t)cy > 1) {
-            const float sy = sinf(ST_PI * ((float)cy + 1.5f) * inv_h);
+            const float sy = sin_(ST_PI * ((float)cy + 1.5f) * inv_h);
             p[2] = texel_lum(S, L, cx, cy + 1) * sy;
             p[3] = texel_lum(S, L, cx + 1, cy + 1) * sy;
         }

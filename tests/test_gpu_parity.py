@@ -1,15 +1,15 @@
 """GPU parity: the HIP path against the CPU oracle on identical PCG streams.
 
-Bars (per the north-star contract):
+Bars (the north-star contract):
   * primary-hit instance|primitive ids: bit-exact;
   * raw closest-hit rays: ids bit-exact, t bit-exact (same watertight test);
   * any-hit occlusion: bit-exact;
-  * per-pixel radiance: |gpu - oracle| <= 1e-4 * max(|oracle|, 1e-3) for at
-    least 99% of pixel channels; the remaining pixels are paths whose sampled
-    directions differ by one ulp of a libm transcendental (GPU ocml vs glibc)
-    and so diverge after oct-snorm16 quantisation;
-  * per-sample path lengths equal for at least 99% of paths;
-  * mean radiance within 1% (no systematic bias).
+  * per-pixel radiance: |gpu - oracle| <= 1e-4 * max(|oracle|, 1e-3) for
+    every pixel channel (the GPU library and the oracle run the same float
+    operation sequences for their transcendentals, so the paths agree
+    vertex for vertex; the share of bit-identical channels is printed);
+  * per-sample path lengths: equal for every path;
+  * ray counts: equal.
 """
 import numpy as np
 import pytest
@@ -31,7 +31,7 @@ def _render_both(gpu_ctx, sd, view, params):
     return g, r
 
 
-def _check_image(g, r, frac_px=0.99, frac_len=0.99):
+def _check_image(g, r):
     vis = g["visibility"].view(np.uint32)
     assert np.array_equal(vis[..., 0], r["visibility"][..., 0]), "primary hit ids differ"
     assert np.array_equal(vis[..., 1], r["visibility"][..., 1]), "primary shading normals differ"
@@ -39,14 +39,14 @@ def _check_image(g, r, frac_px=0.99, frac_len=0.99):
     assert np.isfinite(hg).all()
     tol = 1e-4 * np.maximum(np.abs(hr), 1e-3)
     ok = np.abs(hg - hr) <= tol
-    print(f"pixel channels within 1e-4 rel: {ok.mean():.5f}; mean gpu {hg.mean():.6f} oracle {hr.mean():.6f}")
-    assert ok.mean() >= frac_px
+    rel = np.abs(hg - hr) / np.maximum(np.abs(hr), 1e-3)
+    print(f"pixel channels within 1e-4 rel: {ok.mean():.6f}, bit-identical {(hg == hr).mean():.6f}, "
+          f"largest relative difference {rel.max():.2e}; mean gpu {hg.mean():.6f} oracle {hr.mean():.6f}")
+    assert ok.all()
     pl = g["path_length"].view(np.uint32)
-    assert (pl == r["path_length"]).mean() >= frac_len
-    assert abs(hg.mean() - hr.mean()) <= 0.01 * abs(hr.mean()) + 1e-6
+    assert np.array_equal(pl, r["path_length"])
     rc = g["ray_counts"].view(np.uint64)
-    assert abs(int(rc[0]) - int(r["ray_counts"][0])) <= 0.01 * int(r["ray_counts"][0])
-    assert abs(int(rc[1]) - int(r["ray_counts"][1])) <= 0.01 * int(r["ray_counts"][1]) + 10
+    assert int(rc[0]) == int(r["ray_counts"][0]) and int(rc[1]) == int(r["ray_counts"][1])
 
 
 def test_cornell_disney_reference_mode(gpu_ctx):
@@ -87,7 +87,7 @@ def test_environment_scene(gpu_ctx):
     params = abi.default_params(W, H, max_depth=5, spp=2)
     view = scenes.make_view(W, H, position=(0.0, 3.0, 14.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
     g, r = _render_both(gpu_ctx, sd, view, params)
-    _check_image(g, r, frac_px=0.98, frac_len=0.98)
+    _check_image(g, r)
 
 
 def test_trace_rays_closest_and_any(gpu_ctx):
@@ -239,7 +239,7 @@ def test_restir_di_candidates(gpu_ctx):
     params = abi.default_params(W, H, max_depth=3, spp=2, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI)
     view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
     g, r = _render_both(gpu_ctx, sd, view, params)
-    _check_image(g, r, frac_px=0.98, frac_len=0.98)
+    _check_image(g, r)
 
 
 def _gpu_render(gpu_ctx, sd, view, params):
@@ -289,7 +289,7 @@ def test_headline_frame_modes_vs_oracle(gpu_ctx):
     W, H = 480, 272
     params = abi.default_params(W, H, max_depth=8, spp=8, flags=HEADLINE_FLAGS, rr_start_depth=3)
     g, r = _render_both(gpu_ctx, sd, scenes.spheres_view(W, H), params)
-    _check_image(g, r, frac_px=0.98, frac_len=0.98)
+    _check_image(g, r)
 
 
 def test_alias_env_and_emitters_vs_oracle(gpu_ctx):
@@ -299,7 +299,7 @@ def test_alias_env_and_emitters_vs_oracle(gpu_ctx):
     view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
     params = abi.default_params(W, H, max_depth=5, spp=4, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS)
     g, r = _render_both(gpu_ctx, sd, view, params)
-    _check_image(g, r, frac_px=0.98, frac_len=0.98)
+    _check_image(g, r)
 
 
 @pytest.mark.parametrize("rr", [2, 3])
@@ -314,11 +314,9 @@ def test_rr_start_vs_oracle(gpu_ctx, rr):
 
 
 # ------------------------------------------------------------------- media
-def _check_media_image(g, r, frac_vis=0.995, frac_px=0.98):
-    """Media add free-flight sampling (logf) and transmittance (expf) to every
-    segment, so one-ulp libm differences flip a scattering decision more
-    often than in surface-only scenes: primary ids are held to 99.5%, pixel
-    channels to 98% within 1e-4 relative, the image mean to 1%."""
+def _check_media_image(g, r):
+    """Media add free-flight sampling and transmittance to every segment;
+    with the shared transcendentals the bar is the same as on surfaces."""
     vis = g["visibility"].view(np.uint32)
     fv = (vis[..., 0] == r["visibility"][..., 0]).mean()
     hg, hr = g["hdr"][..., :3], r["hdr"][..., :3]
@@ -326,14 +324,14 @@ def _check_media_image(g, r, frac_vis=0.995, frac_px=0.98):
     ok = np.abs(hg - hr) <= 1e-4 * np.maximum(np.abs(hr), 1e-3)
     pl = (g["path_length"].view(np.uint32) == r["path_length"]).mean()
     rc = g["ray_counts"].view(np.uint64)
-    print(f"primary ids {fv:.5f}; pixel channels within 1e-4 rel {ok.mean():.5f}; path lengths {pl:.5f}; "
-          f"mean gpu {hg.mean():.6f} oracle {hr.mean():.6f}; rays gpu {rc.tolist()} oracle {r['ray_counts'].tolist()}")
-    assert fv >= frac_vis
-    assert ok.mean() >= frac_px
-    assert pl >= frac_px
-    assert abs(hg.mean() - hr.mean()) <= 0.01 * abs(hr.mean()) + 1e-6
+    print(f"primary ids {fv:.5f}; pixel channels within 1e-4 rel {ok.mean():.6f}, bit-identical {(hg == hr).mean():.6f}; "
+          f"path lengths {pl:.5f}; mean gpu {hg.mean():.6f} oracle {hr.mean():.6f}; rays gpu {rc.tolist()} "
+          f"oracle {r['ray_counts'].tolist()}")
+    assert fv == 1.0
+    assert ok.all()
+    assert pl == 1.0
     for j in range(2):
-        assert abs(int(rc[j]) - int(r["ray_counts"][j])) <= 0.01 * int(r["ray_counts"][j]) + 10
+        assert int(rc[j]) == int(r["ray_counts"][j])
 
 
 def test_media_homogeneous_box_in_cornell(gpu_ctx):
@@ -390,7 +388,7 @@ def test_auxiliary_outputs_and_cameras(gpu_ctx, ortho):
     vis = g["visibility"].view(np.uint32)
     same = vis[..., 0] == r["visibility"][..., 0]
     print(f"primary ids equal: {same.mean():.5f}")
-    assert same.mean() >= (0.995 if ortho else 1.0)
+    assert same.all()
     for k in ("albedo", "depth", "prev_uv"):
         a, b = g[k][same], r[k][same]
         fin = np.isfinite(b)
@@ -550,7 +548,7 @@ def test_disney_lobes_shading_normals_16bit_indices_and_transforms(gpu_ctx):
     W = H = 96
     params = abi.default_params(W, H, max_depth=5, spp=2)
     g, r = _render_both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
-    _check_image(g, r, frac_px=0.98, frac_len=0.98)
+    _check_image(g, r)
 
 
 def test_restir_reuse_independent_of_samples_per_wave(gpu_ctx):
@@ -606,7 +604,7 @@ def test_emitter_with_vertex_normals(gpu_ctx, light_trace):
         hg, hr = g["hdr"][..., :3], r["hdr"][..., :3]
         ok = np.abs(hg - hr) <= 1e-4 * np.maximum(np.abs(hr), 1e-3)
         print(f"pixel channels within 1e-4 rel: {ok.mean():.5f}")
-        assert ok.mean() >= 0.99 and abs(hg.mean() - hr.mean()) <= 0.01 * hr.mean()
+        assert ok.all()
     else:
         _check_image(g, r)
 
