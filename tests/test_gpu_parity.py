@@ -57,6 +57,17 @@ def test_cornell_disney_reference_mode(gpu_ctx):
     _check_image(g, r)
 
 
+def test_config0_cornell_256_4spp(gpu_ctx):
+    """The small configuration exactly: seeded Cornell box (24 + 5120
+    triangles), 256x256, 4 spp (seeds 0-3), maxPathLength 4, Disney
+    materials (the reference's parity mode)."""
+    sd = scenes.cornell_box(0)
+    W = H = 256
+    params = abi.default_params(W, H, max_depth=4, spp=4, seed=0)
+    g, r = _render_both(gpu_ctx, sd, scenes.cornell_view(W, H), params)
+    _check_image(g, r)
+
+
 def test_cornell_lambertian(gpu_ctx):
     sd = scenes.cornell_box(3)
     W = H = 96
@@ -347,6 +358,17 @@ def test_media_heterogeneous_cloud(gpu_ctx):
     sd = scenes.cloud(seed=2, n=64, env_size=(256, 128))
     W = H = 64
     params = abi.default_params(W, H, max_depth=8, spp=4)
+    g, r = _render_both(gpu_ctx, sd, scenes.cloud_view(W, H), params)
+    _check_media_image(g, r)
+
+
+def test_config2_cloud_256_grid_path_length_16(gpu_ctx):
+    """The volume configuration's scene exactly (256^3 fBm NanoVDB grid,
+    sigma_t scale 20, albedo 0.9, g 0.3, the 1024x512 sky alias-sampled) at
+    maxPathLength 16, on a 64x64 image at 2 spp."""
+    sd = scenes.cloud(seed=0)
+    W = H = 64
+    params = abi.default_params(W, H, max_depth=16, spp=2, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS)
     g, r = _render_both(gpu_ctx, sd, scenes.cloud_view(W, H), params)
     _check_media_image(g, r)
 
