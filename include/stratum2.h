@@ -256,7 +256,7 @@ int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closes
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
  * results as st_trace_rays.  Its traversal scratch (ray-fetch counter and the
- * HBM part of the per-lane stacks) is one fixed 16 MiB + 256 B device array
+ * HBM part of the per-lane stacks) is one fixed 16 MiB + 1 KiB device array
  * per GPU, held until the process ends; the traversal grid shrinks to what
  * fits it (fewer lanes for deeper trees), and a tree too deep for even one
  * block returns ST_ERROR_LIMIT, as do counts of 2^31 or more.  Calls on one

@@ -10,13 +10,25 @@
  * Parity status of each part (the check that pins it is in tests/):
  *   rng / hash / packing / f16 / rayOffset  pinned: hand-checked constants and
  *                                           closed-form properties (tests/test_oracle_units.py)
+ *   transcendentals (sin, cos, atan2, acos, pinned: float64 error bounds, 1-3.5 ulp
+ *   log, exp, pow)                          (tests/test_oracle_math.py)
  *   watertight triangle test + BVH          pinned: FP64 brute force over all triangles
  *   camera                                  pinned: closed-form pinhole invariants
- *   BSDFs                                   pinned: furnace / reciprocity / pdf integration
+ *   BSDFs                                   pinned: furnace / reciprocity / pdf integration; Disney
+ *                                           eval value by float64 GGX / Burley closed forms
+ *                                           (tests/test_oracle_disney.py)
  *   env mip-warp sampling                   pinned: sample pdf == SampleTexelPdf, integrates to 1
+ *   env alias-table sampling                pinned: table mass == float64 texel weights, chi-square
+ *                                           of sampled texels, in-texel uniformity
+ *                                           (tests/test_oracle_alias.py)
  *   bounce loop                             pinned: closed-form furnace scene (white Lambert
  *                                           sphere interior), emission-only analytic image
+ *   Russian roulette from path length k     pinned: identity beyond the maximum depth; same
+ *                                           expectation as roulette everywhere and never
+ *                                           (tests/test_oracle_headline_modes.py)
  *   ReSTIR DI RIS                           pinned: unbiased against one-sample NEE (test_oracle_units.py)
+ *   ReSTIR DI screen-space reuse            pinned: same expectation as RIS and one-sample NEE;
+ *                                           independent of thread count (test_oracle_headline_modes.py)
  *   NanoVDB read path, delta / ratio        pinned: the reference's PNanoVDB compiled into oracle/_ref,
  *   tracking, Henyey-Greenstein             chord transmittance, furnace, HG pdf (test_oracle_media.py)
  *   light tracing, adjoint BSDF             pinned: estimates the camera-path image, adjoint

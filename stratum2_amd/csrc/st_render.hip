@@ -65,7 +65,13 @@ struct HipError : std::runtime_error {
 
 constexpr int kTraceBlock = 128;
 constexpr int kShadeBlock = 256;
-constexpr int kTraceBlocksPerCU = 16;  // 32 wavefronts per CU (7-8 per SIMD at <= 72 VGPRs), 8 KB LDS each
+constexpr int kTraceBlocksPerCU = 16;
+// WaveState::counters (kCounterWords words): [0..1] queue counts (ping-pong),
+// [2] shadow count, [4..7] u64 ray counts, then the traversal fetch cursors:
+// kXcds per traversal kernel, kFetchStride words (one 128-byte line) apart
+constexpr int kFetchClosest = 256, kFetchShadow = kFetchClosest + (int)(kXcds * kFetchStride),
+              kFetchRays = kFetchShadow + (int)(kXcds * kFetchStride);
+constexpr int kCounterWords = kFetchRays + (int)(kXcds * kFetchStride);  // 32 wavefronts per CU (7-8 per SIMD at <= 72 VGPRs), 8 KB LDS each
 
 // ----------------------------------------------------------------- device state
 struct WaveState {
@@ -239,8 +245,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveStat
         w.counters[0] = n;
         w.counters[1] = 0;
         w.counters[2] = 0;
-        w.counters[16] = 0;
-        w.counters[17] = 0;
+        for (int i = 0; i < 2 * (int)kXcds; i++) w.counters[kFetchClosest + i * kFetchStride] = 0;
     }
 }
 
@@ -264,7 +269,7 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     ClosestSrc src{w, w.queue[qsel]};
     TraverseStats ts{0, 0};
-    trace_queue<false, COUNT>(S, w.counters[qsel], &w.counters[16], src, lds, spill, ts);
+    trace_queue<false, COUNT>(S, w.counters[qsel], &w.counters[kFetchClosest], src, lds, spill, ts);
     if (COUNT) {
         atomicAdd(&stats[0], (unsigned long long)ts.nodes);
         atomicAdd(&stats[1], (unsigned long long)ts.tris);
@@ -336,7 +341,7 @@ __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_e
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     ShadowSrc src{w, flags};
     TraverseStats ts{0, 0};
-    trace_queue<true, COUNT>(S, w.counters[2], &w.counters[17], src, lds, spill, ts);
+    trace_queue<true, COUNT>(S, w.counters[2], &w.counters[kFetchShadow], src, lds, spill, ts);
     if (COUNT) {
         atomicAdd(&stats[2], (unsigned long long)ts.nodes);
         atomicAdd(&stats[3], (unsigned long long)ts.tris);
@@ -1134,10 +1139,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_light_init(DevScene S, RenderCo
         const uint32_t qi = wave_append(&w.counters[0], live);
         if (live) w.queue[0][qi] = p;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        w.counters[16] = 0;
-        w.counters[17] = 0;
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int i = 0; i < 2 * (int)kXcds; i++) w.counters[kFetchClosest + i * kFetchStride] = 0;
 }
 
 // Sub-path MIS quantities after a BSDF sample (ProcessInteraction,
@@ -1616,8 +1619,7 @@ __global__ void k_count(WaveState w, int qsel, int media) {
     }
     w.counters[2] = 0;
     w.counters[qsel] = 0;
-    w.counters[16] = 0;  // traversal fetch cursors
-    w.counters[17] = 0;
+    for (int i = 0; i < 2 * (int)kXcds; i++) w.counters[kFetchClosest + i * kFetchStride] = 0;  // traversal fetch cursors
 }
 
 struct RaySrc {
@@ -1802,7 +1804,7 @@ constexpr uint32_t kWavePaths = 2u << 20;
 // share it.  Host code never passes `scratch` itself to a kernel (on the host
 // it names a stand-in, not device memory): it uses the device address that
 // hipGetSymbolAddress returns, kept per device in g_scratch.
-constexpr size_t kScratchHeader = 256;
+constexpr size_t kScratchHeader = 4 * kXcds * kFetchStride;  // the ray-fetch cursors, one 128-byte line each
 constexpr size_t kScratchSpill = 16u << 20;
 constexpr size_t kScratchBytes = kScratchHeader + kScratchSpill;
 static __device__ __align__(256) uint8_t scratch[kScratchBytes];
@@ -2571,7 +2573,7 @@ static void reserve_wave(WaveBufs& b, uint64_t paths, bool reuse, uint64_t npix_
     b.q1.reserve(paths * 4);
     b.shadow.reserve(paths * 48);
     b.med.reserve(paths * 4);
-    b.counters.reserve(256);
+    b.counters.reserve(4 * kCounterWords);
     if (reuse) {
         b.pri.reserve(npix_k * 16);
         b.psd.reserve(npix_k * 32);
@@ -2682,7 +2684,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         hipStream_t s0 = ctx->stream;
         HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, s0));
         HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
-        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 4 * kCounterWords, s0));
         HIP_CHECK(hipEventRecord(ctx->start_done, s0));
         for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
         begin_timings(ctx);
@@ -2837,7 +2839,7 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         hipStream_t s0 = ctx->stream;
         HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, s0));
         HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
-        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 4 * kCounterWords, s0));
         HIP_CHECK(hipEventRecord(ctx->start_done, s0));
         for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
         RenderConsts c;
@@ -2969,7 +2971,7 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
         hipStream_t s0 = ctx->stream;
         HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, s0));
         HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
-        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 256, s0));
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 4 * kCounterWords, s0));
         HIP_CHECK(hipEventRecord(ctx->start_done, s0));
         for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
         begin_timings(ctx);
@@ -3096,10 +3098,10 @@ int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, u
         const int grid = std::min<int>((count + kTraceBlock - 1) / kTraceBlock, ctx->cus * kTraceBlocksPerCU);
         WaveBufs& B = ctx->slot[0];
         ensure_spill(B, ctx->cus * kTraceBlocksPerCU, kTraceBlock);
-        B.counters.reserve(256);
-        HIP_CHECK(hipMemsetAsync(B.counters.as<uint8_t>() + 4 * 18, 0, 4, ctx->stream));
+        B.counters.reserve(4 * kCounterWords);
+        HIP_CHECK(hipMemsetAsync(B.counters.as<uint32_t>() + kFetchRays, 0, 4 * kXcds * kFetchStride, ctx->stream));
         hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
-                           closest, hits, B.spill.as<uint32_t>(), B.counters.as<uint32_t>() + 18);
+                           closest, hits, B.spill.as<uint32_t>(), B.counters.as<uint32_t>() + kFetchRays);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         return ST_OK;
@@ -3129,7 +3131,7 @@ int st_trace_rays_noalloc(st_scene* s, const float* rays, uint32_t count, int cl
         uint32_t* fetch = reinterpret_cast<uint32_t*>(ctx->scratch);
         uint32_t* spill = reinterpret_cast<uint32_t*>(ctx->scratch + kScratchHeader);
         // the fetch counter starts at zero (a fill kernel: also right inside a captured graph)
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, fetch, 1u, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, fetch, kXcds * kFetchStride, 0u);
         hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
                            closest, hits, spill, fetch);
         HIP_CHECK(hipGetLastError());

@@ -227,14 +227,31 @@ __device__ __forceinline__ Hit traverse(const DevScene& S, v3 o, v3 d, float tma
 }
 
 // Persistent traversal over a queue of n rays with per-lane dynamic fetch:
-// a lane whose ray is finished takes the next ray index from *fetch (one
-// atomic per wavefront per refill), so lanes do not idle behind the slowest
-// ray of their wavefront.  Src supplies load(i, o, d, tmax) and done(i, hit).
+// a lane whose ray is finished takes the next ray index (one atomic per
+// wavefront per refill), so lanes do not idle behind the slowest ray of their
+// wavefront.  Src supplies load(i, o, d, tmax) and done(i, hit).
+//
+// XCD-aware: workgroups are dispatched to the XCDs round robin, so block b
+// runs on XCD b % kXcds.  The queue is dealt out in chunks of kChunk rays,
+// chunk c to XCD c % kXcds, and each XCD has its own cursor over its chunks
+// (fetch[x * kFetchStride], one 128-byte line each).  Rays that are
+// neighbours in the queue (neighbouring pixels, for camera rays) thus meet in
+// one XCD's L2, all XCDs still sweep the queue together (the chunks of one
+// round are spread over the whole machine), and the ray-fetch atomics spread
+// over eight addresses instead of one.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kFetchStride = 32;  // words between the XCDs' cursors
+constexpr uint32_t kChunk = 2048;
 template <bool ANY, bool COUNT, class Src>
 __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint32_t* fetch, Src& src, uint32_t* lds,
                                             uint32_t* spill, TraverseStats& st) {
     const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane_id = __lane_id();
+    const uint32_t home = blockIdx.x % kXcds;
+    // rays dealt to this XCD: whole rounds of kXcds chunks, then its share of the last round
+    const uint32_t round = kChunk * kXcds, rem = n % round;
+    const uint32_t mine = (n / round) * kChunk + (rem > home * kChunk ? min(rem - home * kChunk, kChunk) : 0u);
+    uint32_t* cursor = fetch + home * kFetchStride;
     Lane L;
     bool active = false, drained = false;
     uint32_t idx = 0;
@@ -245,12 +262,13 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
                 const uint32_t cnt = (uint32_t)__popcll(need);
                 const int leader = __ffsll((long long)need) - 1;
                 uint32_t base = 0;
-                if ((int)lane_id == leader) base = atomicAdd(fetch, cnt);
+                if ((int)lane_id == leader) base = atomicAdd(cursor, cnt);
                 base = __shfl(base, leader);
-                if (base + cnt >= n) drained = true;
+                if ((uint64_t)base + cnt >= mine) drained = true;
                 if (!active) {
-                    const uint32_t i = base + (uint32_t)__popcll(need & ((1ull << lane_id) - 1ull));
-                    if (i < n) {
+                    const uint32_t j = base + (uint32_t)__popcll(need & ((1ull << lane_id) - 1ull));
+                    if (j < mine) {
+                        const uint32_t i = ((j / kChunk) * kXcds + home) * kChunk + j % kChunk;
                         v3 o, d;
                         float tmax;
                         src.load(i, o, d, tmax);
