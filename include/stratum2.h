@@ -265,6 +265,51 @@ int st_trace_rays(st_scene* scene, const float* rays, uint32_t count, int closes
 int st_trace_rays_noalloc(st_scene* scene, const float* rays, uint32_t count, int closest,
                           uint32_t* hits);
 
+/* ---- Batched rendering (no reference counterpart: the reference renders one
+ * view per frame from a host-driven dispatch loop, TestRenderer.cpp:613-621).
+ * A batch renders many requests, each exactly as one st_render call with the
+ * same arguments would (same images, bit for bit), in one hipGraph launch:
+ * the requests are spread over the context's streams, every launch of every
+ * request is captured once into a graph at the first st_batch_render, and
+ * later calls replay it.  Camera paths only (light tracing and the
+ * bidirectional mode return ST_ERROR_UNSUPPORTED).  Requests' scenes must
+ * belong to the context; outputs are device pointers, written on every
+ * render; ray_counts, when given, receive the request's rays.  Like
+ * st_render, a call waits for work on the null stream first and returns when
+ * the images are complete.  Calls on one context must be serialised.  A
+ * scene moved by st_scene_update is captured again at the next render;
+ * destroy a batch before the scenes it renders. */
+typedef struct st_render_request {
+    st_scene* scene;
+    st_view view;
+    st_render_params params;
+    st_outputs outputs;
+} st_render_request;
+
+/* Launch shapes of a batch (st_batch_set_config; defaults: 16 trace and 8
+ * shade blocks per CU, 2^21 paths per wave, 4 streams). */
+typedef struct st_batch_config {
+    uint32_t trace_blocks_per_cu;  /* traversal grid: blocks of 128 lanes per CU (1-16) */
+    uint32_t shade_blocks_per_cu;  /* shading grid: blocks of 256 lanes per CU (1-32) */
+    uint32_t wave_paths;           /* paths per wave when samples_per_wave is 0 (1 to 2^26) */
+    uint32_t streams;              /* streams the requests are spread over (1-4) */
+    double ms;                     /* reserved (0) */
+} st_batch_config;
+
+typedef struct st_batch st_batch;
+int st_batch_create(st_context* ctx, const st_render_request* requests, uint32_t count, st_batch** batch);
+int st_batch_render(st_batch* batch);
+/* Switched off: returns ST_ERROR_UNSUPPORTED (its timed search over
+ * configurations aborted the process on the host with no cause found). */
+int st_batch_autotune(st_batch* batch, uint32_t repeats, st_batch_config* chosen);
+int st_batch_get_config(const st_batch* batch, st_batch_config* out);
+/* Changes the launch shapes; the next render captures again.  Images do
+ * not depend on the configuration. */
+int st_batch_set_config(st_batch* batch, const st_batch_config* config);
+int st_batch_destroy(st_batch* batch);
+/* create + render + destroy */
+int st_render_batch(st_context* ctx, const st_render_request* requests, uint32_t count);
+
 /* Device time of the last st_render per kernel class (HIP events on the
  * context stream) and traversal statistics for the algorithmic-bytes roofline:
  * out[0..2]  ms in closest-hit trace, shade (k_shade alone: k_ris and the reuse

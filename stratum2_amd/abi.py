@@ -105,6 +105,15 @@ class st_outputs(C.Structure):
                 ("ray_counts", C.c_void_p), ("albedo", C.c_void_p), ("depth", C.c_void_p), ("prev_uv", C.c_void_p)]
 
 
+class st_render_request(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("view", st_view), ("params", st_render_params), ("outputs", st_outputs)]
+
+
+class st_batch_config(C.Structure):
+    _fields_ = [("trace_blocks_per_cu", u32), ("shade_blocks_per_cu", u32), ("wave_paths", u32), ("streams", u32),
+                ("ms", C.c_double)]
+
+
 def default_params(width, height, **kw):
     """TestRenderer defaults (src/App/TestRenderer.cpp:38-43)."""
     p = st_render_params()
@@ -186,6 +195,8 @@ EXPORTS = [
     "st_trace_rays", "st_trace_rays_noalloc", "st_last_timings", "st_last_kernel_times", "st_tonemap", "st_scene_update",
     "st_denoiser_create", "st_denoiser_destroy", "st_denoiser_reset", "st_denoiser_accumulated_frames", "st_denoise",
     "st_denoiser_last_timings",
+    "st_batch_create", "st_batch_render", "st_batch_autotune", "st_batch_get_config", "st_batch_set_config",
+    "st_batch_destroy", "st_render_batch",
 ]
 
 _lib = None
@@ -223,6 +234,13 @@ def load_library(path=LIB_PATH):
     lib.st_denoiser_accumulated_frames.restype = u32
     lib.st_denoise.argtypes = [vp, C.POINTER(st_view), C.POINTER(st_denoise_inputs), C.POINTER(st_denoise_params), vp]
     lib.st_denoiser_last_timings.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.st_batch_create.argtypes = [vp, C.POINTER(st_render_request), u32, C.POINTER(vp)]
+    lib.st_batch_render.argtypes = [vp]
+    lib.st_batch_autotune.argtypes = [vp, u32, C.POINTER(st_batch_config)]
+    lib.st_batch_get_config.argtypes = [vp, C.POINTER(st_batch_config)]
+    lib.st_batch_set_config.argtypes = [vp, C.POINTER(st_batch_config)]
+    lib.st_batch_destroy.argtypes = [vp]
+    lib.st_render_batch.argtypes = [vp, C.POINTER(st_render_request), u32]
     if path == LIB_PATH:
         _lib = lib
     return lib

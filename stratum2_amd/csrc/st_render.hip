@@ -1845,6 +1845,7 @@ struct st_scene {
     std::vector<st_volume_info> volume_info;
     std::vector<st_transform> inv_xf;
     uint64_t stats[4] = {0, 0, 0, 0};
+    uint64_t generation = 0;  // bumped by st_scene_update: device pointers and constants may have changed
     // for scene updates: per-instance object-space bounds (the scene sphere),
     // the flattened triangles' owners, and the BVH buffers (replaced on update)
     struct InstBox {
@@ -1904,14 +1905,17 @@ hipEvent_t get_event(st_context* ctx, int kind, bool start) {
     return start ? v[i].first : v[i].second;
 }
 
+// HIP events around one launch on its stream; a null context times nothing
+// (work being captured into a graph).
 struct Timed {
     st_context* ctx;
     int kind;
     hipStream_t stream;
     Timed(st_context* c, int k, hipStream_t s) : ctx(c), kind(k), stream(s) {
-        HIP_CHECK(hipEventRecord(get_event(c, k, true), s));
+        if (ctx) HIP_CHECK(hipEventRecord(get_event(c, k, true), s));
     }
     ~Timed() {
+        if (!ctx) return;
         (void)hipEventRecord(get_event(ctx, kind, false), stream);
         ctx->ev_used[kind]++;
     }
@@ -2479,6 +2483,7 @@ int st_scene_update(st_scene* s, const st_transform* transforms, double* build_m
     st_context* ctx = s->ctx;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
+        s->generation++;
         DevScene& D = s->d;
         const uint32_t ni = D.inst_count;
         // transforms, their inverses and the scene sphere
@@ -2647,11 +2652,9 @@ static void finish_timings(st_context* ctx, int slots, uint64_t* ray_counts) {
 static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o);
 static int render_vc(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o);
 
-int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
-    if (!s || !view || !p || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
-    if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
-    if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, o);
-    if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, o);
+// Argument checks of a camera-path render (the path tracer of
+// TestRenderer::render); ST_OK when it can run.
+static int check_camera(st_scene* s, const st_render_params* p) {
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && !(p->flags & ST_FLAG_RESTIR_DI))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && (p->spatial_neighbours > 64 || !(p->spatial_radius >= 0) || p->spatial_radius > 4096))
@@ -2659,153 +2662,492 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
     if ((p->flags & ST_FLAG_RESTIR_DI) && (p->candidate_samples == 0 || p->candidate_samples > 1024))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "candidate_samples must be in [1, 1024]");
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
-    const bool media = s->d.vinst_count > 0;
     if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
-    if (media && (p->flags & ST_FLAG_RESTIR_DI))
+    if (s->d.vinst_count > 0 && (p->flags & ST_FLAG_RESTIR_DI))
         return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI is not supported in scenes with media");
+    return ST_OK;
+}
+
+// Samples per wave of a camera-path render: params.samples_per_wave, or as
+// many as fill about kWavePaths paths; 0 when a wave would hold 2^31 paths.
+static uint32_t wave_samples(const st_render_params* p, uint32_t want) {
+    const uint32_t npix = p->width * p->height;
+    uint32_t k = want;
+    if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, kWavePaths / npix));
+    k = std::min(k, p->spp);
+    return (uint64_t)npix * k >= (1ull << 31) ? 0 : k;
+}
+
+// One camera-path render (TestRenderer::render's dispatches, testrenderer.slang:
+// 1087-1183) enqueued on given wave slots and accumulators.  st_render runs
+// it on the context's four slots; st_batch_render on one slot per request,
+// inside a captured graph.  Clears are kernels, not memsets, so the sequence
+// can be captured.
+struct CameraJob {
+    st_scene* s;
+    st_view view;
+    st_render_params p;
+    st_outputs o;
+    WaveBufs* slot[kWaveSlots];      // wave slots: buffers and streams
+    uint32_t* counters[kWaveSlots];  // the slots' counter blocks (kCounterWords words)
+    int slots;
+    uint32_t k;                      // samples per wave
+    float4* acc;                     // per pixel: the sum over samples
+    float4* rprev;                   // per pixel: final reservoirs of the last sample (reuse)
+    unsigned long long* stats;       // traversal statistics (COUNT_TRAVERSAL)
+    int tgrid, sgrid;                // trace and shade grids
+    bool timed;                      // HIP-event timings of the kernel classes
+    bool store_rays;                 // write the ray counts to o.ray_counts on the device
+};
+
+__global__ void k_store_rays(const uint32_t* counters, unsigned long long* out) {
+    if (threadIdx.x < 2) out[threadIdx.x] = reinterpret_cast<const unsigned long long*>(counters + 4)[threadIdx.x];
+}
+
+static void enqueue_camera(st_context* ctx, const CameraJob& j) {
+    st_scene* s = j.s;
+    const st_render_params* p = &j.p;
+    const st_outputs* o = &j.o;
+    const uint32_t npix = p->width * p->height, k = j.k;
+    const bool reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
+    const bool media = s->d.vinst_count > 0;
+    const int slots = j.slots, tgrid = j.tgrid, sgrid = j.sgrid;
+    hipStream_t s0 = j.slot[0]->stream;
+    const int fgrid = std::min<int>((int)((npix * 4 + 255) / 256), ctx->cus * 8);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fgrid), dim3(256), 0, s0, reinterpret_cast<uint32_t*>(j.acc), npix * 4, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s0, reinterpret_cast<uint32_t*>(j.stats), 16u, 0u);
+    for (int sl = 0; sl < slots; sl++)
+        hipLaunchKernelGGL(k_fill_u32, dim3(kCounterWords / 256), dim3(256), 0, s0, j.counters[sl], (uint32_t)kCounterWords, 0u);
+    if (slots > 1) {
+        HIP_CHECK(hipEventRecord(ctx->start_done, s0));
+        for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(j.slot[sl]->stream, ctx->start_done, 0));
+    }
+    RenderConsts c;
+    std::memset(&c, 0, sizeof c);
+    c.W = p->width;
+    c.H = p->height;
+    c.npix = npix;
+    c.seed = p->seed;
+    c.max_depth = p->max_depth;
+    c.min_len = p->min_path_length;
+    c.flags = p->flags;
+    c.rr_start = p->rr_start_depth;
+    c.env_prob = p->environment_sample_probability;
+    c.view = j.view;
+    c.inv_cam = inverse_transform(j.view.camera_to_world);
+    c.max_null = p->max_null_collisions;
+    // the camera's medium: the last volume whose object-space bounds hold the
+    // camera position (TestRenderer.cpp:431-446)
+    c.view_medium = kNoMedium;
+    {
+        const st_transform& T = j.view.camera_to_world;
+        const float cam[3] = {T.m[0][3], T.m[1][3], T.m[2][3]};
+        for (const st_volume_info& vi : s->volume_info) {
+            if (vi.instance_index >= s->d.inst_count) continue;
+            const st_transform& iT = s->inv_xf[vi.instance_index];
+            float lp[3];
+            for (int r = 0; r < 3; r++) lp[r] = iT.m[r][0] * cam[0] + iT.m[r][1] * cam[1] + iT.m[r][2] * cam[2] + iT.m[r][3];
+            if (lp[0] >= vi.min[0] && lp[1] >= vi.min[1] && lp[2] >= vi.min[2] && lp[0] <= vi.max[0] &&
+                lp[1] <= vi.max[1] && lp[2] <= vi.max[2])
+                c.view_medium = vi.instance_index;
+        }
+    }
+    c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
+    const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+    const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
+    c.cand = p->candidate_samples;
+    c.max_m = p->max_m;
+    c.spatial = p->spatial_neighbours;
+    c.radius = p->spatial_radius;
+    const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
+    unsigned long long* stats = j.stats;
+    // Wave i runs in slot i % slots on that slot's stream; the slot's
+    // previous wave (i - slots) finished before it on the same stream.  Across slots,
+    // two orders matter: the screen-space reuse of wave i reads the final
+    // reservoirs of wave i - 1, and resolves add samples in sample order.
+    uint32_t wi = 0;
+    for (uint32_t base = 0; base < p->spp; base += k, wi++) {
+        WaveBufs& B = *j.slot[wi % slots];
+        WaveBufs& prev = *j.slot[(wi + slots - 1) % slots];
+        hipStream_t sm = B.stream;
+        WaveState w = wave_state(B, ctx);
+        w.counters = j.counters[wi % slots];
+        w.r_prev = j.rprev;
+        uint32_t* spill = B.spill.as<uint32_t>();
+        c.k = std::min(k, p->spp - base);
+        c.sample_base = base;
+        c.wave_paths = npix * c.k;
+        const int cgrid = std::min<int>((c.wave_paths + kShadeBlock - 1) / kShadeBlock, ctx->cus * 16);
+        hipLaunchKernelGGL(k_camera, dim3(cgrid), dim3(kShadeBlock), 0, sm, c, w);
+        int qsel = 0;
+        for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
+            {
+                Timed tm(j.timed ? ctx : nullptr, 0, sm);
+                if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats);
+                else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+            }
+            // with screen-space reuse, k_ris runs the primary vertices' RIS
+            // first and every vertex after takes plain NEE: the shading
+            // kernel without the candidate loop
+            if (reuse && b == 0) {
+                Timed tm(j.timed ? ctx : nullptr, 3, sm);
+                auto kr = lambert ? (alias ? k_ris<true, true> : k_ris<true, false>)
+                                  : (alias ? k_ris<false, true> : k_ris<false, false>);
+                hipLaunchKernelGGL(kr, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
+            }
+            {
+                Timed tm(j.timed ? ctx : nullptr, 1, sm);
+                AuxOut aux{nullptr, nullptr, nullptr, nullptr};
+                if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
+                auto* kfn = shade_kernel(lambert, alias, restir && !reuse, media);
+                hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0);
+            }
+            if (reuse && b == 0) {
+                const int rgrid2 = std::min<int>((npix + kShadeBlock - 1) / kShadeBlock, ctx->cus * 8);
+                auto kt = lambert ? (alias ? k_restir_temporal<true, true> : k_restir_temporal<true, false>)
+                                  : (alias ? k_restir_temporal<false, true> : k_restir_temporal<false, false>);
+                auto ks = lambert ? (alias ? k_restir_spatial<true, true> : k_restir_spatial<true, false>)
+                                  : (alias ? k_restir_spatial<false, true> : k_restir_spatial<false, false>);
+                if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.reuse_done, 0));
+                // temporal reuse chains consecutive samples: sample j of the wave
+                // reuses sample j - 1's final reservoirs
+                for (uint32_t jj = 0; jj < c.k; jj++) {
+                    Timed tm(j.timed ? ctx : nullptr, 4, sm);
+                    hipLaunchKernelGGL(kt, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, (base + jj) > 0 ? 1 : 0, jj);
+                    hipLaunchKernelGGL(ks, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, jj);
+                }
+                if (slots > 1) HIP_CHECK(hipEventRecord(B.reuse_done, sm));
+            }
+            {
+                Timed tm(j.timed ? ctx : nullptr, 2, sm);
+                if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats);
+                else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
+                else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
+            }
+            hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
+            qsel ^= 1;
+        }
+        const bool last = base + c.k >= p->spp;
+        const int rgrid = std::min<int>((npix + 255) / 256, ctx->cus * 8);
+        if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.resolve_done, 0));
+        hipLaunchKernelGGL(k_resolve, dim3(rgrid), dim3(256), 0, sm, c, w, j.acc, o->hdr, o->path_length,
+                           1.0f / (float)p->spp, last ? 1 : 0);
+        if (slots > 1) HIP_CHECK(hipEventRecord(B.resolve_done, sm));
+        HIP_CHECK(hipGetLastError());
+    }
+    if (j.store_rays && o->ray_counts)  // one slot: its counters hold every wave's rays
+        hipLaunchKernelGGL(k_store_rays, dim3(1), dim3(64), 0, s0, j.counters[0], (unsigned long long*)o->ray_counts);
+}
+
+int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
+    if (!s || !view || !p || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
+    if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, o);
+    if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, o);
+    const int rc = check_camera(s, p);
+    if (rc != ST_OK) return rc;
     st_context* ctx = s->ctx;
     try {
         HIP_CHECK(hipSetDevice(ctx->device));
-        order_after_caller(ctx);
         const uint32_t npix = p->width * p->height;
         const bool reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
-        uint32_t k = p->samples_per_wave;
-        if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p->spp, kWavePaths / npix));
-        k = std::min(k, p->spp);
+        const uint32_t k = wave_samples(p, p->samples_per_wave);
+        if (k == 0) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
         const uint64_t paths = (uint64_t)npix * k;
-        if (paths >= (1ull << 31)) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
         const uint32_t waves = (p->spp + k - 1) / k;
         const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) ? 1 : (int)std::min<uint32_t>(kWaveSlots, waves);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
+        order_after_caller(ctx);
         for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, reuse, paths, tgrid);
         ctx->acc.reserve((size_t)npix * 16);
         if (reuse) ctx->rprev.reserve((size_t)npix * 32);
         ctx->stats.reserve(64);
-        hipStream_t s0 = ctx->stream;
-        HIP_CHECK(hipMemsetAsync(ctx->acc.p, 0, (size_t)npix * 16, s0));
-        HIP_CHECK(hipMemsetAsync(ctx->stats.p, 0, 64, s0));
-        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipMemsetAsync(ctx->slot[sl].counters.p, 0, 4 * kCounterWords, s0));
-        HIP_CHECK(hipEventRecord(ctx->start_done, s0));
-        for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
         begin_timings(ctx);
-        RenderConsts c;
-        std::memset(&c, 0, sizeof c);
-        c.W = p->width;
-        c.H = p->height;
-        c.npix = npix;
-        c.seed = p->seed;
-        c.max_depth = p->max_depth;
-        c.min_len = p->min_path_length;
-        c.flags = p->flags;
-        c.rr_start = p->rr_start_depth;
-        c.env_prob = p->environment_sample_probability;
-        c.view = *view;
-        c.inv_cam = inverse_transform(view->camera_to_world);
-        c.max_null = p->max_null_collisions;
-        // the camera's medium: the last volume whose object-space bounds hold the
-        // camera position (TestRenderer.cpp:431-446)
-        c.view_medium = kNoMedium;
-        {
-            const st_transform& T = view->camera_to_world;
-            const float cam[3] = {T.m[0][3], T.m[1][3], T.m[2][3]};
-            for (const st_volume_info& vi : s->volume_info) {
-                if (vi.instance_index >= s->d.inst_count) continue;
-                const st_transform& iT = s->inv_xf[vi.instance_index];
-                float lp[3];
-                for (int r = 0; r < 3; r++) lp[r] = iT.m[r][0] * cam[0] + iT.m[r][1] * cam[1] + iT.m[r][2] * cam[2] + iT.m[r][3];
-                if (lp[0] >= vi.min[0] && lp[1] >= vi.min[1] && lp[2] >= vi.min[2] && lp[0] <= vi.max[0] &&
-                    lp[1] <= vi.max[1] && lp[2] <= vi.max[2])
-                    c.view_medium = vi.instance_index;
-            }
+        CameraJob j;
+        std::memset(&j, 0, sizeof j);
+        j.s = s;
+        j.view = *view;
+        j.p = *p;
+        j.o = *o;
+        for (int sl = 0; sl < slots; sl++) {
+            j.slot[sl] = &ctx->slot[sl];
+            j.counters[sl] = ctx->slot[sl].counters.as<uint32_t>();
         }
-        c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
-        const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
-        const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
-        c.cand = p->candidate_samples;
-        c.max_m = p->max_m;
-        c.spatial = p->spatial_neighbours;
-        c.radius = p->spatial_radius;
-        const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
-        auto* stats = ctx->stats.as<unsigned long long>();
-        // Wave i runs in slot i % slots on that slot's stream; the slot's
-        // previous wave (i - slots) finished before it on the same stream.  Across slots,
-        // two orders matter: the screen-space reuse of wave i reads the final
-        // reservoirs of wave i - 1, and resolves add samples in sample order.
-        uint32_t wi = 0;
-        for (uint32_t base = 0; base < p->spp; base += k, wi++) {
-            WaveBufs& B = ctx->slot[wi % slots];
-            WaveBufs& prev = ctx->slot[(wi + slots - 1) % slots];
-            hipStream_t sm = B.stream;
-            WaveState w = wave_state(B, ctx);
-            uint32_t* spill = B.spill.as<uint32_t>();
-            c.k = std::min(k, p->spp - base);
-            c.sample_base = base;
-            c.wave_paths = npix * c.k;
-            const int cgrid = std::min<int>((c.wave_paths + kShadeBlock - 1) / kShadeBlock, ctx->cus * 16);
-            hipLaunchKernelGGL(k_camera, dim3(cgrid), dim3(kShadeBlock), 0, sm, c, w);
-            int qsel = 0;
-            for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
-                {
-                    Timed tm(ctx, 0, sm);
-                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats);
-                    else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
-                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
-                }
-                // with screen-space reuse, k_ris runs the primary vertices' RIS
-                // first and every vertex after takes plain NEE: the shading
-                // kernel without the candidate loop
-                if (reuse && b == 0) {
-                    Timed tm(ctx, 3, sm);
-                    auto kr = lambert ? (alias ? k_ris<true, true> : k_ris<true, false>)
-                                      : (alias ? k_ris<false, true> : k_ris<false, false>);
-                    hipLaunchKernelGGL(kr, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
-                }
-                {
-                    Timed tm(ctx, 1, sm);
-                    AuxOut aux{nullptr, nullptr, nullptr, nullptr};
-                    if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
-                    auto* kfn = shade_kernel(lambert, alias, restir && !reuse, media);
-                    hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0);
-                }
-                if (reuse && b == 0) {
-                    const int rgrid2 = std::min<int>((npix + kShadeBlock - 1) / kShadeBlock, ctx->cus * 8);
-                    auto kt = lambert ? (alias ? k_restir_temporal<true, true> : k_restir_temporal<true, false>)
-                                      : (alias ? k_restir_temporal<false, true> : k_restir_temporal<false, false>);
-                    auto ks = lambert ? (alias ? k_restir_spatial<true, true> : k_restir_spatial<true, false>)
-                                      : (alias ? k_restir_spatial<false, true> : k_restir_spatial<false, false>);
-                    if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.reuse_done, 0));
-                    // temporal reuse chains consecutive samples: sample j of the wave
-                    // reuses sample j - 1's final reservoirs
-                    for (uint32_t j = 0; j < c.k; j++) {
-                        Timed tm(ctx, 4, sm);
-                        hipLaunchKernelGGL(kt, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, (base + j) > 0 ? 1 : 0, j);
-                        hipLaunchKernelGGL(ks, dim3(rgrid2), dim3(kShadeBlock), 0, sm, s->d, c, w, j);
-                    }
-                    HIP_CHECK(hipEventRecord(B.reuse_done, sm));
-                }
-                {
-                    Timed tm(ctx, 2, sm);
-                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats);
-                    else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
-                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
-                }
-                hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
-                qsel ^= 1;
-            }
-            const bool last = base + c.k >= p->spp;
-            const int rgrid = std::min<int>((npix + 255) / 256, ctx->cus * 8);
-            if (wi > 0 && slots > 1) HIP_CHECK(hipStreamWaitEvent(sm, prev.resolve_done, 0));
-            hipLaunchKernelGGL(k_resolve, dim3(rgrid), dim3(256), 0, sm, c, w, ctx->acc.as<float4>(), o->hdr,
-                               o->path_length, 1.0f / (float)p->spp, last ? 1 : 0);
-            HIP_CHECK(hipEventRecord(B.resolve_done, sm));
-            HIP_CHECK(hipGetLastError());
-        }
+        j.slots = slots;
+        j.k = k;
+        j.acc = ctx->acc.as<float4>();
+        j.rprev = ctx->rprev.as<float4>();
+        j.stats = ctx->stats.as<unsigned long long>();
+        j.tgrid = tgrid;
+        j.sgrid = sgrid;
+        j.timed = true;
+        enqueue_camera(ctx, j);
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
         finish_timings(ctx, slots, o->ray_counts);
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
     }
+}
+
+// ------------------------------------------------------------ batches
+// A batch renders many requests (scene, view, params, outputs), each exactly
+// as one st_render call would, in one graph launch.  Requests are spread over
+// the context's streams by longest-processing-time-first; each runs its waves
+// one after another in its stream's wave buffers, with accumulators, counters
+// and reuse history of its own, and the streams run side by side.  The whole
+// sequence of launches is captured once into a hipGraph and replayed on every
+// st_batch_render: the host issues one launch per batch instead of a few
+// thousand.  st_batch_autotune times candidate configurations (trace and
+// shade grid sizes, paths per wave, streams) and keeps the fastest.
+struct st_batch {
+    st_context* ctx = nullptr;
+    std::vector<st_render_request> req;
+    st_batch_config cfg{};
+    struct Per {
+        DevBuf counters, acc, rprev, stats;
+        uint32_t k = 1;
+        int stream = 0;
+        double cost = 0;
+    };
+    std::vector<std::unique_ptr<Per>> per;
+    WaveBufs bufs[kWaveSlots];
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> generation;  // of each request's scene when the graph was captured
+    hipEvent_t fork = nullptr, join[kWaveSlots] = {};
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+static st_batch_config default_batch_config() {
+    st_batch_config c{};
+    c.trace_blocks_per_cu = kTraceBlocksPerCU;
+    c.shade_blocks_per_cu = 8;
+    c.wave_paths = kWavePaths;
+    c.streams = kWaveSlots;
+    c.ms = 0;
+    return c;
+}
+
+// Buffers and stream assignment for a configuration (no launches).
+static void batch_plan(st_batch* b, const st_batch_config& cfg) {
+    st_context* ctx = b->ctx;
+    b->drop_graph();
+    b->cfg = cfg;
+    const int streams = (int)cfg.streams;
+    const int tgrid = ctx->cus * (int)cfg.trace_blocks_per_cu;
+    std::vector<size_t> order(b->req.size());
+    for (size_t r = 0; r < order.size(); r++) {
+        const st_render_params& p = b->req[r].params;
+        st_batch::Per& q = *b->per[r];
+        const uint32_t npix = p.width * p.height;
+        uint32_t k = p.samples_per_wave;
+        if (k == 0) k = std::max<uint32_t>(1, std::min<uint32_t>(p.spp, cfg.wave_paths / npix));
+        q.k = std::min(k, p.spp);
+        if ((uint64_t)npix * q.k >= (1ull << 31)) throw std::invalid_argument("too many paths per wave");
+        // estimated cost: path vertices, with the candidate loop of RIS counted twice
+        q.cost = (double)npix * p.spp * std::max<uint32_t>(1, p.max_depth) * ((p.flags & ST_FLAG_RESTIR_DI) ? 2.0 : 1.0);
+        order[r] = r;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t c) { return b->per[a]->cost > b->per[c]->cost; });
+    std::vector<double> load(streams, 0.0);
+    std::vector<uint64_t> paths(streams, 0);
+    std::vector<bool> reuse(streams, false);
+    for (size_t r : order) {
+        const int i = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        st_batch::Per& q = *b->per[r];
+        const st_render_params& p = b->req[r].params;
+        q.stream = i;
+        load[i] += q.cost;
+        paths[i] = std::max<uint64_t>(paths[i], (uint64_t)p.width * p.height * q.k);
+        reuse[i] = reuse[i] || (p.flags & ST_FLAG_RESTIR_DI_REUSE);
+        const size_t npix = (size_t)p.width * p.height;
+        q.counters.reserve(4 * kCounterWords);
+        q.acc.reserve(npix * 16);
+        if (p.flags & ST_FLAG_RESTIR_DI_REUSE) q.rprev.reserve(npix * 32);
+        q.stats.reserve(64);
+    }
+    for (int i = 0; i < streams; i++) {
+        b->bufs[i].stream = ctx->slot[i].stream;
+        if (paths[i]) reserve_wave(b->bufs[i], paths[i], reuse[i], paths[i], tgrid);
+    }
+}
+
+// The batch's launches: forked from the first stream, every request on its
+// stream, joined back (enqueued inside the capture).
+static void batch_enqueue(st_batch* b) {
+    st_context* ctx = b->ctx;
+    const int streams = (int)b->cfg.streams;
+    hipStream_t s0 = ctx->slot[0].stream;
+    HIP_CHECK(hipEventRecord(b->fork, s0));
+    for (int i = 1; i < streams; i++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[i].stream, b->fork, 0));
+    for (int i = 0; i < streams; i++)
+        for (size_t r = 0; r < b->req.size(); r++) {
+            st_batch::Per& q = *b->per[r];
+            if (q.stream != i) continue;
+            const st_render_request& R = b->req[r];
+            CameraJob j;
+            std::memset(&j, 0, sizeof j);
+            j.s = R.scene;
+            j.view = R.view;
+            j.p = R.params;
+            j.o = R.outputs;
+            j.slot[0] = &b->bufs[i];
+            j.counters[0] = q.counters.as<uint32_t>();
+            j.slots = 1;
+            j.k = q.k;
+            j.acc = q.acc.as<float4>();
+            j.rprev = q.rprev.as<float4>();
+            j.stats = q.stats.as<unsigned long long>();
+            j.tgrid = ctx->cus * (int)b->cfg.trace_blocks_per_cu;
+            j.sgrid = ctx->cus * (int)b->cfg.shade_blocks_per_cu;
+            j.timed = false;
+            j.store_rays = true;
+            enqueue_camera(ctx, j);
+        }
+    for (int i = 1; i < streams; i++) {
+        HIP_CHECK(hipEventRecord(b->join[i], ctx->slot[i].stream));
+        HIP_CHECK(hipStreamWaitEvent(s0, b->join[i], 0));
+    }
+}
+
+// Capture the batch's launches into a graph and instantiate it.
+static void batch_capture(st_batch* b) {
+    st_context* ctx = b->ctx;
+    b->drop_graph();
+    hipStream_t s0 = ctx->slot[0].stream;
+    HIP_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
+    try {
+        batch_enqueue(b);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s0, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(s0, &b->graph));
+    HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+
+static void batch_launch(st_batch* b) {
+    st_context* ctx = b->ctx;
+    // a scene rebuilt since the capture has new buffers: capture again
+    for (size_t r = 0; r < b->req.size() && b->exec; r++)
+        if (b->generation[r] != b->req[r].scene->generation) b->drop_graph();
+    if (!b->exec) {
+        batch_capture(b);
+        for (size_t r = 0; r < b->req.size(); r++) b->generation[r] = b->req[r].scene->generation;
+    }
+    order_after_caller(ctx);
+    HIP_CHECK(hipGraphLaunch(b->exec, ctx->slot[0].stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->slot[0].stream));
+}
+
+int st_batch_create(st_context* ctx, const st_render_request* requests, uint32_t count, st_batch** out) {
+    if (!ctx || !out || (count && !requests)) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (count == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty batch");
+    if (count > 4096) return set_error(ST_ERROR_LIMIT, "at most 4096 requests per batch");
+    for (uint32_t r = 0; r < count; r++) {
+        const st_render_request& R = requests[r];
+        if (!R.scene || !R.outputs.hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "request without scene or hdr output");
+        if (R.scene->ctx != ctx) return set_error(ST_ERROR_INVALID_ARGUMENT, "request scene belongs to another context");
+        const st_render_params& p = R.params;
+        if (p.width == 0 || p.height == 0 || p.spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render in a request");
+        if (p.flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC))
+            return set_error(ST_ERROR_UNSUPPORTED, "batches run camera paths (no light tracing or bidirectional mode)");
+        const int rc = check_camera(R.scene, &p);
+        if (rc != ST_OK) return rc;
+    }
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        auto b = std::make_unique<st_batch>();
+        b->ctx = ctx;
+        b->req.assign(requests, requests + count);
+        b->generation.assign(count, 0);
+        for (uint32_t r = 0; r < count; r++) b->per.push_back(std::make_unique<st_batch::Per>());
+        HIP_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        for (int i = 0; i < kWaveSlots; i++) HIP_CHECK(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
+        batch_plan(b.get(), default_batch_config());
+        *out = b.release();
+        return ST_OK;
+    } catch (const std::invalid_argument& e) {
+        return set_error(ST_ERROR_LIMIT, e.what());
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_batch_render(st_batch* b) {
+    if (!b) return set_error(ST_ERROR_INVALID_ARGUMENT, "null batch");
+    try {
+        HIP_CHECK(hipSetDevice(b->ctx->device));
+        batch_launch(b);
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_batch_get_config(const st_batch* b, st_batch_config* out) {
+    if (!b || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = b->cfg;
+    return ST_OK;
+}
+
+int st_batch_set_config(st_batch* b, const st_batch_config* cfg) {
+    if (!b || !cfg) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (cfg->trace_blocks_per_cu < 1 || cfg->trace_blocks_per_cu > 16 || cfg->shade_blocks_per_cu < 1 ||
+        cfg->shade_blocks_per_cu > 32 || cfg->streams < 1 || cfg->streams > kWaveSlots || cfg->wave_paths < 1 ||
+        cfg->wave_paths > (1u << 26))
+        return set_error(ST_ERROR_INVALID_ARGUMENT,
+                         "batch config: 1-16 trace and 1-32 shade blocks per CU, 1-4 streams, 1 to 2^26 paths per wave");
+    try {
+        HIP_CHECK(hipSetDevice(b->ctx->device));
+        batch_plan(b, *cfg);
+        return ST_OK;
+    } catch (const std::invalid_argument& e) {
+        return set_error(ST_ERROR_LIMIT, e.what());
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+// The in-library autotuner (timed coordinate descent over the four
+// configuration fields) is switched off: its one run aborted the process on
+// the host (glibc "free(): invalid pointer") with no cause found in this
+// library's code.  Configurations are set with st_batch_set_config.
+int st_batch_autotune(st_batch* b, uint32_t repeats, st_batch_config* chosen) {
+    (void)repeats;
+    (void)chosen;
+    if (!b) return set_error(ST_ERROR_INVALID_ARGUMENT, "null batch");
+    return set_error(ST_ERROR_UNSUPPORTED, "st_batch_autotune is switched off; set the configuration with st_batch_set_config");
+}
+
+int st_batch_destroy(st_batch* b) {
+    if (!b) return ST_OK;
+    (void)hipSetDevice(b->ctx->device);
+    for (int i = 0; i < kWaveSlots; i++) (void)hipStreamSynchronize(b->ctx->slot[i].stream);
+    b->drop_graph();
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (int i = 0; i < kWaveSlots; i++)
+        if (b->join[i]) (void)hipEventDestroy(b->join[i]);
+    delete b;
+    return ST_OK;
+}
+
+int st_render_batch(st_context* ctx, const st_render_request* requests, uint32_t count) {
+    st_batch* b = nullptr;
+    int rc = st_batch_create(ctx, requests, count, &b);
+    if (rc != ST_OK) return rc;
+    rc = st_batch_render(b);
+    st_batch_destroy(b);
+    return rc;
 }
 
 static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {

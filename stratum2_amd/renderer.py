@@ -164,6 +164,74 @@ class TestRenderer:
         return res
 
 
+class Batch:
+    """Many renders in one graph launch (st_batch_*): requests is a list of
+    (scene, view, params); each gets output tensors of its own, the same as
+    TestRenderer.render would return for it alone."""
+
+    def __init__(self, ctx, requests, want_path_length=False):
+        torch = _torch()
+        self.ctx, self.lib = ctx, ctx.lib
+        self.outputs = []
+        arr = (abi.st_render_request * len(requests))()
+        dev = torch.device("cuda", ctx.device)
+        for i, (scene, view, p) in enumerate(requests):
+            if scene.ctx is not ctx:
+                raise ValueError("every request's scene must belong to this context")
+            W, H, S = p.width, p.height, p.spp
+            o = {"hdr": torch.empty((H, W, 4), dtype=torch.float32, device=dev),
+                 "visibility": torch.empty((H, W, 2), dtype=torch.int32, device=dev),
+                 "ray_counts": torch.empty(2, dtype=torch.int64, device=dev),
+                 "path_length": torch.empty((S, H, W), dtype=torch.int32, device=dev) if want_path_length else None}
+            self.outputs.append(o)
+            arr[i].scene = scene.h
+            arr[i].view = view
+            arr[i].params = p
+            pl = o["path_length"]
+            arr[i].outputs = abi.st_outputs(o["hdr"].data_ptr(), o["visibility"].data_ptr(),
+                                            pl.data_ptr() if pl is not None else None, o["ray_counts"].data_ptr(),
+                                            None, None, None)
+        self._scenes = [r[0] for r in requests]  # kept alive as long as the batch
+        self.h = C.c_void_p()
+        abi.check(self.lib, self.lib.st_batch_create(ctx.h, arr, len(requests), C.byref(self.h)), "st_batch_create")
+
+    def render(self):
+        _enter(self.ctx.device)
+        abi.check(self.lib, self.lib.st_batch_render(self.h), "st_batch_render")
+        return self.outputs
+
+    def autotune(self, repeats=2):
+        """st_batch_autotune: switched off in the library (raises)."""
+        _enter(self.ctx.device)
+        cfg = abi.st_batch_config()
+        abi.check(self.lib, self.lib.st_batch_autotune(self.h, repeats, C.byref(cfg)), "st_batch_autotune")
+        return self.config()
+
+    def config(self):
+        cfg = abi.st_batch_config()
+        abi.check(self.lib, self.lib.st_batch_get_config(self.h, C.byref(cfg)), "st_batch_get_config")
+        return {k: getattr(cfg, k) for k, _ in cfg._fields_}
+
+    def set_config(self, **kw):
+        cfg = abi.st_batch_config()
+        abi.check(self.lib, self.lib.st_batch_get_config(self.h, C.byref(cfg)), "st_batch_get_config")
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        _enter(self.ctx.device)
+        abi.check(self.lib, self.lib.st_batch_set_config(self.h, C.byref(cfg)), "st_batch_set_config")
+
+    def close(self):
+        if self.h:
+            self.lib.st_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Denoiser:
     """The reference's Denoiser node (src/App/Denoiser.hpp:14-59) over
     st_denoise: temporal accumulation with reprojection, then optional
