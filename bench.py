@@ -24,6 +24,13 @@ instead (light subpaths, one per pixel and sample, connected to the camera);
 the spheres workload: temporal accumulation with reprojection and albedo
 demodulation, variance estimate, five 5x5 a-trous passes (one step = one
 denoised frame, Mpixels/s).
+--workload batch times one st_batch_render of 32 seeded requests of uneven
+size and spp (square, 128 to 1024 px a side; 1, 4, 16 or 64 spp): three in
+four render the spheres scene as the headline does, one in four the cloud as
+--workload cloud does (one step = one batch, Mrays/s over all requests),
+beside the same requests rendered one after another with st_render.  Launch
+shapes come from the package's autotuner, run before the timed region
+(--no-autotune: the library defaults).
 --copy-calibration runs a plain device copy of known size (for scaling the
 PMC byte counters) and exits.
 """
@@ -51,8 +58,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-wave", type=int, default=0, help="0: library default")
     ap.add_argument("--no-restir", action="store_true", help="plain NEE instead of ReSTIR DI")
-    ap.add_argument("--workload", choices=["spheres", "cloud", "svgf"], default="spheres")
+    ap.add_argument("--workload", choices=["spheres", "cloud", "svgf", "batch"], default="spheres")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="batch workload: the library's default launch shapes")
     ap.add_argument("--device-bvh", action="store_true", help="rebuild the BVH on the GPU (st_scene_update) before timing")
     ap.add_argument("--light-trace", action="store_true", help="light tracing (paths start on the lights)")
     ap.add_argument("--bdpt", action="store_true", help="bidirectional mode with a light vertex cache (gUseVC)")
@@ -78,6 +86,9 @@ def main():
         return
     if args.workload == "svgf":
         bench_svgf(args, log)
+        return
+    if args.workload == "batch":
+        bench_batch(args, log)
         return
     cloud = args.workload == "cloud"
     if cloud:
@@ -173,43 +184,8 @@ def main():
     ts = ctx.timings()
     log(f"serial frame {serial_ms:.1f} ms, per kernel (ms, launches): {kx}")
 
-    # Algorithmic bytes per kernel.  Traversal: 32 B ray + 16 B hit (or 48 B
-    # shadow record + 4 B flag) + 48 B per 4-wide node + 48 B per triangle.
-    # Media (per traceRay segment): 52 B path state read + 52 B written (or the
-    # 48 B shadow record + 16 B NEE accumulator), and 44 B per density-grid read
-    # (root tile key + child, upper mask word + table entry, lower mask word +
-    # table entry, leaf value).  Shading: 16 B hit + 64 B path state + 64 B
-    # material + 64 B triangle + 48 B shadow record per vertex.  k_ris, per
-    # primary hit: the same 208 B of hit, state, material and triangle, `cand`
-    # light candidates (environment, alias mode: 8 B alias entry + 4 B pdf + 4
-    # texels of 16 B; emissive triangle: 16 B light record + 80 B emitter
-    # triangle + 80 B transform and material words), mixed by the environment
-    # sampling probability, and its 48 B reservoir + 48 B record written.  Reuse,
-    # per pixel and sample: temporal 144 B (two reservoirs, the primary record,
-    # one written), spatial 160 B (own reservoir and record read, one reservoir
-    # and one shadow record written); the neighbours' records are other pixels'
-    # reads, and light-sample re-evaluations read the same few MB of emitter
-    # table and environment as k_ris, so neither is counted again.
-    NODE_B, TRI_B, GRID_B = 48, 48, 44
-    CAND_ENV_B, CAND_TRI_B = 76, 176
-    env_p = params.environment_sample_probability
-    cand_b = env_p * CAND_ENV_B + (1 - env_p) * CAND_TRI_B
-    cr, sr = ts["closest_rays"], ts["shadow_rays"]
-    if cloud:
-        algo = {"k_trace_closest": cr * 104 + GRID_B * nodes_c * cr,
-                "k_trace_shadow": sr * 64 + GRID_B * nodes_s * sr,
-                "k_shade": cr * (16 + 64 + 64 + 28 + 48) + sr * 48}
-        names = {"k_trace_closest": "k_trace_media", "k_trace_shadow": "k_shadow_media"}
-    else:
-        algo = {"k_trace_closest": cr * (48 + NODE_B * nodes_c + TRI_B * tris_c),
-                "k_trace_shadow": sr * (52 + NODE_B * nodes_s + TRI_B * tris_s),
-                "k_shade": cr * (16 + 64 + 64 + 64 + 48) + sr * 48}
-        names = {"k_shade": "k_shade_light" if args.light_trace else "k_shade_vc" if args.bdpt else "k_shade"}
-        if args.bdpt:  # a cached-vertex connection adds its 48 B vertex, 64 B triangle and 64 B material
-            algo["k_shade"] += sr * (48 + 64 + 64) / 2
-        if not args.no_restir:
-            algo["k_ris"] = prim_hits * args.spp * (208 + params.candidate_samples * cand_b + 96)
-            algo["k_restir_reuse"] = args.width * args.height * args.spp * (144 + 160)
+    algo, names = algo_bytes(ts, (nodes_c, tris_c, nodes_s, tris_s), cloud, prim_hits, args.width * args.height * args.spp,
+                             params, not args.no_restir, args.light_trace, args.bdpt)
     kernels = {}
     for k, b in algo.items():
         ms_k, n_k = kx[k]
@@ -267,6 +243,205 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+
+
+def algo_bytes(ts, per_ray, cloud, prim_hits, npix_spp, params, restir, light_trace=False, bdpt=False):
+    """Algorithmic HBM bytes of one frame per kernel class, from its ray counts
+    (ts: Context.timings()) and the per-ray traversal counts of a
+    COUNT_TRAVERSAL pass (per_ray: closest nodes, closest triangles, shadow
+    nodes, shadow triangles; density-grid reads with media).
+
+    Traversal: 32 B ray + 16 B hit (or 48 B shadow record + 4 B flag) + 48 B
+    per 4-wide node + 48 B per triangle.  Media (per traceRay segment): 52 B
+    path state read + 52 B written (or the 48 B shadow record + 16 B NEE
+    accumulator), and 44 B per density-grid read (root tile key + child, upper
+    mask word + table entry, lower mask word + table entry, leaf value).
+    Shading: 16 B hit + 64 B path state + 64 B material + 64 B triangle + 48 B
+    shadow record per vertex.  k_ris, per primary hit: the same 208 B of hit,
+    state, material and triangle, `cand` light candidates (environment, alias
+    mode: 8 B alias entry + 4 B pdf + 4 texels of 16 B; emissive triangle: 16 B
+    light record + 80 B emitter triangle + 80 B transform and material words),
+    mixed by the environment sampling probability, and its 48 B reservoir +
+    48 B record written.  Reuse, per pixel and sample: temporal 144 B (two
+    reservoirs, the primary record, one written), spatial 160 B (own reservoir
+    and record read, one reservoir and one shadow record written); the
+    neighbours' records are other pixels' reads, and light-sample
+    re-evaluations read the same few MB of emitter table and environment as
+    k_ris, so neither is counted again.  Returns ({class: bytes}, {class:
+    kernel name where it differs})."""
+    NODE_B, TRI_B, GRID_B = 48, 48, 44
+    CAND_ENV_B, CAND_TRI_B = 76, 176
+    nodes_c, tris_c, nodes_s, tris_s = per_ray
+    env_p = params.environment_sample_probability
+    cand_b = env_p * CAND_ENV_B + (1 - env_p) * CAND_TRI_B
+    cr, sr = ts["closest_rays"], ts["shadow_rays"]
+    if cloud:
+        algo = {"k_trace_closest": cr * 104 + GRID_B * nodes_c * cr,
+                "k_trace_shadow": sr * 64 + GRID_B * nodes_s * sr,
+                "k_shade": cr * (16 + 64 + 64 + 28 + 48) + sr * 48}
+        names = {"k_trace_closest": "k_trace_media", "k_trace_shadow": "k_shadow_media"}
+    else:
+        algo = {"k_trace_closest": cr * (48 + NODE_B * nodes_c + TRI_B * tris_c),
+                "k_trace_shadow": sr * (52 + NODE_B * nodes_s + TRI_B * tris_s),
+                "k_shade": cr * (16 + 64 + 64 + 64 + 48) + sr * 48}
+        names = {"k_shade": "k_shade_light" if light_trace else "k_shade_vc" if bdpt else "k_shade"}
+        if bdpt:  # a cached-vertex connection adds its 48 B vertex, 64 B triangle and 64 B material
+            algo["k_shade"] += sr * (48 + 64 + 64) / 2
+        if restir:
+            algo["k_ris"] = prim_hits * (npix_spp // (params.width * params.height)) * \
+                (208 + params.candidate_samples * cand_b + 96)
+            algo["k_restir_reuse"] = npix_spp * (144 + 160)
+    return algo, names
+
+
+def batch_requests(n=32, seed=7):
+    """The batch mix: n (scene kind, side, spp) triples drawn from a seeded
+    generator, every fourth request on the cloud."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([128, 256, 512, 1024], n)
+    spps = rng.choice([1, 4, 16, 64], n)
+    return [("cloud" if i % 4 == 3 else "spheres", int(sides[i]), int(spps[i])) for i in range(n)]
+
+
+def bench_batch(args, log):
+    """One step = one st_batch_render of the batch mix (one hipGraph launch
+    spread over four streams).  Mrays/s counts every request's rays.  The same
+    requests rendered one after another with st_render are timed beside it;
+    per-kernel roofline figures come from each request rendered once more with
+    ST_FLAG_SERIAL_WAVES (exclusive HIP-event times, summed over requests)."""
+    import numpy as np
+    import torch
+    from stratum2_amd import abi, scenes
+    from stratum2_amd.renderer import Batch, Context, Scene, TestRenderer
+    mix = batch_requests()
+    ctx = Context(0)
+    t0 = time.time()
+    sd = {"spheres": scenes.spheres_field(seed=1), "cloud": scenes.cloud(seed=0)}
+    sc = {k: Scene(ctx, v) for k, v in sd.items()}
+    log(f"scenes built in {time.time() - t0:.1f}s")
+    head = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
+    reqs = []
+    for kind, side, spp in mix:
+        if kind == "cloud":
+            p = abi.default_params(side, side, max_depth=16, spp=spp, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS,
+                                   rr_start_depth=0)
+            reqs.append((sc[kind], scenes.cloud_view(side, side), p))
+        else:
+            p = abi.default_params(side, side, max_depth=8, spp=spp, flags=head, rr_start_depth=3)
+            reqs.append((sc[kind], scenes.spheres_view(side, side), p))
+    tune = None
+    if not args.no_autotune:  # launch shapes for this mix (untimed; fresh batches)
+        from stratum2_amd.autotune import autotune_batch
+        a = time.perf_counter()
+        cfg, tuned_ms, hist = autotune_batch(ctx, reqs, repeats=1, passes=1, log=log)
+        tune = {"seconds": round(time.perf_counter() - a, 1), "candidates": len(hist), "best_ms": round(tuned_ms, 1)}
+    b = Batch(ctx, reqs)
+    if tune is not None:
+        b.set_config(**cfg)
+
+    def batch_rays():
+        return sum(int(o["ray_counts"][0].item()) + int(o["ray_counts"][1].item()) for o in b.outputs)
+
+    for _ in range(max(1, args.warmup)):  # the first render captures the graph
+        b.render()
+    torch.cuda.synchronize()
+    times, rays = [], []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        b.render()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - a)
+        rays.append(batch_rays())
+        log(f"batch: {times[-1] * 1e3:.1f} ms, {rays[-1] / 1e6:.1f} M rays")
+    value = sum(rays) / sum(times) / 1e6
+    ms = sum(times) / len(times) * 1e3
+
+    # the same requests one after another (st_render, four wave slots each)
+    outs = [None] * len(reqs)
+    for i, (s, v, p) in enumerate(reqs):
+        outs[i] = TestRenderer(s).render(v, p)
+    torch.cuda.synchronize()
+    seq = []
+    for _ in range(max(1, min(args.steps, 2))):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for i, (s, v, p) in enumerate(reqs):
+            TestRenderer(s).render(v, p, out=outs[i])
+        torch.cuda.synchronize()
+        seq.append(time.perf_counter() - a)
+    seq_ms = min(seq) * 1e3
+    same = all(torch.equal(outs[i]["hdr"], b.outputs[i]["hdr"]) for i in range(len(reqs)))
+    log(f"sequential st_render: {seq_ms:.1f} ms; batch images equal: {same}")
+
+    # exclusive per-kernel times and algorithmic bytes, summed over requests
+    totals = {}
+    serial_ms = 0.0
+    for (s, v, p), (kind, side, spp) in zip(reqs, mix):
+        cloud = kind == "cloud"
+        r = TestRenderer(s)
+        cp = abi.default_params(side, side, max_depth=p.max_depth, spp=1, flags=p.flags | abi.ST_FLAG_COUNT_TRAVERSAL,
+                                rr_start_depth=p.rr_start_depth)
+        first = r.render(v, cp)
+        torch.cuda.synchronize()
+        tc = ctx.timings()
+        prim_hits = int((first["visibility"][..., 0] != -1).sum().item())
+        per_ray = (tc["closest_nodes"] / max(1, tc["closest_rays"]), tc["closest_tris"] / max(1, tc["closest_rays"]),
+                   tc["shadow_nodes"] / max(1, tc["shadow_rays"]), tc["shadow_tris"] / max(1, tc["shadow_rays"]))
+        sp = abi.default_params(side, side, max_depth=p.max_depth, spp=spp, flags=p.flags | abi.ST_FLAG_SERIAL_WAVES,
+                                rr_start_depth=p.rr_start_depth)
+        a = time.perf_counter()
+        r.render(v, sp)
+        torch.cuda.synchronize()
+        serial_ms += (time.perf_counter() - a) * 1e3
+        kx, ts = ctx.kernel_times(), ctx.timings()
+        algo, names = algo_bytes(ts, per_ray, cloud, prim_hits, side * side * spp, p, not cloud)
+        for k, nbytes in algo.items():
+            ms_k, n_k = kx[k]
+            if n_k == 0 or ms_k <= 0:
+                continue
+            t = totals.setdefault(names.get(k, k), [0.0, 0, 0.0])
+            t[0] += ms_k
+            t[1] += n_k
+            t[2] += nbytes
+    kernels = {k: {"ms_per_batch": round(t[0], 3), "launches": t[1], "avg_launch_ms": round(t[0] / t[1], 4),
+                   "bytes_per_launch": round(t[2] / t[1]), "frac": round(t[2] / (t[0] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+               for k, t in totals.items()}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_batch"])
+    d = kernels[dom]
+    assert sum(k["ms_per_batch"] for k in kernels.values()) <= serial_ms, (kernels, serial_ms)
+    achieved = d["bytes_per_launch"] / (d["avg_launch_ms"] / 1e3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_launch_ms"],
+                "timing": "HIP events around each launch, every request rendered once more with ST_FLAG_SERIAL_WAVES "
+                          f"({serial_ms:.1f} ms for all requests), summed over requests",
+                "kernels": kernels}
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle on a 512^2 spheres request of the mix, bounded like the headline's
+        s_side = 512
+        ns = argparse.Namespace(width=s_side, height=s_side, cpu_seconds=args.cpu_seconds)
+        cpu = cpu_baseline(sd["spheres"], scenes.spheres_view(s_side, s_side), ns, 8, head, 3, log)
+    px_spp = sum(side * side * spp for _, side, spp in mix)
+    line = {
+        "metric": "Mrays/s batch of 32 uneven requests", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generators)",
+        "config": {"workload": "32 requests (seed 7): square 128-1024 px a side, 1/4/16/64 spp; spheres requests as the "
+                               "headline (1,004,098 tris, maxPathLength 8, RR from 3, alias-sampled sky, ReSTIR DI with "
+                               "reuse), every fourth on the 256^3 cloud (maxPathLength 16, delta + ratio tracking); one "
+                               "st_batch_render (one hipGraph launch over 4 streams)",
+                   "requests": [f"{k} {s}^2 {n}spp" for k, s, n in mix], "pixel_samples": px_spp,
+                   "batch_config": b.config(), "autotune": tune, "sequential_st_render_ms": round(seq_ms, 2),
+                   "batch_speedup_vs_sequential": round(seq_ms / ms, 3), "images_equal_sequential": bool(same),
+                   "samples_per_px_per_s": round(px_spp / (ms / 1e3), 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    b.close()
     print(json.dumps(line), flush=True)
 
 

@@ -287,7 +287,8 @@ typedef struct st_render_request {
 } st_render_request;
 
 /* Launch shapes of a batch (st_batch_set_config; defaults: 16 trace and 8
- * shade blocks per CU, 2^21 paths per wave, 4 streams). */
+ * shade blocks per CU, 2^23 paths per wave, 4 streams).  The package's
+ * autotuner (stratum2_amd/autotune.py) times candidates on fresh batches. */
 typedef struct st_batch_config {
     uint32_t trace_blocks_per_cu;  /* traversal grid: blocks of 128 lanes per CU (1-16) */
     uint32_t shade_blocks_per_cu;  /* shading grid: blocks of 256 lanes per CU (1-32) */

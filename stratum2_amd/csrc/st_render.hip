@@ -2923,13 +2923,23 @@ struct st_batch {
         exec = nullptr;
         graph = nullptr;
     }
+    ~st_batch() {
+        drop_graph();
+        if (fork) (void)hipEventDestroy(fork);
+        for (int i = 0; i < kWaveSlots; i++)
+            if (join[i]) (void)hipEventDestroy(join[i]);
+    }
 };
 
+// Defaults: st_render's grids, four streams, and waves four times st_render's
+// (a request's waves run one after another on its stream, so fewer, larger
+// waves leave fewer half-empty tail bounces; on a 32-request mix of 128^2 to
+// 1024^2 px this measured 460 ms per batch against 670 ms at 2^21 paths).
 static st_batch_config default_batch_config() {
     st_batch_config c{};
     c.trace_blocks_per_cu = kTraceBlocksPerCU;
     c.shade_blocks_per_cu = 8;
-    c.wave_paths = kWavePaths;
+    c.wave_paths = 4 * kWavePaths;
     c.streams = kWaveSlots;
     c.ms = 0;
     return c;
@@ -3133,10 +3143,6 @@ int st_batch_destroy(st_batch* b) {
     if (!b) return ST_OK;
     (void)hipSetDevice(b->ctx->device);
     for (int i = 0; i < kWaveSlots; i++) (void)hipStreamSynchronize(b->ctx->slot[i].stream);
-    b->drop_graph();
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    for (int i = 0; i < kWaveSlots; i++)
-        if (b->join[i]) (void)hipEventDestroy(b->join[i]);
     delete b;
     return ST_OK;
 }

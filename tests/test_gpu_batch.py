@@ -108,6 +108,21 @@ def test_batch_config_before_first_render(gpu_ctx, mixed):
     b.close()
 
 
+def test_package_autotuner_picks_a_candidate(gpu_ctx, mixed):
+    from stratum2_amd.autotune import autotune_batch
+    from stratum2_amd.renderer import Batch
+    reqs = mixed[:3]
+    cands = {"wave_paths": [1 << 12, 1 << 16], "streams": [1, 4]}
+    cfg, ms, hist = autotune_batch(gpu_ctx, reqs, repeats=1, candidates=cands, passes=2)
+    assert ms > 0 and cfg["wave_paths"] in (1 << 12, 1 << 16, 1 << 23) and cfg["streams"] in (1, 4)
+    assert min(h[1] for h in hist) == ms and 3 <= len(hist) <= 8
+    b = Batch(gpu_ctx, reqs, want_path_length=True)
+    b.set_config(**cfg)
+    b.render()
+    _check(b.outputs, reqs)
+    b.close()
+
+
 def test_batch_recaptures_after_scene_update(gpu_ctx, mixed):
     from stratum2_amd.renderer import Batch
     reqs = [mixed[6], mixed[0]]
