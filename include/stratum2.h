@@ -240,6 +240,42 @@ int st_scene_stats(const st_scene* scene, uint64_t out[4]);
 int st_render(st_scene* scene, const st_view* view, const st_render_params* params,
               const st_outputs* outputs);
 
+/* ---- Differentiable rendering (no reference counterpart: the reference has
+ * no derivatives; this is the path-replay adjoint of st_render's camera-path
+ * estimator, after Vicini et al., "Path Replay Backpropagation", 2021).
+ *
+ * st_scene_set_triangle_albedo: rgb = 3 floats per flattened triangle (mesh
+ * instances in instance order, primitives in index order; count = the
+ * triangle count of st_scene_stats), host or device memory, copied; the
+ * values replace the material base colour at Lambertian camera-path vertices.
+ * NULL removes the table.  While a table is set, st_render and batches need
+ * ST_FLAG_LAMBERTIAN and plain NEE (no ReSTIR DI, light tracing or the
+ * bidirectional mode): others return ST_ERROR_UNSUPPORTED.
+ *
+ * st_scene_set_volume_grid: replaces the values of grid `volume` (host or
+ * device memory): the same byte size and tree layout as the grid the scene
+ * was created with (only values may differ; the root's maximum value, byte
+ * 36 of the root node, is the tracking majorant and must bound them).
+ *
+ * st_adjoint: the gradient of sum(d_hdr * hdr) over the image that st_render
+ * with the same arguments and ST_FLAG_FLOAT_NEE renders.  d_hdr: RGBA float
+ * per pixel (device).  grad_albedo (3 floats per flattened triangle; needs
+ * the albedo table and ST_FLAG_LAMBERTIAN) and grad_grid (one float per
+ * 4-byte word of grid grad_volume: the gradient of the value stored at byte
+ * 4i is added to grad_grid[i]) are device buffers that the call adds to;
+ * either may be NULL.  Each path is rendered, then replayed from its seed
+ * (no tape): at a Lambertian vertex the radiance still to come is divided by
+ * the albedo, at each delta-tracking or ratio-tracking collision it is
+ * multiplied by the derivative of log(sigma_t) (real) or log(majorant -
+ * sigma_t) (null) with respect to the grid value read.  Sampling decisions
+ * (tracking events, roulette, MIS) are not differentiated, which keeps the
+ * gradient unbiased.  Camera paths with plain NEE only (ReSTIR DI, light
+ * tracing and the bidirectional mode return ST_ERROR_UNSUPPORTED). */
+int st_scene_set_triangle_albedo(st_scene* scene, const float* rgb, uint32_t count);
+int st_scene_set_volume_grid(st_scene* scene, uint32_t volume, const void* grid, uint64_t bytes);
+int st_adjoint(st_scene* scene, const st_view* view, const st_render_params* params, const float* d_hdr,
+               float* grad_albedo, float* grad_grid, uint32_t grad_volume);
+
 /* Primary-ray closest hit only (the minimum slice): visibility for each pixel
  * and, when t_out is not null, the hit distance as the path state stores it
  * (fp16-rounded, inf on a miss).  Device pointers. */

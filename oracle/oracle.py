@@ -68,6 +68,11 @@ def lib():
         l.or_svgf_variance.argtypes = [pv, pp, u32, u32, vp, u32, vp, vp, vp, vp, vp]
         l.or_svgf_atrous.argtypes = [pv, pp, u32, u32, vp, vp, u32, vp, vp]
         l.or_svgf_copy_rgb.argtypes = [u32, u32, vp, vp]
+        l.or_scene_triangle_count.restype = u32
+        l.or_scene_triangle_count.argtypes = [vp]
+        l.or_scene_set_triangle_albedo.argtypes = [vp, vp, u32]
+        l.or_scene_set_weight_grid.argtypes = [vp, u32, vp, C.c_uint64]
+        l.or_adjoint.argtypes = [vp, pv, C.POINTER(abi.st_render_params), vp, vp, vp, u32, C.c_int]
         _lib = l
     return _lib
 
@@ -108,6 +113,43 @@ class OracleScene:
             raise RuntimeError("oracle render: " + self.lib.or_last_error().decode())
         return dict(hdr=hdr, visibility=vis, path_length=pl, ray_counts=rc, traversal=tr, albedo=alb, depth=dep,
                     prev_uv=puv)
+
+    def triangle_count(self):
+        return int(self.lib.or_scene_triangle_count(self.h))
+
+    def set_triangle_albedo(self, rgb):
+        """(triangles, 3) float32 per-triangle albedo (None clears)."""
+        if rgb is None:
+            st = self.lib.or_scene_set_triangle_albedo(self.h, None, 0)
+        else:
+            self._alb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+            st = self.lib.or_scene_set_triangle_albedo(self.h, self._alb.ctypes.data, len(self._alb))
+        if st != 0:
+            raise RuntimeError("oracle albedo: " + self.lib.or_last_error().decode())
+
+    def set_weight_grid(self, vol, grid):
+        """Grid bytes that the path weights read instead of volume `vol`'s (None clears)."""
+        if grid is None:
+            st = self.lib.or_scene_set_weight_grid(self.h, vol, None, 0)
+        else:
+            g = np.ascontiguousarray(grid, np.uint8)
+            st = self.lib.or_scene_set_weight_grid(self.h, vol, g.ctypes.data, g.nbytes)
+        if st != 0:
+            raise RuntimeError("oracle weight grid: " + self.lib.or_last_error().decode())
+
+    def adjoint(self, view, params, d_hdr, albedo=True, grid_vol=None, grid_words=0, threads=0):
+        """Path-replay gradient of sum(d_hdr * hdr): (grad_albedo (T, 3) float64 or None,
+        grad_grid (grid_words,) float64 per 4-byte word of grid grid_vol, or None)."""
+        d = np.ascontiguousarray(d_hdr, np.float32)
+        assert d.shape == (params.height, params.width, 4)
+        ga = np.zeros((self.triangle_count(), 3)) if albedo else None
+        gg = np.zeros(grid_words) if grid_vol is not None else None
+        st = self.lib.or_adjoint(self.h, C.byref(view), C.byref(params), d.ctypes.data,
+                                 ga.ctypes.data if ga is not None else None, gg.ctypes.data if gg is not None else None,
+                                 grid_vol or 0, threads)
+        if st != 0:
+            raise RuntimeError("oracle adjoint: " + self.lib.or_last_error().decode())
+        return ga, gg
 
     def env_alias_table(self, n):
         """(prob, alias, pdf) of the n texels of the environment's level 0."""

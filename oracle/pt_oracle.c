@@ -437,6 +437,14 @@ struct or_scene {
     uint64_t volsz[8];
     uint32_t nvolinfo;
     st_volume_info* volinfo;
+    /* differentiable rendering: the per-triangle albedo (3 floats per
+     * flattened triangle, NULL: none) that replaces the base colour of
+     * Lambertian camera-path vertices, each instance's first flattened
+     * triangle, and "weight" copies of the grids (NULL: the grid itself): the
+     * tracking decisions read vol[], the path weights read volw[] */
+    float* tri_alb;
+    uint32_t* tri_base;
+    uint8_t* volw[8];
 };
 
 static f3 xf_point(const st_transform* t, f3 v) {
@@ -596,7 +604,8 @@ static float shading_normal_correction_x(const shading_t* sd, f3 din, f3 dout, i
 static float shading_normal_correction(const shading_t* sd, f3 din, f3 dout) { return shading_normal_correction_x(sd, din, dout, 0); }
 
 /* --------------------------------------------------------- materials */
-typedef struct { uint32_t p[4]; float escale; } material_t;
+/* has_bo: bo replaces the packed base colour (per-triangle albedo) */
+typedef struct { uint32_t p[4]; float escale; int has_bo; f3 bo; } material_t;
 static material_t load_material(const or_scene* s, uint32_t addr) {
     material_t m;
     memset(&m, 0, sizeof m);
@@ -605,7 +614,9 @@ static material_t load_material(const or_scene* s, uint32_t addr) {
     m.escale = asfloat(s->mat[addr / 4 + 8]);
     return m;
 }
-static f3 m_base(const material_t* m) { return v3(un8(m->p[0], 0), un8(m->p[0], 8), un8(m->p[0], 16)); }
+static f3 m_base(const material_t* m) {
+    return m->has_bo ? m->bo : v3(un8(m->p[0], 0), un8(m->p[0], 8), un8(m->p[0], 16));
+}
 static f3 m_emission(const material_t* m) {
     return mulf(v3(un8(m->p[1], 0), un8(m->p[1], 8), un8(m->p[1], 16)), m->escale);
 }
@@ -894,6 +905,7 @@ static float m_cont_prob(const material_t* m) { return saturate(luminance(m_base
 
 void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], const float dout[3], float o[5]) {
     material_t m;
+    memset(&m, 0, sizeof m);
     memcpy(m.p, mat, 16);
     m.escale = asfloat(mat[4]);
     rer_t r = bsdf_eval(&m, lambertian, v3(din[0], din[1], din[2]), v3(dout[0], dout[1], dout[2]));
@@ -902,6 +914,7 @@ void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], con
 void or_bsdf_eval_adjoint(const uint32_t mat[5], int lambertian, int adjoint, const float din[3], const float dout[3],
                           float o[5]) {
     material_t m;
+    memset(&m, 0, sizeof m);
     memcpy(m.p, mat, 16);
     m.escale = asfloat(mat[4]);
     rer_t r = bsdf_eval_x(&m, lambertian, v3(din[0], din[1], din[2]), v3(dout[0], dout[1], dout[2]), adjoint);
@@ -909,6 +922,7 @@ void or_bsdf_eval_adjoint(const uint32_t mat[5], int lambertian, int adjoint, co
 }
 void or_bsdf_sample(const uint32_t mat[5], int lambertian, const float rnd[3], const float din[3], float o[10]) {
     material_t m;
+    memset(&m, 0, sizeof m);
     memcpy(m.p, mat, 16);
     m.escale = asfloat(mat[4]);
     dsr_t r = bsdf_sample(&m, lambertian, v3(rnd[0], rnd[1], rnd[2]), v3(din[0], din[1], din[2]));
@@ -1260,7 +1274,9 @@ static uint32_t rd32(const uint8_t* b, uint64_t o) { uint32_t v; memcpy(&v, b + 
 static uint64_t rd64(const uint8_t* b, uint64_t o) { uint64_t v; memcpy(&v, b + o, 8); return v; }
 static float rdf(const uint8_t* b, uint64_t o) { float v; memcpy(&v, b + o, 4); return v; }
 static uint64_t nvdb_root(const uint8_t* b) { return 672 + rd64(b, 672 + 24); }
-static float nvdb_read(const uint8_t* b, int i, int j, int k) {
+/* Byte offset of the value that the read at (i, j, k) returns (a leaf voxel,
+ * a tile of an internal node or of the root, or the background). */
+static uint64_t nvdb_addr(const uint8_t* b, int i, int j, int k) {
     const uint64_t root = nvdb_root(b);
     const uint32_t ntiles = rd32(b, root + 24);
     const uint64_t key = (uint64_t)((uint32_t)k >> 12) | ((uint64_t)((uint32_t)j >> 12) << 21) | ((uint64_t)((uint32_t)i >> 12) << 42);
@@ -1268,19 +1284,20 @@ static float nvdb_read(const uint8_t* b, int i, int j, int k) {
         const uint64_t tile = root + 64 + 32 * (uint64_t)t;
         if (rd64(b, tile) != key) continue;
         const int64_t child = (int64_t)rd64(b, tile + 8);
-        if (child == 0) return rdf(b, tile + 20);
+        if (child == 0) return tile + 20;
         const uint64_t up = root + (uint64_t)child;
         const uint32_t n2 = (uint32_t)(((i & 4095) >> 7) << 10) + (uint32_t)(((j & 4095) >> 7) << 5) + (uint32_t)((k & 4095) >> 7);
-        if (!((rd32(b, up + 4128 + 4 * (n2 >> 5)) >> (n2 & 31)) & 1)) return rdf(b, up + 8256 + 8 * (uint64_t)n2);
+        if (!((rd32(b, up + 4128 + 4 * (n2 >> 5)) >> (n2 & 31)) & 1)) return up + 8256 + 8 * (uint64_t)n2;
         const uint64_t lo = up + rd64(b, up + 8256 + 8 * (uint64_t)n2);
         const uint32_t n1 = (uint32_t)(((i & 127) >> 3) << 8) + (uint32_t)(((j & 127) >> 3) << 4) + (uint32_t)((k & 127) >> 3);
-        if (!((rd32(b, lo + 544 + 4 * (n1 >> 5)) >> (n1 & 31)) & 1)) return rdf(b, lo + 1088 + 8 * (uint64_t)n1);
+        if (!((rd32(b, lo + 544 + 4 * (n1 >> 5)) >> (n1 & 31)) & 1)) return lo + 1088 + 8 * (uint64_t)n1;
         const uint64_t leaf = lo + rd64(b, lo + 1088 + 8 * (uint64_t)n1);
         const uint32_t n0 = (uint32_t)((i & 7) << 6) + (uint32_t)((j & 7) << 3) + (uint32_t)(k & 7);
-        return rdf(b, leaf + 96 + 4 * (uint64_t)n0);
+        return leaf + 96 + 4 * (uint64_t)n0;
     }
-    return rdf(b, root + 28); /* background */
+    return root + 28; /* background */
 }
+static float nvdb_read(const uint8_t* b, int i, int j, int k) { return rdf(b, nvdb_addr(b, i, j, k)); }
 static f3 nvdb_map(const uint8_t* b, uint64_t off, f3 v, int translate, int inverse) {
     const uint64_t m = 296 + off;
     if (inverse && translate) v = v3(v.x - rdf(b, 296 + 72), v.y - rdf(b, 296 + 76), v.z - rdf(b, 296 + 80));
@@ -1347,8 +1364,14 @@ void or_hg_sample(float g, float r0, float r1, const float din[3], float out[3])
 static f3 expv(f3 a) { return v3(om_exp(a.x), om_exp(a.y), om_exp(a.z)); }
 static float avg3(f3 a) { return (a.x + a.y + a.z) / 3; }
 /* deltaTrack<Scattering>, medium.hlsli:104-181.  origin/direction in object space. */
+/* Density-gradient sink of the adjoint: at each collision in grid `vol`,
+ * grad[value offset / 4] += sum_c wgt_c * dlog(beta_c) / d(value), where wgt
+ * = dLoss/dL (per channel) times the radiance the collision's factor
+ * multiplies. */
+typedef struct { double* grad; uint32_t vol; f3 wgt; } adjd_t;
+
 static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_t* rng, f3 origin, f3 dir, float tmax, f3* beta,
-                      f3* dirPdf, f3* neePdf, int* scattered, uint32_t max_null) {
+                      f3* dirPdf, f3* neePdf, int* scattered, uint32_t max_null, const adjd_t* ad) {
     *scattered = 0;
     *dirPdf = v3(1, 1, 1);
     *neePdf = v3(1, 1, 1);
@@ -1374,6 +1397,10 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
         return v3(0, 0, 0);
     }
     const uint8_t* g = s->vol[m->dens_vol];
+    /* detached sampling: decisions, pdfs and the majorant read the grid, the
+     * path weights read its weight copy (the same grid unless one is set) */
+    const uint8_t* gw = s->volw[m->dens_vol] ? s->volw[m->dens_vol] : g;
+    const int sink = ad && ad->grad && ad->vol == m->dens_vol;
     const float rmax = rdf(g, nvdb_root(g) + 36);
     const f3 maj = mulf(m->density, rmax);
     if (fget(maj, ch) < 1e-6f) return v3(0, 0, 0);
@@ -1394,8 +1421,9 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
         }
         origin = add(origin, mulf(dir, t));
         tmax -= t;
-        const float dv = nvdb_read(g, (int)floorf(origin.x), (int)floorf(origin.y), (int)floorf(origin.z));
-        const f3 sig_t = mulf(m->density, dv);
+        const uint64_t va = nvdb_addr(g, (int)floorf(origin.x), (int)floorf(origin.y), (int)floorf(origin.z));
+        const float dv = rdf(g, va), dvw = rdf(gw, va);
+        const f3 sig_t = mulf(m->density, dv), sig_tw = mulf(m->density, dvw);
         const f3 sig_s = med_albedo(m); /* readAlbedo is 1 without an albedo volume */
         const f3 tr = mulf(expv(mulf(neg(maj), t)), invmax);
         int real = 0;
@@ -1405,13 +1433,23 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
             real = r2[0] < fget(sig_t, ch) * invc;
         }
         if (real) {
-            *beta = mul(*beta, mul(mul(tr, sig_t), sig_s));
+            /* beta_c gains sigma_t,c = density_c * value: dlog = 1 / value */
+            if (sink && dvw > 0) ad->grad[va / 4] += ((double)ad->wgt.x + (double)ad->wgt.y + (double)ad->wgt.z) / (double)dvw;
+            *beta = mul(*beta, mul(mul(tr, sig_tw), sig_s));
             *dirPdf = mul(*dirPdf, mul(tr, sig_t));
             *scattered = 1;
             return nvdb_index_to_world(g, origin);
         }
-        const f3 d = sub(maj, sig_t);
-        *beta = mul(*beta, mul(tr, d));
+        const f3 d = sub(maj, sig_t), dw = sub(maj, sig_tw);
+        /* beta_c gains maj_c - density_c * value: dlog = -density_c / (maj_c - sigma_t,c) */
+        if (sink) {
+            double gsum = 0;
+            if (dw.x != 0) gsum -= (double)ad->wgt.x * (double)m->density.x / (double)dw.x;
+            if (dw.y != 0) gsum -= (double)ad->wgt.y * (double)m->density.y / (double)dw.y;
+            if (dw.z != 0) gsum -= (double)ad->wgt.z * (double)m->density.z / (double)dw.z;
+            ad->grad[va / 4] += gsum;
+        }
+        *beta = mul(*beta, mul(tr, dw));
         *dirPdf = mul(*dirPdf, mul(tr, d));
         *neePdf = mul(*neePdf, mul(tr, maj));
     }
@@ -1684,8 +1722,10 @@ or_scene* or_scene_create(const st_scene_desc* d) {
     }
     s->tv = (float*)malloc(sizeof(float) * 9 * (ntri + 1));
     s->tid = (uint32_t*)malloc(sizeof(uint32_t) * (ntri + 1));
+    s->tri_base = (uint32_t*)calloc(s->ninst + 1, sizeof(uint32_t));
     for (uint32_t i = 0; i < s->ninst; i++) {
         const st_instance* in = &s->inst[i];
+        s->tri_base[i] = s->ntris;
         uint32_t type = in->type_material_address & 0xfu;
         if (type == 2) { /* volume box corners, Scene.cpp:700-711 */
             f3 mn = v3(-1, -1, -1), mx = v3(1, 1, 1);
@@ -1763,7 +1803,8 @@ void or_scene_destroy(or_scene* s) {
     free(s->alias_prob); free(s->alias_idx); free(s->alias_pdf);
     free(s->light_inst); free(s->tv); free(s->tid); free(s->nodes); free(s->order);
     free(s->inv_xf); free(s->volinfo);
-    for (uint32_t v = 0; v < s->nvols; v++) free(s->vol[v]);
+    for (uint32_t v = 0; v < s->nvols; v++) { free(s->vol[v]); free(s->volw[v]); }
+    free(s->tri_alb); free(s->tri_base);
     free(s);
 }
 
@@ -1875,7 +1916,22 @@ typedef struct {
     shading_t prim_sd;
     uint32_t prim_dir;          /* packed primary ray direction */
     float res[8];               /* seed[4], target, W, M, depth */
+    struct adj_s* adj;          /* path replay of or_adjoint, or NULL */
 } pctx_t;
+
+/* Path replay (or_adjoint).  Both passes trace each shadow ray right after
+ * the vertex that made it and add its fp32 contribution to acc, in the order
+ * the GPU wavefront adds them (emission, then that vertex's NEE); pass 1 ends
+ * with total = acc, pass 2 replays the path and differentiates: the radiance
+ * still to come at any point is total - acc. */
+typedef struct adj_s {
+    int pass;            /* 1: totals, 2: gradients */
+    f3 acc, total;
+    f3 g;                /* dLoss/dL of this path: d_hdr / spp */
+    double* grad_alb;    /* 3 per flattened triangle */
+    double* grad_grid;   /* per 4-byte word of grid grad_vol */
+    uint32_t grad_vol;
+} adj_t;
 
 /* ViewData::toRaster (compat/scene.h:158-165) with ProjectionData::projectPoint
  * (compat/transform.h:139-155); 0, with *ox, *oy unchanged, when p is not visible. */
@@ -1926,6 +1982,7 @@ static void store_aux(pctx_t* x, int has_instance, f3 pos, f3 albedo) {
 static void add_radiance(pctx_t* x, f3 rad, float w) {
     if (w <= 0 || all_le0(rad)) return;
     x->out = add(x->out, mulf(rad, w));
+    if (x->adj) x->adj->acc = add(x->adj->acc, mulf(rad, w));
 }
 /* AddEmission, testrenderer.slang:788-808 (+ --minPathLength gate, vcm.slang:960) */
 static void add_emission(pctx_t* x, f3 e, float directPdfA, float emissionPdfW, float WtoA) {
@@ -2165,8 +2222,15 @@ static int trace_scattering(pctx_t* x, f3 origin, f3 dir, hit_t* h, f3* pos, flo
             const st_transform* iT = &s->inv_xf[ps->medium];
             f3 dp3, np3;
             int sc;
+            adjd_t ad, *adp = NULL;
+            if (x->adj && x->adj->pass == 2) { /* every later contribution carries this segment's factors */
+                ad.grad = x->adj->grad_grid;
+                ad.vol = x->adj->grad_vol;
+                ad.wgt = mul(x->adj->g, sub(x->adj->total, x->adj->acc));
+                adp = &ad;
+            }
             f3 p = delta_track(s, &m, 1, &ps->rng, xf_point(iT, o), xf_vector(iT, dir), h->hit ? h->t : tmax, &ps->thr, &dp3, &np3,
-                               &sc, x->c->max_null);
+                               &sc, x->c->max_null, adp);
             *dirpdf *= avg3(dp3);
             *neepdf *= avg3(np3);
             if (sc) {
@@ -2197,7 +2261,8 @@ static int trace_scattering(pctx_t* x, f3 origin, f3 dir, hit_t* h, f3* pos, flo
  * The deferred-shadow variant of the reference reads an uninitialised static
  * path state here (testrenderer.slang:1153-1158); this uses the vertex's
  * medium and a random stream of its own, keyed by pixel, sample and bounce. */
-static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, uint32_t py, uint32_t seed, f3* beta, float* neepdf) {
+static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, uint32_t py, uint32_t seed, f3* beta, float* neepdf,
+                                   const adjd_t* ad) {
     const or_scene* s = x->s;
     rng_t rng;
     rng_init(&rng, seed, px, py, 0x40000000u | (sr->plen << 16));
@@ -2226,7 +2291,7 @@ static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, u
             f3 dp3, np3;
             int sc;
             delta_track(s, &m, 0, &rng, xf_point(iT, o), xf_vector(iT, sr->d), h.hit ? h.t : tmax, beta, &dp3, &np3, &sc,
-                        x->c->max_null);
+                        x->c->max_null, ad);
             *neepdf *= avg3(np3);
             if (!(beta->x != 0 && beta->y != 0 && beta->z != 0)) return;
         }
@@ -2426,6 +2491,12 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     }
     if (all_le0(ps->thr)) return 0; /* ExtendPath, testrenderer.slang:1062-1063 */
     material_t m = load_material(s, sd.flags_material >> 4);
+    uint32_t alb_tri = 0xffffffffu; /* the vertex's entry of the per-triangle albedo */
+    if (s->tri_alb) {
+        alb_tri = s->tri_base[h.id & 0xffffu] + (h.id >> 16);
+        m.has_bo = 1;
+        m.bo = v3(s->tri_alb[3 * (size_t)alb_tri], s->tri_alb[3 * (size_t)alb_tri + 1], s->tri_alb[3 * (size_t)alb_tri + 2]);
+    }
     f3 ldin = to_local(&sd, neg(dir));
     float cos_in = fabsf(ldin.z);
     const int vc = (x->c->flags & ST_FLAG_BDPT_LVC) != 0;
@@ -2450,6 +2521,16 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
             float pa = ps->pdfa16;
             add_emission(x, e, pa, pa * cos_hemisphere_pdfw(ldin.z), cos_in / pow2(ps->t16));
         }
+    }
+    if (x->adj && x->adj->pass == 2 && alb_tri != 0xffffffffu) {
+        /* Lambertian f = albedo / pi: every later contribution (this vertex's
+         * NEE, the continuation) carries one factor albedo_c of this vertex */
+        const f3 wgt = mul(x->adj->g, sub(x->adj->total, x->adj->acc));
+        const f3 a = m.bo;
+        double* ga = x->adj->grad_alb + 3 * (size_t)alb_tri;
+        if (a.x > 0) ga[0] += (double)(wgt.x / a.x);
+        if (a.y > 0) ga[1] += (double)(wgt.y / a.y);
+        if (a.z > 0) ga[2] += (double)(wgt.z / a.z);
     }
     if (ps->path_len >= x->c->max_depth) return 0;
     if (!m_can_eval(&m)) return 0;
@@ -2486,6 +2567,38 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     ps->origin = ray_offset_dir(sd.pos, unpack_normal(sd.gn), wd);
     if (ps->thr.x != ps->thr.x || ps->thr.y != ps->thr.y || ps->thr.z != ps->thr.z) return 0;
     return 1;
+}
+
+/* The path replay's shadow rays, traced as the GPU wavefront does after each
+ * bounce: the fp32 contribution (ST_FLAG_FLOAT_NEE's) joins acc, and in pass
+ * 2 ratio tracking is replayed with the contribution as the sink weight. */
+static void adj_flush_shadows(pctx_t* x, uint32_t px, uint32_t py, uint32_t seed) {
+    adj_t* A = x->adj;
+    for (int i = 0; i < x->nsh; i++) {
+        shadow_t sr = x->sh[i];
+        f3 c = sr.c;
+        if (x->s->nvolinst > 0) {
+            float neepdf;
+            trace_visibility_media(x, &sr, px, py, seed, &c, &neepdf, NULL);
+            if (all_le0(c) || neepdf <= 0) continue;
+            c = divf(c, neepdf);
+        } else {
+            hit_t h = trace(x->s, sr.o, sr.d, sr.tmax, 1, &x->trav[2], &x->trav[3]);
+            x->rays[1]++;
+            if (h.hit) continue;
+        }
+        if (sr.w <= 0 || all_le0(c)) continue;
+        const f3 rw = mulf(c, sr.w);
+        const f3 add3 = v3(maxf(0, rw.x), maxf(0, rw.y), maxf(0, rw.z));
+        A->acc = add(A->acc, add3);
+        if (A->pass == 2 && x->s->nvolinst > 0) {
+            adjd_t ad = {A->grad_grid, A->grad_vol, mul(A->g, add3)};
+            f3 cb = sr.c;
+            float np;
+            trace_visibility_media(x, &sr, px, py, seed, &cb, &np, &ad);
+        }
+    }
+    x->nsh = 0;
 }
 
 /* One launch (1 spp) for one pixel: Render + RenderIteration x (maxDepth-1)
@@ -2535,14 +2648,16 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
      * throughput is zero or the depth is reached (testrenderer.slang:1105, 1137-1141) */
     for (int first = 1;; first = 0) {
         if (!first && (all_le0(ps->thr) || ps->path_len + 1 > x->c->max_depth)) break;
-        if (!extend_path(x, p, view)) { ps->thr = v3(0, 0, 0); break; }
+        const int alive = extend_path(x, p, view);
+        if (x->adj) adj_flush_shadows(x, px, py, seed);
+        if (!alive) { ps->thr = v3(0, 0, 0); break; }
     }
     /* ProcessShadowRays + AddRadianceInterlocked, testrenderer.slang:237-261, 1147-1166 */
     for (int i = 0; i < x->nsh; i++) {
         shadow_t* sr = &x->sh[i];
         if (x->s->nvolinst > 0) {
             float neepdf;
-            trace_visibility_media(x, sr, px, py, seed, &sr->c, &neepdf);
+            trace_visibility_media(x, sr, px, py, seed, &sr->c, &neepdf, NULL);
             if (all_le0(sr->c) || neepdf <= 0) continue;
             sr->c = divf(sr->c, neepdf);
         } else {
@@ -2649,7 +2764,7 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
         sr.o = o; sr.d = tc; sr.tmax = dist; sr.c = contrib; sr.w = 1;
         sr.medium = x->ps.medium; sr.plen = x->ps.path_len;
         float neepdf;
-        trace_visibility_media(x, &sr, q % x->W, q / x->W, lx->seed, &contrib, &neepdf);
+        trace_visibility_media(x, &sr, q % x->W, q / x->W, lx->seed, &contrib, &neepdf, NULL);
         if (all_le0(contrib) || neepdf <= 0) return;
         contrib = divf(contrib, neepdf);
     } else {
@@ -3249,9 +3364,19 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
     return 0;
 }
 
+/* The per-triangle albedo serves Lambertian camera paths with plain NEE. */
+static int check_tri_albedo(const or_scene* s, uint32_t flags) {
+    if (!s->tri_alb) return 0;
+    if (!(flags & ST_FLAG_LAMBERTIAN)) return fail("a per-triangle albedo needs ST_FLAG_LAMBERTIAN");
+    if (flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC | ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE))
+        return fail("a per-triangle albedo serves camera paths with plain NEE");
+    return 0;
+}
+
 int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
     if (!s || !view || !p || !out) return fail("null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
+    if (check_tri_albedo(s, p->flags)) return -1;
     if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, out, threads);
     if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, out, threads);
     if (s->nvolinst > 0 && (p->flags & ST_FLAG_RESTIR_DI)) return fail("ReSTIR DI with media is not supported");
@@ -3328,6 +3453,117 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
     }
     if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
     if (out->traversal) for (int k = 0; k < 4; k++) out->traversal[k] = trav[k];
+    return 0;
+}
+
+/* ------------------------------------------------------- path replay */
+uint32_t or_scene_triangle_count(const or_scene* s) { return s ? s->ntris : 0; }
+
+int or_scene_set_triangle_albedo(or_scene* s, const float* rgb, uint32_t count) {
+    if (!s) return fail("null scene");
+    if (!rgb) { free(s->tri_alb); s->tri_alb = NULL; return 0; }
+    if (count != s->ntris) return fail("one albedo per flattened triangle");
+    if (!s->tri_alb) s->tri_alb = (float*)malloc(sizeof(float) * 3 * ((size_t)count + 1));
+    memcpy(s->tri_alb, rgb, sizeof(float) * 3 * (size_t)count);
+    return 0;
+}
+
+int or_scene_set_weight_grid(or_scene* s, uint32_t vol, const void* grid, uint64_t size) {
+    if (!s || vol >= s->nvols) return fail("no such volume");
+    if (!grid) { free(s->volw[vol]); s->volw[vol] = NULL; return 0; }
+    if (size != s->volsz[vol]) return fail("the weight grid must have the grid's size");
+    if (!s->volw[vol]) s->volw[vol] = (uint8_t*)malloc(size + 16);
+    memcpy(s->volw[vol], grid, size);
+    return 0;
+}
+
+/* Path-replay adjoint of the camera-path estimator (no tape): for each path,
+ * pass 1 renders it and keeps its fp32 radiance (shadow rays traced after
+ * each bounce), pass 2 replays it from the same seed and, at every
+ * differentiable event, adds dLoss/dL times the radiance still to come times
+ * the event's log-derivative: 1/albedo_c at a Lambertian vertex (the
+ * triangle's entry of the albedo table), and at each delta- or
+ * ratio-tracking collision in grid grad_vol the derivative of log sigma_t
+ * (real) or log(majorant - sigma_t) (null) with respect to the value read.
+ * Sampling decisions (tracking events, roulette, MIS) are held fixed
+ * ("detached"), which keeps the estimator unbiased; a fixed-seed central
+ * difference over a weight grid (or_scene_set_weight_grid) or over the
+ * albedo table with roulette off is exactly its derivative.  NEE is fp32
+ * (ST_FLAG_FLOAT_NEE).  d_hdr: dLoss/dHDR, RGBA per pixel; gradients are
+ * added to grad_alb (3 per flattened triangle) and grad_grid (one per 4-byte
+ * word of grid grad_vol); either may be NULL. */
+int or_adjoint(or_scene* s, const st_view* view, const st_render_params* p, const float* d_hdr, double* grad_alb,
+               double* grad_grid, uint32_t grad_vol, int threads) {
+    if (!s || !view || !p || !d_hdr) return fail("null argument");
+    if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
+    if (p->flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC | ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE))
+        return fail("the adjoint replays camera paths with plain NEE");
+    if (grad_alb && (!s->tri_alb || !(p->flags & ST_FLAG_LAMBERTIAN)))
+        return fail("albedo gradients need a per-triangle albedo and ST_FLAG_LAMBERTIAN");
+    if (grad_grid && grad_vol >= s->nvols) return fail("no such volume");
+    if (check_tri_albedo(s, p->flags)) return -1;
+    cfg_t c;
+    memset(&c, 0, sizeof c);
+    c.flags = p->flags | ST_FLAG_FLOAT_NEE;
+    c.env_prob = p->environment_sample_probability;
+    c.max_depth = p->max_depth;
+    c.min_len = p->min_path_length;
+    c.cand = p->candidate_samples;
+    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    c.max_null = p->max_null_collisions;
+    c.rr_start = p->rr_start_depth;
+    const uint32_t view_medium = camera_medium(s, view);
+    const st_transform inv_cam = inverse_xf(&view->camera_to_world);
+    const uint32_t W = p->width, H = p->height;
+    const size_t na = grad_alb ? 3 * (size_t)s->ntris : 0, ng = grad_grid ? (size_t)(s->volsz[grad_vol] / 4) : 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+#pragma omp parallel
+    {
+        pctx_t x;
+        memset(&x, 0, sizeof x);
+        x.s = s;
+        x.c = &c;
+        x.view_medium = view_medium;
+        x.view = view;
+        x.inv_cam = inv_cam;
+        x.W = W;
+        x.H = H;
+        adj_t A;
+        memset(&A, 0, sizeof A);
+        A.grad_alb = na ? (double*)calloc(na, sizeof(double)) : NULL;
+        A.grad_grid = ng ? (double*)calloc(ng, sizeof(double)) : NULL;
+        A.grad_vol = grad_vol;
+        x.adj = &A;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t pix = 0; pix < (int64_t)W * H; pix++) {
+            const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
+            x.px = px;
+            x.py = py;
+            A.g = v3(d_hdr[4 * pix] / (float)p->spp, d_hdr[4 * pix + 1] / (float)p->spp, d_hdr[4 * pix + 2] / (float)p->spp);
+            for (uint32_t k = 0; k < p->spp; k++) {
+                f3 r;
+                uint32_t pl;
+                A.pass = 1;
+                A.acc = v3(0, 0, 0);
+                render_pixel(&x, p, view, px, py, p->seed + k, &r, &pl);
+                A.total = A.acc;
+                A.pass = 2;
+                A.acc = v3(0, 0, 0);
+                render_pixel(&x, p, view, px, py, p->seed + k, &r, &pl);
+            }
+        }
+#pragma omp critical
+        {
+            for (size_t i = 0; i < na; i++) grad_alb[i] += A.grad_alb[i];
+            for (size_t i = 0; i < ng; i++) grad_grid[i] += A.grad_grid[i];
+        }
+        free(A.grad_alb);
+        free(A.grad_grid);
+    }
     return 0;
 }
 

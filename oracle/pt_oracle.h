@@ -48,6 +48,20 @@ int or_trace_rays(or_scene* s, const float* rays, uint32_t count, int closest, u
  * the watertight test + BVH on tiny scenes). hits: {t (float bits), id, 0, 0}. */
 int or_trace_rays_bruteforce_f64(or_scene* s, const float* rays, uint32_t count, uint32_t* hits);
 
+/* Differentiable rendering.  The per-triangle albedo (3 floats per flattened
+ * triangle: mesh instances in order, primitives in order; NULL clears)
+ * replaces the base colour of Lambertian camera-path vertices.  A weight grid
+ * (same size as volume `vol`; NULL clears) feeds the path weights while the
+ * tracking decisions keep reading the grid: fixed-seed differences over it
+ * are the detached-sampling derivative.  or_adjoint: the path-replay
+ * gradient of sum(d_hdr * hdr) (see pt_oracle.c), added to grad_alb (3 per
+ * triangle) and grad_grid (one per 4-byte word of grid grad_vol). */
+uint32_t or_scene_triangle_count(const or_scene* s);
+int or_scene_set_triangle_albedo(or_scene* s, const float* rgb, uint32_t count);
+int or_scene_set_weight_grid(or_scene* s, uint32_t vol, const void* grid, uint64_t size);
+int or_adjoint(or_scene* s, const st_view* view, const st_render_params* p, const float* d_hdr, double* grad_alb,
+               double* grad_grid, uint32_t grad_vol, int threads);
+
 /* Bit-exact helper exports (for golden vectors). */
 void or_pcg4d(const uint32_t in[4], uint32_t out[4]);
 uint32_t or_xxhash32(uint32_t p);

@@ -196,7 +196,7 @@ EXPORTS = [
     "st_denoiser_create", "st_denoiser_destroy", "st_denoiser_reset", "st_denoiser_accumulated_frames", "st_denoise",
     "st_denoiser_last_timings",
     "st_batch_create", "st_batch_render", "st_batch_autotune", "st_batch_get_config", "st_batch_set_config",
-    "st_batch_destroy", "st_render_batch",
+    "st_batch_destroy", "st_render_batch", "st_scene_set_triangle_albedo", "st_scene_set_volume_grid", "st_adjoint",
 ]
 
 _lib = None
@@ -240,6 +240,9 @@ def load_library(path=LIB_PATH):
     lib.st_batch_get_config.argtypes = [vp, C.POINTER(st_batch_config)]
     lib.st_batch_set_config.argtypes = [vp, C.POINTER(st_batch_config)]
     lib.st_batch_destroy.argtypes = [vp]
+    lib.st_scene_set_triangle_albedo.argtypes = [vp, vp, u32]
+    lib.st_scene_set_volume_grid.argtypes = [vp, u32, vp, C.c_uint64]
+    lib.st_adjoint.argtypes = [vp, C.POINTER(st_view), C.POINTER(st_render_params), vp, vp, vp, u32]
     lib.st_render_batch.argtypes = [vp, C.POINTER(st_render_request), u32]
     if path == LIB_PATH:
         _lib = lib

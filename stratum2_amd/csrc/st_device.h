@@ -187,7 +187,9 @@ __device__ __forceinline__ v3 uv_to_cart(float u, float v) {
 struct Material {
     uint32_t p[4];
     float escale;
-    __device__ __forceinline__ v3 base() const { return mk(un8(p[0], 0), un8(p[0], 8), un8(p[0], 16)); }
+    bool has_bo;  // bo replaces the packed base colour (a per-triangle albedo)
+    v3 bo;
+    __device__ __forceinline__ v3 base() const { return has_bo ? bo : mk(un8(p[0], 0), un8(p[0], 8), un8(p[0], 16)); }
     __device__ __forceinline__ v3 emission() const { return mk(un8(p[1], 0), un8(p[1], 8), un8(p[1], 16)) * escale; }
     __device__ __forceinline__ float spec() const { return un8(p[0], 24); }
     __device__ __forceinline__ float sheen() const { return un8(p[1], 24); }

@@ -69,6 +69,11 @@ struct DevScene {
     uint32_t vinst[kMaxVolumeInstances];
     const st_volume_info* vinfo;   // VolumeInfo records, in the scene's order
     uint32_t vinfo_count;
+    // differentiable rendering: the per-triangle albedo (3 floats per
+    // flattened triangle, null: none) of Lambertian camera paths, and each
+    // instance's first flattened triangle
+    const float* tri_alb;
+    const uint32_t* tri_base;
 };
 
 }  // namespace st

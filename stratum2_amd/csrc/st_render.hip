@@ -112,6 +112,22 @@ struct RenderConsts {
     st_transform inv_cam;  // world to camera (TransformData::inverse of the camera transform)
 };
 
+// The second pass of st_adjoint (path replay): the first pass's fp32 radiance
+// of each path, dLoss/dHDR, and the gradient sinks (null: not wanted).
+struct AdjArgs {
+    const float4* ltot;  // per path
+    const float4* d_hdr; // per pixel, RGBA
+    float* grad_alb;     // 3 per flattened triangle
+    float* grad_grid;    // per 4-byte word of grid grad_vol
+    uint32_t grad_vol, spp;
+};
+
+// dLoss/dL of the path through pixel pix: dLoss/dHDR / spp
+__device__ __forceinline__ v3 adj_pixel_weight(const AdjArgs& a, uint32_t pix) {
+    const float4 d = a.d_hdr[pix];
+    return mk(d.x / (float)a.spp, d.y / (float)a.spp, d.z / (float)a.spp);
+}
+
 // Auxiliary outputs of sample 0's first vertex (StoreAuxillaryData); null = not wanted.
 struct AuxOut {
     uint32_t* visibility;
@@ -390,8 +406,11 @@ __device__ inline uint32_t medium_at(const DevScene& S, v3 pos) {
 // the ray origin; surface hits keep t of the last segment (the shading kernel
 // measures the distance from the origin, intersection.hlsli:244).  One lane
 // per path, grid-stride.
+// ADJ: the path replay of st_adjoint; each collision in the gradient's grid
+// adds its term with the radiance still to come on the path as weight.
+template <bool ADJ = false>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderConsts c, WaveState w, int qsel, uint32_t* spill,
-                                                             unsigned long long* stats) {
+                                                             unsigned long long* stats, AdjArgs adj) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t n = w.counters[qsel];
     const uint32_t* q = w.queue[qsel];
@@ -414,6 +433,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
         uint4 out = make_uint4(0, 0xffffffffu, 0, 0);
         bool scattered = false, done = false;
         v3 spos = mk(0, 0, 0);
+        DensitySink sink{nullptr, 0, mk(0, 0, 0)};
+        if (ADJ && adj.grad_grid) {  // every later contribution carries this ray's tracking factors
+            uint32_t px, py;
+            pixel_of(c, p % c.npix, px, py);
+            const float4 lt = adj.ltot[p], e = w.em[p];
+            sink = DensitySink{adj.grad_grid, adj.grad_vol,
+                               adj_pixel_weight(adj, py * c.W + px) * mk(lt.x - e.x, lt.y - e.y, lt.z - e.z)};
+        }
         // bounded: a ray still crossing boundaries after kMaxMediaSegments
         // traceRay calls ends as a miss (the reference loop is unbounded)
         for (int seg = 0; tmax > 1e-6f && seg < kMaxMediaSegments; seg++) {
@@ -427,8 +454,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
                 const st_transform& iT = S.inv_xf[med];
                 v3 dp3, np3;
                 bool sc;
-                const v3 pp = delta_track<true>(S, m, rng, xf_point(iT, o), xf_vector(iT, dir), h.hit ? h.t : tmax, thr, dp3,
-                                                np3, sc, c.max_null, reads);
+                const v3 pp = delta_track<true, ADJ>(S, m, rng, xf_point(iT, o), xf_vector(iT, dir), h.hit ? h.t : tmax, thr,
+                                                     dp3, np3, sc, c.max_null, reads, &sink);
                 dirpdf *= avg3(dp3);
                 if (sc) {
                     spos = xf_point(S.xf[med], pp);
@@ -466,6 +493,55 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
     if (c.flags & ST_FLAG_COUNT_TRAVERSAL) wave_add_u64(&stats[0], reads);
 }
 
+// One visibility ray through surfaces and media: beta is multiplied by the
+// ratio-tracking transmittance (0 when a surface blocks), neepdf by the
+// tracking pdf.  With ADJ, each collision in the sink's grid adds its term.
+template <bool ADJ>
+__device__ inline void shadow_media_walk(const DevScene& S, const RenderConsts& c, Rng& rng, uint32_t cur, v3 o, v3 d, float tmax,
+                                         v3& beta, float& neepdf, uint32_t* lds, uint32_t* spill, uint32_t stride,
+                                         uint32_t gtid, TraverseStats& ts, uint32_t& segs, uint32_t& reads,
+                                         const DensitySink* sink) {
+    neepdf = 1;
+    for (int seg = 0; tmax > 1e-6f; seg++) {
+        if (seg == kMaxMediaSegments) {  // bounded like the scattering ray: counts as blocked
+            beta = mk(0, 0, 0);
+            neepdf = 0;
+            return;
+        }
+        Hit h = traverse<false, false>(S, o, d, tmax, lds, spill, stride, gtid, ts);
+        segs++;
+        bool is_vol;
+        uint32_t vgn = 0;
+        volume_boxes(S, o, d, h, is_vol, vgn);
+        if (h.hit && !is_vol) {
+            beta = mk(0, 0, 0);
+            neepdf = 0;
+            return;
+        }
+        if (cur != kNoMedium) {
+            const Medium m = load_medium(S, S.inst[cur].type_material_address >> 4);
+            const st_transform& iT = S.inv_xf[cur];
+            v3 dp3, np3;
+            bool sc;
+            delta_track<false, ADJ>(S, m, rng, xf_point(iT, o), xf_vector(iT, d), h.hit ? h.t : tmax, beta, dp3, np3, sc,
+                                    c.max_null, reads, sink);
+            neepdf *= avg3(np3);
+            if (!(beta.x != 0 && beta.y != 0 && beta.z != 0)) return;
+        }
+        if (!h.hit) return;
+        const v3 bp = o + d * h.t;
+        const v3 ng = unpack_normal(vgn);
+        if (dot(ng, d) < 0) {
+            cur = h.id & 0xffffu;
+            o = ray_offset(bp, -ng);
+        } else {
+            cur = kNoMedium;
+            o = ray_offset(bp, ng);
+        }
+        tmax -= h.t;
+    }
+}
+
 // traceVisibilityRay with media (intersection.hlsli:262-338): closest hits,
 // any surface blocks, ratio tracking through each medium crossed; the NEE
 // contribution is divided by the average tracking pdf (testrenderer.slang:
@@ -473,9 +549,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_media(DevScene S, RenderC
 // this uses the vertex's medium and its own random stream keyed like the
 // path's (pixel or light dispatch index, sample) and by path length (offset
 // 0x40000000 | length << 16).  Light tracing keeps the light path in the
-// record's weight slot and adds with weight 1 to the pixel slot.
+// record's weight slot and adds with weight 1 to the pixel slot.  ADJ: the
+// path replay of st_adjoint walks a contributing ray a second time, from the
+// same random stream, with the fp32 contribution as the sink's weight.
+template <bool ADJ = false>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, RenderConsts c, WaveState w, uint32_t* spill,
-                                                              unsigned long long* stats) {
+                                                              unsigned long long* stats, AdjArgs adj) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     const uint32_t n = w.counters[2];
     const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -486,52 +565,31 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_media(DevScene S, Render
         const uint32_t p = __float_as_uint(b.w);
         const uint32_t path = c.light_paths ? __float_as_uint(cc.w) : p;
         const uint32_t plen = w.st[path].y;
-        Rng rng = path_rng(c, path, 0x40000000u | (plen << 16));
-        uint32_t cur = w.med[path];
+        const uint32_t key = 0x40000000u | (plen << 16);
+        Rng rng = path_rng(c, path, key);
         v3 beta = mk(cc.x, cc.y, cc.z);
-        const v3 d = mk(b.x, b.y, b.z);
-        v3 o = mk(a.x, a.y, a.z);
-        float tmax = a.w, neepdf = 1;
-        for (int seg = 0; tmax > 1e-6f; seg++) {
-            if (seg == kMaxMediaSegments) {  // bounded like the scattering ray: counts as blocked
-                beta = mk(0, 0, 0);
-                neepdf = 0;
-                break;
+        float neepdf = 1;
+        shadow_media_walk<false>(S, c, rng, w.med[path], mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, beta, neepdf, lds, spill,
+                                 stride, gtid, ts, segs, reads, nullptr);
+        if (!(all_nonpos(beta) || neepdf <= 0)) {
+            const float wgt = c.light_paths ? 1.0f : cc.w;
+            add_shadow_contribution(w, c.flags, p, beta / neepdf, wgt);
+            if (ADJ && adj.grad_grid && wgt > 0) {
+                const v3 rw = (beta / neepdf) * wgt;
+                const v3 add3 = mk(fmax_(0.0f, rw.x), fmax_(0.0f, rw.y), fmax_(0.0f, rw.z));
+                if (!all_nonpos(add3)) {
+                    uint32_t px, py;
+                    pixel_of(c, p % c.npix, px, py);
+                    const DensitySink sink{adj.grad_grid, adj.grad_vol, adj_pixel_weight(adj, py * c.W + px) * add3};
+                    Rng rng2 = path_rng(c, path, key);
+                    v3 beta2 = mk(cc.x, cc.y, cc.z);
+                    float np2 = 1;
+                    uint32_t segs2 = 0, reads2 = 0;
+                    shadow_media_walk<true>(S, c, rng2, w.med[path], mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, beta2, np2,
+                                            lds, spill, stride, gtid, ts, segs2, reads2, &sink);
+                }
             }
-            Hit h = traverse<false, false>(S, o, d, tmax, lds, spill, stride, gtid, ts);
-            segs++;
-            bool is_vol;
-            uint32_t vgn = 0;
-            volume_boxes(S, o, d, h, is_vol, vgn);
-            if (h.hit && !is_vol) {
-                beta = mk(0, 0, 0);
-                neepdf = 0;
-                break;
-            }
-            if (cur != kNoMedium) {
-                const Medium m = load_medium(S, S.inst[cur].type_material_address >> 4);
-                const st_transform& iT = S.inv_xf[cur];
-                v3 dp3, np3;
-                bool sc;
-                delta_track<false>(S, m, rng, xf_point(iT, o), xf_vector(iT, d), h.hit ? h.t : tmax, beta, dp3, np3, sc,
-                                   c.max_null, reads);
-                neepdf *= avg3(np3);
-                if (!(beta.x != 0 && beta.y != 0 && beta.z != 0)) break;
-            }
-            if (!h.hit) break;
-            const v3 bp = o + d * h.t;
-            const v3 ng = unpack_normal(vgn);
-            if (dot(ng, d) < 0) {
-                cur = h.id & 0xffffu;
-                o = ray_offset(bp, -ng);
-            } else {
-                cur = kNoMedium;
-                o = ray_offset(bp, ng);
-            }
-            tmax -= h.t;
         }
-        if (!(all_nonpos(beta) || neepdf <= 0))
-            add_shadow_contribution(w, c.flags, p, beta / neepdf, c.light_paths ? 1.0f : cc.w);
     }
     wave_add_u64(reinterpret_cast<unsigned long long*>(w.counters + 4) + 1, segs);
     if (c.flags & ST_FLAG_COUNT_TRAVERSAL) wave_add_u64(&stats[2], reads);
@@ -749,9 +807,13 @@ __global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c,
     }
 }
 
-template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA>
+// ADJ: the path replay of st_adjoint, which adds the albedo gradient of each
+// Lambertian vertex with a per-triangle albedo (every contribution after its
+// emission carries one factor albedo_c: dLoss/dL * (the path's first-pass
+// radiance - what it has gathered so far) / albedo_c).
+template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA, bool ADJ = false>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
-                                                       AuxOut aux, int first_wave) {
+                                                       AuxOut aux, int first_wave, AdjArgs adj) {
     const uint32_t n = w.counters[qsel];
     const uint32_t* q = w.queue[qsel];
     uint32_t* qout = w.queue[qsel ^ 1];
@@ -882,7 +944,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 const float thit = MEDIA ? len(sd.pos - mk(r.x, r.y, r.z)) : __uint_as_float(hit.x);
                 const float t16 = q16(thit), pdfa16 = q16(pdfa);
                 const bool live = !MEDIA || !all_nonpos(thr);  // ExtendPath, testrenderer.slang:1062-1063
-                const Material m = load_material(S, sd.material);
+                Material m = load_material(S, sd.material);
+                uint32_t alb_entry = 0xffffffffu;
+                if (LAMBERT && S.tri_alb) apply_triangle_albedo(S, m, ii, prim, alb_entry);
                 if (live && path_len == 1 && first_wave && k == 0)
                     store_aux(c, aux, pix, px, py, hit.y, sd.sn, true, sd.pos, m.base(), t16);
                 const Frame f = make_frame(sd);
@@ -892,6 +956,18 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 if (live && ldin.z > 0) {
                     const v3 e = thr * m.emission();
                     if (any_pos(e)) add_emission(e, pdfa16, cos_in / sq(t16));
+                }
+                if (ADJ && live && alb_entry != 0xffffffffu && adj.grad_alb) {
+                    if (!em_dirty) {
+                        em = w.em[p];
+                        em_dirty = true;
+                    }
+                    const float4 lt = adj.ltot[p];
+                    const v3 wg = adj_pixel_weight(adj, pix) * mk(lt.x - em.x, lt.y - em.y, lt.z - em.z);
+                    float* ga = adj.grad_alb + 3 * (size_t)alb_entry;
+                    if (m.bo.x > 0) atomicAdd(ga, wg.x / m.bo.x);
+                    if (m.bo.y > 0) atomicAdd(ga + 1, wg.y / m.bo.y);
+                    if (m.bo.z > 0) atomicAdd(ga + 2, wg.z / m.bo.z);
                 }
                 bool alive = live && path_len < c.max_depth && any_pos(m.base());
                 // Russian roulette from path length rr_start (0: every bounce, testrenderer.slang:890-892)
@@ -1814,6 +1890,7 @@ static void* g_scratch[kMaxDevices] = {};
 struct WaveBufs {
     DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, pri, psd, rinit, rtemp, med;
     DevBuf vcm, lv_raw, lv, lv_cnt, lv_off, lv_total, scan_tmp;  // bidirectional mode
+    DevBuf ltot;  // st_adjoint: each path's first-pass radiance
     hipStream_t stream = nullptr;        // slot 0: the context's stream
     hipEvent_t reuse_done = nullptr;     // this slot's last wave finished its screen-space reuse
     hipEvent_t resolve_done = nullptr;   // this slot's last wave finished its resolve
@@ -1859,6 +1936,8 @@ struct st_scene {
     const uint32_t* inst_first_dev = nullptr;
     void* bvh_nodes = nullptr;  // hipMalloc'ed by the device builder, else null
     void* bvh_tris = nullptr;
+    std::vector<uint64_t> vol_bytes;  // each grid's size
+    DevBuf tri_alb;                   // st_scene_set_triangle_albedo's copy
     // emitter table, rebuilt when the instances move
     const uint32_t* light_tri_base = nullptr;
     float4* light_rec = nullptr;
@@ -2266,6 +2345,7 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
             roots[v] = validate_nanovdb((const uint8_t*)g.data, g.size_bytes);
             D.vol[v] = s->upload<uint8_t>(g.data, g.size_bytes);
             D.vol_root[v] = roots[v];
+            s->vol_bytes.push_back(g.size_bytes);
         }
         D.vol_count = d->volume_count;
         s->inv_xf.resize(d->instance_count + 1);
@@ -2341,6 +2421,8 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
         s->ntri = ntri;
         s->tri_inst = s->upload<uint32_t>(owner.data(), 4 * owner.size());
         s->inst_first_dev = s->upload<uint32_t>(s->inst_first.data(), 4 * s->inst_first.size());
+        D.tri_base = s->inst_first_dev;  // the per-triangle albedo's index base of each instance
+        D.tri_alb = nullptr;
         D.light_inst = s->upload<uint32_t>(lights.data(), 4 * lights.size());
         D.light_count = (uint32_t)lights.size();
         if (D.light_count > 0) {
@@ -2547,7 +2629,16 @@ int st_scene_stats(const st_scene* s, uint64_t out[4]) {
     return ST_OK;
 }
 
-using ShadeFn = void (*)(DevScene, RenderConsts, WaveState, int, AuxOut, int);
+using ShadeFn = void (*)(DevScene, RenderConsts, WaveState, int, AuxOut, int, AdjArgs);
+// the path replay of st_adjoint (plain NEE)
+static ShadeFn shade_kernel_adj(bool lambert, bool alias, bool media) {
+    static const ShadeFn t[8] = {
+        k_shade<false, false, false, false, true>, k_shade<false, true, false, false, true>,
+        k_shade<true, false, false, false, true>,  k_shade<true, true, false, false, true>,
+        k_shade<false, false, false, true, true>,  k_shade<false, true, false, true, true>,
+        k_shade<true, false, false, true, true>,   k_shade<true, true, false, true, true>};
+    return t[(media ? 4 : 0) | (lambert ? 2 : 0) | (alias ? 1 : 0)];
+}
 static ShadeFn shade_kernel(bool lambert, bool alias, bool restir, bool media) {
     if (media) {  // ReSTIR DI with media is rejected before this
         static const ShadeFn mt[4] = {k_shade<false, false, false, true>, k_shade<false, true, false, true>,
@@ -2654,6 +2745,16 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
 
 // Argument checks of a camera-path render (the path tracer of
 // TestRenderer::render); ST_OK when it can run.
+// A per-triangle albedo serves Lambertian camera paths with plain NEE.
+static int check_triangle_albedo(const st_scene* s, uint32_t flags) {
+    if (!s->d.tri_alb) return ST_OK;
+    if (!(flags & ST_FLAG_LAMBERTIAN))
+        return set_error(ST_ERROR_UNSUPPORTED, "a per-triangle albedo needs ST_FLAG_LAMBERTIAN");
+    if (flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC | ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE))
+        return set_error(ST_ERROR_UNSUPPORTED, "a per-triangle albedo serves camera paths with plain NEE");
+    return ST_OK;
+}
+
 static int check_camera(st_scene* s, const st_render_params* p) {
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && !(p->flags & ST_FLAG_RESTIR_DI))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");
@@ -2665,7 +2766,7 @@ static int check_camera(st_scene* s, const st_render_params* p) {
     if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
     if (s->d.vinst_count > 0 && (p->flags & ST_FLAG_RESTIR_DI))
         return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI is not supported in scenes with media");
-    return ST_OK;
+    return check_triangle_albedo(s, p->flags);
 }
 
 // Samples per wave of a camera-path render: params.samples_per_wave, or as
@@ -2704,6 +2805,44 @@ __global__ void k_store_rays(const uint32_t* counters, unsigned long long* out) 
     if (threadIdx.x < 2) out[threadIdx.x] = reinterpret_cast<const unsigned long long*>(counters + 4)[threadIdx.x];
 }
 
+// The constants of a camera-path render shared by every wave.
+static RenderConsts camera_consts(const st_scene* s, const st_view& view, const st_render_params* p) {
+    RenderConsts c;
+    std::memset(&c, 0, sizeof c);
+    c.W = p->width;
+    c.H = p->height;
+    c.npix = p->width * p->height;
+    c.seed = p->seed;
+    c.max_depth = p->max_depth;
+    c.min_len = p->min_path_length;
+    c.flags = p->flags;
+    c.rr_start = p->rr_start_depth;
+    c.env_prob = p->environment_sample_probability;
+    c.view = view;
+    c.inv_cam = inverse_transform(view.camera_to_world);
+    c.max_null = p->max_null_collisions;
+    // the camera's medium: the last volume whose object-space bounds hold the
+    // camera position (TestRenderer.cpp:431-446)
+    c.view_medium = kNoMedium;
+    const st_transform& T = view.camera_to_world;
+    const float cam[3] = {T.m[0][3], T.m[1][3], T.m[2][3]};
+    for (const st_volume_info& vi : s->volume_info) {
+        if (vi.instance_index >= s->d.inst_count) continue;
+        const st_transform& iT = s->inv_xf[vi.instance_index];
+        float lp[3];
+        for (int r = 0; r < 3; r++) lp[r] = iT.m[r][0] * cam[0] + iT.m[r][1] * cam[1] + iT.m[r][2] * cam[2] + iT.m[r][3];
+        if (lp[0] >= vi.min[0] && lp[1] >= vi.min[1] && lp[2] >= vi.min[2] && lp[0] <= vi.max[0] && lp[1] <= vi.max[1] &&
+            lp[2] <= vi.max[2])
+            c.view_medium = vi.instance_index;
+    }
+    c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
+    c.cand = p->candidate_samples;
+    c.max_m = p->max_m;
+    c.spatial = p->spatial_neighbours;
+    c.radius = p->spatial_radius;
+    return c;
+}
+
 static void enqueue_camera(st_context* ctx, const CameraJob& j) {
     st_scene* s = j.s;
     const st_render_params* p = &j.p;
@@ -2722,43 +2861,9 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
         HIP_CHECK(hipEventRecord(ctx->start_done, s0));
         for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(j.slot[sl]->stream, ctx->start_done, 0));
     }
-    RenderConsts c;
-    std::memset(&c, 0, sizeof c);
-    c.W = p->width;
-    c.H = p->height;
-    c.npix = npix;
-    c.seed = p->seed;
-    c.max_depth = p->max_depth;
-    c.min_len = p->min_path_length;
-    c.flags = p->flags;
-    c.rr_start = p->rr_start_depth;
-    c.env_prob = p->environment_sample_probability;
-    c.view = j.view;
-    c.inv_cam = inverse_transform(j.view.camera_to_world);
-    c.max_null = p->max_null_collisions;
-    // the camera's medium: the last volume whose object-space bounds hold the
-    // camera position (TestRenderer.cpp:431-446)
-    c.view_medium = kNoMedium;
-    {
-        const st_transform& T = j.view.camera_to_world;
-        const float cam[3] = {T.m[0][3], T.m[1][3], T.m[2][3]};
-        for (const st_volume_info& vi : s->volume_info) {
-            if (vi.instance_index >= s->d.inst_count) continue;
-            const st_transform& iT = s->inv_xf[vi.instance_index];
-            float lp[3];
-            for (int r = 0; r < 3; r++) lp[r] = iT.m[r][0] * cam[0] + iT.m[r][1] * cam[1] + iT.m[r][2] * cam[2] + iT.m[r][3];
-            if (lp[0] >= vi.min[0] && lp[1] >= vi.min[1] && lp[2] >= vi.min[2] && lp[0] <= vi.max[0] &&
-                lp[1] <= vi.max[1] && lp[2] <= vi.max[2])
-                c.view_medium = vi.instance_index;
-        }
-    }
-    c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
+    RenderConsts c = camera_consts(s, j.view, p);
     const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
     const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
-    c.cand = p->candidate_samples;
-    c.max_m = p->max_m;
-    c.spatial = p->spatial_neighbours;
-    c.radius = p->spatial_radius;
     const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
     unsigned long long* stats = j.stats;
     // Wave i runs in slot i % slots on that slot's stream; the slot's
@@ -2783,7 +2888,7 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
         for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
             {
                 Timed tm(j.timed ? ctx : nullptr, 0, sm);
-                if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats);
+                if (media) hipLaunchKernelGGL(k_trace_media<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats, AdjArgs{});
                 else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                 else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
             }
@@ -2801,7 +2906,8 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
                 AuxOut aux{nullptr, nullptr, nullptr, nullptr};
                 if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
                 auto* kfn = shade_kernel(lambert, alias, restir && !reuse, media);
-                hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0);
+                hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0,
+                                   AdjArgs{});
             }
             if (reuse && b == 0) {
                 const int rgrid2 = std::min<int>((npix + kShadeBlock - 1) / kShadeBlock, ctx->cus * 8);
@@ -2821,7 +2927,7 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
             }
             {
                 Timed tm(j.timed ? ctx : nullptr, 2, sm);
-                if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats);
+                if (media) hipLaunchKernelGGL(k_shadow_media<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats, AdjArgs{});
                 else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
                 else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
             }
@@ -2885,6 +2991,154 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         enqueue_camera(ctx, j);
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
         finish_timings(ctx, slots, o->ray_counts);
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+// ------------------------------------------------------------ differentiable rendering
+int st_scene_set_triangle_albedo(st_scene* s, const float* rgb, uint32_t count) {
+    if (!s) return set_error(ST_ERROR_INVALID_ARGUMENT, "null scene");
+    try {
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        if (!rgb) {
+            if (s->d.tri_alb) s->generation++;  // captured graphs hold the scene's constants
+            s->d.tri_alb = nullptr;
+            return ST_OK;
+        }
+        if ((uint64_t)count != s->ntri)
+            return set_error(ST_ERROR_INVALID_ARGUMENT, "the albedo table holds 3 floats per flattened triangle");
+        order_after_caller(s->ctx);
+        s->tri_alb.reserve(12ull * count + 16);
+        HIP_CHECK(hipMemcpyAsync(s->tri_alb.p, rgb, 12ull * count, hipMemcpyDefault, s->ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        if (s->d.tri_alb != s->tri_alb.as<float>()) s->generation++;
+        s->d.tri_alb = s->tri_alb.as<float>();
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_scene_set_volume_grid(st_scene* s, uint32_t volume, const void* grid, uint64_t bytes) {
+    if (!s || !grid) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (volume >= s->vol_bytes.size()) return set_error(ST_ERROR_INVALID_ARGUMENT, "no such volume");
+    if (bytes != s->vol_bytes[volume]) return set_error(ST_ERROR_INVALID_ARGUMENT, "the grid must keep its size");
+    try {
+        HIP_CHECK(hipSetDevice(s->ctx->device));
+        order_after_caller(s->ctx);
+        std::vector<uint8_t> host(bytes);
+        HIP_CHECK(hipMemcpyAsync(host.data(), grid, bytes, hipMemcpyDefault, s->ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+        // the same tree: the root (and with it every node offset) where it was
+        if (validate_nanovdb(host.data(), bytes) != s->d.vol_root[volume])
+            return set_error(ST_ERROR_INVALID_ARGUMENT, "the grid must keep its tree layout");
+        HIP_CHECK(hipMemcpy(const_cast<uint8_t*>(s->d.vol[volume]), host.data(), bytes, hipMemcpyHostToDevice));
+        return ST_OK;
+    } catch (const std::invalid_argument& e) {
+        return set_error(ST_ERROR_INVALID_ARGUMENT, e.what());
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+// A wave's fp32 radiance per path, kept for the replay
+__global__ void k_adj_total(uint32_t n, const float4* em, float4* ltot) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) ltot[p] = em[p];
+}
+
+// Path-replay adjoint.  Every wave is rendered twice from the same seeds with
+// fp32 NEE: the first pass keeps each path's radiance, the second replays the
+// path and, at each differentiable event, adds dLoss/dL times the radiance
+// still to come times the event's log-derivative (k_shade<.., ADJ>,
+// k_trace_media<ADJ>, k_shadow_media<ADJ>).  Waves are independent (the
+// gradients are sums), so they run in the context's four slots with no order
+// between them.
+int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, const float* d_hdr, float* grad_albedo,
+               float* grad_grid, uint32_t grad_volume) {
+    if (!s || !view || !p || !d_hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
+    if (p->flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC | ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE))
+        return set_error(ST_ERROR_UNSUPPORTED, "the adjoint replays camera paths with plain NEE");
+    if (grad_albedo && (!s->d.tri_alb || !(p->flags & ST_FLAG_LAMBERTIAN)))
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "albedo gradients need a per-triangle albedo and ST_FLAG_LAMBERTIAN");
+    if (grad_grid && grad_volume >= s->d.vol_count) return set_error(ST_ERROR_INVALID_ARGUMENT, "no such volume");
+    const int rc = check_camera(s, p);
+    if (rc != ST_OK) return rc;
+    st_context* ctx = s->ctx;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        const uint32_t npix = p->width * p->height;
+        const uint32_t k = wave_samples(p, p->samples_per_wave);
+        if (k == 0) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
+        const uint64_t paths = (uint64_t)npix * k;
+        const uint32_t waves = (p->spp + k - 1) / k;
+        const int slots = (int)std::min<uint32_t>(kWaveSlots, waves);
+        const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
+        const bool media = s->d.vinst_count > 0, lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+        const bool alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+        order_after_caller(ctx);
+        for (int sl = 0; sl < slots; sl++) {
+            reserve_wave(ctx->slot[sl], paths, false, paths, tgrid);
+            ctx->slot[sl].ltot.reserve(paths * 16);
+        }
+        ctx->stats.reserve(64);
+        hipStream_t s0 = ctx->slot[0].stream;
+        for (int sl = 0; sl < slots; sl++)
+            hipLaunchKernelGGL(k_fill_u32, dim3(kCounterWords / 256), dim3(256), 0, s0, ctx->slot[sl].counters.as<uint32_t>(),
+                               (uint32_t)kCounterWords, 0u);
+        if (slots > 1) {
+            HIP_CHECK(hipEventRecord(ctx->start_done, s0));
+            for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(ctx->slot[sl].stream, ctx->start_done, 0));
+        }
+        RenderConsts c = camera_consts(s, *view, p);
+        c.flags |= ST_FLAG_FLOAT_NEE;
+        unsigned long long* stats = ctx->stats.as<unsigned long long>();
+        const AuxOut aux{nullptr, nullptr, nullptr, nullptr};
+        auto* shade = shade_kernel(lambert, alias, false, media);
+        auto* shade_adj = shade_kernel_adj(lambert, alias, media);
+        uint32_t wi = 0;
+        for (uint32_t base = 0; base < p->spp; base += k, wi++) {
+            WaveBufs& B = ctx->slot[wi % slots];
+            hipStream_t sm = B.stream;
+            WaveState w = wave_state(B, ctx);
+            uint32_t* spill = B.spill.as<uint32_t>();
+            c.k = std::min(k, p->spp - base);
+            c.sample_base = base;
+            c.wave_paths = npix * c.k;
+            const AdjArgs adj{B.ltot.as<float4>(), reinterpret_cast<const float4*>(d_hdr), grad_albedo, grad_grid, grad_volume,
+                              p->spp};
+            const int cgrid = std::min<int>((c.wave_paths + kShadeBlock - 1) / kShadeBlock, ctx->cus * 16);
+            for (int pass = 0; pass < 2; pass++) {
+                const bool replay = pass == 1;
+                hipLaunchKernelGGL(k_camera, dim3(cgrid), dim3(kShadeBlock), 0, sm, c, w);
+                int qsel = 0;
+                for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
+                    if (media)
+                        hipLaunchKernelGGL(replay ? k_trace_media<true> : k_trace_media<false>, dim3(tgrid), dim3(kTraceBlock), 0,
+                                           sm, s->d, c, w, qsel, spill, stats, adj);
+                    else
+                        hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                    hipLaunchKernelGGL(replay ? shade_adj : shade, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, 0,
+                                       adj);
+                    if (media)
+                        hipLaunchKernelGGL(replay ? k_shadow_media<true> : k_shadow_media<false>, dim3(tgrid), dim3(kTraceBlock),
+                                           0, sm, s->d, c, w, spill, stats, adj);
+                    else
+                        hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill,
+                                           stats);
+                    hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
+                    qsel ^= 1;
+                }
+                if (!replay) {
+                    const int g2 = std::min<int>((c.wave_paths + 255) / 256, ctx->cus * 8);
+                    hipLaunchKernelGGL(k_adj_total, dim3(g2), dim3(256), 0, sm, c.wave_paths, w.em, B.ltot.as<float4>());
+                }
+            }
+            HIP_CHECK(hipGetLastError());
+        }
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
@@ -3158,6 +3412,7 @@ int st_render_batch(st_context* ctx, const st_render_request* requests, uint32_t
 
 static int render_light(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
+    if (check_triangle_albedo(s, p->flags) != ST_OK) return ST_ERROR_UNSUPPORTED;
     if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
     if (!(p->light_subpath_factor >= 0) || !(p->light_subpath_factor <= 64))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "light_subpath_factor must be in [0, 64]");
@@ -3235,7 +3490,7 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
             for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
                 {
                     Timed tm(ctx, 0, sm);
-                    if (media) hipLaunchKernelGGL(k_trace_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats);
+                    if (media) hipLaunchKernelGGL(k_trace_media<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats, AdjArgs{});
                     else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                     else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                 }
@@ -3247,7 +3502,7 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
                 }
                 {
                     Timed tm(ctx, 2, sm);
-                    if (media) hipLaunchKernelGGL(k_shadow_media, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats);
+                    if (media) hipLaunchKernelGGL(k_shadow_media<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats, AdjArgs{});
                     else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
                     else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
                 }
@@ -3277,6 +3532,7 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
 // resolve.  Both passes of a sample run on its slot's stream.
 static int render_vc(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
+    if (check_triangle_albedo(s, p->flags) != ST_OK) return ST_ERROR_UNSUPPORTED;
     if (s->d.vinst_count > 0 || (p->flags & (ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_FLOAT_NEE | ST_FLAG_DIRECT_ONLY)) ||
         p->min_path_length)
         return set_error(ST_ERROR_UNSUPPORTED, "the bidirectional mode supports surfaces with plain NEE only");

@@ -160,7 +160,20 @@ __device__ __forceinline__ Material load_material(const DevScene& S, uint32_t ad
     m.p[2] = w.z;
     m.p[3] = w.w;
     m.escale = __uint_as_float(S.mat[addr / 4 + 8]);
+    m.has_bo = false;
+    m.bo = mk(0, 0, 0);
     return m;
+}
+
+// The per-triangle albedo (st_scene_set_triangle_albedo) of a triangle hit,
+// replacing the material's base colour; entry = the triangle's index in the
+// flattened order (mesh instances in order, primitives in order).
+__device__ __forceinline__ void apply_triangle_albedo(const DevScene& S, Material& m, uint32_t inst, uint32_t prim,
+                                                      uint32_t& entry) {
+    entry = S.tri_base[inst] + prim;
+    const float* a = S.tri_alb + 3 * (size_t)entry;
+    m.bo = mk(a[0], a[1], a[2]);
+    m.has_bo = true;
 }
 
 // ------------------------------------------------------------- environment

@@ -23,9 +23,11 @@ __device__ __forceinline__ uint64_t g64(const uint8_t* b, uint64_t o) {
 }
 __device__ __forceinline__ float gf(const uint8_t* b, uint64_t o) { return *reinterpret_cast<const float*>(b + o); }
 
-// Value at integer index (i, j, k): pnanovdb_readaccessor_get_value_address +
-// pnanovdb_read_float for a float grid.  root = byte offset of the root node.
-__device__ inline float nvdb_read(const uint8_t* b, uint64_t root, int i, int j, int k) {
+// Byte address of the value at integer index (i, j, k):
+// pnanovdb_readaccessor_get_value_address for a float grid (a leaf voxel, a
+// tile of an internal node or of the root, or the background).  root = byte
+// offset of the root node.
+__device__ inline uint64_t nvdb_addr(const uint8_t* b, uint64_t root, int i, int j, int k) {
     const uint32_t ntiles = g32(b, root + 24);
     const uint64_t key = (uint64_t)((uint32_t)k >> 12) | ((uint64_t)((uint32_t)j >> 12) << 21) |
                          ((uint64_t)((uint32_t)i >> 12) << 42);
@@ -33,18 +35,22 @@ __device__ inline float nvdb_read(const uint8_t* b, uint64_t root, int i, int j,
         const uint64_t tile = root + 64 + 32 * (uint64_t)t;
         if (g64(b, tile) != key) continue;
         const int64_t child = (int64_t)g64(b, tile + 8);
-        if (child == 0) return gf(b, tile + 20);
+        if (child == 0) return tile + 20;
         const uint64_t up = root + (uint64_t)child;
         const uint32_t n2 = (uint32_t)(((i & 4095) >> 7) << 10) + (uint32_t)(((j & 4095) >> 7) << 5) + (uint32_t)((k & 4095) >> 7);
-        if (!((g32(b, up + 4128 + 4 * (n2 >> 5)) >> (n2 & 31)) & 1)) return gf(b, up + 8256 + 8 * (uint64_t)n2);
+        if (!((g32(b, up + 4128 + 4 * (n2 >> 5)) >> (n2 & 31)) & 1)) return up + 8256 + 8 * (uint64_t)n2;
         const uint64_t lo = up + g64(b, up + 8256 + 8 * (uint64_t)n2);
         const uint32_t n1 = (uint32_t)(((i & 127) >> 3) << 8) + (uint32_t)(((j & 127) >> 3) << 4) + (uint32_t)((k & 127) >> 3);
-        if (!((g32(b, lo + 544 + 4 * (n1 >> 5)) >> (n1 & 31)) & 1)) return gf(b, lo + 1088 + 8 * (uint64_t)n1);
+        if (!((g32(b, lo + 544 + 4 * (n1 >> 5)) >> (n1 & 31)) & 1)) return lo + 1088 + 8 * (uint64_t)n1;
         const uint64_t leaf = lo + g64(b, lo + 1088 + 8 * (uint64_t)n1);
         const uint32_t n0 = (uint32_t)((i & 7) << 6) + (uint32_t)((j & 7) << 3) + (uint32_t)(k & 7);
-        return gf(b, leaf + 96 + 4 * (uint64_t)n0);
+        return leaf + 96 + 4 * (uint64_t)n0;
     }
-    return gf(b, root + 28);  // background
+    return root + 28;  // background
+}
+// pnanovdb_read_float of that address
+__device__ inline float nvdb_read(const uint8_t* b, uint64_t root, int i, int j, int k) {
+    return gf(b, nvdb_addr(b, root, i, j, k));
 }
 
 // Grid map (GridData::mMap at byte 296: matf 0, invmatf 36, vecf 72),
@@ -108,14 +114,27 @@ __device__ __forceinline__ v3 expv(v3 a) { return mk(exp_(a.x), exp_(a.y), exp_(
 __device__ __forceinline__ float avg3(v3 a) { return (a.x + a.y + a.z) / 3; }
 __device__ __forceinline__ float max3(v3 a) { return fmax_(fmax_(a.x, a.y), a.z); }
 
+// Density-gradient sink of the path-replay adjoint: at each collision in
+// grid `vol`, grad[value address / 4] += sum_c wgt_c * dlog(beta_c)/d(value),
+// with wgt = dLoss/dL times the radiance that the collision's factor
+// multiplies.  Tracking decisions and pdfs are not differentiated
+// (detached sampling).
+struct DensitySink {
+    float* grad;
+    uint32_t vol;
+    v3 wgt;
+};
+
 // deltaTrack<SCATTER> (medium.hlsli:104-181): origin/dir in the medium
 // instance's object space.  Multiplies beta by the transmittance (times
 // sigma_t * sigma_s at a real collision); returns the scattering position in
 // object space when *scattered.
-// *reads counts the grid reads (collisions) for the traffic model.
-template <bool SCATTER>
+// *reads counts the grid reads (collisions) for the traffic model.  With
+// ADJ, each collision in the sink's grid adds its gradient term.
+template <bool SCATTER, bool ADJ = false>
 __device__ inline v3 delta_track(const DevScene& S, const Medium& m, Rng& rng, v3 origin, v3 dir, float tmax, v3& beta,
-                                 v3& dir_pdf, v3& nee_pdf, bool& scattered, uint32_t max_null, uint32_t& reads) {
+                                 v3& dir_pdf, v3& nee_pdf, bool& scattered, uint32_t max_null, uint32_t& reads,
+                                 const DensitySink* sink = nullptr) {
     scattered = false;
     dir_pdf = mk(1, 1, 1);
     nee_pdf = mk(1, 1, 1);
@@ -156,19 +175,30 @@ __device__ inline v3 delta_track(const DevScene& S, const Medium& m, Rng& rng, v
         origin = origin + dir * t;
         tmax -= t;
         reads++;
-        const float dv = nvdb_read(g, root, (int)floorf(origin.x), (int)floorf(origin.y), (int)floorf(origin.z));
+        const uint64_t va = nvdb_addr(g, root, (int)floorf(origin.x), (int)floorf(origin.y), (int)floorf(origin.z));
+        const float dv = gf(g, va);
         const v3 sig_t = m.density * dv;
         const v3 sig_s = m.albedo();  // readAlbedo is 1 without an albedo volume
         const v3 tr = expv((-maj) * t) * invmax;
         bool real = false;
         if (SCATTER) real = rng.nextf().x < comp(sig_t, ch) * invc;
+        const bool sink_here = ADJ && sink->grad && sink->vol == m.dens_vol;
         if (real) {
+            // beta_c gains sigma_t,c = density_c * value: dlog = 1 / value
+            if (sink_here && dv > 0) atomicAdd(sink->grad + va / 4, (sink->wgt.x + sink->wgt.y + sink->wgt.z) / dv);
             beta = beta * ((tr * sig_t) * sig_s);
             dir_pdf = dir_pdf * (tr * sig_t);
             scattered = true;
             return index_to_world(g, origin);
         }
         const v3 d = maj - sig_t;
+        if (sink_here) {  // beta_c gains maj_c - sigma_t,c: dlog = -density_c / (maj_c - sigma_t,c)
+            float gs = 0;
+            if (d.x != 0) gs -= sink->wgt.x * m.density.x / d.x;
+            if (d.y != 0) gs -= sink->wgt.y * m.density.y / d.y;
+            if (d.z != 0) gs -= sink->wgt.z * m.density.z / d.z;
+            atomicAdd(sink->grad + va / 4, gs);
+        }
         beta = beta * (tr * d);
         dir_pdf = dir_pdf * (tr * d);
         nee_pdf = nee_pdf * (tr * maj);

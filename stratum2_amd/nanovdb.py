@@ -25,9 +25,14 @@ def _mask_bytes(bits):
     return np.packbits(bits.astype(np.uint8), bitorder="little").tobytes()
 
 
-def write_float_grid(dense, voxel_size=1.0, background=0.0, name="density"):
+def write_float_grid(dense, voxel_size=1.0, background=0.0, name="density", all_leaves=False, return_offsets=False):
     """dense: float32 array indexed [i, j, k] with 0 <= i, j, k < 4096.
-    Returns the grid as a uint8 numpy array."""
+    Returns the grid as a uint8 numpy array.  all_leaves: store every leaf
+    that covers the dense array, active or not, so that the tree layout (and
+    each voxel's value address) does not depend on the values.
+    return_offsets: also return an int64 array of dense's shape holding the
+    byte offset of each voxel's value in the grid (-1 where no leaf holds
+    it)."""
     dense = np.asarray(dense, np.float32)
     nx, ny, nz = dense.shape
     assert max(nx, ny, nz) <= 4096
@@ -36,6 +41,8 @@ def write_float_grid(dense, voxel_size=1.0, background=0.0, name="density"):
     active = d != background
     LX, LY, LZ = [s // 8 for s in d.shape]
     leaf_active = active.reshape(LX, 8, LY, 8, LZ, 8).any(axis=(1, 3, 5))
+    if all_leaves:
+        leaf_active[:(nx + 7) // 8, :(ny + 7) // 8, :(nz + 7) // 8] = True
     # lower nodes: 128^3 voxels = 16^3 leaves
     WX, WY, WZ = LX // 16, LY // 16, LZ // 16
     lower_has = leaf_active.reshape(WX, 16, WY, 16, WZ, 16).any(axis=(1, 3, 5))
@@ -168,7 +175,18 @@ def write_float_grid(dense, voxel_size=1.0, background=0.0, name="density"):
     put("I", o + 636, GRID_TYPE_FLOAT)
     put("q", o + 640, 0)
     put("I", o + 648, 0)
-    return np.frombuffer(bytes(buf), np.uint8).copy()
+    grid = np.frombuffer(bytes(buf), np.uint8).copy()
+    if not return_offsets:
+        return grid
+    offs = np.full(d.shape, -1, np.int64)
+    local = 96 + 4 * np.arange(512, dtype=np.int64).reshape(8, 8, 8)  # n = i*64 + j*8 + k
+    for (x, y, z), o in leaf_index.items():
+        offs[8 * x:8 * x + 8, 8 * y:8 * y + 8, 8 * z:8 * z + 8] = o + local
+    return grid, offs[:nx, :ny, :nz].copy()
+
+
+ROOT_OFFSET = GRID_SIZE + TREE_SIZE  # where write_float_grid puts the root node
+ROOT_MAX_OFFSET = ROOT_OFFSET + 36   # the root's maximum value: the tracking majorant
 
 
 def fbm_density(n=256, seed=0, octaves=4, threshold=0.4):

@@ -84,6 +84,43 @@ class Scene:
         abi.check(self.lib, self.lib.st_scene_update(self.h, xf.ctypes.data, C.byref(ms)), "st_scene_update")
         return ms.value
 
+    def set_triangle_albedo(self, rgb):
+        """st_scene_set_triangle_albedo: (triangles, 3) float32 device tensor
+        (copied), or None to remove the table."""
+        if rgb is None:
+            abi.check(self.lib, self.lib.st_scene_set_triangle_albedo(self.h, None, 0), "st_scene_set_triangle_albedo")
+            return
+        torch = _torch()
+        n = rgb.shape[0]
+        _check_tensor(rgb, "rgb", (n, 3), torch.float32, self.ctx.device)
+        _enter(self.ctx.device)
+        abi.check(self.lib, self.lib.st_scene_set_triangle_albedo(self.h, rgb.data_ptr(), n), "st_scene_set_triangle_albedo")
+
+    def set_volume_grid(self, volume, grid):
+        """st_scene_set_volume_grid: new values of grid `volume` as a uint8
+        device tensor of the grid's byte size and tree layout."""
+        torch = _torch()
+        _check_tensor(grid, "grid", (grid.numel(),), torch.uint8, self.ctx.device)
+        _enter(self.ctx.device)
+        abi.check(self.lib, self.lib.st_scene_set_volume_grid(self.h, volume, grid.data_ptr(), grid.numel()),
+                  "st_scene_set_volume_grid")
+
+    def adjoint(self, view, params, d_hdr, grad_albedo=None, grad_grid=None, grad_volume=0):
+        """st_adjoint: adds the gradient of sum(d_hdr * hdr) to grad_albedo
+        ((triangles, 3) float32) and grad_grid (float32, one per 4-byte word of
+        the grid); d_hdr is (H, W, 4) float32.  All device tensors."""
+        torch = _torch()
+        W, H = params.width, params.height
+        _check_tensor(d_hdr, "d_hdr", (H, W, 4), torch.float32, self.ctx.device)
+        for name, t in (("grad_albedo", grad_albedo), ("grad_grid", grad_grid)):
+            if t is not None:
+                _check_tensor(t, name, tuple(t.shape), torch.float32, self.ctx.device)
+        _enter(self.ctx.device)
+        abi.check(self.lib, self.lib.st_adjoint(self.h, C.byref(view), C.byref(params), d_hdr.data_ptr(),
+                                                grad_albedo.data_ptr() if grad_albedo is not None else None,
+                                                grad_grid.data_ptr() if grad_grid is not None else None, grad_volume),
+                  "st_adjoint")
+
     def stats(self):
         out = (C.c_uint64 * 4)()
         abi.check(self.lib, self.lib.st_scene_stats(self.h, out), "st_scene_stats")

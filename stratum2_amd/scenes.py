@@ -576,6 +576,24 @@ def cloud(seed=0, n=256, sigma_scale=20.0, albedo=0.9, anisotropy=0.3, env=True,
     return sd
 
 
+def cornell_cloud(seed=0, n=64, sigma_scale=6.0, albedo=0.8, anisotropy=0.2, fbm_threshold=0.3, density=None,
+                  light_radiance=15.0):
+    """The seeded Cornell box with an n^3 density grid (seeded fBm, or
+    `density`, indexed [i, j, k]) filling the box [-0.6, 0.95, -0.7] -
+    [0.6, 1.75, 0.3] above the block and the sphere, for differentiable
+    rendering: the grid stores every leaf (write_float_grid(all_leaves=True)),
+    so each voxel's value address does not depend on the values.  Returns
+    (SceneData, value byte offsets of the n^3 voxels)."""
+    from . import nanovdb
+    sd = cornell_box(seed, light_radiance)
+    if density is None:
+        density = nanovdb.fbm_density(n=n, seed=seed, octaves=4, threshold=fbm_threshold)
+    grid, offs = nanovdb.write_float_grid(np.asarray(density, np.float32), voxel_size=1.0, all_leaves=True,
+                                          return_offsets=True)
+    add_medium_box(sd, (sigma_scale,) * 3, (albedo,) * 3, anisotropy, (-0.6, 0.95, -0.7), (0.6, 1.75, 0.3), grid=grid)
+    return sd, offs
+
+
 def cloud_view(width=1024, height=1024):
     return make_view(width, height, position=(0.0, 0.0, 1.6))
 
