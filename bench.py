@@ -31,6 +31,11 @@ four render the spheres scene as the headline does, one in four the cloud as
 beside the same requests rendered one after another with st_render.  Launch
 shapes come from the package's autotuner, run before the timed region
 (--no-autotune: the library defaults).
+--workload adjoint times the path-replay adjoint (st_adjoint): one step = the
+gradient of an image loss over 16 cameras at 512x512, 8 spp, maxPathLength 8
+(RR from 3), with respect to the per-triangle albedo (5144 triangles) and a
+64^3 density grid of the Cornell box with a cloud; reported in ms per
+gradient and as a multiple of the 16 forward renders.
 --copy-calibration runs a plain device copy of known size (for scaling the
 PMC byte counters) and exits.
 """
@@ -58,7 +63,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-wave", type=int, default=0, help="0: library default")
     ap.add_argument("--no-restir", action="store_true", help="plain NEE instead of ReSTIR DI")
-    ap.add_argument("--workload", choices=["spheres", "cloud", "svgf", "batch"], default="spheres")
+    ap.add_argument("--workload", choices=["spheres", "cloud", "svgf", "batch", "adjoint"], default="spheres")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="batch workload: the library's default launch shapes")
     ap.add_argument("--device-bvh", action="store_true", help="rebuild the BVH on the GPU (st_scene_update) before timing")
@@ -89,6 +94,9 @@ def main():
         return
     if args.workload == "batch":
         bench_batch(args, log)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args, log)
         return
     cloud = args.workload == "cloud"
     if cloud:
@@ -443,6 +451,150 @@ def bench_batch(args, log):
     }
     b.close()
     print(json.dumps(line), flush=True)
+
+
+def bench_adjoint(args, log):
+    """One step = one gradient: st_adjoint over 16 cameras (512x512, 8 spp,
+    maxPathLength 8, RR from 3, Lambertian, fp32 NEE) of a random linear
+    image loss, with respect to the per-triangle albedo and the 64^3 density
+    grid.  The 16 forward renders (st_render, same arguments) are timed
+    beside it; roofline figures come from one more adjoint per camera with
+    ST_FLAG_SERIAL_WAVES (exclusive HIP-event times, both passes)."""
+    import numpy as np
+    import torch
+    from stratum2_amd import abi, scenes
+    from stratum2_amd.renderer import Context, Scene, TestRenderer
+    W = H = 512
+    spp, depth, ncam = 8, 8, 16
+    ctx = Context(0)
+    sd, offs = scenes.cornell_cloud(seed=0, n=64)
+    sc = Scene(ctx, sd)
+    T = int(sc.stats()["triangles"])
+    rng = np.random.default_rng(0)
+    alb = torch.from_numpy(rng.uniform(0.2, 0.9, (T, 3)).astype(np.float32)).cuda()
+    sc.set_triangle_albedo(alb)
+    views = scenes.cornell_cameras(W, H, ncam)
+    flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_LAMBERTIAN | abi.ST_FLAG_FLOAT_NEE
+    params = [abi.default_params(W, H, max_depth=depth, spp=spp, flags=flags, rr_start_depth=3, seed=17 * i)
+              for i in range(ncam)]
+    dh = torch.from_numpy(rng.uniform(0.5, 1.5, (H, W, 4)).astype(np.float32)).cuda()
+    words = sd.volumes[0].nbytes // 4
+    ga = torch.zeros((T, 3), dtype=torch.float32, device="cuda")
+    gg = torch.zeros(words, dtype=torch.float32, device="cuda")
+    r = TestRenderer(sc)
+    outs = [None] * ncam
+
+    def forward():
+        for i in range(ncam):
+            outs[i] = r.render(views[i], params[i], out=outs[i])
+
+    def gradient():
+        for i in range(ncam):
+            sc.adjoint(views[i], params[i], dh, ga, gg, 0)
+
+    for _ in range(max(1, args.warmup)):
+        forward()
+        gradient()
+    torch.cuda.synchronize()
+    fwd, adj = [], []
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        forward()
+        torch.cuda.synchronize()
+        fwd.append(time.perf_counter() - a)
+        ga.zero_()
+        gg.zero_()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        gradient()
+        torch.cuda.synchronize()
+        adj.append(time.perf_counter() - a)
+        log(f"forward {fwd[-1] * 1e3:.1f} ms, gradient {adj[-1] * 1e3:.1f} ms ({adj[-1] / fwd[-1]:.2f}x)")
+    fwd_ms, adj_ms = sum(fwd) / len(fwd) * 1e3, sum(adj) / len(adj) * 1e3
+    gv = gg[torch.as_tensor(offs.reshape(-1) // 4, device="cuda")]
+    assert torch.isfinite(ga).all() and torch.isfinite(gg).all() and float(gv.abs().sum()) > 0
+
+    # exclusive kernel times of the adjoint (both passes) and its ray counts;
+    # per-ray grid reads from a 1-spp counting render of each camera
+    totals, serial_ms = {}, 0.0
+    for i in range(ncam):
+        cp = abi.default_params(W, H, max_depth=depth, spp=1, flags=flags | abi.ST_FLAG_COUNT_TRAVERSAL, rr_start_depth=3,
+                                seed=params[i].seed)
+        r.render(views[i], cp)
+        torch.cuda.synchronize()
+        tc = ctx.timings()
+        per_ray = (tc["closest_nodes"] / max(1, tc["closest_rays"]), 0.0, tc["shadow_nodes"] / max(1, tc["shadow_rays"]), 0.0)
+        sp = abi.default_params(W, H, max_depth=depth, spp=spp, flags=flags | abi.ST_FLAG_SERIAL_WAVES, rr_start_depth=3,
+                                seed=params[i].seed)
+        a = time.perf_counter()
+        sc.adjoint(views[i], sp, dh, ga, gg, 0)
+        torch.cuda.synchronize()
+        serial_ms += (time.perf_counter() - a) * 1e3
+        kx, ts = ctx.kernel_times(), ctx.timings()
+        algo, names = algo_bytes(ts, per_ray, True, 0, W * H * spp, sp, False)
+        for k, nbytes in algo.items():
+            ms_k, n_k = kx[k]
+            if n_k == 0 or ms_k <= 0:
+                continue
+            t = totals.setdefault(names.get(k, k), [0.0, 0, 0.0])
+            t[0] += ms_k
+            t[1] += n_k
+            t[2] += nbytes
+    kernels = {k: {"ms_per_gradient": round(t[0], 3), "launches": t[1], "avg_launch_ms": round(t[0] / t[1], 4),
+                   "bytes_per_launch": round(t[2] / t[1]), "frac": round(t[2] / (t[0] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+               for k, t in totals.items()}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_gradient"])
+    d = kernels[dom]
+    assert sum(k["ms_per_gradient"] for k in kernels.values()) <= serial_ms, (kernels, serial_ms)
+    achieved = d["bytes_per_launch"] / (d["avg_launch_ms"] / 1e3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_launch_ms"],
+                "timing": "HIP events around each launch of one more adjoint per camera with ST_FLAG_SERIAL_WAVES "
+                          f"({serial_ms:.1f} ms for 16 cameras), both passes; the gradient atomics are not counted",
+                "kernels": kernels}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = adjoint_cpu_baseline(sd, views[0], alb.cpu().numpy(), dh.cpu().numpy(), args.cpu_seconds, depth, flags, log)
+    line = {
+        "metric": "adjoint ms/grad", "value": round(adj_ms, 2), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(adj_ms, 2), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded Cornell box and fBm density)",
+        "config": {"workload": f"gradient of a linear image loss over {ncam} cameras at {W}x{H}, {spp} spp, maxPathLength "
+                               f"{depth} (RR from 3), Lambertian, fp32 NEE, with respect to {T} per-triangle albedos and "
+                               "a 64^3 density grid (Cornell box + fBm cloud, sigma 6, albedo 0.8, g 0.2)",
+                   "cameras": ncam, "width": W, "height": H, "spp": spp, "max_path_length": depth,
+                   "forward_ms": round(fwd_ms, 2), "gradient_over_forward": round(adj_ms / fwd_ms, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def adjoint_cpu_baseline(sd, view, alb, dh, seconds, depth, flags, log):
+    """The oracle's or_adjoint (OpenMP over this host's share of cores) on
+    one camera at 512x512, spp grown until it takes about a third of
+    --cpu-seconds; reported in ms per gradient of 16 cameras at 8 spp."""
+    from stratum2_amd import abi
+    from oracle.oracle import OracleScene
+    o = OracleScene(sd)
+    o.set_triangle_albedo(alb)
+    threads = min(16, os.cpu_count() or 1)
+    words = sd.volumes[0].nbytes // 4
+    spp = 1
+    while True:
+        p = abi.default_params(512, 512, max_depth=depth, spp=spp, flags=flags, rr_start_depth=3)
+        a = time.perf_counter()
+        o.adjoint(view, p, dh, albedo=True, grid_vol=0, grid_words=words, threads=threads)
+        dt = time.perf_counter() - a
+        if dt >= seconds / 3 or spp >= 8:
+            break
+        spp = min(8, spp * 2)
+    per_grad = dt / spp * 8 * 16 * 1e3
+    log(f"cpu baseline: or_adjoint 512x512 x {spp} spp in {dt:.2f}s on {threads} threads")
+    return {"value": round(per_grad, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle or_adjoint, one camera at 512x512 and {spp} spp ({dt:.1f} s), scaled to 16 cameras x 8 spp"}
 
 
 def bench_svgf(args, log):

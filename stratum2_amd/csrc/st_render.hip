@@ -122,6 +122,71 @@ struct AdjArgs {
     uint32_t grad_vol, spp;
 };
 
+// Per-block accumulation of the albedo gradient: a 512-entry open-addressing
+// table in LDS (key = albedo entry), flushed to the gradient once at the end
+// of the kernel, so the global atomics per launch scale with the blocks and
+// the distinct triangles they saw, not with the vertices (thousands of paths
+// land on each wall triangle).
+constexpr int kAdjSlots = 512;
+struct AdjTable {
+    uint32_t key[kAdjSlots];
+    float val[3 * kAdjSlots];
+};
+__device__ inline void adj_table_clear(AdjTable& t) {
+    for (int i = threadIdx.x; i < kAdjSlots; i += blockDim.x) {
+        t.key[i] = 0xffffffffu;
+        t.val[3 * i] = t.val[3 * i + 1] = t.val[3 * i + 2] = 0.0f;
+    }
+}
+// into the table (8 probes), else straight to the gradient
+__device__ inline void adj_table_add(AdjTable& t, float* base, uint32_t key, float x, float y, float z) {
+    const uint32_t h = (key * 2654435761u) >> 23;
+    for (uint32_t probe = 0; probe < 8; probe++) {
+        const uint32_t sl = (h + probe) & (kAdjSlots - 1);
+        const uint32_t old = atomicCAS(&t.key[sl], 0xffffffffu, key);
+        if (old == 0xffffffffu || old == key) {
+            atomicAdd(&t.val[3 * sl], x);
+            atomicAdd(&t.val[3 * sl + 1], y);
+            atomicAdd(&t.val[3 * sl + 2], z);
+            return;
+        }
+    }
+    atomicAdd(base + 3 * (size_t)key, x);
+    atomicAdd(base + 3 * (size_t)key + 1, y);
+    atomicAdd(base + 3 * (size_t)key + 2, z);
+}
+__device__ inline void adj_table_flush(const AdjTable& t, float* base) {
+    for (int i = threadIdx.x; i < kAdjSlots; i += blockDim.x) {
+        const uint32_t k = t.key[i];
+        if (k == 0xffffffffu) continue;
+        atomicAdd(base + 3 * (size_t)k, t.val[3 * i]);
+        atomicAdd(base + 3 * (size_t)k + 1, t.val[3 * i + 1]);
+        atomicAdd(base + 3 * (size_t)k + 2, t.val[3 * i + 2]);
+    }
+}
+// Adds v at key for every lane whose key is not ~0u: the lanes that share the
+// first lane's key (a tile of primary rays on one wall triangle) are summed
+// across the wavefront first, the others go to the table one by one.  Every
+// lane of the wavefront must call it.
+__device__ inline void wave_add_keyed3(AdjTable& t, float* base, uint32_t key, v3 v) {
+    bool want = key != 0xffffffffu;
+    const unsigned long long m = __ballot(want);
+    if (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        const uint32_t k = __shfl(key, leader);
+        const bool mine = want && key == k;
+        float x = mine ? v.x : 0.0f, y = mine ? v.y : 0.0f, z = mine ? v.z : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) {
+            x += __shfl_xor(x, off);
+            y += __shfl_xor(y, off);
+            z += __shfl_xor(z, off);
+        }
+        if ((int)__lane_id() == leader) adj_table_add(t, base, k, x, y, z);
+        if (mine) want = false;
+    }
+    if (want) adj_table_add(t, base, key, v.x, v.y, v.z);
+}
+
 // dLoss/dL of the path through pixel pix: dLoss/dHDR / spp
 __device__ __forceinline__ v3 adj_pixel_weight(const AdjArgs& a, uint32_t pix) {
     const float4 d = a.d_hdr[pix];
@@ -814,6 +879,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c,
 template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA, bool ADJ = false>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
                                                        AuxOut aux, int first_wave, AdjArgs adj) {
+    __shared__ AdjTable adj_tab;  // ADJ only
+    if (ADJ) {
+        adj_table_clear(adj_tab);
+        __syncthreads();
+    }
     const uint32_t n = w.counters[qsel];
     const uint32_t* q = w.queue[qsel];
     uint32_t* qout = w.queue[qsel ^ 1];
@@ -827,6 +897,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
         float4 sh0 = make_float4(0, 0, 0, 0), sh1 = sh0, sh2 = sh0;
         uint32_t p = 0;
         float4 ray_out = sh0, thr_out = sh0;
+        uint32_t adj_key = 0xffffffffu;  // ADJ: this vertex's albedo entry and gradient
+        v3 adj_val = mk(0, 0, 0);
         if (valid) {
             p = q[i];
             const uint4 hit = w.hit[i];
@@ -964,10 +1036,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                     }
                     const float4 lt = adj.ltot[p];
                     const v3 wg = adj_pixel_weight(adj, pix) * mk(lt.x - em.x, lt.y - em.y, lt.z - em.z);
-                    float* ga = adj.grad_alb + 3 * (size_t)alb_entry;
-                    if (m.bo.x > 0) atomicAdd(ga, wg.x / m.bo.x);
-                    if (m.bo.y > 0) atomicAdd(ga + 1, wg.y / m.bo.y);
-                    if (m.bo.z > 0) atomicAdd(ga + 2, wg.z / m.bo.z);
+                    adj_key = alb_entry;
+                    adj_val = mk(m.bo.x > 0 ? wg.x / m.bo.x : 0.0f, m.bo.y > 0 ? wg.y / m.bo.y : 0.0f,
+                                 m.bo.z > 0 ? wg.z / m.bo.z : 0.0f);
                 }
                 bool alive = live && path_len < c.max_depth && any_pos(m.base());
                 // Russian roulette from path length rr_start (0: every bounce, testrenderer.slang:890-892)
@@ -1022,6 +1093,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 w.thr[p] = thr_out;
             }
         }
+        if (ADJ) wave_add_keyed3(adj_tab, adj.grad_alb, adj_key, adj_val);
         const uint32_t si = wave_append(&w.counters[2], emit_shadow);
         if (emit_shadow) {
             w.shadow[3 * (size_t)si] = sh0;
@@ -1030,6 +1102,10 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
         }
         const uint32_t qi = wave_append(&w.counters[qsel ^ 1], cont);
         if (cont) qout[qi] = p;
+    }
+    if (ADJ) {
+        __syncthreads();
+        if (adj.grad_alb) adj_table_flush(adj_tab, adj.grad_alb);
     }
 }
 
@@ -3054,7 +3130,9 @@ __global__ void k_adj_total(uint32_t n, const float4* em, float4* ltot) {
 // still to come times the event's log-derivative (k_shade<.., ADJ>,
 // k_trace_media<ADJ>, k_shadow_media<ADJ>).  Waves are independent (the
 // gradients are sums), so they run in the context's four slots with no order
-// between them.
+// between them (one slot with ST_FLAG_SERIAL_WAVES).  Kernel classes are
+// timed as in st_render (st_last_kernel_times; st_last_timings' ray counts
+// cover both passes).
 int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, const float* d_hdr, float* grad_albedo,
                float* grad_grid, uint32_t grad_volume) {
     if (!s || !view || !p || !d_hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
@@ -3074,7 +3152,7 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
         if (k == 0) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
         const uint64_t paths = (uint64_t)npix * k;
         const uint32_t waves = (p->spp + k - 1) / k;
-        const int slots = (int)std::min<uint32_t>(kWaveSlots, waves);
+        const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) ? 1 : (int)std::min<uint32_t>(kWaveSlots, waves);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         const bool media = s->d.vinst_count > 0, lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
         const bool alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
@@ -3084,7 +3162,9 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
             ctx->slot[sl].ltot.reserve(paths * 16);
         }
         ctx->stats.reserve(64);
+        begin_timings(ctx);
         hipStream_t s0 = ctx->slot[0].stream;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s0, ctx->stats.as<uint32_t>(), 16u, 0u);
         for (int sl = 0; sl < slots; sl++)
             hipLaunchKernelGGL(k_fill_u32, dim3(kCounterWords / 256), dim3(256), 0, s0, ctx->slot[sl].counters.as<uint32_t>(),
                                (uint32_t)kCounterWords, 0u);
@@ -3115,19 +3195,29 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
                 hipLaunchKernelGGL(k_camera, dim3(cgrid), dim3(kShadeBlock), 0, sm, c, w);
                 int qsel = 0;
                 for (uint32_t b = 0; b < std::max<uint32_t>(1, p->max_depth); b++) {
-                    if (media)
-                        hipLaunchKernelGGL(replay ? k_trace_media<true> : k_trace_media<false>, dim3(tgrid), dim3(kTraceBlock), 0,
-                                           sm, s->d, c, w, qsel, spill, stats, adj);
-                    else
-                        hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
-                    hipLaunchKernelGGL(replay ? shade_adj : shade, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, 0,
-                                       adj);
-                    if (media)
-                        hipLaunchKernelGGL(replay ? k_shadow_media<true> : k_shadow_media<false>, dim3(tgrid), dim3(kTraceBlock),
-                                           0, sm, s->d, c, w, spill, stats, adj);
-                    else
-                        hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill,
-                                           stats);
+                    {
+                        Timed tm(ctx, 0, sm);
+                        if (media)
+                            hipLaunchKernelGGL(replay ? k_trace_media<true> : k_trace_media<false>, dim3(tgrid), dim3(kTraceBlock),
+                                               0, sm, s->d, c, w, qsel, spill, stats, adj);
+                        else
+                            hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill,
+                                               stats);
+                    }
+                    {
+                        Timed tm(ctx, 1, sm);
+                        hipLaunchKernelGGL(replay ? shade_adj : shade, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux,
+                                           0, adj);
+                    }
+                    {
+                        Timed tm(ctx, 2, sm);
+                        if (media)
+                            hipLaunchKernelGGL(replay ? k_shadow_media<true> : k_shadow_media<false>, dim3(tgrid),
+                                               dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats, adj);
+                        else
+                            hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags,
+                                               spill, stats);
+                    }
                     hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
                     qsel ^= 1;
                 }
@@ -3139,6 +3229,7 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
             HIP_CHECK(hipGetLastError());
         }
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
+        finish_timings(ctx, slots, nullptr);
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());

@@ -277,6 +277,34 @@ def axis_angle_quat(axis, deg):
     return (a[0] * math.sin(h), a[1] * math.sin(h), a[2] * math.sin(h), math.cos(h))
 
 
+def look_at_quat(position, target):
+    """Orientation (x, y, z, w) turning the camera's forward axis (-z) towards
+    target: a yaw about y after a pitch about x (no roll)."""
+    d = np.asarray(target, np.float64) - np.asarray(position, np.float64)
+    d /= np.linalg.norm(d)
+    # quat_to_transform (TransformData) rotates by the conjugate of the
+    # usual convention, hence the negated angles
+    yaw, pitch = -math.atan2(-d[0], -d[2]), -math.asin(max(-1.0, min(1.0, d[1])))
+    qy = (0.0, math.sin(yaw / 2), 0.0, math.cos(yaw / 2))
+    qp = (math.sin(pitch / 2), 0.0, 0.0, math.cos(pitch / 2))
+    x1, y1, z1, w1 = qp
+    x2, y2, z2, w2 = qy
+    return (w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+            w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2)
+
+
+def cornell_cameras(width, height, n=16):
+    """n cameras on a 4-wide grid in front of the Cornell box's open side,
+    each looking at (0, 1, -0.2)."""
+    views = []
+    for i in range(n):
+        x = -0.5 + (i % 4) / 3.0
+        y = 0.7 + (i // 4 % 4) / 4.0
+        pos = (x, y, 2.3 + 0.1 * (i // 16))
+        views.append(make_view(width, height, position=pos, orientation=look_at_quat(pos, (0.0, 1.0, -0.2))))
+    return views
+
+
 # ------------------------------------------------------------ geometry
 def quad(p0, p1, p2, p3):
     """Two triangles (p0,p1,p2), (p0,p2,p3); normal = cross(v0-v2, v1-v2) side."""
