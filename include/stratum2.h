@@ -276,6 +276,41 @@ int st_scene_set_volume_grid(st_scene* scene, uint32_t volume, const void* grid,
 int st_adjoint(st_scene* scene, const st_view* view, const st_render_params* params, const float* d_hdr,
                float* grad_albedo, float* grad_grid, uint32_t grad_volume);
 
+/* ---- Learned radiance cache (no reference counterpart; after Mueller et
+ * al., "Real-time Neural Radiance Caching for Path Tracing", 2021, with the
+ * multiresolution hash encoding of "Instant Neural Graphics Primitives",
+ * 2022).  An fp16 hash-grid table (levels x 2^log2_table entries x 2
+ * features; cells hashed as HashGrid::FindCellIndex does, hashgrid.hlsli:
+ * 56-58) and a 64-64-64-64-3 ReLU MLP on fp16 MFMA with fp32 accumulation,
+ * trained with Adam on a relative L2 loss.
+ * A query is 12 floats: world position, outgoing direction, shading normal,
+ * albedo; positions are normalised by the box given at creation. */
+typedef struct st_cache st_cache;
+typedef struct st_cache_config {
+    uint32_t levels;          /* 1-16 (16) */
+    uint32_t log2_table;      /* entries per level, log2: 8-24 (19) */
+    uint32_t base_resolution; /* cells per axis of the coarsest level (16) */
+    uint32_t max_resolution;  /* of the finest level (2048) */
+    float learning_rate;      /* Adam (1e-2) */
+    float beta1, beta2, epsilon; /* (0.9, 0.99, 1e-15) */
+    uint32_t seed;            /* initial parameters */
+} st_cache_config;
+int st_cache_create(st_context* ctx, const st_cache_config* config, const float bbox_min[3], const float bbox_max[3],
+                    st_cache** cache);
+int st_cache_destroy(st_cache* cache);
+/* queries: n x 12 floats, radiance: n x 3 floats (device) */
+int st_cache_infer(st_cache* cache, const float* queries, uint32_t n, float* radiance);
+/* One Adam step on n samples (queries, n x 3 target radiances; device);
+ * loss (host, may be null) receives the mean relative L2 loss before the step. */
+int st_cache_train(st_cache* cache, const float* queries, const float* targets, uint32_t n, float* loss);
+/* Parameter sizes in floats: the table (levels x entries x 2) and the four
+ * 64 x 64 weight matrices ([layer][out][in]; the output layer's rows 3..63 are 0). */
+int st_cache_param_counts(const st_cache* cache, uint32_t* table_floats, uint32_t* weight_floats);
+/* Host copies of the fp32 parameters and of the last step's gradients (sums
+ * over its samples of the per-sample loss gradient); any may be null. */
+int st_cache_get_params(st_cache* cache, float* table, float* weights, float* table_grad, float* weight_grad);
+int st_cache_set_params(st_cache* cache, const float* table, const float* weights);
+
 /* Primary-ray closest hit only (the minimum slice): visibility for each pixel
  * and, when t_out is not null, the hit distance as the path state stores it
  * (fp16-rounded, inf on a miss).  Device pointers. */

@@ -109,6 +109,20 @@ class st_render_request(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("view", st_view), ("params", st_render_params), ("outputs", st_outputs)]
 
 
+class st_cache_config(C.Structure):
+    _fields_ = [("levels", u32), ("log2_table", u32), ("base_resolution", u32), ("max_resolution", u32),
+                ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("seed", u32)]
+
+
+def default_cache_config(**kw):
+    c = st_cache_config(levels=16, log2_table=19, base_resolution=16, max_resolution=2048, learning_rate=1e-2,
+                        beta1=0.9, beta2=0.99, epsilon=1e-15, seed=0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
 class st_batch_config(C.Structure):
     _fields_ = [("trace_blocks_per_cu", u32), ("shade_blocks_per_cu", u32), ("wave_paths", u32), ("streams", u32),
                 ("ms", C.c_double)]
@@ -197,6 +211,8 @@ EXPORTS = [
     "st_denoiser_last_timings",
     "st_batch_create", "st_batch_render", "st_batch_autotune", "st_batch_get_config", "st_batch_set_config",
     "st_batch_destroy", "st_render_batch", "st_scene_set_triangle_albedo", "st_scene_set_volume_grid", "st_adjoint",
+    "st_cache_create", "st_cache_destroy", "st_cache_infer", "st_cache_train", "st_cache_param_counts", "st_cache_get_params",
+    "st_cache_set_params",
 ]
 
 _lib = None
@@ -244,6 +260,14 @@ def load_library(path=LIB_PATH):
     lib.st_scene_set_volume_grid.argtypes = [vp, u32, vp, C.c_uint64]
     lib.st_adjoint.argtypes = [vp, C.POINTER(st_view), C.POINTER(st_render_params), vp, vp, vp, u32]
     lib.st_render_batch.argtypes = [vp, C.POINTER(st_render_request), u32]
+    f3 = C.c_float * 3
+    lib.st_cache_create.argtypes = [vp, C.POINTER(st_cache_config), f3, f3, C.POINTER(vp)]
+    lib.st_cache_destroy.argtypes = [vp]
+    lib.st_cache_infer.argtypes = [vp, vp, u32, vp]
+    lib.st_cache_train.argtypes = [vp, vp, vp, u32, C.POINTER(C.c_float)]
+    lib.st_cache_param_counts.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
+    lib.st_cache_get_params.argtypes = [vp, vp, vp, vp, vp]
+    lib.st_cache_set_params.argtypes = [vp, vp, vp]
     if path == LIB_PATH:
         _lib = lib
     return lib

@@ -1,0 +1,259 @@
+// st_cache.h — learned radiance cache: a multiresolution hash-grid encoding
+// (fp16 table) and a fully-fused 64-wide ReLU MLP on fp16 MFMA with fp32
+// accumulation, with hand-written inference, backward and Adam kernels.
+//
+// Inputs per query (12 floats): world position, outgoing direction, shading
+// normal, albedo.  Network input (64 fp16): 2 features x L levels of the
+// hash grid (trilinear over each level's cell), (direction + 1) / 2,
+// (normal + 1) / 2, albedo, a constant 1 (the first layer's bias), zeros.
+// Layers: 64 -> 64 -> 64 -> 64 (ReLU) -> 3 (linear; the output matrix is
+// stored 64 x 64 with rows 3..63 zero).  No other biases.
+//
+// Matrix layout on v_mfma_f32_16x16x16f16 (one wavefront, 16 x 16 x 16):
+// lane l holds A[l % 16][4 (l / 16) + j], B[4 (l / 16) + j][l % 16] and
+// D[4 (l / 16) + j][l % 16], j = 0..3.  With A = a 16 x 16 tile of W[out][in]
+// and B = a tile of X^T (k = input feature, column = sample), each lane's A
+// and B are 4 consecutive halves of a row of W and of X, and D is the tile of
+// Y^T whose lane holds 4 consecutive output features of one sample: the
+// B operand of the next layer's matching k block, and 4 consecutive halves
+// of a row of Y.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace st {
+
+constexpr int kMlpWidth = 64;      // input, hidden and (padded) output width
+constexpr int kMlpLayers = 4;      // 3 hidden layers + output
+constexpr int kCacheMaxLevels = 16;
+constexpr int kCacheQuery = 12;    // floats per query
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+struct CacheGrid {
+    float lo[3], inv_ext[3];            // position normalisation: (p - lo) * inv_ext in [0, 1]
+    float res[kCacheMaxLevels];         // cells per axis of each level
+    uint32_t levels, table_mask;        // table entries per level = table_mask + 1 (a power of 2)
+};
+
+// pcg (rng.hlsli:17-21)
+__device__ __forceinline__ uint32_t rc_pcg(uint32_t v) {
+    const uint32_t s = v * 747796405u + 2891336453u;
+    const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+// the cell hash of HashGrid::FindCellIndex (hashgrid.hlsli:56-58), keyed by
+// level instead of cell size
+__device__ __forceinline__ uint32_t rc_hash(int x, int y, int z, uint32_t level) {
+    return rc_pcg(rc_pcg(level) + rc_pcg((uint32_t)z + rc_pcg((uint32_t)y + rc_pcg((uint32_t)x))));
+}
+
+// The 8 corners of a query's cell at one level: table entries and trilinear weights
+__device__ __forceinline__ void rc_corners(const CacheGrid& g, const float* q, uint32_t l, uint32_t idx[8], float wt[8]) {
+    float p[3];
+    int c[3];
+    float f[3];
+    for (int a = 0; a < 3; a++) {
+        float u = (q[a] - g.lo[a]) * g.inv_ext[a];
+        u = fminf(fmaxf(u, 0.0f), 1.0f);
+        p[a] = u * g.res[l];
+        const float fl = floorf(p[a]);
+        c[a] = (int)fl;
+        f[a] = p[a] - fl;
+    }
+    for (int k = 0; k < 8; k++) {
+        const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+        idx[k] = l * (g.table_mask + 1) + (rc_hash(c[0] + dx, c[1] + dy, c[2] + dz, l) & g.table_mask);
+        wt[k] = (dx ? f[0] : 1 - f[0]) * (dy ? f[1] : 1 - f[1]) * (dz ? f[2] : 1 - f[2]);
+    }
+}
+
+// X[s] (64 fp16) from query s: hash features, then the raw inputs
+__global__ void k_rc_encode(CacheGrid g, const _Float16* table, const float* queries, uint32_t n, _Float16* X) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const float* q = queries + (size_t)kCacheQuery * s;
+        _Float16* x = X + (size_t)kMlpWidth * s;
+        for (uint32_t l = 0; l < g.levels; l++) {
+            uint32_t idx[8];
+            float wt[8];
+            rc_corners(g, q, l, idx, wt);
+            float a = 0, b = 0;
+            for (int k = 0; k < 8; k++) {
+                a += wt[k] * (float)table[2 * (size_t)idx[k]];
+                b += wt[k] * (float)table[2 * (size_t)idx[k] + 1];
+            }
+            x[2 * l] = (_Float16)a;
+            x[2 * l + 1] = (_Float16)b;
+        }
+        uint32_t o = 2 * g.levels;
+        for (int a = 0; a < 3; a++) x[o++] = (_Float16)((q[3 + a] + 1) * 0.5f);
+        for (int a = 0; a < 3; a++) x[o++] = (_Float16)((q[6 + a] + 1) * 0.5f);
+        for (int a = 0; a < 3; a++) x[o++] = (_Float16)q[9 + a];
+        x[o++] = (_Float16)1.0f;
+        for (; o < (uint32_t)kMlpWidth; o++) x[o] = (_Float16)0.0f;
+    }
+}
+
+// dTable += sum over queries of weight * dX[s][2 l + f]
+__global__ void k_rc_encode_bwd(CacheGrid g, const float* queries, uint32_t n, const _Float16* dX, float* dtable) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const float* q = queries + (size_t)kCacheQuery * s;
+        const _Float16* d = dX + (size_t)kMlpWidth * s;
+        for (uint32_t l = 0; l < g.levels; l++) {
+            const float ga = (float)d[2 * l], gb = (float)d[2 * l + 1];
+            if (ga == 0 && gb == 0) continue;
+            uint32_t idx[8];
+            float wt[8];
+            rc_corners(g, q, l, idx, wt);
+            for (int k = 0; k < 8; k++) {
+                atomicAdd(dtable + 2 * (size_t)idx[k], wt[k] * ga);
+                atomicAdd(dtable + 2 * (size_t)idx[k] + 1, wt[k] * gb);
+            }
+        }
+    }
+}
+
+// Y[s][o] = act(sum_k X[s][k] W[o][k]) over n rows, one wavefront per 64
+// rows at a time: act = ReLU (relu), or a mask (mask[s][o] > 0 keeps the
+// value: the backward pass through a ReLU), or none.  W: 64 x 64 fp16,
+// row-major [out][in].  n need not be a multiple of 64 (rows past n are
+// neither read nor written).
+__global__ void __launch_bounds__(256) k_mlp_layer(const _Float16* W, const _Float16* X, _Float16* Y, const _Float16* mask,
+                                                    uint32_t n, int relu) {
+    const uint32_t lane = __lane_id(), r = lane % 16, qd = lane / 16;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, waves = gridDim.x * blockDim.x / 64;
+    h4 A[4][4];
+    for (int ob = 0; ob < 4; ob++)
+        for (int kb = 0; kb < 4; kb++) A[ob][kb] = *reinterpret_cast<const h4*>(W + (ob * 16 + r) * kMlpWidth + kb * 16 + 4 * qd);
+    for (uint32_t base = wave * 64; base < n; base += waves * 64) {
+        for (int sb = 0; sb < 4; sb++) {
+            const uint32_t s = base + sb * 16 + r;
+            const bool in = s < n;
+            h4 B[4];
+            for (int kb = 0; kb < 4; kb++) {
+                if (in) B[kb] = *reinterpret_cast<const h4*>(X + (size_t)s * kMlpWidth + kb * 16 + 4 * qd);
+                else B[kb] = h4{0, 0, 0, 0};
+            }
+            f4v acc[4];
+            for (int ob = 0; ob < 4; ob++) {  // every lane runs every MFMA
+                acc[ob] = f4v{0, 0, 0, 0};
+                for (int kb = 0; kb < 4; kb++) acc[ob] = __builtin_amdgcn_mfma_f32_16x16x16f16(A[ob][kb], B[kb], acc[ob], 0, 0, 0);
+            }
+            if (in) {
+                for (int ob = 0; ob < 4; ob++) {
+                    const size_t off = (size_t)s * kMlpWidth + ob * 16 + 4 * qd;
+                    h4 m4 = h4{1, 1, 1, 1};
+                    if (mask) m4 = *reinterpret_cast<const h4*>(mask + off);
+                    h4 y;
+                    for (int j = 0; j < 4; j++) {
+                        float v = acc[ob][j];
+                        if (relu) v = fmaxf(v, 0.0f);
+                        if (mask && !((float)m4[j] > 0.0f)) v = 0.0f;
+                        y[j] = (_Float16)v;
+                    }
+                    *reinterpret_cast<h4*>(Y + off) = y;
+                }
+            }
+        }
+    }
+}
+
+// dW[o][i] += scale * sum_s dY[s][o] X[s][i]: one wavefront per chunk of
+// `chunk` rows (a multiple of 16), all 16 output tiles in registers,
+// then one atomic per element and lane.
+__global__ void __launch_bounds__(256) k_mlp_wgrad(const _Float16* dY, const _Float16* X, float* dW, uint32_t n, uint32_t chunk,
+                                                    float scale) {
+    const uint32_t lane = __lane_id(), r = lane % 16, qd = lane / 16;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t s0 = wave * chunk;
+    if (s0 >= n) return;  // whole wavefronts leave together
+    const uint32_t s1 = min(n, s0 + chunk);
+    f4v acc[4][4];
+    for (int ob = 0; ob < 4; ob++)
+        for (int ib = 0; ib < 4; ib++) acc[ob][ib] = f4v{0, 0, 0, 0};
+    for (uint32_t k0 = s0; k0 < s1; k0 += 16) {
+        // A[o][s] = dY[s][o]: lane holds rows s = k0 + 4 qd + j of column o = ob 16 + r
+        // B[s][i] = X[s][i]: lane holds rows s = k0 + 4 qd + j of column i = ib 16 + r
+        h4 A[4], B[4];
+        for (int t = 0; t < 4; t++)
+            for (int j = 0; j < 4; j++) {
+                const uint32_t s = k0 + 4 * qd + j;
+                A[t][j] = s < s1 ? dY[(size_t)s * kMlpWidth + t * 16 + r] : (_Float16)0.0f;
+                B[t][j] = s < s1 ? X[(size_t)s * kMlpWidth + t * 16 + r] : (_Float16)0.0f;
+            }
+        for (int ob = 0; ob < 4; ob++)
+            for (int ib = 0; ib < 4; ib++) acc[ob][ib] = __builtin_amdgcn_mfma_f32_16x16x16f16(A[ob], B[ib], acc[ob][ib], 0, 0, 0);
+    }
+    // D[o][i]: lane holds o = ob 16 + 4 qd + j, i = ib 16 + r
+    for (int ob = 0; ob < 4; ob++)
+        for (int ib = 0; ib < 4; ib++)
+            for (int j = 0; j < 4; j++)
+                atomicAdd(dW + (ob * 16 + 4 * qd + j) * kMlpWidth + ib * 16 + r, scale * acc[ob][ib][j]);
+}
+
+// Relative L2 loss of the radiance cache (Mueller et al., "Real-time Neural
+// Radiance Caching for Path Tracing", 2021): per sample and channel
+// (pred - target)^2 / (pred^2 + 0.01) / 3, with pred held fixed in the
+// denominator.  dY[s][c] is the per-sample gradient (the 1/n goes into the
+// weights' gradient scale); loss_sum receives the summed loss.
+__global__ void k_rc_loss(const _Float16* Y, const float* targets, uint32_t n, _Float16* dY, float* loss_sum) {
+    float local = 0;
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        _Float16* d = dY + (size_t)kMlpWidth * s;
+        for (int c = 0; c < kMlpWidth; c++) d[c] = (_Float16)0.0f;
+        for (int c = 0; c < 3; c++) {
+            const float p = (float)Y[(size_t)kMlpWidth * s + c], t = targets[3 * (size_t)s + c];
+            const float den = p * p + 0.01f;
+            local += (p - t) * (p - t) / den / 3;
+            d[c] = (_Float16)fminf(fmaxf(2 * (p - t) / den / 3, -30000.0f), 30000.0f);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
+    if (__lane_id() == 0) atomicAdd(loss_sum, local);
+}
+
+// Adam (Kingma and Ba 2015) over n parameters with fp32 master copies; g is
+// scaled by gscale; entries whose gradient is exactly zero keep their
+// moments and value (sparse: table entries no sample touched).  p16: the
+// fp16 copy the kernels read; p16t (optional): its 64 x 64 blocks
+// transposed, for the backward pass.
+__global__ void k_rc_adam(float* p, float* m, float* v, const float* g, uint32_t n, float gscale, float lr, float b1, float b2,
+                          float eps, float c1, float c2, _Float16* p16, _Float16* p16t, int sparse) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = g[i] * gscale;
+        if (!(sparse && gi == 0.0f)) {
+            const float mi = b1 * m[i] + (1 - b1) * gi;
+            const float vi = b2 * v[i] + (1 - b2) * gi * gi;
+            m[i] = mi;
+            v[i] = vi;
+            p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        }
+        p16[i] = (_Float16)p[i];
+        if (p16t) {
+            const uint32_t blk = i / (kMlpWidth * kMlpWidth), e = i % (kMlpWidth * kMlpWidth);
+            const uint32_t o = e / kMlpWidth, in = e % kMlpWidth;
+            p16t[(size_t)blk * kMlpWidth * kMlpWidth + in * kMlpWidth + o] = (_Float16)p[i];
+        }
+    }
+}
+
+// fp16 copies of fp32 parameters (and, optionally, the 64 x 64 blocks transposed)
+__global__ void k_rc_copy16(const float* p, uint32_t n, _Float16* p16, _Float16* p16t) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        p16[i] = (_Float16)p[i];
+        if (p16t) {
+            const uint32_t blk = i / (kMlpWidth * kMlpWidth), e = i % (kMlpWidth * kMlpWidth);
+            p16t[(size_t)blk * kMlpWidth * kMlpWidth + (e % kMlpWidth) * kMlpWidth + e / kMlpWidth] = (_Float16)p[i];
+        }
+    }
+}
+
+// radiance = max(0, Y[s][0..2]) as fp32
+__global__ void k_rc_output(const _Float16* Y, uint32_t n, float* out) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; c++) out[3 * (size_t)s + c] = fmaxf((float)Y[(size_t)kMlpWidth * s + c], 0.0f);
+}
+
+}  // namespace st

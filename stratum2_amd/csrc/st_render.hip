@@ -32,6 +32,7 @@
 
 #include "../../include/stratum2.h"
 #include "st_bvh.h"
+#include "st_cache.h"
 #include "st_layout.h"
 #include "st_lbvh.h"
 #include "st_shade.h"
@@ -3230,6 +3231,261 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
         }
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
         finish_timings(ctx, slots, nullptr);
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+// ------------------------------------------------------------ radiance cache
+struct st_cache {
+    st_context* ctx = nullptr;
+    st_cache_config cfg{};
+    CacheGrid grid{};
+    uint32_t table_floats = 0;  // levels x entries x 2
+    DevBuf table32, table16, tm, tv, tgrad;
+    DevBuf w32, w16, w16t, wm, wv, wgrad;  // 4 layers of 64 x 64
+    DevBuf X0, H1, H2, H3, Y, dA, dB;      // rows x 64 fp16
+    DevBuf loss;
+    uint32_t rows = 0;
+    uint32_t step = 0;
+    void reserve_rows(uint32_t n) {
+        if (n <= rows) return;
+        const size_t b = (size_t)n * kMlpWidth * 2;
+        for (DevBuf* d : {&X0, &H1, &H2, &H3, &Y, &dA, &dB}) d->reserve(b);
+        rows = n;
+    }
+};
+
+namespace {
+constexpr uint32_t kMlpParams = kMlpLayers * kMlpWidth * kMlpWidth;
+constexpr uint32_t kCacheMaxRows = 1u << 24;
+
+int grid_for(int n, int per, int cap) { return std::max(1, std::min((n + per - 1) / per, cap)); }
+
+// X0 -> H1 -> H2 -> H3 -> Y on the context stream
+void cache_forward(st_cache* c, const float* queries, uint32_t n) {
+    hipStream_t st = c->ctx->stream;
+    const int cus = c->ctx->cus;
+    const _Float16* w = c->w16.as<_Float16>();
+    hipLaunchKernelGGL(k_rc_encode, dim3(grid_for((int)n, 256, cus * 16)), dim3(256), 0, st, c->grid, c->table16.as<_Float16>(),
+                       queries, n, c->X0.as<_Float16>());
+    const int lg = grid_for((int)n, 256, cus * 8);  // 4 wavefronts of 64 rows per block
+    _Float16* in[4] = {c->X0.as<_Float16>(), c->H1.as<_Float16>(), c->H2.as<_Float16>(), c->H3.as<_Float16>()};
+    _Float16* out[4] = {c->H1.as<_Float16>(), c->H2.as<_Float16>(), c->H3.as<_Float16>(), c->Y.as<_Float16>()};
+    for (int l = 0; l < kMlpLayers; l++)
+        hipLaunchKernelGGL(k_mlp_layer, dim3(lg), dim3(256), 0, st, w + (size_t)l * kMlpWidth * kMlpWidth, in[l], out[l],
+                           (const _Float16*)nullptr, n, l < kMlpLayers - 1 ? 1 : 0);
+}
+
+void cache_copy16(st_cache* c) {
+    hipStream_t st = c->ctx->stream;
+    hipLaunchKernelGGL(k_rc_copy16, dim3(grid_for((int)c->table_floats, 256, c->ctx->cus * 16)), dim3(256), 0, st,
+                       c->table32.as<float>(), c->table_floats, c->table16.as<_Float16>(), (_Float16*)nullptr);
+    hipLaunchKernelGGL(k_rc_copy16, dim3(grid_for((int)kMlpParams, 256, 64)), dim3(256), 0, st, c->w32.as<float>(), kMlpParams,
+                       c->w16.as<_Float16>(), c->w16t.as<_Float16>());
+}
+}  // namespace
+
+int st_cache_create(st_context* ctx, const st_cache_config* cfg, const float bbox_min[3], const float bbox_max[3], st_cache** out) {
+    if (!ctx || !cfg || !bbox_min || !bbox_max || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (cfg->levels < 1 || cfg->levels > kCacheMaxLevels || cfg->log2_table < 8 || cfg->log2_table > 24 ||
+        cfg->base_resolution < 2 || cfg->max_resolution < cfg->base_resolution || cfg->max_resolution > (1u << 20))
+        return set_error(ST_ERROR_INVALID_ARGUMENT,
+                         "cache: 1-16 levels, 2^8 to 2^24 entries per level, 2 <= base resolution <= max resolution <= 2^20");
+    if (!(cfg->learning_rate > 0) || !(cfg->beta1 >= 0 && cfg->beta1 < 1) || !(cfg->beta2 >= 0 && cfg->beta2 < 1) ||
+        !(cfg->epsilon > 0))
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "cache: learning rate > 0, betas in [0, 1), epsilon > 0");
+    for (int a = 0; a < 3; a++)
+        if (!(bbox_max[a] > bbox_min[a])) return set_error(ST_ERROR_INVALID_ARGUMENT, "cache: empty bounding box");
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        auto c = std::make_unique<st_cache>();
+        c->ctx = ctx;
+        c->cfg = *cfg;
+        CacheGrid& g = c->grid;
+        for (int a = 0; a < 3; a++) {
+            g.lo[a] = bbox_min[a];
+            g.inv_ext[a] = 1.0f / (bbox_max[a] - bbox_min[a]);
+        }
+        g.levels = cfg->levels;
+        g.table_mask = (1u << cfg->log2_table) - 1;
+        // geometric progression of resolutions (Mueller et al., "Instant
+        // Neural Graphics Primitives", 2022, eq. 2-3)
+        const double b = cfg->levels > 1 ? std::exp((std::log((double)cfg->max_resolution) - std::log((double)cfg->base_resolution)) /
+                                                    (double)(cfg->levels - 1))
+                                         : 1.0;
+        for (uint32_t l = 0; l < kCacheMaxLevels; l++)
+            g.res[l] = l < cfg->levels ? (float)std::floor((double)cfg->base_resolution * std::pow(b, (double)l)) : 0.0f;
+        c->table_floats = cfg->levels * (1u << cfg->log2_table) * 2;
+        // initial parameters: table entries uniform in +-1e-4, weights uniform
+        // in +-sqrt(6 / (fan in + fan out)); the output matrix's rows 3..63 zero
+        std::vector<float> t(c->table_floats), w(kMlpParams, 0.0f);
+        uint64_t state = 0x9E3779B97F4A7C15ull ^ cfg->seed;
+        auto uni = [&]() {
+            state = state * 6364136223846793005ull + 1442695040888963407ull;
+            return (float)((state >> 40) * (1.0 / 16777216.0));
+        };
+        for (float& x : t) x = (uni() * 2 - 1) * 1e-4f;
+        for (int l = 0; l < kMlpLayers; l++) {
+            const int fo = l == kMlpLayers - 1 ? 3 : kMlpWidth;
+            const float a = std::sqrt(6.0f / (float)(kMlpWidth + fo));
+            for (int o = 0; o < fo; o++)
+                for (int i = 0; i < kMlpWidth; i++) w[(size_t)l * kMlpWidth * kMlpWidth + o * kMlpWidth + i] = (uni() * 2 - 1) * a;
+        }
+        const size_t tb = 4ull * c->table_floats;
+        c->table32.reserve(tb);
+        c->tm.reserve(tb);
+        c->tv.reserve(tb);
+        c->tgrad.reserve(tb);
+        c->table16.reserve(tb / 2);
+        c->w32.reserve(4ull * kMlpParams);
+        c->wm.reserve(4ull * kMlpParams);
+        c->wv.reserve(4ull * kMlpParams);
+        c->wgrad.reserve(4ull * kMlpParams);
+        c->w16.reserve(2ull * kMlpParams);
+        c->w16t.reserve(2ull * kMlpParams);
+        c->loss.reserve(64);
+        HIP_CHECK(hipMemcpy(c->table32.p, t.data(), tb, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(c->w32.p, w.data(), 4ull * kMlpParams, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemset(c->tm.p, 0, tb));
+        HIP_CHECK(hipMemset(c->tv.p, 0, tb));
+        HIP_CHECK(hipMemset(c->tgrad.p, 0, tb));
+        HIP_CHECK(hipMemset(c->wm.p, 0, 4ull * kMlpParams));
+        HIP_CHECK(hipMemset(c->wv.p, 0, 4ull * kMlpParams));
+        HIP_CHECK(hipMemset(c->wgrad.p, 0, 4ull * kMlpParams));
+        cache_copy16(c.get());
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        *out = c.release();
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_cache_destroy(st_cache* c) {
+    if (!c) return ST_OK;
+    (void)hipSetDevice(c->ctx->device);
+    (void)hipStreamSynchronize(c->ctx->stream);
+    delete c;
+    return ST_OK;
+}
+
+int st_cache_infer(st_cache* c, const float* queries, uint32_t n, float* radiance) {
+    if (!c || (n && (!queries || !radiance))) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (n > kCacheMaxRows) return set_error(ST_ERROR_LIMIT, "at most 2^24 queries per call");
+    if (n == 0) return ST_OK;
+    try {
+        HIP_CHECK(hipSetDevice(c->ctx->device));
+        order_after_caller(c->ctx);
+        c->reserve_rows(n);
+        cache_forward(c, queries, n);
+        hipLaunchKernelGGL(k_rc_output, dim3(grid_for((int)n, 256, c->ctx->cus * 16)), dim3(256), 0, c->ctx->stream, c->Y.as<_Float16>(),
+                           n, radiance);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+namespace {
+// One Adam step on n samples already on the device, on the context stream
+void cache_train_step(st_cache* c, const float* queries, const float* targets, uint32_t n) {
+    st_context* ctx = c->ctx;
+    hipStream_t st = ctx->stream;
+    const int cus = ctx->cus;
+    c->reserve_rows(n);
+    cache_forward(c, queries, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int)c->table_floats, 256, cus * 16)), dim3(256), 0, st, c->tgrad.as<uint32_t>(),
+                       c->table_floats, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int)kMlpParams, 256, 64)), dim3(256), 0, st, c->wgrad.as<uint32_t>(), kMlpParams, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, c->loss.as<uint32_t>(), 16u, 0u);
+    hipLaunchKernelGGL(k_rc_loss, dim3(grid_for((int)n, 256, cus * 8)), dim3(256), 0, st, c->Y.as<_Float16>(), targets, n,
+                       c->dA.as<_Float16>(), c->loss.as<float>());
+    const _Float16* wt = c->w16t.as<_Float16>();
+    float* wg = c->wgrad.as<float>();
+    const uint32_t chunk = 1024;
+    const int wgrid = grid_for((int)((n + chunk - 1) / chunk), 4, 1 << 20);  // 4 wavefronts of `chunk` rows per block
+    const int lg = grid_for((int)n, 256, cus * 8);
+    _Float16* acts[4] = {c->X0.as<_Float16>(), c->H1.as<_Float16>(), c->H2.as<_Float16>(), c->H3.as<_Float16>()};
+    _Float16* d_out = c->dA.as<_Float16>();  // gradient at layer l's output
+    _Float16* d_in = c->dB.as<_Float16>();
+    for (int l = kMlpLayers - 1; l >= 0; l--) {
+        const size_t off = (size_t)l * kMlpWidth * kMlpWidth;
+        hipLaunchKernelGGL(k_mlp_wgrad, dim3(wgrid), dim3(256), 0, st, d_out, acts[l], wg + off, n, chunk, 1.0f);
+        // through the layer (transposed weights) and the ReLU that made its input
+        hipLaunchKernelGGL(k_mlp_layer, dim3(lg), dim3(256), 0, st, wt + off, d_out, d_in, l > 0 ? acts[l] : (const _Float16*)nullptr,
+                           n, 0);
+        std::swap(d_out, d_in);
+    }
+    // d_out now holds dLoss/dX0
+    hipLaunchKernelGGL(k_rc_encode_bwd, dim3(grid_for((int)n, 256, cus * 16)), dim3(256), 0, st, c->grid, queries, n, d_out,
+                       c->tgrad.as<float>());
+    c->step++;
+    const float b1 = c->cfg.beta1, b2 = c->cfg.beta2;
+    const float c1 = 1.0f - (float)std::pow((double)b1, (double)c->step), c2 = 1.0f - (float)std::pow((double)b2, (double)c->step);
+    const float gs = 1.0f / (float)n;
+    hipLaunchKernelGGL(k_rc_adam, dim3(grid_for((int)kMlpParams, 256, 64)), dim3(256), 0, st, c->w32.as<float>(), c->wm.as<float>(),
+                       c->wv.as<float>(), (const float*)wg, kMlpParams, gs, c->cfg.learning_rate, b1, b2, c->cfg.epsilon, c1, c2,
+                       c->w16.as<_Float16>(), c->w16t.as<_Float16>(), 0);
+    hipLaunchKernelGGL(k_rc_adam, dim3(grid_for((int)c->table_floats, 256, cus * 16)), dim3(256), 0, st, c->table32.as<float>(),
+                       c->tm.as<float>(), c->tv.as<float>(), (const float*)c->tgrad.as<float>(), c->table_floats, gs,
+                       c->cfg.learning_rate, b1, b2, c->cfg.epsilon, c1, c2, c->table16.as<_Float16>(), (_Float16*)nullptr, 1);
+    HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+int st_cache_train(st_cache* c, const float* queries, const float* targets, uint32_t n, float* loss) {
+    if (!c || (n && (!queries || !targets))) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (n > kCacheMaxRows) return set_error(ST_ERROR_LIMIT, "at most 2^24 samples per step");
+    if (n == 0) return ST_OK;
+    try {
+        HIP_CHECK(hipSetDevice(c->ctx->device));
+        order_after_caller(c->ctx);
+        cache_train_step(c, queries, targets, n);
+        float l = 0;
+        HIP_CHECK(hipMemcpyAsync(&l, c->loss.p, 4, hipMemcpyDeviceToHost, c->ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+        if (loss) *loss = l / (float)n;
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_cache_param_counts(const st_cache* c, uint32_t* table_floats, uint32_t* weight_floats) {
+    if (!c) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (table_floats) *table_floats = c->table_floats;
+    if (weight_floats) *weight_floats = kMlpParams;
+    return ST_OK;
+}
+
+int st_cache_get_params(st_cache* c, float* table, float* weights, float* table_grad, float* weight_grad) {
+    if (!c) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    try {
+        HIP_CHECK(hipSetDevice(c->ctx->device));
+        HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+        if (table) HIP_CHECK(hipMemcpy(table, c->table32.p, 4ull * c->table_floats, hipMemcpyDeviceToHost));
+        if (weights) HIP_CHECK(hipMemcpy(weights, c->w32.p, 4ull * kMlpParams, hipMemcpyDeviceToHost));
+        if (table_grad) HIP_CHECK(hipMemcpy(table_grad, c->tgrad.p, 4ull * c->table_floats, hipMemcpyDeviceToHost));
+        if (weight_grad) HIP_CHECK(hipMemcpy(weight_grad, c->wgrad.p, 4ull * kMlpParams, hipMemcpyDeviceToHost));
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_cache_set_params(st_cache* c, const float* table, const float* weights) {
+    if (!c) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    try {
+        HIP_CHECK(hipSetDevice(c->ctx->device));
+        HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
+        if (table) HIP_CHECK(hipMemcpy(c->table32.p, table, 4ull * c->table_floats, hipMemcpyHostToDevice));
+        if (weights) HIP_CHECK(hipMemcpy(c->w32.p, weights, 4ull * kMlpParams, hipMemcpyHostToDevice));
+        cache_copy16(c);
+        HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
