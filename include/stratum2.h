@@ -311,6 +311,45 @@ int st_cache_param_counts(const st_cache* cache, uint32_t* table_floats, uint32_
 int st_cache_get_params(st_cache* cache, float* table, float* weights, float* table_grad, float* weight_grad);
 int st_cache_set_params(st_cache* cache, const float* table, const float* weights);
 
+/* Camera paths that end in the cache (no reference counterpart: st_render's
+ * estimator with its paths cut short, Mueller et al. 2021).  A path reaching
+ * a scattering vertex at path length cache_depth (3: after two bounces), with
+ * cache_depth < max_depth, adds that vertex's emission as st_render does and
+ * then, instead of NEE and a continuation, throughput x the cache's radiance
+ * at the vertex (query: position, direction towards the previous vertex,
+ * shading normal (0 at a medium vertex), albedo) and ends.  Training paths
+ * are exempt: of each run of train_stride consecutive paths (path index =
+ * sample * W * H + y * W + x, samples of this call counted from 0) the one at
+ * offset pcg(run ^ pcg(frame)) % train_stride continues exactly as in
+ * st_render, and its cache-depth vertex becomes training row `run`: the query
+ * and target = (the path's final radiance - the radiance it had gathered up
+ * to and including that vertex's emission) / its throughput there.  Rows of
+ * training paths that end before the cache depth, or at a black vertex, have
+ * weight 0.  After the render, train_steps Adam steps on the rows (all of
+ * them each step; train_steps 0 leaves the cache unchanged).  Inference in
+ * this call uses the cache as it was on entry.
+ * train_queries (rows x 12 floats) and train_targets (rows x 4: target rgb,
+ * weight), rows = ceil(spp * W * H / train_stride), are optional device
+ * outputs; loss (host, may be null) receives the mean loss of the first step.
+ * ReSTIR DI is allowed only with ST_FLAG_RESTIR_DI_REUSE (RIS at primary
+ * vertices); light tracing and the bidirectional mode are not. */
+typedef struct st_cache_render_params {
+    uint32_t cache_depth;     /* path length of the vertex that queries the cache (3) */
+    uint32_t train_stride;    /* one training path per this many paths (>= 1) */
+    uint32_t train_steps;     /* Adam steps after the render (1) */
+    uint32_t frame;           /* selects this call's training paths */
+    float* train_queries;     /* optional device output */
+    float* train_targets;     /* optional device output */
+    float* loss;              /* optional host output */
+} st_cache_render_params;
+int st_render_cached(st_scene* scene, st_cache* cache, const st_view* view, const st_render_params* params,
+                     const st_cache_render_params* cache_params, const st_outputs* outputs);
+/* Device time of the last st_render_cached's cache work (HIP events):
+ * out[0] ms of inference + target kernels, out[1] their launches, out[2] ms
+ * of training steps, out[3] steps.  The render's own kernel classes are in
+ * st_last_kernel_times. */
+int st_cache_last_times(st_context* ctx, double out[4]);
+
 /* Primary-ray closest hit only (the minimum slice): visibility for each pixel
  * and, when t_out is not null, the hit distance as the path state stores it
  * (fp16-rounded, inf on a miss).  Device pointers. */

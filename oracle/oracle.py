@@ -73,6 +73,7 @@ def lib():
         l.or_scene_set_triangle_albedo.argtypes = [vp, vp, u32]
         l.or_scene_set_weight_grid.argtypes = [vp, u32, vp, C.c_uint64]
         l.or_adjoint.argtypes = [vp, pv, C.POINTER(abi.st_render_params), vp, vp, vp, u32, C.c_int]
+        l.or_render_cached.argtypes = [vp, pv, C.POINTER(abi.st_render_params), u32, u32, u32, vp, vp, C.c_int]
         _lib = l
     return _lib
 
@@ -113,6 +114,24 @@ class OracleScene:
             raise RuntimeError("oracle render: " + self.lib.or_last_error().decode())
         return dict(hdr=hdr, visibility=vis, path_length=pl, ray_counts=rc, traversal=tr, albedo=alb, depth=dep,
                     prev_uv=puv)
+
+    def render_cached(self, view, params, depth, stride, frame=0, threads=0):
+        """or_render_cached: the render of st_render_cached with the cache
+        answering 0, and recs (spp, H, W, 25) per path: kind (0 none, 1 cache
+        query, 2 training row), query[12], throughput[3], radiance gathered up to
+        the vertex[3], the path's fp32 part[3], its fixed-point part / 16384 [3]."""
+        W, H, S = params.width, params.height, params.spp
+        hdr = np.zeros((H, W, 4), np.float32)
+        vis = np.zeros((H, W, 2), np.uint32)
+        pl = np.zeros((S, H, W), np.uint32)
+        rc = np.zeros(2, np.uint64)
+        recs = np.zeros((S, H, W, 25), np.float32)
+        o = or_outputs(hdr.ctypes.data, vis.ctypes.data, pl.ctypes.data, rc.ctypes.data, None, None, None, None)
+        st = self.lib.or_render_cached(self.h, C.byref(view), C.byref(params), depth, stride, frame, C.byref(o), recs.ctypes.data,
+                                       threads)
+        if st != 0:
+            raise RuntimeError("oracle render_cached: " + self.lib.or_last_error().decode())
+        return dict(hdr=hdr, visibility=vis, path_length=pl, ray_counts=rc), recs
 
     def triangle_count(self):
         return int(self.lib.or_scene_triangle_count(self.h))
@@ -266,3 +285,44 @@ class OracleDenoiser:
             self.accumulated_frames = 0
         self.prev = (color, moments, vis.copy(), depth.copy())
         return out.copy()
+
+
+def cached_image(recs, radiance=None):
+    """The image st_render_cached makes from or_render_cached's records when
+    the cache answers radiance (n, 3) to the n cache queries in (sample,
+    pixel) order (None: 0): per path e = fp32 part (+ throughput x radiance),
+    a float32 running sum over samples of e + the fixed-point part, times
+    1/spp (the GPU's resolve)."""
+    S, H, W = recs.shape[:3]
+    e = recs[..., 19:22].copy()
+    if radiance is not None:
+        q = recs[..., 0] == 1
+        e[q] = (e[q] + (recs[..., 13:16][q] * np.asarray(radiance, np.float32)).astype(np.float32)).astype(np.float32)
+    a = np.zeros((H, W, 3), np.float32)
+    for k in range(S):
+        a = (a + (e[k] + recs[k, ..., 22:25]).astype(np.float32)).astype(np.float32)
+    out = np.ones((H, W, 4), np.float32)
+    out[..., :3] = a * np.float32(np.float32(1.0) / np.float32(S))
+    return out
+
+
+def training_rows(recs, stride):
+    """The training rows st_render_cached writes: queries (rows, 12) and
+    targets (rows, 4) = {(total - gathered) / throughput per channel (0 where
+    the throughput is 0), 1} for the rows that have a training path record,
+    zeros elsewhere."""
+    S, H, W = recs.shape[:3]
+    flat = recs.reshape(-1, 25)
+    rows = (len(flat) + stride - 1) // stride
+    q = np.zeros((rows, 12), np.float32)
+    t = np.zeros((rows, 4), np.float32)
+    g = np.nonzero(flat[:, 0] == 2)[0]
+    r = flat[g]
+    total = (r[:, 19:22] + r[:, 22:25]).astype(np.float32)
+    thr = r[:, 13:16]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        tg = np.where(thr > 0, ((total - r[:, 16:19]).astype(np.float32) / thr).astype(np.float32), np.float32(0))
+    q[g // stride] = r[:, 1:13]
+    t[g // stride, :3] = tg
+    t[g // stride, 3] = 1
+    return q, t

@@ -1917,7 +1917,21 @@ typedef struct {
     uint32_t prim_dir;          /* packed primary ray direction */
     float res[8];               /* seed[4], target, W, M, depth */
     struct adj_s* adj;          /* path replay of or_adjoint, or NULL */
+    struct cache_s* cache;      /* or_render_cached: this path's cache-depth vertex, or NULL */
 } pctx_t;
+
+/* or_render_cached (st_render_cached without the network): a path reaching a
+ * scattering vertex at path length `depth` either ends there in a cache query
+ * or, as the training path of its run of `stride` paths, goes on unchanged
+ * and records its training row.  g = sample * W * H + y * W + x. */
+typedef struct cache_s {
+    uint32_t depth, stride, frame, g;
+    int kind;                /* 0 no record, 1 cache query, 2 training row */
+    float q[12], thr[3];
+    f3 out_before;           /* fp32 radiance (emission) up to and including the vertex */
+    int nsh_before;          /* shadow rays made before the vertex */
+    uint32_t nee_before[3];  /* their fixed-point NEE */
+} cache_t;
 
 /* Path replay (or_adjoint).  Both passes trace each shadow ray right after
  * the vertex that made it and add its fp32 contribution to acc, in the order
@@ -2324,6 +2338,21 @@ static void vc_after_sample(pstate_t* ps, const dsr_t* smp, float cos_out, int f
     }
 }
 
+/* The cache-depth vertex: the training path of a run records its row and
+ * goes on, any other path records its query and ends (returns 1). */
+static int cache_vertex(pctx_t* x, f3 pos, f3 wo, f3 nrm, f3 alb) {
+    cache_t* c = x->cache;
+    const uint32_t row = c->g / c->stride;
+    const int train = c->g % c->stride == or_pcg(row ^ or_pcg(c->frame)) % c->stride;
+    const float q[12] = {pos.x, pos.y, pos.z, wo.x, wo.y, wo.z, nrm.x, nrm.y, nrm.z, alb.x, alb.y, alb.z};
+    memcpy(c->q, q, sizeof q);
+    c->thr[0] = x->ps.thr.x; c->thr[1] = x->ps.thr.y; c->thr[2] = x->ps.thr.z;
+    c->kind = train ? 2 : 1;
+    c->out_before = x->out;
+    c->nsh_before = x->nsh;
+    return !train;
+}
+
 static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
     const or_scene* s = x->s;
     pstate_t* ps = &x->ps;
@@ -2341,6 +2370,7 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
     }
     if (ps->path_len >= x->c->max_depth) return 0;
     if (!med_can_eval(&m)) return 0;
+    if (x->cache && ps->path_len == x->cache->depth && cache_vertex(x, pos, ldin, v3(0, 0, 0), med_albedo(&m))) return 0;
     if (!med_singular(&m) && vc) {
         float rnd[4];
         rng_nextf(&ps->rng, rnd);
@@ -2534,6 +2564,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     }
     if (ps->path_len >= x->c->max_depth) return 0;
     if (!m_can_eval(&m)) return 0;
+    if (x->cache && ps->path_len == x->cache->depth && cache_vertex(x, sd.pos, neg(dir), unpack_normal(sd.sn), m_base(&m))) return 0;
     if (!m_singular(&m, x->c->lambert) && vc) {
         connect_to_light(x, &m, &sd, ldin);
         connect_to_light_subpath(x, &m, &sd, NULL, sd.pos, ldin);
@@ -2610,6 +2641,10 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
     x->out = v3(0, 0, 0);
     x->nee[0] = x->nee[1] = x->nee[2] = 0;
     x->nsh = 0;
+    if (x->cache) {
+        x->cache->kind = 0;
+        x->cache->nee_before[0] = x->cache->nee_before[1] = x->cache->nee_before[2] = 0;
+    }
     rng_init(&ps->rng, seed, px, py, 0);
     float ox = 0.5f, oy = 0.5f;
     if (x->c->flags & ST_FLAG_PIXEL_JITTER) {
@@ -2672,6 +2707,7 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
             if (x->c->flags & ST_FLAG_FLOAT_NEE) continue;
             int32_t iv = q[k] >= 2147483647.0f ? INT_MAX : (int32_t)q[k];
             x->nee[k] += (uint32_t)iv;
+            if (x->cache && x->cache->kind && i < x->cache->nsh_before) x->cache->nee_before[k] += (uint32_t)iv;
         }
         if (x->c->flags & ST_FLAG_FLOAT_NEE) x->out = add(x->out, v3(maxf(0, rw.x), maxf(0, rw.y), maxf(0, rw.z)));
     }
@@ -3185,8 +3221,29 @@ static float mis_pair(float a, float b) { /* mis(a, b), testrenderer.slang:287-2
     return d > 0 ? a2 / d : 0.0f;
 }
 
+/* or_render_cached's record of path g: kind, query, throughput, the radiance
+ * gathered before (training rows), the path's fp32 and fixed-point parts.
+ * nee: the path's final fixed-point sums; extra: fixed point added to the
+ * path before its cache-depth vertex outside its own shadow rays (the
+ * screen-space reuse's primary NEE). */
+#define OR_CACHE_REC 25
+static void cache_store(const cache_t* c, f3 out_final, const uint32_t nee[3], const uint32_t extra[3], float* r) {
+    memset(r, 0, OR_CACHE_REC * sizeof(float));
+    r[0] = (float)c->kind;
+    if (c->kind) {
+        memcpy(r + 1, c->q, sizeof c->q);
+        memcpy(r + 13, c->thr, sizeof c->thr);
+    }
+    if (c->kind == 2) {
+        const float ob[3] = {c->out_before.x, c->out_before.y, c->out_before.z};
+        for (int k = 0; k < 3; k++) r[16 + k] = ob[k] + (float)(c->nee_before[k] + extra[k]) / (float)OR_Q;
+    }
+    r[19] = out_final.x; r[20] = out_final.y; r[21] = out_final.z;
+    for (int k = 0; k < 3; k++) r[22 + k] = (float)nee[k] / (float)OR_Q;
+}
+
 static int render_reuse(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads,
-                        const cfg_t* c) {
+                        const cfg_t* c, const uint32_t* cc, float* recs) {
     const uint32_t W = p->width, H = p->height;
     const size_t npix = (size_t)W * H;
     prim_t* prim = (prim_t*)calloc(npix, sizeof(prim_t));
@@ -3196,8 +3253,10 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
     f3* em = (f3*)calloc(npix, sizeof(f3));
     uint32_t* nee = (uint32_t*)calloc(npix * 3, sizeof(uint32_t));
     f3* sum = (f3*)calloc(npix, sizeof(f3));
-    if (!prim || !rinit || !rtemp || !rprev || !em || !nee || !sum) {
-        free(prim); free(rinit); free(rtemp); free(rprev); free(em); free(nee); free(sum);
+    cache_t* crec = cc ? (cache_t*)calloc(npix, sizeof(cache_t)) : NULL;
+    uint32_t* nee_path = cc ? (uint32_t*)calloc(npix * 3, sizeof(uint32_t)) : NULL;
+    if (!prim || !rinit || !rtemp || !rprev || !em || !nee || !sum || (cc && (!crec || !nee_path))) {
+        free(prim); free(rinit); free(rtemp); free(rprev); free(em); free(nee); free(sum); free(crec); free(nee_path);
         return fail("out of memory");
     }
     const uint32_t view_medium = camera_medium(s, view);
@@ -3223,6 +3282,12 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
             x.W = W;
             x.H = H;
             x.reuse = 1;
+            cache_t xc;
+            if (cc) {
+                memset(&xc, 0, sizeof xc);
+                xc.depth = cc[0]; xc.stride = cc[1]; xc.frame = cc[2];
+                x.cache = &xc;
+            }
             /* 1. paths, primary reservoirs */
 #pragma omp for schedule(dynamic, 64)
             for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
@@ -3233,7 +3298,12 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
                 memset(x.res, 0, sizeof x.res);
                 f3 r;
                 uint32_t pl;
+                if (cc) xc.g = k * (uint32_t)npix + (uint32_t)pix;
                 render_pixel(&x, p, view, px, py, seed, &r, &pl);
+                if (cc) {
+                    crec[pix] = xc;
+                    for (int j = 0; j < 3; j++) nee_path[3 * pix + j] = x.nee[j];
+                }
                 if (k == 0 && out->visibility) { out->visibility[2 * pix] = x.vis[0]; out->visibility[2 * pix + 1] = x.vis[1]; }
                 if (k == 0 && out->albedo) {
                     for (int j = 0; j < 3; j++) out->albedo[4 * pix + j] = x.alb[j];
@@ -3340,6 +3410,11 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
             /* resolve: the running sum over samples (ProcessAtomicOutput) */
 #pragma omp for schedule(static)
             for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+                if (cc && recs) {
+                    uint32_t extra[3];
+                    for (int j = 0; j < 3; j++) extra[j] = nee[3 * pix + j] - nee_path[3 * pix + j];
+                    cache_store(&crec[pix], em[pix], &nee[3 * pix], extra, recs + OR_CACHE_REC * ((size_t)k * npix + (size_t)pix));
+                }
                 sum[pix].x += em[pix].x + (float)nee[3 * pix] / (float)OR_Q;
                 sum[pix].y += em[pix].y + (float)nee[3 * pix + 1] / (float)OR_Q;
                 sum[pix].z += em[pix].z + (float)nee[3 * pix + 2] / (float)OR_Q;
@@ -3360,7 +3435,7 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
         }
     if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
     if (out->traversal) for (int j = 0; j < 4; j++) out->traversal[j] = trav[j];
-    free(prim); free(rinit); free(rtemp); free(rprev); free(em); free(nee); free(sum);
+    free(prim); free(rinit); free(rtemp); free(rprev); free(em); free(nee); free(sum); free(crec); free(nee_path);
     return 0;
 }
 
@@ -3373,7 +3448,28 @@ static int check_tri_albedo(const or_scene* s, uint32_t flags) {
     return 0;
 }
 
+static int render_camera(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads,
+                         const uint32_t* cc, float* recs);
+
 int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
+    return render_camera(s, view, p, out, threads, NULL, NULL);
+}
+
+int or_render_cached(or_scene* s, const st_view* view, const st_render_params* p, uint32_t depth, uint32_t stride,
+                     uint32_t frame, const or_outputs* out, float* recs, int threads) {
+    if (!s || !view || !p || !out) return fail("null argument");
+    if (p->flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC)) return fail("the radiance cache ends camera paths");
+    if ((p->flags & ST_FLAG_RESTIR_DI) && !(p->flags & ST_FLAG_RESTIR_DI_REUSE))
+        return fail("with the radiance cache, ReSTIR DI runs at primary vertices (ST_FLAG_RESTIR_DI_REUSE)");
+    if (p->flags & ST_FLAG_FLOAT_NEE) return fail("fp32 NEE with the radiance cache is not restated");
+    if (depth < 1 || depth >= p->max_depth || stride < 1) return fail("cache depth in [1, max_depth), stride >= 1");
+    if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && depth < 2) return fail("with screen-space reuse: cache depth >= 2");
+    const uint32_t cc[3] = {depth, stride, frame};
+    return render_camera(s, view, p, out, threads, cc, recs);
+}
+
+static int render_camera(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads,
+                         const uint32_t* cc, float* recs) {
     if (!s || !view || !p || !out) return fail("null argument");
     if (p->width == 0 || p->height == 0 || p->spp == 0) return fail("empty render");
     if (check_tri_albedo(s, p->flags)) return -1;
@@ -3394,7 +3490,7 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
     c.max_null = p->max_null_collisions;
     c.rr_start = p->rr_start_depth;
-    if (reuse) return render_reuse(s, view, p, out, threads, &c);
+    if (reuse) return render_reuse(s, view, p, out, threads, &c, cc, recs);
     const uint32_t view_medium = camera_medium(s, view);
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
@@ -3415,6 +3511,12 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
         x.inv_cam = inv_cam;
         x.W = W;
         x.H = H;
+        cache_t xc;
+        if (cc) {
+            memset(&xc, 0, sizeof xc);
+            xc.depth = cc[0]; xc.stride = cc[1]; xc.frame = cc[2];
+            x.cache = &xc;
+        }
 #pragma omp for schedule(dynamic, 64)
         for (int64_t pix = 0; pix < (int64_t)W * H; pix++) {
             uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
@@ -3424,7 +3526,12 @@ int or_render(or_scene* s, const st_view* view, const st_render_params* p, const
             for (uint32_t k = 0; k < p->spp; k++) {
                 f3 r;
                 uint32_t pl;
+                if (cc) xc.g = k * W * H + (uint32_t)pix;
                 render_pixel(&x, p, view, px, py, p->seed + k, &r, &pl);
+                if (cc && recs) {
+                    const uint32_t none[3] = {0, 0, 0};
+                    cache_store(&xc, x.out, x.nee, none, recs + OR_CACHE_REC * ((size_t)k * W * H + (size_t)pix));
+                }
                 if (k == 0 && out->visibility) { out->visibility[2 * pix] = x.vis[0]; out->visibility[2 * pix + 1] = x.vis[1]; }
                 if (k == 0 && out->albedo) {
                     for (int j = 0; j < 3; j++) out->albedo[4 * pix + j] = x.alb[j];

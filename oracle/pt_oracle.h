@@ -41,6 +41,17 @@ const char* or_last_error(void);
 int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out,
               int threads);
 
+/* st_render_cached without the network: as or_render, while a path reaching a
+ * scattering vertex at path length depth ends there in a cache query (adding
+ * nothing here) unless it is the training path of its run of stride paths
+ * (offset pcg(run ^ pcg(frame)) % stride).  recs (may be NULL): 25 floats per
+ * path (index sample * W * H + y * W + x): kind (0 none, 1 cache query,
+ * 2 training row), query[12], throughput[3], the radiance gathered up to and
+ * including the vertex's emission (kind 2)[3], the path's fp32 part[3] and
+ * its fixed-point part / 16384 [3]. */
+int or_render_cached(or_scene* s, const st_view* view, const st_render_params* p, uint32_t depth, uint32_t stride,
+                     uint32_t frame, const or_outputs* out, float* recs, int threads);
+
 /* Closest-hit / any-hit of raw rays, same layout as st_trace_rays. */
 int or_trace_rays(or_scene* s, const float* rays, uint32_t count, int closest, uint32_t* hits);
 

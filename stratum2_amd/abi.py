@@ -123,6 +123,11 @@ def default_cache_config(**kw):
     return c
 
 
+class st_cache_render_params(C.Structure):
+    _fields_ = [("cache_depth", u32), ("train_stride", u32), ("train_steps", u32), ("frame", u32),
+                ("train_queries", C.c_void_p), ("train_targets", C.c_void_p), ("loss", C.POINTER(C.c_float))]
+
+
 class st_batch_config(C.Structure):
     _fields_ = [("trace_blocks_per_cu", u32), ("shade_blocks_per_cu", u32), ("wave_paths", u32), ("streams", u32),
                 ("ms", C.c_double)]
@@ -212,7 +217,7 @@ EXPORTS = [
     "st_batch_create", "st_batch_render", "st_batch_autotune", "st_batch_get_config", "st_batch_set_config",
     "st_batch_destroy", "st_render_batch", "st_scene_set_triangle_albedo", "st_scene_set_volume_grid", "st_adjoint",
     "st_cache_create", "st_cache_destroy", "st_cache_infer", "st_cache_train", "st_cache_param_counts", "st_cache_get_params",
-    "st_cache_set_params",
+    "st_cache_set_params", "st_render_cached", "st_cache_last_times",
 ]
 
 _lib = None
@@ -268,6 +273,9 @@ def load_library(path=LIB_PATH):
     lib.st_cache_param_counts.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
     lib.st_cache_get_params.argtypes = [vp, vp, vp, vp, vp]
     lib.st_cache_set_params.argtypes = [vp, vp, vp]
+    lib.st_render_cached.argtypes = [vp, vp, C.POINTER(st_view), C.POINTER(st_render_params),
+                                     C.POINTER(st_cache_render_params), C.POINTER(st_outputs)]
+    lib.st_cache_last_times.argtypes = [vp, C.POINTER(C.c_double)]
     if path == LIB_PATH:
         _lib = lib
     return lib

@@ -1,11 +1,12 @@
-"""The learned radiance cache over the C ABI (st_cache_*): queries and targets
-are torch device tensors; parameters come back as numpy arrays."""
+"""The learned radiance cache over the C ABI (st_cache_*, st_render_cached):
+queries and targets are torch device tensors; parameters come back as numpy
+arrays."""
 import ctypes as C
 
 import numpy as np
 
 from . import abi
-from .renderer import _check_tensor, _enter, _torch
+from .renderer import TestRenderer, _check_tensor, _enter, _torch
 
 
 class RadianceCache:
@@ -73,3 +74,36 @@ class RadianceCache:
         w = np.ascontiguousarray(weights, np.float32) if weights is not None else None
         abi.check(self.lib, self.lib.st_cache_set_params(self.h, t.ctypes.data if t is not None else None,
                                                          w.ctypes.data if w is not None else None), "st_cache_set_params")
+
+    def render(self, scene, view, params, cache_depth=3, train_stride=16, train_steps=1, frame=0, want_train=False,
+               want_path_length=False, out=None):
+        """st_render_cached: the path tracer with paths ending in this cache at
+        path length cache_depth, one training path per train_stride paths, then
+        train_steps Adam steps on the training rows.  Returns the render's
+        output tensors plus "loss" (mean loss of the first step, or None) and,
+        with want_train, "train_queries" (rows, 12) and "train_targets"
+        (rows, 4: target rgb, weight)."""
+        torch = _torch()
+        if scene.ctx is not self.ctx:
+            raise ValueError("the scene and the cache belong to different contexts")
+        res, o = TestRenderer(scene).outputs(params, want_path_length, out)
+        rows = (params.width * params.height * params.spp + train_stride - 1) // max(1, train_stride)
+        dev = torch.device("cuda", self.ctx.device)
+        tq = torch.empty((rows, 12), dtype=torch.float32, device=dev) if want_train else None
+        tt = torch.empty((rows, 4), dtype=torch.float32, device=dev) if want_train else None
+        loss = C.c_float(0)
+        cp = abi.st_cache_render_params(cache_depth, train_stride, train_steps, frame, tq.data_ptr() if want_train else None,
+                                        tt.data_ptr() if want_train else None, C.pointer(loss))
+        _enter(self.ctx.device)
+        abi.check(self.lib, self.lib.st_render_cached(scene.h, self.h, C.byref(view), C.byref(params), C.byref(cp), C.byref(o)),
+                  "st_render_cached")
+        res["loss"] = loss.value if train_steps else None
+        if want_train:
+            res["train_queries"], res["train_targets"] = tq, tt
+        return res
+
+    def last_times(self):
+        """(inference + target ms, their launches, training ms, steps) of the last render."""
+        out = (C.c_double * 4)()
+        abi.check(self.lib, self.lib.st_cache_last_times(self.ctx.h, out), "st_cache_last_times")
+        return tuple(out)

@@ -70,29 +70,107 @@ __device__ __forceinline__ void rc_corners(const CacheGrid& g, const float* q, u
     }
 }
 
+// The two hash features of level l: trilinear over the level's cell
+__device__ __forceinline__ void rc_level(const CacheGrid& g, const _Float16* table, const float* q, uint32_t l, _Float16& fa,
+                                         _Float16& fb) {
+    uint32_t idx[8];
+    float wt[8];
+    rc_corners(g, q, l, idx, wt);
+    float a = 0, b = 0;
+    for (int k = 0; k < 8; k++) {
+        a += wt[k] * (float)table[2 * (size_t)idx[k]];
+        b += wt[k] * (float)table[2 * (size_t)idx[k] + 1];
+    }
+    fa = (_Float16)a;
+    fb = (_Float16)b;
+}
+// Raw input i after the hash features: (direction + 1) / 2, (normal + 1) / 2,
+// albedo, a constant 1, then zeros
+__device__ __forceinline__ _Float16 rc_raw(const float* q, uint32_t i) {
+    if (i < 6) return (_Float16)((q[3 + i] + 1) * 0.5f);
+    if (i < 9) return (_Float16)q[3 + i];
+    return (_Float16)(i == 9 ? 1.0f : 0.0f);
+}
+// Network inputs f and f + 1 (f even) of query q
+__device__ __forceinline__ void rc_feature_pair(const CacheGrid& g, const _Float16* table, const float* q, uint32_t f, _Float16& a,
+                                                _Float16& b) {
+    if (f / 2 < g.levels) {
+        rc_level(g, table, q, f / 2, a, b);
+    } else {
+        a = rc_raw(q, f - 2 * g.levels);
+        b = rc_raw(q, f + 1 - 2 * g.levels);
+    }
+}
+
 // X[s] (64 fp16) from query s: hash features, then the raw inputs
 __global__ void k_rc_encode(CacheGrid g, const _Float16* table, const float* queries, uint32_t n, _Float16* X) {
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         const float* q = queries + (size_t)kCacheQuery * s;
         _Float16* x = X + (size_t)kMlpWidth * s;
-        for (uint32_t l = 0; l < g.levels; l++) {
-            uint32_t idx[8];
-            float wt[8];
-            rc_corners(g, q, l, idx, wt);
-            float a = 0, b = 0;
-            for (int k = 0; k < 8; k++) {
-                a += wt[k] * (float)table[2 * (size_t)idx[k]];
-                b += wt[k] * (float)table[2 * (size_t)idx[k] + 1];
-            }
-            x[2 * l] = (_Float16)a;
-            x[2 * l + 1] = (_Float16)b;
+        for (uint32_t f = 0; f < (uint32_t)kMlpWidth; f += 2) rc_feature_pair(g, table, q, f, x[f], x[f + 1]);
+    }
+}
+
+// Inference for the paths of a wave that ended in the cache, fused into one
+// kernel: each wavefront takes 16 queries at a time, encodes them straight
+// into the first layer's B operands (lane l: features 16 kb + 4 (l / 16) + j
+// of query l % 16) and runs the four layers in registers, each layer's D
+// tile being the next layer's B operand; the weights sit in LDS.  The
+// arithmetic is that of k_rc_encode + k_mlp_layer + k_rc_output, so a query
+// gets the radiance st_cache_infer gives it.  em[path] += throughput x radiance.
+// cq: 4 float4 per query ({pos, thr.x}, {wo, thr.y}, {normal, thr.z},
+// {albedo, path}); count: queries (device).
+__global__ void __launch_bounds__(256) k_rc_infer_paths(CacheGrid g, const _Float16* table, const _Float16* W16, const float4* cq,
+                                                         const uint32_t* count, float4* em) {
+    constexpr int kH4 = kMlpLayers * kMlpWidth * kMlpWidth / 4;
+    __shared__ h4 Wl[kH4];
+    const h4* Wg = reinterpret_cast<const h4*>(W16);
+    for (int i = threadIdx.x; i < kH4; i += blockDim.x) Wl[i] = Wg[i];
+    __syncthreads();
+    const uint32_t n = *count;
+    const uint32_t lane = __lane_id(), r = lane % 16, qd = lane / 16;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, waves = gridDim.x * blockDim.x / 64;
+    for (uint32_t base = wave * 16; base < n; base += waves * 16) {  // wave-uniform bound: every lane runs every MFMA
+        const uint32_t s = base + r;
+        const bool in = s < n;
+        float q[kCacheQuery];
+        float4 c0 = make_float4(0, 0, 0, 0), c1 = c0, c2 = c0, c3 = c0;
+        if (in) {
+            c0 = cq[4 * (size_t)s];
+            c1 = cq[4 * (size_t)s + 1];
+            c2 = cq[4 * (size_t)s + 2];
+            c3 = cq[4 * (size_t)s + 3];
         }
-        uint32_t o = 2 * g.levels;
-        for (int a = 0; a < 3; a++) x[o++] = (_Float16)((q[3 + a] + 1) * 0.5f);
-        for (int a = 0; a < 3; a++) x[o++] = (_Float16)((q[6 + a] + 1) * 0.5f);
-        for (int a = 0; a < 3; a++) x[o++] = (_Float16)q[9 + a];
-        x[o++] = (_Float16)1.0f;
-        for (; o < (uint32_t)kMlpWidth; o++) x[o] = (_Float16)0.0f;
+        q[0] = c0.x, q[1] = c0.y, q[2] = c0.z, q[3] = c1.x, q[4] = c1.y, q[5] = c1.z;
+        q[6] = c2.x, q[7] = c2.y, q[8] = c2.z, q[9] = c3.x, q[10] = c3.y, q[11] = c3.z;
+        h4 B[4];
+        for (int kb = 0; kb < 4; kb++) {
+            const uint32_t f = kb * 16 + 4 * qd;
+            _Float16 a, b, c, d;
+            rc_feature_pair(g, table, q, f, a, b);
+            rc_feature_pair(g, table, q, f + 2, c, d);
+            B[kb] = in ? h4{a, b, c, d} : h4{0, 0, 0, 0};
+        }
+        f4v acc[4];
+        for (int l = 0; l < kMlpLayers; l++) {
+            const h4* A = Wl + l * (kMlpWidth * kMlpWidth / 4);
+            for (int ob = 0; ob < 4; ob++) {
+                acc[ob] = f4v{0, 0, 0, 0};
+                for (int kb = 0; kb < 4; kb++)
+                    acc[ob] = __builtin_amdgcn_mfma_f32_16x16x16f16(A[(ob * 16 + r) * 16 + kb * 4 + qd], B[kb], acc[ob], 0, 0, 0);
+            }
+            if (l < kMlpLayers - 1)
+                for (int ob = 0; ob < 4; ob++)
+                    for (int j = 0; j < 4; j++) B[ob][j] = (_Float16)fmaxf(acc[ob][j], 0.0f);
+        }
+        if (in && qd == 0) {  // lanes 0..15 hold output features 0..3 of their query
+            const uint32_t p = __float_as_uint(c3.w);
+            float4 e = em[p];
+            e.x += c0.w * fmaxf((float)(_Float16)acc[0][0], 0.0f);
+            e.y += c1.w * fmaxf((float)(_Float16)acc[0][1], 0.0f);
+            e.z += c2.w * fmaxf((float)(_Float16)acc[0][2], 0.0f);
+            em[p] = e;
+        }
     }
 }
 
@@ -197,21 +275,33 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(const _Float16* dY, const _Fl
 // Radiance Caching for Path Tracing", 2021): per sample and channel
 // (pred - target)^2 / (pred^2 + 0.01) / 3, with pred held fixed in the
 // denominator.  dY[s][c] is the per-sample gradient (the 1/n goes into the
-// weights' gradient scale); loss_sum receives the summed loss.
-__global__ void k_rc_loss(const _Float16* Y, const float* targets, uint32_t n, _Float16* dY, float* loss_sum) {
-    float local = 0;
+// weights' gradient scale).  targets: tstride floats per sample, rgb and,
+// with tstride 4, a weight (0: the sample is left out).  loss_sum[0]
+// receives the weighted loss sum, loss_sum[1] the weight sum.
+__global__ void k_rc_loss(const _Float16* Y, const float* targets, uint32_t tstride, uint32_t n, _Float16* dY, float* loss_sum) {
+    float local = 0, wsum = 0;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         _Float16* d = dY + (size_t)kMlpWidth * s;
         for (int c = 0; c < kMlpWidth; c++) d[c] = (_Float16)0.0f;
+        const float* t = targets + (size_t)tstride * s;
+        const float wgt = tstride == 4 ? t[3] : 1.0f;
+        if (!(wgt > 0)) continue;
+        wsum += wgt;
         for (int c = 0; c < 3; c++) {
-            const float p = (float)Y[(size_t)kMlpWidth * s + c], t = targets[3 * (size_t)s + c];
+            const float p = (float)Y[(size_t)kMlpWidth * s + c];
             const float den = p * p + 0.01f;
-            local += (p - t) * (p - t) / den / 3;
-            d[c] = (_Float16)fminf(fmaxf(2 * (p - t) / den / 3, -30000.0f), 30000.0f);
+            local += wgt * ((p - t[c]) * (p - t[c]) / den / 3);
+            d[c] = (_Float16)fminf(fmaxf(wgt * (2 * (p - t[c]) / den / 3), -30000.0f), 30000.0f);
         }
     }
-    for (int off = 32; off > 0; off >>= 1) local += __shfl_xor(local, off);
-    if (__lane_id() == 0) atomicAdd(loss_sum, local);
+    for (int off = 32; off > 0; off >>= 1) {
+        local += __shfl_xor(local, off);
+        wsum += __shfl_xor(wsum, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(loss_sum, local);
+        atomicAdd(loss_sum + 1, wsum);
+    }
 }
 
 // Adam (Kingma and Ba 2015) over n parameters with fp32 master copies; g is

@@ -68,7 +68,7 @@ constexpr int kTraceBlock = 128;
 constexpr int kShadeBlock = 256;
 constexpr int kTraceBlocksPerCU = 16;
 // WaveState::counters (kCounterWords words): [0..1] queue counts (ping-pong),
-// [2] shadow count, [4..7] u64 ray counts, then the traversal fetch cursors:
+// [2] shadow count, [3] radiance-cache queries, [4..7] u64 ray counts, then the traversal fetch cursors:
 // kXcds per traversal kernel, kFetchStride words (one 128-byte line) apart
 constexpr int kFetchClosest = 256, kFetchShadow = kFetchClosest + (int)(kXcds * kFetchStride),
               kFetchRays = kFetchShadow + (int)(kXcds * kFetchStride);
@@ -99,6 +99,14 @@ struct WaveState {
     uint32_t* lv_off;   // exclusive prefix sums of lv_cnt
     float4* lv;         // the cache: vertices in (subpath, depth) order
     uint32_t* lv_total; // vertices in the cache
+    // radiance cache (st_render_cached): queries of paths ending in the cache
+    // ({pos, thr.x}, {wo, thr.y}, {normal, thr.z}, {albedo, path}), counted in
+    // counters[3]; training rows (12-float query, {throughput, 0}, {gathered
+    // radiance, weight} -> {target, weight})
+    float4* cq;
+    float* tq;
+    float4* tthr;
+    float4* tt;
 };
 
 struct RenderConsts {
@@ -106,6 +114,7 @@ struct RenderConsts {
     float env_prob;
     float max_m, radius;
     uint32_t view_medium, max_null;
+    uint32_t cache_depth, train_stride, train_frame;  // radiance cache: query depth, training-path selection
     uint32_t light_paths;  // light tracing and the bidirectional mode: light subpaths per sample (mLightSubpathCount)
     uint32_t lv_stride;    // bidirectional mode: light vertex slots per subpath (max_depth - 1)
     float ipd;             // light tracing: ViewData::imagePlaneDist (compat/scene.h:147)
@@ -327,6 +336,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_camera(RenderConsts c, WaveStat
         w.counters[0] = n;
         w.counters[1] = 0;
         w.counters[2] = 0;
+        w.counters[3] = 0;
         for (int i = 0; i < 2 * (int)kXcds; i++) w.counters[kFetchClosest + i * kFetchStride] = 0;
     }
 }
@@ -877,7 +887,42 @@ __global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c,
 // Lambertian vertex with a per-triangle albedo (every contribution after its
 // emission carries one factor albedo_c: dLoss/dL * (the path's first-pass
 // radiance - what it has gathered so far) / albedo_c).
-template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA, bool ADJ = false>
+// Training-path selection of st_render_cached: of each run of train_stride
+// consecutive paths (g = sample * W * H + pixel) one continues unchanged and
+// gives training row g / train_stride.
+__device__ __forceinline__ bool train_path(const RenderConsts& c, uint32_t g, uint32_t& row) {
+    row = g / c.train_stride;
+    return g % c.train_stride == rc_pcg(row ^ rc_pcg(c.train_frame)) % c.train_stride;
+}
+
+// The cache-depth vertex of st_render_cached: a training path writes its
+// row (query, throughput, radiance gathered so far incl. this vertex's
+// emission) and goes on; any other path fills a cache query and stops.
+__device__ __forceinline__ bool cache_vertex(const RenderConsts& c, const WaveState& w, uint32_t p, uint32_t g, v3 pos, v3 wo,
+                                             v3 nrm, v3 alb, v3 thr, float4& em, bool& em_dirty, float4 cqr[4]) {
+    uint32_t row;
+    if (train_path(c, g, row)) {
+        if (!em_dirty) {
+            em = w.em[p];
+            em_dirty = true;
+        }
+        const uint4 nq = w.nee[p];
+        float* q = w.tq + 12 * (size_t)row;
+        const float v[12] = {pos.x, pos.y, pos.z, wo.x, wo.y, wo.z, nrm.x, nrm.y, nrm.z, alb.x, alb.y, alb.z};
+        for (int i = 0; i < 12; i++) q[i] = v[i];
+        w.tthr[row] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+        w.tt[row] = make_float4(em.x + (float)nq.x / (float)kQuant, em.y + (float)nq.y / (float)kQuant,
+                                em.z + (float)nq.z / (float)kQuant, 1.0f);
+        return false;
+    }
+    cqr[0] = make_float4(pos.x, pos.y, pos.z, thr.x);
+    cqr[1] = make_float4(wo.x, wo.y, wo.z, thr.y);
+    cqr[2] = make_float4(nrm.x, nrm.y, nrm.z, thr.z);
+    cqr[3] = make_float4(alb.x, alb.y, alb.z, __uint_as_float(p));
+    return true;
+}
+
+template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA, bool ADJ = false, bool CACHE = false>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
                                                        AuxOut aux, int first_wave, AdjArgs adj) {
     __shared__ AdjTable adj_tab;  // ADJ only
@@ -900,6 +945,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
         float4 ray_out = sh0, thr_out = sh0;
         uint32_t adj_key = 0xffffffffu;  // ADJ: this vertex's albedo entry and gradient
         v3 adj_val = mk(0, 0, 0);
+        bool to_cache = false;  // CACHE: this path ends in a cache query
+        float4 cqr[4];
         if (valid) {
             p = q[i];
             const uint4 hit = w.hit[i];
@@ -961,6 +1008,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                         store_aux(c, aux, pix, px, py, hit.y, 0, true, pos, m.albedo(), t16);
                     const v3 ldin = -dir;
                     bool alive = path_len < c.max_depth && m.can_eval();
+                    if (CACHE && alive && path_len == c.cache_depth) {
+                        to_cache = cache_vertex(c, w, p, (c.sample_base + k) * c.npix + pix, pos, ldin, mk(0, 0, 0), m.albedo(),
+                                                thr, em, em_dirty, cqr);
+                        if (to_cache) alive = false;
+                    }
                     const bool rr_on = path_len >= c.rr_start;
                     if (alive && !m.singular() && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
                         // ConnectToLight (:751-779) with the phase function; cosOut = 1
@@ -1042,6 +1094,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                                  m.bo.z > 0 ? wg.z / m.bo.z : 0.0f);
                 }
                 bool alive = live && path_len < c.max_depth && any_pos(m.base());
+                if (CACHE && alive && path_len == c.cache_depth) {
+                    to_cache = cache_vertex(c, w, p, (c.sample_base + k) * c.npix + pix, sd.pos, -dir, unpack_normal(sd.sn),
+                                            m.base(), thr, em, em_dirty, cqr);
+                    if (to_cache) alive = false;
+                }
                 // Russian roulette from path length rr_start (0: every bounce, testrenderer.slang:890-892)
                 const bool rr_on = path_len >= c.rr_start;
                 const float cp_eff = rr_on ? m_cont_prob(m) : 1.0f;
@@ -1095,6 +1152,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
             }
         }
         if (ADJ) wave_add_keyed3(adj_tab, adj.grad_alb, adj_key, adj_val);
+        if (CACHE) {
+            const uint32_t ci = wave_append(&w.counters[3], to_cache);
+            if (to_cache)
+                for (int j = 0; j < 4; j++) w.cq[4 * (size_t)ci + j] = cqr[j];
+        }
         const uint32_t si = wave_append(&w.counters[2], emit_shadow);
         if (emit_shadow) {
             w.shadow[3 * (size_t)si] = sh0;
@@ -1762,6 +1824,30 @@ __global__ void k_resolve(RenderConsts c, WaveState w, float4* acc, float* hdr, 
     }
 }
 
+// Training targets of st_render_cached for the rows whose training path is
+// in this wave (rows [row_lo, row_hi)): per channel, (the path's final
+// radiance - the radiance it had gathered at the cache vertex) / its
+// throughput there (0 where that throughput is 0).  Rows without a record
+// keep weight 0.
+__global__ void k_rc_targets(RenderConsts c, WaveState w, uint32_t row_lo, uint32_t row_hi) {
+    const uint64_t g0 = (uint64_t)c.sample_base * c.npix, g1 = g0 + (uint64_t)c.k * c.npix;
+    for (uint32_t row = row_lo + blockIdx.x * blockDim.x + threadIdx.x; row < row_hi; row += gridDim.x * blockDim.x) {
+        const uint64_t g = (uint64_t)row * c.train_stride + rc_pcg(row ^ rc_pcg(c.train_frame)) % c.train_stride;
+        if (g < g0 || g >= g1) continue;
+        const float4 before = w.tt[row];
+        if (!(before.w > 0)) continue;
+        const uint32_t pix = (uint32_t)(g % c.npix);
+        const uint32_t p = (uint32_t)(g / c.npix - c.sample_base) * c.npix + path_of_pixel(c, pix % c.W, pix / c.W);
+        const float4 e = w.em[p];
+        const uint4 nq = w.nee[p];
+        const float4 th = w.tthr[row];
+        const float tx = e.x + (float)nq.x / (float)kQuant, ty = e.y + (float)nq.y / (float)kQuant,
+                    tz = e.z + (float)nq.z / (float)kQuant;
+        w.tt[row] = make_float4(th.x > 0 ? (tx - before.x) / th.x : 0.0f, th.y > 0 ? (ty - before.y) / th.y : 0.0f,
+                                th.z > 0 ? (tz - before.z) / th.z : 0.0f, 1.0f);
+    }
+}
+
 __global__ void k_count(WaveState w, int qsel, int media) {
     // rays traced this bounce: extension queue + shadow queue (the media
     // kernels count their traceRay segments themselves)
@@ -1968,6 +2054,7 @@ struct WaveBufs {
     DevBuf ray, thr, stt, em, nee, hit, q0, q1, shadow, counters, spill, pri, psd, rinit, rtemp, med;
     DevBuf vcm, lv_raw, lv, lv_cnt, lv_off, lv_total, scan_tmp;  // bidirectional mode
     DevBuf ltot;  // st_adjoint: each path's first-pass radiance
+    DevBuf cq;    // st_render_cached: queries of the paths that end in the cache
     hipStream_t stream = nullptr;        // slot 0: the context's stream
     hipEvent_t reuse_done = nullptr;     // this slot's last wave finished its screen-space reuse
     hipEvent_t resolve_done = nullptr;   // this slot's last wave finished its resolve
@@ -1984,11 +2071,13 @@ struct st_context {
     DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
     double timings[12] = {0};
     // HIP-event timed kernel classes: 0 closest-hit trace, 1 shading, 2 shadow
-    // trace, 3 primary-vertex RIS (k_ris), 4 screen-space reuse passes
-    static constexpr int kKinds = 5;
+    // trace, 3 primary-vertex RIS (k_ris), 4 screen-space reuse passes,
+    // 5 radiance-cache inference + targets, 6 radiance-cache training
+    static constexpr int kKinds = 7;
+    static constexpr int kRenderKinds = 5;  // the classes st_last_kernel_times reports
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[kKinds];
-    size_t ev_used[kKinds] = {0, 0, 0, 0, 0};
-    double kernel_ms[kKinds] = {0, 0, 0, 0, 0};
+    size_t ev_used[kKinds] = {0, 0, 0, 0, 0, 0, 0};
+    double kernel_ms[kKinds] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 struct st_scene {
@@ -2730,6 +2819,15 @@ static ShadeFn shade_kernel(bool lambert, bool alias, bool restir, bool media) {
     return table[i];
 }
 
+static ShadeFn shade_kernel_cache(bool lambert, bool alias, bool media) {
+    static const ShadeFn t[8] = {
+        k_shade<false, false, false, false, false, true>, k_shade<false, true, false, false, false, true>,
+        k_shade<true, false, false, false, false, true>,  k_shade<true, true, false, false, false, true>,
+        k_shade<false, false, false, true, false, true>,  k_shade<false, true, false, true, false, true>,
+        k_shade<true, false, false, true, false, true>,   k_shade<true, true, false, true, false, true>};
+    return t[(media ? 4 : 0) | (lambert ? 2 : 0) | (alias ? 1 : 0)];
+}
+
 static void ensure_spill(WaveBufs& b, int grid, int block) {
     b.spill.reserve((size_t)(kMaxStack - kLdsStack + 1) * grid * block * 4);
 }
@@ -2856,6 +2954,18 @@ static uint32_t wave_samples(const st_render_params* p, uint32_t want) {
     return (uint64_t)npix * k >= (1ull << 31) ? 0 : k;
 }
 
+// The radiance cache of an st_render_cached call: the network the queries
+// read and the training rows the training paths write.
+struct CacheJob {
+    CacheGrid grid;
+    const _Float16* table;  // fp16 table and weights
+    const _Float16* w16;
+    float* tq;              // rows x 12
+    float4* tthr;           // rows
+    float4* tt;             // rows
+    uint32_t rows, depth, stride, frame;
+};
+
 // One camera-path render (TestRenderer::render's dispatches, testrenderer.slang:
 // 1087-1183) enqueued on given wave slots and accumulators.  st_render runs
 // it on the context's four slots; st_batch_render on one slot per request,
@@ -2876,7 +2986,9 @@ struct CameraJob {
     int tgrid, sgrid;                // trace and shade grids
     bool timed;                      // HIP-event timings of the kernel classes
     bool store_rays;                 // write the ray counts to o.ray_counts on the device
+    const CacheJob* cache;           // st_render_cached, else null
 };
+
 
 __global__ void k_store_rays(const uint32_t* counters, unsigned long long* out) {
     if (threadIdx.x < 2) out[threadIdx.x] = reinterpret_cast<const unsigned long long*>(counters + 4)[threadIdx.x];
@@ -2939,6 +3051,12 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
         for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(j.slot[sl]->stream, ctx->start_done, 0));
     }
     RenderConsts c = camera_consts(s, j.view, p);
+    const CacheJob* cj = j.cache;
+    if (cj) {
+        c.cache_depth = cj->depth;
+        c.train_stride = cj->stride;
+        c.train_frame = cj->frame;
+    }
     const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
     const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
     const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
@@ -2955,6 +3073,12 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
         WaveState w = wave_state(B, ctx);
         w.counters = j.counters[wi % slots];
         w.r_prev = j.rprev;
+        if (cj) {
+            w.cq = B.cq.as<float4>();
+            w.tq = cj->tq;
+            w.tthr = cj->tthr;
+            w.tt = cj->tt;
+        }
         uint32_t* spill = B.spill.as<uint32_t>();
         c.k = std::min(k, p->spp - base);
         c.sample_base = base;
@@ -2982,9 +3106,15 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
                 Timed tm(j.timed ? ctx : nullptr, 1, sm);
                 AuxOut aux{nullptr, nullptr, nullptr, nullptr};
                 if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
-                auto* kfn = shade_kernel(lambert, alias, restir && !reuse, media);
+                auto* kfn = cj ? shade_kernel_cache(lambert, alias, media) : shade_kernel(lambert, alias, restir && !reuse, media);
                 hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0,
                                    AdjArgs{});
+            }
+            if (cj && b + 1 == cj->depth) {  // the paths that ended in the cache take its radiance
+                Timed tm(j.timed ? ctx : nullptr, 5, sm);
+                const int igrid = std::min<int>((int)((c.wave_paths + 63) / 64), ctx->cus * 4);
+                hipLaunchKernelGGL(k_rc_infer_paths, dim3(igrid), dim3(256), 0, sm, cj->grid, cj->table, cj->w16, w.cq,
+                                   w.counters + 3, w.em);
             }
             if (reuse && b == 0) {
                 const int rgrid2 = std::min<int>((npix + kShadeBlock - 1) / kShadeBlock, ctx->cus * 8);
@@ -3010,6 +3140,15 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
             }
             hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
             qsel ^= 1;
+        }
+        if (cj) {  // training targets of the rows whose path is in this wave
+            const uint64_t g0 = (uint64_t)base * npix, g1 = g0 + (uint64_t)c.k * npix;
+            const uint32_t lo = (uint32_t)(g0 / cj->stride), hi = (uint32_t)std::min<uint64_t>(cj->rows, (g1 - 1) / cj->stride + 1);
+            if (hi > lo) {
+                Timed tm(j.timed ? ctx : nullptr, 5, sm);
+                hipLaunchKernelGGL(k_rc_targets, dim3(std::min<int>((int)((hi - lo + 255) / 256), ctx->cus * 4)), dim3(256), 0, sm, c, w,
+                                   lo, hi);
+            }
         }
         const bool last = base + c.k >= p->spp;
         const int rgrid = std::min<int>((npix + 255) / 256, ctx->cus * 8);
@@ -3247,6 +3386,7 @@ struct st_cache {
     DevBuf w32, w16, w16t, wm, wv, wgrad;  // 4 layers of 64 x 64
     DevBuf X0, H1, H2, H3, Y, dA, dB;      // rows x 64 fp16
     DevBuf loss;
+    DevBuf trq, trthr, trt;  // st_render_cached's training rows: queries, throughputs, targets
     uint32_t rows = 0;
     uint32_t step = 0;
     void reserve_rows(uint32_t n) {
@@ -3391,8 +3531,9 @@ int st_cache_infer(st_cache* c, const float* queries, uint32_t n, float* radianc
 }
 
 namespace {
-// One Adam step on n samples already on the device, on the context stream
-void cache_train_step(st_cache* c, const float* queries, const float* targets, uint32_t n) {
+// One Adam step on n samples already on the device, on the context stream;
+// targets: tstride (3, or 4 with a weight) floats per sample
+void cache_train_step(st_cache* c, const float* queries, const float* targets, uint32_t tstride, uint32_t n) {
     st_context* ctx = c->ctx;
     hipStream_t st = ctx->stream;
     const int cus = ctx->cus;
@@ -3402,8 +3543,8 @@ void cache_train_step(st_cache* c, const float* queries, const float* targets, u
                        c->table_floats, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int)kMlpParams, 256, 64)), dim3(256), 0, st, c->wgrad.as<uint32_t>(), kMlpParams, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, c->loss.as<uint32_t>(), 16u, 0u);
-    hipLaunchKernelGGL(k_rc_loss, dim3(grid_for((int)n, 256, cus * 8)), dim3(256), 0, st, c->Y.as<_Float16>(), targets, n,
-                       c->dA.as<_Float16>(), c->loss.as<float>());
+    hipLaunchKernelGGL(k_rc_loss, dim3(grid_for((int)n, 256, cus * 8)), dim3(256), 0, st, c->Y.as<_Float16>(), targets, tstride,
+                       n, c->dA.as<_Float16>(), c->loss.as<float>());
     const _Float16* wt = c->w16t.as<_Float16>();
     float* wg = c->wgrad.as<float>();
     const uint32_t chunk = 1024;
@@ -3444,11 +3585,11 @@ int st_cache_train(st_cache* c, const float* queries, const float* targets, uint
     try {
         HIP_CHECK(hipSetDevice(c->ctx->device));
         order_after_caller(c->ctx);
-        cache_train_step(c, queries, targets, n);
-        float l = 0;
-        HIP_CHECK(hipMemcpyAsync(&l, c->loss.p, 4, hipMemcpyDeviceToHost, c->ctx->stream));
+        cache_train_step(c, queries, targets, 3, n);
+        float l[2] = {0, 0};
+        HIP_CHECK(hipMemcpyAsync(l, c->loss.p, 8, hipMemcpyDeviceToHost, c->ctx->stream));
         HIP_CHECK(hipStreamSynchronize(c->ctx->stream));
-        if (loss) *loss = l / (float)n;
+        if (loss) *loss = l[0] / (float)n;
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
@@ -3490,6 +3631,116 @@ int st_cache_set_params(st_cache* c, const float* table, const float* weights) {
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
     }
+}
+
+int st_render_cached(st_scene* s, st_cache* cache, const st_view* view, const st_render_params* p,
+                     const st_cache_render_params* cp, const st_outputs* o) {
+    if (!s || !cache || !view || !p || !cp || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
+    if (cache->ctx != s->ctx) return set_error(ST_ERROR_INVALID_ARGUMENT, "the cache belongs to another context");
+    if (p->flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC))
+        return set_error(ST_ERROR_UNSUPPORTED, "the radiance cache ends camera paths of the path tracer");
+    if ((p->flags & ST_FLAG_RESTIR_DI) && !(p->flags & ST_FLAG_RESTIR_DI_REUSE))
+        return set_error(ST_ERROR_UNSUPPORTED, "with the radiance cache, ReSTIR DI runs at primary vertices (ST_FLAG_RESTIR_DI_REUSE)");
+    if (cp->cache_depth < 1 || cp->cache_depth >= p->max_depth)
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "cache_depth must be in [1, max_depth)");
+    if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && cp->cache_depth < 2)
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "with screen-space reuse the primary vertex keeps its NEE: cache_depth >= 2");
+    if (cp->train_stride < 1) return set_error(ST_ERROR_INVALID_ARGUMENT, "train_stride must be at least 1");
+    if (cp->train_steps > 1024) return set_error(ST_ERROR_INVALID_ARGUMENT, "at most 1024 training steps per call");
+    const int rc = check_camera(s, p);
+    if (rc != ST_OK) return rc;
+    const uint64_t npix = (uint64_t)p->width * p->height, total = npix * p->spp;
+    if (total >= (1ull << 32)) return set_error(ST_ERROR_LIMIT, "at most 2^32 - 1 paths per call");
+    const uint64_t rows = (total + cp->train_stride - 1) / cp->train_stride;
+    if (rows > kCacheMaxRows) return set_error(ST_ERROR_LIMIT, "at most 2^24 training rows per call (raise train_stride)");
+    st_context* ctx = s->ctx;
+    try {
+        HIP_CHECK(hipSetDevice(ctx->device));
+        const bool reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
+        const uint32_t k = wave_samples(p, p->samples_per_wave);
+        if (k == 0) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
+        const uint64_t paths = npix * k;
+        const uint32_t waves = (p->spp + k - 1) / k;
+        const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) ? 1 : (int)std::min<uint32_t>(kWaveSlots, waves);
+        const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
+        order_after_caller(ctx);
+        for (int sl = 0; sl < slots; sl++) {
+            reserve_wave(ctx->slot[sl], paths, reuse, paths, tgrid);
+            ctx->slot[sl].cq.reserve(paths * 64);
+        }
+        ctx->acc.reserve((size_t)npix * 16);
+        if (reuse) ctx->rprev.reserve((size_t)npix * 32);
+        ctx->stats.reserve(64);
+        cache->trq.reserve(rows * 48);
+        cache->trthr.reserve(rows * 16);
+        cache->trt.reserve(rows * 16);
+        cache->reserve_rows((uint32_t)rows);
+        begin_timings(ctx);
+        hipStream_t s0 = ctx->stream;
+        // rows start empty: zero queries (finite network inputs) and weights
+        const int fg = ctx->cus * 8;
+        hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, s0, cache->trq.as<uint32_t>(), (uint32_t)(rows * 12), 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, s0, cache->trt.as<uint32_t>(), (uint32_t)(rows * 4), 0u);
+        CacheJob cj;
+        cj.grid = cache->grid;
+        cj.table = cache->table16.as<_Float16>();
+        cj.w16 = cache->w16.as<_Float16>();
+        cj.tq = cache->trq.as<float>();
+        cj.tthr = cache->trthr.as<float4>();
+        cj.tt = cache->trt.as<float4>();
+        cj.rows = (uint32_t)rows;
+        cj.depth = cp->cache_depth;
+        cj.stride = cp->train_stride;
+        cj.frame = cp->frame;
+        CameraJob j;
+        std::memset(&j, 0, sizeof j);
+        j.s = s;
+        j.view = *view;
+        j.p = *p;
+        j.o = *o;
+        for (int sl = 0; sl < slots; sl++) {
+            j.slot[sl] = &ctx->slot[sl];
+            j.counters[sl] = ctx->slot[sl].counters.as<uint32_t>();
+        }
+        j.slots = slots;
+        j.k = k;
+        j.acc = ctx->acc.as<float4>();
+        j.rprev = ctx->rprev.as<float4>();
+        j.stats = ctx->stats.as<unsigned long long>();
+        j.tgrid = tgrid;
+        j.sgrid = sgrid;
+        j.timed = true;
+        j.cache = &cj;
+        enqueue_camera(ctx, j);
+        // every wave's targets are in before the training steps
+        for (int sl = 1; sl < slots; sl++) HIP_CHECK(hipStreamWaitEvent(s0, ctx->slot[sl].resolve_done, 0));
+        if (cp->train_queries)
+            HIP_CHECK(hipMemcpyAsync(cp->train_queries, cache->trq.p, rows * 48, hipMemcpyDeviceToDevice, s0));
+        if (cp->train_targets)
+            HIP_CHECK(hipMemcpyAsync(cp->train_targets, cache->trt.p, rows * 16, hipMemcpyDeviceToDevice, s0));
+        float l[2] = {0, 0};
+        for (uint32_t st = 0; st < cp->train_steps; st++) {
+            Timed tm(ctx, 6, s0);
+            cache_train_step(cache, cache->trq.as<float>(), cache->trt.as<float>(), 4, (uint32_t)rows);
+            if (st == 0 && cp->loss) HIP_CHECK(hipMemcpyAsync(l, cache->loss.p, 8, hipMemcpyDeviceToHost, s0));
+        }
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
+        if (cp->loss) *cp->loss = l[1] > 0 ? l[0] / l[1] : 0.0f;
+        finish_timings(ctx, slots, o->ray_counts);
+        return ST_OK;
+    } catch (const std::exception& e) {
+        return set_error(ST_ERROR_HIP, e.what());
+    }
+}
+
+int st_cache_last_times(st_context* ctx, double out[4]) {
+    if (!ctx || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    out[0] = ctx->kernel_ms[5];
+    out[1] = (double)ctx->ev_used[5];
+    out[2] = ctx->kernel_ms[6];
+    out[3] = (double)ctx->ev_used[6];
+    return ST_OK;
 }
 
 // ------------------------------------------------------------ batches
@@ -4032,9 +4283,9 @@ int st_last_timings(st_context* ctx, double out[12]) {
 
 int st_last_kernel_times(st_context* ctx, double out[10]) {
     if (!ctx || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
-    for (int i = 0; i < st_context::kKinds; i++) {
+    for (int i = 0; i < st_context::kRenderKinds; i++) {
         out[i] = ctx->kernel_ms[i];
-        out[st_context::kKinds + i] = (double)ctx->ev_used[i];
+        out[st_context::kRenderKinds + i] = (double)ctx->ev_used[i];
     }
     return ST_OK;
 }

@@ -167,6 +167,14 @@ class TestRenderer:
         self.scene = scene
 
     def render(self, view, params, want_path_length=False, out=None, want_aux=False):
+        res, o = self.outputs(params, want_path_length, out, want_aux)
+        _enter(self.scene.ctx.device)
+        abi.check(self.scene.lib, self.scene.lib.st_render(self.scene.h, C.byref(view), C.byref(params), C.byref(o)),
+                  "st_render")
+        return res
+
+    def outputs(self, params, want_path_length=False, out=None, want_aux=False):
+        """Output tensors for a render with params (or the caller's, checked) and their st_outputs."""
         torch = _torch()
         dev = torch.device("cuda", self.scene.ctx.device)
         W, H, S = params.width, params.height, params.spp
@@ -195,10 +203,7 @@ class TestRenderer:
             return res[k].data_ptr() if res.get(k) is not None else None
         o = abi.st_outputs(ptr("hdr"), ptr("visibility"), ptr("path_length"), ptr("ray_counts"), ptr("albedo"), ptr("depth"),
                            ptr("prev_uv"))
-        _enter(self.scene.ctx.device)
-        abi.check(self.scene.lib, self.scene.lib.st_render(self.scene.h, C.byref(view), C.byref(params), C.byref(o)),
-                  "st_render")
-        return res
+        return res, o
 
 
 class Batch:

@@ -210,6 +210,33 @@ class SceneData:
     def triangle_count(self):
         return sum(b >> 16 for a, b in self.instances if (a & 0xF) == 0)
 
+    def bounds(self):
+        """World-space box (lo, hi) holding every mesh and volume instance: each
+        instance's object-space box (its position buffer from the mesh's
+        offset on; a volume's bounds) with its 8 corners transformed."""
+        lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
+        boxes = {}
+        vol_box = {int(v.instance_index): (np.array(v.min[:]), np.array(v.max[:])) for v in self.volume_info}
+        for i, ((a, b), xf) in enumerate(zip(self.instances, self.transforms)):
+            kind = a & 0xF
+            if kind == 0:
+                vi = self.vertex_infos[b & 0xFFFF]
+                key = (vi.packed_buffer_indices[0] >> 16, vi.packed_offsets[1])
+                if key not in boxes:
+                    pts = np.frombuffer(self.buffers[key[0]], np.float32)[key[1] // 4:]
+                    pts = pts[:len(pts) // 3 * 3].reshape(-1, 3)
+                    boxes[key] = (pts.min(0), pts.max(0))
+                bmin, bmax = boxes[key]
+            elif kind == 2 and i in vol_box:
+                bmin, bmax = vol_box[i]
+            else:
+                continue
+            c = np.array([[bmax[0] if k & 1 else bmin[0], bmax[1] if k & 2 else bmin[1], bmax[2] if k & 4 else bmin[2]]
+                          for k in range(8)], np.float64)
+            w = c @ np.asarray(xf, np.float64)[:, :3].T + np.asarray(xf, np.float64)[:, 3]
+            lo, hi = np.minimum(lo, w.min(0)), np.maximum(hi, w.max(0))
+        return tuple(lo.astype(np.float32)), tuple(hi.astype(np.float32))
+
 
 # ---------------------------------------------------------------- camera
 def quat_to_transform(pos, q):

@@ -1,0 +1,160 @@
+"""st_render_cached on the GPU against the oracle's restatement
+(or_render_cached) and against st_render.
+
+- With a network that answers 0 (all weights 0) the image must equal the
+  oracle's cached render at the per-pixel bar (every channel within 1e-4
+  relative, path lengths equal), and the training rows (queries, targets,
+  weights) must equal the oracle's: on the Cornell box, on the headline
+  frame's modes (alias table, RIS with screen-space reuse, roulette from 3)
+  and in a medium.
+- With one training path per path (stride 1) nothing ends in the cache: the
+  image is st_render's bit for bit.
+- With a nonzero network each path that ends in the cache gets throughput x
+  the network's radiance at its query: the image is compared with the
+  oracle's records composed with the numpy network (fp16 rounding points as
+  the GPU's; accumulation order differs, so one fp16 unit of slack).
+- The training step of a call uses the rows it wrote: its loss equals the
+  loss st_cache_train reports on the same rows.
+"""
+import numpy as np
+import pytest
+
+from oracle.cache_oracle import CacheOracle
+from oracle.oracle import OracleScene, cached_image, training_rows
+from stratum2_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(levels=8, log2_table=14, base_resolution=4, max_resolution=64)
+
+
+def _case(name):
+    if name == "cornell":
+        sd = scenes.cornell_box(0)
+        W = H = 64
+        view = scenes.cornell_view(W, H)
+        p = abi.default_params(W, H, max_depth=4, spp=4)
+        return sd, view, p, 2, 4
+    if name == "headline":
+        sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+        W, H = 96, 64
+        view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+        flags = (abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE)
+        p = abi.default_params(W, H, max_depth=8, spp=3, flags=flags, rr_start_depth=3, candidate_samples=32, max_m=3.0,
+                               spatial_neighbours=5, spatial_radius=30.0)
+        return sd, view, p, 3, 3
+    if name == "media":
+        sd, _ = scenes.cornell_cloud(0, n=16)
+        W = H = 48
+        view = scenes.cornell_view(W, H)
+        p = abi.default_params(W, H, max_depth=6, spp=2, max_null_collisions=1000)
+        return sd, view, p, 2, 2
+    raise KeyError(name)
+
+
+def _cache(ctx, sd, seed=1):
+    from stratum2_amd.cache import RadianceCache
+    return RadianceCache(ctx, *sd.bounds(), seed=seed, **CFG)
+
+
+def _within(got, want, rel=1e-4, floor=1e-3):
+    err = np.abs(got - want) / np.maximum(np.abs(want), floor)
+    assert err.max() <= rel, (float(err.max()), np.unravel_index(int(err.argmax()), err.shape))
+
+
+@pytest.mark.parametrize("name", ["cornell", "headline", "media"])
+def test_zero_cache_equals_oracle(gpu_ctx, name):
+    from stratum2_amd.renderer import Scene
+    sd, view, p, depth, stride = _case(name)
+    print(f"case {name}: {p.width}x{p.height} {p.spp}spp L{p.max_depth} depth {depth} stride {stride}", flush=True)
+    cache = _cache(gpu_ctx, sd)
+    table, W = cache.params()
+    cache.set_params(weights=np.zeros_like(W))
+    out = cache.render(Scene(gpu_ctx, sd), view, p, cache_depth=depth, train_stride=stride, train_steps=0, frame=5,
+                       want_train=True, want_path_length=True)
+    res, recs = OracleScene(sd).render_cached(view, p, depth, stride, frame=5)
+    assert (recs[..., 0] == 1).sum() > 100 and (recs[..., 0] == 2).sum() > 20
+    _within(out["hdr"].cpu().numpy()[..., :3], res["hdr"][..., :3])
+    assert np.array_equal(out["path_length"].cpu().numpy().view(np.uint32), res["path_length"])
+    q, t = training_rows(recs, stride)
+    gq, gt = out["train_queries"].cpu().numpy(), out["train_targets"].cpu().numpy()
+    assert np.array_equal(gt[:, 3], t[:, 3])
+    assert np.array_equal(gq, q)
+    _within(gt[:, :3], t[:, :3], floor=1e-2)
+    cache.close()
+
+
+def test_stride_one_equals_plain_render(gpu_ctx):
+    from stratum2_amd.renderer import Scene, TestRenderer
+    sd, view, p, depth, _ = _case("headline")
+    sc = Scene(gpu_ctx, sd)
+    cache = _cache(gpu_ctx, sd)
+    got = cache.render(sc, view, p, cache_depth=depth, train_stride=1, train_steps=0, want_path_length=True)
+    want = TestRenderer(sc).render(view, p, want_path_length=True)
+    assert np.array_equal(got["hdr"].cpu().numpy(), want["hdr"].cpu().numpy())
+    assert np.array_equal(got["path_length"].cpu().numpy(), want["path_length"].cpu().numpy())
+    cache.close()
+
+
+@pytest.mark.parametrize("name", ["cornell", "media"])
+def test_network_radiance_matches_oracle(gpu_ctx, name):
+    from stratum2_amd.renderer import Scene
+    sd, view, p, depth, stride = _case(name)
+    cache = _cache(gpu_ctx, sd, seed=4)
+    rng = np.random.default_rng(11)
+    table, W = cache.params()
+    table = rng.normal(scale=0.5, size=table.shape).astype(np.float32)
+    W = (W * 0.7).astype(np.float32)
+    W[3, :3] = np.abs(W[3, :3])  # mostly positive outputs
+    cache.set_params(table=table, weights=W)
+    out = cache.render(Scene(gpu_ctx, sd), view, p, cache_depth=depth, train_stride=stride, train_steps=0, frame=2)
+    _, recs = OracleScene(sd).render_cached(view, p, depth, stride, frame=2)
+    o = CacheOracle(CFG["levels"], CFG["log2_table"], CFG["base_resolution"], CFG["max_resolution"], *sd.bounds())
+    qs = recs[recs[..., 0] == 1][:, 1:13]
+    rad = np.maximum(o.forward(W, o.encode(table, qs))[-1][:, :3], 0).astype(np.float32)
+    assert (rad > 0).mean() > 0.3
+    want = cached_image(recs, rad)[..., :3]
+    got = out["hdr"].cpu().numpy()[..., :3]
+    # one fp16 unit of the network's output (2^-10 relative) per pixel, plus a floor
+    err = np.abs(got - want) / np.maximum(np.abs(want), 1e-2)
+    assert (err <= 2e-3).mean() >= 0.999, float(np.quantile(err, 0.999))
+    assert np.abs(got - want).mean() <= 1e-4 * np.abs(want).mean() + 1e-6
+    cache.close()
+
+
+def test_training_step_uses_the_rows(gpu_ctx):
+    """The loss of a call's first step equals st_cache_train's loss on the
+    weighted rows the call wrote (the rows with weight 1)."""
+    import torch
+    from stratum2_amd.renderer import Scene
+    sd, view, p, depth, stride = _case("cornell")
+    sc = Scene(gpu_ctx, sd)
+    a = _cache(gpu_ctx, sd, seed=9)
+    b = _cache(gpu_ctx, sd, seed=9)
+    out = a.render(sc, view, p, cache_depth=depth, train_stride=stride, train_steps=1, frame=3, want_train=True)
+    tq, tt = out["train_queries"], out["train_targets"]
+    keep = tt[:, 3] > 0
+    lb = b.train(tq[keep].contiguous(), tt[keep][:, :3].contiguous())
+    assert abs(out["loss"] - lb) <= 1e-3 * abs(lb), (out["loss"], lb)
+    # both took one step from the same start on the same samples: same weights
+    wa, wb = a.params()[1], b.params()[1]
+    assert np.abs(wa - wb).max() <= 1e-5 + 1e-3 * np.abs(wb).max()
+    a.close()
+    b.close()
+    del torch
+
+
+def test_argument_checks(gpu_ctx):
+    from stratum2_amd.renderer import Scene
+    sd, view, p, depth, stride = _case("cornell")
+    sc = Scene(gpu_ctx, sd)
+    cache = _cache(gpu_ctx, sd)
+    for kw in (dict(cache_depth=p.max_depth), dict(cache_depth=0), dict(train_stride=0)):
+        args = dict(cache_depth=depth, train_stride=stride, train_steps=0)
+        args.update(kw)
+        with pytest.raises(RuntimeError):
+            cache.render(sc, view, p, **args)
+    p2 = abi.default_params(p.width, p.height, max_depth=4, spp=1, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI)
+    with pytest.raises(RuntimeError):
+        cache.render(sc, view, p2, cache_depth=2)
+    cache.close()
