@@ -36,6 +36,12 @@ gradient of an image loss over 16 cameras at 512x512, 8 spp, maxPathLength 8
 (RR from 3), with respect to the per-triangle albedo (5144 triangles) and a
 64^3 density grid of the Cornell box with a cloud; reported in ms per
 gradient and as a multiple of the 16 forward renders.
+--workload cache measures time-to-RMSE of the learned radiance cache
+(st_render_cached): paths end in the cache at their second vertex, 8-spp
+frames each train one Adam step on 2^16 rows; reported as the time the cached
+form needs to reach the RMSE the plain tracer has after 128 spp, over the
+plain tracer's time (a 1024-spp plain render is the reference), on the volume
+scene (the value) and the 1M-triangle headline scene.
 --copy-calibration runs a plain device copy of known size (for scaling the
 PMC byte counters) and exits.
 """
@@ -63,7 +69,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-wave", type=int, default=0, help="0: library default")
     ap.add_argument("--no-restir", action="store_true", help="plain NEE instead of ReSTIR DI")
-    ap.add_argument("--workload", choices=["spheres", "cloud", "svgf", "batch", "adjoint"], default="spheres")
+    ap.add_argument("--workload", choices=["spheres", "cloud", "svgf", "batch", "adjoint", "cache"], default="spheres")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="batch workload: the library's default launch shapes")
     ap.add_argument("--device-bvh", action="store_true", help="rebuild the BVH on the GPU (st_scene_update) before timing")
@@ -94,6 +100,9 @@ def main():
         return
     if args.workload == "batch":
         bench_batch(args, log)
+        return
+    if args.workload == "cache":
+        bench_cache(args, log)
         return
     if args.workload == "adjoint":
         bench_adjoint(args, log)
@@ -570,6 +579,183 @@ def bench_adjoint(args, log):
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def cache_scene(name):
+    """(scene, view, params(spp, seed)) of the time-to-RMSE comparison: the
+    volume scene at 1024x1024 (maxPathLength 16, roulette everywhere) or the
+    1M-triangle scene at 1920x1080 with the headline frame's flags."""
+    from stratum2_amd import abi, scenes
+    if name == "cloud":
+        sd = scenes.cloud(seed=0)
+        W = H = 1024
+        flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
+
+        def mk(spp, seed, extra=0):
+            return abi.default_params(W, H, max_depth=16, spp=spp, seed=seed, flags=flags | extra, rr_start_depth=0)
+        return sd, scenes.cloud_view(W, H), mk
+    sd = scenes.spheres_field(seed=1)
+    W, H = 1920, 1080
+    flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
+
+    def mk(spp, seed, extra=0):
+        return abi.default_params(W, H, max_depth=8, spp=spp, seed=seed, flags=flags | extra, rr_start_depth=3,
+                                  candidate_samples=32, max_m=3.0, spatial_neighbours=5, spatial_radius=30.0)
+    return sd, scenes.spheres_view(W, H), mk
+
+
+# Adam's learning rate for the online-trained cache: 1e-2 left the frozen
+# cache's image mean 34% high on the cloud after 32 frames, 1e-3 40% low (not
+# yet converged), 3e-3 within 2% on both scenes; a moving average of the
+# parameters lagged behind over so few frames
+CACHE_LR = 3e-3
+
+
+def cache_run(name, depth, frame_spp, total_spp, log):
+    """Plain and cached progressive renders of one scene against a 1024-spp
+    plain reference (seeds 10^6 on).  Plain: st_render calls of 16 spp;
+    cached: st_render_cached frames of frame_spp spp, each training one Adam
+    step on its 2^16 rows.  Only the render calls are timed (synchronised
+    before and after); errors are computed outside the timed calls."""
+    import torch
+    from stratum2_amd.cache import RadianceCache
+    from stratum2_amd.renderer import Context, Scene, TestRenderer
+    sd, view, mk = cache_scene(name)
+    ctx = Context(0)
+    sc = Scene(ctx, sd)
+    r = TestRenderer(sc)
+    p0 = mk(1, 0)
+    W, H = p0.width, p0.height
+    ref = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+    for i in range(16):
+        ref += r.render(view, mk(64, 1_000_000 + 64 * i))["hdr"][..., :3].double()
+    ref /= 16
+
+    def err(acc, n):
+        d = acc / n - ref
+        return d.pow(2).mean().sqrt().item(), (d.pow(2) / (ref.pow(2) + 1e-2)).mean().item()
+
+    P = 16
+    r.render(view, mk(P, 0))
+    acc = torch.zeros_like(ref)
+    el, plain = 0.0, []
+    for c in range(total_spp // P):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        o = r.render(view, mk(P, P * c))
+        torch.cuda.synchronize()
+        el += time.perf_counter() - a
+        acc += o["hdr"][..., :3].double() * P
+        plain.append((P * (c + 1), el) + err(acc, P * (c + 1)))
+    lo, hi = sd.bounds()
+    # one training path per `stride` paths, set from an untimed frame so that
+    # about 2^16 of them reach the cache vertex (rows with weight) per frame
+    stride = max(1, W * H * frame_spp // 65536)
+    warm = RadianceCache(ctx, lo, hi, learning_rate=CACHE_LR)
+    warm.render(sc, view, mk(frame_spp, 0), cache_depth=depth, train_stride=stride, train_steps=1)
+    frac = warm.last_times()[5] / -(-W * H * frame_spp // stride)
+    stride = max(1, int(W * H * frame_spp * frac // 65536))
+    warm.render(sc, view, mk(frame_spp, 0), cache_depth=depth, train_stride=stride, train_steps=1)
+    warm.close()
+    cache = RadianceCache(ctx, lo, hi, learning_rate=CACHE_LR)
+    acc = torch.zeros_like(ref)
+    el, cached, times, losses = 0.0, [], [], []
+    for f in range(total_spp // frame_spp):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        o = cache.render(sc, view, mk(frame_spp, frame_spp * f), cache_depth=depth, train_stride=stride, train_steps=1, frame=f)
+        torch.cuda.synchronize()
+        el += time.perf_counter() - a
+        times.append(cache.last_times())
+        losses.append(o["loss"])
+        acc += o["hdr"][..., :3].double() * frame_spp
+        cached.append((frame_spp * (f + 1), el) + err(acc, frame_spp * (f + 1)))
+    # bias: the trained cache, frozen, over 256 more spp of other seeds
+    acc = torch.zeros_like(ref)
+    for f in range(256 // frame_spp):
+        o = cache.render(sc, view, mk(frame_spp, 500_000 + frame_spp * f), cache_depth=depth, train_stride=stride, train_steps=0,
+                         frame=10_000 + f)
+        acc += o["hdr"][..., :3].double() * frame_spp
+    frozen_rmse = err(acc, 256)[0]
+    bias = ((acc / 256).mean() - ref.mean()).item() / ref.mean().item()
+    cache.close()
+    target_spp = 128
+    tp, ep = [(x[1], x[2]) for x in plain if x[0] == target_spp][0]
+    hit = [x for x in cached if x[2] <= ep]
+    t_c = hit[0][1] if hit else None
+    inf_ms = sum(t[0] for t in times) / len(times)
+    queries = sum(t[4] for t in times) / len(times)
+    train_ms = sum(t[2] for t in times) / len(times)
+    rows = sum(t[5] for t in times) / len(times)
+    res = {"scene": name, "width": W, "height": H, "cache_depth": depth, "frame_spp": frame_spp, "train_stride": stride,
+           "training_rows_per_frame": -(-W * H * frame_spp // stride), "training_rows_with_weight": round(rows),
+           "plain_ms_per_spp": round(plain[-1][1] / plain[-1][0] * 1e3, 3),
+           "cached_ms_per_spp": round(cached[-1][1] / cached[-1][0] * 1e3, 3),
+           "target": f"plain RMSE at {target_spp} spp", "target_rmse": round(ep, 4), "plain_ms": round(tp * 1e3, 1),
+           "cached_ms": round(t_c * 1e3, 1) if t_c else None, "cached_spp_at_target": hit[0][0] if hit else None,
+           "time_ratio": round(t_c / tp, 3) if t_c else None,
+           "rmse_plain": {x[0]: round(x[2], 4) for x in plain if x[0] in (16, 32, 64, 128, 256)},
+           "rmse_cached": {x[0]: round(x[2], 4) for x in cached if x[0] in (16, 32, 64, 128, 256)},
+           "relmse_plain_256": round(plain[-1][3], 4), "relmse_cached_256": round(cached[-1][3], 4),
+           "frozen_cache_rmse_256spp": round(frozen_rmse, 4), "frozen_cache_relative_bias_of_mean": round(bias, 5),
+           "inference_ms_per_frame": round(inf_ms, 3), "queries_per_frame": round(queries), "training_ms_per_frame": round(train_ms, 3),
+           "loss_first_last": [round(losses[0], 3), round(losses[-1], 3)]}
+    log(f"cache {name}: {json.dumps(res)}")
+    return res
+
+
+def bench_cache(args, log):
+    """Time-to-RMSE of the learned radiance cache: the volume scene (the
+    value) and the 1M-triangle headline scene, paths ending in the cache at
+    their second vertex (cache_depth 2), 8-spp frames with 2^16 training rows
+    each; the time the cached form needs to reach the RMSE the plain tracer
+    has after 128 spp, over the plain tracer's time to get there.  Roofline:
+    the fused inference kernel, whose algorithmic bytes per query are the
+    64-byte query, 16 levels x 8 corners x 4 bytes of table and the path's
+    16-byte radiance read and written."""
+    runs = [cache_run("cloud", 2, 8, 256, log), cache_run("spheres", 2, 8, 256, log)]
+    c = runs[0]
+    qbytes = 64 + 16 * 8 * 4 + 32
+    inf_s = c["inference_ms_per_frame"] / 1e3
+    achieved = c["queries_per_frame"] * qbytes / inf_s / 1e9
+    roofline = {"kernel": "k_rc_infer_paths", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_query": qbytes, "queries_per_frame": c["queries_per_frame"],
+                "timing": "HIP events around the inference (and training-target) launches of each cached frame",
+                "mfma_tflops": round(c["queries_per_frame"] * 4 * 2 * 64 * 64 / inf_s / 1e12, 2)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cache_cpu_baseline(args.cpu_seconds, log)
+    line = {
+        "metric": "time-to-RMSE cached / plain", "value": c["time_ratio"], "unit": "ratio", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": c["cached_ms"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 render, fp16 cache (fp32 accumulation)", "data": "synthetic (seeded scene generators)",
+        "config": {"workload": "time for st_render_cached (paths end in the hash-grid + MLP radiance cache at their second "
+                               "vertex; 8-spp frames, one Adam step (lr 3e-3) on about 2^16 training rows per frame) to "
+                               "reach the RMSE "
+                               "st_render has after 128 spp (16-spp calls), both against a 1024-spp st_render reference: "
+                               "1024^2 256^3 fBm cloud (maxPathLength 16) as the value, 1920x1080 1M-triangle ReSTIR scene "
+                               "beside it", "runs": runs},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cache_cpu_baseline(seconds, log):
+    """The oracle's or_render_cached (no network: the cached estimator's path
+    work) on the volume scene at 1024x1024, 1 spp, cache depth 2, OpenMP over
+    this host's share of cores; reported in ms per spp."""
+    from oracle.oracle import OracleScene
+    sd, view, mk = cache_scene("cloud")
+    o = OracleScene(sd)
+    threads = min(16, os.cpu_count() or 1)
+    a = time.perf_counter()
+    o.render_cached(view, mk(1, 0), 2, 128, 0, threads=threads)
+    dt = time.perf_counter() - a
+    log(f"cpu baseline: or_render_cached 1024x1024 x 1 spp in {dt:.2f}s on {threads} threads")
+    return {"value": round(dt * 1e3, 1), "unit": "ms per spp", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle or_render_cached (cache answering 0), 1024x1024 cloud, 1 spp, cache depth 2 ({dt:.1f} s)"}
 
 
 def adjoint_cpu_baseline(sd, view, alb, dh, seconds, depth, flags, log):

@@ -280,9 +280,10 @@ int st_adjoint(st_scene* scene, const st_view* view, const st_render_params* par
  * al., "Real-time Neural Radiance Caching for Path Tracing", 2021, with the
  * multiresolution hash encoding of "Instant Neural Graphics Primitives",
  * 2022).  An fp16 hash-grid table (levels x 2^log2_table entries x 2
- * features; cells hashed as HashGrid::FindCellIndex does, hashgrid.hlsli:
- * 56-58) and a 64-64-64-64-3 ReLU MLP on fp16 MFMA with fp32 accumulation,
- * trained with Adam on a relative L2 loss.
+ * features; a level whose (res + 2)^3 cell corners fit indexes them
+ * directly, finer levels by that paper's spatial hash) and a 64-64-64-64-3
+ * ReLU MLP on fp16 MFMA with fp32 accumulation, trained with Adam on a
+ * relative L2 loss.
  * A query is 12 floats: world position, outgoing direction, shading normal,
  * albedo; positions are normalised by the box given at creation. */
 typedef struct st_cache st_cache;
@@ -294,11 +295,15 @@ typedef struct st_cache_config {
     float learning_rate;      /* Adam (1e-2) */
     float beta1, beta2, epsilon; /* (0.9, 0.99, 1e-15) */
     uint32_t seed;            /* initial parameters */
+    float ema_decay;          /* inference reads the parameters' exponential moving average
+                                 ema = ema_decay ema + (1 - ema_decay) p, updated at each step
+                                 (0: inference reads the parameters themselves) */
 } st_cache_config;
 int st_cache_create(st_context* ctx, const st_cache_config* config, const float bbox_min[3], const float bbox_max[3],
                     st_cache** cache);
 int st_cache_destroy(st_cache* cache);
-/* queries: n x 12 floats, radiance: n x 3 floats (device) */
+/* queries: n x 12 floats, radiance: n x 3 floats (device); with the
+ * parameters inference reads (the moving average when ema_decay > 0) */
 int st_cache_infer(st_cache* cache, const float* queries, uint32_t n, float* radiance);
 /* One Adam step on n samples (queries, n x 3 target radiances; device);
  * loss (host, may be null) receives the mean relative L2 loss before the step. */
@@ -313,7 +318,7 @@ int st_cache_set_params(st_cache* cache, const float* table, const float* weight
 
 /* Camera paths that end in the cache (no reference counterpart: st_render's
  * estimator with its paths cut short, Mueller et al. 2021).  A path reaching
- * a scattering vertex at path length cache_depth (3: after two bounces), with
+ * a scattering vertex at path length cache_depth (2: bounce k is vertex k), with
  * cache_depth < max_depth, adds that vertex's emission as st_render does and
  * then, instead of NEE and a continuation, throughput x the cache's radiance
  * at the vertex (query: position, direction towards the previous vertex,
@@ -325,16 +330,17 @@ int st_cache_set_params(st_cache* cache, const float* table, const float* weight
  * and target = (the path's final radiance - the radiance it had gathered up
  * to and including that vertex's emission) / its throughput there.  Rows of
  * training paths that end before the cache depth, or at a black vertex, have
- * weight 0.  After the render, train_steps Adam steps on the rows (all of
- * them each step; train_steps 0 leaves the cache unchanged).  Inference in
- * this call uses the cache as it was on entry.
+ * weight 0.  After the render, train_steps Adam steps on the rows with
+ * weight (all of them each step, moved to the front in row order;
+ * train_steps 0 leaves the cache unchanged).  Inference in this call uses the
+ * cache as it was on entry.
  * train_queries (rows x 12 floats) and train_targets (rows x 4: target rgb,
  * weight), rows = ceil(spp * W * H / train_stride), are optional device
  * outputs; loss (host, may be null) receives the mean loss of the first step.
  * ReSTIR DI is allowed only with ST_FLAG_RESTIR_DI_REUSE (RIS at primary
  * vertices); light tracing and the bidirectional mode are not. */
 typedef struct st_cache_render_params {
-    uint32_t cache_depth;     /* path length of the vertex that queries the cache (3) */
+    uint32_t cache_depth;     /* path length of the vertex that queries the cache (2) */
     uint32_t train_stride;    /* one training path per this many paths (>= 1) */
     uint32_t train_steps;     /* Adam steps after the render (1) */
     uint32_t frame;           /* selects this call's training paths */
@@ -344,11 +350,13 @@ typedef struct st_cache_render_params {
 } st_cache_render_params;
 int st_render_cached(st_scene* scene, st_cache* cache, const st_view* view, const st_render_params* params,
                      const st_cache_render_params* cache_params, const st_outputs* outputs);
-/* Device time of the last st_render_cached's cache work (HIP events):
- * out[0] ms of inference + target kernels, out[1] their launches, out[2] ms
- * of training steps, out[3] steps.  The render's own kernel classes are in
- * st_last_kernel_times. */
-int st_cache_last_times(st_context* ctx, double out[4]);
+/* The last st_render_cached's cache work: out[0] ms of inference, target and
+ * row-compaction kernels (HIP events), out[1] their launches, out[2] ms of
+ * training steps,
+ * out[3] steps, out[4] paths that ended in a cache query, out[5] training
+ * rows with weight 1 (0 unless the call asked for its loss).  The render's
+ * own kernel classes are in st_last_kernel_times. */
+int st_cache_last_times(st_context* ctx, double out[6]);
 
 /* Primary-ray closest hit only (the minimum slice): visibility for each pixel
  * and, when t_out is not null, the hit distance as the path state stores it

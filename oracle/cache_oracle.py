@@ -8,9 +8,11 @@ layers) and accumulates in float64, so it differs from the GPU only by the
 order of fp32 accumulation.  With half=False everything is float64, which the
 tests use to pin the backward pass against central differences.
 
-The hash of a level's cell corner is pcg(pcg(level) + pcg(z + pcg(y +
-pcg(x)))) with the reference's pcg (rng.hlsli:17-21), as
-HashGrid::FindCellIndex chains it (hashgrid.hlsli:56-58)."""
+A level whose (res + 2)^3 corner points fit its table indexes corner
+(x, y, z) directly as x + n (y + n z), n = res + 2; a finer level hashes it
+with the spatial hash of Mueller et al., "Instant Neural Graphics
+Primitives", 2022: (x xor 2654435761 y xor 805459861 z) mod table size, in
+uint32 arithmetic."""
 import math
 
 import numpy as np
@@ -26,9 +28,10 @@ def pcg(v):
     return ((w >> np.uint64(22)) ^ w) & M32
 
 
-def cell_hash(x, y, z, level):
-    u = lambda a: np.asarray(a, np.int64).astype(np.uint64) & M32  # noqa: E731  (two's complement of negatives)
-    return pcg(pcg(np.uint64(level)) + pcg(u(z) + pcg(u(y) + pcg(u(x)))))
+def cell_hash(x, y, z):
+    """The spatial hash of a corner (uint32 arithmetic; negatives wrap as two's complement)."""
+    u = lambda a: np.asarray(a, np.int64).astype(np.uint64) & M32  # noqa: E731
+    return (u(x) ^ ((u(y) * np.uint64(2654435761)) & M32) ^ ((u(z) * np.uint64(805459861)) & M32)) & M32
 
 
 class CacheOracle:
@@ -38,6 +41,14 @@ class CacheOracle:
         self.inv_ext = (np.float32(1) / (np.asarray(bbox_max, np.float32) - self.lo)).astype(np.float32)
         b = math.exp((math.log(max_resolution) - math.log(base_resolution)) / (levels - 1)) if levels > 1 else 1.0
         self.res = np.array([math.floor(base_resolution * b ** l) for l in range(levels)], np.float32)
+        self.dense = [(float(r) + 2) ** 3 <= self.entries for r in self.res]
+
+    def corner_index(self, x, y, z, l):
+        """Table entry (within level l) of corner (x, y, z)."""
+        if self.dense[l]:
+            n = np.int64(self.res[l]) + 2
+            return (np.asarray(x, np.int64) + n * (np.asarray(y, np.int64) + n * np.asarray(z, np.int64))).astype(np.int64)
+        return (cell_hash(x, y, z) & np.uint64(self.entries - 1)).astype(np.int64)
 
     def corners(self, q, l):
         """Table indices (n, 8) and trilinear weights (n, 8) of level l, in float32 as the GPU."""
@@ -51,8 +62,7 @@ class CacheOracle:
         one = np.float32(1)
         for k in range(8):
             dx, dy, dz = k & 1, (k >> 1) & 1, k >> 2
-            h = cell_hash(c[:, 0] + dx, c[:, 1] + dy, c[:, 2] + dz, l) & np.uint64(self.entries - 1)
-            idx[:, k] = l * self.entries + h.astype(np.int64)
+            idx[:, k] = l * self.entries + self.corner_index(c[:, 0] + dx, c[:, 1] + dy, c[:, 2] + dz, l)
             wx = f[:, 0] if dx else one - f[:, 0]
             wy = f[:, 1] if dy else one - f[:, 1]
             wz = f[:, 2] if dz else one - f[:, 2]

@@ -371,8 +371,14 @@ static f3 sample_cos_hemisphere(float u1, float u2) {
 static float cos_hemisphere_pdfw(float c) { return maxf(c, 0.0f) / OR_PI; }
 static float concentric_disc_pdfa(void) { return 1.0f / OR_PI; }
 static float mis1(float a) { return a * a; }
-/* mis, testrenderer.slang:284-290 */
-static float mis2(float a, float b) { float a2 = mis1(a); return a2 / (a2 + mis1(b)); }
+/* mis, testrenderer.slang:284-290, with its limit where a pdf overflowed
+ * (inf / inf: 1 for the infinite pdf, 0 against it, 1/2 for two) instead of
+ * the NaN the reference's quotient gives */
+static float mis2(float a, float b) {
+    const float a2 = mis1(a), b2 = mis1(b);
+    const float w = a2 / (a2 + b2);
+    return w == w ? w : (a2 > b2 ? 1.0f : (a2 < b2 ? 0.0f : 0.5f));
+}
 
 /* stableAtan2 / cartesianToSphericalUv / sphericalUvToCartesian, compat/common.h:137-150 */
 static float stable_atan2(float y, float x) {

@@ -112,12 +112,12 @@ class st_render_request(C.Structure):
 class st_cache_config(C.Structure):
     _fields_ = [("levels", u32), ("log2_table", u32), ("base_resolution", u32), ("max_resolution", u32),
                 ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
-                ("seed", u32)]
+                ("seed", u32), ("ema_decay", C.c_float)]
 
 
 def default_cache_config(**kw):
     c = st_cache_config(levels=16, log2_table=19, base_resolution=16, max_resolution=2048, learning_rate=1e-2,
-                        beta1=0.9, beta2=0.99, epsilon=1e-15, seed=0)
+                        beta1=0.9, beta2=0.99, epsilon=1e-15, seed=0, ema_decay=0.0)
     for k, v in kw.items():
         setattr(c, k, v)
     return c

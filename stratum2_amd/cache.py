@@ -75,7 +75,7 @@ class RadianceCache:
         abi.check(self.lib, self.lib.st_cache_set_params(self.h, t.ctypes.data if t is not None else None,
                                                          w.ctypes.data if w is not None else None), "st_cache_set_params")
 
-    def render(self, scene, view, params, cache_depth=3, train_stride=16, train_steps=1, frame=0, want_train=False,
+    def render(self, scene, view, params, cache_depth=2, train_stride=16, train_steps=1, frame=0, want_train=False,
                want_path_length=False, out=None):
         """st_render_cached: the path tracer with paths ending in this cache at
         path length cache_depth, one training path per train_stride paths, then
@@ -103,7 +103,8 @@ class RadianceCache:
         return res
 
     def last_times(self):
-        """(inference + target ms, their launches, training ms, steps) of the last render."""
-        out = (C.c_double * 4)()
+        """(inference + target ms, their launches, training ms, steps, cache queries,
+        training rows) of the last render."""
+        out = (C.c_double * 6)()
         abi.check(self.lib, self.lib.st_cache_last_times(self.ctx.h, out), "st_cache_last_times")
         return tuple(out)

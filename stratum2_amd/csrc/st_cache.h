@@ -30,12 +30,14 @@ constexpr int kCacheMaxLevels = 16;
 constexpr int kCacheQuery = 12;    // floats per query
 
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 struct CacheGrid {
     float lo[3], inv_ext[3];            // position normalisation: (p - lo) * inv_ext in [0, 1]
     float res[kCacheMaxLevels];         // cells per axis of each level
     uint32_t levels, table_mask;        // table entries per level = table_mask + 1 (a power of 2)
+    uint32_t dense;                     // bit l: level l's (res + 2)^3 corners fit the table, indexed directly
 };
 
 // pcg (rng.hlsli:17-21)
@@ -44,42 +46,51 @@ __device__ __forceinline__ uint32_t rc_pcg(uint32_t v) {
     const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
     return (w >> 22u) ^ w;
 }
-// the cell hash of HashGrid::FindCellIndex (hashgrid.hlsli:56-58), keyed by
-// level instead of cell size
-__device__ __forceinline__ uint32_t rc_hash(int x, int y, int z, uint32_t level) {
-    return rc_pcg(rc_pcg(level) + rc_pcg((uint32_t)z + rc_pcg((uint32_t)y + rc_pcg((uint32_t)x))));
-}
 
-// The 8 corners of a query's cell at one level: table entries and trilinear weights
-__device__ __forceinline__ void rc_corners(const CacheGrid& g, const float* q, uint32_t l, uint32_t idx[8], float wt[8]) {
+// The 8 corners of a query's cell at one level: table entries and trilinear
+// weights.  A level whose (res + 2)^3 corner points fit its table indexes
+// them directly (x + n (y + n z), n = res + 2); a finer level hashes them
+// with the spatial hash of Mueller et al., "Instant Neural Graphics
+// Primitives", 2022 (x xor 2654435761 y xor 805459861 z), which keeps the
+// x and x + 1 corners in neighbouring entries, mostly one cache line.
+// res: the levels' resolutions in LDS (rc_load_res; a lane-dependent index
+// into the kernel argument would put the argument in scratch)
+__device__ __forceinline__ void rc_corners(const CacheGrid& g, const float* res, const float* q, uint32_t l, uint32_t idx[8],
+                                           float wt[8]) {
     float p[3];
     int c[3];
     float f[3];
     for (int a = 0; a < 3; a++) {
         float u = (q[a] - g.lo[a]) * g.inv_ext[a];
         u = fminf(fmaxf(u, 0.0f), 1.0f);
-        p[a] = u * g.res[l];
+        p[a] = u * res[l];
         const float fl = floorf(p[a]);
         c[a] = (int)fl;
         f[a] = p[a] - fl;
     }
+    const bool dense = (g.dense >> l) & 1u;
+    const uint32_t n = (uint32_t)res[l] + 2;
     for (int k = 0; k < 8; k++) {
         const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
-        idx[k] = l * (g.table_mask + 1) + (rc_hash(c[0] + dx, c[1] + dy, c[2] + dz, l) & g.table_mask);
+        const uint32_t x = (uint32_t)(c[0] + dx), y = (uint32_t)(c[1] + dy), z = (uint32_t)(c[2] + dz);
+        const uint32_t h = dense ? x + n * (y + n * z) : (x ^ (y * 2654435761u) ^ (z * 805459861u)) & g.table_mask;
+        idx[k] = l * (g.table_mask + 1) + h;
         wt[k] = (dx ? f[0] : 1 - f[0]) * (dy ? f[1] : 1 - f[1]) * (dz ? f[2] : 1 - f[2]);
     }
 }
 
 // The two hash features of level l: trilinear over the level's cell
-__device__ __forceinline__ void rc_level(const CacheGrid& g, const _Float16* table, const float* q, uint32_t l, _Float16& fa,
-                                         _Float16& fb) {
+__device__ __forceinline__ void rc_level(const CacheGrid& g, const float* res, const _Float16* table, const float* q, uint32_t l,
+                                         _Float16& fa, _Float16& fb) {
     uint32_t idx[8];
     float wt[8];
-    rc_corners(g, q, l, idx, wt);
+    rc_corners(g, res, q, l, idx, wt);
+    h2 t[8];
+    for (int k = 0; k < 8; k++) t[k] = *reinterpret_cast<const h2*>(table + 2 * (size_t)idx[k]);  // one entry: both features
     float a = 0, b = 0;
     for (int k = 0; k < 8; k++) {
-        a += wt[k] * (float)table[2 * (size_t)idx[k]];
-        b += wt[k] * (float)table[2 * (size_t)idx[k] + 1];
+        a += wt[k] * (float)t[k][0];
+        b += wt[k] * (float)t[k][1];
     }
     fa = (_Float16)a;
     fb = (_Float16)b;
@@ -87,46 +98,65 @@ __device__ __forceinline__ void rc_level(const CacheGrid& g, const _Float16* tab
 // Raw input i after the hash features: (direction + 1) / 2, (normal + 1) / 2,
 // albedo, a constant 1, then zeros
 __device__ __forceinline__ _Float16 rc_raw(const float* q, uint32_t i) {
-    if (i < 6) return (_Float16)((q[3 + i] + 1) * 0.5f);
-    if (i < 9) return (_Float16)q[3 + i];
-    return (_Float16)(i == 9 ? 1.0f : 0.0f);
+    // selects over constant indices: q stays in registers
+    float v = i == 9 ? 1.0f : 0.0f;
+    for (int a = 0; a < 6; a++) v = i == (uint32_t)a ? (q[3 + a] + 1) * 0.5f : v;
+    for (int a = 6; a < 9; a++) v = i == (uint32_t)a ? q[3 + a] : v;
+    return (_Float16)v;
 }
 // Network inputs f and f + 1 (f even) of query q
-__device__ __forceinline__ void rc_feature_pair(const CacheGrid& g, const _Float16* table, const float* q, uint32_t f, _Float16& a,
-                                                _Float16& b) {
+__device__ __forceinline__ void rc_feature_pair(const CacheGrid& g, const float* res, const _Float16* table, const float* q,
+                                                uint32_t f, _Float16& a, _Float16& b) {
     if (f / 2 < g.levels) {
-        rc_level(g, table, q, f / 2, a, b);
+        rc_level(g, res, table, q, f / 2, a, b);
     } else {
         a = rc_raw(q, f - 2 * g.levels);
         b = rc_raw(q, f + 1 - 2 * g.levels);
     }
 }
 
+// The levels' resolutions into LDS (constant indices into the argument);
+// every thread of the block must call it
+__device__ __forceinline__ void rc_load_res(const CacheGrid& g, float* res) {
+    if (threadIdx.x == 0)
+        for (int l = 0; l < kCacheMaxLevels; l++) res[l] = g.res[l];
+    __syncthreads();
+}
+
+// Rows a training kernel covers: n, or fewer when the count is on the device
+__device__ __forceinline__ uint32_t rc_rows(uint32_t n, const uint32_t* n_dev) { return n_dev ? min(*n_dev, n) : n; }
+
 // X[s] (64 fp16) from query s: hash features, then the raw inputs
-__global__ void k_rc_encode(CacheGrid g, const _Float16* table, const float* queries, uint32_t n, _Float16* X) {
+__global__ void k_rc_encode(CacheGrid g, const _Float16* table, const float* queries, uint32_t n_max, const uint32_t* n_dev,
+                            _Float16* X) {
+    __shared__ float res[kCacheMaxLevels];
+    rc_load_res(g, res);
+    const uint32_t n = rc_rows(n_max, n_dev);
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         const float* q = queries + (size_t)kCacheQuery * s;
         _Float16* x = X + (size_t)kMlpWidth * s;
-        for (uint32_t f = 0; f < (uint32_t)kMlpWidth; f += 2) rc_feature_pair(g, table, q, f, x[f], x[f + 1]);
+        for (uint32_t f = 0; f < (uint32_t)kMlpWidth; f += 2) rc_feature_pair(g, res, table, q, f, x[f], x[f + 1]);
     }
 }
 
 // Inference for the paths of a wave that ended in the cache, fused into one
 // kernel: each wavefront takes 16 queries at a time, encodes them straight
 // into the first layer's B operands (lane l: features 16 kb + 4 (l / 16) + j
-// of query l % 16) and runs the four layers in registers, each layer's D
-// tile being the next layer's B operand; the weights sit in LDS.  The
-// arithmetic is that of k_rc_encode + k_mlp_layer + k_rc_output, so a query
-// gets the radiance st_cache_infer gives it.  em[path] += throughput x radiance.
-// cq: 4 float4 per query ({pos, thr.x}, {wo, thr.y}, {normal, thr.z},
-// {albedo, path}); count: queries (device).
+// of query l % 16, so the 4 lanes of a query read 4 levels each) and runs
+// the four layers in registers, each layer's D tile being the next layer's B
+// operand; the weights sit in LDS.  The arithmetic is that of k_rc_encode +
+// k_mlp_layer + k_rc_output, so a query gets the radiance st_cache_infer
+// gives it.  em[path] += throughput x radiance.  cq: 4 float4 per query
+// ({pos, thr.x}, {wo, thr.y}, {normal, thr.z}, {albedo, path}); count:
+// queries (device).
 __global__ void __launch_bounds__(256) k_rc_infer_paths(CacheGrid g, const _Float16* table, const _Float16* W16, const float4* cq,
                                                          const uint32_t* count, float4* em) {
     constexpr int kH4 = kMlpLayers * kMlpWidth * kMlpWidth / 4;
     __shared__ h4 Wl[kH4];
+    __shared__ float res[kCacheMaxLevels];
     const h4* Wg = reinterpret_cast<const h4*>(W16);
     for (int i = threadIdx.x; i < kH4; i += blockDim.x) Wl[i] = Wg[i];
-    __syncthreads();
+    rc_load_res(g, res);
     const uint32_t n = *count;
     const uint32_t lane = __lane_id(), r = lane % 16, qd = lane / 16;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, waves = gridDim.x * blockDim.x / 64;
@@ -147,8 +177,8 @@ __global__ void __launch_bounds__(256) k_rc_infer_paths(CacheGrid g, const _Floa
         for (int kb = 0; kb < 4; kb++) {
             const uint32_t f = kb * 16 + 4 * qd;
             _Float16 a, b, c, d;
-            rc_feature_pair(g, table, q, f, a, b);
-            rc_feature_pair(g, table, q, f + 2, c, d);
+            rc_feature_pair(g, res, table, q, f, a, b);
+            rc_feature_pair(g, res, table, q, f + 2, c, d);
             B[kb] = in ? h4{a, b, c, d} : h4{0, 0, 0, 0};
         }
         f4v acc[4];
@@ -175,7 +205,11 @@ __global__ void __launch_bounds__(256) k_rc_infer_paths(CacheGrid g, const _Floa
 }
 
 // dTable += sum over queries of weight * dX[s][2 l + f]
-__global__ void k_rc_encode_bwd(CacheGrid g, const float* queries, uint32_t n, const _Float16* dX, float* dtable) {
+__global__ void k_rc_encode_bwd(CacheGrid g, const float* queries, uint32_t n_max, const uint32_t* n_dev, const _Float16* dX,
+                                float* dtable) {
+    __shared__ float res[kCacheMaxLevels];
+    rc_load_res(g, res);
+    const uint32_t n = rc_rows(n_max, n_dev);
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         const float* q = queries + (size_t)kCacheQuery * s;
         const _Float16* d = dX + (size_t)kMlpWidth * s;
@@ -184,7 +218,7 @@ __global__ void k_rc_encode_bwd(CacheGrid g, const float* queries, uint32_t n, c
             if (ga == 0 && gb == 0) continue;
             uint32_t idx[8];
             float wt[8];
-            rc_corners(g, q, l, idx, wt);
+            rc_corners(g, res, q, l, idx, wt);
             for (int k = 0; k < 8; k++) {
                 atomicAdd(dtable + 2 * (size_t)idx[k], wt[k] * ga);
                 atomicAdd(dtable + 2 * (size_t)idx[k] + 1, wt[k] * gb);
@@ -199,7 +233,8 @@ __global__ void k_rc_encode_bwd(CacheGrid g, const float* queries, uint32_t n, c
 // row-major [out][in].  n need not be a multiple of 64 (rows past n are
 // neither read nor written).
 __global__ void __launch_bounds__(256) k_mlp_layer(const _Float16* W, const _Float16* X, _Float16* Y, const _Float16* mask,
-                                                    uint32_t n, int relu) {
+                                                    uint32_t n_max, const uint32_t* n_dev, int relu) {
+    const uint32_t n = rc_rows(n_max, n_dev);
     const uint32_t lane = __lane_id(), r = lane % 16, qd = lane / 16;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64, waves = gridDim.x * blockDim.x / 64;
     h4 A[4][4];
@@ -238,15 +273,20 @@ __global__ void __launch_bounds__(256) k_mlp_layer(const _Float16* W, const _Flo
     }
 }
 
-// dW[o][i] += scale * sum_s dY[s][o] X[s][i]: one wavefront per chunk of
-// `chunk` rows (a multiple of 16), all 16 output tiles in registers,
-// then one atomic per element and lane.
-__global__ void __launch_bounds__(256) k_mlp_wgrad(const _Float16* dY, const _Float16* X, float* dW, uint32_t n, uint32_t chunk,
-                                                    float scale) {
+// partial[w][o][i] = sum over wavefront w's chunk of `chunk` rows (a
+// multiple of 16) of dY[s][o] X[s][i], all 16 output tiles in registers;
+// k_mlp_wgrad_reduce adds the partials up.
+__global__ void __launch_bounds__(256) k_mlp_wgrad(const _Float16* dY, const _Float16* X, float* partial, uint32_t n_max,
+                                                    const uint32_t* n_dev, uint32_t chunk) {
     const uint32_t lane = __lane_id(), r = lane % 16, qd = lane / 16;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t n = rc_rows(n_max, n_dev);
     const uint32_t s0 = wave * chunk;
-    if (s0 >= n) return;  // whole wavefronts leave together
+    if (s0 >= n) {  // whole wavefronts leave together; past a device count their partials are 0
+        if (s0 < n_max)
+            for (int e = lane; e < kMlpWidth * kMlpWidth; e += 64) partial[(size_t)wave * kMlpWidth * kMlpWidth + e] = 0.0f;
+        return;
+    }
     const uint32_t s1 = min(n, s0 + chunk);
     f4v acc[4][4];
     for (int ob = 0; ob < 4; ob++)
@@ -265,10 +305,21 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(const _Float16* dY, const _Fl
             for (int ib = 0; ib < 4; ib++) acc[ob][ib] = __builtin_amdgcn_mfma_f32_16x16x16f16(A[ob], B[ib], acc[ob][ib], 0, 0, 0);
     }
     // D[o][i]: lane holds o = ob 16 + 4 qd + j, i = ib 16 + r
+    float* out = partial + (size_t)wave * kMlpWidth * kMlpWidth;
     for (int ob = 0; ob < 4; ob++)
         for (int ib = 0; ib < 4; ib++)
-            for (int j = 0; j < 4; j++)
-                atomicAdd(dW + (ob * 16 + 4 * qd + j) * kMlpWidth + ib * 16 + r, scale * acc[ob][ib][j]);
+            for (int j = 0; j < 4; j++) out[(ob * 16 + 4 * qd + j) * kMlpWidth + ib * 16 + r] = acc[ob][ib][j];
+}
+
+// dW[e] += scale * the sum of the nw partials of element e: block (x, y)
+// sums partials y, y + gridDim.y, ... of 256 consecutive elements, and the
+// gridDim.y sums of an element meet in one atomic add each
+__global__ void k_mlp_wgrad_reduce(const float* partial, uint32_t nw, float* dW, float scale) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (uint32_t)(kMlpWidth * kMlpWidth)) return;
+    float sum = 0;
+    for (uint32_t w = blockIdx.y; w < nw; w += gridDim.y) sum += partial[(size_t)w * kMlpWidth * kMlpWidth + e];
+    atomicAdd(dW + e, scale * sum);
 }
 
 // Relative L2 loss of the radiance cache (Mueller et al., "Real-time Neural
@@ -278,7 +329,9 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(const _Float16* dY, const _Fl
 // weights' gradient scale).  targets: tstride floats per sample, rgb and,
 // with tstride 4, a weight (0: the sample is left out).  loss_sum[0]
 // receives the weighted loss sum, loss_sum[1] the weight sum.
-__global__ void k_rc_loss(const _Float16* Y, const float* targets, uint32_t tstride, uint32_t n, _Float16* dY, float* loss_sum) {
+__global__ void k_rc_loss(const _Float16* Y, const float* targets, uint32_t tstride, uint32_t n_max, const uint32_t* n_dev,
+                          _Float16* dY, float* loss_sum) {
+    const uint32_t n = rc_rows(n_max, n_dev);
     float local = 0, wsum = 0;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         _Float16* d = dY + (size_t)kMlpWidth * s;
@@ -307,10 +360,13 @@ __global__ void k_rc_loss(const _Float16* Y, const float* targets, uint32_t tstr
 // Adam (Kingma and Ba 2015) over n parameters with fp32 master copies; g is
 // scaled by gscale; entries whose gradient is exactly zero keep their
 // moments and value (sparse: table entries no sample touched).  p16: the
-// fp16 copy the kernels read; p16t (optional): its 64 x 64 blocks
-// transposed, for the backward pass.
+// fp16 copy the training kernels read; p16t (optional): its 64 x 64 blocks
+// transposed, for the backward pass.  ema (optional): the exponential moving
+// average of the parameters, ema = decay ema + (1 - decay) p, whose fp16 copy
+// ema16 inference reads (Polyak averaging, as Mueller et al. 2021 infer with).
 __global__ void k_rc_adam(float* p, float* m, float* v, const float* g, uint32_t n, float gscale, float lr, float b1, float b2,
-                          float eps, float c1, float c2, _Float16* p16, _Float16* p16t, int sparse) {
+                          float eps, float c1, float c2, _Float16* p16, _Float16* p16t, int sparse, float* ema, _Float16* ema16,
+                          float decay) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float gi = g[i] * gscale;
         if (!(sparse && gi == 0.0f)) {
@@ -325,6 +381,11 @@ __global__ void k_rc_adam(float* p, float* m, float* v, const float* g, uint32_t
             const uint32_t blk = i / (kMlpWidth * kMlpWidth), e = i % (kMlpWidth * kMlpWidth);
             const uint32_t o = e / kMlpWidth, in = e % kMlpWidth;
             p16t[(size_t)blk * kMlpWidth * kMlpWidth + in * kMlpWidth + o] = (_Float16)p[i];
+        }
+        if (ema) {
+            const float e = decay * ema[i] + (1 - decay) * p[i];
+            ema[i] = e;
+            ema16[i] = (_Float16)e;
         }
     }
 }

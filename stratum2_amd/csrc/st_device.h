@@ -160,10 +160,14 @@ __device__ __forceinline__ v3 sample_cos_hemisphere(float u1, float u2) {
 }
 __device__ __forceinline__ float cos_hemisphere_pdfw(float c) { return fmax_(c, 0.0f) / ST_PI; }
 __device__ __forceinline__ float mis1(float a) { return a * a; }
-// mis, testrenderer.slang:284-290
+// mis, testrenderer.slang:284-290.  Where a pdf overflowed (the fp16 store
+// of a sharp lobe's direction pdf reads back as inf, testrenderer.slang:
+// 93-97) the reference's quotient is inf / inf = NaN and poisons the pixel;
+// here it takes its limit: 1 for the infinite pdf, 0 against it, 1/2 for two.
 __device__ __forceinline__ float mis2(float a, float b) {
-    float a2 = mis1(a);
-    return a2 / (a2 + mis1(b));
+    const float a2 = mis1(a), b2 = mis1(b);
+    const float w = a2 / (a2 + b2);
+    return w == w ? w : (a2 > b2 ? 1.0f : (a2 < b2 ? 0.0f : 0.5f));
 }
 // compat/common.h:137-150
 __device__ __forceinline__ float stable_atan2(float y, float x) {

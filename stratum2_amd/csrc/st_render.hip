@@ -68,7 +68,8 @@ constexpr int kTraceBlock = 128;
 constexpr int kShadeBlock = 256;
 constexpr int kTraceBlocksPerCU = 16;
 // WaveState::counters (kCounterWords words): [0..1] queue counts (ping-pong),
-// [2] shadow count, [3] radiance-cache queries, [4..7] u64 ray counts, then the traversal fetch cursors:
+// [2] shadow count, [3] radiance-cache queries, [4..7] u64 ray counts,
+// [8..9] u64 radiance-cache queries of the call, then the traversal fetch cursors:
 // kXcds per traversal kernel, kFetchStride words (one 128-byte line) apart
 constexpr int kFetchClosest = 256, kFetchShadow = kFetchClosest + (int)(kXcds * kFetchStride),
               kFetchRays = kFetchShadow + (int)(kXcds * kFetchStride);
@@ -1848,6 +1849,26 @@ __global__ void k_rc_targets(RenderConsts c, WaveState w, uint32_t row_lo, uint3
     }
 }
 
+// st_render_cached's training rows with weight, moved in row order to the
+// front of dense arrays (queries, target rgb): flags for the scan, then the
+// move; the last thread writes the count.
+__global__ void k_rc_row_flags(const float4* tt, uint32_t rows, uint32_t* flag) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) flag[i] = tt[i].w > 0 ? 1u : 0u;
+}
+__global__ void k_rc_row_compact(const float* tq, const float4* tt, const uint32_t* flag, const uint32_t* off, uint32_t rows,
+                                 float* q_out, float* t_out, uint32_t* count) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) {
+        if (i == rows - 1) *count = off[i] + flag[i];
+        if (!flag[i]) continue;
+        const uint32_t d = off[i];
+        for (int a = 0; a < 12; a++) q_out[12 * (size_t)d + a] = tq[12 * (size_t)i + a];
+        const float4 t = tt[i];
+        t_out[3 * (size_t)d] = t.x;
+        t_out[3 * (size_t)d + 1] = t.y;
+        t_out[3 * (size_t)d + 2] = t.z;
+    }
+}
+
 __global__ void k_count(WaveState w, int qsel, int media) {
     // rays traced this bounce: extension queue + shadow queue (the media
     // kernels count their traceRay segments themselves)
@@ -1856,7 +1877,9 @@ __global__ void k_count(WaveState w, int qsel, int media) {
         r[0] += w.counters[qsel];
         r[1] += w.counters[2];
     }
+    r[2] += w.counters[3];  // radiance-cache queries
     w.counters[2] = 0;
+    w.counters[3] = 0;
     w.counters[qsel] = 0;
     for (int i = 0; i < 2 * (int)kXcds; i++) w.counters[kFetchClosest + i * kFetchStride] = 0;  // traversal fetch cursors
 }
@@ -2070,6 +2093,8 @@ struct st_context {
     uint8_t* scratch = nullptr;          // this device's fixed scratch (device address of `scratch`)
     DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
     double timings[12] = {0};
+    double cache_queries = 0;            // paths that ended in the radiance cache in the last call
+    double cache_rows = 0;               // its training rows with weight 1 (when its loss was asked for)
     // HIP-event timed kernel classes: 0 closest-hit trace, 1 shading, 2 shadow
     // trace, 3 primary-vertex RIS (k_ris), 4 screen-space reuse passes,
     // 5 radiance-cache inference + targets, 6 radiance-cache training
@@ -2903,11 +2928,13 @@ static void finish_timings(st_context* ctx, int slots, uint64_t* ray_counts) {
     }
     unsigned long long hs[4], rc[2] = {0, 0};
     HIP_CHECK(hipMemcpy(hs, ctx->stats.p, 32, hipMemcpyDeviceToHost));
+    ctx->cache_queries = 0;
     for (int sl = 0; sl < slots; sl++) {
-        unsigned long long r[2];
-        HIP_CHECK(hipMemcpy(r, ctx->slot[sl].counters.as<uint8_t>() + 16, 16, hipMemcpyDeviceToHost));
+        unsigned long long r[3];
+        HIP_CHECK(hipMemcpy(r, ctx->slot[sl].counters.as<uint8_t>() + 16, 24, hipMemcpyDeviceToHost));
         rc[0] += r[0];
         rc[1] += r[1];
+        ctx->cache_queries += (double)r[2];
     }
     for (int i = 0; i < 4; i++) ctx->timings[6 + i] = (double)hs[i];
     ctx->timings[10] = (double)rc[0];
@@ -3384,9 +3411,16 @@ struct st_cache {
     uint32_t table_floats = 0;  // levels x entries x 2
     DevBuf table32, table16, tm, tv, tgrad;
     DevBuf w32, w16, w16t, wm, wv, wgrad;  // 4 layers of 64 x 64
+    DevBuf table_ema, w_ema, table16e, w16e;  // the parameters' moving average (config.ema_decay > 0)
+    bool ema() const { return cfg.ema_decay > 0; }
+    // the fp16 parameters inference reads: the moving average's when kept
+    const _Float16* infer_table() const { return (ema() ? table16e : table16).as<_Float16>(); }
+    const _Float16* infer_weights() const { return (ema() ? w16e : w16).as<_Float16>(); }
     DevBuf X0, H1, H2, H3, Y, dA, dB;      // rows x 64 fp16
+    DevBuf wpart;                          // weight-gradient partials, one 64 x 64 block per wavefront
     DevBuf loss;
     DevBuf trq, trthr, trt;  // st_render_cached's training rows: queries, throughputs, targets
+    DevBuf flag, off, scan_tmp, cq, ct, count;  // and the rows with weight, compacted
     uint32_t rows = 0;
     uint32_t step = 0;
     void reserve_rows(uint32_t n) {
@@ -3403,19 +3437,21 @@ constexpr uint32_t kCacheMaxRows = 1u << 24;
 
 int grid_for(int n, int per, int cap) { return std::max(1, std::min((n + per - 1) / per, cap)); }
 
-// X0 -> H1 -> H2 -> H3 -> Y on the context stream
-void cache_forward(st_cache* c, const float* queries, uint32_t n) {
+// X0 -> H1 -> H2 -> H3 -> Y on the context stream over n rows (at most n
+// when n_dev, a device count, is given), with the training parameters or
+// (infer) the ones inference reads
+void cache_forward(st_cache* c, const float* queries, uint32_t n, const uint32_t* n_dev = nullptr, bool infer = false) {
     hipStream_t st = c->ctx->stream;
     const int cus = c->ctx->cus;
-    const _Float16* w = c->w16.as<_Float16>();
-    hipLaunchKernelGGL(k_rc_encode, dim3(grid_for((int)n, 256, cus * 16)), dim3(256), 0, st, c->grid, c->table16.as<_Float16>(),
-                       queries, n, c->X0.as<_Float16>());
+    const _Float16* w = infer ? c->infer_weights() : c->w16.as<_Float16>();
+    hipLaunchKernelGGL(k_rc_encode, dim3(grid_for((int)n, 256, cus * 16)), dim3(256), 0, st, c->grid,
+                       infer ? c->infer_table() : c->table16.as<_Float16>(), queries, n, n_dev, c->X0.as<_Float16>());
     const int lg = grid_for((int)n, 256, cus * 8);  // 4 wavefronts of 64 rows per block
     _Float16* in[4] = {c->X0.as<_Float16>(), c->H1.as<_Float16>(), c->H2.as<_Float16>(), c->H3.as<_Float16>()};
     _Float16* out[4] = {c->H1.as<_Float16>(), c->H2.as<_Float16>(), c->H3.as<_Float16>(), c->Y.as<_Float16>()};
     for (int l = 0; l < kMlpLayers; l++)
         hipLaunchKernelGGL(k_mlp_layer, dim3(lg), dim3(256), 0, st, w + (size_t)l * kMlpWidth * kMlpWidth, in[l], out[l],
-                           (const _Float16*)nullptr, n, l < kMlpLayers - 1 ? 1 : 0);
+                           (const _Float16*)nullptr, n, n_dev, l < kMlpLayers - 1 ? 1 : 0);
 }
 
 void cache_copy16(st_cache* c) {
@@ -3424,6 +3460,12 @@ void cache_copy16(st_cache* c) {
                        c->table32.as<float>(), c->table_floats, c->table16.as<_Float16>(), (_Float16*)nullptr);
     hipLaunchKernelGGL(k_rc_copy16, dim3(grid_for((int)kMlpParams, 256, 64)), dim3(256), 0, st, c->w32.as<float>(), kMlpParams,
                        c->w16.as<_Float16>(), c->w16t.as<_Float16>());
+    if (c->ema()) {  // the moving average restarts at the parameters
+        HIP_CHECK(hipMemcpyAsync(c->table_ema.p, c->table32.p, 4ull * c->table_floats, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(c->w_ema.p, c->w32.p, 4ull * kMlpParams, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(c->table16e.p, c->table16.p, 2ull * c->table_floats, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(c->w16e.p, c->w16.p, 2ull * kMlpParams, hipMemcpyDeviceToDevice, st));
+    }
 }
 }  // namespace
 
@@ -3434,8 +3476,8 @@ int st_cache_create(st_context* ctx, const st_cache_config* cfg, const float bbo
         return set_error(ST_ERROR_INVALID_ARGUMENT,
                          "cache: 1-16 levels, 2^8 to 2^24 entries per level, 2 <= base resolution <= max resolution <= 2^20");
     if (!(cfg->learning_rate > 0) || !(cfg->beta1 >= 0 && cfg->beta1 < 1) || !(cfg->beta2 >= 0 && cfg->beta2 < 1) ||
-        !(cfg->epsilon > 0))
-        return set_error(ST_ERROR_INVALID_ARGUMENT, "cache: learning rate > 0, betas in [0, 1), epsilon > 0");
+        !(cfg->epsilon > 0) || !(cfg->ema_decay >= 0 && cfg->ema_decay < 1))
+        return set_error(ST_ERROR_INVALID_ARGUMENT, "cache: learning rate > 0, betas and ema_decay in [0, 1), epsilon > 0");
     for (int a = 0; a < 3; a++)
         if (!(bbox_max[a] > bbox_min[a])) return set_error(ST_ERROR_INVALID_ARGUMENT, "cache: empty bounding box");
     try {
@@ -3455,8 +3497,12 @@ int st_cache_create(st_context* ctx, const st_cache_config* cfg, const float bbo
         const double b = cfg->levels > 1 ? std::exp((std::log((double)cfg->max_resolution) - std::log((double)cfg->base_resolution)) /
                                                     (double)(cfg->levels - 1))
                                          : 1.0;
-        for (uint32_t l = 0; l < kCacheMaxLevels; l++)
+        for (uint32_t l = 0; l < kCacheMaxLevels; l++) {
             g.res[l] = l < cfg->levels ? (float)std::floor((double)cfg->base_resolution * std::pow(b, (double)l)) : 0.0f;
+            // levels whose (res + 2)^3 corner points fit the table index them directly
+            const double n = (double)g.res[l] + 2;
+            if (l < cfg->levels && n * n * n <= (double)(1u << cfg->log2_table)) g.dense |= 1u << l;
+        }
         c->table_floats = cfg->levels * (1u << cfg->log2_table) * 2;
         // initial parameters: table entries uniform in +-1e-4, weights uniform
         // in +-sqrt(6 / (fan in + fan out)); the output matrix's rows 3..63 zero
@@ -3485,6 +3531,12 @@ int st_cache_create(st_context* ctx, const st_cache_config* cfg, const float bbo
         c->wgrad.reserve(4ull * kMlpParams);
         c->w16.reserve(2ull * kMlpParams);
         c->w16t.reserve(2ull * kMlpParams);
+        if (c->ema()) {
+            c->table_ema.reserve(tb);
+            c->table16e.reserve(tb / 2);
+            c->w_ema.reserve(4ull * kMlpParams);
+            c->w16e.reserve(2ull * kMlpParams);
+        }
         c->loss.reserve(64);
         HIP_CHECK(hipMemcpy(c->table32.p, t.data(), tb, hipMemcpyHostToDevice));
         HIP_CHECK(hipMemcpy(c->w32.p, w.data(), 4ull * kMlpParams, hipMemcpyHostToDevice));
@@ -3519,7 +3571,7 @@ int st_cache_infer(st_cache* c, const float* queries, uint32_t n, float* radianc
         HIP_CHECK(hipSetDevice(c->ctx->device));
         order_after_caller(c->ctx);
         c->reserve_rows(n);
-        cache_forward(c, queries, n);
+        cache_forward(c, queries, n, nullptr, true);
         hipLaunchKernelGGL(k_rc_output, dim3(grid_for((int)n, 256, c->ctx->cus * 16)), dim3(256), 0, c->ctx->stream, c->Y.as<_Float16>(),
                            n, radiance);
         HIP_CHECK(hipGetLastError());
@@ -3531,38 +3583,47 @@ int st_cache_infer(st_cache* c, const float* queries, uint32_t n, float* radianc
 }
 
 namespace {
-// One Adam step on n samples already on the device, on the context stream;
-// targets: tstride (3, or 4 with a weight) floats per sample
-void cache_train_step(st_cache* c, const float* queries, const float* targets, uint32_t tstride, uint32_t n) {
+// One Adam step on n samples already on the device (at most n when n_dev, a
+// device count, is given), on the context stream; targets: tstride (3, or 4
+// with a weight) floats per sample.  The gradient is scaled by 1 / n
+// (Adam's step does not depend on the scale).
+void cache_train_step(st_cache* c, const float* queries, const float* targets, uint32_t tstride, uint32_t n,
+                      const uint32_t* n_dev = nullptr) {
     st_context* ctx = c->ctx;
     hipStream_t st = ctx->stream;
     const int cus = ctx->cus;
     c->reserve_rows(n);
-    cache_forward(c, queries, n);
+    cache_forward(c, queries, n, n_dev);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int)c->table_floats, 256, cus * 16)), dim3(256), 0, st, c->tgrad.as<uint32_t>(),
                        c->table_floats, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((int)kMlpParams, 256, 64)), dim3(256), 0, st, c->wgrad.as<uint32_t>(), kMlpParams, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, c->loss.as<uint32_t>(), 16u, 0u);
     hipLaunchKernelGGL(k_rc_loss, dim3(grid_for((int)n, 256, cus * 8)), dim3(256), 0, st, c->Y.as<_Float16>(), targets, tstride,
-                       n, c->dA.as<_Float16>(), c->loss.as<float>());
+                       n, n_dev, c->dA.as<_Float16>(), c->loss.as<float>());
     const _Float16* wt = c->w16t.as<_Float16>();
     float* wg = c->wgrad.as<float>();
-    const uint32_t chunk = 1024;
-    const int wgrid = grid_for((int)((n + chunk - 1) / chunk), 4, 1 << 20);  // 4 wavefronts of `chunk` rows per block
+    // rows per wavefront of the weight-gradient product: 128, more when
+    // there would be over 2048 wavefronts (the partials' buffer)
+    const uint32_t chunk = std::max<uint32_t>(128, ((n + 2047) / 2048 + 15) / 16 * 16);
+    const uint32_t nw = (n + chunk - 1) / chunk;
+    c->wpart.reserve((size_t)nw * kMlpWidth * kMlpWidth * 4);
+    const int wgrid = grid_for((int)nw, 4, 1 << 20);  // 4 wavefronts of `chunk` rows per block
     const int lg = grid_for((int)n, 256, cus * 8);
     _Float16* acts[4] = {c->X0.as<_Float16>(), c->H1.as<_Float16>(), c->H2.as<_Float16>(), c->H3.as<_Float16>()};
     _Float16* d_out = c->dA.as<_Float16>();  // gradient at layer l's output
     _Float16* d_in = c->dB.as<_Float16>();
     for (int l = kMlpLayers - 1; l >= 0; l--) {
         const size_t off = (size_t)l * kMlpWidth * kMlpWidth;
-        hipLaunchKernelGGL(k_mlp_wgrad, dim3(wgrid), dim3(256), 0, st, d_out, acts[l], wg + off, n, chunk, 1.0f);
+        hipLaunchKernelGGL(k_mlp_wgrad, dim3(wgrid), dim3(256), 0, st, d_out, acts[l], c->wpart.as<float>(), n, n_dev, chunk);
+        hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(kMlpWidth * kMlpWidth / 256, std::min<uint32_t>(nw, 32)), dim3(256), 0, st,
+                           c->wpart.as<float>(), nw, wg + off, 1.0f);
         // through the layer (transposed weights) and the ReLU that made its input
         hipLaunchKernelGGL(k_mlp_layer, dim3(lg), dim3(256), 0, st, wt + off, d_out, d_in, l > 0 ? acts[l] : (const _Float16*)nullptr,
-                           n, 0);
+                           n, n_dev, 0);
         std::swap(d_out, d_in);
     }
     // d_out now holds dLoss/dX0
-    hipLaunchKernelGGL(k_rc_encode_bwd, dim3(grid_for((int)n, 256, cus * 16)), dim3(256), 0, st, c->grid, queries, n, d_out,
+    hipLaunchKernelGGL(k_rc_encode_bwd, dim3(grid_for((int)n, 256, cus * 16)), dim3(256), 0, st, c->grid, queries, n, n_dev, d_out,
                        c->tgrad.as<float>());
     c->step++;
     const float b1 = c->cfg.beta1, b2 = c->cfg.beta2;
@@ -3570,10 +3631,12 @@ void cache_train_step(st_cache* c, const float* queries, const float* targets, u
     const float gs = 1.0f / (float)n;
     hipLaunchKernelGGL(k_rc_adam, dim3(grid_for((int)kMlpParams, 256, 64)), dim3(256), 0, st, c->w32.as<float>(), c->wm.as<float>(),
                        c->wv.as<float>(), (const float*)wg, kMlpParams, gs, c->cfg.learning_rate, b1, b2, c->cfg.epsilon, c1, c2,
-                       c->w16.as<_Float16>(), c->w16t.as<_Float16>(), 0);
+                       c->w16.as<_Float16>(), c->w16t.as<_Float16>(), 0, c->ema() ? c->w_ema.as<float>() : nullptr,
+                       c->w16e.as<_Float16>(), c->cfg.ema_decay);
     hipLaunchKernelGGL(k_rc_adam, dim3(grid_for((int)c->table_floats, 256, cus * 16)), dim3(256), 0, st, c->table32.as<float>(),
                        c->tm.as<float>(), c->tv.as<float>(), (const float*)c->tgrad.as<float>(), c->table_floats, gs,
-                       c->cfg.learning_rate, b1, b2, c->cfg.epsilon, c1, c2, c->table16.as<_Float16>(), (_Float16*)nullptr, 1);
+                       c->cfg.learning_rate, b1, b2, c->cfg.epsilon, c1, c2, c->table16.as<_Float16>(), (_Float16*)nullptr, 1,
+                       c->ema() ? c->table_ema.as<float>() : nullptr, c->table16e.as<_Float16>(), c->cfg.ema_decay);
     HIP_CHECK(hipGetLastError());
 }
 }  // namespace
@@ -3684,8 +3747,8 @@ int st_render_cached(st_scene* s, st_cache* cache, const st_view* view, const st
         hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, s0, cache->trt.as<uint32_t>(), (uint32_t)(rows * 4), 0u);
         CacheJob cj;
         cj.grid = cache->grid;
-        cj.table = cache->table16.as<_Float16>();
-        cj.w16 = cache->w16.as<_Float16>();
+        cj.table = cache->infer_table();
+        cj.w16 = cache->infer_weights();
         cj.tq = cache->trq.as<float>();
         cj.tthr = cache->trthr.as<float4>();
         cj.tt = cache->trt.as<float4>();
@@ -3720,26 +3783,50 @@ int st_render_cached(st_scene* s, st_cache* cache, const st_view* view, const st
         if (cp->train_targets)
             HIP_CHECK(hipMemcpyAsync(cp->train_targets, cache->trt.p, rows * 16, hipMemcpyDeviceToDevice, s0));
         float l[2] = {0, 0};
+        if (cp->train_steps > 0) {
+            // the rows with weight to the front, in row order: training covers them only
+            Timed tm(ctx, 5, s0);
+            const int cg = std::min<int>((int)((rows + 255) / 256), ctx->cus * 8);
+            cache->flag.reserve(rows * 4);
+            cache->off.reserve(rows * 4);
+            cache->cq.reserve(rows * 48);
+            cache->ct.reserve(rows * 12);
+            cache->count.reserve(64);
+            size_t sb = 0;
+            HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)rows, s0));
+            cache->scan_tmp.reserve(std::max<size_t>(sb, 16));
+            sb = cache->scan_tmp.n;
+            hipLaunchKernelGGL(k_rc_row_flags, dim3(cg), dim3(256), 0, s0, cache->trt.as<float4>(), (uint32_t)rows,
+                               cache->flag.as<uint32_t>());
+            HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(cache->scan_tmp.p, sb, cache->flag.as<uint32_t>(), cache->off.as<uint32_t>(),
+                                                       (int)rows, s0));
+            hipLaunchKernelGGL(k_rc_row_compact, dim3(cg), dim3(256), 0, s0, cache->trq.as<float>(), cache->trt.as<float4>(),
+                               cache->flag.as<uint32_t>(), cache->off.as<uint32_t>(), (uint32_t)rows, cache->cq.as<float>(),
+                               cache->ct.as<float>(), cache->count.as<uint32_t>());
+        }
         for (uint32_t st = 0; st < cp->train_steps; st++) {
             Timed tm(ctx, 6, s0);
-            cache_train_step(cache, cache->trq.as<float>(), cache->trt.as<float>(), 4, (uint32_t)rows);
+            cache_train_step(cache, cache->cq.as<float>(), cache->ct.as<float>(), 3, (uint32_t)rows, cache->count.as<uint32_t>());
             if (st == 0 && cp->loss) HIP_CHECK(hipMemcpyAsync(l, cache->loss.p, 8, hipMemcpyDeviceToHost, s0));
         }
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
         if (cp->loss) *cp->loss = l[1] > 0 ? l[0] / l[1] : 0.0f;
         finish_timings(ctx, slots, o->ray_counts);
+        ctx->cache_rows = cp->loss ? (double)l[1] : 0.0;
         return ST_OK;
     } catch (const std::exception& e) {
         return set_error(ST_ERROR_HIP, e.what());
     }
 }
 
-int st_cache_last_times(st_context* ctx, double out[4]) {
+int st_cache_last_times(st_context* ctx, double out[6]) {
     if (!ctx || !out) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
     out[0] = ctx->kernel_ms[5];
     out[1] = (double)ctx->ev_used[5];
     out[2] = ctx->kernel_ms[6];
     out[3] = (double)ctx->ev_used[6];
+    out[4] = ctx->cache_queries;
+    out[5] = ctx->cache_rows;
     return ST_OK;
 }
 

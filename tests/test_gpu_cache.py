@@ -113,6 +113,38 @@ def test_training_fits_a_smooth_field(gpu_ctx):
     c.close()
 
 
+def test_inference_reads_the_moving_average(gpu_ctx):
+    """With ema_decay d, inference after two steps reads
+    d (d p0 + (1 - d) p1) + (1 - d) p2, the moving average restarted at the
+    parameters set (p0), while training reads the parameters themselves."""
+    import torch
+    from stratum2_amd.cache import RadianceCache
+    d = 0.75
+    c = RadianceCache(gpu_ctx, *BOX, seed=6, ema_decay=d, **CFG)
+    rng = np.random.default_rng(4)
+    table0 = rng.normal(scale=0.5, size=c.params()[0].shape).astype(np.float32)
+    c.set_params(table=table0)
+    W0 = c.params()[1]
+    q = _queries(rng, 3001)
+    got0 = c.infer(torch.from_numpy(q).cuda()).cpu().numpy()
+    ps = [(table0, W0)]
+    for _ in range(2):
+        t = rng.uniform(0, 2, (3001, 3)).astype(np.float32)
+        c.train(torch.from_numpy(q).cuda(), torch.from_numpy(t).cuda())
+        ps.append(c.params())
+    (t0, w0), (t1, w1), (t2, w2) = ps
+    te = d * (d * t0 + (1 - d) * t1) + (1 - d) * t2
+    we = d * (d * w0 + (1 - d) * w1) + (1 - d) * w2
+    got = c.infer(torch.from_numpy(q).cuda()).cpu().numpy()
+    o = CacheOracle(CFG["levels"], CFG["log2_table"], CFG["base_resolution"], CFG["max_resolution"], *BOX)
+    _close(got0, np.maximum(o.forward(W0, o.encode(table0, q))[-1][:, :3], 0))
+    _close(got, np.maximum(o.forward(we, o.encode(te, q))[-1][:, :3], 0))
+    # not the raw parameters
+    raw = np.maximum(o.forward(w2, o.encode(t2, q))[-1][:, :3], 0)
+    assert np.abs(got - raw).mean() > 5 * np.abs(got - np.maximum(o.forward(we, o.encode(te, q))[-1][:, :3], 0)).mean()
+    c.close()
+
+
 def test_cache_argument_errors(gpu_ctx):
     from stratum2_amd.cache import RadianceCache
     with pytest.raises(RuntimeError, match="levels"):

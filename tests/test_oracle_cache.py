@@ -1,8 +1,9 @@
 """Pins the radiance-cache oracle (oracle/cache_oracle.py), the checker of the
 GPU cache kernels:
 
-- its pcg equals the C oracle's or_pcg (itself pinned by the reference's
-  golden vectors);
+- its corner index is the direct index x + n (y + n z) where a level's
+  corners fit the table and the published spatial hash elsewhere, checked
+  against hand-computed values;
 - in float64 its backward pass (weight gradients and the hash-table
   gradient) equals central differences of its forward loss with the loss's
   stop-gradient denominators frozen;
@@ -14,8 +15,7 @@ A transposed weight, a dropped ReLU mask, a wrong loss factor or a table
 gradient credited to the wrong corner fails one of these."""
 import numpy as np
 
-from oracle.cache_oracle import CacheOracle, cell_hash, pcg
-from oracle.oracle import lib
+from oracle.cache_oracle import CacheOracle, cell_hash
 
 
 def _queries(rng, n):
@@ -29,14 +29,21 @@ def _queries(rng, n):
     return q
 
 
-def test_pcg_matches_the_c_oracle():
-    rng = np.random.default_rng(0)
-    v = rng.integers(0, 2 ** 32, 2000, dtype=np.uint64)
-    L = lib()
-    want = np.array([L.or_pcg(int(x)) for x in v], np.uint64)
-    assert np.array_equal(pcg(v), want)
-    # negative cell coordinates wrap as two's complement
-    assert cell_hash(-1, 0, 0, 3) == cell_hash(2 ** 32 - 1, 0, 0, 3)
+def test_corner_index():
+    # the spatial hash in uint32 arithmetic, against values worked by hand
+    assert cell_hash(1, 0, 0) == 1
+    assert cell_hash(0, 1, 0) == 2654435761
+    assert cell_hash(0, 0, 1) == 805459861
+    assert cell_hash(3, 2, 0) == (3 ^ ((2 * 2654435761) % 2 ** 32))
+    assert cell_hash(-1, 0, 0) == 2 ** 32 - 1  # negatives wrap as two's complement
+    o = CacheOracle(3, 12, 4, 30, (0, 0, 0), (1, 1, 1))  # res 4, 10, 30: (res + 2)^3 = 216, 1728 fit 4096, 32768 does not
+    assert o.dense == [True, True, False]
+    assert o.corner_index(1, 2, 3, 0) == 1 + 6 * (2 + 6 * 3)
+    assert o.corner_index(5, 0, 11, 1) == 5 + 12 * 12 * 11
+    assert o.corner_index(7, 9, 4, 2) == (7 ^ ((9 * 2654435761) % 2 ** 32) ^ ((4 * 805459861) % 2 ** 32)) % 4096
+    # the x and x + 1 corners of a hashed level sit in the same 16-entry (64-byte) block when x is even
+    for x in range(0, 40, 2):
+        assert o.corner_index(x, 9, 4, 2) // 2 == o.corner_index(x + 1, 9, 4, 2) // 2
 
 
 def test_backward_equals_central_differences_float64():
@@ -88,8 +95,7 @@ def test_encoding_is_trilinear():
     for l in range(3):
         r = int(o.res[l])
         g = np.stack(np.meshgrid(*[np.arange(r + 2)] * 3, indexing="ij"), -1).reshape(-1, 3)
-        h = cell_hash(g[:, 0], g[:, 1], g[:, 2], l).astype(np.int64) & ((1 << 20) - 1)
-        e = l * (1 << 20) + h
+        e = l * (1 << 20) + o.corner_index(g[:, 0], g[:, 1], g[:, 2], l)
         u, cnt = np.unique(e, return_counts=True)
         shared[u[cnt > 1]] = True
         table[e, 0] = g @ a

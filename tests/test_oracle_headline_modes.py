@@ -152,3 +152,18 @@ def test_reuse_is_a_function_of_its_inputs(field):
     a = osc.render(view, p, threads=1, want_path_length=False)["hdr"]
     b = osc.render(view, p, threads=0, want_path_length=False)["hdr"]
     assert np.array_equal(a, b)
+
+
+def test_overflowed_direction_pdf_gives_a_finite_image():
+    """Regression: at 1920x1080, seed 1000184, pixel (1035, 1554) reaches the
+    sky at path length 3 after a GGX sample whose direction pdf overflows the
+    PathState's fp16 store (inf); the emission MIS weight takes its limit 1
+    instead of inf / inf = NaN."""
+    sd = scenes.spheres_field(seed=1)
+    Wf, Hf = 1920, 1080
+    flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI
+    p = abi.default_params(Wf, Hf, max_depth=3, spp=1, seed=1000184, flags=flags, rr_start_depth=3, candidate_samples=32,
+                           max_m=3.0)
+    hdr = OracleScene(sd).render(scenes.spheres_view(Wf, Hf), p, want_path_length=False)["hdr"]
+    assert np.isfinite(hdr).all()
+    assert (hdr[1035, 1554, :3] > 0).all()
