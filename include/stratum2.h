@@ -169,6 +169,22 @@ typedef struct st_view {
  * for bit; no two kernels of the call overlap, so the per-kernel HIP-event
  * times of st_last_kernel_times are each kernel's own (profiling). */
 #define ST_FLAG_SERIAL_WAVES            (1u << 13)
+/* With ST_FLAG_RESTIR_DI: the reference's own reservoir reuse
+ * (gReSTIR_DI_Reuse; SampleIlluminationReSTIR + ReservoirReuse,
+ * testrenderer.slang:608-726; HashGrid, hashgrid.hlsli:14-120).  Every NEE
+ * vertex runs RIS, then looks up the previous sample's world-space hash grid
+ * at a point jittered in a concentric disc of one cell on its tangent plane,
+ * picks one entry of that cell uniformly, combines it with pairwise MIS, and
+ * appends its own reservoir (M clamped at max_m x candidates) to this
+ * sample's grid.  Cells are hashgrid_cell_size wide (0: the reference's 0.5).
+ * The grid of a sample keeps the first two reservoir vertices of each path
+ * (the reference's capacity, W x H x min(2, maxDepth - 1)), and is built
+ * deterministically: entries sorted by cell checksum, stable in pixel and
+ * vertex order, looked up by binary search (the reference's atomic append
+ * order, prefix sum and linear probing are run-dependent).  Samples run one
+ * after another (one sample per wave, one stream); a call's first sample has
+ * no previous grid.  Not with ST_FLAG_RESTIR_DI_REUSE or media. */
+#define ST_FLAG_RESTIR_DI_HASHGRID      (1u << 14)
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface
@@ -191,6 +207,8 @@ typedef struct st_render_params {
     uint32_t samples_per_wave;         /* 0 = automatic: samples traced concurrently */
     float light_subpath_factor;        /* mLightSubpathCount (TestRenderer.hpp:29, default 1): light paths
                                           per sample = max(1, uint(W*H * factor)) (TestRenderer.cpp:452) */
+    float hashgrid_cell_size;          /* ST_FLAG_RESTIR_DI_HASHGRID: cell size in world units (0: 0.5,
+                                          GpuHashGrid(..., 100000, 0.5f), TestRenderer.cpp:52) */
 } st_render_params;
 
 /* Device outputs.  hdr is required; the others may be NULL. */

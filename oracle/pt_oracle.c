@@ -1166,6 +1166,7 @@ typedef struct {
     uint32_t max_depth, min_len, cand, max_null, seed_px, seed_py, seed;
     uint32_t rr_start; /* Russian roulette from this path length on (0, 1: every vertex) */
     int lambert;
+    float max_m;       /* mMaxM (ReSTIR reuse: M clamped at max_m x candidates) */
 } cfg_t;
 
 /* The continuation probability folded into the BSDF / phase pdfs at a vertex
@@ -1924,7 +1925,27 @@ typedef struct {
     float res[8];               /* seed[4], target, W, M, depth */
     struct adj_s* adj;          /* path replay of or_adjoint, or NULL */
     struct cache_s* cache;      /* or_render_cached: this path's cache-depth vertex, or NULL */
+    /* hash-grid reservoir reuse (ST_FLAG_RESTIR_DI_HASHGRID): the previous
+     * sample's grid (NULL for a call's first sample), this sample's, and this
+     * path's pixel */
+    const struct hgrid_s* hg_prev;
+    struct hgrid_s* hg_cur;
+    uint32_t hg_pix;
 } pctx_t;
+
+/* A sample's hash grid (HashGrid<HashGridData>, hashgrid.hlsli:14-120): the
+ * entries of the first hg_dmax reservoir vertices of each path at slot
+ * vertex * W * H + pixel (16 floats: {object-space position, instance |
+ * primitive << 16}, light-sample seed, {target, W, M, cell checksum},
+ * {local incoming direction, 0}; W = 0: empty), then the slots sorted by
+ * (empty, checksum), stable in slot order, with their keys. */
+typedef struct hgrid_s {
+    float* rec;
+    uint64_t* keys;
+    uint32_t* slots;
+    uint32_t nslots, npix, dmax;
+    float cell;
+} hgrid_t;
 
 /* or_render_cached (st_render_cached without the network): a path reaching a
  * scattering vertex at path length `depth` either ends there in a cache query
@@ -2160,8 +2181,86 @@ static void rsd_init(rsd_t* r, pctx_t* x, const material_t* m, const shading_t* 
         r->vtmax = maxf(L.dist * 0.99f, L.dist - 1e-2f);
     }
 }
-/* SampleIlluminationReSTIR (initial RIS, no reuse) + ConnectToLight (ReSTIR),
- * testrenderer.slang:665-684, 730-747 */
+/* HashGrid::FindCellIndex's checksum of the cell holding p (hashgrid.hlsli:51-58) */
+static uint32_t hg_checksum(f3 p, float cs) {
+    const int ix = (int)floorf(p.x / cs), iy = (int)floorf(p.y / cs), iz = (int)floorf(p.z / cs);
+    const uint32_t h = or_xxhash32(or_xxhash32(asuint(cs)) + or_xxhash32((uint32_t)iz + or_xxhash32((uint32_t)iy + or_xxhash32((uint32_t)ix))));
+    return h > 1u ? h : 1u;
+}
+/* first index of keys[0, n) with key >= k (upper: > k) */
+static uint32_t hg_bound(const uint64_t* keys, uint32_t n, uint64_t k, int upper) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (upper ? keys[mid] <= k : keys[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+static float mis_pair(float a, float b);
+/* SampleIlluminationReSTIR's reuse and store (testrenderer.slang:686-723) with
+ * ReservoirReuse (:608-663): a point jittered over one cell on the tangent
+ * plane, one entry of the previous sample's grid in its cell, pairwise MIS;
+ * then this vertex's reservoir into this sample's grid.  A current reservoir
+ * of target 0 gets weight 0 (the reference's 0/0 makes the sum NaN). */
+static void hg_reuse(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin, rsd_t* sample, float* W) {
+    hgrid_t* g = x->hg_cur;
+    float M = (float)x->c->cand;
+    float target = rsd_target(sample);
+    if (x->hg_prev) {
+        const hgrid_t* pg = x->hg_prev;
+        float dr[4], d[2];
+        rng_nextf(&x->ps.rng, dr);
+        sample_concentric_disc(dr[0], dr[1], d);
+        const f3 qp = add(sd->pos, to_world(sd, v3(d[0] * g->cell, d[1] * g->cell, 0)));
+        const uint64_t key = hg_checksum(qp, g->cell);
+        const uint32_t lo = hg_bound(pg->keys, pg->nslots, key, 0), hi = hg_bound(pg->keys, pg->nslots, key, 1);
+        if (hi > lo) {
+            uint32_t u[4];
+            rng_next(&x->ps.rng, u);
+            const float* e = pg->rec + 16 * (size_t)pg->slots[lo + u[0] % (hi - lo)];
+            rsd_t xn;
+            rsd_init(&xn, x, m, sd, ldin, e + 4);
+            const float tn = rsd_target(&xn);
+            float phq = 0;
+            {
+                const uint32_t id = asuint(e[3]);
+                const f3 lp = v3(e[0], e[1], e[2]);
+                shading_t sq;
+                make_tri_shading_x(x->s, id & 0xffffu, id >> 16, 0, 0, &lp, &sq);
+                const material_t mq = load_material(x->s, sq.flags_material >> 4);
+                rsd_t xq;
+                rsd_init(&xq, x, &mq, &sq, v3(e[12], e[13], e[14]), sample->smp);
+                phq = rsd_target(&xq);
+            }
+            const float qM = e[10];
+            const float m0 = mis_pair(M * target, qM * phq), m1 = mis_pair(qM * e[8], M * tn);
+            const float w0 = m0 * target * *W, w1 = m1 * tn * e[9];
+            M += qM;
+            const float total = w0 + w1;
+            float uu[4];
+            rng_nextf(&x->ps.rng, uu);
+            if (uu[0] * total <= w1) {
+                *sample = xn;
+                target = tn;
+            }
+            *W = target > 0 ? total / target : 0.0f;
+        }
+    }
+    if (*W > 0 && x->ps.path_len - 1 < g->dmax) {
+        float* e = g->rec + 16 * ((size_t)(x->ps.path_len - 1) * g->npix + x->hg_pix);
+        const uint32_t id = x->ps.inst_prim;
+        const f3 lp = xf_point(&x->s->inv_xf[id & 0xffffu], sd->pos);
+        e[0] = lp.x; e[1] = lp.y; e[2] = lp.z; e[3] = asfloat(id);
+        memcpy(e + 4, sample->smp, 4 * sizeof(float));
+        const float mmax = x->c->max_m * (float)x->c->cand;
+        e[8] = target; e[9] = *W; e[10] = M < mmax ? M : mmax; e[11] = asfloat(hg_checksum(sd->pos, g->cell));
+        e[12] = ldin.x; e[13] = ldin.y; e[14] = ldin.z; e[15] = 0;
+    }
+}
+
+/* SampleIlluminationReSTIR (initial RIS; with the hash grid, its reuse) +
+ * ConnectToLight (ReSTIR), testrenderer.slang:665-726, 730-747 */
 static void connect_to_light_restir(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
     rsd_t sample, xi;
     memset(&sample, 0, sizeof sample);
@@ -2178,6 +2277,7 @@ static void connect_to_light_restir(pctx_t* x, const material_t* m, const shadin
     }
     float W = 0;
     if (rsd_target(&sample) > 0) W = wsum / ((float)x->c->cand * rsd_target(&sample));
+    if (x->hg_cur) hg_reuse(x, m, sd, ldin, &sample, &W);
     if (W <= 0) return;
     W *= sample.wtoa;
     f3 contrib = mulf(mul(x->ps.thr, sample.contrib), W);
@@ -3457,6 +3557,120 @@ static int check_tri_albedo(const or_scene* s, uint32_t flags) {
 static int render_camera(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads,
                          const uint32_t* cc, float* recs);
 
+/* TestRenderer with gReSTIR_DI_Reuse: samples one after another, each
+ * reusing the previous sample's hash grid (TestRenderer.cpp:595-628: clear,
+ * render, build).  The build sorts a sample's entries by (empty, checksum),
+ * stable in slot order. */
+static int hg_cmp_n;
+static const uint64_t* hg_cmp_keys;
+static int hg_cmp(const void* a, const void* b) {
+    const uint32_t i = *(const uint32_t*)a, j = *(const uint32_t*)b;
+    if (hg_cmp_keys[i] != hg_cmp_keys[j]) return hg_cmp_keys[i] < hg_cmp_keys[j] ? -1 : 1;
+    return i < j ? -1 : (i > j ? 1 : 0);
+}
+static int render_hashgrid(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads,
+                           const cfg_t* c) {
+    const uint32_t W = p->width, H = p->height;
+    const size_t npix = (size_t)W * H;
+    hgrid_t g[2];
+    const uint32_t dmax = p->max_depth - 1 < 2 ? p->max_depth - 1 : 2;
+    const size_t ns = npix * dmax;
+    f3* sum = (f3*)calloc(npix, sizeof(f3));
+    uint64_t* kin = (uint64_t*)calloc(ns, sizeof(uint64_t));
+    int ok = sum && kin;
+    for (int i = 0; i < 2; i++) {
+        g[i].rec = (float*)calloc(ns * 16, sizeof(float));
+        g[i].keys = (uint64_t*)calloc(ns, sizeof(uint64_t));
+        g[i].slots = (uint32_t*)calloc(ns, sizeof(uint32_t));
+        g[i].nslots = (uint32_t)ns;
+        g[i].npix = (uint32_t)npix;
+        g[i].dmax = dmax;
+        g[i].cell = p->hashgrid_cell_size > 0 ? p->hashgrid_cell_size : 0.5f;
+        ok = ok && g[i].rec && g[i].keys && g[i].slots;
+    }
+    if (!ok) {
+        free(sum); free(kin);
+        for (int i = 0; i < 2; i++) { free(g[i].rec); free(g[i].keys); free(g[i].slots); }
+        return fail("out of memory");
+    }
+    const uint32_t view_medium = camera_medium(s, view);
+    const st_transform inv_cam = inverse_xf(&view->camera_to_world);
+    uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    for (uint32_t k = 0; k < p->spp; k++) {
+        hgrid_t* cur = &g[k & 1];
+        const hgrid_t* prev = k > 0 ? &g[(k + 1) & 1] : NULL;
+        memset(cur->rec, 0, ns * 16 * sizeof(float));
+#pragma omp parallel
+        {
+            pctx_t x;
+            memset(&x, 0, sizeof x);
+            x.s = s;
+            x.c = c;
+            x.view_medium = view_medium;
+            x.view = view;
+            x.inv_cam = inv_cam;
+            x.W = W;
+            x.H = H;
+            x.hg_prev = prev;
+            x.hg_cur = cur;
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
+                const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
+                x.px = px;
+                x.py = py;
+                x.hg_pix = (uint32_t)pix;
+                f3 r;
+                uint32_t pl;
+                render_pixel(&x, p, view, px, py, p->seed + k, &r, &pl);
+                if (k == 0 && out->visibility) { out->visibility[2 * pix] = x.vis[0]; out->visibility[2 * pix + 1] = x.vis[1]; }
+                if (k == 0 && out->albedo) {
+                    for (int j = 0; j < 3; j++) out->albedo[4 * pix + j] = x.alb[j];
+                    out->albedo[4 * pix + 3] = 1;
+                }
+                if (k == 0 && out->depth) {
+                    out->depth[4 * pix] = x.dep[0]; out->depth[4 * pix + 1] = x.dep[1];
+                    out->depth[4 * pix + 2] = 1; out->depth[4 * pix + 3] = 1;
+                }
+                if (k == 0 && out->prev_uv) { out->prev_uv[2 * pix] = x.uv[0]; out->prev_uv[2 * pix + 1] = x.uv[1]; }
+                if (out->path_length) out->path_length[(size_t)k * npix + pix] = pl;
+                sum[pix] = add(sum[pix], r);
+            }
+#pragma omp critical
+            {
+                rays[0] += x.rays[0]; rays[1] += x.rays[1];
+                for (int j = 0; j < 4; j++) trav[j] += x.trav[j];
+            }
+        }
+        /* HashGrid build (GpuHashGrid::build), deterministic */
+        for (size_t i = 0; i < ns; i++) {
+            const float* e = cur->rec + 16 * i;
+            kin[i] = e[9] > 0 ? (uint64_t)asuint(e[11]) : (1ull << 32);
+            cur->slots[i] = (uint32_t)i;
+        }
+        hg_cmp_keys = kin;
+        hg_cmp_n = (int)ns;
+        qsort(cur->slots, ns, sizeof(uint32_t), hg_cmp);
+        for (size_t i = 0; i < ns; i++) cur->keys[i] = kin[cur->slots[i]];
+    }
+    if (out->hdr)
+        for (size_t i = 0; i < npix; i++) {
+            out->hdr[4 * i + 0] = sum[i].x / (float)p->spp;
+            out->hdr[4 * i + 1] = sum[i].y / (float)p->spp;
+            out->hdr[4 * i + 2] = sum[i].z / (float)p->spp;
+            out->hdr[4 * i + 3] = 1;
+        }
+    if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
+    if (out->traversal) for (int j = 0; j < 4; j++) out->traversal[j] = trav[j];
+    free(sum); free(kin);
+    for (int i = 0; i < 2; i++) { free(g[i].rec); free(g[i].keys); free(g[i].slots); }
+    return 0;
+}
+
 int or_render(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads) {
     return render_camera(s, view, p, out, threads, NULL, NULL);
 }
@@ -3496,7 +3710,14 @@ static int render_camera(or_scene* s, const st_view* view, const st_render_param
     c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
     c.max_null = p->max_null_collisions;
     c.rr_start = p->rr_start_depth;
+    c.max_m = p->max_m;
     if (reuse) return render_reuse(s, view, p, out, threads, &c, cc, recs);
+    if (p->flags & ST_FLAG_RESTIR_DI_HASHGRID) {
+        if (!(p->flags & ST_FLAG_RESTIR_DI)) return fail("ST_FLAG_RESTIR_DI_HASHGRID needs ST_FLAG_RESTIR_DI");
+        if (cc) return fail("hash-grid reuse with the radiance cache is not restated");
+        if (p->max_depth < 2) return fail("hash-grid reuse needs max_depth >= 2");
+        return render_hashgrid(s, view, p, out, threads, &c);
+    }
     const uint32_t view_medium = camera_medium(s, view);
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};

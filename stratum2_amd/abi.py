@@ -33,6 +33,7 @@ ST_FLAG_TRACE_FROM_LIGHT = 1 << 10  # TestRenderer light tracing (gTraceFromLigh
 ST_FLAG_BDPT_LVC = 1 << 11  # TestRenderer bidirectional mode with a light vertex cache (gUseVC)
 ST_FLAG_VC_FP32_MIS = 1 << 12  # dVC / dVCM in fp32 instead of the reference's fp16
 ST_FLAG_SERIAL_WAVES = 1 << 13  # waves one after another on one stream (exclusive per-kernel times)
+ST_FLAG_RESTIR_DI_HASHGRID = 1 << 14  # the reference's hash-grid reservoir reuse (gReSTIR_DI_Reuse)
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 # TonemapMode (src/Shaders/compat/tonemap.h:10-23)
@@ -96,7 +97,7 @@ class st_render_params(C.Structure):
         ("environment_sample_probability", f32), ("candidate_samples", u32), ("max_m", f32),
         ("flags", u32), ("rr_start_depth", u32), ("exposure", f32),
         ("spatial_neighbours", u32), ("spatial_radius", f32), ("samples_per_wave", u32),
-        ("light_subpath_factor", f32),
+        ("light_subpath_factor", f32), ("hashgrid_cell_size", f32),
     ]
 
 

@@ -50,6 +50,19 @@ __device__ __forceinline__ float lerp(float a, float b, float t) { return a + (b
 __device__ __forceinline__ float comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 
 // ---------------------------------------------------------------- rng
+// xxhash32 and pcg, src/Shaders/common/rng.hlsli:5-21
+__device__ __forceinline__ uint32_t xxhash32(uint32_t p) {
+    uint32_t h = p + 374761393u;
+    h = 668265263u * ((h << 17) | (h >> 15));
+    h = 2246822519u * (h ^ (h >> 15));
+    h = 3266489917u * (h ^ (h >> 13));
+    return h ^ (h >> 16);
+}
+__device__ __forceinline__ uint32_t pcg1(uint32_t v) {
+    const uint32_t s = v * 747796405u + 2891336453u;
+    const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
 // pcg4d, src/Shaders/common/rng.hlsli:23-35
 __device__ __forceinline__ uint4 pcg4d(uint4 v) {
     v.x = v.x * 1664525u + 1013904223u;

@@ -108,6 +108,14 @@ struct WaveState {
     float* tq;
     float4* tthr;
     float4* tt;
+    // hash-grid reservoir reuse: this sample's entries ({local pos, id},
+    // seed, {target, W, M, checksum}, {local dir in, 0}) at slot
+    // vertex * W * H + pixel, and the previous sample's grid (its entries,
+    // and their slots sorted by cell key)
+    float4* hg_rec;
+    const float4* hg_prev_rec;
+    const unsigned long long* hg_prev_keys;
+    const uint32_t* hg_prev_slots;
 };
 
 struct RenderConsts {
@@ -116,6 +124,8 @@ struct RenderConsts {
     float max_m, radius;
     uint32_t view_medium, max_null;
     uint32_t cache_depth, train_stride, train_frame;  // radiance cache: query depth, training-path selection
+    float hg_cell;                                    // hash-grid reservoir reuse: cell size,
+    uint32_t hg_dmax, hg_prev, hg_nslots;             // reservoir vertices kept per path, previous grid valid, slots
     uint32_t light_paths;  // light tracing and the bidirectional mode: light subpaths per sample (mLightSubpathCount)
     uint32_t lv_stride;    // bidirectional mode: light vertex slots per subpath (max_depth - 1)
     float ipd;             // light tracing: ViewData::imagePlaneDist (compat/scene.h:147)
@@ -718,6 +728,102 @@ __device__ __forceinline__ bool eval_candidate(const DevScene& S, const RenderCo
     return true;
 }
 
+__device__ __forceinline__ float mis_pair(float a, float b) {  // mis(a, b), testrenderer.slang:287-290
+    const float a2 = a * a, d = a2 + b * b;
+    return d > 0 ? a2 / d : 0.0f;
+}
+
+// ------------------------------------------------ hash-grid reservoir reuse
+// HashGrid::FindCellIndex's checksum of the cell holding p (hashgrid.hlsli:
+// 51-58): the cell's identity in the grid
+__device__ __forceinline__ uint32_t hg_checksum(v3 p, float cs) {
+    const int x = (int)floorf(p.x / cs), y = (int)floorf(p.y / cs), z = (int)floorf(p.z / cs);
+    const uint32_t h = xxhash32(xxhash32(__float_as_uint(cs)) + xxhash32((uint32_t)z + xxhash32((uint32_t)y + xxhash32((uint32_t)x))));
+    return h > 1u ? h : 1u;
+}
+// first index in keys[0, n) whose key is >= k (upper: > k)
+__device__ __forceinline__ uint32_t hg_bound(const unsigned long long* keys, uint32_t n, unsigned long long k, bool upper) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const unsigned long long v = keys[mid];
+        if (upper ? v <= k : v < k) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+// The vertex of a hash-grid reuse: where its entry goes and what it holds
+struct HgVertex {
+    const WaveState* w;
+    uint32_t pix, path_len, id;  // linear pixel, path length, instance | primitive << 16
+};
+// Reuse from the previous sample's grid and append to this sample's
+// (SampleIlluminationReSTIR's reuse and store, testrenderer.slang:686-723;
+// ReservoirReuse, :608-663).  sel / sel_target / sel_seed / W: the vertex's
+// reservoir after RIS, updated in place.
+template <bool LAMBERT, bool ALIAS>
+__device__ inline void hash_grid_reuse(const DevScene& S, const RenderConsts& c, const HgVertex& h, Rng& rng, const Material& m,
+                                       const Frame& f, v3 lgn, v3 ldin, v3 pos, float cp_eff, Cand& sel, float& sel_target,
+                                       float4& sel_seed, float& W) {
+    float M = (float)c.cand;
+    if (c.hg_prev) {
+        // a point jittered over one cell on the tangent plane, one entry of its cell
+        const float4 dr = rng.nextf();
+        float dx, dy;
+        concentric_disc(dr.x, dr.y, dx, dy);
+        const v3 qp = pos + f.to_world(mk(dx * c.hg_cell, dy * c.hg_cell, 0.0f));
+        const unsigned long long key = hg_checksum(qp, c.hg_cell);
+        const uint32_t lo = hg_bound(h.w->hg_prev_keys, c.hg_nslots, key, false);
+        const uint32_t hi = hg_bound(h.w->hg_prev_keys, c.hg_nslots, key, true);
+        if (hi > lo) {
+            const uint32_t slot = h.w->hg_prev_slots[lo + rng.next().x % (hi - lo)];
+            const float4* e = h.w->hg_prev_rec + 4 * (size_t)slot;
+            const float4 e0 = e[0], seed = e[1], e2 = e[2], e3 = e[3];
+            // the entry's light sample here
+            Cand xn;
+            float lrn;
+            const bool okn = eval_candidate<LAMBERT, ALIAS>(S, c, m, f, lgn, ldin, pos, seed, cp_eff, xn, lrn);
+            const float tn = (okn && lrn > 0) ? lum(xn.contrib) * xn.wtoa : 0.0f;
+            // this vertex's light sample at the entry's point (HashGridData::getShadingData)
+            float phq = 0;
+            {
+                const uint32_t id = __float_as_uint(e0.w);
+                const v3 lp = mk(e0.x, e0.y, e0.z);
+                Shading sq;
+                tri_shading(S, id & 0xffffu, id >> 16, 0.0f, 0.0f, sq, &lp);
+                const Material mq = load_material(S, sq.material);
+                const Frame fq = make_frame(sq);
+                Cand xq;
+                float lrq;
+                const bool okq = eval_candidate<LAMBERT, ALIAS>(S, c, mq, fq, fq.to_local(unpack_normal(sq.gn)), mk(e3.x, e3.y, e3.z),
+                                                                sq.pos, sel_seed, 1.0f, xq, lrq);
+                if (okq && lrq > 0) phq = lum(xq.contrib) * xq.wtoa;
+            }
+            // pairwise MIS (a current reservoir of target 0 weighs 0 here, where the
+            // reference's 0/0 makes the sum NaN and drops the entry)
+            const float qM = e2.z;
+            const float m0 = mis_pair(M * sel_target, qM * phq), m1 = mis_pair(qM * e2.x, M * tn);
+            const float w0 = m0 * sel_target * W, w1 = m1 * tn * e2.y;
+            M += qM;
+            const float total = w0 + w1;
+            if (rng.nextf().x * total <= w1) {
+                sel = xn;
+                sel_target = tn;
+                sel_seed = seed;
+            }
+            W = sel_target > 0 ? total / sel_target : 0.0f;
+        }
+    }
+    if (W > 0 && h.path_len - 1 < c.hg_dmax) {
+        float4* e = h.w->hg_rec + 4 * ((size_t)(h.path_len - 1) * c.npix + h.pix);
+        const v3 lp = xf_point(S.inv_xf[h.id & 0xffffu], pos);
+        e[0] = make_float4(lp.x, lp.y, lp.z, __uint_as_float(h.id));
+        e[1] = sel_seed;
+        e[2] = make_float4(sel_target, W, fmin_(M, c.max_m * (float)c.cand), __uint_as_float(hg_checksum(pos, c.hg_cell)));
+        e[3] = make_float4(ldin.x, ldin.y, ldin.z, 0.0f);
+    }
+}
+
 // Next-event estimation at a surface vertex.  Plain: one light sample
 // (ConnectToLight, testrenderer.slang:751-779).  RESTIR: resampled importance
 // sampling over c.cand candidates (SampleIlluminationReSTIR, :665-684; the
@@ -725,7 +831,7 @@ __device__ __forceinline__ bool eval_candidate(const DevScene& S, const RenderCo
 template <bool LAMBERT, bool ALIAS, bool RESTIR>
 __device__ __forceinline__ bool next_event(const DevScene& S, const RenderConsts& c, Rng& rng, const Material& m,
                                            const Frame& f, v3 lgn, v3 ldin, v3 pos, uint32_t gn, v3 thr, float cp_eff,
-                                           float4& sh0, float4& sh1, float4& sh2, uint32_t p) {
+                                           float4& sh0, float4& sh1, float4& sh2, uint32_t p, const HgVertex* hg = nullptr) {
     Cand sel;
     float wgt_out;
     v3 contrib;
@@ -745,6 +851,7 @@ __device__ __forceinline__ bool next_event(const DevScene& S, const RenderConsts
         sel.dist = L.dist;
     } else {
         float wsum = 0, sel_target = 0;
+        float4 sel_seed = make_float4(0, 0, 0, 0);
         sel.contrib = mk(0, 0, 0);
         sel.dir = mk(0, 0, 0);
         sel.dist = 0;
@@ -754,16 +861,19 @@ __device__ __forceinline__ bool next_event(const DevScene& S, const RenderConsts
         for (uint32_t i = 0; i < c.cand; i++) {
             Cand x;
             float lr;
-            const bool ok = eval_candidate<LAMBERT, ALIAS>(S, c, m, f, lgn, ldin, pos, rng.nextf(), cp_eff, x, lr);
+            const float4 rnd = rng.nextf();
+            const bool ok = eval_candidate<LAMBERT, ALIAS>(S, c, m, f, lgn, ldin, pos, rnd, cp_eff, x, lr);
             const float target = (ok && lr > 0) ? lum(x.contrib) * x.wtoa : 0.0f;
             if (target <= 0) continue;
             wsum += target / x.src_pdf;
             if (rng.nextf().x <= (target / x.src_pdf) / wsum) {
                 sel = x;
                 sel_target = target;
+                sel_seed = rnd;
             }
         }
         float W = sel_target > 0 ? wsum / ((float)c.cand * sel_target) : 0.0f;
+        if (hg) hash_grid_reuse<LAMBERT, ALIAS>(S, c, *hg, rng, m, f, lgn, ldin, pos, cp_eff, sel, sel_target, sel_seed, W);
         if (W <= 0) return false;
         W *= sel.wtoa;
         contrib = (thr * sel.contrib) * W;
@@ -832,10 +942,6 @@ __device__ __forceinline__ float seed_target(const DevScene& S, const RenderCons
     const float cp = 1 >= c.rr_start ? m_cont_prob(P.m) : 1.0f;  // primary vertex: path length 1
     const bool ok = eval_candidate<LAMBERT, ALIAS>(S, c, P.m, P.f, P.lgn, P.ldin, P.sd.pos, seed, cp, x, lr);
     return (ok && lr > 0) ? lum(x.contrib) * x.wtoa : 0.0f;
-}
-__device__ __forceinline__ float mis_pair(float a, float b) {  // mis(a, b), testrenderer.slang:287-290
-    const float a2 = a * a, d = a2 + b * b;
-    return d > 0 ? a2 / d : 0.0f;
 }
 
 // RIS at the primary vertices of the screen-space reuse mode
@@ -1110,8 +1216,10 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 if (!reuse && alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
                     // the screen-space reuse mode keeps reservoirs at primary vertices only
                     const bool ris_here = RESTIR && !(c.flags & ST_FLAG_RESTIR_DI_REUSE);
+                    const HgVertex hgv{&w, pix, path_len, hit.y};
+                    const HgVertex* hgp = (RESTIR && (c.flags & ST_FLAG_RESTIR_DI_HASHGRID)) ? &hgv : nullptr;
                     emit_shadow = (ris_here ? next_event<LAMBERT, ALIAS, true>(S, c, rng, m, f, lgn, ldin, sd.pos, sd.gn, thr,
-                                                                              cp_eff, sh0, sh1, sh2, p)
+                                                                              cp_eff, sh0, sh1, sh2, p, hgp)
                                             : next_event<LAMBERT, ALIAS, false>(S, c, rng, m, f, lgn, ldin, sd.pos, sd.gn, thr,
                                                                                cp_eff, sh0, sh1, sh2, p)) &&
                                   path_len + 1 >= c.min_len;
@@ -1849,6 +1957,16 @@ __global__ void k_rc_targets(RenderConsts c, WaveState w, uint32_t row_lo, uint3
     }
 }
 
+// Sort keys of a sample's hash-grid entries: the cell checksum of each entry
+// with weight, 2^32 for empty slots (sorted last); values: the slots.
+__global__ void k_hg_keys(const float4* rec, uint32_t n, unsigned long long* keys, uint32_t* slots) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 e2 = rec[4 * (size_t)i + 2];
+        keys[i] = e2.y > 0 ? (unsigned long long)__float_as_uint(e2.w) : (1ull << 32);
+        slots[i] = i;
+    }
+}
+
 // st_render_cached's training rows with weight, moved in row order to the
 // front of dense arrays (queries, target rgb): flags for the scan, then the
 // move; the last thread writes the count.
@@ -2092,6 +2210,8 @@ struct st_context {
     hipEvent_t entry = nullptr;          // the caller's null-stream work at a call's entry
     uint8_t* scratch = nullptr;          // this device's fixed scratch (device address of `scratch`)
     DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
+    // hash-grid reservoir reuse: two samples' entries and sorted keys / slots
+    DevBuf hg_rec[2], hg_keys[2], hg_slots[2], hg_keys_tmp, hg_slots_tmp, hg_sort_tmp;
     double timings[12] = {0};
     double cache_queries = 0;            // paths that ended in the radiance cache in the last call
     double cache_rows = 0;               // its training rows with weight 1 (when its loss was asked for)
@@ -2968,6 +3088,15 @@ static int check_camera(st_scene* s, const st_render_params* p) {
     if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
     if (s->d.vinst_count > 0 && (p->flags & ST_FLAG_RESTIR_DI))
         return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI is not supported in scenes with media");
+    if (p->flags & ST_FLAG_RESTIR_DI_HASHGRID) {
+        if (!(p->flags & ST_FLAG_RESTIR_DI) || (p->flags & ST_FLAG_RESTIR_DI_REUSE))
+            return set_error(ST_ERROR_INVALID_ARGUMENT, "ST_FLAG_RESTIR_DI_HASHGRID needs ST_FLAG_RESTIR_DI and excludes ST_FLAG_RESTIR_DI_REUSE");
+        if (!(p->hashgrid_cell_size >= 0) || p->hashgrid_cell_size > 1e30f)
+            return set_error(ST_ERROR_INVALID_ARGUMENT, "hashgrid_cell_size must be a finite value >= 0");
+        if (p->max_depth < 2) return set_error(ST_ERROR_INVALID_ARGUMENT, "hash-grid reuse needs max_depth >= 2");
+        // two entries of 16 words per pixel, cleared by one 32-bit count
+        if ((uint64_t)p->width * p->height * 32 >= (1ull << 32)) return set_error(ST_ERROR_LIMIT, "image too large for the hash grid");
+    }
     return check_triangle_albedo(s, p->flags);
 }
 
@@ -3014,6 +3143,7 @@ struct CameraJob {
     bool timed;                      // HIP-event timings of the kernel classes
     bool store_rays;                 // write the ray counts to o.ray_counts on the device
     const CacheJob* cache;           // st_render_cached, else null
+    bool hashgrid;                   // ST_FLAG_RESTIR_DI_HASHGRID: one sample per wave, one slot
 };
 
 
@@ -3084,6 +3214,11 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
         c.train_stride = cj->stride;
         c.train_frame = cj->frame;
     }
+    if (j.hashgrid) {
+        c.hg_cell = p->hashgrid_cell_size > 0 ? p->hashgrid_cell_size : 0.5f;
+        c.hg_dmax = std::min<uint32_t>(2, p->max_depth - 1);
+        c.hg_nslots = npix * c.hg_dmax;
+    }
     const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
     const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
     const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
@@ -3105,6 +3240,15 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
             w.tq = cj->tq;
             w.tthr = cj->tthr;
             w.tt = cj->tt;
+        }
+        const int hg_cur = wi & 1;
+        if (j.hashgrid) {  // this sample's grid, and the previous sample's to reuse from
+            c.hg_prev = base > 0 ? 1u : 0u;
+            w.hg_rec = ctx->hg_rec[hg_cur].as<float4>();
+            w.hg_prev_rec = ctx->hg_rec[hg_cur ^ 1].as<float4>();
+            w.hg_prev_keys = ctx->hg_keys[hg_cur ^ 1].as<unsigned long long>();
+            w.hg_prev_slots = ctx->hg_slots[hg_cur ^ 1].as<uint32_t>();
+            hipLaunchKernelGGL(k_fill_u32, dim3(ctx->cus * 8), dim3(256), 0, sm, ctx->hg_rec[hg_cur].as<uint32_t>(), c.hg_nslots * 16, 0u);
         }
         uint32_t* spill = B.spill.as<uint32_t>();
         c.k = std::min(k, p->spp - base);
@@ -3168,6 +3312,15 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
             hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
             qsel ^= 1;
         }
+        if (j.hashgrid) {  // this sample's grid: entries sorted by cell, stable in slot order
+            const int hgrid = std::min<int>((int)((c.hg_nslots + 255) / 256), ctx->cus * 8);
+            hipLaunchKernelGGL(k_hg_keys, dim3(hgrid), dim3(256), 0, sm, w.hg_rec, c.hg_nslots, ctx->hg_keys_tmp.as<unsigned long long>(),
+                               ctx->hg_slots_tmp.as<uint32_t>());
+            size_t sb = ctx->hg_sort_tmp.n;
+            HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(ctx->hg_sort_tmp.p, sb, ctx->hg_keys_tmp.as<unsigned long long>(),
+                                                         ctx->hg_keys[hg_cur].as<unsigned long long>(), ctx->hg_slots_tmp.as<uint32_t>(),
+                                                         ctx->hg_slots[hg_cur].as<uint32_t>(), (int)c.hg_nslots, 0, 33, sm));
+        }
         if (cj) {  // training targets of the rows whose path is in this wave
             const uint64_t g0 = (uint64_t)base * npix, g1 = g0 + (uint64_t)c.k * npix;
             const uint32_t lo = (uint32_t)(g0 / cj->stride), hi = (uint32_t)std::min<uint64_t>(cj->rows, (g1 - 1) / cj->stride + 1);
@@ -3201,14 +3354,30 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         HIP_CHECK(hipSetDevice(ctx->device));
         const uint32_t npix = p->width * p->height;
         const bool reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
-        const uint32_t k = wave_samples(p, p->samples_per_wave);
+        const bool hashgrid = (p->flags & ST_FLAG_RESTIR_DI_HASHGRID) != 0;
+        // hash-grid reuse chains the samples: one per wave, one after another
+        const uint32_t k = hashgrid ? 1 : wave_samples(p, p->samples_per_wave);
         if (k == 0) return set_error(ST_ERROR_LIMIT, "too many paths per wave");
         const uint64_t paths = (uint64_t)npix * k;
         const uint32_t waves = (p->spp + k - 1) / k;
-        const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) ? 1 : (int)std::min<uint32_t>(kWaveSlots, waves);
+        const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) || hashgrid ? 1 : (int)std::min<uint32_t>(kWaveSlots, waves);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         order_after_caller(ctx);
         for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, reuse, paths, tgrid);
+        if (hashgrid) {
+            const uint64_t ns = (uint64_t)npix * std::min<uint32_t>(2, p->max_depth - 1);
+            for (int i = 0; i < 2; i++) {
+                ctx->hg_rec[i].reserve(ns * 64);
+                ctx->hg_keys[i].reserve(ns * 8);
+                ctx->hg_slots[i].reserve(ns * 4);
+            }
+            ctx->hg_keys_tmp.reserve(ns * 8);
+            ctx->hg_slots_tmp.reserve(ns * 4);
+            size_t sb = 0;
+            HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, (unsigned long long*)nullptr, (unsigned long long*)nullptr,
+                                                         (uint32_t*)nullptr, (uint32_t*)nullptr, (int)ns, 0, 33));
+            ctx->hg_sort_tmp.reserve(std::max<size_t>(sb, 16));
+        }
         ctx->acc.reserve((size_t)npix * 16);
         if (reuse) ctx->rprev.reserve((size_t)npix * 32);
         ctx->stats.reserve(64);
@@ -3231,6 +3400,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         j.tgrid = tgrid;
         j.sgrid = sgrid;
         j.timed = true;
+        j.hashgrid = hashgrid;
         enqueue_camera(ctx, j);
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
         finish_timings(ctx, slots, o->ray_counts);
@@ -4010,6 +4180,8 @@ int st_batch_create(st_context* ctx, const st_render_request* requests, uint32_t
         if (p.width == 0 || p.height == 0 || p.spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render in a request");
         if (p.flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC))
             return set_error(ST_ERROR_UNSUPPORTED, "batches run camera paths (no light tracing or bidirectional mode)");
+        if (p.flags & ST_FLAG_RESTIR_DI_HASHGRID)
+            return set_error(ST_ERROR_UNSUPPORTED, "hash-grid reuse chains a request's samples through the context's grids: not in batches");
         const int rc = check_camera(R.scene, &p);
         if (rc != ST_OK) return rc;
     }

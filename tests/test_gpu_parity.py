@@ -253,6 +253,52 @@ def test_restir_di_candidates(gpu_ctx):
     _check_image(g, r)
 
 
+@pytest.mark.parametrize("case", ["direct", "paths", "cornell", "small_cells"])
+def test_restir_hashgrid_reuse_vs_oracle(gpu_ctx, case):
+    """The reference's hash-grid reservoir reuse (gReSTIR_DI_Reuse): every NEE
+    vertex runs RIS, reuses one entry of the previous sample's world-space
+    grid and appends to this sample's; the grid is built deterministically,
+    so the GPU and the oracle agree at the per-pixel bar over a chain of
+    samples: direct light, whole paths (RIS at every vertex, two entries per
+    path), the Disney Cornell box, and cells of 0.05 (more cells, fewer
+    entries each)."""
+    flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_HASHGRID
+    if case == "cornell":
+        sd = scenes.cornell_box(0)
+        W = H = 64
+        view = scenes.cornell_view(W, H)
+        params = abi.default_params(W, H, max_depth=4, spp=4, flags=flags, max_m=3.0)
+    else:
+        sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+        W, H = 96, 64
+        view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+        flags |= abi.ST_FLAG_ENV_ALIAS
+        if case == "direct":
+            flags |= abi.ST_FLAG_DIRECT_ONLY
+        params = abi.default_params(W, H, max_depth=2 if case == "direct" else 4, spp=4, flags=flags, max_m=3.0,
+                                    hashgrid_cell_size=0.05 if case == "small_cells" else 0.0)
+    print(f"case {case}", flush=True)
+    g, r = _render_both(gpu_ctx, sd, view, params)
+    _check_image(g, r)
+
+
+def test_restir_hashgrid_argument_checks(gpu_ctx):
+    from stratum2_amd.renderer import Batch, Scene, TestRenderer
+    sd = scenes.cornell_box(0)
+    sc = Scene(gpu_ctx, sd)
+    view = scenes.cornell_view(16, 16)
+    bad = [abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI_HASHGRID,
+           abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE | abi.ST_FLAG_RESTIR_DI_HASHGRID]
+    for fl in bad:
+        with pytest.raises(RuntimeError):
+            TestRenderer(sc).render(view, abi.default_params(16, 16, max_depth=3, spp=1, flags=fl))
+    ok = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_HASHGRID
+    with pytest.raises(RuntimeError):
+        TestRenderer(sc).render(view, abi.default_params(16, 16, max_depth=3, spp=1, flags=ok, hashgrid_cell_size=-1.0))
+    with pytest.raises(RuntimeError, match="batches"):
+        Batch(gpu_ctx, [(sc, view, abi.default_params(16, 16, max_depth=3, spp=1, flags=ok))])
+
+
 def _gpu_render(gpu_ctx, sd, view, params):
     import torch
     from stratum2_amd.renderer import Scene, TestRenderer

@@ -167,3 +167,45 @@ def test_overflowed_direction_pdf_gives_a_finite_image():
     hdr = OracleScene(sd).render(scenes.spheres_view(Wf, Hf), p, want_path_length=False)["hdr"]
     assert np.isfinite(hdr).all()
     assert (hdr[1035, 1554, :3] > 0).all()
+
+
+HG = abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_HASHGRID
+
+
+def test_hashgrid_reuse_keeps_direct_light_expectation(field):
+    """The reference's hash-grid reuse (each sample reuses one entry of the
+    previous sample's world-space grid; pairwise MIS) estimates the same
+    direct light as RIS alone and one-sample NEE."""
+    osc, view = field
+    d = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_DIRECT_ONLY | abi.ST_FLAG_ENV_ALIAS
+    ris = image_means(osc, view, 8, 8, max_depth=2, flags=d | abi.ST_FLAG_RESTIR_DI)
+    hg = image_means(osc, view, 8, 8, max_depth=2, flags=d | HG, max_m=3.0)
+    nee = image_means(osc, view, 8, 8, max_depth=2, flags=d)
+    assert same_expectation(ris, hg)
+    assert same_expectation(nee, hg)
+
+
+def test_hashgrid_reuse_keeps_path_expectation(field):
+    """The same for whole paths (maxPathLength 4, roulette everywhere): every
+    NEE vertex runs RIS, reuses from the previous sample's grid and appends
+    (the first two vertices of each path)."""
+    osc, view = field
+    f = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
+    ris = image_means(osc, view, 8, 8, max_depth=4, flags=f | abi.ST_FLAG_RESTIR_DI)
+    hg = image_means(osc, view, 8, 8, max_depth=4, flags=f | HG, max_m=3.0)
+    assert same_expectation(ris, hg)
+
+
+def test_hashgrid_reuse_is_deterministic(field):
+    """Two renders with the same seeds are equal bit for bit (the grid is
+    built by a stable sort, not in atomic order); a call's first sample, with
+    no previous grid, is RIS alone; later samples reuse (the image differs)."""
+    osc, view = field
+    d = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_DIRECT_ONLY | abi.ST_FLAG_ENV_ALIAS
+    mk = lambda fl, spp, seed=7: abi.default_params(W, H, spp=spp, seed=seed, max_depth=2, flags=fl, max_m=3.0)  # noqa: E731
+    a = osc.render(view, mk(d | HG, 8))
+    b = osc.render(view, mk(d | HG, 8))
+    assert np.array_equal(a["hdr"], b["hdr"]) and np.array_equal(a["path_length"], b["path_length"])
+    one = osc.render(view, mk(d | HG, 1))["hdr"]
+    assert np.array_equal(one, osc.render(view, mk(d | abi.ST_FLAG_RESTIR_DI, 1))["hdr"])
+    assert not np.array_equal(a["hdr"], osc.render(view, mk(d | abi.ST_FLAG_RESTIR_DI, 8))["hdr"])
