@@ -182,9 +182,16 @@ typedef struct st_view {
  * deterministically: entries sorted by cell checksum, stable in pixel and
  * vertex order, looked up by binary search (the reference's atomic append
  * order, prefix sum and linear probing are run-dependent).  Samples run one
- * after another (one sample per wave, one stream); a call's first sample has
- * no previous grid.  Not with ST_FLAG_RESTIR_DI_REUSE or media. */
+ * after another (one sample per wave, one stream).  As TestRenderer keeps
+ * mPrevHashGrid from frame to frame (TestRenderer.cpp:472-493), a call's
+ * first sample reuses the grid of the previous call's last sample when that
+ * call, on the same context, was the last st_render, used this flag, and had
+ * the same scene and W x H x min(2, maxDepth - 1); otherwise it has none.
+ * Not with ST_FLAG_RESTIR_DI_REUSE or media. */
 #define ST_FLAG_RESTIR_DI_HASHGRID      (1u << 14)
+/* Drop the history kept from earlier calls on the context (the hash grid of
+ * ST_FLAG_RESTIR_DI_HASHGRID), as at a camera cut. */
+#define ST_FLAG_HISTORY_RESET           (1u << 15)
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface

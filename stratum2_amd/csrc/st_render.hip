@@ -20,6 +20,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -2212,6 +2213,12 @@ struct st_context {
     DevBuf acc, stats, raysbuf, outbuf, rprev, tmax;
     // hash-grid reservoir reuse: two samples' entries and sorted keys / slots
     DevBuf hg_rec[2], hg_keys[2], hg_slots[2], hg_keys_tmp, hg_slots_tmp, hg_sort_tmp;
+    // the grid the last sample of the previous hash-grid st_render built: its
+    // buffer (hg_last), scene and entry count; valid until a st_render without
+    // the flag, a failed call or ST_FLAG_HISTORY_RESET
+    bool hg_valid = false;
+    int hg_last = 0;
+    uint64_t hg_scene = 0, hg_ns = 0;
     double timings[12] = {0};
     double cache_queries = 0;            // paths that ended in the radiance cache in the last call
     double cache_rows = 0;               // its training rows with weight 1 (when its loss was asked for)
@@ -2234,6 +2241,7 @@ struct st_scene {
     std::vector<st_transform> inv_xf;
     uint64_t stats[4] = {0, 0, 0, 0};
     uint64_t generation = 0;  // bumped by st_scene_update: device pointers and constants may have changed
+    uint64_t serial = 0;      // unique per created scene (a kept hash grid names its scene by it)
     // for scene updates: per-instance object-space bounds (the scene sphere),
     // the flattened triangles' owners, and the BVH buffers (replaced on update)
     struct InstBox {
@@ -2603,6 +2611,8 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
         validate_desc(d);
         HIP_CHECK(hipSetDevice(ctx->device));
         auto s = std::make_unique<st_scene>();
+        static std::atomic<uint64_t> next_serial{1};
+        s->serial = next_serial++;
         s->ctx = ctx;
         DevScene& D = s->d;
         // reference-format buffers
@@ -3144,6 +3154,8 @@ struct CameraJob {
     bool store_rays;                 // write the ray counts to o.ray_counts on the device
     const CacheJob* cache;           // st_render_cached, else null
     bool hashgrid;                   // ST_FLAG_RESTIR_DI_HASHGRID: one sample per wave, one slot
+    int hg_first;                    // grid buffer of the call's first sample (the other holds the previous grid)
+    bool hg_prev_valid;              // the previous grid (from the previous call) may be reused by the first sample
 };
 
 
@@ -3241,9 +3253,9 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
             w.tthr = cj->tthr;
             w.tt = cj->tt;
         }
-        const int hg_cur = wi & 1;
+        const int hg_cur = (j.hg_first + wi) & 1;
         if (j.hashgrid) {  // this sample's grid, and the previous sample's to reuse from
-            c.hg_prev = base > 0 ? 1u : 0u;
+            c.hg_prev = (base > 0 || j.hg_prev_valid) ? 1u : 0u;
             w.hg_rec = ctx->hg_rec[hg_cur].as<float4>();
             w.hg_prev_rec = ctx->hg_rec[hg_cur ^ 1].as<float4>();
             w.hg_prev_keys = ctx->hg_keys[hg_cur ^ 1].as<unsigned long long>();
@@ -3364,8 +3376,13 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
         order_after_caller(ctx);
         for (int sl = 0; sl < slots; sl++) reserve_wave(ctx->slot[sl], paths, reuse, paths, tgrid);
+        // the previous call's last grid is reused when it was built for this
+        // scene at this entry count (TestRenderer keeps mPrevHashGrid across frames)
+        const uint64_t ns = hashgrid ? (uint64_t)npix * std::min<uint32_t>(2, p->max_depth - 1) : 0;
+        const bool hg_prev_valid = hashgrid && ctx->hg_valid && !(p->flags & ST_FLAG_HISTORY_RESET) && ctx->hg_scene == s->serial &&
+                                   ctx->hg_ns == ns;
+        ctx->hg_valid = false;
         if (hashgrid) {
-            const uint64_t ns = (uint64_t)npix * std::min<uint32_t>(2, p->max_depth - 1);
             for (int i = 0; i < 2; i++) {
                 ctx->hg_rec[i].reserve(ns * 64);
                 ctx->hg_keys[i].reserve(ns * 8);
@@ -3401,8 +3418,16 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         j.sgrid = sgrid;
         j.timed = true;
         j.hashgrid = hashgrid;
+        j.hg_first = ctx->hg_last ^ 1;
+        j.hg_prev_valid = hg_prev_valid;
         enqueue_camera(ctx, j);
         for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
+        if (hashgrid) {
+            ctx->hg_last = (j.hg_first + (int)waves - 1) & 1;
+            ctx->hg_scene = s->serial;
+            ctx->hg_ns = ns;
+            ctx->hg_valid = true;
+        }
         finish_timings(ctx, slots, o->ray_counts);
         return ST_OK;
     } catch (const std::exception& e) {

@@ -299,6 +299,50 @@ def test_restir_hashgrid_argument_checks(gpu_ctx):
         Batch(gpu_ctx, [(sc, view, abi.default_params(16, 16, max_depth=3, spp=1, flags=ok))])
 
 
+def test_restir_hashgrid_kept_across_calls(gpu_ctx):
+    """As TestRenderer keeps the previous frame's hash grid, a call's first
+    sample reuses the grid the previous call's last sample built: two 1-spp
+    calls give the two samples of one 2-spp call (per-sample path lengths
+    equal, radiance within float rounding of the average).  ST_FLAG_HISTORY_RESET,
+    a render without the flag in between, or another scene drops the grid:
+    the first sample is then RIS alone, as in a fresh call."""
+    import torch
+    from stratum2_amd.renderer import Scene, TestRenderer
+    sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+    W, H = 96, 64
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_HASHGRID
+    r = TestRenderer(Scene(gpu_ctx, sd))
+
+    def render(spp, seed, extra=0, renderer=r):
+        out = renderer.render(view, abi.default_params(W, H, max_depth=4, spp=spp, seed=seed, flags=flags | extra, max_m=3.0),
+                              want_path_length=True)
+        torch.cuda.synchronize()
+        return out["hdr"].cpu().numpy()[..., :3].astype(np.float64), out["path_length"].cpu().numpy().view(np.uint32)
+
+    fresh0 = render(1, 7, abi.ST_FLAG_HISTORY_RESET)
+    a = render(1, 7, abi.ST_FLAG_HISTORY_RESET)
+    b = render(1, 8)
+    both = render(2, 7, abi.ST_FLAG_HISTORY_RESET)
+    assert np.array_equal(a[0], fresh0[0])
+    assert np.array_equal(both[1][0], a[1][0]) and np.array_equal(both[1][1], b[1][0])
+    diff = np.abs(2 * both[0] - (a[0] + b[0]))
+    print(f"largest difference of the two-call sum from the 2-spp call: {diff.max():.3e}", flush=True)
+    assert np.all(diff <= 1e-5 * np.maximum(np.abs(a[0] + b[0]), 1e-2))
+    # the second call reused: it differs from a fresh first sample of seed 8
+    fresh8 = render(1, 8, abi.ST_FLAG_HISTORY_RESET)
+    assert not np.array_equal(b[0], fresh8[0])
+    # a render without the flag drops the grid
+    render(1, 7, abi.ST_FLAG_HISTORY_RESET)
+    r.render(view, abi.default_params(W, H, max_depth=4, spp=1, seed=3, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS))
+    torch.cuda.synchronize()
+    assert np.array_equal(render(1, 8)[0], fresh8[0])
+    # another scene drops it too
+    render(1, 7, abi.ST_FLAG_HISTORY_RESET)
+    other = TestRenderer(Scene(gpu_ctx, sd))
+    assert np.array_equal(render(1, 8, renderer=other)[0], fresh8[0])
+
+
 def _gpu_render(gpu_ctx, sd, view, params):
     import torch
     from stratum2_amd.renderer import Scene, TestRenderer
