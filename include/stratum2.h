@@ -192,6 +192,14 @@ typedef struct st_view {
 /* Drop the history kept from earlier calls on the context (the hash grid of
  * ST_FLAG_RESTIR_DI_HASHGRID), as at a camera cut. */
 #define ST_FLAG_HISTORY_RESET           (1u << 15)
+/* Per-material Lambert / GGX in place of the Disney BSDF: a material whose
+ * metallic is at least 0.5 is a GGX conductor (the Disney BSDF's metallic lobe
+ * alone: Schlick's Fresnel about the base colour, anisotropic GGX with
+ * alpha = roughness^2, separable Smith masking, visible-normal sampling in
+ * Heitz's view-aligned disk frame), any other is Lambert with its base colour.  The reference has no such
+ * switch (gLambertian makes every material Lambert, testrenderer.slang:33-37);
+ * ST_FLAG_LAMBERTIAN takes precedence. */
+#define ST_FLAG_LAMBERT_GGX             (1u << 16)
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface

@@ -51,8 +51,41 @@
 #include <omp.h>
 #endif
 
+/* The arithmetic type.  The default build computes in float, with the
+ * renderer's operations in the renderer's order (the parity build).  Built
+ * with -DOR_FP64 the same algorithm computes in double, with libm's double
+ * transcendentals: the high-precision reference for converged images.  Data
+ * in its stored formats (scene buffers, textures, alias tables, fp16 and
+ * snorm packing, fixed-point sums, the caller's arrays) stays in those
+ * formats in both builds. */
+#ifdef OR_FP64
+typedef double real;
+#define sqrtr sqrt
+#define fabsr fabs
+#define floorr floor
+#define fminr fmin
+#define fmaxr fmax
+#define tanr tan
+#define powr pow
+#define nextafterr nextafter
+#define truncr trunc
+#define ldexpr ldexp
+#define OR_PI 3.14159265358979323846
+#else
+typedef float real;
+#define sqrtr sqrtf
+#define fabsr fabsf
+#define floorr floorf
+#define fminr fminf
+#define fmaxr fmaxf
+#define tanr tanf
+#define powr powf
+#define nextafterr nextafterf
+#define truncr truncf
+#define ldexpr ldexpf
 #define OR_PI 3.14159265358979323846f
-#define OR_1_PI (1.0f / OR_PI)
+#endif
+#define OR_1_PI ((real)1 / OR_PI)
 #define OR_Q 16384 /* gRadianceQuantizationFactor, testrenderer.slang:25 */
 
 static _Thread_local char g_err[256];
@@ -63,33 +96,33 @@ static int fail(const char* m) {
 }
 
 /* ------------------------------------------------------------------ math */
-typedef struct { float x, y, z; } f3;
-static inline f3 v3(float x, float y, float z) { f3 r = {x, y, z}; return r; }
+typedef struct { real x, y, z; } f3;
+static inline f3 v3(real x, real y, real z) { f3 r = {x, y, z}; return r; }
 static inline f3 add(f3 a, f3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
 static inline f3 sub(f3 a, f3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 static inline f3 mul(f3 a, f3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }
-static inline f3 mulf(f3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline f3 mulf(f3 a, real s) { return v3(a.x * s, a.y * s, a.z * s); }
 static inline f3 divv(f3 a, f3 b) { return v3(a.x / b.x, a.y / b.y, a.z / b.z); }
-static inline f3 divf(f3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+static inline f3 divf(f3 a, real s) { return v3(a.x / s, a.y / s, a.z / s); }
 static inline f3 neg(f3 a) { return v3(-a.x, -a.y, -a.z); }
-static inline float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline real dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline f3 cross(f3 a, f3 b) {
     return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-static inline float length3(f3 a) { return sqrtf(dot(a, a)); }
-static inline f3 normalize3(f3 a) { return divf(a, sqrtf(dot(a, a))); }
-static inline float fget(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+static inline real length3(f3 a) { return sqrtr(dot(a, a)); }
+static inline f3 normalize3(f3 a) { return divf(a, sqrtr(dot(a, a))); }
+static inline real fget(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 static inline int any_gt0(f3 a) { return a.x > 0 || a.y > 0 || a.z > 0; }
 static inline int all_le0(f3 a) { return a.x <= 0 && a.y <= 0 && a.z <= 0; }
-static inline float minf(float a, float b) { return a < b ? a : b; }
-static inline float maxf(float a, float b) { return a > b ? a : b; }
-static inline float saturate(float a) { return a < 0 ? 0 : (a > 1 ? 1 : a); }
-static inline float max3f(f3 a) { return maxf(maxf(a.x, a.y), a.z); }
-static inline float pow2(float a) { return a * a; }
+static inline real minf(real a, real b) { return a < b ? a : b; }
+static inline real maxf(real a, real b) { return a > b ? a : b; }
+static inline real saturate(real a) { return a < 0 ? 0 : (a > 1 ? 1 : a); }
+static inline real max3f(f3 a) { return maxf(maxf(a.x, a.y), a.z); }
+static inline real pow2(real a) { return a * a; }
 /* max(0, x) of the reference's GPU code: NaN becomes 0 */
-static inline float clamp0(float x) { return x > 0 ? x : 0.0f; }
+static inline real clamp0(real x) { return x > 0 ? x : 0.0f; }
 /* luminance, src/Shaders/compat/common.h:51 */
-static inline float luminance(f3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+static inline real luminance(f3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
 
 static inline uint32_t asuint(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static inline float asfloat(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
@@ -125,7 +158,7 @@ static void rng_init(rng_t* r, uint32_t seed, uint32_t px, uint32_t py, uint32_t
     r->s[0] = px; r->s[1] = py; r->s[2] = seed; r->s[3] = offset;
 }
 static void rng_next(rng_t* r, uint32_t o[4]) { r->s[3]++; or_pcg4d(r->s, o); }
-static void rng_nextf(rng_t* r, float o[4]) {
+static void rng_nextf(rng_t* r, real o[4]) {
     uint32_t v[4];
     rng_next(r, v);
     for (int i = 0; i < 4; i++) o[i] = asfloat(0x3f800000u | (v[i] >> 9)) - 1.0f;
@@ -134,30 +167,30 @@ static void rng_nextf(rng_t* r, float o[4]) {
 /* ------------------------------------------------------------- packing */
 /* D3DX_FLOAT_to_INT / D3DX_INT_to_FLOAT / SaturateSigned,
  * src/Shaders/common/D3DX_DXGIFormatConvert.inl:290-322 */
-static int32_t snorm16(float v) {
+static int32_t snorm16(real v) {
     if (v != v) v = 0;
     v = minf(maxf(v, -1.0f), 1.0f);
-    return (int32_t)truncf(v * 32767.0f + (v >= 0 ? 0.5f : -0.5f));
+    return (int32_t)truncr(v * 32767.0f + (v >= 0 ? 0.5f : -0.5f));
 }
-static float unsnorm16(int32_t i) { return maxf((float)i / 32767.0f, -1.0f); }
+static real unsnorm16(int32_t i) { return maxf((real)i / 32767.0f, -1.0f); }
 /* packNormal2 / packNormal, src/Shaders/common/common.hlsli:3-19 */
 static uint32_t pack_normal(f3 v) {
-    float r = 1.0f / (fabsf(v.x) + fabsf(v.y) + fabsf(v.z));
-    float px = v.x * r, py = v.y * r;
+    real r = 1.0f / (fabsr(v.x) + fabsr(v.y) + fabsr(v.z));
+    real px = v.x * r, py = v.y * r;
     if (v.z <= 0) {
-        float nx = (1 - fabsf(py)) * (px >= 0 ? 1.0f : -1.0f);
-        float ny = (1 - fabsf(px)) * (py >= 0 ? 1.0f : -1.0f);
+        real nx = (1 - fabsr(py)) * (px >= 0 ? 1.0f : -1.0f);
+        real ny = (1 - fabsr(px)) * (py >= 0 ? 1.0f : -1.0f);
         px = nx; py = ny;
     }
     return ((uint32_t)snorm16(px) & 0xffffu) | ((uint32_t)snorm16(py) << 16);
 }
 /* unpackNormal2 / unpackNormal, common.hlsli:8-22 */
 static f3 unpack_normal(uint32_t p) {
-    float x = unsnorm16((int32_t)(p << 16) >> 16), y = unsnorm16((int32_t)(p & 0xffff0000u) >> 16);
-    f3 v = v3(x, y, 1 - (fabsf(x) + fabsf(y)));
+    real x = unsnorm16((int32_t)(p << 16) >> 16), y = unsnorm16((int32_t)(p & 0xffff0000u) >> 16);
+    f3 v = v3(x, y, 1 - (fabsr(x) + fabsr(y)));
     if (v.z < 0) {
-        float nx = (1 - fabsf(v.y)) * (v.x >= 0 ? 1.0f : -1.0f);
-        float ny = (1 - fabsf(v.x)) * (v.y >= 0 ? 1.0f : -1.0f);
+        real nx = (1 - fabsr(v.y)) * (v.x >= 0 ? 1.0f : -1.0f);
+        real ny = (1 - fabsr(v.x)) * (v.y >= 0 ? 1.0f : -1.0f);
         v.x = nx; v.y = ny;
     }
     return normalize3(v);
@@ -186,22 +219,22 @@ uint16_t or_f32_to_f16(float f) {
 float or_f16_to_f32(uint16_t h) {
     uint32_t sign = ((uint32_t)h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
     if (e == 0) {
-        float v = (float)m * (1.0f / 16777216.0f);
+        real v = (real)m * (1.0f / 16777216.0f);
         return sign ? -v : v;
     }
     if (e == 0x1f) return asfloat(sign | 0x7f800000u | (m << 13));
     return asfloat(sign | ((e + 112) << 23) | (m << 13));
 }
-static float q16(float f) { return or_f16_to_f32(or_f32_to_f16(f)); }
+static real q16(real f) { return or_f16_to_f32(or_f32_to_f16(f)); }
 
 /* unorm8 material fields, src/Shaders/compat/material_data.h:43-64 */
-static float un8(uint32_t w, int shift) { return (float)((w >> shift) & 0xffu) / 255.0f; }
+static real un8(uint32_t w, int shift) { return (real)((w >> shift) & 0xffu) / 255.0f; }
 
 /* rayOffset, src/Shaders/common/intersection.hlsli:17-32 */
-static float roff1(float p, float n) {
-    const float origin = 1.0f / 32.0f, fs = 1.0f / 65536.0f;
+static real roff1(real p, real n) {
+    const real origin = 1.0f / 32.0f, fs = 1.0f / 65536.0f;
     int32_t of = (int32_t)(256.0f * n);
-    if (fabsf(p) < origin) return p + fs * n;
+    if (fabsr(p) < origin) return p + fs * n;
     return asfloat((uint32_t)((int32_t)asuint(p) + ((p < 0.0f) ? -of : of)));
 }
 static f3 ray_offset(f3 p, f3 n) { return v3(roff1(p.x, n.x), roff1(p.y, n.y), roff1(p.z, n.z)); }
@@ -222,109 +255,120 @@ void or_ray_offset(const float p[3], const float n[3], float o[3]) {
  * logf.c, expf.c), evaluated in float.  The reference's own precision
  * (Slang -> SPIR-V -> driver) is unpinned; these are pinned against float64
  * by their error bounds in tests/test_oracle_math.py. */
-static float om_poly_sin(float x, float z) {
+#ifndef OR_FP64
+static real om_poly_sin(real x, real z) {
     return ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * x + x;
 }
-static float om_poly_cos(float z) {
-    const float y = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z;
+static real om_poly_cos(real z) {
+    const real y = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z;
     return (y - 0.5f * z) + 1.0f;
 }
-static float om_reduce(float a, int* jo) {
+static real om_reduce(real a, int* jo) {
     int j = (int)(a * 1.27323954473516f);
-    float y = (float)j;
+    real y = (real)j;
     if (j & 1) { j += 1; y += 1.0f; }
     j &= 7;
     *jo = j;
     return ((a - y * 0.78515625f) - y * 2.4187564849853515625e-4f) - y * 3.77489497744594108e-8f;
 }
-static float om_sin(float x) {
-    float sign = 1.0f, a = x;
+static real om_sin(real x) {
+    real sign = 1.0f, a = x;
     if (a < 0) { a = -a; sign = -1.0f; }
     int j;
-    const float r = om_reduce(a, &j);
+    const real r = om_reduce(a, &j);
     if (j > 3) { sign = -sign; j -= 4; }
-    const float z = r * r;
-    const float y = (j == 1 || j == 2) ? om_poly_cos(z) : om_poly_sin(r, z);
+    const real z = r * r;
+    const real y = (j == 1 || j == 2) ? om_poly_cos(z) : om_poly_sin(r, z);
     return sign < 0 ? -y : y;
 }
-static float om_cos(float x) {
-    const float a = x < 0 ? -x : x;
+static real om_cos(real x) {
+    const real a = x < 0 ? -x : x;
     int j;
-    const float r = om_reduce(a, &j);
-    float sign = 1.0f;
+    const real r = om_reduce(a, &j);
+    real sign = 1.0f;
     if (j > 3) { j -= 4; sign = -sign; }
     if (j > 1) sign = -sign;
-    const float z = r * r;
-    const float y = (j == 1 || j == 2) ? om_poly_sin(r, z) : om_poly_cos(z);
+    const real z = r * r;
+    const real y = (j == 1 || j == 2) ? om_poly_sin(r, z) : om_poly_cos(z);
     return sign < 0 ? -y : y;
 }
-static float om_atan_pos(float x) {
-    float y = 0.0f;
+static real om_atan_pos(real x) {
+    real y = 0.0f;
     if (x > 2.414213562373095f) { y = 1.5707963267948966f; x = -(1.0f / x); }
     else if (x > 0.4142135623730950f) { y = 0.7853981633974483f; x = (x - 1.0f) / (x + 1.0f); }
-    const float z = x * x;
+    const real z = x * x;
     return y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z * x + x);
 }
 /* atan2 for x != 0 (stable_atan2 handles x == 0, compat/common.h:137-140) */
-static float om_atan2(float y, float x) {
-    const float q = y / x;
-    const float t = q < 0 ? -om_atan_pos(-q) : om_atan_pos(q);
+static real om_atan2(real y, real x) {
+    const real q = y / x;
+    const real t = q < 0 ? -om_atan_pos(-q) : om_atan_pos(q);
     if (x > 0) return t;
     return y < 0 ? t - 3.14159265358979f : t + 3.14159265358979f;
 }
-static float om_asin_pos(float a, int* big) {
-    float x, z;
+static real om_asin_pos(real a, int* big) {
+    real x, z;
     *big = a > 0.5f;
-    if (*big) { z = 0.5f * (1.0f - a); x = sqrtf(z); }
+    if (*big) { z = 0.5f * (1.0f - a); x = sqrtr(z); }
     else { x = a; z = x * x; }
     return ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z +
             1.6666752422e-1f) * z * x + x;
 }
-static float om_acos(float x) {
-    const float a = x < 0 ? -x : x;
+static real om_acos(real x) {
+    const real a = x < 0 ? -x : x;
     int big;
-    const float s = om_asin_pos(a, &big);
+    const real s = om_asin_pos(a, &big);
     if (big) {
-        const float r = s + s;
+        const real r = s + s;
         return x < 0 ? 3.14159265358979f - r : r;
     }
     return x < 0 ? 1.5707963267948966f + s : 1.5707963267948966f - s;
 }
-static float om_log(float x) {
+static real om_log(real x) {
     if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
     if (x == INFINITY) return x;
     int e = 0;
     if (x < 1.17549435e-38f) { x *= 16777216.0f; e = -24; }
     const uint32_t b = asuint(x);
     e += (int)((b >> 23) & 0xffu) - 126;
-    float m = asfloat((b & 0x007fffffu) | 0x3f000000u);
+    real m = asfloat((b & 0x007fffffu) | 0x3f000000u);
     if (m < 0.707106781186547524f) { e -= 1; m = m + m - 1.0f; }
     else m = m - 1.0f;
-    const float z = m * m;
-    float y = ((((((((7.0376836292e-2f * m - 1.1514610310e-1f) * m + 1.1676998740e-1f) * m - 1.2420140846e-1f) * m +
+    const real z = m * m;
+    real y = ((((((((7.0376836292e-2f * m - 1.1514610310e-1f) * m + 1.1676998740e-1f) * m - 1.2420140846e-1f) * m +
                    1.4249322787e-1f) * m - 1.6668057665e-1f) * m + 2.0000714765e-1f) * m - 2.4999993993e-1f) * m +
                3.3333331174e-1f) * m * z;
-    const float fe = (float)e;
+    const real fe = (real)e;
     y += -2.12194440e-4f * fe;
     y += -0.5f * z;
     return (m + y) + 0.693359375f * fe;
 }
-static float om_exp(float x) {
+static real om_exp(real x) {
     if (x != x) return x;
     if (x > 88.7228391f) return INFINITY;
     if (x < -87.0f) return 0.0f;
-    const float fz = floorf(1.44269504088896341f * x + 0.5f);
-    float r = x - fz * 0.693359375f;
+    const real fz = floorr(1.44269504088896341f * x + 0.5f);
+    real r = x - fz * 0.693359375f;
     r = r - fz * -2.12194440e-4f;
     const int n = (int)fz;
-    const float z = r * r;
-    const float p = ((((((1.9875691500e-4f * r + 1.3981999507e-3f) * r + 8.3334519073e-3f) * r + 4.1665795894e-2f) * r +
+    const real z = r * r;
+    const real p = ((((((1.9875691500e-4f * r + 1.3981999507e-3f) * r + 8.3334519073e-3f) * r + 4.1665795894e-2f) * r +
                       1.6666665459e-1f) * r + 5.0000001201e-1f) * z + r) + 1.0f;
     const int n1 = n / 2, n2 = n - n1;
     return (p * asfloat((uint32_t)(n1 + 127) << 23)) * asfloat((uint32_t)(n2 + 127) << 23);
 }
-static float om_pow(float x, float y) { return om_exp(y * om_log(x)); }
-static float om_eval(int fn, float x, float y) {
+static real om_pow(real x, real y) { return om_exp(y * om_log(x)); }
+#else
+/* the double build: libm's double functions */
+static real om_sin(real x) { return sin(x); }
+static real om_cos(real x) { return cos(x); }
+static real om_atan2(real y, real x) { return atan2(y, x); }
+static real om_acos(real x) { return acos(x); }
+static real om_log(real x) { return log(x); }
+static real om_exp(real x) { return exp(x); }
+static real om_pow(real x, real y) { return pow(x, y); }
+#endif
+static real om_eval(int fn, real x, real y) {
     switch (fn) {
         case 0: return om_sin(x);
         case 1: return om_cos(x);
@@ -348,56 +392,56 @@ static void make_orthonormal(f3 N, f3 r[3]) {
     r[2] = N;
 }
 /* warps, common.hlsli:36-77 */
-static void sample_uniform_triangle(float u1, float u2, float b[2]) {
-    float a = sqrtf(u1);
+static void sample_uniform_triangle(real u1, real u2, real b[2]) {
+    real a = sqrtr(u1);
     b[0] = 1 - a; b[1] = a * u2;
 }
-static f3 sample_uniform_sphere(float u1, float u2) {
-    float z = u2 * 2 - 1, r = sqrtf(maxf(1 - z * z, 0)), phi = 2 * OR_PI * u1;
+static f3 sample_uniform_sphere(real u1, real u2) {
+    real z = u2 * 2 - 1, r = sqrtr(maxf(1 - z * z, 0)), phi = 2 * OR_PI * u1;
     return v3(r * om_cos(phi), r * om_sin(phi), z);
 }
-static void sample_concentric_disc(float u1, float u2, float o[2]) {
-    float ux = 2 * u1 - 1, uy = 2 * u2 - 1, theta, r;
+static void sample_concentric_disc(real u1, real u2, real o[2]) {
+    real ux = 2 * u1 - 1, uy = 2 * u2 - 1, theta, r;
     if (ux == 0 && uy == 0) { o[0] = o[1] = 0; return; }
-    if (fabsf(ux) > fabsf(uy)) { r = ux; theta = (OR_PI / 4) * (uy / ux); }
+    if (fabsr(ux) > fabsr(uy)) { r = ux; theta = (OR_PI / 4) * (uy / ux); }
     else { r = uy; theta = OR_PI / 2 - (OR_PI / 4) * (ux / uy); }
     o[0] = r * om_cos(theta); o[1] = r * om_sin(theta);
 }
-static f3 sample_cos_hemisphere(float u1, float u2) {
-    float xy[2];
+static f3 sample_cos_hemisphere(real u1, real u2) {
+    real xy[2];
     sample_concentric_disc(u1, u2, xy);
-    return v3(xy[0], xy[1], sqrtf(maxf(0, 1 - (xy[0] * xy[0] + xy[1] * xy[1]))));
+    return v3(xy[0], xy[1], sqrtr(maxf(0, 1 - (xy[0] * xy[0] + xy[1] * xy[1]))));
 }
-static float cos_hemisphere_pdfw(float c) { return maxf(c, 0.0f) / OR_PI; }
-static float concentric_disc_pdfa(void) { return 1.0f / OR_PI; }
-static float mis1(float a) { return a * a; }
+static real cos_hemisphere_pdfw(real c) { return maxf(c, 0.0f) / OR_PI; }
+static real concentric_disc_pdfa(void) { return 1.0f / OR_PI; }
+static real mis1(real a) { return a * a; }
 /* mis, testrenderer.slang:284-290, with its limit where a pdf overflowed
  * (inf / inf: 1 for the infinite pdf, 0 against it, 1/2 for two) instead of
  * the NaN the reference's quotient gives */
-static float mis2(float a, float b) {
-    const float a2 = mis1(a), b2 = mis1(b);
-    const float w = a2 / (a2 + b2);
+static real mis2(real a, real b) {
+    const real a2 = mis1(a), b2 = mis1(b);
+    const real w = a2 / (a2 + b2);
     return w == w ? w : (a2 > b2 ? 1.0f : (a2 < b2 ? 0.0f : 0.5f));
 }
 
 /* stableAtan2 / cartesianToSphericalUv / sphericalUvToCartesian, compat/common.h:137-150 */
-static float stable_atan2(float y, float x) {
+static real stable_atan2(real y, real x) {
     return x == 0.0f ? (y == 0 ? 0 : (y < 0 ? -OR_PI / 2 : OR_PI / 2)) : om_atan2(y, x);
 }
-static void cart_to_uv(f3 v, float uv[2]) {
-    float th = stable_atan2(v.z, v.x);
+static void cart_to_uv(f3 v, real uv[2]) {
+    real th = stable_atan2(v.z, v.x);
     uv[0] = th * OR_1_PI * 0.5f + 0.5f;
     uv[1] = om_acos(minf(maxf(v.y, -1.0f), 1.0f)) * OR_1_PI;
 }
-static f3 uv_to_cart(float u, float v) {
+static f3 uv_to_cart(real u, real v) {
     u = u * 2 - 1;
     u *= OR_PI; v *= OR_PI;
-    float sp = om_sin(v);
+    real sp = om_sin(v);
     return v3(sp * om_cos(u), om_cos(v), sp * om_sin(u));
 }
 
 /* ------------------------------------------------------------- scene */
-typedef struct { float lo[3], hi[3]; uint32_t a, n; } bnode; /* n>0: leaf [a, a+n); else children a, a+1 */
+typedef struct { real lo[3], hi[3]; uint32_t a, n; } bnode; /* n>0: leaf [a, a+n); else children a, a+1 */
 typedef struct { int w, h; float* rgba; } level_t;
 
 struct or_scene {
@@ -422,11 +466,11 @@ struct or_scene {
     float* alias_prob;
     uint32_t* alias_idx;
     float* alias_pdf;
-    float alias_jmax; /* largest in-texel jitter: 1 - 2^(ceil(log2(max(W, H))) - 21) */
+    real alias_jmax; /* largest in-texel jitter: 1 - 2^(ceil(log2(max(W, H))) - 21) */
     /* lights: emissive instances in instance order (Scene.cpp:487-491) */
     uint32_t nlights;
     uint32_t* light_inst;
-    float sphere[4];
+    real sphere[4];
     /* flattened world-space triangles */
     uint32_t ntris;
     float* tv;     /* 9 floats each */
@@ -488,11 +532,14 @@ static int load_f3(const or_scene* s, uint32_t buf, uint32_t offset, uint32_t st
     *o = v3(f[0], f[1], f[2]);
     return 1;
 }
-static int load_f2(const or_scene* s, uint32_t buf, uint32_t offset, uint32_t stride, uint32_t idx, float o[2]) {
+static int load_f2(const or_scene* s, uint32_t buf, uint32_t offset, uint32_t stride, uint32_t idx, real o[2]) {
     if (buf >= s->nvb) return 0;
     uint64_t off = (uint64_t)offset + (uint64_t)stride * idx;
     if (off + 8 > s->vbsz[buf]) return 0;
-    memcpy(o, s->vb[buf] + off, 8);
+    float f[2];
+    memcpy(f, s->vb[buf] + off, 8);
+    o[0] = f[0];
+    o[1] = f[1];
     return 1;
 }
 
@@ -500,8 +547,8 @@ typedef struct {
     f3 pos;
     uint32_t flags_material;
     uint32_t gn, sn, tan;
-    float area;
-    float uv[2];
+    real area;
+    real uv[2];
 } shading_t; /* ShadingData, src/Shaders/compat/scene.h:185-198 (fields used here) */
 
 /* makeTriangleShadingDataWithoutPosition + makeTriangleShadingData,
@@ -509,7 +556,7 @@ typedef struct {
 /* makeTriangleShadingData, shading_data.hlsli:142-174: from barycentrics, or
  * (lp != NULL) from an object-space point on the triangle, whose barycentrics
  * are solved from the point and whose world position is T * lp. */
-static void make_tri_shading_x(const or_scene* s, uint32_t ii, uint32_t prim, float bx, float by, const f3* lp, shading_t* r) {
+static void make_tri_shading_x(const or_scene* s, uint32_t ii, uint32_t prim, real bx, real by, const f3* lp, shading_t* r) {
     const st_instance* in = &s->inst[ii];
     const st_transform* T = &s->xf[ii];
     const st_mesh_vertex_info* vi = &s->vi[in->data & 0xffffu];
@@ -522,32 +569,32 @@ static void make_tri_shading_x(const or_scene* s, uint32_t ii, uint32_t prim, fl
     load_f3(s, pb, vi->packed_offsets[1], ps, tri[2], &v2);
     if (lp) {
         const f3 v1v0 = sub(v1, v0), v2v0 = sub(v2, v0), p_v0 = sub(*lp, v0);
-        const float d00 = dot(v1v0, v1v0), d01 = dot(v1v0, v2v0), d11 = dot(v2v0, v2v0);
-        const float d20 = dot(p_v0, v1v0), d21 = dot(p_v0, v2v0);
-        const float den = d00 * d11 - d01 * d01;
+        const real d00 = dot(v1v0, v1v0), d01 = dot(v1v0, v2v0), d11 = dot(v2v0, v2v0);
+        const real d20 = dot(p_v0, v1v0), d21 = dot(p_v0, v2v0);
+        const real den = d00 * d11 - d01 * d01;
         bx = (d11 * d20 - d01 * d21) / den;
         by = (d00 * d21 - d01 * d20) / den;
     }
     r->flags_material = (in->type_material_address >> 4) << 4;
     f3 dPds = xf_vector(T, sub(v0, v2)), dPdt = xf_vector(T, sub(v1, v2));
     f3 gn = cross(dPds, dPdt);
-    float area2 = length3(gn);
+    real area2 = length3(gn);
     gn = divf(gn, area2);
     r->gn = pack_normal(gn);
     r->area = area2 / 2;
-    float t0[2] = {0, 0}, t1[2] = {0, 0}, t2[2] = {0, 0};
+    real t0[2] = {0, 0}, t1[2] = {0, 0}, t2[2] = {0, 0};
     uint32_t tb = vi_field(vi->packed_buffer_indices[1], 16, 16), ts = vi_field(vi->packed_strides, 24, 8);
     if (tb < ST_IMAGE_LIMIT) {
         load_f2(s, tb, vi->packed_offsets[3], ts, tri[0], t0);
         load_f2(s, tb, vi->packed_offsets[3], ts, tri[1], t1);
         load_f2(s, tb, vi->packed_offsets[3], ts, tri[2], t2);
     }
-    float duvds[2] = {t2[0] - t0[0], t2[1] - t0[1]}, duvdt[2] = {t2[0] - t1[0], t2[1] - t1[1]};
+    real duvds[2] = {t2[0] - t0[0], t2[1] - t0[1]}, duvdt[2] = {t2[0] - t1[0], t2[1] - t1[1]};
     r->uv[0] = t0[0] + (t1[0] - t0[0]) * bx + duvds[0] * by;
     r->uv[1] = t0[1] + (t1[1] - t0[1]) * bx + duvds[1] * by;
-    float det = duvds[0] * duvdt[1] - duvdt[0] * duvds[1];
-    float invDet = 1 / det;
-    float dsdu = duvdt[1] * invDet, dtdu = -duvds[1] * invDet;
+    real det = duvds[0] * duvdt[1] - duvdt[0] * duvds[1];
+    real invDet = 1 / det;
+    real dsdu = duvdt[1] * invDet, dtdu = -duvds[1] * invDet;
     f3 dPdu;
     if (det != 0) {
         dPdu = neg(add(mulf(dPds, dsdu), mulf(dPdt, dtdu)));
@@ -579,7 +626,7 @@ static void make_tri_shading_x(const or_scene* s, uint32_t ii, uint32_t prim, fl
     }
     r->pos = lp ? xf_point(T, *lp) : xf_point(T, add(add(v0, mulf(sub(v1, v0), bx)), mulf(sub(v2, v0), by)));
 }
-static void make_tri_shading(const or_scene* s, uint32_t ii, uint32_t prim, float bx, float by, shading_t* r) {
+static void make_tri_shading(const or_scene* s, uint32_t ii, uint32_t prim, real bx, real by, shading_t* r) {
     make_tri_shading_x(s, ii, prim, bx, by, NULL, r);
 }
 
@@ -594,24 +641,24 @@ static f3 to_world(const shading_t* sd, f3 v) {
     f3 b = cross(n, t);
     return add(add(mulf(t, v.x), mulf(b, v.y)), mulf(n, v.z));
 }
-static float signf(float x) { return x > 0 ? 1.0f : (x < 0 ? -1.0f : 0.0f); }
+static real signf(real x) { return x > 0 ? 1.0f : (x < 0 ? -1.0f : 0.0f); }
 /* ShadingData::shadingNormalCorrection, shading_data.hlsli:35-54; adj: light paths (Adjoint) */
-static float shading_normal_correction_x(const shading_t* sd, f3 din, f3 dout, int adj) {
+static real shading_normal_correction_x(const shading_t* sd, f3 din, f3 dout, int adj) {
     f3 lg = to_local(sd, unpack_normal(sd->gn));
-    float ngin = dot(lg, din), ngout = dot(lg, dout);
+    real ngin = dot(lg, din), ngout = dot(lg, dout);
     if (signf(ngout) != signf(dout.z) || signf(ngin) != signf(din.z)) return 0;
-    float G = 1;
+    real G = 1;
     if (adj) {
-        const float num = ngout * din.z, denom = dout.z * ngin;
-        if (fabsf(denom) > 1e-5f) G *= fabsf(num / denom);
+        const real num = ngout * din.z, denom = dout.z * ngin;
+        if (fabsr(denom) > 1e-5f) G *= fabsr(num / denom);
     }
     return G;
 }
-static float shading_normal_correction(const shading_t* sd, f3 din, f3 dout) { return shading_normal_correction_x(sd, din, dout, 0); }
+static real shading_normal_correction(const shading_t* sd, f3 din, f3 dout) { return shading_normal_correction_x(sd, din, dout, 0); }
 
 /* --------------------------------------------------------- materials */
 /* has_bo: bo replaces the packed base colour (per-triangle albedo) */
-typedef struct { uint32_t p[4]; float escale; int has_bo; f3 bo; } material_t;
+typedef struct { uint32_t p[4]; real escale; int has_bo; f3 bo; } material_t;
 static material_t load_material(const or_scene* s, uint32_t addr) {
     material_t m;
     memset(&m, 0, sizeof m);
@@ -640,128 +687,128 @@ static f3 m_emission(const material_t* m) {
 /* microfacet.hlsli:22-179 */
 /* pow(x, 5) of disney.hlsli as (x^2)^2 x: within 3 ulp of the correctly
  * rounded power, the same operations as the GPU library */
-static float powf5(float x) { const float x2 = x * x; return x2 * x2 * x; }
-static f3 schlick3(f3 F0, float c) {
-    float k = powf5(maxf(1 - c, 0.0f));
+static real powf5(real x) { const real x2 = x * x; return x2 * x2 * x; }
+static f3 schlick3(f3 F0, real c) {
+    real k = powf5(maxf(1 - c, 0.0f));
     return v3(F0.x + (1 - F0.x) * k, F0.y + (1 - F0.y) * k, F0.z + (1 - F0.z) * k);
 }
-static float schlick1(float F0, float c) { return F0 + (1 - F0) * powf5(maxf(1 - c, 0.0f)); }
-static float fresnel_dielectric3(float ni, float nt, float eta) {
-    float rs = (ni - eta * nt) / (ni + eta * nt);
-    float rp = (eta * ni - nt) / (eta * ni + nt);
+static real schlick1(real F0, real c) { return F0 + (1 - F0) * powf5(maxf(1 - c, 0.0f)); }
+static real fresnel_dielectric3(real ni, real nt, real eta) {
+    real rs = (ni - eta * nt) / (ni + eta * nt);
+    real rp = (eta * ni - nt) / (eta * ni + nt);
     return (rs * rs + rp * rp) / 2;
 }
-static float fresnel_dielectric(float ni, float eta) {
-    float nt2 = 1 - (1 - ni * ni) / (eta * eta);
+static real fresnel_dielectric(real ni, real eta) {
+    real nt2 = 1 - (1 - ni * ni) / (eta * eta);
     if (nt2 < 0) return 1;
-    return fresnel_dielectric3(fabsf(ni), sqrtf(nt2), eta);
+    return fresnel_dielectric3(fabsr(ni), sqrtr(nt2), eta);
 }
-static float GTR1(float ndh, float a) {
-    float a2 = a * a, t = 1 + (a2 - 1) * ndh * ndh;
+static real GTR1(real ndh, real a) {
+    real a2 = a * a, t = 1 + (a2 - 1) * ndh * ndh;
     return (a2 - 1) / (OR_PI * om_log(a2) * t);
 }
-static float GTR2a(f3 h, float ax, float ay) {
+static real GTR2a(f3 h, real ax, real ay) {
     f3 hs = divv(h, v3(ax, ay, 1));
-    float l2 = dot(hs, hs);
+    real l2 = dot(hs, hs);
     return 1 / (OR_PI * ax * ay * l2 * l2);
 }
-static float smith_gtr2(f3 v, float ax, float ay) {
-    float ax2 = ax * ax, ay2 = ay * ay;
+static real smith_gtr2(f3 v, real ax, real ay) {
+    real ax2 = ax * ax, ay2 = ay * ay;
     f3 v2 = mul(v, v);
-    float L = (-1 + sqrtf(1 + (v2.x * ax2 + v2.y * ay2) / v2.z)) / 2;
+    real L = (-1 + sqrtr(1 + (v2.x * ax2 + v2.y * ay2) / v2.z)) / 2;
     return 1 / (1 + L);
 }
-static float smith_gtr1(f3 v) { return smith_gtr2(v, 0.25f, 0.25f); }
-static f3 sample_vndf(f3 din, float ax, float ay, float r0, float r1) {
+static real smith_gtr1(f3 v) { return smith_gtr2(v, 0.25f, 0.25f); }
+static f3 sample_vndf(f3 din, real ax, real ay, real r0, real r1) {
     int flip = din.z < 0;
     if (flip) din = neg(din);
     f3 hin = normalize3(v3(ax * din.x, ay * din.y, din.z));
-    float r = sqrtf(r0), phi = 2 * OR_PI * r1;
-    float t1 = r * om_cos(phi), t2 = r * om_sin(phi);
-    float s = (1 + hin.z) / 2;
-    t2 = (1 - s) * sqrtf(1 - t1 * t1) + s * t2;
-    f3 dn = v3(t1, t2, sqrtf(maxf(0.0f, 1 - t1 * t1 - t2 * t2)));
+    real r = sqrtr(r0), phi = 2 * OR_PI * r1;
+    real t1 = r * om_cos(phi), t2 = r * om_sin(phi);
+    real s = (1 + hin.z) / 2;
+    t2 = (1 - s) * sqrtr(1 - t1 * t1) + s * t2;
+    f3 dn = v3(t1, t2, sqrtr(maxf(0.0f, 1 - t1 * t1 - t2 * t2)));
     f3 fr[3];
     make_orthonormal(hin, fr);
     f3 hn = add(add(mulf(fr[0], dn.x), mulf(fr[1], dn.y)), mulf(hin, dn.z));
     f3 o = normalize3(v3(ax * hn.x, ay * hn.y, maxf(0.0f, hn.z)));
     return flip ? neg(o) : o;
 }
-static f3 lerp3(f3 a, f3 b, float t) { return add(a, mulf(sub(b, a), t)); }
-static float lerp1(float a, float b, float t) { return a + (b - a) * t; }
+static f3 lerp3(f3 a, f3 b, real t) { return add(a, mulf(sub(b, a), t)); }
+static real lerp1(real a, real b, real t) { return a + (b - a) * t; }
 
 /* Disney eval, src/Shaders/materials/disney.hlsli:9-139; adj: Adjoint (light paths),
  * refraction scaled by eta_factor = 1 / eta^2 (:123) */
 static f3 disney_eval(const material_t* m, f3 din, f3 dout, int adj) {
-    float aspect = sqrtf(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
-    float ax = maxf(1e-4f, alpha / aspect), ay = maxf(1e-4f, alpha * aspect), metallic = M_METAL(m);
+    real aspect = sqrtr(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
+    real ax = maxf(1e-4f, alpha / aspect), ay = maxf(1e-4f, alpha * aspect), metallic = M_METAL(m);
     if (din.z * dout.z > 0) {
-        float ndin = fabsf(din.z), ndout = dout.z;
+        real ndin = fabsr(din.z), ndout = dout.z;
         f3 h = normalize3(add(din, dout));
         if (h.z * din.z < 0) h = neg(h);
-        float ndh = h.z, hdin = dot(h, din), hdout = dot(h, dout);
+        real ndh = h.z, hdin = dot(h, din), hdout = dot(h, dout);
         f3 c = v3(0, 0, 0);
         if (din.z >= 0 && dout.z >= 0) {
-            float Fd90 = 0.5f + 2 * M_ROUGH(m) * hdout * hdout;
-            float sno = powf5(1 - ndout), sni = powf5(1 - ndin), sho = powf5(1 - hdout);
-            float based = (1 + (Fd90 - 1) * sno) * (1 + (Fd90 - 1) * sni);
-            float Fss90 = hdout * hdout * M_ROUGH(m);
-            float Fss = (1 + (Fss90 - 1) * sno) * (1 + (Fss90 - 1) * sni);
-            float ss = 1.25f * (Fss * (1 / (ndout + ndin) - 0.5f) + 0.5f);
+            real Fd90 = 0.5f + 2 * M_ROUGH(m) * hdout * hdout;
+            real sno = powf5(1 - ndout), sni = powf5(1 - ndin), sho = powf5(1 - hdout);
+            real based = (1 + (Fd90 - 1) * sno) * (1 + (Fd90 - 1) * sni);
+            real Fss90 = hdout * hdout * M_ROUGH(m);
+            real Fss = (1 + (Fss90 - 1) * sno) * (1 + (Fss90 - 1) * sni);
+            real ss = 1.25f * (Fss * (1 / (ndout + ndin) - 0.5f) + 0.5f);
             f3 bc = m_base(m);
             c = add(c, mulf(mulf(mulf(bc, (1 - M_TRANS(m)) * (1 - metallic)), lerp1(based, ss, M_SUBSURF(m)) / OR_PI), ndout));
-            float lum = luminance(bc);
+            real lum = luminance(bc);
             f3 ctint = lum > 0 ? divf(bc, lum) : v3(1, 1, 1);
             f3 csheen = lerp3(v3(1, 1, 1), ctint, 0.5f);
             c = add(c, mulf(mulf(mulf(csheen, (1 - metallic) * M_SHEEN(m)), sho), ndout));
             if (ndin > 0 && hdout > 0 && ndh > 0) {
-                float eta = M_ETA1(m);
-                float f0 = (eta - 1) * (eta - 1) / ((eta + 1) * (eta + 1));
+                real eta = M_ETA1(m);
+                real f0 = (eta - 1) * (eta - 1) / ((eta + 1) * (eta + 1));
                 f3 scol = lerp3(v3(1, 1, 1), ctint, 0.5f);
                 f3 c0 = lerp3(mulf(scol, M_SPEC(m) * f0), m_base(m), metallic);
                 f3 F = schlick3(c0, hdout);
-                float D = GTR2a(h, ax, ay), G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
-                float sw = 1 - M_TRANS(m) * (1 - metallic);
+                real D = GTR2a(h, ax, ay), G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
+                real sw = 1 - M_TRANS(m) * (1 - metallic);
                 c = add(c, divf(mulf(mulf(mulf(F, sw), D), G), 4 * ndin));
             }
             if (ndin > 0 && ndh > 0) {
-                float ac = lerp1(0.1f, 0.001f, M_CCGLOSS(m));
-                float Fc = schlick1(0.04f, hdout), Dc = GTR1(ndh, ac), Gc = smith_gtr1(din) * smith_gtr1(dout);
-                float v = M_CC(m) * Fc * Dc * Gc / (4 * ndin);
+                real ac = lerp1(0.1f, 0.001f, M_CCGLOSS(m));
+                real Fc = schlick1(0.04f, hdout), Dc = GTR1(ndh, ac), Gc = smith_gtr1(din) * smith_gtr1(dout);
+                real v = M_CC(m) * Fc * Dc * Gc / (4 * ndin);
                 c = add(c, v3(v, v, v));
             }
         }
-        float gw = (1 - metallic) * M_TRANS(m);
+        real gw = (1 - metallic) * M_TRANS(m);
         if (gw > 0) {
-            float eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
-            float Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay);
-            float G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
-            c = add(c, mulf(m_base(m), gw * (Fg * D * G) / (4 * fabsf(ndin))));
+            real eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+            real Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay);
+            real G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
+            c = add(c, mulf(m_base(m), gw * (Fg * D * G) / (4 * fabsr(ndin))));
         }
         return c;
     }
-    float gw = (1 - metallic) * M_TRANS(m);
+    real gw = (1 - metallic) * M_TRANS(m);
     if (gw <= 0) return v3(0, 0, 0);
-    float eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+    real eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
     f3 h = normalize3(add(din, mulf(dout, eta)));
     if (h.z * din.z < 0) h = neg(h);
-    float hdin = dot(h, din), hdout = dot(h, dout), sd = hdin + eta * hdout;
-    float Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay);
-    float G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
+    real hdin = dot(h, din), hdout = dot(h, dout), sd = hdin + eta * hdout;
+    real Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay);
+    real G = smith_gtr2(din, ax, ay) * smith_gtr2(dout, ax, ay);
     f3 bc = m_base(m);
-    f3 sq = v3(sqrtf(bc.x), sqrtf(bc.y), sqrtf(bc.z));
-    const float eta_factor = adj ? (1 / (eta * eta)) : 1;
-    return mulf(sq, gw * (eta_factor * (1 - Fg) * D * G * eta * eta * fabsf(hdout * hdin)) / (fabsf(din.z) * sd * sd));
+    f3 sq = v3(sqrtr(bc.x), sqrtr(bc.y), sqrtr(bc.z));
+    const real eta_factor = adj ? (1 / (eta * eta)) : 1;
+    return mulf(sq, gw * (eta_factor * (1 - Fg) * D * G * eta * eta * fabsr(hdout * hdin)) / (fabsr(din.z) * sd * sd));
 }
 
 /* Disney evalPdf, disney.hlsli:141-231 */
-static float disney_pdf(const material_t* m, f3 din, f3 dout) {
-    float alpha = pow2(M_ROUGH(m)), metallic = M_METAL(m), aspect = sqrtf(1 - M_ANISO(m) * 0.9f);
-    float ax = maxf(0.0001f, alpha / aspect), ay = maxf(0.0001f, alpha * aspect);
-    float ac = (1 - M_CCGLOSS(m)) * 0.1f + M_CCGLOSS(m) * 0.001f, tr = M_TRANS(m);
-    float dw = (1 - metallic) * (1 - tr), mw = (1 - tr * (1 - metallic)), gw = (1 - metallic) * tr, cw = M_CC(m);
-    float tw = dw + mw + gw + cw;
-    float dp = dw / tw, mp = mw / tw, gp = gw / tw, cp = cw / tw;
+static real disney_pdf(const material_t* m, f3 din, f3 dout) {
+    real alpha = pow2(M_ROUGH(m)), metallic = M_METAL(m), aspect = sqrtr(1 - M_ANISO(m) * 0.9f);
+    real ax = maxf(0.0001f, alpha / aspect), ay = maxf(0.0001f, alpha * aspect);
+    real ac = (1 - M_CCGLOSS(m)) * 0.1f + M_CCGLOSS(m) * 0.001f, tr = M_TRANS(m);
+    real dw = (1 - metallic) * (1 - tr), mw = (1 - tr * (1 - metallic)), gw = (1 - metallic) * tr, cw = M_CC(m);
+    real tw = dw + mw + gw + cw;
+    real dp = dw / tw, mp = mw / tw, gp = gw / tw, cp = cw / tw;
     if (din.z < 0) {
         dp = 0; mp = 0; cp = 0;
         if (gp > 0) gp = 1;
@@ -769,48 +816,48 @@ static float disney_pdf(const material_t* m, f3 din, f3 dout) {
     if (din.z * dout.z > 0) {
         f3 h = normalize3(add(din, dout));
         if (h.z < 0) h = neg(h);
-        float ndin = din.z, ndh = h.z, hdin = dot(h, din), hdout = dot(h, dout);
+        real ndin = din.z, ndh = h.z, hdin = dot(h, din), hdout = dot(h, dout);
         if (din.z >= 0 && dout.z >= 0) {
             dp *= maxf(dout.z, 0.0f) / OR_PI;
             if (ndin > 0) {
-                float D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay);
+                real D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay);
                 mp *= (D * Gi / (4 * ndin));
             } else mp = 0;
             if (ndh > 0 && hdout > 0) {
-                float Dc = GTR1(ndh, ac);
+                real Dc = GTR1(ndh, ac);
                 cp *= (Dc * ndh / (4 * hdout));
             } else cp = 0;
         }
-        float eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
-        float Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay);
-        gp *= (Fg * D * Gi / (4 * fabsf(ndin)));
+        real eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+        real Fg = fresnel_dielectric(hdin, eta), D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay);
+        gp *= (Fg * D * Gi / (4 * fabsr(ndin)));
     } else {
-        float eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+        real eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
         f3 h = normalize3(add(din, mulf(dout, eta)));
         if (h.z < 0) h = neg(h);
-        float hdin = dot(h, din), hdout = dot(h, dout);
-        float D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay), Fg = fresnel_dielectric(hdin, eta);
-        float sd = hdin + eta * hdout, dh = eta * eta * hdout / (sd * sd);
-        gp *= (1 - Fg) * D * Gi * fabsf(dh * hdin / din.z);
+        real hdin = dot(h, din), hdout = dot(h, dout);
+        real D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay), Fg = fresnel_dielectric(hdin, eta);
+        real sd = hdin + eta * hdout, dh = eta * eta * hdout / (sd * sd);
+        gp *= (1 - Fg) * D * Gi * fabsr(dh * hdin / din.z);
     }
     return dp + mp + gp + cp;
 }
 
-typedef struct { f3 dir, f; float fwd, rev, eta, rough; } dsr_t;   /* DirectionSampleRecord */
-typedef struct { f3 f; float fwd, rev; } rer_t;                    /* ReflectanceEvalRecord */
+typedef struct { f3 dir, f; real fwd, rev, eta, rough; } dsr_t;   /* DirectionSampleRecord */
+typedef struct { f3 f; real fwd, rev; } rer_t;                    /* ReflectanceEvalRecord */
 
 /* Disney sample, disney.hlsli:233-380 (returns direction, eta, roughness) */
 static dsr_t disney_sample(const material_t* m, f3 din, f3 rnd) {
     dsr_t r;
     memset(&r, 0, sizeof r);
-    float aspect = sqrtf(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
-    float ax = maxf(0.0001f, alpha / aspect), ay = maxf(0.0001f, alpha * aspect);
-    float ac = (1 - M_CCGLOSS(m)) * 0.1f + M_CCGLOSS(m) * 0.001f;
-    float metallic = M_METAL(m), tr = M_TRANS(m);
-    float dw = (1 - metallic) * (1 - tr), mw = (1 - tr * (1 - metallic)), gw = (1 - metallic) * tr, cw = M_CC(m);
+    real aspect = sqrtr(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
+    real ax = maxf(0.0001f, alpha / aspect), ay = maxf(0.0001f, alpha * aspect);
+    real ac = (1 - M_CCGLOSS(m)) * 0.1f + M_CCGLOSS(m) * 0.001f;
+    real metallic = M_METAL(m), tr = M_TRANS(m);
+    real dw = (1 - metallic) * (1 - tr), mw = (1 - tr * (1 - metallic)), gw = (1 - metallic) * tr, cw = M_CC(m);
     if (din.z >= 0) {
-        float tw = dw + mw + gw + cw;
-        float dp = dw / tw, mp = mw / tw, gp = gw / tw;
+        real tw = dw + mw + gw + cw;
+        real dp = dw / tw, mp = mw / tw, gp = gw / tw;
         if (rnd.z <= dp) {
             r.dir = sample_cos_hemisphere(rnd.x, rnd.y);
             r.eta = 0; r.rough = 1;
@@ -823,59 +870,115 @@ static dsr_t disney_sample(const material_t* m, f3 din, f3 rnd) {
         } else if (rnd.z <= (dp + mp + gp)) {
             if (gp <= 0) return r;
             f3 h = sample_vndf(din, ax, ay, rnd.x, rnd.y);
-            float hdin = dot(h, din), eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
-            float F = fresnel_dielectric(hdin, eta), u = (rnd.z - (dp + mp)) / gp;
+            real hdin = dot(h, din), eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+            real F = fresnel_dielectric(hdin, eta), u = (rnd.z - (dp + mp)) / gp;
             if (u <= F) {
                 r.dir = normalize3(add(neg(din), mulf(h, 2 * dot(din, h))));
                 r.eta = 0;
                 return r; /* roughness stays 0: reference returns before setting it */
             }
-            float hdo2 = 1 - (1 - hdin * hdin) / (eta * eta);
+            real hdo2 = 1 - (1 - hdin * hdin) / (eta * eta);
             if (hdo2 <= 0) { memset(&r, 0, sizeof r); return r; }
             if (hdin < 0) h = neg(h);
-            float hdo = sqrtf(hdo2);
-            r.dir = add(divf(neg(din), eta), mulf(h, fabsf(hdin) / eta - hdo));
+            real hdo = sqrtr(hdo2);
+            r.dir = add(divf(neg(din), eta), mulf(h, fabsr(hdin) / eta - hdo));
             r.eta = eta;
             r.rough = M_ROUGH(m);
             return r;
         } else {
-            float a2 = ac * ac;
-            float ch = sqrtf(maxf(0.0f, (1 - om_pow(a2, 1 - rnd.x)) / (1 - a2)));
-            float sh = sqrtf(maxf(1 - ch * ch, 0.0f)), az = 2 * OR_PI * rnd.y;
+            real a2 = ac * ac;
+            real ch = sqrtr(maxf(0.0f, (1 - om_pow(a2, 1 - rnd.x)) / (1 - a2)));
+            real sh = sqrtr(maxf(1 - ch * ch, 0.0f)), az = 2 * OR_PI * rnd.y;
             f3 mn = v3(sh * om_cos(az), sh * om_sin(az), ch);
             r.dir = normalize3(add(neg(din), mulf(mn, 2 * dot(din, mn))));
             r.eta = 0;
-            r.rough = sqrtf(ac);
+            r.rough = sqrtr(ac);
             return r;
         }
     }
     f3 h = sample_vndf(din, ax, ay, rnd.x, rnd.y);
     if (h.z < 0) h = neg(h);
-    float hdin = dot(h, din), eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
-    float F = fresnel_dielectric(hdin, eta);
+    real hdin = dot(h, din), eta = din.z > 0 ? M_ETA1(m) : 1 / M_ETA1(m);
+    real F = fresnel_dielectric(hdin, eta);
     if (rnd.z <= F) {
         r.dir = normalize3(add(neg(din), mulf(h, 2 * dot(din, h))));
         r.eta = 0;
     } else {
-        float hdo2 = 1 - (1 - hdin * hdin) / (eta * eta);
+        real hdo2 = 1 - (1 - hdin * hdin) / (eta * eta);
         if (hdo2 <= 0) { memset(&r, 0, sizeof r); return r; }
         if (hdin < 0) h = neg(h);
-        float hdo = sqrtf(hdo2);
-        r.dir = add(divf(neg(din), eta), mulf(h, fabsf(hdin) / eta - hdo));
+        real hdo = sqrtr(hdo2);
+        r.dir = add(divf(neg(din), eta), mulf(h, fabsr(hdin) / eta - hdo));
         r.eta = eta;
     }
     r.rough = M_ROUGH(m);
     return r;
 }
 
-/* evaluateReflectance for Disney (disney.hlsli:391-403) or Lambert (lambertian.hlsli:13-24) */
-static rer_t bsdf_eval_x(const material_t* m, int lambert, f3 din, f3 dout, int adj) {
+/* Visible-normal sampling as Heitz publishes it (JCGT 7(4), 2018, listing 1):
+ * the disk frame's first axis horizontal, its second in the plane of the
+ * normal and the view.  sample_vndf (the reference's) builds the frame with
+ * makeOrthonormal, whose axes are not so aligned away from normal incidence,
+ * so its directions are not distributed as the pdf it reports. */
+static f3 sample_vndf_heitz(f3 din, real ax, real ay, real r0, real r1) {
+    const int flip = din.z < 0;
+    if (flip) din = neg(din);
+    const f3 vh = normalize3(v3(ax * din.x, ay * din.y, din.z));
+    const real lensq = vh.x * vh.x + vh.y * vh.y;
+    const f3 e1 = lensq > 0 ? divf(v3(-vh.y, vh.x, 0), sqrtr(lensq)) : v3(1, 0, 0);
+    const f3 e2 = cross(vh, e1);
+    const real r = sqrtr(r0), phi = 2 * OR_PI * r1;
+    real t1 = r * om_cos(phi), t2 = r * om_sin(phi);
+    const real s = (1 + vh.z) / 2;
+    t2 = (1 - s) * sqrtr(1 - t1 * t1) + s * t2;
+    const f3 nh = add(add(mulf(e1, t1), mulf(e2, t2)), mulf(vh, sqrtr(maxf(0.0f, 1 - t1 * t1 - t2 * t2))));
+    const f3 o = normalize3(v3(ax * nh.x, ay * nh.y, maxf(0.0f, nh.z)));
+    return flip ? neg(o) : o;
+}
+/* The GGX conductor of ST_FLAG_LAMBERT_GGX: the Disney BSDF's metallic lobe
+ * alone (disney.hlsli:56-70 and :163-169 at metallic 1, no clearcoat), F
+ * Schlick's about the base colour, reflection only, two-sided. */
+static rer_t ggx_eval(const material_t* m, f3 din, f3 dout) {
     rer_t r;
+    r.f = v3(0, 0, 0); r.fwd = r.rev = 0;
+    if (!(din.z * dout.z > 0)) return r;
+    if (din.z < 0) { din = neg(din); dout = neg(dout); }
+    const real aspect = sqrtr(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
+    const real ax = maxf(1e-4f, alpha / aspect), ay = maxf(1e-4f, alpha * aspect);
+    const f3 h = normalize3(add(din, dout));
+    const real D = GTR2a(h, ax, ay), Gi = smith_gtr2(din, ax, ay), Go = smith_gtr2(dout, ax, ay);
+    const f3 F = schlick3(m_base(m), dot(h, dout));
+    r.f = divf(mulf(F, D * Gi * Go), 4 * din.z);
+    r.fwd = D * Gi / (4 * din.z);
+    r.rev = D * Go / (4 * dout.z);
+    return r;
+}
+static dsr_t ggx_sample(const material_t* m, f3 din, f3 rnd) {
+    const real aspect = sqrtr(1 - M_ANISO(m) * 0.9f), alpha = pow2(M_ROUGH(m));
+    const real ax = maxf(1e-4f, alpha / aspect), ay = maxf(1e-4f, alpha * aspect);
+    const f3 h = sample_vndf_heitz(din, ax, ay, rnd.x, rnd.y);
+    dsr_t r;
+    r.dir = normalize3(add(neg(din), mulf(h, 2 * dot(din, h))));
+    const rer_t f = ggx_eval(m, din, r.dir);
+    r.f = f.f; r.fwd = f.fwd; r.rev = f.rev;
+    r.eta = 0; r.rough = M_ROUGH(m);
+    return r;
+}
+/* the model of a material: 0 Disney, 1 Lambert, 2 GGX conductor.  mode: the
+ * configuration's (0 Disney, 1 all Lambert, 2 per material by metallic) */
+static int m_model(const material_t* m, int mode) { return mode == 2 ? (M_METAL(m) >= 0.5f ? 2 : 1) : mode; }
+
+/* evaluateReflectance for Disney (disney.hlsli:391-403) or Lambert (lambertian.hlsli:13-24),
+ * or per material Lambert or the GGX conductor */
+static rer_t bsdf_eval_x(const material_t* m, int mode, f3 din, f3 dout, int adj) {
+    rer_t r;
+    const int lambert = m_model(m, mode) == 1;
+    if (m_model(m, mode) == 2) return ggx_eval(m, din, dout);
     if (lambert) {
         if (din.z * dout.z < 0) { r.f = v3(0, 0, 0); r.fwd = r.rev = 0; return r; }
-        r.f = divf(mulf(m_base(m), fabsf(dout.z)), OR_PI);
-        r.fwd = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(dout.z)));
-        r.rev = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(din.z)));
+        r.f = divf(mulf(m_base(m), fabsr(dout.z)), OR_PI);
+        r.fwd = cos_hemisphere_pdfw(maxf(1e-5f, fabsr(dout.z)));
+        r.rev = cos_hemisphere_pdfw(maxf(1e-5f, fabsr(din.z)));
         return r;
     }
     r.f = disney_eval(m, din, dout, adj);
@@ -885,14 +988,16 @@ static rer_t bsdf_eval_x(const material_t* m, int lambert, f3 din, f3 dout, int 
 }
 /* sampleDirection (disney.hlsli:415-429, lambertian.hlsli:29-39) */
 static rer_t bsdf_eval(const material_t* m, int lambert, f3 din, f3 dout) { return bsdf_eval_x(m, lambert, din, dout, 0); }
-static dsr_t bsdf_sample_x(const material_t* m, int lambert, f3 rnd, f3 din, int adj) {
+static dsr_t bsdf_sample_x(const material_t* m, int mode, f3 rnd, f3 din, int adj) {
     dsr_t r;
+    const int lambert = m_model(m, mode) == 1;
+    if (m_model(m, mode) == 2) return ggx_sample(m, din, rnd);
     if (lambert) {
         r.dir = sample_cos_hemisphere(rnd.x, rnd.y);
         if (din.z < 0) r.dir = neg(r.dir);
-        r.f = divf(mulf(m_base(m), fabsf(r.dir.z)), OR_PI);
-        r.fwd = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(r.dir.z)));
-        r.rev = cos_hemisphere_pdfw(maxf(1e-5f, fabsf(din.z)));
+        r.f = divf(mulf(m_base(m), fabsr(r.dir.z)), OR_PI);
+        r.fwd = cos_hemisphere_pdfw(maxf(1e-5f, fabsr(r.dir.z)));
+        r.rev = cos_hemisphere_pdfw(maxf(1e-5f, fabsr(din.z)));
         r.rough = 1; r.eta = 0;
         return r;
     }
@@ -902,12 +1007,14 @@ static dsr_t bsdf_sample_x(const material_t* m, int lambert, f3 rnd, f3 din, int
     return r;
 }
 static dsr_t bsdf_sample(const material_t* m, int lambert, f3 rnd, f3 din) { return bsdf_sample_x(m, lambert, rnd, din, 0); }
-static int m_singular(const material_t* m, int lambert) {
-    if (lambert) return 0;
+static int m_singular(const material_t* m, int mode) {
+    const int model = m_model(m, mode);
+    if (model == 1) return 0;
+    if (model == 2) return (1 - M_ROUGH(m)) > 0.75f; /* isSingular at metallic 1 */
     return maxf(maxf(M_SPEC(m), M_METAL(m)), M_TRANS(m)) * (1 - M_ROUGH(m)) > 0.75f;
 }
 static int m_can_eval(const material_t* m) { f3 b = m_base(m); return any_gt0(b); }
-static float m_cont_prob(const material_t* m) { return saturate(luminance(m_base(m))); }
+static real m_cont_prob(const material_t* m) { return saturate(luminance(m_base(m))); }
 
 void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], const float dout[3], float o[5]) {
     material_t m;
@@ -938,7 +1045,7 @@ void or_bsdf_sample(const uint32_t mat[5], int lambertian, const float rnd[3], c
 
 /* ------------------------------------------------------- environment */
 static const float* texel(const level_t* l, int x, int y) { return l->rgba + 4 * ((size_t)y * l->w + x); }
-static float texel_lum(const level_t* l, int x, int y) {
+static real texel_lum(const level_t* l, int x, int y) {
     const float* t = texel(l, x, y);
     return luminance(v3(t[0], t[1], t[2]));
 }
@@ -956,7 +1063,7 @@ static void build_mips(or_scene* s) {
                 int x0 = a->w > 1 ? 2 * x : 0, x1 = a->w > 1 ? 2 * x + 1 : 0;
                 int y0 = a->h > 1 ? 2 * y : 0, y1 = a->h > 1 ? 2 * y + 1 : 0;
                 for (int c = 0; c < 4; c++) {
-                    float v = ((texel(a, x0, y0)[c] + texel(a, x1, y0)[c]) + (texel(a, x0, y1)[c] + texel(a, x1, y1)[c])) * 0.25f;
+                    real v = ((texel(a, x0, y0)[c] + texel(a, x1, y0)[c]) + (texel(a, x0, y1)[c] + texel(a, x1, y1)[c])) * 0.25f;
                     b->rgba[4 * ((size_t)y * b->w + x) + c] = v;
                 }
             }
@@ -965,43 +1072,43 @@ static void build_mips(or_scene* s) {
     s->nlevels = l + 1;
 }
 /* bilinear SampleLevel(uv, 0) with repeat addressing (TestRenderer.cpp:54-57) */
-static f3 env_bilinear(const level_t* l, float u, float v) {
-    float x = u * (float)l->w - 0.5f, y = v * (float)l->h - 0.5f;
-    float fx0 = floorf(x), fy0 = floorf(y), ax = x - fx0, ay = y - fy0;
+static f3 env_bilinear(const level_t* l, real u, real v) {
+    real x = u * (real)l->w - 0.5f, y = v * (real)l->h - 0.5f;
+    real fx0 = floorr(x), fy0 = floorr(y), ax = x - fx0, ay = y - fy0;
     int x0 = (int)fx0, y0 = (int)fy0;
     int xa = ((x0 % l->w) + l->w) % l->w, xb = (((x0 + 1) % l->w) + l->w) % l->w;
     int ya = ((y0 % l->h) + l->h) % l->h, yb = (((y0 + 1) % l->h) + l->h) % l->h;
     const float *t00 = texel(l, xa, ya), *t10 = texel(l, xb, ya), *t01 = texel(l, xa, yb), *t11 = texel(l, xb, yb);
     f3 r;
-    float* rr = &r.x;
+    real* rr = &r.x;
     for (int c = 0; c < 3; c++) {
-        float top = t00[c] * (1 - ax) + t10[c] * ax;
-        float bot = t01[c] * (1 - ax) + t11[c] * ax;
+        real top = t00[c] * (1 - ax) + t10[c] * ax;
+        real bot = t01[c] * (1 - ax) + t11[c] * ax;
         rr[c] = top * (1 - ay) + bot * ay;
     }
     return r;
 }
 /* SampleTexel, src/Shaders/common/lights.hlsli:49-97 */
-static void sample_texel(const or_scene* s, float rx, float ry, float uv[2], float* pdf) {
+static void sample_texel(const or_scene* s, real rx, real ry, real uv[2], real* pdf) {
     int levelCount = s->nlevels, cx = 0, cy = 0, lw = 1, lh = 1;
     *pdf = 1;
     int iend = levelCount - 1 < 11 ? levelCount - 1 : 11;
     for (int i = 1; i < iend; i++) {
         const level_t* L = &s->lv[levelCount - 1 - i];
-        float inv_h = 1 / (float)L->h;
+        real inv_h = 1 / (real)L->h;
         cx *= L->w / lw; cy *= L->h / lh;
-        float p[4] = {0, 0, 0, 0};
+        real p[4] = {0, 0, 0, 0};
         if ((uint32_t)L->w - (uint32_t)cx > 1) {
-            float sy = om_sin(OR_PI * ((float)cy + 0.5f) * inv_h);
+            real sy = om_sin(OR_PI * ((real)cy + 0.5f) * inv_h);
             p[0] = texel_lum(L, cx, cy) * sy;
             p[1] = texel_lum(L, cx + 1, cy) * sy;
         }
         if ((uint32_t)L->h - (uint32_t)cy > 1) {
-            float sy = om_sin(OR_PI * ((float)cy + 1.5f) * inv_h);
+            real sy = om_sin(OR_PI * ((real)cy + 1.5f) * inv_h);
             p[2] = texel_lum(L, cx, cy + 1) * sy;
             p[3] = texel_lum(L, cx + 1, cy + 1) * sy;
         }
-        float sum = p[0] + p[1] + p[2] + p[3];
+        real sum = p[0] + p[1] + p[2] + p[3];
         if (sum < 1e-6f) continue;
         for (int j = 0; j < 4; j++) p[j] = p[j] / sum;
         for (int j = 0; j < 4; j++) {
@@ -1015,39 +1122,39 @@ static void sample_texel(const or_scene* s, float rx, float ry, float uv[2], flo
         }
         lw = L->w; lh = L->h;
     }
-    *pdf *= (float)((uint32_t)lw * (uint32_t)lh);
-    uv[0] = ((float)cx + rx) / (float)lw;
-    uv[1] = ((float)cy + ry) / (float)lh;
+    *pdf *= (real)((uint32_t)lw * (uint32_t)lh);
+    uv[0] = ((real)cx + rx) / (real)lw;
+    uv[1] = ((real)cy + ry) / (real)lh;
 }
 /* SampleTexelPdf, lights.hlsli:98-139 */
-static float sample_texel_pdf(const or_scene* s, float u, float v) {
+static real sample_texel_pdf(const or_scene* s, real u, real v) {
     int levelCount = s->nlevels, lw = 1, lh = 1;
-    float pdf = 1;
+    real pdf = 1;
     int iend = levelCount - 1 < 11 ? levelCount - 1 : 11;
     for (int i = 1; i < iend; i++) {
         const level_t* L = &s->lv[levelCount - 1 - i];
-        float inv_h = 1 / (float)L->h;
-        int ux = (int)((float)L->w * u), uy = (int)((float)L->h * v);
+        real inv_h = 1 / (real)L->h;
+        int ux = (int)((real)L->w * u), uy = (int)((real)L->h * v);
         int cx = (ux / 2) * 2, cy = (uy / 2) * 2;
         int ox = ux - cx < 1 ? ux - cx : 1, oy = uy - cy < 1 ? uy - cy : 1;
-        float p[4] = {0, 0, 0, 0};
+        real p[4] = {0, 0, 0, 0};
         if ((uint32_t)L->w - (uint32_t)cx > 1) {
-            float sy = om_sin(OR_PI * ((float)cy + 0.5f) * inv_h);
+            real sy = om_sin(OR_PI * ((real)cy + 0.5f) * inv_h);
             p[0] = texel_lum(L, cx, cy) * sy;
             p[1] = texel_lum(L, cx + 1, cy) * sy;
         }
         if ((uint32_t)L->h - (uint32_t)cy > 1) {
-            float sy = om_sin(OR_PI * ((float)cy + 1.5f) * inv_h);
+            real sy = om_sin(OR_PI * ((real)cy + 1.5f) * inv_h);
             p[2] = texel_lum(L, cx, cy + 1) * sy;
             p[3] = texel_lum(L, cx + 1, cy + 1) * sy;
         }
-        float sum = p[0] + p[1] + p[2] + p[3];
+        real sum = p[0] + p[1] + p[2] + p[3];
         if (sum < 1e-6f) continue;
         for (int j = 0; j < 4; j++) p[j] = p[j] / sum;
         pdf *= p[oy * 2 + ox];
         lw = L->w; lh = L->h;
     }
-    return pdf * (float)((uint32_t)lw * (uint32_t)lh);
+    return pdf * (real)((uint32_t)lw * (uint32_t)lh);
 }
 /* Alias table over the full-resolution map (the north-star environment
  * sampler, ST_FLAG_ENV_ALIAS; the reference warps the mip pyramid instead).
@@ -1089,13 +1196,13 @@ static int build_alias(or_scene* s) {
     size_t ns = 0, nl = 0;
     for (size_t i = 0; i < N; i++) {
         sc[i] = wgt[i] / total * N;
-        s->alias_pdf[i] = (float)(wgt[i] / total * N);
+        s->alias_pdf[i] = (real)(wgt[i] / total * N);
         if (sc[i] < 1) small[ns++] = (uint32_t)i;
         else large[nl++] = (uint32_t)i;
     }
     while (ns > 0 && nl > 0) {
         const uint32_t a = small[--ns], g = large[nl - 1];
-        s->alias_prob[a] = (float)sc[a];
+        s->alias_prob[a] = (real)sc[a];
         s->alias_idx[a] = g;
         sc[g] = (sc[g] + sc[a]) - 1;
         if (sc[g] < 1) {
@@ -1112,26 +1219,26 @@ static int build_alias(or_scene* s) {
      * evaluated at its point */
     int lg = 0;
     while ((1 << lg) < (l->w > l->h ? l->w : l->h)) lg++;
-    s->alias_jmax = 1.0f - ldexpf(1.0f, lg - 21);
+    s->alias_jmax = 1.0f - ldexpr(1.0f, lg - 21);
     return 0;
 }
 /* One texel by the alias table, a uniform point inside it.  rx picks the
  * column, ra is the accept test, ju / jv place the point in the texel; *pdf
  * is the uv-space density of the texel drawn. */
-static void sample_alias(const or_scene* s, float rx, float ra, float ju, float jv, float uv[2], float* pdf) {
+static void sample_alias(const or_scene* s, real rx, real ra, real ju, real jv, real uv[2], real* pdf) {
     const int W = s->lv[0].w, H = s->lv[0].h;
     const uint32_t N = (uint32_t)W * (uint32_t)H;
-    uint32_t i = (uint32_t)(rx * (float)N);
+    uint32_t i = (uint32_t)(rx * (real)N);
     if (i >= N) i = N - 1;
     if (!(ra < s->alias_prob[i])) i = s->alias_idx[i];
     *pdf = s->alias_pdf[i];
-    uv[0] = ((float)(i % (uint32_t)W) + minf(ju, s->alias_jmax)) / (float)W;
-    uv[1] = ((float)(i / (uint32_t)W) + minf(jv, s->alias_jmax)) / (float)H;
+    uv[0] = ((real)(i % (uint32_t)W) + minf(ju, s->alias_jmax)) / (real)W;
+    uv[1] = ((real)(i / (uint32_t)W) + minf(jv, s->alias_jmax)) / (real)H;
 }
 /* uv-space density of the alias sampler at uv */
-static float alias_pdf(const or_scene* s, float u, float v) {
+static real alias_pdf(const or_scene* s, real u, real v) {
     const int W = s->lv[0].w, H = s->lv[0].h;
-    int x = (int)(u * (float)W), y = (int)(v * (float)H);
+    int x = (int)(u * (real)W), y = (int)(v * (real)H);
     x = x < 0 ? 0 : (x >= W ? W - 1 : x);
     y = y < 0 ? 0 : (y >= H ? H - 1 : y);
     return s->alias_pdf[(size_t)y * W + x];
@@ -1139,7 +1246,13 @@ static float alias_pdf(const or_scene* s, float u, float v) {
 int or_env_alias_sample(or_scene* s, uint32_t n, const float* rnd, float* uv, float* pdf) {
     if (!s->env_has_image) return 1;
     for (uint32_t i = 0; i < n; i++)
-        sample_alias(s, rnd[4 * i], rnd[4 * i + 1], rnd[4 * i + 2], rnd[4 * i + 3], uv + 2 * i, pdf + i);
+    {
+        real u[2], pd;
+        sample_alias(s, rnd[4 * i], rnd[4 * i + 1], rnd[4 * i + 2], rnd[4 * i + 3], u, &pd);
+        uv[2 * i] = u[0];
+        uv[2 * i + 1] = u[1];
+        pdf[i] = pd;
+    }
     return 0;
 }
 float or_env_alias_pdf(or_scene* s, const float uv[2]) { return s->env_has_image ? alias_pdf(s, uv[0], uv[1]) : 0; }
@@ -1153,7 +1266,11 @@ int or_env_alias_table(or_scene* s, float* prob, uint32_t* alias, float* pdf) {
 }
 int or_env_sample_texel(or_scene* s, const float rnd[2], float uv[2], float* pdf) {
     if (!s->env_has_image) return 1;
-    sample_texel(s, rnd[0], rnd[1], uv, pdf);
+    real u[2], pd;
+    sample_texel(s, rnd[0], rnd[1], u, &pd);
+    uv[0] = u[0];
+    uv[1] = u[1];
+    *pdf = pd;
     return 0;
 }
 float or_env_sample_texel_pdf(or_scene* s, const float uv[2]) {
@@ -1162,28 +1279,28 @@ float or_env_sample_texel_pdf(or_scene* s, const float uv[2]) {
 
 typedef struct {
     uint32_t flags;
-    float env_prob;
+    real env_prob;
     uint32_t max_depth, min_len, cand, max_null, seed_px, seed_py, seed;
     uint32_t rr_start; /* Russian roulette from this path length on (0, 1: every vertex) */
-    int lambert;
-    float max_m;       /* mMaxM (ReSTIR reuse: M clamped at max_m x candidates) */
+    int lambert;      /* 0 Disney, 1 every material Lambert (gLambertian), 2 per material Lambert / GGX */
+    real max_m;       /* mMaxM (ReSTIR reuse: M clamped at max_m x candidates) */
 } cfg_t;
 
 /* The continuation probability folded into the BSDF / phase pdfs at a vertex
  * of path length plen: Russian roulette runs at every vertex in the reference
  * (testrenderer.slang:890-892); the north-star mode starts it at rr_start. */
-static float rr_cp(const cfg_t* c, uint32_t plen, float cp) { return plen >= c->rr_start ? cp : 1.0f; }
+static real rr_cp(const cfg_t* c, uint32_t plen, real cp) { return plen >= c->rr_start ? cp : 1.0f; }
 
 /* EvaluateEnvironment, lights.hlsli:143-163 */
-static f3 evaluate_environment(const or_scene* s, const cfg_t* c, f3 d, float* pdfw) {
+static f3 evaluate_environment(const or_scene* s, const cfg_t* c, f3 d, real* pdfw) {
     if (!s->has_env) { *pdfw = 0; return v3(0, 0, 0); }
     f3 e = s->env_emission;
     if (s->env_has_image) {
-        float uv[2];
+        real uv[2];
         cart_to_uv(d, uv);
         e = mul(e, env_bilinear(&s->lv[0], uv[0], uv[1]));
-        const float pu = (c->flags & ST_FLAG_ENV_ALIAS) ? alias_pdf(s, uv[0], uv[1]) : sample_texel_pdf(s, uv[0], uv[1]);
-        *pdfw = pu / (2 * OR_PI * OR_PI * sqrtf(1 - d.y * d.y));
+        const real pu = (c->flags & ST_FLAG_ENV_ALIAS) ? alias_pdf(s, uv[0], uv[1]) : sample_texel_pdf(s, uv[0], uv[1]);
+        *pdfw = pu / (2 * OR_PI * OR_PI * sqrtr(1 - d.y * d.y));
     } else {
         *pdfw = 1 / (4 * OR_PI);
     }
@@ -1193,7 +1310,7 @@ static f3 evaluate_environment(const or_scene* s, const cfg_t* c, f3 d, float* p
 
 typedef struct {
     f3 radiance, dir, pos;
-    float pdf, dist, cos_light;
+    real pdf, dist, cos_light;
     uint32_t normal;
     int finite;
 } isr_t; /* IlluminationSampleRecord, lights.hlsli:32-45 */
@@ -1201,27 +1318,27 @@ typedef struct {
 /* SampleEmission + SampleIllumination, lights.hlsli:166-260.  The reference
  * leaves mDirectionToLight unset for environment samples (lights.hlsli:234-244);
  * here it is the sampled direction. */
-static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const float rnd_in[4], f3 ref) {
+static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const real rnd_in[4], f3 ref) {
     isr_t r;
     memset(&r, 0, sizeof r);
-    float rnd[4] = {rnd_in[0], rnd_in[1], rnd_in[2], rnd_in[3]};
+    real rnd[4] = {rnd_in[0], rnd_in[1], rnd_in[2], rnd_in[3]};
     if (s->has_env) {
         if (s->nlights == 0 || rnd[3] < c->env_prob) {
             f3 d;
-            float pdf;
+            real pdf;
             const int alias = (c->flags & ST_FLAG_ENV_ALIAS) && s->env_has_image;
-            float uv[2] = {0, 0};
+            real uv[2] = {0, 0};
             if (s->env_has_image) {
                 if (alias) {
                     /* rnd[2] is free on this branch; rnd[3], rescaled past the
                      * environment-or-emitter choice, is uniform again */
-                    const float ju = s->nlights > 0 ? rnd[3] / c->env_prob : rnd[3];
+                    const real ju = s->nlights > 0 ? rnd[3] / c->env_prob : rnd[3];
                     sample_alias(s, rnd[0], rnd[2], ju, rnd[1], uv, &pdf);
                 } else {
                     sample_texel(s, rnd[0], rnd[1], uv, &pdf);
                 }
                 d = uv_to_cart(uv[0], uv[1]);
-                pdf /= (2 * OR_PI * OR_PI * sqrtf(1 - pow2(d.y)));
+                pdf /= (2 * OR_PI * OR_PI * sqrtr(1 - pow2(d.y)));
             } else {
                 d = sample_uniform_sphere(rnd[0], rnd[1]);
                 pdf = 1 / (4 * OR_PI);
@@ -1238,7 +1355,7 @@ static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const float 
                  * direction -> uv round trip */
                 r.radiance = mul(s->env_emission, env_bilinear(&s->lv[0], uv[0], uv[1]));
             } else {
-                float tmp;
+                real tmp;
                 r.radiance = evaluate_environment(s, c, d, &tmp);
             }
             return r;
@@ -1246,16 +1363,16 @@ static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const float 
         if (s->nlights > 0) rnd[3] = (rnd[3] - c->env_prob) / (1 - c->env_prob);
     }
     if (s->nlights == 0) return r;
-    uint32_t li = (uint32_t)(rnd[2] * (float)s->nlights) % s->nlights;
+    uint32_t li = (uint32_t)(rnd[2] * (real)s->nlights) % s->nlights;
     uint32_t ii = s->light_inst[li];
-    float pdf = 1 / (float)s->nlights;
+    real pdf = 1 / (real)s->nlights;
     if (s->has_env) pdf *= 1 - c->env_prob;
     const st_instance* in = &s->inst[ii];
     if ((in->type_material_address & 0xfu) != 0) { memset(&r, 0, sizeof r); return r; }
     uint32_t pc = in->data >> 16;
-    uint32_t prim = (uint32_t)(rnd[3] * (float)pc) % pc;
-    pdf /= (float)pc;
-    float b[2];
+    uint32_t prim = (uint32_t)(rnd[3] * (real)pc) % pc;
+    pdf /= (real)pc;
+    real b[2];
     sample_uniform_triangle(rnd[0], rnd[1], b);
     shading_t sd;
     make_tri_shading(s, ii, prim, b[0], b[1], &sd);
@@ -1304,7 +1421,7 @@ static uint64_t nvdb_addr(const uint8_t* b, int i, int j, int k) {
     }
     return root + 28; /* background */
 }
-static float nvdb_read(const uint8_t* b, int i, int j, int k) { return rdf(b, nvdb_addr(b, i, j, k)); }
+static real nvdb_read(const uint8_t* b, int i, int j, int k) { return rdf(b, nvdb_addr(b, i, j, k)); }
 static f3 nvdb_map(const uint8_t* b, uint64_t off, f3 v, int translate, int inverse) {
     const uint64_t m = 296 + off;
     if (inverse && translate) v = v3(v.x - rdf(b, 296 + 72), v.y - rdf(b, 296 + 76), v.z - rdf(b, 296 + 80));
@@ -1328,7 +1445,7 @@ void or_nvdb_world_to_index(const void* grid, const float p[3], int dir, float o
 typedef struct {
     f3 density;
     uint32_t albedo_packed, dens_vol, alb_vol;
-    float g;
+    real g;
 } medium_t;
 static medium_t load_medium(const or_scene* s, uint32_t addr) {
     medium_t m;
@@ -1344,21 +1461,21 @@ static medium_t load_medium(const or_scene* s, uint32_t addr) {
 }
 static f3 med_albedo(const medium_t* m) { return v3(un8(m->albedo_packed, 0), un8(m->albedo_packed, 8), un8(m->albedo_packed, 16)); }
 static int med_can_eval(const medium_t* m) { return any_gt0(m->density) && m->albedo_packed > 0; }
-static int med_singular(const medium_t* m) { return fabsf(m->g) > 0.999f; }
-static float med_cont_prob(const medium_t* m) { return saturate(luminance(mul(med_albedo(m), m->density))); }
+static int med_singular(const medium_t* m) { return fabsr(m->g) > 0.999f; }
+static real med_cont_prob(const medium_t* m) { return saturate(luminance(mul(med_albedo(m), m->density))); }
 /* Henyey-Greenstein evaluateReflectance / sampleDirection, medium.hlsli:44-78 */
-static float hg_eval(float g, f3 din, f3 dout) {
+static real hg_eval(real g, f3 din, f3 dout) {
     /* pow(d, 1.5) as d * sqrt(d), as the GPU library takes it */
-    const float d = 1 + g * g + 2 * g * dot(din, dout);
-    return 1 / (4 * OR_PI) * (1 - g * g) / (d * sqrtf(d));
+    const real d = 1 + g * g + 2 * g * dot(din, dout);
+    return 1 / (4 * OR_PI) * (1 - g * g) / (d * sqrtr(d));
 }
-static f3 hg_sample(float g, float r0, float r1, f3 din) {
-    if (fabsf(g) < 1e-4f) return sample_uniform_sphere(r0, r1);
-    const float g2 = g * g;
-    const float tmp = (g2 - 1) / (2 * r0 * g - (g + 1));
-    const float ce = (tmp * tmp - (1 + g2)) / (2 * g);
-    const float se = sqrtf(maxf(1 - ce * ce, 0));
-    const float az = 2 * OR_PI * r1;
+static f3 hg_sample(real g, real r0, real r1, f3 din) {
+    if (fabsr(g) < 1e-4f) return sample_uniform_sphere(r0, r1);
+    const real g2 = g * g;
+    const real tmp = (g2 - 1) / (2 * r0 * g - (g + 1));
+    const real ce = (tmp * tmp - (1 + g2)) / (2 * g);
+    const real se = sqrtr(maxf(1 - ce * ce, 0));
+    const real az = 2 * OR_PI * r1;
     f3 fr[3];
     make_orthonormal(din, fr);
     return add(add(mulf(fr[0], se * om_cos(az)), mulf(fr[1], se * om_sin(az))), mulf(din, ce));
@@ -1369,7 +1486,7 @@ void or_hg_sample(float g, float r0, float r1, const float din[3], float out[3])
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
 static f3 expv(f3 a) { return v3(om_exp(a.x), om_exp(a.y), om_exp(a.z)); }
-static float avg3(f3 a) { return (a.x + a.y + a.z) / 3; }
+static real avg3(f3 a) { return (a.x + a.y + a.z) / 3; }
 /* deltaTrack<Scattering>, medium.hlsli:104-181.  origin/direction in object space. */
 /* Density-gradient sink of the adjoint: at each collision in grid `vol`,
  * grad[value offset / 4] += sum_c wgt_c * dlog(beta_c) / d(value), where wgt
@@ -1377,7 +1494,7 @@ static float avg3(f3 a) { return (a.x + a.y + a.z) / 3; }
  * multiplies. */
 typedef struct { double* grad; uint32_t vol; f3 wgt; } adjd_t;
 
-static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_t* rng, f3 origin, f3 dir, float tmax, f3* beta,
+static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_t* rng, f3 origin, f3 dir, real tmax, f3* beta,
                       f3* dirPdf, f3* neePdf, int* scattered, uint32_t max_null, const adjd_t* ad) {
     *scattered = 0;
     *dirPdf = v3(1, 1, 1);
@@ -1387,15 +1504,15 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
     const int ch = (int)(u4[0] % 3);
     if (m->dens_vol == 0xFFFFFFFFu || m->dens_vol >= s->nvols) {
         const f3 st = m->density;
-        const float sc = fget(st, ch);
+        const real sc = fget(st, ch);
         if (sc < 1e-6f) return v3(0, 0, 0);
-        float r[4];
+        real r[4];
         rng_nextf(rng, r);
-        const float t = -om_log(1 - r[0]) / sc;
+        const real t = -om_log(1 - r[0]) / sc;
         if (t < tmax && scattering) {
             *scattered = 1;
             *beta = mul(*beta, mul(mul(expv(mulf(neg(st), t)), st), med_albedo(m)));
-            const float e = om_exp(-sc * t) * sc;
+            const real e = om_exp(-sc * t) * sc;
             *dirPdf = mulf(*dirPdf, e);
             return add(origin, mulf(dir, t));
         }
@@ -1408,17 +1525,17 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
      * path weights read its weight copy (the same grid unless one is set) */
     const uint8_t* gw = s->volw[m->dens_vol] ? s->volw[m->dens_vol] : g;
     const int sink = ad && ad->grad && ad->vol == m->dens_vol;
-    const float rmax = rdf(g, nvdb_root(g) + 36);
+    const real rmax = rdf(g, nvdb_root(g) + 36);
     const f3 maj = mulf(m->density, rmax);
     if (fget(maj, ch) < 1e-6f) return v3(0, 0, 0);
     origin = nvdb_world_to_index(g, origin);
     dir = nvdb_world_to_index_dir(g, dir);
-    const float invc = 1 / fget(maj, ch);
-    const float invmax = 1 / max3f(maj);
+    const real invc = 1 / fget(maj, ch);
+    const real invmax = 1 / max3f(maj);
     for (uint32_t it = 0; it < max_null && any_gt0(*beta); it++) {
-        float r[4];
+        real r[4];
         rng_nextf(rng, r);
-        const float t = -om_log(1 - r[0]) * invc;
+        const real t = -om_log(1 - r[0]) * invc;
         if (t >= tmax) {
             const f3 tr = expv(mulf(neg(maj), tmax));
             *beta = mul(*beta, tr);
@@ -1428,18 +1545,18 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
         }
         origin = add(origin, mulf(dir, t));
         tmax -= t;
-        const uint64_t va = nvdb_addr(g, (int)floorf(origin.x), (int)floorf(origin.y), (int)floorf(origin.z));
-        const float dv = rdf(g, va), dvw = rdf(gw, va);
+        const uint64_t va = nvdb_addr(g, (int)floorr(origin.x), (int)floorr(origin.y), (int)floorr(origin.z));
+        const real dv = rdf(g, va), dvw = rdf(gw, va);
         const f3 sig_t = mulf(m->density, dv), sig_tw = mulf(m->density, dvw);
         const f3 sig_s = med_albedo(m); /* readAlbedo is 1 without an albedo volume */
         const f3 tr = mulf(expv(mulf(neg(maj), t)), invmax);
-        int real = 0;
+        int real_coll = 0;
         if (scattering) {
-            float r2[4];
+            real r2[4];
             rng_nextf(rng, r2);
-            real = r2[0] < fget(sig_t, ch) * invc;
+            real_coll = r2[0] < fget(sig_t, ch) * invc;
         }
-        if (real) {
+        if (real_coll) {
             /* beta_c gains sigma_t,c = density_c * value: dlog = 1 / value */
             if (sink && dvw > 0) ad->grad[va / 4] += ((double)ad->wgt.x + (double)ad->wgt.y + (double)ad->wgt.z) / (double)dvw;
             *beta = mul(*beta, mul(mul(tr, sig_tw), sig_s));
@@ -1464,14 +1581,14 @@ static f3 delta_track(const or_scene* s, const medium_t* m, int scattering, rng_
 }
 
 /* ------------------------------------------------------------- BVH */
-static float tri_c(const or_scene* s, uint32_t t, int ax) {
+static real tri_c(const or_scene* s, uint32_t t, int ax) {
     const float* v = s->tv + 9 * (size_t)t;
     return (v[ax] + v[3 + ax] + v[6 + ax]) * (1.0f / 3.0f);
 }
 static void select_nth(const or_scene* s, uint32_t* a, int64_t n, int64_t k, int ax) {
     int64_t lo = 0, hi = n - 1;
     while (lo < hi) {
-        float piv = tri_c(s, a[(lo + hi) / 2], ax);
+        real piv = tri_c(s, a[(lo + hi) / 2], ax);
         int64_t i = lo, j = hi;
         while (i <= j) {
             while (tri_c(s, a[i], ax) < piv) i++;
@@ -1493,16 +1610,16 @@ static void node_bounds(const or_scene* s, bnode* nd, const uint32_t* a, uint32_
                 nd->hi[k] = maxf(nd->hi[k], v[3 * j + k]);
             }
     }
-    for (int k = 0; k < 3; k++) { nd->lo[k] = nextafterf(nd->lo[k], -INFINITY); nd->hi[k] = nextafterf(nd->hi[k], INFINITY); }
+    for (int k = 0; k < 3; k++) { nd->lo[k] = nextafterr(nd->lo[k], -INFINITY); nd->hi[k] = nextafterr(nd->hi[k], INFINITY); }
 }
 static void build_rec(or_scene* s, uint32_t ni, uint32_t start, uint32_t n) {
     bnode* nd = &s->nodes[ni];
     node_bounds(s, nd, s->order + start, n);
     if (n <= 4) { nd->a = start; nd->n = n; return; }
-    float c_lo[3] = {INFINITY, INFINITY, INFINITY}, c_hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    real c_lo[3] = {INFINITY, INFINITY, INFINITY}, c_hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = 0; i < n; i++)
         for (int k = 0; k < 3; k++) {
-            float c = tri_c(s, s->order[start + i], k);
+            real c = tri_c(s, s->order[start + i], k);
             c_lo[k] = minf(c_lo[k], c); c_hi[k] = maxf(c_hi[k], c);
         }
     int ax = 0;
@@ -1518,11 +1635,11 @@ static void build_rec(or_scene* s, uint32_t ni, uint32_t start, uint32_t n) {
 
 /* Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1) 2013).
  * Returns 1 with t, b1 (weight of v1), b2 (weight of v2) on a hit with t > 0. */
-typedef struct { f3 o, d; int kx, ky, kz; float Sx, Sy, Sz; } wray_t;
+typedef struct { f3 o, d; int kx, ky, kz; real Sx, Sy, Sz; } wray_t;
 static wray_t wray_make(f3 o, f3 d) {
     wray_t r;
     r.o = o; r.d = d;
-    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    real ax = fabsr(d.x), ay = fabsr(d.y), az = fabsr(d.z);
     r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
     r.kx = (r.kz + 1) % 3; r.ky = (r.kx + 1) % 3;
     if (fget(d, r.kz) < 0) { int t = r.kx; r.kx = r.ky; r.ky = t; }
@@ -1531,43 +1648,43 @@ static wray_t wray_make(f3 o, f3 d) {
     r.Sz = 1.0f / fget(d, r.kz);
     return r;
 }
-static int wt_intersect(const wray_t* r, const float* v, float* t, float* b1, float* b2) {
+static int wt_intersect(const wray_t* r, const float* v, real* t, real* b1, real* b2) {
     f3 A = sub(v3(v[0], v[1], v[2]), r->o), B = sub(v3(v[3], v[4], v[5]), r->o), C = sub(v3(v[6], v[7], v[8]), r->o);
-    float Ax = fget(A, r->kx) - r->Sx * fget(A, r->kz), Ay = fget(A, r->ky) - r->Sy * fget(A, r->kz);
-    float Bx = fget(B, r->kx) - r->Sx * fget(B, r->kz), By = fget(B, r->ky) - r->Sy * fget(B, r->kz);
-    float Cx = fget(C, r->kx) - r->Sx * fget(C, r->kz), Cy = fget(C, r->ky) - r->Sy * fget(C, r->kz);
-    float U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
+    real Ax = fget(A, r->kx) - r->Sx * fget(A, r->kz), Ay = fget(A, r->ky) - r->Sy * fget(A, r->kz);
+    real Bx = fget(B, r->kx) - r->Sx * fget(B, r->kz), By = fget(B, r->ky) - r->Sy * fget(B, r->kz);
+    real Cx = fget(C, r->kx) - r->Sx * fget(C, r->kz), Cy = fget(C, r->ky) - r->Sy * fget(C, r->kz);
+    real U = Cx * By - Cy * Bx, V = Ax * Cy - Ay * Cx, W = Bx * Ay - By * Ax;
     if (U == 0.0f || V == 0.0f || W == 0.0f) {
-        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
-        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
-        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+        U = (real)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (real)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (real)((double)Bx * (double)Ay - (double)By * (double)Ax);
     }
     if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return 0;
-    float det = U + V + W;
+    real det = U + V + W;
     if (det == 0.0f) return 0;
-    float Az = r->Sz * fget(A, r->kz), Bz = r->Sz * fget(B, r->kz), Cz = r->Sz * fget(C, r->kz);
-    float T = U * Az + V * Bz + W * Cz;
-    float rcp = 1.0f / det;
-    float tt = T * rcp;
+    real Az = r->Sz * fget(A, r->kz), Bz = r->Sz * fget(B, r->kz), Cz = r->Sz * fget(C, r->kz);
+    real T = U * Az + V * Bz + W * Cz;
+    real rcp = 1.0f / det;
+    real tt = T * rcp;
     if (!(tt > 0.0f)) return 0;
     *t = tt; *b1 = V * rcp; *b2 = W * rcp;
     return 1;
 }
 
-typedef struct { float t, b1, b2; uint32_t id; int hit; } hit_t;
+typedef struct { real t, b1, b2; uint32_t id; int hit; } hit_t;
 
-static int box_hit(const bnode* n, f3 o, f3 inv, float tmax, float* tnear) {
-    float t0x = (n->lo[0] - o.x) * inv.x, t1x = (n->hi[0] - o.x) * inv.x;
-    float t0y = (n->lo[1] - o.y) * inv.y, t1y = (n->hi[1] - o.y) * inv.y;
-    float t0z = (n->lo[2] - o.z) * inv.z, t1z = (n->hi[2] - o.z) * inv.z;
-    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.0f));
-    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+static int box_hit(const bnode* n, f3 o, f3 inv, real tmax, real* tnear) {
+    real t0x = (n->lo[0] - o.x) * inv.x, t1x = (n->hi[0] - o.x) * inv.x;
+    real t0y = (n->lo[1] - o.y) * inv.y, t1y = (n->hi[1] - o.y) * inv.y;
+    real t0z = (n->lo[2] - o.z) * inv.z, t1z = (n->hi[2] - o.z) * inv.z;
+    real tn = fmaxr(fmaxr(fminr(t0x, t1x), fminr(t0y, t1y)), fmaxr(fminr(t0z, t1z), 0.0f));
+    real tf = fminr(fminr(fmaxr(t0x, t1x), fmaxr(t0y, t1y)), fminr(fmaxr(t0z, t1z), tmax));
     *tnear = tn;
     return tn <= tf * 1.00000024f;
 }
 
 /* closest (any=0) or any hit (any=1) in (0, tmax) */
-static hit_t trace(const or_scene* s, f3 o, f3 d, float tmax, int any, uint64_t* nodes, uint64_t* tris) {
+static hit_t trace(const or_scene* s, f3 o, f3 d, real tmax, int any, uint64_t* nodes, uint64_t* tris) {
     hit_t h;
     h.hit = 0; h.t = tmax; h.b1 = h.b2 = 0; h.id = 0xffffffffu;
     if (s->nnodes == 0) return h;
@@ -1578,13 +1695,13 @@ static hit_t trace(const or_scene* s, f3 o, f3 d, float tmax, int any, uint64_t*
     stack[sp++] = 0;
     while (sp > 0) {
         const bnode* n = &s->nodes[stack[--sp]];
-        float tn;
+        real tn;
         (*nodes)++;
         if (!box_hit(n, o, inv, h.t, &tn)) continue;
         if (n->n > 0) {
             for (uint32_t i = 0; i < n->n; i++) {
                 uint32_t ti = s->order[n->a + i];
-                float t, b1, b2;
+                real t, b1, b2;
                 (*tris)++;
                 if (!wt_intersect(&wr, s->tv + 9 * (size_t)ti, &t, &b1, &b2)) continue;
                 uint32_t id = s->tid[ti];
@@ -1603,7 +1720,7 @@ static hit_t trace(const or_scene* s, f3 o, f3 d, float tmax, int any, uint64_t*
 
 /* traceRay over triangles and the procedural volume boxes, intersection.hlsli:68-193.
  * *vol_gn receives the packed box normal when the closest hit is a volume box. */
-static hit_t trace_ray(const or_scene* s, f3 o, f3 d, float tmax, int any, int* is_vol, uint32_t* vol_gn, uint64_t* nodes,
+static hit_t trace_ray(const or_scene* s, f3 o, f3 d, real tmax, int any, int* is_vol, uint32_t* vol_gn, uint64_t* nodes,
                        uint64_t* tris) {
     hit_t h = trace(s, o, d, tmax, any, nodes, tris);
     *is_vol = 0;
@@ -1621,15 +1738,15 @@ static hit_t trace_ray(const or_scene* s, f3 o, f3 d, float tmax, int any, int* 
             const uint64_t root = nvdb_root(g);
             org = nvdb_world_to_index(g, oo);
             dir = nvdb_world_to_index_dir(g, od);
-            bmin = v3((float)(int32_t)rd32(g, root), (float)(int32_t)rd32(g, root + 4), (float)(int32_t)rd32(g, root + 8));
-            bmax = add(v3((float)(int32_t)rd32(g, root + 12), (float)(int32_t)rd32(g, root + 16), (float)(int32_t)rd32(g, root + 20)),
+            bmin = v3((real)(int32_t)rd32(g, root), (real)(int32_t)rd32(g, root + 4), (real)(int32_t)rd32(g, root + 8));
+            bmax = add(v3((real)(int32_t)rd32(g, root + 12), (real)(int32_t)rd32(g, root + 16), (real)(int32_t)rd32(g, root + 20)),
                        v3(1, 1, 1));
         }
         const f3 t0 = divv(sub(bmin, org), dir), t1 = divv(sub(bmax, org), dir);
-        const float sx = maxf(maxf(minf(t0.x, t1.x), minf(t0.y, t1.y)), minf(t0.z, t1.z));
-        const float sy = minf(minf(maxf(t0.x, t1.x), maxf(t0.y, t1.y)), maxf(t0.z, t1.z));
+        const real sx = maxf(maxf(minf(t0.x, t1.x), minf(t0.y, t1.y)), minf(t0.z, t1.z));
+        const real sy = minf(minf(maxf(t0.x, t1.x), maxf(t0.y, t1.y)), maxf(t0.z, t1.z));
         if (!(sx <= sy)) continue; /* the box itself is missed */
-        const float t = sx > 0 ? sx : sy;
+        const real t = sx > 0 ? sx : sy;
         if (!(t < h.t && t > 0)) continue;
         f3 ln = sub(v3(t1.x == t, t1.y == t, t1.z == t), v3(t0.x == t, t0.y == t, t0.z == t));
         if (g) ln = nvdb_index_to_world_dir(g, ln);
@@ -1659,8 +1776,8 @@ static st_transform inverse_xf(const st_transform* T) {
     inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
     for (int i = 0; i < 3; i++) {
         double tr = 0;
-        for (int j = 0; j < 3; j++) { r.m[i][j] = (float)inv[i][j]; tr -= inv[i][j] * T->m[j][3]; }
-        r.m[i][3] = (float)tr;
+        for (int j = 0; j < 3; j++) { r.m[i][j] = (real)inv[i][j]; tr -= inv[i][j] * T->m[j][3]; }
+        r.m[i][3] = (real)tr;
     }
     return r;
 }
@@ -1722,7 +1839,7 @@ or_scene* or_scene_create(const st_scene_desc* d) {
     if (s->nvolinfo) memcpy(s->volinfo, d->volume_info, sizeof(st_volume_info) * s->nvolinfo);
     /* lights, scene AABB and flattened triangles */
     s->light_inst = (uint32_t*)malloc(sizeof(uint32_t) * (s->ninst + 1));
-    float amin[3] = {INFINITY, INFINITY, INFINITY}, amax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    real amin[3] = {INFINITY, INFINITY, INFINITY}, amax[3] = {-INFINITY, -INFINITY, -INFINITY};
     uint64_t ntri = 0;
     for (uint32_t i = 0; i < s->ninst; i++) {
         if ((s->inst[i].type_material_address & 0xfu) == 0) ntri += s->inst[i].data >> 16;
@@ -1739,8 +1856,8 @@ or_scene* or_scene_create(const st_scene_desc* d) {
             if (in->data < s->nvols) {
                 double b[6];
                 memcpy(b, s->vol[in->data] + 560, 48);
-                mn = v3((float)b[0], (float)b[1], (float)b[2]);
-                mx = v3((float)b[3], (float)b[4], (float)b[5]);
+                mn = v3((real)b[0], (real)b[1], (real)b[2]);
+                mx = v3((real)b[3], (real)b[4], (real)b[5]);
             }
             /* as the reference does, the running min/max also feed the later
              * corners (Scene.cpp:701-709 updates mn/mx inside the corner loop) */
@@ -1761,7 +1878,7 @@ or_scene* or_scene_create(const st_scene_desc* d) {
         if (vii >= s->nvi) { fail("vertex info index out of range"); or_scene_destroy(s); return NULL; }
         const st_mesh_vertex_info* vi = &s->vi[vii];
         uint32_t pb = vi_field(vi->packed_buffer_indices[0], 16, 16), ps = vi_field(vi->packed_strides, 8, 8);
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        real lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (uint32_t p = 0; p < pc; p++) {
             uint32_t tri[3];
             if (!load_tri(s, vi, p, tri)) { fail("index buffer out of range"); or_scene_destroy(s); return NULL; }
@@ -1855,7 +1972,7 @@ int or_trace_rays_bruteforce_f64(or_scene* s, const float* rays, uint32_t count,
             double tt = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) / det;
             if (tt > 0 && tt < best) { best = tt; bid = s->tid[t]; }
         }
-        hits[4 * i + 0] = asuint((float)best);
+        hits[4 * i + 0] = asuint((real)best);
         hits[4 * i + 1] = bid;
         hits[4 * i + 2] = 0;
         hits[4 * i + 3] = 0;
@@ -1869,16 +1986,16 @@ typedef struct {
     uint32_t dir;       /* packed oct-snorm16 (testrenderer.slang:119-122) */
     uint32_t path_len;  /* mPathLength */
     uint32_t inst_prim; /* mInstancePrimitiveIndex */
-    float t16, pdfa16, dirpdf16; /* FP16-stored fields, kept as their f32 values */
+    real t16, pdfa16, dirpdf16; /* FP16-stored fields, kept as their f32 values */
     rng_t rng;
     uint32_t medium;    /* mCurrentMedium */
-    float dvc, dvcm;    /* dVC, dVCM (FP16-stored, kept as their f32 values) */
+    real dvc, dvcm;    /* dVC, dVCM (FP16-stored, kept as their f32 values) */
     int vc_f32;         /* ST_FLAG_VC_FP32_MIS: dVC, dVCM kept in fp32 */
     int flag31;         /* isSpecular (camera paths) / isFiniteLight (light paths): one bit */
 } pstate_t; /* PathState, testrenderer.slang:66-125 */
 
 /* a dVC / dVCM store: fp16 as the reference's PathState (testrenderer.slang:75-82) */
-static float qvc(const pstate_t* ps, float f) { return ps->vc_f32 ? f : q16(f); }
+static real qvc(const pstate_t* ps, real f) { return ps->vc_f32 ? f : q16(f); }
 
 /* PackedLightVertex, testrenderer.slang:45-64 */
 typedef struct {
@@ -1887,12 +2004,12 @@ typedef struct {
     f3 thr;
     uint32_t dirin;     /* packed local incoming direction */
     uint32_t plen;
-    float dvc, dvcm;
+    real dvc, dvcm;
 } lvert_t;
 
 /* ShadowRay (testrenderer.slang:127-142) + the vertex medium and the key of
  * its own random stream for ratio tracking */
-typedef struct { f3 o, d, c; float tmax, w; uint32_t medium, plen; } shadow_t;
+typedef struct { f3 o, d, c; real tmax, w; uint32_t medium, plen; } shadow_t;
 
 typedef struct {
     const or_scene* s;
@@ -1909,11 +2026,11 @@ typedef struct {
     const st_view* view;
     st_transform inv_cam;
     uint32_t px, py, W, H;
-    float alb[3], dep[2], uv[2];
+    real alb[3], dep[2], uv[2];
     /* bidirectional mode: this sample's light vertex cache and subpath count */
     const lvert_t* lvc;
     uint32_t lvc_count, L;
-    float ipd;
+    real ipd;
     const uint32_t* light_acc;  /* this pixel's light-pass connections (3 x u32) or NULL */
     /* screen-space ReSTIR DI reuse: the primary vertex keeps its RIS reservoir
      * (seed, target, W, M, hit distance) and shading record instead of making
@@ -1922,7 +2039,7 @@ typedef struct {
     int prim_hit;               /* the primary ray hit a surface */
     shading_t prim_sd;
     uint32_t prim_dir;          /* packed primary ray direction */
-    float res[8];               /* seed[4], target, W, M, depth */
+    real res[8];               /* seed[4], target, W, M, depth */
     struct adj_s* adj;          /* path replay of or_adjoint, or NULL */
     struct cache_s* cache;      /* or_render_cached: this path's cache-depth vertex, or NULL */
     /* hash-grid reservoir reuse (ST_FLAG_RESTIR_DI_HASHGRID): the previous
@@ -1940,11 +2057,11 @@ typedef struct {
  * {local incoming direction, 0}; W = 0: empty), then the slots sorted by
  * (empty, checksum), stable in slot order, with their keys. */
 typedef struct hgrid_s {
-    float* rec;
+    real* rec;
     uint64_t* keys;
     uint32_t* slots;
     uint32_t nslots, npix, dmax;
-    float cell;
+    real cell;
 } hgrid_t;
 
 /* or_render_cached (st_render_cached without the network): a path reaching a
@@ -1954,7 +2071,7 @@ typedef struct hgrid_s {
 typedef struct cache_s {
     uint32_t depth, stride, frame, g;
     int kind;                /* 0 no record, 1 cache query, 2 training row */
-    float q[12], thr[3];
+    real q[12], thr[3];
     f3 out_before;           /* fp32 radiance (emission) up to and including the vertex */
     int nsh_before;          /* shadow rays made before the vertex */
     uint32_t nee_before[3];  /* their fixed-point NEE */
@@ -1976,12 +2093,12 @@ typedef struct adj_s {
 
 /* ViewData::toRaster (compat/scene.h:158-165) with ProjectionData::projectPoint
  * (compat/transform.h:139-155); 0, with *ox, *oy unchanged, when p is not visible. */
-static int to_raster(const st_view* V, f3 p, float* ox, float* oy) {
+static int to_raster(const st_view* V, f3 p, real* ox, real* oy) {
     const st_projection* pr = &V->projection;
-    const float sz = p.z > 0 ? 1.0f : (p.z < 0 ? -1.0f : 0.0f);
-    const float sn = pr->near_plane > 0 ? 1.0f : (pr->near_plane < 0 ? -1.0f : 0.0f);
+    const real sz = p.z > 0 ? 1.0f : (p.z < 0 ? -1.0f : 0.0f);
+    const real sn = pr->near_plane > 0 ? 1.0f : (pr->near_plane < 0 ? -1.0f : 0.0f);
     if (sz != sn) return 0;
-    float r0, r1, r2, r3;
+    real r0, r1, r2, r3;
     if (pr->vertical_fov < 0) {
         r0 = p.x * pr->scale[0] + pr->offset[0];
         r1 = p.y * pr->scale[1] + pr->offset[1];
@@ -1990,14 +2107,14 @@ static int to_raster(const st_view* V, f3 p, float* ox, float* oy) {
     } else {
         r0 = p.x * pr->scale[0] + p.z * pr->offset[0];
         r1 = p.y * pr->scale[1] + p.z * pr->offset[1];
-        r2 = fabsf(pr->near_plane);
+        r2 = fabsr(pr->near_plane);
         r3 = p.z * sn;
     }
-    const float aw = fabsf(r3);
-    if (fabsf(r0) >= aw || fabsf(r1) >= aw || fabsf(r2) >= aw) return 0;
-    const float ew = (float)(V->image_max[0] - V->image_min[0]), eh = (float)(V->image_max[1] - V->image_min[1]);
-    *ox = (float)V->image_min[0] + ew * ((r0 / r3) * 0.5f + 0.5f);
-    *oy = (float)V->image_min[1] + eh * ((-r1 / r3) * 0.5f + 0.5f);
+    const real aw = fabsr(r3);
+    if (fabsr(r0) >= aw || fabsr(r1) >= aw || fabsr(r2) >= aw) return 0;
+    const real ew = (real)(V->image_max[0] - V->image_min[0]), eh = (real)(V->image_max[1] - V->image_min[1]);
+    *ox = (real)V->image_min[0] + ew * ((r0 / r3) * 0.5f + 0.5f);
+    *oy = (real)V->image_min[1] + eh * ((-r1 / r3) * 0.5f + 0.5f);
     return 1;
 }
 
@@ -2005,8 +2122,8 @@ static int to_raster(const st_view* V, f3 p, float* ox, float* oy) {
  * state: the instance motion transform is the identity and the previous view
  * is this view.  has_instance: the vertex is a surface or medium vertex. */
 static void store_aux(pctx_t* x, int has_instance, f3 pos, f3 albedo) {
-    float pcx = (float)x->px + 0.5f, pcy = (float)x->py + 0.5f;
-    float prev_depth = x->ps.t16;
+    real pcx = (real)x->px + 0.5f, pcy = (real)x->py + 0.5f;
+    real prev_depth = x->ps.t16;
     if (has_instance) {
         const f3 pc = xf_point(&x->inv_cam, pos);
         prev_depth = length3(pc);
@@ -2015,23 +2132,23 @@ static void store_aux(pctx_t* x, int has_instance, f3 pos, f3 albedo) {
     x->alb[0] = albedo.x; x->alb[1] = albedo.y; x->alb[2] = albedo.z;
     x->dep[0] = x->ps.t16;
     x->dep[1] = prev_depth;
-    x->uv[0] = pcx / (float)x->W;
-    x->uv[1] = pcy / (float)x->H;
+    x->uv[0] = pcx / (real)x->W;
+    x->uv[1] = pcy / (real)x->H;
 }
 
 /* AddRadiance, testrenderer.slang:220-235 */
-static void add_radiance(pctx_t* x, f3 rad, float w) {
+static void add_radiance(pctx_t* x, f3 rad, real w) {
     if (w <= 0 || all_le0(rad)) return;
     x->out = add(x->out, mulf(rad, w));
     if (x->adj) x->adj->acc = add(x->adj->acc, mulf(rad, w));
 }
 /* AddEmission, testrenderer.slang:788-808 (+ --minPathLength gate, vcm.slang:960) */
-static void add_emission(pctx_t* x, f3 e, float directPdfA, float emissionPdfW, float WtoA) {
-    float w = 1;
+static void add_emission(pctx_t* x, f3 e, real directPdfA, real emissionPdfW, real WtoA) {
+    real w = 1;
     if (x->c->flags & ST_FLAG_BDPT_LVC) {
         /* partial eye sub-path MIS weight [tech. rep. (43)] */
         if (x->ps.path_len > 1 && !x->ps.flag31) {
-            const float wc = mis1(directPdfA) * x->ps.dvcm + mis1(emissionPdfW) * x->ps.dvc;
+            const real wc = mis1(directPdfA) * x->ps.dvcm + mis1(emissionPdfW) * x->ps.dvc;
             w = 1 / (1 + wc);
         }
     } else if (x->ps.path_len > 1) { /* isSpecular is never set on camera paths without VC */
@@ -2042,25 +2159,25 @@ static void add_emission(pctx_t* x, f3 e, float directPdfA, float emissionPdfW, 
 }
 /* ConnectToLight (plain NEE), testrenderer.slang:751-779 */
 static void connect_to_light(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
-    float rnd[4];
+    real rnd[4];
     rng_nextf(&x->ps.rng, rnd);
     isr_t r = sample_illumination(x->s, x->c, rnd, sd->pos);
     if (all_le0(r.radiance) || r.pdf <= 0) return;
-    float AtoW = 1;
+    real AtoW = 1;
     if (r.finite) AtoW = pow2(r.dist) / r.cos_light;
-    float dpw = r.pdf * AtoW;
+    real dpw = r.pdf * AtoW;
     f3 ldout = to_local(sd, r.dir);
     rer_t f = bsdf_eval(m, x->c->lambert, ldin, ldout);
     f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
-    float cp = rr_cp(x->c, x->ps.path_len, m_cont_prob(m));
+    real cp = rr_cp(x->c, x->ps.path_len, m_cont_prob(m));
     f.fwd *= cp; f.rev *= cp;
     f3 contrib = divf(mul(mul(x->ps.thr, r.radiance), f.f), dpw);
-    float w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, f.fwd / AtoW); /* ComputeMIS_DI :530-566 */
+    real w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, f.fwd / AtoW); /* ComputeMIS_DI :530-566 */
     if (x->c->flags & ST_FLAG_BDPT_LVC) {
         /* ComputeMIS_DI with gUseVC [tech. rep. (44), (45)] */
-        const float epw = r.pdf * cos_hemisphere_pdfw(r.cos_light);
-        const float wl = mis1(f.fwd / dpw);
-        const float wc = mis1(epw * fabsf(ldout.z) / (dpw * r.cos_light)) * (0 + x->ps.dvcm + x->ps.dvc * mis1(f.rev));
+        const real epw = r.pdf * cos_hemisphere_pdfw(r.cos_light);
+        const real wl = mis1(f.fwd / dpw);
+        const real wc = mis1(epw * fabsr(ldout.z) / (dpw * r.cos_light)) * (0 + x->ps.dvcm + x->ps.dvc * mis1(f.rev));
         w = 1 / (wl + 1 + wc);
     }
     if (x->ps.path_len + 1 < x->c->min_len) return;
@@ -2097,21 +2214,21 @@ static void connect_to_light_subpath(pctx_t* x, const material_t* m, const shadi
         vpos = sv.pos;
     }
     f3 dir = sub(vpos, cpos);
-    const float dist = length3(dir);
+    const real dist = length3(dir);
     dir = divf(dir, dist);
     const f3 vin = unpack_normal(v->dirin);
     rer_t fv;
-    float cos_light, cpv;
+    real cos_light, cpv;
     if (vmed) {
         const medium_t mv = load_medium(x->s, x->s->inst[vi].type_material_address >> 4);
-        const float ph = hg_eval(mv.g, vin, neg(dir));
+        const real ph = hg_eval(mv.g, vin, neg(dir));
         fv.f = v3(ph, ph, ph); fv.fwd = ph; fv.rev = ph;
         cos_light = 1;
         cpv = med_cont_prob(&mv);
     } else {
         const material_t mv = load_material(x->s, sv.flags_material >> 4);
         const f3 vout = to_local(&sv, neg(dir));
-        cos_light = fabsf(vout.z);
+        cos_light = fabsr(vout.z);
         fv = bsdf_eval_x(&mv, x->c->lambert, vin, vout, 1);
         fv.f = mulf(fv.f, shading_normal_correction_x(&sv, vin, vout, 1));
         cpv = m_cont_prob(&mv);
@@ -2119,29 +2236,29 @@ static void connect_to_light_subpath(pctx_t* x, const material_t* m, const shadi
     fv.fwd *= cpv; fv.rev *= cpv;
     if (all_le0(v->thr) || all_le0(fv.f)) return;
     rer_t f;
-    float cos_cam;
+    real cos_cam;
     if (cmed) {
-        const float ph = hg_eval(cmed->g, ldin, dir);
+        const real ph = hg_eval(cmed->g, ldin, dir);
         f.f = v3(ph, ph, ph); f.fwd = ph; f.rev = ph;
         cos_cam = 1;
-        const float cp = med_cont_prob(cmed);
+        const real cp = med_cont_prob(cmed);
         f.fwd *= cp; f.rev *= cp;
     } else {
         const f3 ldout = to_local(sd, dir);
-        cos_cam = fabsf(ldout.z);
+        cos_cam = fabsr(ldout.z);
         f = bsdf_eval(m, x->c->lambert, ldin, ldout);
         f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
-        const float cp = m_cont_prob(m);
+        const real cp = m_cont_prob(m);
         f.fwd *= cp; f.rev *= cp;
     }
-    const float wn = (float)x->lvc_count / (float)x->L;
-    const float inv_d2 = 1 / pow2(dist);
+    const real wn = (real)x->lvc_count / (real)x->L;
+    const real inv_d2 = 1 / pow2(dist);
     const f3 contrib = mul(mul(x->ps.thr, v->thr), mulf(mulf(mul(f.f, fv.f), inv_d2), wn));
-    const float cam_pdfa = f.fwd * cos_light * inv_d2;
-    const float light_pdfa = fv.fwd * cos_cam * inv_d2;
-    const float wl = mis1(cam_pdfa) * (0 + v->dvcm + v->dvc * mis1(fv.rev));   /* [tech. rep. (40)] */
-    const float wc = mis1(light_pdfa) * (0 + x->ps.dvcm + x->ps.dvc * mis1(f.rev)); /* [tech. rep. (41)] */
-    const float w = 1 / (wl + 1 + wc);
+    const real cam_pdfa = f.fwd * cos_light * inv_d2;
+    const real light_pdfa = fv.fwd * cos_cam * inv_d2;
+    const real wl = mis1(cam_pdfa) * (0 + v->dvcm + v->dvc * mis1(fv.rev));   /* [tech. rep. (40)] */
+    const real wc = mis1(light_pdfa) * (0 + x->ps.dvcm + x->ps.dvc * mis1(f.rev)); /* [tech. rep. (41)] */
+    const real w = 1 / (wl + 1 + wc);
     if (x->nsh >= 64) return;
     shadow_t* sr = &x->sh[x->nsh++];
     sr->o = cmed ? cpos : ray_offset_dir(sd->pos, unpack_normal(sd->gn), dir); /* rayOffset of a medium vertex: itself */
@@ -2155,13 +2272,13 @@ static void connect_to_light_subpath(pctx_t* x, const material_t* m, const shadi
 
 /* ReservoirSampleDataDI, testrenderer.slang:568-606 */
 typedef struct {
-    float smp[4];
-    float src_pdf, mis_w, wtoa;
+    real smp[4];
+    real src_pdf, mis_w, wtoa;
     f3 contrib, vo, vd;
-    float vtmax;
+    real vtmax;
 } rsd_t;
-static float rsd_target(const rsd_t* r) { return luminance(r->contrib) * r->wtoa; }
-static void rsd_init(rsd_t* r, pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin, const float rnd[4]) {
+static real rsd_target(const rsd_t* r) { return luminance(r->contrib) * r->wtoa; }
+static void rsd_init(rsd_t* r, pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin, const real rnd[4]) {
     memset(r, 0, sizeof *r);
     memcpy(r->smp, rnd, sizeof r->smp);
     isr_t L = sample_illumination(x->s, x->c, rnd, sd->pos);
@@ -2170,11 +2287,11 @@ static void rsd_init(rsd_t* r, pctx_t* x, const material_t* m, const shading_t* 
         f3 ldout = to_local(sd, L.dir);
         rer_t f = bsdf_eval(m, x->c->lambert, ldin, ldout);
         f.f = mulf(f.f, shading_normal_correction(sd, ldin, ldout));
-        float cp = rr_cp(x->c, x->ps.path_len, m_cont_prob(m));
+        real cp = rr_cp(x->c, x->ps.path_len, m_cont_prob(m));
         f.fwd *= cp; f.rev *= cp;
         r->contrib = mul(f.f, L.radiance);
         r->wtoa = L.finite ? L.cos_light / pow2(L.dist) : 1;
-        float AtoW = L.finite ? pow2(L.dist) / L.cos_light : 1;
+        real AtoW = L.finite ? pow2(L.dist) / L.cos_light : 1;
         r->mis_w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(L.pdf, f.fwd / AtoW); /* ComputeMIS_DI, :530-566 */
         r->vo = ray_offset_dir(sd->pos, unpack_normal(sd->gn), L.dir);
         r->vd = L.dir;
@@ -2182,8 +2299,8 @@ static void rsd_init(rsd_t* r, pctx_t* x, const material_t* m, const shading_t* 
     }
 }
 /* HashGrid::FindCellIndex's checksum of the cell holding p (hashgrid.hlsli:51-58) */
-static uint32_t hg_checksum(f3 p, float cs) {
-    const int ix = (int)floorf(p.x / cs), iy = (int)floorf(p.y / cs), iz = (int)floorf(p.z / cs);
+static uint32_t hg_checksum(f3 p, real cs) {
+    const int ix = (int)floorr(p.x / cs), iy = (int)floorr(p.y / cs), iz = (int)floorr(p.z / cs);
     const uint32_t h = or_xxhash32(or_xxhash32(asuint(cs)) + or_xxhash32((uint32_t)iz + or_xxhash32((uint32_t)iy + or_xxhash32((uint32_t)ix))));
     return h > 1u ? h : 1u;
 }
@@ -2197,19 +2314,19 @@ static uint32_t hg_bound(const uint64_t* keys, uint32_t n, uint64_t k, int upper
     }
     return lo;
 }
-static float mis_pair(float a, float b);
+static real mis_pair(real a, real b);
 /* SampleIlluminationReSTIR's reuse and store (testrenderer.slang:686-723) with
  * ReservoirReuse (:608-663): a point jittered over one cell on the tangent
  * plane, one entry of the previous sample's grid in its cell, pairwise MIS;
  * then this vertex's reservoir into this sample's grid.  A current reservoir
  * of target 0 gets weight 0 (the reference's 0/0 makes the sum NaN). */
-static void hg_reuse(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin, rsd_t* sample, float* W) {
+static void hg_reuse(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin, rsd_t* sample, real* W) {
     hgrid_t* g = x->hg_cur;
-    float M = (float)x->c->cand;
-    float target = rsd_target(sample);
+    real M = (real)x->c->cand;
+    real target = rsd_target(sample);
     if (x->hg_prev) {
         const hgrid_t* pg = x->hg_prev;
-        float dr[4], d[2];
+        real dr[4], d[2];
         rng_nextf(&x->ps.rng, dr);
         sample_concentric_disc(dr[0], dr[1], d);
         const f3 qp = add(sd->pos, to_world(sd, v3(d[0] * g->cell, d[1] * g->cell, 0)));
@@ -2218,11 +2335,11 @@ static void hg_reuse(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldi
         if (hi > lo) {
             uint32_t u[4];
             rng_next(&x->ps.rng, u);
-            const float* e = pg->rec + 16 * (size_t)pg->slots[lo + u[0] % (hi - lo)];
+            const real* e = pg->rec + 16 * (size_t)pg->slots[lo + u[0] % (hi - lo)];
             rsd_t xn;
             rsd_init(&xn, x, m, sd, ldin, e + 4);
-            const float tn = rsd_target(&xn);
-            float phq = 0;
+            const real tn = rsd_target(&xn);
+            real phq = 0;
             {
                 const uint32_t id = asuint(e[3]);
                 const f3 lp = v3(e[0], e[1], e[2]);
@@ -2233,12 +2350,12 @@ static void hg_reuse(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldi
                 rsd_init(&xq, x, &mq, &sq, v3(e[12], e[13], e[14]), sample->smp);
                 phq = rsd_target(&xq);
             }
-            const float qM = e[10];
-            const float m0 = mis_pair(M * target, qM * phq), m1 = mis_pair(qM * e[8], M * tn);
-            const float w0 = m0 * target * *W, w1 = m1 * tn * e[9];
+            const real qM = e[10];
+            const real m0 = mis_pair(M * target, qM * phq), m1 = mis_pair(qM * e[8], M * tn);
+            const real w0 = m0 * target * *W, w1 = m1 * tn * e[9];
             M += qM;
-            const float total = w0 + w1;
-            float uu[4];
+            const real total = w0 + w1;
+            real uu[4];
             rng_nextf(&x->ps.rng, uu);
             if (uu[0] * total <= w1) {
                 *sample = xn;
@@ -2248,12 +2365,12 @@ static void hg_reuse(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldi
         }
     }
     if (*W > 0 && x->ps.path_len - 1 < g->dmax) {
-        float* e = g->rec + 16 * ((size_t)(x->ps.path_len - 1) * g->npix + x->hg_pix);
+        real* e = g->rec + 16 * ((size_t)(x->ps.path_len - 1) * g->npix + x->hg_pix);
         const uint32_t id = x->ps.inst_prim;
         const f3 lp = xf_point(&x->s->inv_xf[id & 0xffffu], sd->pos);
         e[0] = lp.x; e[1] = lp.y; e[2] = lp.z; e[3] = asfloat(id);
-        memcpy(e + 4, sample->smp, 4 * sizeof(float));
-        const float mmax = x->c->max_m * (float)x->c->cand;
+        memcpy(e + 4, sample->smp, 4 * sizeof(real));
+        const real mmax = x->c->max_m * (real)x->c->cand;
         e[8] = target; e[9] = *W; e[10] = M < mmax ? M : mmax; e[11] = asfloat(hg_checksum(sd->pos, g->cell));
         e[12] = ldin.x; e[13] = ldin.y; e[14] = ldin.z; e[15] = 0;
     }
@@ -2264,19 +2381,19 @@ static void hg_reuse(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldi
 static void connect_to_light_restir(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
     rsd_t sample, xi;
     memset(&sample, 0, sizeof sample);
-    float wsum = 0;
+    real wsum = 0;
     for (uint32_t i = 0; i < x->c->cand; i++) {
-        float rnd[4];
+        real rnd[4];
         rng_nextf(&x->ps.rng, rnd);
         rsd_init(&xi, x, m, sd, ldin, rnd);
         if (rsd_target(&xi) <= 0) continue;
         wsum += rsd_target(&xi) / xi.src_pdf;
-        float u[4];
+        real u[4];
         rng_nextf(&x->ps.rng, u);
         if (u[0] <= (rsd_target(&xi) / xi.src_pdf) / wsum) sample = xi;
     }
-    float W = 0;
-    if (rsd_target(&sample) > 0) W = wsum / ((float)x->c->cand * rsd_target(&sample));
+    real W = 0;
+    if (rsd_target(&sample) > 0) W = wsum / ((real)x->c->cand * rsd_target(&sample));
     if (x->hg_cur) hg_reuse(x, m, sd, ldin, &sample, &W);
     if (W <= 0) return;
     W *= sample.wtoa;
@@ -2298,16 +2415,16 @@ static void connect_to_light_restir(pctx_t* x, const material_t* m, const shadin
  * with the winning seed kept for reuse instead of a shadow ray. */
 static void ris_reservoir(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldin) {
     rsd_t xi;
-    float wsum = 0, target = 0;
+    real wsum = 0, target = 0;
     for (int j = 0; j < 4; j++) x->res[j] = 0;
     for (uint32_t i = 0; i < x->c->cand; i++) {
-        float rnd[4];
+        real rnd[4];
         rng_nextf(&x->ps.rng, rnd);
         rsd_init(&xi, x, m, sd, ldin, rnd);
-        const float t = rsd_target(&xi);
+        const real t = rsd_target(&xi);
         if (t <= 0) continue;
         wsum += t / xi.src_pdf;
-        float u[4];
+        real u[4];
         rng_nextf(&x->ps.rng, u);
         if (u[0] <= (t / xi.src_pdf) / wsum) {
             memcpy(x->res, rnd, sizeof rnd);
@@ -2315,21 +2432,21 @@ static void ris_reservoir(pctx_t* x, const material_t* m, const shading_t* sd, f
         }
     }
     x->res[4] = target;
-    x->res[5] = target > 0 ? wsum / ((float)x->c->cand * target) : 0.0f;
-    x->res[6] = target > 0 ? (float)x->c->cand : 0.0f;
+    x->res[5] = target > 0 ? wsum / ((real)x->c->cand * target) : 0.0f;
+    x->res[6] = target > 0 ? (real)x->c->cand : 0.0f;
 }
 
 #define OR_MAX_MEDIA_SEGMENTS 256
 /* traceScatteringRay, intersection.hlsli:195-260: closest hits, delta tracking
  * through the current medium, medium boundaries crossed in place.
  * Returns 0 miss, 1 surface hit (h), 2 medium scattering at *pos. */
-static int trace_scattering(pctx_t* x, f3 origin, f3 dir, hit_t* h, f3* pos, float* dirpdf, float* neepdf) {
+static int trace_scattering(pctx_t* x, f3 origin, f3 dir, hit_t* h, f3* pos, real* dirpdf, real* neepdf) {
     const or_scene* s = x->s;
     pstate_t* ps = &x->ps;
     *dirpdf = 1;
     *neepdf = 1;
     f3 o = origin;
-    float tmax = INFINITY;
+    real tmax = INFINITY;
     /* at most OR_MAX_MEDIA_SEGMENTS traceRay calls; a ray still crossing
      * boundaries after that ends as a miss (the reference loop is unbounded) */
     for (int seg = 0; tmax > 1e-6f && seg < OR_MAX_MEDIA_SEGMENTS; seg++) {
@@ -2381,7 +2498,7 @@ static int trace_scattering(pctx_t* x, f3 origin, f3 dir, hit_t* h, f3* pos, flo
  * The deferred-shadow variant of the reference reads an uninitialised static
  * path state here (testrenderer.slang:1153-1158); this uses the vertex's
  * medium and a random stream of its own, keyed by pixel, sample and bounce. */
-static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, uint32_t py, uint32_t seed, f3* beta, float* neepdf,
+static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, uint32_t py, uint32_t seed, f3* beta, real* neepdf,
                                    const adjd_t* ad) {
     const or_scene* s = x->s;
     rng_t rng;
@@ -2389,7 +2506,7 @@ static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, u
     *neepdf = 1;
     uint32_t cur = sr->medium;
     f3 o = sr->o;
-    float tmax = sr->tmax;
+    real tmax = sr->tmax;
     for (int seg = 0; tmax > 1e-6f; seg++) {
         if (seg == OR_MAX_MEDIA_SEGMENTS) { /* bounded like the scattering ray: counts as blocked */
             *beta = v3(0, 0, 0);
@@ -2433,7 +2550,7 @@ static void trace_visibility_media(pctx_t* x, const shadow_t* sr, uint32_t px, u
 /* Sub-path MIS quantities after a BSDF sample, ProcessInteraction
  * (testrenderer.slang:909-922): [tech. rep. (53)-(55)] for a singular sample,
  * else (34)-(36); a camera path is no longer specular after a non-singular one. */
-static void vc_after_sample(pstate_t* ps, const dsr_t* smp, float cos_out, int from_light) {
+static void vc_after_sample(pstate_t* ps, const dsr_t* smp, real cos_out, int from_light) {
     if (smp->rough == 0) {
         ps->dvcm = qvc(ps, 0.0f);
         ps->dvc = qvc(ps, ps->dvc * mis1(cos_out));
@@ -2450,7 +2567,7 @@ static int cache_vertex(pctx_t* x, f3 pos, f3 wo, f3 nrm, f3 alb) {
     cache_t* c = x->cache;
     const uint32_t row = c->g / c->stride;
     const int train = c->g % c->stride == or_pcg(row ^ or_pcg(c->frame)) % c->stride;
-    const float q[12] = {pos.x, pos.y, pos.z, wo.x, wo.y, wo.z, nrm.x, nrm.y, nrm.z, alb.x, alb.y, alb.z};
+    const real q[12] = {pos.x, pos.y, pos.z, wo.x, wo.y, wo.z, nrm.x, nrm.y, nrm.z, alb.x, alb.y, alb.z};
     memcpy(c->q, q, sizeof q);
     c->thr[0] = x->ps.thr.x; c->thr[1] = x->ps.thr.y; c->thr[2] = x->ps.thr.z;
     c->kind = train ? 2 : 1;
@@ -2478,21 +2595,21 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
     if (!med_can_eval(&m)) return 0;
     if (x->cache && ps->path_len == x->cache->depth && cache_vertex(x, pos, ldin, v3(0, 0, 0), med_albedo(&m))) return 0;
     if (!med_singular(&m) && vc) {
-        float rnd[4];
+        real rnd[4];
         rng_nextf(&ps->rng, rnd);
         isr_t r = sample_illumination(s, x->c, rnd, pos);
         if (!(all_le0(r.radiance) || r.pdf <= 0)) {
-            float AtoW = 1;
+            real AtoW = 1;
             if (r.finite) AtoW = pow2(r.dist) / r.cos_light;
-            const float dpw = r.pdf * AtoW;
-            const float v = hg_eval(m.g, ldin, r.dir);
-            const float cp = med_cont_prob(&m);
-            const float fwd = v * cp, rev = v * cp;
+            const real dpw = r.pdf * AtoW;
+            const real v = hg_eval(m.g, ldin, r.dir);
+            const real cp = med_cont_prob(&m);
+            const real fwd = v * cp, rev = v * cp;
             const f3 contrib = divf(mulf(mul(ps->thr, r.radiance), v), dpw);
             /* ComputeMIS_DI with gUseVC; cosOut of a medium vertex is 1 */
-            const float epw = r.pdf * cos_hemisphere_pdfw(r.cos_light);
-            const float wl = mis1(fwd / dpw);
-            const float wc = mis1(epw * 1.0f / (dpw * r.cos_light)) * (0 + ps->dvcm + ps->dvc * mis1(rev));
+            const real epw = r.pdf * cos_hemisphere_pdfw(r.cos_light);
+            const real wl = mis1(fwd / dpw);
+            const real wc = mis1(epw * 1.0f / (dpw * r.cos_light)) * (0 + ps->dvcm + ps->dvc * mis1(rev));
             if (x->nsh < 64) {
                 shadow_t* sr = &x->sh[x->nsh++];
                 sr->o = pos;
@@ -2506,18 +2623,18 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
         }
         connect_to_light_subpath(x, NULL, NULL, &m, pos, ldin);
     } else if (!med_singular(&m) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
-        float rnd[4];
+        real rnd[4];
         rng_nextf(&ps->rng, rnd);
         isr_t r = sample_illumination(s, x->c, rnd, pos);
         if (!(all_le0(r.radiance) || r.pdf <= 0)) {
-            float AtoW = 1;
+            real AtoW = 1;
             if (r.finite) AtoW = pow2(r.dist) / r.cos_light;
-            const float dpw = r.pdf * AtoW;
-            const float v = hg_eval(m.g, ldin, r.dir);
-            const float cp = rr_cp(x->c, ps->path_len, med_cont_prob(&m));
-            const float fwd = v * cp;
+            const real dpw = r.pdf * AtoW;
+            const real v = hg_eval(m.g, ldin, r.dir);
+            const real cp = rr_cp(x->c, ps->path_len, med_cont_prob(&m));
+            const real fwd = v * cp;
             const f3 contrib = divf(mulf(mul(ps->thr, r.radiance), v), dpw);
-            const float w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, fwd / AtoW);
+            const real w = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, fwd / AtoW);
             if (ps->path_len + 1 >= x->c->min_len && x->nsh < 64) {
                 shadow_t* sr = &x->sh[x->nsh++];
                 sr->o = pos; /* rayOffset of a non-surface point is the point itself */
@@ -2531,22 +2648,22 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
         }
     }
     if (x->c->flags & ST_FLAG_DIRECT_ONLY) return 0;
-    const float cp = minf(1, med_cont_prob(&m));
+    const real cp = minf(1, med_cont_prob(&m));
     const int rr_on = ps->path_len >= x->c->rr_start;
-    float rr[4];
+    real rr[4];
     rng_nextf(&ps->rng, rr);
     if (rr_on && rr[0] > cp) return 0;
-    float rb[4];
+    real rb[4];
     rng_nextf(&ps->rng, rb);
     const f3 sdir = hg_sample(m.g, rb[0], rb[1], ldin);
-    const float v = hg_eval(m.g, ldin, sdir);
-    const float fwd = v * (rr_on ? cp : 1.0f);
+    const real v = hg_eval(m.g, ldin, sdir);
+    const real fwd = v * (rr_on ? cp : 1.0f);
     ps->thr = mulf(ps->thr, v / fwd);
-    ps->dirpdf16 = (1 - fabsf(m.g)) == 0 ? 0 : q16(fwd);
+    ps->dirpdf16 = (1 - fabsr(m.g)) == 0 ? 0 : q16(fwd);
     if (vc) {
         dsr_t smp;
         memset(&smp, 0, sizeof smp);
-        smp.fwd = fwd; smp.rev = fwd; smp.rough = (1 - fabsf(m.g)) == 0 ? 0.0f : 1.0f;
+        smp.fwd = fwd; smp.rev = fwd; smp.rough = (1 - fabsr(m.g)) == 0 ? 0.0f : 1.0f;
         vc_after_sample(ps, &smp, 1.0f, 0);
     }
     ps->dir = pack_normal(sdir);
@@ -2566,7 +2683,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     int kind;
     f3 mpos = v3(0, 0, 0);
     if (s->nvolinst > 0) {
-        float dirpdf, neepdf;
+        real dirpdf, neepdf;
         kind = trace_scattering(x, ps->origin, dir, &h, &mpos, &dirpdf, &neepdf);
         ps->thr = divf(ps->thr, dirpdf); /* TraceRay, testrenderer.slang:1030 */
     } else {
@@ -2577,14 +2694,14 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     ps->path_len++;
     shading_t sd;
     memset(&sd, 0, sizeof sd);
-    float pick = 0;
+    real pick = 0;
     if (kind == 2) {
         /* medium scattering: pickPdf 0 and shape area 0 (intersection.hlsli:232-236) */
         ps->inst_prim = h.id;
         ps->t16 = q16(h.t);
-        float pdfa = 0;
+        real pdfa = 0;
         if (s->nlights > 0) {
-            pdfa = 0.0f / (0.0f * (float)s->nlights);
+            pdfa = 0.0f / (0.0f * (real)s->nlights);
             if (s->has_env) pdfa *= 1 - x->c->env_prob;
         }
         ps->pdfa16 = q16(pdfa);
@@ -2593,7 +2710,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     }
     if (h.hit) {
         uint32_t ii = h.id & 0xffffu, prim = h.id >> 16;
-        pick = 1.0f / (float)(s->inst[ii].data >> 16);
+        pick = 1.0f / (real)(s->inst[ii].data >> 16);
         make_tri_shading(s, ii, prim, h.b1, h.b2, &sd);
         ps->inst_prim = h.id;
         if (s->nvolinst > 0) h.t = length3(sub(sd.pos, ps->origin)); /* intersection.hlsli:244 */
@@ -2604,9 +2721,9 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     }
     ps->t16 = q16(h.hit ? h.t : INFINITY);
     /* LightSamplePdfA, lights.hlsli:264-271 */
-    float pdfa = 0;
+    real pdfa = 0;
     if (s->nlights > 0) {
-        pdfa = pick / (sd.area * (float)s->nlights);
+        pdfa = pick / (sd.area * (real)s->nlights);
         if (s->has_env) pdfa *= 1 - x->c->env_prob;
     }
     ps->pdfa16 = q16(pdfa);
@@ -2615,10 +2732,10 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
             x->vis[0] = ps->inst_prim; x->vis[1] = sd.sn;
             store_aux(x, 0, v3(0, 0, 0), v3(1, 1, 1));
         }
-        float dpw;
+        real dpw;
         f3 e = mul(ps->thr, evaluate_environment(s, x->c, dir, &dpw));
         if (any_gt0(e)) {
-            float dpa = concentric_disc_pdfa() / pow2(s->sphere[3]);
+            real dpa = concentric_disc_pdfa() / pow2(s->sphere[3]);
             if (x->c->flags & ST_FLAG_ENV_MIS_BALANCED) dpa = dpw;
             add_emission(x, e, dpa, dpw, 1);
         }
@@ -2634,7 +2751,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
         m.bo = v3(s->tri_alb[3 * (size_t)alb_tri], s->tri_alb[3 * (size_t)alb_tri + 1], s->tri_alb[3 * (size_t)alb_tri + 2]);
     }
     f3 ldin = to_local(&sd, neg(dir));
-    float cos_in = fabsf(ldin.z);
+    real cos_in = fabsr(ldin.z);
     const int vc = (x->c->flags & ST_FLAG_BDPT_LVC) != 0;
     if (vc) { /* [tech. rep. (31)-(33)], completed at the hit (testrenderer.slang:826-834) */
         ps->dvcm = qvc(ps, ps->dvcm * mis1(pow2(ps->t16)));
@@ -2654,7 +2771,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     if (ldin.z > 0) {
         f3 e = mul(ps->thr, m_emission(&m));
         if (any_gt0(e)) {
-            float pa = ps->pdfa16;
+            real pa = ps->pdfa16;
             add_emission(x, e, pa, pa * cos_hemisphere_pdfw(ldin.z), cos_in / pow2(ps->t16));
         }
     }
@@ -2684,20 +2801,20 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     if (x->c->flags & ST_FLAG_DIRECT_ONLY) return 0; /* gSampleDirectIlluminationOnly, testrenderer.slang:886-888 */
     /* Russian roulette (testrenderer.slang:890-892); the draw is made at every
      * vertex, the test only from rr_start on */
-    float cp = minf(1, m_cont_prob(&m));
+    real cp = minf(1, m_cont_prob(&m));
     const int rr_on = ps->path_len >= x->c->rr_start;
-    float rr[4];
+    real rr[4];
     rng_nextf(&ps->rng, rr);
     if (rr_on && rr[0] > cp) return 0;
     if (!rr_on) cp = 1;
-    float rb[4];
+    real rb[4];
     rng_nextf(&ps->rng, rb);
     dsr_t smp = bsdf_sample(&m, x->c->lambert, v3(rb[0], rb[1], rb[2]), ldin);
     smp.fwd *= cp; smp.rev *= cp;
     ps->thr = mul(ps->thr, divf(smp.f, smp.fwd));
     if (smp.rough == 0) ps->dirpdf16 = 0;
     else ps->dirpdf16 = q16(smp.fwd);
-    if (vc) vc_after_sample(ps, &smp, fabsf(smp.dir.z), 0);
+    if (vc) vc_after_sample(ps, &smp, fabsr(smp.dir.z), 0);
     ps->thr = mulf(ps->thr, shading_normal_correction(&sd, ldin, smp.dir));
     f3 wd = to_world(&sd, smp.dir);
     ps->dir = pack_normal(wd);
@@ -2715,7 +2832,7 @@ static void adj_flush_shadows(pctx_t* x, uint32_t px, uint32_t py, uint32_t seed
         shadow_t sr = x->sh[i];
         f3 c = sr.c;
         if (x->s->nvolinst > 0) {
-            float neepdf;
+            real neepdf;
             trace_visibility_media(x, &sr, px, py, seed, &c, &neepdf, NULL);
             if (all_le0(c) || neepdf <= 0) continue;
             c = divf(c, neepdf);
@@ -2731,7 +2848,7 @@ static void adj_flush_shadows(pctx_t* x, uint32_t px, uint32_t py, uint32_t seed
         if (A->pass == 2 && x->s->nvolinst > 0) {
             adjd_t ad = {A->grad_grid, A->grad_vol, mul(A->g, add3)};
             f3 cb = sr.c;
-            float np;
+            real np;
             trace_visibility_media(x, &sr, px, py, seed, &cb, &np, &ad);
         }
     }
@@ -2752,19 +2869,19 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
         x->cache->nee_before[0] = x->cache->nee_before[1] = x->cache->nee_before[2] = 0;
     }
     rng_init(&ps->rng, seed, px, py, 0);
-    float ox = 0.5f, oy = 0.5f;
+    real ox = 0.5f, oy = 0.5f;
     if (x->c->flags & ST_FLAG_PIXEL_JITTER) {
-        float j[4];
+        real j[4];
         rng_nextf(&ps->rng, j);
         ox = j[0]; oy = j[1];
     }
     /* InitializeCameraPath, testrenderer.slang:938-982; ViewData::toWorld scene.h:152-157;
      * ProjectionData::backProject transform.h:159-170 */
-    float ew = (float)(view->image_max[0] - view->image_min[0]), eh = (float)(view->image_max[1] - view->image_min[1]);
-    float u = (((float)px + ox) - (float)view->image_min[0]) / ew, v = (((float)py + oy) - (float)view->image_min[1]) / eh;
-    float cx = 2 * u - 1, cy = -(2 * v - 1);
+    real ew = (real)(view->image_max[0] - view->image_min[0]), eh = (real)(view->image_max[1] - view->image_min[1]);
+    real u = (((real)px + ox) - (real)view->image_min[0]) / ew, v = (((real)py + oy) - (real)view->image_min[1]) / eh;
+    real cx = 2 * u - 1, cy = -(2 * v - 1);
     const st_projection* pr = &view->projection;
-    float sg = pr->near_plane > 0 ? 1.0f : (pr->near_plane < 0 ? -1.0f : 0.0f);
+    real sg = pr->near_plane > 0 ? 1.0f : (pr->near_plane < 0 ? -1.0f : 0.0f);
     f3 bp = v3(pr->near_plane * (cx * sg - pr->offset[0]) / pr->scale[0], pr->near_plane * (cy * sg - pr->offset[1]) / pr->scale[1],
                pr->near_plane);
     if (pr->vertical_fov < 0) bp = v3((cx - pr->offset[0]) / pr->scale[0], (cy - pr->offset[1]) / pr->scale[1], pr->near_plane);
@@ -2776,13 +2893,13 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
     ps->thr = v3(1, 1, 1);
     ps->dir = pack_normal(normalize3(xf_vector(T, ldir)));
     if (x->c->flags & ST_FLAG_BDPT_LVC) { /* testrenderer.slang:958-977 */
-        const float cos_cam = fabsf(ldir.z);
-        const float ipcd = x->ipd / cos_cam;
-        const float cam_pdfw = pow2(ipcd) / cos_cam;
+        const real cos_cam = fabsr(ldir.z);
+        const real ipcd = x->ipd / cos_cam;
+        const real cam_pdfw = pow2(ipcd) / cos_cam;
         ps->dirpdf16 = q16(cam_pdfw);
         ps->flag31 = 1;
         ps->vc_f32 = (x->c->flags & ST_FLAG_VC_FP32_MIS) != 0;
-        ps->dvcm = qvc(ps, mis1((float)x->L / cam_pdfw));
+        ps->dvcm = qvc(ps, mis1((real)x->L / cam_pdfw));
         ps->dvc = qvc(ps, 0.0f);
     }
     /* Render always extends once; RenderIteration returns early when the
@@ -2797,7 +2914,7 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
     for (int i = 0; i < x->nsh; i++) {
         shadow_t* sr = &x->sh[i];
         if (x->s->nvolinst > 0) {
-            float neepdf;
+            real neepdf;
             trace_visibility_media(x, sr, px, py, seed, &sr->c, &neepdf, NULL);
             if (all_le0(sr->c) || neepdf <= 0) continue;
             sr->c = divf(sr->c, neepdf);
@@ -2808,7 +2925,7 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
         }
         if (sr->w <= 0 || all_le0(sr->c)) continue;
         f3 rw = mulf(sr->c, sr->w);
-        float q[3] = {clamp0(rw.x) * (float)OR_Q, clamp0(rw.y) * (float)OR_Q, clamp0(rw.z) * (float)OR_Q};
+        real q[3] = {clamp0(rw.x) * (real)OR_Q, clamp0(rw.y) * (real)OR_Q, clamp0(rw.z) * (real)OR_Q};
         for (int k = 0; k < 3; k++) {
             if (x->c->flags & ST_FLAG_FLOAT_NEE) continue;
             int32_t iv = q[k] >= 2147483647.0f ? INT_MAX : (int32_t)q[k];
@@ -2821,9 +2938,9 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
      * holds every connection (with VC, also the light pass's camera connections) */
     if (x->light_acc)
         for (int k = 0; k < 3; k++) x->nee[k] += x->light_acc[k];
-    result->x = x->out.x + (float)x->nee[0] / (float)OR_Q;
-    result->y = x->out.y + (float)x->nee[1] / (float)OR_Q;
-    result->z = x->out.z + (float)x->nee[2] / (float)OR_Q;
+    result->x = x->out.x + (real)x->nee[0] / (real)OR_Q;
+    result->y = x->out.y + (real)x->nee[1] / (real)OR_Q;
+    result->z = x->out.z + (real)x->nee[2] / (real)OR_Q;
     *plen = ps->path_len;
 }
 
@@ -2835,7 +2952,7 @@ static void render_pixel(pctx_t* x, const st_render_params* p, const st_view* vi
 typedef struct {
     pctx_t b;       /* path state, scene, counters, view */
     uint32_t L;     /* light paths per sample */
-    float ipd;      /* ViewData::imagePlaneDist */
+    real ipd;      /* ViewData::imagePlaneDist */
     uint32_t seed;  /* this sample's seed */
     uint32_t* acc;  /* this thread's mOutputAtomic, 3 per pixel */
     lvert_t* verts; /* bidirectional mode: max_depth - 1 slots per light subpath */
@@ -2866,37 +2983,37 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
                               uint32_t q) {
     pctx_t* x = &lx->b;
     const f3 pc = xf_point(&x->inv_cam, pos);
-    float pcx = 0, pcy = 0;
+    real pcx = 0, pcy = 0;
     if (!to_raster(x->view, pc, &pcx, &pcy)) return;
-    const float cos_cam = fabsf(normalize3(pc).z);
+    const real cos_cam = fabsr(normalize3(pc).z);
     const st_transform* T = &x->view->camera_to_world;
     f3 tc = sub(v3(T->m[0][3], T->m[1][3], T->m[2][3]), pos);
-    const float dist = length3(tc);
+    const real dist = length3(tc);
     tc = divf(tc, dist);
     f3 f, o;
-    float cos_out, frev = 0;
+    real cos_out, frev = 0;
     if (med) {
-        const float v = hg_eval(med->g, ldin, tc);
+        const real v = hg_eval(med->g, ldin, tc);
         f = v3(v, v, v);
         cos_out = 1;
         o = pos;
         frev = v * med_cont_prob(med);
     } else {
         const f3 ldout = to_local(sd, tc);
-        cos_out = fabsf(ldout.z);
+        cos_out = fabsr(ldout.z);
         rer_t r = bsdf_eval_x(m, x->c->lambert, ldin, ldout, 1);
         f = mulf(r.f, shading_normal_correction_x(sd, ldin, ldout, 1));
         frev = r.rev * m_cont_prob(m);
         o = ray_offset_dir(pos, unpack_normal(sd->gn), tc);
     }
-    const float image_point_to_camera = lx->ipd / cos_cam;
-    const float image_to_solid_angle = pow2(image_point_to_camera) / cos_cam;
-    const float image_to_surface = image_to_solid_angle * cos_out / pow2(dist);
-    const float surface_to_image = 1.0f / image_to_surface;
-    f3 contrib = divf(mul(x->ps.thr, f), (float)lx->L * surface_to_image * cos_out);
-    float wmis = 1;
+    const real image_point_to_camera = lx->ipd / cos_cam;
+    const real image_to_solid_angle = pow2(image_point_to_camera) / cos_cam;
+    const real image_to_surface = image_to_solid_angle * cos_out / pow2(dist);
+    const real surface_to_image = 1.0f / image_to_surface;
+    f3 contrib = divf(mul(x->ps.thr, f), (real)lx->L * surface_to_image * cos_out);
+    real wmis = 1;
     if (x->c->flags & ST_FLAG_BDPT_LVC) { /* [tech. rep. (46)]; the eye sub-path weight is 0 (47) */
-        const float wl = mis1(image_to_surface / (float)lx->L) * (0 + x->ps.dvcm + x->ps.dvc * mis1(frev));
+        const real wl = mis1(image_to_surface / (real)lx->L) * (0 + x->ps.dvcm + x->ps.dvc * mis1(frev));
         wmis = 1.f / (wl + 1.f);
     }
     const uint32_t pix = (uint32_t)(int32_t)pcy * x->W + (uint32_t)(int32_t)pcx;
@@ -2905,7 +3022,7 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
         shadow_t sr;
         sr.o = o; sr.d = tc; sr.tmax = dist; sr.c = contrib; sr.w = 1;
         sr.medium = x->ps.medium; sr.plen = x->ps.path_len;
-        float neepdf;
+        real neepdf;
         trace_visibility_media(x, &sr, q % x->W, q / x->W, lx->seed, &contrib, &neepdf, NULL);
         if (all_le0(contrib) || neepdf <= 0) return;
         contrib = divf(contrib, neepdf);
@@ -2917,7 +3034,7 @@ static void connect_to_camera(lctx_t* lx, const material_t* m, const shading_t* 
     if (all_le0(contrib) || wmis <= 0) return;
     /* AddRadianceInterlocked, testrenderer.slang:237-261 */
     contrib = mulf(contrib, wmis);
-    const float qv[3] = {clamp0(contrib.x) * (float)OR_Q, clamp0(contrib.y) * (float)OR_Q, clamp0(contrib.z) * (float)OR_Q};
+    const real qv[3] = {clamp0(contrib.x) * (real)OR_Q, clamp0(contrib.y) * (real)OR_Q, clamp0(contrib.z) * (real)OR_Q};
     for (int k = 0; k < 3; k++) lx->acc[3 * (size_t)pix + k] += (uint32_t)(qv[k] >= 2147483647.0f ? INT_MAX : (int32_t)qv[k]);
 }
 
@@ -2933,25 +3050,25 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
     pstate_t* ps = &x->ps;
     memset(ps, 0, sizeof *ps);
     rng_init(&ps->rng, lx->seed, q % x->W, q / x->W, 0xFFFFu);
-    float rnd[4], dr[4];
+    real rnd[4], dr[4];
     rng_nextf(&ps->rng, rnd);
     const isr_t r = sample_illumination(s, c, rnd, v3(0, 0, 0));
     rng_nextf(&ps->rng, dr);
     f3 thr = divf(r.radiance, r.pdf), o, d;
-    float direct_pdfa = 1, emission_pdfw = 1, cos_light = 1;
+    real direct_pdfa = 1, emission_pdfw = 1, cos_light = 1;
     if (r.finite) {
         const f3 ld = sample_cos_hemisphere(dr[0], dr[1]);
         f3 onb[3];
         make_orthonormal(unpack_normal(r.normal), onb);
         o = ray_offset(r.pos, onb[2]);
         d = normalize3(add(add(mulf(onb[0], ld.x), mulf(onb[1], ld.y)), mulf(onb[2], ld.z)));
-        const float dirpdf = q16(cos_hemisphere_pdfw(ld.z)); /* mDirPdfW is stored in fp16 */
+        const real dirpdf = q16(cos_hemisphere_pdfw(ld.z)); /* mDirPdfW is stored in fp16 */
         thr = mulf(thr, ld.z / dirpdf);
         direct_pdfa = r.pdf;
         emission_pdfw = r.pdf * dirpdf;
         cos_light = ld.z;
     } else {
-        float xy[2];
+        real xy[2];
         sample_concentric_disc(dr[0], dr[1], xy);
         f3 onb[3];
         make_orthonormal(neg(r.dir), onb);
@@ -2977,7 +3094,7 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
         int kind;
         f3 mpos = v3(0, 0, 0);
         if (s->nvolinst > 0) {
-            float dirpdf, neepdf;
+            real dirpdf, neepdf;
             kind = trace_scattering(x, ps->origin, wd, &h, &mpos, &dirpdf, &neepdf);
             ps->thr = divf(ps->thr, dirpdf);
         } else {
@@ -3009,20 +3126,20 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
                 v->dvcm = ps->dvcm;
             }
             if (!med_singular(&m) && ps->path_len + 1 >= c->min_len) connect_to_camera(lx, NULL, NULL, &m, mpos, ldin, q);
-            const float cp = minf(1, med_cont_prob(&m));
+            const real cp = minf(1, med_cont_prob(&m));
             const int rr_on = ps->path_len >= c->rr_start;
-            float rr[4], rb[4];
+            real rr[4], rb[4];
             rng_nextf(&ps->rng, rr);
             if (rr_on && rr[0] > cp) return;
             rng_nextf(&ps->rng, rb);
             const f3 sdir = hg_sample(m.g, rb[0], rb[1], ldin);
-            const float v = hg_eval(m.g, ldin, sdir);
-            const float fwd = v * (rr_on ? cp : 1.0f);
+            const real v = hg_eval(m.g, ldin, sdir);
+            const real fwd = v * (rr_on ? cp : 1.0f);
             ps->thr = mulf(ps->thr, v / fwd);
             if (vc) {
                 dsr_t smp;
                 memset(&smp, 0, sizeof smp);
-                smp.fwd = fwd; smp.rev = fwd; smp.rough = (1 - fabsf(m.g)) == 0 ? 0.0f : 1.0f;
+                smp.fwd = fwd; smp.rev = fwd; smp.rough = (1 - fabsr(m.g)) == 0 ? 0.0f : 1.0f;
                 vc_after_sample(ps, &smp, 1.0f, 1);
             }
             ps->dir = pack_normal(sdir);
@@ -3033,7 +3150,7 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
             const material_t m = load_material(s, sd.flags_material >> 4);
             const f3 ldin = to_local(&sd, neg(wd));
             if (vc) { /* testrenderer.slang:812-824 */
-                const float cos_in = fabsf(ldin.z);
+                const real cos_in = fabsr(ldin.z);
                 ps->t16 = q16(h.t);
                 if (ps->path_len > 1 || ps->flag31) ps->dvcm = qvc(ps, ps->dvcm * mis1(pow2(ps->t16)));
                 ps->dvcm = qvc(ps, ps->dvcm / mis1(cos_in));
@@ -3052,17 +3169,17 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
                 v->dvcm = ps->dvcm;
             }
             if (!m_singular(&m, c->lambert) && ps->path_len + 1 >= c->min_len) connect_to_camera(lx, &m, &sd, NULL, sd.pos, ldin, q);
-            const float cp0 = minf(1, m_cont_prob(&m));
+            const real cp0 = minf(1, m_cont_prob(&m));
             const int rr_on = ps->path_len >= c->rr_start;
-            float rr[4], rb[4];
+            real rr[4], rb[4];
             rng_nextf(&ps->rng, rr);
             if (rr_on && rr[0] > cp0) return;
-            const float cp = rr_on ? cp0 : 1.0f;
+            const real cp = rr_on ? cp0 : 1.0f;
             rng_nextf(&ps->rng, rb);
             dsr_t smp = bsdf_sample_x(&m, c->lambert, v3(rb[0], rb[1], rb[2]), ldin, 1);
             smp.fwd *= cp; smp.rev *= cp;
             ps->thr = mul(ps->thr, divf(smp.f, smp.fwd));
-            if (vc) vc_after_sample(ps, &smp, fabsf(smp.dir.z), 1);
+            if (vc) vc_after_sample(ps, &smp, fabsr(smp.dir.z), 1);
             ps->thr = mulf(ps->thr, shading_normal_correction_x(&sd, ldin, smp.dir, 1));
             const f3 w = to_world(&sd, smp.dir);
             ps->dir = pack_normal(w);
@@ -3082,17 +3199,17 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
     c.env_prob = p->environment_sample_probability;
     c.max_depth = p->max_depth;
     c.min_len = p->min_path_length;
-    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) ? 1 : ((p->flags & ST_FLAG_LAMBERT_GGX) ? 2 : 0);
     c.max_null = p->max_null_collisions;
     c.rr_start = p->rr_start_depth;
     const uint32_t W = p->width, H = p->height, npix = W * H;
     /* mLightSubpathCount = max(1, uint(W * H * factor)), TestRenderer.cpp:452 */
-    const float Lf = (float)npix * p->light_subpath_factor; /* bounded before the conversion */
+    const real Lf = (real)npix * p->light_subpath_factor; /* bounded before the conversion */
     if (!(Lf < 2147483648.0f)) return fail("too many light subpaths");
     uint32_t L = (uint32_t)Lf;
     if (L < 1) L = 1;
     /* ViewData::imagePlaneDist, compat/scene.h:147 */
-    const float ipd = (float)abs(view->image_max[1] - view->image_min[1]) / (2 * tanf(view->projection.vertical_fov / 2));
+    const real ipd = (real)abs(view->image_max[1] - view->image_min[1]) / (2 * tanr(view->projection.vertical_fov / 2));
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     uint32_t* total = calloc((size_t)npix * 3, 4);
     f3* sum = calloc(npix, sizeof(f3));
@@ -3130,16 +3247,16 @@ static int render_light(or_scene* s, const st_view* view, const st_render_params
         if (oom) { free(total); free(sum); return fail("out of memory"); }
         /* ProcessAtomicOutput with gClearImage, then the mean over samples */
         for (uint32_t i = 0; i < npix; i++) {
-            const f3 r = v3(0.0f + (float)total[3 * i] / (float)OR_Q, 0.0f + (float)total[3 * i + 1] / (float)OR_Q,
-                            0.0f + (float)total[3 * i + 2] / (float)OR_Q);
+            const f3 r = v3(0.0f + (real)total[3 * i] / (real)OR_Q, 0.0f + (real)total[3 * i + 1] / (real)OR_Q,
+                            0.0f + (real)total[3 * i + 2] / (real)OR_Q);
             sum[i] = add(sum[i], r);
         }
     }
     if (out->hdr)
         for (uint32_t i = 0; i < npix; i++) {
-            out->hdr[4 * i + 0] = sum[i].x / (float)p->spp;
-            out->hdr[4 * i + 1] = sum[i].y / (float)p->spp;
-            out->hdr[4 * i + 2] = sum[i].z / (float)p->spp;
+            out->hdr[4 * i + 0] = sum[i].x / (real)p->spp;
+            out->hdr[4 * i + 1] = sum[i].y / (real)p->spp;
+            out->hdr[4 * i + 2] = sum[i].z / (real)p->spp;
             out->hdr[4 * i + 3] = 1;
         }
     if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
@@ -3188,15 +3305,15 @@ static int render_vc(or_scene* s, const st_view* view, const st_render_params* p
     c.flags = p->flags;
     c.env_prob = p->environment_sample_probability;
     c.max_depth = p->max_depth;
-    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) ? 1 : ((p->flags & ST_FLAG_LAMBERT_GGX) ? 2 : 0);
     c.max_null = p->max_null_collisions;
     c.rr_start = p->rr_start_depth;
     const uint32_t W = p->width, H = p->height, npix = W * H;
-    const float Lf = (float)npix * p->light_subpath_factor; /* bounded before the conversion */
+    const real Lf = (real)npix * p->light_subpath_factor; /* bounded before the conversion */
     if (!(Lf < 2147483648.0f)) return fail("too many light subpaths");
     uint32_t L = (uint32_t)Lf;
     if (L < 1) L = 1;
-    const float ipd = (float)abs(view->image_max[1] - view->image_min[1]) / (2 * tanf(view->projection.vertical_fov / 2));
+    const real ipd = (real)abs(view->image_max[1] - view->image_min[1]) / (2 * tanr(view->projection.vertical_fov / 2));
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     const size_t slots = (size_t)L * (p->max_depth - 1 ? p->max_depth - 1 : 1);
     const uint32_t view_medium = camera_medium(s, view);
@@ -3280,9 +3397,9 @@ static int render_vc(or_scene* s, const st_view* view, const st_render_params* p
     }
     if (!oom && out->hdr)
         for (uint32_t i = 0; i < npix; i++) {
-            out->hdr[4 * i + 0] = sum[i].x / (float)p->spp;
-            out->hdr[4 * i + 1] = sum[i].y / (float)p->spp;
-            out->hdr[4 * i + 2] = sum[i].z / (float)p->spp;
+            out->hdr[4 * i + 0] = sum[i].x / (real)p->spp;
+            out->hdr[4 * i + 1] = sum[i].y / (real)p->spp;
+            out->hdr[4 * i + 2] = sum[i].z / (real)p->spp;
             out->hdr[4 * i + 3] = 1;
         }
     if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
@@ -3317,13 +3434,13 @@ static void prim_make(const or_scene* s, const shading_t* sd, uint32_t dir, prim
     P->ldin = to_local(sd, neg(unpack_normal(dir)));
 }
 /* target function of a light-sample seed at a primary vertex (0 when unusable) */
-static float seed_target(pctx_t* x, const prim_t* P, const float seed[4], rsd_t* r) {
+static real seed_target(pctx_t* x, const prim_t* P, const real seed[4], rsd_t* r) {
     x->ps.path_len = 1;
     rsd_init(r, x, &P->m, &P->sd, P->ldin, seed);
     return r->src_pdf > 0 ? rsd_target(r) : 0.0f;
 }
-static float mis_pair(float a, float b) { /* mis(a, b), testrenderer.slang:287-290 */
-    const float a2 = a * a, d = a2 + b * b;
+static real mis_pair(real a, real b) { /* mis(a, b), testrenderer.slang:287-290 */
+    const real a2 = a * a, d = a2 + b * b;
     return d > 0 ? a2 / d : 0.0f;
 }
 
@@ -3337,15 +3454,15 @@ static void cache_store(const cache_t* c, f3 out_final, const uint32_t nee[3], c
     memset(r, 0, OR_CACHE_REC * sizeof(float));
     r[0] = (float)c->kind;
     if (c->kind) {
-        memcpy(r + 1, c->q, sizeof c->q);
-        memcpy(r + 13, c->thr, sizeof c->thr);
+        for (int k = 0; k < 12; k++) r[1 + k] = c->q[k];
+        for (int k = 0; k < 3; k++) r[13 + k] = c->thr[k];
     }
     if (c->kind == 2) {
-        const float ob[3] = {c->out_before.x, c->out_before.y, c->out_before.z};
-        for (int k = 0; k < 3; k++) r[16 + k] = ob[k] + (float)(c->nee_before[k] + extra[k]) / (float)OR_Q;
+        const real ob[3] = {c->out_before.x, c->out_before.y, c->out_before.z};
+        for (int k = 0; k < 3; k++) r[16 + k] = ob[k] + (real)(c->nee_before[k] + extra[k]) / (real)OR_Q;
     }
     r[19] = out_final.x; r[20] = out_final.y; r[21] = out_final.z;
-    for (int k = 0; k < 3; k++) r[22 + k] = (float)nee[k] / (float)OR_Q;
+    for (int k = 0; k < 3; k++) r[22 + k] = (real)nee[k] / (real)OR_Q;
 }
 
 static int render_reuse(or_scene* s, const st_view* view, const st_render_params* p, const or_outputs* out, int threads,
@@ -3353,9 +3470,9 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
     const uint32_t W = p->width, H = p->height;
     const size_t npix = (size_t)W * H;
     prim_t* prim = (prim_t*)calloc(npix, sizeof(prim_t));
-    float* rinit = (float*)calloc(npix * 8, sizeof(float));
-    float* rtemp = (float*)calloc(npix * 8, sizeof(float));
-    float* rprev = (float*)calloc(npix * 8, sizeof(float));
+    real* rinit = (real*)calloc(npix * 8, sizeof(real));
+    real* rtemp = (real*)calloc(npix * 8, sizeof(real));
+    real* rprev = (real*)calloc(npix * 8, sizeof(real));
     f3* em = (f3*)calloc(npix, sizeof(f3));
     uint32_t* nee = (uint32_t*)calloc(npix * 3, sizeof(uint32_t));
     f3* sum = (f3*)calloc(npix, sizeof(f3));
@@ -3368,7 +3485,7 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
     const uint32_t view_medium = camera_medium(s, view);
     const st_transform inv_cam = inverse_xf(&view->camera_to_world);
     uint64_t rays[2] = {0, 0}, trav[4] = {0, 0, 0, 0};
-    const float mmax = p->max_m * (float)c->cand;
+    const real mmax = p->max_m * (real)c->cand;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
@@ -3432,23 +3549,23 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
 #pragma omp for schedule(dynamic, 64)
             for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
                 const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
-                float* b = &rtemp[8 * pix];
-                memcpy(b, &rinit[8 * pix], 8 * sizeof(float));
-                const float* q = &rprev[8 * pix];
+                real* b = &rtemp[8 * pix];
+                memcpy(b, &rinit[8 * pix], 8 * sizeof(real));
+                const real* q = &rprev[8 * pix];
                 if (b[6] > 0 && k > 0 && q[6] > 0) {
                     rsd_t xr;
-                    const float tq = seed_target(&x, &prim[pix], q, &xr);
+                    const real tq = seed_target(&x, &prim[pix], q, &xr);
                     rng_t rng;
                     rng_init(&rng, seed, px, py, 1u << 20);
-                    float u[4];
+                    real u[4];
                     rng_nextf(&rng, u);
-                    const float m0 = mis_pair(b[6] * b[4], q[6] * b[4]), m1 = mis_pair(q[6] * q[4], b[6] * tq);
-                    const float w0 = m0 * b[4] * b[5], w1 = m1 * tq * q[5];
-                    const float total = w0 + w1;
-                    const float M = b[6] + q[6];
-                    float target = b[4];
+                    const real m0 = mis_pair(b[6] * b[4], q[6] * b[4]), m1 = mis_pair(q[6] * q[4], b[6] * tq);
+                    const real w0 = m0 * b[4] * b[5], w1 = m1 * tq * q[5];
+                    const real total = w0 + w1;
+                    const real M = b[6] + q[6];
+                    real target = b[4];
                     if (u[0] * total <= w1 && tq > 0) {
-                        memcpy(b, q, 4 * sizeof(float));
+                        memcpy(b, q, 4 * sizeof(real));
                         target = tq;
                     }
                     b[4] = target;
@@ -3460,11 +3577,11 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
 #pragma omp for schedule(dynamic, 64)
             for (int64_t pix = 0; pix < (int64_t)npix; pix++) {
                 const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
-                float b[8];
+                real b[8];
                 memcpy(b, &rtemp[8 * pix], sizeof b);
                 if (!(b[6] > 0)) {
-                    memset(&rprev[8 * pix], 0, 8 * sizeof(float));
-                    memcpy(&rprev[8 * pix], b, 4 * sizeof(float));
+                    memset(&rprev[8 * pix], 0, 8 * sizeof(real));
+                    memcpy(&rprev[8 * pix], b, 4 * sizeof(real));
                     continue;
                 }
                 const prim_t* P = &prim[pix];
@@ -3474,26 +3591,26 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
                 rng_t rng;
                 rng_init(&rng, seed, px, py, (1u << 20) + 64);
                 for (uint32_t n = 0; n < p->spatial_neighbours; n++) {
-                    float u[4], d[2];
+                    real u[4], d[2];
                     rng_nextf(&rng, u);
                     sample_concentric_disc(u[0], u[1], d);
-                    const int qx = (int)px + (int)floorf(d[0] * p->spatial_radius + 0.5f);
-                    const int qy = (int)py + (int)floorf(d[1] * p->spatial_radius + 0.5f);
+                    const int qx = (int)px + (int)floorr(d[0] * p->spatial_radius + 0.5f);
+                    const int qy = (int)py + (int)floorr(d[1] * p->spatial_radius + 0.5f);
                     if (qx < 0 || qy < 0 || qx >= (int)W || qy >= (int)H || (qx == (int)px && qy == (int)py)) continue;
                     const size_t qp = (size_t)qy * W + (size_t)qx;
-                    const float* q = &rtemp[8 * qp];
+                    const real* q = &rtemp[8 * qp];
                     if (!(q[6] > 0)) continue;
-                    if (fabsf(q[7] - b[7]) > 0.1f * b[7]) continue;
+                    if (fabsr(q[7] - b[7]) > 0.1f * b[7]) continue;
                     const prim_t* Q = &prim[qp];
                     if (dot(unpack_normal(Q->sd.sn), nc) < 0.9f) continue;
                     rsd_t xn, xq;
-                    const float tn = seed_target(&x, P, q, &xn);   /* the neighbour's seed here */
-                    const float tcq = seed_target(&x, Q, b, &xq);  /* our seed there */
-                    const float m0 = mis_pair(b[6] * b[4], q[6] * tcq), m1 = mis_pair(q[6] * q[4], b[6] * tn);
-                    const float w0 = m0 * b[4] * b[5], w1 = m1 * tn * q[5];
-                    const float total = w0 + w1;
+                    const real tn = seed_target(&x, P, q, &xn);   /* the neighbour's seed here */
+                    const real tcq = seed_target(&x, Q, b, &xq);  /* our seed there */
+                    const real m0 = mis_pair(b[6] * b[4], q[6] * tcq), m1 = mis_pair(q[6] * q[4], b[6] * tn);
+                    const real w0 = m0 * b[4] * b[5], w1 = m1 * tn * q[5];
+                    const real total = w0 + w1;
                     if (u[2] * total <= w1 && tn > 0) {
-                        memcpy(b, q, 4 * sizeof(float));
+                        memcpy(b, q, 4 * sizeof(real));
                         b[4] = tn;
                         cur = xn;
                     }
@@ -3502,13 +3619,13 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
                 }
                 memcpy(&rprev[8 * pix], b, sizeof b);
                 if (b[5] > 0 && b[4] > 0 && 2 >= c->min_len) {
-                    const float Wp = b[5] * cur.wtoa; /* 1/pdfA -> 1/pdfW */
+                    const real Wp = b[5] * cur.wtoa; /* 1/pdfA -> 1/pdfW */
                     const f3 contrib = mulf(cur.contrib, Wp); /* primary throughput is 1 */
                     hit_t h = trace(s, cur.vo, cur.vd, cur.vtmax, 1, &x.trav[2], &x.trav[3]);
                     x.rays[1]++;
                     if (!h.hit && cur.mis_w > 0 && !all_le0(contrib)) {
                         const f3 rw = mulf(contrib, cur.mis_w);
-                        const float qv[3] = {clamp0(rw.x) * (float)OR_Q, clamp0(rw.y) * (float)OR_Q, clamp0(rw.z) * (float)OR_Q};
+                        const real qv[3] = {clamp0(rw.x) * (real)OR_Q, clamp0(rw.y) * (real)OR_Q, clamp0(rw.z) * (real)OR_Q};
                         for (int j = 0; j < 3; j++) nee[3 * pix + j] += (uint32_t)(qv[j] >= 2147483647.0f ? INT_MAX : (int32_t)qv[j]);
                     }
                 }
@@ -3521,9 +3638,9 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
                     for (int j = 0; j < 3; j++) extra[j] = nee[3 * pix + j] - nee_path[3 * pix + j];
                     cache_store(&crec[pix], em[pix], &nee[3 * pix], extra, recs + OR_CACHE_REC * ((size_t)k * npix + (size_t)pix));
                 }
-                sum[pix].x += em[pix].x + (float)nee[3 * pix] / (float)OR_Q;
-                sum[pix].y += em[pix].y + (float)nee[3 * pix + 1] / (float)OR_Q;
-                sum[pix].z += em[pix].z + (float)nee[3 * pix + 2] / (float)OR_Q;
+                sum[pix].x += em[pix].x + (real)nee[3 * pix] / (real)OR_Q;
+                sum[pix].y += em[pix].y + (real)nee[3 * pix + 1] / (real)OR_Q;
+                sum[pix].z += em[pix].z + (real)nee[3 * pix + 2] / (real)OR_Q;
             }
 #pragma omp critical
             {
@@ -3534,9 +3651,9 @@ static int render_reuse(or_scene* s, const st_view* view, const st_render_params
     }
     if (out->hdr)
         for (size_t i = 0; i < npix; i++) {
-            out->hdr[4 * i + 0] = sum[i].x / (float)p->spp;
-            out->hdr[4 * i + 1] = sum[i].y / (float)p->spp;
-            out->hdr[4 * i + 2] = sum[i].z / (float)p->spp;
+            out->hdr[4 * i + 0] = sum[i].x / (real)p->spp;
+            out->hdr[4 * i + 1] = sum[i].y / (real)p->spp;
+            out->hdr[4 * i + 2] = sum[i].z / (real)p->spp;
             out->hdr[4 * i + 3] = 1;
         }
     if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
@@ -3579,7 +3696,7 @@ static int render_hashgrid(or_scene* s, const st_view* view, const st_render_par
     uint64_t* kin = (uint64_t*)calloc(ns, sizeof(uint64_t));
     int ok = sum && kin;
     for (int i = 0; i < 2; i++) {
-        g[i].rec = (float*)calloc(ns * 16, sizeof(float));
+        g[i].rec = (real*)calloc(ns * 16, sizeof(real));
         g[i].keys = (uint64_t*)calloc(ns, sizeof(uint64_t));
         g[i].slots = (uint32_t*)calloc(ns, sizeof(uint32_t));
         g[i].nslots = (uint32_t)ns;
@@ -3604,7 +3721,7 @@ static int render_hashgrid(or_scene* s, const st_view* view, const st_render_par
     for (uint32_t k = 0; k < p->spp; k++) {
         hgrid_t* cur = &g[k & 1];
         const hgrid_t* prev = k > 0 ? &g[(k + 1) & 1] : NULL;
-        memset(cur->rec, 0, ns * 16 * sizeof(float));
+        memset(cur->rec, 0, ns * 16 * sizeof(real));
 #pragma omp parallel
         {
             pctx_t x;
@@ -3648,7 +3765,7 @@ static int render_hashgrid(or_scene* s, const st_view* view, const st_render_par
         }
         /* HashGrid build (GpuHashGrid::build), deterministic */
         for (size_t i = 0; i < ns; i++) {
-            const float* e = cur->rec + 16 * i;
+            const real* e = cur->rec + 16 * i;
             kin[i] = e[9] > 0 ? (uint64_t)asuint(e[11]) : (1ull << 32);
             cur->slots[i] = (uint32_t)i;
         }
@@ -3659,9 +3776,9 @@ static int render_hashgrid(or_scene* s, const st_view* view, const st_render_par
     }
     if (out->hdr)
         for (size_t i = 0; i < npix; i++) {
-            out->hdr[4 * i + 0] = sum[i].x / (float)p->spp;
-            out->hdr[4 * i + 1] = sum[i].y / (float)p->spp;
-            out->hdr[4 * i + 2] = sum[i].z / (float)p->spp;
+            out->hdr[4 * i + 0] = sum[i].x / (real)p->spp;
+            out->hdr[4 * i + 1] = sum[i].y / (real)p->spp;
+            out->hdr[4 * i + 2] = sum[i].z / (real)p->spp;
             out->hdr[4 * i + 3] = 1;
         }
     if (out->ray_counts) { out->ray_counts[0] = rays[0]; out->ray_counts[1] = rays[1]; }
@@ -3707,7 +3824,7 @@ static int render_camera(or_scene* s, const st_view* view, const st_render_param
     c.max_depth = p->max_depth;
     c.min_len = p->min_path_length;
     c.cand = p->candidate_samples;
-    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) ? 1 : ((p->flags & ST_FLAG_LAMBERT_GGX) ? 2 : 0);
     c.max_null = p->max_null_collisions;
     c.rr_start = p->rr_start_depth;
     c.max_m = p->max_m;
@@ -3773,9 +3890,9 @@ static int render_camera(or_scene* s, const st_view* view, const st_render_param
                 sum = add(sum, r);
             }
             if (out->hdr) {
-                out->hdr[4 * pix + 0] = sum.x / (float)p->spp;
-                out->hdr[4 * pix + 1] = sum.y / (float)p->spp;
-                out->hdr[4 * pix + 2] = sum.z / (float)p->spp;
+                out->hdr[4 * pix + 0] = sum.x / (real)p->spp;
+                out->hdr[4 * pix + 1] = sum.y / (real)p->spp;
+                out->hdr[4 * pix + 2] = sum.z / (real)p->spp;
                 out->hdr[4 * pix + 3] = 1;
             }
         }
@@ -3843,7 +3960,7 @@ int or_adjoint(or_scene* s, const st_view* view, const st_render_params* p, cons
     c.max_depth = p->max_depth;
     c.min_len = p->min_path_length;
     c.cand = p->candidate_samples;
-    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+    c.lambert = (p->flags & ST_FLAG_LAMBERTIAN) ? 1 : ((p->flags & ST_FLAG_LAMBERT_GGX) ? 2 : 0);
     c.max_null = p->max_null_collisions;
     c.rr_start = p->rr_start_depth;
     const uint32_t view_medium = camera_medium(s, view);
@@ -3877,7 +3994,7 @@ int or_adjoint(or_scene* s, const st_view* view, const st_render_params* p, cons
             const uint32_t px = (uint32_t)(pix % W), py = (uint32_t)(pix / W);
             x.px = px;
             x.py = py;
-            A.g = v3(d_hdr[4 * pix] / (float)p->spp, d_hdr[4 * pix + 1] / (float)p->spp, d_hdr[4 * pix + 2] / (float)p->spp);
+            A.g = v3(d_hdr[4 * pix] / (real)p->spp, d_hdr[4 * pix + 1] / (real)p->spp, d_hdr[4 * pix + 2] / (real)p->spp);
             for (uint32_t k = 0; k < p->spp; k++) {
                 f3 r;
                 uint32_t pl;
@@ -3906,36 +4023,36 @@ int or_adjoint(or_scene* s, const st_view* view, const st_render_params* p, cons
  * src/Shaders/kernels/tonemap.slang:124-192, operators :30-117, viridisQuintic
  * and rgbToSrgb of compat/common.h:88-104, in the reference's operation order. */
 static f3 tm_div1p(f3 c) { return v3(c.x / (1 + c.x), c.y / (1 + c.y), c.z / (1 + c.z)); }
-static float tm_u2(float x) {
-    const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
+static real tm_u2(real x) {
+    const real A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
     return ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
 }
-static float tm_filmic1(float c) {
+static real tm_filmic1(real c) {
     c = maxf(0.0f, c - 0.004f);
     return (c * (6.2f * c + 0.5f)) / (c * (6.2f * c + 1.7f) + 0.06f);
 }
-static float tm_rtt1(float v) {
-    const float a = v * (v + 0.0245786f) - 0.000090537f;
-    const float b = v * (0.983729f * v + 0.4329510f) + 0.238081f;
+static real tm_rtt1(real v) {
+    const real a = v * (v + 0.0245786f) - 0.000090537f;
+    const real b = v * (0.983729f * v + 0.4329510f) + 0.238081f;
     return a / b;
 }
-static float tm_aces_approx1(float v) {
-    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+static real tm_aces_approx1(real v) {
+    const real a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
     v *= 0.6f;
     return saturate((v * (a * v + b)) / (v * (c * v + d) + e));
 }
-static f3 tm_viridis(float x) {
-    const float x2 = x * x, x3 = x2 * x, x4 = x * x3, x5 = x2 * x3;
+static f3 tm_viridis(real x) {
+    const real x2 = x * x, x3 = x2 * x, x4 = x * x3, x5 = x2 * x3;
     return v3((0.280268003f + -0.143510503f * x + 2.225793877f * x2 + -14.815088879f * x3) + (25.212752309f * x4 + -11.772589584f * x5),
               (-0.002117546f + 1.617109353f * x + -1.909305070f * x2 + 2.701152864f * x3) + (-1.685288385f * x4 + 0.178738871f * x5),
               (0.300805501f + 2.614650302f * x + -12.019139090f * x2 + 28.933559110f * x3) + (-33.491294770f * x4 + 13.762053843f * x5));
 }
-static float tm_srgb1(float c) { return c <= 0.0031308f ? c * 12.92f : powf(c * 1.055f, 1 / 2.4f) - 0.055f; }
+static real tm_srgb1(real c) { return c <= 0.0031308f ? c * 12.92f : powr(c * 1.055f, 1 / 2.4f) - 0.055f; }
 
-static f3 tm_apply(int mode, f3 c, const float mx[4]) {
+static f3 tm_apply(int mode, f3 c, const real mx[4]) {
     switch (mode) {
         case ST_TONEMAP_REINHARD: {
-            const float l = luminance(c);
+            const real l = luminance(c);
             const f3 tc = tm_div1p(c), a = divf(c, 1 + l);
             return add(a, mul(sub(tc, a), tc));
         }
@@ -3944,16 +4061,16 @@ static f3 tm_apply(int mode, f3 c, const float mx[4]) {
             return mul(tm_div1p(c), add(v3(1, 1, 1), divv(c, mul(m, m))));
         }
         case ST_TONEMAP_REINHARD_LUMINANCE: {
-            const float l = luminance(c), l1 = l / (1 + l);
+            const real l = luminance(c), l1 = l / (1 + l);
             return mulf(c, l1 / l);
         }
         case ST_TONEMAP_REINHARD_LUMINANCE_EXTENDED: {
-            const float l = luminance(c), m = mx[3] == 0 ? 1.0f : mx[3];
-            const float l1 = (l / (1 + l)) * (1 + l / (m * m));
+            const real l = luminance(c), m = mx[3] == 0 ? 1.0f : mx[3];
+            const real l1 = (l / (1 + l)) * (1 + l / (m * m));
             return mulf(c, l1 / l);
         }
         case ST_TONEMAP_UNCHARTED2: {
-            const float w = tm_u2(mx[3] == 0 ? 1.0f : mx[3]);
+            const real w = tm_u2(mx[3] == 0 ? 1.0f : mx[3]);
             return divf(v3(tm_u2(c.x), tm_u2(c.y), tm_u2(c.z)), w);
         }
         case ST_TONEMAP_FILMIC: return v3(tm_filmic1(c.x), tm_filmic1(c.y), tm_filmic1(c.z));
@@ -3980,19 +4097,19 @@ int or_tonemap(const float* hdr, const float* albedo, uint32_t w, uint32_t h, fl
     for (size_t i = 0; i < n; i++) {
         f3 v = v3(hdr[4 * i], hdr[4 * i + 1], hdr[4 * i + 2]);
         if (albedo) v = mul(v, v3(albedo[4 * i], albedo[4 * i + 1], albedo[4 * i + 2]));
-        const float l = luminance(v);
+        const real l = luminance(v);
         if (v.x != v.x || v.y != v.y || v.z != v.z || l != l || l <= 0) continue;
-        const float q[4] = {v.x * 16384.0f, v.y * 16384.0f, v.z * 16384.0f, l * 16384.0f};
+        const real q[4] = {v.x * 16384.0f, v.y * 16384.0f, v.z * 16384.0f, l * 16384.0f};
         for (int k = 0; k < 4; k++) {
-            const float cl = minf(maxf(q[k], 0.0f), 4294967296.0f);
+            const real cl = minf(maxf(q[k], 0.0f), 4294967296.0f);
             const uint32_t u = cl >= 4294967296.0f ? 0xffffffffu : (uint32_t)cl;
             if (u > mq[k]) mq[k] = u;
         }
     }
-    float mx[4];
-    for (int k = 0; k < 4; k++) mx[k] = (float)mq[k] / 16384.0f;
+    real mx[4];
+    for (int k = 0; k < 4; k++) mx[k] = (real)mq[k] / 16384.0f;
     if (max_out) for (int k = 0; k < 4; k++) max_out[k] = mx[k];
-    const float scale = powf(2.0f, exposure);
+    const real scale = powr(2.0f, exposure);
     for (size_t i = 0; i < n; i++) {
         f3 r = v3(hdr[4 * i], hdr[4 * i + 1], hdr[4 * i + 2]);
         if (albedo) r = mul(r, add(v3(1e-2f, 1e-2f, 1e-2f), v3(albedo[4 * i], albedo[4 * i + 1], albedo[4 * i + 2])));

@@ -35,6 +35,7 @@ ST_FLAG_VC_FP32_MIS = 1 << 12  # dVC / dVCM in fp32 instead of the reference's f
 ST_FLAG_SERIAL_WAVES = 1 << 13  # waves one after another on one stream (exclusive per-kernel times)
 ST_FLAG_RESTIR_DI_HASHGRID = 1 << 14  # the reference's hash-grid reservoir reuse (gReSTIR_DI_Reuse)
 ST_FLAG_HISTORY_RESET = 1 << 15  # drop the hash grid kept from the previous call (a camera cut)
+ST_FLAG_LAMBERT_GGX = 1 << 16  # per material: GGX conductor (metallic >= 0.5) or Lambert
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 # TonemapMode (src/Shaders/compat/tonemap.h:10-23)

@@ -205,6 +205,7 @@ struct Material {
     uint32_t p[4];
     float escale;
     bool has_bo;  // bo replaces the packed base colour (a per-triangle albedo)
+    uint32_t model;  // 2: the GGX conductor of ST_FLAG_LAMBERT_GGX (read by the Lambert instantiations only)
     v3 bo;
     __device__ __forceinline__ v3 base() const { return has_bo ? bo : mk(un8(p[0], 0), un8(p[0], 8), un8(p[0], 16)); }
     __device__ __forceinline__ v3 emission() const { return mk(un8(p[1], 0), un8(p[1], 8), un8(p[1], 16)) * escale; }
@@ -459,21 +460,102 @@ __device__ inline Sample disney_sample(const Material& m, v3 din, float rx, floa
     return r;
 }
 
-// evaluateReflectance: Lambert (lambertian.hlsli:13-24) or Disney (disney.hlsli:391-403)
-template <bool LAMBERT, bool ADJ = false>
-__device__ inline Eval bsdf_eval(const Material& m, v3 din, v3 dout) {
+// Visible-normal sampling as Heitz publishes it (JCGT 7(4), 2018, "Sampling the
+// GGX Distribution of Visible Normals", listing 1): the disk's first axis is
+// horizontal and its second lies in the plane of the normal and the view, so
+// the half-disk warp follows the visible hemisphere.  (sample_vndf, the
+// reference's, builds the disk frame with makeOrthonormal instead, whose axes
+// are not so aligned away from normal incidence: its directions are then not
+// distributed as the pdf it reports.)
+__device__ __forceinline__ v3 sample_vndf_heitz(v3 din, float ax, float ay, float r0, float r1) {
+    const bool flip = din.z < 0;
+    if (flip) din = -din;
+    const v3 vh = normalize(mk(ax * din.x, ay * din.y, din.z));
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    const v3 e1 = lensq > 0 ? mk(-vh.y, vh.x, 0) / sqrtf(lensq) : mk(1, 0, 0);
+    const v3 e2 = cross(vh, e1);
+    const float r = sqrtf(r0), phi = 2 * ST_PI * r1;
+    float t1 = r * cos_(phi), t2 = r * sin_(phi);
+    const float s = (1 + vh.z) / 2;
+    t2 = (1 - s) * sqrtf(1 - t1 * t1) + s * t2;
+    const v3 nh = e1 * t1 + e2 * t2 + vh * sqrtf(fmax_(0.0f, 1 - t1 * t1 - t2 * t2));
+    const v3 o = normalize(mk(ax * nh.x, ay * nh.y, fmax_(0.0f, nh.z)));
+    return flip ? -o : o;
+}
+
+// The GGX conductor of ST_FLAG_LAMBERT_GGX: the metallic lobe of the Disney
+// BSDF alone (disney.hlsli:56-70 and :163-169 at metallic 1, no clearcoat):
+// f = F(h.o) D(h) G1(i) G1(o) / (4 |n.i|) (the cosine of o included), F
+// Schlick's about the base colour, D the anisotropic GGX (GTR2) with Disney's
+// (alpha / aspect, alpha * aspect), alpha = roughness^2 (at least 1e-4), G1
+// Smith's; directions from the visible normals (sample_vndf_heitz), pdf =
+// D G1(i) / (4 |n.i|).
+// Reflection only, two-sided as the Lambert model.
+__device__ inline Eval ggx_eval(const Material& m, v3 din, v3 dout) {
     Eval r;
-    if (LAMBERT) {
-        if (din.z * dout.z < 0) {
-            r.f = mk(0, 0, 0);
-            r.fwd = r.rev = 0;
-            return r;
-        }
-        r.f = (m.base() * fabsf(dout.z)) / ST_PI;
-        r.fwd = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(dout.z)));
-        r.rev = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(din.z)));
+    r.f = mk(0, 0, 0);
+    r.fwd = r.rev = 0;
+    if (!(din.z * dout.z > 0)) return r;
+    if (din.z < 0) {
+        din = -din;
+        dout = -dout;
+    }
+    const float aspect = sqrtf(1 - m.aniso() * 0.9f), alpha = sq(m.rough());
+    const float ax = fmax_(1e-4f, alpha / aspect), ay = fmax_(1e-4f, alpha * aspect);
+    const v3 h = normalize(din + dout);
+    const float D = gtr2(h, ax, ay), Gi = smith2(din, ax, ay), Go = smith2(dout, ax, ay);
+    const v3 F = schlick3(m.base(), dot(h, dout));
+    r.f = (F * (D * Gi * Go)) / (4 * din.z);
+    r.fwd = D * Gi / (4 * din.z);
+    r.rev = D * Go / (4 * dout.z);
+    return r;
+}
+__device__ inline Sample ggx_sample(const Material& m, v3 din, float rx, float ry) {
+    const float aspect = sqrtf(1 - m.aniso() * 0.9f), alpha = sq(m.rough());
+    const float ax = fmax_(1e-4f, alpha / aspect), ay = fmax_(1e-4f, alpha * aspect);
+    const v3 h = sample_vndf_heitz(din, ax, ay, rx, ry);
+    Sample r;
+    r.dir = normalize(-din + h * (2 * dot(din, h)));
+    const Eval f = ggx_eval(m, din, r.dir);
+    r.f = f.f;
+    r.fwd = f.fwd;
+    r.rev = f.rev;
+    r.eta = 0;
+    r.rough = m.rough();
+    return r;
+}
+__device__ inline Eval lambert_eval(const Material& m, v3 din, v3 dout) {
+    Eval r;
+    if (din.z * dout.z < 0) {
+        r.f = mk(0, 0, 0);
+        r.fwd = r.rev = 0;
         return r;
     }
+    r.f = (m.base() * fabsf(dout.z)) / ST_PI;
+    r.fwd = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(dout.z)));
+    r.rev = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(din.z)));
+    return r;
+}
+__device__ inline Sample lambert_sample(const Material& m, v3 din, float rx, float ry) {
+    Sample r;
+    r.dir = sample_cos_hemisphere(rx, ry);
+    if (din.z < 0) r.dir = -r.dir;
+    r.f = (m.base() * fabsf(r.dir.z)) / ST_PI;
+    r.fwd = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(r.dir.z)));
+    r.rev = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(din.z)));
+    r.rough = 1;
+    r.eta = 0;
+    return r;
+}
+
+// evaluateReflectance: Lambert (lambertian.hlsli:13-24) or Disney (disney.hlsli:391-403).
+// The Lambert instantiations also serve ST_FLAG_LAMBERT_GGX, whose metallic
+// materials are GGX conductors (model 2), so the Disney kernels carry no
+// per-material branch.
+template <bool LAMBERT, bool ADJ = false>
+__device__ inline Eval bsdf_eval(const Material& m, v3 din, v3 dout) {
+    if (LAMBERT) return m.model == 2 ? ggx_eval(m, din, dout) : lambert_eval(m, din, dout);
+    Eval r;
     r.f = disney_eval<ADJ>(m, din, dout);
     r.fwd = disney_pdf(m, din, dout);
     r.rev = disney_pdf(m, dout, din);
@@ -482,17 +564,7 @@ __device__ inline Eval bsdf_eval(const Material& m, v3 din, v3 dout) {
 // sampleDirection: lambertian.hlsli:29-39, disney.hlsli:415-429
 template <bool LAMBERT, bool ADJ = false>
 __device__ inline Sample bsdf_sample(const Material& m, float rx, float ry, float rz, v3 din) {
-    if (LAMBERT) {
-        Sample r;
-        r.dir = sample_cos_hemisphere(rx, ry);
-        if (din.z < 0) r.dir = -r.dir;
-        r.f = (m.base() * fabsf(r.dir.z)) / ST_PI;
-        r.fwd = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(r.dir.z)));
-        r.rev = cos_hemisphere_pdfw(fmax_(1e-5f, fabsf(din.z)));
-        r.rough = 1;
-        r.eta = 0;
-        return r;
-    }
+    if (LAMBERT) return m.model == 2 ? ggx_sample(m, din, rx, ry) : lambert_sample(m, din, rx, ry);
     Sample r = disney_sample(m, din, rx, ry, rz);
     Eval f = bsdf_eval<false, ADJ>(m, din, r.dir);
     r.f = f.f;
@@ -502,7 +574,7 @@ __device__ inline Sample bsdf_sample(const Material& m, float rx, float ry, floa
 }
 template <bool LAMBERT>
 __device__ __forceinline__ bool m_singular(const Material& m) {
-    if (LAMBERT) return false;
+    if (LAMBERT) return m.model == 2 && (1 - m.rough()) > 0.75f;  // a GGX conductor: isSingular at metallic 1
     return fmax_(fmax_(m.spec(), m.metal()), m.trans()) * (1 - m.rough()) > 0.75f;
 }
 __device__ __forceinline__ float m_cont_prob(const Material& m) { return sat(lum(m.base())); }

@@ -792,7 +792,7 @@ __device__ inline void hash_grid_reuse(const DevScene& S, const RenderConsts& c,
                 const v3 lp = mk(e0.x, e0.y, e0.z);
                 Shading sq;
                 tri_shading(S, id & 0xffffu, id >> 16, 0.0f, 0.0f, sq, &lp);
-                const Material mq = load_material(S, sq.material);
+                const Material mq = load_material(S, sq.material, c.flags);
                 const Frame fq = make_frame(sq);
                 Cand xq;
                 float lrq;
@@ -922,7 +922,7 @@ struct Prim {
 };
 // The triangle's shading data was computed once, by k_ris, and stored bit for
 // bit, so the reuse passes read one 32-byte record instead of the mesh buffers.
-__device__ __forceinline__ void rebuild_primary(const DevScene& S, const WaveState& w, size_t idx, Prim& P) {
+__device__ __forceinline__ void rebuild_primary(const DevScene& S, const WaveState& w, size_t idx, Prim& P, uint32_t flags) {
     const uint4 rec = w.pri[idx];
     const float4 a = w.psd[2 * idx], b = w.psd[2 * idx + 1];
     P.sd.pos = mk(a.x, a.y, a.z);
@@ -931,7 +931,7 @@ __device__ __forceinline__ void rebuild_primary(const DevScene& S, const WaveSta
     P.sd.sn = __float_as_uint(b.y);
     P.sd.tan = __float_as_uint(b.z);
     P.sd.area = b.w;
-    P.m = load_material(S, P.sd.material);
+    P.m = load_material(S, P.sd.material, flags);
     P.f = make_frame(P.sd);
     P.lgn = P.f.to_local(unpack_normal(P.sd.gn));
     P.ldin = P.f.to_local(-unpack_normal(rec.w));
@@ -970,7 +970,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c,
         Rng rng{px, py, c.seed + c.sample_base + k, s4.x};
         Shading sd;
         tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
-        const Material m = load_material(S, sd.material);
+        const Material m = load_material(S, sd.material, c.flags);
         const Frame f = make_frame(sd);
         const v3 ldin = f.to_local(-unpack_normal(__float_as_uint(r.w)));
         const v3 lgn = f.to_local(unpack_normal(sd.gn));
@@ -1177,7 +1177,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 const float thit = MEDIA ? len(sd.pos - mk(r.x, r.y, r.z)) : __uint_as_float(hit.x);
                 const float t16 = q16(thit), pdfa16 = q16(pdfa);
                 const bool live = !MEDIA || !all_nonpos(thr);  // ExtendPath, testrenderer.slang:1062-1063
-                Material m = load_material(S, sd.material);
+                Material m = load_material(S, sd.material, c.flags);
                 uint32_t alb_entry = 0xffffffffu;
                 if (LAMBERT && S.tri_alb) apply_triangle_albedo(S, m, ii, prim, alb_entry);
                 if (live && path_len == 1 && first_wave && k == 0)
@@ -1297,7 +1297,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_temporal(DevScene S, Ren
             const float4 qs = w.r_prev[2 * pix], qb = w.r_prev[2 * pix + 1];
             if (qb.z > 0) {
                 Prim P;
-                rebuild_primary(S, w, rp, P);
+                rebuild_primary(S, w, rp, P, c.flags);
                 Cand x;
                 const float tq = seed_target<LAMBERT, ALIAS>(S, c, P, qs, x);  // q's seed here
                 const uint32_t px = pix % c.W, py = pix / c.W;
@@ -1334,7 +1334,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
             float4 seed = w.r_temp[2 * (jb + pix)], b = w.r_temp[2 * (jb + pix) + 1];
             if (b.z > 0) {
                 Prim P;
-                rebuild_primary(S, w, jb + pix, P);
+                rebuild_primary(S, w, jb + pix, P, c.flags);
                 const v3 nc = unpack_normal(P.sd.sn);
                 Cand cur;
                 seed_target<LAMBERT, ALIAS>(S, c, P, seed, cur);
@@ -1352,7 +1352,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_restir_spatial(DevScene S, Rend
                     // similar surfaces only: depth within 10%, normals within ~25 degrees
                     if (fabsf(qb.w - b.w) > 0.1f * b.w) continue;
                     Prim Q;
-                    rebuild_primary(S, w, jb + qpix, Q);
+                    rebuild_primary(S, w, jb + qpix, Q, c.flags);
                     if (dot(unpack_normal(Q.sd.sn), nc) < 0.9f) continue;
                     Cand xn, xq;
                     const float tn = seed_target<LAMBERT, ALIAS>(S, c, P, qs, xn);     // neighbour's seed here
@@ -1597,7 +1597,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
                 } else {
                     Shading sd;
                     tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
-                    const Material m = load_material(S, sd.material);
+                    const Material m = load_material(S, sd.material, c.flags);
                     const Frame f = make_frame(sd);
                     const v3 ldin = f.to_local(-dir);
                     const v3 lgn = f.to_local(unpack_normal(sd.gn));
@@ -1756,7 +1756,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_vc(DevScene S, RenderCons
                     if (S.has_env) pdfa *= 1 - c.env_prob;
                 }
                 const float t16 = q16(__uint_as_float(hit.x)), pdfa16 = q16(pdfa);
-                const Material m = load_material(S, sd.material);
+                const Material m = load_material(S, sd.material, c.flags);
                 const Frame f = make_frame(sd);
                 const v3 ldin = f.to_local(-dir);
                 const v3 lgn = f.to_local(unpack_normal(sd.gn));
@@ -1809,7 +1809,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_vc(DevScene S, RenderCons
                             v3 d = sv.pos - sd.pos;
                             const float dist = len(d);
                             d = d / dist;
-                            const Material mv = load_material(S, sv.material);
+                            const Material mv = load_material(S, sv.material, c.flags);
                             const Frame fv = make_frame(sv);
                             const v3 vin = unpack_normal(__float_as_uint(v1.w));
                             const v3 vout = fv.to_local(-d);
@@ -3231,7 +3231,7 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
         c.hg_dmax = std::min<uint32_t>(2, p->max_depth - 1);
         c.hg_nslots = npix * c.hg_dmax;
     }
-    const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+    const bool lambert = (p->flags & (ST_FLAG_LAMBERTIAN | ST_FLAG_LAMBERT_GGX)) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
     const bool restir = (p->flags & ST_FLAG_RESTIR_DI) != 0;
     const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
     unsigned long long* stats = j.stats;
@@ -3516,7 +3516,7 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
         const uint32_t waves = (p->spp + k - 1) / k;
         const int slots = (p->flags & ST_FLAG_SERIAL_WAVES) ? 1 : (int)std::min<uint32_t>(kWaveSlots, waves);
         const int tgrid = ctx->cus * kTraceBlocksPerCU, sgrid = ctx->cus * 8;
-        const bool media = s->d.vinst_count > 0, lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0;
+        const bool media = s->d.vinst_count > 0, lambert = (p->flags & (ST_FLAG_LAMBERTIAN | ST_FLAG_LAMBERT_GGX)) != 0;
         const bool alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
         order_after_caller(ctx);
         for (int sl = 0; sl < slots; sl++) {
@@ -4349,7 +4349,7 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
         // ViewData::imagePlaneDist, compat/scene.h:147
         c.ipd = (float)std::abs(view->image_max[1] - view->image_min[1]) / (2 * std::tan(view->projection.vertical_fov / 2));
         c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
-        const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+        const bool lambert = (p->flags & (ST_FLAG_LAMBERTIAN | ST_FLAG_LAMBERT_GGX)) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
         const bool media = s->d.vinst_count > 0;
         auto* stats = ctx->stats.as<unsigned long long>();
@@ -4480,7 +4480,7 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
         c.ipd = (float)std::abs(view->image_max[1] - view->image_min[1]) / (2 * std::tan(view->projection.vertical_fov / 2));
         c.tiled = (p->width % 8 == 0 && p->height % 8 == 0) ? 1 : 0;
         c.k = 1;
-        const bool lambert = (p->flags & ST_FLAG_LAMBERTIAN) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
+        const bool lambert = (p->flags & (ST_FLAG_LAMBERTIAN | ST_FLAG_LAMBERT_GGX)) != 0, alias = (p->flags & ST_FLAG_ENV_ALIAS) != 0;
         const bool count = (p->flags & ST_FLAG_COUNT_TRAVERSAL) != 0;
         auto* stats = ctx->stats.as<unsigned long long>();
         const uint32_t depth = std::max<uint32_t>(1, p->max_depth);

@@ -152,7 +152,9 @@ __device__ __forceinline__ float shading_normal_correction(const Frame& f, v3 lg
 }
 
 // LoadMaterial, material.hlsli:15-21 (configurations carry no material textures)
-__device__ __forceinline__ Material load_material(const DevScene& S, uint32_t addr) {
+// flags: the render's ST_FLAG_*; with ST_FLAG_LAMBERT_GGX (and not
+// ST_FLAG_LAMBERTIAN) a material of metallic at least 0.5 is a GGX conductor
+__device__ __forceinline__ Material load_material(const DevScene& S, uint32_t addr, uint32_t flags) {
     Material m;
     const uint4 w = *reinterpret_cast<const uint4*>(S.mat + addr / 4);
     m.p[0] = w.x;
@@ -162,6 +164,7 @@ __device__ __forceinline__ Material load_material(const DevScene& S, uint32_t ad
     m.escale = __uint_as_float(S.mat[addr / 4 + 8]);
     m.has_bo = false;
     m.bo = mk(0, 0, 0);
+    m.model = ((flags & (ST_FLAG_LAMBERT_GGX | ST_FLAG_LAMBERTIAN)) == ST_FLAG_LAMBERT_GGX && m.metal() >= 0.5f) ? 2u : 0u;
     return m;
 }
 

@@ -1,15 +1,20 @@
 """Converged-image check of the Cornell-box frame (256x256, maxPathLength 4,
-Disney materials, 4096 spp): the HIP renderer against the CPU oracle.
+Disney materials, 4096 spp): the HIP renderer against the CPU oracle, in its
+float build (the renderer's own arithmetic) and its double build (the same
+algorithm computing in double, libm's double transcendentals).
 
-    python tests/converged_rmse.py gpu    OUT.npy            # on a GPU machine
-    python tests/converged_rmse.py oracle SEED OUT.npy       # CPU, seeds SEED..SEED+4095
-    python tests/converged_rmse.py compare GPU.npy ORACLE_SAME.npy ORACLE_INDEP.npy
+    python tests/converged_rmse.py gpu      OUT.npy          # on a GPU machine
+    python tests/converged_rmse.py oracle   SEED OUT.npy     # CPU float build, seeds SEED..SEED+4095
+    python tests/converged_rmse.py oracle64 SEED OUT.npy     # CPU double build
+    python tests/converged_rmse.py compare GPU.npy F32_SAME.npy F32_INDEP.npy [F64_SAME.npy F64_INDEP.npy]
 
-`compare` prints the RMSE of the GPU image against the oracle on the same PCG
-streams (seeds 0..4095: parity at convergence) and against the oracle on an
-independent seed range, next to the oracle-vs-oracle RMSE of the two seed
-ranges (the Monte Carlo noise floor).  An unbiased renderer sits at the floor.
-Not collected by pytest: the oracle takes minutes per image.
+`compare` prints, for each pair, the RMSE over pixels and the difference of
+the image means with its standard error.  "Same" images use the GPU's seeds
+(0..4095, the same PCG streams); "indep" images use seeds 4096..8191.  The
+RMSE of two independent renders is the Monte Carlo noise floor: an unbiased
+renderer sits at it.  The mean difference is judged against its standard
+error, estimated from the spread of the 256 block means (16x16 pixels) of an
+independent pair.  Not collected by pytest: an oracle image takes minutes.
 """
 import os
 import sys
@@ -39,9 +44,10 @@ def render_gpu(out):
     np.save(out, img.cpu().numpy()[..., :3])
 
 
-def render_oracle(seed, out):
+def render_oracle(seed, out, fp64=False):
     from oracle.oracle import OracleScene
-    img = OracleScene(scenes.cornell_box(0)).render(scenes.cornell_view(W, H), params(seed), want_path_length=False)["hdr"]
+    img = OracleScene(scenes.cornell_box(0), fp64=fp64).render(scenes.cornell_view(W, H), params(seed),
+                                                               want_path_length=False)["hdr"]
     np.save(out, img[..., :3])
 
 
@@ -49,13 +55,35 @@ def rmse(a, b):
     return float(np.sqrt(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2)))
 
 
-def compare(g, o_same, o_indep):
-    g, a, b = np.load(g), np.load(o_same), np.load(o_indep)
-    m = float(a.mean())
-    for name, v in [("gpu vs oracle, same streams", rmse(g, a)), ("gpu vs oracle, independent seeds", rmse(g, b)),
-                    ("oracle vs oracle, independent seeds (noise floor)", rmse(a, b))]:
-        print(f"{name}: RMSE {v:.6g} ({v / m:.4%} of the mean {m:.5g})")
-    print(f"image means: gpu {g.mean():.6g}, oracle {a.mean():.6g}, oracle (independent) {b.mean():.6g}")
+def block_means(a, n=16):
+    a = a.astype(np.float64).mean(-1)
+    return a.reshape(H // n, n, W // n, n).mean((1, 3)).ravel()
+
+
+def mean_diff(a, b, se_pair):
+    """Difference of the image means and its standard error: the spread of the
+    block-mean differences of an independent pair, over sqrt(blocks)."""
+    d = block_means(se_pair[0]) - block_means(se_pair[1])
+    se = float(d.std(ddof=1) / np.sqrt(d.size))
+    return float(a.astype(np.float64).mean() - b.astype(np.float64).mean()), se
+
+
+def compare(g, f32_same, f32_indep, f64_same=None, f64_indep=None):
+    g, a, b = np.load(g), np.load(f32_same), np.load(f32_indep)
+    pairs = [("gpu vs float oracle, same streams", g, a), ("gpu vs float oracle, independent seeds", g, b),
+             ("float oracle vs float oracle, independent seeds (noise floor)", a, b)]
+    se_pair = (a, b)
+    if f64_same:
+        c, e = np.load(f64_same), np.load(f64_indep)
+        pairs += [("gpu vs double oracle, same streams", g, c), ("gpu vs double oracle, independent seeds", g, e),
+                  ("float oracle vs double oracle, same streams", a, c),
+                  ("double oracle vs double oracle, independent seeds (noise floor)", c, e)]
+    m = float(a.astype(np.float64).mean())
+    for name, x, y in pairs:
+        v = rmse(x, y)
+        d, se = mean_diff(x, y, se_pair)
+        print(f"{name}: RMSE {v:.6g} ({v / m:.4%} of the mean {m:.5g}); mean difference {d:+.3e} "
+              f"(standard error of an independent pair {se:.2e}, {d / se:+.2f} of it)")
 
 
 if __name__ == "__main__":
@@ -63,5 +91,7 @@ if __name__ == "__main__":
         render_gpu(sys.argv[2])
     elif sys.argv[1] == "oracle":
         render_oracle(int(sys.argv[2]), sys.argv[3])
+    elif sys.argv[1] == "oracle64":
+        render_oracle(int(sys.argv[2]), sys.argv[3], fp64=True)
     else:
-        compare(*sys.argv[2:5])
+        compare(*sys.argv[2:7])

@@ -282,6 +282,32 @@ def test_restir_hashgrid_reuse_vs_oracle(gpu_ctx, case):
     _check_image(g, r)
 
 
+@pytest.mark.parametrize("case", ["cornell_256", "field_restir", "field_reuse"])
+def test_lambert_ggx_vs_oracle(gpu_ctx, case):
+    """Per-material Lambert / GGX (ST_FLAG_LAMBERT_GGX): the seeded Cornell box
+    of 50% Lambert / 50% GGX materials at 256x256, 4 spp, maxPathLength 4; and
+    the sphere field with the alias-sampled environment, RIS at every vertex,
+    and the screen-space reuse with roulette from length 3."""
+    flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_LAMBERT_GGX
+    if case == "cornell_256":
+        sd = scenes.cornell_box(0)
+        W = H = 256
+        view = scenes.cornell_view(W, H)
+        params = abi.default_params(W, H, max_depth=4, spp=4, flags=flags)
+    else:
+        sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+        W, H = 96, 64
+        view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+        flags |= abi.ST_FLAG_ENV_ALIAS | abi.ST_FLAG_RESTIR_DI
+        if case == "field_reuse":
+            flags |= abi.ST_FLAG_RESTIR_DI_REUSE
+        params = abi.default_params(W, H, max_depth=6, spp=4, flags=flags, rr_start_depth=3, candidate_samples=32,
+                                    max_m=3.0, spatial_neighbours=5, spatial_radius=30.0)
+    print(f"case {case}", flush=True)
+    g, r = _render_both(gpu_ctx, sd, view, params)
+    _check_image(g, r)
+
+
 def test_restir_hashgrid_argument_checks(gpu_ctx):
     from stratum2_amd.renderer import Batch, Scene, TestRenderer
     sd = scenes.cornell_box(0)
