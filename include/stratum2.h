@@ -200,6 +200,14 @@ typedef struct st_view {
  * switch (gLambertian makes every material Lambert, testrenderer.slang:33-37);
  * ST_FLAG_LAMBERTIAN takes precedence. */
 #define ST_FLAG_LAMBERT_GGX             (1u << 16)
+/* Camera-path st_render only: return once the work is enqueued on the
+ * context's streams, joined into the context stream, which waits for the
+ * caller's null-stream work at entry.  The outputs (ray_counts included,
+ * written on the device) are complete after st_sync(ctx); calls on one
+ * context run in order.  No timings or statistics are collected for such a
+ * call (st_last_timings and st_last_kernel_times keep the last synchronous
+ * call's). */
+#define ST_FLAG_ASYNC                   (1u << 17)
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface

@@ -36,6 +36,7 @@ ST_FLAG_SERIAL_WAVES = 1 << 13  # waves one after another on one stream (exclusi
 ST_FLAG_RESTIR_DI_HASHGRID = 1 << 14  # the reference's hash-grid reservoir reuse (gReSTIR_DI_Reuse)
 ST_FLAG_HISTORY_RESET = 1 << 15  # drop the hash grid kept from the previous call (a camera cut)
 ST_FLAG_LAMBERT_GGX = 1 << 16  # per material: GGX conductor (metallic >= 0.5) or Lambert
+ST_FLAG_ASYNC = 1 << 17  # st_render returns once enqueued; outputs complete after st_sync
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 # TonemapMode (src/Shaders/compat/tonemap.h:10-23)

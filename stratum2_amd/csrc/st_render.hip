@@ -2200,6 +2200,7 @@ struct WaveBufs {
     hipStream_t stream = nullptr;        // slot 0: the context's stream
     hipEvent_t reuse_done = nullptr;     // this slot's last wave finished its screen-space reuse
     hipEvent_t resolve_done = nullptr;   // this slot's last wave finished its resolve
+    hipEvent_t done = nullptr;           // this slot's part of an ST_FLAG_ASYNC call is enqueued up to here
 };
 
 struct st_context {
@@ -2564,6 +2565,7 @@ int st_context_create(int device, st_context** out) {
         for (WaveBufs& b : ctx->slot) {
             HIP_CHECK(hipEventCreateWithFlags(&b.reuse_done, hipEventDisableTiming));
             HIP_CHECK(hipEventCreateWithFlags(&b.resolve_done, hipEventDisableTiming));
+            HIP_CHECK(hipEventCreateWithFlags(&b.done, hipEventDisableTiming));
         }
         HIP_CHECK(hipEventCreateWithFlags(&ctx->start_done, hipEventDisableTiming));
         HIP_CHECK(hipEventCreateWithFlags(&ctx->entry, hipEventDisableTiming));
@@ -2588,6 +2590,7 @@ int st_context_destroy(st_context* ctx) {
     for (WaveBufs& b : ctx->slot) {
         if (b.reuse_done) (void)hipEventDestroy(b.reuse_done);
         if (b.resolve_done) (void)hipEventDestroy(b.resolve_done);
+        if (b.done) (void)hipEventDestroy(b.done);
     }
     if (ctx->start_done) (void)hipEventDestroy(ctx->start_done);
     if (ctx->entry) (void)hipEventDestroy(ctx->entry);
@@ -3162,6 +3165,17 @@ struct CameraJob {
 __global__ void k_store_rays(const uint32_t* counters, unsigned long long* out) {
     if (threadIdx.x < 2) out[threadIdx.x] = reinterpret_cast<const unsigned long long*>(counters + 4)[threadIdx.x];
 }
+// the ray counts of an ST_FLAG_ASYNC call: the sum over its wave slots' counters
+struct SlotCounters {
+    const uint32_t* c[kWaveSlots];
+};
+__global__ void k_sum_rays(SlotCounters sc, int slots, unsigned long long* out) {
+    if (threadIdx.x < 2) {
+        unsigned long long v = 0;
+        for (int sl = 0; sl < slots; sl++) v += reinterpret_cast<const unsigned long long*>(sc.c[sl] + 4)[threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+}
 
 // The constants of a camera-path render shared by every wave.
 static RenderConsts camera_consts(const st_scene* s, const st_view& view, const st_render_params* p) {
@@ -3420,14 +3434,28 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
         j.hashgrid = hashgrid;
         j.hg_first = ctx->hg_last ^ 1;
         j.hg_prev_valid = hg_prev_valid;
+        const bool async = (p->flags & ST_FLAG_ASYNC) != 0;
+        j.timed = !async;
         enqueue_camera(ctx, j);
-        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
         if (hashgrid) {
             ctx->hg_last = (j.hg_first + (int)waves - 1) & 1;
             ctx->hg_scene = s->serial;
             ctx->hg_ns = ns;
             ctx->hg_valid = true;
         }
+        if (async) {  // join the wave slots into the context stream and return
+            SlotCounters sc{};
+            for (int sl = 0; sl < slots; sl++) sc.c[sl] = j.counters[sl];
+            for (int sl = 1; sl < slots; sl++) {
+                HIP_CHECK(hipEventRecord(ctx->slot[sl].done, ctx->slot[sl].stream));
+                HIP_CHECK(hipStreamWaitEvent(ctx->stream, ctx->slot[sl].done, 0));
+            }
+            if (o->ray_counts)
+                hipLaunchKernelGGL(k_sum_rays, dim3(1), dim3(64), 0, ctx->stream, sc, slots, (unsigned long long*)o->ray_counts);
+            HIP_CHECK(hipGetLastError());
+            return ST_OK;
+        }
+        for (int sl = 0; sl < slots; sl++) HIP_CHECK(hipStreamSynchronize(ctx->slot[sl].stream));
         finish_timings(ctx, slots, o->ray_counts);
         return ST_OK;
     } catch (const std::exception& e) {

@@ -48,6 +48,10 @@ class Context:
         except Exception:
             pass
 
+    def sync(self):
+        """st_sync: wait for the context stream (ST_FLAG_ASYNC renders)."""
+        abi.check(self.lib, self.lib.st_sync(self.h), "st_sync")
+
     def timings(self):
         out = (C.c_double * 12)()
         abi.check(self.lib, self.lib.st_last_timings(self.h, out), "st_last_timings")

@@ -369,6 +369,34 @@ def test_restir_hashgrid_kept_across_calls(gpu_ctx):
     assert np.array_equal(render(1, 8, renderer=other)[0], fresh8[0])
 
 
+def test_async_render_equals_sync(gpu_ctx):
+    """ST_FLAG_ASYNC: st_render returns once enqueued; after st_sync the
+    outputs of several calls in a row (four wave slots each, one with the
+    screen-space reuse) equal those of synchronous calls bit for bit,
+    ray counts included (summed over the slots on the device)."""
+    import torch
+    from stratum2_amd.renderer import Scene, TestRenderer
+    sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)
+    W, H = 96, 64
+    view = scenes.make_view(W, H, position=(0.0, 3.0, 12.0), orientation=scenes.axis_angle_quat((1, 0, 0), -10.0))
+    r = TestRenderer(Scene(gpu_ctx, sd))
+    base = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_ENV_ALIAS
+    cases = [dict(flags=base, spp=16, seed=1), dict(flags=base | abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE, spp=12, seed=5),
+             dict(flags=base | abi.ST_FLAG_LAMBERT_GGX, spp=9, seed=9)]
+    mk = lambda c, extra=0: abi.default_params(W, H, max_depth=6, spp=c["spp"], seed=c["seed"], flags=c["flags"] | extra,  # noqa
+                                                rr_start_depth=3, samples_per_wave=2)
+    want = []
+    for c in cases:
+        o = r.render(view, mk(c), want_path_length=True)
+        torch.cuda.synchronize()
+        want.append({k: v.cpu().numpy() for k, v in o.items() if v is not None})
+    got = [r.render(view, mk(c, abi.ST_FLAG_ASYNC), want_path_length=True) for c in cases]
+    gpu_ctx.sync()
+    for a, b in zip(want, got):
+        for k, v in a.items():
+            assert np.array_equal(v, b[k].cpu().numpy()), k
+
+
 def _gpu_render(gpu_ctx, sd, view, params):
     import torch
     from stratum2_amd.renderer import Scene, TestRenderer
