@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--samples-per-wave", type=int, default=0, help="0: library default")
     ap.add_argument("--no-restir", action="store_true", help="plain NEE instead of ReSTIR DI")
+    ap.add_argument("--restir-deep", action="store_true",
+                    help="RIS at the NEE vertices past the primary one too (ST_FLAG_RESTIR_DI_DEEP)")
     ap.add_argument("--workload", choices=["spheres", "cloud", "svgf", "batch", "adjoint", "cache"], default="spheres")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="batch workload: the library's default launch shapes")
@@ -146,6 +148,8 @@ def main():
         flags |= abi.ST_FLAG_BDPT_LVC
     if not args.no_restir:  # ReSTIR DI: 32 candidates, temporal + spatial (5 neighbours, 30 px) reuse
         flags |= abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_REUSE
+        if args.restir_deep:
+            flags |= abi.ST_FLAG_RESTIR_DI_DEEP
     # roulette from path length 3 (north star); every vertex for the volume
     # and for the bidirectional mode (whose MIS weights assume it)
     rr = 0 if (cloud or args.bdpt) else 3
@@ -242,8 +246,8 @@ def main():
     else:
         workload = f"{args.width}x{args.height} {args.spp}spp maxPathLength 8"
     if not cloud and not args.no_restir:
-        workload += (", ReSTIR DI at primary vertices (32 candidates, temporal + spatial reuse over 5 neighbours in 30 px), "
-                     "one-sample NEE deeper")
+        workload += (", ReSTIR DI at primary vertices (32 candidates, temporal + spatial reuse over 5 neighbours in 30 px), " +
+                     ("RIS over 32 candidates at deeper vertices" if args.restir_deep else "one-sample NEE deeper"))
     if args.light_trace:
         workload += ", light tracing (1 light subpath per pixel and sample, connected to the camera)"
     if args.bdpt:
@@ -252,7 +256,8 @@ def main():
         "metric": "Mrays/s volume 1024^2" if cloud else "Mrays/s 1080p/1M tris", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": {"workload": workload, "restir_di": not args.no_restir, "light_tracing": args.light_trace,
+        "config": {"workload": workload, "restir_di": not args.no_restir, "restir_deep": args.restir_deep and not args.no_restir,
+                   "light_tracing": args.light_trace,
                    "bdpt": args.bdpt,
                    "width": args.width, "height": args.height, "spp": args.spp, "max_path_length": max_depth,
                    "triangles": sd.triangle_count, "bvh": "device linear BVH" if args.device_bvh else "host binned SAH",

@@ -208,6 +208,11 @@ typedef struct st_view {
  * call (st_last_timings and st_last_kernel_times keep the last synchronous
  * call's). */
 #define ST_FLAG_ASYNC                   (1u << 17)
+/* With ST_FLAG_RESTIR_DI_REUSE: RIS (no reuse) at the NEE vertices past the
+ * primary one too, as the reference runs SampleIlluminationReSTIR at every
+ * NEE vertex (testrenderer.slang:869-884); without it they take one-sample
+ * NEE.  Not with the radiance cache. */
+#define ST_FLAG_RESTIR_DI_DEEP          (1u << 18)
 #define ST_FLAG_DEFAULT (ST_FLAG_SAMPLE_DIRECT)
 
 /* PushConstants of testrenderer.slang:1-21 plus the CLI surface

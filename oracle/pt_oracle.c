@@ -2792,10 +2792,12 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
         connect_to_light(x, &m, &sd, ldin);
         connect_to_light_subpath(x, &m, &sd, NULL, sd.pos, ldin);
     } else if (!m_singular(&m, x->c->lambert) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
-        /* with screen-space reuse, RIS runs at the primary vertex only (its
-         * shadow ray comes after the reuse passes); deeper vertices take plain NEE */
+        /* with screen-space reuse, the primary vertex keeps its RIS reservoir
+         * (its shadow ray comes after the reuse passes); deeper vertices take
+         * RIS with ST_FLAG_RESTIR_DI_DEEP, else plain NEE */
         if (x->reuse && ps->path_len == 1) ris_reservoir(x, &m, &sd, ldin);
-        else if ((x->c->flags & ST_FLAG_RESTIR_DI) && !x->reuse) connect_to_light_restir(x, &m, &sd, ldin);
+        else if ((x->c->flags & ST_FLAG_RESTIR_DI) && (!x->reuse || (x->c->flags & ST_FLAG_RESTIR_DI_DEEP)))
+            connect_to_light_restir(x, &m, &sd, ldin);
         else connect_to_light(x, &m, &sd, ldin);
     }
     if (x->c->flags & ST_FLAG_DIRECT_ONLY) return 0; /* gSampleDirectIlluminationOnly, testrenderer.slang:886-888 */
@@ -3798,6 +3800,7 @@ int or_render_cached(or_scene* s, const st_view* view, const st_render_params* p
     if (p->flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC)) return fail("the radiance cache ends camera paths");
     if ((p->flags & ST_FLAG_RESTIR_DI) && !(p->flags & ST_FLAG_RESTIR_DI_REUSE))
         return fail("with the radiance cache, ReSTIR DI runs at primary vertices (ST_FLAG_RESTIR_DI_REUSE)");
+    if (p->flags & ST_FLAG_RESTIR_DI_DEEP) return fail("with the radiance cache, deeper vertices take one-sample NEE");
     if (p->flags & ST_FLAG_FLOAT_NEE) return fail("fp32 NEE with the radiance cache is not restated");
     if (depth < 1 || depth >= p->max_depth || stride < 1) return fail("cache depth in [1, max_depth), stride >= 1");
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && depth < 2) return fail("with screen-space reuse: cache depth >= 2");

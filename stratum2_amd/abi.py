@@ -37,6 +37,7 @@ ST_FLAG_RESTIR_DI_HASHGRID = 1 << 14  # the reference's hash-grid reservoir reus
 ST_FLAG_HISTORY_RESET = 1 << 15  # drop the hash grid kept from the previous call (a camera cut)
 ST_FLAG_LAMBERT_GGX = 1 << 16  # per material: GGX conductor (metallic >= 0.5) or Lambert
 ST_FLAG_ASYNC = 1 << 17  # st_render returns once enqueued; outputs complete after st_sync
+ST_FLAG_RESTIR_DI_DEEP = 1 << 18  # with the screen-space reuse: RIS at the deeper NEE vertices too
 ST_FLAG_DEFAULT = ST_FLAG_SAMPLE_DIRECT
 
 # TonemapMode (src/Shaders/compat/tonemap.h:10-23)

@@ -1215,8 +1215,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 // made after temporal + spatial reuse
                 const bool reuse = (c.flags & ST_FLAG_RESTIR_DI_REUSE) && path_len == 1;
                 if (!reuse && alive && !m_singular<LAMBERT>(m) && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
-                    // the screen-space reuse mode keeps reservoirs at primary vertices only
-                    const bool ris_here = RESTIR && !(c.flags & ST_FLAG_RESTIR_DI_REUSE);
+                    // the screen-space reuse mode keeps reservoirs at primary vertices;
+                    // deeper vertices run RIS with ST_FLAG_RESTIR_DI_DEEP, else plain NEE
+                    const bool ris_here = RESTIR && (!(c.flags & ST_FLAG_RESTIR_DI_REUSE) || (c.flags & ST_FLAG_RESTIR_DI_DEEP));
                     const HgVertex hgv{&w, pix, path_len, hit.y};
                     const HgVertex* hgp = (RESTIR && (c.flags & ST_FLAG_RESTIR_DI_HASHGRID)) ? &hgv : nullptr;
                     emit_shadow = (ris_here ? next_event<LAMBERT, ALIAS, true>(S, c, rng, m, f, lgn, ldin, sd.pos, sd.gn, thr,
@@ -3303,7 +3304,7 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
                 Timed tm(j.timed ? ctx : nullptr, 1, sm);
                 AuxOut aux{nullptr, nullptr, nullptr, nullptr};
                 if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
-                auto* kfn = cj ? shade_kernel_cache(lambert, alias, media) : shade_kernel(lambert, alias, restir && !reuse, media);
+                auto* kfn = cj ? shade_kernel_cache(lambert, alias, media) : shade_kernel(lambert, alias, restir && (!reuse || (p->flags & ST_FLAG_RESTIR_DI_DEEP)), media);
                 hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0,
                                    AdjArgs{});
             }
@@ -3928,6 +3929,8 @@ int st_render_cached(st_scene* s, st_cache* cache, const st_view* view, const st
         return set_error(ST_ERROR_UNSUPPORTED, "the radiance cache ends camera paths of the path tracer");
     if ((p->flags & ST_FLAG_RESTIR_DI) && !(p->flags & ST_FLAG_RESTIR_DI_REUSE))
         return set_error(ST_ERROR_UNSUPPORTED, "with the radiance cache, ReSTIR DI runs at primary vertices (ST_FLAG_RESTIR_DI_REUSE)");
+    if (p->flags & ST_FLAG_RESTIR_DI_DEEP)
+        return set_error(ST_ERROR_UNSUPPORTED, "with the radiance cache, deeper vertices take one-sample NEE (no ST_FLAG_RESTIR_DI_DEEP)");
     if (cp->cache_depth < 1 || cp->cache_depth >= p->max_depth)
         return set_error(ST_ERROR_INVALID_ARGUMENT, "cache_depth must be in [1, max_depth)");
     if ((p->flags & ST_FLAG_RESTIR_DI_REUSE) && cp->cache_depth < 2)

@@ -447,6 +447,17 @@ def test_headline_frame_modes_vs_oracle(gpu_ctx):
     _check_image(g, r)
 
 
+def test_headline_frame_with_deep_ris_vs_oracle(gpu_ctx):
+    """The headline frame's modes with ST_FLAG_RESTIR_DI_DEEP (RIS at every
+    NEE vertex past the primary one too), 240x136 and 4 spp, against the
+    oracle."""
+    sd = scenes.spheres_field(seed=1)
+    W, H = 240, 136
+    params = abi.default_params(W, H, max_depth=8, spp=4, flags=HEADLINE_FLAGS | abi.ST_FLAG_RESTIR_DI_DEEP, rr_start_depth=3)
+    g, r = _render_both(gpu_ctx, sd, scenes.spheres_view(W, H), params)
+    _check_image(g, r)
+
+
 def test_alias_env_and_emitters_vs_oracle(gpu_ctx):
     """Alias-table environment sampling with emitters, one-sample NEE, path length 5."""
     sd = scenes.spheres_field(seed=4, n_spheres=30, n_emitters=12)

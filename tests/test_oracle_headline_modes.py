@@ -169,6 +169,19 @@ def test_overflowed_direction_pdf_gives_a_finite_image():
     assert (hdr[1035, 1554, :3] > 0).all()
 
 
+def test_deep_ris_with_reuse_keeps_expectation(field):
+    """ST_FLAG_RESTIR_DI_DEEP: with the screen-space reuse at the primary
+    vertex, RIS at the deeper NEE vertices estimates the same image as
+    one-sample NEE there (maxPathLength 4, roulette from length 3); the
+    images differ sample by sample."""
+    osc, view = field
+    kw = dict(max_depth=4, rr_start_depth=3, candidate_samples=32, max_m=3.0, spatial_neighbours=5, spatial_radius=30.0)
+    plain = image_means(osc, view, 8, 8, flags=HEADLINE, **kw)
+    deep = image_means(osc, view, 8, 8, flags=HEADLINE | abi.ST_FLAG_RESTIR_DI_DEEP, **kw)
+    assert same_expectation(plain, deep)
+    assert not np.array_equal(plain, deep)
+
+
 HG = abi.ST_FLAG_RESTIR_DI | abi.ST_FLAG_RESTIR_DI_HASHGRID
 
 
