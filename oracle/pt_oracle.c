@@ -495,6 +495,9 @@ struct or_scene {
     float* tri_alb;
     uint32_t* tri_base;
     uint8_t* volw[8];
+    /* procedural sphere instances (SphereInstanceData, compat/scene.h:60-69) */
+    uint32_t nsph;
+    uint32_t* sph_inst;
 };
 
 static f3 xf_point(const st_transform* t, f3 v) {
@@ -556,7 +559,31 @@ typedef struct {
 /* makeTriangleShadingData, shading_data.hlsli:142-174: from barycentrics, or
  * (lp != NULL) from an object-space point on the triangle, whose barycentrics
  * are solved from the point and whose world position is T * lp. */
+/* makeSphereShadingData, shading_data.hlsli:174-189, at the object-space
+ * point lp: normal T lp normalised, area 4 pi r^2 (object space), the tangent
+ * from the spherical uv taken as angles, as the reference writes it. */
+static void make_sphere_shading(const or_scene* s, uint32_t ii, f3 lp, shading_t* r) {
+    const st_instance* in = &s->inst[ii];
+    const st_transform* T = &s->xf[ii];
+    const f3 n = normalize3(xf_vector(T, lp));
+    r->pos = xf_point(T, lp);
+    r->flags_material = (in->type_material_address >> 4) << 4;
+    r->gn = r->sn = pack_normal(n);
+    const real rad = asfloat(in->data);
+    r->area = 4 * OR_PI * rad * rad;
+    real uv[2];
+    cart_to_uv(normalize3(lp), uv);
+    r->uv[0] = uv[0];
+    r->uv[1] = uv[1];
+    const real su = om_sin(uv[0]), cu = om_cos(uv[0]), sv = om_sin(uv[1]);
+    const f3 dpdu = xf_vector(T, v3(-su * sv, 0.0f, cu * sv));
+    r->tan = pack_normal(normalize3(sub(dpdu, mulf(n, dot(n, dpdu)))));
+}
 static void make_tri_shading_x(const or_scene* s, uint32_t ii, uint32_t prim, real bx, real by, const f3* lp, shading_t* r) {
+    if (prim == 0xFFFFu) { /* a sphere instance, at lp */
+        make_sphere_shading(s, ii, *lp, r);
+        return;
+    }
     const st_instance* in = &s->inst[ii];
     const st_transform* T = &s->xf[ii];
     const st_mesh_vertex_info* vi = &s->vi[in->data & 0xffffu];
@@ -1368,14 +1395,21 @@ static isr_t sample_illumination(const or_scene* s, const cfg_t* c, const real r
     real pdf = 1 / (real)s->nlights;
     if (s->has_env) pdf *= 1 - c->env_prob;
     const st_instance* in = &s->inst[ii];
-    if ((in->type_material_address & 0xfu) != 0) { memset(&r, 0, sizeof r); return r; }
-    uint32_t pc = in->data >> 16;
-    uint32_t prim = (uint32_t)(rnd[3] * (real)pc) % pc;
-    pdf /= (real)pc;
-    real b[2];
-    sample_uniform_triangle(rnd[0], rnd[1], b);
+    const uint32_t type = in->type_material_address & 0xfu;
     shading_t sd;
-    make_tri_shading(s, ii, prim, b[0], b[1], &sd);
+    if (type == 1) { /* sphere: makeSphereShadingData at radius * sampleUniformSphere (lights.hlsli:215-220) */
+        make_sphere_shading(s, ii, mulf(sample_uniform_sphere(rnd[0], rnd[1]), asfloat(in->data)), &sd);
+    } else if (type == 0) {
+        uint32_t pc = in->data >> 16;
+        uint32_t prim = (uint32_t)(rnd[3] * (real)pc) % pc;
+        pdf /= (real)pc;
+        real b[2];
+        sample_uniform_triangle(rnd[0], rnd[1], b);
+        make_tri_shading(s, ii, prim, b[0], b[1], &sd);
+    } else {
+        memset(&r, 0, sizeof r);
+        return r;
+    }
     pdf /= sd.area;
     r.pdf = pdf;
     r.finite = 1;
@@ -1684,7 +1718,54 @@ static int box_hit(const bnode* n, f3 o, f3 inv, real tmax, real* tnear) {
 }
 
 /* closest (any=0) or any hit (any=1) in (0, tmax) */
+/* raySphere (compat/common.h:124-135) about the origin */
+static void ray_sphere(f3 o, f3 d, real r, real* t0, real* t1) {
+    const real a = dot(d, d), b = dot(o, d);
+    const f3 l = sub(mulf(o, a), mulf(d, b));
+    real det = pow2(a * r) - dot(l, l);
+    if (det < 0) { *t0 = *t1 = 0; return; }
+    const real inv_a = 1 / a;
+    det = sqrtr(det * inv_a) * inv_a;
+    const real m = -b * inv_a;
+    *t0 = m - det;
+    *t1 = m + det;
+}
+/* the procedural sphere hits (intersection.hlsli:82-90) after the triangles:
+ * the entry distance, or the exit one from inside, when it is in (0, h.t);
+ * ties go to the smaller id */
+static void trace_spheres(const or_scene* s, f3 o, f3 d, int any, hit_t* h) {
+    for (uint32_t k = 0; k < s->nsph; k++) {
+        const uint32_t ii = s->sph_inst[k];
+        const f3 oo = xf_point(&s->inv_xf[ii], o), od = xf_vector(&s->inv_xf[ii], d);
+        real t0, t1;
+        ray_sphere(oo, od, asfloat(s->inst[ii].data), &t0, &t1);
+        if (!(t0 < t1)) continue;
+        const real t = t0 > 0 ? t0 : t1;
+        const uint32_t id = ii | 0xffff0000u;
+        if (!(t > 0) || !(t < h->t || (t == h->t && h->hit && id < h->id))) continue;
+        h->t = t; h->b1 = h->b2 = 0; h->id = id; h->hit = 1;
+        if (any) return;
+    }
+}
+static hit_t trace_tris(const or_scene* s, f3 o, f3 d, real tmax, int any, uint64_t* nodes, uint64_t* tris);
 static hit_t trace(const or_scene* s, f3 o, f3 d, real tmax, int any, uint64_t* nodes, uint64_t* tris) {
+    hit_t h = trace_tris(s, o, d, tmax, any, nodes, tris);
+    if (s->nsph && !(any && h.hit)) trace_spheres(s, o, d, any, &h);
+    return h;
+}
+/* the shading data of a hit of the ray (o, d): a triangle at its
+ * barycentrics, a sphere at the object-space point o' + d' t
+ * (intersection.hlsli:178-181) */
+static void make_hit_shading(const or_scene* s, const hit_t* h, f3 o, f3 d, shading_t* r) {
+    const uint32_t ii = h->id & 0xffffu, prim = h->id >> 16;
+    if (prim == 0xFFFFu) {
+        const f3 oo = xf_point(&s->inv_xf[ii], o), od = xf_vector(&s->inv_xf[ii], d);
+        make_sphere_shading(s, ii, add(oo, mulf(od, h->t)), r);
+        return;
+    }
+    make_tri_shading(s, ii, prim, h->b1, h->b2, r);
+}
+static hit_t trace_tris(const or_scene* s, f3 o, f3 d, real tmax, int any, uint64_t* nodes, uint64_t* tris) {
     hit_t h;
     h.hit = 0; h.t = tmax; h.b1 = h.b2 = 0; h.id = 0xffffffffu;
     if (s->nnodes == 0) return h;
@@ -1839,6 +1920,7 @@ or_scene* or_scene_create(const st_scene_desc* d) {
     if (s->nvolinfo) memcpy(s->volinfo, d->volume_info, sizeof(st_volume_info) * s->nvolinfo);
     /* lights, scene AABB and flattened triangles */
     s->light_inst = (uint32_t*)malloc(sizeof(uint32_t) * (s->ninst + 1));
+    s->sph_inst = (uint32_t*)malloc(sizeof(uint32_t) * (s->ninst + 1));
     real amin[3] = {INFINITY, INFINITY, INFINITY}, amax[3] = {-INFINITY, -INFINITY, -INFINITY};
     uint64_t ntri = 0;
     for (uint32_t i = 0; i < s->ninst; i++) {
@@ -1869,6 +1951,21 @@ or_scene* or_scene_create(const st_scene_desc* d) {
             }
             amin[0] = minf(amin[0], mn.x); amin[1] = minf(amin[1], mn.y); amin[2] = minf(amin[2], mn.z);
             amax[0] = maxf(amax[0], mx.x); amax[1] = maxf(amax[1], mx.y); amax[2] = maxf(amax[2], mx.z);
+        }
+        if (type == 1) { /* sphere: lights, the transformed corners of [-r, r]^3 */
+            const real rad = asfloat(in->data);
+            if (!(rad > 0) || !isfinite(rad)) { fail("sphere radius must be positive and finite"); or_scene_destroy(s); return NULL; }
+            if ((uint64_t)(in->type_material_address >> 4) / 4 + 9 > s->nmatw) { fail("material address out of range"); or_scene_destroy(s); return NULL; }
+            material_t m = load_material(s, in->type_material_address >> 4);
+            if (any_gt0(v3(un8(m.p[1], 0), un8(m.p[1], 8), un8(m.p[1], 16)))) s->light_inst[s->nlights++] = i;
+            s->sph_inst[s->nsph++] = i;
+            for (int c = 0; c < 8; c++) {
+                f3 corner = v3((c % 2) == 0 ? -rad : rad, ((c % 4) / 2) == 0 ? -rad : rad, (c / 4) == 0 ? -rad : rad);
+                corner = xf_point(&s->xf[i], corner);
+                amin[0] = minf(amin[0], corner.x); amin[1] = minf(amin[1], corner.y); amin[2] = minf(amin[2], corner.z);
+                amax[0] = maxf(amax[0], corner.x); amax[1] = maxf(amax[1], corner.y); amax[2] = maxf(amax[2], corner.z);
+            }
+            continue;
         }
         if (type != 0) continue;
         material_t m = load_material(s, in->type_material_address >> 4);
@@ -1904,6 +2001,7 @@ or_scene* or_scene_create(const st_scene_desc* d) {
             amax[0] = maxf(amax[0], corner.x); amax[1] = maxf(amax[1], corner.y); amax[2] = maxf(amax[2], corner.z);
         }
     }
+    if (s->nsph && s->nvolinst) { fail("sphere instances together with media are not supported"); or_scene_destroy(s); return NULL; }
     /* scene sphere, TestRenderer.cpp:458-461 */
     f3 cen = divf(add(v3(amax[0], amax[1], amax[2]), v3(amin[0], amin[1], amin[2])), 2);
     s->sphere[0] = cen.x; s->sphere[1] = cen.y; s->sphere[2] = cen.z;
@@ -1926,7 +2024,7 @@ void or_scene_destroy(or_scene* s) {
     for (int l = 0; l < 16; l++) free(s->lv[l].rgba);
     free(s->alias_prob); free(s->alias_idx); free(s->alias_pdf);
     free(s->light_inst); free(s->tv); free(s->tid); free(s->nodes); free(s->order);
-    free(s->inv_xf); free(s->volinfo);
+    free(s->inv_xf); free(s->volinfo); free(s->sph_inst);
     for (uint32_t v = 0; v < s->nvols; v++) { free(s->vol[v]); free(s->volw[v]); }
     free(s->tri_alb); free(s->tri_base);
     free(s);
@@ -2710,8 +2808,8 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
     }
     if (h.hit) {
         uint32_t ii = h.id & 0xffffu, prim = h.id >> 16;
-        pick = 1.0f / (real)(s->inst[ii].data >> 16);
-        make_tri_shading(s, ii, prim, h.b1, h.b2, &sd);
+        pick = prim == 0xFFFFu ? 1.0f : 1.0f / (real)(s->inst[ii].data >> 16);
+        make_hit_shading(s, &h, ps->origin, dir, &sd);
         ps->inst_prim = h.id;
         if (s->nvolinst > 0) h.t = length3(sub(sd.pos, ps->origin)); /* intersection.hlsli:244 */
     } else {
@@ -3148,7 +3246,7 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
             ps->origin = mpos;
         } else {
             shading_t sd;
-            make_tri_shading(s, h.id & 0xffffu, h.id >> 16, h.b1, h.b2, &sd);
+            make_hit_shading(s, &h, ps->origin, wd, &sd);
             const material_t m = load_material(s, sd.flags_material >> 4);
             const f3 ldin = to_local(&sd, neg(wd));
             if (vc) { /* testrenderer.slang:812-824 */
@@ -3916,6 +4014,7 @@ uint32_t or_scene_triangle_count(const or_scene* s) { return s ? s->ntris : 0; }
 int or_scene_set_triangle_albedo(or_scene* s, const float* rgb, uint32_t count) {
     if (!s) return fail("null scene");
     if (!rgb) { free(s->tri_alb); s->tri_alb = NULL; return 0; }
+    if (s->nsph) return fail("a per-triangle albedo in a scene with sphere instances");
     if (count != s->ntris) return fail("one albedo per flattened triangle");
     if (!s->tri_alb) s->tri_alb = (float*)malloc(sizeof(float) * 3 * ((size_t)count + 1));
     memcpy(s->tri_alb, rgb, sizeof(float) * 3 * (size_t)count);

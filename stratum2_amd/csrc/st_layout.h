@@ -74,6 +74,12 @@ struct DevScene {
     // instance's first flattened triangle
     const float* tri_alb;
     const uint32_t* tri_base;
+    // procedural sphere instances (SphereInstanceData, compat/scene.h:60-69): a
+    // second 4-wide tree over their world boxes, whose leaf records carry the
+    // instance index in the first record's w
+    const float4* sph_nodes;
+    const float4* sph_tris;
+    uint32_t sph_node_count;
 };
 
 }  // namespace st

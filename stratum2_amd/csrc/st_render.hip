@@ -367,13 +367,13 @@ struct ClosestSrc {
     }
 };
 
-template <bool COUNT>
+template <bool COUNT, bool SPH = false>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_trace_closest(DevScene S, WaveState w, int qsel, uint32_t* spill,
                                                                unsigned long long* stats) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     ClosestSrc src{w, w.queue[qsel]};
     TraverseStats ts{0, 0};
-    trace_queue<false, COUNT>(S, w.counters[qsel], &w.counters[kFetchClosest], src, lds, spill, ts);
+    trace_queue<false, COUNT, ClosestSrc, SPH>(S, w.counters[qsel], &w.counters[kFetchClosest], src, lds, spill, ts);
     if (COUNT) {
         atomicAdd(&stats[0], (unsigned long long)ts.nodes);
         atomicAdd(&stats[1], (unsigned long long)ts.tris);
@@ -439,17 +439,32 @@ struct ShadowSrc {
     }
 };
 
-template <bool COUNT>
+template <bool COUNT, bool SPH = false>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_trace_shadow(DevScene S, WaveState w, uint32_t flags, uint32_t* spill,
                                                               unsigned long long* stats) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     ShadowSrc src{w, flags};
     TraverseStats ts{0, 0};
-    trace_queue<true, COUNT>(S, w.counters[2], &w.counters[kFetchShadow], src, lds, spill, ts);
+    trace_queue<true, COUNT, ShadowSrc, SPH>(S, w.counters[2], &w.counters[kFetchShadow], src, lds, spill, ts);
     if (COUNT) {
         atomicAdd(&stats[2], (unsigned long long)ts.nodes);
         atomicAdd(&stats[3], (unsigned long long)ts.tris);
     }
+}
+
+// The trace kernels of a scene: with sphere instances, the lanes walk the
+// spheres' tree after the triangles'.
+using ClosestFn = void (*)(DevScene, WaveState, int, uint32_t*, unsigned long long*);
+using ShadowFn = void (*)(DevScene, WaveState, uint32_t, uint32_t*, unsigned long long*);
+static ClosestFn closest_kernel(bool count, const DevScene& S) {
+    const bool sph = S.sph_node_count != 0;
+    return count ? (sph ? k_trace_closest<true, true> : k_trace_closest<true, false>)
+                 : (sph ? k_trace_closest<false, true> : k_trace_closest<false, false>);
+}
+static ShadowFn shadow_kernel(bool count, const DevScene& S) {
+    const bool sph = S.sph_node_count != 0;
+    return count ? (sph ? k_trace_shadow<true, true> : k_trace_shadow<true, false>)
+                 : (sph ? k_trace_shadow<false, true> : k_trace_shadow<false, false>);
 }
 
 // Sum of a per-lane count over the wavefront, added once per wave.
@@ -950,7 +965,8 @@ __device__ __forceinline__ float seed_target(const DevScene& S, const RenderCons
 // k_shade: it keeps each pixel's reservoir and primary record, and advances
 // the path's random counter past the candidates, where k_shade continues the
 // path (emission, RR and BSDF sampling draw no numbers before NEE).
-template <bool LAMBERT, bool ALIAS>
+// SPH: the scene has sphere instances (their shading inlined only then)
+template <bool LAMBERT, bool ALIAS, bool SPH = false>
 __global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c, WaveState w, int qsel) {
     const uint32_t n = w.counters[qsel];
     const uint32_t* q = w.queue[qsel];
@@ -969,7 +985,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c,
         uint4 s4 = w.st[p];
         Rng rng{px, py, c.seed + c.sample_base + k, s4.x};
         Shading sd;
-        tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
+        if (SPH) hit_shading(S, hit, mk(r.x, r.y, r.z), unpack_normal(__float_as_uint(r.w)), sd);
+        else tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
         const Material m = load_material(S, sd.material, c.flags);
         const Frame f = make_frame(sd);
         const v3 ldin = f.to_local(-unpack_normal(__float_as_uint(r.w)));
@@ -1166,8 +1183,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
             } else {
                 const uint32_t ii = hit.y & 0xffffu, prim = hit.y >> 16;
                 Shading sd;
-                tri_shading(S, ii, prim, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
-                const float pick = 1.0f / (float)(S.inst[ii].data >> 16);
+                hit_shading(S, hit, mk(r.x, r.y, r.z), dir, sd);
+                const float pick = prim_pick(S, hit.y);
                 float pdfa = 0;  // LightSamplePdfA, lights.hlsli:264-271
                 if (S.light_count > 0) {
                     pdfa = pick / (sd.area * (float)S.light_count);
@@ -1597,7 +1614,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
                     }
                 } else {
                     Shading sd;
-                    tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
+                    hit_shading(S, hit, mk(r.x, r.y, r.z), dir, sd);
                     const Material m = load_material(S, sd.material, c.flags);
                     const Frame f = make_frame(sd);
                     const v3 ldin = f.to_local(-dir);
@@ -1749,8 +1766,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_vc(DevScene S, RenderCons
             } else {
                 const uint32_t ii = hit.y & 0xffffu, prim = hit.y >> 16;
                 Shading sd;
-                tri_shading(S, ii, prim, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
-                const float pick = 1.0f / (float)(S.inst[ii].data >> 16);
+                hit_shading(S, hit, mk(r.x, r.y, r.z), dir, sd);
+                const float pick = prim_pick(S, hit.y);
                 float pdfa = 0;  // LightSamplePdfA, lights.hlsli:264-271
                 if (S.light_count > 0) {
                     pdfa = pick / (sd.area * (float)S.light_count);
@@ -2023,13 +2040,14 @@ struct RaySrc {
     }
 };
 
+template <bool SPH>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int closest,
                                                             uint32_t* out, uint32_t* spill, uint32_t* fetch) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     RaySrc src{rays, out, closest};
     TraverseStats ts{0, 0};
-    if (closest) trace_queue<false, false>(S, n, fetch, src, lds, spill, ts);
-    else trace_queue<true, false>(S, n, fetch, src, lds, spill, ts);
+    if (closest) trace_queue<false, false, RaySrc, SPH>(S, n, fetch, src, lds, spill, ts);
+    else trace_queue<true, false, RaySrc, SPH>(S, n, fetch, src, lds, spill, ts);
 }
 
 // World-space triangle records (v0.xyz|id, v1, v2) of every mesh instance
@@ -2064,14 +2082,16 @@ __global__ void k_light_table(DevScene S, const uint32_t* tri_base, float4* rec,
     if (li >= S.light_count) return;
     const uint32_t ii = S.light_inst[li];
     const st_instance in = S.inst[ii];
-    const uint32_t pc = in.data >> 16, base = tri_base[li];
+    const bool sphere = (in.type_material_address & 0xfu) == 1;
+    const uint32_t pc = sphere ? 0u : in.data >> 16, base = tri_base[li];
     if (threadIdx.x == 0) {
         const st_transform& T = S.xf[ii];
         for (int r = 0; r < 3; r++) rec[4 * (size_t)li + r] = make_float4(T.m[r][0], T.m[r][1], T.m[r][2], T.m[r][3]);
         const uint32_t addr = in.type_material_address >> 4;
-        rec[4 * (size_t)li + 3] = make_float4(__uint_as_float(pc), __uint_as_float(base), __uint_as_float(S.mat[addr / 4 + 1]),
-                                              __uint_as_float(S.mat[addr / 4 + 8]));
+        rec[4 * (size_t)li + 3] = make_float4(__uint_as_float(pc), sphere ? __uint_as_float(in.data) : __uint_as_float(base),
+                                              __uint_as_float(S.mat[addr / 4 + 1]), __uint_as_float(S.mat[addr / 4 + 8]));
     }
+    if (sphere) return;
     const st_mesh_vertex_info vi = S.vi[in.data & 0xffffu];
     const uint32_t pb = vi.packed_buffer_indices[0] >> 16, ps = (vi.packed_strides >> 8) & 0xffu;
     const uint32_t nb = vi.packed_buffer_indices[1] & 0xffffu, ns = (vi.packed_strides >> 16) & 0xffu;
@@ -2263,6 +2283,49 @@ struct st_scene {
     const uint32_t* light_tri_base = nullptr;
     float4* light_rec = nullptr;
     float4* light_tri = nullptr;
+    // sphere instances (index, radius) and their tree, rebuilt when the instances move
+    std::vector<uint32_t> sph_inst;
+    std::vector<float> sph_rad;
+    DevBuf sph_nodes, sph_tris;
+    // The tree over the sphere instances' world boxes (the transformed corners
+    // of [-r, r]^3, widened by 2^-20 of their magnitude): the triangle builder
+    // over records whose first two vertices are the box corners.
+    void build_sphere_tree(const st_transform* xf) {
+        const uint32_t n = (uint32_t)sph_inst.size();
+        d.sph_node_count = 0;
+        if (n == 0) return;
+        std::vector<float> verts(9 * (size_t)n);
+        std::vector<uint32_t> ids(n);
+        for (uint32_t k = 0; k < n; k++) {
+            const st_transform& T = xf[sph_inst[k]];
+            const float r = sph_rad[k];
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int c = 0; c < 8; c++) {
+                const float v[3] = {(c & 1) ? r : -r, (c & 2) ? r : -r, (c & 4) ? r : -r};
+                for (int a = 0; a < 3; a++) {
+                    const float w = T.m[a][0] * v[0] + T.m[a][1] * v[1] + T.m[a][2] * v[2] + T.m[a][3];
+                    lo[a] = std::min(lo[a], w);
+                    hi[a] = std::max(hi[a], w);
+                }
+            }
+            for (int a = 0; a < 3; a++) {
+                const float m = std::max(std::fabs(lo[a]), std::fabs(hi[a])) * 0x1p-20f + 1e-30f;
+                verts[9 * (size_t)k + a] = lo[a] - m;
+                verts[9 * (size_t)k + 3 + a] = hi[a] + m;
+                verts[9 * (size_t)k + 6 + a] = lo[a] - m;
+            }
+            ids[k] = sph_inst[k];
+        }
+        BvhOut b = build_bvh(verts.data(), ids.data(), n);
+        if (3 * b.depth4 + 1 > kMaxStack) throw std::runtime_error("sphere tree deeper than the traversal stack");
+        sph_nodes.reserve(b.nodes4.size() * sizeof(Node4Out));
+        sph_tris.reserve(b.tris4.size() * sizeof(TriOut));
+        HIP_CHECK(hipMemcpy(sph_nodes.p, b.nodes4.data(), b.nodes4.size() * sizeof(Node4Out), hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(sph_tris.p, b.tris4.data(), b.tris4.size() * sizeof(TriOut), hipMemcpyHostToDevice));
+        d.sph_nodes = sph_nodes.as<float4>();
+        d.sph_tris = sph_tris.as<float4>();
+        d.sph_node_count = (uint32_t)b.nodes4.size();
+    }
     ~st_scene() {
         for (auto* b : bufs) delete b;
         if (bvh_nodes) (void)hipFree(bvh_nodes);
@@ -2342,7 +2405,7 @@ void validate_desc(const st_scene_desc* d) {
         if (ii >= d->instance_count || (d->instances[ii].type_material_address & 0xfu) != 2)
             throw std::invalid_argument("volume info must name a volume instance");
     }
-    uint32_t nvol = 0;
+    uint32_t nvol = 0, nsph = 0;
     for (uint32_t i = 0; i < d->instance_count; i++) {
         const st_instance& in = d->instances[i];
         const uint32_t type = in.type_material_address & 0xfu;
@@ -2356,13 +2419,22 @@ void validate_desc(const st_scene_desc* d) {
             if (d->material_data[addr / 4 + 5] != 0xFFFFFFFFu) throw std::domain_error("albedo volumes are not supported");
             continue;
         }
-        if (type != 0) throw std::domain_error("only mesh and volume instances are supported");
+        if (type == 1) {  // SphereInstanceData (compat/scene.h:60-69): the radius in data
+            float r;
+            std::memcpy(&r, &in.data, 4);
+            if (!(r > 0) || !std::isfinite(r)) throw std::invalid_argument("sphere radius must be positive and finite");
+        } else if (type != 0) {
+            throw std::domain_error("only mesh, sphere and volume instances are supported");
+        }
         const uint32_t addr = in.type_material_address >> 4;
         if ((uint64_t)addr / 4 + 16 > d->material_word_count) throw std::invalid_argument("material address out of range");
         for (int k = 0; k < 4; k++)
             if (d->material_data[addr / 4 + 4 + k] < ST_IMAGE_LIMIT) throw std::domain_error("material textures are not supported");
-        if ((in.data & 0xffffu) >= d->mesh_vertex_info_count) throw std::invalid_argument("vertex info index out of range");
+        if (type == 0 && (in.data & 0xffffu) >= d->mesh_vertex_info_count) throw std::invalid_argument("vertex info index out of range");
+        if (type == 1) nsph++;
     }
+    // a medium's scattering vertices take the primitive id 0xffff that marks sphere hits
+    if (nsph && nvol) throw std::domain_error("sphere instances together with media are not supported");
 }
 
 // Scene AABB (Scene.cpp:622-631 mesh instances, :672-711 volumes, where the
@@ -2714,6 +2786,20 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
             const uint32_t addr = in.type_material_address >> 4;
             const uint32_t ew = d->material_data[addr / 4 + 1];
             if ((ew & 0xffu) || ((ew >> 8) & 0xffu) || ((ew >> 16) & 0xffu)) lights.push_back(i);  // Scene.cpp:487-491
+            if ((in.type_material_address & 0xfu) == 1) {
+                // sphere: the scene box takes the transformed corners of [-r, r]^3
+                // (the reference's center +- r, its sphere transforms being translations)
+                float r;
+                std::memcpy(&r, &in.data, 4);
+                s->sph_inst.push_back(i);
+                s->sph_rad.push_back(r);
+                ib0.kind = 1;
+                for (int k = 0; k < 3; k++) {
+                    ib0.lo[k] = -r;
+                    ib0.hi[k] = r;
+                }
+                continue;
+            }
             const st_mesh_vertex_info& vi = d->mesh_vertex_info[in.data & 0xffffu];
             const uint32_t pc = in.data >> 16;
             const uint32_t ib = vi.packed_buffer_indices[0] & 0xffffu, pb = vi.packed_buffer_indices[0] >> 16;
@@ -2756,7 +2842,7 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
             uint64_t total = 0;
             for (size_t li = 0; li < lights.size(); li++) {
                 base[li] = (uint32_t)total;
-                total += d->instances[lights[li]].data >> 16;
+                if ((d->instances[lights[li]].type_material_address & 0xfu) == 0) total += d->instances[lights[li]].data >> 16;
             }
             if (total >= (1ull << 31)) throw std::domain_error("too many emissive triangles");
             s->light_tri_base = s->upload<uint32_t>(base.data(), 4 * base.size());
@@ -2766,6 +2852,7 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
             D.light_tri = s->light_tri;
             s->build_light_table(ctx->stream);
         }
+        s->build_sphere_tree(d->transforms);
         BvhOut bvh = build_bvh(verts.data(), ids.data(), (uint32_t)ntri);
         D.nodes = s->upload<float4>(bvh.nodes4.data(), bvh.nodes4.size() * sizeof(Node4Out));
         D.tris = s->upload<float4>(bvh.tris4.data(), bvh.tris4.size() * sizeof(TriOut));
@@ -2899,6 +2986,7 @@ int st_scene_update(st_scene* s, const st_transform* transforms, double* build_m
         HIP_CHECK(hipMemcpy(const_cast<st_transform*>(D.inv_xf), s->inv_xf.data(), sizeof(st_transform) * ni, hipMemcpyHostToDevice));
         scene_sphere(s->boxes, transforms, D.sphere);
         s->build_light_table(ctx->stream);  // the emitters' transforms moved too
+        s->build_sphere_tree(transforms);
         double ms = 0;
         if (s->ntri > 0) {
             // flatten on the device and rebuild the BVH there
@@ -3288,16 +3376,19 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
             {
                 Timed tm(j.timed ? ctx : nullptr, 0, sm);
                 if (media) hipLaunchKernelGGL(k_trace_media<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats, AdjArgs{});
-                else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
-                else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                else if (count) hipLaunchKernelGGL(closest_kernel(true, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                else hipLaunchKernelGGL(closest_kernel(false, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
             }
             // with screen-space reuse, k_ris runs the primary vertices' RIS
             // first and every vertex after takes plain NEE: the shading
             // kernel without the candidate loop
             if (reuse && b == 0) {
                 Timed tm(j.timed ? ctx : nullptr, 3, sm);
-                auto kr = lambert ? (alias ? k_ris<true, true> : k_ris<true, false>)
-                                  : (alias ? k_ris<false, true> : k_ris<false, false>);
+                const bool sph = s->d.sph_node_count != 0;
+                auto kr = sph ? (lambert ? (alias ? k_ris<true, true, true> : k_ris<true, false, true>)
+                                         : (alias ? k_ris<false, true, true> : k_ris<false, false, true>))
+                              : (lambert ? (alias ? k_ris<true, true> : k_ris<true, false>)
+                                         : (alias ? k_ris<false, true> : k_ris<false, false>));
                 hipLaunchKernelGGL(kr, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel);
             }
             {
@@ -3333,8 +3424,8 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
             {
                 Timed tm(j.timed ? ctx : nullptr, 2, sm);
                 if (media) hipLaunchKernelGGL(k_shadow_media<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats, AdjArgs{});
-                else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
-                else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
+                else if (count) hipLaunchKernelGGL(shadow_kernel(true, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
+                else hipLaunchKernelGGL(shadow_kernel(false, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
             }
             hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
             qsel ^= 1;
@@ -3467,6 +3558,7 @@ int st_render(st_scene* s, const st_view* view, const st_render_params* p, const
 // ------------------------------------------------------------ differentiable rendering
 int st_scene_set_triangle_albedo(st_scene* s, const float* rgb, uint32_t count) {
     if (!s) return set_error(ST_ERROR_INVALID_ARGUMENT, "null scene");
+    if (rgb && !s->sph_inst.empty()) return set_error(ST_ERROR_UNSUPPORTED, "a per-triangle albedo in a scene with sphere instances");
     try {
         HIP_CHECK(hipSetDevice(s->ctx->device));
         if (!rgb) {
@@ -3592,7 +3684,7 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
                             hipLaunchKernelGGL(replay ? k_trace_media<true> : k_trace_media<false>, dim3(tgrid), dim3(kTraceBlock),
                                                0, sm, s->d, c, w, qsel, spill, stats, adj);
                         else
-                            hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill,
+                            hipLaunchKernelGGL(closest_kernel(false, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill,
                                                stats);
                     }
                     {
@@ -3606,7 +3698,7 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
                             hipLaunchKernelGGL(replay ? k_shadow_media<true> : k_shadow_media<false>, dim3(tgrid),
                                                dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats, adj);
                         else
-                            hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags,
+                            hipLaunchKernelGGL(shadow_kernel(false, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags,
                                                spill, stats);
                     }
                     hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
@@ -4404,8 +4496,8 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
                 {
                     Timed tm(ctx, 0, sm);
                     if (media) hipLaunchKernelGGL(k_trace_media<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, qsel, spill, stats, AdjArgs{});
-                    else if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
-                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                    else if (count) hipLaunchKernelGGL(closest_kernel(true, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                    else hipLaunchKernelGGL(closest_kernel(false, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                 }
                 {
                     Timed tm(ctx, 1, sm);
@@ -4416,8 +4508,8 @@ static int render_light(st_scene* s, const st_view* view, const st_render_params
                 {
                     Timed tm(ctx, 2, sm);
                     if (media) hipLaunchKernelGGL(k_shadow_media<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, c, w, spill, stats, AdjArgs{});
-                    else if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
-                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
+                    else if (count) hipLaunchKernelGGL(shadow_kernel(true, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
+                    else hipLaunchKernelGGL(shadow_kernel(false, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, c.flags, spill, stats);
                 }
                 hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, media ? 1 : 0);
                 qsel ^= 1;
@@ -4521,8 +4613,8 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
             for (uint32_t b = 0; b < depth; b++) {
                 {
                     Timed tm(ctx, 0, sm);
-                    if (count) hipLaunchKernelGGL(k_trace_closest<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
-                    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                    if (count) hipLaunchKernelGGL(closest_kernel(true, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
+                    else hipLaunchKernelGGL(closest_kernel(false, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, qsel, spill, stats);
                 }
                 {
                     Timed tm(ctx, 1, sm);
@@ -4537,8 +4629,8 @@ static int render_vc(st_scene* s, const st_view* view, const st_render_params* p
                 }
                 {
                     Timed tm(ctx, 2, sm);
-                    if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, cc.flags, spill, stats);
-                    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, cc.flags, spill, stats);
+                    if (count) hipLaunchKernelGGL(shadow_kernel(true, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, cc.flags, spill, stats);
+                    else hipLaunchKernelGGL(shadow_kernel(false, s->d), dim3(tgrid), dim3(kTraceBlock), 0, sm, s->d, w, cc.flags, spill, stats);
                 }
                 hipLaunchKernelGGL(k_count, dim3(1), dim3(1), 0, sm, w, qsel, 0);
                 qsel ^= 1;
@@ -4617,7 +4709,7 @@ int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, u
         ensure_spill(B, ctx->cus * kTraceBlocksPerCU, kTraceBlock);
         B.counters.reserve(4 * kCounterWords);
         HIP_CHECK(hipMemsetAsync(B.counters.as<uint32_t>() + kFetchRays, 0, 4 * kXcds * kFetchStride, ctx->stream));
-        hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
+        hipLaunchKernelGGL(s->d.sph_node_count ? k_trace_rays<true> : k_trace_rays<false>, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
                            closest, hits, B.spill.as<uint32_t>(), B.counters.as<uint32_t>() + kFetchRays);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -4649,7 +4741,7 @@ int st_trace_rays_noalloc(st_scene* s, const float* rays, uint32_t count, int cl
         uint32_t* spill = reinterpret_cast<uint32_t*>(ctx->scratch + kScratchHeader);
         // the fetch counter starts at zero (a fill kernel: also right inside a captured graph)
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, fetch, kXcds * kFetchStride, 0u);
-        hipLaunchKernelGGL(k_trace_rays, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
+        hipLaunchKernelGGL(s->d.sph_node_count ? k_trace_rays<true> : k_trace_rays<false>, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
                            closest, hits, spill, fetch);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));

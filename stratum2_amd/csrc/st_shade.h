@@ -53,12 +53,38 @@ __device__ __forceinline__ void load_tri(const DevScene& S, const st_mesh_vertex
     }
 }
 
+// makeSphereShadingData, shading_data.hlsli:174-189: the sphere instance ii
+// at the object-space point lp.  The area is the object-space 4 pi r^2 (the
+// reference's scene build leaves sphere transforms a translation), and the
+// tangent comes from the spherical uv taken as angles, as the reference
+// writes it.
+__device__ __attribute__((noinline)) void sphere_shading(const DevScene& S, uint32_t ii, v3 lp, Shading& r) {
+    const st_instance in = S.inst[ii];
+    const st_transform T = S.xf[ii];
+    const v3 n = normalize(xf_vector(T, lp));
+    r.pos = xf_point(T, lp);
+    r.material = in.type_material_address >> 4;
+    r.gn = r.sn = pack_normal(n);
+    const float rad = __uint_as_float(in.data);
+    r.area = 4 * ST_PI * rad * rad;
+    float u, v;
+    cart_to_uv(normalize(lp), u, v);
+    const float su = sin_(u), cu = cos_(u), sv = sin_(v);
+    const v3 dpdu = xf_vector(T, mk(-su * sv, 0.0f, cu * sv));
+    r.tan = pack_normal(normalize(dpdu - n * dot(n, dpdu)));
+}
+
 // makeTriangleShadingData(WithoutPosition), shading_data.hlsli:59-151
 // lp != nullptr: the shading data at an object-space point on the triangle
 // (makeTriangleShadingData from a local position, shading_data.hlsli:153-174):
-// barycentrics solved from the point, world position T * lp.
+// barycentrics solved from the point, world position T * lp.  A sphere
+// instance (prim 0xffff) takes its shading data at lp.
 __device__ inline void tri_shading(const DevScene& S, uint32_t ii, uint32_t prim, float bx, float by, Shading& r,
                                    const v3* lp = nullptr) {
+    if (prim == 0xffffu) {
+        sphere_shading(S, ii, *lp, r);
+        return;
+    }
     const st_instance in = S.inst[ii];
     const st_transform T = S.xf[ii];
     const st_mesh_vertex_info vi = S.vi[in.data & 0xffffu];
@@ -121,6 +147,29 @@ __device__ inline void tri_shading(const DevScene& S, uint32_t ii, uint32_t prim
         if (dot(sn, gn) < 0) r.gn = pack_normal(-gn);
     }
     r.pos = lp ? xf_point(T, *lp) : xf_point(T, (v0 + (v1 - v0) * bx) + (v2 - v0) * by);
+}
+
+// The shading data of a traversal hit {t, id, b1, b2} of the ray (o, d): a
+// triangle at its barycentrics, or a sphere at the object-space point o' + d' t
+// (CommittedObjectRayOrigin + CommittedObjectRayDirection * t,
+// intersection.hlsli:178-181).  Its primitive pick pdf is 1 for a sphere.
+__device__ inline void hit_shading(const DevScene& S, uint4 hit, v3 o, v3 d, Shading& sd) {
+    const uint32_t ii = hit.y & 0xffffu, prim = hit.y >> 16;
+    if (prim == 0xffffu) {
+        const st_transform& T = S.inv_xf[ii];
+        const float t = __uint_as_float(hit.x);
+        const v3 oo = mk(T.m[0][0] * o.x + T.m[0][1] * o.y + T.m[0][2] * o.z + T.m[0][3],
+                         T.m[1][0] * o.x + T.m[1][1] * o.y + T.m[1][2] * o.z + T.m[1][3],
+                         T.m[2][0] * o.x + T.m[2][1] * o.y + T.m[2][2] * o.z + T.m[2][3]);
+        const v3 od = mk(T.m[0][0] * d.x + T.m[0][1] * d.y + T.m[0][2] * d.z, T.m[1][0] * d.x + T.m[1][1] * d.y + T.m[1][2] * d.z,
+                         T.m[2][0] * d.x + T.m[2][1] * d.y + T.m[2][2] * d.z);
+        sphere_shading(S, ii, oo + od * t, sd);
+        return;
+    }
+    tri_shading(S, ii, prim, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
+}
+__device__ __forceinline__ float prim_pick(const DevScene& S, uint32_t id) {
+    return (id >> 16) == 0xffffu ? 1.0f : 1.0f / (float)(S.inst[id & 0xffffu].data >> 16);
 }
 
 // ShadingData::toLocal / toWorld / shadingNormalCorrection, shading_data.hlsli:24-54
@@ -352,6 +401,19 @@ __device__ inline void light_point(const DevScene& S, uint32_t li, uint32_t prim
     pos = xf_point(T, (v0 + (v1 - v0) * bx) + (v2 - v0) * by);
 }
 
+// A point on an emissive sphere instance (SampleEmission's sphere branch,
+// lights.hlsli:215-220): makeSphereShadingData at radius * sampleUniformSphere.
+__device__ inline void light_sphere(const DevScene& S, uint32_t li, float rad, float u1, float u2, v3& pos, uint32_t& gnp,
+                                    float& area) {
+    const float4* L = S.light_rec + 4 * (size_t)li;
+    const float4 r0 = L[0], r1 = L[1], r2 = L[2];
+    const st_transform T = {{{r0.x, r0.y, r0.z, r0.w}, {r1.x, r1.y, r1.z, r1.w}, {r2.x, r2.y, r2.z, r2.w}}};
+    const v3 lp = sample_uniform_sphere(u1, u2) * rad;
+    gnp = pack_normal(normalize(xf_vector(T, lp)));
+    pos = xf_point(T, lp);
+    area = 4 * ST_PI * rad * rad;
+}
+
 struct LightSample {  // IlluminationSampleRecord, lights.hlsli:32-45
     v3 radiance, dir;
     float pdf, dist, cos_light;
@@ -417,15 +479,20 @@ __device__ inline LightSample sample_illumination(const DevScene& S, float env_p
     const uint32_t li = (uint32_t)(rnd.z * (float)S.light_count) % S.light_count;
     float pdf = 1 / (float)S.light_count;
     if (S.has_env) pdf *= 1 - env_prob;
-    const float4 l3 = S.light_rec[4 * (size_t)li + 3];  // {triangles, first, emission word, scale}
+    // {triangles, first, emission word, scale}; a sphere instance: {0, radius, ...}
+    const float4 l3 = S.light_rec[4 * (size_t)li + 3];
     const uint32_t pc = __float_as_uint(l3.x);
-    const uint32_t prim = (uint32_t)(rnd.w * (float)pc) % pc;
-    pdf /= (float)pc;
-    const float a = sqrtf(rnd.x);  // sampleUniformTriangle, common.hlsli:36-39
     v3 lpos;
     uint32_t lgn;
     float larea;
-    light_point(S, li, prim, 1 - a, a * rnd.y, lpos, lgn, larea);
+    if (pc == 0) {
+        light_sphere(S, li, l3.y, rnd.x, rnd.y, lpos, lgn, larea);
+    } else {
+        const uint32_t prim = (uint32_t)(rnd.w * (float)pc) % pc;
+        pdf /= (float)pc;
+        const float a = sqrtf(rnd.x);  // sampleUniformTriangle, common.hlsli:36-39
+        light_point(S, li, prim, 1 - a, a * rnd.y, lpos, lgn, larea);
+    }
     pdf /= larea;
     const uint32_t ew = __float_as_uint(l3.z);  // Material::emission, as load_material's record holds it
     r.radiance = mk(un8(ew, 0), un8(ew, 8), un8(ew, 16)) * l3.w;

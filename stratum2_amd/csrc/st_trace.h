@@ -96,18 +96,46 @@ struct TraverseStats {
     uint32_t nodes, tris;
 };
 
+// raySphere (compat/common.h:124-135) about the origin: the entry and exit
+// parameters of o + t d, or (0, 0) when the ray misses
+__device__ __forceinline__ void ray_sphere(v3 o, v3 d, float r, float& t0, float& t1) {
+    const float a = dot(d, d), b = dot(o, d);
+    const v3 l = o * a - d * b;
+    float det = sq(a * r) - dot(l, l);
+    if (det < 0) {
+        t0 = t1 = 0;
+        return;
+    }
+    const float inv_a = 1 / a;
+    det = sqrtf(det * inv_a) * inv_a;
+    const float m = -b * inv_a;
+    t0 = m - det;
+    t1 = m + det;
+}
+// The object-space ray of instance ii (CandidateObjectRayOrigin/Direction)
+__device__ __forceinline__ void object_ray(const DevScene& S, uint32_t ii, v3 o, v3 d, v3& oo, v3& od) {
+    const st_transform& T = S.inv_xf[ii];
+    oo = mk(T.m[0][0] * o.x + T.m[0][1] * o.y + T.m[0][2] * o.z + T.m[0][3],
+            T.m[1][0] * o.x + T.m[1][1] * o.y + T.m[1][2] * o.z + T.m[1][3],
+            T.m[2][0] * o.x + T.m[2][1] * o.y + T.m[2][2] * o.z + T.m[2][3]);
+    od = mk(T.m[0][0] * d.x + T.m[0][1] * d.y + T.m[0][2] * d.z, T.m[1][0] * d.x + T.m[1][1] * d.y + T.m[1][2] * d.z,
+            T.m[2][0] * d.x + T.m[2][1] * d.y + T.m[2][2] * d.z);
+}
+
 // Traversal state of one lane.  step() visits one 4-wide node: its leaf
 // children are intersected at once, the nearest hit inner child is next and
 // the others are pushed.  Returns false when the ray is finished.
 struct Lane {
     WRay wr;
     v3 inv;
+    v3 dir;  // read by the sphere tests only
     Hit h;
     uint32_t node;
     int sp;
 
     __device__ __forceinline__ void start(v3 o, v3 d, float tmax) {
         wr = wray_make(o, d);
+        dir = d;
         inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
         h.hit = false;
         h.t = tmax;
@@ -134,6 +162,26 @@ struct Lane {
         return false;
     }
 
+    // a sphere instance's procedural hit (intersection.hlsli:82-90): the entry
+    // distance, or the exit one from inside, when it is in (0, t); ties go to
+    // the smaller id as for triangles
+    __device__ __forceinline__ bool sph_test(const DevScene& S, uint32_t k) {
+        const uint32_t ii = __float_as_uint(S.sph_tris[3 * (size_t)k].w);
+        v3 oo, od;
+        object_ray(S, ii, wr.o, dir, oo, od);
+        float t0, t1;
+        ray_sphere(oo, od, __uint_as_float(S.inst[ii].data), t0, t1);
+        if (!(t0 < t1)) return false;
+        const float t = t0 > 0 ? t0 : t1;
+        const uint32_t id = ii | 0xffff0000u;
+        if (!(t > 0) || !(t < h.t || (t == h.t && h.id != 0xffffffffu && id < h.id))) return false;
+        h.t = t;
+        h.b1 = h.b2 = 0;
+        h.id = id;
+        h.hit = true;
+        return true;
+    }
+
     __device__ __forceinline__ void push(uint32_t v, uint32_t* lds, uint32_t* spill, uint32_t stride, uint32_t gtid) {
         if (sp < kLdsStack) lds[sp * (int)blockDim.x + (int)threadIdx.x] = v;
         else spill[(size_t)(sp - kLdsStack) * stride + gtid] = v;
@@ -143,10 +191,11 @@ struct Lane {
     // One 4-wide node: decode four quantised child boxes (o + q * 2^e, exact
     // in float), slab-test them, intersect hit leaf children at once, continue
     // with the nearest hit inner child and push the others farthest first.
-    template <bool ANY, bool COUNT>
+    // SPH: a step in the sphere instances' tree (leaf records are spheres)
+    template <bool ANY, bool COUNT, bool SPH = false>
     __device__ __forceinline__ bool step(const DevScene& S, uint32_t* lds, uint32_t* spill, uint32_t spill_stride,
                                          uint32_t gtid, TraverseStats& st) {
-        const float4* np = S.nodes + 3 * (size_t)node;
+        const float4* np = (SPH ? S.sph_nodes : S.nodes) + 3 * (size_t)node;
         const float4 w0 = np[0], w1 = np[1], w2 = np[2];
         if (COUNT) st.nodes++;
         const uint32_t x0 = __float_as_uint(w0.w), wi = __float_as_uint(w2.z);
@@ -186,7 +235,7 @@ struct Lane {
             const uint32_t j = (uint32_t)__builtin_ctz(trimask);
             trimask &= trimask - 1u;
             if (COUNT) st.tris++;
-            if (tri_test(S, base_tri + j) && ANY) return false;
+            if ((SPH ? sph_test(S, base_tri + j) : tri_test(S, base_tri + j)) && ANY) return false;
         }
         // sort the (at most 4) inner candidates by entry distance
 #define ST_CSWAP(i, j)                                   \
@@ -220,8 +269,14 @@ __device__ __forceinline__ Hit traverse(const DevScene& S, v3 o, v3 d, float tma
                                         uint32_t spill_stride, uint32_t gtid, TraverseStats& st) {
     Lane L;
     L.start(o, d, tmax);
-    if (S.node_count == 0) return L.h;
-    while (L.step<ANY, COUNT>(S, lds, spill, spill_stride, gtid, st)) {
+    if (S.node_count != 0)
+        while (L.step<ANY, COUNT>(S, lds, spill, spill_stride, gtid, st)) {
+        }
+    if (S.sph_node_count != 0 && !(ANY && L.h.hit)) {  // then the sphere instances, up to the nearest hit
+        L.node = 0;
+        L.sp = 0;
+        while (L.step<ANY, COUNT, true>(S, lds, spill, spill_stride, gtid, st)) {
+        }
     }
     return L.h;
 }
@@ -242,7 +297,10 @@ __device__ __forceinline__ Hit traverse(const DevScene& S, v3 o, v3 d, float tma
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kFetchStride = 32;  // words between the XCDs' cursors
 constexpr uint32_t kChunk = 2048;
-template <bool ANY, bool COUNT, class Src>
+// SPH: the scene has sphere instances; a lane done with the triangles walks
+// the spheres' tree next, up to its nearest hit (any hit: only when it found
+// none).
+template <bool ANY, bool COUNT, class Src, bool SPH = false>
 __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint32_t* fetch, Src& src, uint32_t* lds,
                                             uint32_t* spill, TraverseStats& st) {
     const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -253,7 +311,7 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
     const uint32_t mine = (n / round) * kChunk + (rem > home * kChunk ? min(rem - home * kChunk, kChunk) : 0u);
     uint32_t* cursor = fetch + home * kFetchStride;
     Lane L;
-    bool active = false, drained = false;
+    bool active = false, drained = false, sph = false;
     uint32_t idx = 0;
     while (true) {
         if (!drained) {
@@ -274,14 +332,27 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
                         src.load(i, o, d, tmax);
                         L.start(o, d, tmax);
                         idx = i;
-                        active = S.node_count != 0;
+                        sph = SPH && S.node_count == 0;
+                        active = S.node_count != 0 || SPH;
                         if (!active) src.done(i, L.h);
                     }
                 }
             }
         }
         if (!__any(active)) break;
-        if (active && !L.step<ANY, COUNT>(S, lds, spill, stride, gtid, st)) {
+        if (SPH) {
+            if (active && !(sph ? L.step<ANY, COUNT, true>(S, lds, spill, stride, gtid, st)
+                                : L.step<ANY, COUNT>(S, lds, spill, stride, gtid, st))) {
+                if (!sph && !(ANY && L.h.hit)) {
+                    sph = true;
+                    L.node = 0;
+                    L.sp = 0;
+                } else {
+                    src.done(idx, L.h);
+                    active = false;
+                }
+            }
+        } else if (active && !L.step<ANY, COUNT>(S, lds, spill, stride, gtid, st)) {
             src.done(idx, L.h);
             active = false;
         }

@@ -127,6 +127,14 @@ class SceneData:
         self.transforms.append(np.eye(3, 4, dtype=np.float32) if transform is None else np.asarray(transform, np.float32))
         return len(self.instances) - 1
 
+    def add_sphere_instance(self, material_address, radius, transform=None):
+        """SphereInstanceData (compat/scene.h:60-69): a sphere of `radius` about
+        the instance's object-space origin.  The reference's scene build keeps
+        only the translation of a sphere's node transform (Scene.cpp:642-645)."""
+        self.instances.append((1 | (material_address << 4), int(np.float32(radius).view(np.uint32))))
+        self.transforms.append(np.eye(3, 4, dtype=np.float32) if transform is None else np.asarray(transform, np.float32))
+        return len(self.instances) - 1
+
     def add_volume(self, grid):
         """A NanoVDB float grid (bytes, as stratum2_amd.nanovdb writes it)."""
         self.volumes.append(np.ascontiguousarray(grid, np.uint8))
@@ -478,6 +486,27 @@ def cornell_box_smooth(seed=0, subdiv=1, light_radiance=15.0):
     vi, nt = s.add_mesh(v.astype(np.float32), f.reshape(-1).astype(np.uint32), normals=n.astype(np.float32))
     a, _ = s.instances[7]
     s.instances[7] = (a, vi | (nt << 16))
+    return s
+
+
+def translation(x, y, z):
+    t = np.eye(3, 4, dtype=np.float32)
+    t[:, 3] = (x, y, z)
+    return t
+
+
+def cornell_spheres(seed=0, light_radiance=15.0, sphere_light=True):
+    """The seeded Cornell box with three procedural sphere instances: a
+    diffuse one, a rough anisotropic metal one, and (sphere_light) a small
+    emissive one of radiance 8 under the ceiling."""
+    s = cornell_box(seed, light_radiance)
+    s.add_sphere_instance(s.add_material(material_record(base_color=(0.7, 0.3, 0.2), specular=0.3, roughness=0.6)), 0.25,
+                          translation(0.45, 1.3, 0.3))
+    s.add_sphere_instance(s.add_material(material_record(base_color=(0.9, 0.8, 0.5), metallic=1.0, roughness=0.45,
+                                                         anisotropic=0.6)), 0.2, translation(-0.5, 0.95, 0.25))
+    if sphere_light:
+        s.add_sphere_instance(s.add_material(material_record(base_color=(0.8, 0.8, 0.8), specular=0.0, roughness=1.0,
+                                                             emission=(8.0, 6.0, 4.0))), 0.12, translation(0.0, 1.6, 0.1))
     return s
 
 
