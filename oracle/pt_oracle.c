@@ -2720,6 +2720,42 @@ static int medium_interaction(pctx_t* x, f3 pos, f3 dir) {
             }
         }
         connect_to_light_subpath(x, NULL, NULL, &m, pos, ldin);
+    } else if (!med_singular(&m) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT) && (x->c->flags & ST_FLAG_RESTIR_DI)) {
+        /* SampleIlluminationReSTIR at a medium vertex (testrenderer.slang:568-606,
+         * 665-684, 730-747): the phase function, cosOut 1, no ray offset */
+        const real cp = rr_cp(x->c, ps->path_len, med_cont_prob(&m));
+        real wsum = 0, sel_t = 0, sel_wtoa = 0, sel_mis = 0, sel_dist = 0;
+        f3 sel_c = v3(0, 0, 0), sel_dir = v3(0, 0, 0);
+        for (uint32_t i = 0; i < x->c->cand; i++) {
+            real rnd[4];
+            rng_nextf(&ps->rng, rnd);
+            isr_t r = sample_illumination(s, x->c, rnd, pos);
+            if (!(luminance(r.radiance) > 0 && r.pdf > 0)) continue;
+            const real v = hg_eval(m.g, ldin, r.dir);
+            const real AtoW = r.finite ? pow2(r.dist) / r.cos_light : 1;
+            const f3 xc = mul(v3(v, v, v), r.radiance);
+            const real xw = r.finite ? r.cos_light / pow2(r.dist) : 1;
+            const real target = luminance(xc) * xw;
+            if (target <= 0) continue;
+            wsum += target / r.pdf;
+            real u[4];
+            rng_nextf(&ps->rng, u);
+            if (u[0] <= (target / r.pdf) / wsum) {
+                sel_t = target; sel_c = xc; sel_wtoa = xw; sel_dir = r.dir; sel_dist = r.dist;
+                sel_mis = (x->c->flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(r.pdf, (v * cp) / AtoW);
+            }
+        }
+        const real W = sel_t > 0 ? wsum / ((real)x->c->cand * sel_t) : 0;
+        if (W > 0 && ps->path_len + 1 >= x->c->min_len && x->nsh < 64) {
+            shadow_t* sr = &x->sh[x->nsh++];
+            sr->o = pos;
+            sr->d = sel_dir;
+            sr->tmax = maxf(sel_dist * 0.99f, sel_dist - 1e-2f);
+            sr->c = mulf(mul(ps->thr, sel_c), W * sel_wtoa);
+            sr->w = sel_mis;
+            sr->medium = ps->medium;
+            sr->plen = ps->path_len;
+        }
     } else if (!med_singular(&m) && (x->c->flags & ST_FLAG_SAMPLE_DIRECT)) {
         real rnd[4];
         rng_nextf(&ps->rng, rnd);
@@ -3913,7 +3949,8 @@ static int render_camera(or_scene* s, const st_view* view, const st_render_param
     if (check_tri_albedo(s, p->flags)) return -1;
     if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, out, threads);
     if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, out, threads);
-    if (s->nvolinst > 0 && (p->flags & ST_FLAG_RESTIR_DI)) return fail("ReSTIR DI with media is not supported");
+    if (s->nvolinst > 0 && (p->flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_RESTIR_DI_HASHGRID)))
+        return fail("ReSTIR DI reuse with media is not supported (RIS is)");
     const int reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
     if (reuse && !(p->flags & ST_FLAG_RESTIR_DI)) return fail("ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");
     if (reuse && (p->flags & (ST_FLAG_FLOAT_NEE | ST_FLAG_PIXEL_JITTER)))

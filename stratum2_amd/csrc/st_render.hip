@@ -1139,7 +1139,42 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                         if (to_cache) alive = false;
                     }
                     const bool rr_on = path_len >= c.rr_start;
-                    if (alive && !m.singular() && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
+                    if (RESTIR && alive && !m.singular() && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
+                        // SampleIlluminationReSTIR at a medium vertex (testrenderer.slang:568-606,
+                        // 665-684, 730-747): the phase function, cosOut 1, no offset
+                        const float cpm = rr_on ? m.cont_prob() : 1.0f;
+                        float wsum = 0, sel_t = 0, sel_pdf = 0, sel_wtoa = 0, sel_mis = 0, sel_dist = 0;
+                        v3 sel_c = mk(0, 0, 0), sel_dir = mk(0, 0, 0);
+                        for (uint32_t ci = 0; ci < c.cand; ci++) {
+                            const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), pos);
+                            if (!(lum(L.radiance) > 0 && L.pdf > 0)) continue;
+                            const float v = hg_eval(m.g, ldin, L.dir);
+                            const float AtoW = L.finite ? sq(L.dist) / L.cos_light : 1.0f;
+                            const v3 xc = mk(v, v, v) * L.radiance;
+                            const float xw = L.finite ? L.cos_light / sq(L.dist) : 1.0f;
+                            const float target = lum(xc) * xw;
+                            if (target <= 0) continue;
+                            wsum += target / L.pdf;
+                            if (rng.nextf().x <= (target / L.pdf) / wsum) {
+                                sel_t = target;
+                                sel_pdf = L.pdf;
+                                sel_c = xc;
+                                sel_wtoa = xw;
+                                sel_dir = L.dir;
+                                sel_dist = L.dist;
+                                sel_mis = (c.flags & ST_FLAG_DIRECT_ONLY) ? 1.0f : mis2(L.pdf, (v * cpm) / AtoW);
+                            }
+                        }
+                        const float Wr = sel_t > 0 ? wsum / ((float)c.cand * sel_t) : 0.0f;
+                        if (Wr > 0 && path_len + 1 >= c.min_len) {
+                            const v3 contrib = (thr * sel_c) * (Wr * sel_wtoa);
+                            sh0 = make_float4(pos.x, pos.y, pos.z, fmax_(sel_dist * 0.99f, sel_dist - 1e-2f));
+                            sh1 = make_float4(sel_dir.x, sel_dir.y, sel_dir.z, __uint_as_float(p));
+                            sh2 = make_float4(contrib.x, contrib.y, contrib.z, sel_mis);
+                            emit_shadow = true;
+                        }
+                        (void)sel_pdf;
+                    } else if (alive && !m.singular() && (c.flags & ST_FLAG_SAMPLE_DIRECT)) {
                         // ConnectToLight (:751-779) with the phase function; cosOut = 1
                         const LightSample L = sample_illumination<ALIAS>(S, c.env_prob, rng.nextf(), pos);
                         if (!(all_nonpos(L.radiance) || L.pdf <= 0)) {
@@ -3053,10 +3088,12 @@ static ShadeFn shade_kernel_adj(bool lambert, bool alias, bool media) {
     return t[(media ? 4 : 0) | (lambert ? 2 : 0) | (alias ? 1 : 0)];
 }
 static ShadeFn shade_kernel(bool lambert, bool alias, bool restir, bool media) {
-    if (media) {  // ReSTIR DI with media is rejected before this
-        static const ShadeFn mt[4] = {k_shade<false, false, false, true>, k_shade<false, true, false, true>,
-                                      k_shade<true, false, false, true>, k_shade<true, true, false, true>};
-        return mt[(lambert ? 2 : 0) | (alias ? 1 : 0)];
+    if (media) {  // RIS at every vertex (no reuse) with media
+        static const ShadeFn mt[8] = {k_shade<false, false, false, true>, k_shade<false, false, true, true>,
+                                      k_shade<false, true, false, true>,  k_shade<false, true, true, true>,
+                                      k_shade<true, false, false, true>,  k_shade<true, false, true, true>,
+                                      k_shade<true, true, false, true>,   k_shade<true, true, true, true>};
+        return mt[(lambert ? 4 : 0) | (alias ? 2 : 0) | (restir ? 1 : 0)];
     }
     const int i = (lambert ? 4 : 0) | (alias ? 2 : 0) | (restir ? 1 : 0);
     static const ShadeFn table[8] = {
@@ -3188,8 +3225,8 @@ static int check_camera(st_scene* s, const st_render_params* p) {
         return set_error(ST_ERROR_INVALID_ARGUMENT, "candidate_samples must be in [1, 1024]");
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
     if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
-    if (s->d.vinst_count > 0 && (p->flags & ST_FLAG_RESTIR_DI))
-        return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI is not supported in scenes with media");
+    if (s->d.vinst_count > 0 && (p->flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_RESTIR_DI_HASHGRID)))
+        return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI reuse is not supported in scenes with media (RIS is)");
     if (p->flags & ST_FLAG_RESTIR_DI_HASHGRID) {
         if (!(p->flags & ST_FLAG_RESTIR_DI) || (p->flags & ST_FLAG_RESTIR_DI_REUSE))
             return set_error(ST_ERROR_INVALID_ARGUMENT, "ST_FLAG_RESTIR_DI_HASHGRID needs ST_FLAG_RESTIR_DI and excludes ST_FLAG_RESTIR_DI_REUSE");

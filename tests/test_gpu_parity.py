@@ -509,6 +509,27 @@ def test_media_homogeneous_box_in_cornell(gpu_ctx):
     _check_media_image(g, r)
 
 
+@pytest.mark.parametrize("scene", ["cornell_box", "cloud"])
+def test_media_ris_vs_oracle(gpu_ctx, scene):
+    """ReSTIR DI's RIS (32 candidates) at every vertex of a scene with media:
+    medium vertices weigh the candidates by the phase function (cosOut 1) and
+    their shadow rays ratio-track through the media."""
+    flags = abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI
+    if scene == "cornell_box":
+        sd = scenes.cornell_box(0)
+        scenes.add_medium_box(sd, (0.8, 1.0, 1.3), (0.8, 0.8, 0.8), 0.3, (-0.55, 0.45, -0.7), (0.55, 1.35, 0.3))
+        W = H = 64
+        view = scenes.cornell_view(W, H)
+    else:
+        sd = scenes.cloud(seed=2, n=64, env_size=(256, 128))
+        W = H = 64
+        view = scenes.cloud_view(W, H)
+        flags |= abi.ST_FLAG_ENV_ALIAS
+    params = abi.default_params(W, H, max_depth=6, spp=4, flags=flags)
+    g, r = _render_both(gpu_ctx, sd, view, params)
+    _check_media_image(g, r)
+
+
 def test_media_heterogeneous_cloud(gpu_ctx):
     sd = scenes.cloud(seed=2, n=64, env_size=(256, 128))
     W = H = 64

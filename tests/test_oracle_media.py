@@ -204,6 +204,28 @@ def test_lit_floor_under_absorbing_slab(use_grid):
     assert np.abs(rel).max() < 0.03, rel
 
 
+@pytest.mark.parametrize("paths", ["direct", "full"])
+def test_ris_in_media_keeps_expectation(paths):
+    """RIS over 32 candidates at surface and medium vertices (ReSTIR DI
+    without reuse) estimates the image of one-sample NEE: a Cornell box with
+    a scattering block, paired over 8 seed ranges."""
+    from scipy import stats
+    sd = scenes.cornell_box(0)
+    scenes.add_medium_box(sd, (0.8, 1.0, 1.3), (0.8, 0.8, 0.8), 0.3, (-0.55, 0.45, -0.7), (0.55, 1.35, 0.3))
+    W = H = 32
+    view = scenes.cornell_view(W, H)
+    osc = OracleScene(sd)
+    flags = abi.ST_FLAG_DEFAULT | (abi.ST_FLAG_DIRECT_ONLY if paths == "direct" else 0)
+    means = {}
+    for mode in ("nee", "ris"):
+        fl = flags | (abi.ST_FLAG_RESTIR_DI if mode == "ris" else 0)
+        means[mode] = np.array([osc.render(view, abi.default_params(W, H, max_depth=4, spp=16, seed=1000 * k, flags=fl),
+                                           want_path_length=False)["hdr"][..., :3].mean() for k in range(8)])
+    t = stats.ttest_rel(means["nee"], means["ris"])
+    print(means["nee"].mean(), means["ris"].mean(), t.pvalue)
+    assert t.pvalue > 1e-3
+
+
 @pytest.mark.parametrize("g", [0.0, 0.3])
 def test_scattering_medium_furnace(g):
     """A white (albedo 1) scattering medium inside a closed box whose walls
@@ -220,3 +242,7 @@ def test_scattering_medium_furnace(g):
     got = out["hdr"][..., :3].astype(np.float64)
     assert abs(got.mean() - 1) < 0.02, got.mean()
     assert out["path_length"].max() > 3  # paths did scatter
+    # RIS at the medium and wall vertices keeps L = Le
+    p = abi.default_params(W, H, max_depth=64, spp=64, flags=abi.ST_FLAG_DEFAULT | abi.ST_FLAG_RESTIR_DI, candidate_samples=8)
+    got = OracleScene(sd).render(view, p, want_path_length=False)["hdr"][..., :3].astype(np.float64)
+    assert abs(got.mean() - 1) < 0.02, got.mean()
