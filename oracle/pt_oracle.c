@@ -498,6 +498,10 @@ struct or_scene {
     /* procedural sphere instances (SphereInstanceData, compat/scene.h:60-69) */
     uint32_t nsph;
     uint32_t* sph_inst;
+    /* material images (Material::store's image indices), each with its mip
+     * chain; ntex = number of images, tex[i].n = 0 when no material uses it */
+    uint32_t ntex;
+    struct tex_s { int n; level_t lv[16]; }* tex;
 };
 
 static f3 xf_point(const st_transform* t, f3 v) {
@@ -552,6 +556,7 @@ typedef struct {
     uint32_t gn, sn, tan;
     real area;
     real uv[2];
+    real uvs; /* mTexcoordScreenSize */
 } shading_t; /* ShadingData, src/Shaders/compat/scene.h:185-198 (fields used here) */
 
 /* makeTriangleShadingDataWithoutPosition + makeTriangleShadingData,
@@ -575,9 +580,11 @@ static void make_sphere_shading(const or_scene* s, uint32_t ii, f3 lp, shading_t
     cart_to_uv(normalize3(lp), uv);
     r->uv[0] = uv[0];
     r->uv[1] = uv[1];
-    const real su = om_sin(uv[0]), cu = om_cos(uv[0]), sv = om_sin(uv[1]);
+    const real su = om_sin(uv[0]), cu = om_cos(uv[0]), sv = om_sin(uv[1]), cv = om_cos(uv[1]);
     const f3 dpdu = xf_vector(T, v3(-su * sv, 0.0f, cu * sv));
+    const f3 dpdv = xf_vector(T, v3(cu * cv, -sv, su * cv));
     r->tan = pack_normal(normalize3(sub(dpdu, mulf(n, dot(n, dpdu)))));
+    r->uvs = 1 / maxf(length3(dpdu), length3(dpdv));
 }
 static void make_tri_shading_x(const or_scene* s, uint32_t ii, uint32_t prim, real bx, real by, const f3* lp, shading_t* r) {
     if (prim == 0xFFFFu) { /* a sphere instance, at lp */
@@ -622,13 +629,17 @@ static void make_tri_shading_x(const or_scene* s, uint32_t ii, uint32_t prim, re
     real det = duvds[0] * duvdt[1] - duvdt[0] * duvds[1];
     real invDet = 1 / det;
     real dsdu = duvdt[1] * invDet, dtdu = -duvds[1] * invDet;
+    real dsdv = duvdt[0] * invDet, dtdv = -duvds[0] * invDet; /* as the reference writes them */
     f3 dPdu;
     if (det != 0) {
         dPdu = neg(add(mulf(dPds, dsdu), mulf(dPdt, dtdu)));
+        const f3 dPdv = neg(add(mulf(dPds, dsdv), mulf(dPdt, dtdv)));
+        r->uvs = 1 / maxf(length3(dPdu), length3(dPdv));
     } else {
         f3 onb[3];
         make_orthonormal(gn, onb);
         dPdu = onb[0];
+        r->uvs = 1;
     }
     int snvalid = 0;
     f3 sn = v3(0, 0, 0);
@@ -1042,6 +1053,47 @@ static int m_singular(const material_t* m, int mode) {
 }
 static int m_can_eval(const material_t* m) { f3 b = m_base(m); return any_gt0(b); }
 static real m_cont_prob(const material_t* m) { return saturate(luminance(m_base(m))); }
+static void tex_sample(const or_scene* s, uint32_t img, real u, real v, real uvs, real out[4]);
+static uint32_t pack_unorm4(const real v[4]);
+static void unpack_unorm4(uint32_t w, real v[4]);
+/* LoadMaterial with its images (material.hlsli:22-45) at the shading point's
+ * uv and screen size: base colour times the image (setBaseColor re-quantises
+ * to 8 bits), the metallic/roughness/anisotropy/subsurface word modulated
+ * (roughness through smoothness), the clearcoat/transmission/eta word
+ * modulated.  Emission images are refused at scene creation. */
+static material_t load_material_sd(const or_scene* s, const shading_t* sd) {
+    const uint32_t addr = sd->flags_material >> 4;
+    material_t m = load_material(s, addr);
+    if (!s->ntex || (uint64_t)addr / 4 + 8 > s->nmatw) return m;
+    const uint32_t* ii = s->mat + addr / 4 + 4;
+    if (ii[0] < s->ntex && s->tex[ii[0]].n) {
+        real t[4];
+        tex_sample(s, ii[0], sd->uv[0], sd->uv[1], sd->uvs, t);
+        const f3 b = m_base(&m);
+        const real bb[3] = {b.x * t[0], b.y * t[1], b.z * t[2]};
+        m.p[0] &= 0xff000000u;
+        for (int c = 0; c < 3; c++) m.p[0] |= (uint32_t)floorr(saturate(bb[c]) * 255 + 0.5f) << (8 * c);
+    }
+    if (ii[2] < s->ntex && s->tex[ii[2]].n) {
+        real v[4], t[4];
+        unpack_unorm4(m.p[2], v);
+        tex_sample(s, ii[2], sd->uv[0], sd->uv[1], sd->uvs, t);
+        v[0] *= t[0];
+        v[2] *= t[2];
+        v[3] *= t[3];
+        v[1] = 1 - (1 - v[1]) * (1 - t[1]);
+        m.p[2] = pack_unorm4(v);
+    }
+    if (ii[3] < s->ntex && s->tex[ii[3]].n) {
+        real v[4], t[4];
+        unpack_unorm4(m.p[3], v);
+        tex_sample(s, ii[3], sd->uv[0], sd->uv[1], sd->uvs, t);
+        for (int c = 0; c < 4; c++) v[c] *= t[c];
+        m.p[3] = pack_unorm4(v);
+    }
+    return m;
+}
+
 
 void or_bsdf_eval(const uint32_t mat[5], int lambertian, const float din[3], const float dout[3], float o[5]) {
     material_t m;
@@ -1077,11 +1129,11 @@ static real texel_lum(const level_t* l, int x, int y) {
     return luminance(v3(t[0], t[1], t[2]));
 }
 /* mip chain by 2x2 box (vkCmdBlitImage with linear filter, src/Core/Image.cpp:282-308) */
-static void build_mips(or_scene* s) {
+static int build_level_chain(level_t* lv) {
     int l = 0;
-    while (s->lv[l].w > 1 || s->lv[l].h > 1) {
-        level_t* a = &s->lv[l];
-        level_t* b = &s->lv[l + 1];
+    while ((lv[l].w > 1 || lv[l].h > 1) && l < 15) {
+        level_t* a = &lv[l];
+        level_t* b = &lv[l + 1];
         b->w = a->w > 1 ? a->w / 2 : 1;
         b->h = a->h > 1 ? a->h / 2 : 1;
         b->rgba = (float*)malloc(sizeof(float) * 4 * (size_t)b->w * b->h);
@@ -1096,8 +1148,9 @@ static void build_mips(or_scene* s) {
             }
         l++;
     }
-    s->nlevels = l + 1;
+    return l + 1;
 }
+static void build_mips(or_scene* s) { s->nlevels = build_level_chain(s->lv); }
 /* bilinear SampleLevel(uv, 0) with repeat addressing (TestRenderer.cpp:54-57) */
 static f3 env_bilinear(const level_t* l, real u, real v) {
     real x = u * (real)l->w - 0.5f, y = v * (real)l->h - 0.5f;
@@ -1115,6 +1168,73 @@ static f3 env_bilinear(const level_t* l, real u, real v) {
     }
     return r;
 }
+/* bilinear SampleLevel of all four channels at one level, repeat addressing */
+static void bilinear4(const level_t* l, real u, real v, real out[4]) {
+    real x = u * (real)l->w - 0.5f, y = v * (real)l->h - 0.5f;
+    real fx0 = floorr(x), fy0 = floorr(y), ax = x - fx0, ay = y - fy0;
+    int x0 = (int)fx0, y0 = (int)fy0;
+    int xa = ((x0 % l->w) + l->w) % l->w, xb = (((x0 + 1) % l->w) + l->w) % l->w;
+    int ya = ((y0 % l->h) + l->h) % l->h, yb = (((y0 + 1) % l->h) + l->h) % l->h;
+    const float *t00 = texel(l, xa, ya), *t10 = texel(l, xb, ya), *t01 = texel(l, xa, yb), *t11 = texel(l, xb, yb);
+    for (int c = 0; c < 4; c++) {
+        real top = t00[c] * (1 - ax) + t10[c] * ax;
+        real bot = t01[c] * (1 - ax) + t11[c] * ax;
+        out[c] = top * (1 - ay) + bot * ay;
+    }
+}
+/* SampleImage4 (scene.hlsli:101-109): SampleLevel at lod = log2(max(uvScreenSize
+ * * max(w, h), 1e-6)) (0 without a screen size), with the static sampler's
+ * linear filter between the two nearest levels (TestRenderer.cpp:54-57); the
+ * lod is clamped to the chain.  log2 is the renderer's log over ln 2. */
+static void tex_sample(const or_scene* s, uint32_t img, real u, real v, real uvs, real out[4]) {
+    const struct tex_s* t = &s->tex[img];
+    real lod = 0;
+    if (uvs > 0) {
+        const real wh = (real)(t->lv[0].w > t->lv[0].h ? t->lv[0].w : t->lv[0].h);
+        lod = om_log(maxf(uvs * wh, 1e-6f)) * (real)1.44269504088896340736;
+    }
+    if (!(lod > 0)) lod = 0;
+    if (lod > (real)(t->n - 1)) lod = (real)(t->n - 1);
+    const int l0 = (int)floorr(lod), l1 = l0 + 1 < t->n ? l0 + 1 : l0;
+    const real f = lod - (real)l0;
+    real a[4], b[4];
+    bilinear4(&t->lv[l0], u, v, a);
+    if (f > 0 && l1 != l0) {
+        bilinear4(&t->lv[l1], u, v, b);
+        for (int c = 0; c < 4; c++) out[c] = a[c] * (1 - f) + b[c] * f;
+    } else {
+        for (int c = 0; c < 4; c++) out[c] = a[c];
+    }
+}
+/* D3DX_FLOAT4_to_R8G8B8A8_UNORM: saturate, scale by 255, round half up */
+static uint32_t pack_unorm4(const real v[4]) {
+    uint32_t w = 0;
+    for (int c = 0; c < 4; c++) w |= (uint32_t)(saturate(v[c]) * 255 + 0.5f) << (8 * c);
+    return w;
+}
+static void unpack_unorm4(uint32_t w, real v[4]) {
+    for (int c = 0; c < 4; c++) v[c] = (real)((w >> (8 * c)) & 0xffu) / 255.0f;
+}
+
+/* unit entry: SampleImage4 of one RGBA32F image (its mip chain built here) */
+int or_tex_sample(const float* rgba, uint32_t w, uint32_t h, float u, float v, float uvs, float out[4]) {
+    or_scene tmp;
+    memset(&tmp, 0, sizeof tmp);
+    struct tex_s t;
+    memset(&t, 0, sizeof t);
+    t.lv[0].w = (int)w; t.lv[0].h = (int)h;
+    t.lv[0].rgba = (float*)malloc(sizeof(float) * 4 * (size_t)w * h);
+    memcpy(t.lv[0].rgba, rgba, sizeof(float) * 4 * (size_t)w * h);
+    t.n = build_level_chain(t.lv);
+    tmp.tex = &t;
+    tmp.ntex = 1;
+    real o[4];
+    tex_sample(&tmp, 0, u, v, uvs, o);
+    for (int c = 0; c < 4; c++) out[c] = o[c];
+    for (int l = 0; l < 16; l++) free(t.lv[l].rgba);
+    return 0;
+}
+
 /* SampleTexel, src/Shaders/common/lights.hlsli:49-97 */
 static void sample_texel(const or_scene* s, real rx, real ry, real uv[2], real* pdf) {
     int levelCount = s->nlevels, cx = 0, cy = 0, lw = 1, lh = 1;
@@ -2002,6 +2122,31 @@ or_scene* or_scene_create(const st_scene_desc* d) {
         }
     }
     if (s->nsph && s->nvolinst) { fail("sphere instances together with media are not supported"); or_scene_destroy(s); return NULL; }
+    /* material images: base colour, packed words 2 and 3 (Material::store's
+     * image indices); emission images, alpha masks and normal maps are refused */
+    s->tex = (struct tex_s*)calloc(d->image_count + 1, sizeof(struct tex_s));
+    for (uint32_t i = 0; i < s->ninst; i++) {
+        const uint32_t type = s->inst[i].type_material_address & 0xfu;
+        if (type > 1) continue;
+        const uint32_t a = (s->inst[i].type_material_address >> 4) / 4;
+        if ((uint64_t)a + 14 > s->nmatw) continue;
+        if (s->mat[a + 5] < ST_IMAGE_LIMIT || s->mat[a + 9] < ST_IMAGE_LIMIT || s->mat[a + 12] < ST_IMAGE_LIMIT) {
+            fail("emission images, normal maps and alpha masks are not supported"); or_scene_destroy(s); return NULL;
+        }
+        for (int k = 4; k < 8; k++) {
+            const uint32_t img = s->mat[a + k];
+            if (img >= ST_IMAGE_LIMIT) continue;
+            if (img >= d->image_count) { fail("material image index out of range"); or_scene_destroy(s); return NULL; }
+            struct tex_s* t = &s->tex[img];
+            if (t->n) continue;
+            const st_image_desc* im = &d->images[img];
+            t->lv[0].w = (int)im->width; t->lv[0].h = (int)im->height;
+            t->lv[0].rgba = (float*)malloc(sizeof(float) * 4 * (size_t)im->width * im->height);
+            memcpy(t->lv[0].rgba, im->rgba, sizeof(float) * 4 * (size_t)im->width * im->height);
+            t->n = build_level_chain(t->lv);
+            s->ntex = d->image_count;
+        }
+    }
     /* scene sphere, TestRenderer.cpp:458-461 */
     f3 cen = divf(add(v3(amax[0], amax[1], amax[2]), v3(amin[0], amin[1], amin[2])), 2);
     s->sphere[0] = cen.x; s->sphere[1] = cen.y; s->sphere[2] = cen.z;
@@ -2025,6 +2170,10 @@ void or_scene_destroy(or_scene* s) {
     free(s->alias_prob); free(s->alias_idx); free(s->alias_pdf);
     free(s->light_inst); free(s->tv); free(s->tid); free(s->nodes); free(s->order);
     free(s->inv_xf); free(s->volinfo); free(s->sph_inst);
+    if (s->tex)
+        for (uint32_t i = 0; i < s->ntex; i++)
+            for (int l = 0; l < 16; l++) free(s->tex[i].lv[l].rgba);
+    free(s->tex);
     for (uint32_t v = 0; v < s->nvols; v++) { free(s->vol[v]); free(s->volw[v]); }
     free(s->tri_alb); free(s->tri_base);
     free(s);
@@ -2324,7 +2473,7 @@ static void connect_to_light_subpath(pctx_t* x, const material_t* m, const shadi
         cos_light = 1;
         cpv = med_cont_prob(&mv);
     } else {
-        const material_t mv = load_material(x->s, sv.flags_material >> 4);
+        const material_t mv = load_material_sd(x->s, &sv);
         const f3 vout = to_local(&sv, neg(dir));
         cos_light = fabsr(vout.z);
         fv = bsdf_eval_x(&mv, x->c->lambert, vin, vout, 1);
@@ -2443,7 +2592,7 @@ static void hg_reuse(pctx_t* x, const material_t* m, const shading_t* sd, f3 ldi
                 const f3 lp = v3(e[0], e[1], e[2]);
                 shading_t sq;
                 make_tri_shading_x(x->s, id & 0xffffu, id >> 16, 0, 0, &lp, &sq);
-                const material_t mq = load_material(x->s, sq.flags_material >> 4);
+                const material_t mq = load_material_sd(x->s, &sq);
                 rsd_t xq;
                 rsd_init(&xq, x, &mq, &sq, v3(e[12], e[13], e[14]), sample->smp);
                 phq = rsd_target(&xq);
@@ -2877,7 +3026,7 @@ static int extend_path(pctx_t* x, const st_render_params* p, const st_view* view
         return 0;
     }
     if (all_le0(ps->thr)) return 0; /* ExtendPath, testrenderer.slang:1062-1063 */
-    material_t m = load_material(s, sd.flags_material >> 4);
+    material_t m = load_material_sd(s, &sd);
     uint32_t alb_tri = 0xffffffffu; /* the vertex's entry of the per-triangle albedo */
     if (s->tri_alb) {
         alb_tri = s->tri_base[h.id & 0xffffu] + (h.id >> 16);
@@ -3283,7 +3432,7 @@ static void trace_light_path(lctx_t* lx, uint32_t q) {
         } else {
             shading_t sd;
             make_hit_shading(s, &h, ps->origin, wd, &sd);
-            const material_t m = load_material(s, sd.flags_material >> 4);
+            const material_t m = load_material_sd(s, &sd);
             const f3 ldin = to_local(&sd, neg(wd));
             if (vc) { /* testrenderer.slang:812-824 */
                 const real cos_in = fabsr(ldin.z);
@@ -3566,7 +3715,7 @@ typedef struct {
 static void prim_make(const or_scene* s, const shading_t* sd, uint32_t dir, prim_t* P) {
     P->hit = 1;
     P->sd = *sd;
-    P->m = load_material(s, sd->flags_material >> 4);
+    P->m = load_material_sd(s, sd);
     P->ldin = to_local(sd, neg(unpack_normal(dir)));
 }
 /* target function of a light-sample seed at a primary vertex (0 when unusable) */
@@ -3951,6 +4100,7 @@ static int render_camera(or_scene* s, const st_view* view, const st_render_param
     if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, out, threads);
     if (s->nvolinst > 0 && (p->flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_RESTIR_DI_HASHGRID)))
         return fail("ReSTIR DI reuse with media is not supported (RIS is)");
+    if (s->ntex && (p->flags & ST_FLAG_RESTIR_DI_REUSE)) return fail("the screen-space reuse with material images");
     const int reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
     if (reuse && !(p->flags & ST_FLAG_RESTIR_DI)) return fail("ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");
     if (reuse && (p->flags & (ST_FLAG_FLOAT_NEE | ST_FLAG_PIXEL_JITTER)))

@@ -92,6 +92,9 @@ float or_env_sample_texel_pdf(or_scene* s, const float uv[2]);
  * be null). */
 int or_env_alias_sample(or_scene* s, uint32_t n, const float* rnd, float* uv, float* pdf);
 float or_env_alias_pdf(or_scene* s, const float uv[2]);
+/* SampleImage4 (scene.hlsli:101-109) of one RGBA32F image with its mip chain,
+ * at uv with texture-coordinate screen size uvs (0: level 0). */
+int or_tex_sample(const float* rgba, uint32_t w, uint32_t h, float u, float v, float uvs, float out[4]);
 int or_env_alias_table(or_scene* s, float* prob, uint32_t* alias, float* pdf);
 
 /* BSDF restated: evaluateReflectance / sampleDirection (disney.hlsli:391-429,

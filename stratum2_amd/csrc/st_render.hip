@@ -2465,6 +2465,8 @@ void validate_desc(const st_scene_desc* d) {
         if ((uint64_t)addr / 4 + 16 > d->material_word_count) throw std::invalid_argument("material address out of range");
         for (int k = 0; k < 4; k++)
             if (d->material_data[addr / 4 + 4 + k] < ST_IMAGE_LIMIT) throw std::domain_error("material textures are not supported");
+        if (d->material_data[addr / 4 + 9] < ST_IMAGE_LIMIT || d->material_data[addr / 4 + 12] < ST_IMAGE_LIMIT)
+            throw std::domain_error("material textures are not supported (normal maps, alpha masks)");
         if (type == 0 && (in.data & 0xffffu) >= d->mesh_vertex_info_count) throw std::invalid_argument("vertex info index out of range");
         if (type == 1) nsph++;
     }

@@ -510,6 +510,47 @@ def cornell_spheres(seed=0, light_radiance=15.0, sphere_light=True):
     return s
 
 
+def checker_image(width, height, cells, c0, c1, rng=None):
+    """RGBA32F checkerboard of cells x cells squares alternating c0 and c1,
+    with per-texel noise of 0.05 when rng is given."""
+    y, x = np.mgrid[0:height, 0:width]
+    sel = ((x * cells // width) + (y * cells // height)) % 2
+    img = np.where(sel[..., None] == 0, np.asarray(c0, np.float32), np.asarray(c1, np.float32)).astype(np.float32)
+    if rng is not None:
+        img = img + rng.uniform(-0.05, 0.05, img.shape).astype(np.float32)
+    return np.clip(img, 0, 1).astype(np.float32)
+
+
+def cornell_textured(seed=0):
+    """The seeded Cornell box with two textured quads: a floor tile whose base
+    colour is a 64x64 checker (texture coordinates repeating twice) and a
+    back panel whose roughness/metallic word comes from a 16x32 image
+    (material.hlsli:22-45 modulation), plus a textured sphere."""
+    rng = np.random.default_rng(seed + 100)
+    s = cornell_box(seed)
+    base = len(s.images)
+    s.images.append(checker_image(64, 64, 8, (0.9, 0.2, 0.1, 1), (0.1, 0.8, 0.3, 1), rng))
+    s.images.append(checker_image(16, 32, 4, (1.0, 0.1, 1.0, 1.0), (0.3, 0.9, 0.5, 0.4), rng))
+    p, i = quad((-0.9, 0.003, -0.9), (0.9, 0.003, -0.9), (0.9, 0.003, 0.9), (-0.9, 0.003, 0.9))
+    f = _tri_normal_ok(p, i.reshape(-1, 3), (0, 1, 0))
+    uv = np.array([[0, 0], [2, 0], [2, 2], [0, 2]], np.float32)
+    vi, n = s.add_mesh(p, f.reshape(-1), texcoords=uv)
+    w = material_record(base_color=(1, 1, 1), specular=0.4, roughness=0.5)
+    w[4] = base
+    s.add_mesh_instance(vi, n, s.add_material(w))
+    p, i = quad((-0.8, 0.2, -0.99), (0.8, 0.2, -0.99), (0.8, 1.6, -0.99), (-0.8, 1.6, -0.99))
+    f = _tri_normal_ok(p, i.reshape(-1, 3), (0, 0, 1))
+    vi, n = s.add_mesh(p, f.reshape(-1), texcoords=np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32))
+    w = material_record(base_color=(0.8, 0.7, 0.6), metallic=1.0, roughness=0.6, anisotropic=0.5, clearcoat=0.5)
+    w[6] = base + 1
+    w[7] = base + 1
+    s.add_mesh_instance(vi, n, s.add_material(w))
+    w = material_record(base_color=(1, 1, 1), specular=0.5, roughness=0.4)
+    w[4] = base
+    s.add_sphere_instance(s.add_material(w), 0.22, translation(0.4, 1.2, 0.3))
+    return s
+
+
 def cornell_view(width=256, height=256):
     return make_view(width, height, position=(0.0, 1.0, 2.4))
 
