@@ -80,6 +80,14 @@ struct DevScene {
     const float4* sph_nodes;
     const float4* sph_tris;
     uint32_t sph_node_count;
+    // material images (Material::store's image indices 0, 2, 3): every level
+    // of every used image in tex_data; per level {first float4, w, h, 0};
+    // per image {first level, level count} (count 0: no material uses it)
+    const float4* tex_data;
+    const uint4* tex_lv;
+    const uint2* tex_img;
+    uint32_t tex_count;
+    int has_tex;
 };
 
 }  // namespace st

@@ -807,7 +807,7 @@ __device__ inline void hash_grid_reuse(const DevScene& S, const RenderConsts& c,
                 const v3 lp = mk(e0.x, e0.y, e0.z);
                 Shading sq;
                 tri_shading(S, id & 0xffffu, id >> 16, 0.0f, 0.0f, sq, &lp);
-                const Material mq = load_material(S, sq.material, c.flags);
+                const Material mq = load_material<false>(S, sq, c.flags);
                 const Frame fq = make_frame(sq);
                 Cand xq;
                 float lrq;
@@ -987,7 +987,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_ris(DevScene S, RenderConsts c,
         Shading sd;
         if (SPH) hit_shading(S, hit, mk(r.x, r.y, r.z), unpack_normal(__float_as_uint(r.w)), sd);
         else tri_shading(S, hit.y & 0xffffu, hit.y >> 16, __uint_as_float(hit.z), __uint_as_float(hit.w), sd);
-        const Material m = load_material(S, sd.material, c.flags);
+        const Material m = load_material<false>(S, sd, c.flags);
         const Frame f = make_frame(sd);
         const v3 ldin = f.to_local(-unpack_normal(__float_as_uint(r.w)));
         const v3 lgn = f.to_local(unpack_normal(sd.gn));
@@ -1047,7 +1047,8 @@ __device__ __forceinline__ bool cache_vertex(const RenderConsts& c, const WaveSt
     return true;
 }
 
-template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA, bool ADJ = false, bool CACHE = false>
+// TEX: the scene has material images (their sampling compiled only here)
+template <bool LAMBERT, bool ALIAS, bool RESTIR, bool MEDIA, bool ADJ = false, bool CACHE = false, bool TEX = false>
 __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts c, WaveState w, int qsel,
                                                        AuxOut aux, int first_wave, AdjArgs adj) {
     __shared__ AdjTable adj_tab;  // ADJ only
@@ -1229,7 +1230,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene S, RenderConsts 
                 const float thit = MEDIA ? len(sd.pos - mk(r.x, r.y, r.z)) : __uint_as_float(hit.x);
                 const float t16 = q16(thit), pdfa16 = q16(pdfa);
                 const bool live = !MEDIA || !all_nonpos(thr);  // ExtendPath, testrenderer.slang:1062-1063
-                Material m = load_material(S, sd.material, c.flags);
+                Material m = load_material<TEX>(S, sd, c.flags);
                 uint32_t alb_entry = 0xffffffffu;
                 if (LAMBERT && S.tri_alb) apply_triangle_albedo(S, m, ii, prim, alb_entry);
                 if (live && path_len == 1 && first_wave && k == 0)
@@ -1650,7 +1651,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_light(DevScene S, RenderC
                 } else {
                     Shading sd;
                     hit_shading(S, hit, mk(r.x, r.y, r.z), dir, sd);
-                    const Material m = load_material(S, sd.material, c.flags);
+                    const Material m = load_material<false>(S, sd, c.flags);
                     const Frame f = make_frame(sd);
                     const v3 ldin = f.to_local(-dir);
                     const v3 lgn = f.to_local(unpack_normal(sd.gn));
@@ -1809,7 +1810,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_vc(DevScene S, RenderCons
                     if (S.has_env) pdfa *= 1 - c.env_prob;
                 }
                 const float t16 = q16(__uint_as_float(hit.x)), pdfa16 = q16(pdfa);
-                const Material m = load_material(S, sd.material, c.flags);
+                const Material m = load_material<false>(S, sd, c.flags);
                 const Frame f = make_frame(sd);
                 const v3 ldin = f.to_local(-dir);
                 const v3 lgn = f.to_local(unpack_normal(sd.gn));
@@ -1862,7 +1863,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade_vc(DevScene S, RenderCons
                             v3 d = sv.pos - sd.pos;
                             const float dist = len(d);
                             d = d / dist;
-                            const Material mv = load_material(S, sv.material, c.flags);
+                            const Material mv = load_material<false>(S, sv, c.flags);
                             const Frame fv = make_frame(sv);
                             const v3 vin = unpack_normal(__float_as_uint(v1.w));
                             const v3 vout = fv.to_local(-d);
@@ -2463,8 +2464,15 @@ void validate_desc(const st_scene_desc* d) {
         }
         const uint32_t addr = in.type_material_address >> 4;
         if ((uint64_t)addr / 4 + 16 > d->material_word_count) throw std::invalid_argument("material address out of range");
-        for (int k = 0; k < 4; k++)
-            if (d->material_data[addr / 4 + 4 + k] < ST_IMAGE_LIMIT) throw std::domain_error("material textures are not supported");
+        // images: base colour and packed words 2, 3; emission images, normal
+        // maps and alpha masks are refused
+        const uint32_t* mw = d->material_data + addr / 4;
+        if (mw[5] < ST_IMAGE_LIMIT || mw[9] < ST_IMAGE_LIMIT || mw[12] < ST_IMAGE_LIMIT)
+            throw std::domain_error("emission images, normal maps and alpha masks are not supported");
+        for (int k : {4, 6, 7})
+            if (mw[k] < ST_IMAGE_LIMIT && (mw[k] >= d->image_count || !d->images || !d->images[mw[k]].rgba ||
+                                           d->images[mw[k]].width == 0 || d->images[mw[k]].height == 0))
+                throw std::invalid_argument("material image index out of range or empty");
         if (d->material_data[addr / 4 + 9] < ST_IMAGE_LIMIT || d->material_data[addr / 4 + 12] < ST_IMAGE_LIMIT)
             throw std::domain_error("material textures are not supported (normal maps, alpha masks)");
         if (type == 0 && (in.data & 0xffffu) >= d->mesh_vertex_info_count) throw std::invalid_argument("vertex info index out of range");
@@ -2472,6 +2480,60 @@ void validate_desc(const st_scene_desc* d) {
     }
     // a medium's scattering vertices take the primitive id 0xffff that marks sphere hits
     if (nsph && nvol) throw std::domain_error("sphere instances together with media are not supported");
+}
+
+// The material images' mip chains (2x2 box, as Image::generateMipMaps blits,
+// Image.cpp:282-308, in the same float operations as the environment's), all
+// levels of all used images in one buffer.
+void build_material_images(st_scene* s, const st_scene_desc* d) {
+    DevScene& D = s->d;
+    D.has_tex = 0;
+    D.tex_count = 0;
+    std::vector<uint2> img(d->image_count + 1, make_uint2(0, 0));
+    std::vector<uint4> lvt;
+    std::vector<float> data;
+    bool any = false;
+    for (uint32_t i = 0; i < d->instance_count; i++) {
+        const uint32_t type = d->instances[i].type_material_address & 0xfu;
+        if (type > 1) continue;
+        const uint32_t* mw = d->material_data + (d->instances[i].type_material_address >> 4) / 4;
+        for (int k : {4, 6, 7}) {
+            const uint32_t id = mw[k];
+            if (id >= ST_IMAGE_LIMIT || img[id].y) continue;
+            const st_image_desc& im = d->images[id];
+            std::vector<float> cur(im.rgba, im.rgba + 4ull * im.width * im.height);
+            int w0 = (int)im.width, h0 = (int)im.height;
+            img[id].x = (uint32_t)lvt.size();
+            for (int l = 0; l < 16; l++) {
+                lvt.push_back(make_uint4((uint32_t)(data.size() / 4), (uint32_t)w0, (uint32_t)h0, 0u));
+                data.insert(data.end(), cur.begin(), cur.end());
+                img[id].y++;
+                if (!(w0 > 1 || h0 > 1) || l == 15) break;
+                const int w1 = w0 > 1 ? w0 / 2 : 1, h1 = h0 > 1 ? h0 / 2 : 1;
+                std::vector<float> nx(4ull * w1 * h1);
+                for (int y = 0; y < h1; y++)
+                    for (int x = 0; x < w1; x++) {
+                        const int x0 = w0 > 1 ? 2 * x : 0, x1 = w0 > 1 ? 2 * x + 1 : 0;
+                        const int y0 = h0 > 1 ? 2 * y : 0, y1 = h0 > 1 ? 2 * y + 1 : 0;
+                        for (int ch = 0; ch < 4; ch++) {
+                            auto T = [&](int xx, int yy) { return cur[4 * ((size_t)yy * w0 + xx) + ch]; };
+                            nx[4 * ((size_t)y * w1 + x) + ch] = ((T(x0, y0) + T(x1, y0)) + (T(x0, y1) + T(x1, y1))) * 0.25f;
+                        }
+                    }
+                cur.swap(nx);
+                w0 = w1;
+                h0 = h1;
+            }
+            any = true;
+        }
+    }
+    if (!any) return;
+    if (data.size() / 4 >= (1ull << 32)) throw std::domain_error("material images too large");
+    D.tex_data = s->upload<float4>(data.data(), data.size() * sizeof(float));
+    D.tex_lv = s->upload<uint4>(lvt.data(), lvt.size() * sizeof(uint4));
+    D.tex_img = s->upload<uint2>(img.data(), img.size() * sizeof(uint2));
+    D.tex_count = d->image_count;
+    D.has_tex = 1;
 }
 
 // Scene AABB (Scene.cpp:622-631 mesh instances, :672-711 volumes, where the
@@ -2890,6 +2952,7 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
             s->build_light_table(ctx->stream);
         }
         s->build_sphere_tree(d->transforms);
+        build_material_images(s.get(), d);
         BvhOut bvh = build_bvh(verts.data(), ids.data(), (uint32_t)ntri);
         D.nodes = s->upload<float4>(bvh.nodes4.data(), bvh.nodes4.size() * sizeof(Node4Out));
         D.tris = s->upload<float4>(bvh.tris4.data(), bvh.tris4.size() * sizeof(TriOut));
@@ -3089,7 +3152,15 @@ static ShadeFn shade_kernel_adj(bool lambert, bool alias, bool media) {
         k_shade<true, false, false, true, true>,   k_shade<true, true, false, true, true>};
     return t[(media ? 4 : 0) | (lambert ? 2 : 0) | (alias ? 1 : 0)];
 }
-static ShadeFn shade_kernel(bool lambert, bool alias, bool restir, bool media) {
+static ShadeFn shade_kernel(bool lambert, bool alias, bool restir, bool media, bool tex = false) {
+    if (tex) {  // material images: no media (refused with them)
+        static const ShadeFn tt[8] = {
+            k_shade<false, false, false, false, false, false, true>, k_shade<false, false, true, false, false, false, true>,
+            k_shade<false, true, false, false, false, false, true>,  k_shade<false, true, true, false, false, false, true>,
+            k_shade<true, false, false, false, false, false, true>,  k_shade<true, false, true, false, false, false, true>,
+            k_shade<true, true, false, false, false, false, true>,   k_shade<true, true, true, false, false, false, true>};
+        return tt[(lambert ? 4 : 0) | (alias ? 2 : 0) | (restir ? 1 : 0)];
+    }
     if (media) {  // RIS at every vertex (no reuse) with media
         static const ShadeFn mt[8] = {k_shade<false, false, false, true>, k_shade<false, false, true, true>,
                                       k_shade<false, true, false, true>,  k_shade<false, true, true, true>,
@@ -3227,6 +3298,7 @@ static int check_camera(st_scene* s, const st_render_params* p) {
         return set_error(ST_ERROR_INVALID_ARGUMENT, "candidate_samples must be in [1, 1024]");
     if ((uint64_t)p->width * p->height > (1ull << 28)) return set_error(ST_ERROR_LIMIT, "image too large");
     if (p->max_null_collisions > (1u << 20)) return set_error(ST_ERROR_INVALID_ARGUMENT, "max_null_collisions above 2^20");
+
     if (s->d.vinst_count > 0 && (p->flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_RESTIR_DI_HASHGRID)))
         return set_error(ST_ERROR_UNSUPPORTED, "ReSTIR DI reuse is not supported in scenes with media (RIS is)");
     if (p->flags & ST_FLAG_RESTIR_DI_HASHGRID) {
@@ -3434,7 +3506,7 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
                 Timed tm(j.timed ? ctx : nullptr, 1, sm);
                 AuxOut aux{nullptr, nullptr, nullptr, nullptr};
                 if (base == 0) aux = AuxOut{o->visibility, o->albedo, o->depth, o->prev_uv};
-                auto* kfn = cj ? shade_kernel_cache(lambert, alias, media) : shade_kernel(lambert, alias, restir && (!reuse || (p->flags & ST_FLAG_RESTIR_DI_DEEP)), media);
+                auto* kfn = cj ? shade_kernel_cache(lambert, alias, media) : shade_kernel(lambert, alias, restir && (!reuse || (p->flags & ST_FLAG_RESTIR_DI_DEEP)), media, s->d.has_tex != 0);
                 hipLaunchKernelGGL(kfn, dim3(sgrid), dim3(kShadeBlock), 0, sm, s->d, c, w, qsel, aux, base == 0 ? 1 : 0,
                                    AdjArgs{});
             }
@@ -3499,8 +3571,18 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
         hipLaunchKernelGGL(k_store_rays, dim3(1), dim3(64), 0, s0, j.counters[0], (unsigned long long*)o->ray_counts);
 }
 
+// Material images serve camera paths with plain NEE or RIS, in scenes without media.
+static int check_images(const st_scene* s, uint32_t flags) {
+    if (!s->d.has_tex) return ST_OK;
+    if (s->d.vinst_count > 0 || (flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_RESTIR_DI_HASHGRID | ST_FLAG_TRACE_FROM_LIGHT |
+                                           ST_FLAG_BDPT_LVC)))
+        return set_error(ST_ERROR_UNSUPPORTED, "material images serve camera paths with plain NEE or RIS, without media or reuse");
+    return ST_OK;
+}
+
 int st_render(st_scene* s, const st_view* view, const st_render_params* p, const st_outputs* o) {
     if (!s || !view || !p || !o || !o->hdr) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
+    if (check_images(s, p->flags) != ST_OK) return ST_ERROR_UNSUPPORTED;
     if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
     if (p->flags & ST_FLAG_TRACE_FROM_LIGHT) return render_light(s, view, p, o);
     if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, o);
@@ -3661,6 +3743,7 @@ int st_adjoint(st_scene* s, const st_view* view, const st_render_params* p, cons
     if (p->width == 0 || p->height == 0 || p->spp == 0) return set_error(ST_ERROR_INVALID_ARGUMENT, "empty render");
     if (p->flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC | ST_FLAG_RESTIR_DI | ST_FLAG_RESTIR_DI_REUSE))
         return set_error(ST_ERROR_UNSUPPORTED, "the adjoint replays camera paths with plain NEE");
+    if (s->d.has_tex) return set_error(ST_ERROR_UNSUPPORTED, "the adjoint in a scene with material images");
     if (grad_albedo && (!s->d.tri_alb || !(p->flags & ST_FLAG_LAMBERTIAN)))
         return set_error(ST_ERROR_INVALID_ARGUMENT, "albedo gradients need a per-triangle albedo and ST_FLAG_LAMBERTIAN");
     if (grad_grid && grad_volume >= s->d.vol_count) return set_error(ST_ERROR_INVALID_ARGUMENT, "no such volume");
@@ -4058,6 +4141,7 @@ int st_render_cached(st_scene* s, st_cache* cache, const st_view* view, const st
     if (cache->ctx != s->ctx) return set_error(ST_ERROR_INVALID_ARGUMENT, "the cache belongs to another context");
     if (p->flags & (ST_FLAG_TRACE_FROM_LIGHT | ST_FLAG_BDPT_LVC))
         return set_error(ST_ERROR_UNSUPPORTED, "the radiance cache ends camera paths of the path tracer");
+    if (s->d.has_tex) return set_error(ST_ERROR_UNSUPPORTED, "the radiance cache in a scene with material images");
     if ((p->flags & ST_FLAG_RESTIR_DI) && !(p->flags & ST_FLAG_RESTIR_DI_REUSE))
         return set_error(ST_ERROR_UNSUPPORTED, "with the radiance cache, ReSTIR DI runs at primary vertices (ST_FLAG_RESTIR_DI_REUSE)");
     if (p->flags & ST_FLAG_RESTIR_DI_DEEP)
@@ -4369,6 +4453,7 @@ int st_batch_create(st_context* ctx, const st_render_request* requests, uint32_t
             return set_error(ST_ERROR_UNSUPPORTED, "batches run camera paths (no light tracing or bidirectional mode)");
         if (p.flags & ST_FLAG_RESTIR_DI_HASHGRID)
             return set_error(ST_ERROR_UNSUPPORTED, "hash-grid reuse chains a request's samples through the context's grids: not in batches");
+        if (check_images(R.scene, p.flags) != ST_OK) return ST_ERROR_UNSUPPORTED;
         const int rc = check_camera(R.scene, &p);
         if (rc != ST_OK) return rc;
     }

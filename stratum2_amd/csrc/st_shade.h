@@ -13,6 +13,7 @@ struct Shading {  // ShadingData fields used on camera paths (compat/scene.h:185
     uint32_t material;  // byte address
     uint32_t gn, sn, tan;
     float area;
+    float u, v, uvs;    // mTexcoord, mTexcoordScreenSize (read by the textured kernels only)
 };
 
 __device__ __forceinline__ v3 xf_point(const st_transform& T, v3 v) {
@@ -72,6 +73,11 @@ __device__ __attribute__((noinline)) void sphere_shading(const DevScene& S, uint
     const float su = sin_(u), cu = cos_(u), sv = sin_(v);
     const v3 dpdu = xf_vector(T, mk(-su * sv, 0.0f, cu * sv));
     r.tan = pack_normal(normalize(dpdu - n * dot(n, dpdu)));
+    r.u = u;
+    r.v = v;
+    const float cv = cos_(v);
+    const v3 dpdv = xf_vector(T, mk(cu * cv, -sv, su * cv));
+    r.uvs = 1 / fmax_(len(dpdu), len(dpdv));
 }
 
 // makeTriangleShadingData(WithoutPosition), shading_data.hlsli:59-151
@@ -117,12 +123,18 @@ __device__ inline void tri_shading(const DevScene& S, uint32_t ii, uint32_t prim
         __builtin_memcpy(t2, Tp + (size_t)ts * tri[2], 8);
     }
     const float duvds0 = t2[0] - t0[0], duvds1 = t2[1] - t0[1], duvdt0 = t2[0] - t1[0], duvdt1 = t2[1] - t1[1];
+    r.u = t0[0] + (t1[0] - t0[0]) * bx + duvds0 * by;
+    r.v = t0[1] + (t1[1] - t0[1]) * bx + duvds1 * by;
+    r.uvs = 1;
     const float det = duvds0 * duvdt1 - duvdt0 * duvds1;
     v3 dPdu;
     if (det != 0) {
         const float invDet = 1 / det;
         const float dsdu = duvdt1 * invDet, dtdu = -duvds1 * invDet;
         dPdu = -(dPds * dsdu + dPdt * dtdu);
+        const float dsdv = duvdt0 * invDet, dtdv = -duvds0 * invDet;  // as the reference writes them
+        const v3 dPdv = -(dPds * dsdv + dPdt * dtdv);
+        r.uvs = 1 / fmax_(len(dPdu), len(dPdv));
     } else {
         v3 o1;
         make_onb(gn, dPdu, o1);
@@ -172,6 +184,82 @@ __device__ __forceinline__ float prim_pick(const DevScene& S, uint32_t id) {
     return (id >> 16) == 0xffffu ? 1.0f : 1.0f / (float)(S.inst[id & 0xffffu].data >> 16);
 }
 
+// SampleImage4 (scene.hlsli:101-109) of material image img: SampleLevel at
+// lod = log2(max(uvs * max(w, h), 1e-6)) (0 without a screen size), linear
+// between the two nearest levels as the static sampler filters
+// (TestRenderer.cpp:54-57), bilinear with repeat addressing in each; the lod
+// is clamped to the chain and log2 is the renderer's log over ln 2.
+__device__ inline float4 tex_bilinear(const DevScene& S, uint32_t lvi, float u, float v) {
+    const uint4 L = S.tex_lv[lvi];
+    const int w = (int)L.y, h = (int)L.z;
+    const float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    const float fx0 = floorf(x), fy0 = floorf(y), ax = x - fx0, ay = y - fy0;
+    const int x0 = (int)fx0, y0 = (int)fy0;
+    const int xa = ((x0 % w) + w) % w, xb = (((x0 + 1) % w) + w) % w;
+    const int ya = ((y0 % h) + h) % h, yb = (((y0 + 1) % h) + h) % h;
+    const float4* P = S.tex_data + L.x;
+    const float4 t00 = P[(size_t)ya * w + xa], t10 = P[(size_t)ya * w + xb], t01 = P[(size_t)yb * w + xa],
+                 t11 = P[(size_t)yb * w + xb];
+    const float4 top = make_float4(t00.x * (1 - ax) + t10.x * ax, t00.y * (1 - ax) + t10.y * ax, t00.z * (1 - ax) + t10.z * ax,
+                                   t00.w * (1 - ax) + t10.w * ax);
+    const float4 bot = make_float4(t01.x * (1 - ax) + t11.x * ax, t01.y * (1 - ax) + t11.y * ax, t01.z * (1 - ax) + t11.z * ax,
+                                   t01.w * (1 - ax) + t11.w * ax);
+    return make_float4(top.x * (1 - ay) + bot.x * ay, top.y * (1 - ay) + bot.y * ay, top.z * (1 - ay) + bot.z * ay,
+                       top.w * (1 - ay) + bot.w * ay);
+}
+__device__ inline float4 tex_sample(const DevScene& S, uint32_t img, float u, float v, float uvs) {
+    const uint2 I = S.tex_img[img];
+    const uint4 L0 = S.tex_lv[I.x];
+    float lod = 0;
+    if (uvs > 0) lod = log_(fmax_(uvs * (float)(L0.y > L0.z ? L0.y : L0.z), 1e-6f)) * 1.44269504088896340736f;
+    if (!(lod > 0)) lod = 0;
+    if (lod > (float)(I.y - 1)) lod = (float)(I.y - 1);
+    const int l0 = (int)floorf(lod), l1 = l0 + 1 < (int)I.y ? l0 + 1 : l0;
+    const float f = lod - (float)l0;
+    const float4 a = tex_bilinear(S, I.x + (uint32_t)l0, u, v);
+    if (!(f > 0) || l1 == l0) return a;
+    const float4 b = tex_bilinear(S, I.x + (uint32_t)l1, u, v);
+    return make_float4(a.x * (1 - f) + b.x * f, a.y * (1 - f) + b.y * f, a.z * (1 - f) + b.z * f, a.w * (1 - f) + b.w * f);
+}
+// D3DX_FLOAT4_to_R8G8B8A8_UNORM and back
+__device__ __forceinline__ uint32_t pack_unorm4(float4 v) {
+    return (uint32_t)(sat(v.x) * 255 + 0.5f) | ((uint32_t)(sat(v.y) * 255 + 0.5f) << 8) |
+           ((uint32_t)(sat(v.z) * 255 + 0.5f) << 16) | ((uint32_t)(sat(v.w) * 255 + 0.5f) << 24);
+}
+__device__ __forceinline__ float4 unpack_unorm4(uint32_t w) {
+    return make_float4((float)(w & 0xffu) / 255.0f, (float)((w >> 8) & 0xffu) / 255.0f, (float)((w >> 16) & 0xffu) / 255.0f,
+                       (float)(w >> 24) / 255.0f);
+}
+// LoadMaterial with its images (material.hlsli:22-45): base colour times the
+// image (setBaseColor re-quantises to 8 bits), packed word 2 modulated with
+// roughness through smoothness, packed word 3 modulated.  Emission images are
+// refused at scene creation.
+__device__ inline void apply_textures(const DevScene& S, Material& m, uint32_t addr, float u, float v,
+                                                         float uvs) {
+    const uint4 ii = *reinterpret_cast<const uint4*>(S.mat + addr / 4 + 4);
+    if (ii.x < S.tex_count && S.tex_img[ii.x].y) {
+        const float4 t = tex_sample(S, ii.x, u, v, uvs);
+        const v3 b = m.base();
+        const float bb[3] = {b.x * t.x, b.y * t.y, b.z * t.z};
+        uint32_t w = m.p[0] & 0xff000000u;
+        for (int c = 0; c < 3; c++) w |= (uint32_t)floorf(sat(bb[c]) * 255 + 0.5f) << (8 * c);
+        m.p[0] = w;
+    }
+    if (ii.z < S.tex_count && S.tex_img[ii.z].y) {
+        float4 x = unpack_unorm4(m.p[2]);
+        const float4 t = tex_sample(S, ii.z, u, v, uvs);
+        x.x *= t.x;
+        x.z *= t.z;
+        x.w *= t.w;
+        x.y = 1 - (1 - x.y) * (1 - t.y);
+        m.p[2] = pack_unorm4(x);
+    }
+    if (ii.w < S.tex_count && S.tex_img[ii.w].y) {
+        const float4 x = unpack_unorm4(m.p[3]), t = tex_sample(S, ii.w, u, v, uvs);
+        m.p[3] = pack_unorm4(make_float4(x.x * t.x, x.y * t.y, x.z * t.z, x.w * t.w));
+    }
+}
+
 // ShadingData::toLocal / toWorld / shadingNormalCorrection, shading_data.hlsli:24-54
 struct Frame {
     v3 t, b, n;
@@ -214,6 +302,14 @@ __device__ __forceinline__ Material load_material(const DevScene& S, uint32_t ad
     m.has_bo = false;
     m.bo = mk(0, 0, 0);
     m.model = ((flags & (ST_FLAG_LAMBERT_GGX | ST_FLAG_LAMBERTIAN)) == ST_FLAG_LAMBERT_GGX && m.metal() >= 0.5f) ? 2u : 0u;
+    return m;
+}
+// load_material at a shading point; TEX: with its images (the kernels
+// instantiated for scenes with material images)
+template <bool TEX>
+__device__ __forceinline__ Material load_material(const DevScene& S, const Shading& sd, uint32_t flags) {
+    Material m = load_material(S, sd.material, flags);
+    if (TEX) apply_textures(S, m, sd.material, sd.u, sd.v, sd.uvs);
     return m;
 }
 
