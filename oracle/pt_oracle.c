@@ -501,6 +501,7 @@ struct or_scene {
     /* material images (Material::store's image indices), each with its mip
      * chain; ntex = number of images, tex[i].n = 0 when no material uses it */
     uint32_t ntex;
+    int has_alpha; /* a mesh material carries an alpha mask */
     struct tex_s { int n; level_t lv[16]; }* tex;
 };
 
@@ -1885,6 +1886,34 @@ static void make_hit_shading(const or_scene* s, const hit_t* h, f3 o, f3 d, shad
     }
     make_tri_shading(s, ii, prim, h->b1, h->b2, r);
 }
+/* The alpha test of a candidate triangle hit (intersection.hlsli:125-147):
+ * a triangle whose material names an alpha mask (material word 12, the
+ * cutoff in word 13, material.hlsli:72-76) is committed only where channel 0
+ * of the mask, sampled bilinearly at level 0 at the candidate's texture
+ * coordinate, reaches the cutoff. */
+static int alpha_pass(const or_scene* s, uint32_t id, real b1, real b2) {
+    const st_instance* in = &s->inst[id & 0xffffu];
+    const uint32_t a = (in->type_material_address >> 4) / 4, mask = s->mat[a + 12];
+    if (mask >= ST_IMAGE_LIMIT) return 1;
+    float cutoff;
+    memcpy(&cutoff, &s->mat[a + 13], 4);
+    const st_mesh_vertex_info* vi = &s->vi[in->data & 0xffffu];
+    uint32_t tri[3] = {0, 0, 0};
+    load_tri(s, vi, id >> 16, tri);
+    real t0[2] = {0, 0}, t1[2] = {0, 0}, t2[2] = {0, 0};
+    const uint32_t tb = vi_field(vi->packed_buffer_indices[1], 16, 16), ts = vi_field(vi->packed_strides, 24, 8);
+    if (tb < ST_IMAGE_LIMIT) {
+        load_f2(s, tb, vi->packed_offsets[3], ts, tri[0], t0);
+        load_f2(s, tb, vi->packed_offsets[3], ts, tri[1], t1);
+        load_f2(s, tb, vi->packed_offsets[3], ts, tri[2], t2);
+    }
+    const real u = t0[0] + (t1[0] - t0[0]) * b1 + (t2[0] - t0[0]) * b2;
+    const real v = t0[1] + (t1[1] - t0[1]) * b1 + (t2[1] - t0[1]) * b2;
+    real o[4];
+    bilinear4(&s->tex[mask].lv[0], u, v, o);
+    return o[0] >= (real)cutoff;
+}
+
 static hit_t trace_tris(const or_scene* s, f3 o, f3 d, real tmax, int any, uint64_t* nodes, uint64_t* tris) {
     hit_t h;
     h.hit = 0; h.t = tmax; h.b1 = h.b2 = 0; h.id = 0xffffffffu;
@@ -1907,6 +1936,7 @@ static hit_t trace_tris(const or_scene* s, f3 o, f3 d, real tmax, int any, uint6
                 if (!wt_intersect(&wr, s->tv + 9 * (size_t)ti, &t, &b1, &b2)) continue;
                 uint32_t id = s->tid[ti];
                 if (t < h.t || (t == h.t && h.hit && id < h.id)) {
+                    if (s->has_alpha && !alpha_pass(s, id, b1, b2)) continue;
                     h.t = t; h.b1 = b1; h.b2 = b2; h.id = id; h.hit = 1;
                     if (any) return h;
                 }
@@ -2123,17 +2153,23 @@ or_scene* or_scene_create(const st_scene_desc* d) {
     }
     if (s->nsph && s->nvolinst) { fail("sphere instances together with media are not supported"); or_scene_destroy(s); return NULL; }
     /* material images: base colour, packed words 2 and 3 (Material::store's
-     * image indices); emission images, alpha masks and normal maps are refused */
+     * image indices) and the alpha masks of mesh materials (word 12); emission
+     * images and normal maps are refused */
     s->tex = (struct tex_s*)calloc(d->image_count + 1, sizeof(struct tex_s));
     for (uint32_t i = 0; i < s->ninst; i++) {
         const uint32_t type = s->inst[i].type_material_address & 0xfu;
         if (type > 1) continue;
         const uint32_t a = (s->inst[i].type_material_address >> 4) / 4;
         if ((uint64_t)a + 14 > s->nmatw) continue;
-        if (s->mat[a + 5] < ST_IMAGE_LIMIT || s->mat[a + 9] < ST_IMAGE_LIMIT || s->mat[a + 12] < ST_IMAGE_LIMIT) {
-            fail("emission images, normal maps and alpha masks are not supported"); or_scene_destroy(s); return NULL;
+        if (s->mat[a + 5] < ST_IMAGE_LIMIT || s->mat[a + 9] < ST_IMAGE_LIMIT) {
+            fail("emission images and normal maps are not supported"); or_scene_destroy(s); return NULL;
         }
-        for (int k = 4; k < 8; k++) {
+        if (s->mat[a + 12] < ST_IMAGE_LIMIT && type != 0) {
+            fail("alpha masks apply to mesh materials only"); or_scene_destroy(s); return NULL;
+        }
+        if (s->mat[a + 12] < ST_IMAGE_LIMIT) s->has_alpha = 1;
+        for (int k = 4; k < 13; k++) {
+            if (k == 8) k = 12;
             const uint32_t img = s->mat[a + k];
             if (img >= ST_IMAGE_LIMIT) continue;
             if (img >= d->image_count) { fail("material image index out of range"); or_scene_destroy(s); return NULL; }
@@ -4100,6 +4136,7 @@ static int render_camera(or_scene* s, const st_view* view, const st_render_param
     if (p->flags & ST_FLAG_BDPT_LVC) return render_vc(s, view, p, out, threads);
     if (s->nvolinst > 0 && (p->flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_RESTIR_DI_HASHGRID)))
         return fail("ReSTIR DI reuse with media is not supported (RIS is)");
+    if (s->ntex && (p->flags & ST_FLAG_RESTIR_DI_REUSE)) return fail("the screen-space reuse with material images");
     if (s->ntex && (p->flags & ST_FLAG_RESTIR_DI_REUSE)) return fail("the screen-space reuse with material images");
     const int reuse = (p->flags & ST_FLAG_RESTIR_DI_REUSE) != 0;
     if (reuse && !(p->flags & ST_FLAG_RESTIR_DI)) return fail("ST_FLAG_RESTIR_DI_REUSE needs ST_FLAG_RESTIR_DI");

@@ -88,6 +88,7 @@ struct DevScene {
     const uint2* tex_img;
     uint32_t tex_count;
     int has_tex;
+    int has_alpha;  // some mesh material has an alpha mask (its triangles' records carry 1 in the second w)
 };
 
 }  // namespace st

@@ -367,13 +367,13 @@ struct ClosestSrc {
     }
 };
 
-template <bool COUNT, bool SPH = false>
+template <bool COUNT, bool SPH = false, bool ALPHA = false>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_trace_closest(DevScene S, WaveState w, int qsel, uint32_t* spill,
                                                                unsigned long long* stats) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     ClosestSrc src{w, w.queue[qsel]};
     TraverseStats ts{0, 0};
-    trace_queue<false, COUNT, ClosestSrc, SPH>(S, w.counters[qsel], &w.counters[kFetchClosest], src, lds, spill, ts);
+    trace_queue<false, COUNT, ClosestSrc, SPH, ALPHA>(S, w.counters[qsel], &w.counters[kFetchClosest], src, lds, spill, ts);
     if (COUNT) {
         atomicAdd(&stats[0], (unsigned long long)ts.nodes);
         atomicAdd(&stats[1], (unsigned long long)ts.tris);
@@ -439,13 +439,13 @@ struct ShadowSrc {
     }
 };
 
-template <bool COUNT, bool SPH = false>
+template <bool COUNT, bool SPH = false, bool ALPHA = false>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_trace_shadow(DevScene S, WaveState w, uint32_t flags, uint32_t* spill,
                                                               unsigned long long* stats) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     ShadowSrc src{w, flags};
     TraverseStats ts{0, 0};
-    trace_queue<true, COUNT, ShadowSrc, SPH>(S, w.counters[2], &w.counters[kFetchShadow], src, lds, spill, ts);
+    trace_queue<true, COUNT, ShadowSrc, SPH, ALPHA>(S, w.counters[2], &w.counters[kFetchShadow], src, lds, spill, ts);
     if (COUNT) {
         atomicAdd(&stats[2], (unsigned long long)ts.nodes);
         atomicAdd(&stats[3], (unsigned long long)ts.tris);
@@ -458,11 +458,17 @@ using ClosestFn = void (*)(DevScene, WaveState, int, uint32_t*, unsigned long lo
 using ShadowFn = void (*)(DevScene, WaveState, uint32_t, uint32_t*, unsigned long long*);
 static ClosestFn closest_kernel(bool count, const DevScene& S) {
     const bool sph = S.sph_node_count != 0;
+    if (S.has_alpha)
+        return count ? (sph ? k_trace_closest<true, true, true> : k_trace_closest<true, false, true>)
+                     : (sph ? k_trace_closest<false, true, true> : k_trace_closest<false, false, true>);
     return count ? (sph ? k_trace_closest<true, true> : k_trace_closest<true, false>)
                  : (sph ? k_trace_closest<false, true> : k_trace_closest<false, false>);
 }
 static ShadowFn shadow_kernel(bool count, const DevScene& S) {
     const bool sph = S.sph_node_count != 0;
+    if (S.has_alpha)
+        return count ? (sph ? k_trace_shadow<true, true, true> : k_trace_shadow<true, false, true>)
+                     : (sph ? k_trace_shadow<false, true, true> : k_trace_shadow<false, false, true>);
     return count ? (sph ? k_trace_shadow<true, true> : k_trace_shadow<true, false>)
                  : (sph ? k_trace_shadow<false, true> : k_trace_shadow<false, false>);
 }
@@ -2076,14 +2082,14 @@ struct RaySrc {
     }
 };
 
-template <bool SPH>
+template <bool SPH, bool ALPHA = false>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_rays(DevScene S, const float4* rays, uint32_t n, int closest,
                                                             uint32_t* out, uint32_t* spill, uint32_t* fetch) {
     __shared__ uint32_t lds[kLdsStack * kTraceBlock];
     RaySrc src{rays, out, closest};
     TraverseStats ts{0, 0};
-    if (closest) trace_queue<false, false, RaySrc, SPH>(S, n, fetch, src, lds, spill, ts);
-    else trace_queue<true, false, RaySrc, SPH>(S, n, fetch, src, lds, spill, ts);
+    if (closest) trace_queue<false, false, RaySrc, SPH, ALPHA>(S, n, fetch, src, lds, spill, ts);
+    else trace_queue<true, false, RaySrc, SPH, ALPHA>(S, n, fetch, src, lds, spill, ts);
 }
 
 // World-space triangle records (v0.xyz|id, v1, v2) of every mesh instance
@@ -2102,7 +2108,8 @@ __global__ void k_flatten(DevScene S, const uint32_t* tri_inst, const uint32_t* 
         const v3 a = xf_point(T, ld_v3(P + (size_t)ps * tri[0])), b = xf_point(T, ld_v3(P + (size_t)ps * tri[1])),
                  c = xf_point(T, ld_v3(P + (size_t)ps * tri[2]));
         out[3 * (size_t)t] = make_float4(a.x, a.y, a.z, __uint_as_float(i | (p << 16)));
-        out[3 * (size_t)t + 1] = make_float4(b.x, b.y, b.z, 0.0f);
+        const uint32_t alpha = S.mat[(in.type_material_address >> 4) / 4 + 12] < ST_IMAGE_LIMIT ? 1u : 0u;
+        out[3 * (size_t)t + 1] = make_float4(b.x, b.y, b.z, __uint_as_float(alpha));
         out[3 * (size_t)t + 2] = make_float4(c.x, c.y, c.z, 0.0f);
     }
 }
@@ -2467,14 +2474,14 @@ void validate_desc(const st_scene_desc* d) {
         // images: base colour and packed words 2, 3; emission images, normal
         // maps and alpha masks are refused
         const uint32_t* mw = d->material_data + addr / 4;
-        if (mw[5] < ST_IMAGE_LIMIT || mw[9] < ST_IMAGE_LIMIT || mw[12] < ST_IMAGE_LIMIT)
-            throw std::domain_error("emission images, normal maps and alpha masks are not supported");
-        for (int k : {4, 6, 7})
+        if (mw[5] < ST_IMAGE_LIMIT || mw[9] < ST_IMAGE_LIMIT)
+            throw std::domain_error("emission images and normal maps are not supported");
+        if (mw[12] < ST_IMAGE_LIMIT && type != 0) throw std::domain_error("alpha masks apply to mesh materials only");
+        for (int k : {4, 6, 7, 12})
             if (mw[k] < ST_IMAGE_LIMIT && (mw[k] >= d->image_count || !d->images || !d->images[mw[k]].rgba ||
                                            d->images[mw[k]].width == 0 || d->images[mw[k]].height == 0))
                 throw std::invalid_argument("material image index out of range or empty");
-        if (d->material_data[addr / 4 + 9] < ST_IMAGE_LIMIT || d->material_data[addr / 4 + 12] < ST_IMAGE_LIMIT)
-            throw std::domain_error("material textures are not supported (normal maps, alpha masks)");
+        if (d->material_data[addr / 4 + 9] < ST_IMAGE_LIMIT) throw std::domain_error("normal maps are not supported");
         if (type == 0 && (in.data & 0xffffu) >= d->mesh_vertex_info_count) throw std::invalid_argument("vertex info index out of range");
         if (type == 1) nsph++;
     }
@@ -2488,6 +2495,7 @@ void validate_desc(const st_scene_desc* d) {
 void build_material_images(st_scene* s, const st_scene_desc* d) {
     DevScene& D = s->d;
     D.has_tex = 0;
+    D.has_alpha = 0;
     D.tex_count = 0;
     std::vector<uint2> img(d->image_count + 1, make_uint2(0, 0));
     std::vector<uint4> lvt;
@@ -2497,7 +2505,8 @@ void build_material_images(st_scene* s, const st_scene_desc* d) {
         const uint32_t type = d->instances[i].type_material_address & 0xfu;
         if (type > 1) continue;
         const uint32_t* mw = d->material_data + (d->instances[i].type_material_address >> 4) / 4;
-        for (int k : {4, 6, 7}) {
+        if (type == 0 && mw[12] < ST_IMAGE_LIMIT) D.has_alpha = 1;
+        for (int k : {4, 6, 7, 12}) {
             const uint32_t id = mw[k];
             if (id >= ST_IMAGE_LIMIT || img[id].y) continue;
             const st_image_desc& im = d->images[id];
@@ -2954,6 +2963,13 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
         s->build_sphere_tree(d->transforms);
         build_material_images(s.get(), d);
         BvhOut bvh = build_bvh(verts.data(), ids.data(), (uint32_t)ntri);
+        if (D.has_alpha)
+            for (TriOut& t : bvh.tris4) {
+                uint32_t id;
+                std::memcpy(&id, &t.f[3], 4);
+                const uint32_t alpha = d->material_data[(d->instances[id & 0xffffu].type_material_address >> 4) / 4 + 12] < ST_IMAGE_LIMIT;
+                std::memcpy(&t.f[7], &alpha, 4);
+            }
         D.nodes = s->upload<float4>(bvh.nodes4.data(), bvh.nodes4.size() * sizeof(Node4Out));
         D.tris = s->upload<float4>(bvh.tris4.data(), bvh.tris4.size() * sizeof(TriOut));
         D.node_count = ntri ? (uint32_t)bvh.nodes4.size() : 0;
@@ -3073,6 +3089,7 @@ int st_scene_create(st_context* ctx, const st_scene_desc* d, st_scene** out) {
 }
 
 int st_scene_update(st_scene* s, const st_transform* transforms, double* build_ms) {
+    if (s && s->d.has_alpha) return set_error(ST_ERROR_UNSUPPORTED, "st_scene_update with alpha masks is not supported");
     if (!s || !transforms) return set_error(ST_ERROR_INVALID_ARGUMENT, "null argument");
     st_context* ctx = s->ctx;
     try {
@@ -3574,6 +3591,7 @@ static void enqueue_camera(st_context* ctx, const CameraJob& j) {
 // Material images serve camera paths with plain NEE or RIS, in scenes without media.
 static int check_images(const st_scene* s, uint32_t flags) {
     if (!s->d.has_tex) return ST_OK;
+    // (alpha masks are material images too: the same modes, and no media)
     if (s->d.vinst_count > 0 || (flags & (ST_FLAG_RESTIR_DI_REUSE | ST_FLAG_RESTIR_DI_HASHGRID | ST_FLAG_TRACE_FROM_LIGHT |
                                            ST_FLAG_BDPT_LVC)))
         return set_error(ST_ERROR_UNSUPPORTED, "material images serve camera paths with plain NEE or RIS, without media or reuse");
@@ -4833,7 +4851,8 @@ int st_trace_rays(st_scene* s, const float* rays, uint32_t count, int closest, u
         ensure_spill(B, ctx->cus * kTraceBlocksPerCU, kTraceBlock);
         B.counters.reserve(4 * kCounterWords);
         HIP_CHECK(hipMemsetAsync(B.counters.as<uint32_t>() + kFetchRays, 0, 4 * kXcds * kFetchStride, ctx->stream));
-        hipLaunchKernelGGL(s->d.sph_node_count ? k_trace_rays<true> : k_trace_rays<false>, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
+        hipLaunchKernelGGL(s->d.has_alpha ? (s->d.sph_node_count ? k_trace_rays<true, true> : k_trace_rays<false, true>)
+                                             : (s->d.sph_node_count ? k_trace_rays<true> : k_trace_rays<false>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
                            closest, hits, B.spill.as<uint32_t>(), B.counters.as<uint32_t>() + kFetchRays);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -4865,7 +4884,8 @@ int st_trace_rays_noalloc(st_scene* s, const float* rays, uint32_t count, int cl
         uint32_t* spill = reinterpret_cast<uint32_t*>(ctx->scratch + kScratchHeader);
         // the fetch counter starts at zero (a fill kernel: also right inside a captured graph)
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, fetch, kXcds * kFetchStride, 0u);
-        hipLaunchKernelGGL(s->d.sph_node_count ? k_trace_rays<true> : k_trace_rays<false>, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
+        hipLaunchKernelGGL(s->d.has_alpha ? (s->d.sph_node_count ? k_trace_rays<true, true> : k_trace_rays<false, true>)
+                                             : (s->d.sph_node_count ? k_trace_rays<true> : k_trace_rays<false>), dim3(grid), dim3(kTraceBlock), 0, ctx->stream, s->d, (const float4*)rays, count,
                            closest, hits, spill, fetch);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));

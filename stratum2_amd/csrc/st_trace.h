@@ -14,6 +14,7 @@
 #pragma once
 
 #include "st_device.h"
+#include "st_shade.h"
 
 namespace st {
 
@@ -96,6 +97,32 @@ struct TraverseStats {
     uint32_t nodes, tris;
 };
 
+// The alpha test of a candidate hit on a triangle whose material has an
+// alpha mask (intersection.hlsli:125-147): commit when the mask, sampled at
+// level 0 at the hit's texture coordinate, reaches the material's cutoff.
+// The mask is channel 0 of the material image (the reference's mImage1s).
+__device__ inline bool alpha_pass(const DevScene& S, uint32_t id, float b1, float b2) {
+    const uint32_t ii = id & 0xffffu, prim = id >> 16;
+    const st_instance in = S.inst[ii];
+    const uint32_t* mw = S.mat + (in.type_material_address >> 4) / 4;
+    const uint32_t mask = mw[12];
+    const float cutoff = __uint_as_float(mw[13]);
+    const st_mesh_vertex_info vi = S.vi[in.data & 0xffffu];
+    uint32_t tri[3];
+    load_tri(S, vi, prim, tri);
+    float t0[2] = {0, 0}, t1[2] = {0, 0}, t2[2] = {0, 0};
+    const uint32_t tb = vi.packed_buffer_indices[1] >> 16, ts = vi.packed_strides >> 24;
+    if (tb < ST_IMAGE_LIMIT) {
+        const uint8_t* Tp = S.vb[tb] + vi.packed_offsets[3];
+        __builtin_memcpy(t0, Tp + (size_t)ts * tri[0], 8);
+        __builtin_memcpy(t1, Tp + (size_t)ts * tri[1], 8);
+        __builtin_memcpy(t2, Tp + (size_t)ts * tri[2], 8);
+    }
+    const float u = t0[0] + (t1[0] - t0[0]) * b1 + (t2[0] - t0[0]) * b2;
+    const float v = t0[1] + (t1[1] - t0[1]) * b1 + (t2[1] - t0[1]) * b2;
+    return tex_bilinear(S, S.tex_img[mask].x, u, v).x >= cutoff;
+}
+
 // raySphere (compat/common.h:124-135) about the origin: the entry and exit
 // parameters of o + t d, or (0, 0) when the ray misses
 __device__ __forceinline__ void ray_sphere(v3 o, v3 d, float r, float& t0, float& t1) {
@@ -145,12 +172,16 @@ struct Lane {
         sp = 0;
     }
 
+    template <bool ALPHA = false>
     __device__ __forceinline__ bool tri_test(const DevScene& S, uint32_t ti) {
         const float4* tp = S.tris + 3 * (size_t)ti;
         const float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
         float t, b1, b2;
         if (!wt_intersect(wr, mk(t0.x, t0.y, t0.z), mk(t1.x, t1.y, t1.z), mk(t2.x, t2.y, t2.z), t, b1, b2)) return false;
         const uint32_t id = __float_as_uint(t0.w);
+        if (ALPHA && __float_as_uint(t1.w) && (t < h.t || (t == h.t && h.id != 0xffffffffu && id < h.id)) &&
+            !alpha_pass(S, id, b1, b2))
+            return false;
         if (t < h.t || (t == h.t && h.id != 0xffffffffu && id < h.id)) {
             h.t = t;
             h.b1 = b1;
@@ -191,8 +222,9 @@ struct Lane {
     // One 4-wide node: decode four quantised child boxes (o + q * 2^e, exact
     // in float), slab-test them, intersect hit leaf children at once, continue
     // with the nearest hit inner child and push the others farthest first.
-    // SPH: a step in the sphere instances' tree (leaf records are spheres)
-    template <bool ANY, bool COUNT, bool SPH = false>
+    // SPH: a step in the sphere instances' tree (leaf records are spheres);
+    // ALPHA: triangles of alpha-masked materials take the alpha test
+    template <bool ANY, bool COUNT, bool SPH = false, bool ALPHA = false>
     __device__ __forceinline__ bool step(const DevScene& S, uint32_t* lds, uint32_t* spill, uint32_t spill_stride,
                                          uint32_t gtid, TraverseStats& st) {
         const float4* np = (SPH ? S.sph_nodes : S.nodes) + 3 * (size_t)node;
@@ -235,7 +267,7 @@ struct Lane {
             const uint32_t j = (uint32_t)__builtin_ctz(trimask);
             trimask &= trimask - 1u;
             if (COUNT) st.tris++;
-            if ((SPH ? sph_test(S, base_tri + j) : tri_test(S, base_tri + j)) && ANY) return false;
+            if ((SPH ? sph_test(S, base_tri + j) : tri_test<ALPHA>(S, base_tri + j)) && ANY) return false;
         }
         // sort the (at most 4) inner candidates by entry distance
 #define ST_CSWAP(i, j)                                   \
@@ -300,7 +332,7 @@ constexpr uint32_t kChunk = 2048;
 // SPH: the scene has sphere instances; a lane done with the triangles walks
 // the spheres' tree next, up to its nearest hit (any hit: only when it found
 // none).
-template <bool ANY, bool COUNT, class Src, bool SPH = false>
+template <bool ANY, bool COUNT, class Src, bool SPH = false, bool ALPHA = false>
 __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint32_t* fetch, Src& src, uint32_t* lds,
                                             uint32_t* spill, TraverseStats& st) {
     const uint32_t stride = gridDim.x * blockDim.x, gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -342,7 +374,7 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
         if (!__any(active)) break;
         if (SPH) {
             if (active && !(sph ? L.step<ANY, COUNT, true>(S, lds, spill, stride, gtid, st)
-                                : L.step<ANY, COUNT>(S, lds, spill, stride, gtid, st))) {
+                                : L.step<ANY, COUNT, false, ALPHA>(S, lds, spill, stride, gtid, st))) {
                 if (!sph && !(ANY && L.h.hit)) {
                     sph = true;
                     L.node = 0;
@@ -352,7 +384,7 @@ __device__ __forceinline__ void trace_queue(const DevScene& S, uint32_t n, uint3
                     active = false;
                 }
             }
-        } else if (active && !L.step<ANY, COUNT>(S, lds, spill, stride, gtid, st)) {
+        } else if (active && !L.step<ANY, COUNT, false, ALPHA>(S, lds, spill, stride, gtid, st)) {
             src.done(idx, L.h);
             active = false;
         }

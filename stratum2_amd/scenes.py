@@ -551,6 +551,40 @@ def cornell_textured(seed=0):
     return s
 
 
+def cornell_alpha(seed=0, cutoff=0.5):
+    """The seeded Cornell box with two alpha-masked quads (material word 12 the
+    mask image, word 13 the cutoff, material.hlsli:72-76): a panel facing the
+    camera (texture coordinates 0..1 over x in [-0.5, 0.5], y in [0.4, 1.4] at
+    z = 0) and a grille under the ceiling light (x, z in [-0.4, 0.4] at y =
+    1.7, facing down) that casts masked shadows.  The mask is a 32x32 image
+    whose channel 0 is a 4x4 checker of 1 and 0 (cell (0, 0) solid) with a
+    soft ramp across the grille's cells."""
+    s = cornell_box(seed)
+    base = len(s.images)
+    y, x = np.mgrid[0:32, 0:32]
+    mask = np.zeros((32, 32, 4), np.float32)
+    mask[..., 0] = (((x // 8) + (y // 8)) % 2 == 0).astype(np.float32)
+    mask[..., 3] = 1
+    s.images.append(mask)
+    ramp = np.zeros((32, 32, 4), np.float32)
+    ramp[..., 0] = (x + 0.5) / 32 * (((x // 8) + (y // 8)) % 2 == 0)
+    ramp[..., 3] = 1
+    s.images.append(ramp)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32)
+    for k, (corners, nrm, mat) in enumerate((
+            (((-0.5, 0.4, 0.0), (0.5, 0.4, 0.0), (0.5, 1.4, 0.0), (-0.5, 1.4, 0.0)), (0, 0, 1),
+             material_record(base_color=(0.9, 0.5, 0.2), specular=0.5, roughness=0.4)),
+            (((-0.4, 1.7, -0.4), (0.4, 1.7, -0.4), (0.4, 1.7, 0.4), (-0.4, 1.7, 0.4)), (0, -1, 0),
+             material_record(base_color=(0.7, 0.7, 0.7), specular=0.3, roughness=0.8)))):
+        p, i = quad(*corners)
+        f = _tri_normal_ok(p, i.reshape(-1, 3), nrm)
+        vi, n = s.add_mesh(p, f.reshape(-1), texcoords=uv)
+        mat[12] = base + k
+        mat[13] = int(np.float32(cutoff).view(np.uint32))
+        s.add_mesh_instance(vi, n, s.add_material(mat))
+    return s
+
+
 def cornell_view(width=256, height=256):
     return make_view(width, height, position=(0.0, 1.0, 2.4))
 

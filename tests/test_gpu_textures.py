@@ -41,7 +41,7 @@ def test_textured_refusals(gpu_ctx):
         with pytest.raises(RuntimeError, match="material images"):
             TestRenderer(sc).render(scenes.cornell_view(16, 16),
                                     abi.default_params(16, 16, max_depth=3, spp=1, flags=abi.ST_FLAG_DEFAULT | fl))
-    for word in (5, 9, 12):
+    for word in (5, 9):  # alpha masks are taken (the alpha-mask tests)
         sd = scenes.cornell_box(0)
         sd.images.append(np.ones((4, 4, 4), np.float32))
         w = scenes.material_record()
