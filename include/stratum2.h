@@ -275,7 +275,8 @@ int st_scene_destroy(st_scene* scene);
  * tree) collapsed to the 4-wide node format.  Replaces the reference's
  * per-update acceleration-structure rebuild (src/App/Scene.cpp:726-754).
  * build_ms (may be null) receives the device build time.  Hits do not depend
- * on the tree: ties resolve to the smaller instance|primitive id. */
+ * on the tree: ties resolve to the smaller instance|primitive id.  A scene
+ * whose mesh materials carry alpha masks returns ST_ERROR_UNSUPPORTED. */
 int st_scene_update(st_scene* scene, const st_transform* transforms, double* build_ms);
 /* BVH statistics: node count, triangle count, max depth, bytes on device. */
 int st_scene_stats(const st_scene* scene, uint64_t out[4]);
